@@ -1,0 +1,374 @@
+"""Python binding of libcmm.so (the MI355X CMM engine) over its C-ABI (include/cmm.h).
+
+Device buffers are torch CUDA tensors of dtype float64 (torch is only the allocator and the
+stream); every call goes straight to the HIP library.  There is no CPU fallback: loading fails
+loudly when the library has not been built.
+
+The operator names follow the reference's host interface (Cuda-CMM/src/simulation/cmm-simulation-host.h:23-50,
+Cuda-CMM/src/numerical/cmm-particles.h:29-32, Cuda-CMM/src/numerical/cmm-fft.h:51).
+"""
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "build", "libcmm.so")
+TWO_PI = 6.283185307179586476925286766559005768394338798750211641949889184615632812572
+
+# initial-condition ids (include/cmm_types.h); names as in Cuda-CMM/src/ui/settings.h:313-324
+IC = {"4_nodes": 0, "quadropole": 1, "one_vortex": 2, "two_votices": 3, "three_vortices": 4,
+      "single_shear_layer": 5, "tanh_shear_layer": 6, "shielded_vortex": 9, "two_cosine": 10,
+      "vortex_sheets": 11, "double_shear_layer": 100, "random_spectrum": 101}
+# time integration names, Cuda-CMM/src/ui/settings.h:386-398
+TIME_INTEGRATION = {"EulerExp": 10, "RK1": 10, "AB2": 20, "RK2": 21, "Heun": 21, "RK3": 30, "RK3Mod": 31,
+                    "RK4": 40, "RK4Mod": 41}
+MAP_UPDATE_ORDER = {"2nd": 0, "4th": 1, "6th": 2}
+
+
+class CmmError(RuntimeError):
+    pass
+
+
+class Grid(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("x0", C.c_double), ("x1", C.c_double),
+                ("y0", C.c_double), ("y1", C.c_double)]
+
+    @property
+    def n(self):
+        return self.nx * self.ny
+
+
+class AdvectCoeffs(C.Structure):
+    _fields_ = [("L1", C.c_double * 4), ("L12", C.c_double * 4), ("c1", C.c_double * 12),
+                ("cx", C.c_double * 12), ("cy", C.c_double * 12), ("cxy", C.c_double * 12),
+                ("dt", C.c_double), ("eps", C.c_double), ("time_integration_num", C.c_int),
+                ("map_update_order_num", C.c_int), ("l_order", C.c_int), ("direction", C.c_int)]
+
+
+class EulerParams(C.Structure):
+    _fields_ = [("g_coarse", Grid), ("g_fine", Grid), ("g_psi", Grid), ("g_vort", Grid),
+                ("dt", C.c_double),
+                ("time_integration_num", C.c_int), ("map_update_order_num", C.c_int), ("l_order", C.c_int),
+                ("lagrange_init_higher_order", C.c_int),
+                ("map_epsilon", C.c_double), ("incomp_threshold", C.c_double), ("freq_cut_psi", C.c_double),
+                ("molly_stencil", C.c_int), ("skip_remapping", C.c_int),
+                ("ic", C.c_int), ("ic_params", C.c_double * 10),
+                ("discrete", C.c_int), ("g_disc", Grid), ("disc_real", C.POINTER(C.c_double)),
+                ("seed", C.c_uint64), ("stack_capacity", C.c_int)]
+
+
+class StepInfo(C.Structure):
+    _fields_ = [("t", C.c_double), ("incomp", C.c_double), ("remapped", C.c_int), ("stack_size", C.c_int),
+                ("saturated", C.c_int)]
+
+
+PROF_NAMES = ["advect", "incomp", "psi_row_r2c", "psi_col", "psi_row_c2r", "particles", "remap"]
+PROF_SLOTS = len(PROF_NAMES)
+FIELD = {"chix": 0, "chiy": 1, "psi": 2, "w_fine": 3, "w_coarse": 4, "disc": 5}
+
+# every symbol include/cmm.h declares
+SYMBOLS = ["cmm_version", "cmm_last_error", "cmm_device_info", "cmm_lagrange_coeff", "cmm_advect_coeffs_fill",
+           "cmm_map_init_identity", "cmm_map_advect", "cmm_incomp_error", "cmm_compose_positions",
+           "cmm_sample_init", "cmm_translate_workspace_bytes", "cmm_translate_ic", "cmm_hermite_workspace_bytes",
+           "cmm_hermite_from_real", "cmm_hermite_from_spectrum", "cmm_psi_workspace_bytes", "cmm_psi_from_vort",
+           "cmm_sample_vort", "cmm_particles_advect", "cmm_euler_create", "cmm_euler_destroy", "cmm_euler_stream",
+           "cmm_euler_add_particles", "cmm_euler_init", "cmm_euler_step", "cmm_euler_get_field",
+           "cmm_euler_get_particles", "cmm_euler_diagnostics", "cmm_euler_state", "cmm_euler_profile", "cmm_euler_profile_read"]
+
+_lib = None
+DP = C.POINTER(C.c_double)
+VP = C.c_void_p
+
+
+def lib():
+    """Load libcmm.so; raises when it is missing (there is no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise CmmError(f"{LIB_PATH} not built: run __graft_entry__.build() or make -C cmm-turbulence_amd")
+        L = C.CDLL(LIB_PATH)
+        L.cmm_last_error.restype = C.c_char_p
+        L.cmm_lagrange_coeff.restype = C.c_double
+        L.cmm_lagrange_coeff.argtypes = [DP, C.c_double, C.c_int, C.c_int, C.c_int]
+        L.cmm_advect_coeffs_fill.argtypes = [DP, DP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
+                                             C.POINTER(AdvectCoeffs)]
+        L.cmm_map_init_identity.argtypes = [Grid, VP, VP, VP]
+        L.cmm_map_advect.argtypes = [Grid, Grid, VP, VP, C.POINTER(VP), C.POINTER(AdvectCoeffs), VP, VP, VP]
+        L.cmm_incomp_error.argtypes = [Grid, Grid, VP, VP, DP, VP]
+        L.cmm_compose_positions.argtypes = [Grid, Grid, VP, VP, VP, C.c_int, VP, VP]
+        L.cmm_sample_init.argtypes = [Grid, Grid, VP, VP, VP, C.c_int, C.c_int, DP, C.c_int, Grid, VP, VP, VP]
+        L.cmm_translate_workspace_bytes.restype = C.c_size_t
+        L.cmm_translate_workspace_bytes.argtypes = [Grid]
+        L.cmm_translate_ic.argtypes = [Grid, Grid, VP, VP, VP, C.c_int, C.c_int, DP, C.c_int, Grid, VP, VP, VP,
+                                       C.c_size_t, VP]
+        L.cmm_hermite_workspace_bytes.restype = C.c_size_t
+        L.cmm_hermite_workspace_bytes.argtypes = [Grid]
+        L.cmm_hermite_from_real.argtypes = [Grid, VP, VP, VP, C.c_size_t, VP]
+        L.cmm_hermite_from_spectrum.argtypes = [Grid, VP, VP, VP, C.c_size_t, VP]
+        L.cmm_psi_workspace_bytes.restype = C.c_size_t
+        L.cmm_psi_workspace_bytes.argtypes = [Grid, Grid, C.c_double]
+        L.cmm_psi_from_vort.argtypes = [Grid, Grid, Grid, Grid, VP, VP, VP, C.c_int, C.c_double, VP, VP,
+                                        C.c_size_t, VP]
+        L.cmm_sample_vort.argtypes = [Grid, Grid, Grid, VP, VP, VP, C.c_int, VP, VP]
+        L.cmm_particles_advect.argtypes = [Grid, C.POINTER(VP), C.c_int, DP, C.c_double, C.c_int, C.c_int, VP, VP]
+        L.cmm_euler_create.argtypes = [C.POINTER(EulerParams), C.POINTER(VP)]
+        L.cmm_euler_destroy.argtypes = [VP]
+        L.cmm_euler_stream.restype = VP
+        L.cmm_euler_stream.argtypes = [VP]
+        L.cmm_euler_add_particles.argtypes = [VP, C.c_int, C.c_int, DP, C.POINTER(C.c_int)]
+        L.cmm_euler_init.argtypes = [VP, DP, C.POINTER(C.c_int)]
+        L.cmm_euler_step.argtypes = [VP, C.c_double, C.POINTER(StepInfo)]
+        L.cmm_euler_get_field.argtypes = [VP, C.c_int, DP]
+        L.cmm_euler_get_particles.argtypes = [VP, C.c_int, DP]
+        L.cmm_euler_diagnostics.argtypes = [VP, DP]
+        L.cmm_euler_state.argtypes = [VP, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_int)]
+        L.cmm_euler_profile.argtypes = [VP, C.c_int]
+        L.cmm_euler_profile_read.argtypes = [VP, DP, C.POINTER(C.c_int)]
+        _lib = L
+    return _lib
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise CmmError(f"{what} failed ({rc}): {lib().cmm_last_error().decode()}")
+
+
+def grid(n, ny=None, bounds=(0.0, TWO_PI, 0.0, TWO_PI)):
+    return Grid(n, n if ny is None else ny, bounds[0], bounds[1], bounds[2], bounds[3])
+
+
+def _p(t):
+    """device pointer of a torch tensor (None -> NULL)"""
+    if t is None:
+        return None
+    import torch
+    assert t.is_cuda and t.dtype == torch.float64 and t.is_contiguous(), "expected contiguous float64 cuda tensor"
+    return C.c_void_p(t.data_ptr())
+
+
+def _stream():
+    import torch
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _levels(levels):
+    arr = (VP * 4)()
+    for i, t in enumerate(levels):
+        arr[i] = t.data_ptr()
+    return arr
+
+
+def advect_coeffs(t, dt, loop_ctr, l_order, tin, muo, eps, direction=-1):
+    import numpy as np
+    c = AdvectCoeffs()
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    dt = np.ascontiguousarray(dt, dtype=np.float64)
+    _check(lib().cmm_advect_coeffs_fill(t.ctypes.data_as(DP), dt.ctypes.data_as(DP), loop_ctr, l_order, tin, muo,
+                                        eps, direction, C.byref(c)), "cmm_advect_coeffs_fill")
+    return c
+
+
+def map_init_identity(g, chix, chiy):
+    _check(lib().cmm_map_init_identity(g, _p(chix), _p(chiy), _stream()), "cmm_map_init_identity")
+
+
+def advect_using_stream_hermite(g_map, g_psi, chix, chiy, psi_levels, coeffs, chix_new, chiy_new):
+    _check(lib().cmm_map_advect(g_map, g_psi, _p(chix), _p(chiy), _levels(psi_levels), C.byref(coeffs),
+                                _p(chix_new), _p(chiy_new), _stream()), "cmm_map_advect")
+
+
+def incompressibility_check(g_check, g_map, chix, chiy):
+    out = C.c_double()
+    _check(lib().cmm_incomp_error(g_map, g_check, _p(chix), _p(chiy), C.byref(out), _stream()), "cmm_incomp_error")
+    return out.value
+
+
+def apply_map_stack(g_target, g_map, chix, chiy, stack, n_stack, xy):
+    _check(lib().cmm_compose_positions(g_map, g_target, _p(chix), _p(chiy), _p(stack), n_stack, _p(xy), _stream()),
+           "cmm_compose_positions")
+
+
+def sample_init(g_map, g_target, chix, chiy, stack, n_stack, ic, out, ic_params=None, disc=None, g_disc=None):
+    prm = (C.c_double * 10)(*([0.0] * 10 if ic_params is None else list(ic_params) + [0.0] * (10 - len(ic_params))))
+    gd = g_disc if g_disc is not None else g_target
+    _check(lib().cmm_sample_init(g_map, g_target, _p(chix), _p(chiy), _p(stack), n_stack, ic,
+                                 C.cast(prm, DP), 1 if disc is not None else 0, gd, _p(disc), _p(out), _stream()),
+           "cmm_sample_init")
+
+
+def translate_initial_condition_through_map_stack(g_map, g_fine, chix, chiy, stack, n_stack, ic, w_herm,
+                                                  ic_params=None, disc=None, g_disc=None):
+    prm = (C.c_double * 10)(*([0.0] * 10 if ic_params is None else list(ic_params) + [0.0] * (10 - len(ic_params))))
+    gd = g_disc if g_disc is not None else g_fine
+    _check(lib().cmm_translate_ic(g_map, g_fine, _p(chix), _p(chiy), _p(stack), n_stack, ic, C.cast(prm, DP),
+                                  1 if disc is not None else 0, gd, _p(disc), _p(w_herm), None, 0, _stream()),
+           "cmm_translate_ic")
+
+
+def hermite_from_real(g, real, herm):
+    _check(lib().cmm_hermite_from_real(g, _p(real), _p(herm), None, 0, _stream()), "cmm_hermite_from_real")
+
+
+def hermite_from_spectrum(g, spec_cols, herm):
+    _check(lib().cmm_hermite_from_spectrum(g, _p(spec_cols), _p(herm), None, 0, _stream()),
+           "cmm_hermite_from_spectrum")
+
+
+def evaluate_stream_hermite(g_map, g_vort, g_fine, g_psi, chix, chiy, w_herm, psi_out, molly_stencil=0,
+                            freq_cut_psi=128.0):
+    _check(lib().cmm_psi_from_vort(g_map, g_vort, g_fine, g_psi, _p(chix), _p(chiy), _p(w_herm), molly_stencil,
+                                   freq_cut_psi, _p(psi_out), None, 0, _stream()), "cmm_psi_from_vort")
+
+
+def sample_vort(g_map, g_target, g_fine, chix, chiy, w_herm, out, molly_stencil=0):
+    _check(lib().cmm_sample_vort(g_map, g_target, g_fine, _p(chix), _p(chiy), _p(w_herm), molly_stencil, _p(out),
+                                 _stream()), "cmm_sample_vort")
+
+
+def particles_advect(g_psi, psi_levels, l_order, Lp12, dt, scheme, pos):
+    Lp = (C.c_double * 4)(*(list(Lp12) + [0.0] * (4 - len(Lp12))))
+    _check(lib().cmm_particles_advect(g_psi, _levels(psi_levels), l_order, C.cast(Lp, DP), dt, scheme,
+                                      pos.numel() // 2, _p(pos), _stream()), "cmm_particles_advect")
+
+
+class Euler:
+    """The device Euler time loop (cmm_euler_* in include/cmm.h)."""
+
+    def __init__(self, g_coarse, g_fine, g_psi, g_vort, dt, time_integration="RK3", map_update_order="4th",
+                 l_order=None, lagrange_init_higher_order=True, map_epsilon=1e-3, incomp_threshold=1e-4,
+                 freq_cut_psi=128.0, molly_stencil=0, skip_remapping=False, initial_condition="4_nodes",
+                 ic_params=None, discrete_real=None, g_disc=None, seed=0, stack_capacity=64):
+        import numpy as np
+        tin = TIME_INTEGRATION[time_integration]
+        if l_order is None:
+            l_order = {10: 1, 20: 2, 21: 2, 30: 3, 31: 3, 40: 4, 41: 4}[tin]
+        p = EulerParams()
+        p.g_coarse, p.g_fine, p.g_psi, p.g_vort = g_coarse, g_fine, g_psi, g_vort
+        p.dt = dt
+        p.time_integration_num = tin
+        p.map_update_order_num = MAP_UPDATE_ORDER[map_update_order]
+        p.l_order = l_order
+        p.lagrange_init_higher_order = int(lagrange_init_higher_order)
+        p.map_epsilon, p.incomp_threshold, p.freq_cut_psi = map_epsilon, incomp_threshold, freq_cut_psi
+        p.molly_stencil, p.skip_remapping = molly_stencil, int(skip_remapping)
+        p.ic = IC[initial_condition]
+        for i, v in enumerate(ic_params or []):
+            p.ic_params[i] = v
+        self._keep = []
+        if initial_condition == "random_spectrum" or discrete_real is not None:
+            p.discrete = 1
+            p.g_disc = g_disc if g_disc is not None else g_fine
+            if discrete_real is not None:
+                d = np.ascontiguousarray(discrete_real, dtype=np.float64)
+                self._keep.append(d)
+                p.disc_real = d.ctypes.data_as(DP)
+        p.seed = seed
+        p.stack_capacity = stack_capacity
+        self.params = p
+        self.l_order = l_order
+        h = C.c_void_p()
+        _check(lib().cmm_euler_create(C.byref(p), C.byref(h)), "cmm_euler_create")
+        self.h = h
+        self.sets = []
+
+    def close(self):
+        if self.h:
+            lib().cmm_euler_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_particles(self, pos, scheme=30):
+        import numpy as np
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        s = C.c_int()
+        _check(lib().cmm_euler_add_particles(self.h, pos.size // 2, scheme, pos.ctypes.data_as(DP), C.byref(s)),
+               "cmm_euler_add_particles")
+        self.sets.append(pos.size // 2)
+        return s.value
+
+    def init(self):
+        w = (C.c_double * 8)()
+        n = C.c_int()
+        _check(lib().cmm_euler_init(self.h, C.cast(w, DP), C.byref(n)), "cmm_euler_init")
+        return [w[i] for i in range(n.value)]
+
+    def step(self, dt_now):
+        info = StepInfo()
+        _check(lib().cmm_euler_step(self.h, dt_now, C.byref(info)), "cmm_euler_step")
+        return info
+
+    def field(self, name):
+        import numpy as np
+        p = self.params
+        size = {"chix": 4 * p.g_coarse.n, "chiy": 4 * p.g_coarse.n, "psi": 4 * p.g_psi.n * self.l_order,
+                "w_fine": 4 * p.g_fine.n, "w_coarse": p.g_coarse.n, "disc": 4 * p.g_disc.n}[name]
+        out = np.zeros(size)
+        _check(lib().cmm_euler_get_field(self.h, FIELD[name], out.ctypes.data_as(DP)), "cmm_euler_get_field")
+        return out
+
+    def particles(self, s):
+        import numpy as np
+        out = np.zeros(2 * self.sets[s])
+        _check(lib().cmm_euler_get_particles(self.h, s, out.ctypes.data_as(DP)), "cmm_euler_get_particles")
+        return out
+
+    def diagnostics(self):
+        """energy, enstrophy, palinstrophy, max |omega| (as the reference's Mesure files)"""
+        out = (C.c_double * 4)()
+        _check(lib().cmm_euler_diagnostics(self.h, C.cast(out, DP)), "cmm_euler_diagnostics")
+        return list(out)
+
+    def state(self):
+        st, t, ns = C.c_int(), C.c_double(), C.c_int()
+        _check(lib().cmm_euler_state(self.h, C.byref(st), C.byref(t), C.byref(ns)), "cmm_euler_state")
+        return st.value, t.value, ns.value
+
+    def profile(self, enable=True):
+        _check(lib().cmm_euler_profile(self.h, int(enable)), "cmm_euler_profile")
+
+    def profile_read(self):
+        ms = (C.c_double * PROF_SLOTS)()
+        cnt = (C.c_int * PROF_SLOTS)()
+        _check(lib().cmm_euler_profile_read(self.h, C.cast(ms, DP), C.cast(cnt, C.POINTER(C.c_int))),
+               "cmm_euler_profile_read")
+        names = PROF_NAMES
+        return {n: (ms[i], cnt[i]) for i, n in enumerate(names)}
+
+
+def device_info():
+    buf = C.create_string_buffer(512)
+    _check(lib().cmm_device_info(buf, 512), "cmm_device_info")
+    return buf.value.decode()
+
+
+def _mix64(z):
+    """SplitMix64 finaliser (Steele, Lea, Flood 2014) on numpy uint64 arrays."""
+    import numpy as np
+    with np.errstate(over="ignore"):
+        z = (z + np.uint64(0x9E3779B97F4A7C15)).astype(np.uint64)
+        z = ((z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)).astype(np.uint64)
+        z = ((z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)).astype(np.uint64)
+        return z ^ (z >> np.uint64(31))
+
+
+def uniform_particles(n, seed, center=(TWO_PI / 2, TWO_PI / 2), width=(TWO_PI, TWO_PI), bounds=(0.0, TWO_PI, 0.0, TWO_PI)):
+    """Positions of the 'uniform' particle initial condition: u in [0,1) from a counter-based
+    SplitMix64 stream (u_j = mix64(mix64(seed) + j) >> 11 / 2^53, j = 2i for x and 2i+1 for y) in
+    place of the reference's cuRAND generator, then the reference's rescale and periodic wrap
+    (Cuda-CMM/src/numerical/cmm-particles.cu:30-46)."""
+    import numpy as np
+    j = np.arange(2 * n, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = _mix64((_mix64(np.array([seed], dtype=np.uint64))[0] + j).astype(np.uint64))
+    u = (z >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+    pos = np.empty(2 * n)
+    lx, ly = bounds[1] - bounds[0], bounds[3] - bounds[2]
+    px = center[0] + width[0] * (u[0::2] - 0.5)
+    py = center[1] + width[1] * (u[1::2] - 0.5)
+    pos[0::2] = px - np.floor((px - bounds[0]) / lx) * lx
+    pos[1::2] = py - np.floor((py - bounds[2]) / ly) * ly
+    return pos

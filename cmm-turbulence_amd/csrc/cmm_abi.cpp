@@ -1,0 +1,665 @@
+// C-ABI entry points (include/cmm.h) and the Euler engine that runs the reference's time loop on
+// the device.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "cmm_internal.hpp"
+#include "../../include/cmm.h"
+
+using namespace cmm;
+
+namespace {
+
+hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+bool grid_ok(const cmm_grid &g) { return g.nx > 0 && g.ny > 0 && g.x1 > g.x0 && g.y1 > g.y0; }
+
+PsiLevels levels(const double *const *p, int L) {
+    PsiLevels P;
+    for (int l = 0; l < 4; ++l) P.p[l] = p[l < L ? l : 0];
+    return P;
+}
+
+// SplitMix64 finaliser (Steele, Lea, Flood 2014)
+uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+}  // namespace
+
+// ================================================================================================
+extern "C" {
+
+int cmm_version(void) { return 1; }
+
+const char *cmm_last_error(void) { return cmm::last_error(); }
+
+int cmm_device_info(char *buf, int len) {
+    int dev;
+    if (int rc = hip_check(hipGetDevice(&dev), "hipGetDevice")) return rc;
+    hipDeviceProp_t p;
+    if (int rc = hip_check(hipGetDeviceProperties(&p, dev), "hipGetDeviceProperties")) return rc;
+    std::snprintf(buf, len, "%s, %s, CUs %d, LDS/block %zu, HBM %zu", p.name, p.gcnArchName, p.multiProcessorCount,
+                  (size_t)p.sharedMemPerBlock, (size_t)p.totalGlobalMem);
+    return CMM_OK;
+}
+
+double cmm_lagrange_coeff(const double *t, double t_next, int loop_ctr, int i_point, int l_order) {
+    double coeff = 1;
+    for (int i_p = 0; i_p < l_order; ++i_p)
+        if (i_p != i_point) coeff *= (t_next - t[loop_ctr - i_p]) / (t[loop_ctr - i_point] - t[loop_ctr - i_p]);
+    return coeff;
+}
+
+int cmm_advect_coeffs_fill(const double *t, const double *dt, int loop_ctr, int l_order, int tin, int muo, double eps,
+                           int direction, cmm_advect_coeffs *c) {
+    if (!t || !dt || !c || l_order < 1 || l_order > 4 || muo < 0 || muo > 2 || !(eps > 0)) {
+        set_error("cmm_advect_coeffs_fill: bad argument");
+        return CMM_ERR_ARG;
+    }
+    std::memset(c, 0, sizeof(*c));
+    const int lc = loop_ctr + l_order - 1;
+    for (int i = 0; i < l_order; ++i) {
+        c->L1[i] = cmm_lagrange_coeff(t, t[lc + 1], lc, i, l_order);
+        c->L12[i] = cmm_lagrange_coeff(t, t[lc] + dt[lc + 1] / 2.0, lc, i, l_order);
+    }
+    // stencil weights of the 2nd/4th/6th order map update with signs and 1/eps folded in,
+    // Cuda-CMM/src/simulation/cmm-simulation-host.cu:75-90
+    double h[3] = {0, 0, 0};
+    if (muo == 2) { h[0] = 3.0 / 8.0; h[1] = -3.0 / 20.0; h[2] = 1.0 / 40.0; }
+    else if (muo == 1) { h[0] = 1.0 / 3.0; h[1] = -1.0 / 12.0; }
+    else { h[0] = 1.0 / 4.0; }
+    for (int f = 0; f < 4 + 4 * muo; ++f) {
+        const int ring = f / 4;
+        c->c1[f] = h[ring];
+        c->cx[f] = h[ring] * (1 - 2 * ((f / 2) % 2)) / eps / double(ring + 1);
+        c->cy[f] = h[ring] * (1 - 2 * (((f + 1) / 2) % 2)) / eps / double(ring + 1);
+        c->cxy[f] = h[ring] * (1 - 2 * (f % 2)) / eps / eps / double(ring + 1) / double(ring + 1);
+    }
+    c->dt = dt[lc + 1];
+    c->eps = eps;
+    c->time_integration_num = tin;
+    c->map_update_order_num = muo;
+    c->l_order = l_order;
+    c->direction = direction;
+    return CMM_OK;
+}
+
+int cmm_map_init_identity(cmm_grid g, double *cx, double *cy, void *stream) {
+    if (!grid_ok(g) || !cx || !cy) { set_error("cmm_map_init_identity: bad argument"); return CMM_ERR_ARG; }
+    return op_map_identity(g, cx, cy, S(stream));
+}
+
+int cmm_map_advect(cmm_grid gm, cmm_grid gp, const double *cx, const double *cy, const double *const *psi,
+                   const cmm_advect_coeffs *c, double *ncx, double *ncy, void *stream) {
+    if (!grid_ok(gm) || !grid_ok(gp) || !cx || !cy || !psi || !c || !ncx || !ncy) {
+        set_error("cmm_map_advect: bad argument");
+        return CMM_ERR_ARG;
+    }
+    if (c->direction != -1) { set_error("cmm_map_advect: only the backward map is supported"); return CMM_ERR_UNSUPPORTED; }
+    if (c->l_order < 1 || c->l_order > 4 || c->map_update_order_num < 0 || c->map_update_order_num > 2) {
+        set_error("cmm_map_advect: bad coefficients");
+        return CMM_ERR_ARG;
+    }
+    if (ncx == cx || ncy == cy || ncx == cy || ncy == cx) { set_error("cmm_map_advect: output aliases input"); return CMM_ERR_ARG; }
+    for (int l = 0; l < c->l_order; ++l)
+        if (!psi[l]) { set_error("cmm_map_advect: null psi level"); return CMM_ERR_ARG; }
+    return op_map_advect(gm, gp, cx, cy, levels(psi, c->l_order), *c, ncx, ncy, S(stream));
+}
+
+int cmm_incomp_error(cmm_grid gm, cmm_grid gc, const double *cx, const double *cy, double *out, void *stream) {
+    if (!grid_ok(gm) || !grid_ok(gc) || !cx || !cy || !out) { set_error("cmm_incomp_error: bad argument"); return CMM_ERR_ARG; }
+    return op_incomp(gm, gc, cx, cy, out, S(stream));
+}
+
+int cmm_compose_positions(cmm_grid gm, cmm_grid gt, const double *cx, const double *cy, const double *stack,
+                          int ns, double *xy, void *stream) {
+    if (!grid_ok(gm) || !grid_ok(gt) || !cx || !cy || !xy || ns < 0 || (ns > 0 && !stack)) {
+        set_error("cmm_compose_positions: bad argument");
+        return CMM_ERR_ARG;
+    }
+    return op_compose_positions(gm, gt, cx, cy, stack, ns, xy, S(stream));
+}
+
+int cmm_sample_init(cmm_grid gm, cmm_grid gt, const double *cx, const double *cy, const double *stack, int ns, int ic,
+                    const double *prm, int discrete, cmm_grid gd, const double *disc, double *out, void *stream) {
+    if (!grid_ok(gm) || !grid_ok(gt) || !cx || !cy || !out || ns < 0 || (ns > 0 && !stack) ||
+        (discrete && (!disc || !grid_ok(gd)))) {
+        set_error("cmm_sample_init: bad argument");
+        return CMM_ERR_ARG;
+    }
+    return op_sample_init(gm, gt, cx, cy, stack, ns, ic, prm, discrete, discrete ? gd : gt, disc, out, S(stream));
+}
+
+size_t cmm_translate_workspace_bytes(cmm_grid g) { return op_hermite_workspace(g); }
+
+int cmm_translate_ic(cmm_grid gm, cmm_grid gf, const double *cx, const double *cy, const double *stack, int ns, int ic,
+                     const double *prm, int discrete, cmm_grid gd, const double *disc, double *W, void *work,
+                     size_t wb, void *stream) {
+    if (!grid_ok(gm) || !grid_ok(gf) || !cx || !cy || !W || ns < 0 || (ns > 0 && !stack) ||
+        (discrete && (!disc || !grid_ok(gd)))) {
+        set_error("cmm_translate_ic: bad argument");
+        return CMM_ERR_ARG;
+    }
+    return op_translate_ic(gm, gf, cx, cy, stack, ns, ic, prm, discrete, discrete ? gd : gf, disc, W, work, wb, S(stream));
+}
+
+size_t cmm_hermite_workspace_bytes(cmm_grid g) { return op_hermite_workspace(g); }
+
+int cmm_hermite_from_real(cmm_grid g, const double *real, double *herm, void *work, size_t wb, void *stream) {
+    if (!real || !herm) { set_error("cmm_hermite_from_real: null pointer"); return CMM_ERR_ARG; }
+    return op_hermite_from_real(g, real, herm, work, wb, S(stream));
+}
+
+int cmm_hermite_from_spectrum(cmm_grid g, const double *spec, double *herm, void *work, size_t wb, void *stream) {
+    if (!spec || !herm) { set_error("cmm_hermite_from_spectrum: null pointer"); return CMM_ERR_ARG; }
+    return op_hermite_from_spectrum(g, spec, herm, work, wb, S(stream));
+}
+
+size_t cmm_psi_workspace_bytes(cmm_grid gv, cmm_grid gp, double fcut) { return op_psi_workspace(gv, gp, fcut); }
+
+int cmm_psi_from_vort(cmm_grid gm, cmm_grid gv, cmm_grid gf, cmm_grid gp, const double *cx, const double *cy,
+                      const double *W, int molly, double fcut, double *psi, void *work, size_t wb, void *stream) {
+    if (!grid_ok(gm) || !grid_ok(gf) || !cx || !cy || !W || !psi || (molly != 0 && molly != 4 && molly != 8)) {
+        set_error("cmm_psi_from_vort: bad argument");
+        return CMM_ERR_ARG;
+    }
+    return op_psi_from_vort(gm, gv, gf, gp, cx, cy, W, molly, fcut, psi, work, wb, S(stream));
+}
+
+int cmm_sample_vort(cmm_grid gm, cmm_grid gt, cmm_grid gf, const double *cx, const double *cy, const double *W,
+                    int molly, double *out, void *stream) {
+    if (!grid_ok(gm) || !grid_ok(gt) || !grid_ok(gf) || !cx || !cy || !W || !out) {
+        set_error("cmm_sample_vort: bad argument");
+        return CMM_ERR_ARG;
+    }
+    return op_sample_vort(gm, gt, gf, cx, cy, W, molly, out, S(stream));
+}
+
+int cmm_particles_advect(cmm_grid gp, const double *const *psi, int L, const double *Lp, double dt, int scheme, int n,
+                         double *pos, void *stream) {
+    if (!grid_ok(gp) || !psi || L < 1 || L > 4 || n < 0 || (n > 0 && !pos)) {
+        set_error("cmm_particles_advect: bad argument");
+        return CMM_ERR_ARG;
+    }
+    for (int l = 0; l < L; ++l)
+        if (!psi[l]) { set_error("cmm_particles_advect: null psi level"); return CMM_ERR_ARG; }
+    if (L < 2 && scheme != 10 && scheme != 20) {
+        set_error("cmm_particles_advect: scheme needs at least two psi levels");
+        return CMM_ERR_ARG;
+    }
+    return op_particles(gp, levels(psi, L), L, Lp, dt, scheme, n, pos, S(stream));
+}
+
+}  // extern "C"
+
+// ================================================================================================
+// Engine
+
+struct cmm_euler {
+    cmm_euler_params p;
+    hipStream_t st = nullptr;
+    long nc = 0, nf = 0, np = 0, nd = 0;
+    double *cx = nullptr, *cy = nullptr, *ncx = nullptr, *ncy = nullptr;
+    double *psi_buf[4] = {nullptr, nullptr, nullptr, nullptr};
+    double *lvl[4] = {nullptr, nullptr, nullptr, nullptr};  // level -> buffer, newest first
+    double *W = nullptr, *disc = nullptr, *stack = nullptr, *tmp = nullptr, *tmp4 = nullptr;
+    int nstack = 0;
+    void *work = nullptr;
+    size_t work_bytes = 0;
+    unsigned long long *slot = nullptr;
+    std::vector<double *> part;
+    std::vector<int> part_n, part_scheme;
+    std::vector<double> tv, dtv;
+    int loop = 0;
+    bool initialised = false;
+    // per-kernel event timing: [advect, incomp, psi_row, psi_col, psi_c2r, particles, remap]
+    bool prof = false;
+    hipEvent_t ev[12] = {};
+    double ms[CMM_PROF_SLOTS] = {};
+    int cnt[CMM_PROF_SLOTS] = {};
+};
+
+namespace {
+
+int dalloc(double **p, size_t n) {
+    if (hipMalloc(p, sizeof(double) * n) != hipSuccess) {
+        *p = nullptr;
+        set_error("hipMalloc failed (" + std::to_string(sizeof(double) * n) + " bytes)");
+        return CMM_ERR_ALLOC;
+    }
+    return CMM_OK;
+}
+
+
+int engine_translate(cmm_euler *e) {
+    return op_translate_ic(e->p.g_coarse, e->p.g_fine, e->cx, e->cy, e->stack, e->nstack, e->p.ic, e->p.ic_params,
+                           e->p.discrete, e->p.discrete ? e->p.g_disc : e->p.g_fine, e->disc, e->W, e->work,
+                           e->work_bytes, e->st);
+}
+
+int engine_psi(cmm_euler *e, double *out, hipEvent_t *ev = nullptr) {
+    return op_psi_from_vort(e->p.g_coarse, e->p.g_vort, e->p.g_fine, e->p.g_psi, e->cx, e->cy, e->W,
+                            e->p.molly_stencil, e->p.freq_cut_psi, out, e->work, e->work_bytes, e->st, ev);
+}
+
+int engine_advect(cmm_euler *e, const double *t, const double *dt, int loop_ctr, int L, int tin) {
+    cmm_advect_coeffs c;
+    if (int rc = cmm_advect_coeffs_fill(t, dt, loop_ctr, L, tin, e->p.map_update_order_num, e->p.map_epsilon, -1, &c))
+        return rc;
+    PsiLevels P;
+    for (int l = 0; l < 4; ++l) P.p[l] = e->lvl[l < L ? l : 0];
+    if (int rc = op_map_advect(e->p.g_coarse, e->p.g_psi, e->cx, e->cy, P, c, e->ncx, e->ncy, e->st)) return rc;
+    std::swap(e->cx, e->ncx);
+    std::swap(e->cy, e->ncy);
+    return CMM_OK;
+}
+
+int engine_incomp(cmm_euler *e, double *out) {
+    if (int rc = op_incomp_launch(e->p.g_coarse, e->p.g_fine, e->cx, e->cy, e->slot, e->st)) return rc;
+    unsigned long long h = 0;
+    if (int rc = hip_check(hipMemcpyAsync(&h, e->slot, sizeof(h), hipMemcpyDeviceToHost, e->st), "hipMemcpyAsync")) return rc;
+    if (int rc = hip_check(hipStreamSynchronize(e->st), "hipStreamSynchronize")) return rc;
+    std::memcpy(out, &h, sizeof(double));
+    return CMM_OK;
+}
+
+int copy_level(cmm_euler *e, int dst, int src) {
+    return hip_check(hipMemcpyAsync(e->lvl[dst], e->lvl[src], sizeof(double) * 4 * e->np, hipMemcpyDeviceToDevice, e->st),
+                     "hipMemcpyAsync psi");
+}
+
+void prof_mark(cmm_euler *e, int idx) {
+    if (e->prof) (void)hipEventRecord(e->ev[idx], e->st);
+}
+
+void prof_acc(cmm_euler *e, int phase, int a, int b) {
+    if (!e->prof) return;
+    float m = 0;
+    if (hipEventSynchronize(e->ev[b]) == hipSuccess && hipEventElapsedTime(&m, e->ev[a], e->ev[b]) == hipSuccess) {
+        e->ms[phase] += m;
+        e->cnt[phase] += 1;
+    }
+}
+
+// Random-spectrum field (no reference counterpart), as a column-major half spectrum:
+// omega_hat = A sqrt(k E(k)/pi) exp(i theta), E(k) = k^4 exp(-2 (k/6)^2), theta = 2 pi U(seed,kx,ky),
+// theta(0,-ky) = -theta(0,ky), |kx|,|ky| <= min(64, n/2-1), A such that 0.5 h^2 sum omega^2 = 1.
+void random_spectrum_columns(int n, uint64_t seed, std::vector<std::complex<double>> &cols) {
+    const int nh = n / 2 + 1;
+    const int km = std::min(64, n / 2 - 1);
+    const double pi = kPi;
+    cols.assign((size_t)nh * n, std::complex<double>(0, 0));
+    double s2 = 0;
+    for (int kx = 0; kx <= km; ++kx)
+        for (int ky = -km; ky <= km; ++ky) {
+            if (kx == 0 && ky <= 0) continue;
+            double k = std::sqrt((double)kx * kx + (double)ky * ky);
+            double E = k * k * k * k * std::exp(-2.0 * (k / 6.0) * (k / 6.0));
+            double a = std::sqrt(k * E / pi);
+            uint64_t key = ((uint64_t)(kx + 32768) << 32) | (uint64_t)(ky + 32768);
+            double u = (double)(mix64(mix64(seed) + key) >> 11) * (1.0 / 9007199254740992.0);
+            double th = 2.0 * pi * u;
+            std::complex<double> v(a * std::cos(th), a * std::sin(th));
+            int row = ky < 0 ? ky + n : ky;
+            cols[(size_t)kx * n + row] = v;
+            if (kx == 0) cols[(size_t)(n - ky)] = std::conj(v);
+            s2 += 2.0 * std::norm(v);
+        }
+    const double A = 1.0 / std::sqrt(2.0 * pi * pi * s2);
+    for (auto &v : cols) v *= A;
+}
+
+int engine_setup_discrete(cmm_euler *e) {
+    const cmm_grid &gd = e->p.g_disc;
+    e->nd = (long)gd.nx * gd.ny;
+    if (int rc = dalloc(&e->disc, 4 * e->nd)) return rc;
+    double *real = nullptr;
+    if (int rc = dalloc(&real, e->nd)) return rc;
+    int rc = CMM_OK;
+    if (e->p.ic == CMM_IC_RANDOM_SPECTRUM) {
+        std::vector<std::complex<double>> cols;
+        random_spectrum_columns(gd.nx, e->p.seed, cols);
+        double *dspec = nullptr;
+        rc = dalloc(&dspec, 2 * cols.size());
+        if (!rc) rc = hip_check(hipMemcpyAsync(dspec, cols.data(), sizeof(double) * 2 * cols.size(), hipMemcpyHostToDevice, e->st), "upload spectrum");
+        // real field = C2R of the spectrum (plane 0 of its Hermite form)
+        if (!rc) rc = op_hermite_from_spectrum(gd, dspec, e->disc, e->work, e->work_bytes, e->st);
+        if (!rc) rc = hip_check(hipMemcpyAsync(real, e->disc, sizeof(double) * e->nd, hipMemcpyDeviceToDevice, e->st), "copy field");
+        if (!rc) rc = hip_check(hipStreamSynchronize(e->st), "sync");
+        if (dspec) (void)hipFree(dspec);
+    } else {
+        if (!e->p.disc_real) { set_error("discrete initial condition without data"); rc = CMM_ERR_ARG; }
+        if (!rc) rc = hip_check(hipMemcpyAsync(real, e->p.disc_real, sizeof(double) * e->nd, hipMemcpyHostToDevice, e->st), "upload field");
+    }
+    // Hermite form of the discrete field, Cuda-CMM/src/simulation/cmm-euler2d.cu:325-344
+    if (!rc) rc = op_hermite_from_real(gd, real, e->disc, e->work, e->work_bytes, e->st);
+    if (!rc) rc = hip_check(hipStreamSynchronize(e->st), "sync");
+    (void)hipFree(real);
+    return rc;
+}
+
+bool pow2(int n) { return n >= 8 && n <= 4096 && (n & (n - 1)) == 0; }
+
+}  // namespace
+
+extern "C" {
+
+int cmm_euler_create(const cmm_euler_params *p, cmm_euler **out) {
+    if (!p || !out) { set_error("cmm_euler_create: null argument"); return CMM_ERR_ARG; }
+    *out = nullptr;
+    if (!grid_ok(p->g_coarse) || !grid_ok(p->g_fine) || !grid_ok(p->g_psi) || !grid_ok(p->g_vort) ||
+        !pow2(p->g_fine.nx) || !pow2(p->g_fine.ny) || !pow2(p->g_psi.nx) || !pow2(p->g_vort.nx) ||
+        p->g_psi.nx != p->g_psi.ny || p->g_vort.nx != p->g_vort.ny) {
+        set_error("cmm_euler_create: grids must be square powers of two between 8 and 4096 (fine, psi, vort)");
+        return CMM_ERR_ARG;
+    }
+    if (p->l_order < 1 || p->l_order > 4 || !(p->dt > 0) || p->stack_capacity < 0 || p->map_update_order_num < 0 ||
+        p->map_update_order_num > 2) {
+        set_error("cmm_euler_create: bad time-stepping parameters");
+        return CMM_ERR_ARG;
+    }
+    if (p->discrete && (!grid_ok(p->g_disc) || !pow2(p->g_disc.nx) || p->g_disc.nx != p->g_disc.ny)) {
+        set_error("cmm_euler_create: discrete grid must be a square power of two");
+        return CMM_ERR_ARG;
+    }
+    cmm_euler *e = new cmm_euler();
+    e->p = *p;
+    e->nc = (long)p->g_coarse.nx * p->g_coarse.ny;
+    e->nf = (long)p->g_fine.nx * p->g_fine.ny;
+    e->np = (long)p->g_psi.nx * p->g_psi.ny;
+    int rc = hip_check(hipStreamCreateWithFlags(&e->st, hipStreamNonBlocking), "hipStreamCreate");
+    if (!rc) rc = dalloc(&e->cx, 4 * e->nc);
+    if (!rc) rc = dalloc(&e->cy, 4 * e->nc);
+    if (!rc) rc = dalloc(&e->ncx, 4 * e->nc);
+    if (!rc) rc = dalloc(&e->ncy, 4 * e->nc);
+    for (int l = 0; l < p->l_order && !rc; ++l) {
+        rc = dalloc(&e->psi_buf[l], 4 * e->np);
+        e->lvl[l] = e->psi_buf[l];
+    }
+    if (!rc) rc = dalloc(&e->W, 4 * e->nf);
+    if (!rc && p->stack_capacity > 0) rc = dalloc(&e->stack, (size_t)8 * e->nc * p->stack_capacity);
+    if (!rc) {
+        size_t w = std::max(op_psi_workspace(p->g_vort, p->g_psi, p->freq_cut_psi), op_hermite_workspace(p->g_fine));
+        if (p->discrete) w = std::max(w, op_hermite_workspace(p->g_disc));
+        e->work_bytes = w;
+        if (hipMalloc(&e->work, w) != hipSuccess) { set_error("hipMalloc workspace"); rc = CMM_ERR_ALLOC; }
+    }
+    if (!rc && hipMalloc(&e->slot, 256) != hipSuccess) { set_error("hipMalloc slot"); rc = CMM_ERR_ALLOC; }
+    for (int i = 0; i < 12 && !rc; ++i) rc = hip_check(hipEventCreate(&e->ev[i]), "hipEventCreate");
+    if (!rc && p->discrete) rc = engine_setup_discrete(e);
+    if (rc) {
+        cmm_euler_destroy(e);
+        return rc;
+    }
+    *out = e;
+    return CMM_OK;
+}
+
+int cmm_euler_destroy(cmm_euler *e) {
+    if (!e) return CMM_OK;
+    if (e->st) (void)hipStreamSynchronize(e->st);
+    double *bufs[] = {e->cx, e->cy, e->ncx, e->ncy, e->W, e->disc, e->stack, e->tmp, e->tmp4,
+                      e->psi_buf[0], e->psi_buf[1], e->psi_buf[2], e->psi_buf[3]};
+    for (double *b : bufs)
+        if (b) (void)hipFree(b);
+    for (double *b : e->part)
+        if (b) (void)hipFree(b);
+    if (e->work) (void)hipFree(e->work);
+    if (e->slot) (void)hipFree(e->slot);
+    for (int i = 0; i < 12; ++i)
+        if (e->ev[i]) (void)hipEventDestroy(e->ev[i]);
+    if (e->st) (void)hipStreamDestroy(e->st);
+    delete e;
+    return CMM_OK;
+}
+
+void *cmm_euler_stream(cmm_euler *e) { return e ? (void *)e->st : nullptr; }
+
+int cmm_euler_add_particles(cmm_euler *e, int n, int scheme, const double *pos, int *set) {
+    if (!e || n < 0 || (n > 0 && !pos)) { set_error("cmm_euler_add_particles: bad argument"); return CMM_ERR_ARG; }
+    if (scheme != 10 && scheme != 20 && scheme != 30 && scheme != 31 && scheme != 40 && scheme != 41) {
+        set_error("cmm_euler_add_particles: unknown scheme");
+        return CMM_ERR_ARG;
+    }
+    double *d = nullptr;
+    if (n > 0) {
+        if (int rc = dalloc(&d, 2 * (size_t)n)) return rc;
+        if (int rc = hip_check(hipMemcpy(d, pos, sizeof(double) * 2 * n, hipMemcpyHostToDevice), "upload particles")) return rc;
+    }
+    e->part.push_back(d);
+    e->part_n.push_back(n);
+    e->part_scheme.push_back(scheme);
+    if (set) *set = (int)e->part.size() - 1;
+    return CMM_OK;
+}
+
+int cmm_euler_init(cmm_euler *e, double *warm, int *nwarm) {
+    if (!e) { set_error("cmm_euler_init: null engine"); return CMM_ERR_ARG; }
+    const cmm_euler_params &p = e->p;
+    const int L = p.l_order;
+    int nw = 0;
+    // Cuda-CMM/src/simulation/cmm-euler2d.cu:495-512
+    if (int rc = op_map_identity(p.g_coarse, e->cx, e->cy, e->st)) return rc;
+    e->nstack = 0;
+    if (int rc = engine_translate(e)) return rc;
+    if (int rc = engine_psi(e, e->lvl[0])) return rc;
+    // Lagrange warm-up, Cuda-CMM/src/simulation/cmm-euler2d.cu:520-619
+    if (L > 1) {
+        const double dtn = p.dt * std::pow(-1.0, L);
+        double t_init[5] = {0, dtn, 2 * dtn, 3 * dtn, 4 * dtn};
+        double dt_init[5] = {dtn, dtn, dtn, dtn, dtn};
+        for (int order = 1; order <= L; ++order) {
+            const int nsteps = order - (order == L);
+            if (!p.lagrange_init_higher_order && order != L) {
+                for (int i = 0; i < 5; ++i) { dt_init[i] = -dt_init[i]; t_init[i] = -t_init[i]; }
+                continue;
+            }
+            int tin = 10, lo = 1;
+            if (p.lagrange_init_higher_order) {
+                lo = order;
+                tin = order == 1 ? 10 : order == 2 ? 21 : order == 3 ? 30 : 40;
+            }
+            for (int s = 1; s <= nsteps; ++s) {
+                if (int rc = engine_advect(e, t_init, dt_init, 0, lo, tin)) return rc;
+                double einc;
+                if (int rc = engine_incomp(e, &einc)) return rc;
+                if (nw < 8 && warm) warm[nw] = einc;
+                nw++;
+                for (int il = 1; il <= nsteps; ++il)
+                    if (int rc = copy_level(e, nsteps - il + 1, nsteps - il)) return rc;
+                if (int rc = engine_psi(e, e->lvl[0])) return rc;
+            }
+            for (int i = 0; i < 5; ++i) { dt_init[i] = -dt_init[i]; t_init[i] = -t_init[i]; }
+            std::swap(e->lvl[0], e->lvl[nsteps]);
+            if (nsteps >= 3) std::swap(e->lvl[1], e->lvl[2]);
+            if (int rc = op_map_identity(p.g_coarse, e->cx, e->cy, e->st)) return rc;
+        }
+    }
+    if (nwarm) *nwarm = std::min(nw, 8);
+    // time vectors, Cuda-CMM/src/simulation/cmm-euler2d.cu:709-715
+    e->tv.assign(L, 0.0);
+    e->dtv.assign(L, p.dt);
+    for (int i = 0; i < L; ++i) e->tv[i] = -(L - 1 - i) * p.dt;
+    e->loop = 0;
+    e->initialised = true;
+    return hip_check(hipStreamSynchronize(e->st), "hipStreamSynchronize");
+}
+
+int cmm_euler_step(cmm_euler *e, double dt_now, cmm_step_info *info) {
+    if (!e || !e->initialised) { set_error("cmm_euler_step: engine not initialised"); return CMM_ERR_ARG; }
+    if (!(dt_now > 0)) { set_error("cmm_euler_step: dt must be positive"); return CMM_ERR_ARG; }
+    const cmm_euler_params &p = e->p;
+    const int L = p.l_order;
+    const int lc = e->loop + L - 1;
+    e->tv.resize(lc + 2);
+    e->dtv.resize(lc + 2);
+    e->tv[lc + 1] = e->tv[lc] + dt_now;
+    e->dtv[lc + 1] = dt_now;
+    // advection, Cuda-CMM/src/simulation/cmm-euler2d.cu:774-778
+    prof_mark(e, 0);
+    if (int rc = engine_advect(e, e->tv.data(), e->dtv.data(), e->loop, L, p.time_integration_num)) return rc;
+    prof_mark(e, 1);
+    // incompressibility on the fine grid, :796
+    double einc;
+    if (int rc = engine_incomp(e, &einc)) return rc;
+    prof_mark(e, 2);
+    prof_acc(e, 0, 0, 1);
+    prof_acc(e, 1, 1, 2);
+    int remapped = 0, saturated = 0;
+    // remap, :814-853 (translate before push; the map is reset afterwards)
+    if (einc > p.incomp_threshold && !p.skip_remapping) {
+        if (e->nstack >= p.stack_capacity) {
+            saturated = 1;
+        } else {
+            prof_mark(e, 3);
+            if (int rc = engine_translate(e)) return rc;
+            double *slotx = e->stack + (size_t)8 * e->nc * e->nstack;
+            if (int rc = hip_check(hipMemcpyAsync(slotx, e->cx, sizeof(double) * 4 * e->nc, hipMemcpyDeviceToDevice, e->st), "push map")) return rc;
+            if (int rc = hip_check(hipMemcpyAsync(slotx + 4 * e->nc, e->cy, sizeof(double) * 4 * e->nc, hipMemcpyDeviceToDevice, e->st), "push map")) return rc;
+            e->nstack++;
+            if (int rc = op_map_identity(p.g_coarse, e->cx, e->cy, e->st)) return rc;
+            prof_mark(e, 4);
+            prof_acc(e, 6, 3, 4);
+            remapped = 1;
+        }
+    }
+    // psi history: the oldest level becomes the newest (same contents as the reference's
+    // copies at :865-868), then the new psi, :871
+    double *oldest = e->lvl[L - 1];
+    for (int l = L - 1; l > 0; --l) e->lvl[l] = e->lvl[l - 1];
+    e->lvl[0] = oldest;
+    if (int rc = engine_psi(e, e->lvl[0], e->prof ? &e->ev[5] : nullptr)) return rc;
+    // particles, :881 with Cuda-CMM/src/numerical/cmm-particles.cu:63-87
+    if (!e->part.empty()) {
+        double Lp[4] = {0, 0, 0, 0};
+        for (int i = 0; i < L; ++i) Lp[i] = cmm_lagrange_coeff(e->tv.data(), e->tv[lc] + e->dtv[lc + 1] / 2.0, lc + 1, i, L);
+        PsiLevels P;
+        for (int l = 0; l < 4; ++l) P.p[l] = e->lvl[l < L ? l : 0];
+        for (size_t s = 0; s < e->part.size(); ++s) {
+            if (e->part_n[s] <= 0) continue;
+            if (L < 2 && e->part_scheme[s] != 10 && e->part_scheme[s] != 20) continue;
+            if (int rc = op_particles(p.g_psi, P, L, Lp, e->dtv[lc + 1], e->part_scheme[s], e->part_n[s], e->part[s], e->st)) return rc;
+        }
+    }
+    prof_mark(e, 9);
+    prof_acc(e, 2, 5, 6);
+    prof_acc(e, 3, 6, 7);
+    prof_acc(e, 4, 7, 8);
+    prof_acc(e, 5, 8, 9);
+    e->loop++;
+    if (info) {
+        info->t = e->tv[lc + 1];
+        info->incomp = einc;
+        info->remapped = remapped;
+        info->stack_size = e->nstack;
+        info->saturated = saturated;
+    }
+    return CMM_OK;
+}
+
+int cmm_euler_get_field(cmm_euler *e, int field, double *out) {
+    if (!e || !out) { set_error("cmm_euler_get_field: bad argument"); return CMM_ERR_ARG; }
+    int rc = CMM_OK;
+    switch (field) {
+    case CMM_FIELD_CHIX: rc = hip_check(hipMemcpyAsync(out, e->cx, sizeof(double) * 4 * e->nc, hipMemcpyDeviceToHost, e->st), "get"); break;
+    case CMM_FIELD_CHIY: rc = hip_check(hipMemcpyAsync(out, e->cy, sizeof(double) * 4 * e->nc, hipMemcpyDeviceToHost, e->st), "get"); break;
+    case CMM_FIELD_PSI:
+        for (int l = 0; l < e->p.l_order && !rc; ++l)
+            rc = hip_check(hipMemcpyAsync(out + (size_t)4 * e->np * l, e->lvl[l], sizeof(double) * 4 * e->np, hipMemcpyDeviceToHost, e->st), "get");
+        break;
+    case CMM_FIELD_W_FINE: rc = hip_check(hipMemcpyAsync(out, e->W, sizeof(double) * 4 * e->nf, hipMemcpyDeviceToHost, e->st), "get"); break;
+    case CMM_FIELD_W_COARSE: {
+        if (!e->tmp) rc = dalloc(&e->tmp, e->nc);
+        if (!rc) rc = op_sample_vort(e->p.g_coarse, e->p.g_coarse, e->p.g_fine, e->cx, e->cy, e->W, 0, e->tmp, e->st);
+        if (!rc) rc = hip_check(hipMemcpyAsync(out, e->tmp, sizeof(double) * e->nc, hipMemcpyDeviceToHost, e->st), "get");
+        break;
+    }
+    case CMM_FIELD_DISC:
+        if (!e->disc) { set_error("no discrete initial condition"); return CMM_ERR_ARG; }
+        rc = hip_check(hipMemcpyAsync(out, e->disc, sizeof(double) * 4 * e->nd, hipMemcpyDeviceToHost, e->st), "get");
+        break;
+    default: set_error("cmm_euler_get_field: unknown field"); return CMM_ERR_ARG;
+    }
+    if (rc) return rc;
+    return hip_check(hipStreamSynchronize(e->st), "hipStreamSynchronize");
+}
+
+int cmm_euler_get_particles(cmm_euler *e, int set, double *out) {
+    if (!e || set < 0 || set >= (int)e->part.size() || !out) { set_error("cmm_euler_get_particles: bad argument"); return CMM_ERR_ARG; }
+    if (e->part_n[set] <= 0) return CMM_OK;
+    if (int rc = hip_check(hipMemcpyAsync(out, e->part[set], sizeof(double) * 2 * e->part_n[set], hipMemcpyDeviceToHost, e->st), "get")) return rc;
+    return hip_check(hipStreamSynchronize(e->st), "hipStreamSynchronize");
+}
+
+int cmm_euler_diagnostics(cmm_euler *e, double *out) {
+    if (!e || !out) { set_error("cmm_euler_diagnostics: bad argument"); return CMM_ERR_ARG; }
+    const cmm_grid &gc = e->p.g_coarse, &gp = e->p.g_psi;
+    int rc = CMM_OK;
+    if (!e->tmp) rc = dalloc(&e->tmp, e->nc);
+    if (!rc && !e->tmp4) rc = dalloc(&e->tmp4, 4 * e->nc);
+    // coarse vorticity and its spectral gradient (Hermite planes 1, 2 are grad_x, grad_y of
+    // Cuda-CMM/src/numerical/cmm-fft.cu:457-474)
+    if (!rc) rc = op_sample_vort(gc, gc, e->p.g_fine, e->cx, e->cy, e->W, 0, e->tmp, e->st);
+    const bool fft_ok = gc.nx == gc.ny && gc.nx >= 8 && gc.nx <= 4096 && (gc.nx & (gc.nx - 1)) == 0;
+    if (!rc && fft_ok) rc = op_hermite_from_real(gc, e->tmp, e->tmp4, nullptr, 0, e->st);
+    std::vector<double> w(fft_ok ? 4 * e->nc : e->nc), ps(2 * e->np);
+    if (!rc) rc = hip_check(hipMemcpyAsync(w.data(), fft_ok ? e->tmp4 : e->tmp, sizeof(double) * w.size(), hipMemcpyDeviceToHost, e->st), "get");
+    if (!rc) rc = hip_check(hipMemcpyAsync(ps.data(), e->lvl[0] + e->np, sizeof(double) * 2 * e->np, hipMemcpyDeviceToHost, e->st), "get");
+    if (!rc) rc = hip_check(hipStreamSynchronize(e->st), "hipStreamSynchronize");
+    if (rc) return rc;
+    // Cuda-CMM/src/numerical/cmm-mesure.cu:30-89 and cmm-simulation-host.cu:667-673
+    const DGrid dc = make_dgrid(gc), dp = make_dgrid(gp);
+    double se = 0, sz = 0, sp = 0, wmax = 0, wmin = 0;
+    for (long i = 0; i < 2 * e->np; ++i) se += ps[i] * ps[i];
+    for (long i = 0; i < e->nc; ++i) {
+        sz += w[i] * w[i];
+        wmax = std::max(wmax, w[i]);
+        wmin = std::min(wmin, w[i]);
+    }
+    double spx = 0, spy = 0;
+    if (fft_ok)
+        for (long i = 0; i < e->nc; ++i) { spx += w[e->nc + i] * w[e->nc + i]; spy += w[2 * e->nc + i] * w[2 * e->nc + i]; }
+    sp = 0.5 * dc.hx * dc.hy * spx + 0.5 * dc.hx * dc.hy * spy;
+    out[0] = 0.5 * dp.hx * dp.hy * se;
+    out[1] = 0.5 * dc.hx * dc.hy * sz;
+    out[2] = fft_ok ? sp : NAN;
+    out[3] = std::max(wmax, -wmin);
+    return CMM_OK;
+}
+
+int cmm_euler_state(cmm_euler *e, int *steps, double *t, int *stack_size) {
+    if (!e) { set_error("cmm_euler_state: null engine"); return CMM_ERR_ARG; }
+    if (steps) *steps = e->loop;
+    if (t) *t = e->tv.empty() ? 0.0 : e->tv[e->loop + e->p.l_order - 1];
+    if (stack_size) *stack_size = e->nstack;
+    return CMM_OK;
+}
+
+int cmm_euler_profile(cmm_euler *e, int enable) {
+    if (!e) { set_error("cmm_euler_profile: null engine"); return CMM_ERR_ARG; }
+    e->prof = enable != 0;
+    for (int i = 0; i < CMM_PROF_SLOTS; ++i) { e->ms[i] = 0; e->cnt[i] = 0; }
+    return CMM_OK;
+}
+
+int cmm_euler_profile_read(cmm_euler *e, double *ms, int *cnt) {
+    if (!e) { set_error("cmm_euler_profile_read: null engine"); return CMM_ERR_ARG; }
+    for (int i = 0; i < CMM_PROF_SLOTS; ++i) {
+        if (ms) ms[i] = e->ms[i];
+        if (cnt) cnt[i] = e->cnt[i];
+    }
+    return CMM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,282 @@
+// Device-side core of the CMM engine: grid descriptor, periodic cell lookup, cubic Hermite
+// evaluation and the workgroup FFT building blocks.  Everything here is __host__ __device__
+// so the index arithmetic can also be exercised by host-side unit tests.
+#pragma once
+
+#include <stdint.h>
+#include <math.h>
+#include "../../include/cmm_types.h"
+
+#ifdef __HIPCC__
+#include <hip/hip_runtime.h>
+#define CMM_HD __host__ __device__ __forceinline__
+#else
+#define CMM_HD inline
+#endif
+
+namespace cmm {
+
+constexpr double kPi = 3.141592653589793238462643383279502884197169399375105820974944592307816406286;
+constexpr double kTwoPi = 6.283185307179586476925286766559005768394338798750211641949889184615632812572;
+
+// Grid as the kernels see it.  h = (x1-x0)/(float)nx as in Cuda-CMM/src/grid/cmm-grid2d.cu:27-28;
+// jx/jy are the jumps nx*h added to a map's value plane on a periodic wrap
+// (Cuda-CMM/src/numerical/cmm-hermite.cu:450), lx/ly the domain lengths used to wrap positions
+// (Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:316-320).
+struct DGrid {
+    int nx, ny, n;
+    double hx, hy, x0, y0, jx, jy, lx, ly;
+};
+
+inline DGrid make_dgrid(const cmm_grid &g) {
+    DGrid d;
+    d.nx = g.nx; d.ny = g.ny; d.n = g.nx * g.ny;
+    d.hx = (g.x1 - g.x0) / (float)g.nx;
+    d.hy = (g.y1 - g.y0) / (float)g.ny;
+    d.x0 = g.x0; d.y0 = g.y0;
+    d.jx = g.nx * d.hx; d.jy = g.ny * d.hy;
+    d.lx = g.x1 - g.x0; d.ly = g.y1 - g.y0;
+    return d;
+}
+
+// Four corner indices of the cell holding (x,y), fractional offsets, and the periodic wrap counts
+// of the corners.  Cuda-CMM/src/numerical/cmm-hermite.cu:152-164 / :191-208.  The wrap is a true
+// floor division (identical to the reference except at exact negative multiples of the grid
+// size, where the reference indexes outside the array); the floor is clamped to +-1e9 cells so
+// that non-finite positions still address memory inside the grid.
+struct Cell {
+    int i00, i10, i01, i11;
+    int wx0, wx1, wy0, wy1;
+    double dx, dy;
+};
+
+CMM_HD int floordiv(int a, int n) { return a >= 0 ? a / n : -((-a - 1) / n) - 1; }
+
+CMM_HD void find_cell(const DGrid &g, double x, double y, Cell &c) {
+    double fx = (x - g.x0) / g.hx, fy = (y - g.y0) / g.hy;
+    double flx = floor(fx), fly = floor(fy);
+    c.dx = fx - flx;
+    c.dy = fy - fly;
+    flx = fmin(fmax(flx, -1.0e9), 1.0e9);
+    fly = fmin(fmax(fly, -1.0e9), 1.0e9);
+    int ix = (int)flx, iy = (int)fly;
+    c.wx0 = floordiv(ix, g.nx);
+    c.wy0 = floordiv(iy, g.ny);
+    int rx0 = ix - c.wx0 * g.nx, ry0 = iy - c.wy0 * g.ny;
+    int rx1 = rx0 + 1, ry1 = ry0 + 1;
+    c.wx1 = c.wx0; c.wy1 = c.wy0;
+    if (rx1 == g.nx) { rx1 = 0; c.wx1++; }
+    if (ry1 == g.ny) { ry1 = 0; c.wy1++; }
+    c.i00 = ry0 * g.nx + rx0;
+    c.i10 = ry0 * g.nx + rx1;
+    c.i01 = ry1 * g.nx + rx0;
+    c.i11 = ry1 * g.nx + rx1;
+}
+
+// Cubic Hermite basis in the f3/f4 form of Cuda-CMM/src/numerical/cmm-hermite.h:27-30:
+// b[0], b[1] weight the corner values, b[2], b[3] the corner derivatives (unit cell).
+CMM_HD void basis(double d, double b[4]) {
+    double e = 1 - d;
+    b[0] = (3 - 2 * e) * e * e;
+    b[1] = (3 - 2 * d) * d * d;
+    b[2] = -((e - 1) * e * e);
+    b[3] = (d - 1) * d * d;
+}
+CMM_HD void basis_d(double d, double b[4]) {
+    double e = 1 - d;
+    b[0] = -((6 - 6 * e) * e);
+    b[1] = (6 - 6 * d) * d;
+    b[2] = (3 * e - 2) * e;
+    b[3] = (3 * d - 2) * d;
+}
+
+// Tensor-product sum over the planes [f, fx, fy, fxy] of one Hermite field
+// (Cuda-CMM/src/numerical/cmm-hermite.cu:31-45); derivative planes are physical, so they are
+// scaled by h.
+CMM_HD double hsum(const double *__restrict__ H, int N, const double bx[4], const double by[4], const Cell &c,
+                   double hx, double hy) {
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
+    t0 += by[0] * bx[0] * H[c.i00]; t0 += by[0] * bx[1] * H[c.i10];
+    t0 += by[1] * bx[0] * H[c.i01]; t0 += by[1] * bx[1] * H[c.i11];
+    const double *H1 = H + N, *H2 = H + 2 * (long)N, *H3 = H + 3 * (long)N;
+    t1 += by[0] * bx[2] * H1[c.i00]; t1 += by[0] * bx[3] * H1[c.i10];
+    t1 += by[1] * bx[2] * H1[c.i01]; t1 += by[1] * bx[3] * H1[c.i11];
+    t2 += by[2] * bx[0] * H2[c.i00]; t2 += by[2] * bx[1] * H2[c.i10];
+    t2 += by[3] * bx[0] * H2[c.i01]; t2 += by[3] * bx[1] * H2[c.i11];
+    t3 += by[2] * bx[2] * H3[c.i00]; t3 += by[2] * bx[3] * H3[c.i10];
+    t3 += by[3] * bx[2] * H3[c.i01]; t3 += by[3] * bx[3] * H3[c.i11];
+    double out = t0;
+    out += t1 * hx;
+    out += t2 * hy;
+    t3 *= hx;
+    t3 *= hy;
+    out += t3;
+    return out;
+}
+
+// Same with the wrap jump j*w added to the value plane at each corner
+// (Cuda-CMM/src/numerical/cmm-hermite.cu:67-90).
+CMM_HD double hsum_warp(const double *__restrict__ H, int N, const double bx[4], const double by[4], const Cell &c,
+                        int w00, int w10, int w01, int w11, double jump, double hx, double hy) {
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
+    t0 += by[0] * bx[0] * (H[c.i00] + w00 * jump); t0 += by[0] * bx[1] * (H[c.i10] + w10 * jump);
+    t0 += by[1] * bx[0] * (H[c.i01] + w01 * jump); t0 += by[1] * bx[1] * (H[c.i11] + w11 * jump);
+    const double *H1 = H + N, *H2 = H + 2 * (long)N, *H3 = H + 3 * (long)N;
+    t1 += by[0] * bx[2] * H1[c.i00]; t1 += by[0] * bx[3] * H1[c.i10];
+    t1 += by[1] * bx[2] * H1[c.i01]; t1 += by[1] * bx[3] * H1[c.i11];
+    t2 += by[2] * bx[0] * H2[c.i00]; t2 += by[2] * bx[1] * H2[c.i10];
+    t2 += by[3] * bx[0] * H2[c.i01]; t2 += by[3] * bx[1] * H2[c.i11];
+    t3 += by[2] * bx[2] * H3[c.i00]; t3 += by[2] * bx[3] * H3[c.i10];
+    t3 += by[3] * bx[2] * H3[c.i01]; t3 += by[3] * bx[3] * H3[c.i11];
+    double out = t0;
+    out += t1 * hx;
+    out += t2 * hy;
+    t3 *= hx;
+    t3 *= hy;
+    out += t3;
+    return out;
+}
+
+// Value of a Hermite field, Cuda-CMM/src/numerical/cmm-hermite.cu:323-331
+CMM_HD double herm_eval(const DGrid &g, const double *__restrict__ H, double x, double y) {
+    Cell c;
+    find_cell(g, x, y, c);
+    double bx[4], by[4];
+    basis(c.dx, bx);
+    basis(c.dy, by);
+    return hsum(H, g.n, bx, by, c, g.hx, g.hy);
+}
+
+// Velocity (d psi/dy, -d psi/dx) of up to four psi levels at one point,
+// Cuda-CMM/src/numerical/cmm-hermite.cu:396-420.  u[2l] = dpsi_l/dy, u[2l+1] = -dpsi_l/dx.
+struct PsiLevels {
+    const double *p[4];
+};
+
+CMM_HD void herm_vel(const DGrid &g, const PsiLevels &P, int nl, double x, double y, double *u) {
+    Cell c;
+    find_cell(g, x, y, c);
+    double bx[4], by[4];
+    basis_d(c.dx, bx);
+    basis(c.dy, by);
+    for (int l = 0; l < nl; ++l) u[2 * l + 1] = -hsum(P.p[l], g.n, bx, by, c, g.hx, g.hy) / g.hx;
+    basis(c.dx, bx);
+    basis_d(c.dy, by);
+    for (int l = 0; l < nl; ++l) u[2 * l] = hsum(P.p[l], g.n, bx, by, c, g.hx, g.hy) / g.hy;
+}
+
+// Map evaluation with periodic jumps, Cuda-CMM/src/numerical/cmm-hermite.cu:446-460
+CMM_HD void diffeo(const DGrid &g, const double *__restrict__ Hx, const double *__restrict__ Hy, double x, double y,
+                   double &xo, double &yo) {
+    Cell c;
+    find_cell(g, x, y, c);
+    double bx[4], by[4];
+    basis(c.dx, bx);
+    basis(c.dy, by);
+    xo = hsum_warp(Hx, g.n, bx, by, c, c.wx0, c.wx1, c.wx0, c.wx1, g.jx, g.hx, g.hy);
+    yo = hsum_warp(Hy, g.n, bx, by, c, c.wy0, c.wy0, c.wy1, c.wy1, g.jy, g.hx, g.hy);
+}
+
+// det(grad X), Cuda-CMM/src/numerical/cmm-hermite.cu:479-503
+CMM_HD double diffeo_det(const DGrid &g, const double *__restrict__ Hx, const double *__restrict__ Hy, double x,
+                         double y) {
+    Cell c;
+    find_cell(g, x, y, c);
+    double bx[4], by[4];
+    basis_d(c.dx, bx);
+    basis(c.dy, by);
+    double Xx = hsum_warp(Hx, g.n, bx, by, c, c.wx0, c.wx1, c.wx0, c.wx1, g.jx, g.hx, g.hy) / g.hx;
+    double Yx = hsum_warp(Hy, g.n, bx, by, c, c.wy0, c.wy0, c.wy1, c.wy1, g.jy, g.hx, g.hy) / g.hy;
+    basis(c.dx, bx);
+    basis_d(c.dy, by);
+    double Xy = hsum_warp(Hx, g.n, bx, by, c, c.wx0, c.wx1, c.wx0, c.wx1, g.jx, g.hx, g.hy) / g.hx;
+    double Yy = hsum_warp(Hy, g.n, bx, by, c, c.wy0, c.wy0, c.wy1, c.wy1, g.jy, g.hx, g.hy) / g.hy;
+    return Xx * Yy - Xy * Yx;
+}
+
+CMM_HD double wrap(double v, double o, double L) { return v - floor((v - o) / L) * L; }
+
+// ------------------------------------------------------------------------------------------------
+// Workgroup FFT building blocks (Stockham autosort, radix 8/4/2, complex double).
+// A transform of length n lives in LDS as n complex values at swizzled slots sw(i); each stage
+// reads R values at stride n/R, twiddles, applies an R-point DFT and writes them at stride Ns.
+struct cd {
+    double x, y;
+};
+
+CMM_HD cd cmul(cd a, cd b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+CMM_HD cd cadd(cd a, cd b) { return {a.x + b.x, a.y + b.y}; }
+CMM_HD cd csub(cd a, cd b) { return {a.x - b.x, a.y - b.y}; }
+// multiply by -i (forward) or +i (inverse)
+CMM_HD cd mul_mi(cd a, bool inv) { return inv ? cd{-a.y, a.x} : cd{a.y, -a.x}; }
+
+// XOR swizzle inside aligned groups of 64 slots: spreads stride-8 accesses over the LDS banks.
+CMM_HD int sw(int i) { return i ^ ((i >> 3) & 7); }
+
+template <int R>
+CMM_HD void dft(cd *v, bool inv);
+
+template <>
+CMM_HD void dft<2>(cd *v, bool) {
+    cd a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+}
+
+template <>
+CMM_HD void dft<4>(cd *v, bool inv) {
+    cd t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+    cd t2 = cadd(v[1], v[3]), t3 = mul_mi(csub(v[1], v[3]), inv);
+    v[0] = cadd(t0, t2);
+    v[2] = csub(t0, t2);
+    v[1] = cadd(t1, t3);
+    v[3] = csub(t1, t3);
+}
+
+template <>
+CMM_HD void dft<8>(cd *v, bool inv) {
+    const double r = 0.70710678118654752440084436210484903928;
+    cd e[4] = {v[0], v[2], v[4], v[6]};
+    cd o[4] = {v[1], v[3], v[5], v[7]};
+    dft<4>(e, inv);
+    dft<4>(o, inv);
+    // o[k] *= exp(-+ 2 pi i k / 8)
+    cd w1 = inv ? cd{r, r} : cd{r, -r};
+    cd w3 = inv ? cd{-r, r} : cd{-r, -r};
+    o[1] = cmul(o[1], w1);
+    o[2] = mul_mi(o[2], inv);
+    o[3] = cmul(o[3], w3);
+    for (int k = 0; k < 4; ++k) {
+        v[k] = cadd(e[k], o[k]);
+        v[k + 4] = csub(e[k], o[k]);
+    }
+}
+
+// Radix sequence for a length 2^logn: radix 8 while three or more bits remain, then 4 or 2.
+CMM_HD int stage_radix_log(int remaining) { return remaining >= 3 ? 3 : remaining; }
+
+// Load the R inputs of butterfly j (stride n/R).
+template <int R>
+CMM_HD void stage_load(const cd *s, cd *v, int j, int nb) {
+    for (int r = 0; r < R; ++r) v[r] = s[sw(j + r * nb)];
+}
+
+// Twiddle, DFT and store butterfly j.  Ns = product of the earlier radices (power of two),
+// tw[m] = exp(-2 pi i m / n); the twiddle for input r is W_{Ns R}^{(j mod Ns) r}.
+template <int R>
+CMM_HD void stage_store(cd *s, cd *v, int j, int logNs, int logn, const cd *tw, bool inv) {
+    const int Ns = 1 << logNs;
+    const int k = j & (Ns - 1);
+    const int rlog = (R == 8) ? 3 : (R == 4 ? 2 : 1);
+    const int shift = logn - logNs - rlog;
+    for (int r = 1; r < R; ++r) {
+        cd w = tw[(k * r) << shift];
+        if (inv) w.y = -w.y;
+        v[r] = cmul(v[r], w);
+    }
+    dft<R>(v, inv);
+    const int base = ((j >> logNs) << (logNs + rlog)) + k;
+    for (int r = 0; r < R; ++r) s[sw(base + r * Ns)] = v[r];
+}
+
+}  // namespace cmm

@@ -1,0 +1,48 @@
+// Internal (C++) interface between the kernels, the C-ABI layer and the engine.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <string>
+
+#include "cmm_core.hpp"
+
+namespace cmm {
+
+void set_error(const std::string &s);
+const char *last_error();
+int hip_check(hipError_t e, const char *what);
+
+int twiddles(int n, const cd **out);
+int workspace(size_t bytes, void **out);
+int scalar_slot(unsigned long long **out);
+
+int op_map_identity(const cmm_grid &g, double *cx, double *cy, hipStream_t st);
+int op_map_advect(const cmm_grid &gm, const cmm_grid &gp, const double *cx, const double *cy, const PsiLevels &P,
+                  const cmm_advect_coeffs &c, double *ncx, double *ncy, hipStream_t st);
+int op_incomp_launch(const cmm_grid &gm, const cmm_grid &gc, const double *cx, const double *cy,
+                     unsigned long long *slot, hipStream_t st);
+int op_incomp(const cmm_grid &gm, const cmm_grid &gc, const double *cx, const double *cy, double *out,
+              hipStream_t st);
+size_t op_psi_workspace(const cmm_grid &gv, const cmm_grid &gp, double fcut);
+int op_psi_from_vort(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf, const cmm_grid &gp, const double *cx,
+                     const double *cy, const double *W, int molly, double fcut, double *psi, void *work,
+                     size_t work_bytes, hipStream_t st, hipEvent_t *ev = nullptr);
+size_t op_hermite_workspace(const cmm_grid &g);
+int op_hermite_from_real(const cmm_grid &g, const double *real, double *herm, void *work, size_t work_bytes,
+                         hipStream_t st);
+int op_hermite_from_spectrum(const cmm_grid &g, const double *spec_cols, double *herm, void *work,
+                             size_t work_bytes, hipStream_t st);
+int op_translate_ic(const cmm_grid &gm, const cmm_grid &gf, const double *cx, const double *cy, const double *stack,
+                    int nstack, int ic, const double *prm, int discrete, const cmm_grid &gd, const double *disc,
+                    double *herm, void *work, size_t work_bytes, hipStream_t st);
+int op_compose_positions(const cmm_grid &gm, const cmm_grid &gt, const double *cx, const double *cy,
+                         const double *stack, int nstack, double *xy, hipStream_t st);
+int op_sample_init(const cmm_grid &gm, const cmm_grid &gt, const double *cx, const double *cy, const double *stack,
+                   int nstack, int ic, const double *prm, int discrete, const cmm_grid &gd, const double *disc,
+                   double *out, hipStream_t st);
+int op_sample_vort(const cmm_grid &gm, const cmm_grid &gt, const cmm_grid &gf, const double *cx, const double *cy,
+                   const double *W, int molly, double *out, hipStream_t st);
+int op_particles(const cmm_grid &gp, const PsiLevels &P, int L, const double *Lp, double dt, int scheme, int n,
+                 double *pos, hipStream_t st);
+
+}  // namespace cmm

@@ -1,0 +1,1040 @@
+// HIP kernels of the CMM map-advection path for gfx950, and the C-ABI launchers declared in
+// include/cmm.h.  One thread per grid point for the gather kernels; one workgroup per row or
+// column for the LDS-resident FFT kernels.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "cmm_core.hpp"
+#include "cmm_internal.hpp"
+#include "../../include/cmm.h"
+
+using namespace cmm;
+
+namespace cmm {
+
+// ------------------------------------------------------------------------------------------------
+// error state
+static thread_local std::string g_last_error;
+
+void set_error(const std::string &s) { g_last_error = s; }
+const char *last_error() { return g_last_error.c_str(); }
+
+int hip_check(hipError_t e, const char *what) {
+    if (e != hipSuccess) {
+        set_error(std::string(what) + ": " + hipGetErrorString(e));
+        return CMM_ERR_HIP;
+    }
+    return CMM_OK;
+}
+
+// XCD-aware block order: the hardware deals consecutive workgroups round-robin over the 8 XCDs;
+// this returns the logical index so that each XCD works on one contiguous range (shared L2 lines).
+__device__ __forceinline__ int xcd_block(int bid, int nblocks) {
+    const int xcd = bid & 7, slot = bid >> 3;
+    const int per = nblocks >> 3, extra = nblocks & 7;
+    return xcd * per + (xcd < extra ? xcd : extra) + slot;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Map kernels
+
+// identity map, Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:27-41
+__global__ void k_map_identity(DGrid g, double *__restrict__ cx, double *__restrict__ cy) {
+    int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    int iy = blockIdx.y * blockDim.y + threadIdx.y;
+    if (ix >= g.nx || iy >= g.ny) return;
+    int i = iy * g.nx + ix;
+    cx[i] = g.x0 + ix * g.hx;
+    cy[i] = g.y0 + iy * g.hy;
+    cx[g.n + i] = 1.0;
+    cy[2 * g.n + i] = 1.0;
+    cx[2 * g.n + i] = 0.0;
+    cy[g.n + i] = 0.0;
+    cx[3 * g.n + i] = 0.0;
+    cy[3 * g.n + i] = 0.0;
+}
+
+__device__ __forceinline__ void lsum(const double *w, const double *u, int nl, double &kx, double &ky) {
+    kx = w[0] * u[0];
+    ky = w[0] * u[1];
+    for (int l = 1; l < nl; ++l) {
+        kx += w[l] * u[2 * l];
+        ky += w[l] * u[2 * l + 1];
+    }
+}
+
+// Backward footpoint integration for one start point; schemes of
+// Cuda-CMM/src/numerical/cmm-timestep.cu (EulerExp :35-45, AB2 :115-137, Heun :68-85,
+// RK3 :175-207, RK3Mod :357-386, RK4 :261-304, RK4Mod :441-470).
+template <int TIN>
+__device__ __forceinline__ void step_backward(const DGrid &g, const PsiLevels &P, const cmm_advect_coeffs &c,
+                                              double x0, double y0, double &xo, double &yo) {
+    const int L = c.l_order;
+    const double dt = c.dt;
+    double u[8];
+    if constexpr (TIN == 10) {
+        herm_vel(g, P, L, x0, y0, u);
+        double kx, ky;
+        lsum(c.L1, u, L, kx, ky);
+        xo = x0 - dt * kx;
+        yo = y0 - dt * ky;
+    } else if constexpr (TIN == 20) {
+        PsiLevels P1 = {{P.p[1], P.p[1], P.p[1], P.p[1]}};
+        double uu[4], xp = x0, yp = y0;
+        xo = x0;
+        yo = y0;
+        for (int it = 0; it < 10; ++it) {
+            herm_vel(g, P1, 1, xp, yp, uu + 2);
+            xp = xo - dt * uu[2];
+            yp = yo - dt * uu[3];
+            herm_vel(g, P1, 1, xp, yp, uu + 2);
+            herm_vel(g, P, 1, xo, yo, uu);
+            xo = x0 - dt * (1.5 * uu[0] - 0.5 * uu[2]);
+            yo = y0 - dt * (1.5 * uu[1] - 0.5 * uu[3]);
+        }
+    } else if constexpr (TIN == 21) {
+        herm_vel(g, P, L, x0, y0, u);
+        double k1x, k1y;
+        lsum(c.L1, u, L, k1x, k1y);
+        herm_vel(g, P, 1, x0 - dt * k1x, y0 - dt * k1y, u);
+        xo = x0 - dt * (k1x + u[0]) / 2;
+        yo = y0 - dt * (k1y + u[1]) / 2;
+    } else if constexpr (TIN == 30) {
+        double k[6];
+        herm_vel(g, P, L, x0, y0, u);
+        lsum(c.L1, u, L, k[0], k[1]);
+        herm_vel(g, P, L, x0 - dt * k[0] / 2.0, y0 - dt * k[1] / 2.0, u);
+        lsum(c.L12, u, L, k[2], k[3]);
+        herm_vel(g, P, 1, x0 + dt * k[0] - 2 * dt * k[2], y0 + dt * k[1] - 2 * dt * k[3], k + 4);
+        xo = x0 - dt * (k[0] + 4 * k[2] + k[4]) / 6.0;
+        yo = y0 - dt * (k[1] + 4 * k[3] + k[5]) / 6.0;
+    } else if constexpr (TIN == 31) {
+        double k[6];
+        herm_vel(g, P, L, x0, y0, u);
+        lsum(c.L1, u, L, k[0], k[1]);
+        herm_vel(g, P, 1, x0 - dt * k[0], y0 - dt * k[1], k + 2);
+        PsiLevels P1 = {{P.p[1], P.p[1], P.p[1], P.p[1]}};
+        herm_vel(g, P1, 1, x0 + 2 * dt * (-2 * k[0] + k[2]), y0 + 2 * dt * (-2 * k[1] + k[3]), k + 4);
+        xo = x0 - dt * (5 * k[0] + 8 * k[2] - k[4]) / 12.0;
+        yo = y0 - dt * (5 * k[1] + 8 * k[3] - k[5]) / 12.0;
+    } else if constexpr (TIN == 40) {
+        double k[8];
+        herm_vel(g, P, L, x0, y0, u);
+        lsum(c.L1, u, L, k[0], k[1]);
+        herm_vel(g, P, L, x0 - dt * k[0] / 2.0, y0 - dt * k[1] / 2.0, u);
+        lsum(c.L12, u, L, k[2], k[3]);
+        herm_vel(g, P, L, x0 - dt * k[2] / 2.0, y0 - dt * k[3] / 2.0, u);
+        lsum(c.L12, u, L, k[4], k[5]);
+        herm_vel(g, P, 1, x0 - dt * k[4], y0 - dt * k[5], k + 6);
+        xo = x0 - dt * (k[0] + 2 * k[2] + 2 * k[4] + k[6]) / 6.0;
+        yo = y0 - dt * (k[1] + 2 * k[3] + 2 * k[5] + k[7]) / 6.0;
+    } else {  // 41
+        double k[8];
+        herm_vel(g, P, L, x0, y0, u);
+        lsum(c.L1, u, L, k[0], k[1]);
+        herm_vel(g, P, 1, x0 - dt * k[0], y0 - dt * k[1], k + 2);
+        herm_vel(g, P, L, x0 - dt * (3 * k[0] + k[2]) / 8.0, y0 - dt * (3 * k[1] + k[3]) / 8.0, u);
+        lsum(c.L12, u, L, k[4], k[5]);
+        herm_vel(g, P, 1, x0 + dt * (k[2] - 4 * k[4]) / 3.0, y0 + dt * (k[3] - 4 * k[5]) / 3.0, k + 6);
+        xo = x0 - dt * (2 * k[0] - k[2] + 8 * k[4] + 3 * k[6]) / 12.0;
+        yo = y0 - dt * (2 * k[1] - k[3] + 8 * k[5] + 3 * k[7]) / 12.0;
+    }
+}
+
+// Map advection with the footpoint stencil, Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:207-298.
+// Block 64x4; blocks are visited in XCD-contiguous order.
+template <int TIN>
+__global__ void __launch_bounds__(256) k_map_advect(DGrid gm, DGrid gp, const double *__restrict__ cx,
+                                                    const double *__restrict__ cy, PsiLevels P, cmm_advect_coeffs c,
+                                                    double *__restrict__ ncx, double *__restrict__ ncy, int tiles_x,
+                                                    int tiles_total) {
+    const int t = xcd_block(blockIdx.x, tiles_total);
+    const int ix = (t % tiles_x) * 64 + threadIdx.x;
+    const int iy = (t / tiles_x) * 4 + threadIdx.y;
+    if (ix >= gm.nx || iy >= gm.ny) return;
+    double ax0 = 0, ax1 = 0, ax2 = 0, ax3 = 0, ay0 = 0, ay1 = 0, ay2 = 0, ay3 = 0;
+    const int nfoot = 4 + 4 * c.map_update_order_num;
+    const double bx = gm.x0 + ix * gm.hx, by = gm.y0 + iy * gm.hy;
+    for (int k = 0; k < nfoot; ++k) {
+        const double xe = bx + ((k >> 2) + 1) * c.eps * (1 - 2 * ((k >> 1) & 1));
+        const double ye = by + ((k >> 2) + 1) * c.eps * (1 - 2 * (((k + 1) >> 1) & 1));
+        double xf, yf;
+        step_backward<TIN>(gp, P, c, xe, ye, xf, yf);
+        double px, py;
+        diffeo(gm, cx, cy, xf, yf, px, py);
+        ax0 += px * c.c1[k]; ay0 += py * c.c1[k];
+        ax1 += px * c.cx[k]; ay1 += py * c.cx[k];
+        ax2 += px * c.cy[k]; ay2 += py * c.cy[k];
+        ax3 += px * c.cxy[k]; ay3 += py * c.cxy[k];
+    }
+    const int i = iy * gm.nx + ix;
+    ncx[i] = ax0; ncx[gm.n + i] = ax1; ncx[2 * gm.n + i] = ax2; ncx[3 * gm.n + i] = ax3;
+    ncy[i] = ay0; ncy[gm.n + i] = ay1; ncy[2 * gm.n + i] = ay2; ncy[3 * gm.n + i] = ay3;
+}
+
+// max |1 - det grad X| over cell centres of gc; per-block maximum then one 64-bit atomic max on the
+// bit pattern (order-preserving for non-negative doubles).  Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:159-173
+// and the Thrust max-reduction of cmm-simulation-host.cu:44-50.
+__global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
+
+__global__ void __launch_bounds__(256) k_incomp(DGrid gm, DGrid gc, const double *__restrict__ cx,
+                                                const double *__restrict__ cy, unsigned long long *__restrict__ out,
+                                                int tiles_x, int tiles_total) {
+    const int t = xcd_block(blockIdx.x, tiles_total);
+    const int ix = (t % tiles_x) * 64 + threadIdx.x;
+    const int iy = (t / tiles_x) * 4 + threadIdx.y;
+    double v = 0.0;
+    if (ix < gc.nx && iy < gc.ny) {
+        const double x = gm.x0 + ix * gc.hx + 0.5 * gc.hx;
+        const double y = gm.y0 + iy * gc.hy + 0.5 * gc.hy;
+        v = fabs(1 - diffeo_det(gm, cx, cy, x, y));
+    }
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    __shared__ double red[4];
+    if (threadIdx.x == 0) red[threadIdx.y] = v;
+    __syncthreads();
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        double m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+        unsigned long long b = (unsigned long long)__double_as_longlong(m);
+        atomicMax(out, b);
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Initial conditions on the device, Cuda-CMM/src/simulation/cmm-init.cu:30-241 (+ new conditions)
+struct ICParams {
+    int ic;
+    double prm[10];
+};
+
+__device__ double init_vorticity(double x, double y, const ICParams &p) {
+    switch (p.ic) {
+    case CMM_IC_4_NODES: {
+        x -= floor(x / kTwoPi) * kTwoPi;
+        y -= floor(y / kTwoPi) * kTwoPi;
+        return cos(x) + cos(y) + 0.6 * cos(2 * x) + 0.2 * cos(3 * x);
+    }
+    case CMM_IC_QUADROPOLE: {
+        double ret = 0;
+        for (int jy = -2; jy <= 2; jy++)
+            for (int jx = -2; jx <= 2; jx++) {
+                double dx = x - kPi / 2 + jx * 2 * kPi, dy = y - kPi / 2 + jy * 2 * kPi;
+                double A = 0.6258473, s = 0.5;
+                double B = A / (s * s * s * s) * (dx * dy) * (dx * dx + dy * dy - 6 * s * s);
+                double D = (dx * dx + dy * dy) / (2 * s * s);
+                ret += B * exp(-D);
+            }
+        return ret;
+    }
+    case CMM_IC_ONE_VORTEX: {
+        double sigma = kTwoPi * 0.125;
+        return 1.0 / (kTwoPi * sigma * sigma) * exp(-((x - kPi) * (x - kPi) + (y - kPi) * (y - kPi)) / (2 * sigma * sigma));
+    }
+    case CMM_IC_SINGLE_SHEAR_LAYER:
+        return (1 + 0.01 * cos(2 * x)) * exp(-50.0 * (y - kPi) * (y - kPi));
+    case CMM_IC_TANH_SHEAR_LAYER: {
+        double d = 14 * 2.0 / kTwoPi;
+        double e = exp(-d * (y - kPi)) + exp(d * (y - kPi));
+        return 5.0 * (1 + 0.01 * cos(2.0 * x)) * 4 / e / e;
+    }
+    case CMM_IC_SHIELDED_VORTEX: {
+        double nu = 2e-1, nu_fac = 1 / (2 * nu * nu * nu), nu_c = 4 * nu, nu_s = 4 * nu;
+        double xr = kPi - x, yr = kPi - y;
+        return nu_fac * (nu_c - xr * xr - yr * yr) * exp(-(xr * xr + yr * yr) / nu_s);
+    }
+    case CMM_IC_TWO_COSINE: {
+        x -= floor(x / kTwoPi) * kTwoPi;
+        y -= floor(y / kTwoPi) * kTwoPi;
+        return cos(x) * cos(y);
+    }
+    case CMM_IC_VORTEX_SHEETS: {
+        double d = 1 / sqrt(p.prm[0]);
+        double s = y - kPi - sin(x) / 2.0;
+        return exp(-s * s / (2 * d * d)) / sqrt(2 * kPi) / d;
+    }
+    case CMM_IC_DOUBLE_SHEAR_LAYER: {
+        const double delta = kPi / 15.0;
+        double ret = 0.05 * cos(x);
+        for (int im = -2; im <= 2; ++im) {
+            double a = (y + im * kTwoPi - kPi / 2) / delta;
+            double b = (y + im * kTwoPi - 3 * kPi / 2) / delta;
+            double sa = 1.0 / cosh(a), sb = 1.0 / cosh(b);
+            ret += (sa * sa - sb * sb) / delta;
+        }
+        return ret;
+    }
+    default:
+        return 0.0;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Row samplers feeding the R2C row transform (one value per grid point of the row grid).
+
+// plain real array
+struct SampleReal {
+    const double *__restrict__ f;
+    int nx;
+    __device__ double operator()(int ix, int iy) const { return f[(long)iy * nx + ix]; }
+};
+
+// omega(X(x)) through the current map and the fine Hermite vorticity (mollifier 0/4/8),
+// Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:402-461; positions use the map grid's origin.
+struct SampleVort {
+    DGrid gm, gv, gf;
+    const double *__restrict__ cx;
+    const double *__restrict__ cy;
+    const double *__restrict__ W;
+    int molly;
+    __device__ double operator()(int ix, int iy) const {
+        const double x = gm.x0 + ix * gv.hx, y = gm.y0 + iy * gv.hy;
+        double x2, y2, v;
+        if (molly == 4) {
+            diffeo(gm, cx, cy, x, y, x2, y2);
+            v = herm_eval(gf, W, x2, y2) / 3.0;
+            for (int m = 0; m < 4; ++m) {
+                // the reference's (i_molly>>1+1)&1 parses as (i_molly>>2)&1
+                x2 = x + gv.hx / 2.0 * ((m >> 2) & 1) * (-1 + 2 * (m & 1));
+                y2 = y + gv.hy / 2.0 * ((m >> 1) & 1) * (-1 + 2 * (m & 1));
+                diffeo(gm, cx, cy, x2, y2, x2, y2);
+                v += herm_eval(gf, W, x2, y2) / 6.0;
+            }
+        } else if (molly == 8) {
+            v = 0;
+            for (int m = 0; m < 9; ++m) {
+                x2 = x + gv.hx * (-1 + m % 3) / 2.0;
+                y2 = y + gv.hy * (-1 + m / 3) / 2.0;
+                diffeo(gm, cx, cy, x2, y2, x2, y2);
+                // the reference's (1 + (i%3)&1) parses as ((1 + i%3) & 1)
+                v += ((1 + (m % 3)) & 1) * ((1 + (m / 3)) & 1) * herm_eval(gf, W, x2, y2) / 16.0;
+            }
+        } else {
+            diffeo(gm, cx, cy, x, y, x2, y2);
+            v = herm_eval(gf, W, x2, y2);
+        }
+        return v;
+    }
+};
+
+// Initial condition at X_1(...X_M(X_cur(x))): the current map then the stack from newest to
+// oldest, wrapping after each layer (Cuda-CMM/src/simulation/cmm-simulation-host.cu:112-152,
+// Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:302-373).  All layers are applied in
+// registers; the stack is device resident, map m at stack + 8*Nc*m (X planes, then Y planes).
+struct SampleCompose {
+    DGrid gm, gt, gd;
+    const double *__restrict__ cx;
+    const double *__restrict__ cy;
+    const double *__restrict__ stack;
+    int nstack;
+    ICParams ic;
+    int discrete;
+    const double *__restrict__ disc;
+    __device__ void position(int ix, int iy, double &x, double &y) const {
+        x = gt.x0 + ix * gt.hx;
+        y = gt.y0 + iy * gt.hy;
+        double px, py;
+        diffeo(gm, cx, cy, x, y, px, py);
+        x = px - floor((px - gm.x0) / gm.lx) * gm.lx;
+        y = py - floor((py - gm.y0) / gm.ly) * gm.ly;
+        const long mstride = 8l * gm.n;
+        for (int m = nstack - 1; m >= 0; --m) {
+            const double *sx = stack + mstride * m;
+            diffeo(gm, sx, sx + 4l * gm.n, x, y, px, py);
+            x = px - floor((px - gm.x0) / gm.lx) * gm.lx;
+            y = py - floor((py - gm.y0) / gm.ly) * gm.ly;
+        }
+    }
+    __device__ double operator()(int ix, int iy) const {
+        double x, y;
+        position(ix, iy, x, y);
+        return discrete ? herm_eval(gd, disc, x, y) : init_vorticity(x, y, ic);
+    }
+};
+
+// ------------------------------------------------------------------------------------------------
+// Workgroup FFT in LDS (length n <= 16*T, power of two).
+constexpr int kFftThreads = 256;
+
+template <int R>
+__device__ __forceinline__ void run_stage(cd *s, int n, int logn, int logNs, const cd *__restrict__ tw, bool inv) {
+    constexpr int BPT = 16 / R;
+    const int nb = n / R;
+    cd v[16];
+#pragma unroll
+    for (int b = 0; b < BPT; ++b) {
+        int j = threadIdx.x + b * kFftThreads;
+        if (j < nb) stage_load<R>(s, v + b * R, j, nb);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int b = 0; b < BPT; ++b) {
+        int j = threadIdx.x + b * kFftThreads;
+        if (j < nb) stage_store<R>(s, v + b * R, j, logNs, logn, tw, inv);
+    }
+    __syncthreads();
+}
+
+__device__ void wg_fft(cd *s, int n, int logn, const cd *__restrict__ tw, bool inv) {
+    int logNs = 0, rem = logn;
+    while (rem > 0) {
+        int rl = stage_radix_log(rem);
+        if (rl == 3) run_stage<8>(s, n, logn, logNs, tw, inv);
+        else if (rl == 2) run_stage<4>(s, n, logn, logNs, tw, inv);
+        else run_stage<2>(s, n, logn, logNs, tw, inv);
+        logNs += rl;
+        rem -= rl;
+    }
+}
+
+// Row R2C of two rows at once (z = row_a + i row_b), samples produced by the sampler.  Writes
+// the half spectrum columns kx <= kx_max transposed: T[kx*ny + y].  Unnormalised.
+template <class S>
+__global__ void __launch_bounds__(kFftThreads) k_row_r2c(S sampler, int nx, int ny, int logn, int kx_max,
+                                                          const cd *__restrict__ tw, cd *__restrict__ T) {
+    extern __shared__ cd s[];
+    const int pair = xcd_block(blockIdx.x, gridDim.x);
+    const int ya = 2 * pair, yb = ya + 1;
+    for (int x = threadIdx.x; x < nx; x += kFftThreads) {
+        double a = sampler(x, ya);
+        double b = sampler(x, yb);
+        s[sw(x)] = cd{a, b};
+    }
+    __syncthreads();
+    wg_fft(s, nx, logn, tw, false);
+    for (int k = threadIdx.x; k <= kx_max; k += kFftThreads) {
+        cd z = s[sw(k)];
+        cd zc = s[sw((nx - k) & (nx - 1))];
+        // X_a = (Z[k] + conj Z[n-k]) / 2,  X_b = (Z[k] - conj Z[n-k]) / (2i)
+        cd xa = {(z.x + zc.x) * 0.5, (z.y - zc.y) * 0.5};
+        cd xb = {(z.y + zc.y) * 0.5, -(z.x - zc.x) * 0.5};
+        T[(long)k * ny + ya] = xa;
+        T[(long)k * ny + yb] = xb;
+    }
+}
+
+// Column stage.  For column kx <= kx_max: forward transform along y of T (length nin), scale
+// by 1/norm, spectral operator, then two inverse transforms along y (length nout):
+//   G0 = IFFT_y(P), G1 = IFFT_y(i ky P).
+// mode 0 (Hermite, nin == nout): P = F.
+// mode 1 (psi): P = -F/|k|^2 after the spectrum move between grids and the circular cut-off
+//   (Cuda-CMM/src/numerical/cmm-fft.cu:294-323, :267-286, :55-78).
+// mode 2 (spectrum given): T already holds the column spectrum; no forward transform, no scaling.
+struct ColParams {
+    int nin, login, nout, logout, nx_in, nx_out;  // nx_* = grid sizes along x (for the move rule)
+    double inv_norm;
+    double fcut;
+    double fkx, fky;  // 2 pi/(h*N) of the output grid along x and y
+    int mode;
+    int kx_max;
+};
+
+__global__ void __launch_bounds__(kFftThreads) k_col(ColParams p, const cd *__restrict__ T,
+                                                     const cd *__restrict__ tw_in, const cd *__restrict__ tw_out,
+                                                     cd *__restrict__ G0, cd *__restrict__ G1) {
+    extern __shared__ cd s[];
+    const int kx = blockIdx.x;
+    const cd *col = T + (long)kx * p.nin;
+    for (int y = threadIdx.x; y < p.nin; y += kFftThreads) s[sw(y)] = col[y];
+    __syncthreads();
+    if (p.mode != 2) wg_fft(s, p.nin, p.login, tw_in, false);
+    // build P for the output column in registers (each thread owns rows threadIdx.x + m*T)
+    constexpr int MAXR = 16;
+    cd P[MAXR];
+    const double kxv = p.fkx * kx;
+#pragma unroll
+    for (int m = 0; m < MAXR; ++m) {
+        const int y = threadIdx.x + m * kFftThreads;
+        cd v = {0.0, 0.0};
+        if (y < p.nout) {
+            if (p.mode == 1) {
+                // spectrum move (only when the grids differ), Cuda-CMM/src/numerical/cmm-fft.cu:294-323
+                bool zero = false;
+                int ys = y;
+                if (p.nout != p.nin || p.nx_out != p.nx_in) {
+                    int mxo = kx + (kx > p.nx_out / 2.0) * (p.nx_out - 2 * kx);
+                    int myo = y + (y > p.nout / 2.0) * (p.nout - 2 * y);
+                    zero = (mxo > p.nx_in / 2.0) || (myo > p.nin / 2.0);
+                    ys = y + (y > p.nout / 2.0) * (p.nin - p.nout);
+                }
+                if (!zero) {
+                    cd f = s[sw(ys)];
+                    v = {f.x * p.inv_norm, f.y * p.inv_norm};
+                }
+                // cut-off, Cuda-CMM/src/numerical/cmm-fft.cu:267-286 (DC removed)
+                int mx = kx + (kx > p.nx_out / 2.0) * (p.nx_out - 2 * kx);
+                int my = y + (y > p.nout / 2.0) * (p.nout - 2 * y);
+                if ((mx * mx + my * my) > p.fcut * p.fcut || (kx == 0 && y == 0)) v = {0.0, 0.0};
+                // inverse Laplacian, Cuda-CMM/src/numerical/cmm-fft.cu:55-78
+                if (!(kx == 0 && y == 0)) {
+                    double kyv = p.fky * (y - (y > p.nout / 2) * p.nout);
+                    double k2 = kxv * kxv + kyv * kyv;
+                    v = {-v.x / k2, -v.y / k2};
+                }
+            } else if (p.mode == 0) {
+                cd f = s[sw(y)];
+                v = {f.x * p.inv_norm, f.y * p.inv_norm};
+            } else {
+                v = s[sw(y)];
+            }
+        }
+        P[m] = v;
+    }
+    __syncthreads();
+    // G0 = IFFT_y(P)
+#pragma unroll
+    for (int m = 0; m < MAXR; ++m) {
+        const int y = threadIdx.x + m * kFftThreads;
+        if (y < p.nout) s[sw(y)] = P[m];
+    }
+    __syncthreads();
+    wg_fft(s, p.nout, p.logout, tw_out, true);
+    cd *g0 = G0 + (long)kx * p.nout;
+    for (int y = threadIdx.x; y < p.nout; y += kFftThreads) g0[y] = s[sw(y)];
+    __syncthreads();
+    // G1 = IFFT_y(i ky P), Cuda-CMM/src/numerical/cmm-fft.cu:169-187 (ky with the Nyquist index positive)
+#pragma unroll
+    for (int m = 0; m < MAXR; ++m) {
+        const int y = threadIdx.x + m * kFftThreads;
+        if (y < p.nout) {
+            double kyv = p.fky * (y - (y > p.nout / 2) * p.nout);
+            s[sw(y)] = cd{-P[m].y * kyv, P[m].x * kyv};
+        }
+    }
+    __syncthreads();
+    wg_fft(s, p.nout, p.logout, tw_out, true);
+    cd *g1 = G1 + (long)kx * p.nout;
+    for (int y = threadIdx.x; y < p.nout; y += kFftThreads) g1[y] = s[sw(y)];
+}
+
+// Row C2R producing the four Hermite planes of one row from the column results:
+//   f = C2R(G0), fx = C2R(i kx G0), fy = C2R(G1), fxy = C2R(i kx G1)
+// (Cuda-CMM/src/numerical/cmm-fft.cu:398-429).  Two real outputs come from one complex inverse
+// transform of the Hermitian extensions A~ + i B~; the kx = 0 and kx = n/2 bins keep only their
+// real parts (C2R semantics).  Columns above kx_max are zero.
+__global__ void __launch_bounds__(kFftThreads) k_row_c2r4(int nx, int ny, int logn, int kx_max, double fkx,
+                                                          const cd *__restrict__ G0, const cd *__restrict__ G1,
+                                                          const cd *__restrict__ tw, double *__restrict__ out) {
+    extern __shared__ cd s[];
+    const int y = xcd_block(blockIdx.x, gridDim.x);
+    const long N = (long)nx * ny;
+    const int h = nx / 2;
+    for (int pass = 0; pass < 2; ++pass) {
+        const cd *G = pass == 0 ? G0 : G1;
+        for (int k = threadIdx.x; k <= h; k += kFftThreads) {
+            cd A = {0.0, 0.0};
+            if (k <= kx_max) A = G[(long)k * ny + y];
+            const double kv = fkx * k;
+            cd B = {-A.y * kv, A.x * kv};  // i kx A
+            if (k == 0 || k == h) {
+                s[sw(k)] = cd{A.x, B.x};
+            } else {
+                s[sw(k)] = cd{A.x - B.y, A.y + B.x};       // A + iB
+                s[sw(nx - k)] = cd{A.x + B.y, -A.y + B.x};  // conj(A) + i conj(B)
+            }
+        }
+        __syncthreads();
+        wg_fft(s, nx, logn, tw, true);
+        double *o0 = out + (2 * pass) * N + (long)y * nx;
+        double *o1 = out + (2 * pass + 1) * N + (long)y * nx;
+        for (int x = threadIdx.x; x < nx; x += kFftThreads) {
+            cd v = s[sw(x)];
+            o0[x] = v.x;
+            o1[x] = v.y;
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Point kernels used outside the fused transforms
+
+template <class S>
+__global__ void __launch_bounds__(256) k_sample_grid(S sampler, int nx, int ny, double *__restrict__ out) {
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iy = blockIdx.y * blockDim.y + threadIdx.y;
+    if (ix >= nx || iy >= ny) return;
+    out[(long)iy * nx + ix] = sampler(ix, iy);
+}
+
+__global__ void __launch_bounds__(256) k_compose_positions(SampleCompose sc, double *__restrict__ xy) {
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iy = blockIdx.y * blockDim.y + threadIdx.y;
+    if (ix >= sc.gt.nx || iy >= sc.gt.ny) return;
+    double x, y;
+    sc.position(ix, iy, x, y);
+    const long i = (long)iy * sc.gt.nx + ix;
+    xy[2 * i] = x;
+    xy[2 * i + 1] = y;
+}
+
+// Fluid particles, Cuda-CMM/src/numerical/cmm-particles.cu:119-276 (level 0 = t_{n+1}, 1 = t_n)
+struct PartParams {
+    double Lp[4];
+    double dt;
+    int L, scheme, n;
+};
+
+__global__ void __launch_bounds__(256) k_particles(DGrid g, PsiLevels P, PartParams q, double *__restrict__ pos) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= q.n) return;
+    double p0 = pos[2 * i], p1 = pos[2 * i + 1];
+    const double dt = q.dt;
+    const int L = q.L;
+    PsiLevels P1 = {{P.p[1], P.p[1], P.p[1], P.p[1]}};
+    PsiLevels P2 = {{P.p[2], P.p[2], P.p[2], P.p[2]}};
+    double u[8], k[8];
+    switch (q.scheme) {
+    case 10:
+        herm_vel(g, P1, 1, p0, p1, u);
+        p0 += dt * u[0];
+        p1 += dt * u[1];
+        break;
+    case 20:
+        herm_vel(g, P1, 1, p0, p1, k);
+        herm_vel(g, P, 1, p0 + dt * k[0], p1 + dt * k[1], k + 2);
+        p0 += dt * (k[0] + k[2]) / 2;
+        p1 += dt * (k[1] + k[3]) / 2;
+        break;
+    case 30:
+        herm_vel(g, P1, 1, p0, p1, k);
+        herm_vel(g, P, L, p0 + dt * k[0] / 2.0, p1 + dt * k[1] / 2.0, u);
+        lsum(q.Lp, u, L, k[2], k[3]);
+        herm_vel(g, P, 1, p0 - dt * k[0] + 2 * dt * k[2], p1 - dt * k[1] + 2 * dt * k[3], k + 4);
+        p0 += dt * (k[0] + 4 * k[2] + k[4]) / 6.0;
+        p1 += dt * (k[1] + 4 * k[3] + k[5]) / 6.0;
+        break;
+    case 40:
+        herm_vel(g, P1, 1, p0, p1, k);
+        herm_vel(g, P, L, p0 + dt * k[0] / 2.0, p1 + dt * k[1] / 2.0, u);
+        lsum(q.Lp, u, L, k[2], k[3]);
+        herm_vel(g, P, L, p0 + dt * k[2] / 2.0, p1 + dt * k[3] / 2.0, u);
+        lsum(q.Lp, u, L, k[4], k[5]);
+        herm_vel(g, P, 1, p0 + dt * k[4], p1 + dt * k[5], k + 6);
+        p0 += dt * (k[0] + 2 * k[2] + 2 * k[4] + k[6]) / 6.0;
+        p1 += dt * (k[1] + 2 * k[3] + 2 * k[5] + k[7]) / 6.0;
+        break;
+    case 31:
+        herm_vel(g, P1, 1, p0, p1, k);
+        herm_vel(g, P, 1, p0 + dt * k[0], p1 + dt * k[1], k + 2);
+        herm_vel(g, P2, 1, p0 + dt * (k[0] - 2 * k[2]), p1 + dt * (k[1] - 2 * k[3]), k + 4);
+        p0 += dt * (8 * k[0] + 5 * k[2] - k[4]) / 12.0;
+        p1 += dt * (8 * k[1] + 5 * k[3] - k[5]) / 12.0;
+        break;
+    case 41:
+        herm_vel(g, P1, 1, p0, p1, k);
+        herm_vel(g, P, 1, p0 + dt * k[0], p1 + dt * k[1], k + 2);
+        herm_vel(g, P, L, p0 + dt / 8.0 * (3 * k[0] + k[2]), p1 + dt / 8.0 * (3 * k[1] + k[3]), u);
+        lsum(q.Lp, u, L, k[4], k[5]);
+        herm_vel(g, P, 1, p0 + dt / 3.0 * (-k[2] + 4 * k[4]), p1 + dt / 3.0 * (-k[3] + 4 * k[5]), k + 6);
+        p0 += dt * (2 * k[0] - k[2] + 8 * k[4] + 3 * k[6]) / 12.0;
+        p1 += dt * (2 * k[1] - k[3] + 8 * k[5] + 3 * k[7]) / 12.0;
+        break;
+    default:
+        p0 = p1 = 0;
+    }
+    pos[2 * i] = p0 - floor((p0 - g.x0) / g.lx) * g.lx;
+    pos[2 * i + 1] = p1 - floor((p1 - g.y0) / g.ly) * g.ly;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Host-side helpers: twiddle tables and workspaces, cached per device.
+
+struct DeviceCache {
+    std::map<int, cd *> tw;                 // n -> exp(-2 pi i m/n), m < n
+    void *work = nullptr;
+    size_t work_bytes = 0;
+    unsigned long long *scalar = nullptr;
+};
+
+static std::mutex g_mutex;
+static std::map<int, DeviceCache> g_cache;
+
+static int current_device(int &dev) { return hip_check(hipGetDevice(&dev), "hipGetDevice"); }
+
+int twiddles(int n, const cd **out) {
+    int dev;
+    if (int rc = current_device(dev)) return rc;
+    std::lock_guard<std::mutex> lk(g_mutex);
+    DeviceCache &c = g_cache[dev];
+    auto it = c.tw.find(n);
+    if (it != c.tw.end()) {
+        *out = it->second;
+        return CMM_OK;
+    }
+    std::vector<cd> h(n);
+    for (int m = 0; m < n; ++m) {
+        long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)m / (long double)n;
+        h[m] = cd{(double)cosl(a), (double)sinl(a)};
+    }
+    cd *d = nullptr;
+    if (hipMalloc(&d, sizeof(cd) * n) != hipSuccess) {
+        set_error("hipMalloc twiddles");
+        return CMM_ERR_ALLOC;
+    }
+    if (int rc = hip_check(hipMemcpy(d, h.data(), sizeof(cd) * n, hipMemcpyHostToDevice), "hipMemcpy twiddles")) return rc;
+    c.tw[n] = d;
+    *out = d;
+    return CMM_OK;
+}
+
+int workspace(size_t bytes, void **out) {
+    int dev;
+    if (int rc = current_device(dev)) return rc;
+    std::lock_guard<std::mutex> lk(g_mutex);
+    DeviceCache &c = g_cache[dev];
+    if (c.work_bytes < bytes) {
+        if (c.work) (void)hipFree(c.work);
+        c.work = nullptr;
+        c.work_bytes = 0;
+        if (hipMalloc(&c.work, bytes) != hipSuccess) {
+            set_error("hipMalloc workspace");
+            return CMM_ERR_ALLOC;
+        }
+        c.work_bytes = bytes;
+    }
+    *out = c.work;
+    return CMM_OK;
+}
+
+int scalar_slot(unsigned long long **out) {
+    int dev;
+    if (int rc = current_device(dev)) return rc;
+    std::lock_guard<std::mutex> lk(g_mutex);
+    DeviceCache &c = g_cache[dev];
+    if (!c.scalar) {
+        if (hipMalloc(&c.scalar, 256) != hipSuccess) {
+            set_error("hipMalloc scalar");
+            return CMM_ERR_ALLOC;
+        }
+    }
+    *out = c.scalar;
+    return CMM_OK;
+}
+
+static bool is_pow2(int n) { return n >= 2 && (n & (n - 1)) == 0; }
+static int ilog2(int n) {
+    int l = 0;
+    while ((1 << l) < n) ++l;
+    return l;
+}
+static bool grid_ok(const cmm_grid &g) { return g.nx > 0 && g.ny > 0 && g.x1 > g.x0 && g.y1 > g.y0; }
+static int fft_grid_ok(const cmm_grid &g, const char *what) {
+    if (!grid_ok(g) || !is_pow2(g.nx) || !is_pow2(g.ny) || g.nx > 16 * kFftThreads || g.ny > 16 * kFftThreads || g.nx < 8 ||
+        g.ny < 8) {
+        set_error(std::string(what) + ": FFT grids must be powers of two between 8 and 4096");
+        return CMM_ERR_ARG;
+    }
+    return CMM_OK;
+}
+
+static int launch_check(const char *what) { return hip_check(hipGetLastError(), what); }
+
+static int set_lds(const void *fn, size_t bytes) {
+    if (bytes > 65536) return hip_check(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes), "hipFuncSetAttribute");
+    return CMM_OK;
+}
+
+// Spectral pipeline: row R2C (with a sampler) -> column stage -> row C2R of four planes.
+struct SpecPlan {
+    cmm_grid gin, gout;
+    int mode;
+    double fcut;
+    int kx_max;
+    size_t t_bytes, g_bytes;
+};
+
+static SpecPlan plan_spec(const cmm_grid &gin, const cmm_grid &gout, int mode, double fcut) {
+    SpecPlan s;
+    s.gin = gin;
+    s.gout = gout;
+    s.mode = mode;
+    s.fcut = fcut;
+    int k = gout.nx / 2;
+    if (mode == 1) {
+        k = std::min(k, gin.nx / 2);
+        if (fcut < 0) k = -1;
+        else k = std::min<double>(k, std::floor(fcut));
+    }
+    s.kx_max = k;
+    const int ncol = k + 1 > 0 ? k + 1 : 1;
+    s.t_bytes = sizeof(cd) * (size_t)ncol * gin.ny;
+    s.g_bytes = sizeof(cd) * (size_t)ncol * gout.ny;
+    return s;
+}
+
+size_t spec_workspace_bytes(const cmm_grid &gin, const cmm_grid &gout, int mode, double fcut) {
+    SpecPlan s = plan_spec(gin, gout, mode, fcut);
+    return s.t_bytes + 2 * s.g_bytes + 256;
+}
+
+// ev (optional): four events recorded before the row transform and after each of the three kernels
+template <class S>
+static int run_spec(const SpecPlan &sp, const S &sampler, double *out, void *work, hipStream_t st,
+                    hipEvent_t *ev = nullptr) {
+    const cd *tw_in, *tw_out, *tw_x_in, *tw_x_out;
+    if (int rc = twiddles(sp.gin.nx, &tw_x_in)) return rc;
+    if (int rc = twiddles(sp.gin.ny, &tw_in)) return rc;
+    if (int rc = twiddles(sp.gout.ny, &tw_out)) return rc;
+    if (int rc = twiddles(sp.gout.nx, &tw_x_out)) return rc;
+    cd *T = (cd *)work;
+    cd *G0 = (cd *)((char *)work + sp.t_bytes);
+    cd *G1 = (cd *)((char *)work + sp.t_bytes + sp.g_bytes);
+    const int kxm = sp.kx_max;
+    const DGrid dout = make_dgrid(sp.gout);
+    if (ev) (void)hipEventRecord(ev[0], st);
+    if (kxm >= 0) {
+        if (sp.mode != 2) {
+            size_t lds = sizeof(cd) * sp.gin.nx;
+            if (int rc = set_lds((const void *)k_row_r2c<S>, lds)) return rc;
+            hipLaunchKernelGGL(k_row_r2c<S>, dim3(sp.gin.ny / 2), dim3(kFftThreads), lds, st, sampler, sp.gin.nx, sp.gin.ny,
+                               ilog2(sp.gin.nx), kxm, tw_x_in, T);
+            if (int rc = launch_check("k_row_r2c")) return rc;
+        }
+        if (ev) (void)hipEventRecord(ev[1], st);
+        ColParams p;
+        p.nin = sp.gin.ny;
+        p.login = ilog2(sp.gin.ny);
+        p.nout = sp.gout.ny;
+        p.logout = ilog2(sp.gout.ny);
+        p.nx_in = sp.gin.nx;
+        p.nx_out = sp.gout.nx;
+        p.inv_norm = 1.0 / ((double)sp.gin.nx * (double)sp.gin.ny);
+        p.fcut = sp.fcut;
+        p.fkx = kTwoPi / (dout.hx * sp.gout.nx);
+        p.fky = kTwoPi / (dout.hy * sp.gout.ny);
+        p.mode = sp.mode;
+        p.kx_max = kxm;
+        size_t lds = sizeof(cd) * std::max(sp.gin.ny, sp.gout.ny);
+        if (int rc = set_lds((const void *)k_col, lds)) return rc;
+        hipLaunchKernelGGL(k_col, dim3(kxm + 1), dim3(kFftThreads), lds, st, p, T, tw_in, tw_out, G0, G1);
+        if (int rc = launch_check("k_col")) return rc;
+    } else if (ev) {
+        (void)hipEventRecord(ev[1], st);
+    }
+    if (ev) (void)hipEventRecord(ev[2], st);
+    {
+        size_t lds = sizeof(cd) * sp.gout.nx;
+        if (int rc = set_lds((const void *)k_row_c2r4, lds)) return rc;
+        hipLaunchKernelGGL(k_row_c2r4, dim3(sp.gout.ny), dim3(kFftThreads), lds, st, sp.gout.nx, sp.gout.ny,
+                           ilog2(sp.gout.nx), kxm, kTwoPi / (dout.hx * sp.gout.nx), G0, G1, tw_x_out, out);
+        if (int rc = launch_check("k_row_c2r4")) return rc;
+    }
+    if (ev) (void)hipEventRecord(ev[3], st);
+    return CMM_OK;
+}
+
+static int get_work(void *work, size_t work_bytes, size_t need, void **w) {
+    if (work) {
+        if (work_bytes < need) {
+            set_error("workspace too small");
+            return CMM_ERR_ARG;
+        }
+        *w = work;
+        return CMM_OK;
+    }
+    return workspace(need, w);
+}
+
+static ICParams make_ic(int ic, const double *prm) {
+    ICParams p;
+    p.ic = ic;
+    for (int i = 0; i < 10; ++i) p.prm[i] = prm ? prm[i] : 0.0;
+    return p;
+}
+
+static dim3 grid2d(int nx, int ny, dim3 b) { return dim3((nx + b.x - 1) / b.x, (ny + b.y - 1) / b.y); }
+
+// ------------------------------------------------------------------------------------------------
+// Operations shared by the C-ABI and the engine
+
+int op_map_identity(const cmm_grid &g, double *cx, double *cy, hipStream_t st) {
+    DGrid d = make_dgrid(g);
+    dim3 b(64, 4);
+    hipLaunchKernelGGL(k_map_identity, grid2d(g.nx, g.ny, b), b, 0, st, d, cx, cy);
+    return launch_check("k_map_identity");
+}
+
+int op_map_advect(const cmm_grid &gm, const cmm_grid &gp, const double *cx, const double *cy, const PsiLevels &P,
+                  const cmm_advect_coeffs &c, double *ncx, double *ncy, hipStream_t st) {
+    DGrid dm = make_dgrid(gm), dp = make_dgrid(gp);
+    dim3 b(64, 4);
+    const int tx = (gm.nx + 63) / 64, ty = (gm.ny + 3) / 4, tt = tx * ty;
+#define CMM_ADV(T) hipLaunchKernelGGL(k_map_advect<T>, dim3(tt), b, 0, st, dm, dp, cx, cy, P, c, ncx, ncy, tx, tt)
+    switch (c.time_integration_num) {
+    case 10: CMM_ADV(10); break;
+    case 20: CMM_ADV(20); break;
+    case 21: CMM_ADV(21); break;
+    case 30: CMM_ADV(30); break;
+    case 31: CMM_ADV(31); break;
+    case 40: CMM_ADV(40); break;
+    case 41: CMM_ADV(41); break;
+    default: set_error("unknown time integration"); return CMM_ERR_ARG;
+    }
+#undef CMM_ADV
+    return launch_check("k_map_advect");
+}
+
+int op_incomp_launch(const cmm_grid &gm, const cmm_grid &gc, const double *cx, const double *cy,
+                     unsigned long long *slot, hipStream_t st) {
+    DGrid dm = make_dgrid(gm), dc = make_dgrid(gc);
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, slot);
+    dim3 b(64, 4);
+    const int tx = (gc.nx + 63) / 64, ty = (gc.ny + 3) / 4, tt = tx * ty;
+    hipLaunchKernelGGL(k_incomp, dim3(tt), b, 0, st, dm, dc, cx, cy, slot, tx, tt);
+    return launch_check("k_incomp");
+}
+
+int op_incomp(const cmm_grid &gm, const cmm_grid &gc, const double *cx, const double *cy, double *out,
+              hipStream_t st) {
+    unsigned long long *slot;
+    if (int rc = scalar_slot(&slot)) return rc;
+    if (int rc = op_incomp_launch(gm, gc, cx, cy, slot, st)) return rc;
+    unsigned long long h = 0;
+    if (int rc = hip_check(hipMemcpyAsync(&h, slot, sizeof(h), hipMemcpyDeviceToHost, st), "hipMemcpyAsync")) return rc;
+    if (int rc = hip_check(hipStreamSynchronize(st), "hipStreamSynchronize")) return rc;
+    double v;
+    std::memcpy(&v, &h, sizeof(v));
+    *out = v;
+    return CMM_OK;
+}
+
+size_t op_psi_workspace(const cmm_grid &gv, const cmm_grid &gp, double fcut) {
+    return spec_workspace_bytes(gv, gp, 1, fcut);
+}
+
+int op_psi_from_vort(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf, const cmm_grid &gp, const double *cx,
+                     const double *cy, const double *W, int molly, double fcut, double *psi, void *work,
+                     size_t work_bytes, hipStream_t st, hipEvent_t *ev) {
+    if (int rc = fft_grid_ok(gv, "vort grid")) return rc;
+    if (int rc = fft_grid_ok(gp, "psi grid")) return rc;
+    SpecPlan sp = plan_spec(gv, gp, 1, fcut);
+    void *w;
+    if (int rc = get_work(work, work_bytes, sp.t_bytes + 2 * sp.g_bytes + 256, &w)) return rc;
+    SampleVort s;
+    s.gm = make_dgrid(gm);
+    s.gv = make_dgrid(gv);
+    s.gf = make_dgrid(gf);
+    s.cx = cx;
+    s.cy = cy;
+    s.W = W;
+    s.molly = molly;
+    return run_spec(sp, s, psi, w, st, ev);
+}
+
+size_t op_hermite_workspace(const cmm_grid &g) { return spec_workspace_bytes(g, g, 0, 0.0); }
+
+int op_hermite_from_real(const cmm_grid &g, const double *real, double *herm, void *work, size_t work_bytes,
+                         hipStream_t st) {
+    if (int rc = fft_grid_ok(g, "grid")) return rc;
+    SpecPlan sp = plan_spec(g, g, 0, 0.0);
+    void *w;
+    if (int rc = get_work(work, work_bytes, sp.t_bytes + 2 * sp.g_bytes + 256, &w)) return rc;
+    SampleReal s{real, g.nx};
+    return run_spec(sp, s, herm, w, st);
+}
+
+int op_hermite_from_spectrum(const cmm_grid &g, const double *spec_cols, double *herm, void *work,
+                             size_t work_bytes, hipStream_t st) {
+    if (int rc = fft_grid_ok(g, "grid")) return rc;
+    SpecPlan sp = plan_spec(g, g, 2, 0.0);
+    void *w;
+    if (int rc = get_work(work, work_bytes, sp.t_bytes + 2 * sp.g_bytes + 256, &w)) return rc;
+    if (int rc = hip_check(hipMemcpyAsync(w, spec_cols, sp.t_bytes, hipMemcpyDeviceToDevice, st), "hipMemcpyAsync")) return rc;
+    SampleReal s{nullptr, g.nx};
+    return run_spec(sp, s, herm, w, st);
+}
+
+int op_translate_ic(const cmm_grid &gm, const cmm_grid &gf, const double *cx, const double *cy, const double *stack,
+                    int nstack, int ic, const double *prm, int discrete, const cmm_grid &gd, const double *disc,
+                    double *herm, void *work, size_t work_bytes, hipStream_t st) {
+    if (int rc = fft_grid_ok(gf, "fine grid")) return rc;
+    SpecPlan sp = plan_spec(gf, gf, 0, 0.0);
+    void *w;
+    if (int rc = get_work(work, work_bytes, sp.t_bytes + 2 * sp.g_bytes + 256, &w)) return rc;
+    SampleCompose s;
+    s.gm = make_dgrid(gm);
+    s.gt = make_dgrid(gf);
+    s.gd = make_dgrid(gd);
+    s.cx = cx;
+    s.cy = cy;
+    s.stack = stack;
+    s.nstack = nstack;
+    s.ic = make_ic(ic, prm);
+    s.discrete = discrete;
+    s.disc = disc;
+    return run_spec(sp, s, herm, w, st);
+}
+
+int op_compose_positions(const cmm_grid &gm, const cmm_grid &gt, const double *cx, const double *cy,
+                         const double *stack, int nstack, double *xy, hipStream_t st) {
+    SampleCompose s;
+    s.gm = make_dgrid(gm);
+    s.gt = make_dgrid(gt);
+    s.gd = s.gt;
+    s.cx = cx;
+    s.cy = cy;
+    s.stack = stack;
+    s.nstack = nstack;
+    s.ic = make_ic(0, nullptr);
+    s.discrete = 0;
+    s.disc = nullptr;
+    dim3 b(64, 4);
+    hipLaunchKernelGGL(k_compose_positions, grid2d(gt.nx, gt.ny, b), b, 0, st, s, xy);
+    return launch_check("k_compose_positions");
+}
+
+int op_sample_init(const cmm_grid &gm, const cmm_grid &gt, const double *cx, const double *cy, const double *stack,
+                   int nstack, int ic, const double *prm, int discrete, const cmm_grid &gd, const double *disc,
+                   double *out, hipStream_t st) {
+    SampleCompose s;
+    s.gm = make_dgrid(gm);
+    s.gt = make_dgrid(gt);
+    s.gd = make_dgrid(gd);
+    s.cx = cx;
+    s.cy = cy;
+    s.stack = stack;
+    s.nstack = nstack;
+    s.ic = make_ic(ic, prm);
+    s.discrete = discrete;
+    s.disc = disc;
+    dim3 b(64, 4);
+    hipLaunchKernelGGL(k_sample_grid<SampleCompose>, grid2d(gt.nx, gt.ny, b), b, 0, st, s, gt.nx, gt.ny, out);
+    return launch_check("k_sample_grid");
+}
+
+int op_sample_vort(const cmm_grid &gm, const cmm_grid &gt, const cmm_grid &gf, const double *cx, const double *cy,
+                   const double *W, int molly, double *out, hipStream_t st) {
+    SampleVort s;
+    s.gm = make_dgrid(gm);
+    s.gv = make_dgrid(gt);
+    s.gf = make_dgrid(gf);
+    s.cx = cx;
+    s.cy = cy;
+    s.W = W;
+    s.molly = molly;
+    dim3 b(64, 4);
+    hipLaunchKernelGGL(k_sample_grid<SampleVort>, grid2d(gt.nx, gt.ny, b), b, 0, st, s, gt.nx, gt.ny, out);
+    return launch_check("k_sample_grid");
+}
+
+int op_particles(const cmm_grid &gp, const PsiLevels &P, int L, const double *Lp, double dt, int scheme, int n,
+                 double *pos, hipStream_t st) {
+    if (n <= 0) return CMM_OK;
+    PartParams q;
+    for (int i = 0; i < 4; ++i) q.Lp[i] = (Lp && i < L) ? Lp[i] : 0.0;
+    q.dt = dt;
+    q.L = L;
+    q.scheme = scheme;
+    q.n = n;
+    hipLaunchKernelGGL(k_particles, dim3((n + 255) / 256), dim3(256), 0, st, make_dgrid(gp), P, q, pos);
+    return launch_check("k_particles");
+}
+
+}  // namespace cmm

@@ -1,0 +1,521 @@
+// Euler time loop of a CMM run, driving libcmm.so through its C-ABI (include/cmm.h).
+// Order of operations, monitoring files and output layout follow Cuda-CMM/src/simulation/cmm-euler2d.cu
+// and Cuda-CMM/src/ui/cmm-io.cu.
+#include "driver.hpp"
+
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <ctime>
+#include <dirent.h>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/cmm.h"
+
+namespace cmmhost {
+
+namespace {
+
+using Clock = std::chrono::high_resolution_clock;
+
+std::string to_s(int v) { return std::to_string(v); }
+std::string zpad(int v, int w) {
+    std::ostringstream os;
+    os.width(w);
+    os.fill('0');
+    os << v;
+    return os.str();
+}
+
+std::string now_string() {
+    time_t t = time(nullptr);
+    struct tm tm = *localtime(&t);
+    char buf[80];
+    strftime(buf, sizeof(buf), "%Y-%m-%d.%X", &tm);
+    return buf;
+}
+
+std::string duration(double sec) {
+    return fmt3(std::floor(sec / 3600.0)) + "h " + fmt3(std::floor(std::fmod(sec, 3600) / 60.0)) + "m " +
+           fmt3(std::fmod(sec, 60)) + "s";
+}
+
+// log.txt plus stdout, Cuda-CMM/src/ui/cmm-io.cu:692-731
+struct Logger {
+    std::string path;
+    explicit Logger(const Settings &s) {
+        path = s.workspace + "data/" + s.file_name + "/log.txt";
+        std::ofstream f(path, s.restart_time == 0 ? std::ios::out : std::ios::app);
+        if (!f) {
+            std::cerr << "Unable to open log file.. exitting" << std::endl;
+            std::exit(0);
+        }
+        if (s.restart_time == 0) f << s.file_name << std::endl;
+        else f << "\nContinuing simulation\n" << std::endl;
+    }
+    void push(const std::string &m) const {
+        std::ofstream f(path, std::ios::app);
+        if (f) f << "[" << now_string() << "]\t" << m << std::endl;
+        std::cout << m << std::endl;
+    }
+};
+
+void mkdir_p(const std::string &p) { mkdir(p.c_str(), 0777); }
+
+void remove_tree(const std::string &p) {
+    DIR *d = opendir(p.c_str());
+    if (!d) return;
+    while (struct dirent *e = readdir(d)) {
+        std::string n = e->d_name;
+        if (n == "." || n == "..") continue;
+        std::string q = p + "/" + n;
+        struct stat st;
+        if (lstat(q.c_str(), &st) == 0 && S_ISDIR(st.st_mode)) remove_tree(q);
+        else unlink(q.c_str());
+    }
+    closedir(d);
+    rmdir(p.c_str());
+}
+
+// Cuda-CMM/src/ui/cmm-io.cu:23-64
+void create_directories(const Settings &s) {
+    const std::string base = s.workspace + "data/" + s.file_name;
+    mkdir_p(s.workspace + "data");
+    mkdir_p(base);
+    if (s.restart_time == 0) remove_tree(base + "/Monitoring_data");
+    mkdir_p(base + "/Monitoring_data");
+    mkdir_p(base + "/Monitoring_data/Mesure");
+    if (s.verbose >= 4) mkdir_p(base + "/Monitoring_data/Debug_globals");
+    mkdir_p(base + "/Time_data");
+    if (!s.save_zoom.empty()) mkdir_p(base + "/Zoom_data");
+    if (!s.particles_advected.empty() || !s.particles_forwarded.empty()) mkdir_p(base + "/Particle_data");
+}
+
+// raw little-endian doubles, Cuda-CMM/src/ui/cmm-io.cu:73-103
+void write_file(const Settings &s, const std::string &name, const double *v, size_t n, bool append) {
+    std::string f = s.workspace + "data/" + s.file_name + name + ".data";
+    std::ofstream o(f, std::ios::binary | (append ? std::ios::app : std::ios::out));
+    if (!o) {
+        std::cout << "Error saving file. Unable to open : " << f << std::endl;
+        return;
+    }
+    o.write(reinterpret_cast<const char *>(v), sizeof(double) * n);
+}
+
+void append1(const Settings &s, const std::string &name, double v) { write_file(s, name, &v, 1, true); }
+
+// Debug statistics instead of arrays at verbose >= 4, Cuda-CMM/src/ui/cmm-io.cu:508-534 and
+// Cuda-CMM/src/numerical/cmm-mesure.cu:220-250: on a grid "Avg" is max(0, max) h^2 and "L2" is
+// sum h^2; on a plain array they are max(0, max)/N and sum/N.  The hash files are not written.
+void debug_globals(const Settings &s, const std::string &name, const double *v, size_t n, double area) {
+    double mn = v[0], mx = v[0], mxz = 0.0, sum = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        mn = std::min(mn, v[i]);
+        mx = std::max(mx, v[i]);
+        mxz = std::max(mxz, v[i]);
+        sum += v[i];
+    }
+    const std::string p = "/Monitoring_data/Debug_globals/" + name;
+    append1(s, p + "_Min", mn);
+    append1(s, p + "_Max", mx);
+    append1(s, p + "_Avg", area > 0 ? area * mxz : mxz / (double)n);
+    append1(s, p + "_L2", area > 0 ? area * sum : sum / (double)n);
+}
+
+uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+// Particle initial positions, Cuda-CMM/src/simulation/cmm-init.cu:311-455.  Random sets draw
+// u_j = mix64(mix64(seed) + j) >> 11 / 2^53 (j = 2i, 2i+1) in place of cuRAND.
+std::vector<double> init_particles(const ParticlesAdvected &p, const double *b) {
+    const int n = p.num;
+    std::vector<double> pos(2 * (size_t)n);
+    const double lx = b[1] - b[0], ly = b[3] - b[2];
+    const uint64_t s0 = mix64(p.seed);
+    auto u = [&](uint64_t j) { return (double)(mix64(s0 + j) >> 11) * (1.0 / 9007199254740992.0); };
+    const int len = (int)std::ceil(std::sqrt((double)n));
+    for (int i = 0; i < n; ++i) {
+        double x = 0, y = 0;
+        switch (p.init_num) {
+        case 0:
+            x = p.p1 + p.p3 * (u(2 * (uint64_t)i) - 0.5);
+            y = p.p2 + p.p4 * (u(2 * (uint64_t)i + 1) - 0.5);
+            break;
+        case 1: {  // normal around 0.5 with unit deviation (Box-Muller), then the same rescale
+            double u1 = std::max(u(2 * (uint64_t)i), 1e-300), u2 = u(2 * (uint64_t)i + 1);
+            double r = std::sqrt(-2.0 * std::log(u1));
+            x = p.p1 + p.p3 * (0.5 + r * std::cos(kTwoPi * u2) - 0.5);
+            y = p.p2 + p.p4 * (0.5 + r * std::sin(kTwoPi * u2) - 0.5);
+            break;
+        }
+        case 2:
+            x = p.p1 + p.p3 * std::cos(i / (double)n * kTwoPi);
+            y = p.p2 + p.p4 * std::sin(i / (double)n * kTwoPi);
+            break;
+        case 3:
+            x = p.p1 + p.p3 * (-1 + 2 * (i % len) / (double)len);
+            y = p.p2 + p.p4 * (-1 + 2 * (i / len) / (double)len);
+            break;
+        case 4:
+            x = i / (double)n * kTwoPi + p.p1;
+            y = std::sin(i / (double)n * kTwoPi) / 2.0 + p.p2;
+            break;
+        default:
+            break;
+        }
+        pos[2 * i] = x - std::floor((x - b[0]) / lx) * lx;
+        pos[2 * i + 1] = y - std::floor((y - b[2]) / ly) * ly;
+    }
+    return pos;
+}
+
+bool save_due(bool is_instant, double ts, double te, double tstep, double t_now, double dt_now, double dt) {
+    // Cuda-CMM/src/ui/cmm-io.cu:300-312
+    if (is_instant) return t_now - ts + dt * 1e-5 < dt_now && t_now - ts + dt * 1e-5 >= 0;
+    return (std::fmod(t_now - ts + dt * 1e-5, tstep) < dt_now && t_now + dt * 1e-5 >= ts && t_now - dt * 1e-5 <= te) ||
+           t_now == te;
+}
+
+struct Run {
+    Settings &s;
+    Logger log;
+    cmm_euler *e = nullptr;
+    cmm_grid gc, gf, gp;
+    int L;
+    std::vector<int> part_set;  // engine set per advected particle entry (-1 if not advected here)
+    Run(Settings &st) : s(st), log(st) {}
+
+    void check(int rc, const char *what) {
+        if (rc != CMM_OK) {
+            log.push(std::string("Exitted early : ") + what + " failed: " + cmm_last_error());
+            std::exit(0);
+        }
+    }
+
+    std::vector<double> field(int f, size_t n) {
+        std::vector<double> v(n);
+        check(cmm_euler_get_field(e, f, v.data()), "cmm_euler_get_field");
+        return v;
+    }
+
+    void write_var(const std::string &sub, const std::string &name, const double *v, size_t n, double area) {
+        if (s.verbose < 4) write_file(s, sub + "/" + name, v, n, false);
+        else debug_globals(s, name, v, n, area);
+    }
+
+    // writeTimeStep, Cuda-CMM/src/ui/cmm-io.cu:295-437
+    std::string write_time_step(double t_now, double dt_now, double dt) {
+        bool now = false;
+        std::string var;
+        for (auto &c : s.save_computational)
+            if (save_due(c.is_instant, c.time_start, c.time_end, c.time_step, t_now, dt_now, dt)) {
+                now = true;
+                var += c.var;
+            }
+        if (!now) return "";
+        std::string msg;
+        std::string ts = std::abs(t_now - kTMax) < 1 ? "final" : fmt3(t_now);
+        const bool all = var.find("All") != std::string::npos;
+        auto has = [&](const char *k) { return var.find(k) != std::string::npos; };
+        const long nc = (long)gc.nx * gc.ny, np = (long)gp.nx * gp.ny;
+        const double ac = ((gc.x1 - gc.x0) / (float)gc.nx) * ((gc.y1 - gc.y0) / (float)gc.ny);
+        const double ap = ((gp.x1 - gp.x0) / (float)gp.nx) * ((gp.y1 - gp.y0) / (float)gp.ny);
+        const std::string sub = "/Time_data/Time_" + ts;
+        bool saved_any = false;
+        if (has("Vorticity") || has("W") || has("Stream") || has("Psi") || has("Map") || has("Chi") || has("Velocity") ||
+            has("U") || has("Dist") || has("F") || all) {
+            mkdir_p(s.workspace + "data/" + s.file_name + sub);
+            if (s.verbose >= 4) append1(s, "/Monitoring_data/Debug_globals/Time_s", t_now);
+            saved_any = true;
+            msg = "Processed computational data";
+        }
+        if (has("Vorticity") || has("W") || all || has("Dist") || has("F")) {
+            auto w = field(CMM_FIELD_W_COARSE, nc);
+            if (has("Vorticity") || has("W") || all) write_var(sub, "Vorticity_W_coarse", w.data(), nc, ac);
+            if (has("Dist") || has("F") || all) write_var(sub, "Distribution_coarse", w.data(), nc, ac);
+        }
+        if (has("Stream") || has("Psi") || has("Velocity") || has("U") || all) {
+            auto psi = field(CMM_FIELD_PSI, 4 * np * L);
+            if (has("Stream") || has("Psi") || all) write_var(sub, "Stream_function_Psi_psi", psi.data(), np, ap);
+            if (has("Stream_H") || has("Psi_H") || all) write_var(sub, "Stream_function_Psi_H_psi", psi.data(), 4 * np, 0);
+            if (has("Velocity") || has("U") || all) {
+                write_var(sub, "Velocity_UX_psi", psi.data() + np, np, ap);
+                write_var(sub, "Velocity_UY_psi", psi.data() + 2 * np, np, ap);
+            }
+        }
+        if (has("Map_b") || has("Chi_b") || has("Map_H") || has("Chi_H") || all) {
+            auto cx = field(CMM_FIELD_CHIX, 4 * nc), cy = field(CMM_FIELD_CHIY, 4 * nc);
+            if (has("Map_b") || has("Chi_b") || all) {
+                write_var(sub, "Map_ChiX_coarse", cx.data(), nc, ac);
+                write_var(sub, "Map_ChiY_coarse", cy.data(), nc, ac);
+            }
+            if (has("Map_H") || has("Chi_H") || all) {
+                write_var(sub, "Map_ChiX_H_coarse", cx.data(), 4 * nc, 0);
+                write_var(sub, "Map_ChiY_H_coarse", cy.data(), 4 * nc, 0);
+            }
+        }
+        // particle positions, Cuda-CMM/src/ui/cmm-io.cu:396-434
+        const std::string psub = "/Particle_data/Time_" + ts;
+        if (has("PartA_") || all) {
+            mkdir_p(s.workspace + "data/" + s.file_name + psub);
+            if (!saved_any && s.verbose >= 4) append1(s, "/Monitoring_data/Debug_globals/Time_s", t_now);
+            msg = "Processed particle data";
+        }
+        for (size_t i = 0; i < s.particles_advected.size(); ++i) {
+            const std::string tag = zpad((int)i + 1, 2);
+            if ((has(("PartA_" + tag).c_str()) || all) && part_set[i] >= 0) {
+                std::vector<double> pos(2 * (size_t)s.particles_advected[i].num);
+                check(cmm_euler_get_particles(e, part_set[i], pos.data()), "cmm_euler_get_particles");
+                if (s.verbose < 4) write_file(s, psub + "/Particles_advected_pos_P" + tag, pos.data(), pos.size(), false);
+                else debug_globals(s, "/Particles_advected_pos_P" + tag, pos.data(), pos.size(), 0);
+            }
+        }
+        return msg;
+    }
+
+    // compute_conservation_targets, Cuda-CMM/src/simulation/cmm-simulation-host.cu:615-698
+    std::string conservation(double t_now, double dt_now, double dt) {
+        bool now = false;
+        for (auto &c : s.save_computational)
+            if (c.conv && save_due(c.is_instant, c.time_start, c.time_end, c.time_step, t_now, dt_now, dt)) now = true;
+        if (!now) return "";
+        double m[4];
+        check(cmm_euler_diagnostics(e, m), "cmm_euler_diagnostics");
+        append1(s, "/Monitoring_data/Mesure/Time_s", t_now);
+        append1(s, "/Monitoring_data/Mesure/Energy", m[0]);
+        append1(s, "/Monitoring_data/Mesure/Enstrophy", m[1]);
+        append1(s, "/Monitoring_data/Mesure/Palinstrophy", m[2]);
+        append1(s, "/Monitoring_data/Mesure/Max_vorticity", m[3]);
+        auto f8 = [](double v) {
+            std::ostringstream os;
+            os.precision(8);
+            os << v;
+            return os.str();
+        };
+        return "Computed coarse Cons : Energ = " + f8(m[0]) + " \t Enstr = " + f8(m[1]) + " \t Palinstr = " + f8(m[2]) +
+               " \t Wmax = " + f8(m[3]);
+    }
+
+    void save(double t_now, double dt_now, double dt) {
+        std::string m = write_time_step(t_now, dt_now, dt);
+        if (s.verbose >= 3 && !m.empty()) log.push(m);
+        m = conservation(t_now, dt_now, dt);
+        if (s.verbose >= 3 && !m.empty()) log.push(m);
+    }
+};
+
+}  // namespace
+
+double compute_next_timestep(const Settings &s, double t, double dt) {
+    double dt_now = dt;
+    const double e = dt * 1e-5;
+    for (auto &p : s.particles_advected)
+        if (t + e - p.init_time < 0 && t + e + dt - p.init_time > 0) dt_now = std::fmin(dt_now, p.init_time - t);
+    for (auto &p : s.particles_forwarded)
+        if (t + e - p.init_time < 0 && t + e + dt - p.init_time > 0) dt_now = std::fmin(dt_now, p.init_time - t);
+    auto targets = [&](bool inst, double ts, double te, double step) {
+        if (inst && t + e - ts < 0 && t + e + dt - ts > 0) dt_now = std::fmin(dt_now, ts - t);
+        if (!inst &&
+            (std::fmod(t + e - ts, step) > std::fmod(t + dt + e - ts, step) ||
+             (std::fmod(t + e - ts, step) < 0 && std::fmod(t + dt + e - ts, step) > 0)) &&
+            t + dt + e >= ts && t - e <= te)
+            dt_now = std::fmin(dt_now, step - std::fmod(t, step));
+    };
+    for (auto &c : s.save_computational) targets(c.is_instant, c.time_start, c.time_end, c.time_step);
+    for (auto &c : s.save_sample) targets(c.is_instant, c.time_start, c.time_end, c.time_step);
+    for (auto &c : s.save_zoom) targets(c.is_instant, c.time_start, c.time_end, c.time_step);
+    return dt_now;
+}
+
+int run_euler(Settings &s) {
+    const auto begin = Clock::now();
+    auto ctime = [&]() { return std::chrono::duration_cast<std::chrono::microseconds>(Clock::now() - begin).count() / 1e6; };
+
+    // dt and Lagrange order, Cuda-CMM/src/simulation/cmm-euler2d.cu:62-78
+    const double dt = s.set_dt_by_steps ? 1.0 / s.steps_per_sec
+                                        : 1.0 / std::max(s.grid_coarse, s.grid_coarse) / s.factor_dt_by_grid;
+    for (auto &p : s.particles_advected) {
+        int need = (p.time_integration_num == 20) ? 2 : (p.time_integration_num == 30 || p.time_integration_num == 31) ? 3
+                 : (p.time_integration_num == 40 || p.time_integration_num == 41) ? 4 : 0;
+        if (s.lagrange_order < need) s.lagrange_order = need;
+    }
+    if (s.lagrange_override != -1) s.lagrange_order = s.lagrange_override;
+    const int iter_max = (int)(1.05 * std::ceil((s.final_time - s.restart_time) / dt));
+    s.file_name = s.sim_name + "_" + s.initial_condition + "_C" + to_s(s.grid_coarse) + "_F" + to_s(s.grid_fine) + "_t" +
+                  to_s((int)(1.0 / dt)) + "_T" + fmt3(s.final_time);
+    create_directories(s);
+    Run r(s);
+    Logger &log = r.log;
+    if (s.verbose >= 2) log.push("Starting memory initialization \t C-Time = " + duration(ctime()));
+    if (s.verbose >= 3) {
+        char buf[512];
+        if (cmm_device_info(buf, sizeof(buf)) == CMM_OK) log.push(std::string("Device = ") + buf);
+    }
+    if (s.verbose >= 1) {
+        log.push("Solving = " + s.simulation_type);
+        log.push("Initial condition = " + s.initial_condition);
+        log.push("Iter max = " + to_s(iter_max));
+        log.push("Name of simulation = " + s.file_name);
+    }
+    // parts of the reference that are not on this engine's path stop here with a message
+    if (s.forward_map) fail("forward_map is not supported by this engine (backward map only)", 1);
+    if (s.restart_time != 0) fail("restart from a saved map stack is not supported by this engine yet", 1);
+    if (s.initial_condition_num >= 7 && s.initial_condition_num <= 8)
+        fail("initial condition " + s.initial_condition + " draws from cuRAND/std distributions and is not provided", 230305);
+    if (!s.save_sample.empty() && s.verbose >= 2) log.push("Note: save_sample outputs are not produced by this engine");
+    if (!s.save_zoom.empty() && s.verbose >= 2) log.push("Note: save_zoom outputs are not produced by this engine");
+
+    const double *b = s.domain_bounds;
+    r.gc = cmm_grid{s.grid_coarse, s.grid_coarse, b[0], b[1], b[2], b[3]};
+    r.gf = cmm_grid{s.grid_fine, s.grid_fine, b[0], b[1], b[2], b[3]};
+    r.gp = cmm_grid{s.grid_psi, s.grid_psi, b[0], b[1], b[2], b[3]};
+    cmm_grid gv{s.grid_vort, s.grid_vort, b[0], b[1], b[2], b[3]};
+    r.L = s.lagrange_order;
+
+    cmm_euler_params p{};
+    p.g_coarse = r.gc;
+    p.g_fine = r.gf;
+    p.g_psi = r.gp;
+    p.g_vort = gv;
+    p.dt = dt;
+    p.time_integration_num = s.time_integration_num;
+    p.map_update_order_num = s.map_update_order_num;
+    p.l_order = s.lagrange_order;
+    p.lagrange_init_higher_order = s.lagrange_init_higher_order;
+    p.map_epsilon = s.map_epsilon;
+    p.incomp_threshold = s.incomp_threshold;
+    p.freq_cut_psi = s.freq_cut_psi;
+    p.molly_stencil = s.molly_stencil;
+    p.skip_remapping = s.skip_remapping;
+    p.ic = s.initial_condition_num;
+    for (int i = 0; i < 10; ++i) p.ic_params[i] = s.initial_params[i];
+    std::vector<double> disc;
+    if (s.initial_condition_num == CMM_IC_RANDOM_SPECTRUM) {
+        p.discrete = 1;
+        p.g_disc = r.gf;
+        p.seed = s.seed;
+    } else if (s.initial_discrete) {
+        // discrete initial condition read from file, Cuda-CMM/src/simulation/cmm-euler2d.cu:325-344
+        p.discrete = 1;
+        p.g_disc = cmm_grid{s.initial_discrete_grid, s.initial_discrete_grid, b[0], b[1], b[2], b[3]};
+        disc.resize((size_t)s.initial_discrete_grid * s.initial_discrete_grid);
+        std::ifstream f(s.initial_discrete_location, std::ios::binary);
+        if (!f || !f.read(reinterpret_cast<char *>(disc.data()), sizeof(double) * disc.size())) {
+            log.push("Unable to read discrete initial condition .. exitting");
+            std::exit(0);
+        }
+        p.disc_real = disc.data();
+        if (s.verbose >= 1) log.push("Using discrete initial condition");
+    }
+    // stack capacity from the memory budget, Cuda-CMM/src/simulation/cmm-euler2d.cu:366-379
+    const double map_size = 8.0 * sizeof(double) * r.gc.nx * r.gc.ny / 1e6;
+    const int cap = (int)(double(s.mem_RAM_CPU_remaps) / map_size);
+    p.stack_capacity = cap;
+    if (s.verbose >= 1) {
+        log.push("Map size in MB = " + to_s((int)map_size));
+        log.push("Map stack length on device = " + to_s(cap));
+    }
+    r.check(cmm_euler_create(&p, &r.e), "cmm_euler_create");
+
+    // particles, Cuda-CMM/src/simulation/cmm-euler2d.cu:423-450 (fluid particles)
+    for (size_t i = 0; i < s.particles_advected.size(); ++i) {
+        const auto &pa = s.particles_advected[i];
+        r.part_set.push_back(-1);
+        if (pa.tau != 0) {
+            if (s.verbose >= 1) log.push("Particles set P" + zpad((int)i + 1, 2) + " : inertial particles (tau != 0) are not advected by this engine");
+            continue;
+        }
+        auto pos = init_particles(pa, b);
+        int set = -1;
+        r.check(cmm_euler_add_particles(r.e, pa.num, pa.time_integration_num, pos.data(), &set), "cmm_euler_add_particles");
+        r.part_set.back() = set;
+        if (s.verbose >= 1) {
+            std::ostringstream os;
+            os.precision(8);
+            os << "Particles set P" << zpad((int)i + 1, 2) << " : Num = " << pa.num << " \t Tau = " << pa.tau;
+            log.push(os.str());
+        }
+    }
+    if (s.verbose >= 1) log.push("Memory initialization finished \t C-Time = " + duration(ctime()));
+    save_param_file(s, s.workspace + "data/" + s.file_name + "/params.txt");
+    if (s.verbose >= 2) log.push("Starting simulation initialization \t C-Time = " + duration(ctime()));
+
+    double warm[8];
+    int nwarm = 0;
+    r.check(cmm_euler_init(r.e, warm, &nwarm), "cmm_euler_init");
+    // warm-up log in the reference's format, Cuda-CMM/src/simulation/cmm-euler2d.cu:557-596
+    if (s.verbose >= 2 && r.L > 1) {
+        int k = 0;
+        for (int order = 1; order <= r.L; ++order) {
+            int n = order - (order == r.L);
+            if (!s.lagrange_init_higher_order && order != r.L) continue;
+            const char *method = !s.lagrange_init_higher_order ? "EulerExp"
+                                 : order == 1 ? "EulerExp" : order == 2 ? "RK2" : order == 3 ? "RK3" : "RK4";
+            log.push("Init order = " + to_s(order) + "/" + to_s(r.L) + " \t Method = " + method);
+            for (int st = 1; st <= n && k < nwarm; ++st, ++k)
+                log.push("   Init step = " + to_s(st) + "/" + to_s(n) + " \t IncompErr = " + fmt3(warm[k]));
+        }
+    }
+    r.save(s.restart_time, dt, dt);
+
+    std::vector<double> tv(s.lagrange_order), dtv(s.lagrange_order);
+    for (int i = 0; i < s.lagrange_order; ++i) { tv[i] = s.restart_time - (s.lagrange_order - 1 - i) * dt; dtv[i] = dt; }
+    append1(s, "/Monitoring_data/Time_c", ctime());
+    if (s.verbose >= 1) log.push("Simulation initialization finished \t C-Time = " + duration(ctime()));
+    if (s.verbose >= 2) log.push("Starting simulation loop \t C-Time = " + duration(ctime()));
+
+    int loop = 0, old = 0;
+    bool cont = true;
+    const int L = s.lagrange_order;
+    while (s.final_time - tv[loop + L - 1] > dt * 1e-5 && loop < iter_max && cont) {
+        const int lc = loop + L - 1;
+        double dt_now = compute_next_timestep(s, tv[lc], dt);
+        if (tv[lc] + dt_now > s.final_time) dt_now = s.final_time - tv[lc];
+        cmm_step_info info;
+        r.check(cmm_euler_step(r.e, dt_now, &info), "cmm_euler_step");
+        tv.push_back(info.t);
+        dtv.push_back(dt_now);
+        if (info.saturated) {
+            log.push("Stack Saturated : Exitting");
+            cont = false;
+        }
+        if (info.remapped) {
+            if (s.verbose >= 2)
+                log.push("Refining Map : Step = " + to_s(loop) + " \t Maps = " + to_s(info.stack_size) + " \t Gap = " + to_s(loop - old));
+            old = loop;
+        }
+        const double gap = loop - old, counter = info.stack_size;
+        r.save(info.t, dt_now, dt);
+        loop++;
+        append1(s, "/Monitoring_data/Time_s", info.t);
+        append1(s, "/Monitoring_data/Error_incompressibility", info.incomp);
+        append1(s, "/Monitoring_data/Map_gaps", gap);
+        append1(s, "/Monitoring_data/Map_counter", counter);
+        const double c_time = ctime();
+        if (s.verbose >= 2)
+            log.push("Step = " + to_s(loop) + " \t S-Time = " + fmt3(info.t) + "/" + fmt3(s.final_time) + " \t E-inc = " +
+                     fmt3(info.incomp) + " \t C-Time = " + duration(c_time));
+        append1(s, "/Monitoring_data/Time_c", c_time);
+    }
+    if (s.verbose >= 2) log.push("Simulation loop finished \t C-Time = " + duration(ctime()));
+    r.save(kTMax, dt, dt);
+    cmm_euler_destroy(r.e);
+    append1(s, "/Monitoring_data/Time_c", ctime());
+    if (s.verbose >= 1) log.push("Finished simulation \t C-Time = " + duration(ctime()));
+    return 0;
+}
+
+}  // namespace cmmhost

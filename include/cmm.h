@@ -1,0 +1,175 @@
+/*
+ * C-ABI of the MI355X CMM engine (libcmm.so).
+ *
+ * Plain C: raw device pointers (allocated by the caller, e.g. hipMalloc or torch), sizes, and a
+ * hipStream_t passed as void*.  Every call returns 0 on success or a negative CMM_ERR_* code;
+ * cmm_last_error() gives the message of the calling thread's last failure.  Calls that need
+ * scratch memory take (work, work_bytes); with work == NULL the library uses a per-device buffer
+ * it allocates and keeps (calls on one device then share it and must not overlap).
+ *
+ * Each entry point replaces a host operator of the reference (Cuda-CMM/src/...) as cited.
+ * Layouts: include/cmm_types.h.
+ */
+#ifndef CMM_H
+#define CMM_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include "cmm_types.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- library ----------------------------------------------------------------------------- */
+int cmm_version(void);
+const char *cmm_last_error(void);
+/* Writes "name, arch, CUs, LDS per block, HBM bytes" of the current device into buf. */
+int cmm_device_info(char *buf, int len);
+
+/* ---- coefficients (host only) --------------------------------------------------------------
+ * Lagrange-in-time weights and footpoint stencil weights of one advection step; replaces the
+ * host half of advect_using_stream_hermite (Cuda-CMM/src/simulation/cmm-simulation-host.cu:64-90)
+ * and get_L_coefficient (Cuda-CMM/src/numerical/cmm-timestep.cu:474-484).  t and dt are the
+ * time vectors of the driver, loop_ctr the step counter (Cuda-CMM/src/simulation/cmm-euler2d.cu:709-767). */
+double cmm_lagrange_coeff(const double *t, double t_next, int loop_ctr, int i_point, int l_order);
+int cmm_advect_coeffs_fill(const double *t, const double *dt, int loop_ctr, int l_order, int time_integration_num,
+                           int map_update_order_num, double eps, int direction, cmm_advect_coeffs *c);
+
+/* ---- map ----------------------------------------------------------------------------------- */
+/* identity map; replaces k_init_diffeo (Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:27-41) */
+int cmm_map_init_identity(cmm_grid g, double *chix, double *chiy, void *stream);
+
+/* Backward advection of the Hermite map through psi; replaces advect_using_stream_hermite +
+ * k_advect_using_stream_hermite (Cuda-CMM/src/simulation/cmm-simulation-host.cu:64-104,
+ * Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:207-298).  psi_levels is a host array of
+ * c->l_order device pointers (newest first, 4*Npsi doubles each).  chix_new/chiy_new must not
+ * alias chix/chiy. */
+int cmm_map_advect(cmm_grid g_map, cmm_grid g_psi, const double *chix, const double *chiy,
+                   const double *const *psi_levels, const cmm_advect_coeffs *c, double *chix_new,
+                   double *chiy_new, void *stream);
+
+/* max |1 - det grad X| over the cell centres of g_check, returned to the host (synchronises the
+ * stream); replaces incompressibility_check (Cuda-CMM/src/simulation/cmm-simulation-host.cu:44-50). */
+int cmm_incomp_error(cmm_grid g_map, cmm_grid g_check, const double *chix, const double *chiy, double *out,
+                     void *stream);
+
+/* ---- stack composition and initial condition ---------------------------------------------------
+ * The stack is device resident: map m at stack + 8*Nc*m (ChiX planes then ChiY planes), m = 0
+ * oldest.  Positions X_1(...X_M(X_cur(x))) on g_target; replaces apply_map_stack
+ * (Cuda-CMM/src/simulation/cmm-simulation-host.cu:112-152). xy: 2*Nt interleaved. */
+int cmm_compose_positions(cmm_grid g_map, cmm_grid g_target, const double *chix, const double *chiy,
+                          const double *stack, int n_stack, double *xy, void *stream);
+/* Initial condition at the composed positions; k_h_sample_from_init
+ * (Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:342-373).  discrete != 0 samples the Hermite
+ * field disc_herm on g_disc instead of the analytic condition ic. */
+int cmm_sample_init(cmm_grid g_map, cmm_grid g_target, const double *chix, const double *chiy, const double *stack,
+                    int n_stack, int ic, const double *ic_params, int discrete, cmm_grid g_disc,
+                    const double *disc_herm, double *out, void *stream);
+/* New fine vorticity Hermite after a remap: composition + initial condition + R2C + /N +
+ * fourier_hermite in one pass; replaces translate_initial_condition_through_map_stack
+ * (Cuda-CMM/src/simulation/cmm-simulation-host.cu:299-327). */
+size_t cmm_translate_workspace_bytes(cmm_grid g_fine);
+int cmm_translate_ic(cmm_grid g_map, cmm_grid g_fine, const double *chix, const double *chiy, const double *stack,
+                     int n_stack, int ic, const double *ic_params, int discrete, cmm_grid g_disc,
+                     const double *disc_herm, double *w_herm, void *work, size_t work_bytes, void *stream);
+
+/* ---- spectral ------------------------------------------------------------------------------ */
+/* Hermite planes of a real periodic field (R2C, /N, fourier_hermite); replaces the
+ * cufftExecD2Z + k_normalize_h + fourier_hermite sequence (Cuda-CMM/src/numerical/cmm-fft.cu:398-429,
+ * Cuda-CMM/src/simulation/cmm-euler2d.cu:335-339). Powers of two, 8..4096. */
+size_t cmm_hermite_workspace_bytes(cmm_grid g);
+int cmm_hermite_from_real(cmm_grid g, const double *real, double *herm, void *work, size_t work_bytes,
+                          void *stream);
+/* Hermite planes from a half spectrum given column-major (spec[kx*ny + ky], kx <= nx/2, complex
+ * interleaved), no scaling: the C2R half of fourier_hermite. */
+int cmm_hermite_from_spectrum(cmm_grid g, const double *spec_cols, double *herm, void *work, size_t work_bytes,
+                              void *stream);
+/* psi Hermite from the map and the fine vorticity: sample omega(X(x)) on g_vort (mollifier 0/4/8),
+ * R2C, /N, move to the psi spectrum, circular cut-off at freq_cut, -1/|k|^2, fourier_hermite;
+ * replaces evaluate_stream_hermite (Cuda-CMM/src/simulation/cmm-simulation-host.cu:333-375).
+ * Sign convention of the reference: Laplace(psi) = omega, velocity (dpsi/dy, -dpsi/dx). */
+size_t cmm_psi_workspace_bytes(cmm_grid g_vort, cmm_grid g_psi, double freq_cut);
+int cmm_psi_from_vort(cmm_grid g_map, cmm_grid g_vort, cmm_grid g_fine, cmm_grid g_psi, const double *chix,
+                      const double *chiy, const double *w_herm, int molly_stencil, double freq_cut, double *psi_out,
+                      void *work, size_t work_bytes, void *stream);
+/* omega(X(x)) on g_target without transform (coarse vorticity output); k_apply_map_and_sample_from_hermite
+ * with padd_inline_fft = false (Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:402-461). */
+int cmm_sample_vort(cmm_grid g_map, cmm_grid g_target, cmm_grid g_fine, const double *chix, const double *chiy,
+                    const double *w_herm, int molly_stencil, double *out, void *stream);
+
+/* ---- particles ----------------------------------------------------------------------------- */
+/* Fluid particles, one step; replaces particles_advect + Particle_advect
+ * (Cuda-CMM/src/numerical/cmm-particles.cu:63-87, :119-276).  psi_levels: host array of l_order
+ * device pointers, level 0 at t_{n+1}, level 1 at t_n; Lp12: Lagrange weights at t_n + dt/2. */
+int cmm_particles_advect(cmm_grid g_psi, const double *const *psi_levels, int l_order, const double *Lp12, double dt,
+                         int scheme, int n, double *pos, void *stream);
+
+/* ---- engine: the Euler time loop on the device ------------------------------------------------
+ * Mirrors cuda_euler_2d (Cuda-CMM/src/simulation/cmm-euler2d.cu:45-960) for the backward map:
+ * initial sampling, Lagrange warm-up, and per step advection, incompressibility check, remap with
+ * a device-resident stack, psi update and particles. */
+typedef struct cmm_euler cmm_euler;
+
+typedef struct cmm_euler_params {
+    cmm_grid g_coarse, g_fine, g_psi, g_vort;
+    double dt;                      /* nominal step */
+    int time_integration_num, map_update_order_num, l_order, lagrange_init_higher_order;
+    double map_epsilon, incomp_threshold, freq_cut_psi;
+    int molly_stencil, skip_remapping;
+    int ic;                         /* CMM_IC_* */
+    double ic_params[10];
+    int discrete;                   /* initial condition from a discrete field */
+    cmm_grid g_disc;
+    const double *disc_real;        /* host real field on g_disc (NULL with CMM_IC_RANDOM_SPECTRUM) */
+    uint64_t seed;                  /* CMM_IC_RANDOM_SPECTRUM seed */
+    int stack_capacity;             /* maps the device stack holds */
+} cmm_euler_params;
+
+typedef struct cmm_step_info {
+    double t;                       /* time after the step */
+    double incomp;                  /* E_inc after advection (before a remap) */
+    int remapped;                   /* 1 if the map was pushed and reset */
+    int stack_size;                 /* maps on the stack after the step */
+    int saturated;                  /* 1 if a remap was needed but the stack was full */
+} cmm_step_info;
+
+enum {
+    CMM_FIELD_CHIX = 0, CMM_FIELD_CHIY = 1, /* 4*Nc, current map */
+    CMM_FIELD_PSI = 2,                      /* l_order*4*Npsi, newest level first */
+    CMM_FIELD_W_FINE = 3,                   /* 4*Nf fine vorticity Hermite */
+    CMM_FIELD_W_COARSE = 4,                 /* Nc, omega(X(x)) sampled on the coarse grid */
+    CMM_FIELD_DISC = 5                      /* 4*Ndisc discrete initial Hermite */
+};
+
+int cmm_euler_create(const cmm_euler_params *p, cmm_euler **out);
+int cmm_euler_destroy(cmm_euler *e);
+/* Stream the engine runs on (hipStream_t as void*). */
+void *cmm_euler_stream(cmm_euler *e);
+/* Adds a fluid-particle set (positions copied from host); returns its index in *set. */
+int cmm_euler_add_particles(cmm_euler *e, int n, int scheme, const double *pos_host, int *set);
+/* Initial state and Lagrange warm-up; warmup_incomp[8] receives the logged errors. */
+int cmm_euler_init(cmm_euler *e, double *warmup_incomp, int *n_warmup);
+/* One step of length dt_now (the driver clips dt to save targets and the final time). */
+int cmm_euler_step(cmm_euler *e, double dt_now, cmm_step_info *info);
+/* Copies a field to host memory (size per CMM_FIELD_*). */
+int cmm_euler_get_field(cmm_euler *e, int field, double *host_out);
+int cmm_euler_get_particles(cmm_euler *e, int set, double *host_out);
+/* Conservation diagnostics of compute_conservation_targets
+ * (Cuda-CMM/src/simulation/cmm-simulation-host.cu:615-698): out[0] energy 0.5 h^2 sum |grad psi|^2,
+ * out[1] enstrophy and out[2] palinstrophy of the coarse vorticity, out[3] max |omega|. */
+int cmm_euler_diagnostics(cmm_euler *e, double *out);
+/* Steps taken, current time, stack size. */
+int cmm_euler_state(cmm_euler *e, int *steps, double *t, int *stack_size);
+/* Per-kernel timing with events on the engine stream: enable (clears the counters), then read
+ * the accumulated milliseconds and launch counts of CMM_PROF_SLOTS slots:
+ * [map advection, incompressibility check, psi sample+row R2C, psi column stage, psi row C2R,
+ *  particles, remap (translate + push)]. */
+#define CMM_PROF_SLOTS 7
+int cmm_euler_profile(cmm_euler *e, int enable);
+int cmm_euler_profile_read(cmm_euler *e, double *ms_out, int *count_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

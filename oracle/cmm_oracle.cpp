@@ -952,7 +952,7 @@ int cmmo_euler_run(const cmmo_euler_params *p, cmmo_euler_result *r) {
     double dummy_t = 0;
     (void)dummy_t;
     while (p->final_time - tv[loop + L - 1] > p->dt * 1e-5 && loop < iter_max && cont) {
-        if (p->max_steps > 0 && loop >= p->max_steps) break;
+        if (p->max_steps != 0 && loop >= p->max_steps) break;  // max_steps < 0: warm-up only
         int lc = loop + L - 1;
         double dtn = p->dt;
         if (tv[lc] + dtn > p->final_time) dtn = p->final_time - tv[lc];

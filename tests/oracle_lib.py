@@ -65,6 +65,11 @@ def lib():
         L.cmmo_grid_h.restype = C.c_double
         L.cmmo_grid_h.argtypes = [C.c_int, C.c_double, C.c_double]
         L.cmmo_ic_random_spectrum.argtypes = [C.c_int, C.c_uint64, DP]
+        L.cmmo_hermite_eval.argtypes = [Grid, DP, C.c_double, C.c_double]
+        L.cmmo_hermite_grad.argtypes = [Grid, DP, C.c_int, C.c_double, C.c_double, DP]
+        L.cmmo_diffeo_eval.argtypes = [Grid, DP, DP, C.c_double, C.c_double, DP, DP]
+        L.cmmo_diffeo_det.argtypes = [Grid, DP, DP, C.c_double, C.c_double]
+        L.cmmo_init_vorticity.argtypes = [C.c_double, C.c_double, C.c_int, DP]
         _lib = L
     return _lib
 

@@ -1,0 +1,74 @@
+"""The cmm2d driver on the reference's own test cases, compared the way the reference's harness
+compares them (Cuda-CMM/tests/test_functions.py:256-311): run the binary with the case's
+parameter file, require "Finished simulation" on stdout, then compare Monitoring_data.
+
+Compared: Time_s and Map_counter exactly; Error_incompressibility, Mesure energy / enstrophy /
+palinstrophy / max vorticity and the Debug_globals statistics of the map, psi, velocity and coarse
+vorticity to 1e-10 relative (+1e-12 absolute for values that are rounding noise around zero).
+Not compared: particle statistics (the reference seeds them with cuRAND, which this engine
+replaces), inertial particles and save_sample / zoom outputs (not produced by this engine), hashes.
+"""
+import json
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "cmm-turbulence_amd", "build", "cmm2d")
+FIX = json.load(open(os.path.join(ROOT, "tests", "golden", "fixtures.json")))
+
+SKIP_PREFIX = ("Debug_globals/Particles_", "Debug_globals/Scalar_", "Debug_globals/GradX_W_256", "Debug_globals/GradY_W_256",
+               "Debug_globals/Laplacian_W_256", "Debug_globals/Map_ChiX_256", "Debug_globals/Map_ChiY_256",
+               "Debug_globals/Stream_function_Psi_256", "Debug_globals/Velocity_UX_256", "Debug_globals/Velocity_UY_256",
+               "Debug_globals/Vorticity_W_256", "Debug_globals/Distribution_F_256", "Debug_globals/Time_s_256",
+               "Mesure/Energy_256", "Mesure/Enstrophy_256", "Mesure/Palinstrophy_256", "Mesure/Max_vorticity_256",
+               "Mesure/Scalar_integral_256", "Mesure/Time_s_256", "Time_c")
+
+
+def run_case(tmp_path, case, name):
+    params = tmp_path / "params.txt"
+    params.write_text(FIX[case]["params_input"])
+    ws = str(tmp_path) + "/"
+    print("running", case, flush=True)
+    p = subprocess.run([BIN, f"param_file={params}", f"workspace={ws}"], capture_output=True, text=True, timeout=240)
+    print(p.stdout[-3000:])
+    assert p.returncode == 0, p.stderr
+    assert "Finished simulation" in p.stdout
+    return os.path.join(ws, "data", name, "Monitoring_data")
+
+
+def compare(mon, case):
+    fx = FIX[case]
+    checked = 0
+    for key, ref in fx.items():
+        if key in ("params_saved", "params_input", "log_warmup_incomp") or key.startswith(SKIP_PREFIX):
+            continue
+        path = os.path.join(mon, key + ".data")
+        assert os.path.exists(path), f"missing output {key}"
+        got = np.fromfile(path, dtype="<f8")
+        ref = np.array(ref)
+        assert got.shape == ref.shape, f"{key}: {got.shape} vs {ref.shape}"
+        if key in ("Time_s", "Map_counter", "Map_gaps", "Mesure/Time_s", "Debug_globals/Time_s"):
+            assert np.array_equal(got, ref), key
+        else:
+            # E_inc is 1 - det, a cancellation: its rounding level is absolute (~1e-12)
+            tol = 1e-10 * np.abs(ref) + (2e-11 if key == "Error_incompressibility" else 1e-12)
+            bad = np.abs(got - ref) > tol
+            assert not bad.any(), f"{key}: got {got[bad][:3]} expected {ref[bad][:3]}"
+        checked += 1
+    print("compared", checked, "outputs")
+    return checked
+
+
+def test_cli_one_step(dev, tmp_path):
+    mon = run_case(tmp_path, "one_step", "one_step_4_nodes_C256_F512_t1_T1")
+    assert compare(mon, "one_step") >= 40
+
+
+def test_cli_remapping_11b(dev, tmp_path):
+    mon = run_case(tmp_path, "remap_11b", "11b-4nodes-reference-remapping_4_nodes_C256_F512_t32_T2")
+    assert compare(mon, "remap_11b") >= 40

@@ -1,0 +1,172 @@
+"""GPU parity: every HIP kernel of the map-advection path against the CPU oracle on identical
+seeded inputs (C-ABI calls through cmm.py), at the fixture sizes C256 / F512 / psi256.
+
+Tolerance: relative L-infinity (max |gpu - oracle| / max |oracle|) <= 1e-12 for the values of
+omega, psi, the map and positions.  Derivative planes of Hermite fields are held to the same 1e-12
+measured in the derivative's own scale (gpu_util.hermite_errors): max|f| * (pi/h)^m for
+spectral Hermite fields, where FFT rounding in mode k grows by k^m, and max|X| / eps^m for a map
+built by the footpoint stencil, where footpoint rounding grows by 1/eps^m.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+from gpu_util import dev_array, hermite_errors, host, rel_linf
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+def kmax(n):
+    return np.pi / (O.TWO_PI / n)
+
+GC, GF, GP = 256, 512, 256
+
+
+def grids():
+    import cmm
+    return cmm.grid(GC), cmm.grid(GF), cmm.grid(GP)
+
+
+def ogrids():
+    return O.grid(GC), O.grid(GF), O.grid(GP)
+
+
+def random_field(n, seed):
+    rng = np.random.default_rng(seed)
+    k = np.fft.fftfreq(n) * n
+    F = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) * np.exp(-(k[None, :] ** 2 + k[:, None] ** 2) / 200.0)
+    return np.real(np.fft.ifft2(F)) * n
+
+
+@pytest.fixture(scope="module")
+def base():
+    """Oracle state after one strong step from the 4_nodes condition: map, psi history, fine omega."""
+    gc, gf, gp = ogrids()
+    r = O.euler_run(gc, gf, gp, gf, dt=1.0, final_time=1.0, skip_remapping=1,
+                    want=("chix", "chiy", "psi", "w_herm"))
+    return r
+
+
+def test_hermite_from_real(dev):
+    import cmm
+    for n, seed in ((256, 1), (512, 2), (64, 3)):
+        f = random_field(n, seed)
+        ref = O.hermite_from_real(O.grid(n), f.ravel())
+        out = dev_array(np.zeros(4 * n * n))
+        cmm.hermite_from_real(cmm.grid(n), dev_array(f.ravel()), out)
+        errs = hermite_errors(host(out), ref, n * n, kmax(n))
+        print(f"n={n} per-plane errors {errs}")
+        assert max(errs) <= TOL
+
+
+def test_translate_ic_empty_stack(dev):
+    import cmm
+    gc, gf, gp = grids()
+    ogc, ogf, ogp = ogrids()
+    cx, cy = O.identity_map(ogc)
+    xy = O.compose_positions(ogc, ogf, cx, cy, None, None, 0)
+    samp = O.sample_init(ogf, xy, 0)
+    ref = O.hermite_from_real(ogf, samp)
+    W = dev_array(np.zeros(4 * GF * GF))
+    cmm.translate_initial_condition_through_map_stack(gc, gf, dev_array(cx), dev_array(cy), None, 0, 0, W)
+    errs = hermite_errors(host(W), ref, GF * GF, kmax(GF))
+    print("translate_ic per-plane errors", errs)
+    assert max(errs) <= TOL
+
+
+def test_compose_and_sample_with_stack(dev, base):
+    import cmm
+    gc, gf, gp = grids()
+    ogc, ogf, ogp = ogrids()
+    # current map and a stack of two maps, all equal to the strongly deformed map
+    mx, my = base["chix"], base["chiy"]
+    stack = np.concatenate([mx, my, mx, my])
+    # the oracle takes the stack as separate X and Y arrays of maps
+    sx = np.concatenate([mx, mx])
+    sy = np.concatenate([my, my])
+    ref_xy = O.compose_positions(ogc, ogf, mx, my, sx, sy, 2)
+    xy = dev_array(np.zeros(2 * GF * GF))
+    cmm.apply_map_stack(gf, gc, dev_array(mx), dev_array(my), dev_array(stack), 2, xy)
+    e = rel_linf(host(xy), ref_xy)
+    print("compose rel Linf", e)
+    assert e <= TOL
+    ref_s = O.sample_init(ogf, ref_xy, 0)
+    out = dev_array(np.zeros(GF * GF))
+    cmm.sample_init(gc, gf, dev_array(mx), dev_array(my), dev_array(stack), 2, 0, out)
+    e = rel_linf(host(out), ref_s)
+    print("sample_init rel Linf", e)
+    assert e <= TOL
+
+
+def test_psi_from_vort(dev, base):
+    import cmm
+    gc, gf, gp = grids()
+    ogc, ogf, ogp = ogrids()
+    for fcut in (128.0, 40.5, 1000.0):
+        ref = O.psi_from_vort(ogc, ogf, ogf, ogp, base["chix"], base["chiy"], base["w_herm"], 0, fcut)
+        out = dev_array(np.zeros(4 * GP * GP))
+        cmm.evaluate_stream_hermite(gc, gf, gf, gp, dev_array(base["chix"]), dev_array(base["chiy"]),
+                                    dev_array(base["w_herm"]), out, 0, fcut)
+        errs = hermite_errors(host(out), ref, GP * GP, kmax(GP))
+        print(f"psi fcut={fcut} per-plane errors {errs}")
+        assert max(errs) <= TOL
+
+
+def test_map_advect_rk3(dev, base):
+    import cmm
+    gc, gf, gp = grids()
+    ogc, ogf, ogp = ogrids()
+    psi = base["psi"]
+    n4 = 4 * GP * GP
+    for dt, start in ((1.0, "deformed"), (1.0 / 32, "identity")):
+        t = np.array([-2 * dt, -dt, 0.0, dt])
+        dts = np.full(4, dt)
+        oc = O.advect_coeffs(t, dts, 0, 3, 30, 1, 1e-3)
+        c = cmm.advect_coeffs(t, dts, 0, 3, 30, 1, 1e-3)
+        if start == "identity":
+            mx, my = O.identity_map(ogc)
+        else:
+            mx, my = base["chix"], base["chiy"]
+        rx, ry = O.map_advect(ogc, ogp, mx, my, psi, oc)
+        lv = [dev_array(psi[i * n4:(i + 1) * n4]) for i in range(3)]
+        ox = dev_array(np.zeros(4 * GC * GC))
+        oy = dev_array(np.zeros(4 * GC * GC))
+        cmm.advect_using_stream_hermite(gc, gp, dev_array(mx), dev_array(my), lv, c, ox, oy)
+        gx, gy = host(ox), host(oy)
+        n = GC * GC
+        for name, g, r in (("X", gx, rx), ("Y", gy, ry)):
+            errs = hermite_errors(g, r, n, 1.0 / 1e-3)
+            print(f"advect dt={dt} {start} {name} per-plane errors {errs}")
+            assert max(errs) <= TOL
+
+
+def test_incompressibility(dev, base):
+    import cmm
+    gc, gf, gp = grids()
+    ogc, ogf, ogp = ogrids()
+    ref = O.incomp_error(ogc, ogf, base["chix"], base["chiy"])
+    got = cmm.incompressibility_check(gf, gc, dev_array(base["chix"]), dev_array(base["chiy"]))
+    print("incomp", got, ref)
+    assert abs(got - ref) <= 1e-12 * max(1.0, abs(ref))
+    cx, cy = O.identity_map(ogc)
+    # identity map: det = 1 up to the rounding of the basis partition of unity (fused multiply-adds)
+    assert cmm.incompressibility_check(gf, gc, dev_array(cx), dev_array(cy)) <= 1e-12
+
+
+def test_particles_rk3(dev, base):
+    import cmm
+    gc, gf, gp = grids()
+    ogc, ogf, ogp = ogrids()
+    rng = np.random.default_rng(5)
+    pos = rng.uniform(0, O.TWO_PI, 2 * 1000)
+    psi = base["psi"]
+    n4 = 4 * GP * GP
+    Lp = [0.375, 0.75, -0.125]
+    ref = O.particles_advect(ogp, psi, 3, Lp, 0.05, 30, pos)
+    d = dev_array(pos)
+    cmm.particles_advect(gp, [dev_array(psi[i * n4:(i + 1) * n4]) for i in range(3)], 3, Lp, 0.05, 30, d)
+    e = rel_linf(host(d), ref)
+    print("particles rel Linf", e)
+    assert e <= TOL

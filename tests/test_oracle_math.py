@@ -1,0 +1,315 @@
+"""Pins of the CPU oracle by what the mathematics fixes (closed forms, invariants, brute force),
+chosen so that a dropped term, a wrong sign or index, or a transposed operand fails.
+
+Covers: Hermite evaluation / velocity / map evaluation / determinant, Lagrange and stencil
+coefficients, FFT and C2R semantics (numpy.fft), fourier_hermite and the psi solve, map advection
+with zero and linear psi, composition, initial conditions new to this engine, particles.
+"""
+import numpy as np
+import pytest
+
+import oracle_lib as O
+
+TWO_PI = O.TWO_PI
+
+
+def herm_from_poly(n, f, fx, fy, fxy, L=TWO_PI):
+    """Hermite planes of a smooth function sampled on the grid (exact derivative planes)."""
+    h = L / float(np.float32(n))
+    x = np.arange(n) * h
+    X, Y = np.meshgrid(x, x)
+    return np.concatenate([g(X, Y).ravel() for g in (f, fx, fy, fxy)]), h
+
+
+def test_hermite_reproduces_bicubic_locally():
+    # a bicubic polynomial in the cell-local coordinates is reproduced exactly inside one cell
+    n = 16
+    rng = np.random.default_rng(0)
+    c = rng.standard_normal((4, 4))
+    g = O.grid(n, b=(0.0, 1.0, 0.0, 1.0))
+    h = 1.0 / float(np.float32(n))
+
+    def p(x, y, dx=0, dy=0):
+        s = 0.0
+        for i in range(4):
+            for j in range(4):
+                if i < dx or j < dy:
+                    continue
+                fi = np.prod(range(i - dx + 1, i + 1)) if dx else 1
+                fj = np.prod(range(j - dy + 1, j + 1)) if dy else 1
+                s = s + c[i, j] * fi * fj * x ** (i - dx) * y ** (j - dy)
+        return s
+
+    H, _ = herm_from_poly(n, lambda X, Y: p(X, Y), lambda X, Y: p(X, Y, 1, 0), lambda X, Y: p(X, Y, 0, 1),
+                          lambda X, Y: p(X, Y, 1, 1), L=1.0)
+    for (x, y) in ((0.3 * h + 5 * h, 0.71 * h + 7 * h), (2 * h + 0.5 * h, 3 * h + 0.99 * h)):
+        v = O.lib().cmmo_hermite_eval(g, O.ptr(H), x, y)
+        assert abs(v - p(x, y)) < 1e-12 * max(1, abs(p(x, y)))
+        u = np.zeros(2)
+        O.lib().cmmo_hermite_grad(g, O.ptr(H), 1, O.C.c_double(x), O.C.c_double(y), O.ptr(u))
+        assert abs(u[0] - p(x, y, 0, 1)) < 1e-10          # u = (d/dy, -d/dx)
+        assert abs(u[1] + p(x, y, 1, 0)) < 1e-10
+
+
+def test_hermite_tensor_formula_brute_force():
+    # direct tensor-product Hermite formula on a 4x4 periodic grid with random data
+    n = 4
+    rng = np.random.default_rng(1)
+    H = rng.standard_normal(4 * n * n)
+    g = O.grid(n, b=(0.0, 1.0, 0.0, 1.0))
+    h = 1.0 / float(np.float32(n))
+    x, y = 0.37, 0.81
+    ix, iy = int(np.floor(x / h)), int(np.floor(y / h))
+    dx, dy = x / h - ix, y / h - iy
+    h00 = lambda t: 2 * t ** 3 - 3 * t ** 2 + 1
+    h01 = lambda t: -2 * t ** 3 + 3 * t ** 2
+    h10 = lambda t: t ** 3 - 2 * t ** 2 + t
+    h11 = lambda t: t ** 3 - t ** 2
+    v = 0.0
+    for cx, wx, dwx in ((ix, h00(dx), h10(dx)), ((ix + 1) % n, h01(dx), h11(dx))):
+        for cy, wy, dwy in ((iy, h00(dy), h10(dy)), ((iy + 1) % n, h01(dy), h11(dy))):
+            k = cy * n + cx
+            v += wx * wy * H[k] + dwx * h * wy * H[n * n + k] + wx * dwy * h * H[2 * n * n + k] \
+                + dwx * dwy * h * h * H[3 * n * n + k]
+    got = O.lib().cmmo_hermite_eval(g, O.ptr(H), O.C.c_double(x), O.C.c_double(y))
+    assert abs(got - v) < 1e-14
+
+
+def test_identity_map_and_determinant():
+    g = O.grid(64)
+    cx, cy = O.identity_map(g)
+    xo, yo = O.C.c_double(), O.C.c_double()
+    for (x, y) in ((0.1, 0.2), (-3.3, 7.9), (TWO_PI * 2.5, -0.01)):
+        O.lib().cmmo_diffeo_eval(g, O.ptr(cx), O.ptr(cy), O.C.c_double(x), O.C.c_double(y), O.C.byref(xo), O.C.byref(yo))
+        assert abs(xo.value - x) < 1e-12 and abs(yo.value - y) < 1e-12   # wrap jumps restore x
+    assert O.incomp_error(g, O.grid(128), cx, cy) < 1e-13
+
+
+def test_area_preserving_shear_det():
+    # X = x + a sin(y), Y = y has det 1; Hermite interpolation keeps it to interpolation accuracy
+    n, a = 128, 0.3
+    g = O.grid(n)
+    Hx, _ = herm_from_poly(n, lambda X, Y: X + a * np.sin(Y), lambda X, Y: 1 + 0 * X, lambda X, Y: a * np.cos(Y),
+                           lambda X, Y: 0 * X)
+    Hy, _ = herm_from_poly(n, lambda X, Y: Y, lambda X, Y: 0 * X, lambda X, Y: 1 + 0 * X, lambda X, Y: 0 * X)
+    e = O.incomp_error(g, O.grid(256), Hx, Hy)
+    assert e < 1e-6
+    # a wrong sign in the cross term would break it: X = x + a sin(y) + a sin(x) is not area preserving
+    Hx2, _ = herm_from_poly(n, lambda X, Y: X + a * np.sin(Y) + a * np.sin(X), lambda X, Y: 1 + a * np.cos(X),
+                            lambda X, Y: a * np.cos(Y), lambda X, Y: 0 * X)
+    assert O.incomp_error(g, O.grid(256), Hx2, Hy) > 0.1
+
+
+def test_lagrange_and_stencil_coefficients():
+    dt = 0.1
+    t = np.array([-2 * dt, -dt, 0.0, dt])
+    c = O.advect_coeffs(t, np.full(4, dt), 0, 3, 30, 1, 1e-3)
+    assert np.allclose(list(c.L1)[:3], [3, -3, 1], atol=1e-13)
+    assert np.allclose(list(c.L12)[:3], [1.875, -1.25, 0.375], atol=1e-13)
+    # 4th-order stencil: weights of the value sum to 1; derivative weights vanish on constants
+    assert abs(sum(list(c.c1)[:8]) - 1) < 1e-15
+    assert abs(sum(list(c.cx)[:8])) < 1e-9 and abs(sum(list(c.cy)[:8])) < 1e-9 and abs(sum(list(c.cxy)[:8])) < 1e-3
+    # and reproduce d/dx of a linear function at the footpoint offsets (x = +-eps, +-2 eps)
+    eps = 1e-3
+    offs_x = [((k >> 2) + 1) * eps * (1 - 2 * ((k >> 1) & 1)) for k in range(8)]
+    offs_y = [((k >> 2) + 1) * eps * (1 - 2 * (((k + 1) >> 1) & 1)) for k in range(8)]
+    assert abs(sum(w * o for w, o in zip(list(c.cx)[:8], offs_x)) - 1) < 1e-12
+    assert abs(sum(w * o for w, o in zip(list(c.cy)[:8], offs_y)) - 1) < 1e-12
+    assert abs(sum(w * ox * oy for w, ox, oy in zip(list(c.cxy)[:8], offs_x, offs_y)) - 1) < 1e-9
+
+
+@pytest.mark.parametrize("n", [8, 64, 256])
+def test_fft_against_numpy(n):
+    rng = np.random.default_rng(n)
+    a = rng.standard_normal((n, n))
+    F = O.rfft2(a)
+    assert np.max(np.abs(F - np.fft.rfft2(a))) < 1e-12 * n * n
+    # C2R with arbitrary (non-Hermitian) input incl. imaginary DC / Nyquist bins: numpy's semantics
+    G = rng.standard_normal((n, n // 2 + 1)) + 1j * rng.standard_normal((n, n // 2 + 1))
+    assert np.max(np.abs(O.irfft2(G, n) - np.fft.irfft2(G, s=(n, n)) * n * n)) < 1e-11 * n * n
+
+
+def test_fourier_hermite_of_trig_field():
+    # f = sin(2x) cos(3y) + cos(x): derivative planes exact
+    n = 64
+    H_exact, h = herm_from_poly(n, lambda X, Y: np.sin(2 * X) * np.cos(3 * Y) + np.cos(X),
+                                lambda X, Y: 2 * np.cos(2 * X) * np.cos(3 * Y) - np.sin(X),
+                                lambda X, Y: -3 * np.sin(2 * X) * np.sin(3 * Y),
+                                lambda X, Y: -6 * np.cos(2 * X) * np.sin(3 * Y))
+    H = O.hermite_from_real(O.grid(n), H_exact[:n * n])
+    assert np.max(np.abs(H - H_exact)) < 1e-12
+
+
+def test_psi_solve_sign_and_cutoff():
+    # omega = cos(2x) + cos(y) sampled through the identity map: psi = -(cos(2x)/4 + cos(y));
+    # with freq_cut 1.5 the k=2 mode is removed
+    n = 64
+    g = O.grid(n)
+    cx, cy = O.identity_map(g)
+    W, h = herm_from_poly(n, lambda X, Y: np.cos(2 * X) + np.cos(Y), lambda X, Y: -2 * np.sin(2 * X),
+                          lambda X, Y: -np.sin(Y), lambda X, Y: 0 * X)
+    x = np.arange(n) * h
+    X, Y = np.meshgrid(x, x)
+    psi = O.psi_from_vort(g, g, g, g, cx, cy, W, 0, 32.0)
+    assert np.max(np.abs(psi[:n * n] - (-(np.cos(2 * X) / 4 + np.cos(Y))).ravel())) < 1e-12
+    assert np.max(np.abs(psi[n * n:2 * n * n] - (np.sin(2 * X) / 2).ravel())) < 1e-12
+    psi = O.psi_from_vort(g, g, g, g, cx, cy, W, 0, 1.5)
+    assert np.max(np.abs(psi[:n * n] - (-np.cos(Y)).ravel())) < 1e-12
+    # psi grid smaller than the vorticity grid (spectrum move)
+    gp = O.grid(32)
+    psi = O.psi_from_vort(g, g, g, gp, cx, cy, W, 0, 32.0)
+    xp = np.arange(32) * (TWO_PI / 32.0)
+    Xp, Yp = np.meshgrid(xp, xp)
+    assert np.max(np.abs(psi[:32 * 32] - (-(np.cos(2 * Xp) / 4 + np.cos(Yp))).ravel())) < 1e-12
+
+
+def psi_linear(n, a, b):
+    """Hermite psi = a*y - b*x is not periodic; use psi = a sin(y) - b sin(x) near the origin
+    region instead: here we build u = (psi_y, -psi_x) = (a, b) constant via psi = a*y - b*x with the
+    Hermite planes set directly (values are never wrapped in the advection)."""
+    H, h = herm_from_poly(n, lambda X, Y: a * Y - b * X, lambda X, Y: -b + 0 * X, lambda X, Y: a + 0 * X,
+                          lambda X, Y: 0 * X)
+    return H
+
+
+def test_advection_zero_psi_keeps_identity():
+    n = 32
+    g = O.grid(n)
+    cx, cy = O.identity_map(g)
+    psi = np.zeros(3 * 4 * n * n)
+    dt = 0.1
+    c = O.advect_coeffs(np.array([-2 * dt, -dt, 0, dt]), np.full(4, dt), 0, 3, 30, 1, 1e-3)
+    nx, ny = O.map_advect(g, g, cx, cy, psi, c)
+    # derivative planes are footpoint differences over eps (eps^2 for the cross plane), so rounding
+    # of the footpoint positions (~1e-15) grows by 1/eps per derivative
+    nn = n * n
+    for new, old in ((nx, cx), (ny, cy)):
+        for p, tol in enumerate((1e-13, 1e-10, 1e-10, 1e-8)):
+            assert np.max(np.abs(new[p * nn:(p + 1) * nn] - old[p * nn:(p + 1) * nn])) < tol * TWO_PI
+
+
+def test_advection_uniform_velocity_is_translation():
+    # psi planes encode the constant velocity u = (a, b) inside the domain interior: the footpoint
+    # moves back by dt*(a, b) and the new map is the translated identity in the interior nodes
+    n = 32
+    g = O.grid(n)
+    a, b = 0.3, -0.2
+    P = psi_linear(n, a, b)
+    psi = np.concatenate([P, P, P])
+    cx, cy = O.identity_map(g)
+    dt = 0.05
+    for tin in (10, 21, 30, 40):
+        L = {10: 1, 21: 2, 30: 3, 40: 4}[tin]
+        t = np.array([-(L - 1 - i) * dt for i in range(L)] + [dt])
+        c = O.advect_coeffs(t, np.full(L + 1, dt), 0, L, tin, 1, 1e-3)
+        psiL = np.concatenate([P] * L)
+        nx, ny = O.map_advect(g, g, cx, cy, psiL, c)
+        nn = n * n
+        inner = np.zeros((n, n), bool)
+        inner[4:-4, 4:-4] = True
+        inner = inner.ravel()
+        assert np.max(np.abs(nx[:nn][inner] - (cx[:nn][inner] - dt * a))) < 1e-12
+        assert np.max(np.abs(ny[:nn][inner] - (cy[:nn][inner] - dt * b))) < 1e-12
+        assert np.max(np.abs(nx[nn:2 * nn][inner] - 1)) < 1e-9
+        assert np.max(np.abs(nx[2 * nn:3 * nn][inner])) < 1e-9 and np.max(np.abs(nx[3 * nn:][inner])) < 1e-6
+
+
+def test_composition_of_translations():
+    n = 32
+    gm = O.grid(n)
+    gt = O.grid(64)
+    cx, cy = O.identity_map(gm)
+    shifts = [(0.1, -0.2), (0.05, 0.3), (-0.4, 0.01)]
+    sx, sy = [], []
+    for dx, dy in shifts:
+        x, y = O.identity_map(gm)
+        x[:n * n] += dx
+        y[:n * n] += dy
+        sx.append(x)
+        sy.append(y)
+    xy = O.compose_positions(gm, gt, cx, cy, np.concatenate(sx), np.concatenate(sy), 3)
+    h = TWO_PI / 64.0
+    g1 = np.arange(64) * h
+    X, Y = np.meshgrid(g1, g1)
+    ex = np.mod(X + sum(s[0] for s in shifts), TWO_PI).ravel()
+    ey = np.mod(Y + sum(s[1] for s in shifts), TWO_PI).ravel()
+    dx = np.abs(xy[0::2] - ex)
+    dx = np.minimum(dx, TWO_PI - dx)
+    assert dx.max() < 1e-12
+    dy = np.abs(xy[1::2] - ey)
+    dy = np.minimum(dy, TWO_PI - dy)
+    assert dy.max() < 1e-12
+    # empty stack with the identity map reproduces the initial condition on the grid
+    xy0 = O.compose_positions(gm, gt, cx, cy, None, None, 0)
+    w = O.sample_init(gt, xy0, 0)
+    assert np.max(np.abs(w - (np.cos(X) + np.cos(Y) + 0.6 * np.cos(2 * X) + 0.2 * np.cos(3 * X)).ravel())) < 1e-13
+
+
+def test_random_spectrum_initial_condition():
+    n = 256
+    w = O.ic_random_spectrum(n, 0).reshape(n, n)
+    h = TWO_PI / float(np.float32(n))
+    assert abs(0.5 * h * h * np.sum(w ** 2) - 1.0) < 1e-12     # enstrophy 1
+    assert abs(w.mean()) < 1e-14                                 # zero mean
+    F = np.fft.rfft2(w)
+    ky = np.abs(np.fft.fftfreq(n, 1 / n))[:, None] + 0 * np.arange(n // 2 + 1)[None, :]
+    kx = np.arange(n // 2 + 1)[None, :] + 0 * ky
+    assert np.abs(F[(kx > 64) | (ky > 64)]).max() < 1e-12 * np.abs(F).max()   # support |kx|,|ky| <= 64
+    # amplitude law |omega_hat|^2 = A^2 k E(k)/pi: ratio of two modes is fixed by E(k)
+    E = lambda k: k ** 4 * np.exp(-2 * (k / 6) ** 2)
+    a1, a2 = np.abs(F[0, 3]), np.abs(F[4, 0])        # k = 3 and k = 4
+    assert abs((a1 / a2) ** 2 - (3 * E(3)) / (4 * E(4))) < 1e-10
+    assert not np.allclose(O.ic_random_spectrum(n, 1), w.ravel())   # the seed matters
+
+
+def test_double_shear_layer_is_periodic_and_odd():
+    n = 256
+    f = lambda x, y: O.lib().cmmo_init_vorticity(O.C.c_double(x), O.C.c_double(y), 100, None)
+    assert abs(f(0.3, 0.0) - f(0.3, TWO_PI)) < 1e-12
+    assert abs(f(0.0, 1.1) - f(TWO_PI, 1.1)) < 1e-12
+    delta = np.pi / 15
+    # at y = pi/2 the upper layer peaks: 1/delta + tail of the lower one + 0.05 cos x
+    # at x = pi/2 (cos x = 0) and y = pi/2 the upper layer peaks at 1/delta; the lower layer
+    # contributes -sech^2(15)/delta ~ -2e-12
+    assert abs(f(np.pi / 2, np.pi / 2) - 1 / delta) < 1e-10
+    # odd under y -> 2 pi - y about the layers' midpoint for the shear part: omega(x, pi - y) = -omega
+    assert abs(f(0.7, np.pi / 2 + 0.3) + f(0.7 + np.pi, 3 * np.pi / 2 + 0.3)) < 1e-12
+
+
+def test_particles_uniform_velocity_straight_lines():
+    n = 32
+    g = O.grid(n)
+    a, b = 0.3, -0.2
+    P = psi_linear(n, a, b)
+    psi = np.concatenate([P, P, P])
+    pos = np.array([1.0, 2.0, 3.0, 1.5, 2.5, 2.5])
+    for scheme in (10, 20, 30, 40, 31, 41):
+        out = O.particles_advect(g, psi, 3, [0.375, 0.75, -0.125], 0.1, scheme, pos)
+        assert np.max(np.abs(out[0::2] - (pos[0::2] + 0.1 * a))) < 1e-12
+        assert np.max(np.abs(out[1::2] - (pos[1::2] + 0.1 * b))) < 1e-12
+
+
+def test_particles_rk3_third_order():
+    # rotation psi = -(x^2+y^2)/2 around the domain centre is not periodic; use the steady cellular
+    # flow psi = sin x sin y and compare one step against a fine RK4 reference: error ~ dt^4
+    n = 256
+    g = O.grid(n)
+    P, h = herm_from_poly(n, lambda X, Y: np.sin(X) * np.sin(Y), lambda X, Y: np.cos(X) * np.sin(Y),
+                          lambda X, Y: np.sin(X) * np.cos(Y), lambda X, Y: np.cos(X) * np.cos(Y))
+    psi = np.concatenate([P, P, P])
+    p0 = np.array([1.0, 0.5])
+
+    def ref(p, T, m=2000):
+        p = p.copy()
+        dt = T / m
+        f = lambda q: np.array([np.sin(q[0]) * np.cos(q[1]), -np.cos(q[0]) * np.sin(q[1])])
+        for _ in range(m):
+            k1 = f(p); k2 = f(p + dt / 2 * k1); k3 = f(p + dt / 2 * k2); k4 = f(p + dt * k3)
+            p = p + dt / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+        return p
+
+    errs = []
+    for dt in (0.2, 0.1):
+        out = O.particles_advect(g, psi, 3, [0.375, 0.75, -0.125], dt, 30, p0)
+        errs.append(np.max(np.abs(out - ref(p0, dt))))
+    assert errs[1] < errs[0] / 10        # local error of a third-order step scales like dt^4

@@ -1,0 +1,58 @@
+"""The parameter-file boundary: the reference's own saved parameter files load and re-save byte
+for byte (Cuda-CMM/src/ui/cmm-param.cu:39-229), and command-line values override the file
+(Cuda-CMM/src/main2d.cpp:72-89).  Runs cmm2d in its save_params_only mode, which exits before any
+GPU work."""
+import json
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "cmm-turbulence_amd", "build", "cmm2d")
+FIX = json.load(open(os.path.join(ROOT, "tests", "golden", "fixtures.json")))
+
+
+def roundtrip(tmp_path, text, *args):
+    src = tmp_path / "in.txt"
+    out = tmp_path / "out.txt"
+    src.write_text(text)
+    p = subprocess.run([BIN, f"param_file={src}", *args, f"save_params_only={out}"], capture_output=True,
+                       text=True, timeout=60)
+    assert p.returncode == 0, p.stdout + p.stderr
+    return out.read_text()
+
+
+@pytest.mark.parametrize("case", ["one_step", "remap_11b", "vlasov_15"])
+def test_saved_param_files_roundtrip(tmp_path, case):
+    text = FIX[case]["params_saved"]
+    if case == "vlasov_15":
+        # the Vlasov file has no particle block; its initial condition names are Vlasov names
+        pass
+    assert roundtrip(tmp_path, text) == text
+
+
+def test_command_line_overrides_file(tmp_path):
+    text = FIX["one_step"]["params_saved"]
+    out = roundtrip(tmp_path, text, "grid_coarse=128", "final_time=2.5", "time_integration=RK4")
+    assert "grid_coarse\t=\t128\n" in out
+    assert "final_time\t=\t2.5\n" in out
+    assert "time_integration\t=\tRK4\n" in out
+
+
+def test_presets_without_file(tmp_path):
+    out = tmp_path / "p.txt"
+    p = subprocess.run([BIN, f"save_params_only={out}"], capture_output=True, text=True, timeout=60)
+    assert p.returncode == 0
+    t = out.read_text()
+    # presets of Cuda-CMM/src/ui/settings.cu:18-327, including freq_cut_psi = 512/4
+    for line in ("grid_coarse\t=\t512", "grid_fine\t=\t2048", "freq_cut_psi\t=\t128", "incomp_threshold\t=\t1e-05",
+                 "save_computational_num\t=\t4", "particles_advected_num\t=\t2"):
+        assert line + "\n" in t
+
+
+def test_bad_initial_condition_exits_with_reference_code(tmp_path):
+    p = subprocess.run([BIN, "initial_condition=nonsense", f"save_params_only={tmp_path / 'x'}"], capture_output=True,
+                       text=True, timeout=60)
+    assert p.returncode == 230305 % 256
+    assert "existing initial condition" in p.stdout
