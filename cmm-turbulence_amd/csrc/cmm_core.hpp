@@ -22,17 +22,27 @@ constexpr double kTwoPi = 6.2831853071795864769252867665590057683943387987502116
 // Grid as the kernels see it.  h = (x1-x0)/(float)nx as in Cuda-CMM/src/grid/cmm-grid2d.cu:27-28;
 // jx/jy are the jumps nx*h added to a map's value plane on a periodic wrap
 // (Cuda-CMM/src/numerical/cmm-hermite.cu:450), lx/ly the domain lengths used to wrap positions
-// (Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:316-320).
+// (Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:316-320).  ihx/ihy = 1/h; lgx/lgy = log2 of
+// the point counts when they are powers of two (else -1).
 struct DGrid {
-    int nx, ny, n;
-    double hx, hy, x0, y0, jx, jy, lx, ly;
+    int nx, ny, n, lgx, lgy;
+    double hx, hy, ihx, ihy, x0, y0, jx, jy, lx, ly;
 };
+
+inline int log2_or_neg(int n) {
+    if (n <= 0 || (n & (n - 1))) return -1;
+    int l = 0;
+    while ((1 << l) < n) ++l;
+    return l;
+}
 
 inline DGrid make_dgrid(const cmm_grid &g) {
     DGrid d;
     d.nx = g.nx; d.ny = g.ny; d.n = g.nx * g.ny;
+    d.lgx = log2_or_neg(g.nx); d.lgy = log2_or_neg(g.ny);
     d.hx = (g.x1 - g.x0) / (float)g.nx;
     d.hy = (g.y1 - g.y0) / (float)g.ny;
+    d.ihx = 1.0 / d.hx; d.ihy = 1.0 / d.hy;
     d.x0 = g.x0; d.y0 = g.y0;
     d.jx = g.nx * d.hx; d.jy = g.ny * d.hy;
     d.lx = g.x1 - g.x0; d.ly = g.y1 - g.y0;
@@ -43,7 +53,10 @@ inline DGrid make_dgrid(const cmm_grid &g) {
 // of the corners.  Cuda-CMM/src/numerical/cmm-hermite.cu:152-164 / :191-208.  The wrap is a true
 // floor division (identical to the reference except at exact negative multiples of the grid
 // size, where the reference indexes outside the array); the floor is clamped to +-1e9 cells so
-// that non-finite positions still address memory inside the grid.
+// that non-finite positions still address memory inside the grid.  The cell coordinate is
+// (x - x0) * (1/h): it can differ from the reference's (x - x0) / h in the last bit, which moves a
+// point lying on a cell edge to the neighbouring cell, where the C1 Hermite interpolant takes the
+// same value.
 struct Cell {
     int i00, i10, i01, i11;
     int wx0, wx1, wy0, wy1;
@@ -51,17 +64,18 @@ struct Cell {
 };
 
 CMM_HD int floordiv(int a, int n) { return a >= 0 ? a / n : -((-a - 1) / n) - 1; }
+CMM_HD int floordiv_g(int a, int n, int lg) { return lg >= 0 ? (a >> lg) : floordiv(a, n); }
 
 CMM_HD void find_cell(const DGrid &g, double x, double y, Cell &c) {
-    double fx = (x - g.x0) / g.hx, fy = (y - g.y0) / g.hy;
+    double fx = (x - g.x0) * g.ihx, fy = (y - g.y0) * g.ihy;
     double flx = floor(fx), fly = floor(fy);
     c.dx = fx - flx;
     c.dy = fy - fly;
     flx = fmin(fmax(flx, -1.0e9), 1.0e9);
     fly = fmin(fmax(fly, -1.0e9), 1.0e9);
     int ix = (int)flx, iy = (int)fly;
-    c.wx0 = floordiv(ix, g.nx);
-    c.wy0 = floordiv(iy, g.ny);
+    c.wx0 = floordiv_g(ix, g.nx, g.lgx);
+    c.wy0 = floordiv_g(iy, g.ny, g.lgy);
     int rx0 = ix - c.wx0 * g.nx, ry0 = iy - c.wy0 * g.ny;
     int rx1 = rx0 + 1, ry1 = ry0 + 1;
     c.wx1 = c.wx0; c.wy1 = c.wy0;
@@ -90,21 +104,39 @@ CMM_HD void basis_d(double d, double b[4]) {
     b[3] = (3 * d - 2) * d;
 }
 
-// Tensor-product sum over the planes [f, fx, fy, fxy] of one Hermite field
-// (Cuda-CMM/src/numerical/cmm-hermite.cu:31-45); derivative planes are physical, so they are
-// scaled by h.
-CMM_HD double hsum(const double *__restrict__ H, int N, const double bx[4], const double by[4], const Cell &c,
-                   double hx, double hy) {
+// The 16 values a cell evaluation needs: planes [f, fx, fy, fxy] at corners [00, 10, 01, 11].
+CMM_HD void load16(const double *__restrict__ H, int N, const Cell &c, double v[16]) {
+    const long n = N;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        const double *Hp = H + p * n;
+        v[4 * p + 0] = Hp[c.i00];
+        v[4 * p + 1] = Hp[c.i10];
+        v[4 * p + 2] = Hp[c.i01];
+        v[4 * p + 3] = Hp[c.i11];
+    }
+}
+
+// Tensor-product sum over the planes of one Hermite field (Cuda-CMM/src/numerical/cmm-hermite.cu:31-45);
+// derivative planes are physical, so they are scaled by h.  wjump adds the wrap jump of a map's
+// value plane per corner (Cuda-CMM/src/numerical/cmm-hermite.cu:67-90).
+template <bool WARP>
+CMM_HD double hsum_t(const double v[16], const double bx[4], const double by[4], double hx, double hy,
+                     const double *wj) {
     double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
-    t0 += by[0] * bx[0] * H[c.i00]; t0 += by[0] * bx[1] * H[c.i10];
-    t0 += by[1] * bx[0] * H[c.i01]; t0 += by[1] * bx[1] * H[c.i11];
-    const double *H1 = H + N, *H2 = H + 2 * (long)N, *H3 = H + 3 * (long)N;
-    t1 += by[0] * bx[2] * H1[c.i00]; t1 += by[0] * bx[3] * H1[c.i10];
-    t1 += by[1] * bx[2] * H1[c.i01]; t1 += by[1] * bx[3] * H1[c.i11];
-    t2 += by[2] * bx[0] * H2[c.i00]; t2 += by[2] * bx[1] * H2[c.i10];
-    t2 += by[3] * bx[0] * H2[c.i01]; t2 += by[3] * bx[1] * H2[c.i11];
-    t3 += by[2] * bx[2] * H3[c.i00]; t3 += by[2] * bx[3] * H3[c.i10];
-    t3 += by[3] * bx[2] * H3[c.i01]; t3 += by[3] * bx[3] * H3[c.i11];
+    if constexpr (WARP) {
+        t0 += by[0] * bx[0] * (v[0] + wj[0]); t0 += by[0] * bx[1] * (v[1] + wj[1]);
+        t0 += by[1] * bx[0] * (v[2] + wj[2]); t0 += by[1] * bx[1] * (v[3] + wj[3]);
+    } else {
+        t0 += by[0] * bx[0] * v[0]; t0 += by[0] * bx[1] * v[1];
+        t0 += by[1] * bx[0] * v[2]; t0 += by[1] * bx[1] * v[3];
+    }
+    t1 += by[0] * bx[2] * v[4]; t1 += by[0] * bx[3] * v[5];
+    t1 += by[1] * bx[2] * v[6]; t1 += by[1] * bx[3] * v[7];
+    t2 += by[2] * bx[0] * v[8]; t2 += by[2] * bx[1] * v[9];
+    t2 += by[3] * bx[0] * v[10]; t2 += by[3] * bx[1] * v[11];
+    t3 += by[2] * bx[2] * v[12]; t3 += by[2] * bx[3] * v[13];
+    t3 += by[3] * bx[2] * v[14]; t3 += by[3] * bx[3] * v[15];
     double out = t0;
     out += t1 * hx;
     out += t2 * hy;
@@ -114,37 +146,23 @@ CMM_HD double hsum(const double *__restrict__ H, int N, const double bx[4], cons
     return out;
 }
 
-// Same with the wrap jump j*w added to the value plane at each corner
-// (Cuda-CMM/src/numerical/cmm-hermite.cu:67-90).
-CMM_HD double hsum_warp(const double *__restrict__ H, int N, const double bx[4], const double by[4], const Cell &c,
-                        int w00, int w10, int w01, int w11, double jump, double hx, double hy) {
-    double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
-    t0 += by[0] * bx[0] * (H[c.i00] + w00 * jump); t0 += by[0] * bx[1] * (H[c.i10] + w10 * jump);
-    t0 += by[1] * bx[0] * (H[c.i01] + w01 * jump); t0 += by[1] * bx[1] * (H[c.i11] + w11 * jump);
-    const double *H1 = H + N, *H2 = H + 2 * (long)N, *H3 = H + 3 * (long)N;
-    t1 += by[0] * bx[2] * H1[c.i00]; t1 += by[0] * bx[3] * H1[c.i10];
-    t1 += by[1] * bx[2] * H1[c.i01]; t1 += by[1] * bx[3] * H1[c.i11];
-    t2 += by[2] * bx[0] * H2[c.i00]; t2 += by[2] * bx[1] * H2[c.i10];
-    t2 += by[3] * bx[0] * H2[c.i01]; t2 += by[3] * bx[1] * H2[c.i11];
-    t3 += by[2] * bx[2] * H3[c.i00]; t3 += by[2] * bx[3] * H3[c.i10];
-    t3 += by[3] * bx[2] * H3[c.i01]; t3 += by[3] * bx[3] * H3[c.i11];
-    double out = t0;
-    out += t1 * hx;
-    out += t2 * hy;
-    t3 *= hx;
-    t3 *= hy;
-    out += t3;
-    return out;
+CMM_HD double hsum_v(const double v[16], const double bx[4], const double by[4], double hx, double hy) {
+    return hsum_t<false>(v, bx, by, hx, hy, nullptr);
+}
+CMM_HD double hsum_v(const double v[16], const double bx[4], const double by[4], double hx, double hy,
+                     const double wj[4]) {
+    return hsum_t<true>(v, bx, by, hx, hy, wj);
 }
 
 // Value of a Hermite field, Cuda-CMM/src/numerical/cmm-hermite.cu:323-331
 CMM_HD double herm_eval(const DGrid &g, const double *__restrict__ H, double x, double y) {
     Cell c;
     find_cell(g, x, y, c);
-    double bx[4], by[4];
+    double bx[4], by[4], v[16];
     basis(c.dx, bx);
     basis(c.dy, by);
-    return hsum(H, g.n, bx, by, c, g.hx, g.hy);
+    load16(H, g.n, c, v);
+    return hsum_v(v, bx, by, g.hx, g.hy);
 }
 
 // Velocity (d psi/dy, -d psi/dx) of up to four psi levels at one point,
@@ -156,13 +174,17 @@ struct PsiLevels {
 CMM_HD void herm_vel(const DGrid &g, const PsiLevels &P, int nl, double x, double y, double *u) {
     Cell c;
     find_cell(g, x, y, c);
-    double bx[4], by[4];
-    basis_d(c.dx, bx);
-    basis(c.dy, by);
-    for (int l = 0; l < nl; ++l) u[2 * l + 1] = -hsum(P.p[l], g.n, bx, by, c, g.hx, g.hy) / g.hx;
+    double bx[4], by[4], dbx[4], dby[4];
     basis(c.dx, bx);
-    basis_d(c.dy, by);
-    for (int l = 0; l < nl; ++l) u[2 * l] = hsum(P.p[l], g.n, bx, by, c, g.hx, g.hy) / g.hy;
+    basis(c.dy, by);
+    basis_d(c.dx, dbx);
+    basis_d(c.dy, dby);
+    for (int l = 0; l < nl; ++l) {
+        double v[16];
+        load16(P.p[l], g.n, c, v);
+        u[2 * l + 1] = -hsum_v(v, dbx, by, g.hx, g.hy) * g.ihx;
+        u[2 * l] = hsum_v(v, bx, dby, g.hx, g.hy) * g.ihy;
+    }
 }
 
 // Map evaluation with periodic jumps, Cuda-CMM/src/numerical/cmm-hermite.cu:446-460
@@ -170,11 +192,15 @@ CMM_HD void diffeo(const DGrid &g, const double *__restrict__ Hx, const double *
                    double &xo, double &yo) {
     Cell c;
     find_cell(g, x, y, c);
-    double bx[4], by[4];
+    double bx[4], by[4], v[16];
     basis(c.dx, bx);
     basis(c.dy, by);
-    xo = hsum_warp(Hx, g.n, bx, by, c, c.wx0, c.wx1, c.wx0, c.wx1, g.jx, g.hx, g.hy);
-    yo = hsum_warp(Hy, g.n, bx, by, c, c.wy0, c.wy0, c.wy1, c.wy1, g.jy, g.hx, g.hy);
+    const double wjx[4] = {c.wx0 * g.jx, c.wx1 * g.jx, c.wx0 * g.jx, c.wx1 * g.jx};
+    const double wjy[4] = {c.wy0 * g.jy, c.wy0 * g.jy, c.wy1 * g.jy, c.wy1 * g.jy};
+    load16(Hx, g.n, c, v);
+    xo = hsum_v(v, bx, by, g.hx, g.hy, wjx);
+    load16(Hy, g.n, c, v);
+    yo = hsum_v(v, bx, by, g.hx, g.hy, wjy);
 }
 
 // det(grad X), Cuda-CMM/src/numerical/cmm-hermite.cu:479-503
@@ -182,15 +208,19 @@ CMM_HD double diffeo_det(const DGrid &g, const double *__restrict__ Hx, const do
                          double y) {
     Cell c;
     find_cell(g, x, y, c);
-    double bx[4], by[4];
-    basis_d(c.dx, bx);
-    basis(c.dy, by);
-    double Xx = hsum_warp(Hx, g.n, bx, by, c, c.wx0, c.wx1, c.wx0, c.wx1, g.jx, g.hx, g.hy) / g.hx;
-    double Yx = hsum_warp(Hy, g.n, bx, by, c, c.wy0, c.wy0, c.wy1, c.wy1, g.jy, g.hx, g.hy) / g.hy;
+    double bx[4], by[4], dbx[4], dby[4], vx[16], vy[16];
     basis(c.dx, bx);
-    basis_d(c.dy, by);
-    double Xy = hsum_warp(Hx, g.n, bx, by, c, c.wx0, c.wx1, c.wx0, c.wx1, g.jx, g.hx, g.hy) / g.hx;
-    double Yy = hsum_warp(Hy, g.n, bx, by, c, c.wy0, c.wy0, c.wy1, c.wy1, g.jy, g.hx, g.hy) / g.hy;
+    basis(c.dy, by);
+    basis_d(c.dx, dbx);
+    basis_d(c.dy, dby);
+    const double wjx[4] = {c.wx0 * g.jx, c.wx1 * g.jx, c.wx0 * g.jx, c.wx1 * g.jx};
+    const double wjy[4] = {c.wy0 * g.jy, c.wy0 * g.jy, c.wy1 * g.jy, c.wy1 * g.jy};
+    load16(Hx, g.n, c, vx);
+    load16(Hy, g.n, c, vy);
+    const double Xx = hsum_v(vx, dbx, by, g.hx, g.hy, wjx) * g.ihx;
+    const double Yx = hsum_v(vy, dbx, by, g.hx, g.hy, wjy) * g.ihy;
+    const double Xy = hsum_v(vx, bx, dby, g.hx, g.hy, wjx) * g.ihx;
+    const double Yy = hsum_v(vy, bx, dby, g.hx, g.hy, wjy) * g.ihy;
     return Xx * Yy - Xy * Yx;
 }
 

@@ -149,52 +149,66 @@ __device__ __forceinline__ void step_backward(const DGrid &g, const PsiLevels &P
 }
 
 // Map advection with the footpoint stencil, Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:207-298.
-// Block 64x4; blocks are visited in XCD-contiguous order.
-template <int TIN>
+// One lane per footpoint: G consecutive lanes (G >= number of footpoints) hold the footpoints of
+// one coarse node, so a wavefront's gathers of psi and of the old map concentrate on the few
+// cells around 64/G neighbouring nodes.  After the footpoints are traced back, lane j of the group
+// forms output plane j by reading the group's positions with cross-lane shuffles and summing them
+// in footpoint order with the stencil weights, the order of the reference's accumulation.
+template <int TIN, int G>
 __global__ void __launch_bounds__(256) k_map_advect(DGrid gm, DGrid gp, const double *__restrict__ cx,
                                                     const double *__restrict__ cy, PsiLevels P, cmm_advect_coeffs c,
-                                                    double *__restrict__ ncx, double *__restrict__ ncy, int tiles_x,
-                                                    int tiles_total) {
-    const int t = xcd_block(blockIdx.x, tiles_total);
-    const int ix = (t % tiles_x) * 64 + threadIdx.x;
-    const int iy = (t / tiles_x) * 4 + threadIdx.y;
-    if (ix >= gm.nx || iy >= gm.ny) return;
-    double ax0 = 0, ax1 = 0, ax2 = 0, ax3 = 0, ay0 = 0, ay1 = 0, ay2 = 0, ay3 = 0;
+                                                    double *__restrict__ ncx, double *__restrict__ ncy, int nblocks) {
+    constexpr int NPB = 256 / G;  // nodes per block
+    const int blk = xcd_block(blockIdx.x, nblocks);
+    const int node = blk * NPB + threadIdx.x / G;
+    const int k = threadIdx.x % G;
     const int nfoot = 4 + 4 * c.map_update_order_num;
-    const double bx = gm.x0 + ix * gm.hx, by = gm.y0 + iy * gm.hy;
-    for (int k = 0; k < nfoot; ++k) {
-        const double xe = bx + ((k >> 2) + 1) * c.eps * (1 - 2 * ((k >> 1) & 1));
-        const double ye = by + ((k >> 2) + 1) * c.eps * (1 - 2 * (((k + 1) >> 1) & 1));
+    const bool active = node < gm.n;
+    double px = 0.0, py = 0.0;
+    if (active && k < nfoot) {
+        const int ix = node % gm.nx, iy = node / gm.nx;
+        const double xe = gm.x0 + ix * gm.hx + ((k >> 2) + 1) * c.eps * (1 - 2 * ((k >> 1) & 1));
+        const double ye = gm.y0 + iy * gm.hy + ((k >> 2) + 1) * c.eps * (1 - 2 * (((k + 1) >> 1) & 1));
         double xf, yf;
         step_backward<TIN>(gp, P, c, xe, ye, xf, yf);
-        double px, py;
         diffeo(gm, cx, cy, xf, yf, px, py);
-        ax0 += px * c.c1[k]; ay0 += py * c.c1[k];
-        ax1 += px * c.cx[k]; ay1 += py * c.cx[k];
-        ax2 += px * c.cy[k]; ay2 += py * c.cy[k];
-        ax3 += px * c.cxy[k]; ay3 += py * c.cxy[k];
     }
-    const int i = iy * gm.nx + ix;
-    ncx[i] = ax0; ncx[gm.n + i] = ax1; ncx[2 * gm.n + i] = ax2; ncx[3 * gm.n + i] = ax3;
-    ncy[i] = ay0; ncy[gm.n + i] = ay1; ncy[2 * gm.n + i] = ay2; ncy[3 * gm.n + i] = ay3;
+    const int base = (threadIdx.x & 63) - k;  // first lane of the group within the wavefront
+    // every lane takes part in every shuffle (the same trip count for all lanes)
+#pragma unroll
+    for (int r = 0; r < (8 + G - 1) / G; ++r) {
+        const int o = k + r * G;
+        const int plane = o & 3;
+        const double *w = plane == 0 ? c.c1 : plane == 1 ? c.cx : plane == 2 ? c.cy : c.cxy;
+        double acc = 0.0;
+        for (int m = 0; m < nfoot; ++m) {
+            const double vx = __shfl(px, base + m, 64);
+            const double vy = __shfl(py, base + m, 64);
+            acc += (o < 4 ? vx : vy) * w[m];
+        }
+        if (active && o < 8) (o < 4 ? ncx : ncy)[(long)plane * gm.n + node] = acc;
+    }
 }
 
-// max |1 - det grad X| over cell centres of gc; per-block maximum then one 64-bit atomic max on the
-// bit pattern (order-preserving for non-negative doubles).  Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:159-173
-// and the Thrust max-reduction of cmm-simulation-host.cu:44-50.
+// max |1 - det grad X| over cell centres of gc; each block reduces a strided set of 64x4 tiles,
+// then one 64-bit atomic max on the bit pattern (order-preserving for non-negative doubles) per
+// block.  Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:159-173 and the Thrust max-reduction of
+// Cuda-CMM/src/simulation/cmm-simulation-host.cu:44-50.
 __global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
 
 __global__ void __launch_bounds__(256) k_incomp(DGrid gm, DGrid gc, const double *__restrict__ cx,
                                                 const double *__restrict__ cy, unsigned long long *__restrict__ out,
                                                 int tiles_x, int tiles_total) {
-    const int t = xcd_block(blockIdx.x, tiles_total);
-    const int ix = (t % tiles_x) * 64 + threadIdx.x;
-    const int iy = (t / tiles_x) * 4 + threadIdx.y;
     double v = 0.0;
-    if (ix < gc.nx && iy < gc.ny) {
-        const double x = gm.x0 + ix * gc.hx + 0.5 * gc.hx;
-        const double y = gm.y0 + iy * gc.hy + 0.5 * gc.hy;
-        v = fabs(1 - diffeo_det(gm, cx, cy, x, y));
+    for (int tt = blockIdx.x; tt < tiles_total; tt += gridDim.x) {
+        const int t = xcd_block(tt, tiles_total);
+        const int ix = (t % tiles_x) * 64 + threadIdx.x;
+        const int iy = (t / tiles_x) * 4 + threadIdx.y;
+        if (ix < gc.nx && iy < gc.ny) {
+            const double x = gm.x0 + ix * gc.hx + 0.5 * gc.hx;
+            const double y = gm.y0 + iy * gc.hy + 0.5 * gc.hy;
+            v = fmax(v, fabs(1 - diffeo_det(gm, cx, cy, x, y)));
+        }
     }
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
     __shared__ double red[4];
@@ -202,8 +216,7 @@ __global__ void __launch_bounds__(256) k_incomp(DGrid gm, DGrid gc, const double
     __syncthreads();
     if (threadIdx.x == 0 && threadIdx.y == 0) {
         double m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
-        unsigned long long b = (unsigned long long)__double_as_longlong(m);
-        atomicMax(out, b);
+        atomicMax(out, (unsigned long long)__double_as_longlong(m));
     }
 }
 
@@ -864,19 +877,22 @@ int op_map_identity(const cmm_grid &g, double *cx, double *cy, hipStream_t st) {
 int op_map_advect(const cmm_grid &gm, const cmm_grid &gp, const double *cx, const double *cy, const PsiLevels &P,
                   const cmm_advect_coeffs &c, double *ncx, double *ncy, hipStream_t st) {
     DGrid dm = make_dgrid(gm), dp = make_dgrid(gp);
-    dim3 b(64, 4);
-    const int tx = (gm.nx + 63) / 64, ty = (gm.ny + 3) / 4, tt = tx * ty;
-#define CMM_ADV(T) hipLaunchKernelGGL(k_map_advect<T>, dim3(tt), b, 0, st, dm, dp, cx, cy, P, c, ncx, ncy, tx, tt)
+    const int nfoot = 4 + 4 * c.map_update_order_num;
+    const int G = nfoot <= 4 ? 4 : nfoot <= 8 ? 8 : 16;
+    const int nb = (int)(((long)dm.n * G + 255) / 256);
+#define CMM_ADV(T, GG) hipLaunchKernelGGL((k_map_advect<T, GG>), dim3(nb), dim3(256), 0, st, dm, dp, cx, cy, P, c, ncx, ncy, nb)
+#define CMM_ADV_G(T) do { if (G == 4) CMM_ADV(T, 4); else if (G == 8) CMM_ADV(T, 8); else CMM_ADV(T, 16); } while (0)
     switch (c.time_integration_num) {
-    case 10: CMM_ADV(10); break;
-    case 20: CMM_ADV(20); break;
-    case 21: CMM_ADV(21); break;
-    case 30: CMM_ADV(30); break;
-    case 31: CMM_ADV(31); break;
-    case 40: CMM_ADV(40); break;
-    case 41: CMM_ADV(41); break;
+    case 10: CMM_ADV_G(10); break;
+    case 20: CMM_ADV_G(20); break;
+    case 21: CMM_ADV_G(21); break;
+    case 30: CMM_ADV_G(30); break;
+    case 31: CMM_ADV_G(31); break;
+    case 40: CMM_ADV_G(40); break;
+    case 41: CMM_ADV_G(41); break;
     default: set_error("unknown time integration"); return CMM_ERR_ARG;
     }
+#undef CMM_ADV_G
 #undef CMM_ADV
     return launch_check("k_map_advect");
 }
@@ -887,7 +903,8 @@ int op_incomp_launch(const cmm_grid &gm, const cmm_grid &gc, const double *cx, c
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, slot);
     dim3 b(64, 4);
     const int tx = (gc.nx + 63) / 64, ty = (gc.ny + 3) / 4, tt = tx * ty;
-    hipLaunchKernelGGL(k_incomp, dim3(tt), b, 0, st, dm, dc, cx, cy, slot, tx, tt);
+    const int nb = std::min(tt, 2048);
+    hipLaunchKernelGGL(k_incomp, dim3(nb), b, 0, st, dm, dc, cx, cy, slot, tx, tt);
     return launch_check("k_incomp");
 }
 

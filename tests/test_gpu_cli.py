@@ -55,8 +55,15 @@ def compare(mon, case):
         if key in ("Time_s", "Map_counter", "Map_gaps", "Mesure/Time_s", "Debug_globals/Time_s"):
             assert np.array_equal(got, ref), key
         else:
-            # E_inc is 1 - det, a cancellation: its rounding level is absolute (~1e-12)
-            tol = 1e-10 * np.abs(ref) + (2e-11 if key == "Error_incompressibility" else 1e-12)
+            # E_inc is 1 - det, a cancellation: its rounding level is absolute (~1e-12).  Statistics of
+            # whole map Hermite arrays include the cross-derivative plane, footpoint differences over
+            # eps^2 = 1e-6 whose rounding is ~1e-9 (Min/Max/Avg land there).
+            if key == "Error_incompressibility":
+                tol = 1e-10 * np.abs(ref) + 2e-11
+            elif "_H_coarse_" in key:
+                tol = 1e-10 * np.abs(ref) + 1e-8
+            else:
+                tol = 1e-10 * np.abs(ref) + 1e-12
             bad = np.abs(got - ref) > tol
             assert not bad.any(), f"{key}: got {got[bad][:3]} expected {ref[bad][:3]}"
         checked += 1
