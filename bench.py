@@ -97,6 +97,8 @@ def main():
     ap.add_argument("--steps", type=int, default=504)
     ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one 1 GiB device copy, the reference kernel for counter scaling")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs one engine instance per GPU and has no multi-GPU "
@@ -110,6 +112,12 @@ def main():
         sys.exit(1)
     c = dict(C2)
     log("device:", cmm.device_info())
+    if args.copy_calibration:
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     g = cmm.grid
     t_setup = time.perf_counter()
     eng = cmm.Euler(g(c["n_coarse"]), g(c["n_fine"]), g(c["n_psi"]), g(c["n_vort"]), c["dt"],
