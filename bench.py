@@ -7,7 +7,11 @@ vorticity Hermite), psi solve, and RK3 advection of 4096 particles.  The run sta
 random-spectrum vorticity (E(k) ~ k^4 exp(-2 (k/6)^2), enstrophy 1), dt = 1/256; W untimed
 steps then K timed steps (default W + K = 512 steps = T 2).  Prints one JSON line.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--copy-calibration]
+  python bench.py summarize-profiles KT_DIR FETCH_DIR WRITE_DIR OUT_DIR "kt cmd" "fetch cmd" "write cmd"
+
+roofline.traffic is the dominant kernel's HBM bytes per launch from the committed counter summary
+(FETCH_SIZE + WRITE_SIZE, scaled by the 1 GiB copy calibration), null when that file is absent.
 """
 import argparse
 import json
@@ -59,10 +63,11 @@ def uniform_particles(n, seed):
     return cmm.uniform_particles(n, seed)
 
 
-def cpu_baseline(c, threads):
-    """The oracle (C++/OpenMP, FP64) timed on one step's worth of the hot-path operators at the same
-    sizes (advect 1024^2, check on 4096^2, psi solve 4096^2 -> 2048^2, 4096 particles) from a
-    random-spectrum state and the identity map; returns steps/s."""
+def cpu_baseline(c, threads, budget_s=12.0):
+    """The oracle (C++/OpenMP, FP64, same algorithm) timed on the hot-path operators of one step at
+    full size (advect 1024^2 with psi 2048^2, check on 4096^2, psi solve 4096^2 -> 2048^2, 4096 RK3
+    particles) from a random-spectrum state, repeated until budget_s of CPU work has run (at least
+    two repetitions); returns (steps/s, repetitions, seconds)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     os.environ["OMP_NUM_THREADS"] = str(threads)
     import numpy as np
@@ -77,21 +82,115 @@ def cpu_baseline(c, threads):
     dt = c["dt"]
     t = np.array([-2 * dt, -dt, 0.0, dt])
     coeff = O.advect_coeffs(t, np.full(4, dt), 0, 3, 30, 1, c["eps"])
-    t0 = time.perf_counter()
-    nx, ny = O.map_advect(gc, gp, cx, cy, psi, coeff)
-    t1 = time.perf_counter()
-    O.incomp_error(gc, gf, nx, ny)
-    t2 = time.perf_counter()
-    O.psi_from_vort(gc, gf, gf, gp, nx, ny, W, 0, c["freq_cut_psi"])
-    t3 = time.perf_counter()
-    O.particles_advect(gp, psi, 3, [0.375, 0.75, -0.125], dt, 30, pos)
-    t4 = time.perf_counter()
-    log(f"cpu baseline ({threads} threads): advect {t1 - t0:.2f}s incomp {t2 - t1:.2f}s psi {t3 - t2:.2f}s "
-        f"particles {t4 - t3:.3f}s")
-    return 1.0 / (t4 - t0), t4 - t0
+    parts = np.zeros(4)
+    reps = 0
+    t_start = time.perf_counter()
+    while reps < 2 or time.perf_counter() - t_start < budget_s:
+        t0 = time.perf_counter()
+        nx, ny = O.map_advect(gc, gp, cx, cy, psi, coeff)
+        t1 = time.perf_counter()
+        O.incomp_error(gc, gf, nx, ny)
+        t2 = time.perf_counter()
+        O.psi_from_vort(gc, gf, gf, gp, nx, ny, W, 0, c["freq_cut_psi"])
+        t3 = time.perf_counter()
+        O.particles_advect(gp, psi, 3, [0.375, 0.75, -0.125], dt, 30, pos.copy())
+        t4 = time.perf_counter()
+        parts += np.array([t1 - t0, t2 - t1, t3 - t2, t4 - t3])
+        reps += 1
+    total = time.perf_counter() - t_start
+    parts /= reps
+    log(f"cpu baseline ({threads} threads, {reps} reps, {total:.1f}s): advect {parts[0]:.3f}s incomp {parts[1]:.3f}s "
+        f"psi {parts[2]:.3f}s particles {parts[3]:.4f}s per step")
+    return reps / total, reps, total
+
+
+# bench slot -> kernel-name pattern of the launches it times (engine psi solve samples omega with SampleVort)
+SLOT_KERNELS = {"advect": "k_map_advect<30, 8>", "incomp": "k_incomp(", "psi_row_r2c": "k_row_r2c<cmm::SampleVort>",
+                "psi_col": "k_col(", "psi_row_c2r": "k_row_c2r4(", "particles": "k_particles("}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "cmm_euler_c2_hbm_traffic.json")
+FETCH_SCALE_DEFAULT, WRITE_SCALE_DEFAULT = 2.0, 1.0
+
+
+def _read_csv(path):
+    import csv
+    with open(path, newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def _find(d, suffix):
+    for root, _, files in os.walk(d):
+        for f in files:
+            if f.endswith(suffix):
+                return os.path.join(root, f)
+    raise FileNotFoundError(f"no *{suffix} under {d}")
+
+
+def summarize_profiles(kt_dir, fetch_dir, write_dir, out_dir, commands):
+    """Turns rocprofv3 output (a --kernel-trace --stats run and two --pmc runs, FETCH_SIZE and
+    WRITE_SIZE) into the text summaries kept with the sources: per-kernel durations, and HBM bytes
+    per dispatch scaled by what the same counters read for the 1 GiB copy (rocclr copyBuffer)."""
+    import collections
+    stats = _read_csv(_find(kt_dir, "kernel_stats.csv"))
+    lines = ["# rocprofv3 kernel statistics, 1024^2 map / 2048^2 psi / 4096^2 fine Euler workload (MI355X)", "",
+             "command: " + commands["kt"], "",
+             f"{'kernel':78s} {'calls':>6s} {'total_ms':>10s} {'avg_us':>10s} {'pct':>6s}"]
+    for r in sorted(stats, key=lambda r: -float(r["TotalDurationNs"])):
+        lines.append(f"{r['Name'][:78]:78s} {int(r['Calls']):6d} {float(r['TotalDurationNs']) / 1e6:10.3f} "
+                     f"{float(r['AverageNs']) / 1e3:10.2f} {float(r['Percentage']):6.2f}")
+    with open(os.path.join(out_dir, "cmm_euler_c2_kernel_stats.txt"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+    def per_kernel(d, counter):
+        # (kernel, grid size) -> values; the time loop's launches are the most frequent grid size
+        acc = collections.defaultdict(list)
+        for r in _read_csv(_find(d, "counter_collection.csv")):
+            if r["Counter_Name"] == counter:
+                acc[(r["Kernel_Name"], r["Grid_Size"])].append(float(r["Counter_Value"]))
+        best = {}
+        for (k, g), vs in acc.items():
+            if k not in best or len(vs) > len(best[k]):
+                best[k] = vs
+        copies = [v for (k, g), vs in acc.items() if "copyBuffer" in k for v in vs]
+        return best, copies
+    (fetch, cf), (write, cw) = per_kernel(fetch_dir, "FETCH_SIZE"), per_kernel(write_dir, "WRITE_SIZE")
+    copy_kib = float(1 << 20)  # the calibration copy moves 1 GiB each way
+    fs = copy_kib / max(cf) if cf else FETCH_SCALE_DEFAULT
+    ws = copy_kib / max(cw) if cw else WRITE_SCALE_DEFAULT
+    c = dict(C2)
+    kx_max = int(min(c["n_psi"] // 2, c["n_vort"] // 2, c["freq_cut_psi"]))
+    lines = ["# HBM traffic per dispatch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (KiB per dispatch),",
+             "# scaled by what the counters read for a 1 GiB device copy, 1024^2 / 2048^2 / 4096^2 Euler workload",
+             "", "fetch command: " + commands["fetch"], "write command: " + commands["write"], "",
+             f"copy calibration: FETCH_SIZE {max(cf) if cf else float('nan'):.1f} KiB, WRITE_SIZE "
+             f"{max(cw) if cw else float('nan'):.1f} KiB for 1048576 KiB -> scale fetch x{fs:.3f}, write x{ws:.3f}", "",
+             f"{'slot':12s} {'kernel':40s} {'n':>4s} {'fetch_MiB':>10s} {'write_MiB':>10s} {'alg_MiB':>10s} "
+             f"{'ratio':>6s}"]
+    out = {"fetch_scale": fs, "write_scale": ws, "slots": {}}
+    for slot, pat in SLOT_KERNELS.items():
+        ks = [k for k in fetch if pat in k]
+        if not ks:
+            continue
+        fv = [v for k in ks for v in fetch[k]]
+        wv = [v for k in write if pat in k for v in write[k]]
+        fb = 1024.0 * fs * sum(fv) / len(fv)
+        wb = 1024.0 * ws * sum(wv) / len(wv) if wv else float("nan")
+        alg = algorithmic(slot, c, kx_max)[0]
+        out["slots"][slot] = {"fetch_bytes": fb, "write_bytes": wb, "algorithmic_bytes": alg, "dispatches": len(fv)}
+        lines.append(f"{slot:12s} {ks[0].split('(')[0][:40]:40s} {len(fv):4d} {fb / 2**20:10.1f} {wb / 2**20:10.1f} "
+                     f"{alg / 2**20:10.1f} {(fb + wb) / alg:6.2f}")
+    with open(os.path.join(out_dir, "cmm_euler_c2_hbm_traffic.txt"), "w") as f:
+        f.write("\n".join(lines) + "\n")
+    with open(os.path.join(out_dir, "cmm_euler_c2_hbm_traffic.json"), "w") as f:
+        json.dump(out, f, indent=1)
+    print("\n".join(lines))
 
 
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "summarize-profiles":
+        kt, fe, wr, out = sys.argv[2:6]
+        cmds = dict(zip(("kt", "fetch", "write"), sys.argv[6:9]))
+        summarize_profiles(kt, fe, wr, out, cmds)
+        return
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=504)
@@ -169,16 +268,22 @@ def main():
     else:
         roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(gbs / HBM_PEAK_GBS, 4)}
-    roof.update({"kernel": dom, "traffic": None, "ms_avg": round(d["ms_avg"], 5),
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        tr = json.load(open(TRAFFIC_FILE))["slots"].get(dom)
+        if tr:
+            traffic = int(tr["fetch_bytes"] + tr["write_bytes"])
+    roof.update({"kernel": dom, "traffic": traffic, "ms_avg": round(d["ms_avg"], 5),
                  "algorithmic_bytes": d["bytes"], "algorithmic_flops": d["flops"]})
 
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
-        v, secs = cpu_baseline(c, threads)
+        v, reps, secs = cpu_baseline(c, threads)
         cpu = {"value": round(v, 5), "unit": "steps/s", "cores": threads, "kind": "port",
-               "sample": f"one step's worth of the hot-path operators at full size (advect 1024^2, check 4096^2, "
-                         f"psi 4096^2->2048^2, 4096 particles), {secs:.1f} s of CPU work"}
+               "sample": f"{reps} repetitions of one step's hot-path operators at full size (advect 1024^2 map with "
+                         f"2048^2 psi, check on 4096^2, psi solve 4096^2->2048^2, 4096 RK3 particles) by the "
+                         f"OpenMP oracle, {secs:.1f} s of CPU work"}
 
     out = {
         "metric": METRIC, "value": round(sps, 3), "unit": "steps/s", "n_gpus": 1, "steps": args.steps,

@@ -73,76 +73,83 @@ __device__ __forceinline__ void lsum(const double *w, const double *u, int nl, d
 // Backward footpoint integration for one start point; schemes of
 // Cuda-CMM/src/numerical/cmm-timestep.cu (EulerExp :35-45, AB2 :115-137, Heun :68-85,
 // RK3 :175-207, RK3Mod :357-386, RK4 :261-304, RK4Mod :441-470).
-template <int TIN>
-__device__ __forceinline__ void step_backward(const DGrid &g, const PsiLevels &P, const cmm_advect_coeffs &c,
-                                              double x0, double y0, double &xo, double &yo) {
+// velocity of psi levels 0..nl-1 read straight from memory
+struct VelGlobal {
+    DGrid g;
+    PsiLevels P;
+    __device__ __forceinline__ void operator()(int nl, double x, double y, double *u) const { herm_vel(g, P, nl, x, y, u); }
+    __device__ __forceinline__ VelGlobal level1() const { return VelGlobal{g, PsiLevels{{P.p[1], P.p[1], P.p[1], P.p[1]}}}; }
+};
+
+template <int TIN, class V>
+__device__ __forceinline__ void step_backward(const V &vel, const cmm_advect_coeffs &c, double x0, double y0,
+                                              double &xo, double &yo) {
     const int L = c.l_order;
     const double dt = c.dt;
     double u[8];
     if constexpr (TIN == 10) {
-        herm_vel(g, P, L, x0, y0, u);
+        vel(L, x0, y0, u);
         double kx, ky;
         lsum(c.L1, u, L, kx, ky);
         xo = x0 - dt * kx;
         yo = y0 - dt * ky;
     } else if constexpr (TIN == 20) {
-        PsiLevels P1 = {{P.p[1], P.p[1], P.p[1], P.p[1]}};
+        const auto v1 = vel.level1();
         double uu[4], xp = x0, yp = y0;
         xo = x0;
         yo = y0;
         for (int it = 0; it < 10; ++it) {
-            herm_vel(g, P1, 1, xp, yp, uu + 2);
+            v1(1, xp, yp, uu + 2);
             xp = xo - dt * uu[2];
             yp = yo - dt * uu[3];
-            herm_vel(g, P1, 1, xp, yp, uu + 2);
-            herm_vel(g, P, 1, xo, yo, uu);
+            v1(1, xp, yp, uu + 2);
+            vel(1, xo, yo, uu);
             xo = x0 - dt * (1.5 * uu[0] - 0.5 * uu[2]);
             yo = y0 - dt * (1.5 * uu[1] - 0.5 * uu[3]);
         }
     } else if constexpr (TIN == 21) {
-        herm_vel(g, P, L, x0, y0, u);
+        vel(L, x0, y0, u);
         double k1x, k1y;
         lsum(c.L1, u, L, k1x, k1y);
-        herm_vel(g, P, 1, x0 - dt * k1x, y0 - dt * k1y, u);
+        vel(1, x0 - dt * k1x, y0 - dt * k1y, u);
         xo = x0 - dt * (k1x + u[0]) / 2;
         yo = y0 - dt * (k1y + u[1]) / 2;
     } else if constexpr (TIN == 30) {
         double k[6];
-        herm_vel(g, P, L, x0, y0, u);
+        vel(L, x0, y0, u);
         lsum(c.L1, u, L, k[0], k[1]);
-        herm_vel(g, P, L, x0 - dt * k[0] / 2.0, y0 - dt * k[1] / 2.0, u);
+        vel(L, x0 - dt * k[0] / 2.0, y0 - dt * k[1] / 2.0, u);
         lsum(c.L12, u, L, k[2], k[3]);
-        herm_vel(g, P, 1, x0 + dt * k[0] - 2 * dt * k[2], y0 + dt * k[1] - 2 * dt * k[3], k + 4);
+        vel(1, x0 + dt * k[0] - 2 * dt * k[2], y0 + dt * k[1] - 2 * dt * k[3], k + 4);
         xo = x0 - dt * (k[0] + 4 * k[2] + k[4]) / 6.0;
         yo = y0 - dt * (k[1] + 4 * k[3] + k[5]) / 6.0;
     } else if constexpr (TIN == 31) {
         double k[6];
-        herm_vel(g, P, L, x0, y0, u);
+        vel(L, x0, y0, u);
         lsum(c.L1, u, L, k[0], k[1]);
-        herm_vel(g, P, 1, x0 - dt * k[0], y0 - dt * k[1], k + 2);
-        PsiLevels P1 = {{P.p[1], P.p[1], P.p[1], P.p[1]}};
-        herm_vel(g, P1, 1, x0 + 2 * dt * (-2 * k[0] + k[2]), y0 + 2 * dt * (-2 * k[1] + k[3]), k + 4);
+        vel(1, x0 - dt * k[0], y0 - dt * k[1], k + 2);
+        vel.level1()(1, x0 + 2 * dt * (-2 * k[0] + k[2]), y0 + 2 * dt * (-2 * k[1] + k[3]), k + 4);
         xo = x0 - dt * (5 * k[0] + 8 * k[2] - k[4]) / 12.0;
         yo = y0 - dt * (5 * k[1] + 8 * k[3] - k[5]) / 12.0;
     } else if constexpr (TIN == 40) {
         double k[8];
-        herm_vel(g, P, L, x0, y0, u);
+        vel(L, x0, y0, u);
         lsum(c.L1, u, L, k[0], k[1]);
-        herm_vel(g, P, L, x0 - dt * k[0] / 2.0, y0 - dt * k[1] / 2.0, u);
+        vel(L, x0 - dt * k[0] / 2.0, y0 - dt * k[1] / 2.0, u);
         lsum(c.L12, u, L, k[2], k[3]);
-        herm_vel(g, P, L, x0 - dt * k[2] / 2.0, y0 - dt * k[3] / 2.0, u);
+        vel(L, x0 - dt * k[2] / 2.0, y0 - dt * k[3] / 2.0, u);
         lsum(c.L12, u, L, k[4], k[5]);
-        herm_vel(g, P, 1, x0 - dt * k[4], y0 - dt * k[5], k + 6);
+        vel(1, x0 - dt * k[4], y0 - dt * k[5], k + 6);
         xo = x0 - dt * (k[0] + 2 * k[2] + 2 * k[4] + k[6]) / 6.0;
         yo = y0 - dt * (k[1] + 2 * k[3] + 2 * k[5] + k[7]) / 6.0;
     } else {  // 41
         double k[8];
-        herm_vel(g, P, L, x0, y0, u);
+        vel(L, x0, y0, u);
         lsum(c.L1, u, L, k[0], k[1]);
-        herm_vel(g, P, 1, x0 - dt * k[0], y0 - dt * k[1], k + 2);
-        herm_vel(g, P, L, x0 - dt * (3 * k[0] + k[2]) / 8.0, y0 - dt * (3 * k[1] + k[3]) / 8.0, u);
+        vel(1, x0 - dt * k[0], y0 - dt * k[1], k + 2);
+        vel(L, x0 - dt * (3 * k[0] + k[2]) / 8.0, y0 - dt * (3 * k[1] + k[3]) / 8.0, u);
         lsum(c.L12, u, L, k[4], k[5]);
-        herm_vel(g, P, 1, x0 + dt * (k[2] - 4 * k[4]) / 3.0, y0 + dt * (k[3] - 4 * k[5]) / 3.0, k + 6);
+        vel(1, x0 + dt * (k[2] - 4 * k[4]) / 3.0, y0 + dt * (k[3] - 4 * k[5]) / 3.0, k + 6);
         xo = x0 - dt * (2 * k[0] - k[2] + 8 * k[4] + 3 * k[6]) / 12.0;
         yo = y0 - dt * (2 * k[1] - k[3] + 8 * k[5] + 3 * k[7]) / 12.0;
     }
@@ -170,7 +177,7 @@ __global__ void __launch_bounds__(256) k_map_advect(DGrid gm, DGrid gp, const do
         const double xe = gm.x0 + ix * gm.hx + ((k >> 2) + 1) * c.eps * (1 - 2 * ((k >> 1) & 1));
         const double ye = gm.y0 + iy * gm.hy + ((k >> 2) + 1) * c.eps * (1 - 2 * (((k + 1) >> 1) & 1));
         double xf, yf;
-        step_backward<TIN>(gp, P, c, xe, ye, xf, yf);
+        step_backward<TIN>(VelGlobal{gp, P}, c, xe, ye, xf, yf);
         diffeo(gm, cx, cy, xf, yf, px, py);
     }
     const int base = (threadIdx.x & 63) - k;  // first lane of the group within the wavefront

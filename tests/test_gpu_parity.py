@@ -142,6 +142,41 @@ def test_map_advect_rk3(dev, base):
             assert max(errs) <= TOL
 
 
+@pytest.mark.parametrize("ratio", [1, 2])
+def test_map_advect_schemes(dev, base, ratio):
+    """Every time scheme (Euler, AB2, RK2, RK3, RK3-mod, RK4, RK4-mod) and footpoint stencil order
+    (2, 4, 6) on the deformed map, with psi on the map grid and on a 2x finer grid, at a small
+    (1/32) and a large (1/2) step."""
+    import cmm
+    gc, gf, _ = grids()
+    ogc, ogf, _ = ogrids()
+    npsi = GC * ratio
+    ogp, gp = O.grid(npsi), cmm.grid(npsi)
+    if ratio == 1:
+        psi = base["psi"]
+    else:
+        p1 = O.psi_from_vort(ogc, ogf, ogf, ogp, base["chix"], base["chiy"], base["w_herm"], 0, 128.0)
+        psi = np.concatenate([p1, 0.9 * p1, 0.8 * p1])
+    n4 = 4 * npsi * npsi
+    lv = [dev_array(psi[i * n4:(i + 1) * n4]) for i in range(3)]
+    mx, my = dev_array(base["chix"]), dev_array(base["chiy"])
+    n = GC * GC
+    for dt in (1.0 / 32, 0.5):
+        t = np.array([-2 * dt, -dt, 0.0, dt])
+        dts = np.full(4, dt)
+        for tin in (10, 20, 21, 30, 31, 40, 41):
+            for muo in (0, 1, 2):
+                oc = O.advect_coeffs(t, dts, 0, 3, tin, muo, 1e-3)
+                c = cmm.advect_coeffs(t, dts, 0, 3, tin, muo, 1e-3)
+                rx, ry = O.map_advect(ogc, ogp, base["chix"], base["chiy"], psi, oc)
+                ox = dev_array(np.zeros(4 * n))
+                oy = dev_array(np.zeros(4 * n))
+                cmm.advect_using_stream_hermite(gc, gp, mx, my, lv, c, ox, oy)
+                e = max(max(hermite_errors(host(ox), rx, n, 1.0 / 1e-3)), max(hermite_errors(host(oy), ry, n, 1.0 / 1e-3)))
+                print(f"advect r={ratio} dt={dt} tin={tin} order={muo}: {e:.2e}")
+                assert e <= TOL, (ratio, dt, tin, muo, e)
+
+
 def test_incompressibility(dev, base):
     import cmm
     gc, gf, gp = grids()
