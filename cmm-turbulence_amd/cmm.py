@@ -67,7 +67,7 @@ FIELD = {"chix": 0, "chiy": 1, "psi": 2, "w_fine": 3, "w_coarse": 4, "disc": 5}
 
 # every symbol include/cmm.h declares
 SYMBOLS = ["cmm_version", "cmm_last_error", "cmm_device_info", "cmm_lagrange_coeff", "cmm_advect_coeffs_fill",
-           "cmm_map_init_identity", "cmm_map_advect", "cmm_incomp_error", "cmm_compose_positions",
+           "cmm_map_init_identity", "cmm_advect_workspace_bytes", "cmm_map_advect", "cmm_incomp_error", "cmm_compose_positions",
            "cmm_sample_init", "cmm_translate_workspace_bytes", "cmm_translate_ic", "cmm_hermite_workspace_bytes",
            "cmm_hermite_from_real", "cmm_hermite_from_spectrum", "cmm_psi_workspace_bytes", "cmm_psi_from_vort",
            "cmm_sample_vort", "cmm_particles_advect", "cmm_euler_create", "cmm_euler_destroy", "cmm_euler_stream",
@@ -92,7 +92,10 @@ def lib():
         L.cmm_advect_coeffs_fill.argtypes = [DP, DP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double, C.c_int,
                                              C.POINTER(AdvectCoeffs)]
         L.cmm_map_init_identity.argtypes = [Grid, VP, VP, VP]
-        L.cmm_map_advect.argtypes = [Grid, Grid, VP, VP, C.POINTER(VP), C.POINTER(AdvectCoeffs), VP, VP, VP]
+        L.cmm_advect_workspace_bytes.restype = C.c_size_t
+        L.cmm_advect_workspace_bytes.argtypes = [Grid, C.POINTER(AdvectCoeffs)]
+        L.cmm_map_advect.argtypes = [Grid, Grid, VP, VP, C.POINTER(VP), C.POINTER(AdvectCoeffs), VP, VP, VP,
+                                     C.c_size_t, VP]
         L.cmm_incomp_error.argtypes = [Grid, Grid, VP, VP, DP, VP]
         L.cmm_compose_positions.argtypes = [Grid, Grid, VP, VP, VP, C.c_int, VP, VP]
         L.cmm_sample_init.argtypes = [Grid, Grid, VP, VP, VP, C.c_int, C.c_int, DP, C.c_int, Grid, VP, VP, VP]
@@ -171,9 +174,17 @@ def map_init_identity(g, chix, chiy):
     _check(lib().cmm_map_init_identity(g, _p(chix), _p(chiy), _stream()), "cmm_map_init_identity")
 
 
-def advect_using_stream_hermite(g_map, g_psi, chix, chiy, psi_levels, coeffs, chix_new, chiy_new):
+def advect_workspace_bytes(g_psi, coeffs):
+    return lib().cmm_advect_workspace_bytes(g_psi, C.byref(coeffs))
+
+
+def advect_using_stream_hermite(g_map, g_psi, chix, chiy, psi_levels, coeffs, chix_new, chiy_new, work=None):
+    """work: None interpolates every psi level per evaluation (the reference's order); a device
+    buffer of advect_workspace_bytes() bytes lets the call combine the levels first."""
+    wb = 0 if work is None else work.numel() * work.element_size()
     _check(lib().cmm_map_advect(g_map, g_psi, _p(chix), _p(chiy), _levels(psi_levels), C.byref(coeffs),
-                                _p(chix_new), _p(chiy_new), _stream()), "cmm_map_advect")
+                                _p(chix_new), _p(chiy_new), _p(work) if work is not None else None, wb, _stream()),
+           "cmm_map_advect")
 
 
 def incompressibility_check(g_check, g_map, chix, chiy):

@@ -100,8 +100,14 @@ int cmm_map_init_identity(cmm_grid g, double *cx, double *cy, void *stream) {
     return op_map_identity(g, cx, cy, S(stream));
 }
 
+size_t cmm_advect_workspace_bytes(cmm_grid gp, const cmm_advect_coeffs *c) {
+    if (!grid_ok(gp) || !c) return 0;
+    return op_advect_workspace(gp, *c);
+}
+
 int cmm_map_advect(cmm_grid gm, cmm_grid gp, const double *cx, const double *cy, const double *const *psi,
-                   const cmm_advect_coeffs *c, double *ncx, double *ncy, void *stream) {
+                   const cmm_advect_coeffs *c, double *ncx, double *ncy, void *work, size_t work_bytes,
+                   void *stream) {
     if (!grid_ok(gm) || !grid_ok(gp) || !cx || !cy || !psi || !c || !ncx || !ncy) {
         set_error("cmm_map_advect: bad argument");
         return CMM_ERR_ARG;
@@ -114,7 +120,7 @@ int cmm_map_advect(cmm_grid gm, cmm_grid gp, const double *cx, const double *cy,
     if (ncx == cx || ncy == cy || ncx == cy || ncy == cx) { set_error("cmm_map_advect: output aliases input"); return CMM_ERR_ARG; }
     for (int l = 0; l < c->l_order; ++l)
         if (!psi[l]) { set_error("cmm_map_advect: null psi level"); return CMM_ERR_ARG; }
-    return op_map_advect(gm, gp, cx, cy, levels(psi, c->l_order), *c, ncx, ncy, S(stream));
+    return op_map_advect(gm, gp, cx, cy, levels(psi, c->l_order), *c, ncx, ncy, work, work_bytes, S(stream));
 }
 
 int cmm_incomp_error(cmm_grid gm, cmm_grid gc, const double *cx, const double *cy, double *out, void *stream) {
@@ -259,7 +265,10 @@ int engine_advect(cmm_euler *e, const double *t, const double *dt, int loop_ctr,
         return rc;
     PsiLevels P;
     for (int l = 0; l < 4; ++l) P.p[l] = e->lvl[l < L ? l : 0];
-    if (int rc = op_map_advect(e->p.g_coarse, e->p.g_psi, e->cx, e->cy, P, c, e->ncx, e->ncy, e->st)) return rc;
+    // the workspace is shared with the psi solve, which runs later on the same stream
+    if (int rc = op_map_advect(e->p.g_coarse, e->p.g_psi, e->cx, e->cy, P, c, e->ncx, e->ncy, e->work, e->work_bytes,
+                               e->st))
+        return rc;
     std::swap(e->cx, e->ncx);
     std::swap(e->cy, e->ncy);
     return CMM_OK;
@@ -392,6 +401,8 @@ int cmm_euler_create(const cmm_euler_params *p, cmm_euler **out) {
     if (!rc) {
         size_t w = std::max(op_psi_workspace(p->g_vort, p->g_psi, p->freq_cut_psi), op_hermite_workspace(p->g_fine));
         if (p->discrete) w = std::max(w, op_hermite_workspace(p->g_disc));
+        // Lagrange-weighted psi combinations of the map advection (two Hermite fields on g_psi)
+        if (p->l_order >= 2) w = std::max(w, 8 * sizeof(double) * (size_t)e->np);
         e->work_bytes = w;
         if (hipMalloc(&e->work, w) != hipSuccess) { set_error("hipMalloc workspace"); rc = CMM_ERR_ALLOC; }
     }

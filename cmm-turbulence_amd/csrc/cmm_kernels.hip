@@ -70,88 +70,158 @@ __device__ __forceinline__ void lsum(const double *w, const double *u, int nl, d
     }
 }
 
+// Velocity sources for the footpoint integration.  w1 / w12: velocity extrapolated in time with the
+// Lagrange weights L1 (to t_{n+1}) / L12 (to t_n + dt/2); lev(l): velocity of psi level l.
+//
+// VelLevels evaluates every level at the point and weights the velocities, the reference's order
+// (Cuda-CMM/src/numerical/cmm-timestep.cu:175-207 with cmm-hermite.cu:396-420).
+struct VelLevels {
+    DGrid g;
+    PsiLevels P;
+    // herm_vel over L levels followed by lsum, unrolled over the (at most 4) levels so that no
+    // array is indexed at run time
+    __device__ __forceinline__ void w(const double *Lw, int L, double x, double y, double &kx, double &ky) const {
+        Cell cc;
+        find_cell(g, x, y, cc);
+        double bx[4], by[4], dbx[4], dby[4];
+        basis(cc.dx, bx);
+        basis(cc.dy, by);
+        basis_d(cc.dx, dbx);
+        basis_d(cc.dy, dby);
+#pragma unroll
+        for (int l = 0; l < 4; ++l) {
+            if (l < L) {
+                double v[16];
+                load16(P.p[l], g.n, cc, v);
+                const double uy = -hsum_v(v, dbx, by, g.hx, g.hy) * g.ihx;
+                const double ux = hsum_v(v, bx, dby, g.hx, g.hy) * g.ihy;
+                if (l == 0) {
+                    kx = Lw[0] * ux;
+                    ky = Lw[0] * uy;
+                } else {
+                    kx += Lw[l] * ux;
+                    ky += Lw[l] * uy;
+                }
+            }
+        }
+    }
+    __device__ __forceinline__ void w1(const cmm_advect_coeffs &c, double x, double y, double &kx, double &ky) const {
+        w(c.L1, c.l_order, x, y, kx, ky);
+    }
+    __device__ __forceinline__ void w12(const cmm_advect_coeffs &c, double x, double y, double &kx, double &ky) const {
+        w(c.L12, c.l_order, x, y, kx, ky);
+    }
+    __device__ __forceinline__ void lev(int l, double x, double y, double &ux, double &uy) const {
+        double u[2];
+        herm_vel(g, PsiLevels{{P.p[l], P.p[l], P.p[l], P.p[l]}}, 1, x, y, u);
+        ux = u[0];
+        uy = u[1];
+    }
+};
+
+// VelCombined reads the precombined fields C1 = sum_l L1[l] psi_l and C12 = sum_l L12[l] psi_l
+// (Hermite interpolation is linear in the field, so this equals the weighted velocities up to
+// rounding) with one gather of 16 values per evaluation instead of 16 per level.
+struct VelCombined {
+    DGrid g;
+    PsiLevels P;
+    const double *C1, *C12;
+    __device__ __forceinline__ void one(const double *H, double x, double y, double &ux, double &uy) const {
+        double u[2];
+        herm_vel(g, PsiLevels{{H, H, H, H}}, 1, x, y, u);
+        ux = u[0];
+        uy = u[1];
+    }
+    __device__ __forceinline__ void w1(const cmm_advect_coeffs &, double x, double y, double &kx, double &ky) const {
+        one(C1, x, y, kx, ky);
+    }
+    __device__ __forceinline__ void w12(const cmm_advect_coeffs &, double x, double y, double &kx, double &ky) const {
+        one(C12, x, y, kx, ky);
+    }
+    __device__ __forceinline__ void lev(int l, double x, double y, double &ux, double &uy) const {
+        one(P.p[l], x, y, ux, uy);
+    }
+};
+
 // Backward footpoint integration for one start point; schemes of
 // Cuda-CMM/src/numerical/cmm-timestep.cu (EulerExp :35-45, AB2 :115-137, Heun :68-85,
 // RK3 :175-207, RK3Mod :357-386, RK4 :261-304, RK4Mod :441-470).
-// velocity of psi levels 0..nl-1 read straight from memory
-struct VelGlobal {
-    DGrid g;
-    PsiLevels P;
-    __device__ __forceinline__ void operator()(int nl, double x, double y, double *u) const { herm_vel(g, P, nl, x, y, u); }
-    __device__ __forceinline__ VelGlobal level1() const { return VelGlobal{g, PsiLevels{{P.p[1], P.p[1], P.p[1], P.p[1]}}}; }
-};
-
 template <int TIN, class V>
 __device__ __forceinline__ void step_backward(const V &vel, const cmm_advect_coeffs &c, double x0, double y0,
                                               double &xo, double &yo) {
-    const int L = c.l_order;
     const double dt = c.dt;
-    double u[8];
     if constexpr (TIN == 10) {
-        vel(L, x0, y0, u);
         double kx, ky;
-        lsum(c.L1, u, L, kx, ky);
+        vel.w1(c, x0, y0, kx, ky);
         xo = x0 - dt * kx;
         yo = y0 - dt * ky;
     } else if constexpr (TIN == 20) {
-        const auto v1 = vel.level1();
-        double uu[4], xp = x0, yp = y0;
+        double u0x, u0y, u1x, u1y, xp = x0, yp = y0;
         xo = x0;
         yo = y0;
         for (int it = 0; it < 10; ++it) {
-            v1(1, xp, yp, uu + 2);
-            xp = xo - dt * uu[2];
-            yp = yo - dt * uu[3];
-            v1(1, xp, yp, uu + 2);
-            vel(1, xo, yo, uu);
-            xo = x0 - dt * (1.5 * uu[0] - 0.5 * uu[2]);
-            yo = y0 - dt * (1.5 * uu[1] - 0.5 * uu[3]);
+            vel.lev(1, xp, yp, u1x, u1y);
+            xp = xo - dt * u1x;
+            yp = yo - dt * u1y;
+            vel.lev(1, xp, yp, u1x, u1y);
+            vel.lev(0, xo, yo, u0x, u0y);
+            xo = x0 - dt * (1.5 * u0x - 0.5 * u1x);
+            yo = y0 - dt * (1.5 * u0y - 0.5 * u1y);
         }
     } else if constexpr (TIN == 21) {
-        vel(L, x0, y0, u);
-        double k1x, k1y;
-        lsum(c.L1, u, L, k1x, k1y);
-        vel(1, x0 - dt * k1x, y0 - dt * k1y, u);
-        xo = x0 - dt * (k1x + u[0]) / 2;
-        yo = y0 - dt * (k1y + u[1]) / 2;
+        double k1x, k1y, ux, uy;
+        vel.w1(c, x0, y0, k1x, k1y);
+        vel.lev(0, x0 - dt * k1x, y0 - dt * k1y, ux, uy);
+        xo = x0 - dt * (k1x + ux) / 2;
+        yo = y0 - dt * (k1y + uy) / 2;
     } else if constexpr (TIN == 30) {
         double k[6];
-        vel(L, x0, y0, u);
-        lsum(c.L1, u, L, k[0], k[1]);
-        vel(L, x0 - dt * k[0] / 2.0, y0 - dt * k[1] / 2.0, u);
-        lsum(c.L12, u, L, k[2], k[3]);
-        vel(1, x0 + dt * k[0] - 2 * dt * k[2], y0 + dt * k[1] - 2 * dt * k[3], k + 4);
+        vel.w1(c, x0, y0, k[0], k[1]);
+        vel.w12(c, x0 - dt * k[0] / 2.0, y0 - dt * k[1] / 2.0, k[2], k[3]);
+        vel.lev(0, x0 + dt * k[0] - 2 * dt * k[2], y0 + dt * k[1] - 2 * dt * k[3], k[4], k[5]);
         xo = x0 - dt * (k[0] + 4 * k[2] + k[4]) / 6.0;
         yo = y0 - dt * (k[1] + 4 * k[3] + k[5]) / 6.0;
     } else if constexpr (TIN == 31) {
         double k[6];
-        vel(L, x0, y0, u);
-        lsum(c.L1, u, L, k[0], k[1]);
-        vel(1, x0 - dt * k[0], y0 - dt * k[1], k + 2);
-        vel.level1()(1, x0 + 2 * dt * (-2 * k[0] + k[2]), y0 + 2 * dt * (-2 * k[1] + k[3]), k + 4);
+        vel.w1(c, x0, y0, k[0], k[1]);
+        vel.lev(0, x0 - dt * k[0], y0 - dt * k[1], k[2], k[3]);
+        vel.lev(1, x0 + 2 * dt * (-2 * k[0] + k[2]), y0 + 2 * dt * (-2 * k[1] + k[3]), k[4], k[5]);
         xo = x0 - dt * (5 * k[0] + 8 * k[2] - k[4]) / 12.0;
         yo = y0 - dt * (5 * k[1] + 8 * k[3] - k[5]) / 12.0;
     } else if constexpr (TIN == 40) {
         double k[8];
-        vel(L, x0, y0, u);
-        lsum(c.L1, u, L, k[0], k[1]);
-        vel(L, x0 - dt * k[0] / 2.0, y0 - dt * k[1] / 2.0, u);
-        lsum(c.L12, u, L, k[2], k[3]);
-        vel(L, x0 - dt * k[2] / 2.0, y0 - dt * k[3] / 2.0, u);
-        lsum(c.L12, u, L, k[4], k[5]);
-        vel(1, x0 - dt * k[4], y0 - dt * k[5], k + 6);
+        vel.w1(c, x0, y0, k[0], k[1]);
+        vel.w12(c, x0 - dt * k[0] / 2.0, y0 - dt * k[1] / 2.0, k[2], k[3]);
+        vel.w12(c, x0 - dt * k[2] / 2.0, y0 - dt * k[3] / 2.0, k[4], k[5]);
+        vel.lev(0, x0 - dt * k[4], y0 - dt * k[5], k[6], k[7]);
         xo = x0 - dt * (k[0] + 2 * k[2] + 2 * k[4] + k[6]) / 6.0;
         yo = y0 - dt * (k[1] + 2 * k[3] + 2 * k[5] + k[7]) / 6.0;
     } else {  // 41
         double k[8];
-        vel(L, x0, y0, u);
-        lsum(c.L1, u, L, k[0], k[1]);
-        vel(1, x0 - dt * k[0], y0 - dt * k[1], k + 2);
-        vel(L, x0 - dt * (3 * k[0] + k[2]) / 8.0, y0 - dt * (3 * k[1] + k[3]) / 8.0, u);
-        lsum(c.L12, u, L, k[4], k[5]);
-        vel(1, x0 + dt * (k[2] - 4 * k[4]) / 3.0, y0 + dt * (k[3] - 4 * k[5]) / 3.0, k + 6);
+        vel.w1(c, x0, y0, k[0], k[1]);
+        vel.lev(0, x0 - dt * k[0], y0 - dt * k[1], k[2], k[3]);
+        vel.w12(c, x0 - dt * (3 * k[0] + k[2]) / 8.0, y0 - dt * (3 * k[1] + k[3]) / 8.0, k[4], k[5]);
+        vel.lev(0, x0 + dt * (k[2] - 4 * k[4]) / 3.0, y0 + dt * (k[3] - 4 * k[5]) / 3.0, k[6], k[7]);
         xo = x0 - dt * (2 * k[0] - k[2] + 8 * k[4] + 3 * k[6]) / 12.0;
         yo = y0 - dt * (2 * k[1] - k[3] + 8 * k[5] + 3 * k[7]) / 12.0;
+    }
+}
+
+// C1 = sum_l L1[l] psi_l and (when C12 is set) C12 = sum_l L12[l] psi_l over the 4 Hermite planes,
+// summed in level order like the weighted velocities.
+__global__ void __launch_bounds__(256) k_psi_combine(long n4, PsiLevels P, int L, cmm_advect_coeffs c,
+                                                     double *__restrict__ C1, double *__restrict__ C12) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n4) return;
+    double v[4];
+    for (int l = 0; l < L; ++l) v[l] = P.p[l][i];
+    double a = c.L1[0] * v[0];
+    for (int l = 1; l < L; ++l) a += c.L1[l] * v[l];
+    C1[i] = a;
+    if (C12) {
+        double b = c.L12[0] * v[0];
+        for (int l = 1; l < L; ++l) b += c.L12[l] * v[l];
+        C12[i] = b;
     }
 }
 
@@ -161,10 +231,11 @@ __device__ __forceinline__ void step_backward(const V &vel, const cmm_advect_coe
 // cells around 64/G neighbouring nodes.  After the footpoints are traced back, lane j of the group
 // forms output plane j by reading the group's positions with cross-lane shuffles and summing them
 // in footpoint order with the stencil weights, the order of the reference's accumulation.
-template <int TIN, int G>
+template <int TIN, int G, bool COMB>
 __global__ void __launch_bounds__(256) k_map_advect(DGrid gm, DGrid gp, const double *__restrict__ cx,
-                                                    const double *__restrict__ cy, PsiLevels P, cmm_advect_coeffs c,
-                                                    double *__restrict__ ncx, double *__restrict__ ncy, int nblocks) {
+                                                    const double *__restrict__ cy, PsiLevels P, const double *C1,
+                                                    const double *C12, cmm_advect_coeffs c, double *__restrict__ ncx,
+                                                    double *__restrict__ ncy, int nblocks) {
     constexpr int NPB = 256 / G;  // nodes per block
     const int blk = xcd_block(blockIdx.x, nblocks);
     const int node = blk * NPB + threadIdx.x / G;
@@ -177,7 +248,10 @@ __global__ void __launch_bounds__(256) k_map_advect(DGrid gm, DGrid gp, const do
         const double xe = gm.x0 + ix * gm.hx + ((k >> 2) + 1) * c.eps * (1 - 2 * ((k >> 1) & 1));
         const double ye = gm.y0 + iy * gm.hy + ((k >> 2) + 1) * c.eps * (1 - 2 * (((k + 1) >> 1) & 1));
         double xf, yf;
-        step_backward<TIN>(VelGlobal{gp, P}, c, xe, ye, xf, yf);
+        if constexpr (COMB)
+            step_backward<TIN>(VelCombined{gp, P, C1, C12}, c, xe, ye, xf, yf);
+        else
+            step_backward<TIN>(VelLevels{gp, P}, c, xe, ye, xf, yf);
         diffeo(gm, cx, cy, xf, yf, px, py);
     }
     const int base = (threadIdx.x & 63) - k;  // first lane of the group within the wavefront
@@ -215,6 +289,68 @@ __global__ void __launch_bounds__(256) k_incomp(DGrid gm, DGrid gc, const double
             const double x = gm.x0 + ix * gc.hx + 0.5 * gc.hx;
             const double y = gm.y0 + iy * gc.hy + 0.5 * gc.hy;
             v = fmax(v, fabs(1 - diffeo_det(gm, cx, cy, x, y)));
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    __shared__ double red[4];
+    if (threadIdx.x == 0) red[threadIdx.y] = v;
+    __syncthreads();
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        double m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+        atomicMax(out, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+// Same check when the check grid refines the map grid by an integer ratio r over the same domain:
+// one lane per map cell loads the cell's 32 corner values once and evaluates the r*r check points
+// inside it (a check point whose cell differs, never expected, reads its own corners), so each
+// point computes exactly what k_incomp computes with 1/r^2 of the gathers.
+__global__ void __launch_bounds__(256) k_incomp_cells(DGrid gm, DGrid gc, const double *__restrict__ cx,
+                                                      const double *__restrict__ cy, unsigned long long *__restrict__ out,
+                                                      int r, int tiles_x, int tiles_total) {
+    double v = 0.0;
+    for (int tt = blockIdx.x; tt < tiles_total; tt += gridDim.x) {
+        const int t = xcd_block(tt, tiles_total);
+        const int mx = (t % tiles_x) * 64 + threadIdx.x;
+        const int my = (t / tiles_x) * 4 + threadIdx.y;
+        if (mx >= gm.nx || my >= gm.ny) continue;
+        Cell c0;
+        find_cell(gm, gm.x0 + (mx + 0.5) * gm.hx, gm.y0 + (my + 0.5) * gm.hy, c0);
+        double vx[16], vy[16];
+        load16(cx, gm.n, c0, vx);
+        load16(cy, gm.n, c0, vy);
+        for (int b = 0; b < r; ++b) {
+            const int iy = my * r + b;
+            const double y = gm.y0 + iy * gc.hy + 0.5 * gc.hy;
+            for (int a = 0; a < r; ++a) {
+                const int ix = mx * r + a;
+                const double x = gm.x0 + ix * gc.hx + 0.5 * gc.hx;
+                Cell c;
+                find_cell(gm, x, y, c);
+                double bx[4], by[4], dbx[4], dby[4];
+                basis(c.dx, bx);
+                basis(c.dy, by);
+                basis_d(c.dx, dbx);
+                basis_d(c.dy, dby);
+                const double wjx[4] = {c.wx0 * gm.jx, c.wx1 * gm.jx, c.wx0 * gm.jx, c.wx1 * gm.jx};
+                const double wjy[4] = {c.wy0 * gm.jy, c.wy0 * gm.jy, c.wy1 * gm.jy, c.wy1 * gm.jy};
+                double Xx, Yx, Xy, Yy;
+                if (c.i00 == c0.i00 && c.wx0 == c0.wx0 && c.wy0 == c0.wy0) {
+                    Xx = hsum_v(vx, dbx, by, gm.hx, gm.hy, wjx) * gm.ihx;
+                    Yx = hsum_v(vy, dbx, by, gm.hx, gm.hy, wjy) * gm.ihy;
+                    Xy = hsum_v(vx, bx, dby, gm.hx, gm.hy, wjx) * gm.ihx;
+                    Yy = hsum_v(vy, bx, dby, gm.hx, gm.hy, wjy) * gm.ihy;
+                } else {
+                    double ux[16], uy[16];
+                    load16(cx, gm.n, c, ux);
+                    load16(cy, gm.n, c, uy);
+                    Xx = hsum_v(ux, dbx, by, gm.hx, gm.hy, wjx) * gm.ihx;
+                    Yx = hsum_v(uy, dbx, by, gm.hx, gm.hy, wjy) * gm.ihy;
+                    Xy = hsum_v(ux, bx, dby, gm.hx, gm.hy, wjx) * gm.ihx;
+                    Yy = hsum_v(uy, bx, dby, gm.hx, gm.hy, wjy) * gm.ihy;
+                }
+                v = fmax(v, fabs(1 - (Xx * Yy - Xy * Yx)));
+            }
         }
     }
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
@@ -881,24 +1017,65 @@ int op_map_identity(const cmm_grid &g, double *cx, double *cy, hipStream_t st) {
     return launch_check("k_map_identity");
 }
 
+// Schemes whose multi-level velocity evaluations use only the L1 / L12 weights can run on
+// precombined fields; AB2 (20) and RK3Mod (31) read level 1 on its own.
+static bool advect_combinable(const cmm_advect_coeffs &c) {
+    const int t = c.time_integration_num;
+    return c.l_order >= 2 && (t == 10 || t == 21 || t == 30 || t == 40 || t == 41);
+}
+
+static bool advect_needs_c12(const cmm_advect_coeffs &c) {
+    const int t = c.time_integration_num;
+    return t == 30 || t == 40 || t == 41;
+}
+
+size_t op_advect_workspace(const cmm_grid &gp, const cmm_advect_coeffs &c) {
+    if (!advect_combinable(c)) return 0;
+    return (advect_needs_c12(c) ? 2 : 1) * sizeof(double) * 4 * (size_t)gp.nx * gp.ny;
+}
+
 int op_map_advect(const cmm_grid &gm, const cmm_grid &gp, const double *cx, const double *cy, const PsiLevels &P,
-                  const cmm_advect_coeffs &c, double *ncx, double *ncy, hipStream_t st) {
+                  const cmm_advect_coeffs &c, double *ncx, double *ncy, void *work, size_t work_bytes,
+                  hipStream_t st) {
     DGrid dm = make_dgrid(gm), dp = make_dgrid(gp);
     const int nfoot = 4 + 4 * c.map_update_order_num;
     const int G = nfoot <= 4 ? 4 : nfoot <= 8 ? 8 : 16;
     const int nb = (int)(((long)dm.n * G + 255) / 256);
-#define CMM_ADV(T, GG) hipLaunchKernelGGL((k_map_advect<T, GG>), dim3(nb), dim3(256), 0, st, dm, dp, cx, cy, P, c, ncx, ncy, nb)
-#define CMM_ADV_G(T) do { if (G == 4) CMM_ADV(T, 4); else if (G == 8) CMM_ADV(T, 8); else CMM_ADV(T, 16); } while (0)
+    const size_t need = op_advect_workspace(gp, c);
+    const bool comb = work && need > 0 && work_bytes >= need;
+    double *C1 = nullptr, *C12 = nullptr;
+    if (comb) {
+        const long n4 = 4L * dp.n;
+        C1 = static_cast<double *>(work);
+        C12 = advect_needs_c12(c) ? C1 + n4 : nullptr;
+        hipLaunchKernelGGL(k_psi_combine, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, st, n4, P, c.l_order, c, C1,
+                           C12);
+        if (int rc = launch_check("k_psi_combine")) return rc;
+    }
+#define CMM_ADV(T, GG, CB)                                                                                             \
+    hipLaunchKernelGGL((k_map_advect<T, GG, CB>), dim3(nb), dim3(256), 0, st, dm, dp, cx, cy, P, C1, C12, c, ncx, ncy, nb)
+#define CMM_ADV_G(T, CB)                                                                                               \
+    do {                                                                                                               \
+        if (G == 4) CMM_ADV(T, 4, CB);                                                                                 \
+        else if (G == 8) CMM_ADV(T, 8, CB);                                                                            \
+        else CMM_ADV(T, 16, CB);                                                                                       \
+    } while (0)
+#define CMM_ADV_T(T)                                                                                                   \
+    do {                                                                                                               \
+        if (comb) CMM_ADV_G(T, true);                                                                                  \
+        else CMM_ADV_G(T, false);                                                                                      \
+    } while (0)
     switch (c.time_integration_num) {
-    case 10: CMM_ADV_G(10); break;
-    case 20: CMM_ADV_G(20); break;
-    case 21: CMM_ADV_G(21); break;
-    case 30: CMM_ADV_G(30); break;
-    case 31: CMM_ADV_G(31); break;
-    case 40: CMM_ADV_G(40); break;
-    case 41: CMM_ADV_G(41); break;
+    case 10: CMM_ADV_T(10); break;
+    case 20: CMM_ADV_G(20, false); break;
+    case 21: CMM_ADV_T(21); break;
+    case 30: CMM_ADV_T(30); break;
+    case 31: CMM_ADV_G(31, false); break;
+    case 40: CMM_ADV_T(40); break;
+    case 41: CMM_ADV_T(41); break;
     default: set_error("unknown time integration"); return CMM_ERR_ARG;
     }
+#undef CMM_ADV_T
 #undef CMM_ADV_G
 #undef CMM_ADV
     return launch_check("k_map_advect");
@@ -909,6 +1086,14 @@ int op_incomp_launch(const cmm_grid &gm, const cmm_grid &gc, const double *cx, c
     DGrid dm = make_dgrid(gm), dc = make_dgrid(gc);
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, slot);
     dim3 b(64, 4);
+    const int r = gc.nx / gm.nx;
+    if (r >= 1 && r <= 8 && gc.nx == r * gm.nx && gc.ny == r * gm.ny && gc.x0 == gm.x0 && gc.x1 == gm.x1 &&
+        gc.y0 == gm.y0 && gc.y1 == gm.y1) {
+        const int tx = (gm.nx + 63) / 64, ty = (gm.ny + 3) / 4, tt = tx * ty;
+        const int nb = std::min(tt, 2048);
+        hipLaunchKernelGGL(k_incomp_cells, dim3(nb), b, 0, st, dm, dc, cx, cy, slot, r, tx, tt);
+        return launch_check("k_incomp_cells");
+    }
     const int tx = (gc.nx + 63) / 64, ty = (gc.ny + 3) / 4, tt = tx * ty;
     const int nb = std::min(tt, 2048);
     hipLaunchKernelGGL(k_incomp, dim3(nb), b, 0, st, dm, dc, cx, cy, slot, tx, tt);

@@ -44,10 +44,16 @@ int cmm_map_init_identity(cmm_grid g, double *chix, double *chiy, void *stream);
  * k_advect_using_stream_hermite (Cuda-CMM/src/simulation/cmm-simulation-host.cu:64-104,
  * Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:207-298).  psi_levels is a host array of
  * c->l_order device pointers (newest first, 4*Npsi doubles each).  chix_new/chiy_new must not
- * alias chix/chiy. */
+ * alias chix/chiy.
+ * work may be NULL: every footpoint evaluation then interpolates each psi level and weights the
+ * velocities (the reference's order of operations).  With work_bytes >=
+ * cmm_advect_workspace_bytes(g_psi, c) (> 0 for l_order >= 2 and schemes 10, 21, 30, 40, 41) the
+ * call first forms the Lagrange-weighted psi combinations in work, once, and interpolates those:
+ * one gather per evaluation instead of one per level; equal up to rounding. */
+size_t cmm_advect_workspace_bytes(cmm_grid g_psi, const cmm_advect_coeffs *c);
 int cmm_map_advect(cmm_grid g_map, cmm_grid g_psi, const double *chix, const double *chiy,
                    const double *const *psi_levels, const cmm_advect_coeffs *c, double *chix_new,
-                   double *chiy_new, void *stream);
+                   double *chiy_new, void *work, size_t work_bytes, void *stream);
 
 /* max |1 - det grad X| over the cell centres of g_check, returned to the host (synchronises the
  * stream); replaces incompressibility_check (Cuda-CMM/src/simulation/cmm-simulation-host.cu:44-50). */

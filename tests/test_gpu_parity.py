@@ -146,7 +146,9 @@ def test_map_advect_rk3(dev, base):
 def test_map_advect_schemes(dev, base, ratio):
     """Every time scheme (Euler, AB2, RK2, RK3, RK3-mod, RK4, RK4-mod) and footpoint stencil order
     (2, 4, 6) on the deformed map, with psi on the map grid and on a 2x finer grid, at a small
-    (1/32) and a large (1/2) step."""
+    (1/32) and a large (1/2) step; each without a workspace (levels interpolated one by one) and
+    with one (Lagrange-weighted psi combinations formed first)."""
+    import torch
     import cmm
     gc, gf, _ = grids()
     ogc, ogf, _ = ogrids()
@@ -169,12 +171,16 @@ def test_map_advect_schemes(dev, base, ratio):
                 oc = O.advect_coeffs(t, dts, 0, 3, tin, muo, 1e-3)
                 c = cmm.advect_coeffs(t, dts, 0, 3, tin, muo, 1e-3)
                 rx, ry = O.map_advect(ogc, ogp, base["chix"], base["chiy"], psi, oc)
-                ox = dev_array(np.zeros(4 * n))
-                oy = dev_array(np.zeros(4 * n))
-                cmm.advect_using_stream_hermite(gc, gp, mx, my, lv, c, ox, oy)
-                e = max(max(hermite_errors(host(ox), rx, n, 1.0 / 1e-3)), max(hermite_errors(host(oy), ry, n, 1.0 / 1e-3)))
-                print(f"advect r={ratio} dt={dt} tin={tin} order={muo}: {e:.2e}")
-                assert e <= TOL, (ratio, dt, tin, muo, e)
+                wb = cmm.advect_workspace_bytes(gp, c)
+                assert wb == (0 if tin in (20, 31) else (2 if tin in (30, 40, 41) else 1) * 32 * npsi * npsi)
+                for work in (None, torch.zeros(max(wb, 8) // 8, dtype=torch.float64, device="cuda")):
+                    ox = dev_array(np.zeros(4 * n))
+                    oy = dev_array(np.zeros(4 * n))
+                    cmm.advect_using_stream_hermite(gc, gp, mx, my, lv, c, ox, oy, work=work)
+                    e = max(max(hermite_errors(host(ox), rx, n, 1.0 / 1e-3)),
+                            max(hermite_errors(host(oy), ry, n, 1.0 / 1e-3)))
+                    print(f"advect r={ratio} dt={dt} tin={tin} order={muo} work={work is not None}: {e:.2e}")
+                    assert e <= TOL, (ratio, dt, tin, muo, work is not None, e)
 
 
 def test_incompressibility(dev, base):
@@ -185,6 +191,13 @@ def test_incompressibility(dev, base):
     got = cmm.incompressibility_check(gf, gc, dev_array(base["chix"]), dev_array(base["chiy"]))
     print("incomp", got, ref)
     assert abs(got - ref) <= 1e-12 * max(1.0, abs(ref))
+    # check grids that refine the map grid by 1 and 4 (one lane per map cell) and by 1.5 (one lane
+    # per check point)
+    for nchk in (256, 1024, 384):
+        ref = O.incomp_error(ogc, O.grid(nchk), base["chix"], base["chiy"])
+        got = cmm.incompressibility_check(cmm.grid(nchk), gc, dev_array(base["chix"]), dev_array(base["chiy"]))
+        print("incomp", nchk, got, ref)
+        assert abs(got - ref) <= 1e-12 * max(1.0, abs(ref))
     cx, cy = O.identity_map(ogc)
     # identity map: det = 1 up to the rounding of the basis partition of unity (fused multiply-adds)
     assert cmm.incompressibility_check(gf, gc, dev_array(cx), dev_array(cy)) <= 1e-12
