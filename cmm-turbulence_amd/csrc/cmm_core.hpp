@@ -203,6 +203,20 @@ CMM_HD void diffeo(const DGrid &g, const double *__restrict__ Hx, const double *
     yo = hsum_v(v, bx, by, g.hx, g.hy, wjy);
 }
 
+// diffeo() at a point whose cell c has been found, with the corner values of both fields given
+CMM_HD void diffeo_vals(const DGrid &g, const Cell &c, const double vx[16], const double vy[16], double &xo,
+                        double &yo) {
+    double bx[4], by[4];
+    basis(c.dx, bx);
+    basis(c.dy, by);
+    const double wjx[4] = {c.wx0 * g.jx, c.wx1 * g.jx, c.wx0 * g.jx, c.wx1 * g.jx};
+    const double wjy[4] = {c.wy0 * g.jy, c.wy0 * g.jy, c.wy1 * g.jy, c.wy1 * g.jy};
+    xo = hsum_v(vx, bx, by, g.hx, g.hy, wjx);
+    yo = hsum_v(vy, bx, by, g.hx, g.hy, wjy);
+}
+
+CMM_HD bool same_cell(const Cell &a, const Cell &b) { return a.i00 == b.i00 && a.wx0 == b.wx0 && a.wy0 == b.wy0; }
+
 // det(grad X), Cuda-CMM/src/numerical/cmm-hermite.cu:479-503
 CMM_HD double diffeo_det(const DGrid &g, const double *__restrict__ Hx, const double *__restrict__ Hy, double x,
                          double y) {

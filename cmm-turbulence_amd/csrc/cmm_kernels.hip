@@ -9,6 +9,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "cmm_core.hpp"
@@ -477,6 +478,11 @@ struct SampleVort {
         }
         return v;
     }
+    // points x..x+3 of rows ya and yb (no mollifier) into the packed row pair s[sw(x+i)] =
+    // (row ya, row yb): map cell corners are loaded once per run of points in the same cell (one
+    // load for all 8 when the vorticity grid refines the map grid 4x or more); the values equal
+    // operator().
+    __device__ void quad(int x, int ya, int yb, cd *s) const;
 };
 
 // Initial condition at X_1(...X_M(X_cur(x))): the current map then the stack from newest to
@@ -514,15 +520,53 @@ struct SampleCompose {
     }
 };
 
+__device__ void SampleVort::quad(int x, int ya, int yb, cd *s) const {
+    if (molly != 0) {
+        for (int i = 0; i < 4; ++i) s[sw(x + i)] = cd{(*this)(x + i, ya), (*this)(x + i, yb)};
+        return;
+    }
+    // corner values of the map cell last loaded (a one-entry cache keyed by the cell)
+    Cell c0;
+    c0.i00 = -1;
+    double vx[16], vy[16];
+    for (int p = 0; p < 8; ++p) {
+        const int i = p & 3, j = p >> 2;
+        const double xp = gm.x0 + (x + i) * gv.hx, yp = gm.y0 + (j ? yb : ya) * gv.hy;
+        Cell c;
+        find_cell(gm, xp, yp, c);
+        if (!same_cell(c, c0)) {
+            load16(cx, gm.n, c, vx);
+            load16(cy, gm.n, c, vy);
+            c0 = c;
+        }
+        double x2, y2;
+        diffeo_vals(gm, c, vx, vy, x2, y2);
+        const double v = herm_eval(gf, W, x2, y2);
+        double *d = &s[sw(x + i)].x;
+        d[j] = v;
+    }
+}
+
+template <class S>
+__device__ __forceinline__ void sample_quad(const S &smp, int x, int ya, int yb, cd *s) {
+    if constexpr (std::is_same_v<S, SampleVort>) {
+        smp.quad(x, ya, yb, s);
+    } else {
+        for (int i = 0; i < 4; ++i) s[sw(x + i)] = cd{smp(x + i, ya), smp(x + i, yb)};
+    }
+}
+
 // ------------------------------------------------------------------------------------------------
-// Workgroup FFT in LDS (length n <= 16*T, power of two).
-constexpr int kFftThreads = 256;
+// Workgroup FFT in LDS (length n <= kFftMaxN, power of two), kFftThreads lanes per transform.
+constexpr int kFftMaxN = 4096;
+constexpr int kFftThreads = 512;
+constexpr int kFftPer = kFftMaxN / kFftThreads;  // elements per lane of a full-length transform
 
 template <int R>
 __device__ __forceinline__ void run_stage(cd *s, int n, int logn, int logNs, const cd *__restrict__ tw, bool inv) {
-    constexpr int BPT = 16 / R;
+    constexpr int BPT = kFftPer / R > 0 ? kFftPer / R : 1;
     const int nb = n / R;
-    cd v[16];
+    cd v[BPT * R];
 #pragma unroll
     for (int b = 0; b < BPT; ++b) {
         int j = threadIdx.x + b * kFftThreads;
@@ -537,7 +581,7 @@ __device__ __forceinline__ void run_stage(cd *s, int n, int logn, int logNs, con
     __syncthreads();
 }
 
-__device__ void wg_fft(cd *s, int n, int logn, const cd *__restrict__ tw, bool inv) {
+__device__ __forceinline__ void wg_fft(cd *s, int n, int logn, const cd *__restrict__ tw, bool inv) {
     int logNs = 0, rem = logn;
     while (rem > 0) {
         int rl = stage_radix_log(rem);
@@ -551,17 +595,21 @@ __device__ void wg_fft(cd *s, int n, int logn, const cd *__restrict__ tw, bool i
 
 // Row R2C of two rows at once (z = row_a + i row_b), samples produced by the sampler.  Writes
 // the half spectrum columns kx <= kx_max transposed: T[kx*ny + y].  Unnormalised.
+// waves per SIMD the row kernel is compiled for: 4 (two 512-lane blocks per CU) unless the
+// sampler's gathers need more registers than that leaves
 template <class S>
-__global__ void __launch_bounds__(kFftThreads) k_row_r2c(S sampler, int nx, int ny, int logn, int kx_max,
+constexpr int r2c_min_waves() {
+    return std::is_same_v<S, SampleVort> || std::is_same_v<S, SampleCompose> ? 2 : 4;
+}
+
+template <class S>
+__global__ void __launch_bounds__(kFftThreads, r2c_min_waves<S>()) k_row_r2c(S sampler, int nx, int ny, int logn, int kx_max,
                                                           const cd *__restrict__ tw, cd *__restrict__ T) {
     extern __shared__ cd s[];
     const int pair = xcd_block(blockIdx.x, gridDim.x);
     const int ya = 2 * pair, yb = ya + 1;
-    for (int x = threadIdx.x; x < nx; x += kFftThreads) {
-        double a = sampler(x, ya);
-        double b = sampler(x, yb);
-        s[sw(x)] = cd{a, b};
-    }
+    // each lane samples 4 consecutive points of both rows (nx is a power of two >= 8)
+    for (int x = 4 * threadIdx.x; x < nx; x += 4 * kFftThreads) sample_quad(sampler, x, ya, yb, s);
     __syncthreads();
     wg_fft(s, nx, logn, tw, false);
     for (int k = threadIdx.x; k <= kx_max; k += kFftThreads) {
@@ -591,7 +639,7 @@ struct ColParams {
     int kx_max;
 };
 
-__global__ void __launch_bounds__(kFftThreads) k_col(ColParams p, const cd *__restrict__ T,
+__global__ void __launch_bounds__(kFftThreads, 4) k_col(ColParams p, const cd *__restrict__ T,
                                                      const cd *__restrict__ tw_in, const cd *__restrict__ tw_out,
                                                      cd *__restrict__ G0, cd *__restrict__ G1) {
     extern __shared__ cd s[];
@@ -601,7 +649,7 @@ __global__ void __launch_bounds__(kFftThreads) k_col(ColParams p, const cd *__re
     __syncthreads();
     if (p.mode != 2) wg_fft(s, p.nin, p.login, tw_in, false);
     // build P for the output column in registers (each thread owns rows threadIdx.x + m*T)
-    constexpr int MAXR = 16;
+    constexpr int MAXR = kFftPer;
     cd P[MAXR];
     const double kxv = p.fkx * kx;
 #pragma unroll
@@ -674,7 +722,7 @@ __global__ void __launch_bounds__(kFftThreads) k_col(ColParams p, const cd *__re
 // (Cuda-CMM/src/numerical/cmm-fft.cu:398-429).  Two real outputs come from one complex inverse
 // transform of the Hermitian extensions A~ + i B~; the kx = 0 and kx = n/2 bins keep only their
 // real parts (C2R semantics).  Columns above kx_max are zero.
-__global__ void __launch_bounds__(kFftThreads) k_row_c2r4(int nx, int ny, int logn, int kx_max, double fkx,
+__global__ void __launch_bounds__(kFftThreads, 4) k_row_c2r4(int nx, int ny, int logn, int kx_max, double fkx,
                                                           const cd *__restrict__ G0, const cd *__restrict__ G1,
                                                           const cd *__restrict__ tw, double *__restrict__ out) {
     extern __shared__ cd s[];
@@ -882,7 +930,7 @@ static int ilog2(int n) {
 }
 static bool grid_ok(const cmm_grid &g) { return g.nx > 0 && g.ny > 0 && g.x1 > g.x0 && g.y1 > g.y0; }
 static int fft_grid_ok(const cmm_grid &g, const char *what) {
-    if (!grid_ok(g) || !is_pow2(g.nx) || !is_pow2(g.ny) || g.nx > 16 * kFftThreads || g.ny > 16 * kFftThreads || g.nx < 8 ||
+    if (!grid_ok(g) || !is_pow2(g.nx) || !is_pow2(g.ny) || g.nx > kFftMaxN || g.ny > kFftMaxN || g.nx < 8 ||
         g.ny < 8) {
         set_error(std::string(what) + ": FFT grids must be powers of two between 8 and 4096");
         return CMM_ERR_ARG;
