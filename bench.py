@@ -40,8 +40,12 @@ def algorithmic(slot, c, kx_max, n_stack=0):
     nc, npsi, nf, nv = c["n_coarse"] ** 2, c["n_psi"] ** 2, c["n_fine"] ** 2, c["n_vort"] ** 2
     L = 3
     if slot == "advect":
-        # map in (8 doubles/node) + psi L levels x 4 planes + map out
-        return 64 * nc + 32 * L * npsi + 64 * nc, 9.2e3 * nc
+        # map in (8 doubles/node) + the two Lagrange-weighted psi fields and psi level 0 (4 planes
+        # each) + map out
+        return 64 * nc + 32 * 3 * npsi + 64 * nc, 9.2e3 * nc
+    if slot == "psi_combine":
+        # L psi levels read, two combined fields written (4 planes each)
+        return 32 * (L + 2) * npsi, 2 * 2 * L * 4 * npsi
     if slot == "incomp":
         return 64 * nc, 320.0 * nf
     if slot == "psi_row_r2c":
@@ -105,8 +109,9 @@ def cpu_baseline(c, threads, budget_s=12.0):
 
 
 # bench slot -> kernel-name pattern of the launches it times (engine psi solve samples omega with SampleVort)
-SLOT_KERNELS = {"advect": "k_map_advect<30, 8>", "incomp": "k_incomp(", "psi_row_r2c": "k_row_r2c<cmm::SampleVort>",
-                "psi_col": "k_col(", "psi_row_c2r": "k_row_c2r4(", "particles": "k_particles("}
+SLOT_KERNELS = {"advect": "k_map_advect<30, 8, true>", "psi_combine": "k_psi_combine<3>", "incomp": "k_incomp_cells(",
+                "psi_row_r2c": "k_row_r2c<cmm::SampleVort>", "psi_col": "k_col(", "psi_row_c2r": "k_row_c2r4(",
+                "particles": "k_particles("}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "cmm_euler_c2_hbm_traffic.json")
 FETCH_SCALE_DEFAULT, WRITE_SCALE_DEFAULT = 2.0, 1.0
 

@@ -61,7 +61,7 @@ class StepInfo(C.Structure):
                 ("saturated", C.c_int)]
 
 
-PROF_NAMES = ["advect", "incomp", "psi_row_r2c", "psi_col", "psi_row_c2r", "particles", "remap"]
+PROF_NAMES = ["advect", "incomp", "psi_row_r2c", "psi_col", "psi_row_c2r", "particles", "remap", "psi_combine"]
 PROF_SLOTS = len(PROF_NAMES)
 FIELD = {"chix": 0, "chiy": 1, "psi": 2, "w_fine": 3, "w_coarse": 4, "disc": 5}
 

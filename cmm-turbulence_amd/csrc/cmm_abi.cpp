@@ -229,7 +229,7 @@ struct cmm_euler {
     std::vector<double> tv, dtv;
     int loop = 0;
     bool initialised = false;
-    // per-kernel event timing: [advect, incomp, psi_row, psi_col, psi_c2r, particles, remap]
+    // per-kernel event timing: [advect, incomp, psi_row, psi_col, psi_c2r, particles, remap, psi_combine]
     bool prof = false;
     hipEvent_t ev[12] = {};
     double ms[CMM_PROF_SLOTS] = {};
@@ -266,8 +266,9 @@ int engine_advect(cmm_euler *e, const double *t, const double *dt, int loop_ctr,
     PsiLevels P;
     for (int l = 0; l < 4; ++l) P.p[l] = e->lvl[l < L ? l : 0];
     // the workspace is shared with the psi solve, which runs later on the same stream
+    // event 10 separates the psi combination from the advection kernel in the profile
     if (int rc = op_map_advect(e->p.g_coarse, e->p.g_psi, e->cx, e->cy, P, c, e->ncx, e->ncy, e->work, e->work_bytes,
-                               e->st))
+                               e->st, e->prof ? e->ev[10] : nullptr))
         return rc;
     std::swap(e->cx, e->ncx);
     std::swap(e->cy, e->ncy);
@@ -525,7 +526,8 @@ int cmm_euler_step(cmm_euler *e, double dt_now, cmm_step_info *info) {
     double einc;
     if (int rc = engine_incomp(e, &einc)) return rc;
     prof_mark(e, 2);
-    prof_acc(e, 0, 0, 1);
+    prof_acc(e, 7, 0, 10);
+    prof_acc(e, 0, 10, 1);
     prof_acc(e, 1, 1, 2);
     int remapped = 0, saturated = 0;
     // remap, :814-853 (translate before push; the map is reset afterwards)

@@ -209,21 +209,35 @@ __device__ __forceinline__ void step_backward(const V &vel, const cmm_advect_coe
 }
 
 // C1 = sum_l L1[l] psi_l and (when C12 is set) C12 = sum_l L12[l] psi_l over the 4 Hermite planes,
-// summed in level order like the weighted velocities.
-__global__ void __launch_bounds__(256) k_psi_combine(long n4, PsiLevels P, int L, cmm_advect_coeffs c,
+// summed in level order like the weighted velocities.  Streaming: each lane moves 4 consecutive
+// doubles per level as two 16-byte loads (n4 is a multiple of 4: 4 planes).
+template <int L>
+__global__ void __launch_bounds__(256) k_psi_combine(long n4, PsiLevels P, cmm_advect_coeffs c,
                                                      double *__restrict__ C1, double *__restrict__ C12) {
-    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    const long i = 4 * ((long)blockIdx.x * 256 + threadIdx.x);
     if (i >= n4) return;
-    double v[4];
-    for (int l = 0; l < L; ++l) v[l] = P.p[l][i];
-    double a = c.L1[0] * v[0];
-    for (int l = 1; l < L; ++l) a += c.L1[l] * v[l];
-    C1[i] = a;
-    if (C12) {
-        double b = c.L12[0] * v[0];
-        for (int l = 1; l < L; ++l) b += c.L12[l] * v[l];
-        C12[i] = b;
+    double2 v[L][2];
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+        const double2 *src = reinterpret_cast<const double2 *>(P.p[l] + i);
+        v[l][0] = src[0];
+        v[l][1] = src[1];
     }
+    auto comb = [&](const double *w, double *dst) {
+        double o[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            double a = w[0] * (q & 1 ? v[0][q >> 1].y : v[0][q >> 1].x);
+#pragma unroll
+            for (int l = 1; l < L; ++l) a += w[l] * (q & 1 ? v[l][q >> 1].y : v[l][q >> 1].x);
+            o[q] = a;
+        }
+        double2 *d = reinterpret_cast<double2 *>(dst + i);
+        d[0] = double2{o[0], o[1]};
+        d[1] = double2{o[2], o[3]};
+    };
+    comb(c.L1, C1);
+    if (C12) comb(c.L12, C12);
 }
 
 // Map advection with the footpoint stencil, Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:207-298.
@@ -1084,7 +1098,7 @@ size_t op_advect_workspace(const cmm_grid &gp, const cmm_advect_coeffs &c) {
 
 int op_map_advect(const cmm_grid &gm, const cmm_grid &gp, const double *cx, const double *cy, const PsiLevels &P,
                   const cmm_advect_coeffs &c, double *ncx, double *ncy, void *work, size_t work_bytes,
-                  hipStream_t st) {
+                  hipStream_t st, hipEvent_t mid) {
     DGrid dm = make_dgrid(gm), dp = make_dgrid(gp);
     const int nfoot = 4 + 4 * c.map_update_order_num;
     const int G = nfoot <= 4 ? 4 : nfoot <= 8 ? 8 : 16;
@@ -1096,10 +1110,15 @@ int op_map_advect(const cmm_grid &gm, const cmm_grid &gp, const double *cx, cons
         const long n4 = 4L * dp.n;
         C1 = static_cast<double *>(work);
         C12 = advect_needs_c12(c) ? C1 + n4 : nullptr;
-        hipLaunchKernelGGL(k_psi_combine, dim3((unsigned)((n4 + 255) / 256)), dim3(256), 0, st, n4, P, c.l_order, c, C1,
-                           C12);
+        const dim3 g((unsigned)((n4 / 4 + 255) / 256));
+        switch (c.l_order) {
+        case 2: hipLaunchKernelGGL(k_psi_combine<2>, g, dim3(256), 0, st, n4, P, c, C1, C12); break;
+        case 3: hipLaunchKernelGGL(k_psi_combine<3>, g, dim3(256), 0, st, n4, P, c, C1, C12); break;
+        default: hipLaunchKernelGGL(k_psi_combine<4>, g, dim3(256), 0, st, n4, P, c, C1, C12); break;
+        }
         if (int rc = launch_check("k_psi_combine")) return rc;
     }
+    if (mid) (void)hipEventRecord(mid, st);
 #define CMM_ADV(T, GG, CB)                                                                                             \
     hipLaunchKernelGGL((k_map_advect<T, GG, CB>), dim3(nb), dim3(256), 0, st, dm, dp, cx, cy, P, C1, C12, c, ncx, ncy, nb)
 #define CMM_ADV_G(T, CB)                                                                                               \
