@@ -8,7 +8,7 @@
  *
  * Pinning (what checks each function, all in tests/):
  *   hermite eval/grad/diffeo/det ... bicubic-polynomial exactness, identity map, brute-force
- *                                    tensor-product formula on tiny grids (test_oracle_hermite.py)
+ *                                    tensor-product formula on tiny grids (test_oracle_math.py)
  *   lagrange / stencil coefficients  closed forms (3,-3,1), (1.875,-1.25,0.375), weights sum
  *   map advection .................. psi=0 identity, linear psi translation, fixture
  *                                    euler2D_one_step map statistics after warm-up + 1 step
