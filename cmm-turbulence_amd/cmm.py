@@ -17,7 +17,10 @@ TWO_PI = 6.283185307179586476925286766559005768394338798750211641949889184615632
 # initial-condition ids (include/cmm_types.h); names as in Cuda-CMM/src/ui/settings.h:313-324
 IC = {"4_nodes": 0, "quadropole": 1, "one_vortex": 2, "two_votices": 3, "three_vortices": 4,
       "single_shear_layer": 5, "tanh_shear_layer": 6, "shielded_vortex": 9, "two_cosine": 10,
-      "vortex_sheets": 11, "double_shear_layer": 100, "random_spectrum": 101}
+      "vortex_sheets": 11, "double_shear_layer": 100, "random_spectrum": 101,
+      # distribution functions of the Vlasov-Poisson flow, Cuda-CMM/src/ui/settings.h:300-303
+      "landau_damping": 200, "two_stream": 201, "nonlin_landau_damping": 202}
+SIM = {"cmm_euler_2d": 0, "cmm_vlasov_poisson_1d": 1}
 # time integration names, Cuda-CMM/src/ui/settings.h:386-398
 TIME_INTEGRATION = {"EulerExp": 10, "RK1": 10, "AB2": 20, "RK2": 21, "Heun": 21, "RK3": 30, "RK3Mod": 31,
                     "RK4": 40, "RK4Mod": 41}
@@ -53,7 +56,7 @@ class EulerParams(C.Structure):
                 ("molly_stencil", C.c_int), ("skip_remapping", C.c_int),
                 ("ic", C.c_int), ("ic_params", C.c_double * 10),
                 ("discrete", C.c_int), ("g_disc", Grid), ("disc_real", C.POINTER(C.c_double)),
-                ("seed", C.c_uint64), ("stack_capacity", C.c_int)]
+                ("seed", C.c_uint64), ("stack_capacity", C.c_int), ("sim", C.c_int)]
 
 
 class StepInfo(C.Structure):
@@ -70,9 +73,9 @@ SYMBOLS = ["cmm_version", "cmm_last_error", "cmm_device_info", "cmm_lagrange_coe
            "cmm_map_init_identity", "cmm_advect_workspace_bytes", "cmm_map_advect", "cmm_incomp_error", "cmm_compose_positions",
            "cmm_sample_init", "cmm_translate_workspace_bytes", "cmm_translate_ic", "cmm_hermite_workspace_bytes",
            "cmm_hermite_from_real", "cmm_hermite_from_spectrum", "cmm_psi_workspace_bytes", "cmm_psi_from_vort",
-           "cmm_sample_vort", "cmm_particles_advect", "cmm_euler_create", "cmm_euler_destroy", "cmm_euler_stream",
+           "cmm_sample_vort", "cmm_vp_psi_workspace_bytes", "cmm_vp_psi_from_f", "cmm_particles_advect", "cmm_euler_create", "cmm_euler_destroy", "cmm_euler_stream",
            "cmm_euler_add_particles", "cmm_euler_init", "cmm_euler_step", "cmm_euler_get_field",
-           "cmm_euler_get_particles", "cmm_euler_diagnostics", "cmm_euler_state", "cmm_euler_profile", "cmm_euler_profile_read"]
+           "cmm_euler_get_particles", "cmm_euler_diagnostics", "cmm_euler_vp_diagnostics", "cmm_euler_state", "cmm_euler_profile", "cmm_euler_profile_read"]
 
 _lib = None
 DP = C.POINTER(C.c_double)
@@ -112,6 +115,9 @@ def lib():
         L.cmm_psi_from_vort.argtypes = [Grid, Grid, Grid, Grid, VP, VP, VP, C.c_int, C.c_double, VP, VP,
                                         C.c_size_t, VP]
         L.cmm_sample_vort.argtypes = [Grid, Grid, Grid, VP, VP, VP, C.c_int, VP, VP]
+        L.cmm_vp_psi_workspace_bytes.restype = C.c_size_t
+        L.cmm_vp_psi_workspace_bytes.argtypes = [Grid, Grid]
+        L.cmm_vp_psi_from_f.argtypes = [Grid, Grid, Grid, Grid, VP, VP, VP, C.c_int, VP, VP, C.c_size_t, VP]
         L.cmm_particles_advect.argtypes = [Grid, C.POINTER(VP), C.c_int, DP, C.c_double, C.c_int, C.c_int, VP, VP]
         L.cmm_euler_create.argtypes = [C.POINTER(EulerParams), C.POINTER(VP)]
         L.cmm_euler_destroy.argtypes = [VP]
@@ -123,6 +129,7 @@ def lib():
         L.cmm_euler_get_field.argtypes = [VP, C.c_int, DP]
         L.cmm_euler_get_particles.argtypes = [VP, C.c_int, DP]
         L.cmm_euler_diagnostics.argtypes = [VP, DP]
+        L.cmm_euler_vp_diagnostics.argtypes = [VP, DP]
         L.cmm_euler_state.argtypes = [VP, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_int)]
         L.cmm_euler_profile.argtypes = [VP, C.c_int]
         L.cmm_euler_profile_read.argtypes = [VP, DP, C.POINTER(C.c_int)]
@@ -230,6 +237,18 @@ def evaluate_stream_hermite(g_map, g_vort, g_fine, g_psi, chix, chiy, w_herm, ps
                                    freq_cut_psi, _p(psi_out), None, 0, _stream()), "cmm_psi_from_vort")
 
 
+def evaluate_potential_from_density_hermite(g_map, g_vort, g_fine, g_psi, chix, chiy, f_herm, psi_out,
+                                             molly_stencil=0):
+    _check(lib().cmm_vp_psi_from_f(g_map, g_vort, g_fine, g_psi, _p(chix), _p(chiy), _p(f_herm), molly_stencil,
+                                   _p(psi_out), None, 0, _stream()), "cmm_vp_psi_from_f")
+
+
+def vp_grid(n, nv=None):
+    """(x, v) grid of the Vlasov-Poisson flow on [0, 4 pi) x [-2.5 pi, 2.5 pi), the reference's
+    bounds {0, 4, -2.5, 2.5} times pi (Cuda-CMM/src/simulation/cmm-vlasov1d.cu:64-75)."""
+    return grid(n, nv, (0.0, 2 * TWO_PI, -1.25 * TWO_PI, 1.25 * TWO_PI))
+
+
 def sample_vort(g_map, g_target, g_fine, chix, chiy, w_herm, out, molly_stencil=0):
     _check(lib().cmm_sample_vort(g_map, g_target, g_fine, _p(chix), _p(chiy), _p(w_herm), molly_stencil, _p(out),
                                  _stream()), "cmm_sample_vort")
@@ -247,7 +266,8 @@ class Euler:
     def __init__(self, g_coarse, g_fine, g_psi, g_vort, dt, time_integration="RK3", map_update_order="4th",
                  l_order=None, lagrange_init_higher_order=True, map_epsilon=1e-3, incomp_threshold=1e-4,
                  freq_cut_psi=128.0, molly_stencil=0, skip_remapping=False, initial_condition="4_nodes",
-                 ic_params=None, discrete_real=None, g_disc=None, seed=0, stack_capacity=64):
+                 ic_params=None, discrete_real=None, g_disc=None, seed=0, stack_capacity=64,
+                 simulation_type="cmm_euler_2d"):
         import numpy as np
         tin = TIME_INTEGRATION[time_integration]
         if l_order is None:
@@ -274,6 +294,7 @@ class Euler:
                 p.disc_real = d.ctypes.data_as(DP)
         p.seed = seed
         p.stack_capacity = stack_capacity
+        p.sim = SIM[simulation_type]
         self.params = p
         self.l_order = l_order
         h = C.c_void_p()
@@ -331,6 +352,12 @@ class Euler:
         """energy, enstrophy, palinstrophy, max |omega| (as the reference's Mesure files)"""
         out = (C.c_double * 4)()
         _check(lib().cmm_euler_diagnostics(self.h, C.cast(out, DP)), "cmm_euler_diagnostics")
+        return list(out)
+
+    def vp_diagnostics(self):
+        """Vlasov-Poisson: Etot, Ekin, Epot, Mass, Momentum, Emode1..4"""
+        out = (C.c_double * 9)()
+        _check(lib().cmm_euler_vp_diagnostics(self.h, C.cast(out, DP)), "cmm_euler_vp_diagnostics")
         return list(out)
 
     def state(self):

@@ -174,6 +174,21 @@ int cmm_hermite_from_spectrum(cmm_grid g, const double *spec, double *herm, void
 
 size_t cmm_psi_workspace_bytes(cmm_grid gv, cmm_grid gp, double fcut) { return op_psi_workspace(gv, gp, fcut); }
 
+size_t cmm_vp_psi_workspace_bytes(cmm_grid gv, cmm_grid gp) {
+    if (!grid_ok(gv) || !grid_ok(gp)) return 0;
+    return op_vp_psi_workspace(gv, gp);
+}
+
+int cmm_vp_psi_from_f(cmm_grid gm, cmm_grid gv, cmm_grid gf, cmm_grid gp, const double *cx, const double *cy,
+                      const double *f, int molly, double *psi, void *work, size_t wb, void *stream) {
+    if (!grid_ok(gm) || !grid_ok(gv) || !grid_ok(gf) || !grid_ok(gp) || !cx || !cy || !f || !psi ||
+        (molly != 0 && molly != 4 && molly != 8)) {
+        set_error("cmm_vp_psi_from_f: bad argument");
+        return CMM_ERR_ARG;
+    }
+    return op_vp_psi_from_f(gm, gv, gf, gp, cx, cy, f, molly, psi, work, wb, S(stream));
+}
+
 int cmm_psi_from_vort(cmm_grid gm, cmm_grid gv, cmm_grid gf, cmm_grid gp, const double *cx, const double *cy,
                       const double *W, int molly, double fcut, double *psi, void *work, size_t wb, void *stream) {
     if (!grid_ok(gm) || !grid_ok(gf) || !cx || !cy || !W || !psi || (molly != 0 && molly != 4 && molly != 8)) {
@@ -255,6 +270,15 @@ int engine_translate(cmm_euler *e) {
 }
 
 int engine_psi(cmm_euler *e, double *out, hipEvent_t *ev = nullptr) {
+    if (e->p.sim == CMM_SIM_VLASOV_POISSON_1D) {
+        // one operator; its whole time goes to the first psi profile slot
+        if (ev) (void)hipEventRecord(ev[0], e->st);
+        int rc = op_vp_psi_from_f(e->p.g_coarse, e->p.g_vort, e->p.g_fine, e->p.g_psi, e->cx, e->cy, e->W,
+                                  e->p.molly_stencil, out, e->work, e->work_bytes, e->st);
+        if (ev)
+            for (int i = 1; i < 4; ++i) (void)hipEventRecord(ev[i], e->st);
+        return rc;
+    }
     return op_psi_from_vort(e->p.g_coarse, e->p.g_vort, e->p.g_fine, e->p.g_psi, e->cx, e->cy, e->W,
                             e->p.molly_stencil, e->p.freq_cut_psi, out, e->work, e->work_bytes, e->st, ev);
 }
@@ -374,6 +398,14 @@ int cmm_euler_create(const cmm_euler_params *p, cmm_euler **out) {
         set_error("cmm_euler_create: grids must be square powers of two between 8 and 4096 (fine, psi, vort)");
         return CMM_ERR_ARG;
     }
+    if (p->sim != CMM_SIM_EULER_2D && p->sim != CMM_SIM_VLASOV_POISSON_1D) {
+        set_error("cmm_euler_create: unknown simulation type");
+        return CMM_ERR_ARG;
+    }
+    if (p->sim == CMM_SIM_VLASOV_POISSON_1D && (p->g_vort.nx > p->g_psi.nx || p->g_vort.ny > p->g_psi.ny)) {
+        set_error("cmm_euler_create: vorticity grid should be smaller than phi_out grid (Vlasov-Poisson needs grid_vort <= grid_psi)");
+        return CMM_ERR_ARG;
+    }
     if (p->l_order < 1 || p->l_order > 4 || !(p->dt > 0) || p->stack_capacity < 0 || p->map_update_order_num < 0 ||
         p->map_update_order_num > 2) {
         set_error("cmm_euler_create: bad time-stepping parameters");
@@ -404,6 +436,7 @@ int cmm_euler_create(const cmm_euler_params *p, cmm_euler **out) {
         if (p->discrete) w = std::max(w, op_hermite_workspace(p->g_disc));
         // Lagrange-weighted psi combinations of the map advection (two Hermite fields on g_psi)
         if (p->l_order >= 2) w = std::max(w, 8 * sizeof(double) * (size_t)e->np);
+        if (p->sim == CMM_SIM_VLASOV_POISSON_1D) w = std::max(w, op_vp_psi_workspace(p->g_vort, p->g_psi));
         e->work_bytes = w;
         if (hipMalloc(&e->work, w) != hipSuccess) { set_error("hipMalloc workspace"); rc = CMM_ERR_ALLOC; }
     }
@@ -648,6 +681,47 @@ int cmm_euler_diagnostics(cmm_euler *e, double *out) {
     out[1] = 0.5 * dc.hx * dc.hy * sz;
     out[2] = fft_ok ? sp : NAN;
     out[3] = std::max(wmax, -wmin);
+    return CMM_OK;
+}
+
+int cmm_euler_vp_diagnostics(cmm_euler *e, double *out) {
+    if (!e || !out) { set_error("cmm_euler_vp_diagnostics: bad argument"); return CMM_ERR_ARG; }
+    const cmm_grid &gc = e->p.g_coarse, &gp = e->p.g_psi;
+    int rc = CMM_OK;
+    if (!e->tmp) rc = dalloc(&e->tmp, e->nc);
+    if (!rc) rc = op_sample_vort(gc, gc, e->p.g_fine, e->cx, e->cy, e->W, 0, e->tmp, e->st);
+    std::vector<double> f(e->nc), ex(gp.nx);
+    if (!rc) rc = hip_check(hipMemcpyAsync(f.data(), e->tmp, sizeof(double) * e->nc, hipMemcpyDeviceToHost, e->st), "get");
+    if (!rc) rc = hip_check(hipMemcpyAsync(ex.data(), e->lvl[0] + e->np, sizeof(double) * gp.nx, hipMemcpyDeviceToHost, e->st), "get");
+    if (!rc) rc = hip_check(hipStreamSynchronize(e->st), "hipStreamSynchronize");
+    if (rc) return rc;
+    const DGrid dc = make_dgrid(gc), dp = make_dgrid(gp);
+    double mass = 0, kin = 0, mom = 0, pot = 0;
+    for (int iy = 0; iy < gc.ny; ++iy) {
+        const double v = gc.y0 + iy * dc.hy;
+        for (int ix = 0; ix < gc.nx; ++ix) {
+            const double fv = f[(size_t)iy * gc.nx + ix];
+            mass += fv;
+            kin += v * v * fv;
+            mom += v * fv;
+        }
+    }
+    for (int ix = 0; ix < gp.nx; ++ix) pot += ex[ix] * ex[ix];
+    out[1] = 0.5 * dc.hy * dc.hx * kin;
+    out[2] = 0.5 * dp.hx * pot;
+    out[0] = out[1] + out[2];
+    out[3] = dc.hy * dc.hx * mass;
+    out[4] = dc.hy * dc.hx * mom;
+    // Compute_kth_Mode: |R2C(dpsi/dx row)[k]| / NX for k = 1..4
+    for (int k = 1; k <= 4; ++k) {
+        double re = 0, im = 0;
+        for (int ix = 0; ix < gp.nx; ++ix) {
+            const double a = -2.0 * 3.141592653589793238462643383279502884 * (double)((long)k * ix % gp.nx) / gp.nx;
+            re += ex[ix] * std::cos(a);
+            im += ex[ix] * std::sin(a);
+        }
+        out[4 + k] = k <= gp.nx / 2 ? std::sqrt(re * re + im * im) / gp.nx : 0.0;
+    }
     return CMM_OK;
 }
 

@@ -387,6 +387,14 @@ struct ICParams {
 
 __device__ double init_vorticity(double x, double y, const ICParams &p) {
     switch (p.ic) {
+    // distribution functions f(x, v), d_init_distirbution_function (Cuda-CMM/src/simulation/cmm-init.cu:457-493)
+    case CMM_IC_LANDAU_DAMPING:
+        return (1 + 0.05 * cos(0.5 * x)) * exp(-y * y * 0.5) / sqrt(2 * kPi);
+    case CMM_IC_TWO_STREAM:
+        return 1 / (2 * sqrt(2 * kPi)) * (1 + 0.05 * cos(0.2 * x)) *
+               (exp(-(y - 3) * (y - 3) * 0.5) + exp(-(y + 3) * (y + 3) * 0.5));
+    case CMM_IC_NONLIN_LANDAU_DAMPING:
+        return (1 + 0.5 * cos(0.5 * x)) * exp(-y * y * 0.5) / sqrt(2 * kPi);
     case CMM_IC_4_NODES: {
         x -= floor(x / kTwoPi) * kTwoPi;
         y -= floor(y / kTwoPi) * kTwoPi;
@@ -862,6 +870,132 @@ __global__ void __launch_bounds__(256) k_particles(DGrid g, PsiLevels P, PartPar
 }
 
 // ------------------------------------------------------------------------------------------------
+// Vlasov-Poisson psi (x, v), evaluate_potential_from_density_hermite
+// (Cuda-CMM/src/simulation/cmm-simulation-host.cu:385-471, :478-494, :534-571).
+//   f(X(x)) on the vort grid -> I(x) = hv (f(x, v0) + sum_v f) - 1 (integral_v, whose output row
+//   aliases the first input row, so v0 counts twice) -> phi_hat = F/(NX k^2), DC 0, zero padded to
+//   the psi grid, C2R -> phi(x);  bump sigma(v), centred and scaled to max 1 -> H_hat = S/(NY k^2),
+//   C2R -> H(v);  psi = phi(x) - H(v) -> Hermite planes.
+
+// partial column sums over chunks of `rows` rows: part[c*nx + x] = sum_{y in chunk c} f[y*nx + x]
+__global__ void __launch_bounds__(256) k_vp_colsum(const double *__restrict__ f, int nx, int ny, int rows,
+                                                   double *__restrict__ part) {
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    if (x >= nx) return;
+    const int y0 = blockIdx.y * rows, y1 = min(ny, y0 + rows);
+    double a = 0.0;
+    for (int y = y0; y < y1; ++y) a += f[(long)y * nx + x];
+    part[(long)blockIdx.y * nx + x] = a;
+}
+
+struct VP1D {
+    int nv, lognv;       // vort grid along x
+    int np, lognp;       // psi grid along x (= along v)
+    int nchunks;
+    double hv;           // vort grid spacing along v
+    double fkx_v;        // 2 pi / (hx NX) of the vort grid
+    double fky_p;        // 2 pi / (hy NY) of the psi grid
+    double v0, hv_p, Lv; // psi grid v origin, spacing, length
+};
+
+// block-wide sum / max over kFftThreads lanes (fixed tree order)
+__device__ double block_reduce(double v, double *red, bool is_max) {
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = kFftThreads / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = is_max ? fmax(red[threadIdx.x], red[threadIdx.x + s])
+                                                            : red[threadIdx.x] + red[threadIdx.x + s];
+        __syncthreads();
+    }
+    const double r = red[0];
+    __syncthreads();
+    return r;
+}
+
+// s holds the forward transform of a real signal of length n_in (bins k <= n_in/2 valid).  Writes
+// the Hermitian spectrum of length n_out with X[k] = s[k] * scale / (fk k)^2 for 1 <= k <= n_in/2
+// (zero elsewhere and at DC; only the real part at n_out/2), then inverse-transforms in place:
+// the C2R of the reference's zero-padded 1-D spectrum.
+__device__ void inv_lap_c2r(cd *s, int n_in, int n_out, int logn_out, double scale, double fk,
+                            const cd *__restrict__ tw_out) {
+    constexpr int PER = kFftMaxN / 2 / kFftThreads + 1;
+    cd keep[PER];
+#pragma unroll
+    for (int m = 0; m < PER; ++m) {
+        const int k = threadIdx.x + m * kFftThreads;
+        keep[m] = (k <= n_in / 2 && k > 0) ? s[sw(k)] : cd{0.0, 0.0};
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n_out; i += kFftThreads) s[sw(i)] = cd{0.0, 0.0};
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < PER; ++m) {
+        const int k = threadIdx.x + m * kFftThreads;
+        if (k > 0 && k <= n_in / 2 && k <= n_out / 2) {
+            const double kk = fk * k, k2 = kk * kk;
+            cd v{keep[m].x * scale / k2, keep[m].y * scale / k2};
+            if (k == n_out / 2) {
+                s[sw(k)] = cd{v.x, 0.0};
+            } else {
+                s[sw(k)] = v;
+                s[sw(n_out - k)] = cd{v.x, -v.y};
+            }
+        }
+    }
+    __syncthreads();
+    wg_fft(s, n_out, logn_out, tw_out, true);
+}
+
+// one workgroup: phi (np values) then H (np values) into phiH
+__global__ void __launch_bounds__(kFftThreads) k_vp_1d(VP1D q, const double *__restrict__ f,
+                                                       const double *__restrict__ part, const cd *__restrict__ tw_v,
+                                                       const cd *__restrict__ tw_p, double *__restrict__ phiH) {
+    extern __shared__ cd s[];
+    __shared__ double red[kFftThreads];
+    // I(x): the row v0 once more, then the chunk sums in order
+    for (int x = threadIdx.x; x < q.nv; x += kFftThreads) {
+        double a = f[x];
+        for (int c = 0; c < q.nchunks; ++c) a += part[(long)c * q.nv + x];
+        a *= q.hv;
+        a += -1.0;
+        s[sw(x)] = cd{a, 0.0};
+    }
+    __syncthreads();
+    wg_fft(s, q.nv, q.lognv, tw_v, false);
+    inv_lap_c2r(s, q.nv, q.np, q.lognp, 1.0 / (double)q.nv, q.fkx_v, tw_p);
+    for (int x = threadIdx.x; x < q.np; x += kFftThreads) phiH[x] = s[sw(x)].x;
+    __syncthreads();
+    // bump function over v on the psi grid
+    double sum = 0.0;
+    for (int i = threadIdx.x; i < q.np; i += kFftThreads) {
+        const double v = q.v0 + i * q.hv_p;
+        const double b = 0.5 - 0.5 * tanh(2 * kPi * (fabs(v) - 0.4 * q.Lv) / (0.1 * q.Lv));
+        s[sw(i)] = cd{b, 0.0};
+        sum += b;
+    }
+    const double mean = block_reduce(sum, red, false) / (double)q.np;
+    double mx = -1e300;
+    for (int i = threadIdx.x; i < q.np; i += kFftThreads) {
+        const double b = s[sw(i)].x - mean;
+        s[sw(i)].x = b;
+        mx = fmax(mx, b);
+    }
+    mx = block_reduce(mx, red, true);
+    for (int i = threadIdx.x; i < q.np; i += kFftThreads) s[sw(i)].x /= mx;
+    __syncthreads();
+    wg_fft(s, q.np, q.lognp, tw_p, false);
+    inv_lap_c2r(s, q.np, q.np, q.lognp, 1.0 / (double)q.np, q.fky_p, tw_p);
+    for (int i = threadIdx.x; i < q.np; i += kFftThreads) phiH[q.np + i] = s[sw(i)].x;
+}
+
+// psi(x, v) = phi(x) - H(v) on the psi grid (k_assemble_psi_periodified)
+struct SamplePsiVP {
+    const double *__restrict__ phiH;
+    int np;
+    __device__ double operator()(int ix, int iy) const { return phiH[ix] - phiH[np + iy]; }
+};
+
+// ------------------------------------------------------------------------------------------------
 // Host-side helpers: twiddle tables and workspaces, cached per device.
 
 struct DeviceCache {
@@ -1283,6 +1417,73 @@ int op_sample_init(const cmm_grid &gm, const cmm_grid &gt, const double *cx, con
     dim3 b(64, 4);
     hipLaunchKernelGGL(k_sample_grid<SampleCompose>, grid2d(gt.nx, gt.ny, b), b, 0, st, s, gt.nx, gt.ny, out);
     return launch_check("k_sample_grid");
+}
+
+static constexpr int kVpRows = 64;
+
+static int vp_check(const cmm_grid &gv, const cmm_grid &gp) {
+    if (int rc = fft_grid_ok(gv, "vort grid")) return rc;
+    if (int rc = fft_grid_ok(gp, "psi grid")) return rc;
+    if (gv.nx > gp.nx || gv.ny > gp.ny) {
+        // the reference stops with error 1013 (Cuda-CMM/src/simulation/cmm-simulation-host.cu:462-465)
+        set_error("vorticity grid should be smaller than phi_out grid (vort grid finer than psi grid)");
+        return CMM_ERR_ARG;
+    }
+    if (gp.nx != gp.ny) {
+        // reference error 20230904 (Cuda-CMM/src/simulation/cmm-simulation-host.cu:555)
+        set_error("velocity periodification assumes NX==NY on the psi grid");
+        return CMM_ERR_ARG;
+    }
+    return CMM_OK;
+}
+
+size_t op_vp_psi_workspace(const cmm_grid &gv, const cmm_grid &gp) {
+    const size_t nchunks = (gv.ny + kVpRows - 1) / kVpRows;
+    const size_t head = sizeof(double) * ((size_t)gv.nx * gv.ny + nchunks * gv.nx + 2 * (size_t)gp.nx);
+    return ((head + 255) & ~(size_t)255) + spec_workspace_bytes(gp, gp, 0, 0.0);
+}
+
+int op_vp_psi_from_f(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf, const cmm_grid &gp, const double *cx,
+                     const double *cy, const double *W, int molly, double *psi, void *work, size_t work_bytes,
+                     hipStream_t st) {
+    if (int rc = vp_check(gv, gp)) return rc;
+    void *w;
+    if (int rc = get_work(work, work_bytes, op_vp_psi_workspace(gv, gp), &w)) return rc;
+    const int nchunks = (gv.ny + kVpRows - 1) / kVpRows;
+    double *f = static_cast<double *>(w);
+    double *part = f + (size_t)gv.nx * gv.ny;
+    double *phiH = part + (size_t)nchunks * gv.nx;
+    const size_t head = sizeof(double) * ((size_t)gv.nx * gv.ny + (size_t)nchunks * gv.nx + 2 * (size_t)gp.nx);
+    void *wspec = static_cast<char *>(w) + ((head + 255) & ~(size_t)255);
+    if (int rc = op_sample_vort(gm, gv, gf, cx, cy, W, molly, f, st)) return rc;
+    hipLaunchKernelGGL(k_vp_colsum, dim3((gv.nx + 255) / 256, nchunks), dim3(256), 0, st, f, gv.nx, gv.ny, kVpRows,
+                       part);
+    if (int rc = launch_check("k_vp_colsum")) return rc;
+    const cd *tw_v, *tw_p;
+    if (int rc = twiddles(gv.nx, &tw_v)) return rc;
+    if (int rc = twiddles(gp.nx, &tw_p)) return rc;
+    const DGrid dv = make_dgrid(gv), dp = make_dgrid(gp);
+    VP1D q;
+    q.nv = gv.nx;
+    q.lognv = ilog2(gv.nx);
+    q.np = gp.nx;
+    q.lognp = ilog2(gp.nx);
+    q.nchunks = nchunks;
+    q.hv = dv.hy;
+    q.fkx_v = kTwoPi / (dv.hx * gv.nx);
+    q.fky_p = kTwoPi / (dp.hy * gp.ny);
+    q.v0 = gp.y0;
+    q.hv_p = dp.hy;
+    q.Lv = gp.y1 - gp.y0;
+    const size_t lds = sizeof(cd) * std::max(gv.nx, gp.nx);
+    // 64 KiB of dynamic LDS plus the static reduction array: raise the limit explicitly
+    if (int rc = hip_check(hipFuncSetAttribute((const void *)k_vp_1d, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+                           "hipFuncSetAttribute"))
+        return rc;
+    hipLaunchKernelGGL(k_vp_1d, dim3(1), dim3(kFftThreads), lds, st, q, f, part, tw_v, tw_p, phiH);
+    if (int rc = launch_check("k_vp_1d")) return rc;
+    SamplePsiVP s{phiH, gp.nx};
+    return run_spec(plan_spec(gp, gp, 0, 0.0), s, psi, wspec, st);
 }
 
 int op_sample_vort(const cmm_grid &gm, const cmm_grid &gt, const cmm_grid &gf, const double *cx, const double *cy,

@@ -292,6 +292,25 @@ struct Run {
         for (auto &c : s.save_computational)
             if (c.conv && save_due(c.is_instant, c.time_start, c.time_end, c.time_step, t_now, dt_now, dt)) now = true;
         if (!now) return "";
+        auto f8 = [](double v) {
+            std::ostringstream os;
+            os.precision(8);
+            os << v;
+            return os.str();
+        };
+        if (s.simulation_type == "cmm_vlasov_poisson_1d") {
+            double m[9];
+            check(cmm_euler_vp_diagnostics(e, m), "cmm_euler_vp_diagnostics");
+            append1(s, "/Monitoring_data/Mesure/Time_s", t_now);
+            append1(s, "/Monitoring_data/Mesure/Etot", m[0]);
+            append1(s, "/Monitoring_data/Mesure/Ekin", m[1]);
+            append1(s, "/Monitoring_data/Mesure/Epot", m[2]);
+            append1(s, "/Monitoring_data/Mesure/Mass", m[3]);
+            append1(s, "/Monitoring_data/Mesure/Momentum", m[4]);
+            for (int k = 0; k < 4; ++k) append1(s, "/Monitoring_data/Mesure/Emode" + to_s(k + 1), m[5 + k]);
+            return "Computed coarse Cons : Etot = " + f8(m[0]) + " \t Momenta = " + f8(m[4]) + " \t Ekin = " + f8(m[1]) +
+                   " \t Epot = " + f8(m[2]) + " \t Mass = " + f8(m[3]);
+        }
         double m[4];
         check(cmm_euler_diagnostics(e, m), "cmm_euler_diagnostics");
         append1(s, "/Monitoring_data/Mesure/Time_s", t_now);
@@ -299,12 +318,6 @@ struct Run {
         append1(s, "/Monitoring_data/Mesure/Enstrophy", m[1]);
         append1(s, "/Monitoring_data/Mesure/Palinstrophy", m[2]);
         append1(s, "/Monitoring_data/Mesure/Max_vorticity", m[3]);
-        auto f8 = [](double v) {
-            std::ostringstream os;
-            os.precision(8);
-            os << v;
-            return os.str();
-        };
         return "Computed coarse Cons : Energ = " + f8(m[0]) + " \t Enstr = " + f8(m[1]) + " \t Palinstr = " + f8(m[2]) +
                " \t Wmax = " + f8(m[3]);
     }
@@ -342,6 +355,11 @@ double compute_next_timestep(const Settings &s, double t, double dt) {
 
 int run_euler(Settings &s) {
     const auto begin = Clock::now();
+    // the Vlasov-Poisson loop (Cuda-CMM/src/simulation/cmm-vlasov1d.cu) shares this driver: bounds
+    // in units of pi (:64-75), f in place of omega, psi from the density, E_inc sanity abort (:838-840)
+    const bool vp = s.simulation_type == "cmm_vlasov_poisson_1d";
+    if (vp)
+        for (int i = 0; i < 4; ++i) s.domain_bounds[i] *= kPi;
     auto ctime = [&]() { return std::chrono::duration_cast<std::chrono::microseconds>(Clock::now() - begin).count() / 1e6; };
 
     // dt and Lagrange order, Cuda-CMM/src/simulation/cmm-euler2d.cu:62-78
@@ -373,7 +391,7 @@ int run_euler(Settings &s) {
     // parts of the reference that are not on this engine's path stop here with a message
     if (s.forward_map) fail("forward_map is not supported by this engine (backward map only)", 1);
     if (s.restart_time != 0) fail("restart from a saved map stack is not supported by this engine yet", 1);
-    if (s.initial_condition_num >= 7 && s.initial_condition_num <= 8)
+    if (!vp && s.initial_condition_num >= 7 && s.initial_condition_num <= 8)
         fail("initial condition " + s.initial_condition + " draws from cuRAND/std distributions and is not provided", 230305);
     if (!s.save_sample.empty() && s.verbose >= 2) log.push("Note: save_sample outputs are not produced by this engine");
     if (!s.save_zoom.empty() && s.verbose >= 2) log.push("Note: save_zoom outputs are not produced by this engine");
@@ -400,10 +418,13 @@ int run_euler(Settings &s) {
     p.freq_cut_psi = s.freq_cut_psi;
     p.molly_stencil = s.molly_stencil;
     p.skip_remapping = s.skip_remapping;
-    p.ic = s.initial_condition_num;
+    p.ic = vp ? CMM_IC_LANDAU_DAMPING + s.initial_condition_num : s.initial_condition_num;
+    p.sim = vp ? CMM_SIM_VLASOV_POISSON_1D : CMM_SIM_EULER_2D;
     for (int i = 0; i < 10; ++i) p.ic_params[i] = s.initial_params[i];
     std::vector<double> disc;
-    if (s.initial_condition_num == CMM_IC_RANDOM_SPECTRUM) {
+    if (vp && s.initial_discrete) fail("a discrete initial condition is not provided for the Vlasov-Poisson flow", 1);
+    if (vp && s.grid_vort > s.grid_psi) fail("vorticity grid should be smaller than phi_out grid", 1013);
+    if (!vp && s.initial_condition_num == CMM_IC_RANDOM_SPECTRUM) {
         p.discrete = 1;
         p.g_disc = r.gf;
         p.seed = s.seed;
@@ -492,6 +513,8 @@ int run_euler(Settings &s) {
             log.push("Stack Saturated : Exitting");
             cont = false;
         }
+        if (vp && (info.incomp <= 0 || info.incomp > 1e15))
+            fail("Dear captain there is an incompressibility error in coarse grid. Aborting.", 2345);
         if (info.remapped) {
             if (s.verbose >= 2)
                 log.push("Refining Map : Step = " + to_s(loop) + " \t Maps = " + to_s(info.stack_size) + " \t Gap = " + to_s(loop - old));

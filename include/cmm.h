@@ -104,6 +104,20 @@ int cmm_psi_from_vort(cmm_grid g_map, cmm_grid g_vort, cmm_grid g_fine, cmm_grid
 int cmm_sample_vort(cmm_grid g_map, cmm_grid g_target, cmm_grid g_fine, const double *chix, const double *chiy,
                     const double *w_herm, int molly_stencil, double *out, void *stream);
 
+/* psi(x, v) Hermite of the 1D-1V Vlasov-Poisson flow from the distribution function's fine
+ * Hermite field: sample f(X(x)) on g_vort (mollifier 0/4/8), integrate over v, solve the 1-D
+ * Poisson equation spectrally, zero-pad to g_psi, add the periodified -v^2/2 part, Hermite planes;
+ * replaces evaluate_potential_from_density_hermite with calc_phi, integral_v, solve_1D_laplace,
+ * zero_padding_1D, compute_periodification_function and k_assemble_psi_periodified
+ * (Cuda-CMM/src/simulation/cmm-simulation-host.cu:385-471, :478-494, :534-571,
+ * Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:504-512, :570-586, Cuda-CMM/src/numerical/cmm-fft.cu:367-391).
+ * g_vort must not be finer than g_psi and g_psi must be square (the reference's errors 1013 and
+ * 20230904); x and v bounds are g_psi's (x0, x1) and (y0, y1). */
+size_t cmm_vp_psi_workspace_bytes(cmm_grid g_vort, cmm_grid g_psi);
+int cmm_vp_psi_from_f(cmm_grid g_map, cmm_grid g_vort, cmm_grid g_fine, cmm_grid g_psi, const double *chix,
+                      const double *chiy, const double *f_herm, int molly_stencil, double *psi_out, void *work,
+                      size_t work_bytes, void *stream);
+
 /* ---- particles ----------------------------------------------------------------------------- */
 /* Fluid particles, one step; replaces particles_advect + Particle_advect
  * (Cuda-CMM/src/numerical/cmm-particles.cu:63-87, :119-276).  psi_levels: host array of l_order
@@ -130,7 +144,13 @@ typedef struct cmm_euler_params {
     const double *disc_real;        /* host real field on g_disc (NULL with CMM_IC_RANDOM_SPECTRUM) */
     uint64_t seed;                  /* CMM_IC_RANDOM_SPECTRUM seed */
     int stack_capacity;             /* maps the device stack holds */
+    int sim;                        /* CMM_SIM_EULER_2D or CMM_SIM_VLASOV_POISSON_1D */
 } cmm_euler_params;
+
+/* The 1D-1V Vlasov-Poisson loop (cuda_vlasov_1d, Cuda-CMM/src/simulation/cmm-vlasov1d.cu:48-1148)
+ * is the same loop with f in place of omega: its initial conditions are CMM_IC_LANDAU_DAMPING..
+ * and its psi comes from cmm_vp_psi_from_f; the map lives on (x, v) = (x0..x1, y0..y1). */
+enum { CMM_SIM_EULER_2D = 0, CMM_SIM_VLASOV_POISSON_1D = 1 };
 
 typedef struct cmm_step_info {
     double t;                       /* time after the step */
@@ -165,6 +185,12 @@ int cmm_euler_get_particles(cmm_euler *e, int set, double *host_out);
  * (Cuda-CMM/src/simulation/cmm-simulation-host.cu:615-698): out[0] energy 0.5 h^2 sum |grad psi|^2,
  * out[1] enstrophy and out[2] palinstrophy of the coarse vorticity, out[3] max |omega|. */
 int cmm_euler_diagnostics(cmm_euler *e, double *out);
+/* Vlasov-Poisson diagnostics of compute_conservation_targets (Cuda-CMM/src/simulation/cmm-simulation-host.cu:642-661,
+ * Cuda-CMM/src/numerical/cmm-mesure.cu:97-202) on the coarse f sample and psi level 0: out[0] Etot,
+ * out[1] Ekin = 1/2 hx hv sum f v^2, out[2] Epot = 1/2 hx sum (dpsi/dx)^2 over the first row,
+ * out[3] Mass = hx hv sum f, out[4] Momentum = hx hv sum f v, out[5..8] |E_k|, k = 1..4, of the
+ * first row of dpsi/dx (R2C / NX). */
+int cmm_euler_vp_diagnostics(cmm_euler *e, double *out);
 /* Steps taken, current time, stack size. */
 int cmm_euler_state(cmm_euler *e, int *steps, double *t, int *stack_size);
 /* Per-kernel timing with events on the engine stream: enable (clears the counters), then read

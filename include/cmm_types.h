@@ -40,8 +40,10 @@ typedef struct cmm_advect_coeffs {
     int direction;             /* -1: backward map (the only direction on this path) */
 } cmm_advect_coeffs;
 
-/* Initial-condition identifiers (vorticity).  0..11 follow Cuda-CMM/src/ui/settings.h:313-324;
- * 100 and up are conditions the reference does not have. */
+/* Initial-condition identifiers.  Vorticity: 0..11 follow Cuda-CMM/src/ui/settings.h:313-324;
+ * 100 and 101 are conditions the reference does not have.  Distribution functions of the 1D-1V
+ * Vlasov-Poisson flow (x, v): 200 + the reference's number (Cuda-CMM/src/ui/settings.h:300-303,
+ * Cuda-CMM/src/simulation/cmm-init.cu:457-493). */
 enum {
     CMM_IC_4_NODES = 0,
     CMM_IC_QUADROPOLE = 1,
@@ -54,7 +56,10 @@ enum {
     CMM_IC_TWO_COSINE = 10,
     CMM_IC_VORTEX_SHEETS = 11,
     CMM_IC_DOUBLE_SHEAR_LAYER = 100,   /* delta^-1 sech^2((y-pi/2)/delta) - delta^-1 sech^2((y-3pi/2)/delta) + 0.05 cos x */
-    CMM_IC_RANDOM_SPECTRUM = 101        /* discrete: seeded phases, E(k) ~ k^4 exp(-2 (k/6)^2), enstrophy 1 */
+    CMM_IC_RANDOM_SPECTRUM = 101,       /* discrete: seeded phases, E(k) ~ k^4 exp(-2 (k/6)^2), enstrophy 1 */
+    CMM_IC_LANDAU_DAMPING = 200,        /* (1 + 0.05 cos 0.5x) exp(-v^2/2)/sqrt(2 pi) */
+    CMM_IC_TWO_STREAM = 201,            /* (1 + 0.05 cos 0.2x)(exp(-(v-3)^2/2) + exp(-(v+3)^2/2))/(2 sqrt(2 pi)) */
+    CMM_IC_NONLIN_LANDAU_DAMPING = 202  /* (1 + 0.5 cos 0.5x) exp(-v^2/2)/sqrt(2 pi) */
 };
 
 /* Error codes returned by every C-ABI call (0 = success). */

@@ -425,6 +425,14 @@ void fourier_hermite(const Grid &g, const cplx *F, double *out) {
 // Initial conditions, Cuda-CMM/src/simulation/cmm-init.cu:30-241 (and new ones)
 double init_vorticity(double x, double y, int ic, const double *prm) {
     switch (ic) {
+    // distribution functions f(x, v), d_init_distirbution_function (Cuda-CMM/src/simulation/cmm-init.cu:457-493)
+    case CMM_IC_LANDAU_DAMPING:
+        return (1 + 0.05 * std::cos(0.5 * x)) * std::exp(-y * y * 0.5) / std::sqrt(2 * kPi);
+    case CMM_IC_TWO_STREAM:
+        return 1 / (2 * std::sqrt(2 * kPi)) * (1 + 0.05 * std::cos(0.2 * x)) *
+               (std::exp(-(y - 3) * (y - 3) * 0.5) + std::exp(-(y + 3) * (y + 3) * 0.5));
+    case CMM_IC_NONLIN_LANDAU_DAMPING:
+        return (1 + 0.5 * std::cos(0.5 * x)) * std::exp(-y * y * 0.5) / std::sqrt(2 * kPi);
     case CMM_IC_4_NODES: {
         x -= std::floor(x / kTwoPi) * kTwoPi;
         y -= std::floor(y / kTwoPi) * kTwoPi;
@@ -599,6 +607,104 @@ void psi_from_vort(const Grid &gm, const Grid &gv, const Grid &gf, const Grid &g
 }
 
 // Cuda-CMM/src/numerical/cmm-particles.cu:119-276 (fluid particles; psi level 0 = t_{n+1}, 1 = t_n)
+// 1-D transforms of length n (power of two), unnormalised.  irfft1 has the C2R semantics of the
+// 2-D version: the imaginary parts of bins 0 and n/2 are dropped.
+void rfft1(int n, const double *in, cplx *out) {
+    FFTPlan pl = make_plan(n);
+    std::vector<cplx> a(n);
+    for (int i = 0; i < n; ++i) a[i] = cplx(in[i], 0.0);
+    fft1(pl, a.data(), -1);
+    for (int k = 0; k <= n / 2; ++k) out[k] = a[k];
+}
+
+void irfft1(int n, const cplx *in, double *out) {
+    FFTPlan pl = make_plan(n);
+    std::vector<cplx> a(n);
+    a[0] = cplx(in[0].real(), 0.0);
+    a[n / 2] = cplx(in[n / 2].real(), 0.0);
+    for (int k = 1; k < n / 2; ++k) { a[k] = in[k]; a[n - k] = std::conj(in[k]); }
+    fft1(pl, a.data(), +1);
+    for (int i = 0; i < n; ++i) out[i] = a[i].real();
+}
+
+// Vlasov-Poisson psi (x, v) from the distribution function,
+// evaluate_potential_from_density_hermite (Cuda-CMM/src/simulation/cmm-simulation-host.cu:385-415):
+//  1. f(X(x)) on the vort grid (k_apply_map_and_sample_from_hermite, as for the vorticity);
+//  2. integral_v (Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:570-586): I(x) = hv sum_v f - 1.
+//     The reference's output row aliases the input's first row (both are the temporary array),
+//     so the row v0 enters twice: I(x) = hv (f(x, v0) + sum_iv f(x, v_iv)) - 1;
+//  3. solve_1D_laplace (:417-432): R2C, /NX, then division by +kx^2 (k_fft_iLap_hx_1D,
+//     Cuda-CMM/src/numerical/cmm-fft.cu), DC set to 0;
+//  4. calc_phi (:436-471): zero_padding_1D (Cuda-CMM/src/numerical/cmm-fft.cu:367-391) keeps the bins
+//     k <= NX_vort/2 on the psi grid, then C2R -> phi(x); a vort grid finer than the psi grid is
+//     rejected (reference error 1013);
+//  5. compute_periodification_function (:534-571) with compute_bump_function (:478-494) on the psi
+//     grid (NX == NY required, reference error 20230904): sigma(v) = 1/2 - 1/2 tanh(2 pi (|v| - 0.4 Lv)
+//     / (0.1 Lv)), minus its mean, divided by its maximum; R2C, /NY, division by +ky^2, DC 0, C2R
+//     -> H(v);
+//  6. k_assemble_psi_periodified (Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:504-512):
+//     psi(x, v) = phi(x) - H(v), then R2C, /N and fourier_hermite.
+int vp_psi_from_f(const Grid &gm, const Grid &gv, const Grid &gf, const Grid &gp, const double *cx, const double *cy,
+                  const double *Hf, int molly, double *psi) {
+    if (gv.nx > gp.nx || gv.ny > gp.ny || gp.nx != gp.ny) return CMM_ERR_ARG;
+    std::vector<double> f(gv.n);
+    sample_vort(gm, gv, gf, cx, cy, Hf, molly, f.data());
+    const int nv = gv.nx, np = gp.nx;
+    std::vector<double> I(nv);
+    for (int ix = 0; ix < nv; ++ix) {
+        double a = f[ix];
+        for (int iy = 0; iy < gv.ny; ++iy) a += f[(long)iy * nv + ix];
+        a *= gv.hy;
+        a += -1.0;
+        I[ix] = a;
+    }
+    std::vector<cplx> F(nv / 2 + 1), P(np / 2 + 1, cplx(0, 0));
+    rfft1(nv, I.data(), F.data());
+    const double fkx = kTwoPi / (gv.hx * gv.nx);
+    for (int k = 0; k <= nv / 2; ++k) {
+        cplx v(F[k].real() / (double)nv, F[k].imag() / (double)nv);
+        if (k == 0) { P[0] = 0; continue; }
+        const double kx = fkx * k, k2 = kx * kx;
+        P[k] = cplx(v.real() / k2, v.imag() / k2);
+    }
+    std::vector<double> phi(np);
+    irfft1(np, P.data(), phi.data());
+    // bump function and its double integral along v
+    const int ny = gp.ny;
+    const double Lv = gp.y1 - gp.y0;
+    std::vector<double> sg(ny);
+    double mean = 0;
+    for (int iy = 0; iy < ny; ++iy) {
+        const double v = gp.y0 + iy * gp.hy;
+        sg[iy] = 0.5 - 0.5 * std::tanh(2 * kPi * (std::fabs(v) - 0.4 * Lv) / (0.1 * Lv));
+        mean += sg[iy];
+    }
+    mean /= (double)ny;
+    double mx = -1e300;
+    for (int iy = 0; iy < ny; ++iy) { sg[iy] -= mean; mx = std::max(mx, sg[iy]); }
+    for (int iy = 0; iy < ny; ++iy) sg[iy] /= mx;
+    std::vector<cplx> S(ny / 2 + 1);
+    rfft1(ny, sg.data(), S.data());
+    const double fky = kTwoPi / (gp.hy * gp.ny);
+    for (int k = 0; k <= ny / 2; ++k) {
+        cplx v(S[k].real() / (double)ny, S[k].imag() / (double)ny);
+        if (k == 0) { S[0] = 0; continue; }
+        const double ky = fky * k, k2 = ky * ky;
+        S[k] = cplx(v.real() / k2, v.imag() / k2);
+    }
+    std::vector<double> H(ny);
+    irfft1(ny, S.data(), H.data());
+    std::vector<double> real(gp.n);
+    for (int iy = 0; iy < ny; ++iy)
+        for (int ix = 0; ix < np; ++ix) real[(long)iy * np + ix] = phi[ix] - H[iy];
+    const int nh = gp.nx / 2 + 1;
+    std::vector<cplx> R((size_t)nh * gp.ny);
+    rfft2(gp.nx, gp.ny, real.data(), R.data());
+    for (auto &v : R) v = cplx(v.real() / (double)gp.n, v.imag() / (double)gp.n);
+    fourier_hermite(gp, R.data(), psi);
+    return CMM_OK;
+}
+
 void particle_step(const Grid &g, const double *psi, int L, const double *Lp, double dt, int scheme, double p[2]) {
     const long N4 = 4 * g.n;
     double u[8], k[8];
@@ -832,6 +938,11 @@ int cmmo_psi_from_vort(cmm_grid gm, cmm_grid gv, cmm_grid gf, cmm_grid gp, const
     return CMM_OK;
 }
 
+int cmmo_vp_psi_from_f(cmm_grid gm, cmm_grid gv, cmm_grid gf, cmm_grid gp, const double *cx, const double *cy,
+                       const double *f_herm, int molly, double *psi_out) {
+    return vp_psi_from_f(mk(gm), mk(gv), mk(gf), mk(gp), cx, cy, f_herm, molly, psi_out);
+}
+
 int cmmo_particles_advect(cmm_grid gg, const double *psi, int L, const double *Lp, double dt, int scheme, int n,
                           double *pos) {
     Grid g = mk(gg);
@@ -879,6 +990,7 @@ int cmmo_ic_random_spectrum(int n, uint64_t seed, double *out) {
 // Full Euler run in the order of Cuda-CMM/src/simulation/cmm-euler2d.cu:495-960
 int cmmo_euler_run(const cmmo_euler_params *p, cmmo_euler_result *r) {
     if (!p || !r) return CMM_ERR_ARG;
+    if (p->sim == 1 && (p->g_vort.nx > p->g_psi.nx || p->g_psi.nx != p->g_psi.ny)) return CMM_ERR_ARG;
     const Grid gc = mk(p->g_coarse), gf = mk(p->g_fine), gp = mk(p->g_psi);
     const int L = p->l_order;
     const long Nc = gc.n, Nf = gf.n, Np = gp.n;
@@ -894,8 +1006,11 @@ int cmmo_euler_run(const cmmo_euler_params *p, cmmo_euler_result *r) {
         cmmo_hermite_from_real(p->g_fine, samp.data(), W.data());
     };
     auto eval_psi = [&]() {
-        psi_from_vort(gc, mk(p->g_vort), gf, gp, cx.data(), cy.data(), W.data(), p->molly_stencil, p->freq_cut_psi,
-                      psi.data());
+        if (p->sim == 1)
+            vp_psi_from_f(gc, mk(p->g_vort), gf, gp, cx.data(), cy.data(), W.data(), p->molly_stencil, psi.data());
+        else
+            psi_from_vort(gc, mk(p->g_vort), gf, gp, cx.data(), cy.data(), W.data(), p->molly_stencil,
+                          p->freq_cut_psi, psi.data());
     };
     auto advect = [&](const double *t, const double *dt, int loop_ctr, int lo, int tin) {
         cmm_advect_coeffs c;

@@ -62,6 +62,10 @@ int cmmo_hermite_from_real(cmm_grid g, const double *real, double *herm);
 int cmmo_psi_from_vort(cmm_grid gm, cmm_grid gvort, cmm_grid gfine, cmm_grid gpsi,
                        const double *chix, const double *chiy, const double *w_herm,
                        int molly, double freq_cut, double *psi_out);
+/* Vlasov-Poisson psi(x, v) from the distribution function's Hermite field,
+ * Cuda-CMM/src/simulation/cmm-simulation-host.cu:385-471, :478-494, :534-571 (see cmm_oracle.cpp) */
+int cmmo_vp_psi_from_f(cmm_grid gm, cmm_grid gvort, cmm_grid gfine, cmm_grid gpsi, const double *chix,
+                       const double *chiy, const double *f_herm, int molly, double *psi_out);
 /* Fluid particles, Cuda-CMM/src/numerical/cmm-particles.cu:119-276 */
 int cmmo_particles_advect(cmm_grid gpsi, const double *psi, int l_order, const double *Lp12,
                           double dt, int scheme, int n, double *pos);
@@ -85,6 +89,7 @@ typedef struct cmmo_euler_params {
     int n_particles, particle_scheme;
     double *particles;         /* in/out, 2*n_particles */
     int max_steps;             /* stop after this many steps (<=0: run to final_time) */
+    int sim;                   /* 0: 2D Euler (cmm-euler2d.cu), 1: 1D-1V Vlasov-Poisson (cmm-vlasov1d.cu) */
 } cmmo_euler_params;
 
 typedef struct cmmo_euler_result {

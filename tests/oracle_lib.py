@@ -33,7 +33,7 @@ class EulerParams(C.Structure):
                 ("ic", C.c_int), ("ic_params", C.c_double * 10),
                 ("discrete", C.c_int), ("g_disc", Grid), ("disc_herm", C.POINTER(C.c_double)),
                 ("stack_capacity", C.c_int), ("n_particles", C.c_int), ("particle_scheme", C.c_int),
-                ("particles", C.POINTER(C.c_double)), ("max_steps", C.c_int)]
+                ("particles", C.POINTER(C.c_double)), ("max_steps", C.c_int), ("sim", C.c_int)]
 
 
 class EulerResult(C.Structure):
@@ -184,11 +184,25 @@ def ic_random_spectrum(n, seed):
     return out
 
 
+def vp_psi_from_f(gm, gv, gf, gp, cx, cy, f_herm, molly=0):
+    out = np.zeros(4 * gp.nx * gp.ny)
+    rc = lib().cmmo_vp_psi_from_f(gm, gv, gf, gp, ptr(cx), ptr(cy), ptr(f_herm), molly, ptr(out))
+    assert rc == 0, rc
+    return out
+
+
+def vp_grid(n, nv=None):
+    """Vlasov-Poisson grid on x in [0, 4 pi), v in [-2.5 pi, 2.5 pi) (the reference's bounds
+    {0, 4, -2.5, 2.5} times pi)."""
+    return grid(n, nv, (0.0, 2 * TWO_PI, -1.25 * TWO_PI, 1.25 * TWO_PI))
+
+
 def euler_run(g_coarse, g_fine, g_psi, g_vort, dt, final_time, tin=30, muo=1, l_order=3, lihi=1,
               eps=1e-3, threshold=1e-4, fcut=128.0, molly=0, skip_remapping=0, ic=0, ic_params=None,
               disc=None, g_disc=None, stack_capacity=1000, particles=None, particle_scheme=30,
-              max_steps=0, want=("incomp", "time", "map_counter")):
+              max_steps=0, want=("incomp", "time", "map_counter"), sim=0):
     p = EulerParams()
+    p.sim = sim
     p.g_coarse, p.g_fine, p.g_psi, p.g_vort = g_coarse, g_fine, g_psi, g_vort
     p.dt, p.final_time = dt, final_time
     p.time_integration_num, p.map_update_order_num, p.l_order = tin, muo, l_order

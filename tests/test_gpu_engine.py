@@ -109,3 +109,43 @@ def test_remapping_11b(dev):
     assert abs(energy - fx["Mesure/Energy"][4]) <= 1e-10 * fx["Mesure/Energy"][4]
     assert abs(enstrophy - fx["Mesure/Enstrophy"][4]) <= 1e-10 * fx["Mesure/Enstrophy"][4]
     e.close()
+
+
+def test_vlasov_poisson_against_oracle(dev):
+    """The 1D-1V Vlasov-Poisson loop (nonlinear Landau damping, map 64^2, fine = psi = vort 128^2
+    on [0, 4 pi) x [-2.5 pi, 2.5 pi), RK3, dt 1/16, T 1, remap at E_inc > 1e-4) on the GPU against
+    the oracle's run: the same remap steps, E_inc to 1e-10 relative, psi to 1e-10 relative L-inf;
+    diagnostics against the closed-form initial values."""
+    import cmm
+    gc, gf = cmm.vp_grid(64), cmm.vp_grid(128)
+    dt, T = 1.0 / 16, 1.0
+    e = cmm.Euler(gc, gf, gf, gf, dt, initial_condition="nonlin_landau_damping", incomp_threshold=1e-4,
+                  stack_capacity=32, simulation_type="cmm_vlasov_poisson_1d")
+    warm = e.init()
+    d0 = e.vp_diagnostics()
+    infos = [e.step(dt) for _ in range(int(round(T / dt)))]
+    einc = np.array([i.incomp for i in infos])
+    counter = np.array([i.stack_size for i in infos], dtype=float)
+    psi = e.field("psi")
+    e.close()
+    r = O.euler_run(O.vp_grid(64), O.vp_grid(128), O.vp_grid(128), O.vp_grid(128), dt=dt, final_time=T, ic=202,
+                    sim=1, threshold=1e-4, stack_capacity=32, want=("incomp", "time", "map_counter", "psi"))
+    print("warm-up", warm, list(r["warmup_incomp"]))
+    print("E_inc", einc, r["incomp"])
+    print("maps", counter, r["map_counter"])
+    assert counter.max() >= 1, "the case should remap at least once"
+    assert np.array_equal(counter, r["map_counter"])
+    assert np.max(np.abs(einc - r["incomp"]) / np.abs(r["incomp"])) <= 1e-10
+    for a, b in zip(warm, r["warmup_incomp"]):
+        assert abs(a - b) <= 1e-10 * abs(b)
+    err = rel_linf(psi, r["psi"])
+    print("psi rel Linf", err)
+    assert err <= 1e-10
+    # initial diagnostics: mass 4 pi (Maxwellian over [-2.5 pi, 2.5 pi) times (1 + 0.5 cos(x/2))),
+    # kinetic energy 2 pi, potential energy 1/2 int (0.5 sin(x/2)/0.5 ... ) = 1/2 * 4 pi/2 * 1 = pi,
+    # momentum 0, first field mode |E_1| = 0.5
+    Etot, Ekin, Epot, mass, mom = d0[:5]
+    print("diagnostics at t=0", d0)
+    assert abs(mass - 4 * np.pi) < 1e-9 and abs(Ekin - 2 * np.pi) < 1e-9
+    assert abs(Epot - np.pi) < 1e-9 and abs(Etot - (Ekin + Epot)) < 1e-12 and abs(mom) < 1e-9
+    assert abs(d0[5] - 0.5) < 1e-9 and max(d0[6:9]) < 1e-9

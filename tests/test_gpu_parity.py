@@ -218,3 +218,33 @@ def test_particles_rk3(dev, base):
     e = rel_linf(host(d), ref)
     print("particles rel Linf", e)
     assert e <= TOL
+
+
+@pytest.fixture(scope="module")
+def vp_base():
+    """Oracle Vlasov-Poisson state after 4 steps of nonlinear Landau damping (map 64^2, fine 128^2)."""
+    g64, g128 = O.vp_grid(64), O.vp_grid(128)
+    return O.euler_run(g64, g128, g128, g128, dt=0.1, final_time=0.4, ic=202, skip_remapping=1, sim=1,
+                       want=("chix", "chiy", "psi", "w_herm"))
+
+
+def test_vp_psi_from_f(dev, vp_base):
+    """Vlasov-Poisson psi on the GPU against the oracle: vort grid equal to the psi grid and half of
+    it (zero padding), mollifiers 0/4/8, through the deformed map."""
+    import cmm
+    gm, gf = cmm.vp_grid(64), cmm.vp_grid(128)
+    ogm, ogf = O.vp_grid(64), O.vp_grid(128)
+    mx, my, W = vp_base["chix"], vp_base["chiy"], vp_base["w_herm"]
+    for nv in (128, 64):
+        for molly in (0, 4, 8):
+            ref = O.vp_psi_from_f(ogm, O.vp_grid(nv), ogf, ogf, mx, my, W, molly)
+            out = dev_array(np.zeros(4 * 128 * 128))
+            cmm.evaluate_potential_from_density_hermite(gm, cmm.vp_grid(nv), gf, gf, dev_array(mx), dev_array(my),
+                                                        dev_array(W), out, molly)
+            errs = hermite_errors(host(out), ref, 128 * 128, 128 / 4.0)
+            print(f"vp psi vort {nv} molly {molly} per-plane errors {errs}")
+            assert max(errs) <= TOL
+    # a vort grid finer than the psi grid is refused, as in the reference (error 1013)
+    with pytest.raises(cmm.CmmError):
+        cmm.evaluate_potential_from_density_hermite(gm, gf, gf, cmm.vp_grid(64), dev_array(mx), dev_array(my),
+                                                    dev_array(W), dev_array(np.zeros(4 * 64 * 64)), 0)

@@ -313,3 +313,82 @@ def test_particles_rk3_third_order():
         out = O.particles_advect(g, psi, 3, [0.375, 0.75, -0.125], dt, 30, p0)
         errs.append(np.max(np.abs(out - ref(p0, dt))))
     assert errs[1] < errs[0] / 10        # local error of a third-order step scales like dt^4
+
+
+def vp_bump_H(n, v0, hv, Lv):
+    """H(v) of compute_periodification_function computed with numpy's FFT: the centred bump
+    divided by its maximum, then H_hat = S_hat / (n ky^2), DC 0."""
+    v = v0 + np.arange(n) * hv
+    sg = 0.5 - 0.5 * np.tanh(2 * np.pi * (np.abs(v) - 0.4 * Lv) / (0.1 * Lv))
+    sg -= sg.mean()
+    sg /= sg.max()
+    S = np.fft.rfft(sg) / n
+    ky = 2 * np.pi / (hv * n) * np.arange(n // 2 + 1)
+    S[1:] /= ky[1:] ** 2
+    S[0] = 0
+    return np.fft.irfft(S, n) * n, sg
+
+
+def test_vp_psi_closed_form():
+    """f(x, v) = A(x)/Lv on the vort grid (64^2), identity map, psi grid 128^2:
+    I(x) = hv (f(x,v0) + sum_v f) - 1 = (1 + 1/Nv) A(x) - 1, so with
+    A = 1 + eps cos(k1 x) + c cos(pi ix) (the vort grid's Nyquist mode)
+    phi = (1 + 1/Nv) [eps cos(k1 x)/k1^2 + 2 c cos(kN x)/kN^2] on the psi grid: +1/k^2 (the
+    reference divides by +k^2), the double-counted first row (1 + 1/Nv), and the Nyquist bin copied
+    into an interior bin by the zero padding (factor 2).  psi = phi(x) - H(v) with H'' = -sigma_c."""
+    nv, npsi = 64, 128
+    gv, gp = O.vp_grid(nv), O.vp_grid(npsi)
+    Lx, Lv = gv.x1 - gv.x0, gv.y1 - gv.y0
+    hx = Lx / float(np.float32(nv))
+    hv = Lv / float(np.float32(nv))
+    eps, c = 0.1, 0.03
+    ix = np.arange(nv)
+    A = 1 + eps * np.cos(2 * np.pi / Lx * ix * hx) + c * np.cos(np.pi * ix)
+    f = np.tile(A / Lv, (nv, 1))
+    F = O.hermite_from_real(gv, f.ravel())
+    cx, cy = O.identity_map(gv)
+    psi = O.vp_psi_from_f(gv, gv, gv, gp, cx, cy, F)
+    n = npsi * npsi
+    hxp = Lx / float(np.float32(npsi))
+    hvp = Lv / float(np.float32(npsi))
+    xp = np.arange(npsi) * hxp
+    k1, kN = 2 * np.pi / Lx, 2 * np.pi / Lx * (nv // 2)
+    fac = 1 + 1 / nv
+    phi = fac * (eps * np.cos(k1 * xp) / k1 ** 2 + 2 * c * np.cos(kN * xp) / kN ** 2)
+    dphi = -fac * (eps * np.sin(k1 * xp) / k1 + 2 * c * np.sin(kN * xp) / kN)
+    H, sg = vp_bump_H(npsi, gp.y0, hvp, Lv)
+    expect = phi[None, :] - H[:, None]
+    assert np.max(np.abs(psi[:n] - expect.ravel())) < 1e-12
+    assert np.max(np.abs(psi[n:2 * n] - np.tile(dphi, npsi))) < 1e-11
+    # d2H/dv2 = -sigma_c: spectral second derivative of -H gives back the centred bump
+    Hk = np.fft.rfft(H)
+    kv = 2 * np.pi / (hvp * npsi) * np.arange(npsi // 2 + 1)
+    d2 = np.fft.irfft(-(kv ** 2) * Hk, npsi)
+    assert np.max(np.abs(-d2 - sg)) < 1e-9
+    # the same density on equal grids: no padding, the Nyquist mode keeps its single weight
+    psi_eq = O.vp_psi_from_f(gv, gv, gv, gv, cx, cy, F)
+    x = ix * hx
+    phi_eq = fac * (eps * np.cos(k1 * x) / k1 ** 2 + c * np.cos(kN * x) / kN ** 2)
+    H_eq, _ = vp_bump_H(nv, gv.y0, hv, Lv)
+    assert np.max(np.abs(psi_eq[:nv * nv] - (phi_eq[None, :] - H_eq[:, None]).ravel())) < 1e-12
+
+
+def test_vp_psi_landau_density():
+    """Landau initial condition through the identity map: rho - 1 = 0.05 cos(x/2) (the Maxwellian
+    over [-2.5 pi, 2.5 pi) integrates to 1 within 1e-13), so phi = 0.05/0.25 cos(x/2) and
+    dpsi/dx = -0.1 sin(x/2)."""
+    n = 128
+    g = O.vp_grid(n)
+    cx, cy = O.identity_map(g)
+    xy = O.compose_positions(g, g, cx, cy, None, None, 0)
+    f = O.sample_init(g, xy, 200)
+    F = O.hermite_from_real(g, f)
+    psi = O.vp_psi_from_f(g, g, g, g, cx, cy, F)
+    h = (g.x1 - g.x0) / float(np.float32(n))
+    x = np.arange(n) * h
+    nn = n * n
+    dpsi = psi[nn:2 * nn].reshape(n, n)
+    assert np.max(np.abs(dpsi - (-0.1 * np.sin(0.5 * x))[None, :])) < 1e-11
+    # a vort grid finer than the psi grid is rejected (the reference's error 1013)
+    with pytest.raises(AssertionError):
+        O.vp_psi_from_f(g, g, g, O.vp_grid(64), cx, cy, F)
