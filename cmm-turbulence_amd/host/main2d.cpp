@@ -24,12 +24,10 @@ int main(int argc, char **argv) {
             return 0;
         }
     }
-    if (s.simulation_type == "cmm_euler_2d") return cmmhost::run_euler(s);
-    if (s.simulation_type == "cmm_vlasov_poisson_1d") {
-        std::cout << "Simulation type cmm_vlasov_poisson_1d is not provided by this engine yet" << std::endl;
-        return 1;
-    }
+    // both loops share the driver (Cuda-CMM/src/main2d.cpp:92-95 dispatches to cuda_euler_2d or cuda_vlasov_1d)
+    if (s.simulation_type == "cmm_euler_2d" || s.simulation_type == "cmm_vlasov_poisson_1d") return cmmhost::run_euler(s);
     std::cout << "Simulation type not recognized, currently supported:" << std::endl;
     std::cout << "\tcmm_euler_2d" << std::endl;
+    std::cout << "\tcmm_vlasov_poisson_1d" << std::endl;
     return 0;
 }
