@@ -96,7 +96,7 @@ def lib():
                                              C.POINTER(AdvectCoeffs)]
         L.cmm_map_init_identity.argtypes = [Grid, VP, VP, VP]
         L.cmm_advect_workspace_bytes.restype = C.c_size_t
-        L.cmm_advect_workspace_bytes.argtypes = [Grid, C.POINTER(AdvectCoeffs)]
+        L.cmm_advect_workspace_bytes.argtypes = [Grid, Grid, C.POINTER(AdvectCoeffs)]
         L.cmm_map_advect.argtypes = [Grid, Grid, VP, VP, C.POINTER(VP), C.POINTER(AdvectCoeffs), VP, VP, VP,
                                      C.c_size_t, VP]
         L.cmm_incomp_error.argtypes = [Grid, Grid, VP, VP, DP, VP]
@@ -181,8 +181,8 @@ def map_init_identity(g, chix, chiy):
     _check(lib().cmm_map_init_identity(g, _p(chix), _p(chiy), _stream()), "cmm_map_init_identity")
 
 
-def advect_workspace_bytes(g_psi, coeffs):
-    return lib().cmm_advect_workspace_bytes(g_psi, C.byref(coeffs))
+def advect_workspace_bytes(g_map, g_psi, coeffs):
+    return lib().cmm_advect_workspace_bytes(g_map, g_psi, C.byref(coeffs))
 
 
 def advect_using_stream_hermite(g_map, g_psi, chix, chiy, psi_levels, coeffs, chix_new, chiy_new, work=None):

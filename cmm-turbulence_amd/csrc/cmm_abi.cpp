@@ -100,9 +100,9 @@ int cmm_map_init_identity(cmm_grid g, double *cx, double *cy, void *stream) {
     return op_map_identity(g, cx, cy, S(stream));
 }
 
-size_t cmm_advect_workspace_bytes(cmm_grid gp, const cmm_advect_coeffs *c) {
-    if (!grid_ok(gp) || !c) return 0;
-    return op_advect_workspace(gp, *c);
+size_t cmm_advect_workspace_bytes(cmm_grid gm, cmm_grid gp, const cmm_advect_coeffs *c) {
+    if (!grid_ok(gm) || !grid_ok(gp) || !c) return 0;
+    return op_advect_workspace(gm, gp, *c);
 }
 
 int cmm_map_advect(cmm_grid gm, cmm_grid gp, const double *cx, const double *cy, const double *const *psi,
@@ -434,8 +434,9 @@ int cmm_euler_create(const cmm_euler_params *p, cmm_euler **out) {
     if (!rc) {
         size_t w = std::max(op_psi_workspace(p->g_vort, p->g_psi, p->freq_cut_psi), op_hermite_workspace(p->g_fine));
         if (p->discrete) w = std::max(w, op_hermite_workspace(p->g_disc));
-        // Lagrange-weighted psi combinations of the map advection (two Hermite fields on g_psi)
-        if (p->l_order >= 2) w = std::max(w, 8 * sizeof(double) * (size_t)e->np);
+        // Lagrange-weighted psi combinations of the map advection (up to three Hermite fields on
+        // g_psi) and the interleaved map
+        if (p->l_order >= 2) w = std::max(w, 12 * sizeof(double) * (size_t)e->np + 8 * sizeof(double) * (size_t)e->nc);
         if (p->sim == CMM_SIM_VLASOV_POISSON_1D) w = std::max(w, op_vp_psi_workspace(p->g_vort, p->g_psi));
         e->work_bytes = w;
         if (hipMalloc(&e->work, w) != hipSuccess) { set_error("hipMalloc workspace"); rc = CMM_ERR_ALLOC; }

@@ -187,6 +187,30 @@ CMM_HD void herm_vel(const DGrid &g, const PsiLevels &P, int nl, double x, doubl
     }
 }
 
+// herm_vel of one field stored node-interleaved (H4[4*i + plane]): the four planes of a corner
+// are one 32-byte record read with two 16-byte loads; same arithmetic as herm_vel.
+CMM_HD void herm_vel_il(const DGrid &g, const double *__restrict__ H4, double x, double y, double &ux, double &uy) {
+    Cell c;
+    find_cell(g, x, y, c);
+    double bx[4], by[4], dbx[4], dby[4], v[16];
+    basis(c.dx, bx);
+    basis(c.dy, by);
+    basis_d(c.dx, dbx);
+    basis_d(c.dy, dby);
+    const int idx[4] = {c.i00, c.i10, c.i01, c.i11};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const double2 *r = reinterpret_cast<const double2 *>(H4 + 4 * (long)idx[q]);
+        const double2 a = r[0], b = r[1];
+        v[q] = a.x;
+        v[4 + q] = a.y;
+        v[8 + q] = b.x;
+        v[12 + q] = b.y;
+    }
+    uy = -hsum_v(v, dbx, by, g.hx, g.hy) * g.ihx;
+    ux = hsum_v(v, bx, dby, g.hx, g.hy) * g.ihy;
+}
+
 // Map evaluation with periodic jumps, Cuda-CMM/src/numerical/cmm-hermite.cu:446-460
 CMM_HD void diffeo(const DGrid &g, const double *__restrict__ Hx, const double *__restrict__ Hy, double x, double y,
                    double &xo, double &yo) {
@@ -216,6 +240,29 @@ CMM_HD void diffeo_vals(const DGrid &g, const Cell &c, const double vx[16], cons
 }
 
 CMM_HD bool same_cell(const Cell &a, const Cell &b) { return a.i00 == b.i00 && a.wx0 == b.wx0 && a.wy0 == b.wy0; }
+
+// diffeo() on a map stored node-interleaved (M8[8*i + p]: planes of X, then of Y), each corner one
+// 64-byte record read with four 16-byte loads; same arithmetic as diffeo.
+CMM_HD void diffeo_il(const DGrid &g, const double *__restrict__ M8, double x, double y, double &xo, double &yo) {
+    Cell c;
+    find_cell(g, x, y, c);
+    double vx[16], vy[16];
+    const int idx[4] = {c.i00, c.i10, c.i01, c.i11};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const double2 *r = reinterpret_cast<const double2 *>(M8 + 8 * (long)idx[q]);
+        const double2 a = r[0], b = r[1], d = r[2], e = r[3];
+        vx[q] = a.x;
+        vx[4 + q] = a.y;
+        vx[8 + q] = b.x;
+        vx[12 + q] = b.y;
+        vy[q] = d.x;
+        vy[4 + q] = d.y;
+        vy[8 + q] = e.x;
+        vy[12 + q] = e.y;
+    }
+    diffeo_vals(g, c, vx, vy, xo, yo);
+}
 
 // det(grad X), Cuda-CMM/src/numerical/cmm-hermite.cu:479-503
 CMM_HD double diffeo_det(const DGrid &g, const double *__restrict__ Hx, const double *__restrict__ Hy, double x,

@@ -17,7 +17,7 @@ int workspace(size_t bytes, void **out);
 int scalar_slot(unsigned long long **out);
 
 int op_map_identity(const cmm_grid &g, double *cx, double *cy, hipStream_t st);
-size_t op_advect_workspace(const cmm_grid &gp, const cmm_advect_coeffs &c);
+size_t op_advect_workspace(const cmm_grid &gm, const cmm_grid &gp, const cmm_advect_coeffs &c);
 int op_map_advect(const cmm_grid &gm, const cmm_grid &gp, const double *cx, const double *cy, const PsiLevels &P,
                   const cmm_advect_coeffs &c, double *ncx, double *ncy, void *work, size_t work_bytes,
                   hipStream_t st, hipEvent_t mid = nullptr);

@@ -122,25 +122,20 @@ struct VelLevels {
 
 // VelCombined reads the precombined fields C1 = sum_l L1[l] psi_l and C12 = sum_l L12[l] psi_l
 // (Hermite interpolation is linear in the field, so this equals the weighted velocities up to
-// rounding) with one gather of 16 values per evaluation instead of 16 per level.
+// rounding) and a copy C0 of level 0, all three node-interleaved: one 16-value gather per
+// evaluation instead of one per level, issued as 8 16-byte loads instead of 16 8-byte loads.
+// Only level 0 is read on its own by the combinable schemes.
 struct VelCombined {
     DGrid g;
-    PsiLevels P;
-    const double *C1, *C12;
-    __device__ __forceinline__ void one(const double *H, double x, double y, double &ux, double &uy) const {
-        double u[2];
-        herm_vel(g, PsiLevels{{H, H, H, H}}, 1, x, y, u);
-        ux = u[0];
-        uy = u[1];
-    }
+    const double *C1, *C12, *C0;
     __device__ __forceinline__ void w1(const cmm_advect_coeffs &, double x, double y, double &kx, double &ky) const {
-        one(C1, x, y, kx, ky);
+        herm_vel_il(g, C1, x, y, kx, ky);
     }
     __device__ __forceinline__ void w12(const cmm_advect_coeffs &, double x, double y, double &kx, double &ky) const {
-        one(C12, x, y, kx, ky);
+        herm_vel_il(g, C12, x, y, kx, ky);
     }
-    __device__ __forceinline__ void lev(int l, double x, double y, double &ux, double &uy) const {
-        one(P.p[l], x, y, ux, uy);
+    __device__ __forceinline__ void lev(int, double x, double y, double &ux, double &uy) const {
+        herm_vel_il(g, C0, x, y, ux, uy);
     }
 };
 
@@ -208,36 +203,40 @@ __device__ __forceinline__ void step_backward(const V &vel, const cmm_advect_coe
     }
 }
 
-// C1 = sum_l L1[l] psi_l and (when C12 is set) C12 = sum_l L12[l] psi_l over the 4 Hermite planes,
-// summed in level order like the weighted velocities.  Streaming: each lane moves 4 consecutive
-// doubles per level as two 16-byte loads (n4 is a multiple of 4: 4 planes).
+// Node-interleaved fields for the advection: C1[4j + p] = sum_l L1[l] psi_l[p*n + j], likewise C12
+// with L12 (when set), and C0[4j + p] = psi_0[p*n + j] (when set); sums in level order like the
+// weighted velocities.  One lane per node: plane reads coalesce across lanes, and each lane writes
+// its 32-byte records with 16-byte stores.
 template <int L>
-__global__ void __launch_bounds__(256) k_psi_combine(long n4, PsiLevels P, cmm_advect_coeffs c,
-                                                     double *__restrict__ C1, double *__restrict__ C12) {
-    const long i = 4 * ((long)blockIdx.x * 256 + threadIdx.x);
-    if (i >= n4) return;
-    double2 v[L][2];
+__global__ void __launch_bounds__(256) k_psi_combine(long n, PsiLevels P, cmm_advect_coeffs c, double *__restrict__ C1,
+                                                     double *__restrict__ C12, double *__restrict__ C0) {
+    const long j = (long)blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    double v[L][4];
 #pragma unroll
-    for (int l = 0; l < L; ++l) {
-        const double2 *src = reinterpret_cast<const double2 *>(P.p[l] + i);
-        v[l][0] = src[0];
-        v[l][1] = src[1];
-    }
+    for (int l = 0; l < L; ++l)
+#pragma unroll
+        for (int p = 0; p < 4; ++p) v[l][p] = P.p[l][p * n + j];
     auto comb = [&](const double *w, double *dst) {
         double o[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            double a = w[0] * (q & 1 ? v[0][q >> 1].y : v[0][q >> 1].x);
+        for (int p = 0; p < 4; ++p) {
+            double a = w[0] * v[0][p];
 #pragma unroll
-            for (int l = 1; l < L; ++l) a += w[l] * (q & 1 ? v[l][q >> 1].y : v[l][q >> 1].x);
-            o[q] = a;
+            for (int l = 1; l < L; ++l) a += w[l] * v[l][p];
+            o[p] = a;
         }
-        double2 *d = reinterpret_cast<double2 *>(dst + i);
+        double2 *d = reinterpret_cast<double2 *>(dst + 4 * j);
         d[0] = double2{o[0], o[1]};
         d[1] = double2{o[2], o[3]};
     };
     comb(c.L1, C1);
     if (C12) comb(c.L12, C12);
+    if (C0) {
+        double2 *d = reinterpret_cast<double2 *>(C0 + 4 * j);
+        d[0] = double2{v[0][0], v[0][1]};
+        d[1] = double2{v[0][2], v[0][3]};
+    }
 }
 
 // Map advection with the footpoint stencil, Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:207-298.
@@ -246,10 +245,24 @@ __global__ void __launch_bounds__(256) k_psi_combine(long n4, PsiLevels P, cmm_a
 // cells around 64/G neighbouring nodes.  After the footpoints are traced back, lane j of the group
 // forms output plane j by reading the group's positions with cross-lane shuffles and summing them
 // in footpoint order with the stencil weights, the order of the reference's accumulation.
+// the map node-interleaved for the advection's map evaluations: M8[8j + p] = ChiX plane p,
+// M8[8j + 4 + p] = ChiY plane p
+__global__ void __launch_bounds__(256) k_interleave_map(long n, const double *__restrict__ cx,
+                                                        const double *__restrict__ cy, double *__restrict__ M8) {
+    const long j = (long)blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    double2 *d = reinterpret_cast<double2 *>(M8 + 8 * j);
+    d[0] = double2{cx[j], cx[n + j]};
+    d[1] = double2{cx[2 * n + j], cx[3 * n + j]};
+    d[2] = double2{cy[j], cy[n + j]};
+    d[3] = double2{cy[2 * n + j], cy[3 * n + j]};
+}
+
 template <int TIN, int G, bool COMB>
 __global__ void __launch_bounds__(256) k_map_advect(DGrid gm, DGrid gp, const double *__restrict__ cx,
                                                     const double *__restrict__ cy, PsiLevels P, const double *C1,
-                                                    const double *C12, cmm_advect_coeffs c, double *__restrict__ ncx,
+                                                    const double *C12, const double *C0, const double *M8,
+                                                    cmm_advect_coeffs c, double *__restrict__ ncx,
                                                     double *__restrict__ ncy, int nblocks) {
     constexpr int NPB = 256 / G;  // nodes per block
     const int blk = xcd_block(blockIdx.x, nblocks);
@@ -263,11 +276,13 @@ __global__ void __launch_bounds__(256) k_map_advect(DGrid gm, DGrid gp, const do
         const double xe = gm.x0 + ix * gm.hx + ((k >> 2) + 1) * c.eps * (1 - 2 * ((k >> 1) & 1));
         const double ye = gm.y0 + iy * gm.hy + ((k >> 2) + 1) * c.eps * (1 - 2 * (((k + 1) >> 1) & 1));
         double xf, yf;
-        if constexpr (COMB)
-            step_backward<TIN>(VelCombined{gp, P, C1, C12}, c, xe, ye, xf, yf);
-        else
+        if constexpr (COMB) {
+            step_backward<TIN>(VelCombined{gp, C1, C12, C0}, c, xe, ye, xf, yf);
+            diffeo_il(gm, M8, xf, yf, px, py);
+        } else {
             step_backward<TIN>(VelLevels{gp, P}, c, xe, ye, xf, yf);
-        diffeo(gm, cx, cy, xf, yf, px, py);
+            diffeo(gm, cx, cy, xf, yf, px, py);
+        }
     }
     const int base = (threadIdx.x & 63) - k;  // first lane of the group within the wavefront
     // every lane takes part in every shuffle (the same trip count for all lanes)
@@ -1225,9 +1240,12 @@ static bool advect_needs_c12(const cmm_advect_coeffs &c) {
     return t == 30 || t == 40 || t == 41;
 }
 
-size_t op_advect_workspace(const cmm_grid &gp, const cmm_advect_coeffs &c) {
+static bool advect_needs_c0(const cmm_advect_coeffs &c) { return c.time_integration_num != 10; }
+
+size_t op_advect_workspace(const cmm_grid &gm, const cmm_grid &gp, const cmm_advect_coeffs &c) {
     if (!advect_combinable(c)) return 0;
-    return (advect_needs_c12(c) ? 2 : 1) * sizeof(double) * 4 * (size_t)gp.nx * gp.ny;
+    const int fields = 1 + advect_needs_c12(c) + advect_needs_c0(c);
+    return fields * sizeof(double) * 4 * (size_t)gp.nx * gp.ny + sizeof(double) * 8 * (size_t)gm.nx * gm.ny;
 }
 
 int op_map_advect(const cmm_grid &gm, const cmm_grid &gp, const double *cx, const double *cy, const PsiLevels &P,
@@ -1237,24 +1255,33 @@ int op_map_advect(const cmm_grid &gm, const cmm_grid &gp, const double *cx, cons
     const int nfoot = 4 + 4 * c.map_update_order_num;
     const int G = nfoot <= 4 ? 4 : nfoot <= 8 ? 8 : 16;
     const int nb = (int)(((long)dm.n * G + 255) / 256);
-    const size_t need = op_advect_workspace(gp, c);
+    const size_t need = op_advect_workspace(gm, gp, c);
     const bool comb = work && need > 0 && work_bytes >= need;
-    double *C1 = nullptr, *C12 = nullptr;
+    double *C1 = nullptr, *C12 = nullptr, *C0 = nullptr, *M8 = nullptr;
     if (comb) {
-        const long n4 = 4L * dp.n;
-        C1 = static_cast<double *>(work);
-        C12 = advect_needs_c12(c) ? C1 + n4 : nullptr;
-        const dim3 g((unsigned)((n4 / 4 + 255) / 256));
+        const long n = dp.n, n4 = 4L * n;
+        double *next = static_cast<double *>(work);
+        M8 = next;
+        next += 8L * dm.n;
+        C1 = next;
+        next += n4;
+        if (advect_needs_c12(c)) { C12 = next; next += n4; }
+        if (advect_needs_c0(c)) C0 = next;
+        hipLaunchKernelGGL(k_interleave_map, dim3((unsigned)((dm.n + 255) / 256)), dim3(256), 0, st, (long)dm.n, cx, cy,
+                           M8);
+        if (int rc = launch_check("k_interleave_map")) return rc;
+        const dim3 g((unsigned)((n + 255) / 256));
         switch (c.l_order) {
-        case 2: hipLaunchKernelGGL(k_psi_combine<2>, g, dim3(256), 0, st, n4, P, c, C1, C12); break;
-        case 3: hipLaunchKernelGGL(k_psi_combine<3>, g, dim3(256), 0, st, n4, P, c, C1, C12); break;
-        default: hipLaunchKernelGGL(k_psi_combine<4>, g, dim3(256), 0, st, n4, P, c, C1, C12); break;
+        case 2: hipLaunchKernelGGL(k_psi_combine<2>, g, dim3(256), 0, st, n, P, c, C1, C12, C0); break;
+        case 3: hipLaunchKernelGGL(k_psi_combine<3>, g, dim3(256), 0, st, n, P, c, C1, C12, C0); break;
+        default: hipLaunchKernelGGL(k_psi_combine<4>, g, dim3(256), 0, st, n, P, c, C1, C12, C0); break;
         }
         if (int rc = launch_check("k_psi_combine")) return rc;
     }
     if (mid) (void)hipEventRecord(mid, st);
 #define CMM_ADV(T, GG, CB)                                                                                             \
-    hipLaunchKernelGGL((k_map_advect<T, GG, CB>), dim3(nb), dim3(256), 0, st, dm, dp, cx, cy, P, C1, C12, c, ncx, ncy, nb)
+    hipLaunchKernelGGL((k_map_advect<T, GG, CB>), dim3(nb), dim3(256), 0, st, dm, dp, cx, cy, P, C1, C12, C0, M8, c, ncx, \
+                       ncy, nb)
 #define CMM_ADV_G(T, CB)                                                                                               \
     do {                                                                                                               \
         if (G == 4) CMM_ADV(T, 4, CB);                                                                                 \

@@ -47,10 +47,11 @@ int cmm_map_init_identity(cmm_grid g, double *chix, double *chiy, void *stream);
  * alias chix/chiy.
  * work may be NULL: every footpoint evaluation then interpolates each psi level and weights the
  * velocities (the reference's order of operations).  With work_bytes >=
- * cmm_advect_workspace_bytes(g_psi, c) (> 0 for l_order >= 2 and schemes 10, 21, 30, 40, 41) the
- * call first forms the Lagrange-weighted psi combinations in work, once, and interpolates those:
- * one gather per evaluation instead of one per level; equal up to rounding. */
-size_t cmm_advect_workspace_bytes(cmm_grid g_psi, const cmm_advect_coeffs *c);
+ * cmm_advect_workspace_bytes(g_map, g_psi, c) (> 0 for l_order >= 2 and schemes 10, 21, 30, 40, 41)
+ * the call first forms the Lagrange-weighted psi combinations (and copies of level 0 and of the
+ * map) in work, once, node-interleaved, and interpolates those: one gather per evaluation instead
+ * of one per level, in 16-byte loads; equal up to rounding. */
+size_t cmm_advect_workspace_bytes(cmm_grid g_map, cmm_grid g_psi, const cmm_advect_coeffs *c);
 int cmm_map_advect(cmm_grid g_map, cmm_grid g_psi, const double *chix, const double *chiy,
                    const double *const *psi_levels, const cmm_advect_coeffs *c, double *chix_new,
                    double *chiy_new, void *work, size_t work_bytes, void *stream);

@@ -51,12 +51,12 @@ def test_argument_errors_without_device():
     arr = (C.c_void_p * 4)(1, 1, 1, 1)
     assert lib.cmm_map_advect(g, g, C.c_void_p(8), C.c_void_p(16), arr, C.byref(c), C.c_void_p(24),
                               C.c_void_p(32), None, 0, None) == -4
-    # workspace sizes: two combined Hermite fields for RK3/RK4, one for Euler/RK2, none for AB2 and
-    # RK3-mod (they read level 1 on its own) or a single level
-    for tin, L, fields in ((30, 3, 2), (40, 3, 2), (41, 4, 2), (10, 3, 1), (21, 2, 1), (20, 3, 0), (31, 3, 0),
+    # workspace sizes: the combined fields C1 (all), C12 (RK3/RK4 variants) and a copy of level 0
+    # (all but Euler), none for AB2 and RK3-mod (they read level 1 on its own) or a single level
+    for tin, L, fields in ((30, 3, 3), (40, 3, 3), (41, 4, 3), (10, 3, 1), (21, 2, 2), (20, 3, 0), (31, 3, 0),
                            (30, 1, 0)):
         c.time_integration_num, c.l_order = tin, L
-        assert lib.cmm_advect_workspace_bytes(g, C.byref(c)) == fields * 4 * 8 * 32 * 32
+        assert lib.cmm_advect_workspace_bytes(g, g, C.byref(c)) == (fields * 4 * 8 * 32 * 32 + 64 * 32 * 32 if fields else 0)
 
 
 def test_lagrange_coefficients_match_closed_form():

@@ -171,8 +171,11 @@ def test_map_advect_schemes(dev, base, ratio):
                 oc = O.advect_coeffs(t, dts, 0, 3, tin, muo, 1e-3)
                 c = cmm.advect_coeffs(t, dts, 0, 3, tin, muo, 1e-3)
                 rx, ry = O.map_advect(ogc, ogp, base["chix"], base["chiy"], psi, oc)
-                wb = cmm.advect_workspace_bytes(gp, c)
-                assert wb == (0 if tin in (20, 31) else (2 if tin in (30, 40, 41) else 1) * 32 * npsi * npsi)
+                wb = cmm.advect_workspace_bytes(gc, gp, c)
+                # C1, C12 (RK3/RK4 variants) and the copy of level 0 (all but Euler), 4 planes each,
+                # and the interleaved map (8 values per map node)
+                nf = 3 if tin in (30, 40, 41) else 2 if tin == 21 else 1
+                assert wb == (0 if tin in (20, 31) else nf * 32 * npsi * npsi + 64 * GC * GC)
                 for work in (None, torch.zeros(max(wb, 8) // 8, dtype=torch.float64, device="cuda")):
                     ox = dev_array(np.zeros(4 * n))
                     oy = dev_array(np.zeros(4 * n))
