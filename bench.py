@@ -44,8 +44,9 @@ def algorithmic(slot, c, kx_max, n_stack=0):
         # each) + map out
         return 64 * nc + 32 * 3 * npsi + 64 * nc, 9.2e3 * nc
     if slot == "psi_combine":
-        # L psi levels read, two combined fields written (4 planes each)
-        return 32 * (L + 2) * npsi, 2 * 2 * L * 4 * npsi
+        # L psi levels read, C1, C12 and the level-0 copy written (4 planes each); the map read and
+        # written interleaved (the slot times both kernels)
+        return 32 * (L + 3) * npsi + 2 * 64 * nc, 2 * 2 * L * 4 * npsi
     if slot == "incomp":
         return 64 * nc, 320.0 * nf
     if slot == "psi_row_r2c":
@@ -110,7 +111,7 @@ def cpu_baseline(c, threads, budget_s=12.0):
 
 # bench slot -> kernel-name pattern of the launches it times (engine psi solve samples omega with SampleVort)
 SLOT_KERNELS = {"advect": "k_map_advect<30, 8, true>", "psi_combine": "k_psi_combine<3>", "incomp": "k_incomp_cells(",
-                "psi_row_r2c": "k_row_r2c<cmm::SampleVort>", "psi_col": "k_col(", "psi_row_c2r": "k_row_c2r4(",
+                "psi_row_r2c": "k_row_r2c<cmm::SampleVortT<true>", "psi_col": "k_col(", "psi_row_c2r": "k_row_c2r4(",
                 "particles": "k_particles("}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "cmm_euler_c2_hbm_traffic.json")
 FETCH_SCALE_DEFAULT, WRITE_SCALE_DEFAULT = 2.0, 1.0

@@ -235,6 +235,7 @@ struct cmm_euler {
     double *psi_buf[4] = {nullptr, nullptr, nullptr, nullptr};
     double *lvl[4] = {nullptr, nullptr, nullptr, nullptr};  // level -> buffer, newest first
     double *W = nullptr, *disc = nullptr, *stack = nullptr, *tmp = nullptr, *tmp4 = nullptr;
+    double *W4 = nullptr;  // W node-interleaved (Euler: read by the psi sampler)
     int nstack = 0;
     void *work = nullptr;
     size_t work_bytes = 0;
@@ -264,9 +265,12 @@ int dalloc(double **p, size_t n) {
 
 
 int engine_translate(cmm_euler *e) {
-    return op_translate_ic(e->p.g_coarse, e->p.g_fine, e->cx, e->cy, e->stack, e->nstack, e->p.ic, e->p.ic_params,
-                           e->p.discrete, e->p.discrete ? e->p.g_disc : e->p.g_fine, e->disc, e->W, e->work,
-                           e->work_bytes, e->st);
+    if (int rc = op_translate_ic(e->p.g_coarse, e->p.g_fine, e->cx, e->cy, e->stack, e->nstack, e->p.ic,
+                                 e->p.ic_params, e->p.discrete, e->p.discrete ? e->p.g_disc : e->p.g_fine, e->disc,
+                                 e->W, e->work, e->work_bytes, e->st))
+        return rc;
+    // the psi sampler reads the fine vorticity node-interleaved (one 32-byte record per corner)
+    return e->W4 ? op_interleave4(e->p.g_fine, e->W, e->W4, e->st) : CMM_OK;
 }
 
 int engine_psi(cmm_euler *e, double *out, hipEvent_t *ev = nullptr) {
@@ -280,7 +284,7 @@ int engine_psi(cmm_euler *e, double *out, hipEvent_t *ev = nullptr) {
         return rc;
     }
     return op_psi_from_vort(e->p.g_coarse, e->p.g_vort, e->p.g_fine, e->p.g_psi, e->cx, e->cy, e->W,
-                            e->p.molly_stencil, e->p.freq_cut_psi, out, e->work, e->work_bytes, e->st, ev);
+                            e->p.molly_stencil, e->p.freq_cut_psi, out, e->work, e->work_bytes, e->st, ev, e->W4);
 }
 
 int engine_advect(cmm_euler *e, const double *t, const double *dt, int loop_ctr, int L, int tin) {
@@ -430,6 +434,7 @@ int cmm_euler_create(const cmm_euler_params *p, cmm_euler **out) {
         e->lvl[l] = e->psi_buf[l];
     }
     if (!rc) rc = dalloc(&e->W, 4 * e->nf);
+    if (!rc && p->sim == CMM_SIM_EULER_2D) rc = dalloc(&e->W4, 4 * e->nf);
     if (!rc && p->stack_capacity > 0) rc = dalloc(&e->stack, (size_t)8 * e->nc * p->stack_capacity);
     if (!rc) {
         size_t w = std::max(op_psi_workspace(p->g_vort, p->g_psi, p->freq_cut_psi), op_hermite_workspace(p->g_fine));
@@ -455,7 +460,7 @@ int cmm_euler_create(const cmm_euler_params *p, cmm_euler **out) {
 int cmm_euler_destroy(cmm_euler *e) {
     if (!e) return CMM_OK;
     if (e->st) (void)hipStreamSynchronize(e->st);
-    double *bufs[] = {e->cx, e->cy, e->ncx, e->ncy, e->W, e->disc, e->stack, e->tmp, e->tmp4,
+    double *bufs[] = {e->cx, e->cy, e->ncx, e->ncy, e->W, e->W4, e->disc, e->stack, e->tmp, e->tmp4,
                       e->psi_buf[0], e->psi_buf[1], e->psi_buf[2], e->psi_buf[3]};
     for (double *b : bufs)
         if (b) (void)hipFree(b);

@@ -165,6 +165,26 @@ CMM_HD double herm_eval(const DGrid &g, const double *__restrict__ H, double x, 
     return hsum_v(v, bx, by, g.hx, g.hy);
 }
 
+// herm_eval of a field stored node-interleaved (H4[4*i + plane]); same arithmetic as herm_eval
+CMM_HD double herm_eval_il(const DGrid &g, const double *__restrict__ H4, double x, double y) {
+    Cell c;
+    find_cell(g, x, y, c);
+    double bx[4], by[4], v[16];
+    basis(c.dx, bx);
+    basis(c.dy, by);
+    const int idx[4] = {c.i00, c.i10, c.i01, c.i11};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const double2 *r = reinterpret_cast<const double2 *>(H4 + 4 * (long)idx[q]);
+        const double2 a = r[0], b = r[1];
+        v[q] = a.x;
+        v[4 + q] = a.y;
+        v[8 + q] = b.x;
+        v[12 + q] = b.y;
+    }
+    return hsum_v(v, bx, by, g.hx, g.hy);
+}
+
 // Velocity (d psi/dy, -d psi/dx) of up to four psi levels at one point,
 // Cuda-CMM/src/numerical/cmm-hermite.cu:396-420.  u[2l] = dpsi_l/dy, u[2l+1] = -dpsi_l/dx.
 struct PsiLevels {

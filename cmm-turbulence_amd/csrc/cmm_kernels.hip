@@ -481,24 +481,30 @@ struct SampleReal {
 
 // omega(X(x)) through the current map and the fine Hermite vorticity (mollifier 0/4/8),
 // Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:402-461; positions use the map grid's origin.
-struct SampleVort {
+// IL: the fine vorticity is given node-interleaved (W[4*i + plane]) instead of plane-major.
+template <bool IL>
+struct SampleVortT {
     DGrid gm, gv, gf;
     const double *__restrict__ cx;
     const double *__restrict__ cy;
     const double *__restrict__ W;
     int molly;
+    __device__ __forceinline__ double wval(double x, double y) const {
+        if constexpr (IL) return herm_eval_il(gf, W, x, y);
+        else return herm_eval(gf, W, x, y);
+    }
     __device__ double operator()(int ix, int iy) const {
         const double x = gm.x0 + ix * gv.hx, y = gm.y0 + iy * gv.hy;
         double x2, y2, v;
         if (molly == 4) {
             diffeo(gm, cx, cy, x, y, x2, y2);
-            v = herm_eval(gf, W, x2, y2) / 3.0;
+            v = wval(x2, y2) / 3.0;
             for (int m = 0; m < 4; ++m) {
                 // the reference's (i_molly>>1+1)&1 parses as (i_molly>>2)&1
                 x2 = x + gv.hx / 2.0 * ((m >> 2) & 1) * (-1 + 2 * (m & 1));
                 y2 = y + gv.hy / 2.0 * ((m >> 1) & 1) * (-1 + 2 * (m & 1));
                 diffeo(gm, cx, cy, x2, y2, x2, y2);
-                v += herm_eval(gf, W, x2, y2) / 6.0;
+                v += wval(x2, y2) / 6.0;
             }
         } else if (molly == 8) {
             v = 0;
@@ -507,11 +513,11 @@ struct SampleVort {
                 y2 = y + gv.hy * (-1 + m / 3) / 2.0;
                 diffeo(gm, cx, cy, x2, y2, x2, y2);
                 // the reference's (1 + (i%3)&1) parses as ((1 + i%3) & 1)
-                v += ((1 + (m % 3)) & 1) * ((1 + (m / 3)) & 1) * herm_eval(gf, W, x2, y2) / 16.0;
+                v += ((1 + (m % 3)) & 1) * ((1 + (m / 3)) & 1) * wval(x2, y2) / 16.0;
             }
         } else {
             diffeo(gm, cx, cy, x, y, x2, y2);
-            v = herm_eval(gf, W, x2, y2);
+            v = wval(x2, y2);
         }
         return v;
     }
@@ -521,6 +527,11 @@ struct SampleVort {
     // operator().
     __device__ void quad(int x, int ya, int yb, cd *s) const;
 };
+using SampleVort = SampleVortT<false>;
+using SampleVortIL = SampleVortT<true>;
+
+template <class S>
+constexpr bool is_vort_sampler = std::is_same_v<S, SampleVort> || std::is_same_v<S, SampleVortIL>;
 
 // Initial condition at X_1(...X_M(X_cur(x))): the current map then the stack from newest to
 // oldest, wrapping after each layer (Cuda-CMM/src/simulation/cmm-simulation-host.cu:112-152,
@@ -557,7 +568,8 @@ struct SampleCompose {
     }
 };
 
-__device__ void SampleVort::quad(int x, int ya, int yb, cd *s) const {
+template <bool IL>
+__device__ void SampleVortT<IL>::quad(int x, int ya, int yb, cd *s) const {
     if (molly != 0) {
         for (int i = 0; i < 4; ++i) s[sw(x + i)] = cd{(*this)(x + i, ya), (*this)(x + i, yb)};
         return;
@@ -578,7 +590,7 @@ __device__ void SampleVort::quad(int x, int ya, int yb, cd *s) const {
         }
         double x2, y2;
         diffeo_vals(gm, c, vx, vy, x2, y2);
-        const double v = herm_eval(gf, W, x2, y2);
+        const double v = wval(x2, y2);
         double *d = &s[sw(x + i)].x;
         d[j] = v;
     }
@@ -586,7 +598,7 @@ __device__ void SampleVort::quad(int x, int ya, int yb, cd *s) const {
 
 template <class S>
 __device__ __forceinline__ void sample_quad(const S &smp, int x, int ya, int yb, cd *s) {
-    if constexpr (std::is_same_v<S, SampleVort>) {
+    if constexpr (is_vort_sampler<S>) {
         smp.quad(x, ya, yb, s);
     } else {
         for (int i = 0; i < 4; ++i) s[sw(x + i)] = cd{smp(x + i, ya), smp(x + i, yb)};
@@ -636,7 +648,7 @@ __device__ __forceinline__ void wg_fft(cd *s, int n, int logn, const cd *__restr
 // sampler's gathers need more registers than that leaves
 template <class S>
 constexpr int r2c_min_waves() {
-    return std::is_same_v<S, SampleVort> || std::is_same_v<S, SampleCompose> ? 2 : 4;
+    return is_vort_sampler<S> || std::is_same_v<S, SampleCompose> ? 2 : 4;
 }
 
 template <class S>
@@ -1348,21 +1360,44 @@ size_t op_psi_workspace(const cmm_grid &gv, const cmm_grid &gp, double fcut) {
 
 int op_psi_from_vort(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf, const cmm_grid &gp, const double *cx,
                      const double *cy, const double *W, int molly, double fcut, double *psi, void *work,
-                     size_t work_bytes, hipStream_t st, hipEvent_t *ev) {
+                     size_t work_bytes, hipStream_t st, hipEvent_t *ev, const double *W4) {
     if (int rc = fft_grid_ok(gv, "vort grid")) return rc;
     if (int rc = fft_grid_ok(gp, "psi grid")) return rc;
     SpecPlan sp = plan_spec(gv, gp, 1, fcut);
     void *w;
     if (int rc = get_work(work, work_bytes, sp.t_bytes + 2 * sp.g_bytes + 256, &w)) return rc;
+    auto fill = [&](auto &s, const double *field) {
+        s.gm = make_dgrid(gm);
+        s.gv = make_dgrid(gv);
+        s.gf = make_dgrid(gf);
+        s.cx = cx;
+        s.cy = cy;
+        s.W = field;
+        s.molly = molly;
+    };
+    if (W4) {
+        SampleVortIL s;
+        fill(s, W4);
+        return run_spec(sp, s, psi, w, st, ev);
+    }
     SampleVort s;
-    s.gm = make_dgrid(gm);
-    s.gv = make_dgrid(gv);
-    s.gf = make_dgrid(gf);
-    s.cx = cx;
-    s.cy = cy;
-    s.W = W;
-    s.molly = molly;
+    fill(s, W);
     return run_spec(sp, s, psi, w, st, ev);
+}
+
+// node-interleaved copy of a Hermite field: H4[4j + p] = H[p*n + j]
+__global__ void __launch_bounds__(256) k_interleave4(long n, const double *__restrict__ H, double *__restrict__ H4) {
+    const long j = (long)blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    double2 *d = reinterpret_cast<double2 *>(H4 + 4 * j);
+    d[0] = double2{H[j], H[n + j]};
+    d[1] = double2{H[2 * n + j], H[3 * n + j]};
+}
+
+int op_interleave4(const cmm_grid &g, const double *H, double *H4, hipStream_t st) {
+    const long n = (long)g.nx * g.ny;
+    hipLaunchKernelGGL(k_interleave4, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, H, H4);
+    return launch_check("k_interleave4");
 }
 
 size_t op_hermite_workspace(const cmm_grid &g) { return spec_workspace_bytes(g, g, 0, 0.0); }
