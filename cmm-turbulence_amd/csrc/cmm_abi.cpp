@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -240,6 +241,8 @@ struct cmm_euler {
     void *work = nullptr;
     size_t work_bytes = 0;
     unsigned long long *slot = nullptr;
+    unsigned long long *h_slot = nullptr;  // pinned host copy of slot
+    hipEvent_t ev_inc = nullptr;           // set after the E_inc copy of a step
     std::vector<double *> part;
     std::vector<int> part_n, part_scheme;
     std::vector<double> tv, dtv;
@@ -247,7 +250,13 @@ struct cmm_euler {
     bool initialised = false;
     // per-kernel event timing: [advect, incomp, psi_row, psi_col, psi_c2r, particles, remap, psi_combine]
     bool prof = false;
-    hipEvent_t ev[12] = {};
+    // event sets of the last kProfSets steps; a set's intervals are read when the set comes round
+    // again (its events long complete), so profiling never makes the host wait inside a step
+    static constexpr int kProfSets = 4;
+    hipEvent_t evs[kProfSets][12] = {};
+    hipEvent_t *ev = evs[0];
+    int cur = 0;  // index of ev in evs
+    std::vector<std::array<int, 3>> pend[kProfSets];  // (slot, from, to) still to be read
     double ms[CMM_PROF_SLOTS] = {};
     int cnt[CMM_PROF_SLOTS] = {};
 };
@@ -321,13 +330,23 @@ void prof_mark(cmm_euler *e, int idx) {
     if (e->prof) (void)hipEventRecord(e->ev[idx], e->st);
 }
 
+// queue the interval [ev a, ev b] of the current step for slot `phase`
 void prof_acc(cmm_euler *e, int phase, int a, int b) {
     if (!e->prof) return;
-    float m = 0;
-    if (hipEventSynchronize(e->ev[b]) == hipSuccess && hipEventElapsedTime(&m, e->ev[a], e->ev[b]) == hipSuccess) {
-        e->ms[phase] += m;
-        e->cnt[phase] += 1;
+    e->pend[e->cur].push_back({phase, a, b});
+}
+
+// read the queued intervals of event set s
+void prof_drain(cmm_euler *e, int s) {
+    for (const auto &q : e->pend[s]) {
+        float m = 0;
+        if (hipEventSynchronize(e->evs[s][q[2]]) == hipSuccess &&
+            hipEventElapsedTime(&m, e->evs[s][q[1]], e->evs[s][q[2]]) == hipSuccess) {
+            e->ms[q[0]] += m;
+            e->cnt[q[0]] += 1;
+        }
     }
+    e->pend[s].clear();
 }
 
 // Random-spectrum field (no reference counterpart), as a column-major half spectrum:
@@ -447,7 +466,14 @@ int cmm_euler_create(const cmm_euler_params *p, cmm_euler **out) {
         if (hipMalloc(&e->work, w) != hipSuccess) { set_error("hipMalloc workspace"); rc = CMM_ERR_ALLOC; }
     }
     if (!rc && hipMalloc(&e->slot, 256) != hipSuccess) { set_error("hipMalloc slot"); rc = CMM_ERR_ALLOC; }
-    for (int i = 0; i < 12 && !rc; ++i) rc = hip_check(hipEventCreate(&e->ev[i]), "hipEventCreate");
+    if (!rc && hipHostMalloc((void **)&e->h_slot, 256, hipHostMallocDefault) != hipSuccess) {
+        e->h_slot = nullptr;
+        set_error("hipHostMalloc slot");
+        rc = CMM_ERR_ALLOC;
+    }
+    if (!rc) rc = hip_check(hipEventCreateWithFlags(&e->ev_inc, hipEventDisableTiming), "hipEventCreate");
+    for (int s = 0; s < cmm_euler::kProfSets && !rc; ++s)
+        for (int i = 0; i < 12 && !rc; ++i) rc = hip_check(hipEventCreate(&e->evs[s][i]), "hipEventCreate");
     if (!rc && p->discrete) rc = engine_setup_discrete(e);
     if (rc) {
         cmm_euler_destroy(e);
@@ -468,8 +494,11 @@ int cmm_euler_destroy(cmm_euler *e) {
         if (b) (void)hipFree(b);
     if (e->work) (void)hipFree(e->work);
     if (e->slot) (void)hipFree(e->slot);
-    for (int i = 0; i < 12; ++i)
-        if (e->ev[i]) (void)hipEventDestroy(e->ev[i]);
+    if (e->h_slot) (void)hipHostFree(e->h_slot);
+    if (e->ev_inc) (void)hipEventDestroy(e->ev_inc);
+    for (int s = 0; s < cmm_euler::kProfSets; ++s)
+        for (int i = 0; i < 12; ++i)
+            if (e->evs[s][i]) (void)hipEventDestroy(e->evs[s][i]);
     if (e->st) (void)hipStreamDestroy(e->st);
     delete e;
     return CMM_OK;
@@ -557,17 +586,35 @@ int cmm_euler_step(cmm_euler *e, double dt_now, cmm_step_info *info) {
     e->dtv.resize(lc + 2);
     e->tv[lc + 1] = e->tv[lc] + dt_now;
     e->dtv[lc + 1] = dt_now;
+    if (e->prof) {
+        const int s = e->loop % cmm_euler::kProfSets;
+        prof_drain(e, s);
+        e->cur = s;
+        e->ev = e->evs[s];
+    }
     // advection, Cuda-CMM/src/simulation/cmm-euler2d.cu:774-778
     prof_mark(e, 0);
     if (int rc = engine_advect(e, e->tv.data(), e->dtv.data(), e->loop, L, p.time_integration_num)) return rc;
     prof_mark(e, 1);
-    // incompressibility on the fine grid, :796
-    double einc;
-    if (int rc = engine_incomp(e, &einc)) return rc;
+    // incompressibility on the fine grid, :796: the check and the copy of E_inc to pinned host
+    // memory are queued, and the host waits for that copy only after the psi solve is queued too
+    if (int rc = op_incomp_launch(p.g_coarse, p.g_fine, e->cx, e->cy, e->slot, e->st)) return rc;
+    if (int rc = hip_check(hipMemcpyAsync(e->h_slot, e->slot, sizeof(unsigned long long), hipMemcpyDeviceToHost, e->st),
+                           "hipMemcpyAsync"))
+        return rc;
+    if (int rc = hip_check(hipEventRecord(e->ev_inc, e->st), "hipEventRecord")) return rc;
     prof_mark(e, 2);
-    prof_acc(e, 7, 0, 10);
-    prof_acc(e, 0, 10, 1);
-    prof_acc(e, 1, 1, 2);
+    // psi history: the oldest level becomes the newest (same contents as the reference's
+    // copies at :865-868), then the new psi, :871.  It is queued before E_inc is known: without a
+    // remap (the common case) it is the step's psi; a remap below recomputes it afterwards in
+    // stream order, from the reset map and the new fine vorticity, as the reference orders it.
+    double *oldest = e->lvl[L - 1];
+    for (int l = L - 1; l > 0; --l) e->lvl[l] = e->lvl[l - 1];
+    e->lvl[0] = oldest;
+    if (int rc = engine_psi(e, e->lvl[0], e->prof ? &e->ev[5] : nullptr)) return rc;
+    if (int rc = hip_check(hipEventSynchronize(e->ev_inc), "hipEventSynchronize")) return rc;
+    double einc;
+    std::memcpy(&einc, e->h_slot, sizeof(double));
     int remapped = 0, saturated = 0;
     // remap, :814-853 (translate before push; the map is reset afterwards)
     if (einc > p.incomp_threshold && !p.skip_remapping) {
@@ -581,17 +628,11 @@ int cmm_euler_step(cmm_euler *e, double dt_now, cmm_step_info *info) {
             if (int rc = hip_check(hipMemcpyAsync(slotx + 4 * e->nc, e->cy, sizeof(double) * 4 * e->nc, hipMemcpyDeviceToDevice, e->st), "push map")) return rc;
             e->nstack++;
             if (int rc = op_map_identity(p.g_coarse, e->cx, e->cy, e->st)) return rc;
+            if (int rc = engine_psi(e, e->lvl[0])) return rc;
             prof_mark(e, 4);
-            prof_acc(e, 6, 3, 4);
             remapped = 1;
         }
     }
-    // psi history: the oldest level becomes the newest (same contents as the reference's
-    // copies at :865-868), then the new psi, :871
-    double *oldest = e->lvl[L - 1];
-    for (int l = L - 1; l > 0; --l) e->lvl[l] = e->lvl[l - 1];
-    e->lvl[0] = oldest;
-    if (int rc = engine_psi(e, e->lvl[0], e->prof ? &e->ev[5] : nullptr)) return rc;
     // particles, :881 with Cuda-CMM/src/numerical/cmm-particles.cu:63-87
     if (!e->part.empty()) {
         double Lp[4] = {0, 0, 0, 0};
@@ -605,6 +646,10 @@ int cmm_euler_step(cmm_euler *e, double dt_now, cmm_step_info *info) {
         }
     }
     prof_mark(e, 9);
+    prof_acc(e, 7, 0, 10);
+    prof_acc(e, 0, 10, 1);
+    prof_acc(e, 1, 1, 2);
+    if (remapped) prof_acc(e, 6, 3, 4);
     prof_acc(e, 2, 5, 6);
     prof_acc(e, 3, 6, 7);
     prof_acc(e, 4, 7, 8);
@@ -741,6 +786,7 @@ int cmm_euler_state(cmm_euler *e, int *steps, double *t, int *stack_size) {
 
 int cmm_euler_profile(cmm_euler *e, int enable) {
     if (!e) { set_error("cmm_euler_profile: null engine"); return CMM_ERR_ARG; }
+    for (int s = 0; s < cmm_euler::kProfSets; ++s) prof_drain(e, s);
     e->prof = enable != 0;
     for (int i = 0; i < CMM_PROF_SLOTS; ++i) { e->ms[i] = 0; e->cnt[i] = 0; }
     return CMM_OK;
@@ -748,6 +794,7 @@ int cmm_euler_profile(cmm_euler *e, int enable) {
 
 int cmm_euler_profile_read(cmm_euler *e, double *ms, int *cnt) {
     if (!e) { set_error("cmm_euler_profile_read: null engine"); return CMM_ERR_ARG; }
+    for (int s = 0; s < cmm_euler::kProfSets; ++s) prof_drain(e, s);
     for (int i = 0; i < CMM_PROF_SLOTS; ++i) {
         if (ms) ms[i] = e->ms[i];
         if (cnt) cnt[i] = e->cnt[i];

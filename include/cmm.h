@@ -197,7 +197,8 @@ int cmm_euler_state(cmm_euler *e, int *steps, double *t, int *stack_size);
 /* Per-kernel timing with events on the engine stream: enable (clears the counters), then read
  * the accumulated milliseconds and launch counts of CMM_PROF_SLOTS slots:
  * [map advection, incompressibility check, psi sample+row R2C, psi column stage, psi row C2R,
- *  particles, remap (translate + push), Lagrange-weighted psi combination before the advection]. */
+ *  particles, remap (translate + push + the psi recomputed after it), Lagrange-weighted psi
+ *  combination before the advection]. */
 #define CMM_PROF_SLOTS 8
 int cmm_euler_profile(cmm_euler *e, int enable);
 int cmm_euler_profile_read(cmm_euler *e, double *ms_out, int *count_out);
