@@ -237,6 +237,9 @@ struct cmm_euler {
     double *lvl[4] = {nullptr, nullptr, nullptr, nullptr};  // level -> buffer, newest first
     double *W = nullptr, *disc = nullptr, *stack = nullptr, *tmp = nullptr, *tmp4 = nullptr;
     double *W4 = nullptr;  // W node-interleaved (Euler: read by the psi sampler)
+    double *m8[2] = {nullptr, nullptr};  // the map node-interleaved (ping-pong), see engine_advect
+    int m8_cur = 0;
+    bool m8_valid = false;
     int nstack = 0;
     void *work = nullptr;
     size_t work_bytes = 0;
@@ -304,11 +307,18 @@ int engine_advect(cmm_euler *e, const double *t, const double *dt, int loop_ctr,
     for (int l = 0; l < 4; ++l) P.p[l] = e->lvl[l < L ? l : 0];
     // the workspace is shared with the psi solve, which runs later on the same stream
     // event 10 separates the psi combination from the advection kernel in the profile
+    // m8: the map node-interleaved, written by the previous advection (valid until the map is
+    // reset); with a workspace the kernel reads it and writes the next one
+    const bool comb = op_advect_workspace(e->p.g_coarse, e->p.g_psi, c) > 0;
+    const double *m8_in = comb && e->m8_valid ? e->m8[e->m8_cur] : nullptr;
+    double *m8_out = comb ? e->m8[1 - e->m8_cur] : nullptr;
     if (int rc = op_map_advect(e->p.g_coarse, e->p.g_psi, e->cx, e->cy, P, c, e->ncx, e->ncy, e->work, e->work_bytes,
-                               e->st, e->prof ? e->ev[10] : nullptr))
+                               e->st, e->prof ? e->ev[10] : nullptr, m8_in, m8_out))
         return rc;
     std::swap(e->cx, e->ncx);
     std::swap(e->cy, e->ncy);
+    e->m8_valid = comb;
+    if (comb) e->m8_cur = 1 - e->m8_cur;
     return CMM_OK;
 }
 
@@ -453,6 +463,8 @@ int cmm_euler_create(const cmm_euler_params *p, cmm_euler **out) {
         e->lvl[l] = e->psi_buf[l];
     }
     if (!rc) rc = dalloc(&e->W, 4 * e->nf);
+    if (!rc && p->l_order >= 2) rc = dalloc(&e->m8[0], 8 * e->nc);
+    if (!rc && p->l_order >= 2) rc = dalloc(&e->m8[1], 8 * e->nc);
     if (!rc && p->sim == CMM_SIM_EULER_2D) rc = dalloc(&e->W4, 4 * e->nf);
     if (!rc && p->stack_capacity > 0) rc = dalloc(&e->stack, (size_t)8 * e->nc * p->stack_capacity);
     if (!rc) {
@@ -486,7 +498,7 @@ int cmm_euler_create(const cmm_euler_params *p, cmm_euler **out) {
 int cmm_euler_destroy(cmm_euler *e) {
     if (!e) return CMM_OK;
     if (e->st) (void)hipStreamSynchronize(e->st);
-    double *bufs[] = {e->cx, e->cy, e->ncx, e->ncy, e->W, e->W4, e->disc, e->stack, e->tmp, e->tmp4,
+    double *bufs[] = {e->cx, e->cy, e->ncx, e->ncy, e->W, e->W4, e->m8[0], e->m8[1], e->disc, e->stack, e->tmp, e->tmp4,
                       e->psi_buf[0], e->psi_buf[1], e->psi_buf[2], e->psi_buf[3]};
     for (double *b : bufs)
         if (b) (void)hipFree(b);
@@ -531,6 +543,7 @@ int cmm_euler_init(cmm_euler *e, double *warm, int *nwarm) {
     int nw = 0;
     // Cuda-CMM/src/simulation/cmm-euler2d.cu:495-512
     if (int rc = op_map_identity(p.g_coarse, e->cx, e->cy, e->st)) return rc;
+    e->m8_valid = false;
     e->nstack = 0;
     if (int rc = engine_translate(e)) return rc;
     if (int rc = engine_psi(e, e->lvl[0])) return rc;
@@ -564,6 +577,7 @@ int cmm_euler_init(cmm_euler *e, double *warm, int *nwarm) {
             std::swap(e->lvl[0], e->lvl[nsteps]);
             if (nsteps >= 3) std::swap(e->lvl[1], e->lvl[2]);
             if (int rc = op_map_identity(p.g_coarse, e->cx, e->cy, e->st)) return rc;
+            e->m8_valid = false;
         }
     }
     if (nwarm) *nwarm = std::min(nw, 8);
@@ -628,6 +642,7 @@ int cmm_euler_step(cmm_euler *e, double dt_now, cmm_step_info *info) {
             if (int rc = hip_check(hipMemcpyAsync(slotx + 4 * e->nc, e->cy, sizeof(double) * 4 * e->nc, hipMemcpyDeviceToDevice, e->st), "push map")) return rc;
             e->nstack++;
             if (int rc = op_map_identity(p.g_coarse, e->cx, e->cy, e->st)) return rc;
+            e->m8_valid = false;
             if (int rc = engine_psi(e, e->lvl[0])) return rc;
             prof_mark(e, 4);
             remapped = 1;

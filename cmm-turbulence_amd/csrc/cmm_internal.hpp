@@ -20,7 +20,7 @@ int op_map_identity(const cmm_grid &g, double *cx, double *cy, hipStream_t st);
 size_t op_advect_workspace(const cmm_grid &gm, const cmm_grid &gp, const cmm_advect_coeffs &c);
 int op_map_advect(const cmm_grid &gm, const cmm_grid &gp, const double *cx, const double *cy, const PsiLevels &P,
                   const cmm_advect_coeffs &c, double *ncx, double *ncy, void *work, size_t work_bytes,
-                  hipStream_t st, hipEvent_t mid = nullptr);
+                  hipStream_t st, hipEvent_t mid = nullptr, const double *m8_in = nullptr, double *m8_out = nullptr);
 int op_incomp_launch(const cmm_grid &gm, const cmm_grid &gc, const double *cx, const double *cy,
                      unsigned long long *slot, hipStream_t st);
 int op_incomp(const cmm_grid &gm, const cmm_grid &gc, const double *cx, const double *cy, double *out,

@@ -263,7 +263,8 @@ __global__ void __launch_bounds__(256) k_map_advect(DGrid gm, DGrid gp, const do
                                                     const double *__restrict__ cy, PsiLevels P, const double *C1,
                                                     const double *C12, const double *C0, const double *M8,
                                                     cmm_advect_coeffs c, double *__restrict__ ncx,
-                                                    double *__restrict__ ncy, int nblocks) {
+                                                    double *__restrict__ ncy, double *__restrict__ M8out,
+                                                    int nblocks) {
     constexpr int NPB = 256 / G;  // nodes per block
     const int blk = xcd_block(blockIdx.x, nblocks);
     const int node = blk * NPB + threadIdx.x / G;
@@ -297,7 +298,11 @@ __global__ void __launch_bounds__(256) k_map_advect(DGrid gm, DGrid gp, const do
             const double vy = __shfl(py, base + m, 64);
             acc += (o < 4 ? vx : vy) * w[m];
         }
-        if (active && o < 8) (o < 4 ? ncx : ncy)[(long)plane * gm.n + node] = acc;
+        if (active && o < 8) {
+            (o < 4 ? ncx : ncy)[(long)plane * gm.n + node] = acc;
+            // the new map node-interleaved as well, for the next step's map evaluations
+            if (M8out) M8out[8L * node + o] = acc;
+        }
     }
 }
 
@@ -1262,26 +1267,32 @@ size_t op_advect_workspace(const cmm_grid &gm, const cmm_grid &gp, const cmm_adv
 
 int op_map_advect(const cmm_grid &gm, const cmm_grid &gp, const double *cx, const double *cy, const PsiLevels &P,
                   const cmm_advect_coeffs &c, double *ncx, double *ncy, void *work, size_t work_bytes,
-                  hipStream_t st, hipEvent_t mid) {
+                  hipStream_t st, hipEvent_t mid, const double *m8_in, double *m8_out) {
     DGrid dm = make_dgrid(gm), dp = make_dgrid(gp);
     const int nfoot = 4 + 4 * c.map_update_order_num;
     const int G = nfoot <= 4 ? 4 : nfoot <= 8 ? 8 : 16;
     const int nb = (int)(((long)dm.n * G + 255) / 256);
     const size_t need = op_advect_workspace(gm, gp, c);
     const bool comb = work && need > 0 && work_bytes >= need;
-    double *C1 = nullptr, *C12 = nullptr, *C0 = nullptr, *M8 = nullptr;
+    const double *M8 = nullptr;
+    double *C1 = nullptr, *C12 = nullptr, *C0 = nullptr;
     if (comb) {
         const long n = dp.n, n4 = 4L * n;
         double *next = static_cast<double *>(work);
-        M8 = next;
+        double *m8_work = next;
         next += 8L * dm.n;
         C1 = next;
         next += n4;
         if (advect_needs_c12(c)) { C12 = next; next += n4; }
         if (advect_needs_c0(c)) C0 = next;
-        hipLaunchKernelGGL(k_interleave_map, dim3((unsigned)((dm.n + 255) / 256)), dim3(256), 0, st, (long)dm.n, cx, cy,
-                           M8);
-        if (int rc = launch_check("k_interleave_map")) return rc;
+        if (m8_in) {
+            M8 = m8_in;
+        } else {
+            hipLaunchKernelGGL(k_interleave_map, dim3((unsigned)((dm.n + 255) / 256)), dim3(256), 0, st, (long)dm.n, cx,
+                               cy, m8_work);
+            if (int rc = launch_check("k_interleave_map")) return rc;
+            M8 = m8_work;
+        }
         const dim3 g((unsigned)((n + 255) / 256));
         switch (c.l_order) {
         case 2: hipLaunchKernelGGL(k_psi_combine<2>, g, dim3(256), 0, st, n, P, c, C1, C12, C0); break;
@@ -1293,7 +1304,7 @@ int op_map_advect(const cmm_grid &gm, const cmm_grid &gp, const double *cx, cons
     if (mid) (void)hipEventRecord(mid, st);
 #define CMM_ADV(T, GG, CB)                                                                                             \
     hipLaunchKernelGGL((k_map_advect<T, GG, CB>), dim3(nb), dim3(256), 0, st, dm, dp, cx, cy, P, C1, C12, C0, M8, c, ncx, \
-                       ncy, nb)
+                       ncy, m8_out, nb)
 #define CMM_ADV_G(T, CB)                                                                                               \
     do {                                                                                                               \
         if (G == 4) CMM_ADV(T, 4, CB);                                                                                 \
