@@ -246,6 +246,11 @@ struct cmm_euler {
     unsigned long long *slot = nullptr;
     unsigned long long *h_slot = nullptr;  // pinned host copy of slot
     hipEvent_t ev_inc = nullptr;           // set after the E_inc copy of a step
+    // particles run on a second stream after the psi solve and overlap the next step's advection;
+    // the next psi solve (which overwrites the oldest level) waits for them
+    hipStream_t st2 = nullptr;
+    hipEvent_t ev_psi = nullptr, ev_part = nullptr;
+    bool part_pending = false;
     std::vector<double *> part;
     std::vector<int> part_n, part_scheme;
     std::vector<double> tv, dtv;
@@ -484,6 +489,9 @@ int cmm_euler_create(const cmm_euler_params *p, cmm_euler **out) {
         rc = CMM_ERR_ALLOC;
     }
     if (!rc) rc = hip_check(hipEventCreateWithFlags(&e->ev_inc, hipEventDisableTiming), "hipEventCreate");
+    if (!rc) rc = hip_check(hipEventCreateWithFlags(&e->ev_psi, hipEventDisableTiming), "hipEventCreate");
+    if (!rc) rc = hip_check(hipEventCreateWithFlags(&e->ev_part, hipEventDisableTiming), "hipEventCreate");
+    if (!rc) rc = hip_check(hipStreamCreateWithFlags(&e->st2, hipStreamNonBlocking), "hipStreamCreate");
     for (int s = 0; s < cmm_euler::kProfSets && !rc; ++s)
         for (int i = 0; i < 12 && !rc; ++i) rc = hip_check(hipEventCreate(&e->evs[s][i]), "hipEventCreate");
     if (!rc && p->discrete) rc = engine_setup_discrete(e);
@@ -506,8 +514,12 @@ int cmm_euler_destroy(cmm_euler *e) {
         if (b) (void)hipFree(b);
     if (e->work) (void)hipFree(e->work);
     if (e->slot) (void)hipFree(e->slot);
+    if (e->st2) (void)hipStreamSynchronize(e->st2);
     if (e->h_slot) (void)hipHostFree(e->h_slot);
     if (e->ev_inc) (void)hipEventDestroy(e->ev_inc);
+    if (e->ev_psi) (void)hipEventDestroy(e->ev_psi);
+    if (e->ev_part) (void)hipEventDestroy(e->ev_part);
+    if (e->st2) (void)hipStreamDestroy(e->st2);
     for (int s = 0; s < cmm_euler::kProfSets; ++s)
         for (int i = 0; i < 12; ++i)
             if (e->evs[s][i]) (void)hipEventDestroy(e->evs[s][i]);
@@ -625,6 +637,8 @@ int cmm_euler_step(cmm_euler *e, double dt_now, cmm_step_info *info) {
     double *oldest = e->lvl[L - 1];
     for (int l = L - 1; l > 0; --l) e->lvl[l] = e->lvl[l - 1];
     e->lvl[0] = oldest;
+    if (e->part_pending)  // the previous step's particles still read the level about to be rewritten
+        if (int rc = hip_check(hipStreamWaitEvent(e->st, e->ev_part, 0), "hipStreamWaitEvent")) return rc;
     if (int rc = engine_psi(e, e->lvl[0], e->prof ? &e->ev[5] : nullptr)) return rc;
     if (int rc = hip_check(hipEventSynchronize(e->ev_inc), "hipEventSynchronize")) return rc;
     double einc;
@@ -648,7 +662,8 @@ int cmm_euler_step(cmm_euler *e, double dt_now, cmm_step_info *info) {
             remapped = 1;
         }
     }
-    // particles, :881 with Cuda-CMM/src/numerical/cmm-particles.cu:63-87
+    // particles, :881 with Cuda-CMM/src/numerical/cmm-particles.cu:63-87, on the second stream
+    bool moved = false;
     if (!e->part.empty()) {
         double Lp[4] = {0, 0, 0, 0};
         for (int i = 0; i < L; ++i) Lp[i] = cmm_lagrange_coeff(e->tv.data(), e->tv[lc] + e->dtv[lc + 1] / 2.0, lc + 1, i, L);
@@ -657,10 +672,19 @@ int cmm_euler_step(cmm_euler *e, double dt_now, cmm_step_info *info) {
         for (size_t s = 0; s < e->part.size(); ++s) {
             if (e->part_n[s] <= 0) continue;
             if (L < 2 && e->part_scheme[s] != 10 && e->part_scheme[s] != 20) continue;
-            if (int rc = op_particles(p.g_psi, P, L, Lp, e->dtv[lc + 1], e->part_scheme[s], e->part_n[s], e->part[s], e->st)) return rc;
+            if (!moved) {
+                if (int rc = hip_check(hipEventRecord(e->ev_psi, e->st), "hipEventRecord")) return rc;
+                if (int rc = hip_check(hipStreamWaitEvent(e->st2, e->ev_psi, 0), "hipStreamWaitEvent")) return rc;
+                moved = true;
+            }
+            if (int rc = op_particles(p.g_psi, P, L, Lp, e->dtv[lc + 1], e->part_scheme[s], e->part_n[s], e->part[s], e->st2)) return rc;
         }
     }
-    prof_mark(e, 9);
+    if (moved) {
+        if (int rc = hip_check(hipEventRecord(e->ev_part, e->st2), "hipEventRecord")) return rc;
+        e->part_pending = true;
+    }
+    if (e->prof) (void)hipEventRecord(e->ev[9], moved ? e->st2 : e->st);
     prof_acc(e, 7, 0, 10);
     prof_acc(e, 0, 10, 1);
     prof_acc(e, 1, 1, 2);
@@ -710,8 +734,8 @@ int cmm_euler_get_field(cmm_euler *e, int field, double *out) {
 int cmm_euler_get_particles(cmm_euler *e, int set, double *out) {
     if (!e || set < 0 || set >= (int)e->part.size() || !out) { set_error("cmm_euler_get_particles: bad argument"); return CMM_ERR_ARG; }
     if (e->part_n[set] <= 0) return CMM_OK;
-    if (int rc = hip_check(hipMemcpyAsync(out, e->part[set], sizeof(double) * 2 * e->part_n[set], hipMemcpyDeviceToHost, e->st), "get")) return rc;
-    return hip_check(hipStreamSynchronize(e->st), "hipStreamSynchronize");
+    if (int rc = hip_check(hipMemcpyAsync(out, e->part[set], sizeof(double) * 2 * e->part_n[set], hipMemcpyDeviceToHost, e->st2), "get")) return rc;
+    return hip_check(hipStreamSynchronize(e->st2), "hipStreamSynchronize");
 }
 
 int cmm_euler_diagnostics(cmm_euler *e, double *out) {

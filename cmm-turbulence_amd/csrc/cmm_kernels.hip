@@ -369,21 +369,15 @@ __global__ void __launch_bounds__(256) k_incomp_cells(DGrid gm, DGrid gc, const 
                 basis_d(c.dy, dby);
                 const double wjx[4] = {c.wx0 * gm.jx, c.wx1 * gm.jx, c.wx0 * gm.jx, c.wx1 * gm.jx};
                 const double wjy[4] = {c.wy0 * gm.jy, c.wy0 * gm.jy, c.wy1 * gm.jy, c.wy1 * gm.jy};
-                double Xx, Yx, Xy, Yy;
-                if (c.i00 == c0.i00 && c.wx0 == c0.wx0 && c.wy0 == c0.wy0) {
-                    Xx = hsum_v(vx, dbx, by, gm.hx, gm.hy, wjx) * gm.ihx;
-                    Yx = hsum_v(vy, dbx, by, gm.hx, gm.hy, wjy) * gm.ihy;
-                    Xy = hsum_v(vx, bx, dby, gm.hx, gm.hy, wjx) * gm.ihx;
-                    Yy = hsum_v(vy, bx, dby, gm.hx, gm.hy, wjy) * gm.ihy;
-                } else {
-                    double ux[16], uy[16];
-                    load16(cx, gm.n, c, ux);
-                    load16(cy, gm.n, c, uy);
-                    Xx = hsum_v(ux, dbx, by, gm.hx, gm.hy, wjx) * gm.ihx;
-                    Yx = hsum_v(uy, dbx, by, gm.hx, gm.hy, wjy) * gm.ihy;
-                    Xy = hsum_v(ux, bx, dby, gm.hx, gm.hy, wjx) * gm.ihx;
-                    Yy = hsum_v(uy, bx, dby, gm.hx, gm.hy, wjy) * gm.ihy;
+                if (!same_cell(c, c0)) {  // never expected: reload for this point's cell
+                    load16(cx, gm.n, c, vx);
+                    load16(cy, gm.n, c, vy);
+                    c0 = c;
                 }
+                const double Xx = hsum_v(vx, dbx, by, gm.hx, gm.hy, wjx) * gm.ihx;
+                const double Yx = hsum_v(vy, dbx, by, gm.hx, gm.hy, wjy) * gm.ihy;
+                const double Xy = hsum_v(vx, bx, dby, gm.hx, gm.hy, wjx) * gm.ihx;
+                const double Yy = hsum_v(vy, bx, dby, gm.hx, gm.hy, wjy) * gm.ihy;
                 v = fmax(v, fabs(1 - (Xx * Yy - Xy * Yx)));
             }
         }

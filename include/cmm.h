@@ -171,7 +171,8 @@ enum {
 
 int cmm_euler_create(const cmm_euler_params *p, cmm_euler **out);
 int cmm_euler_destroy(cmm_euler *e);
-/* Stream the engine runs on (hipStream_t as void*). */
+/* Stream the engine runs on (hipStream_t as void*).  Particle sets advance on a second engine
+ * stream, ordered against this one with events; cmm_euler_get_particles waits for them. */
 void *cmm_euler_stream(cmm_euler *e);
 /* Adds a fluid-particle set (positions copied from host); returns its index in *set. */
 int cmm_euler_add_particles(cmm_euler *e, int n, int scheme, const double *pos_host, int *set);

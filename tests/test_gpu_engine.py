@@ -151,3 +151,27 @@ def test_vlasov_poisson_against_oracle(dev):
     assert abs(mass - 4 * np.pi) < 1e-9 and abs(Ekin - 2 * np.pi) < 1e-9
     assert abs(Epot - np.pi) < 1e-9 and abs(Etot - (Ekin + Epot)) < 1e-12 and abs(mom) < 1e-9
     assert abs(d0[5] - 0.5) < 1e-9 and max(d0[6:9]) < 1e-9
+
+
+def test_particles_in_the_loop(dev):
+    """Fluid particles advanced by the device loop (on its second stream, overlapping the next
+    step) against the oracle's run: 4_nodes, map 64^2, fine = psi = vort 128^2, dt 1/16, 16 steps
+    with a remap threshold that remaps, 500 particles, RK3 and RK4; positions to 1e-12 relative."""
+    import cmm
+    rng = np.random.default_rng(3)
+    pos = rng.uniform(0, TWO_PI, 1000)
+    for scheme, tin, L in ((30, "RK3", 3), (40, "RK4", 4)):
+        e = cmm.Euler(cmm.grid(64), cmm.grid(128), cmm.grid(128), cmm.grid(128), 1.0 / 16, time_integration=tin,
+                      incomp_threshold=1e-4, stack_capacity=64)
+        s = e.add_particles(pos, scheme=scheme)
+        e.init()
+        infos = [e.step(1.0 / 16) for _ in range(16)]
+        got = e.particles(s)
+        e.close()
+        r = O.euler_run(O.grid(64), O.grid(128), O.grid(128), O.grid(128), dt=1.0 / 16, final_time=1.0, tin=scheme,
+                        l_order=L, threshold=1e-4, stack_capacity=64, particles=pos, particle_scheme=scheme,
+                        want=("incomp", "time", "map_counter", "particles"))
+        maps = [i.stack_size for i in infos]
+        print(scheme, "maps", maps[-1], "particles rel Linf", rel_linf(got, r["particles"]))
+        assert maps == list(r["map_counter"]) and maps[-1] >= 1
+        assert rel_linf(got, r["particles"]) <= 1e-12
