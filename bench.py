@@ -40,13 +40,12 @@ def algorithmic(slot, c, kx_max, n_stack=0):
     nc, npsi, nf, nv = c["n_coarse"] ** 2, c["n_psi"] ** 2, c["n_fine"] ** 2, c["n_vort"] ** 2
     L = 3
     if slot == "advect":
-        # map in (8 doubles/node) + the two Lagrange-weighted psi fields and psi level 0 (4 planes
-        # each) + map out
-        return 64 * nc + 32 * 3 * npsi + 64 * nc, 9.2e3 * nc
+        # map in (8 doubles/node, interleaved) + the two Lagrange-weighted psi fields and psi level 0
+        # (4 planes each) + map out, plane-major and interleaved for the next step
+        return 64 * nc + 32 * 3 * npsi + 2 * 64 * nc, 9.2e3 * nc
     if slot == "psi_combine":
-        # L psi levels read, C1, C12 and the level-0 copy written (4 planes each); the map read and
-        # written interleaved (the slot times both kernels)
-        return 32 * (L + 3) * npsi + 2 * 64 * nc, 2 * 2 * L * 4 * npsi
+        # L psi levels read, C1, C12 and the level-0 copy written (4 planes each)
+        return 32 * (L + 3) * npsi, 2 * 2 * L * 4 * npsi
     if slot == "incomp":
         return 64 * nc, 320.0 * nf
     if slot == "psi_row_r2c":

@@ -73,9 +73,9 @@ SYMBOLS = ["cmm_version", "cmm_last_error", "cmm_device_info", "cmm_lagrange_coe
            "cmm_map_init_identity", "cmm_advect_workspace_bytes", "cmm_map_advect", "cmm_incomp_error", "cmm_compose_positions",
            "cmm_sample_init", "cmm_translate_workspace_bytes", "cmm_translate_ic", "cmm_hermite_workspace_bytes",
            "cmm_hermite_from_real", "cmm_hermite_from_spectrum", "cmm_psi_workspace_bytes", "cmm_psi_from_vort",
-           "cmm_sample_vort", "cmm_vp_psi_workspace_bytes", "cmm_vp_psi_from_f", "cmm_particles_advect", "cmm_euler_create", "cmm_euler_destroy", "cmm_euler_stream",
+           "cmm_spectral_workspace_bytes", "cmm_spectral_from_real", "cmm_sample_vort", "cmm_vp_psi_workspace_bytes", "cmm_vp_psi_from_f", "cmm_particles_advect", "cmm_euler_create", "cmm_euler_destroy", "cmm_euler_stream",
            "cmm_euler_add_particles", "cmm_euler_init", "cmm_euler_step", "cmm_euler_get_field",
-           "cmm_euler_get_particles", "cmm_euler_diagnostics", "cmm_euler_vp_diagnostics", "cmm_euler_state", "cmm_euler_profile", "cmm_euler_profile_read"]
+           "cmm_euler_get_particles", "cmm_euler_diagnostics", "cmm_euler_vp_diagnostics", "cmm_euler_sample", "cmm_euler_state", "cmm_euler_profile", "cmm_euler_profile_read"]
 
 _lib = None
 DP = C.POINTER(C.c_double)
@@ -115,6 +115,9 @@ def lib():
         L.cmm_psi_from_vort.argtypes = [Grid, Grid, Grid, Grid, VP, VP, VP, C.c_int, C.c_double, VP, VP,
                                         C.c_size_t, VP]
         L.cmm_sample_vort.argtypes = [Grid, Grid, Grid, VP, VP, VP, C.c_int, VP, VP]
+        L.cmm_spectral_workspace_bytes.restype = C.c_size_t
+        L.cmm_spectral_workspace_bytes.argtypes = [Grid]
+        L.cmm_spectral_from_real.argtypes = [Grid, VP, C.c_int, VP, VP, C.c_size_t, VP]
         L.cmm_vp_psi_workspace_bytes.restype = C.c_size_t
         L.cmm_vp_psi_workspace_bytes.argtypes = [Grid, Grid]
         L.cmm_vp_psi_from_f.argtypes = [Grid, Grid, Grid, Grid, VP, VP, VP, C.c_int, VP, VP, C.c_size_t, VP]
@@ -130,6 +133,7 @@ def lib():
         L.cmm_euler_get_particles.argtypes = [VP, C.c_int, DP]
         L.cmm_euler_diagnostics.argtypes = [VP, DP]
         L.cmm_euler_vp_diagnostics.argtypes = [VP, DP]
+        L.cmm_euler_sample.argtypes = [VP, Grid, C.c_int, DP]
         L.cmm_euler_state.argtypes = [VP, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_int)]
         L.cmm_euler_profile.argtypes = [VP, C.c_int]
         L.cmm_euler_profile_read.argtypes = [VP, DP, C.POINTER(C.c_int)]
@@ -224,6 +228,11 @@ def translate_initial_condition_through_map_stack(g_map, g_fine, chix, chiy, sta
 
 def hermite_from_real(g, real, herm):
     _check(lib().cmm_hermite_from_real(g, _p(real), _p(herm), None, 0, _stream()), "cmm_hermite_from_real")
+
+
+def spectral_from_real(g, real, op, herm):
+    """Hermite planes of F (op 0), of i_laplacian(F) (op 1) or of laplacian(F) (op 2)."""
+    _check(lib().cmm_spectral_from_real(g, _p(real), op, _p(herm), None, 0, _stream()), "cmm_spectral_from_real")
 
 
 def hermite_from_spectrum(g, spec_cols, herm):
@@ -353,6 +362,16 @@ class Euler:
         out = (C.c_double * 4)()
         _check(lib().cmm_euler_diagnostics(self.h, C.cast(out, DP)), "cmm_euler_diagnostics")
         return list(out)
+
+    def sample(self, g_sample, scalar_ic=-1):
+        """Fields through the whole map stack on g_sample (cmm_euler_sample): a dict of arrays
+        chix, chiy, w, theta, lap_w, gradx_w, grady_w, psi, ux, uy."""
+        import numpy as np
+        n = g_sample.nx * g_sample.ny
+        out = np.zeros(10 * n)
+        _check(lib().cmm_euler_sample(self.h, g_sample, scalar_ic, out.ctypes.data_as(DP)), "cmm_euler_sample")
+        names = ("chix", "chiy", "w", "theta", "lap_w", "gradx_w", "grady_w", "psi", "ux", "uy")
+        return {k: out[i * n:(i + 1) * n] for i, k in enumerate(names)}
 
     def vp_diagnostics(self):
         """Vlasov-Poisson: Etot, Ekin, Epot, Mass, Momentum, Emode1..4"""

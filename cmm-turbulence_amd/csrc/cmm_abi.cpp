@@ -175,6 +175,13 @@ int cmm_hermite_from_spectrum(cmm_grid g, const double *spec, double *herm, void
 
 size_t cmm_psi_workspace_bytes(cmm_grid gv, cmm_grid gp, double fcut) { return op_psi_workspace(gv, gp, fcut); }
 
+size_t cmm_spectral_workspace_bytes(cmm_grid g) { return grid_ok(g) ? op_hermite_workspace(g) : 0; }
+
+int cmm_spectral_from_real(cmm_grid g, const double *real, int op, double *herm, void *work, size_t wb, void *stream) {
+    if (!grid_ok(g) || !real || !herm) { set_error("cmm_spectral_from_real: bad argument"); return CMM_ERR_ARG; }
+    return op_spectral_from_real(g, real, op, herm, work, wb, S(stream));
+}
+
 size_t cmm_vp_psi_workspace_bytes(cmm_grid gv, cmm_grid gp) {
     if (!grid_ok(gv) || !grid_ok(gp)) return 0;
     return op_vp_psi_workspace(gv, gp);
@@ -238,6 +245,8 @@ struct cmm_euler {
     double *W = nullptr, *disc = nullptr, *stack = nullptr, *tmp = nullptr, *tmp4 = nullptr;
     double *W4 = nullptr;  // W node-interleaved (Euler: read by the psi sampler)
     double *m8[2] = {nullptr, nullptr};  // the map node-interleaved (ping-pong), see engine_advect
+    double *sbuf = nullptr;              // sample-grid buffers of cmm_euler_sample
+    size_t sbuf_n = 0;
     int m8_cur = 0;
     bool m8_valid = false;
     int nstack = 0;
@@ -506,7 +515,7 @@ int cmm_euler_create(const cmm_euler_params *p, cmm_euler **out) {
 int cmm_euler_destroy(cmm_euler *e) {
     if (!e) return CMM_OK;
     if (e->st) (void)hipStreamSynchronize(e->st);
-    double *bufs[] = {e->cx, e->cy, e->ncx, e->ncy, e->W, e->W4, e->m8[0], e->m8[1], e->disc, e->stack, e->tmp, e->tmp4,
+    double *bufs[] = {e->cx, e->cy, e->ncx, e->ncy, e->W, e->W4, e->m8[0], e->m8[1], e->sbuf, e->disc, e->stack, e->tmp, e->tmp4,
                       e->psi_buf[0], e->psi_buf[1], e->psi_buf[2], e->psi_buf[3]};
     for (double *b : bufs)
         if (b) (void)hipFree(b);
@@ -812,6 +821,53 @@ int cmm_euler_vp_diagnostics(cmm_euler *e, double *out) {
         }
         out[4 + k] = k <= gp.nx / 2 ? std::sqrt(re * re + im * im) / gp.nx : 0.0;
     }
+    return CMM_OK;
+}
+
+int cmm_euler_sample(cmm_euler *e, cmm_grid gs, int scalar_ic, double *out) {
+    if (!e || !out || !grid_ok(gs)) { set_error("cmm_euler_sample: bad argument"); return CMM_ERR_ARG; }
+    const cmm_grid &gc = e->p.g_coarse;
+    if (gs.x0 != gc.x0 || gs.x1 != gc.x1 || gs.y0 != gc.y0 || gs.y1 != gc.y1) {
+        set_error("cmm_euler_sample: the sample grid must have the map's bounds");
+        return CMM_ERR_ARG;
+    }
+    const size_t n = (size_t)gs.nx * gs.ny;
+    // device buffer: positions (2n), W (n), theta (n), one Hermite result (4n), kept for the next call
+    if (e->sbuf_n < n) {
+        if (e->sbuf) (void)hipFree(e->sbuf);
+        e->sbuf = nullptr;
+        e->sbuf_n = 0;
+        if (int rc = dalloc(&e->sbuf, 8 * n)) return rc;
+        e->sbuf_n = n;
+    }
+    double *xy = e->sbuf, *w = xy + 2 * n, *th = w + n, *h = th + n;
+    std::vector<double> hxy(2 * n);
+    auto get = [&](double *dst, const double *src, size_t cnt) {
+        return hip_check(hipMemcpyAsync(dst, src, sizeof(double) * cnt, hipMemcpyDeviceToHost, e->st), "get");
+    };
+    int rc = op_compose_positions(gc, gs, e->cx, e->cy, e->stack, e->nstack, xy, e->st);
+    if (!rc) rc = get(hxy.data(), xy, 2 * n);
+    if (!rc) rc = op_sample_init(gc, gs, e->cx, e->cy, e->stack, e->nstack, e->p.ic, e->p.ic_params, e->p.discrete,
+                                 e->p.discrete ? e->p.g_disc : e->p.g_fine, e->disc, w, e->st);
+    if (!rc) rc = get(out + CMM_SAMPLE_W * n, w, n);
+    if (!rc && scalar_ic >= 0)
+        rc = op_sample_init(gc, gs, e->cx, e->cy, e->stack, e->nstack, scalar_ic, e->p.ic_params, 0, gs, nullptr, th,
+                            e->st);
+    if (!rc && scalar_ic >= 0) rc = get(out + CMM_SAMPLE_THETA * n, th, n);
+    // spectral derivatives of the sampled field (the psi-solve workspace is free between steps)
+    if (!rc) rc = op_spectral_from_real(gs, w, 2, h, nullptr, 0, e->st);
+    if (!rc) rc = get(out + CMM_SAMPLE_LAP_W * n, h, n);
+    if (!rc) rc = op_spectral_from_real(gs, w, 0, h, nullptr, 0, e->st);
+    if (!rc) rc = get(out + CMM_SAMPLE_GRADX_W * n, h + n, 2 * n);
+    if (!rc) rc = op_spectral_from_real(gs, w, 1, h, nullptr, 0, e->st);
+    if (!rc) rc = get(out + CMM_SAMPLE_PSI * n, h, 3 * n);
+    if (!rc) rc = hip_check(hipStreamSynchronize(e->st), "hipStreamSynchronize");
+    if (rc) return rc;
+    for (size_t i = 0; i < n; ++i) {
+        out[CMM_SAMPLE_CHIX * n + i] = hxy[2 * i];
+        out[CMM_SAMPLE_CHIY * n + i] = hxy[2 * i + 1];
+    }
+    if (scalar_ic < 0) std::fill(out + CMM_SAMPLE_THETA * n, out + (CMM_SAMPLE_THETA + 1) * n, 0.0);
     return CMM_OK;
 }
 

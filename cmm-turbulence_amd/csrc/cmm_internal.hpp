@@ -48,6 +48,8 @@ size_t op_vp_psi_workspace(const cmm_grid &gv, const cmm_grid &gp);
 int op_vp_psi_from_f(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf, const cmm_grid &gp, const double *cx,
                      const double *cy, const double *W, int molly, double *psi, void *work, size_t work_bytes,
                      hipStream_t st);
+int op_spectral_from_real(const cmm_grid &g, const double *real, int op, double *herm, void *work, size_t work_bytes,
+                          hipStream_t st);
 int op_sample_vort(const cmm_grid &gm, const cmm_grid &gt, const cmm_grid &gf, const double *cx, const double *cy,
                    const double *W, int molly, double *out, hipStream_t st);
 int op_particles(const cmm_grid &gp, const PsiLevels &P, int L, const double *Lp, double dt, int scheme, int n,

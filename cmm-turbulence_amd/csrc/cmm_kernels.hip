@@ -409,6 +409,22 @@ __device__ double init_vorticity(double x, double y, const ICParams &p) {
                (exp(-(y - 3) * (y - 3) * 0.5) + exp(-(y + 3) * (y + 3) * 0.5));
     case CMM_IC_NONLIN_LANDAU_DAMPING:
         return (1 + 0.5 * cos(0.5 * x)) * exp(-y * y * 0.5) / sqrt(2 * kPi);
+    // passive scalars, d_init_scalar (Cuda-CMM/src/simulation/cmm-init.cu:249-301)
+    case CMM_IC_SCALAR_RECTANGLE: {
+        x -= floor(x / kTwoPi) * kTwoPi;
+        y -= floor(y / kTwoPi) * kTwoPi;
+        const double cx0 = kPi, cy0 = kPi / 2.0, wx = 2 * kPi, wy = kPi;
+        return (x > cx0 - wx / 2.0 && x < cx0 + wx / 2.0 && y > cy0 - wy / 2.0 && y < cy0 + wy / 2.0) ? 1.0 : 0.0;
+    }
+    case CMM_IC_SCALAR_GAUSSIAN: {
+        const double mu = kPi, var = kPi / 4.0;
+        return 1.0 / (kTwoPi * var * var) * exp(-((x - mu) * (x - mu) / (2 * var * var) + (y - mu) * (y - mu) / (2 * var * var)));
+    }
+    case CMM_IC_SCALAR_CIRCULAR_RING: {
+        const double r = kPi / 2.0, rw = r / 8.0, s1 = 10.0, s2 = 10.0;
+        const double d2 = (x - kPi) * (x - kPi) + (y - kPi) * (y - kPi);
+        return (tanh(s1 * (d2 - (r + rw) * (r + rw))) - tanh(s2 * (d2 - (r - rw) * (r - rw)))) / 2.0;
+    }
     case CMM_IC_4_NODES: {
         x -= floor(x / kTwoPi) * kTwoPi;
         y -= floor(y / kTwoPi) * kTwoPi;
@@ -678,6 +694,7 @@ __global__ void __launch_bounds__(kFftThreads, r2c_min_waves<S>()) k_row_r2c(S s
 // mode 1 (psi): P = -F/|k|^2 after the spectrum move between grids and the circular cut-off
 //   (Cuda-CMM/src/numerical/cmm-fft.cu:294-323, :267-286, :55-78).
 // mode 2 (spectrum given): T already holds the column spectrum; no forward transform, no scaling.
+// mode 3 (Laplacian, nin == nout): P = -|k|^2 F.
 struct ColParams {
     int nin, login, nout, logout, nx_in, nx_out;  // nx_* = grid sizes along x (for the move rule)
     double inv_norm;
@@ -732,6 +749,12 @@ __global__ void __launch_bounds__(kFftThreads, 4) k_col(ColParams p, const cd *_
             } else if (p.mode == 0) {
                 cd f = s[sw(y)];
                 v = {f.x * p.inv_norm, f.y * p.inv_norm};
+            } else if (p.mode == 3) {
+                // Laplacian, k_fft_lap_h (Cuda-CMM/src/numerical/cmm-fft.cu:28-51)
+                cd f = s[sw(y)];
+                const double kyv = p.fky * (y - (y > p.nout / 2) * p.nout);
+                const double k2 = kxv * kxv + kyv * kyv;
+                v = {-(f.x * p.inv_norm) * k2, -(f.y * p.inv_norm) * k2};
             } else {
                 v = s[sw(y)];
             }
@@ -1406,6 +1429,24 @@ int op_interleave4(const cmm_grid &g, const double *H, double *H4, hipStream_t s
 }
 
 size_t op_hermite_workspace(const cmm_grid &g) { return spec_workspace_bytes(g, g, 0, 0.0); }
+
+// Hermite planes of F (op 0), of the inverse Laplacian -F/|k|^2 with DC 0 (op 1: i_laplacian,
+// Cuda-CMM/src/numerical/cmm-fft.cu:441-447) or of the Laplacian -|k|^2 F (op 2: laplacian,
+// :432-438) of a real field; planes 1 and 2 are grad_x and grad_y of the result (:450-474).
+int op_spectral_from_real(const cmm_grid &g, const double *real, int op, double *herm, void *work, size_t work_bytes,
+                          hipStream_t st) {
+    if (int rc = fft_grid_ok(g, "grid")) return rc;
+    if (op < 0 || op > 2) {
+        set_error("spectral operator must be 0 (Hermite), 1 (inverse Laplacian) or 2 (Laplacian)");
+        return CMM_ERR_ARG;
+    }
+    const int mode = op == 0 ? 0 : op == 1 ? 1 : 3;
+    SpecPlan sp = plan_spec(g, g, mode, op == 1 ? 1e300 : 0.0);
+    void *w;
+    if (int rc = get_work(work, work_bytes, sp.t_bytes + 2 * sp.g_bytes + 256, &w)) return rc;
+    SampleReal s{real, g.nx};
+    return run_spec(sp, s, herm, w, st);
+}
 
 int op_hermite_from_real(const cmm_grid &g, const double *real, double *herm, void *work, size_t work_bytes,
                          hipStream_t st) {

@@ -322,10 +322,96 @@ struct Run {
                " \t Wmax = " + f8(m[3]);
     }
 
+    // sample_compute_and_write, Cuda-CMM/src/simulation/cmm-simulation-host.cu:706-893: fields
+    // through the whole map stack on each due sample grid, written like the computational data,
+    // with conservation values on that grid
+    std::string sample(double t_now, double dt_now, double dt) {
+        if (s.simulation_type != "cmm_euler_2d") return "";
+        std::string msg;
+        auto f8 = [](double v) {
+            std::ostringstream os;
+            os.precision(8);
+            os << v;
+            return os.str();
+        };
+        const double *b = s.domain_bounds;
+        for (size_t i = 0; i < s.save_sample.size(); ++i) {
+            const auto &c = s.save_sample[i];
+            if (!save_due(c.is_instant, c.time_start, c.time_end, c.time_step, t_now, dt_now, dt)) continue;
+            const int N = c.grid;
+            const std::string tag = to_s(N);
+            const cmm_grid gs{N, N, b[0], b[1], b[2], b[3]};
+            const size_t n = (size_t)N * N;
+            const double area = ((b[1] - b[0]) / (float)N) * ((b[3] - b[2]) / (float)N);
+            const std::string &var = c.var;
+            const bool all = var.find("All") != std::string::npos;
+            auto has = [&](const char *k) { return var.find(k) != std::string::npos; };
+            if (s.verbose >= 4) append1(s, "/Monitoring_data/Debug_globals/Time_s_" + tag, t_now);
+            const bool scalar = has("Scalar") || has("Theta") || all;
+            if (scalar && s.scalar_discrete) fail("a discrete passive scalar is not provided by this engine", 1);
+            // scalar numbers, Cuda-CMM/src/ui/settings.h:431-434
+            const int scalar_num = s.scalar_name == "rectangle" ? 0 : s.scalar_name == "gaussian" ? 1
+                                 : s.scalar_name == "circular_ring" ? 2 : -1;
+            const int scalar_ic = scalar && scalar_num >= 0 ? CMM_IC_SCALAR_RECTANGLE + scalar_num : -1;
+            std::vector<double> out(CMM_SAMPLE_FIELDS * n);
+            check(cmm_euler_sample(e, gs, scalar_ic, out.data()), "cmm_euler_sample");
+            auto f = [&](int k) { return out.data() + (size_t)k * n; };
+            std::string ts = std::abs(t_now - kTMax) < 1 ? "final" : fmt3(t_now);
+            const std::string sub = "/Time_data/Time_" + ts;
+            mkdir_p(s.workspace + "data/" + s.file_name + sub);
+            if (has("Map_b") || has("Chi_b") || all) {
+                write_var(sub, "Map_ChiX_" + tag, f(CMM_SAMPLE_CHIX), n, area);
+                write_var(sub, "Map_ChiY_" + tag, f(CMM_SAMPLE_CHIY), n, area);
+            }
+            if (scalar) {
+                write_var(sub, "Scalar_Theta_" + tag, f(CMM_SAMPLE_THETA), n, area);
+                // Compute_avg (Cuda-CMM/src/numerical/cmm-mesure.cu:225-228): h^2 max(0, max)
+                double mx = 0.0;
+                for (size_t k = 0; k < n; ++k) mx = std::max(mx, f(CMM_SAMPLE_THETA)[k]);
+                append1(s, "/Monitoring_data/Mesure/Scalar_integral_" + tag, area * mx);
+            }
+            const double *w = f(CMM_SAMPLE_W);
+            if (has("Vorticity") || has("W") || has("F") || all) write_var(sub, "Vorticity_W_" + tag, w, n, area);
+            double ens = 0, pal = 0, eng = 0, wmax = w[0];
+            for (size_t k = 0; k < n; ++k) {
+                ens += w[k] * w[k];
+                pal += f(CMM_SAMPLE_GRADX_W)[k] * f(CMM_SAMPLE_GRADX_W)[k] + f(CMM_SAMPLE_GRADY_W)[k] * f(CMM_SAMPLE_GRADY_W)[k];
+                eng += f(CMM_SAMPLE_UX)[k] * f(CMM_SAMPLE_UX)[k] + f(CMM_SAMPLE_UY)[k] * f(CMM_SAMPLE_UY)[k];
+                wmax = std::max(wmax, w[k]);
+            }
+            double m[4];
+            m[0] = 0.5 * area * eng;
+            m[1] = 0.5 * area * ens;
+            m[2] = 0.5 * area * pal;
+            // the reference takes the maximum twice (Compute_max for w_max and w_min), so this is |max w|
+            m[3] = std::max(wmax, -wmax);
+            if (has("Laplacian_W") || all) write_var(sub, "Laplacian_W_" + tag, f(CMM_SAMPLE_LAP_W), n, area);
+            if (has("Grad_W") || all) {
+                write_var(sub, "GradX_W_" + tag, f(CMM_SAMPLE_GRADX_W), n, area);
+                write_var(sub, "GradY_W_" + tag, f(CMM_SAMPLE_GRADY_W), n, area);
+            }
+            if (has("Stream") || has("Psi") || all) write_var(sub, "Stream_function_Psi_" + tag, f(CMM_SAMPLE_PSI), n, area);
+            if (has("Velocity") || has("U") || all) {
+                write_var(sub, "Velocity_UX_" + tag, f(CMM_SAMPLE_UX), n, area);
+                write_var(sub, "Velocity_UY_" + tag, f(CMM_SAMPLE_UY), n, area);
+            }
+            append1(s, "/Monitoring_data/Mesure/Time_s_" + tag, t_now);
+            append1(s, "/Monitoring_data/Mesure/Energy_" + tag, m[0]);
+            append1(s, "/Monitoring_data/Mesure/Enstrophy_" + tag, m[1]);
+            append1(s, "/Monitoring_data/Mesure/Palinstrophy_" + tag, m[2]);
+            append1(s, "/Monitoring_data/Mesure/Max_vorticity_" + tag, m[3]);
+            msg += "Processed sample data " + to_s((int)i + 1) + " on grid " + tag + ", Cons : Energ = " + f8(m[0]) +
+                   " \t Enstr = " + f8(m[1]) + " \t Palinstr = " + f8(m[2]) + " \t Wmax = " + f8(m[3]);
+        }
+        return msg;
+    }
+
     void save(double t_now, double dt_now, double dt) {
         std::string m = write_time_step(t_now, dt_now, dt);
         if (s.verbose >= 3 && !m.empty()) log.push(m);
         m = conservation(t_now, dt_now, dt);
+        if (s.verbose >= 3 && !m.empty()) log.push(m);
+        m = sample(t_now, dt_now, dt);
         if (s.verbose >= 3 && !m.empty()) log.push(m);
     }
 };
@@ -393,7 +479,8 @@ int run_euler(Settings &s) {
     if (s.restart_time != 0) fail("restart from a saved map stack is not supported by this engine yet", 1);
     if (!vp && s.initial_condition_num >= 7 && s.initial_condition_num <= 8)
         fail("initial condition " + s.initial_condition + " draws from cuRAND/std distributions and is not provided", 230305);
-    if (!s.save_sample.empty() && s.verbose >= 2) log.push("Note: save_sample outputs are not produced by this engine");
+    if (vp && !s.save_sample.empty() && s.verbose >= 2)
+        log.push("Note: save_sample outputs of the Vlasov-Poisson flow are not produced by this engine");
     if (!s.save_zoom.empty() && s.verbose >= 2) log.push("Note: save_zoom outputs are not produced by this engine");
 
     const double *b = s.domain_bounds;

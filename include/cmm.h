@@ -92,6 +92,13 @@ int cmm_hermite_from_real(cmm_grid g, const double *real, double *herm, void *wo
  * interleaved), no scaling: the C2R half of fourier_hermite. */
 int cmm_hermite_from_spectrum(cmm_grid g, const double *spec_cols, double *herm, void *work, size_t work_bytes,
                               void *stream);
+/* Hermite planes [g, g_x, g_y, g_xy] of g = F (op 0), of the inverse Laplacian -F/|k|^2 with the
+ * mean removed (op 1) or of the Laplacian -|k|^2 F (op 2) of a real periodic field F; replaces
+ * fourier_hermite, i_laplacian, laplacian, grad_x and grad_y on a sample grid
+ * (Cuda-CMM/src/numerical/cmm-fft.cu:398-474).  Powers of two, 8..4096. */
+size_t cmm_spectral_workspace_bytes(cmm_grid g);
+int cmm_spectral_from_real(cmm_grid g, const double *real, int op, double *herm, void *work, size_t work_bytes,
+                           void *stream);
 /* psi Hermite from the map and the fine vorticity: sample omega(X(x)) on g_vort (mollifier 0/4/8),
  * R2C, /N, move to the psi spectrum, circular cut-off at freq_cut, -1/|k|^2, fourier_hermite;
  * replaces evaluate_stream_hermite (Cuda-CMM/src/simulation/cmm-simulation-host.cu:333-375).
@@ -193,6 +200,19 @@ int cmm_euler_diagnostics(cmm_euler *e, double *out);
  * out[3] Mass = hx hv sum f, out[4] Momentum = hx hv sum f v, out[5..8] |E_k|, k = 1..4, of the
  * first row of dpsi/dx (R2C / NX). */
 int cmm_euler_vp_diagnostics(cmm_euler *e, double *out);
+/* Fields through the whole map stack on a sample grid, the computational part of
+ * sample_compute_and_write (Cuda-CMM/src/simulation/cmm-simulation-host.cu:706-893): host_out
+ * receives 10 arrays of g_sample.nx*g_sample.ny doubles: [CMM_SAMPLE_CHIX, CMM_SAMPLE_CHIY] the
+ * composed positions, CMM_SAMPLE_W the initial condition there, CMM_SAMPLE_THETA the passive
+ * scalar scalar_ic there (zeros when scalar_ic < 0), CMM_SAMPLE_LAP_W, CMM_SAMPLE_GRADX_W,
+ * CMM_SAMPLE_GRADY_W, CMM_SAMPLE_PSI (inverse Laplacian of W), CMM_SAMPLE_UX, CMM_SAMPLE_UY (its
+ * x and y gradients).  g_sample: powers of two, 8..4096, same bounds as the map. */
+enum {
+    CMM_SAMPLE_CHIX = 0, CMM_SAMPLE_CHIY = 1, CMM_SAMPLE_W = 2, CMM_SAMPLE_THETA = 3, CMM_SAMPLE_LAP_W = 4,
+    CMM_SAMPLE_GRADX_W = 5, CMM_SAMPLE_GRADY_W = 6, CMM_SAMPLE_PSI = 7, CMM_SAMPLE_UX = 8, CMM_SAMPLE_UY = 9,
+    CMM_SAMPLE_FIELDS = 10
+};
+int cmm_euler_sample(cmm_euler *e, cmm_grid g_sample, int scalar_ic, double *host_out);
 /* Steps taken, current time, stack size. */
 int cmm_euler_state(cmm_euler *e, int *steps, double *t, int *stack_size);
 /* Per-kernel timing with events on the engine stream: enable (clears the counters), then read

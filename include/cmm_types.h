@@ -59,7 +59,12 @@ enum {
     CMM_IC_RANDOM_SPECTRUM = 101,       /* discrete: seeded phases, E(k) ~ k^4 exp(-2 (k/6)^2), enstrophy 1 */
     CMM_IC_LANDAU_DAMPING = 200,        /* (1 + 0.05 cos 0.5x) exp(-v^2/2)/sqrt(2 pi) */
     CMM_IC_TWO_STREAM = 201,            /* (1 + 0.05 cos 0.2x)(exp(-(v-3)^2/2) + exp(-(v+3)^2/2))/(2 sqrt(2 pi)) */
-    CMM_IC_NONLIN_LANDAU_DAMPING = 202  /* (1 + 0.5 cos 0.5x) exp(-v^2/2)/sqrt(2 pi) */
+    CMM_IC_NONLIN_LANDAU_DAMPING = 202, /* (1 + 0.5 cos 0.5x) exp(-v^2/2)/sqrt(2 pi) */
+    /* passive scalars, 300 + the reference's number (Cuda-CMM/src/ui/settings.h:431-434,
+     * d_init_scalar, Cuda-CMM/src/simulation/cmm-init.cu:249-301) */
+    CMM_IC_SCALAR_RECTANGLE = 300,
+    CMM_IC_SCALAR_GAUSSIAN = 301,
+    CMM_IC_SCALAR_CIRCULAR_RING = 302
 };
 
 /* Error codes returned by every C-ABI call (0 = success). */

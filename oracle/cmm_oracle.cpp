@@ -433,6 +433,23 @@ double init_vorticity(double x, double y, int ic, const double *prm) {
                (std::exp(-(y - 3) * (y - 3) * 0.5) + std::exp(-(y + 3) * (y + 3) * 0.5));
     case CMM_IC_NONLIN_LANDAU_DAMPING:
         return (1 + 0.5 * std::cos(0.5 * x)) * std::exp(-y * y * 0.5) / std::sqrt(2 * kPi);
+    // passive scalars, d_init_scalar (Cuda-CMM/src/simulation/cmm-init.cu:249-301)
+    case CMM_IC_SCALAR_RECTANGLE: {
+        x -= std::floor(x / kTwoPi) * kTwoPi;
+        y -= std::floor(y / kTwoPi) * kTwoPi;
+        const double cx0 = kPi, cy0 = kPi / 2.0, wx = 2 * kPi, wy = kPi;
+        return (x > cx0 - wx / 2.0 && x < cx0 + wx / 2.0 && y > cy0 - wy / 2.0 && y < cy0 + wy / 2.0) ? 1.0 : 0.0;
+    }
+    case CMM_IC_SCALAR_GAUSSIAN: {
+        const double mu = kPi, var = kPi / 4.0;
+        return 1.0 / (kTwoPi * var * var) *
+               std::exp(-((x - mu) * (x - mu) / (2 * var * var) + (y - mu) * (y - mu) / (2 * var * var)));
+    }
+    case CMM_IC_SCALAR_CIRCULAR_RING: {
+        const double r = kPi / 2.0, rw = r / 8.0, s1 = 10.0, s2 = 10.0;
+        const double d2 = (x - kPi) * (x - kPi) + (y - kPi) * (y - kPi);
+        return (std::tanh(s1 * (d2 - (r + rw) * (r + rw))) - std::tanh(s2 * (d2 - (r - rw) * (r - rw)))) / 2.0;
+    }
     case CMM_IC_4_NODES: {
         x -= std::floor(x / kTwoPi) * kTwoPi;
         y -= std::floor(y / kTwoPi) * kTwoPi;
@@ -935,6 +952,29 @@ int cmmo_psi_from_vort(cmm_grid gm, cmm_grid gv, cmm_grid gf, cmm_grid gp, const
                        const double *H, int molly, double fcut, double *psi) {
     if (!pow2(gv.nx) || !pow2(gp.nx) || gv.nx != gv.ny || gp.nx != gp.ny) return CMM_ERR_ARG;
     psi_from_vort(mk(gm), mk(gv), mk(gf), mk(gp), cx, cy, H, molly, fcut, psi);
+    return CMM_OK;
+}
+
+// Hermite planes of F, of i_laplacian(F) (-F/|k|^2, DC 0) or of laplacian(F) (-|k|^2 F),
+// Cuda-CMM/src/numerical/cmm-fft.cu:28-78, :398-474
+int cmmo_spectral_from_real(cmm_grid gg, const double *real, int op, double *herm) {
+    const Grid g = mk(gg);
+    if (!pow2(g.nx) || !pow2(g.ny) || op < 0 || op > 2) return CMM_ERR_ARG;
+    const int nh = g.nx / 2 + 1;
+    std::vector<cplx> F((size_t)nh * g.ny);
+    rfft2(g.nx, g.ny, real, F.data());
+    const double fx = kfac(g, 0), fy = kfac(g, 1);
+    for (int iy = 0; iy < g.ny; ++iy)
+        for (int ix = 0; ix < nh; ++ix) {
+            const long i = (long)iy * nh + ix;
+            cplx v(F[i].real() / (double)g.n, F[i].imag() / (double)g.n);
+            const double kx = fx * (ix - (ix > g.nx / 2) * g.nx), ky = fy * (iy - (iy > g.ny / 2) * g.ny);
+            const double k2 = kx * kx + ky * ky;
+            if (op == 1) v = i == 0 ? cplx(0, 0) : cplx(-v.real() / k2, -v.imag() / k2);
+            if (op == 2) v = cplx(-v.real() * k2, -v.imag() * k2);
+            F[i] = v;
+        }
+    fourier_hermite(g, F.data(), herm);
     return CMM_OK;
 }
 

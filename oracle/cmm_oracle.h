@@ -62,6 +62,9 @@ int cmmo_hermite_from_real(cmm_grid g, const double *real, double *herm);
 int cmmo_psi_from_vort(cmm_grid gm, cmm_grid gvort, cmm_grid gfine, cmm_grid gpsi,
                        const double *chix, const double *chiy, const double *w_herm,
                        int molly, double freq_cut, double *psi_out);
+/* Hermite planes of F (op 0), i_laplacian(F) (op 1) or laplacian(F) (op 2),
+ * Cuda-CMM/src/numerical/cmm-fft.cu:28-78, :398-474 */
+int cmmo_spectral_from_real(cmm_grid g, const double *real, int op, double *herm);
 /* Vlasov-Poisson psi(x, v) from the distribution function's Hermite field,
  * Cuda-CMM/src/simulation/cmm-simulation-host.cu:385-471, :478-494, :534-571 (see cmm_oracle.cpp) */
 int cmmo_vp_psi_from_f(cmm_grid gm, cmm_grid gvort, cmm_grid gfine, cmm_grid gpsi, const double *chix,

@@ -184,6 +184,13 @@ def ic_random_spectrum(n, seed):
     return out
 
 
+def spectral_from_real(g, real, op):
+    out = np.zeros(4 * g.nx * g.ny)
+    rc = lib().cmmo_spectral_from_real(g, ptr(np.ascontiguousarray(real, dtype=np.float64).ravel()), op, ptr(out))
+    assert rc == 0, rc
+    return out
+
+
 def vp_psi_from_f(gm, gv, gf, gp, cx, cy, f_herm, molly=0):
     out = np.zeros(4 * gp.nx * gp.ny)
     rc = lib().cmmo_vp_psi_from_f(gm, gv, gf, gp, ptr(cx), ptr(cy), ptr(f_herm), molly, ptr(out))

@@ -251,3 +251,18 @@ def test_vp_psi_from_f(dev, vp_base):
     with pytest.raises(cmm.CmmError):
         cmm.evaluate_potential_from_density_hermite(gm, gf, gf, cmm.vp_grid(64), dev_array(mx), dev_array(my),
                                                     dev_array(W), dev_array(np.zeros(4 * 64 * 64)), 0)
+
+
+def test_spectral_operators(dev):
+    """Hermite planes of F, i_laplacian(F) and laplacian(F) of a random field on the GPU against
+    the oracle (derivative planes in their own scale)."""
+    import cmm
+    for n in (64, 256):
+        f = random_field(n, 11)
+        for op in (0, 1, 2):
+            ref = O.spectral_from_real(O.grid(n), f.ravel(), op)
+            out = dev_array(np.zeros(4 * n * n))
+            cmm.spectral_from_real(cmm.grid(n), dev_array(f.ravel()), op, out)
+            errs = hermite_errors(host(out), ref, n * n, kmax(n))
+            print(f"spectral n={n} op={op} per-plane errors {errs}")
+            assert max(errs) <= TOL

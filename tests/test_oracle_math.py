@@ -392,3 +392,24 @@ def test_vp_psi_landau_density():
     # a vort grid finer than the psi grid is rejected (the reference's error 1013)
     with pytest.raises(AssertionError):
         O.vp_psi_from_f(g, g, g, O.vp_grid(64), cx, cy, F)
+
+
+def test_spectral_operators_closed_form():
+    """laplacian(cos 2x cos 3y) = -13 cos 2x cos 3y; i_laplacian gives -1/13 of it (mean removed:
+    a constant added to F disappears); grad planes are the exact derivatives."""
+    n = 64
+    g = O.grid(n)
+    h = TWO_PI / float(np.float32(n))
+    x = np.arange(n) * h
+    X, Y = np.meshgrid(x, x)
+    F = np.cos(2 * X) * np.cos(3 * Y) + 0.7
+    base = np.cos(2 * X) * np.cos(3 * Y)
+    nn = n * n
+    lap = O.spectral_from_real(g, F.ravel(), 2)
+    assert np.max(np.abs(lap[:nn] - (-13 * base).ravel())) < 1e-11
+    il = O.spectral_from_real(g, F.ravel(), 1)
+    assert np.max(np.abs(il[:nn] - (-base / 13).ravel())) < 1e-13
+    assert np.max(np.abs(il[nn:2 * nn] - (2 * np.sin(2 * X) * np.cos(3 * Y) / 13).ravel())) < 1e-13
+    assert np.max(np.abs(il[2 * nn:3 * nn] - (3 * np.cos(2 * X) * np.sin(3 * Y) / 13).ravel())) < 1e-13
+    he = O.spectral_from_real(g, F.ravel(), 0)
+    assert np.max(np.abs(he - O.hermite_from_real(g, F.ravel()))) == 0.0
