@@ -165,13 +165,8 @@ CMM_HD double herm_eval(const DGrid &g, const double *__restrict__ H, double x, 
     return hsum_v(v, bx, by, g.hx, g.hy);
 }
 
-// herm_eval of a field stored node-interleaved (H4[4*i + plane]); same arithmetic as herm_eval
-CMM_HD double herm_eval_il(const DGrid &g, const double *__restrict__ H4, double x, double y) {
-    Cell c;
-    find_cell(g, x, y, c);
-    double bx[4], by[4], v[16];
-    basis(c.dx, bx);
-    basis(c.dy, by);
+// load16 of a field stored node-interleaved (H4[4*i + plane]): two 16-byte loads per corner
+CMM_HD void load16_il(const double *__restrict__ H4, const Cell &c, double v[16]) {
     const int idx[4] = {c.i00, c.i10, c.i01, c.i11};
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -182,7 +177,23 @@ CMM_HD double herm_eval_il(const DGrid &g, const double *__restrict__ H4, double
         v[8 + q] = b.x;
         v[12 + q] = b.y;
     }
+}
+
+// herm_eval at a point whose cell c has been found, with the corner values given
+CMM_HD double herm_eval_vals(const DGrid &g, const Cell &c, const double v[16]) {
+    double bx[4], by[4];
+    basis(c.dx, bx);
+    basis(c.dy, by);
     return hsum_v(v, bx, by, g.hx, g.hy);
+}
+
+// herm_eval of a field stored node-interleaved (H4[4*i + plane]); same arithmetic as herm_eval
+CMM_HD double herm_eval_il(const DGrid &g, const double *__restrict__ H4, double x, double y) {
+    Cell c;
+    find_cell(g, x, y, c);
+    double v[16];
+    load16_il(H4, c, v);
+    return herm_eval_vals(g, c, v);
 }
 
 // Velocity (d psi/dy, -d psi/dx) of up to four psi levels at one point,
@@ -261,12 +272,9 @@ CMM_HD void diffeo_vals(const DGrid &g, const Cell &c, const double vx[16], cons
 
 CMM_HD bool same_cell(const Cell &a, const Cell &b) { return a.i00 == b.i00 && a.wx0 == b.wx0 && a.wy0 == b.wy0; }
 
-// diffeo() on a map stored node-interleaved (M8[8*i + p]: planes of X, then of Y), each corner one
-// 64-byte record read with four 16-byte loads; same arithmetic as diffeo.
-CMM_HD void diffeo_il(const DGrid &g, const double *__restrict__ M8, double x, double y, double &xo, double &yo) {
-    Cell c;
-    find_cell(g, x, y, c);
-    double vx[16], vy[16];
+// the corner values of a map stored node-interleaved (M8[8*i + p]: planes of X, then of Y), each
+// corner one 64-byte record read with four 16-byte loads
+CMM_HD void load_map_il(const double *__restrict__ M8, const Cell &c, double vx[16], double vy[16]) {
     const int idx[4] = {c.i00, c.i10, c.i01, c.i11};
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
@@ -281,6 +289,14 @@ CMM_HD void diffeo_il(const DGrid &g, const double *__restrict__ M8, double x, d
         vy[8 + q] = e.x;
         vy[12 + q] = e.y;
     }
+}
+
+// diffeo() on a map stored node-interleaved; same arithmetic as diffeo
+CMM_HD void diffeo_il(const DGrid &g, const double *__restrict__ M8, double x, double y, double &xo, double &yo) {
+    Cell c;
+    find_cell(g, x, y, c);
+    double vx[16], vy[16];
+    load_map_il(M8, c, vx, vy);
     diffeo_vals(g, c, vx, vy, xo, yo);
 }
 

@@ -589,10 +589,52 @@ __device__ void SampleVortT<IL>::quad(int x, int ya, int yb, cd *s) const {
         for (int i = 0; i < 4; ++i) s[sw(x + i)] = cd{(*this)(x + i, ya), (*this)(x + i, yb)};
         return;
     }
+    double vx[16], vy[16];
+    {
+        // floor is monotone, so the first and last point sharing a map cell puts all eight there
+        // (always, when the vorticity grid refines the map grid 4x or more): the positions are
+        // computed first, then the vorticity gathers are issued four points at a time so that
+        // their latencies overlap
+        Cell cf, cl;
+        find_cell(gm, gm.x0 + x * gv.hx, gm.y0 + ya * gv.hy, cf);
+        find_cell(gm, gm.x0 + (x + 3) * gv.hx, gm.y0 + yb * gv.hy, cl);
+        if (same_cell(cf, cl)) {
+            load16(cx, gm.n, cf, vx);
+            load16(cy, gm.n, cf, vy);
+            double px[8], py[8];
+#pragma unroll
+            for (int p = 0; p < 8; ++p) {
+                const int i = p & 3, j = p >> 2;
+                Cell c;
+                find_cell(gm, gm.x0 + (x + i) * gv.hx, gm.y0 + (j ? yb : ya) * gv.hy, c);
+                diffeo_vals(gm, c, vx, vy, px[p], py[p]);
+            }
+            // one point per batch: the unrolled loop lets the compiler overlap the gathers of
+            // several points as far as registers allow (larger explicit batches spill)
+            constexpr int B = 1;
+#pragma unroll
+            for (int h = 0; h < 8 / B; ++h) {
+                Cell wc[B];
+                double wv[B][16];
+#pragma unroll
+                for (int i = 0; i < B; ++i) {
+                    find_cell(gf, px[B * h + i], py[B * h + i], wc[i]);
+                    if constexpr (IL) load16_il(W, wc[i], wv[i]);
+                    else load16(W, gf.n, wc[i], wv[i]);
+                }
+#pragma unroll
+                for (int i = 0; i < B; ++i) {
+                    const int p = B * h + i;
+                    double *d = &s[sw(x + (p & 3))].x;
+                    d[p >> 2] = herm_eval_vals(gf, wc[i], wv[i]);
+                }
+            }
+            return;
+        }
+    }
     // corner values of the map cell last loaded (a one-entry cache keyed by the cell)
     Cell c0;
     c0.i00 = -1;
-    double vx[16], vy[16];
     for (int p = 0; p < 8; ++p) {
         const int i = p & 3, j = p >> 2;
         const double xp = gm.x0 + (x + i) * gv.hx, yp = gm.y0 + (j ? yb : ya) * gv.hy;

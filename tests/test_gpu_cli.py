@@ -77,6 +77,10 @@ def compare(mon, case):
                 tol = 1e-10 * np.abs(ref) + 1e-12
             elif "_H_coarse_" in key:
                 tol = 1e-10 * np.abs(ref) + 1e-8
+            elif "Laplacian_W_" in key:
+                # the spectral Laplacian on the 256^2 sample grid multiplies the rounding noise of the
+                # sampled vorticity (~1e-14 after the map layers) by up to k_max^2 = 128^2
+                tol = 1e-10 * np.abs(ref) + 1e-9
             else:
                 tol = 1e-10 * np.abs(ref) + 1e-12
             bad = np.abs(got - ref) > tol

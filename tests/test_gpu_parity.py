@@ -89,7 +89,10 @@ def test_compose_and_sample_with_stack(dev, base):
     ref_xy = O.compose_positions(ogc, ogf, mx, my, sx, sy, 2)
     xy = dev_array(np.zeros(2 * GF * GF))
     cmm.apply_map_stack(gf, gc, dev_array(mx), dev_array(my), dev_array(stack), 2, xy)
-    e = rel_linf(host(xy), ref_xy)
+    # positions are wrapped into [0, 2 pi) after each layer: one within rounding of the boundary
+    # can come out on either side, so they are compared as points of the torus
+    d = np.mod(host(xy) - ref_xy + np.pi, 2 * np.pi) - np.pi
+    e = np.max(np.abs(d)) / np.max(np.abs(ref_xy))
     print("compose rel Linf", e)
     assert e <= TOL
     ref_s = O.sample_init(ogf, ref_xy, 0)
