@@ -47,7 +47,8 @@ def algorithmic(slot, c, kx_max, n_stack=0):
         # L psi levels read, C1, C12 and the level-0 copy written (4 planes each)
         return 32 * (L + 3) * npsi, 2 * 2 * L * 4 * npsi
     if slot == "incomp":
-        return 64 * nc, 320.0 * nf
+        # map read once; separable Hermite sums over 4x4 tiles of check points, about 85 flop each
+        return 64 * nc, 85.0 * nf
     if slot == "psi_row_r2c":
         # map + fine omega Hermite read, transposed half-spectrum columns kx <= kx_max written
         return 64 * nc + 32 * nf + 16 * (kx_max + 1) * c["n_vort"], 270.0 * nv + 2.5 * nv * 24
@@ -109,7 +110,7 @@ def cpu_baseline(c, threads, budget_s=12.0):
 
 
 # bench slot -> kernel-name pattern of the launches it times (engine psi solve samples omega with SampleVort)
-SLOT_KERNELS = {"advect": "k_map_advect<30, 8, true>", "psi_combine": "k_psi_combine<3>", "incomp": "k_incomp_cells(",
+SLOT_KERNELS = {"advect": "k_map_advect<30, 8, true>", "psi_combine": "k_psi_combine<3>", "incomp": "k_incomp_tensor<",
                 "psi_row_r2c": "k_row_r2c<cmm::SampleVortT<true>", "psi_col": "k_col(", "psi_row_c2r": "k_row_c2r4(",
                 "particles": "k_particles("}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "cmm_euler_c2_hbm_traffic.json")

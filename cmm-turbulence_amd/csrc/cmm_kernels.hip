@@ -336,57 +336,97 @@ __global__ void __launch_bounds__(256) k_incomp(DGrid gm, DGrid gc, const double
     }
 }
 
-// Same check when the check grid refines the map grid by an integer ratio r over the same domain:
-// one lane per map cell loads the cell's 32 corner values once and evaluates the r*r check points
-// inside it (a check point whose cell differs, never expected, reads its own corners), so each
-// point computes exactly what k_incomp computes with 1/r^2 of the gathers.
-__global__ void __launch_bounds__(256) k_incomp_cells(DGrid gm, DGrid gc, const double *__restrict__ cx,
-                                                      const double *__restrict__ cy, unsigned long long *__restrict__ out,
-                                                      int r, int tiles_x, int tiles_total) {
+// Same check when the check grid refines the map grid by an integer ratio r over the same domain,
+// with the Hermite sums in separable form: a lane takes a TX x TY tile of check points inside one
+// map cell (TX and TY divide r), loads the cell's corner values once, contracts them along x for
+// the tile's TX columns and then along y per point.  The terms are those of hsum_v (derivative planes scaled by h, value planes with their
+// wrap jumps) grouped by the x basis first, so the values agree with diffeo_det to rounding.  A
+// tile whose first and last points fall in different cells is evaluated point by point.
+template <int TX, int TY>
+__global__ void __launch_bounds__(256) k_incomp_tensor(DGrid gm, DGrid gc, const double *__restrict__ cx,
+                                                       const double *__restrict__ cy,
+                                                       unsigned long long *__restrict__ out, int r, int ntx,
+                                                       long ntiles) {
     double v = 0.0;
-    for (int tt = blockIdx.x; tt < tiles_total; tt += gridDim.x) {
-        const int t = xcd_block(tt, tiles_total);
-        const int mx = (t % tiles_x) * 64 + threadIdx.x;
-        const int my = (t / tiles_x) * 4 + threadIdx.y;
-        if (mx >= gm.nx || my >= gm.ny) continue;
-        Cell c0;
-        find_cell(gm, gm.x0 + (mx + 0.5) * gm.hx, gm.y0 + (my + 0.5) * gm.hy, c0);
+    for (long t0 = (long)blockIdx.x * 256; t0 < ntiles; t0 += (long)gridDim.x * 256) {
+        const long t = t0 + threadIdx.x;
+        if (t >= ntiles) continue;
+        const int ix0 = (int)(t % ntx) * TX, iy0 = (int)(t / ntx) * TY;
+        double xs[TX], ys[TY];
+#pragma unroll
+        for (int a = 0; a < TX; ++a) xs[a] = gm.x0 + (ix0 + a) * gc.hx + 0.5 * gc.hx;
+#pragma unroll
+        for (int b = 0; b < TY; ++b) ys[b] = gm.y0 + (iy0 + b) * gc.hy + 0.5 * gc.hy;
+        Cell c0, c1;
+        find_cell(gm, xs[0], ys[0], c0);
+        find_cell(gm, xs[TX - 1], ys[TY - 1], c1);
+        if (!same_cell(c0, c1)) {
+#pragma unroll 1
+            for (int i = 0; i < TX * TY; ++i) {
+                const int a = i % TX, b = i / TX;
+                const double x = gm.x0 + (ix0 + a) * gc.hx + 0.5 * gc.hx, y = gm.y0 + (iy0 + b) * gc.hy + 0.5 * gc.hy;
+                v = fmax(v, fabs(1 - diffeo_det(gm, cx, cy, x, y)));
+            }
+            continue;
+        }
         double vx[16], vy[16];
         load16(cx, gm.n, c0, vx);
         load16(cy, gm.n, c0, vy);
-        for (int b = 0; b < r; ++b) {
-            const int iy = my * r + b;
-            const double y = gm.y0 + iy * gc.hy + 0.5 * gc.hy;
-            for (int a = 0; a < r; ++a) {
-                const int ix = mx * r + a;
-                const double x = gm.x0 + ix * gc.hx + 0.5 * gc.hx;
-                Cell c;
-                find_cell(gm, x, y, c);
-                double bx[4], by[4], dbx[4], dby[4];
-                basis(c.dx, bx);
-                basis(c.dy, by);
-                basis_d(c.dx, dbx);
-                basis_d(c.dy, dby);
-                const double wjx[4] = {c.wx0 * gm.jx, c.wx1 * gm.jx, c.wx0 * gm.jx, c.wx1 * gm.jx};
-                const double wjy[4] = {c.wy0 * gm.jy, c.wy0 * gm.jy, c.wy1 * gm.jy, c.wy1 * gm.jy};
-                if (!same_cell(c, c0)) {  // never expected: reload for this point's cell
-                    load16(cx, gm.n, c, vx);
-                    load16(cy, gm.n, c, vy);
-                    c0 = c;
+        const double wjx[4] = {c0.wx0 * gm.jx, c0.wx1 * gm.jx, c0.wx0 * gm.jx, c0.wx1 * gm.jx};
+        const double wjy[4] = {c0.wy0 * gm.jy, c0.wy0 * gm.jy, c0.wy1 * gm.jy, c0.wy1 * gm.jy};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            vx[q] += wjx[q];
+            vy[q] += wjy[q];
+        }
+        // y basis of the tile's rows
+        double by[TY][4], dby[TY][4];
+#pragma unroll
+        for (int b = 0; b < TY; ++b) {
+            const double fy = (ys[b] - gm.y0) * gm.ihy;
+            const double d = fy - floor(fy);
+            basis(d, by[b]);
+            basis_d(d, dby[b]);
+        }
+#pragma unroll
+        for (int a = 0; a < TX; ++a) {
+            const double fx = (xs[a] - gm.x0) * gm.ihx;
+            const double d = fx - floor(fx);
+            double bx[4], dbx[4];
+            basis(d, bx);
+            basis_d(d, dbx);
+            // P[k][n]: x-contraction of field k (0: X with dbx, 1: X with bx, 2: Y with dbx, 3: Y
+            // with bx) at y node n (0, 1) for the value planes (n) and the y-derivative planes (2 + n)
+            double P[4][4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const double *w = (k & 1) ? bx : dbx;
+                const double *f = (k < 2) ? vx : vy;
+#pragma unroll
+                for (int n = 0; n < 2; ++n) {
+                    P[k][n] = w[0] * f[2 * n] + w[1] * f[2 * n + 1] + gm.hx * (w[2] * f[4 + 2 * n] + w[3] * f[5 + 2 * n]);
+                    P[k][2 + n] =
+                        w[0] * f[8 + 2 * n] + w[1] * f[9 + 2 * n] + gm.hx * (w[2] * f[12 + 2 * n] + w[3] * f[13 + 2 * n]);
                 }
-                const double Xx = hsum_v(vx, dbx, by, gm.hx, gm.hy, wjx) * gm.ihx;
-                const double Yx = hsum_v(vy, dbx, by, gm.hx, gm.hy, wjy) * gm.ihy;
-                const double Xy = hsum_v(vx, bx, dby, gm.hx, gm.hy, wjx) * gm.ihx;
-                const double Yy = hsum_v(vy, bx, dby, gm.hx, gm.hy, wjy) * gm.ihy;
+            }
+#pragma unroll
+            for (int b = 0; b < TY; ++b) {
+                auto ysum = [&](const double *p, const double *c) {
+                    return c[0] * p[0] + c[1] * p[1] + gm.hy * (c[2] * p[2] + c[3] * p[3]);
+                };
+                const double Xx = ysum(P[0], by[b]) * gm.ihx;
+                const double Xy = ysum(P[1], dby[b]) * gm.ihx;
+                const double Yx = ysum(P[2], by[b]) * gm.ihy;
+                const double Yy = ysum(P[3], dby[b]) * gm.ihy;
                 v = fmax(v, fabs(1 - (Xx * Yy - Xy * Yx)));
             }
         }
     }
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
     __shared__ double red[4];
-    if (threadIdx.x == 0) red[threadIdx.y] = v;
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
-    if (threadIdx.x == 0 && threadIdx.y == 0) {
+    if (threadIdx.x == 0) {
         double m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
         atomicMax(out, (unsigned long long)__double_as_longlong(m));
     }
@@ -1397,12 +1437,18 @@ int op_incomp_launch(const cmm_grid &gm, const cmm_grid &gc, const double *cx, c
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, slot);
     dim3 b(64, 4);
     const int r = gc.nx / gm.nx;
-    if (r >= 1 && r <= 8 && gc.nx == r * gm.nx && gc.ny == r * gm.ny && gc.x0 == gm.x0 && gc.x1 == gm.x1 &&
+    if (r >= 1 && gc.nx == r * gm.nx && gc.ny == r * gm.ny && gc.x0 == gm.x0 && gc.x1 == gm.x1 &&
         gc.y0 == gm.y0 && gc.y1 == gm.y1) {
-        const int tx = (gm.nx + 63) / 64, ty = (gm.ny + 3) / 4, tt = tx * ty;
-        const int nb = std::min(tt, 2048);
-        hipLaunchKernelGGL(k_incomp_cells, dim3(nb), b, 0, st, dm, dc, cx, cy, slot, r, tx, tt);
-        return launch_check("k_incomp_cells");
+        // tiles of T x T points, T the largest of 4, 3, 2, 1 that divides r
+        const int T = r % 4 == 0 ? 4 : r % 3 == 0 ? 3 : r % 2 == 0 ? 2 : 1;
+        const int ntx = gc.nx / T;
+        const long ntiles = (long)ntx * (gc.ny / T);
+        const unsigned nb = (unsigned)std::min<long>((ntiles + 255) / 256, 8192);
+        if (T == 4) hipLaunchKernelGGL((k_incomp_tensor<4, 4>), dim3(nb), dim3(256), 0, st, dm, dc, cx, cy, slot, r, ntx, ntiles);
+        else if (T == 3) hipLaunchKernelGGL((k_incomp_tensor<3, 3>), dim3(nb), dim3(256), 0, st, dm, dc, cx, cy, slot, r, ntx, ntiles);
+        else if (T == 2) hipLaunchKernelGGL((k_incomp_tensor<2, 2>), dim3(nb), dim3(256), 0, st, dm, dc, cx, cy, slot, r, ntx, ntiles);
+        else hipLaunchKernelGGL((k_incomp_tensor<1, 1>), dim3(nb), dim3(256), 0, st, dm, dc, cx, cy, slot, r, ntx, ntiles);
+        return launch_check("k_incomp_tensor");
     }
     const int tx = (gc.nx + 63) / 64, ty = (gc.ny + 3) / 4, tt = tx * ty;
     const int nb = std::min(tt, 2048);

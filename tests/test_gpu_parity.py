@@ -197,9 +197,9 @@ def test_incompressibility(dev, base):
     got = cmm.incompressibility_check(gf, gc, dev_array(base["chix"]), dev_array(base["chiy"]))
     print("incomp", got, ref)
     assert abs(got - ref) <= 1e-12 * max(1.0, abs(ref))
-    # check grids that refine the map grid by 1 and 4 (one lane per map cell) and by 1.5 (one lane
-    # per check point)
-    for nchk in (256, 1024, 384):
+    # check grids that refine the map grid by 1, 3, 4 and 8 (tiles of 1, 9, 16 points inside one
+    # map cell; four tiles per cell for 8) and by 1.5 (one lane per check point)
+    for nchk in (256, 768, 1024, 2048, 384):
         ref = O.incomp_error(ogc, O.grid(nchk), base["chix"], base["chiy"])
         got = cmm.incompressibility_check(cmm.grid(nchk), gc, dev_array(base["chix"]), dev_array(base["chiy"]))
         print("incomp", nchk, got, ref)
