@@ -66,6 +66,9 @@ struct Cell {
 CMM_HD int floordiv(int a, int n) { return a >= 0 ? a / n : -((-a - 1) / n) - 1; }
 CMM_HD int floordiv_g(int a, int n, int lg) { return lg >= 0 ? (a >> lg) : floordiv(a, n); }
 
+// P2: the grid's point counts are known to be powers of two (lgx, lgy >= 0), so the wrap counts
+// are shifts, with no branch on the grid
+template <bool P2 = false>
 CMM_HD void find_cell(const DGrid &g, double x, double y, Cell &c) {
     double fx = (x - g.x0) * g.ihx, fy = (y - g.y0) * g.ihy;
     double flx = floor(fx), fly = floor(fy);
@@ -74,8 +77,13 @@ CMM_HD void find_cell(const DGrid &g, double x, double y, Cell &c) {
     flx = fmin(fmax(flx, -1.0e9), 1.0e9);
     fly = fmin(fmax(fly, -1.0e9), 1.0e9);
     int ix = (int)flx, iy = (int)fly;
-    c.wx0 = floordiv_g(ix, g.nx, g.lgx);
-    c.wy0 = floordiv_g(iy, g.ny, g.lgy);
+    if constexpr (P2) {
+        c.wx0 = ix >> g.lgx;
+        c.wy0 = iy >> g.lgy;
+    } else {
+        c.wx0 = floordiv_g(ix, g.nx, g.lgx);
+        c.wy0 = floordiv_g(iy, g.ny, g.lgy);
+    }
     int rx0 = ix - c.wx0 * g.nx, ry0 = iy - c.wy0 * g.ny;
     int rx1 = rx0 + 1, ry1 = ry0 + 1;
     c.wx1 = c.wx0; c.wy1 = c.wy0;
@@ -218,28 +226,26 @@ CMM_HD void herm_vel(const DGrid &g, const PsiLevels &P, int nl, double x, doubl
     }
 }
 
-// herm_vel of one field stored node-interleaved (H4[4*i + plane]): the four planes of a corner
-// are one 32-byte record read with two 16-byte loads; same arithmetic as herm_vel.
-CMM_HD void herm_vel_il(const DGrid &g, const double *__restrict__ H4, double x, double y, double &ux, double &uy) {
-    Cell c;
-    find_cell(g, x, y, c);
-    double bx[4], by[4], dbx[4], dby[4], v[16];
+// herm_vel of one field at a point whose cell c has been found, with the corner values given
+CMM_HD void herm_vel_vals(const DGrid &g, const Cell &c, const double v[16], double &ux, double &uy) {
+    double bx[4], by[4], dbx[4], dby[4];
     basis(c.dx, bx);
     basis(c.dy, by);
     basis_d(c.dx, dbx);
     basis_d(c.dy, dby);
-    const int idx[4] = {c.i00, c.i10, c.i01, c.i11};
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const double2 *r = reinterpret_cast<const double2 *>(H4 + 4 * (long)idx[q]);
-        const double2 a = r[0], b = r[1];
-        v[q] = a.x;
-        v[4 + q] = a.y;
-        v[8 + q] = b.x;
-        v[12 + q] = b.y;
-    }
     uy = -hsum_v(v, dbx, by, g.hx, g.hy) * g.ihx;
     ux = hsum_v(v, bx, dby, g.hx, g.hy) * g.ihy;
+}
+
+// herm_vel of one field stored node-interleaved (H4[4*i + plane]): the four planes of a corner
+// are one 32-byte record read with two 16-byte loads; same arithmetic as herm_vel.
+template <bool P2 = false>
+CMM_HD void herm_vel_il(const DGrid &g, const double *__restrict__ H4, double x, double y, double &ux, double &uy) {
+    Cell c;
+    find_cell<P2>(g, x, y, c);
+    double v[16];
+    load16_il(H4, c, v);
+    herm_vel_vals(g, c, v, ux, uy);
 }
 
 // Map evaluation with periodic jumps, Cuda-CMM/src/numerical/cmm-hermite.cu:446-460
@@ -292,9 +298,10 @@ CMM_HD void load_map_il(const double *__restrict__ M8, const Cell &c, double vx[
 }
 
 // diffeo() on a map stored node-interleaved; same arithmetic as diffeo
+template <bool P2 = false>
 CMM_HD void diffeo_il(const DGrid &g, const double *__restrict__ M8, double x, double y, double &xo, double &yo) {
     Cell c;
-    find_cell(g, x, y, c);
+    find_cell<P2>(g, x, y, c);
     double vx[16], vy[16];
     load_map_il(M8, c, vx, vy);
     diffeo_vals(g, c, vx, vy, xo, yo);

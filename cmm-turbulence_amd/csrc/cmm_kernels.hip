@@ -125,17 +125,18 @@ struct VelLevels {
 // rounding) and a copy C0 of level 0, all three node-interleaved: one 16-value gather per
 // evaluation instead of one per level, issued as 8 16-byte loads instead of 16 8-byte loads.
 // Only level 0 is read on its own by the combinable schemes.
+template <bool P2>
 struct VelCombined {
     DGrid g;
     const double *C1, *C12, *C0;
     __device__ __forceinline__ void w1(const cmm_advect_coeffs &, double x, double y, double &kx, double &ky) const {
-        herm_vel_il(g, C1, x, y, kx, ky);
+        herm_vel_il<P2>(g, C1, x, y, kx, ky);
     }
     __device__ __forceinline__ void w12(const cmm_advect_coeffs &, double x, double y, double &kx, double &ky) const {
-        herm_vel_il(g, C12, x, y, kx, ky);
+        herm_vel_il<P2>(g, C12, x, y, kx, ky);
     }
     __device__ __forceinline__ void lev(int, double x, double y, double &ux, double &uy) const {
-        herm_vel_il(g, C0, x, y, ux, uy);
+        herm_vel_il<P2>(g, C0, x, y, ux, uy);
     }
 };
 
@@ -258,7 +259,8 @@ __global__ void __launch_bounds__(256) k_interleave_map(long n, const double *__
     d[3] = double2{cy[2 * n + j], cy[3 * n + j]};
 }
 
-template <int TIN, int G, bool COMB>
+// P2 (with COMB): both grids have power-of-two point counts
+template <int TIN, int G, bool COMB, bool P2 = false>
 __global__ void __launch_bounds__(256) k_map_advect(DGrid gm, DGrid gp, const double *__restrict__ cx,
                                                     const double *__restrict__ cy, PsiLevels P, const double *C1,
                                                     const double *C12, const double *C0, const double *M8,
@@ -278,8 +280,8 @@ __global__ void __launch_bounds__(256) k_map_advect(DGrid gm, DGrid gp, const do
         const double ye = gm.y0 + iy * gm.hy + ((k >> 2) + 1) * c.eps * (1 - 2 * (((k + 1) >> 1) & 1));
         double xf, yf;
         if constexpr (COMB) {
-            step_backward<TIN>(VelCombined{gp, C1, C12, C0}, c, xe, ye, xf, yf);
-            diffeo_il(gm, M8, xf, yf, px, py);
+            step_backward<TIN>(VelCombined<P2>{gp, C1, C12, C0}, c, xe, ye, xf, yf);
+            diffeo_il<P2>(gm, M8, xf, yf, px, py);
         } else {
             step_backward<TIN>(VelLevels{gp, P}, c, xe, ye, xf, yf);
             diffeo(gm, cx, cy, xf, yf, px, py);
@@ -1401,6 +1403,7 @@ int op_map_advect(const cmm_grid &gm, const cmm_grid &gp, const double *cx, cons
         if (int rc = launch_check("k_psi_combine")) return rc;
     }
     if (mid) (void)hipEventRecord(mid, st);
+    const bool p2 = dm.lgx >= 0 && dm.lgy >= 0 && dp.lgx >= 0 && dp.lgy >= 0;
 #define CMM_ADV(T, GG, CB)                                                                                             \
     hipLaunchKernelGGL((k_map_advect<T, GG, CB>), dim3(nb), dim3(256), 0, st, dm, dp, cx, cy, P, C1, C12, C0, M8, c, ncx, \
                        ncy, m8_out, nb)
@@ -1412,7 +1415,10 @@ int op_map_advect(const cmm_grid &gm, const cmm_grid &gp, const double *cx, cons
     } while (0)
 #define CMM_ADV_T(T)                                                                                                   \
     do {                                                                                                               \
-        if (comb) CMM_ADV_G(T, true);                                                                                  \
+        if (comb && p2 && G == 8)                                                                                      \
+            hipLaunchKernelGGL((k_map_advect<T, 8, true, true>), dim3(nb), dim3(256), 0, st, dm, dp, cx, cy, P, C1,  \
+                               C12, C0, M8, c, ncx, ncy, m8_out, nb);                                                 \
+        else if (comb) CMM_ADV_G(T, true);                                                                             \
         else CMM_ADV_G(T, false);                                                                                      \
     } while (0)
     switch (c.time_integration_num) {
