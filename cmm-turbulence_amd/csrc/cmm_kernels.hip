@@ -578,17 +578,47 @@ struct SampleVortT {
         }
         return v;
     }
-    // points x..x+3 of rows ya and yb (no mollifier) into the packed row pair s[sw(x+i)] =
-    // (row ya, row yb): map cell corners are loaded once per run of points in the same cell (one
-    // load for all 8 when the vorticity grid refines the map grid 4x or more); the values equal
-    // operator().
-    __device__ void quad(int x, int ya, int yb, cd *s) const;
+    // R points (x, y0 .. y0 + R - 1) of one column: the map cell corners are loaded once when all
+    // R share the cell (the vorticity grid refines the map grid R times or more along y), and the
+    // R vorticity gathers are independent.  The values equal operator().
+    template <int R>
+    __device__ __forceinline__ void column(int x, int y0, double *out) const {
+        if (molly != 0) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) out[r] = (*this)(x, y0 + r);
+            return;
+        }
+        const double xp = gm.x0 + x * gv.hx;
+        Cell cf, cl;
+        find_cell(gm, xp, gm.y0 + y0 * gv.hy, cf);
+        find_cell(gm, xp, gm.y0 + (y0 + R - 1) * gv.hy, cl);
+        if (!same_cell(cf, cl)) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) out[r] = (*this)(x, y0 + r);
+            return;
+        }
+        double vx[16], vy[16];
+        load16(cx, gm.n, cf, vx);
+        load16(cy, gm.n, cf, vy);
+        Cell wc[R];
+        double wv[R][16];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            Cell c;
+            find_cell(gm, xp, gm.y0 + (y0 + r) * gv.hy, c);
+            double px, py;
+            diffeo_vals(gm, c, vx, vy, px, py);
+            find_cell(gf, px, py, wc[r]);
+            if constexpr (IL) load16_il(W, wc[r], wv[r]);
+            else load16(W, gf.n, wc[r], wv[r]);
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) out[r] = herm_eval_vals(gf, wc[r], wv[r]);
+    }
 };
 using SampleVort = SampleVortT<false>;
 using SampleVortIL = SampleVortT<true>;
 
-template <class S>
-constexpr bool is_vort_sampler = std::is_same_v<S, SampleVort> || std::is_same_v<S, SampleVortIL>;
 
 // Initial condition at X_1(...X_M(X_cur(x))): the current map then the stack from newest to
 // oldest, wrapping after each layer (Cuda-CMM/src/simulation/cmm-simulation-host.cu:112-152,
@@ -625,83 +655,28 @@ struct SampleCompose {
     }
 };
 
-template <bool IL>
-__device__ void SampleVortT<IL>::quad(int x, int ya, int yb, cd *s) const {
-    if (molly != 0) {
-        for (int i = 0; i < 4; ++i) s[sw(x + i)] = cd{(*this)(x + i, ya), (*this)(x + i, yb)};
-        return;
-    }
-    double vx[16], vy[16];
-    {
-        // floor is monotone, so the first and last point sharing a map cell puts all eight there
-        // (always, when the vorticity grid refines the map grid 4x or more): the positions are
-        // computed first, then the vorticity gathers are issued four points at a time so that
-        // their latencies overlap
-        Cell cf, cl;
-        find_cell(gm, gm.x0 + x * gv.hx, gm.y0 + ya * gv.hy, cf);
-        find_cell(gm, gm.x0 + (x + 3) * gv.hx, gm.y0 + yb * gv.hy, cl);
-        if (same_cell(cf, cl)) {
-            load16(cx, gm.n, cf, vx);
-            load16(cy, gm.n, cf, vy);
-            double px[8], py[8];
+// omega through the map on the whole vorticity grid into a real row-major field f, for the row
+// transform to read back.  A lane takes kSampleRows points of one column, so that consecutive
+// lanes gather from consecutive vorticity nodes (coalesced 16-byte loads) and share map cells;
+// without the FFT's LDS and barriers the gathers also keep more loads in flight than inside the
+// row kernel.
+constexpr int kSampleRows = 4;
+
+template <class S>
+__global__ void __launch_bounds__(256, 2) k_sample_field(S smp, int nx, int ny, double *__restrict__ f) {
+    constexpr int R = kSampleRows;
+    const long t = (long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (long)nx * (ny / R)) return;
+    const int x = (int)(t % nx), y0 = R * (int)(t / nx);
+    double v[R];
+    smp.template column<R>(x, y0, v);
 #pragma unroll
-            for (int p = 0; p < 8; ++p) {
-                const int i = p & 3, j = p >> 2;
-                Cell c;
-                find_cell(gm, gm.x0 + (x + i) * gv.hx, gm.y0 + (j ? yb : ya) * gv.hy, c);
-                diffeo_vals(gm, c, vx, vy, px[p], py[p]);
-            }
-            // one point per batch: the unrolled loop lets the compiler overlap the gathers of
-            // several points as far as registers allow (larger explicit batches spill)
-            constexpr int B = 1;
-#pragma unroll
-            for (int h = 0; h < 8 / B; ++h) {
-                Cell wc[B];
-                double wv[B][16];
-#pragma unroll
-                for (int i = 0; i < B; ++i) {
-                    find_cell(gf, px[B * h + i], py[B * h + i], wc[i]);
-                    if constexpr (IL) load16_il(W, wc[i], wv[i]);
-                    else load16(W, gf.n, wc[i], wv[i]);
-                }
-#pragma unroll
-                for (int i = 0; i < B; ++i) {
-                    const int p = B * h + i;
-                    double *d = &s[sw(x + (p & 3))].x;
-                    d[p >> 2] = herm_eval_vals(gf, wc[i], wv[i]);
-                }
-            }
-            return;
-        }
-    }
-    // corner values of the map cell last loaded (a one-entry cache keyed by the cell)
-    Cell c0;
-    c0.i00 = -1;
-    for (int p = 0; p < 8; ++p) {
-        const int i = p & 3, j = p >> 2;
-        const double xp = gm.x0 + (x + i) * gv.hx, yp = gm.y0 + (j ? yb : ya) * gv.hy;
-        Cell c;
-        find_cell(gm, xp, yp, c);
-        if (!same_cell(c, c0)) {
-            load16(cx, gm.n, c, vx);
-            load16(cy, gm.n, c, vy);
-            c0 = c;
-        }
-        double x2, y2;
-        diffeo_vals(gm, c, vx, vy, x2, y2);
-        const double v = wval(x2, y2);
-        double *d = &s[sw(x + i)].x;
-        d[j] = v;
-    }
+    for (int r = 0; r < R; ++r) f[(long)(y0 + r) * nx + x] = v[r];
 }
 
 template <class S>
 __device__ __forceinline__ void sample_quad(const S &smp, int x, int ya, int yb, cd *s) {
-    if constexpr (is_vort_sampler<S>) {
-        smp.quad(x, ya, yb, s);
-    } else {
-        for (int i = 0; i < 4; ++i) s[sw(x + i)] = cd{smp(x + i, ya), smp(x + i, yb)};
-    }
+    for (int i = 0; i < 4; ++i) s[sw(x + i)] = cd{smp(x + i, ya), smp(x + i, yb)};
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -747,7 +722,7 @@ __device__ __forceinline__ void wg_fft(cd *s, int n, int logn, const cd *__restr
 // sampler's gathers need more registers than that leaves
 template <class S>
 constexpr int r2c_min_waves() {
-    return is_vort_sampler<S> || std::is_same_v<S, SampleCompose> ? 2 : 4;
+    return std::is_same_v<S, SampleCompose> ? 2 : 4;
 }
 
 template <class S>
@@ -1262,7 +1237,7 @@ size_t spec_workspace_bytes(const cmm_grid &gin, const cmm_grid &gout, int mode,
 // ev (optional): four events recorded before the row transform and after each of the three kernels
 template <class S>
 static int run_spec(const SpecPlan &sp, const S &sampler, double *out, void *work, hipStream_t st,
-                    hipEvent_t *ev = nullptr) {
+                    hipEvent_t *ev = nullptr, bool ev0 = true) {
     const cd *tw_in, *tw_out, *tw_x_in, *tw_x_out;
     if (int rc = twiddles(sp.gin.nx, &tw_x_in)) return rc;
     if (int rc = twiddles(sp.gin.ny, &tw_in)) return rc;
@@ -1273,7 +1248,7 @@ static int run_spec(const SpecPlan &sp, const S &sampler, double *out, void *wor
     cd *G1 = (cd *)((char *)work + sp.t_bytes + sp.g_bytes);
     const int kxm = sp.kx_max;
     const DGrid dout = make_dgrid(sp.gout);
-    if (ev) (void)hipEventRecord(ev[0], st);
+    if (ev && ev0) (void)hipEventRecord(ev[0], st);
     if (kxm >= 0) {
         if (sp.mode != 2) {
             size_t lds = sizeof(cd) * sp.gin.nx;
@@ -1476,8 +1451,9 @@ int op_incomp(const cmm_grid &gm, const cmm_grid &gc, const double *cx, const do
     return CMM_OK;
 }
 
+// spectral buffers and the sampled vorticity field
 size_t op_psi_workspace(const cmm_grid &gv, const cmm_grid &gp, double fcut) {
-    return spec_workspace_bytes(gv, gp, 1, fcut);
+    return spec_workspace_bytes(gv, gp, 1, fcut) + sizeof(double) * (size_t)gv.nx * gv.ny;
 }
 
 int op_psi_from_vort(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf, const cmm_grid &gp, const double *cx,
@@ -1486,25 +1462,29 @@ int op_psi_from_vort(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf,
     if (int rc = fft_grid_ok(gv, "vort grid")) return rc;
     if (int rc = fft_grid_ok(gp, "psi grid")) return rc;
     SpecPlan sp = plan_spec(gv, gp, 1, fcut);
+    const size_t base = sp.t_bytes + 2 * sp.g_bytes + 256;
     void *w;
-    if (int rc = get_work(work, work_bytes, sp.t_bytes + 2 * sp.g_bytes + 256, &w)) return rc;
-    auto fill = [&](auto &s, const double *field) {
-        s.gm = make_dgrid(gm);
-        s.gv = make_dgrid(gv);
-        s.gf = make_dgrid(gf);
-        s.cx = cx;
-        s.cy = cy;
-        s.W = field;
-        s.molly = molly;
+    if (int rc = get_work(work, work_bytes, base + sizeof(double) * (size_t)gv.nx * gv.ny, &w)) return rc;
+    // omega(X(x)) on the vorticity grid into a real field after the spectral buffers, then the
+    // plain row transform of that field; the sampling counts to the row transform's profile slot
+    double *f = reinterpret_cast<double *>(static_cast<char *>(w) + base);
+    const long lanes = (long)gv.nx * (gv.ny / kSampleRows);
+    const dim3 grid((unsigned)((lanes + 255) / 256));
+    auto launch = [&](auto smp, const double *field) {
+        smp.gm = make_dgrid(gm);
+        smp.gv = make_dgrid(gv);
+        smp.gf = make_dgrid(gf);
+        smp.cx = cx;
+        smp.cy = cy;
+        smp.W = field;
+        smp.molly = molly;
+        hipLaunchKernelGGL(k_sample_field<decltype(smp)>, grid, dim3(256), 0, st, smp, gv.nx, gv.ny, f);
     };
-    if (W4) {
-        SampleVortIL s;
-        fill(s, W4);
-        return run_spec(sp, s, psi, w, st, ev);
-    }
-    SampleVort s;
-    fill(s, W);
-    return run_spec(sp, s, psi, w, st, ev);
+    if (ev) (void)hipEventRecord(ev[0], st);
+    if (W4) launch(SampleVortIL{}, W4);
+    else launch(SampleVort{}, W);
+    if (int rc = launch_check("k_sample_field")) return rc;
+    return run_spec(sp, SampleReal{f, gv.nx}, psi, w, st, ev, false);
 }
 
 // node-interleaved copy of a Hermite field: H4[4j + p] = H[p*n + j]

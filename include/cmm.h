@@ -102,7 +102,9 @@ int cmm_spectral_from_real(cmm_grid g, const double *real, int op, double *herm,
 /* psi Hermite from the map and the fine vorticity: sample omega(X(x)) on g_vort (mollifier 0/4/8),
  * R2C, /N, move to the psi spectrum, circular cut-off at freq_cut, -1/|k|^2, fourier_hermite;
  * replaces evaluate_stream_hermite (Cuda-CMM/src/simulation/cmm-simulation-host.cu:333-375).
- * Sign convention of the reference: Laplace(psi) = omega, velocity (dpsi/dy, -dpsi/dx). */
+ * Sign convention of the reference: Laplace(psi) = omega, velocity (dpsi/dy, -dpsi/dx).
+ * The workspace holds the transposed half spectrum, two psi-grid column buffers and the sampled
+ * vorticity field (8 bytes per g_vort point). */
 size_t cmm_psi_workspace_bytes(cmm_grid g_vort, cmm_grid g_psi, double freq_cut);
 int cmm_psi_from_vort(cmm_grid g_map, cmm_grid g_vort, cmm_grid g_fine, cmm_grid g_psi, const double *chix,
                       const double *chiy, const double *w_herm, int molly_stencil, double freq_cut, double *psi_out,
