@@ -50,8 +50,9 @@ def algorithmic(slot, c, kx_max, n_stack=0):
         # map read once; separable Hermite sums over 4x4 tiles of check points, about 85 flop each
         return 64 * nc, 85.0 * nf
     if slot == "psi_row_r2c":
-        # map + fine omega Hermite read, transposed half-spectrum columns kx <= kx_max written
-        return 64 * nc + 32 * nf + 16 * (kx_max + 1) * c["n_vort"], 270.0 * nv + 2.5 * nv * 24
+        # sampling: map + fine omega Hermite read, omega(X(x)) written; row transform: that field read,
+        # transposed half-spectrum columns kx <= kx_max written
+        return 64 * nc + 32 * nf + 16 * nv + 16 * (kx_max + 1) * c["n_vort"], 270.0 * nv + 2.5 * nv * 24
     if slot == "psi_col":
         return 16 * (kx_max + 1) * (c["n_vort"] + 2 * c["n_psi"]), 5.0 * (kx_max + 1) * 36 * c["n_psi"]
     if slot == "psi_row_c2r":
@@ -109,10 +110,12 @@ def cpu_baseline(c, threads, budget_s=12.0):
     return reps / total, reps, total
 
 
-# bench slot -> kernel-name pattern of the launches it times (engine psi solve samples omega with SampleVort)
-SLOT_KERNELS = {"advect": "k_map_advect<30, 8, true>", "psi_combine": "k_psi_combine<3>", "incomp": "k_incomp_tensor<",
-                "psi_row_r2c": "k_row_r2c<cmm::SampleVortT<true>", "psi_col": "k_col(", "psi_row_c2r": "k_row_c2r4(",
-                "particles": "k_particles("}
+# bench slot -> kernel-name patterns of the launches it times (the row-transform slot of the psi solve
+# is the sampling of omega(X(x)) followed by the row transform of the sampled field)
+SLOT_KERNELS = {"advect": ("k_map_advect<30, 8, true",), "psi_combine": ("k_psi_combine<3>",),
+                "incomp": ("k_incomp_tensor<",),
+                "psi_row_r2c": ("k_sample_field<cmm::SampleVortT<true", "k_row_r2c<cmm::SampleReal>"),
+                "psi_col": ("k_col(",), "psi_row_c2r": ("k_row_c2r4(",), "particles": ("k_particles(",)}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "cmm_euler_c2_hbm_traffic.json")
 FETCH_SCALE_DEFAULT, WRITE_SCALE_DEFAULT = 2.0, 1.0
 
@@ -172,18 +175,26 @@ def summarize_profiles(kt_dir, fetch_dir, write_dir, out_dir, commands):
              f"{'slot':12s} {'kernel':40s} {'n':>4s} {'fetch_MiB':>10s} {'write_MiB':>10s} {'alg_MiB':>10s} "
              f"{'ratio':>6s}"]
     out = {"fetch_scale": fs, "write_scale": ws, "slots": {}}
-    for slot, pat in SLOT_KERNELS.items():
-        ks = [k for k in fetch if pat in k]
-        if not ks:
-            continue
-        fv = [v for k in ks for v in fetch[k]]
-        wv = [v for k in write if pat in k for v in write[k]]
-        fb = 1024.0 * fs * sum(fv) / len(fv)
-        wb = 1024.0 * ws * sum(wv) / len(wv) if wv else float("nan")
-        alg = algorithmic(slot, c, kx_max)[0]
-        out["slots"][slot] = {"fetch_bytes": fb, "write_bytes": wb, "algorithmic_bytes": alg, "dispatches": len(fv)}
-        lines.append(f"{slot:12s} {ks[0].split('(')[0][:40]:40s} {len(fv):4d} {fb / 2**20:10.1f} {wb / 2**20:10.1f} "
-                     f"{alg / 2**20:10.1f} {(fb + wb) / alg:6.2f}")
+    for slot, pats in SLOT_KERNELS.items():
+        # per-dispatch bytes of each kernel of the slot, summed over its kernels
+        fb = wb = 0.0
+        names, n = [], None
+        for pat in pats:
+            ks = [k for k in fetch if pat in k]
+            if not ks:
+                break
+            fv = [v for k in ks for v in fetch[k]]
+            wv = [v for k in write if pat in k for v in write[k]]
+            fb += 1024.0 * fs * sum(fv) / len(fv)
+            wb += 1024.0 * ws * sum(wv) / len(wv) if wv else float("nan")
+            names.append(ks[0].split("(")[0].replace("void cmm::", ""))
+            n = len(fv) if n is None else min(n, len(fv))
+        else:
+            alg = algorithmic(slot, c, kx_max)[0]
+            out["slots"][slot] = {"fetch_bytes": fb, "write_bytes": wb, "algorithmic_bytes": alg, "dispatches": n,
+                                  "kernels": names}
+            lines.append(f"{slot:12s} {' + '.join(names)[:40]:40s} {n:4d} {fb / 2**20:10.1f} {wb / 2**20:10.1f} "
+                         f"{alg / 2**20:10.1f} {(fb + wb) / alg:6.2f}")
     with open(os.path.join(out_dir, "cmm_euler_c2_hbm_traffic.txt"), "w") as f:
         f.write("\n".join(lines) + "\n")
     with open(os.path.join(out_dir, "cmm_euler_c2_hbm_traffic.json"), "w") as f:
