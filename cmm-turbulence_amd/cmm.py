@@ -75,7 +75,7 @@ SYMBOLS = ["cmm_version", "cmm_last_error", "cmm_device_info", "cmm_lagrange_coe
            "cmm_hermite_from_real", "cmm_hermite_from_spectrum", "cmm_psi_workspace_bytes", "cmm_psi_from_vort",
            "cmm_spectral_workspace_bytes", "cmm_spectral_from_real", "cmm_sample_vort", "cmm_vp_psi_workspace_bytes", "cmm_vp_psi_from_f", "cmm_particles_advect", "cmm_euler_create", "cmm_euler_destroy", "cmm_euler_stream",
            "cmm_euler_add_particles", "cmm_euler_init", "cmm_euler_step", "cmm_euler_get_field",
-           "cmm_euler_get_particles", "cmm_euler_diagnostics", "cmm_euler_vp_diagnostics", "cmm_euler_sample", "cmm_euler_state", "cmm_euler_profile", "cmm_euler_profile_read"]
+           "cmm_euler_get_particles", "cmm_euler_diagnostics", "cmm_euler_vp_diagnostics", "cmm_euler_sample", "cmm_euler_zoom", "cmm_euler_state", "cmm_euler_profile", "cmm_euler_profile_read"]
 
 _lib = None
 DP = C.POINTER(C.c_double)
@@ -134,6 +134,7 @@ def lib():
         L.cmm_euler_diagnostics.argtypes = [VP, DP]
         L.cmm_euler_vp_diagnostics.argtypes = [VP, DP]
         L.cmm_euler_sample.argtypes = [VP, Grid, C.c_int, DP]
+        L.cmm_euler_zoom.argtypes = [VP, Grid, C.POINTER(C.c_int), C.c_int, C.c_int, DP]
         L.cmm_euler_state.argtypes = [VP, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_int)]
         L.cmm_euler_profile.argtypes = [VP, C.c_int]
         L.cmm_euler_profile_read.argtypes = [VP, DP, C.POINTER(C.c_int)]
@@ -372,6 +373,21 @@ class Euler:
         _check(lib().cmm_euler_sample(self.h, g_sample, scalar_ic, out.ctypes.data_as(DP)), "cmm_euler_sample")
         names = ("chix", "chiy", "w", "theta", "lap_w", "gradx_w", "grady_w", "psi", "ux", "uy")
         return {k: out[i * n:(i + 1) * n] for i, k in enumerate(names)}
+
+    def zoom(self, g_zoom, ics=(), want_psi=True):
+        """A zoom window (cmm_euler_zoom): positions x, y through the whole stack on g_zoom, the
+        initial conditions ics there (-1: the run's own), and psi at the window's points."""
+        import numpy as np
+        n = g_zoom.nx * g_zoom.ny
+        k = len(ics)
+        out = np.zeros((3 + k) * n)
+        arr = (C.c_int * max(k, 1))(*ics)
+        _check(lib().cmm_euler_zoom(self.h, g_zoom, arr, k, 1 if want_psi else 0, out.ctypes.data_as(DP)),
+               "cmm_euler_zoom")
+        res = {"x": out[:n], "y": out[n:2 * n], "ic": [out[(2 + i) * n:(3 + i) * n] for i in range(k)]}
+        if want_psi:
+            res["psi"] = out[(2 + k) * n:(3 + k) * n]
+        return res
 
     def vp_diagnostics(self):
         """Vlasov-Poisson: Etot, Ekin, Epot, Mass, Momentum, Emode1..4"""

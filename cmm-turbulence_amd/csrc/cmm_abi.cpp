@@ -871,6 +871,49 @@ int cmm_euler_sample(cmm_euler *e, cmm_grid gs, int scalar_ic, double *out) {
     return CMM_OK;
 }
 
+int cmm_euler_zoom(cmm_euler *e, cmm_grid gz, const int *ics, int n_ics, int want_psi, double *out) {
+    if (!e || !out || gz.nx < 1 || gz.ny < 1 || n_ics < 0 || (n_ics > 0 && !ics) || !(gz.x1 > gz.x0) ||
+        !(gz.y1 > gz.y0)) {
+        set_error("cmm_euler_zoom: bad argument");
+        return CMM_ERR_ARG;
+    }
+    const cmm_grid &gc = e->p.g_coarse;
+    const size_t n = (size_t)gz.nx * gz.ny;
+    if (e->sbuf_n < n) {
+        if (e->sbuf) (void)hipFree(e->sbuf);
+        e->sbuf = nullptr;
+        e->sbuf_n = 0;
+        if (int rc = dalloc(&e->sbuf, 8 * n)) return rc;
+        e->sbuf_n = n;
+    }
+    double *xy = e->sbuf, *f = xy + 2 * n;
+    std::vector<double> hxy(2 * n);
+    auto get = [&](double *dst, const double *src, size_t cnt) {
+        if (int rc = hip_check(hipMemcpyAsync(dst, src, sizeof(double) * cnt, hipMemcpyDeviceToHost, e->st), "get"))
+            return rc;
+        return hip_check(hipStreamSynchronize(e->st), "hipStreamSynchronize");
+    };
+    int rc = op_compose_positions(gc, gz, e->cx, e->cy, e->stack, e->nstack, xy, e->st);
+    if (!rc) rc = get(hxy.data(), xy, 2 * n);
+    for (int k = 0; !rc && k < n_ics; ++k) {
+        const bool own = ics[k] < 0;
+        const int disc = own ? e->p.discrete : 0;
+        rc = op_sample_init(gc, gz, e->cx, e->cy, e->stack, e->nstack, own ? e->p.ic : ics[k], e->p.ic_params, disc,
+                            disc ? e->p.g_disc : gz, disc ? e->disc : nullptr, f, e->st);
+        if (!rc) rc = get(out + (2 + k) * n, f, n);
+    }
+    if (!rc && want_psi) {
+        rc = op_sample_hermite(e->p.g_psi, e->lvl[0], gz, f, e->st);
+        if (!rc) rc = get(out + (2 + n_ics) * n, f, n);
+    }
+    if (rc) return rc;
+    for (size_t i = 0; i < n; ++i) {
+        out[i] = hxy[2 * i];
+        out[n + i] = hxy[2 * i + 1];
+    }
+    return CMM_OK;
+}
+
 int cmm_euler_state(cmm_euler *e, int *steps, double *t, int *stack_size) {
     if (!e) { set_error("cmm_euler_state: null engine"); return CMM_ERR_ARG; }
     if (steps) *steps = e->loop;

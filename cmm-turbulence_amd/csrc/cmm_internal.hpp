@@ -52,6 +52,8 @@ int op_spectral_from_real(const cmm_grid &g, const double *real, int op, double 
                           hipStream_t st);
 int op_sample_vort(const cmm_grid &gm, const cmm_grid &gt, const cmm_grid &gf, const double *cx, const double *cy,
                    const double *W, int molly, double *out, hipStream_t st);
+// value plane of the Hermite field H (grid gh) at the points of gt
+int op_sample_hermite(const cmm_grid &gh, const double *H, const cmm_grid &gt, double *out, hipStream_t st);
 int op_particles(const cmm_grid &gp, const PsiLevels &P, int L, const double *Lp, double dt, int scheme, int n,
                  double *pos, hipStream_t st);
 

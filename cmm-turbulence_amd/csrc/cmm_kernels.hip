@@ -536,6 +536,16 @@ struct SampleReal {
     __device__ double operator()(int ix, int iy) const { return f[(long)iy * nx + ix]; }
 };
 
+// value of a Hermite field at the points of another grid (its own origin and spacing),
+// k_h_sample (Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:122-135)
+struct SampleHermite {
+    DGrid gh, gt;
+    const double *__restrict__ H;
+    __device__ double operator()(int ix, int iy) const {
+        return herm_eval(gh, H, gt.x0 + ix * gt.hx, gt.y0 + iy * gt.hy);
+    }
+};
+
 // omega(X(x)) through the current map and the fine Hermite vorticity (mollifier 0/4/8),
 // Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:402-461; positions use the map grid's origin.
 // IL: the fine vorticity is given node-interleaved (W[4*i + plane]) instead of plane-major.
@@ -1680,6 +1690,13 @@ int op_sample_vort(const cmm_grid &gm, const cmm_grid &gt, const cmm_grid &gf, c
     s.molly = molly;
     dim3 b(64, 4);
     hipLaunchKernelGGL(k_sample_grid<SampleVort>, grid2d(gt.nx, gt.ny, b), b, 0, st, s, gt.nx, gt.ny, out);
+    return launch_check("k_sample_grid");
+}
+
+int op_sample_hermite(const cmm_grid &gh, const double *H, const cmm_grid &gt, double *out, hipStream_t st) {
+    SampleHermite s{make_dgrid(gh), make_dgrid(gt), H};
+    dim3 b(64, 4);
+    hipLaunchKernelGGL(k_sample_grid<SampleHermite>, grid2d(gt.nx, gt.ny, b), b, 0, st, s, gt.nx, gt.ny, out);
     return launch_check("k_sample_grid");
 }
 

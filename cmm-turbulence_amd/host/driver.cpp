@@ -406,12 +406,85 @@ struct Run {
         return msg;
     }
 
+    // compute_zoom, Cuda-CMM/src/simulation/cmm-simulation-host.cu:1039-1227: for each due zoom, rep
+    // windows around (pos_x, pos_y), the widths shrinking by rep_fac, each with the fields through
+    // the whole map stack on a grid x grid window.  The statistics use the cell area of the full
+    // domain on that grid, as the reference's zoom variable keeps the domain's grid.
+    std::string zoom(double t_now, double dt_now, double dt) {
+        std::string msg;
+        const double *b = s.domain_bounds;
+        const bool vp = s.simulation_type == "cmm_vlasov_poisson_1d";
+        for (size_t i = 0; i < s.save_zoom.size(); ++i) {
+            const auto &c = s.save_zoom[i];
+            if (!save_due(c.is_instant, c.time_start, c.time_end, c.time_step, t_now, dt_now, dt)) continue;
+            const int N = c.grid;
+            const std::string tag = to_s(N), zname = "Zoom_" + to_s((int)i + 1);
+            const size_t n = (size_t)N * N;
+            const double area = ((b[1] - b[0]) / (float)N) * ((b[3] - b[2]) / (float)N);
+            const std::string &var = c.var;
+            const bool all = var.find("All") != std::string::npos;
+            auto has = [&](const char *k) { return var.find(k) != std::string::npos; };
+            msg += "Processed zoom data " + to_s((int)i + 1) + " on grid " + tag;
+            if (s.verbose >= 4) append1(s, "/Monitoring_data/Debug_globals/Time_s_" + tag + "_" + zname, t_now);
+            // the reference's k_h_sample_from_init switches: vorticity and distribution from the
+            // initial condition number of the run (so a Vlasov run's "vorticity" is the vorticity
+            // condition of the same number), the passive scalar from scalar_name
+            const bool want_map = has("Map_b") || has("Chi_b") || all;
+            const bool want_scalar = has("Scalar") || has("Theta") || all;
+            const bool want_w = has("Vorticity") || has("W") || all;
+            const bool want_f = has("Dist") || has("F") || all;
+            const bool want_psi = has("Stream") || has("Psi") || all;
+            if (want_scalar && s.scalar_discrete) fail("a discrete passive scalar is not provided by this engine", 1);
+            const int scalar_num = s.scalar_name == "rectangle" ? 0 : s.scalar_name == "gaussian" ? 1
+                                 : s.scalar_name == "circular_ring" ? 2 : -1;
+            std::vector<int> ics;
+            std::vector<std::string> names;
+            if (want_scalar) {
+                ics.push_back(scalar_num >= 0 ? CMM_IC_SCALAR_RECTANGLE + scalar_num : 399);
+                names.push_back("Scalar_Theta_" + tag);
+            }
+            if (want_w) {
+                ics.push_back(vp && !s.initial_discrete ? s.initial_condition_num : -1);
+                names.push_back("Vorticity_W_" + tag);
+            }
+            if (want_f) {
+                ics.push_back(vp || s.initial_discrete ? -1 : CMM_IC_LANDAU_DAMPING + s.initial_condition_num);
+                names.push_back("Distribution_" + tag);
+            }
+            std::string ts = std::abs(t_now - kTMax) < 1 ? "final" : fmt3(t_now);
+            double wx = c.width_x / 2.0, wy = c.width_y / 2.0;
+            std::vector<double> out((3 + ics.size()) * n);
+            for (int r = 0; r < c.rep; ++r) {
+                const cmm_grid gz{N, N, c.pos_x - wx, c.pos_x + wx, c.pos_y - wy, c.pos_y + wy};
+                check(cmm_euler_zoom(e, gz, ics.data(), (int)ics.size(), want_psi ? 1 : 0, out.data()), "cmm_euler_zoom");
+                const std::string rep = zname + "_rep_" + to_s(r);
+                const std::string sub = "/Zoom_data/Time_" + ts + "/" + rep;
+                if (s.verbose < 4) mkdir_p(s.workspace + "data/" + s.file_name + sub);
+                auto put = [&](const std::string &name, const double *v) {
+                    if (s.verbose < 4) write_file(s, sub + "/" + name, v, n, false);
+                    else debug_globals(s, name + "_" + rep, v, n, area);
+                };
+                if (want_map) {
+                    put("Map_ChiX_" + tag, out.data());
+                    put("Map_ChiY_" + tag, out.data() + n);
+                }
+                for (size_t k = 0; k < ics.size(); ++k) put(names[k], out.data() + (2 + k) * n);
+                if (want_psi) put("Stream_function_Psi_" + tag, out.data() + (2 + ics.size()) * n);
+                wx *= c.rep_fac;
+                wy *= c.rep_fac;
+            }
+        }
+        return msg;
+    }
+
     void save(double t_now, double dt_now, double dt) {
         std::string m = write_time_step(t_now, dt_now, dt);
         if (s.verbose >= 3 && !m.empty()) log.push(m);
         m = conservation(t_now, dt_now, dt);
         if (s.verbose >= 3 && !m.empty()) log.push(m);
         m = sample(t_now, dt_now, dt);
+        if (s.verbose >= 3 && !m.empty()) log.push(m);
+        m = zoom(t_now, dt_now, dt);
         if (s.verbose >= 3 && !m.empty()) log.push(m);
     }
 };
@@ -481,7 +554,7 @@ int run_euler(Settings &s) {
         fail("initial condition " + s.initial_condition + " draws from cuRAND/std distributions and is not provided", 230305);
     if (vp && !s.save_sample.empty() && s.verbose >= 2)
         log.push("Note: save_sample outputs of the Vlasov-Poisson flow are not produced by this engine");
-    if (!s.save_zoom.empty() && s.verbose >= 2) log.push("Note: save_zoom outputs are not produced by this engine");
+
 
     const double *b = s.domain_bounds;
     r.gc = cmm_grid{s.grid_coarse, s.grid_coarse, b[0], b[1], b[2], b[3]};

@@ -215,6 +215,13 @@ enum {
     CMM_SAMPLE_FIELDS = 10
 };
 int cmm_euler_sample(cmm_euler *e, cmm_grid g_sample, int scalar_ic, double *host_out);
+/* A zoom window of compute_zoom (Cuda-CMM/src/simulation/cmm-simulation-host.cu:1039-1227) on
+ * g_zoom (any size >= 1, any bounds): host_out receives, each g_zoom.nx*g_zoom.ny doubles, the
+ * positions X and Y through the current map and the whole stack (wrapped into the domain after
+ * each layer), then for each k < n_ics the initial condition ics[k] at those positions (ics[k] < 0:
+ * the engine's own initial condition, discrete or not; else a CMM_IC_* value), then, when
+ * want_psi, the newest psi level evaluated at the zoom grid points themselves (k_h_sample). */
+int cmm_euler_zoom(cmm_euler *e, cmm_grid g_zoom, const int *ics, int n_ics, int want_psi, double *host_out);
 /* Steps taken, current time, stack size. */
 int cmm_euler_state(cmm_euler *e, int *steps, double *t, int *stack_size);
 /* Per-kernel timing with events on the engine stream: enable (clears the counters), then read

@@ -7,9 +7,9 @@ palinstrophy / max vorticity and the Debug_globals statistics of the map, psi, v
 vorticity to 1e-10 relative (+1e-12 absolute for values that are rounding noise around zero).
 Also compared: the sample-grid outputs (fields through the whole map stack on the 256^2 sample
 grid: map, vorticity, passive scalar, Laplacian, gradients, stream function, velocity, and their
-conservation values).  Not compared: particle statistics (the reference seeds them with cuRAND,
-which this engine replaces), inertial particles and zoom outputs (not produced by this engine),
-hashes.
+conservation values) and the zoom windows.  Not compared: particle statistics (the reference
+seeds them with cuRAND, which this engine replaces), inertial particles (not produced by this
+engine), hashes.
 """
 import json
 import os
@@ -93,6 +93,18 @@ def compare(mon, case):
 def test_cli_one_step(dev, tmp_path):
     mon = run_case(tmp_path, "one_step", "one_step_4_nodes_C256_F512_t1_T1")
     assert compare(mon, "one_step") >= 40
+
+
+@pytest.mark.parametrize("kind", ["none", "conv", "comp", "sample", "zoom", "particles"])
+def test_cli_init_fields(dev, tmp_path, kind):
+    """The reference's initialisation cases (Cuda-CMM/tests/euler2D_init_fields): final_time 0, so
+    the Lagrange warm-up (with dt = 1) and the saves at t = 0 and at the end, one save mode each:
+    computational statistics, conservation values, sample grid, zoom windows (two repetitions,
+    the second at half width) and particles."""
+    mon = run_case(tmp_path, "init_" + kind, f"init_fields_{kind}_4_nodes_C256_F512_t1_T0")
+    expect = sum(1 for k in FIX["init_" + kind]
+                 if k not in ("params_saved", "params_input", "log_warmup_incomp") and not k.startswith(SKIP_PREFIX))
+    assert compare(mon, "init_" + kind) == expect
 
 
 def test_cli_remapping_11b(dev, tmp_path):
