@@ -75,7 +75,7 @@ SYMBOLS = ["cmm_version", "cmm_last_error", "cmm_device_info", "cmm_lagrange_coe
            "cmm_hermite_from_real", "cmm_hermite_from_spectrum", "cmm_psi_workspace_bytes", "cmm_psi_from_vort",
            "cmm_spectral_workspace_bytes", "cmm_spectral_from_real", "cmm_sample_vort", "cmm_vp_psi_workspace_bytes", "cmm_vp_psi_from_f", "cmm_particles_advect", "cmm_euler_create", "cmm_euler_destroy", "cmm_euler_stream",
            "cmm_euler_add_particles", "cmm_euler_init", "cmm_euler_step", "cmm_euler_get_field",
-           "cmm_euler_get_particles", "cmm_euler_diagnostics", "cmm_euler_vp_diagnostics", "cmm_euler_sample", "cmm_euler_zoom", "cmm_euler_state", "cmm_euler_profile", "cmm_euler_profile_read"]
+           "cmm_euler_get_particles", "cmm_euler_diagnostics", "cmm_euler_vp_diagnostics", "cmm_euler_sample", "cmm_euler_zoom", "cmm_euler_restart", "cmm_euler_get_stack_map", "cmm_euler_state", "cmm_euler_profile", "cmm_euler_profile_read"]
 
 _lib = None
 DP = C.POINTER(C.c_double)
@@ -135,6 +135,8 @@ def lib():
         L.cmm_euler_vp_diagnostics.argtypes = [VP, DP]
         L.cmm_euler_sample.argtypes = [VP, Grid, C.c_int, DP]
         L.cmm_euler_zoom.argtypes = [VP, Grid, C.POINTER(C.c_int), C.c_int, C.c_int, DP]
+        L.cmm_euler_restart.argtypes = [VP, C.c_double, C.c_int, DP, DP, DP, DP]
+        L.cmm_euler_get_stack_map.argtypes = [VP, C.c_int, DP, DP]
         L.cmm_euler_state.argtypes = [VP, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_int)]
         L.cmm_euler_profile.argtypes = [VP, C.c_int]
         L.cmm_euler_profile_read.argtypes = [VP, DP, C.POINTER(C.c_int)]
@@ -388,6 +390,27 @@ class Euler:
         if want_psi:
             res["psi"] = out[(2 + k) * n:(3 + k) * n]
         return res
+
+    def restart(self, t_restart, stack_maps, chix, chiy, psi_old):
+        """Restart from a saved state (cmm_euler_restart): stack_maps is a list of (chix, chiy) host
+        arrays (oldest first), chix/chiy the active map, psi_old the older psi levels."""
+        import numpy as np
+        st = np.ascontiguousarray(np.concatenate([np.concatenate([x, y]) for x, y in stack_maps])
+                                  if stack_maps else np.zeros(1))
+        cx, cy = np.ascontiguousarray(chix, dtype=np.float64), np.ascontiguousarray(chiy, dtype=np.float64)
+        po = np.ascontiguousarray(np.concatenate(list(psi_old)) if len(psi_old) else np.zeros(1))
+        _check(lib().cmm_euler_restart(self.h, float(t_restart), len(stack_maps), st.ctypes.data_as(DP),
+                                       cx.ctypes.data_as(DP), cy.ctypes.data_as(DP), po.ctypes.data_as(DP)),
+               "cmm_euler_restart")
+
+    def stack_map(self, m):
+        """Map m of the stack (0 = oldest) as host arrays (chix, chiy)."""
+        import numpy as np
+        n = 4 * self.params.g_coarse.n
+        x, y = np.zeros(n), np.zeros(n)
+        _check(lib().cmm_euler_get_stack_map(self.h, m, x.ctypes.data_as(DP), y.ctypes.data_as(DP)),
+               "cmm_euler_get_stack_map")
+        return x, y
 
     def vp_diagnostics(self):
         """Vlasov-Poisson: Etot, Ekin, Epot, Mass, Momentum, Emode1..4"""

@@ -611,6 +611,59 @@ int cmm_euler_init(cmm_euler *e, double *warm, int *nwarm) {
     return hip_check(hipStreamSynchronize(e->st), "hipStreamSynchronize");
 }
 
+int cmm_euler_restart(cmm_euler *e, double t_restart, int n_maps, const double *stack_host, const double *chix_host,
+                      const double *chiy_host, const double *psi_old_host) {
+    if (!e || !chix_host || !chiy_host || n_maps < 0 || (n_maps > 0 && !stack_host) ||
+        (e->p.l_order > 1 && !psi_old_host)) {
+        set_error("cmm_euler_restart: bad argument");
+        return CMM_ERR_ARG;
+    }
+    const cmm_euler_params &p = e->p;
+    const int L = p.l_order;
+    if (n_maps > p.stack_capacity) {
+        set_error("cmm_euler_restart: more maps than the stack holds");
+        return CMM_ERR_ARG;
+    }
+    auto put = [&](double *dst, const double *src, size_t cnt) {
+        return hip_check(hipMemcpyAsync(dst, src, sizeof(double) * cnt, hipMemcpyHostToDevice, e->st), "put");
+    };
+    // readMapStack (Cuda-CMM/src/simulation/cmm-simulation-host.cu:1255-1302): the stacked maps, the
+    // initial condition translated through them with the active map still the identity, then the
+    // active map and the older psi levels
+    int rc = n_maps > 0 ? put(e->stack, stack_host, (size_t)8 * e->nc * n_maps) : CMM_OK;
+    e->nstack = n_maps;
+    if (!rc) rc = op_map_identity(p.g_coarse, e->cx, e->cy, e->st);
+    e->m8_valid = false;
+    if (!rc) rc = engine_translate(e);
+    if (!rc) rc = put(e->cx, chix_host, (size_t)4 * e->nc);
+    if (!rc) rc = put(e->cy, chiy_host, (size_t)4 * e->nc);
+    for (int l = 1; l < L && !rc; ++l) rc = put(e->lvl[l], psi_old_host + (size_t)4 * e->np * (l - 1), (size_t)4 * e->np);
+    // the newest psi from the loaded state (Cuda-CMM/src/simulation/cmm-euler2d.cu:664-666); no warm-up
+    if (!rc) rc = engine_psi(e, e->lvl[0]);
+    if (rc) return rc;
+    // time vectors from the restart time, Cuda-CMM/src/simulation/cmm-euler2d.cu:709-715
+    e->tv.assign(L, 0.0);
+    e->dtv.assign(L, p.dt);
+    for (int i = 0; i < L; ++i) e->tv[i] = t_restart - (L - 1 - i) * p.dt;
+    e->loop = 0;
+    e->initialised = true;
+    return hip_check(hipStreamSynchronize(e->st), "hipStreamSynchronize");
+}
+
+int cmm_euler_get_stack_map(cmm_euler *e, int m, double *chix_host, double *chiy_host) {
+    if (!e || m < 0 || m >= e->nstack || !chix_host || !chiy_host) {
+        set_error("cmm_euler_get_stack_map: bad argument");
+        return CMM_ERR_ARG;
+    }
+    const double *slot = e->stack + (size_t)8 * e->nc * m;
+    int rc = hip_check(hipMemcpyAsync(chix_host, slot, sizeof(double) * 4 * e->nc, hipMemcpyDeviceToHost, e->st), "get");
+    if (!rc)
+        rc = hip_check(hipMemcpyAsync(chiy_host, slot + 4 * e->nc, sizeof(double) * 4 * e->nc, hipMemcpyDeviceToHost, e->st),
+                       "get");
+    if (rc) return rc;
+    return hip_check(hipStreamSynchronize(e->st), "hipStreamSynchronize");
+}
+
 int cmm_euler_step(cmm_euler *e, double dt_now, cmm_step_info *info) {
     if (!e || !e->initialised) { set_error("cmm_euler_step: engine not initialised"); return CMM_ERR_ARG; }
     if (!(dt_now > 0)) { set_error("cmm_euler_step: dt must be positive"); return CMM_ERR_ARG; }

@@ -113,6 +113,26 @@ void write_file(const Settings &s, const std::string &name, const double *v, siz
 
 void append1(const Settings &s, const std::string &name, double v) { write_file(s, name, &v, 1, true); }
 
+// readAllRealFromBinaryFile: exactly n doubles from a file (false when it is missing or short)
+bool read_doubles(const std::string &path, double *v, size_t n) {
+    std::ifstream f(path, std::ios::binary);
+    return f && f.read(reinterpret_cast<char *>(v), sizeof(double) * n) && f.gcount() == (std::streamsize)(sizeof(double) * n);
+}
+
+// countFilesWithString (Cuda-CMM/src/simulation/cmm-simulation-host.cu:1234-1252): regular files
+// in dir whose name contains key
+int count_files_with(const std::string &dir, const std::string &key) {
+    int count = 0;
+    if (DIR *d = opendir(dir.c_str())) {
+        while (dirent *ent = readdir(d))
+            if (ent->d_type == DT_REG && std::string(ent->d_name).find(key) != std::string::npos) count++;
+        closedir(d);
+    } else {
+        std::cerr << "Error: Could not open directory " << dir << std::endl;
+    }
+    return count;
+}
+
 // Debug statistics instead of arrays at verbose >= 4, Cuda-CMM/src/ui/cmm-io.cu:508-534 and
 // Cuda-CMM/src/numerical/cmm-mesure.cu:220-250: on a grid "Avg" is max(0, max) h^2 and "L2" is
 // sum h^2; on a plain array they are max(0, max)/N and sum/N.  The hash files are not written.
@@ -477,6 +497,30 @@ struct Run {
         return msg;
     }
 
+    // writeMapStack (Cuda-CMM/src/ui/cmm-io.cu:611-635): map m of the stack (m = -1: the active map,
+    // file suffix "coarse") and psi level `level` written once per older level, as the reference
+    // writes its newest level into every Stream_function_Psi_H_psi_i file
+    void write_map_stack(int m, int level) {
+        mkdir_p(s.workspace + "data/" + s.file_name + "/MapStack");
+        const size_t nc4 = 4 * (size_t)gc.nx * gc.ny, np4 = 4 * (size_t)gp.nx * gp.ny;
+        std::vector<double> cx(nc4), cy(nc4);
+        if (m < 0) {
+            check(cmm_euler_get_field(e, CMM_FIELD_CHIX, cx.data()), "cmm_euler_get_field");
+            check(cmm_euler_get_field(e, CMM_FIELD_CHIY, cy.data()), "cmm_euler_get_field");
+        } else {
+            check(cmm_euler_get_stack_map(e, m, cx.data(), cy.data()), "cmm_euler_get_stack_map");
+        }
+        const std::string name = m < 0 ? "coarse" : to_s(m);
+        write_file(s, "/MapStack/Map_ChiX_H_" + name, cx.data(), nc4, false);
+        write_file(s, "/MapStack/Map_ChiY_H_" + name, cy.data(), nc4, false);
+        if (L > 1) {
+            std::vector<double> psi(np4 * L);
+            check(cmm_euler_get_field(e, CMM_FIELD_PSI, psi.data()), "cmm_euler_get_field");
+            for (int i = 1; i < L; ++i)
+                write_file(s, "/MapStack/Stream_function_Psi_H_psi_" + to_s(i), psi.data() + np4 * level, np4, false);
+        }
+    }
+
     void save(double t_now, double dt_now, double dt) {
         std::string m = write_time_step(t_now, dt_now, dt);
         if (s.verbose >= 3 && !m.empty()) log.push(m);
@@ -510,6 +554,11 @@ double compute_next_timestep(const Settings &s, double t, double dt) {
     for (auto &c : s.save_sample) targets(c.is_instant, c.time_start, c.time_end, c.time_step);
     for (auto &c : s.save_zoom) targets(c.is_instant, c.time_start, c.time_end, c.time_step);
     return dt_now;
+}
+
+// readMapStack's folder: restart_location, or the run's own MapStack folder when it is empty
+std::string restart_dir(const Settings &s) {
+    return s.restart_location.empty() ? s.workspace + "data/" + s.file_name + "/MapStack" : s.restart_location;
 }
 
 int run_euler(Settings &s) {
@@ -549,7 +598,6 @@ int run_euler(Settings &s) {
     }
     // parts of the reference that are not on this engine's path stop here with a message
     if (s.forward_map) fail("forward_map is not supported by this engine (backward map only)", 1);
-    if (s.restart_time != 0) fail("restart from a saved map stack is not supported by this engine yet", 1);
     if (!vp && s.initial_condition_num >= 7 && s.initial_condition_num <= 8)
         fail("initial condition " + s.initial_condition + " draws from cuRAND/std distributions and is not provided", 230305);
     if (vp && !s.save_sample.empty() && s.verbose >= 2)
@@ -619,7 +667,19 @@ int run_euler(Settings &s) {
             if (s.verbose >= 1) log.push("Particles set P" + zpad((int)i + 1, 2) + " : inertial particles (tau != 0) are not advected by this engine");
             continue;
         }
-        auto pos = init_particles(pa, b);
+        std::vector<double> pos;
+        if (s.restart_time != 0) {
+            // readParticlesState, Cuda-CMM/src/ui/cmm-io.cu:659-688
+            pos.resize(2 * (size_t)pa.num);
+            const std::string f = restart_dir(s) + "/Particles_advected_pos_P" + zpad((int)i + 1, 2) + ".data";
+            if (!read_doubles(f, pos.data(), pos.size())) {
+                log.push("Failed to read in particles pos " + to_s((int)i) + " at location " + f);
+                log.push("Exitting ..");
+                std::exit(0);
+            }
+        } else {
+            pos = init_particles(pa, b);
+        }
         int set = -1;
         r.check(cmm_euler_add_particles(r.e, pa.num, pa.time_integration_num, pos.data(), &set), "cmm_euler_add_particles");
         r.part_set.back() = set;
@@ -636,9 +696,45 @@ int run_euler(Settings &s) {
 
     double warm[8];
     int nwarm = 0;
-    r.check(cmm_euler_init(r.e, warm, &nwarm), "cmm_euler_init");
+    if (s.restart_time != 0) {
+        // restart from a saved map stack, Cuda-CMM/src/simulation/cmm-euler2d.cu:634-684 with readMapStack
+        // (Cuda-CMM/src/simulation/cmm-simulation-host.cu:1255-1302)
+        if (s.verbose >= 2) log.push("Initializing by loading previous state");
+        const std::string dir = restart_dir(s);
+        const size_t nc4 = 4 * (size_t)r.gc.nx * r.gc.ny, np4 = 4 * (size_t)r.gp.nx * r.gp.ny;
+        const int n_maps = std::max(0, count_files_with(dir, "Map_ChiX_H") - 1);
+        std::vector<double> stack(2 * nc4 * n_maps), cx(nc4), cy(nc4), psi_old(np4 * std::max(0, r.L - 1));
+        std::string msg;
+        for (int m = 0; m < n_maps; ++m) {
+            for (int k = 0; k < 2; ++k) {
+                const std::string f = dir + "/Map_Chi" + (k ? "Y" : "X") + "_H_" + to_s(m) + ".data";
+                if (!read_doubles(f, stack.data() + (2 * m + k) * nc4, nc4))
+                    msg += "Failed to read in map " + to_s(m) + " at location " + f + "\n";
+            }
+        }
+        for (int k = 0; k < 2; ++k) {
+            const std::string f = dir + "/Map_Chi" + (k ? "Y" : "X") + "_H_coarse.data";
+            if (!read_doubles(f, (k ? cy : cx).data(), nc4)) msg += "Failed to read in active map at location " + f + "\n";
+        }
+        for (int l = 1; l < r.L; ++l) {
+            const std::string f = dir + "/Stream_function_Psi_H_psi_" + to_s(l) + ".data";
+            if (!read_doubles(f, psi_old.data() + (l - 1) * np4, np4))
+                msg += "Failed to read in stream function" + to_s(l) + " at location " + f + "\n";
+        }
+        if (!msg.empty()) {
+            log.push(msg);
+            log.push("Exitting ..");
+            std::exit(0);
+        }
+        r.check(cmm_euler_restart(r.e, s.restart_time, n_maps, stack.data(), cx.data(), cy.data(), psi_old.data()),
+                "cmm_euler_restart");
+        if (s.verbose >= 2) log.push("Loaded " + to_s(n_maps) + " maps to map stack");
+        if (!s.particles_advected.empty() && s.verbose >= 2) log.push("Loaded data for particle sets");
+    } else {
+        r.check(cmm_euler_init(r.e, warm, &nwarm), "cmm_euler_init");
+    }
     // warm-up log in the reference's format, Cuda-CMM/src/simulation/cmm-euler2d.cu:557-596
-    if (s.verbose >= 2 && r.L > 1) {
+    if (s.restart_time == 0 && s.verbose >= 2 && r.L > 1) {
         int k = 0;
         for (int order = 1; order <= r.L; ++order) {
             int n = order - (order == r.L);
@@ -679,6 +775,9 @@ int run_euler(Settings &s) {
             if (s.verbose >= 2)
                 log.push("Refining Map : Step = " + to_s(loop) + " \t Maps = " + to_s(info.stack_size) + " \t Gap = " + to_s(loop - old));
             old = loop;
+            // writeMapStack at the remap (Cuda-CMM/src/simulation/cmm-euler2d.cu:841): the pushed map and
+            // the psi level that was newest during the step (level 1 once the step has run)
+            if (s.save_map_stack) r.write_map_stack(info.stack_size - 1, 1);
         }
         const double gap = loop - old, counter = info.stack_size;
         r.save(info.t, dt_now, dt);
@@ -695,6 +794,27 @@ int run_euler(Settings &s) {
     }
     if (s.verbose >= 2) log.push("Simulation loop finished \t C-Time = " + duration(ctime()));
     r.save(kTMax, dt, dt);
+    if (s.save_map_stack) {
+        // Cuda-CMM/src/simulation/cmm-euler2d.cu:1033-1051: the active map, the newest psi level and
+        // the particle state
+        int steps = 0, n_stack = 0;
+        double t_end = 0;
+        cmm_euler_state(r.e, &steps, &t_end, &n_stack);
+        if (s.verbose >= 2) {
+            std::ostringstream os;
+            os.precision(16);
+            os << t_end;
+            log.push("Saving last Map of MapStack : Maps = " + to_s(n_stack) + " \t Total size = " +
+                     fmt3((n_stack + 1) * map_size) + "mb \t S-time = " + os.str());
+        }
+        r.write_map_stack(-1, 0);
+        for (size_t i = 0; i < s.particles_advected.size(); ++i) {
+            if (r.part_set[i] < 0) continue;
+            std::vector<double> pos(2 * (size_t)s.particles_advected[i].num);
+            r.check(cmm_euler_get_particles(r.e, r.part_set[i], pos.data()), "cmm_euler_get_particles");
+            write_file(s, "/MapStack/Particles_advected_pos_P" + zpad((int)i + 1, 2), pos.data(), pos.size(), false);
+        }
+    }
     cmm_euler_destroy(r.e);
     append1(s, "/Monitoring_data/Time_c", ctime());
     if (s.verbose >= 1) log.push("Finished simulation \t C-Time = " + duration(ctime()));

@@ -187,6 +187,18 @@ void *cmm_euler_stream(cmm_euler *e);
 int cmm_euler_add_particles(cmm_euler *e, int n, int scheme, const double *pos_host, int *set);
 /* Initial state and Lagrange warm-up; warmup_incomp[8] receives the logged errors. */
 int cmm_euler_init(cmm_euler *e, double *warmup_incomp, int *n_warmup);
+/* Restart from a saved state instead of cmm_euler_init: readMapStack and the restart branch of the
+ * time loop (Cuda-CMM/src/simulation/cmm-simulation-host.cu:1255-1302,
+ * Cuda-CMM/src/simulation/cmm-euler2d.cu:634-684).  stack_host holds n_maps maps (8*Nc doubles each:
+ * ChiX planes then ChiY planes, oldest first); the fine vorticity is the initial condition
+ * translated through them; chix_host/chiy_host (4*Nc) become the active map; psi_old_host holds
+ * l_order-1 psi levels (4*Npsi each) for levels 1.. ; level 0 is computed.  Times restart at
+ * t_restart; there is no Lagrange warm-up. */
+int cmm_euler_restart(cmm_euler *e, double t_restart, int n_maps, const double *stack_host, const double *chix_host,
+                      const double *chiy_host, const double *psi_old_host);
+/* Map m of the stack (0 = oldest) to host memory, 4*Nc doubles each (writeMapStack,
+ * Cuda-CMM/src/ui/cmm-io.cu:611-635). */
+int cmm_euler_get_stack_map(cmm_euler *e, int m, double *chix_host, double *chiy_host);
 /* One step of length dt_now (the driver clips dt to save targets and the final time). */
 int cmm_euler_step(cmm_euler *e, double dt_now, cmm_step_info *info);
 /* Copies a field to host memory (size per CMM_FIELD_*). */

@@ -43,7 +43,7 @@ def compare(mon, case):
     fx = FIX[case]
     checked = 0
     for key, ref in fx.items():
-        if key in ("params_saved", "params_input", "log_warmup_incomp") or key.startswith(SKIP_PREFIX):
+        if key in ("params_saved", "params_input", "params_input_first", "log_warmup_incomp") or key.startswith(SKIP_PREFIX):
             continue
         path = os.path.join(mon, key + ".data")
         assert os.path.exists(path), f"missing output {key}"
@@ -103,8 +103,37 @@ def test_cli_init_fields(dev, tmp_path, kind):
     the second at half width) and particles."""
     mon = run_case(tmp_path, "init_" + kind, f"init_fields_{kind}_4_nodes_C256_F512_t1_T0")
     expect = sum(1 for k in FIX["init_" + kind]
-                 if k not in ("params_saved", "params_input", "log_warmup_incomp") and not k.startswith(SKIP_PREFIX))
+                 if k not in ("params_saved", "params_input", "params_input_first", "log_warmup_incomp")
+                 and not k.startswith(SKIP_PREFIX))
     assert compare(mon, "init_" + kind) == expect
+
+
+def test_cli_restart_from_map_stack(dev, tmp_path):
+    """The reference's restart case (Cuda-CMM/tests/euler2D_remap): a first run to T = 1 saves its
+    map stack (save_map_stack = 1, the maps pushed at each remap, the active map, the psi files and
+    the particle state); a second run restarts from that folder at restart_time = 1 and runs to
+    T = 2.  The second run's outputs are compared with the reference's: the remaps, E_inc, the
+    map counter from the 3 loaded maps on, the conservation values."""
+    ws = str(tmp_path) + "/"
+    for key in ("params_input_first", "params_input"):
+        params = tmp_path / (key + ".txt")
+        params.write_text(FIX["restart"][key])
+        print("running", key, flush=True)
+        # restart_location is relative to the working directory, as in the reference's harness
+        p = subprocess.run([BIN, f"param_file={params}", f"workspace={ws}"], capture_output=True, text=True,
+                           timeout=240, cwd=ws)
+        print(p.stdout[-3000:])
+        assert p.returncode == 0, p.stderr
+        assert "Finished simulation" in p.stdout
+    first = os.path.join(ws, "data", "remap_4_nodes_C256_F512_t32_T1", "MapStack")
+    names = sorted(os.listdir(first))
+    print(names)
+    assert "Map_ChiX_H_coarse.data" in names and "Stream_function_Psi_H_psi_2.data" in names
+    assert sum(1 for n in names if n.startswith("Map_ChiX_H_")) == 4  # 3 stacked maps + the active one
+    mon = os.path.join(ws, "data", "remap_4_nodes_C256_F512_t32_T2", "Monitoring_data")
+    counter = np.fromfile(os.path.join(mon, "Map_counter.data"), dtype="<f8")
+    assert counter[0] == 3
+    assert compare(mon, "restart") >= 40
 
 
 def test_cli_remapping_11b(dev, tmp_path):
