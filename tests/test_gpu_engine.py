@@ -176,3 +176,65 @@ def test_particles_in_the_loop(dev):
         print(scheme, "maps", maps[-1], "particles rel Linf", rel_linf(got, r["particles"]))
         assert maps == list(r["map_counter"]) and maps[-1] >= 1
         assert rel_linf(got, r["particles"]) <= 1e-12
+
+
+def test_zoom_and_restart_against_oracle(dev):
+    """A run that remaps (map 64^2, fine 128^2, threshold 1e-5), then: a zoom window (non-square,
+    off-centre, cmm_euler_zoom) against the oracle's composition through the same map and stack,
+    its initial conditions (own, passive scalar ring, Landau distribution) and psi there; and a
+    second engine restarted from that state (cmm_euler_restart) against the oracle's translate and
+    psi solve."""
+    import cmm
+    gc, gf = cmm.grid(64), cmm.grid(128)
+    e = cmm.Euler(gc, gf, gf, gf, 1.0 / 32, incomp_threshold=1e-5, stack_capacity=16)
+    e.init()
+    for _ in range(24):
+        info = e.step(1.0 / 32)
+    nstack = info.stack_size
+    assert nstack >= 2, "the run should have remapped"
+    cx, cy = e.field("chix"), e.field("chiy")
+    maps = [e.stack_map(m) for m in range(nstack)]
+    psi = e.field("psi")
+    npsi = 4 * 128 * 128
+    gz = cmm.grid(48, 40, (1.0, 3.5, 2.0, 4.25))
+    ics = [-1, 302, 200]
+    z = e.zoom(gz, ics)
+    ogc, ogf = O.grid(64), O.grid(128)
+    ogz = O.grid(48, 40, (1.0, 3.5, 2.0, 4.25))
+    sx = np.concatenate([m[0] for m in maps])
+    sy = np.concatenate([m[1] for m in maps])
+    xy = O.compose_positions(ogc, ogz, cx, cy, sx, sy, nstack)
+    # positions on the torus (a point within rounding of the wrap can land on either side)
+    d = np.concatenate([z["x"] - xy[0::2], z["y"] - xy[1::2]])
+    d = np.mod(d + np.pi, 2 * np.pi) - np.pi
+    print("zoom positions", np.abs(d).max())
+    assert np.abs(d).max() <= 1e-12 * TWO_PI
+    for k, ic in enumerate(ics):
+        ref = O.sample_init(ogz, xy, 0 if ic < 0 else ic)
+        print("zoom ic", ic, rel_linf(z["ic"][k], ref))
+        assert rel_linf(z["ic"][k], ref) <= 1e-12
+    L = O.lib()
+    # the zoom grid's spacing is (x1 - x0)/(float)N as for every grid
+    p0 = np.ascontiguousarray(psi[:npsi])
+    hx, hy = L.cmmo_grid_h(48, 1.0, 3.5), L.cmmo_grid_h(40, 2.0, 4.25)
+    ref_psi = np.array([L.cmmo_hermite_eval(ogf, O.ptr(p0), 1.0 + i * hx, 2.0 + j * hy)
+                        for j in range(40) for i in range(48)])
+    print("zoom psi", rel_linf(z["psi"], ref_psi))
+    assert rel_linf(z["psi"], ref_psi) <= 1e-12
+    # restart a second engine from this state
+    r = cmm.Euler(gc, gf, gf, gf, 1.0 / 32, incomp_threshold=1e-5, stack_capacity=16)
+    r.restart(0.75, maps, cx, cy, [psi[npsi:2 * npsi], psi[2 * npsi:3 * npsi]])
+    w_ref = O.hermite_from_real(ogf, O.sample_init(ogf, O.compose_positions(ogc, ogf, *O.identity_map(ogc), sx, sy,
+                                                                            nstack), 0))
+    w = r.field("w_fine")
+    print("restart omega", rel_linf(w[:128 * 128], w_ref[:128 * 128]))
+    assert rel_linf(w[:128 * 128], w_ref[:128 * 128]) <= 1e-12
+    psi_ref = O.psi_from_vort(ogc, ogf, ogf, ogf, cx, cy, w_ref, 0, 128.0)
+    rp = r.field("psi")
+    print("restart psi", rel_linf(rp[:128 * 128], psi_ref[:128 * 128]))
+    assert rel_linf(rp[:128 * 128], psi_ref[:128 * 128]) <= 1e-12
+    assert np.array_equal(rp[npsi:], psi[npsi:])
+    info = r.step(1.0 / 32)
+    assert abs(info.t - (0.75 + 1.0 / 32)) < 1e-15
+    e.close()
+    r.close()
