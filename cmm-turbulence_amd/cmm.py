@@ -70,7 +70,7 @@ FIELD = {"chix": 0, "chiy": 1, "psi": 2, "w_fine": 3, "w_coarse": 4, "disc": 5}
 
 # every symbol include/cmm.h declares
 SYMBOLS = ["cmm_version", "cmm_last_error", "cmm_device_info", "cmm_lagrange_coeff", "cmm_advect_coeffs_fill",
-           "cmm_map_init_identity", "cmm_advect_workspace_bytes", "cmm_map_advect", "cmm_incomp_error", "cmm_compose_positions",
+           "cmm_map_init_identity", "cmm_advect_workspace_bytes", "cmm_map_advect", "cmm_incomp_error", "cmm_invert_error", "cmm_compose_positions",
            "cmm_sample_init", "cmm_translate_workspace_bytes", "cmm_translate_ic", "cmm_hermite_workspace_bytes",
            "cmm_hermite_from_real", "cmm_hermite_from_spectrum", "cmm_psi_workspace_bytes", "cmm_psi_from_vort",
            "cmm_spectral_workspace_bytes", "cmm_spectral_from_real", "cmm_sample_vort", "cmm_vp_psi_workspace_bytes", "cmm_vp_psi_from_f", "cmm_particles_advect", "cmm_euler_create", "cmm_euler_destroy", "cmm_euler_stream",
@@ -100,6 +100,7 @@ def lib():
         L.cmm_map_advect.argtypes = [Grid, Grid, VP, VP, C.POINTER(VP), C.POINTER(AdvectCoeffs), VP, VP, VP,
                                      C.c_size_t, VP]
         L.cmm_incomp_error.argtypes = [Grid, Grid, VP, VP, DP, VP]
+        L.cmm_invert_error.argtypes = [Grid, Grid, Grid, VP, VP, VP, VP, DP, VP]
         L.cmm_compose_positions.argtypes = [Grid, Grid, VP, VP, VP, C.c_int, VP, VP]
         L.cmm_sample_init.argtypes = [Grid, Grid, VP, VP, VP, C.c_int, C.c_int, DP, C.c_int, Grid, VP, VP, VP]
         L.cmm_translate_workspace_bytes.restype = C.c_size_t
@@ -204,6 +205,14 @@ def advect_using_stream_hermite(g_map, g_psi, chix, chiy, psi_levels, coeffs, ch
 def incompressibility_check(g_check, g_map, chix, chiy):
     out = C.c_double()
     _check(lib().cmm_incomp_error(g_map, g_check, _p(chix), _p(chiy), C.byref(out), _stream()), "cmm_incomp_error")
+    return out.value
+
+
+def invertibility_check(g_check, g_back, chix_b, chiy_b, g_fwd, chix_f, chiy_f):
+    """max |x - X_b(X_f(x))| over the cell centres of g_check (cmm_invert_error)"""
+    out = C.c_double()
+    _check(lib().cmm_invert_error(g_back, g_fwd, g_check, _p(chix_b), _p(chiy_b), _p(chix_f), _p(chiy_f),
+                                  C.byref(out), _stream()), "cmm_invert_error")
     return out.value
 
 

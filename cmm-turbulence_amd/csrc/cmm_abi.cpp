@@ -113,7 +113,7 @@ int cmm_map_advect(cmm_grid gm, cmm_grid gp, const double *cx, const double *cy,
         set_error("cmm_map_advect: bad argument");
         return CMM_ERR_ARG;
     }
-    if (c->direction != -1) { set_error("cmm_map_advect: only the backward map is supported"); return CMM_ERR_UNSUPPORTED; }
+    if (c->direction != -1 && c->direction != 1) { set_error("cmm_map_advect: direction must be -1 or 1"); return CMM_ERR_ARG; }
     if (c->l_order < 1 || c->l_order > 4 || c->map_update_order_num < 0 || c->map_update_order_num > 2) {
         set_error("cmm_map_advect: bad coefficients");
         return CMM_ERR_ARG;
@@ -127,6 +127,15 @@ int cmm_map_advect(cmm_grid gm, cmm_grid gp, const double *cx, const double *cy,
 int cmm_incomp_error(cmm_grid gm, cmm_grid gc, const double *cx, const double *cy, double *out, void *stream) {
     if (!grid_ok(gm) || !grid_ok(gc) || !cx || !cy || !out) { set_error("cmm_incomp_error: bad argument"); return CMM_ERR_ARG; }
     return op_incomp(gm, gc, cx, cy, out, S(stream));
+}
+
+int cmm_invert_error(cmm_grid gb, cmm_grid gf, cmm_grid gc, const double *bx, const double *by, const double *fx,
+                     const double *fy, double *out, void *stream) {
+    if (!grid_ok(gb) || !grid_ok(gf) || !grid_ok(gc) || !bx || !by || !fx || !fy || !out) {
+        set_error("cmm_invert_error: bad argument");
+        return CMM_ERR_ARG;
+    }
+    return op_invert(gb, gf, gc, bx, by, fx, fy, out, S(stream));
 }
 
 int cmm_compose_positions(cmm_grid gm, cmm_grid gt, const double *cx, const double *cy, const double *stack,

@@ -25,6 +25,11 @@ int op_incomp_launch(const cmm_grid &gm, const cmm_grid &gc, const double *cx, c
                      unsigned long long *slot, hipStream_t st);
 int op_incomp(const cmm_grid &gm, const cmm_grid &gc, const double *cx, const double *cy, double *out,
               hipStream_t st);
+// max |x - X_b(X_f(x))| over the cell centres of gc (forward map on gf, backward on gb)
+int op_invert_launch(const cmm_grid &gb, const cmm_grid &gf, const cmm_grid &gc, const double *bx, const double *by,
+                     const double *fx, const double *fy, unsigned long long *slot, hipStream_t st);
+int op_invert(const cmm_grid &gb, const cmm_grid &gf, const cmm_grid &gc, const double *bx, const double *by,
+              const double *fx, const double *fy, double *out, hipStream_t st);
 size_t op_psi_workspace(const cmm_grid &gv, const cmm_grid &gp, double fcut);
 int op_psi_from_vort(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf, const cmm_grid &gp, const double *cx,
                      const double *cy, const double *W, int molly, double fcut, double *psi, void *work,

@@ -204,6 +204,65 @@ __device__ __forceinline__ void step_backward(const V &vel, const cmm_advect_coe
     }
 }
 
+// Forward footpoint integration of the forward map, the reference's forward schemes
+// (Cuda-CMM/src/numerical/cmm-timestep.cu: EulerExp :28-33, Heun :48-65, AB2 :86-109, RK3 :143-174,
+// RK4 :218-260, RK3Mod :314-337, RK4Mod :384-430).  As in the reference, RK4 weights its second
+// stage with L1 and its fourth with L12, and RK4Mod is the backward formula.
+template <int TIN, class V>
+__device__ __forceinline__ void step_forward(const V &vel, const cmm_advect_coeffs &c, double x0, double y0,
+                                             double &xo, double &yo) {
+    const double dt = c.dt;
+    if constexpr (TIN == 10) {
+        double ux, uy;
+        vel.lev(0, x0, y0, ux, uy);
+        xo = x0 + dt * ux;
+        yo = y0 + dt * uy;
+    } else if constexpr (TIN == 20) {
+        double u0x, u0y, u1x, u1y, xp = x0, yp = y0;
+        xo = x0;
+        yo = y0;
+        for (int it = 0; it < 10; ++it) {
+            vel.lev(1, xp, yp, u1x, u1y);
+            xp = xo + dt * u1x;
+            yp = yo + dt * u1y;
+            vel.lev(1, xp, yp, u1x, u1y);
+            vel.lev(0, xo, yo, u0x, u0y);
+            xo = x0 + dt * (1.5 * u0x - 0.5 * u1x);
+            yo = y0 + dt * (1.5 * u0y - 0.5 * u1y);
+        }
+    } else if constexpr (TIN == 21) {
+        double k[4];
+        vel.lev(0, x0, y0, k[0], k[1]);
+        vel.w1(c, x0 + dt * k[0], y0 + dt * k[1], k[2], k[3]);
+        xo = x0 + dt * (k[0] + k[2]) / 2;
+        yo = y0 + dt * (k[1] + k[3]) / 2;
+    } else if constexpr (TIN == 30) {
+        double k[6];
+        vel.lev(0, x0, y0, k[0], k[1]);
+        vel.w12(c, x0 + dt * k[0] / 2.0, y0 + dt * k[1] / 2.0, k[2], k[3]);
+        vel.w1(c, x0 - dt * k[0] + 2 * dt * k[2], y0 - dt * k[1] + 2 * dt * k[3], k[4], k[5]);
+        xo = x0 + dt * (k[0] + 4 * k[2] + k[4]) / 6.0;
+        yo = y0 + dt * (k[1] + 4 * k[3] + k[5]) / 6.0;
+    } else if constexpr (TIN == 31) {
+        double k[6];
+        vel.lev(0, x0, y0, k[0], k[1]);
+        vel.lev(1, x0 - dt * k[0], y0 - dt * k[1], k[2], k[3]);
+        vel.lev(2, x0 - dt * (8 * k[0] + 2 * k[2]) / 5.0, y0 - dt * (8 * k[1] + 2 * k[3]) / 5.0, k[4], k[5]);
+        xo = x0 + dt * (23 * k[0] - 16 * k[2] + 5 * k[4]) / 12.0;
+        yo = y0 + dt * (23 * k[1] - 16 * k[3] + 5 * k[5]) / 12.0;
+    } else if constexpr (TIN == 40) {
+        double k[8];
+        vel.lev(0, x0, y0, k[0], k[1]);
+        vel.w1(c, x0 + dt * k[0] / 2.0, y0 + dt * k[1] / 2.0, k[2], k[3]);
+        vel.w12(c, x0 + dt * k[2] / 2.0, y0 + dt * k[3] / 2.0, k[4], k[5]);
+        vel.w12(c, x0 + dt * k[4], y0 + dt * k[5], k[6], k[7]);
+        xo = x0 + dt * (k[0] + 2 * k[2] + 2 * k[4] + k[6]) / 6.0;
+        yo = y0 + dt * (k[1] + 2 * k[3] + 2 * k[5] + k[7]) / 6.0;
+    } else {  // 41
+        step_backward<41>(vel, c, x0, y0, xo, yo);
+    }
+}
+
 // Node-interleaved fields for the advection: C1[4j + p] = sum_l L1[l] psi_l[p*n + j], likewise C12
 // with L12 (when set), and C0[4j + p] = psi_0[p*n + j] (when set); sums in level order like the
 // weighted velocities.  One lane per node: plane reads coalesce across lanes, and each lane writes
@@ -260,6 +319,34 @@ __global__ void __launch_bounds__(256) k_interleave_map(long n, const double *__
 }
 
 // P2 (with COMB): both grids have power-of-two point counts
+// Output stage of the advection: lane k of a node's group forms output plane k (and k + G) by
+// reading the group's traced positions with cross-lane shuffles and summing them in footpoint
+// order with the stencil weights, the order of the reference's accumulation.
+template <int G>
+__device__ __forceinline__ void advect_store(const DGrid &gm, const cmm_advect_coeffs &c, int nfoot, int node,
+                                             bool active, int k, double px, double py, double *__restrict__ ncx,
+                                             double *__restrict__ ncy, double *__restrict__ M8out) {
+    const int base = (threadIdx.x & 63) - k;  // first lane of the group within the wavefront
+    // every lane takes part in every shuffle (the same trip count for all lanes)
+#pragma unroll
+    for (int r = 0; r < (8 + G - 1) / G; ++r) {
+        const int o = k + r * G;
+        const int plane = o & 3;
+        const double *w = plane == 0 ? c.c1 : plane == 1 ? c.cx : plane == 2 ? c.cy : c.cxy;
+        double acc = 0.0;
+        for (int m = 0; m < nfoot; ++m) {
+            const double vx = __shfl(px, base + m, 64);
+            const double vy = __shfl(py, base + m, 64);
+            acc += (o < 4 ? vx : vy) * w[m];
+        }
+        if (active && o < 8) {
+            (o < 4 ? ncx : ncy)[(long)plane * gm.n + node] = acc;
+            // the new map node-interleaved as well, for the next step's map evaluations
+            if (M8out) M8out[8L * node + o] = acc;
+        }
+    }
+}
+
 template <int TIN, int G, bool COMB, bool P2 = false>
 __global__ void __launch_bounds__(256) k_map_advect(DGrid gm, DGrid gp, const double *__restrict__ cx,
                                                     const double *__restrict__ cy, PsiLevels P, const double *C1,
@@ -287,25 +374,31 @@ __global__ void __launch_bounds__(256) k_map_advect(DGrid gm, DGrid gp, const do
             diffeo(gm, cx, cy, xf, yf, px, py);
         }
     }
-    const int base = (threadIdx.x & 63) - k;  // first lane of the group within the wavefront
-    // every lane takes part in every shuffle (the same trip count for all lanes)
-#pragma unroll
-    for (int r = 0; r < (8 + G - 1) / G; ++r) {
-        const int o = k + r * G;
-        const int plane = o & 3;
-        const double *w = plane == 0 ? c.c1 : plane == 1 ? c.cx : plane == 2 ? c.cy : c.cxy;
-        double acc = 0.0;
-        for (int m = 0; m < nfoot; ++m) {
-            const double vx = __shfl(px, base + m, 64);
-            const double vy = __shfl(py, base + m, 64);
-            acc += (o < 4 ? vx : vy) * w[m];
-        }
-        if (active && o < 8) {
-            (o < 4 ? ncx : ncy)[(long)plane * gm.n + node] = acc;
-            // the new map node-interleaved as well, for the next step's map evaluations
-            if (M8out) M8out[8L * node + o] = acc;
-        }
+    advect_store<G>(gm, c, nfoot, node, active, k, px, py, ncx, ncy, M8out);
+}
+
+// The forward map (Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:254-274): the old forward map
+// is evaluated at the footpoint first, then the point moves forward in time; psi per level.
+template <int TIN, int G>
+__global__ void __launch_bounds__(256) k_map_advect_fwd(DGrid gm, DGrid gp, const double *__restrict__ cx,
+                                                        const double *__restrict__ cy, PsiLevels P, cmm_advect_coeffs c,
+                                                        double *__restrict__ ncx, double *__restrict__ ncy, int nblocks) {
+    constexpr int NPB = 256 / G;
+    const int blk = xcd_block(blockIdx.x, nblocks);
+    const int node = blk * NPB + threadIdx.x / G;
+    const int k = threadIdx.x % G;
+    const int nfoot = 4 + 4 * c.map_update_order_num;
+    const bool active = node < gm.n;
+    double px = 0.0, py = 0.0;
+    if (active && k < nfoot) {
+        const int ix = node % gm.nx, iy = node / gm.nx;
+        const double xe = gm.x0 + ix * gm.hx + ((k >> 2) + 1) * c.eps * (1 - 2 * ((k >> 1) & 1));
+        const double ye = gm.y0 + iy * gm.hy + ((k >> 2) + 1) * c.eps * (1 - 2 * (((k + 1) >> 1) & 1));
+        double xf, yf;
+        diffeo(gm, cx, cy, xe, ye, xf, yf);
+        step_forward<TIN>(VelLevels{gp, P}, c, xf, yf, px, py);
     }
+    advect_store<G>(gm, c, nfoot, node, active, k, px, py, ncx, ncy, nullptr);
 }
 
 // max |1 - det grad X| over cell centres of gc; each block reduces a strided set of 64x4 tiles,
@@ -326,6 +419,37 @@ __global__ void __launch_bounds__(256) k_incomp(DGrid gm, DGrid gc, const double
             const double x = gm.x0 + ix * gc.hx + 0.5 * gc.hx;
             const double y = gm.y0 + iy * gc.hy + 0.5 * gc.hy;
             v = fmax(v, fabs(1 - diffeo_det(gm, cx, cy, x, y)));
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    __shared__ double red[4];
+    if (threadIdx.x == 0) red[threadIdx.y] = v;
+    __syncthreads();
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        double m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+        atomicMax(out, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+// max over cell centres of gc of |x - X_b(X_f(x))|, k_invertibility_check and invertibility_check
+// (Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:177-198,
+// Cuda-CMM/src/simulation/cmm-simulation-host.cu:52-60); the same reduction as k_incomp
+__global__ void __launch_bounds__(256) k_invert(DGrid gb, DGrid gf, DGrid gc, const double *__restrict__ bx,
+                                                const double *__restrict__ by, const double *__restrict__ fx,
+                                                const double *__restrict__ fy, unsigned long long *__restrict__ out,
+                                                int tiles_x, int tiles_total) {
+    double v = 0.0;
+    for (int tt = blockIdx.x; tt < tiles_total; tt += gridDim.x) {
+        const int t = xcd_block(tt, tiles_total);
+        const int ix = (t % tiles_x) * 64 + threadIdx.x;
+        const int iy = (t / tiles_x) * 4 + threadIdx.y;
+        if (ix < gc.nx && iy < gc.ny) {
+            const double x = gc.x0 + ix * gc.hx + 0.5 * gc.hx;
+            const double y = gc.y0 + iy * gc.hy + 0.5 * gc.hy;
+            double xi, yi;
+            diffeo(gf, fx, fy, x, y, xi, yi);
+            diffeo(gb, bx, by, xi, yi, xi, yi);
+            v = fmax(v, hypot(x - xi, y - yi));
         }
     }
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
@@ -1358,6 +1482,29 @@ int op_map_advect(const cmm_grid &gm, const cmm_grid &gp, const double *cx, cons
     const int nfoot = 4 + 4 * c.map_update_order_num;
     const int G = nfoot <= 4 ? 4 : nfoot <= 8 ? 8 : 16;
     const int nb = (int)(((long)dm.n * G + 255) / 256);
+    if (c.direction == 1) {
+        if (mid) (void)hipEventRecord(mid, st);
+#define CMM_FWD(T, GG) hipLaunchKernelGGL((k_map_advect_fwd<T, GG>), dim3(nb), dim3(256), 0, st, dm, dp, cx, cy, P, c, ncx, ncy, nb)
+#define CMM_FWD_G(T)                                                                                                   \
+    do {                                                                                                               \
+        if (G == 4) CMM_FWD(T, 4);                                                                                     \
+        else if (G == 8) CMM_FWD(T, 8);                                                                                \
+        else CMM_FWD(T, 16);                                                                                           \
+    } while (0)
+        switch (c.time_integration_num) {
+        case 10: CMM_FWD_G(10); break;
+        case 20: CMM_FWD_G(20); break;
+        case 21: CMM_FWD_G(21); break;
+        case 30: CMM_FWD_G(30); break;
+        case 31: CMM_FWD_G(31); break;
+        case 40: CMM_FWD_G(40); break;
+        case 41: CMM_FWD_G(41); break;
+        default: set_error("unknown time integration"); return CMM_ERR_ARG;
+        }
+#undef CMM_FWD_G
+#undef CMM_FWD
+        return launch_check("k_map_advect_fwd");
+    }
     const size_t need = op_advect_workspace(gm, gp, c);
     const bool comb = work && need > 0 && work_bytes >= need;
     const double *M8 = nullptr;
@@ -1447,6 +1594,15 @@ int op_incomp_launch(const cmm_grid &gm, const cmm_grid &gc, const double *cx, c
     return launch_check("k_incomp");
 }
 
+int op_invert_launch(const cmm_grid &gb, const cmm_grid &gf, const cmm_grid &gc, const double *bx, const double *by,
+                     const double *fx, const double *fy, unsigned long long *slot, hipStream_t st) {
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, slot);
+    const int tx = (gc.nx + 63) / 64, ty = (gc.ny + 3) / 4, tt = tx * ty;
+    hipLaunchKernelGGL(k_invert, dim3(std::min(tt, 2048)), dim3(64, 4), 0, st, make_dgrid(gb), make_dgrid(gf),
+                       make_dgrid(gc), bx, by, fx, fy, slot, tx, tt);
+    return launch_check("k_invert");
+}
+
 int op_incomp(const cmm_grid &gm, const cmm_grid &gc, const double *cx, const double *cy, double *out,
               hipStream_t st) {
     unsigned long long *slot;
@@ -1458,6 +1614,18 @@ int op_incomp(const cmm_grid &gm, const cmm_grid &gc, const double *cx, const do
     double v;
     std::memcpy(&v, &h, sizeof(v));
     *out = v;
+    return CMM_OK;
+}
+
+int op_invert(const cmm_grid &gb, const cmm_grid &gf, const cmm_grid &gc, const double *bx, const double *by,
+              const double *fx, const double *fy, double *out, hipStream_t st) {
+    unsigned long long *slot;
+    if (int rc = scalar_slot(&slot)) return rc;
+    if (int rc = op_invert_launch(gb, gf, gc, bx, by, fx, fy, slot, st)) return rc;
+    unsigned long long h = 0;
+    if (int rc = hip_check(hipMemcpyAsync(&h, slot, sizeof(h), hipMemcpyDeviceToHost, st), "hipMemcpyAsync")) return rc;
+    if (int rc = hip_check(hipStreamSynchronize(st), "hipStreamSynchronize")) return rc;
+    std::memcpy(out, &h, sizeof(double));
     return CMM_OK;
 }
 

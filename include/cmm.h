@@ -40,9 +40,12 @@ int cmm_advect_coeffs_fill(const double *t, const double *dt, int loop_ctr, int 
 /* identity map; replaces k_init_diffeo (Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:27-41) */
 int cmm_map_init_identity(cmm_grid g, double *chix, double *chiy, void *stream);
 
-/* Backward advection of the Hermite map through psi; replaces advect_using_stream_hermite +
+/* Advection of the Hermite map through psi; replaces advect_using_stream_hermite +
  * k_advect_using_stream_hermite (Cuda-CMM/src/simulation/cmm-simulation-host.cu:64-104,
- * Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:207-298).  psi_levels is a host array of
+ * Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:207-298).  c->direction -1 advects a backward
+ * map (trace back, then the old map); +1 a forward map (the old map, then forward in time, with
+ * the forward schemes of Cuda-CMM/src/numerical/cmm-timestep.cu; the workspace is not used).
+ * psi_levels is a host array of
  * c->l_order device pointers (newest first, 4*Npsi doubles each).  chix_new/chiy_new must not
  * alias chix/chiy.
  * work may be NULL: every footpoint evaluation then interpolates each psi level and weights the
@@ -60,6 +63,11 @@ int cmm_map_advect(cmm_grid g_map, cmm_grid g_psi, const double *chix, const dou
  * stream); replaces incompressibility_check (Cuda-CMM/src/simulation/cmm-simulation-host.cu:44-50). */
 int cmm_incomp_error(cmm_grid g_map, cmm_grid g_check, const double *chix, const double *chiy, double *out,
                      void *stream);
+/* Invertibility of the backward and forward maps: max over the cell centres of g_check of
+ * |x - X_b(X_f(x))|; replaces invertibility_check (Cuda-CMM/src/simulation/cmm-simulation-host.cu:52-60)
+ * with k_invertibility_check (Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:177-198). */
+int cmm_invert_error(cmm_grid g_back, cmm_grid g_fwd, cmm_grid g_check, const double *chix_b, const double *chiy_b,
+                     const double *chix_f, const double *chiy_f, double *out, void *stream);
 
 /* ---- stack composition and initial condition ---------------------------------------------------
  * The stack is device resident: map m at stack + 8*Nc*m (ChiX planes then ChiY planes), m = 0

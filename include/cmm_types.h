@@ -37,7 +37,7 @@ typedef struct cmm_advect_coeffs {
     int time_integration_num;  /* 10 EulerExp, 20 AB2, 21 RK2/Heun, 30 RK3, 31 RK3Mod, 40 RK4, 41 RK4Mod */
     int map_update_order_num;  /* 0: 2nd (4 footpoints), 1: 4th (8), 2: 6th (12) */
     int l_order;               /* number of psi levels (1..4) */
-    int direction;             /* -1: backward map (the only direction on this path) */
+    int direction;             /* -1: backward map, +1: forward map */
 } cmm_advect_coeffs;
 
 /* Initial-condition identifiers.  Vorticity: 0..11 follow Cuda-CMM/src/ui/settings.h:313-324;

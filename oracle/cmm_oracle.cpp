@@ -281,6 +281,108 @@ void step_backward(const Grid &g, const double *psi, const cmm_advect_coeffs &c,
     }
 }
 
+// forward time integrators of the forward map, Cuda-CMM/src/numerical/cmm-timestep.cu:28-33 (EulerExp),
+// :48-65 (Heun), :86-109 (AB2), :143-174 (RK3), :218-260 (RK4), :314-337 (RK3Mod), :384-430 (RK4Mod).
+// As in the reference: RK4 weights its second stage with L1 and its fourth with L12, and RK4Mod is
+// the backward formula.
+void step_forward(const Grid &g, const double *psi, const cmm_advect_coeffs &c, const double xin[2], double xout[2]) {
+    const int L = c.l_order;
+    const double dt = c.dt;
+    const long N4 = 4 * g.n;
+    double u[8];
+    switch (c.time_integration_num) {
+    case 10: {
+        herm_grad(g, psi, 1, xin[0], xin[1], u);
+        xout[0] = xin[0] + dt * u[0]; xout[1] = xin[1] + dt * u[1];
+        break;
+    }
+    case 20: {
+        double uu[4], xp[2] = {xin[0], xin[1]};
+        xout[0] = xin[0]; xout[1] = xin[1];
+        for (int it = 0; it < 10; ++it) {
+            herm_grad(g, psi + N4, 1, xp[0], xp[1], uu + 2);
+            xp[0] = xout[0] + dt * uu[2]; xp[1] = xout[1] + dt * uu[3];
+            herm_grad(g, psi + N4, 1, xp[0], xp[1], uu + 2);
+            herm_grad(g, psi, 1, xout[0], xout[1], uu);
+            xout[0] = xin[0] + dt * (1.5 * uu[0] - 0.5 * uu[2]);
+            xout[1] = xin[1] + dt * (1.5 * uu[1] - 0.5 * uu[3]);
+        }
+        break;
+    }
+    case 21: {
+        double k[4];
+        herm_grad(g, psi, 1, xin[0], xin[1], k);
+        double p[2] = {xin[0] + dt * k[0], xin[1] + dt * k[1]};
+        herm_grad(g, psi, L, p[0], p[1], u);
+        lsum(c.L1, u, L, k[2], k[3]);
+        xout[0] = xin[0] + dt * (k[0] + k[2]) / 2;
+        xout[1] = xin[1] + dt * (k[1] + k[3]) / 2;
+        break;
+    }
+    case 30: {
+        double k[6];
+        herm_grad(g, psi, 1, xin[0], xin[1], k);
+        double p[2] = {xin[0] + dt * k[0] / 2.0, xin[1] + dt * k[1] / 2.0};
+        herm_grad(g, psi, L, p[0], p[1], u);
+        lsum(c.L12, u, L, k[2], k[3]);
+        p[0] = xin[0] - dt * k[0] + 2 * dt * k[2];
+        p[1] = xin[1] - dt * k[1] + 2 * dt * k[3];
+        herm_grad(g, psi, L, p[0], p[1], u);
+        lsum(c.L1, u, L, k[4], k[5]);
+        xout[0] = xin[0] + dt * (k[0] + 4 * k[2] + k[4]) / 6.0;
+        xout[1] = xin[1] + dt * (k[1] + 4 * k[3] + k[5]) / 6.0;
+        break;
+    }
+    case 31: {
+        double k[6];
+        herm_grad(g, psi, 1, xin[0], xin[1], k);
+        double p[2] = {xin[0] - dt * k[0], xin[1] - dt * k[1]};
+        herm_grad(g, psi + N4, 1, p[0], p[1], k + 2);
+        p[0] = xin[0] - dt * (8 * k[0] + 2 * k[2]) / 5.0;
+        p[1] = xin[1] - dt * (8 * k[1] + 2 * k[3]) / 5.0;
+        herm_grad(g, psi + 2 * N4, 1, p[0], p[1], k + 4);
+        xout[0] = xin[0] + dt * (23 * k[0] - 16 * k[2] + 5 * k[4]) / 12.0;
+        xout[1] = xin[1] + dt * (23 * k[1] - 16 * k[3] + 5 * k[5]) / 12.0;
+        break;
+    }
+    case 40: {
+        double k[8];
+        herm_grad(g, psi, 1, xin[0], xin[1], k);
+        double p[2] = {xin[0] + dt * k[0] / 2.0, xin[1] + dt * k[1] / 2.0};
+        herm_grad(g, psi, L, p[0], p[1], u);
+        lsum(c.L1, u, L, k[2], k[3]);
+        p[0] = xin[0] + dt * k[2] / 2.0; p[1] = xin[1] + dt * k[3] / 2.0;
+        herm_grad(g, psi, L, p[0], p[1], u);
+        lsum(c.L12, u, L, k[4], k[5]);
+        p[0] = xin[0] + dt * k[4]; p[1] = xin[1] + dt * k[5];
+        herm_grad(g, psi, L, p[0], p[1], u);
+        lsum(c.L12, u, L, k[6], k[7]);
+        xout[0] = xin[0] + dt * (k[0] + 2 * k[2] + 2 * k[4] + k[6]) / 6.0;
+        xout[1] = xin[1] + dt * (k[1] + 2 * k[3] + 2 * k[5] + k[7]) / 6.0;
+        break;
+    }
+    case 41: {
+        double k[8];
+        herm_grad(g, psi, L, xin[0], xin[1], u);
+        lsum(c.L1, u, L, k[0], k[1]);
+        double p[2] = {xin[0] - dt * k[0], xin[1] - dt * k[1]};
+        herm_grad(g, psi, 1, p[0], p[1], k + 2);
+        p[0] = xin[0] - dt * (3 * k[0] + k[2]) / 8.0;
+        p[1] = xin[1] - dt * (3 * k[1] + k[3]) / 8.0;
+        herm_grad(g, psi, L, p[0], p[1], u);
+        lsum(c.L12, u, L, k[4], k[5]);
+        p[0] = xin[0] + dt * (k[2] - 4 * k[4]) / 3.0;
+        p[1] = xin[1] + dt * (k[3] - 4 * k[5]) / 3.0;
+        herm_grad(g, psi, 1, p[0], p[1], k + 6);
+        xout[0] = xin[0] - dt * (2 * k[0] - k[2] + 8 * k[4] + 3 * k[6]) / 12.0;
+        xout[1] = xin[1] - dt * (2 * k[1] - k[3] + 8 * k[5] + 3 * k[7]) / 12.0;
+        break;
+    }
+    default:
+        xout[0] = xout[1] = 0;
+    }
+}
+
 // ----------------------------------------------------------------------------------------------
 // FFT: iterative radix-2 complex transform, twiddles from long double.
 typedef std::complex<double> cplx;
@@ -868,7 +970,7 @@ int cmmo_map_init_identity(cmm_grid gg, double *cx, double *cy) {
 
 int cmmo_map_advect(cmm_grid ggm, cmm_grid ggp, const double *cx, const double *cy, const double *psi,
                     const cmm_advect_coeffs *c, double *nx_, double *ny_) {
-    if (!c || c->direction != -1) return CMM_ERR_UNSUPPORTED;
+    if (!c || (c->direction != -1 && c->direction != 1)) return CMM_ERR_UNSUPPORTED;
     Grid gm = mk(ggm), gp = mk(ggp);
     const int nfoot = 4 + 4 * c->map_update_order_num;
 #pragma omp parallel for schedule(static)
@@ -879,9 +981,19 @@ int cmmo_map_advect(cmm_grid ggm, cmm_grid ggp, const double *cx, const double *
                 double xe[2], xf[2];
                 xe[0] = gm.x0 + ix * gm.hx + ((k >> 2) + 1) * c->eps * (1 - 2 * ((k >> 1) & 1));
                 xe[1] = gm.y0 + iy * gm.hy + ((k >> 2) + 1) * c->eps * (1 - 2 * (((k + 1) >> 1) & 1));
-                step_backward(gp, psi, *c, xe, xf);
                 double px, py;
-                diffeo(gm, cx, cy, xf[0], xf[1], px, py);
+                if (c->direction == -1) {
+                    // backward: trace back, then the old map, Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:232-252
+                    step_backward(gp, psi, *c, xe, xf);
+                    diffeo(gm, cx, cy, xf[0], xf[1], px, py);
+                } else {
+                    // forward: the old map first, then forward in time, :254-274
+                    diffeo(gm, cx, cy, xe[0], xe[1], xf[0], xf[1]);
+                    double xo[2];
+                    step_forward(gp, psi, *c, xf, xo);
+                    px = xo[0];
+                    py = xo[1];
+                }
                 ax[0] += px * c->c1[k]; ay[0] += py * c->c1[k];
                 ax[1] += px * c->cx[k]; ay[1] += py * c->cx[k];
                 ax[2] += px * c->cy[k]; ay[2] += py * c->cy[k];
@@ -892,6 +1004,27 @@ int cmmo_map_advect(cmm_grid ggm, cmm_grid ggp, const double *cx, const double *
         }
     }
     return CMM_OK;
+}
+
+// max over cell centres of the check grid of |x - X_b(X_f(x))|, k_invertibility_check and
+// invertibility_check (Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:177-198,
+// Cuda-CMM/src/simulation/cmm-simulation-host.cu:52-60)
+double cmmo_invert_error(cmm_grid ggb, cmm_grid ggf, cmm_grid ggc, const double *bx, const double *by, const double *fx,
+                         const double *fy) {
+    Grid gb = mk(ggb), gf = mk(ggf), gc = mk(ggc);
+    double mx = 0.0;
+#pragma omp parallel for reduction(max : mx)
+    for (int iy = 0; iy < gc.ny; ++iy)
+        for (int ix = 0; ix < gc.nx; ++ix) {
+            double x = gc.x0 + ix * gc.hx + 0.5 * gc.hx;
+            double y = gc.y0 + iy * gc.hy + 0.5 * gc.hy;
+            double xi, yi;
+            diffeo(gf, fx, fy, x, y, xi, yi);
+            diffeo(gb, bx, by, xi, yi, xi, yi);
+            double d = std::hypot(x - xi, y - yi);
+            if (d > mx) mx = d;
+        }
+    return mx;
 }
 
 double cmmo_incomp_error(cmm_grid ggm, cmm_grid ggc, const double *cx, const double *cy) {
@@ -1037,6 +1170,8 @@ int cmmo_euler_run(const cmmo_euler_params *p, cmmo_euler_result *r) {
     std::vector<double> cx(4 * Nc), cy(4 * Nc), ncx(4 * Nc), ncy(4 * Nc);
     std::vector<double> W(4 * Nf), psi((size_t)4 * Np * L, 0.0), xy(2 * Nf), samp(Nf);
     std::vector<double> stx, sty;
+    // forward map and its stack (only with p->forward_map)
+    std::vector<double> fx(4 * Nc), fy(4 * Nc), nfx(4 * Nc), nfy(4 * Nc), fstx, fsty;
     int nstack = 0;
     r->n_warmup = 0;
 
@@ -1060,6 +1195,7 @@ int cmmo_euler_run(const cmmo_euler_params *p, cmmo_euler_result *r) {
     };
 
     cmmo_map_init_identity(p->g_coarse, cx.data(), cy.data());
+    cmmo_map_init_identity(p->g_coarse, fx.data(), fy.data());
     translate();
     eval_psi();
 
@@ -1114,7 +1250,21 @@ int cmmo_euler_run(const cmmo_euler_params *p, cmmo_euler_result *r) {
         tv[lc + 1] = tv[lc] + dtn;
         dtv[lc + 1] = dtn;
         advect(tv.data(), dtv.data(), loop, L, p->time_integration_num);
+        double einc_f = 0, einv = 0;
+        if (p->forward_map) {
+            // the forward map with the same psi history and coefficients, :780-785
+            cmm_advect_coeffs c;
+            cmmo_advect_coeffs_fill(tv.data(), dtv.data(), loop, L, p->time_integration_num, p->map_update_order_num,
+                                    p->map_epsilon, 1, &c);
+            cmmo_map_advect(p->g_coarse, p->g_psi, fx.data(), fy.data(), psi.data(), &c, nfx.data(), nfy.data());
+            std::swap(fx, nfx); std::swap(fy, nfy);
+        }
         double einc = cmmo_incomp_error(p->g_coarse, p->g_fine, cx.data(), cy.data());
+        if (p->forward_map) {
+            // :799-802, both on the coarse grid
+            einv = cmmo_invert_error(p->g_coarse, p->g_coarse, p->g_coarse, cx.data(), cy.data(), fx.data(), fy.data());
+            einc_f = cmmo_incomp_error(p->g_coarse, p->g_coarse, fx.data(), fy.data());
+        }
         if (einc > p->incomp_threshold && !p->skip_remapping) {
             if (nstack >= p->stack_capacity) {
                 cont = false;
@@ -1124,8 +1274,15 @@ int cmmo_euler_run(const cmmo_euler_params *p, cmmo_euler_result *r) {
                 sty.insert(sty.end(), cy.begin(), cy.end());
                 nstack++;
                 cmmo_map_init_identity(p->g_coarse, cx.data(), cy.data());
+                if (p->forward_map) {
+                    fstx.insert(fstx.end(), fx.begin(), fx.end());
+                    fsty.insert(fsty.end(), fy.begin(), fy.end());
+                    cmmo_map_init_identity(p->g_coarse, fx.data(), fy.data());
+                }
             }
         }
+        if (r->incomp_fwd) r->incomp_fwd[loop] = einc_f;
+        if (r->invert) r->invert[loop] = einv;
         for (int il = 1; il < L; ++il)
             std::memcpy(&psi[(size_t)4 * Np * (L - il)], &psi[(size_t)4 * Np * (L - il - 1)], sizeof(double) * 4 * Np);
         eval_psi();
@@ -1147,6 +1304,8 @@ int cmmo_euler_run(const cmmo_euler_params *p, cmmo_euler_result *r) {
     if (r->psi) std::memcpy(r->psi, psi.data(), sizeof(double) * 4 * Np * L);
     if (r->w_herm) std::memcpy(r->w_herm, W.data(), sizeof(double) * 4 * Nf);
     if (r->vort_coarse) sample_vort(gc, gc, gf, cx.data(), cy.data(), W.data(), 0, r->vort_coarse);
+    if (r->fchix) std::memcpy(r->fchix, fx.data(), sizeof(double) * 4 * Nc);
+    if (r->fchiy) std::memcpy(r->fchiy, fy.data(), sizeof(double) * 4 * Nc);
     return CMM_OK;
 }
 

@@ -42,6 +42,8 @@ int cmmo_map_advect(cmm_grid gm, cmm_grid gpsi, const double *chix, const double
                     const double *psi, const cmm_advect_coeffs *c, double *chix_new, double *chiy_new);
 /* max |1 - det grad X| on cell centres of gcheck, Cuda-CMM/src/simulation/cmm-simulation-host.cu:44-50 */
 double cmmo_incomp_error(cmm_grid gm, cmm_grid gcheck, const double *chix, const double *chiy);
+double cmmo_invert_error(cmm_grid g_back, cmm_grid g_fwd, cmm_grid gcheck, const double *bx, const double *by,
+                         const double *fx, const double *fy);
 /* Positions X_1(...X_M(X_cur(x))) on gt, Cuda-CMM/src/simulation/cmm-simulation-host.cu:112-152 */
 int cmmo_compose_positions(cmm_grid gm, cmm_grid gt, const double *chix, const double *chiy,
                            const double *stack_x, const double *stack_y, int n_stack, double *xy);
@@ -93,6 +95,7 @@ typedef struct cmmo_euler_params {
     double *particles;         /* in/out, 2*n_particles */
     int max_steps;             /* stop after this many steps (<=0: run to final_time) */
     int sim;                   /* 0: 2D Euler (cmm-euler2d.cu), 1: 1D-1V Vlasov-Poisson (cmm-vlasov1d.cu) */
+    int forward_map;           /* also advect the forward map (cmm-euler2d.cu:780-785) */
 } cmmo_euler_params;
 
 typedef struct cmmo_euler_result {
@@ -106,6 +109,9 @@ typedef struct cmmo_euler_result {
     double *psi;               /* final psi history (optional, l_order*4*Npsi) */
     double *w_herm;            /* final fine vorticity Hermite (optional, 4*Nf) */
     double *vort_coarse;       /* final coarse vorticity sample (optional, Nc) */
+    double *incomp_fwd;        /* [steps] E_inc of the forward map on the coarse grid (optional) */
+    double *invert;            /* [steps] invertibility error (optional) */
+    double *fchix, *fchiy;     /* final forward map (optional, 4*Nc each) */
 } cmmo_euler_result;
 
 int cmmo_euler_run(const cmmo_euler_params *p, cmmo_euler_result *r);

@@ -33,7 +33,8 @@ class EulerParams(C.Structure):
                 ("ic", C.c_int), ("ic_params", C.c_double * 10),
                 ("discrete", C.c_int), ("g_disc", Grid), ("disc_herm", C.POINTER(C.c_double)),
                 ("stack_capacity", C.c_int), ("n_particles", C.c_int), ("particle_scheme", C.c_int),
-                ("particles", C.POINTER(C.c_double)), ("max_steps", C.c_int), ("sim", C.c_int)]
+                ("particles", C.POINTER(C.c_double)), ("max_steps", C.c_int), ("sim", C.c_int),
+                ("forward_map", C.c_int)]
 
 
 class EulerResult(C.Structure):
@@ -42,7 +43,9 @@ class EulerResult(C.Structure):
                 ("map_counter", C.POINTER(C.c_double)),
                 ("chix", C.POINTER(C.c_double)), ("chiy", C.POINTER(C.c_double)),
                 ("psi", C.POINTER(C.c_double)), ("w_herm", C.POINTER(C.c_double)),
-                ("vort_coarse", C.POINTER(C.c_double))]
+                ("vort_coarse", C.POINTER(C.c_double)),
+                ("incomp_fwd", C.POINTER(C.c_double)), ("invert", C.POINTER(C.c_double)),
+                ("fchix", C.POINTER(C.c_double)), ("fchiy", C.POINTER(C.c_double))]
 
 
 TWO_PI = 6.283185307179586476925286766559005768394338798750211641949889184615632812572
@@ -58,6 +61,7 @@ def lib():
             raise RuntimeError("oracle library missing: run make -C oracle")
         L = C.CDLL(LIB_PATH)
         L.cmmo_incomp_error.restype = C.c_double
+        L.cmmo_invert_error.restype = C.c_double
         L.cmmo_hermite_eval.restype = C.c_double
         L.cmmo_diffeo_det.restype = C.c_double
         L.cmmo_lagrange_coeff.restype = C.c_double
@@ -207,9 +211,10 @@ def vp_grid(n, nv=None):
 def euler_run(g_coarse, g_fine, g_psi, g_vort, dt, final_time, tin=30, muo=1, l_order=3, lihi=1,
               eps=1e-3, threshold=1e-4, fcut=128.0, molly=0, skip_remapping=0, ic=0, ic_params=None,
               disc=None, g_disc=None, stack_capacity=1000, particles=None, particle_scheme=30,
-              max_steps=0, want=("incomp", "time", "map_counter"), sim=0):
+              max_steps=0, want=("incomp", "time", "map_counter"), sim=0, forward_map=0):
     p = EulerParams()
     p.sim = sim
+    p.forward_map = forward_map
     p.g_coarse, p.g_fine, p.g_psi, p.g_vort = g_coarse, g_fine, g_psi, g_vort
     p.dt, p.final_time = dt, final_time
     p.time_integration_num, p.map_update_order_num, p.l_order = tin, muo, l_order
@@ -237,14 +242,15 @@ def euler_run(g_coarse, g_fine, g_psi, g_vort, dt, final_time, tin=30, muo=1, l_
     iter_max = int(1.05 * np.ceil(final_time / dt)) + 1
     res = EulerResult()
     out = {}
-    for name in ("time", "incomp", "map_counter"):
+    for name in ("time", "incomp", "map_counter", "incomp_fwd", "invert"):
         a = np.zeros(iter_max)
         out[name] = a
         setattr(res, name, ptr(a))
     nc = g_coarse.nx * g_coarse.ny
     npsi = g_psi.nx * g_psi.ny
     nf = g_fine.nx * g_fine.ny
-    sizes = {"chix": 4 * nc, "chiy": 4 * nc, "psi": 4 * npsi * l_order, "w_herm": 4 * nf, "vort_coarse": nc}
+    sizes = {"chix": 4 * nc, "chiy": 4 * nc, "psi": 4 * npsi * l_order, "w_herm": 4 * nf, "vort_coarse": nc,
+             "fchix": 4 * nc, "fchiy": 4 * nc}
     for name, n in sizes.items():
         if name in want:
             a = np.zeros(n)
@@ -253,7 +259,7 @@ def euler_run(g_coarse, g_fine, g_psi, g_vort, dt, final_time, tin=30, muo=1, l_
     rc = lib().cmmo_euler_run(C.byref(p), C.byref(res))
     assert rc == 0
     steps = res.steps
-    for name in ("time", "incomp", "map_counter"):
+    for name in ("time", "incomp", "map_counter", "incomp_fwd", "invert"):
         out[name] = out[name][:steps]
     out["steps"] = steps
     out["warmup_incomp"] = [res.warmup_incomp[i] for i in range(res.n_warmup)]

@@ -44,13 +44,14 @@ def test_argument_errors_without_device():
     h = C.c_void_p()
     assert lib.cmm_euler_create(C.byref(p), C.byref(h)) == -1
     assert h.value is None
-    # backward map only
-    c.direction = 1
+    # the direction is -1 (backward map) or +1 (forward map)
+    c.direction = 0
     c.l_order = 3
     g = cmm.grid(32)
     arr = (C.c_void_p * 4)(1, 1, 1, 1)
     assert lib.cmm_map_advect(g, g, C.c_void_p(8), C.c_void_p(16), arr, C.byref(c), C.c_void_p(24),
-                              C.c_void_p(32), None, 0, None) == -4
+                              C.c_void_p(32), None, 0, None) == -1
+    assert b"direction" in lib.cmm_last_error()
     # workspace sizes: the combined fields C1 (all), C12 (RK3/RK4 variants) and a copy of level 0
     # (all but Euler), none for AB2 and RK3-mod (they read level 1 on its own) or a single level
     for tin, L, fields in ((30, 3, 3), (40, 3, 3), (41, 4, 3), (10, 3, 1), (21, 2, 2), (20, 3, 0), (31, 3, 0),

@@ -189,6 +189,52 @@ def test_map_advect_schemes(dev, base, ratio):
                     assert e <= TOL, (ratio, dt, tin, muo, work is not None, e)
 
 
+@pytest.mark.parametrize("ratio", [1, 2])
+def test_map_advect_forward(dev, base, ratio):
+    """The forward map (direction +1): every scheme and stencil order advects the deformed map
+    forward with the reference's forward schemes, against the oracle; then the invertibility check
+    of the backward and forward maps of the same step against the oracle's."""
+    import cmm
+    gc, gf, _ = grids()
+    ogc, ogf, _ = ogrids()
+    npsi = GC * ratio
+    ogp, gp = O.grid(npsi), cmm.grid(npsi)
+    if ratio == 1:
+        psi = base["psi"]
+    else:
+        p1 = O.psi_from_vort(ogc, ogf, ogf, ogp, base["chix"], base["chiy"], base["w_herm"], 0, 128.0)
+        psi = np.concatenate([p1, 0.9 * p1, 0.8 * p1])
+    n4 = 4 * npsi * npsi
+    lv = [dev_array(psi[i * n4:(i + 1) * n4]) for i in range(3)]
+    mx, my = dev_array(base["chix"]), dev_array(base["chiy"])
+    n = GC * GC
+    dt = 1.0 / 32
+    t = np.array([-2 * dt, -dt, 0.0, dt])
+    dts = np.full(4, dt)
+    for tin in (10, 20, 21, 30, 31, 40, 41):
+        for muo in (0, 1, 2):
+            oc = O.advect_coeffs(t, dts, 0, 3, tin, muo, 1e-3, direction=1)
+            c = cmm.advect_coeffs(t, dts, 0, 3, tin, muo, 1e-3, direction=1)
+            rx, ry = O.map_advect(ogc, ogp, base["chix"], base["chiy"], psi, oc)
+            ox = dev_array(np.zeros(4 * n))
+            oy = dev_array(np.zeros(4 * n))
+            cmm.advect_using_stream_hermite(gc, gp, mx, my, lv, c, ox, oy)
+            e = max(max(hermite_errors(host(ox), rx, n, 1.0 / 1e-3)), max(hermite_errors(host(oy), ry, n, 1.0 / 1e-3)))
+            print(f"forward r={ratio} tin={tin} order={muo}: {e:.2e}")
+            assert e <= TOL, (ratio, tin, muo, e)
+    # invertibility of this step's backward and forward maps from the identity
+    ix, iy = O.identity_map(ogc)
+    cb = O.advect_coeffs(t, dts, 0, 3, 30, 1, 1e-3)
+    cf = O.advect_coeffs(t, dts, 0, 3, 30, 1, 1e-3, direction=1)
+    bx, by = O.map_advect(ogc, ogp, ix, iy, psi, cb)
+    fx, fy = O.map_advect(ogc, ogp, ix, iy, psi, cf)
+    ref = O.lib().cmmo_invert_error(ogc, ogc, ogc, O.ptr(bx), O.ptr(by), O.ptr(fx), O.ptr(fy))
+    got = cmm.invertibility_check(gc, gc, dev_array(bx), dev_array(by), gc, dev_array(fx), dev_array(fy))
+    print("invertibility", got, ref)
+    # a distance |x - X_b(X_f(x))| ~ 1e-6 between positions of size ~1: rounding is absolute, ~1e-15
+    assert ref > 0 and abs(got - ref) <= 1e-10 * ref + 1e-13
+
+
 def test_incompressibility(dev, base):
     import cmm
     gc, gf, gp = grids()
