@@ -56,26 +56,29 @@ class EulerParams(C.Structure):
                 ("molly_stencil", C.c_int), ("skip_remapping", C.c_int),
                 ("ic", C.c_int), ("ic_params", C.c_double * 10),
                 ("discrete", C.c_int), ("g_disc", Grid), ("disc_real", C.POINTER(C.c_double)),
-                ("seed", C.c_uint64), ("stack_capacity", C.c_int), ("sim", C.c_int)]
+                ("seed", C.c_uint64), ("stack_capacity", C.c_int), ("sim", C.c_int), ("forward_map", C.c_int)]
 
 
 class StepInfo(C.Structure):
     _fields_ = [("t", C.c_double), ("incomp", C.c_double), ("remapped", C.c_int), ("stack_size", C.c_int),
-                ("saturated", C.c_int)]
+                ("saturated", C.c_int), ("incomp_forward", C.c_double), ("invertibility", C.c_double)]
 
 
 PROF_NAMES = ["advect", "incomp", "psi_row_r2c", "psi_col", "psi_row_c2r", "particles", "remap", "psi_combine"]
 PROF_SLOTS = len(PROF_NAMES)
-FIELD = {"chix": 0, "chiy": 1, "psi": 2, "w_fine": 3, "w_coarse": 4, "disc": 5}
+FIELD = {"chix": 0, "chiy": 1, "psi": 2, "w_fine": 3, "w_coarse": 4, "disc": 5, "chix_f": 6, "chiy_f": 7}
 
 # every symbol include/cmm.h declares
 SYMBOLS = ["cmm_version", "cmm_last_error", "cmm_device_info", "cmm_lagrange_coeff", "cmm_advect_coeffs_fill",
            "cmm_map_init_identity", "cmm_advect_workspace_bytes", "cmm_map_advect", "cmm_incomp_error", "cmm_invert_error", "cmm_compose_positions",
+           "cmm_forward_positions", "cmm_forward_points",
            "cmm_sample_init", "cmm_translate_workspace_bytes", "cmm_translate_ic", "cmm_hermite_workspace_bytes",
            "cmm_hermite_from_real", "cmm_hermite_from_spectrum", "cmm_psi_workspace_bytes", "cmm_psi_from_vort",
            "cmm_spectral_workspace_bytes", "cmm_spectral_from_real", "cmm_sample_vort", "cmm_vp_psi_workspace_bytes", "cmm_vp_psi_from_f", "cmm_particles_advect", "cmm_euler_create", "cmm_euler_destroy", "cmm_euler_stream",
            "cmm_euler_add_particles", "cmm_euler_init", "cmm_euler_step", "cmm_euler_get_field",
-           "cmm_euler_get_particles", "cmm_euler_diagnostics", "cmm_euler_vp_diagnostics", "cmm_euler_sample", "cmm_euler_zoom", "cmm_euler_restart", "cmm_euler_get_stack_map", "cmm_euler_state", "cmm_euler_profile", "cmm_euler_profile_read"]
+           "cmm_euler_get_particles", "cmm_euler_diagnostics", "cmm_euler_vp_diagnostics", "cmm_euler_sample", "cmm_euler_zoom", "cmm_euler_restart", "cmm_euler_get_stack_map",
+           "cmm_euler_restart_forward", "cmm_euler_get_forward_stack_map", "cmm_euler_request_remap",
+           "cmm_euler_forward_positions", "cmm_euler_forward_points", "cmm_euler_state", "cmm_euler_profile", "cmm_euler_profile_read"]
 
 _lib = None
 DP = C.POINTER(C.c_double)
@@ -102,6 +105,8 @@ def lib():
         L.cmm_incomp_error.argtypes = [Grid, Grid, VP, VP, DP, VP]
         L.cmm_invert_error.argtypes = [Grid, Grid, Grid, VP, VP, VP, VP, DP, VP]
         L.cmm_compose_positions.argtypes = [Grid, Grid, VP, VP, VP, C.c_int, VP, VP]
+        L.cmm_forward_positions.argtypes = [Grid, Grid, VP, VP, VP, C.c_int, VP, VP]
+        L.cmm_forward_points.argtypes = [Grid, Grid, VP, VP, VP, C.c_int, C.c_int, C.c_int, C.c_int, VP, VP, VP]
         L.cmm_sample_init.argtypes = [Grid, Grid, VP, VP, VP, C.c_int, C.c_int, DP, C.c_int, Grid, VP, VP, VP]
         L.cmm_translate_workspace_bytes.restype = C.c_size_t
         L.cmm_translate_workspace_bytes.argtypes = [Grid]
@@ -138,6 +143,11 @@ def lib():
         L.cmm_euler_zoom.argtypes = [VP, Grid, C.POINTER(C.c_int), C.c_int, C.c_int, DP]
         L.cmm_euler_restart.argtypes = [VP, C.c_double, C.c_int, DP, DP, DP, DP]
         L.cmm_euler_get_stack_map.argtypes = [VP, C.c_int, DP, DP]
+        L.cmm_euler_restart_forward.argtypes = [VP, C.c_int, DP, DP, DP]
+        L.cmm_euler_get_forward_stack_map.argtypes = [VP, C.c_int, DP, DP]
+        L.cmm_euler_request_remap.argtypes = [VP]
+        L.cmm_euler_forward_positions.argtypes = [VP, Grid, DP]
+        L.cmm_euler_forward_points.argtypes = [VP, Grid, C.c_int, DP, C.c_int, C.c_int, DP]
         L.cmm_euler_state.argtypes = [VP, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_int)]
         L.cmm_euler_profile.argtypes = [VP, C.c_int]
         L.cmm_euler_profile_read.argtypes = [VP, DP, C.POINTER(C.c_int)]
@@ -221,6 +231,20 @@ def apply_map_stack(g_target, g_map, chix, chiy, stack, n_stack, xy):
            "cmm_compose_positions")
 
 
+def apply_map_stack_forward(g_target, g_map, chix, chiy, stack, n_stack, xy):
+    """forward positions: the oldest stacked forward map first, the current one last"""
+    _check(lib().cmm_forward_positions(g_map, g_target, _p(chix), _p(chiy), _p(stack), n_stack, _p(xy), _stream()),
+           "cmm_forward_positions")
+
+
+def apply_map_stack_points_forward(g_bounds, g_map, chix, chiy, stack, n_stack, first_map, apply_current, pos_in,
+                                   pos_out):
+    """forwarded particles through stacked forward maps first_map.. and then the current one"""
+    _check(lib().cmm_forward_points(g_map, g_bounds, _p(chix), _p(chiy), _p(stack), n_stack, first_map,
+                                    int(apply_current), pos_in.numel() // 2, _p(pos_in), _p(pos_out), _stream()),
+           "cmm_forward_points")
+
+
 def sample_init(g_map, g_target, chix, chiy, stack, n_stack, ic, out, ic_params=None, disc=None, g_disc=None):
     prm = (C.c_double * 10)(*([0.0] * 10 if ic_params is None else list(ic_params) + [0.0] * (10 - len(ic_params))))
     gd = g_disc if g_disc is not None else g_target
@@ -288,7 +312,7 @@ class Euler:
                  l_order=None, lagrange_init_higher_order=True, map_epsilon=1e-3, incomp_threshold=1e-4,
                  freq_cut_psi=128.0, molly_stencil=0, skip_remapping=False, initial_condition="4_nodes",
                  ic_params=None, discrete_real=None, g_disc=None, seed=0, stack_capacity=64,
-                 simulation_type="cmm_euler_2d"):
+                 simulation_type="cmm_euler_2d", forward_map=False):
         import numpy as np
         tin = TIME_INTEGRATION[time_integration]
         if l_order is None:
@@ -316,6 +340,7 @@ class Euler:
         p.seed = seed
         p.stack_capacity = stack_capacity
         p.sim = SIM[simulation_type]
+        p.forward_map = int(forward_map)
         self.params = p
         self.l_order = l_order
         h = C.c_void_p()
@@ -357,7 +382,8 @@ class Euler:
     def field(self, name):
         import numpy as np
         p = self.params
-        size = {"chix": 4 * p.g_coarse.n, "chiy": 4 * p.g_coarse.n, "psi": 4 * p.g_psi.n * self.l_order,
+        size = {"chix": 4 * p.g_coarse.n, "chiy": 4 * p.g_coarse.n, "chix_f": 4 * p.g_coarse.n,
+                "chiy_f": 4 * p.g_coarse.n, "psi": 4 * p.g_psi.n * self.l_order,
                 "w_fine": 4 * p.g_fine.n, "w_coarse": p.g_coarse.n, "disc": 4 * p.g_disc.n}[name]
         out = np.zeros(size)
         _check(lib().cmm_euler_get_field(self.h, FIELD[name], out.ctypes.data_as(DP)), "cmm_euler_get_field")
@@ -420,6 +446,46 @@ class Euler:
         _check(lib().cmm_euler_get_stack_map(self.h, m, x.ctypes.data_as(DP), y.ctypes.data_as(DP)),
                "cmm_euler_get_stack_map")
         return x, y
+
+    def forward_stack_map(self, m):
+        """Forward map m of the forward stack (0 = oldest) as host arrays (chix, chiy)."""
+        import numpy as np
+        n = 4 * self.params.g_coarse.n
+        x, y = np.zeros(n), np.zeros(n)
+        _check(lib().cmm_euler_get_forward_stack_map(self.h, m, x.ctypes.data_as(DP), y.ctypes.data_as(DP)),
+               "cmm_euler_get_forward_stack_map")
+        return x, y
+
+    def restart_forward(self, stack_maps, chix, chiy):
+        """The forward part of a restart (cmm_euler_restart_forward), after restart()."""
+        import numpy as np
+        st = np.ascontiguousarray(np.concatenate([np.concatenate([x, y]) for x, y in stack_maps])
+                                  if stack_maps else np.zeros(1))
+        cx, cy = np.ascontiguousarray(chix, dtype=np.float64), np.ascontiguousarray(chiy, dtype=np.float64)
+        _check(lib().cmm_euler_restart_forward(self.h, len(stack_maps), st.ctypes.data_as(DP), cx.ctypes.data_as(DP),
+                                               cy.ctypes.data_as(DP)), "cmm_euler_restart_forward")
+
+    def request_remap(self):
+        """The next step remaps whatever E_inc is (forwarded-particle start)."""
+        _check(lib().cmm_euler_request_remap(self.h), "cmm_euler_request_remap")
+
+    def forward_positions(self, g_target):
+        """The forward map through the forward stack on g_target: (x, y) host arrays."""
+        import numpy as np
+        n = g_target.nx * g_target.ny
+        out = np.zeros(2 * n)
+        _check(lib().cmm_euler_forward_positions(self.h, g_target, out.ctypes.data_as(DP)),
+               "cmm_euler_forward_positions")
+        return out[:n], out[n:]
+
+    def forward_points(self, g_bounds, pos, first_map=0, apply_current=True):
+        """Forwarded particles (x, y interleaved) through forward maps first_map.. and the current one."""
+        import numpy as np
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        out = np.zeros_like(pos)
+        _check(lib().cmm_euler_forward_points(self.h, g_bounds, pos.size // 2, pos.ctypes.data_as(DP), first_map,
+                                              int(apply_current), out.ctypes.data_as(DP)), "cmm_euler_forward_points")
+        return out
 
     def vp_diagnostics(self):
         """Vlasov-Poisson: Etot, Ekin, Epot, Mass, Momentum, Emode1..4"""

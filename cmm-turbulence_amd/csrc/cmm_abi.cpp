@@ -147,6 +147,24 @@ int cmm_compose_positions(cmm_grid gm, cmm_grid gt, const double *cx, const doub
     return op_compose_positions(gm, gt, cx, cy, stack, ns, xy, S(stream));
 }
 
+int cmm_forward_positions(cmm_grid gm, cmm_grid gt, const double *cx, const double *cy, const double *stack,
+                          int ns, double *xy, void *stream) {
+    if (!grid_ok(gm) || !grid_ok(gt) || !cx || !cy || !xy || ns < 0 || (ns > 0 && !stack)) {
+        set_error("cmm_forward_positions: bad argument");
+        return CMM_ERR_ARG;
+    }
+    return op_forward_positions(gm, gt, cx, cy, stack, ns, xy, S(stream));
+}
+
+int cmm_forward_points(cmm_grid gm, cmm_grid gb, const double *cx, const double *cy, const double *stack, int ns,
+                       int first_map, int apply_current, int n, const double *in, double *out, void *stream) {
+    if (!grid_ok(gm) || !cx || !cy || ns < 0 || (ns > 0 && !stack) || n < 0 || (n > 0 && (!in || !out || in == out))) {
+        set_error("cmm_forward_points: bad argument");
+        return CMM_ERR_ARG;
+    }
+    return op_forward_points(gm, gb, stack, first_map, ns, cx, cy, apply_current, in, out, n, S(stream));
+}
+
 int cmm_sample_init(cmm_grid gm, cmm_grid gt, const double *cx, const double *cy, const double *stack, int ns, int ic,
                     const double *prm, int discrete, cmm_grid gd, const double *disc, double *out, void *stream) {
     if (!grid_ok(gm) || !grid_ok(gt) || !cx || !cy || !out || ns < 0 || (ns > 0 && !stack) ||
@@ -259,6 +277,9 @@ struct cmm_euler {
     int m8_cur = 0;
     bool m8_valid = false;
     int nstack = 0;
+    // forward map (forward_map only): current, next, and its stack, pushed with the backward one
+    double *fx = nullptr, *fy = nullptr, *nfx = nullptr, *nfy = nullptr, *fstack = nullptr;
+    bool force_remap = false;  // set by cmm_euler_request_remap for the next step
     void *work = nullptr;
     size_t work_bytes = 0;
     unsigned long long *slot = nullptr;
@@ -342,6 +363,21 @@ int engine_advect(cmm_euler *e, const double *t, const double *dt, int loop_ctr,
     std::swap(e->cy, e->ncy);
     e->m8_valid = comb;
     if (comb) e->m8_cur = 1 - e->m8_cur;
+    return CMM_OK;
+}
+
+// the forward map with the same psi levels and times, Cuda-CMM/src/simulation/cmm-euler2d.cu:780-785
+int engine_advect_forward(cmm_euler *e, const double *t, const double *dt, int loop_ctr, int L, int tin) {
+    cmm_advect_coeffs c;
+    if (int rc = cmm_advect_coeffs_fill(t, dt, loop_ctr, L, tin, e->p.map_update_order_num, e->p.map_epsilon, 1, &c))
+        return rc;
+    PsiLevels P;
+    for (int l = 0; l < 4; ++l) P.p[l] = e->lvl[l < L ? l : 0];
+    if (int rc = op_map_advect(e->p.g_coarse, e->p.g_psi, e->fx, e->fy, P, c, e->nfx, e->nfy, nullptr, 0, e->st, nullptr,
+                               nullptr, nullptr))
+        return rc;
+    std::swap(e->fx, e->nfx);
+    std::swap(e->fy, e->nfy);
     return CMM_OK;
 }
 
@@ -467,6 +503,10 @@ int cmm_euler_create(const cmm_euler_params *p, cmm_euler **out) {
         set_error("cmm_euler_create: bad time-stepping parameters");
         return CMM_ERR_ARG;
     }
+    if (p->forward_map != 0 && p->forward_map != 1) {
+        set_error("cmm_euler_create: forward_map must be 0 or 1");
+        return CMM_ERR_ARG;
+    }
     if (p->discrete && (!grid_ok(p->g_disc) || !pow2(p->g_disc.nx) || p->g_disc.nx != p->g_disc.ny)) {
         set_error("cmm_euler_create: discrete grid must be a square power of two");
         return CMM_ERR_ARG;
@@ -490,6 +530,13 @@ int cmm_euler_create(const cmm_euler_params *p, cmm_euler **out) {
     if (!rc && p->l_order >= 2) rc = dalloc(&e->m8[1], 8 * e->nc);
     if (!rc && p->sim == CMM_SIM_EULER_2D) rc = dalloc(&e->W4, 4 * e->nf);
     if (!rc && p->stack_capacity > 0) rc = dalloc(&e->stack, (size_t)8 * e->nc * p->stack_capacity);
+    if (p->forward_map) {
+        if (!rc) rc = dalloc(&e->fx, 4 * e->nc);
+        if (!rc) rc = dalloc(&e->fy, 4 * e->nc);
+        if (!rc) rc = dalloc(&e->nfx, 4 * e->nc);
+        if (!rc) rc = dalloc(&e->nfy, 4 * e->nc);
+        if (!rc && p->stack_capacity > 0) rc = dalloc(&e->fstack, (size_t)8 * e->nc * p->stack_capacity);
+    }
     if (!rc) {
         size_t w = std::max(op_psi_workspace(p->g_vort, p->g_psi, p->freq_cut_psi), op_hermite_workspace(p->g_fine));
         if (p->discrete) w = std::max(w, op_hermite_workspace(p->g_disc));
@@ -525,7 +572,7 @@ int cmm_euler_destroy(cmm_euler *e) {
     if (!e) return CMM_OK;
     if (e->st) (void)hipStreamSynchronize(e->st);
     double *bufs[] = {e->cx, e->cy, e->ncx, e->ncy, e->W, e->W4, e->m8[0], e->m8[1], e->sbuf, e->disc, e->stack, e->tmp, e->tmp4,
-                      e->psi_buf[0], e->psi_buf[1], e->psi_buf[2], e->psi_buf[3]};
+                      e->psi_buf[0], e->psi_buf[1], e->psi_buf[2], e->psi_buf[3], e->fx, e->fy, e->nfx, e->nfy, e->fstack};
     for (double *b : bufs)
         if (b) (void)hipFree(b);
     for (double *b : e->part)
@@ -573,8 +620,11 @@ int cmm_euler_init(cmm_euler *e, double *warm, int *nwarm) {
     int nw = 0;
     // Cuda-CMM/src/simulation/cmm-euler2d.cu:495-512
     if (int rc = op_map_identity(p.g_coarse, e->cx, e->cy, e->st)) return rc;
+    if (p.forward_map)
+        if (int rc = op_map_identity(p.g_coarse, e->fx, e->fy, e->st)) return rc;
     e->m8_valid = false;
     e->nstack = 0;
+    e->force_remap = false;
     if (int rc = engine_translate(e)) return rc;
     if (int rc = engine_psi(e, e->lvl[0])) return rc;
     // Lagrange warm-up, Cuda-CMM/src/simulation/cmm-euler2d.cu:520-619
@@ -642,7 +692,9 @@ int cmm_euler_restart(cmm_euler *e, double t_restart, int n_maps, const double *
     int rc = n_maps > 0 ? put(e->stack, stack_host, (size_t)8 * e->nc * n_maps) : CMM_OK;
     e->nstack = n_maps;
     if (!rc) rc = op_map_identity(p.g_coarse, e->cx, e->cy, e->st);
+    if (!rc && p.forward_map) rc = op_map_identity(p.g_coarse, e->fx, e->fy, e->st);
     e->m8_valid = false;
+    e->force_remap = false;
     if (!rc) rc = engine_translate(e);
     if (!rc) rc = put(e->cx, chix_host, (size_t)4 * e->nc);
     if (!rc) rc = put(e->cy, chiy_host, (size_t)4 * e->nc);
@@ -673,6 +725,42 @@ int cmm_euler_get_stack_map(cmm_euler *e, int m, double *chix_host, double *chiy
     return hip_check(hipStreamSynchronize(e->st), "hipStreamSynchronize");
 }
 
+int cmm_euler_restart_forward(cmm_euler *e, int n_maps, const double *stack_host, const double *chix_host,
+                              const double *chiy_host) {
+    if (!e || !e->fx || !chix_host || !chiy_host || n_maps != e->nstack || (n_maps > 0 && !stack_host)) {
+        set_error("cmm_euler_restart_forward: bad argument (forward map engine, restarted, with as many forward maps as backward maps)");
+        return CMM_ERR_ARG;
+    }
+    auto put = [&](double *dst, const double *src, size_t cnt) {
+        return hip_check(hipMemcpyAsync(dst, src, sizeof(double) * cnt, hipMemcpyHostToDevice, e->st), "put");
+    };
+    int rc = n_maps > 0 ? put(e->fstack, stack_host, (size_t)8 * e->nc * n_maps) : CMM_OK;
+    if (!rc) rc = put(e->fx, chix_host, (size_t)4 * e->nc);
+    if (!rc) rc = put(e->fy, chiy_host, (size_t)4 * e->nc);
+    if (rc) return rc;
+    return hip_check(hipStreamSynchronize(e->st), "hipStreamSynchronize");
+}
+
+int cmm_euler_get_forward_stack_map(cmm_euler *e, int m, double *chix_host, double *chiy_host) {
+    if (!e || !e->fstack || m < 0 || m >= e->nstack || !chix_host || !chiy_host) {
+        set_error("cmm_euler_get_forward_stack_map: bad argument");
+        return CMM_ERR_ARG;
+    }
+    const double *slot = e->fstack + (size_t)8 * e->nc * m;
+    int rc = hip_check(hipMemcpyAsync(chix_host, slot, sizeof(double) * 4 * e->nc, hipMemcpyDeviceToHost, e->st), "get");
+    if (!rc)
+        rc = hip_check(hipMemcpyAsync(chiy_host, slot + 4 * e->nc, sizeof(double) * 4 * e->nc, hipMemcpyDeviceToHost, e->st),
+                       "get");
+    if (rc) return rc;
+    return hip_check(hipStreamSynchronize(e->st), "hipStreamSynchronize");
+}
+
+int cmm_euler_request_remap(cmm_euler *e) {
+    if (!e) { set_error("cmm_euler_request_remap: null engine"); return CMM_ERR_ARG; }
+    e->force_remap = true;
+    return CMM_OK;
+}
+
 int cmm_euler_step(cmm_euler *e, double dt_now, cmm_step_info *info) {
     if (!e || !e->initialised) { set_error("cmm_euler_step: engine not initialised"); return CMM_ERR_ARG; }
     if (!(dt_now > 0)) { set_error("cmm_euler_step: dt must be positive"); return CMM_ERR_ARG; }
@@ -692,11 +780,20 @@ int cmm_euler_step(cmm_euler *e, double dt_now, cmm_step_info *info) {
     // advection, Cuda-CMM/src/simulation/cmm-euler2d.cu:774-778
     prof_mark(e, 0);
     if (int rc = engine_advect(e, e->tv.data(), e->dtv.data(), e->loop, L, p.time_integration_num)) return rc;
+    if (p.forward_map)
+        if (int rc = engine_advect_forward(e, e->tv.data(), e->dtv.data(), e->loop, L, p.time_integration_num)) return rc;
     prof_mark(e, 1);
     // incompressibility on the fine grid, :796: the check and the copy of E_inc to pinned host
     // memory are queued, and the host waits for that copy only after the psi solve is queued too
     if (int rc = op_incomp_launch(p.g_coarse, p.g_fine, e->cx, e->cy, e->slot, e->st)) return rc;
-    if (int rc = hip_check(hipMemcpyAsync(e->h_slot, e->slot, sizeof(unsigned long long), hipMemcpyDeviceToHost, e->st),
+    // forward map: invertibility and E_inc on the coarse grid, :798-803
+    if (p.forward_map) {
+        if (int rc = op_invert_launch(p.g_coarse, p.g_coarse, p.g_coarse, e->cx, e->cy, e->fx, e->fy, e->slot + 2, e->st))
+            return rc;
+        if (int rc = op_incomp_launch(p.g_coarse, p.g_coarse, e->fx, e->fy, e->slot + 1, e->st)) return rc;
+    }
+    if (int rc = hip_check(hipMemcpyAsync(e->h_slot, e->slot, sizeof(unsigned long long) * (p.forward_map ? 3 : 1),
+                                          hipMemcpyDeviceToHost, e->st),
                            "hipMemcpyAsync"))
         return rc;
     if (int rc = hip_check(hipEventRecord(e->ev_inc, e->st), "hipEventRecord")) return rc;
@@ -712,11 +809,17 @@ int cmm_euler_step(cmm_euler *e, double dt_now, cmm_step_info *info) {
         if (int rc = hip_check(hipStreamWaitEvent(e->st, e->ev_part, 0), "hipStreamWaitEvent")) return rc;
     if (int rc = engine_psi(e, e->lvl[0], e->prof ? &e->ev[5] : nullptr)) return rc;
     if (int rc = hip_check(hipEventSynchronize(e->ev_inc), "hipEventSynchronize")) return rc;
-    double einc;
+    double einc, einc_f = 0, einv = 0;
     std::memcpy(&einc, e->h_slot, sizeof(double));
+    if (p.forward_map) {
+        std::memcpy(&einc_f, e->h_slot + 1, sizeof(double));
+        std::memcpy(&einv, e->h_slot + 2, sizeof(double));
+    }
     int remapped = 0, saturated = 0;
+    const bool forced = e->force_remap;
+    e->force_remap = false;
     // remap, :814-853 (translate before push; the map is reset afterwards)
-    if (einc > p.incomp_threshold && !p.skip_remapping) {
+    if ((einc > p.incomp_threshold && !p.skip_remapping) || forced) {
         if (e->nstack >= p.stack_capacity) {
             saturated = 1;
         } else {
@@ -725,6 +828,12 @@ int cmm_euler_step(cmm_euler *e, double dt_now, cmm_step_info *info) {
             double *slotx = e->stack + (size_t)8 * e->nc * e->nstack;
             if (int rc = hip_check(hipMemcpyAsync(slotx, e->cx, sizeof(double) * 4 * e->nc, hipMemcpyDeviceToDevice, e->st), "push map")) return rc;
             if (int rc = hip_check(hipMemcpyAsync(slotx + 4 * e->nc, e->cy, sizeof(double) * 4 * e->nc, hipMemcpyDeviceToDevice, e->st), "push map")) return rc;
+            if (p.forward_map) {  // :847-851
+                double *fslot = e->fstack + (size_t)8 * e->nc * e->nstack;
+                if (int rc = hip_check(hipMemcpyAsync(fslot, e->fx, sizeof(double) * 4 * e->nc, hipMemcpyDeviceToDevice, e->st), "push forward map")) return rc;
+                if (int rc = hip_check(hipMemcpyAsync(fslot + 4 * e->nc, e->fy, sizeof(double) * 4 * e->nc, hipMemcpyDeviceToDevice, e->st), "push forward map")) return rc;
+                if (int rc = op_map_identity(p.g_coarse, e->fx, e->fy, e->st)) return rc;
+            }
             e->nstack++;
             if (int rc = op_map_identity(p.g_coarse, e->cx, e->cy, e->st)) return rc;
             e->m8_valid = false;
@@ -771,6 +880,8 @@ int cmm_euler_step(cmm_euler *e, double dt_now, cmm_step_info *info) {
         info->remapped = remapped;
         info->stack_size = e->nstack;
         info->saturated = saturated;
+        info->incomp_forward = einc_f;
+        info->invertibility = einv;
     }
     return CMM_OK;
 }
@@ -792,6 +903,12 @@ int cmm_euler_get_field(cmm_euler *e, int field, double *out) {
         if (!rc) rc = hip_check(hipMemcpyAsync(out, e->tmp, sizeof(double) * e->nc, hipMemcpyDeviceToHost, e->st), "get");
         break;
     }
+    case CMM_FIELD_CHIX_F:
+    case CMM_FIELD_CHIY_F:
+        if (!e->fx) { set_error("cmm_euler_get_field: the engine has no forward map"); return CMM_ERR_ARG; }
+        rc = hip_check(hipMemcpyAsync(out, field == CMM_FIELD_CHIX_F ? e->fx : e->fy, sizeof(double) * 4 * e->nc,
+                                      hipMemcpyDeviceToHost, e->st), "get");
+        break;
     case CMM_FIELD_DISC:
         if (!e->disc) { set_error("no discrete initial condition"); return CMM_ERR_ARG; }
         rc = hip_check(hipMemcpyAsync(out, e->disc, sizeof(double) * 4 * e->nd, hipMemcpyDeviceToHost, e->st), "get");
@@ -974,6 +1091,50 @@ int cmm_euler_zoom(cmm_euler *e, cmm_grid gz, const int *ics, int n_ics, int wan
         out[n + i] = hxy[2 * i + 1];
     }
     return CMM_OK;
+}
+
+int cmm_euler_forward_positions(cmm_euler *e, cmm_grid gt, double *out) {
+    if (!e || !out || !e->fx || gt.nx < 1 || gt.ny < 1 || !(gt.x1 > gt.x0) || !(gt.y1 > gt.y0)) {
+        set_error("cmm_euler_forward_positions: bad argument (needs a forward map engine)");
+        return CMM_ERR_ARG;
+    }
+    const size_t n = (size_t)gt.nx * gt.ny;
+    if (e->sbuf_n < n) {
+        if (e->sbuf) (void)hipFree(e->sbuf);
+        e->sbuf = nullptr;
+        e->sbuf_n = 0;
+        if (int rc = dalloc(&e->sbuf, 8 * n)) return rc;
+        e->sbuf_n = n;
+    }
+    std::vector<double> hxy(2 * n);
+    int rc = op_forward_positions(e->p.g_coarse, gt, e->fx, e->fy, e->fstack, e->nstack, e->sbuf, e->st);
+    if (!rc) rc = hip_check(hipMemcpyAsync(hxy.data(), e->sbuf, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, e->st), "get");
+    if (!rc) rc = hip_check(hipStreamSynchronize(e->st), "hipStreamSynchronize");
+    if (rc) return rc;
+    for (size_t i = 0; i < n; ++i) {
+        out[i] = hxy[2 * i];
+        out[n + i] = hxy[2 * i + 1];
+    }
+    return CMM_OK;
+}
+
+int cmm_euler_forward_points(cmm_euler *e, cmm_grid gb, int n, const double *pos, int first_map, int apply_current,
+                             double *out) {
+    if (!e || !e->fx || n < 0 || (n > 0 && (!pos || !out))) {
+        set_error("cmm_euler_forward_points: bad argument (needs a forward map engine)");
+        return CMM_ERR_ARG;
+    }
+    if (n == 0) return CMM_OK;
+    double *d = nullptr;
+    if (int rc = dalloc(&d, 4 * (size_t)n)) return rc;
+    int rc = hip_check(hipMemcpyAsync(d, pos, sizeof(double) * 2 * n, hipMemcpyHostToDevice, e->st), "put");
+    if (!rc)
+        rc = op_forward_points(e->p.g_coarse, gb, e->fstack, first_map, e->nstack, e->fx, e->fy, apply_current, d,
+                               d + 2 * (size_t)n, n, e->st);
+    if (!rc) rc = hip_check(hipMemcpyAsync(out, d + 2 * (size_t)n, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, e->st), "get");
+    if (!rc) rc = hip_check(hipStreamSynchronize(e->st), "hipStreamSynchronize");
+    (void)hipFree(d);
+    return rc;
 }
 
 int cmm_euler_state(cmm_euler *e, int *steps, double *t, int *stack_size) {

@@ -46,6 +46,10 @@ int op_translate_ic(const cmm_grid &gm, const cmm_grid &gf, const double *cx, co
                     double *herm, void *work, size_t work_bytes, hipStream_t st);
 int op_compose_positions(const cmm_grid &gm, const cmm_grid &gt, const double *cx, const double *cy,
                          const double *stack, int nstack, double *xy, hipStream_t st);
+int op_forward_positions(const cmm_grid &gm, const cmm_grid &gt, const double *cx, const double *cy,
+                         const double *stack, int nstack, double *xy, hipStream_t st);
+int op_forward_points(const cmm_grid &gm, const cmm_grid &gb, const double *stack, int first, int nstack,
+                      const double *cx, const double *cy, int cur, const double *in, double *out, int n, hipStream_t st);
 int op_sample_init(const cmm_grid &gm, const cmm_grid &gt, const double *cx, const double *cy, const double *stack,
                    int nstack, int ic, const double *prm, int discrete, const cmm_grid &gd, const double *disc,
                    double *out, hipStream_t st);

@@ -782,6 +782,21 @@ struct SampleCompose {
             y = py - floor((py - gm.y0) / gm.ly) * gm.ly;
         }
     }
+    // forward order, Cuda-CMM/src/simulation/cmm-simulation-host.cu:128-148: the oldest stacked
+    // forward map first, the current forward map last
+    __device__ void position_forward(int ix, int iy, double &x, double &y) const {
+        x = gt.x0 + ix * gt.hx;
+        y = gt.y0 + iy * gt.hy;
+        const long mstride = 8l * gm.n;
+        for (int m = 0; m <= nstack; ++m) {
+            const double *sx = m < nstack ? stack + mstride * m : cx;
+            const double *sy = m < nstack ? sx + 4l * gm.n : cy;
+            double px, py;
+            diffeo(gm, sx, sy, x, y, px, py);
+            x = px - floor((px - gm.x0) / gm.lx) * gm.lx;
+            y = py - floor((py - gm.y0) / gm.ly) * gm.ly;
+        }
+    }
     __device__ double operator()(int ix, int iy) const {
         double x, y;
         position(ix, iy, x, y);
@@ -1040,6 +1055,48 @@ __global__ void __launch_bounds__(256) k_compose_positions(SampleCompose sc, dou
     const long i = (long)iy * sc.gt.nx + ix;
     xy[2 * i] = x;
     xy[2 * i + 1] = y;
+}
+
+__global__ void __launch_bounds__(256) k_forward_positions(SampleCompose sc, double *__restrict__ xy) {
+    const int ix = blockIdx.x * blockDim.x + threadIdx.x;
+    const int iy = blockIdx.y * blockDim.y + threadIdx.y;
+    if (ix >= sc.gt.nx || iy >= sc.gt.ny) return;
+    double x, y;
+    sc.position_forward(ix, iy, x, y);
+    const long i = (long)iy * sc.gt.nx + ix;
+    xy[2 * i] = x;
+    xy[2 * i + 1] = y;
+}
+
+// Forwarded particles: k_h_sample_points_map (Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:97-118)
+// once per map, stacked maps first..nstack-1 then (cur) the current map.  A point strictly inside
+// the bounds b goes through the map and is wrapped by x - floor(x / Lx) Lx; others become NaN.
+struct PointBounds {
+    double x0, x1, y0, y1;
+};
+
+__global__ void __launch_bounds__(256) k_forward_points(DGrid gm, PointBounds b, const double *__restrict__ stack,
+                                                        int first, int nstack, const double *__restrict__ cx,
+                                                        const double *__restrict__ cy, int cur,
+                                                        const double *__restrict__ in, double *__restrict__ out, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double x = in[2 * i], y = in[2 * i + 1];
+    const long mstride = 8l * gm.n;
+    for (int m = first; m < nstack + cur; ++m) {
+        const double *sx = m < nstack ? stack + mstride * m : cx;
+        const double *sy = m < nstack ? sx + 4l * gm.n : cy;
+        if (x > b.x0 && x < b.x1 && y > b.y0 && y < b.y1) {
+            double px, py;
+            diffeo(gm, sx, sy, x, y, px, py);
+            x = px - floor(px / gm.lx) * gm.lx;
+            y = py - floor(py / gm.ly) * gm.ly;
+        } else {
+            x = y = __builtin_nan("");
+        }
+    }
+    out[2 * i] = x;
+    out[2 * i + 1] = y;
 }
 
 // Fluid particles, Cuda-CMM/src/numerical/cmm-particles.cu:119-276 (level 0 = t_{n+1}, 1 = t_n)
@@ -1758,6 +1815,34 @@ int op_compose_positions(const cmm_grid &gm, const cmm_grid &gt, const double *c
     dim3 b(64, 4);
     hipLaunchKernelGGL(k_compose_positions, grid2d(gt.nx, gt.ny, b), b, 0, st, s, xy);
     return launch_check("k_compose_positions");
+}
+
+int op_forward_positions(const cmm_grid &gm, const cmm_grid &gt, const double *cx, const double *cy,
+                         const double *stack, int nstack, double *xy, hipStream_t st) {
+    SampleCompose s;
+    s.gm = make_dgrid(gm);
+    s.gt = make_dgrid(gt);
+    s.gd = s.gt;
+    s.cx = cx;
+    s.cy = cy;
+    s.stack = stack;
+    s.nstack = nstack;
+    s.ic = make_ic(0, nullptr);
+    s.discrete = 0;
+    s.disc = nullptr;
+    dim3 b(64, 4);
+    hipLaunchKernelGGL(k_forward_positions, grid2d(gt.nx, gt.ny, b), b, 0, st, s, xy);
+    return launch_check("k_forward_positions");
+}
+
+int op_forward_points(const cmm_grid &gm, const cmm_grid &gb, const double *stack, int first, int nstack,
+                      const double *cx, const double *cy, int cur, const double *in, double *out, int n, hipStream_t st) {
+    if (n <= 0) return CMM_OK;
+    PointBounds b{gb.x0, gb.x1, gb.y0, gb.y1};
+    const int f = std::min(std::max(first, 0), nstack);  // maps f..nstack-1, then the current one
+    hipLaunchKernelGGL(k_forward_points, dim3((n + 255) / 256), dim3(256), 0, st, make_dgrid(gm), b, stack, f, nstack,
+                       cx, cy, cur ? 1 : 0, in, out, n);
+    return launch_check("k_forward_points");
 }
 
 int op_sample_init(const cmm_grid &gm, const cmm_grid &gt, const double *cx, const double *cy, const double *stack,

@@ -202,6 +202,19 @@ std::vector<double> init_particles(const ParticlesAdvected &p, const double *b) 
     return pos;
 }
 
+// forwarded particles start from the same distributions (init_particles with type 1)
+std::vector<double> init_particles(const ParticlesForwarded &p, const double *b) {
+    ParticlesAdvected a;
+    a.num = p.num;
+    a.seed = p.seed;
+    a.init_num = p.init_num;
+    a.p1 = p.p1;
+    a.p2 = p.p2;
+    a.p3 = p.p3;
+    a.p4 = p.p4;
+    return init_particles(a, b);
+}
+
 bool save_due(bool is_instant, double ts, double te, double tstep, double t_now, double dt_now, double dt) {
     // Cuda-CMM/src/ui/cmm-io.cu:300-312
     if (is_instant) return t_now - ts + dt * 1e-5 < dt_now && t_now - ts + dt * 1e-5 >= 0;
@@ -216,7 +229,29 @@ struct Run {
     cmm_grid gc, gf, gp;
     int L;
     std::vector<int> part_set;  // engine set per advected particle entry (-1 if not advected here)
+    // forwarded particles: start positions and the stack index their maps start from (init_map of
+    // Cuda-CMM/src/simulation/cmm-euler2d.cu:804-811; 0 until they start)
+    std::vector<std::vector<double>> fpos;
+    std::vector<int> finit_map;
     Run(Settings &st) : s(st), log(st) {}
+
+    // apply_map_stack_points with direction 1 (Cuda-CMM/src/simulation/cmm-simulation-host.cu:213-291)
+    // for forwarded set i: with an empty stack the current forward map; else the stacked maps from 0
+    // (a set started at t = 0) or from max(1, init_map), and the current map once the set has started
+    std::vector<double> forwarded(size_t i, const cmm_grid &gb) {
+        const auto &pf = s.particles_forwarded[i];
+        int steps = 0, ns = 0;
+        double t = 0;
+        cmm_euler_state(e, &steps, &t, &ns);
+        int first = 0, cur = 1;
+        if (ns > 0 && pf.init_time != 0) {
+            first = std::max(1, finit_map[i]);
+            cur = finit_map[i] != 0;
+        }
+        std::vector<double> out(fpos[i].size());
+        check(cmm_euler_forward_points(e, gb, pf.num, fpos[i].data(), first, cur, out.data()), "cmm_euler_forward_points");
+        return out;
+    }
 
     void check(int rc, const char *what) {
         if (rc != CMM_OK) {
@@ -285,6 +320,16 @@ struct Run {
             if (has("Map_H") || has("Chi_H") || all) {
                 write_var(sub, "Map_ChiX_H_coarse", cx.data(), 4 * nc, 0);
                 write_var(sub, "Map_ChiY_H_coarse", cy.data(), 4 * nc, 0);
+            }
+        }
+        // forward map on the coarse grid, Cuda-CMM/src/ui/cmm-io.cu:379-389
+        if (s.forward_map && (has("Map_f") || has("Chi_f") || all)) {
+            auto fx = field(CMM_FIELD_CHIX_F, 4 * nc), fy = field(CMM_FIELD_CHIY_F, 4 * nc);
+            write_var(sub, "Map_ChiX_f_coarse", fx.data(), nc, ac);
+            write_var(sub, "Map_ChiY_f_coarse", fy.data(), nc, ac);
+            if (has("Map_f_H") || has("Chi_f_H") || all) {
+                write_var(sub, "Map_ChiX_f_H_coarse", fx.data(), 4 * nc, 0);
+                write_var(sub, "Map_ChiY_f_H_coarse", fy.data(), 4 * nc, 0);
             }
         }
         // particle positions, Cuda-CMM/src/ui/cmm-io.cu:396-434
@@ -379,6 +424,23 @@ struct Run {
             std::string ts = std::abs(t_now - kTMax) < 1 ? "final" : fmt3(t_now);
             const std::string sub = "/Time_data/Time_" + ts;
             mkdir_p(s.workspace + "data/" + s.file_name + sub);
+            // the forward map through the forward stack and the forwarded particles, :740-783
+            if (s.forward_map && (has("Map_f") || has("Chi_f") || all || !s.particles_forwarded.empty())) {
+                if (has("Map_f") || has("Chi_f") || all) {
+                    std::vector<double> fxy(2 * n);
+                    check(cmm_euler_forward_positions(e, gs, fxy.data()), "cmm_euler_forward_positions");
+                    write_var(sub, "Map_ChiX_f_" + tag, fxy.data(), n, area);
+                    write_var(sub, "Map_ChiY_f_" + tag, fxy.data() + n, n, area);
+                }
+                for (size_t k = 0; k < s.particles_forwarded.size(); ++k) {
+                    const std::string ptag = zpad((int)k + 1, 2);
+                    if (!(has(("PartF_" + ptag).c_str()) || all)) continue;
+                    auto pos = forwarded(k, gs);
+                    const std::string psub = "/Particle_data/Time_" + ts;
+                    mkdir_p(s.workspace + "data/" + s.file_name + psub);
+                    write_file(s, psub + "/Particles_forwarded_pos_P" + ptag, pos.data(), pos.size(), false);
+                }
+            }
             if (has("Map_b") || has("Chi_b") || all) {
                 write_var(sub, "Map_ChiX_" + tag, f(CMM_SAMPLE_CHIX), n, area);
                 write_var(sub, "Map_ChiY_" + tag, f(CMM_SAMPLE_CHIY), n, area);
@@ -484,6 +546,22 @@ struct Run {
                     if (s.verbose < 4) write_file(s, sub + "/" + name, v, n, false);
                     else debug_globals(s, name + "_" + rep, v, n, area);
                 };
+                // forward map and forwarded particles of the window, :1091-1140 (the particle files
+                // carry the set's 0-based index, and only an explicit PartF_ entry saves them)
+                if (s.forward_map && (has("Map_f") || has("Chi_f") || all || !s.particles_forwarded.empty())) {
+                    if (has("Map_f") || has("Chi_f") || all) {
+                        std::vector<double> fxy(2 * n);
+                        check(cmm_euler_forward_positions(e, gz, fxy.data()), "cmm_euler_forward_positions");
+                        put("Map_ChiX_f_" + tag, fxy.data());
+                        put("Map_ChiY_f_" + tag, fxy.data() + n);
+                    }
+                    for (size_t k = 0; k < s.particles_forwarded.size(); ++k) {
+                        if (!has(("PartF_" + zpad((int)k + 1, 2)).c_str())) continue;
+                        auto pos = forwarded(k, gz);
+                        mkdir_p(s.workspace + "data/" + s.file_name + sub);
+                        write_file(s, sub + "/Particles_forwarded_pos_P" + zpad((int)k, 2), pos.data(), pos.size(), false);
+                    }
+                }
                 if (want_map) {
                     put("Map_ChiX_" + tag, out.data());
                     put("Map_ChiY_" + tag, out.data() + n);
@@ -500,20 +578,24 @@ struct Run {
     // writeMapStack (Cuda-CMM/src/ui/cmm-io.cu:611-635): map m of the stack (m = -1: the active map,
     // file suffix "coarse") and psi level `level` written once per older level, as the reference
     // writes its newest level into every Stream_function_Psi_H_psi_i file
-    void write_map_stack(int m, int level) {
+    // forward = true: the forward map, files Map_ChiX_f_H_*, and no psi
+    void write_map_stack(int m, int level, bool forward = false) {
         mkdir_p(s.workspace + "data/" + s.file_name + "/MapStack");
         const size_t nc4 = 4 * (size_t)gc.nx * gc.ny, np4 = 4 * (size_t)gp.nx * gp.ny;
         std::vector<double> cx(nc4), cy(nc4);
         if (m < 0) {
-            check(cmm_euler_get_field(e, CMM_FIELD_CHIX, cx.data()), "cmm_euler_get_field");
-            check(cmm_euler_get_field(e, CMM_FIELD_CHIY, cy.data()), "cmm_euler_get_field");
+            check(cmm_euler_get_field(e, forward ? CMM_FIELD_CHIX_F : CMM_FIELD_CHIX, cx.data()), "cmm_euler_get_field");
+            check(cmm_euler_get_field(e, forward ? CMM_FIELD_CHIY_F : CMM_FIELD_CHIY, cy.data()), "cmm_euler_get_field");
+        } else if (forward) {
+            check(cmm_euler_get_forward_stack_map(e, m, cx.data(), cy.data()), "cmm_euler_get_forward_stack_map");
         } else {
             check(cmm_euler_get_stack_map(e, m, cx.data(), cy.data()), "cmm_euler_get_stack_map");
         }
         const std::string name = m < 0 ? "coarse" : to_s(m);
-        write_file(s, "/MapStack/Map_ChiX_H_" + name, cx.data(), nc4, false);
-        write_file(s, "/MapStack/Map_ChiY_H_" + name, cy.data(), nc4, false);
-        if (L > 1) {
+        const std::string f = forward ? "_f" : "";
+        write_file(s, "/MapStack/Map_ChiX" + f + "_H_" + name, cx.data(), nc4, false);
+        write_file(s, "/MapStack/Map_ChiY" + f + "_H_" + name, cy.data(), nc4, false);
+        if (L > 1 && !forward) {
             std::vector<double> psi(np4 * L);
             check(cmm_euler_get_field(e, CMM_FIELD_PSI, psi.data()), "cmm_euler_get_field");
             for (int i = 1; i < L; ++i)
@@ -597,7 +679,6 @@ int run_euler(Settings &s) {
         log.push("Name of simulation = " + s.file_name);
     }
     // parts of the reference that are not on this engine's path stop here with a message
-    if (s.forward_map) fail("forward_map is not supported by this engine (backward map only)", 1);
     if (!vp && s.initial_condition_num >= 7 && s.initial_condition_num <= 8)
         fail("initial condition " + s.initial_condition + " draws from cuRAND/std distributions and is not provided", 230305);
     if (vp && !s.save_sample.empty() && s.verbose >= 2)
@@ -651,13 +732,25 @@ int run_euler(Settings &s) {
     }
     // stack capacity from the memory budget, Cuda-CMM/src/simulation/cmm-euler2d.cu:366-379
     const double map_size = 8.0 * sizeof(double) * r.gc.nx * r.gc.ny / 1e6;
-    const int cap = (int)(double(s.mem_RAM_CPU_remaps) / map_size);
+    int cap = (int)(double(s.mem_RAM_CPU_remaps) / map_size);
+    if (s.forward_map) cap = (int)(cap / 2.0);  // the forward stack takes the other half, :368
     p.stack_capacity = cap;
+    p.forward_map = s.forward_map ? 1 : 0;
     if (s.verbose >= 1) {
         log.push("Map size in MB = " + to_s((int)map_size));
         log.push("Map stack length on device = " + to_s(cap));
     }
     r.check(cmm_euler_create(&p, &r.e), "cmm_euler_create");
+
+    // forwarded particles, Cuda-CMM/src/simulation/cmm-euler2d.cu:399-421: start positions that stay
+    // fixed; the forward maps carry them at each save
+    if (s.forward_map)
+        for (size_t i = 0; i < s.particles_forwarded.size(); ++i) {
+            r.fpos.push_back(init_particles(s.particles_forwarded[i], b));
+            r.finit_map.push_back(0);
+            if (s.verbose >= 1)
+                log.push("Particles set F" + zpad((int)i + 1, 2) + " : Num = " + to_s(s.particles_forwarded[i].num));
+        }
 
     // particles, Cuda-CMM/src/simulation/cmm-euler2d.cu:423-450 (fluid particles)
     for (size_t i = 0; i < s.particles_advected.size(); ++i) {
@@ -729,6 +822,26 @@ int run_euler(Settings &s) {
         r.check(cmm_euler_restart(r.e, s.restart_time, n_maps, stack.data(), cx.data(), cy.data(), psi_old.data()),
                 "cmm_euler_restart");
         if (s.verbose >= 2) log.push("Loaded " + to_s(n_maps) + " maps to map stack");
+        if (s.forward_map) {
+            // the forward maps of the same folder (readMapStack with isForward, files Map_ChiX_f_H_*)
+            for (int m = 0; m < n_maps; ++m)
+                for (int k = 0; k < 2; ++k) {
+                    const std::string f = dir + "/Map_Chi" + (k ? "Y" : "X") + "_f_H_" + to_s(m) + ".data";
+                    if (!read_doubles(f, stack.data() + (2 * m + k) * nc4, nc4))
+                        msg += "Failed to read in map " + to_s(m) + " at location " + f + "\n";
+                }
+            for (int k = 0; k < 2; ++k) {
+                const std::string f = dir + "/Map_Chi" + (k ? "Y" : "X") + "_f_H_coarse.data";
+                if (!read_doubles(f, (k ? cy : cx).data(), nc4)) msg += "Failed to read in active map at location " + f + "\n";
+            }
+            if (!msg.empty()) {
+                log.push(msg);
+                log.push("Exitting ..");
+                std::exit(0);
+            }
+            r.check(cmm_euler_restart_forward(r.e, n_maps, stack.data(), cx.data(), cy.data()), "cmm_euler_restart_forward");
+            if (s.verbose >= 2) log.push("Loaded " + to_s(n_maps) + " maps to forward map stack");
+        }
         if (!s.particles_advected.empty() && s.verbose >= 2) log.push("Loaded data for particle sets");
     } else {
         r.check(cmm_euler_init(r.e, warm, &nwarm), "cmm_euler_init");
@@ -761,6 +874,22 @@ int run_euler(Settings &s) {
         const int lc = loop + L - 1;
         double dt_now = compute_next_timestep(s, tv[lc], dt);
         if (tv[lc] + dt_now > s.final_time) dt_now = s.final_time - tv[lc];
+        // forwarded particles starting in this step remap both maps, :804-811
+        if (s.forward_map) {
+            const double t_next = tv[lc] + dt_now;
+            bool start = false;
+            int steps = 0, ns = 0;
+            double t_cur = 0;
+            cmm_euler_state(r.e, &steps, &t_cur, &ns);
+            for (size_t i = 0; i < s.particles_forwarded.size(); ++i) {
+                const double d = t_next - s.particles_forwarded[i].init_time + dt * 1e-5;
+                if (d < dt_now && d >= 0) {
+                    start = true;
+                    r.finit_map[i] = ns + 1;
+                }
+            }
+            if (start) r.check(cmm_euler_request_remap(r.e), "cmm_euler_request_remap");
+        }
         cmm_step_info info;
         r.check(cmm_euler_step(r.e, dt_now, &info), "cmm_euler_step");
         tv.push_back(info.t);
@@ -778,18 +907,26 @@ int run_euler(Settings &s) {
             // writeMapStack at the remap (Cuda-CMM/src/simulation/cmm-euler2d.cu:841): the pushed map and
             // the psi level that was newest during the step (level 1 once the step has run)
             if (s.save_map_stack) r.write_map_stack(info.stack_size - 1, 1);
+            if (s.save_map_stack && s.forward_map) r.write_map_stack(info.stack_size - 1, 1, true);
         }
         const double gap = loop - old, counter = info.stack_size;
         r.save(info.t, dt_now, dt);
         loop++;
         append1(s, "/Monitoring_data/Time_s", info.t);
         append1(s, "/Monitoring_data/Error_incompressibility", info.incomp);
+        if (s.forward_map) {
+            append1(s, "/Monitoring_data/Error_incompressibility_forward", info.incomp_forward);
+            append1(s, "/Monitoring_data/Error_invertibility", info.invertibility);
+        }
         append1(s, "/Monitoring_data/Map_gaps", gap);
         append1(s, "/Monitoring_data/Map_counter", counter);
         const double c_time = ctime();
         if (s.verbose >= 2)
             log.push("Step = " + to_s(loop) + " \t S-Time = " + fmt3(info.t) + "/" + fmt3(s.final_time) + " \t E-inc = " +
-                     fmt3(info.incomp) + " \t C-Time = " + duration(c_time));
+                     fmt3(info.incomp) +
+                     (s.forward_map ? " \t E-inc_f = " + fmt3(info.incomp_forward) + " \t E-inv = " + fmt3(info.invertibility)
+                                    : std::string()) +
+                     " \t C-Time = " + duration(c_time));
         append1(s, "/Monitoring_data/Time_c", c_time);
     }
     if (s.verbose >= 2) log.push("Simulation loop finished \t C-Time = " + duration(ctime()));
@@ -808,6 +945,12 @@ int run_euler(Settings &s) {
                      fmt3((n_stack + 1) * map_size) + "mb \t S-time = " + os.str());
         }
         r.write_map_stack(-1, 0);
+        if (s.forward_map) {
+            if (s.verbose >= 2)
+                log.push("Saving last map of forward MapStack : Maps = " + to_s(n_stack) + " \t Total size = " +
+                         fmt3((n_stack + 1) * map_size) + "mb");
+            r.write_map_stack(-1, 0, true);
+        }
         for (size_t i = 0; i < s.particles_advected.size(); ++i) {
             if (r.part_set[i] < 0) continue;
             std::vector<double> pos(2 * (size_t)s.particles_advected[i].num);

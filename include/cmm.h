@@ -75,6 +75,19 @@ int cmm_invert_error(cmm_grid g_back, cmm_grid g_fwd, cmm_grid g_check, const do
  * (Cuda-CMM/src/simulation/cmm-simulation-host.cu:112-152). xy: 2*Nt interleaved. */
 int cmm_compose_positions(cmm_grid g_map, cmm_grid g_target, const double *chix, const double *chiy,
                           const double *stack, int n_stack, double *xy, void *stream);
+/* Forward positions X_cur(X_{M-1}(...X_0(x))) on g_target through a stack of forward maps (same
+ * layout), apply_map_stack with direction 1 (Cuda-CMM/src/simulation/cmm-simulation-host.cu:128-148).
+ * xy: 2*Nt interleaved. */
+int cmm_forward_positions(cmm_grid g_map, cmm_grid g_target, const double *chix, const double *chiy,
+                          const double *stack, int n_stack, double *xy, void *stream);
+/* Points through forward stacked maps first_map..n_stack-1 and then, if apply_current, the current
+ * forward map: k_h_sample_points_map (Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:97-118) as
+ * apply_map_stack_points applies it (Cuda-CMM/src/simulation/cmm-simulation-host.cu:213-291).  A point
+ * strictly inside g_bounds' box is mapped and wrapped (x - floor(x / Lx) Lx), any other becomes NaN.
+ * in/out: 2*n interleaved device arrays (distinct). */
+int cmm_forward_points(cmm_grid g_map, cmm_grid g_bounds, const double *chix, const double *chiy, const double *stack,
+                       int n_stack, int first_map, int apply_current, int n, const double *in, double *out,
+                       void *stream);
 /* Initial condition at the composed positions; k_h_sample_from_init
  * (Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:342-373).  discrete != 0 samples the Hermite
  * field disc_herm on g_disc instead of the analytic condition ic. */
@@ -163,6 +176,9 @@ typedef struct cmm_euler_params {
     uint64_t seed;                  /* CMM_IC_RANDOM_SPECTRUM seed */
     int stack_capacity;             /* maps the device stack holds */
     int sim;                        /* CMM_SIM_EULER_2D or CMM_SIM_VLASOV_POISSON_1D */
+    int forward_map;                /* 1: also advect the forward map on g_coarse, with its own stack
+                                       of stack_capacity maps (Cuda-CMM/src/simulation/cmm-euler2d.cu:156-159,
+                                       386-397, 781-811) */
 } cmm_euler_params;
 
 /* The 1D-1V Vlasov-Poisson loop (cuda_vlasov_1d, Cuda-CMM/src/simulation/cmm-vlasov1d.cu:48-1148)
@@ -176,6 +192,8 @@ typedef struct cmm_step_info {
     int remapped;                   /* 1 if the map was pushed and reset */
     int stack_size;                 /* maps on the stack after the step */
     int saturated;                  /* 1 if a remap was needed but the stack was full */
+    double incomp_forward;          /* forward map: E_inc on g_coarse (0 without a forward map) */
+    double invertibility;           /* forward map: max |X_f(X_b(x)) - x| on g_coarse (0 without) */
 } cmm_step_info;
 
 enum {
@@ -183,7 +201,8 @@ enum {
     CMM_FIELD_PSI = 2,                      /* l_order*4*Npsi, newest level first */
     CMM_FIELD_W_FINE = 3,                   /* 4*Nf fine vorticity Hermite */
     CMM_FIELD_W_COARSE = 4,                 /* Nc, omega(X(x)) sampled on the coarse grid */
-    CMM_FIELD_DISC = 5                      /* 4*Ndisc discrete initial Hermite */
+    CMM_FIELD_DISC = 5,                     /* 4*Ndisc discrete initial Hermite */
+    CMM_FIELD_CHIX_F = 6, CMM_FIELD_CHIY_F = 7 /* 4*Nc, current forward map (forward_map only) */
 };
 
 int cmm_euler_create(const cmm_euler_params *p, cmm_euler **out);
@@ -207,6 +226,17 @@ int cmm_euler_restart(cmm_euler *e, double t_restart, int n_maps, const double *
 /* Map m of the stack (0 = oldest) to host memory, 4*Nc doubles each (writeMapStack,
  * Cuda-CMM/src/ui/cmm-io.cu:611-635). */
 int cmm_euler_get_stack_map(cmm_euler *e, int m, double *chix_host, double *chiy_host);
+/* Forward-map part of a restart (forward_map only), after cmm_euler_restart: n_maps forward maps
+ * (8*Nc doubles each, oldest first) and the active forward map (readMapStack with isForward,
+ * Cuda-CMM/src/simulation/cmm-simulation-host.cu:1262-1291).  n_maps equals the backward count. */
+int cmm_euler_restart_forward(cmm_euler *e, int n_maps, const double *stack_host, const double *chix_host,
+                              const double *chiy_host);
+/* Forward map m of the forward stack (0 = oldest), as cmm_euler_get_stack_map. */
+int cmm_euler_get_forward_stack_map(cmm_euler *e, int m, double *chix_host, double *chiy_host);
+/* Makes the next cmm_euler_step remap whatever E_inc is (the forwarded-particle start of
+ * Cuda-CMM/src/simulation/cmm-euler2d.cu:804-814, which pushes both maps at the particles' start
+ * time).  The stack limit still applies. */
+int cmm_euler_request_remap(cmm_euler *e);
 /* One step of length dt_now (the driver clips dt to save targets and the final time). */
 int cmm_euler_step(cmm_euler *e, double dt_now, cmm_step_info *info);
 /* Copies a field to host memory (size per CMM_FIELD_*). */
@@ -242,6 +272,18 @@ int cmm_euler_sample(cmm_euler *e, cmm_grid g_sample, int scalar_ic, double *hos
  * the engine's own initial condition, discrete or not; else a CMM_IC_* value), then, when
  * want_psi, the newest psi level evaluated at the zoom grid points themselves (k_h_sample). */
 int cmm_euler_zoom(cmm_euler *e, cmm_grid g_zoom, const int *ics, int n_ics, int want_psi, double *host_out);
+/* The forward map through the forward stack on the points of g_target (apply_map_stack with
+ * direction 1, Cuda-CMM/src/simulation/cmm-simulation-host.cu:128-148): the oldest stacked forward
+ * map first, then the newer ones, then the current forward map, each result wrapped into the
+ * domain.  host_out: X positions (n) then Y positions (n), n = g_target.nx * g_target.ny, row-major. */
+int cmm_euler_forward_positions(cmm_euler *e, cmm_grid g_target, double *host_out);
+/* Forwarded particles (apply_map_stack_points, direction 1, Cuda-CMM/src/simulation/cmm-simulation-host.cu:213-291):
+ * n points (x, y interleaved) through the forward stacked maps first_map .. stack_size-1 and then,
+ * if apply_current, the current forward map; with nothing to apply the points are copied.  Each
+ * application takes a point strictly inside the bounds of g_bounds through the map and wraps the
+ * result by the domain lengths (x - floor(x / Lx) Lx); any other point becomes NaN. */
+int cmm_euler_forward_points(cmm_euler *e, cmm_grid g_bounds, int n, const double *pos_host, int first_map,
+                             int apply_current, double *host_out);
 /* Steps taken, current time, stack size. */
 int cmm_euler_state(cmm_euler *e, int *steps, double *t, int *stack_size);
 /* Per-kernel timing with events on the engine stream: enable (clears the counters), then read

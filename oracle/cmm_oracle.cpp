@@ -24,6 +24,7 @@
 #include "cmm_oracle.h"
 
 #include <cmath>
+#include <limits>
 #include <cstring>
 #include <cstdlib>
 #include <vector>
@@ -645,6 +646,58 @@ void compose(const Grid &gm, const Grid &gt, const double *cx, const double *cy,
     }
 }
 
+// Forward composition, apply_map_stack with direction 1 (Cuda-CMM/src/simulation/cmm-simulation-host.cu:128-148):
+// the oldest stacked forward map first (k_h_sample_map_compact), the newer ones after it and the
+// current forward map last (k_apply_map_compact, Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:302-340),
+// each result wrapped into the domain; with an empty stack the current map alone.
+void compose_forward(const Grid &gm, const Grid &gt, const double *cx, const double *cy, const double *sx,
+                     const double *sy, int ns, double *xy) {
+    const double LX = gm.x1 - gm.x0, LY = gm.y1 - gm.y0;
+#pragma omp parallel for
+    for (int iy = 0; iy < gt.ny; ++iy) {
+        for (int ix = 0; ix < gt.nx; ++ix) {
+            long i = (long)iy * gt.nx + ix;
+            double x = gt.x0 + ix * gt.hx, y = gt.y0 + iy * gt.hy;
+            double px, py;
+            for (int m = 0; m <= ns; ++m) {
+                const double *mx = m < ns ? sx + 4 * gm.n * (long)m : cx;
+                const double *my = m < ns ? sy + 4 * gm.n * (long)m : cy;
+                diffeo(gm, mx, my, x, y, px, py);
+                x = px - std::floor((px - gm.x0) / LX) * LX;
+                y = py - std::floor((py - gm.y0) / LY) * LY;
+            }
+            xy[2 * i] = x; xy[2 * i + 1] = y;
+        }
+    }
+}
+
+// Forwarded particles: k_h_sample_points_map (Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:97-118)
+// applied with the stacked forward maps first..ns-1 and then, if cur, the current forward map
+// (the selection of Cuda-CMM/src/simulation/cmm-simulation-host.cu:213-291).  A point strictly inside
+// the bounds b = (x0, x1, y0, y1) goes through the map and is wrapped by x - floor(x / Lx) Lx, with
+// Lx the map domain's length; any other point becomes NaN.
+void forward_points(const Grid &gm, const double *b, const double *cx, const double *cy, const double *sx,
+                    const double *sy, int ns, int first, int cur, int n, const double *in, double *out) {
+    const double LX = gm.x1 - gm.x0, LY = gm.y1 - gm.y0;
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    for (int i = 0; i < n; ++i) {
+        double x = in[2 * i], y = in[2 * i + 1];
+        auto apply = [&](const double *mx, const double *my) {
+            if (x > b[0] && x < b[1] && y > b[2] && y < b[3]) {
+                double px, py;
+                diffeo(gm, mx, my, x, y, px, py);
+                x = px - std::floor(px / LX) * LX;
+                y = py - std::floor(py / LY) * LY;
+            } else {
+                x = y = nan;
+            }
+        };
+        for (int m = first < 0 ? 0 : first; m < ns; ++m) apply(sx + 4 * gm.n * (long)m, sy + 4 * gm.n * (long)m);
+        if (cur) apply(cx, cy);
+        out[2 * i] = x; out[2 * i + 1] = y;
+    }
+}
+
 // Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:402-461; positions use the map grid's origin
 void sample_vort(const Grid &gm, const Grid &gv, const Grid &gf, const double *cx, const double *cy, const double *H,
                  int molly, double *out) {
@@ -1044,6 +1097,18 @@ double cmmo_incomp_error(cmm_grid ggm, cmm_grid ggc, const double *cx, const dou
 int cmmo_compose_positions(cmm_grid gm, cmm_grid gt, const double *cx, const double *cy, const double *sx,
                            const double *sy, int ns, double *xy) {
     compose(mk(gm), mk(gt), cx, cy, sx, sy, ns, xy);
+    return CMM_OK;
+}
+
+int cmmo_forward_positions(cmm_grid gm, cmm_grid gt, const double *cx, const double *cy, const double *sx,
+                           const double *sy, int ns, double *xy) {
+    compose_forward(mk(gm), mk(gt), cx, cy, sx, sy, ns, xy);
+    return CMM_OK;
+}
+
+int cmmo_forward_points(cmm_grid gm, const double *bounds, const double *cx, const double *cy, const double *sx,
+                        const double *sy, int ns, int first, int cur, int n, const double *in, double *out) {
+    forward_points(mk(gm), bounds, cx, cy, sx, sy, ns, first, cur, n, in, out);
     return CMM_OK;
 }
 

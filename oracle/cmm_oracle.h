@@ -47,6 +47,15 @@ double cmmo_invert_error(cmm_grid g_back, cmm_grid g_fwd, cmm_grid gcheck, const
 /* Positions X_1(...X_M(X_cur(x))) on gt, Cuda-CMM/src/simulation/cmm-simulation-host.cu:112-152 */
 int cmmo_compose_positions(cmm_grid gm, cmm_grid gt, const double *chix, const double *chiy,
                            const double *stack_x, const double *stack_y, int n_stack, double *xy);
+/* Forward positions X_cur(X_{M-1}(...X_0(x))) on gt, Cuda-CMM/src/simulation/cmm-simulation-host.cu:128-148 */
+int cmmo_forward_positions(cmm_grid gm, cmm_grid gt, const double *chix, const double *chiy,
+                           const double *stack_x, const double *stack_y, int n_stack, double *xy);
+/* Forwarded particles through stacked maps first..n_stack-1 then (if apply_current) the current map,
+ * Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:97-118 with cmm-simulation-host.cu:213-291;
+ * bounds = (x0, x1, y0, y1) of the target grid */
+int cmmo_forward_points(cmm_grid gm, const double *bounds, const double *chix, const double *chiy,
+                        const double *stack_x, const double *stack_y, int n_stack, int first_map, int apply_current,
+                        int n, const double *in, double *out);
 /* Initial condition at positions, Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:342-373 */
 int cmmo_sample_init(cmm_grid gt, const double *xy, int ic, const double *ic_params,
                      const double *disc_herm, cmm_grid gdisc, int discrete, double *out);

@@ -7,7 +7,7 @@ Sources (Cuda-CMM/tests/...):
   euler2D_11b_4nodes_reference_remapping/data/reference-results_4_nodes_C256_F512_t32_T2/Monitoring_data/
   vlasov1D_15_vp_landau_damping/data/reference-results_C256_F512_t1024_T0.2/Monitoring_data/
   euler2D_init_fields/data/reference-results_<kind>_4_nodes_C256_F512_t1_T0/Monitoring_data/
-    (kind = none, comp, conv, sample, zoom, particles: the t = 0 outputs of each save mode)
+    (kind = none, comp, conv, sample, zoom, particles, forward: the t = 0 outputs of each save mode)
   euler2D_remap/data/reference-results_4_nodes_C256_F512_t32_T2/Monitoring_data/
     (the second half of a run restarted from the map stack the first half saved)
 Also the log.txt warm-up lines "IncompErr = ..." (3 significant digits), the saved params.txt of each
@@ -27,7 +27,7 @@ CASES = {
     "vlasov_15": "vlasov1D_15_vp_landau_damping/data/reference-results_C256_F512_t1024_T0.2",
     "restart": "euler2D_remap/data/reference-results_4_nodes_C256_F512_t32_T2",
 }
-INIT_KINDS = ("none", "comp", "conv", "sample", "zoom", "particles")
+INIT_KINDS = ("none", "comp", "conv", "sample", "zoom", "particles", "forward")
 for _k in INIT_KINDS:
     CASES["init_" + _k] = f"euler2D_init_fields/data/reference-results_{_k}_4_nodes_C256_F512_t1_T0"
 # the parameter files the reference's harness runs each case with (Cuda-CMM/tests/*/run_test.py)

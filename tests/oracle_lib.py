@@ -126,6 +126,25 @@ def compose_positions(gm, gt, cx, cy, stack_x, stack_y, n_stack):
     return xy
 
 
+def forward_positions(gm, gt, cx, cy, stack_x, stack_y, n_stack):
+    xy = np.zeros(2 * gt.nx * gt.ny)
+    sx = stack_x if stack_x is not None else np.zeros(1)
+    sy = stack_y if stack_y is not None else np.zeros(1)
+    lib().cmmo_forward_positions(gm, gt, ptr(cx), ptr(cy), ptr(sx), ptr(sy), n_stack, ptr(xy))
+    return xy
+
+
+def forward_points(gm, bounds, cx, cy, stack_x, stack_y, n_stack, first_map, apply_current, pos):
+    pos = np.ascontiguousarray(pos, dtype=np.float64)
+    out = np.zeros_like(pos)
+    b = np.ascontiguousarray(bounds, dtype=np.float64)
+    sx = stack_x if stack_x is not None else np.zeros(1)
+    sy = stack_y if stack_y is not None else np.zeros(1)
+    lib().cmmo_forward_points(gm, ptr(b), ptr(cx), ptr(cy), ptr(sx), ptr(sy), n_stack, first_map, int(apply_current),
+                              pos.size // 2, ptr(pos), ptr(out))
+    return out
+
+
 def sample_init(gt, xy, ic, params=None, disc=None, gdisc=None, discrete=0):
     out = np.zeros(gt.nx * gt.ny)
     prm = np.zeros(10) if params is None else np.ascontiguousarray(params, dtype=np.float64)

@@ -95,7 +95,7 @@ def test_cli_one_step(dev, tmp_path):
     assert compare(mon, "one_step") >= 40
 
 
-@pytest.mark.parametrize("kind", ["none", "conv", "comp", "sample", "zoom", "particles"])
+@pytest.mark.parametrize("kind", ["none", "conv", "comp", "sample", "zoom", "particles", "forward"])
 def test_cli_init_fields(dev, tmp_path, kind):
     """The reference's initialisation cases (Cuda-CMM/tests/euler2D_init_fields): final_time 0, so
     the Lagrange warm-up (with dt = 1) and the saves at t = 0 and at the end, one save mode each:

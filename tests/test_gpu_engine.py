@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 import oracle_lib as O
-from gpu_util import rel_linf
+from gpu_util import hermite_errors, rel_linf
 
 pytestmark = pytest.mark.gpu
 
@@ -238,3 +238,65 @@ def test_zoom_and_restart_against_oracle(dev):
     assert abs(info.t - (0.75 + 1.0 / 32)) < 1e-15
     e.close()
     r.close()
+
+
+def test_forward_map_in_the_loop(dev):
+    """forward_map = 1 (map 64^2, fine 128^2, threshold 1e-5 so that the run remaps): E_inc of both
+    maps and the invertibility error per step and the map counter against the oracle's loop, the
+    final forward map, its composition through the forward stack and forwarded points; then a
+    remap requested below the threshold (cmm_euler_request_remap) pushes both maps and resets the
+    forward map to the identity."""
+    import cmm
+    gc, gf = cmm.grid(64), cmm.grid(128)
+    dt, steps = 1.0 / 32, 24
+    e = cmm.Euler(gc, gf, gf, gf, dt, incomp_threshold=1e-5, stack_capacity=16, forward_map=True)
+    e.init()
+    assert np.array_equal(e.field("chix_f"), O.identity_map(O.grid(64))[0])
+    inc, inc_f, inv, ctr = [], [], [], []
+    for _ in range(steps):
+        info = e.step(dt)
+        inc.append(info.incomp)
+        inc_f.append(info.incomp_forward)
+        inv.append(info.invertibility)
+        ctr.append(info.stack_size)
+    ogc, ogf = O.grid(64), O.grid(128)
+    r = O.euler_run(ogc, ogf, ogf, ogf, dt=dt, final_time=steps * dt, threshold=1e-5, stack_capacity=16, forward_map=1,
+                    want=("incomp", "time", "map_counter", "fchix", "fchiy"))
+    print("E_inc", inc[:3], r["incomp"][:3])
+    print("E_inc_f", inc_f[:3], r["incomp_fwd"][:3])
+    print("E_inv", inv[:3], r["invert"][:3])
+    assert np.array_equal(np.array(ctr, dtype=float), r["map_counter"]) and ctr[-1] >= 2
+    assert np.all(np.abs(np.array(inc) - r["incomp"]) <= 1e-10 * np.abs(r["incomp"]) + 2e-11)
+    assert np.all(np.abs(np.array(inc_f) - r["incomp_fwd"]) <= 1e-10 * np.abs(r["incomp_fwd"]) + 2e-11)
+    # |x - X_b(X_f(x))| is a cancellation between positions of size 2 pi: its rounding level is
+    # absolute, ~1e-13 after 24 steps of accumulated map rounding
+    assert np.all(np.abs(np.array(inv) - r["invert"]) <= 1e-10 * np.abs(r["invert"]) + 2e-12)
+    fx, fy = e.field("chix_f"), e.field("chiy_f")
+    for name, g, ref in (("X_f", fx, r["fchix"]), ("Y_f", fy, r["fchiy"])):
+        errs = hermite_errors(g, ref, 64 * 64, 1.0 / 1e-3)
+        print("forward map", name, errs)
+        assert max(errs) <= 1e-10
+    n_st = ctr[-1]
+    maps = [e.forward_stack_map(m) for m in range(n_st)]
+    sx = np.concatenate([m[0] for m in maps])
+    sy = np.concatenate([m[1] for m in maps])
+    gt = cmm.grid(96)
+    x, y = e.forward_positions(gt)
+    xy = O.forward_positions(ogc, O.grid(96), fx, fy, sx, sy, n_st)
+    d = np.mod(np.concatenate([x - xy[0::2], y - xy[1::2]]) + np.pi, 2 * np.pi) - np.pi
+    print("forward positions", np.abs(d).max())
+    assert np.abs(d).max() <= 1e-12 * TWO_PI
+    pts = np.random.default_rng(5).uniform(0.01, TWO_PI - 0.01, size=(300, 2)).ravel()
+    for first, cur in ((0, True), (1, True), (1, False)):
+        got = e.forward_points(gc, pts, first, cur)
+        ref = O.forward_points(ogc, (0.0, TWO_PI, 0.0, TWO_PI), fx, fy, sx, sy, n_st, first, cur, pts)
+        assert np.array_equal(np.isnan(got), np.isnan(ref))
+        dd = np.mod(got - ref + np.pi, 2 * np.pi) - np.pi
+        assert np.nanmax(np.abs(dd)) <= 1e-12 * TWO_PI
+    # a requested remap below the threshold
+    e.request_remap()
+    info = e.step(dt)
+    assert info.remapped == 1 and info.stack_size == n_st + 1
+    assert np.array_equal(e.field("chix_f"), O.identity_map(ogc)[0])
+    assert not np.array_equal(e.forward_stack_map(n_st)[0], O.identity_map(ogc)[0])
+    e.close()

@@ -103,6 +103,53 @@ def test_compose_and_sample_with_stack(dev, base):
     assert e <= TOL
 
 
+def _shear_maps(n, a, b):
+    """Hermite planes of A(x, y) = (x + a sin y, y) and B(x, y) = (x, y + b sin x) on an n^2 grid."""
+    h = 2 * np.pi / float(np.float32(n))
+    X, Y = np.meshgrid(np.arange(n) * h, np.arange(n) * h)
+    one, zero = np.ones_like(X), np.zeros_like(X)
+    ax = np.concatenate([p.ravel() for p in (X + a * np.sin(Y), one, a * np.cos(Y), zero)])
+    ay = np.concatenate([p.ravel() for p in (Y, zero, one, zero)])
+    bx = np.concatenate([p.ravel() for p in (X, one, zero, zero)])
+    by = np.concatenate([p.ravel() for p in (Y + b * np.sin(X), b * np.cos(X), one, zero)])
+    return ax, ay, bx, by
+
+
+def test_forward_composition_and_points(dev, base):
+    """Forward positions through a forward stack (oldest first, current last) and forwarded points
+    with every map selection, against the oracle."""
+    import cmm
+    gc, gf, gp = grids()
+    ogc, ogf, ogp = ogrids()
+    ax, ay, bx, by = _shear_maps(GC, 0.3, 0.3)
+    mx, my = base["chix"], base["chiy"]
+    stack = np.concatenate([mx, my, ax, ay])
+    sx, sy = np.concatenate([mx, ax]), np.concatenate([my, ay])
+    ref_xy = O.forward_positions(ogc, ogf, bx, by, sx, sy, 2)
+    xy = dev_array(np.zeros(2 * GF * GF))
+    cmm.apply_map_stack_forward(gf, gc, dev_array(bx), dev_array(by), dev_array(stack), 2, xy)
+    d = np.mod(host(xy) - ref_xy + np.pi, 2 * np.pi) - np.pi
+    e = np.max(np.abs(d)) / np.max(np.abs(ref_xy))
+    print("forward compose rel Linf", e)
+    assert e <= TOL
+    rng = np.random.default_rng(11)
+    pts = rng.uniform(0.0, 2 * np.pi, size=(5000, 2)).ravel()
+    pts[:4] = [0.0, 1.0, 1.0, 0.0]  # on the boundary: NaN
+    zoom = cmm.Grid(64, 64, 1.0, 5.0, 0.5, 4.5)
+    for gb, bounds in ((gc, (0.0, 2 * np.pi, 0.0, 2 * np.pi)), (zoom, (1.0, 5.0, 0.5, 4.5))):
+        for first, cur in ((0, 1), (1, 1), (2, 1), (0, 0), (2, 0), (5, 1)):
+            ref = O.forward_points(ogc, bounds, bx, by, sx, sy, 2, first, cur, pts)
+            out = dev_array(np.zeros_like(pts))
+            cmm.apply_map_stack_points_forward(gb, gc, dev_array(bx), dev_array(by), dev_array(stack), 2, first, cur,
+                                               dev_array(pts), out)
+            got = host(out)
+            assert np.array_equal(np.isnan(got), np.isnan(ref)), (first, cur)
+            ok = ~np.isnan(ref)
+            dd = np.mod(got[ok] - ref[ok] + np.pi, 2 * np.pi) - np.pi
+            print("forward points", bounds, first, cur, "nan", int((~ok).sum()), "max diff", np.max(np.abs(dd)))
+            assert np.max(np.abs(dd)) <= TOL * 2 * np.pi
+
+
 def test_psi_from_vort(dev, base):
     import cmm
     gc, gf, gp = grids()

@@ -490,3 +490,73 @@ def test_forward_rk3_is_third_order_and_inverts_the_backward_map():
         worst = max(worst, np.hypot(x - xo.value, y - yo.value))
     print("invertibility", e, worst)
     assert worst <= e < 1e-4
+
+
+def _shears(n, a, b):
+    """Hermite maps of two shears that do not commute: A(x, y) = (x + a sin y, y), B(x, y) = (x, y + b sin x)."""
+    one, zero = (lambda X, Y: 1 + 0 * X), (lambda X, Y: 0 * X)
+    ax, _ = herm_from_poly(n, lambda X, Y: X + a * np.sin(Y), one, lambda X, Y: a * np.cos(Y), zero)
+    ay, _ = herm_from_poly(n, lambda X, Y: Y, zero, one, zero)
+    bx, _ = herm_from_poly(n, lambda X, Y: X, one, zero, zero)
+    by, _ = herm_from_poly(n, lambda X, Y: Y + b * np.sin(X), lambda X, Y: b * np.cos(X), one, zero)
+    return (ax, ay), (bx, by)
+
+
+def _torus_err(got, want):
+    d = np.abs(np.asarray(got) - np.asarray(want)) % TWO_PI
+    return np.max(np.minimum(d, TWO_PI - d))
+
+
+def test_forward_composition_order_with_non_commuting_maps():
+    """Forward composition applies the oldest stacked map first and the current map last; backward
+    composition the other way round.  Two shears that do not commute tell the orders apart by
+    O(a b) = 0.09, far above the bicubic interpolation error (~1e-6 at n = 128)."""
+    n, a, b = 128, 0.3, 0.3
+    gm, gt = O.grid(n), O.grid(48)
+    (ax, ay), (bx, by) = _shears(n, a, b)
+    h = TWO_PI / 48.0
+    X, Y = np.meshgrid(np.arange(48) * h, np.arange(48) * h)
+    X, Y = X.ravel(), Y.ravel()
+    # forward: stack [A], current B -> B(A(p))
+    xy = O.forward_positions(gm, gt, bx, by, ax, ay, 1)
+    x1 = X + a * np.sin(Y)
+    assert _torus_err(xy[0::2], x1) < 1e-5 and _torus_err(xy[1::2], Y + b * np.sin(x1)) < 1e-5
+    # backward with the same inputs: A(B(p))
+    xyb = O.compose_positions(gm, gt, bx, by, ax, ay, 1)
+    y1 = Y + b * np.sin(X)
+    assert _torus_err(xyb[0::2], X + a * np.sin(y1)) < 1e-5 and _torus_err(xyb[1::2], y1) < 1e-5
+    assert _torus_err(xy[0::2], xyb[0::2]) > 0.05
+    # empty stack: the current map alone
+    xy0 = O.forward_positions(gm, gt, ax, ay, None, None, 0)
+    assert _torus_err(xy0[0::2], x1) < 1e-5 and _torus_err(xy0[1::2], Y) < 1e-12
+
+
+def test_forward_points_selection_and_bounds():
+    """Forwarded particles: maps first_map.. of the stack then (optionally) the current map, each
+    application only for points strictly inside the bounds (others become NaN); with nothing to
+    apply the points are copied."""
+    n, a, b = 128, 0.3, 0.3
+    gm = O.grid(n)
+    (ax, ay), (bx, by) = _shears(n, a, b)
+    cx, cy = O.identity_map(gm)
+    # stack [identity, A], current B
+    sx, sy = np.concatenate([cx, ax]), np.concatenate([cy, ay])
+    rng = np.random.default_rng(3)
+    p = rng.uniform(0.1, TWO_PI - 0.1, size=(200, 2)).ravel()
+    px, py = p[0::2], p[1::2]
+    full = (0.0, TWO_PI, 0.0, TWO_PI)
+    both = O.forward_points(gm, full, bx, by, sx, sy, 2, 0, True, p)
+    x1 = px + a * np.sin(py)
+    assert _torus_err(both[0::2], x1) < 1e-5 and _torus_err(both[1::2], py + b * np.sin(np.mod(x1, TWO_PI))) < 1e-5
+    # the same as the forward positions of those points on a grid when all maps apply
+    only_cur = O.forward_points(gm, full, bx, by, sx, sy, 2, 2, True, p)
+    assert _torus_err(only_cur[0::2], px) < 1e-12 and _torus_err(only_cur[1::2], py + b * np.sin(px)) < 1e-5
+    no_cur = O.forward_points(gm, full, bx, by, sx, sy, 2, 1, False, p)
+    assert _torus_err(no_cur[0::2], x1) < 1e-5 and _torus_err(no_cur[1::2], py) < 1e-12
+    copy = O.forward_points(gm, full, bx, by, sx, sy, 2, 2, False, p)
+    assert np.array_equal(copy, p)
+    # bounds: a point on the boundary (x = 0) or outside a window is NaN after the first application
+    q = np.array([0.0, 1.0, 1.0, 1.0, 3.0, 3.0])
+    out = O.forward_points(gm, (0.5, 2.0, 0.5, 2.0), bx, by, sx, sy, 2, 2, True, q)
+    assert np.isnan(out[0]) and np.isnan(out[1]) and np.isnan(out[4]) and np.isnan(out[5])
+    assert abs(out[2] - 1.0) < 1e-12 and abs(out[3] - (1.0 + b * np.sin(1.0))) < 1e-5

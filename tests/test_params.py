@@ -24,7 +24,7 @@ def roundtrip(tmp_path, text, *args):
 
 
 @pytest.mark.parametrize("case", ["one_step", "remap_11b", "vlasov_15", "restart", "init_none", "init_comp", "init_conv",
-                                  "init_sample", "init_zoom", "init_particles"])
+                                  "init_sample", "init_zoom", "init_particles", "init_forward"])
 def test_saved_param_files_roundtrip(tmp_path, case):
     text = FIX[case]["params_saved"]
     if case == "vlasov_15":
