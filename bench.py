@@ -202,6 +202,46 @@ def summarize_profiles(kt_dir, fetch_dir, write_dir, out_dir, commands):
     print("\n".join(lines))
 
 
+def run_form(cmm, torch, c, precision, warmup, steps, dt=None):
+    """One engine run of the workload: W untimed steps, then K steps timed between two device
+    synchronisations with per-kernel events; returns the rate, the profile and the final fields
+    (coarse vorticity, newest psi level, particles) for the error comparison."""
+    import numpy as np
+    g = cmm.grid
+    dt = c["dt"] if dt is None else dt
+    t_setup = time.perf_counter()
+    eng = cmm.Euler(g(c["n_coarse"]), g(c["n_fine"]), g(c["n_psi"]), g(c["n_vort"]), dt,
+                    incomp_threshold=c["threshold"], map_epsilon=c["eps"], freq_cut_psi=c["freq_cut_psi"],
+                    initial_condition="random_spectrum", seed=c["seed"], stack_capacity=c["stack_capacity"],
+                    precision=precision)
+    eng.add_particles(uniform_particles(c["n_particles"], c["seed"]), scheme=30)
+    warm = eng.init()
+    t_init = time.perf_counter() - t_setup
+    log(f"[{precision}] init + warm-up {t_init:.2f}s, warm-up E_inc {warm}")
+    einc = []
+    for i in range(warmup):
+        einc.append(eng.step(dt).incomp)
+    torch.cuda.synchronize()
+    eng.profile(True)
+    remaps = 0
+    t0 = time.perf_counter()
+    for i in range(steps):
+        info = eng.step(dt)
+        remaps += info.remapped
+        einc.append(info.incomp)
+        if (i + 1) % 128 == 0:
+            log(f"[{precision}] step {warmup + i + 1}: t={info.t:.4f} E_inc={info.incomp:.3e} maps={info.stack_size}")
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prof = eng.profile_read()
+    _, t_now, n_stack = eng.state()
+    out = {"sps": steps / (t1 - t0), "elapsed": t1 - t0, "prof": prof, "maps": n_stack, "t_end": t_now,
+           "remaps": remaps, "init_s": t_init, "einc": np.array(einc), "w": eng.field("w_coarse"),
+           "psi": eng.field("psi")[:4 * c["n_psi"] ** 2], "part": eng.particles(0)}
+    eng.close()
+    return out
+
+
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "summarize-profiles":
         kt, fe, wr, out = sys.argv[2:6]
@@ -213,6 +253,7 @@ def main():
     ap.add_argument("--steps", type=int, default=504)
     ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-split", action="store_true", help="skip the split-precision form")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 1 GiB device copy, the reference kernel for counter scaling")
     args = ap.parse_args()
@@ -221,6 +262,7 @@ def main():
               f"path (requested --gpus {args.gpus}).", file=sys.stderr)
         sys.exit(2)
 
+    import numpy as np
     import torch
     import cmm
     if not torch.cuda.is_available():
@@ -234,34 +276,9 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    g = cmm.grid
-    t_setup = time.perf_counter()
-    eng = cmm.Euler(g(c["n_coarse"]), g(c["n_fine"]), g(c["n_psi"]), g(c["n_vort"]), c["dt"],
-                    incomp_threshold=c["threshold"], map_epsilon=c["eps"], freq_cut_psi=c["freq_cut_psi"],
-                    initial_condition="random_spectrum", seed=c["seed"], stack_capacity=c["stack_capacity"])
-    eng.add_particles(uniform_particles(c["n_particles"], c["seed"]), scheme=30)
-    warm = eng.init()
-    t_init = time.perf_counter() - t_setup
-    log(f"init + warm-up {t_init:.2f}s, warm-up E_inc {warm}")
-    dt = c["dt"]
-    t = 0.0
-    for i in range(args.warmup):
-        info = eng.step(dt)
-    torch.cuda.synchronize()
-    eng.profile(True)
-    remaps = 0
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        info = eng.step(dt)
-        remaps += info.remapped
-        if (i + 1) % 128 == 0:
-            log(f"step {args.warmup + i + 1}: t={info.t:.4f} E_inc={info.incomp:.3e} maps={info.stack_size}")
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    prof = eng.profile_read()
-    steps, t_now, n_stack = eng.state()
-    elapsed = t1 - t0
-    sps = args.steps / elapsed
+    plain = run_form(cmm, torch, c, "fp64", args.warmup, args.steps)
+    sps, elapsed, prof, n_stack, t_now, remaps, t_init = (plain[k] for k in (
+        "sps", "elapsed", "prof", "maps", "t_end", "remaps", "init_s"))
 
     # dominant kernel and its roofline
     kx_max = int(min(c["n_psi"] // 2, c["n_vort"] // 2, c["freq_cut_psi"]))
@@ -293,6 +310,37 @@ def main():
     roof.update({"kernel": dom, "traffic": traffic, "ms_avg": round(d["ms_avg"], 5),
                  "algorithmic_bytes": d["bytes"], "algorithmic_flops": d["flops"]})
 
+    # the chosen form: split precision at the same step, with its errors against the FP64 run
+    chosen = None
+    if not args.no_split:
+        sp = run_form(cmm, torch, c, "split", args.warmup, args.steps)
+        rl2 = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+        rli = lambda a, b: float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+        dpart = np.abs(sp["part"] - plain["part"])
+        dpart = np.minimum(dpart, 2 * np.pi - dpart)
+        chosen = {"form": "split precision (map = identity + FP32 displacement, FP32 combined psi and fine omega "
+                          "records, FP64 arithmetic, FP64 psi levels / FFT / Poisson divide)",
+                  "value": round(sp["sps"], 3), "ms_per_step": round(1e3 * sp["elapsed"] / args.steps, 4),
+                  "speedup": round(sp["sps"] / sps, 3),
+                  "errors_vs_fp64_at_t_end": {
+                      "omega_coarse_rel_l2": rl2(sp["w"], plain["w"]), "omega_coarse_rel_linf": rli(sp["w"], plain["w"]),
+                      "psi_rel_l2": rl2(sp["psi"], plain["psi"]), "psi_rel_linf": rli(sp["psi"], plain["psi"]),
+                      "particles_max_abs": float(dpart.max()),
+                      "e_inc_max_abs_diff": float(np.max(np.abs(sp["einc"] - plain["einc"])))},
+                  "remaps_timed": sp["remaps"], "maps": sp["maps"],
+                  "kernels_ms_avg": {k: round(ms / cnt, 5) for k, (ms, cnt) in sp["prof"].items() if cnt}}
+        log(f"split form: {sp['sps']:.1f} steps/s ({chosen['speedup']:.2f}x), errors {chosen['errors_vs_fp64_at_t_end']}")
+        # the same forms at twice the step (plain Lagrange extrapolation of psi), to the same end
+        # time: their error against the FP64 run at the nominal step, and the rate in simulated time
+        big = {}
+        for prec in ("fp64", "split"):
+            r = run_form(cmm, torch, c, prec, args.warmup // 2, args.steps // 2, dt=2 * c["dt"])
+            big[prec] = {"dt": 2 * c["dt"], "steps_per_s": round(r["sps"], 3),
+                         "speedup_in_simulated_time": round(2 * r["sps"] / sps, 3), "t_end": r["t_end"],
+                         "omega_coarse_rel_l2": rl2(r["w"], plain["w"]), "psi_rel_l2": rl2(r["psi"], plain["psi"])}
+            log(f"[{prec} at 2 dt] {big[prec]}")
+        chosen["at_twice_the_step"] = big
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
@@ -314,8 +362,8 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels_ms_avg": {k: round(v["ms_avg"], 5) for k, v in per.items()},
+        "chosen_form": chosen,
     }
-    eng.close()
     print(json.dumps(out), flush=True)
 
 

@@ -56,7 +56,8 @@ class EulerParams(C.Structure):
                 ("molly_stencil", C.c_int), ("skip_remapping", C.c_int),
                 ("ic", C.c_int), ("ic_params", C.c_double * 10),
                 ("discrete", C.c_int), ("g_disc", Grid), ("disc_real", C.POINTER(C.c_double)),
-                ("seed", C.c_uint64), ("stack_capacity", C.c_int), ("sim", C.c_int), ("forward_map", C.c_int)]
+                ("seed", C.c_uint64), ("stack_capacity", C.c_int), ("sim", C.c_int), ("forward_map", C.c_int),
+                ("precision", C.c_int)]
 
 
 class StepInfo(C.Structure):
@@ -78,7 +79,11 @@ SYMBOLS = ["cmm_version", "cmm_last_error", "cmm_device_info", "cmm_lagrange_coe
            "cmm_euler_add_particles", "cmm_euler_init", "cmm_euler_step", "cmm_euler_get_field",
            "cmm_euler_get_particles", "cmm_euler_diagnostics", "cmm_euler_vp_diagnostics", "cmm_euler_sample", "cmm_euler_zoom", "cmm_euler_restart", "cmm_euler_get_stack_map",
            "cmm_euler_restart_forward", "cmm_euler_get_forward_stack_map", "cmm_euler_request_remap",
-           "cmm_euler_forward_positions", "cmm_euler_forward_points", "cmm_euler_state", "cmm_euler_profile", "cmm_euler_profile_read"]
+           "cmm_euler_forward_positions", "cmm_euler_forward_points", "cmm_euler_state", "cmm_euler_profile", "cmm_euler_profile_read",
+           "cmm_map_to_split", "cmm_map_from_split", "cmm_hermite_to_split", "cmm_map_advect_split_workspace_bytes",
+           "cmm_map_advect_split", "cmm_incomp_error_split", "cmm_psi_from_vort_split"]
+# engine precision forms (CMM_PRECISION_*)
+PRECISION = {"fp64": 0, "split": 1}
 
 _lib = None
 DP = C.POINTER(C.c_double)
@@ -151,6 +156,15 @@ def lib():
         L.cmm_euler_state.argtypes = [VP, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_int)]
         L.cmm_euler_profile.argtypes = [VP, C.c_int]
         L.cmm_euler_profile_read.argtypes = [VP, DP, C.POINTER(C.c_int)]
+        L.cmm_map_to_split.argtypes = [Grid, VP, VP, VP, VP]
+        L.cmm_map_from_split.argtypes = [Grid, VP, VP, VP, VP]
+        L.cmm_hermite_to_split.argtypes = [Grid, VP, VP, VP]
+        L.cmm_map_advect_split_workspace_bytes.restype = C.c_size_t
+        L.cmm_map_advect_split_workspace_bytes.argtypes = [Grid, C.POINTER(AdvectCoeffs)]
+        L.cmm_map_advect_split.argtypes = [Grid, Grid, VP, C.POINTER(VP), C.POINTER(AdvectCoeffs), VP, VP, C.c_size_t, VP]
+        L.cmm_incomp_error_split.argtypes = [Grid, Grid, VP, DP, VP]
+        L.cmm_psi_from_vort_split.argtypes = [Grid, Grid, Grid, Grid, VP, VP, C.c_int, C.c_double, VP, VP, C.c_size_t,
+                                              VP]
         _lib = L
     return _lib
 
@@ -170,6 +184,13 @@ def _p(t):
         return None
     import torch
     assert t.is_cuda and t.dtype == torch.float64 and t.is_contiguous(), "expected contiguous float64 cuda tensor"
+    return C.c_void_p(t.data_ptr())
+
+
+def _pf(t):
+    """device pointer of a float32 torch tensor (split-precision records)"""
+    import torch
+    assert t.is_cuda and t.dtype == torch.float32 and t.is_contiguous(), "expected contiguous float32 cuda tensor"
     return C.c_void_p(t.data_ptr())
 
 
@@ -288,6 +309,42 @@ def evaluate_potential_from_density_hermite(g_map, g_vort, g_fine, g_psi, chix, 
                                    _p(psi_out), None, 0, _stream()), "cmm_vp_psi_from_f")
 
 
+# ---- split-precision form (no counterpart in the reference): map records of 8 floats per node
+# (X - x, dX/dx - 1, dX/dy, d2X/dxdy, Y - y, dY/dx, dY/dy - 1, d2Y/dxdy), FP32 Hermite node records
+
+def map_to_split(g, chix, chiy, m_split):
+    _check(lib().cmm_map_to_split(g, _p(chix), _p(chiy), _pf(m_split), _stream()), "cmm_map_to_split")
+
+
+def map_from_split(g, m_split, chix, chiy):
+    _check(lib().cmm_map_from_split(g, _pf(m_split), _p(chix), _p(chiy), _stream()), "cmm_map_from_split")
+
+
+def hermite_to_split(g, herm, h_split):
+    _check(lib().cmm_hermite_to_split(g, _p(herm), _pf(h_split), _stream()), "cmm_hermite_to_split")
+
+
+def advect_split(g_map, g_psi, m_in, psi_levels, coeffs, m_out):
+    import torch
+    wb = lib().cmm_map_advect_split_workspace_bytes(g_psi, C.byref(coeffs))
+    work = torch.empty(max(wb, 8), dtype=torch.uint8, device=m_in.device)
+    _check(lib().cmm_map_advect_split(g_map, g_psi, _pf(m_in), _levels(psi_levels), C.byref(coeffs), _pf(m_out),
+                                      C.c_void_p(work.data_ptr()), wb, _stream()), "cmm_map_advect_split")
+
+
+def incompressibility_check_split(g_check, g_map, m_split):
+    out = C.c_double()
+    _check(lib().cmm_incomp_error_split(g_map, g_check, _pf(m_split), C.byref(out), _stream()),
+           "cmm_incomp_error_split")
+    return out.value
+
+
+def evaluate_stream_hermite_split(g_map, g_vort, g_fine, g_psi, m_split, w_split, psi_out, molly_stencil=0,
+                                  freq_cut_psi=128.0):
+    _check(lib().cmm_psi_from_vort_split(g_map, g_vort, g_fine, g_psi, _pf(m_split), _pf(w_split), molly_stencil,
+                                         freq_cut_psi, _p(psi_out), None, 0, _stream()), "cmm_psi_from_vort_split")
+
+
 def vp_grid(n, nv=None):
     """(x, v) grid of the Vlasov-Poisson flow on [0, 4 pi) x [-2.5 pi, 2.5 pi), the reference's
     bounds {0, 4, -2.5, 2.5} times pi (Cuda-CMM/src/simulation/cmm-vlasov1d.cu:64-75)."""
@@ -312,7 +369,7 @@ class Euler:
                  l_order=None, lagrange_init_higher_order=True, map_epsilon=1e-3, incomp_threshold=1e-4,
                  freq_cut_psi=128.0, molly_stencil=0, skip_remapping=False, initial_condition="4_nodes",
                  ic_params=None, discrete_real=None, g_disc=None, seed=0, stack_capacity=64,
-                 simulation_type="cmm_euler_2d", forward_map=False):
+                 simulation_type="cmm_euler_2d", forward_map=False, precision="fp64"):
         import numpy as np
         tin = TIME_INTEGRATION[time_integration]
         if l_order is None:
@@ -341,6 +398,7 @@ class Euler:
         p.stack_capacity = stack_capacity
         p.sim = SIM[simulation_type]
         p.forward_map = int(forward_map)
+        p.precision = PRECISION[precision]
         self.params = p
         self.l_order = l_order
         h = C.c_void_p()

@@ -257,6 +257,55 @@ int cmm_particles_advect(cmm_grid gp, const double *const *psi, int L, const dou
     return op_particles(gp, levels(psi, L), L, Lp, dt, scheme, n, pos, S(stream));
 }
 
+int cmm_map_to_split(cmm_grid g, const double *cx, const double *cy, float *M, void *stream) {
+    if (!grid_ok(g) || !cx || !cy || !M) { set_error("cmm_map_to_split: bad argument"); return CMM_ERR_ARG; }
+    return op_map_to32(g, cx, cy, M, S(stream));
+}
+
+int cmm_map_from_split(cmm_grid g, const float *M, double *cx, double *cy, void *stream) {
+    if (!grid_ok(g) || !cx || !cy || !M) { set_error("cmm_map_from_split: bad argument"); return CMM_ERR_ARG; }
+    return op_map_from32(g, M, cx, cy, S(stream));
+}
+
+int cmm_hermite_to_split(cmm_grid g, const double *H, float *H4, void *stream) {
+    if (!grid_ok(g) || !H || !H4) { set_error("cmm_hermite_to_split: bad argument"); return CMM_ERR_ARG; }
+    return op_interleave4_32(g, H, H4, S(stream));
+}
+
+size_t cmm_map_advect_split_workspace_bytes(cmm_grid gp, const cmm_advect_coeffs *c) {
+    if (!grid_ok(gp) || !c) return 0;
+    return op_advect32_workspace(gp, *c);
+}
+
+int cmm_map_advect_split(cmm_grid gm, cmm_grid gp, const float *M_in, const double *const *psi,
+                         const cmm_advect_coeffs *c, float *M_out, void *work, size_t work_bytes, void *stream) {
+    if (!grid_ok(gm) || !grid_ok(gp) || !M_in || !M_out || !psi || !c || M_in == M_out) {
+        set_error("cmm_map_advect_split: bad argument");
+        return CMM_ERR_ARG;
+    }
+    if (c->l_order < 2 || c->l_order > 4 || c->map_update_order_num < 0 || c->map_update_order_num > 2) {
+        set_error("cmm_map_advect_split: bad coefficients");
+        return CMM_ERR_ARG;
+    }
+    for (int l = 0; l < c->l_order; ++l)
+        if (!psi[l]) { set_error("cmm_map_advect_split: null psi level"); return CMM_ERR_ARG; }
+    return op_map_advect32(gm, gp, levels(psi, c->l_order), *c, M_in, M_out, work, work_bytes, S(stream));
+}
+
+int cmm_incomp_error_split(cmm_grid gm, cmm_grid gc, const float *M, double *out, void *stream) {
+    if (!grid_ok(gm) || !grid_ok(gc) || !M || !out) { set_error("cmm_incomp_error_split: bad argument"); return CMM_ERR_ARG; }
+    return op_incomp32(gm, gc, M, out, S(stream));
+}
+
+int cmm_psi_from_vort_split(cmm_grid gm, cmm_grid gv, cmm_grid gf, cmm_grid gp, const float *M, const float *W4,
+                            int molly, double fcut, double *psi, void *work, size_t wb, void *stream) {
+    if (!grid_ok(gm) || !grid_ok(gf) || !M || !W4 || !psi || (molly != 0 && molly != 4 && molly != 8)) {
+        set_error("cmm_psi_from_vort_split: bad argument");
+        return CMM_ERR_ARG;
+    }
+    return op_psi_from_vort32(gm, gv, gf, gp, M, W4, molly, fcut, psi, work, wb, S(stream));
+}
+
 }  // extern "C"
 
 // ================================================================================================
@@ -276,6 +325,13 @@ struct cmm_euler {
     size_t sbuf_n = 0;
     int m8_cur = 0;
     bool m8_valid = false;
+    // split-precision form: the map as displacement records (12 floats per node; ping-pong) and the fine
+    // vorticity as FP32 node records.  cx/cy and m32[m32_cur] hold the same map when both are
+    // fresh; a consumer brings the form it reads up to date, a writer marks the other stale.
+    float *m32[2] = {nullptr, nullptr};
+    float *W4f = nullptr;
+    int m32_cur = 0;
+    bool m32_fresh = false, planes_fresh = true;
     int nstack = 0;
     // forward map (forward_map only): current, next, and its stack, pushed with the backward one
     double *fx = nullptr, *fy = nullptr, *nfx = nullptr, *nfy = nullptr, *fstack = nullptr;
@@ -319,25 +375,65 @@ int dalloc(double **p, size_t n) {
     return CMM_OK;
 }
 
+int falloc(float **p, size_t n) {
+    if (hipMalloc(p, sizeof(float) * n) != hipSuccess) {
+        *p = nullptr;
+        set_error("hipMalloc failed (" + std::to_string(sizeof(float) * n) + " bytes)");
+        return CMM_ERR_ALLOC;
+    }
+    return CMM_OK;
+}
+
+bool split(const cmm_euler *e) { return e->p.precision == CMM_PRECISION_SPLIT; }
+
+int sync_planes(cmm_euler *e) {
+    if (e->planes_fresh) return CMM_OK;
+    if (int rc = op_map_from32(e->p.g_coarse, e->m32[e->m32_cur], e->cx, e->cy, e->st)) return rc;
+    e->planes_fresh = true;
+    return CMM_OK;
+}
+
+int sync_m32(cmm_euler *e) {
+    if (e->m32_fresh) return CMM_OK;
+    if (int rc = op_map_to32(e->p.g_coarse, e->cx, e->cy, e->m32[e->m32_cur], e->st)) return rc;
+    e->m32_fresh = true;
+    return CMM_OK;
+}
+
+// after a write to cx/cy (identity, restart, an FP64 advection)
+void planes_written(cmm_euler *e) {
+    e->planes_fresh = true;
+    e->m32_fresh = false;
+    e->m8_valid = false;
+}
 
 int engine_translate(cmm_euler *e) {
+    if (int rc = sync_planes(e)) return rc;
     if (int rc = op_translate_ic(e->p.g_coarse, e->p.g_fine, e->cx, e->cy, e->stack, e->nstack, e->p.ic,
                                  e->p.ic_params, e->p.discrete, e->p.discrete ? e->p.g_disc : e->p.g_fine, e->disc,
                                  e->W, e->work, e->work_bytes, e->st))
         return rc;
-    // the psi sampler reads the fine vorticity node-interleaved (one 32-byte record per corner)
+    // the psi sampler reads the fine vorticity node-interleaved (one 32-byte record per corner,
+    // 16 bytes in the split form)
+    if (e->W4f) return op_interleave4_32(e->p.g_fine, e->W, e->W4f, e->st);
     return e->W4 ? op_interleave4(e->p.g_fine, e->W, e->W4, e->st) : CMM_OK;
 }
 
 int engine_psi(cmm_euler *e, double *out, hipEvent_t *ev = nullptr) {
     if (e->p.sim == CMM_SIM_VLASOV_POISSON_1D) {
         // one operator; its whole time goes to the first psi profile slot
+        if (int rc = sync_planes(e)) return rc;
         if (ev) (void)hipEventRecord(ev[0], e->st);
         int rc = op_vp_psi_from_f(e->p.g_coarse, e->p.g_vort, e->p.g_fine, e->p.g_psi, e->cx, e->cy, e->W,
                                   e->p.molly_stencil, out, e->work, e->work_bytes, e->st);
         if (ev)
             for (int i = 1; i < 4; ++i) (void)hipEventRecord(ev[i], e->st);
         return rc;
+    }
+    if (split(e)) {
+        if (int rc = sync_m32(e)) return rc;
+        return op_psi_from_vort32(e->p.g_coarse, e->p.g_vort, e->p.g_fine, e->p.g_psi, e->m32[e->m32_cur], e->W4f,
+                                  e->p.molly_stencil, e->p.freq_cut_psi, out, e->work, e->work_bytes, e->st, ev);
     }
     return op_psi_from_vort(e->p.g_coarse, e->p.g_vort, e->p.g_fine, e->p.g_psi, e->cx, e->cy, e->W,
                             e->p.molly_stencil, e->p.freq_cut_psi, out, e->work, e->work_bytes, e->st, ev, e->W4);
@@ -349,6 +445,19 @@ int engine_advect(cmm_euler *e, const double *t, const double *dt, int loop_ctr,
         return rc;
     PsiLevels P;
     for (int l = 0; l < 4; ++l) P.p[l] = e->lvl[l < L ? l : 0];
+    if (split(e) && op_advect32_supported(c)) {
+        if (int rc = sync_m32(e)) return rc;
+        if (int rc = op_map_advect32(e->p.g_coarse, e->p.g_psi, P, c, e->m32[e->m32_cur], e->m32[1 - e->m32_cur],
+                                     e->work, e->work_bytes, e->st, e->prof ? e->ev[10] : nullptr))
+            return rc;
+        e->m32_cur = 1 - e->m32_cur;
+        e->m32_fresh = true;
+        e->planes_fresh = false;
+        return CMM_OK;
+    }
+    // the split form runs the schemes that read psi levels on their own (and the one-level
+    // warm-up steps) on the FP64 planes
+    if (int rc = sync_planes(e)) return rc;
     // the workspace is shared with the psi solve, which runs later on the same stream
     // event 10 separates the psi combination from the advection kernel in the profile
     // m8: the map node-interleaved, written by the previous advection (valid until the map is
@@ -363,6 +472,7 @@ int engine_advect(cmm_euler *e, const double *t, const double *dt, int loop_ctr,
     std::swap(e->cy, e->ncy);
     e->m8_valid = comb;
     if (comb) e->m8_cur = 1 - e->m8_cur;
+    e->m32_fresh = false;
     return CMM_OK;
 }
 
@@ -381,8 +491,15 @@ int engine_advect_forward(cmm_euler *e, const double *t, const double *dt, int l
     return CMM_OK;
 }
 
+// the check on whichever form of the map is current
+int launch_incomp(cmm_euler *e) {
+    if (split(e) && e->m32_fresh) return op_incomp32_launch(e->p.g_coarse, e->p.g_fine, e->m32[e->m32_cur], e->slot, e->st);
+    if (int rc = sync_planes(e)) return rc;
+    return op_incomp_launch(e->p.g_coarse, e->p.g_fine, e->cx, e->cy, e->slot, e->st);
+}
+
 int engine_incomp(cmm_euler *e, double *out) {
-    if (int rc = op_incomp_launch(e->p.g_coarse, e->p.g_fine, e->cx, e->cy, e->slot, e->st)) return rc;
+    if (int rc = launch_incomp(e)) return rc;
     unsigned long long h = 0;
     if (int rc = hip_check(hipMemcpyAsync(&h, e->slot, sizeof(h), hipMemcpyDeviceToHost, e->st), "hipMemcpyAsync")) return rc;
     if (int rc = hip_check(hipStreamSynchronize(e->st), "hipStreamSynchronize")) return rc;
@@ -503,6 +620,14 @@ int cmm_euler_create(const cmm_euler_params *p, cmm_euler **out) {
         set_error("cmm_euler_create: bad time-stepping parameters");
         return CMM_ERR_ARG;
     }
+    if (p->precision != CMM_PRECISION_FP64 && p->precision != CMM_PRECISION_SPLIT) {
+        set_error("cmm_euler_create: precision must be CMM_PRECISION_FP64 or CMM_PRECISION_SPLIT");
+        return CMM_ERR_ARG;
+    }
+    if (p->precision == CMM_PRECISION_SPLIT && (p->sim != CMM_SIM_EULER_2D || p->forward_map)) {
+        set_error("cmm_euler_create: the split-precision form runs the Euler flow with the backward map only");
+        return CMM_ERR_ARG;
+    }
     if (p->forward_map != 0 && p->forward_map != 1) {
         set_error("cmm_euler_create: forward_map must be 0 or 1");
         return CMM_ERR_ARG;
@@ -526,9 +651,13 @@ int cmm_euler_create(const cmm_euler_params *p, cmm_euler **out) {
         e->lvl[l] = e->psi_buf[l];
     }
     if (!rc) rc = dalloc(&e->W, 4 * e->nf);
-    if (!rc && p->l_order >= 2) rc = dalloc(&e->m8[0], 8 * e->nc);
-    if (!rc && p->l_order >= 2) rc = dalloc(&e->m8[1], 8 * e->nc);
-    if (!rc && p->sim == CMM_SIM_EULER_2D) rc = dalloc(&e->W4, 4 * e->nf);
+    const bool sp = p->precision == CMM_PRECISION_SPLIT;
+    if (!rc && p->l_order >= 2 && !sp) rc = dalloc(&e->m8[0], 8 * e->nc);
+    if (!rc && p->l_order >= 2 && !sp) rc = dalloc(&e->m8[1], 8 * e->nc);
+    if (!rc && p->sim == CMM_SIM_EULER_2D && !sp) rc = dalloc(&e->W4, 4 * e->nf);
+    if (!rc && sp) rc = falloc(&e->m32[0], 12 * e->nc);
+    if (!rc && sp) rc = falloc(&e->m32[1], 12 * e->nc);
+    if (!rc && sp) rc = falloc(&e->W4f, 4 * e->nf);
     if (!rc && p->stack_capacity > 0) rc = dalloc(&e->stack, (size_t)8 * e->nc * p->stack_capacity);
     if (p->forward_map) {
         if (!rc) rc = dalloc(&e->fx, 4 * e->nc);
@@ -577,6 +706,8 @@ int cmm_euler_destroy(cmm_euler *e) {
         if (b) (void)hipFree(b);
     for (double *b : e->part)
         if (b) (void)hipFree(b);
+    for (float *b : {e->m32[0], e->m32[1], e->W4f})
+        if (b) (void)hipFree(b);
     if (e->work) (void)hipFree(e->work);
     if (e->slot) (void)hipFree(e->slot);
     if (e->st2) (void)hipStreamSynchronize(e->st2);
@@ -622,7 +753,7 @@ int cmm_euler_init(cmm_euler *e, double *warm, int *nwarm) {
     if (int rc = op_map_identity(p.g_coarse, e->cx, e->cy, e->st)) return rc;
     if (p.forward_map)
         if (int rc = op_map_identity(p.g_coarse, e->fx, e->fy, e->st)) return rc;
-    e->m8_valid = false;
+    planes_written(e);
     e->nstack = 0;
     e->force_remap = false;
     if (int rc = engine_translate(e)) return rc;
@@ -657,7 +788,7 @@ int cmm_euler_init(cmm_euler *e, double *warm, int *nwarm) {
             std::swap(e->lvl[0], e->lvl[nsteps]);
             if (nsteps >= 3) std::swap(e->lvl[1], e->lvl[2]);
             if (int rc = op_map_identity(p.g_coarse, e->cx, e->cy, e->st)) return rc;
-            e->m8_valid = false;
+            planes_written(e);
         }
     }
     if (nwarm) *nwarm = std::min(nw, 8);
@@ -693,11 +824,12 @@ int cmm_euler_restart(cmm_euler *e, double t_restart, int n_maps, const double *
     e->nstack = n_maps;
     if (!rc) rc = op_map_identity(p.g_coarse, e->cx, e->cy, e->st);
     if (!rc && p.forward_map) rc = op_map_identity(p.g_coarse, e->fx, e->fy, e->st);
-    e->m8_valid = false;
+    planes_written(e);
     e->force_remap = false;
     if (!rc) rc = engine_translate(e);
     if (!rc) rc = put(e->cx, chix_host, (size_t)4 * e->nc);
     if (!rc) rc = put(e->cy, chiy_host, (size_t)4 * e->nc);
+    planes_written(e);
     for (int l = 1; l < L && !rc; ++l) rc = put(e->lvl[l], psi_old_host + (size_t)4 * e->np * (l - 1), (size_t)4 * e->np);
     // the newest psi from the loaded state (Cuda-CMM/src/simulation/cmm-euler2d.cu:664-666); no warm-up
     if (!rc) rc = engine_psi(e, e->lvl[0]);
@@ -785,7 +917,7 @@ int cmm_euler_step(cmm_euler *e, double dt_now, cmm_step_info *info) {
     prof_mark(e, 1);
     // incompressibility on the fine grid, :796: the check and the copy of E_inc to pinned host
     // memory are queued, and the host waits for that copy only after the psi solve is queued too
-    if (int rc = op_incomp_launch(p.g_coarse, p.g_fine, e->cx, e->cy, e->slot, e->st)) return rc;
+    if (int rc = launch_incomp(e)) return rc;
     // forward map: invertibility and E_inc on the coarse grid, :798-803
     if (p.forward_map) {
         if (int rc = op_invert_launch(p.g_coarse, p.g_coarse, p.g_coarse, e->cx, e->cy, e->fx, e->fy, e->slot + 2, e->st))
@@ -836,7 +968,7 @@ int cmm_euler_step(cmm_euler *e, double dt_now, cmm_step_info *info) {
             }
             e->nstack++;
             if (int rc = op_map_identity(p.g_coarse, e->cx, e->cy, e->st)) return rc;
-            e->m8_valid = false;
+            planes_written(e);
             if (int rc = engine_psi(e, e->lvl[0])) return rc;
             prof_mark(e, 4);
             remapped = 1;
@@ -888,7 +1020,8 @@ int cmm_euler_step(cmm_euler *e, double dt_now, cmm_step_info *info) {
 
 int cmm_euler_get_field(cmm_euler *e, int field, double *out) {
     if (!e || !out) { set_error("cmm_euler_get_field: bad argument"); return CMM_ERR_ARG; }
-    int rc = CMM_OK;
+    int rc = sync_planes(e);
+    if (rc) return rc;
     switch (field) {
     case CMM_FIELD_CHIX: rc = hip_check(hipMemcpyAsync(out, e->cx, sizeof(double) * 4 * e->nc, hipMemcpyDeviceToHost, e->st), "get"); break;
     case CMM_FIELD_CHIY: rc = hip_check(hipMemcpyAsync(out, e->cy, sizeof(double) * 4 * e->nc, hipMemcpyDeviceToHost, e->st), "get"); break;
@@ -929,7 +1062,7 @@ int cmm_euler_get_particles(cmm_euler *e, int set, double *out) {
 int cmm_euler_diagnostics(cmm_euler *e, double *out) {
     if (!e || !out) { set_error("cmm_euler_diagnostics: bad argument"); return CMM_ERR_ARG; }
     const cmm_grid &gc = e->p.g_coarse, &gp = e->p.g_psi;
-    int rc = CMM_OK;
+    int rc = sync_planes(e);
     if (!e->tmp) rc = dalloc(&e->tmp, e->nc);
     if (!rc && !e->tmp4) rc = dalloc(&e->tmp4, 4 * e->nc);
     // coarse vorticity and its spectral gradient (Hermite planes 1, 2 are grad_x, grad_y of
@@ -965,7 +1098,7 @@ int cmm_euler_diagnostics(cmm_euler *e, double *out) {
 int cmm_euler_vp_diagnostics(cmm_euler *e, double *out) {
     if (!e || !out) { set_error("cmm_euler_vp_diagnostics: bad argument"); return CMM_ERR_ARG; }
     const cmm_grid &gc = e->p.g_coarse, &gp = e->p.g_psi;
-    int rc = CMM_OK;
+    int rc = sync_planes(e);
     if (!e->tmp) rc = dalloc(&e->tmp, e->nc);
     if (!rc) rc = op_sample_vort(gc, gc, e->p.g_fine, e->cx, e->cy, e->W, 0, e->tmp, e->st);
     std::vector<double> f(e->nc), ex(gp.nx);
@@ -1019,6 +1152,7 @@ int cmm_euler_sample(cmm_euler *e, cmm_grid gs, int scalar_ic, double *out) {
         if (int rc = dalloc(&e->sbuf, 8 * n)) return rc;
         e->sbuf_n = n;
     }
+    if (int rc = sync_planes(e)) return rc;
     double *xy = e->sbuf, *w = xy + 2 * n, *th = w + n, *h = th + n;
     std::vector<double> hxy(2 * n);
     auto get = [&](double *dst, const double *src, size_t cnt) {
@@ -1065,6 +1199,7 @@ int cmm_euler_zoom(cmm_euler *e, cmm_grid gz, const int *ics, int n_ics, int wan
         if (int rc = dalloc(&e->sbuf, 8 * n)) return rc;
         e->sbuf_n = n;
     }
+    if (int rc = sync_planes(e)) return rc;
     double *xy = e->sbuf, *f = xy + 2 * n;
     std::vector<double> hxy(2 * n);
     auto get = [&](double *dst, const double *src, size_t cnt) {

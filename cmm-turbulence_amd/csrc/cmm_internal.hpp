@@ -66,4 +66,24 @@ int op_sample_hermite(const cmm_grid &gh, const double *H, const cmm_grid &gt, d
 int op_particles(const cmm_grid &gp, const PsiLevels &P, int L, const double *Lp, double dt, int scheme, int n,
                  double *pos, hipStream_t st);
 
+// Split-precision form: the map as identity + FP32 displacement records (M[8j + p]: planes of
+// X - x, then of Y - y, identity derivatives removed), FP32 Hermite records for the gathered
+// fields, FP64 arithmetic, FP64 psi levels and FFTs.
+int op_map_to32(const cmm_grid &g, const double *cx, const double *cy, float *M, hipStream_t st);
+int op_map_from32(const cmm_grid &g, const float *M, double *cx, double *cy, hipStream_t st);
+int op_interleave4_32(const cmm_grid &g, const double *H, float *H4, hipStream_t st);
+bool op_advect32_supported(const cmm_advect_coeffs &c);
+size_t op_advect32_workspace(const cmm_grid &gp, const cmm_advect_coeffs &c);
+int op_map_advect32(const cmm_grid &gm, const cmm_grid &gp, const PsiLevels &P, const cmm_advect_coeffs &c,
+                    const float *M_in, float *M_out, void *work, size_t work_bytes, hipStream_t st,
+                    hipEvent_t mid = nullptr);
+int op_incomp32_launch(const cmm_grid &gm, const cmm_grid &gc, const float *M, unsigned long long *slot,
+                       hipStream_t st);
+int op_incomp32(const cmm_grid &gm, const cmm_grid &gc, const float *M, double *out, hipStream_t st);
+int op_psi_from_vort32(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf, const cmm_grid &gp, const float *M,
+                       const float *W4, int molly, double fcut, double *psi, void *work, size_t work_bytes,
+                       hipStream_t st, hipEvent_t *ev = nullptr);
+int op_sample_vort32(const cmm_grid &gm, const cmm_grid &gt, const cmm_grid &gf, const float *M, const float *W4,
+                     int molly, double *out, hipStream_t st);
+
 }  // namespace cmm

@@ -1966,4 +1966,549 @@ int op_particles(const cmm_grid &gp, const PsiLevels &P, int L, const double *Lp
     return launch_check("k_particles");
 }
 
+// ================================================================================================
+// Split-precision form.
+//
+// The map is stored as the identity plus a displacement D = X - identity, one 48-byte record of
+// three float4 per node:
+//   M[3j]     = (Dx hi, Dx lo, Dy hi, Dy lo)        value planes as FP32 pairs (hi + lo, ~48 bits)
+//   M[3j + 1] = (dX/dx - 1, dX/dy, dY/dx, dY/dy - 1)
+//   M[3j + 2] = (d2X/dxdy, d2Y/dxdy, 0, 0)
+// with physical derivatives and no wrap jumps (the displacement is periodic).  The value planes
+// keep a low part because they reach the Jacobian through the basis derivative (1/h times their
+// rounding) and the advection carries them from step to step; the derivative planes enter both
+// with weight h and keep one FP32 rounding.  The Hermite fields the gathers read (the
+// Lagrange-combined psi fields, the fine vorticity) are FP32 records (f, fx, fy, fxy), one
+// 16-byte load per corner.  The arithmetic stays FP64: values are widened on load, the identity
+// is added back exactly (X(p) = p + D(p), since bicubic Hermite reproduces the identity), the
+// footpoint integration and the stencil sums accumulate in FP64, and the Jacobian check forms
+// 1 - det from the displacement derivatives, -(a + d + a d - b c), without the cancellation of
+// 1 - det.  The psi levels, the FFTs and the Poisson divide stay FP64.
+
+__device__ __forceinline__ void load16_f4(const float4 *__restrict__ H, const Cell &c, double v[16]) {
+    const int idx[4] = {c.i00, c.i10, c.i01, c.i11};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const float4 a = H[idx[q]];
+        v[q] = a.x;
+        v[4 + q] = a.y;
+        v[8 + q] = a.z;
+        v[12 + q] = a.w;
+    }
+}
+
+// corner values of the split map; LO: add the low parts of the value planes
+template <bool LO = true>
+__device__ __forceinline__ void load_map32(const float4 *__restrict__ M, const Cell &c, double vx[16], double vy[16]) {
+    const int idx[4] = {c.i00, c.i10, c.i01, c.i11};
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const float4 *r = M + 3 * (long)idx[q];
+        const float4 a = r[0], b = r[1], d = r[2];
+        vx[q] = LO ? (double)a.x + (double)a.y : (double)a.x;
+        vy[q] = LO ? (double)a.z + (double)a.w : (double)a.z;
+        vx[4 + q] = b.x;
+        vx[8 + q] = b.y;
+        vy[4 + q] = b.z;
+        vy[8 + q] = b.w;
+        vx[12 + q] = d.x;
+        vy[12 + q] = d.y;
+    }
+}
+
+// X(p) = p + D(p) with the corner displacements of p's cell given
+__device__ __forceinline__ void diffeo32_vals(const DGrid &g, const Cell &c, const double vx[16], const double vy[16],
+                                              double x, double y, double &xo, double &yo) {
+    double bx[4], by[4];
+    basis(c.dx, bx);
+    basis(c.dy, by);
+    xo = x + hsum_v(vx, bx, by, g.hx, g.hy);
+    yo = y + hsum_v(vy, bx, by, g.hx, g.hy);
+}
+
+template <bool P2 = false, bool LO = true>
+__device__ __forceinline__ void diffeo32(const DGrid &g, const float4 *__restrict__ M, double x, double y, double &xo,
+                                         double &yo) {
+    Cell c;
+    find_cell<P2>(g, x, y, c);
+    double vx[16], vy[16];
+    load_map32<LO>(M, c, vx, vy);
+    diffeo32_vals(g, c, vx, vy, x, y, xo, yo);
+}
+
+// |1 - det grad X| at (x, y) from the displacement derivatives (the derivative scalings of diffeo_det)
+__device__ __forceinline__ double incomp32_point(const DGrid &g, const float4 *__restrict__ M, double x, double y) {
+    Cell c;
+    find_cell(g, x, y, c);
+    double bx[4], by[4], dbx[4], dby[4], vx[16], vy[16];
+    basis(c.dx, bx);
+    basis(c.dy, by);
+    basis_d(c.dx, dbx);
+    basis_d(c.dy, dby);
+    load_map32(M, c, vx, vy);
+    const double a = hsum_v(vx, dbx, by, g.hx, g.hy) * g.ihx;
+    const double cc = hsum_v(vy, dbx, by, g.hx, g.hy) * g.ihy;
+    const double b = hsum_v(vx, bx, dby, g.hx, g.hy) * g.ihx;
+    const double d = hsum_v(vy, bx, dby, g.hx, g.hy) * g.ihy;
+    return fabs(a + d + (a * d - b * cc));
+}
+
+// velocity sources of the footpoint integration on FP32 combined fields (see VelCombined)
+template <bool P2>
+struct VelCombined32 {
+    DGrid g;
+    const float4 *C1, *C12, *C0;
+    __device__ __forceinline__ void ev(const float4 *__restrict__ H, double x, double y, double &ux, double &uy) const {
+        Cell c;
+        find_cell<P2>(g, x, y, c);
+        double v[16];
+        load16_f4(H, c, v);
+        herm_vel_vals(g, c, v, ux, uy);
+    }
+    __device__ __forceinline__ void w1(const cmm_advect_coeffs &, double x, double y, double &kx, double &ky) const {
+        ev(C1, x, y, kx, ky);
+    }
+    __device__ __forceinline__ void w12(const cmm_advect_coeffs &, double x, double y, double &kx, double &ky) const {
+        ev(C12, x, y, kx, ky);
+    }
+    __device__ __forceinline__ void lev(int, double x, double y, double &ux, double &uy) const { ev(C0, x, y, ux, uy); }
+};
+
+// k_psi_combine with FP32 records: the sums in FP64, rounded once
+template <int L>
+__global__ void __launch_bounds__(256) k_psi_combine32(long n, PsiLevels P, cmm_advect_coeffs c, float4 *__restrict__ C1,
+                                                       float4 *__restrict__ C12, float4 *__restrict__ C0) {
+    const long j = (long)blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    double v[L][4];
+#pragma unroll
+    for (int l = 0; l < L; ++l)
+#pragma unroll
+        for (int p = 0; p < 4; ++p) v[l][p] = P.p[l][p * n + j];
+    auto comb = [&](const double *w, float4 *dst) {
+        double o[4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            double a = w[0] * v[0][p];
+#pragma unroll
+            for (int l = 1; l < L; ++l) a += w[l] * v[l][p];
+            o[p] = a;
+        }
+        dst[j] = float4{(float)o[0], (float)o[1], (float)o[2], (float)o[3]};
+    };
+    comb(c.L1, C1);
+    if (C12) comb(c.L12, C12);
+    if (C0) C0[j] = float4{(float)v[0][0], (float)v[0][1], (float)v[0][2], (float)v[0][3]};
+}
+
+// Output stage of the split advection: lane k of a node's group forms displacement plane k (and
+// k + G) from the group's footpoint displacements q_m = X(footpoint m) - node (FP64), in footpoint
+// order with the stencil weights.  The stencil sums of the derivative planes annihilate the node
+// coordinate (their weights sum to zero) and the value weights sum to one, so these are the
+// planes of X - identity; the identity's unit derivatives are removed before rounding to FP32.
+// record position of plane o (0-3: X planes, 4-7: Y planes) in the 12 floats of a node
+__device__ __forceinline__ int split_slot(int o) {
+    const int p = o & 3, y = o >> 2;
+    return p == 0 ? 2 * y : p == 3 ? 8 + y : 4 + 2 * y + (p - 1);
+}
+
+template <int G>
+__device__ __forceinline__ void advect_store32(const cmm_advect_coeffs &c, int nfoot, int node, bool active, int k,
+                                               double qx, double qy, float *__restrict__ Mout) {
+    const int base = (threadIdx.x & 63) - k;
+#pragma unroll
+    for (int r = 0; r < (8 + G - 1) / G; ++r) {
+        const int o = k + r * G;
+        const int plane = o & 3;
+        const double *w = plane == 0 ? c.c1 : plane == 1 ? c.cx : plane == 2 ? c.cy : c.cxy;
+        double acc = 0.0;
+        for (int m = 0; m < nfoot; ++m) {
+            const double vx = __shfl(qx, base + m, 64);
+            const double vy = __shfl(qy, base + m, 64);
+            acc += (o < 4 ? vx : vy) * w[m];
+        }
+        if (o == 1 || o == 6) acc -= 1.0;
+        if (active && o < 8) {
+            const float hi = (float)acc;
+            float *rec = Mout + 12L * node;
+            rec[split_slot(o)] = hi;
+            if (plane == 0) rec[split_slot(o) + 1] = (float)(acc - (double)hi);
+            else if (plane == 3) rec[10 + (o >> 2)] = 0.0f;
+        }
+    }
+}
+
+template <int TIN, int G, bool P2>
+__global__ void __launch_bounds__(256) k_map_advect32(DGrid gm, DGrid gp, const float4 *C1, const float4 *C12,
+                                                      const float4 *C0, const float4 *__restrict__ M,
+                                                      cmm_advect_coeffs c, float *__restrict__ Mout, int nblocks) {
+    constexpr int NPB = 256 / G;
+    const int blk = xcd_block(blockIdx.x, nblocks);
+    const int node = blk * NPB + threadIdx.x / G;
+    const int k = threadIdx.x % G;
+    const int nfoot = 4 + 4 * c.map_update_order_num;
+    const bool active = node < gm.n;
+    double qx = 0.0, qy = 0.0;
+    if (active && k < nfoot) {
+        const int ix = node % gm.nx, iy = node / gm.nx;
+        const double xn = gm.x0 + ix * gm.hx, yn = gm.y0 + iy * gm.hy;
+        const double xe = xn + ((k >> 2) + 1) * c.eps * (1 - 2 * ((k >> 1) & 1));
+        const double ye = yn + ((k >> 2) + 1) * c.eps * (1 - 2 * (((k + 1) >> 1) & 1));
+        double xf, yf, px, py;
+        step_backward<TIN>(VelCombined32<P2>{gp, C1, C12, C0}, c, xe, ye, xf, yf);
+        diffeo32<P2>(gm, M, xf, yf, px, py);
+        qx = px - xn;
+        qy = py - yn;
+    }
+    advect_store32<G>(c, nfoot, node, active, k, qx, qy, Mout);
+}
+
+// k_incomp_tensor on the split map: the same separable sums over the displacement records
+template <int TX, int TY>
+__global__ void __launch_bounds__(256) k_incomp_tensor32(DGrid gm, DGrid gc, const float4 *__restrict__ M,
+                                                         unsigned long long *__restrict__ out, int ntx, long ntiles) {
+    double v = 0.0;
+    for (long t0 = (long)blockIdx.x * 256; t0 < ntiles; t0 += (long)gridDim.x * 256) {
+        const long t = t0 + threadIdx.x;
+        if (t >= ntiles) continue;
+        const int ix0 = (int)(t % ntx) * TX, iy0 = (int)(t / ntx) * TY;
+        double xs[TX], ys[TY];
+#pragma unroll
+        for (int a = 0; a < TX; ++a) xs[a] = gm.x0 + (ix0 + a) * gc.hx + 0.5 * gc.hx;
+#pragma unroll
+        for (int b = 0; b < TY; ++b) ys[b] = gm.y0 + (iy0 + b) * gc.hy + 0.5 * gc.hy;
+        Cell c0, c1;
+        find_cell(gm, xs[0], ys[0], c0);
+        find_cell(gm, xs[TX - 1], ys[TY - 1], c1);
+        if (!same_cell(c0, c1)) {
+#pragma unroll 1
+            for (int i = 0; i < TX * TY; ++i) v = fmax(v, incomp32_point(gm, M, xs[i % TX], ys[i / TX]));
+            continue;
+        }
+        double vx[16], vy[16];
+        load_map32(M, c0, vx, vy);
+        double by[TY][4], dby[TY][4];
+#pragma unroll
+        for (int b = 0; b < TY; ++b) {
+            const double fy = (ys[b] - gm.y0) * gm.ihy;
+            const double d = fy - floor(fy);
+            basis(d, by[b]);
+            basis_d(d, dby[b]);
+        }
+#pragma unroll
+        for (int a = 0; a < TX; ++a) {
+            const double fx = (xs[a] - gm.x0) * gm.ihx;
+            const double d = fx - floor(fx);
+            double bx[4], dbx[4];
+            basis(d, bx);
+            basis_d(d, dbx);
+            double P[4][4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const double *w = (k & 1) ? bx : dbx;
+                const double *f = (k < 2) ? vx : vy;
+#pragma unroll
+                for (int n = 0; n < 2; ++n) {
+                    P[k][n] = w[0] * f[2 * n] + w[1] * f[2 * n + 1] + gm.hx * (w[2] * f[4 + 2 * n] + w[3] * f[5 + 2 * n]);
+                    P[k][2 + n] =
+                        w[0] * f[8 + 2 * n] + w[1] * f[9 + 2 * n] + gm.hx * (w[2] * f[12 + 2 * n] + w[3] * f[13 + 2 * n]);
+                }
+            }
+#pragma unroll
+            for (int b = 0; b < TY; ++b) {
+                auto ysum = [&](const double *p, const double *cw) {
+                    return cw[0] * p[0] + cw[1] * p[1] + gm.hy * (cw[2] * p[2] + cw[3] * p[3]);
+                };
+                const double ea = ysum(P[0], by[b]) * gm.ihx;
+                const double eb = ysum(P[1], dby[b]) * gm.ihx;
+                const double ec = ysum(P[2], by[b]) * gm.ihy;
+                const double ed = ysum(P[3], dby[b]) * gm.ihy;
+                v = fmax(v, fabs(ea + ed + (ea * ed - eb * ec)));
+            }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    __shared__ double red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+        atomicMax(out, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+// the check point by point (check grids that do not refine the map grid by an integer ratio)
+__global__ void __launch_bounds__(256) k_incomp32(DGrid gm, DGrid gc, const float4 *__restrict__ M,
+                                                  unsigned long long *__restrict__ out, int tiles_x, int tiles_total) {
+    double v = 0.0;
+    for (int tt = blockIdx.x; tt < tiles_total; tt += gridDim.x) {
+        const int t = xcd_block(tt, tiles_total);
+        const int ix = (t % tiles_x) * 64 + threadIdx.x;
+        const int iy = (t / tiles_x) * 4 + threadIdx.y;
+        if (ix < gc.nx && iy < gc.ny)
+            v = fmax(v, incomp32_point(gm, M, gm.x0 + ix * gc.hx + 0.5 * gc.hx, gm.y0 + iy * gc.hy + 0.5 * gc.hy));
+    }
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    __shared__ double red[4];
+    if (threadIdx.x == 0) red[threadIdx.y] = v;
+    __syncthreads();
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        double m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+        atomicMax(out, (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+// omega(X(x)) through the split map and the FP32 fine vorticity records; SampleVortT's mollifiers
+// and column sampling with the split evaluations.  The positions use the displacement without its
+// low parts: their error, 2^-24 |X - x|, moves the sampled vorticity by that times |grad omega|.
+struct SampleVort32 {
+    DGrid gm, gv, gf;
+    const float4 *__restrict__ M;
+    const float4 *__restrict__ W;
+    int molly;
+    __device__ __forceinline__ double wval(double x, double y) const {
+        Cell c;
+        find_cell(gf, x, y, c);
+        double v[16];
+        load16_f4(W, c, v);
+        return herm_eval_vals(gf, c, v);
+    }
+    __device__ double operator()(int ix, int iy) const {
+        const double x = gm.x0 + ix * gv.hx, y = gm.y0 + iy * gv.hy;
+        double x2, y2, v;
+        if (molly == 4) {
+            diffeo32<false, false>(gm, M, x, y, x2, y2);
+            v = wval(x2, y2) / 3.0;
+            for (int m = 0; m < 4; ++m) {
+                x2 = x + gv.hx / 2.0 * ((m >> 2) & 1) * (-1 + 2 * (m & 1));
+                y2 = y + gv.hy / 2.0 * ((m >> 1) & 1) * (-1 + 2 * (m & 1));
+                diffeo32<false, false>(gm, M, x2, y2, x2, y2);
+                v += wval(x2, y2) / 6.0;
+            }
+        } else if (molly == 8) {
+            v = 0;
+            for (int m = 0; m < 9; ++m) {
+                x2 = x + gv.hx * (-1 + m % 3) / 2.0;
+                y2 = y + gv.hy * (-1 + m / 3) / 2.0;
+                diffeo32<false, false>(gm, M, x2, y2, x2, y2);
+                v += ((1 + (m % 3)) & 1) * ((1 + (m / 3)) & 1) * wval(x2, y2) / 16.0;
+            }
+        } else {
+            diffeo32<false, false>(gm, M, x, y, x2, y2);
+            v = wval(x2, y2);
+        }
+        return v;
+    }
+    template <int R>
+    __device__ __forceinline__ void column(int x, int y0, double *out) const {
+        if (molly != 0) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) out[r] = (*this)(x, y0 + r);
+            return;
+        }
+        const double xp = gm.x0 + x * gv.hx;
+        Cell cf, cl;
+        find_cell(gm, xp, gm.y0 + y0 * gv.hy, cf);
+        find_cell(gm, xp, gm.y0 + (y0 + R - 1) * gv.hy, cl);
+        if (!same_cell(cf, cl)) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) out[r] = (*this)(x, y0 + r);
+            return;
+        }
+        double vx[16], vy[16];
+        load_map32<false>(M, cf, vx, vy);
+        Cell wc[R];
+        double wv[R][16];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            Cell c;
+            const double yp = gm.y0 + (y0 + r) * gv.hy;
+            find_cell(gm, xp, yp, c);
+            double px, py;
+            diffeo32_vals(gm, c, vx, vy, xp, yp, px, py);
+            find_cell(gf, px, py, wc[r]);
+            load16_f4(W, wc[r], wv[r]);
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) out[r] = herm_eval_vals(gf, wc[r], wv[r]);
+    }
+};
+
+// plane-major FP64 map <-> split records
+__global__ void __launch_bounds__(256) k_map_to32(DGrid g, const double *__restrict__ cx, const double *__restrict__ cy,
+                                                  float4 *__restrict__ M) {
+    const long j = (long)blockIdx.x * 256 + threadIdx.x;
+    if (j >= g.n) return;
+    const long n = g.n;
+    const double xn = g.x0 + (int)(j % g.nx) * g.hx, yn = g.y0 + (int)(j / g.nx) * g.hy;
+    const double dx = cx[j] - xn, dy = cy[j] - yn;
+    const float hx = (float)dx, hy = (float)dy;
+    M[3 * j] = float4{hx, (float)(dx - (double)hx), hy, (float)(dy - (double)hy)};
+    M[3 * j + 1] = float4{(float)(cx[n + j] - 1.0), (float)cx[2 * n + j], (float)cy[n + j], (float)(cy[2 * n + j] - 1.0)};
+    M[3 * j + 2] = float4{(float)cx[3 * n + j], (float)cy[3 * n + j], 0.0f, 0.0f};
+}
+
+__global__ void __launch_bounds__(256) k_map_from32(DGrid g, const float4 *__restrict__ M, double *__restrict__ cx,
+                                                    double *__restrict__ cy) {
+    const long j = (long)blockIdx.x * 256 + threadIdx.x;
+    if (j >= g.n) return;
+    const long n = g.n;
+    const double xn = g.x0 + (int)(j % g.nx) * g.hx, yn = g.y0 + (int)(j / g.nx) * g.hy;
+    const float4 a = M[3 * j], b = M[3 * j + 1], d = M[3 * j + 2];
+    cx[j] = xn + ((double)a.x + (double)a.y);
+    cx[n + j] = 1.0 + (double)b.x;
+    cx[2 * n + j] = b.y;
+    cx[3 * n + j] = d.x;
+    cy[j] = yn + ((double)a.z + (double)a.w);
+    cy[n + j] = b.z;
+    cy[2 * n + j] = 1.0 + (double)b.w;
+    cy[3 * n + j] = d.y;
+}
+
+// FP32 node records of a plane-major FP64 Hermite field
+__global__ void __launch_bounds__(256) k_interleave4_32(long n, const double *__restrict__ H, float4 *__restrict__ H4) {
+    const long j = (long)blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    H4[j] = float4{(float)H[j], (float)H[n + j], (float)H[2 * n + j], (float)H[3 * n + j]};
+}
+
+int op_map_to32(const cmm_grid &g, const double *cx, const double *cy, float *M, hipStream_t st) {
+    const DGrid d = make_dgrid(g);
+    hipLaunchKernelGGL(k_map_to32, dim3((unsigned)((d.n + 255) / 256)), dim3(256), 0, st, d, cx, cy, (float4 *)M);
+    return launch_check("k_map_to32");
+}
+
+int op_map_from32(const cmm_grid &g, const float *M, double *cx, double *cy, hipStream_t st) {
+    const DGrid d = make_dgrid(g);
+    hipLaunchKernelGGL(k_map_from32, dim3((unsigned)((d.n + 255) / 256)), dim3(256), 0, st, d, (const float4 *)M, cx, cy);
+    return launch_check("k_map_from32");
+}
+
+int op_interleave4_32(const cmm_grid &g, const double *H, float *H4, hipStream_t st) {
+    const long n = (long)g.nx * g.ny;
+    hipLaunchKernelGGL(k_interleave4_32, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, H, (float4 *)H4);
+    return launch_check("k_interleave4_32");
+}
+
+bool op_advect32_supported(const cmm_advect_coeffs &c) { return c.direction != 1 && advect_combinable(c); }
+
+size_t op_advect32_workspace(const cmm_grid &gp, const cmm_advect_coeffs &c) {
+    if (!op_advect32_supported(c)) return 0;
+    const int fields = 1 + advect_needs_c12(c) + advect_needs_c0(c);
+    return fields * sizeof(float4) * (size_t)gp.nx * gp.ny;
+}
+
+int op_map_advect32(const cmm_grid &gm, const cmm_grid &gp, const PsiLevels &P, const cmm_advect_coeffs &c,
+                    const float *M_in, float *M_out, void *work, size_t work_bytes, hipStream_t st, hipEvent_t mid) {
+    if (!op_advect32_supported(c)) {
+        set_error("split-precision advection: the time scheme reads psi levels on their own (AB2, RK3Mod), "
+                  "needs two or more psi levels, and is backward only");
+        return CMM_ERR_ARG;
+    }
+    const size_t need = op_advect32_workspace(gp, c);
+    void *w;
+    if (int rc = get_work(work, work_bytes, need, &w)) return rc;
+    DGrid dm = make_dgrid(gm), dp = make_dgrid(gp);
+    const long n = dp.n;
+    float4 *next = static_cast<float4 *>(w);
+    float4 *C1 = next, *C12 = nullptr, *C0 = nullptr;
+    next += n;
+    if (advect_needs_c12(c)) { C12 = next; next += n; }
+    if (advect_needs_c0(c)) C0 = next;
+    const dim3 g((unsigned)((n + 255) / 256));
+    switch (c.l_order) {
+    case 2: hipLaunchKernelGGL(k_psi_combine32<2>, g, dim3(256), 0, st, n, P, c, C1, C12, C0); break;
+    case 3: hipLaunchKernelGGL(k_psi_combine32<3>, g, dim3(256), 0, st, n, P, c, C1, C12, C0); break;
+    default: hipLaunchKernelGGL(k_psi_combine32<4>, g, dim3(256), 0, st, n, P, c, C1, C12, C0); break;
+    }
+    if (int rc = launch_check("k_psi_combine32")) return rc;
+    if (mid) (void)hipEventRecord(mid, st);
+    const int nfoot = 4 + 4 * c.map_update_order_num;
+    const int G = nfoot <= 4 ? 4 : nfoot <= 8 ? 8 : 16;
+    const int nb = (int)(((long)dm.n * G + 255) / 256);
+    const bool p2 = dm.lgx >= 0 && dm.lgy >= 0 && dp.lgx >= 0 && dp.lgy >= 0;
+    const float4 *Mi = (const float4 *)M_in;
+#define CMM_A32(T, GG, PP)                                                                                             \
+    hipLaunchKernelGGL((k_map_advect32<T, GG, PP>), dim3(nb), dim3(256), 0, st, dm, dp, C1, C12, C0, Mi, c, M_out, nb)
+#define CMM_A32_T(T)                                                                                                   \
+    do {                                                                                                               \
+        if (G == 8 && p2) CMM_A32(T, 8, true);                                                                         \
+        else if (G == 4) CMM_A32(T, 4, false);                                                                         \
+        else if (G == 8) CMM_A32(T, 8, false);                                                                         \
+        else CMM_A32(T, 16, false);                                                                                    \
+    } while (0)
+    switch (c.time_integration_num) {
+    case 10: CMM_A32_T(10); break;
+    case 21: CMM_A32_T(21); break;
+    case 30: CMM_A32_T(30); break;
+    case 40: CMM_A32_T(40); break;
+    case 41: CMM_A32_T(41); break;
+    default: set_error("unknown time integration"); return CMM_ERR_ARG;
+    }
+#undef CMM_A32_T
+#undef CMM_A32
+    return launch_check("k_map_advect32");
+}
+
+int op_incomp32_launch(const cmm_grid &gm, const cmm_grid &gc, const float *Mf, unsigned long long *slot,
+                       hipStream_t st) {
+    DGrid dm = make_dgrid(gm), dc = make_dgrid(gc);
+    const float4 *M = (const float4 *)Mf;
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, slot);
+    const int r = gc.nx / gm.nx;
+    if (r >= 1 && gc.nx == r * gm.nx && gc.ny == r * gm.ny && gc.x0 == gm.x0 && gc.x1 == gm.x1 &&
+        gc.y0 == gm.y0 && gc.y1 == gm.y1) {
+        const int T = r % 4 == 0 ? 4 : r % 3 == 0 ? 3 : r % 2 == 0 ? 2 : 1;
+        const int ntx = gc.nx / T;
+        const long ntiles = (long)ntx * (gc.ny / T);
+        const unsigned nb = (unsigned)std::min<long>((ntiles + 255) / 256, 8192);
+        if (T == 4) hipLaunchKernelGGL((k_incomp_tensor32<4, 4>), dim3(nb), dim3(256), 0, st, dm, dc, M, slot, ntx, ntiles);
+        else if (T == 3) hipLaunchKernelGGL((k_incomp_tensor32<3, 3>), dim3(nb), dim3(256), 0, st, dm, dc, M, slot, ntx, ntiles);
+        else if (T == 2) hipLaunchKernelGGL((k_incomp_tensor32<2, 2>), dim3(nb), dim3(256), 0, st, dm, dc, M, slot, ntx, ntiles);
+        else hipLaunchKernelGGL((k_incomp_tensor32<1, 1>), dim3(nb), dim3(256), 0, st, dm, dc, M, slot, ntx, ntiles);
+        return launch_check("k_incomp_tensor32");
+    }
+    const int tx = (gc.nx + 63) / 64, ty = (gc.ny + 3) / 4, tt = tx * ty;
+    hipLaunchKernelGGL(k_incomp32, dim3(std::min(tt, 2048)), dim3(64, 4), 0, st, dm, dc, M, slot, tx, tt);
+    return launch_check("k_incomp32");
+}
+
+int op_incomp32(const cmm_grid &gm, const cmm_grid &gc, const float *M, double *out, hipStream_t st) {
+    unsigned long long *slot;
+    if (int rc = scalar_slot(&slot)) return rc;
+    if (int rc = op_incomp32_launch(gm, gc, M, slot, st)) return rc;
+    unsigned long long h = 0;
+    if (int rc = hip_check(hipMemcpyAsync(&h, slot, sizeof(h), hipMemcpyDeviceToHost, st), "hipMemcpyAsync")) return rc;
+    if (int rc = hip_check(hipStreamSynchronize(st), "hipStreamSynchronize")) return rc;
+    std::memcpy(out, &h, sizeof(double));
+    return CMM_OK;
+}
+
+int op_psi_from_vort32(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf, const cmm_grid &gp, const float *M,
+                       const float *W4, int molly, double fcut, double *psi, void *work, size_t work_bytes,
+                       hipStream_t st, hipEvent_t *ev) {
+    if (int rc = fft_grid_ok(gv, "vort grid")) return rc;
+    if (int rc = fft_grid_ok(gp, "psi grid")) return rc;
+    SpecPlan sp = plan_spec(gv, gp, 1, fcut);
+    const size_t base = sp.t_bytes + 2 * sp.g_bytes + 256;
+    void *w;
+    if (int rc = get_work(work, work_bytes, base + sizeof(double) * (size_t)gv.nx * gv.ny, &w)) return rc;
+    double *f = reinterpret_cast<double *>(static_cast<char *>(w) + base);
+    const long lanes = (long)gv.nx * (gv.ny / kSampleRows);
+    SampleVort32 smp{make_dgrid(gm), make_dgrid(gv), make_dgrid(gf), (const float4 *)M, (const float4 *)W4, molly};
+    if (ev) (void)hipEventRecord(ev[0], st);
+    hipLaunchKernelGGL(k_sample_field<SampleVort32>, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, st, smp, gv.nx,
+                       gv.ny, f);
+    if (int rc = launch_check("k_sample_field")) return rc;
+    return run_spec(sp, SampleReal{f, gv.nx}, psi, w, st, ev, false);
+}
+
+int op_sample_vort32(const cmm_grid &gm, const cmm_grid &gt, const cmm_grid &gf, const float *M, const float *W4,
+                     int molly, double *out, hipStream_t st) {
+    SampleVort32 s{make_dgrid(gm), make_dgrid(gt), make_dgrid(gf), (const float4 *)M, (const float4 *)W4, molly};
+    dim3 b(64, 4);
+    hipLaunchKernelGGL(k_sample_grid<SampleVort32>, grid2d(gt.nx, gt.ny, b), b, 0, st, s, gt.nx, gt.ny, out);
+    return launch_check("k_sample_grid");
+}
+
 }  // namespace cmm

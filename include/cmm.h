@@ -156,6 +156,31 @@ int cmm_vp_psi_from_f(cmm_grid g_map, cmm_grid g_vort, cmm_grid g_fine, cmm_grid
 int cmm_particles_advect(cmm_grid g_psi, const double *const *psi_levels, int l_order, const double *Lp12, double dt,
                          int scheme, int n, double *pos, void *stream);
 
+/* ---- split-precision form (no counterpart in the reference) ----------------------------------
+ * The map is stored as the identity plus a displacement D = X - identity, one 48-byte record of
+ * 12 floats per node: (Dx hi, Dx lo, Dy hi, Dy lo), the value planes as FP32 pairs hi + lo;
+ * (dX/dx - 1, dX/dy, dY/dx, dY/dy - 1); (d2X/dxdy, d2Y/dxdy, 0, 0).  The gathered Hermite fields
+ * (the Lagrange-combined psi fields, the fine vorticity) are FP32 node records
+ * (f, fx, fy, fxy).  Arithmetic, the footpoint integration, the Jacobian check, the psi levels,
+ * the FFTs and the Poisson divide stay FP64.  These operators are the split forms of
+ * cmm_map_advect, cmm_incomp_error and cmm_psi_from_vort. */
+enum { CMM_PRECISION_FP64 = 0, CMM_PRECISION_SPLIT = 1 };
+/* plane-major FP64 map (4*N doubles per component) <-> split map (12*N floats) */
+int cmm_map_to_split(cmm_grid g, const double *chix, const double *chiy, float *map_split, void *stream);
+int cmm_map_from_split(cmm_grid g, const float *map_split, double *chix, double *chiy, void *stream);
+/* FP32 node records (4*N floats) of a plane-major FP64 Hermite field */
+int cmm_hermite_to_split(cmm_grid g, const double *herm, float *herm_split, void *stream);
+/* Backward advection of the split map (schemes 10, 21, 30, 40, 41 with l_order >= 2); the
+ * workspace (cmm_map_advect_split_workspace_bytes) holds the FP32 combined psi fields. */
+size_t cmm_map_advect_split_workspace_bytes(cmm_grid g_psi, const cmm_advect_coeffs *c);
+int cmm_map_advect_split(cmm_grid g_map, cmm_grid g_psi, const float *map_in, const double *const *psi_levels,
+                         const cmm_advect_coeffs *c, float *map_out, void *work, size_t work_bytes, void *stream);
+int cmm_incomp_error_split(cmm_grid g_map, cmm_grid g_check, const float *map_split, double *out, void *stream);
+/* workspace as cmm_psi_workspace_bytes; vort_split: FP32 records of the fine vorticity */
+int cmm_psi_from_vort_split(cmm_grid g_map, cmm_grid g_vort, cmm_grid g_fine, cmm_grid g_psi, const float *map_split,
+                            const float *vort_split, int molly_stencil, double freq_cut, double *psi_out, void *work,
+                            size_t work_bytes, void *stream);
+
 /* ---- engine: the Euler time loop on the device ------------------------------------------------
  * Mirrors cuda_euler_2d (Cuda-CMM/src/simulation/cmm-euler2d.cu:45-960) for the backward map:
  * initial sampling, Lagrange warm-up, and per step advection, incompressibility check, remap with
@@ -179,6 +204,9 @@ typedef struct cmm_euler_params {
     int forward_map;                /* 1: also advect the forward map on g_coarse, with its own stack
                                        of stack_capacity maps (Cuda-CMM/src/simulation/cmm-euler2d.cu:156-159,
                                        386-397, 781-811) */
+    int precision;                  /* CMM_PRECISION_FP64, or CMM_PRECISION_SPLIT: the split-precision
+                                       form of the map, the combined psi fields and the fine vorticity
+                                       (Euler, backward map only; no counterpart in the reference) */
 } cmm_euler_params;
 
 /* The 1D-1V Vlasov-Poisson loop (cuda_vlasov_1d, Cuda-CMM/src/simulation/cmm-vlasov1d.cu:48-1148)
