@@ -59,6 +59,16 @@ def algorithmic(slot, c, kx_max, n_stack=0):
         return 2 * 16 * (kx_max + 1) * c["n_psi"] + 32 * npsi, 2 * 5.0 * npsi * 11
     if slot == "particles":
         return 16 * c["n_particles"], 0.0
+    # split-precision form: 48-byte map records, 16-byte records of the combined psi fields and of
+    # the fine vorticity
+    if slot == "advect_split":
+        return 48 * nc + 16 * 3 * npsi + 48 * nc, 9.2e3 * nc
+    if slot == "psi_combine_split":
+        return 32 * L * npsi + 16 * 3 * npsi, 2 * 2 * L * 4 * npsi
+    if slot == "incomp_split":
+        return 48 * nc, 85.0 * nf
+    if slot == "psi_row_r2c_split":
+        return 48 * nc + 16 * nf + 16 * nv + 16 * (kx_max + 1) * c["n_vort"], 270.0 * nv + 2.5 * nv * 24
     if slot == "remap":
         return (64 * nc * (n_stack + 1) + 32 * nf), 170.0 * nf * (n_stack + 1)
     return 0, 0.0
@@ -115,7 +125,11 @@ def cpu_baseline(c, threads, budget_s=12.0):
 SLOT_KERNELS = {"advect": ("k_map_advect<30, 8, true",), "psi_combine": ("k_psi_combine<3>",),
                 "incomp": ("k_incomp_tensor<",),
                 "psi_row_r2c": ("k_sample_field<cmm::SampleVortT<true", "k_row_r2c<cmm::SampleReal>"),
-                "psi_col": ("k_col(",), "psi_row_c2r": ("k_row_c2r4(",), "particles": ("k_particles(",)}
+                "psi_col": ("k_col(",), "psi_row_c2r": ("k_row_c2r4(",), "particles": ("k_particles(",),
+                # the split-precision form's own kernels (its column and C2R stages are the FP64 ones)
+                "advect_split": ("k_map_advect32<30, 8, true",), "psi_combine_split": ("k_psi_combine32<3>",),
+                "incomp_split": ("k_incomp_tensor32<",),
+                "psi_row_r2c_split": ("k_sample_field<cmm::SampleVort32", "k_row_r2c<cmm::SampleReal>")}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "cmm_euler_c2_hbm_traffic.json")
 FETCH_SCALE_DEFAULT, WRITE_SCALE_DEFAULT = 2.0, 1.0
 
@@ -202,6 +216,42 @@ def summarize_profiles(kt_dir, fetch_dir, write_dir, out_dir, commands):
     print("\n".join(lines))
 
 
+def roofline(c, prof, n_stack, suffix=""):
+    """Per-slot rates from the engine's event profile and the roofline object of the dominant
+    kernel: algorithmic bytes and flops per launch over its average duration; traffic from the
+    committed counter summary.  suffix selects the algorithmic model of a form's slots."""
+    kx_max = int(min(c["n_psi"] // 2, c["n_vort"] // 2, c["freq_cut_psi"]))
+    per = {}
+    for name, (ms, cnt) in prof.items():
+        if cnt:
+            model = name + suffix if algorithmic(name + suffix, c, kx_max, n_stack)[0] else name
+            b, f = algorithmic(model, c, kx_max, n_stack)
+            per[name] = dict(ms_total=ms, launches=cnt, ms_avg=ms / cnt, bytes=b, flops=f, model=model)
+    for name, d in sorted(per.items(), key=lambda kv: -kv[1]["ms_total"]):
+        gbs = d["bytes"] / (d["ms_avg"] * 1e-3) / 1e9
+        tfs = d["flops"] / (d["ms_avg"] * 1e-3) / 1e12
+        log(f"  {name + suffix:18s} {d['ms_total']:9.2f} ms total  {d['launches']:5d} launches  {d['ms_avg']:.4f} ms avg"
+            f"  {gbs:8.1f} GB/s  {tfs:6.2f} TF/s")
+    dom = max((k for k in per if k != "remap"), key=lambda k: per[k]["ms_total"])
+    d = per[dom]
+    gbs = d["bytes"] / (d["ms_avg"] * 1e-3) / 1e9
+    tfs = d["flops"] / (d["ms_avg"] * 1e-3) / 1e12
+    if tfs / FP64_PEAK_TFLOPS > gbs / HBM_PEAK_GBS:
+        roof = {"bound": "mfma", "achieved": round(tfs, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(tfs / FP64_PEAK_TFLOPS, 4)}
+    else:
+        roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        tr = json.load(open(TRAFFIC_FILE))["slots"].get(d["model"])
+        if tr:
+            traffic = int(tr["fetch_bytes"] + tr["write_bytes"])
+    roof.update({"kernel": d["model"], "traffic": traffic, "ms_avg": round(d["ms_avg"], 5),
+                 "algorithmic_bytes": d["bytes"], "algorithmic_flops": d["flops"]})
+    return roof, per
+
+
 def run_form(cmm, torch, c, precision, warmup, steps, dt=None):
     """One engine run of the workload: W untimed steps, then K steps timed between two device
     synchronisations with per-kernel events; returns the rate, the profile and the final fields
@@ -280,35 +330,7 @@ def main():
     sps, elapsed, prof, n_stack, t_now, remaps, t_init = (plain[k] for k in (
         "sps", "elapsed", "prof", "maps", "t_end", "remaps", "init_s"))
 
-    # dominant kernel and its roofline
-    kx_max = int(min(c["n_psi"] // 2, c["n_vort"] // 2, c["freq_cut_psi"]))
-    per = {}
-    for name, (ms, cnt) in prof.items():
-        if cnt:
-            b, f = algorithmic(name, c, kx_max, n_stack)
-            per[name] = dict(ms_total=ms, launches=cnt, ms_avg=ms / cnt, bytes=b, flops=f)
-    for name, d in sorted(per.items(), key=lambda kv: -kv[1]["ms_total"]):
-        gbs = d["bytes"] / (d["ms_avg"] * 1e-3) / 1e9
-        tfs = d["flops"] / (d["ms_avg"] * 1e-3) / 1e12
-        log(f"  {name:12s} {d['ms_total']:9.2f} ms total  {d['launches']:5d} launches  {d['ms_avg']:.4f} ms avg"
-            f"  {gbs:8.1f} GB/s  {tfs:6.2f} TF/s")
-    dom = max((k for k in per if k != "remap"), key=lambda k: per[k]["ms_total"])
-    d = per[dom]
-    gbs = d["bytes"] / (d["ms_avg"] * 1e-3) / 1e9
-    tfs = d["flops"] / (d["ms_avg"] * 1e-3) / 1e12
-    if tfs / FP64_PEAK_TFLOPS > gbs / HBM_PEAK_GBS:
-        roof = {"bound": "mfma", "achieved": round(tfs, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(tfs / FP64_PEAK_TFLOPS, 4)}
-    else:
-        roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(gbs / HBM_PEAK_GBS, 4)}
-    traffic = None
-    if os.path.exists(TRAFFIC_FILE):
-        tr = json.load(open(TRAFFIC_FILE))["slots"].get(dom)
-        if tr:
-            traffic = int(tr["fetch_bytes"] + tr["write_bytes"])
-    roof.update({"kernel": dom, "traffic": traffic, "ms_avg": round(d["ms_avg"], 5),
-                 "algorithmic_bytes": d["bytes"], "algorithmic_flops": d["flops"]})
+    roof, per = roofline(c, prof, n_stack)
 
     # the chosen form: split precision at the same step, with its errors against the FP64 run
     chosen = None
@@ -328,6 +350,7 @@ def main():
                       "particles_max_abs": float(dpart.max()),
                       "e_inc_max_abs_diff": float(np.max(np.abs(sp["einc"] - plain["einc"])))},
                   "remaps_timed": sp["remaps"], "maps": sp["maps"],
+                  "roofline": roofline(c, sp["prof"], sp["maps"], "_split")[0],
                   "kernels_ms_avg": {k: round(ms / cnt, 5) for k, (ms, cnt) in sp["prof"].items() if cnt}}
         log(f"split form: {sp['sps']:.1f} steps/s ({chosen['speedup']:.2f}x), errors {chosen['errors_vs_fp64_at_t_end']}")
         # the same forms at twice the step (plain Lagrange extrapolation of psi), to the same end

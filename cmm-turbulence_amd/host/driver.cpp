@@ -736,6 +736,9 @@ int run_euler(Settings &s) {
     if (s.forward_map) cap = (int)(cap / 2.0);  // the forward stack takes the other half, :368
     p.stack_capacity = cap;
     p.forward_map = s.forward_map ? 1 : 0;
+    if (s.map_precision == "split") p.precision = CMM_PRECISION_SPLIT;
+    else if (s.map_precision == "fp64") p.precision = CMM_PRECISION_FP64;
+    else fail("map_precision must be fp64 or split", 1020);
     if (s.verbose >= 1) {
         log.push("Map size in MB = " + to_s((int)map_size));
         log.push("Map stack length on device = " + to_s(cap));

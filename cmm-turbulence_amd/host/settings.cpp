@@ -323,6 +323,7 @@ Settings::Settings() {
     for (auto &p : ps) { ParticlesAdvected a; a.set_all(p); particles_advected.push_back(a); }
     particles_steps = -1;
     seed = 0;
+    map_precision = "fp64";
 }
 
 int Settings::set_variable(const std::string &line, const std::string &delim) {
@@ -373,6 +374,7 @@ int Settings::set_variable(const std::string &line, const std::string &delim) {
     else if (c == "particles_advected_num") { particles_advected.assign(std::stoi(v), ParticlesAdvected()); return 4; }
     else if (c == "particles_steps") particles_steps = std::stoi(v);
     else if (c == "seed") seed = std::stoull(v);
+    else if (c == "map_precision") map_precision = v;
     return 0;
 }
 

@@ -114,6 +114,7 @@ struct Settings {
     std::vector<ParticlesAdvected> particles_advected;
     int particles_steps;
     unsigned long long seed;  // seed of the random-spectrum initial condition (new key)
+    std::string map_precision;  // "fp64" or "split": the engine's precision form (new key)
 
     Settings();                 // presets, Cuda-CMM/src/ui/settings.cu:18-327
     // one COMMAND<delim>VALUE; returns the block kind (1..5) when the next lines hold a block
