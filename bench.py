@@ -292,6 +292,93 @@ def run_form(cmm, torch, c, precision, warmup, steps, dt=None):
     return out
 
 
+def ensemble_specs(n_inst, seed):
+    """The mixed ensemble: map grids cycling 256^2, 512^2, 1024^2 (fine 4x, psi and vort 2x),
+    Landau damping and two-stream distributions alternating, amplitudes uniform in [0.005, 0.05]
+    from a seeded generator."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    sizes = (256, 512, 1024)
+    return [(sizes[i % 3], "landau_amp" if i % 2 == 0 else "two_stream_v2", float(rng.uniform(0.005, 0.05)))
+            for i in range(n_inst)]
+
+
+def run_ensemble(args):
+    """Batched 1D-1V Vlasov-Poisson ensemble: every instance's step queued into one captured graph
+    per step phase and replayed with one launch per ensemble step, in the FP64 and split forms;
+    the rate against stepping the instances one by one, and the split form's errors."""
+    import numpy as np
+    import torch
+    import cmm
+    dt, t_end = 1.0 / 20, args.t_end
+    steps = int(round(t_end / dt))
+    specs = ensemble_specs(args.instances, 0)
+    g = cmm.ensemble_grid
+
+    def build(prec):
+        es = []
+        for n, ic, amp in specs:
+            e = cmm.Euler(g(n), g(4 * n), g(2 * n), g(2 * n), dt, initial_condition=ic, ic_params=[amp],
+                          simulation_type="cmm_vlasov_poisson_1d", incomp_threshold=1e-4, stack_capacity=64,
+                          precision=prec)
+            e.init()
+            es.append(e)
+        return es
+
+    out = {"workload": f"{args.instances} Vlasov-Poisson instances, maps 256^2/512^2/1024^2 (fine 4x, psi and vort 2x), "
+                       f"x in [0, 4 pi), v in [-8, 8), Landau / two-stream alternating, amplitudes U[0.005, 0.05] "
+                       f"seed 0, dt 1/20, T {t_end}", "steps": steps, "forms": {}}
+    fields = {}
+    for prec in ("fp64", "split"):
+        es = build(prec)
+        tunes = [cmm.autotune(e) for e in es]
+        b = cmm.Batch(es, use_graphs=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        remaps = 0
+        for i in range(steps):
+            infos = b.step(dt)
+            remaps += sum(x.remapped for x in infos)
+            if (i + 1) % 200 == 0:
+                log(f"[ensemble {prec}] step {i + 1}/{steps}, remaps so far {remaps}")
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        st = b.stats()
+        b.close()
+        fields[prec] = [(e.field("w_coarse"), e.field("psi")[:4 * (2 * n) ** 2]) for e, (n, _, _) in zip(es, specs)]
+        out["forms"][prec] = {"ensemble_steps_per_s": round(steps / el, 3), "instance_steps_per_s": round(
+            steps * len(es) / el, 1), "remaps": remaps, "graph": st,
+            "check_tiles": sorted({(t["tile"], t["max_blocks"]) for t in tunes})}
+        log(f"[ensemble {prec}] {steps} steps in {el:.2f}s: {out['forms'][prec]}")
+        # the same batch without graphs (each instance's work queued on its own stream, one wait per
+        # ensemble step), then the instances stepped one by one (cmm_euler_step each, one wait per
+        # instance step), each for a bounded number of steps
+        ns = min(steps, 100)
+        b = cmm.Batch(es, use_graphs=False)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(ns):
+            b.step(dt)
+        torch.cuda.synchronize()
+        out["forms"][prec]["eager_batch_ensemble_steps_per_s"] = round(ns / (time.perf_counter() - t0), 3)
+        b.close()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(ns):
+            for e in es:
+                e.step(dt)
+        torch.cuda.synchronize()
+        el1 = time.perf_counter() - t0
+        out["forms"][prec]["one_by_one_ensemble_steps_per_s"] = round(ns / el1, 3)
+        for e in es:
+            e.close()
+    rl2 = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    errs = [(rl2(fs[0], f6[0]), rl2(fs[1], f6[1])) for fs, f6 in zip(fields["split"], fields["fp64"])]
+    out["split_vs_fp64_at_t_end"] = {"f_coarse_rel_l2_max": max(e[0] for e in errs),
+                                     "psi_rel_l2_max": max(e[1] for e in errs)}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "summarize-profiles":
         kt, fe, wr, out = sys.argv[2:6]
@@ -304,9 +391,16 @@ def main():
     ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-split", action="store_true", help="skip the split-precision form")
+    ap.add_argument("--ensemble", action="store_true",
+                    help="run the batched Vlasov-Poisson ensemble instead (its own JSON line)")
+    ap.add_argument("--instances", type=int, default=32)
+    ap.add_argument("--t-end", type=float, default=40.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 1 GiB device copy, the reference kernel for counter scaling")
     args = ap.parse_args()
+    if args.ensemble:
+        run_ensemble(args)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs one engine instance per GPU and has no multi-GPU "
               f"path (requested --gpus {args.gpus}).", file=sys.stderr)

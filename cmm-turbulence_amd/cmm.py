@@ -19,7 +19,9 @@ IC = {"4_nodes": 0, "quadropole": 1, "one_vortex": 2, "two_votices": 3, "three_v
       "single_shear_layer": 5, "tanh_shear_layer": 6, "shielded_vortex": 9, "two_cosine": 10,
       "vortex_sheets": 11, "double_shear_layer": 100, "random_spectrum": 101,
       # distribution functions of the Vlasov-Poisson flow, Cuda-CMM/src/ui/settings.h:300-303
-      "landau_damping": 200, "two_stream": 201, "nonlin_landau_damping": 202}
+      "landau_damping": 200, "two_stream": 201, "nonlin_landau_damping": 202,
+      # ensemble conditions with amplitude ic_params[0] (no reference counterpart)
+      "landau_amp": 203, "two_stream_v2": 204}
 SIM = {"cmm_euler_2d": 0, "cmm_vlasov_poisson_1d": 1}
 # time integration names, Cuda-CMM/src/ui/settings.h:386-398
 TIME_INTEGRATION = {"EulerExp": 10, "RK1": 10, "AB2": 20, "RK2": 21, "Heun": 21, "RK3": 30, "RK3Mod": 31,
@@ -81,7 +83,8 @@ SYMBOLS = ["cmm_version", "cmm_last_error", "cmm_device_info", "cmm_lagrange_coe
            "cmm_euler_restart_forward", "cmm_euler_get_forward_stack_map", "cmm_euler_request_remap",
            "cmm_euler_forward_positions", "cmm_euler_forward_points", "cmm_euler_state", "cmm_euler_profile", "cmm_euler_profile_read",
            "cmm_map_to_split", "cmm_map_from_split", "cmm_hermite_to_split", "cmm_map_advect_split_workspace_bytes",
-           "cmm_map_advect_split", "cmm_incomp_error_split", "cmm_psi_from_vort_split"]
+           "cmm_map_advect_split", "cmm_incomp_error_split", "cmm_psi_from_vort_split",
+           "cmm_batch_create", "cmm_batch_destroy", "cmm_batch_step", "cmm_batch_stats", "cmm_euler_autotune"]
 # engine precision forms (CMM_PRECISION_*)
 PRECISION = {"fp64": 0, "split": 1}
 
@@ -165,6 +168,11 @@ def lib():
         L.cmm_incomp_error_split.argtypes = [Grid, Grid, VP, DP, VP]
         L.cmm_psi_from_vort_split.argtypes = [Grid, Grid, Grid, Grid, VP, VP, C.c_int, C.c_double, VP, VP, C.c_size_t,
                                               VP]
+        L.cmm_batch_create.argtypes = [C.POINTER(VP), C.c_int, C.c_int, C.POINTER(VP)]
+        L.cmm_batch_destroy.argtypes = [VP]
+        L.cmm_batch_step.argtypes = [VP, C.c_double, C.POINTER(StepInfo)]
+        L.cmm_batch_stats.argtypes = [VP, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.cmm_euler_autotune.argtypes = [VP, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -601,3 +609,48 @@ def uniform_particles(n, seed, center=(TWO_PI / 2, TWO_PI / 2), width=(TWO_PI, T
     pos[0::2] = px - np.floor((px - bounds[0]) / lx) * lx
     pos[1::2] = py - np.floor((py - bounds[2]) / ly) * ly
     return pos
+
+
+class Batch:
+    """Engines stepped together (cmm_batch_*): one wait per step, and with graphs the queued work
+    of all engines replayed from one captured graph per step phase."""
+
+    def __init__(self, engines, use_graphs=True):
+        self.engines = list(engines)
+        arr = (VP * len(self.engines))(*[e.h.value for e in self.engines])
+        h = C.c_void_p()
+        _check(lib().cmm_batch_create(arr, len(self.engines), int(use_graphs), C.byref(h)), "cmm_batch_create")
+        self.h = h
+
+    def step(self, dt):
+        infos = (StepInfo * len(self.engines))()
+        _check(lib().cmm_batch_step(self.h, dt, infos), "cmm_batch_step")
+        return list(infos)
+
+    def stats(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().cmm_batch_stats(self.h, C.byref(a), C.byref(b), C.byref(c)), "cmm_batch_stats")
+        return {"graphs_captured": a.value, "graph_launches": b.value, "eager_steps": c.value}
+
+    def close(self):
+        if self.h:
+            lib().cmm_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def autotune(engine):
+    """cmm_euler_autotune: the check's tile edge and block cap, timed on the engine's map"""
+    t, b, ms = C.c_int(), C.c_int(), C.c_double()
+    _check(lib().cmm_euler_autotune(engine.h, C.byref(t), C.byref(b), C.byref(ms)), "cmm_euler_autotune")
+    return {"tile": t.value, "max_blocks": b.value, "ms": ms.value}
+
+
+def ensemble_grid(n, nv=None):
+    """(x, v) grid of the ensemble workload: x in [0, 4 pi), v in [-8, 8)"""
+    return grid(n, nv, (0.0, 2 * TWO_PI, -8.0, 8.0))

@@ -6,6 +6,7 @@
 #include <array>
 #include <cmath>
 #include <complex>
+#include <map>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -332,6 +333,10 @@ struct cmm_euler {
     float *W4f = nullptr;
     int m32_cur = 0;
     bool m32_fresh = false, planes_fresh = true;
+    // dry: a step's host state advances without device work (the work is replayed from a captured
+    // graph); capturing: the step's work is being captured (no event records)
+    bool dry = false, capturing = false;
+    IncompTune tune;  // the check's tile and grid, from cmm_euler_autotune
     int nstack = 0;
     // forward map (forward_map only): current, next, and its stack, pushed with the backward one
     double *fx = nullptr, *fy = nullptr, *nfx = nullptr, *nfy = nullptr, *fstack = nullptr;
@@ -388,14 +393,16 @@ bool split(const cmm_euler *e) { return e->p.precision == CMM_PRECISION_SPLIT; }
 
 int sync_planes(cmm_euler *e) {
     if (e->planes_fresh) return CMM_OK;
-    if (int rc = op_map_from32(e->p.g_coarse, e->m32[e->m32_cur], e->cx, e->cy, e->st)) return rc;
+    if (!e->dry)
+        if (int rc = op_map_from32(e->p.g_coarse, e->m32[e->m32_cur], e->cx, e->cy, e->st)) return rc;
     e->planes_fresh = true;
     return CMM_OK;
 }
 
 int sync_m32(cmm_euler *e) {
     if (e->m32_fresh) return CMM_OK;
-    if (int rc = op_map_to32(e->p.g_coarse, e->cx, e->cy, e->m32[e->m32_cur], e->st)) return rc;
+    if (!e->dry)
+        if (int rc = op_map_to32(e->p.g_coarse, e->cx, e->cy, e->m32[e->m32_cur], e->st)) return rc;
     e->m32_fresh = true;
     return CMM_OK;
 }
@@ -420,6 +427,7 @@ int engine_translate(cmm_euler *e) {
 }
 
 int engine_psi(cmm_euler *e, double *out, hipEvent_t *ev = nullptr) {
+    if (e->dry) return split(e) ? (e->p.sim == CMM_SIM_VLASOV_POISSON_1D ? sync_planes(e) : sync_m32(e)) : CMM_OK;
     if (e->p.sim == CMM_SIM_VLASOV_POISSON_1D) {
         // one operator; its whole time goes to the first psi profile slot
         if (int rc = sync_planes(e)) return rc;
@@ -447,9 +455,13 @@ int engine_advect(cmm_euler *e, const double *t, const double *dt, int loop_ctr,
     for (int l = 0; l < 4; ++l) P.p[l] = e->lvl[l < L ? l : 0];
     if (split(e) && op_advect32_supported(c)) {
         if (int rc = sync_m32(e)) return rc;
-        if (int rc = op_map_advect32(e->p.g_coarse, e->p.g_psi, P, c, e->m32[e->m32_cur], e->m32[1 - e->m32_cur],
-                                     e->work, e->work_bytes, e->st, e->prof ? e->ev[10] : nullptr))
-            return rc;
+        if (!e->dry)
+            // the Vlasov-Poisson stream function -H(v) + phi(x) has a large smooth part in v: its
+            // combined fields stay FP64 records
+            if (int rc = op_map_advect32(e->p.g_coarse, e->p.g_psi, P, c, e->m32[e->m32_cur], e->m32[1 - e->m32_cur],
+                                         e->work, e->work_bytes, e->st, e->prof ? e->ev[10] : nullptr,
+                                         e->p.sim == CMM_SIM_VLASOV_POISSON_1D))
+                return rc;
         e->m32_cur = 1 - e->m32_cur;
         e->m32_fresh = true;
         e->planes_fresh = false;
@@ -465,9 +477,10 @@ int engine_advect(cmm_euler *e, const double *t, const double *dt, int loop_ctr,
     const bool comb = op_advect_workspace(e->p.g_coarse, e->p.g_psi, c) > 0;
     const double *m8_in = comb && e->m8_valid ? e->m8[e->m8_cur] : nullptr;
     double *m8_out = comb ? e->m8[1 - e->m8_cur] : nullptr;
-    if (int rc = op_map_advect(e->p.g_coarse, e->p.g_psi, e->cx, e->cy, P, c, e->ncx, e->ncy, e->work, e->work_bytes,
-                               e->st, e->prof ? e->ev[10] : nullptr, m8_in, m8_out))
-        return rc;
+    if (!e->dry)
+        if (int rc = op_map_advect(e->p.g_coarse, e->p.g_psi, e->cx, e->cy, P, c, e->ncx, e->ncy, e->work,
+                                   e->work_bytes, e->st, e->prof ? e->ev[10] : nullptr, m8_in, m8_out))
+            return rc;
     std::swap(e->cx, e->ncx);
     std::swap(e->cy, e->ncy);
     e->m8_valid = comb;
@@ -493,9 +506,10 @@ int engine_advect_forward(cmm_euler *e, const double *t, const double *dt, int l
 
 // the check on whichever form of the map is current
 int launch_incomp(cmm_euler *e) {
-    if (split(e) && e->m32_fresh) return op_incomp32_launch(e->p.g_coarse, e->p.g_fine, e->m32[e->m32_cur], e->slot, e->st);
+    if (split(e) && e->m32_fresh)
+        return e->dry ? CMM_OK : op_incomp32_launch(e->p.g_coarse, e->p.g_fine, e->m32[e->m32_cur], e->slot, e->st, e->tune);
     if (int rc = sync_planes(e)) return rc;
-    return op_incomp_launch(e->p.g_coarse, e->p.g_fine, e->cx, e->cy, e->slot, e->st);
+    return e->dry ? CMM_OK : op_incomp_launch(e->p.g_coarse, e->p.g_fine, e->cx, e->cy, e->slot, e->st, e->tune);
 }
 
 int engine_incomp(cmm_euler *e, double *out) {
@@ -624,8 +638,8 @@ int cmm_euler_create(const cmm_euler_params *p, cmm_euler **out) {
         set_error("cmm_euler_create: precision must be CMM_PRECISION_FP64 or CMM_PRECISION_SPLIT");
         return CMM_ERR_ARG;
     }
-    if (p->precision == CMM_PRECISION_SPLIT && (p->sim != CMM_SIM_EULER_2D || p->forward_map)) {
-        set_error("cmm_euler_create: the split-precision form runs the Euler flow with the backward map only");
+    if (p->precision == CMM_PRECISION_SPLIT && p->forward_map) {
+        set_error("cmm_euler_create: the split-precision form runs the backward map only");
         return CMM_ERR_ARG;
     }
     if (p->forward_map != 0 && p->forward_map != 1) {
@@ -657,7 +671,7 @@ int cmm_euler_create(const cmm_euler_params *p, cmm_euler **out) {
     if (!rc && p->sim == CMM_SIM_EULER_2D && !sp) rc = dalloc(&e->W4, 4 * e->nf);
     if (!rc && sp) rc = falloc(&e->m32[0], 12 * e->nc);
     if (!rc && sp) rc = falloc(&e->m32[1], 12 * e->nc);
-    if (!rc && sp) rc = falloc(&e->W4f, 4 * e->nf);
+    if (!rc && sp && p->sim == CMM_SIM_EULER_2D) rc = falloc(&e->W4f, 4 * e->nf);
     if (!rc && p->stack_capacity > 0) rc = dalloc(&e->stack, (size_t)8 * e->nc * p->stack_capacity);
     if (p->forward_map) {
         if (!rc) rc = dalloc(&e->fx, 4 * e->nc);
@@ -893,9 +907,12 @@ int cmm_euler_request_remap(cmm_euler *e) {
     return CMM_OK;
 }
 
-int cmm_euler_step(cmm_euler *e, double dt_now, cmm_step_info *info) {
-    if (!e || !e->initialised) { set_error("cmm_euler_step: engine not initialised"); return CMM_ERR_ARG; }
-    if (!(dt_now > 0)) { set_error("cmm_euler_step: dt must be positive"); return CMM_ERR_ARG; }
+namespace {
+
+// The device work of a step up to the psi solve, queued without a host wait: advection, check,
+// E_inc copied to pinned memory, psi history rotated and the new psi.  With e->dry only the host
+// state advances (the work comes from a captured graph replay).
+int step_queue(cmm_euler *e, double dt_now) {
     const cmm_euler_params &p = e->p;
     const int L = p.l_order;
     const int lc = e->loop + L - 1;
@@ -924,11 +941,13 @@ int cmm_euler_step(cmm_euler *e, double dt_now, cmm_step_info *info) {
             return rc;
         if (int rc = op_incomp_launch(p.g_coarse, p.g_coarse, e->fx, e->fy, e->slot + 1, e->st)) return rc;
     }
-    if (int rc = hip_check(hipMemcpyAsync(e->h_slot, e->slot, sizeof(unsigned long long) * (p.forward_map ? 3 : 1),
-                                          hipMemcpyDeviceToHost, e->st),
-                           "hipMemcpyAsync"))
-        return rc;
-    if (int rc = hip_check(hipEventRecord(e->ev_inc, e->st), "hipEventRecord")) return rc;
+    if (!e->dry)
+        if (int rc = hip_check(hipMemcpyAsync(e->h_slot, e->slot, sizeof(unsigned long long) * (p.forward_map ? 3 : 1),
+                                              hipMemcpyDeviceToHost, e->st),
+                               "hipMemcpyAsync"))
+            return rc;
+    if (!e->dry && !e->capturing)
+        if (int rc = hip_check(hipEventRecord(e->ev_inc, e->st), "hipEventRecord")) return rc;
     prof_mark(e, 2);
     // psi history: the oldest level becomes the newest (same contents as the reference's
     // copies at :865-868), then the new psi, :871.  It is queued before E_inc is known: without a
@@ -940,7 +959,17 @@ int cmm_euler_step(cmm_euler *e, double dt_now, cmm_step_info *info) {
     if (e->part_pending)  // the previous step's particles still read the level about to be rewritten
         if (int rc = hip_check(hipStreamWaitEvent(e->st, e->ev_part, 0), "hipStreamWaitEvent")) return rc;
     if (int rc = engine_psi(e, e->lvl[0], e->prof ? &e->ev[5] : nullptr)) return rc;
-    if (int rc = hip_check(hipEventSynchronize(e->ev_inc), "hipEventSynchronize")) return rc;
+    return CMM_OK;
+}
+
+// The rest of a step: E_inc (waited for unless the caller has synchronised), remap, particles,
+// bookkeeping.
+int step_finish(cmm_euler *e, cmm_step_info *info, bool waited) {
+    const cmm_euler_params &p = e->p;
+    const int L = p.l_order;
+    const int lc = e->loop + L - 1;
+    if (!waited)
+        if (int rc = hip_check(hipEventSynchronize(e->ev_inc), "hipEventSynchronize")) return rc;
     double einc, einc_f = 0, einv = 0;
     std::memcpy(&einc, e->h_slot, sizeof(double));
     if (p.forward_map) {
@@ -1015,6 +1044,209 @@ int cmm_euler_step(cmm_euler *e, double dt_now, cmm_step_info *info) {
         info->incomp_forward = einc_f;
         info->invertibility = einv;
     }
+    return CMM_OK;
+}
+
+}  // namespace
+
+int cmm_euler_step(cmm_euler *e, double dt_now, cmm_step_info *info) {
+    if (!e || !e->initialised) { set_error("cmm_euler_step: engine not initialised"); return CMM_ERR_ARG; }
+    if (!(dt_now > 0)) { set_error("cmm_euler_step: dt must be positive"); return CMM_ERR_ARG; }
+    if (int rc = step_queue(e, dt_now)) return rc;
+    return step_finish(e, info, false);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Ensembles: several engines stepped together, their queued step work in one captured graph
+
+struct cmm_batch {
+    std::vector<cmm_euler *> e;
+    bool graphs = true;
+    hipStream_t master = nullptr;
+    hipEvent_t fork = nullptr;
+    std::vector<hipEvent_t> join;
+    // one graph per (step phase, state flags of every engine): the psi ring and the map ping-pong
+    // rotate device pointers with periods dividing 12, so a graph replays exactly the launches of
+    // the step it was captured on
+    std::map<std::string, hipGraphExec_t> cache;
+    int captured = 0, launched = 0, eager = 0;
+};
+
+namespace {
+
+std::string batch_key(const cmm_batch *b) {
+    std::string k;
+    for (const cmm_euler *e : b->e) {
+        k += std::to_string(e->loop % 12);
+        k += e->m8_valid ? 'v' : '-';
+        k += e->m32_fresh ? 'm' : '-';
+        k += e->planes_fresh ? 'p' : '-';
+        k += ';';
+    }
+    return k;
+}
+
+// after a remap, bring the forms of the map a replayed step reads up to date, so that the next
+// step's launches are those of the captured steady state
+int batch_normalise(cmm_euler *e) {
+    if (split(e)) return sync_m32(e);
+    if (e->m8[0] && !e->m8_valid) {
+        if (int rc = op_interleave_map(e->p.g_coarse, e->cx, e->cy, e->m8[e->m8_cur], e->st)) return rc;
+        e->m8_valid = true;
+    }
+    return CMM_OK;
+}
+
+int batch_capture(cmm_batch *b, double dt, hipGraphExec_t *out) {
+    int rc = hip_check(hipStreamBeginCapture(b->master, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
+    if (rc) return rc;
+    rc = hip_check(hipEventRecord(b->fork, b->master), "hipEventRecord");
+    for (size_t i = 0; i < b->e.size() && !rc; ++i) {
+        cmm_euler *e = b->e[i];
+        rc = hip_check(hipStreamWaitEvent(e->st, b->fork, 0), "hipStreamWaitEvent");
+        e->capturing = true;
+        if (!rc) rc = step_queue(e, dt);
+        e->capturing = false;
+        if (!rc) rc = hip_check(hipEventRecord(b->join[i], e->st), "hipEventRecord");
+        if (!rc) rc = hip_check(hipStreamWaitEvent(b->master, b->join[i], 0), "hipStreamWaitEvent");
+    }
+    hipGraph_t g = nullptr;
+    const int rc_end = hip_check(hipStreamEndCapture(b->master, &g), "hipStreamEndCapture");
+    if (!rc) rc = rc_end;
+    if (!rc) rc = hip_check(hipGraphInstantiate(out, g, nullptr, nullptr, 0), "hipGraphInstantiate");
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+}
+
+}  // namespace
+
+int cmm_batch_create(cmm_euler *const *engines, int n, int use_graphs, cmm_batch **out) {
+    if (!engines || n <= 0 || !out) { set_error("cmm_batch_create: bad argument"); return CMM_ERR_ARG; }
+    *out = nullptr;
+    for (int i = 0; i < n; ++i) {
+        const cmm_euler *e = engines[i];
+        if (!e || !e->initialised || e->p.forward_map || !e->part.empty()) {
+            set_error("cmm_batch_create: engines must be initialised, without a forward map or particles");
+            return CMM_ERR_ARG;
+        }
+        for (int j = 0; j < i; ++j)
+            if (engines[j] == e) { set_error("cmm_batch_create: an engine appears twice"); return CMM_ERR_ARG; }
+    }
+    cmm_batch *b = new cmm_batch();
+    b->e.assign(engines, engines + n);
+    b->graphs = use_graphs != 0;
+    int rc = hip_check(hipStreamCreateWithFlags(&b->master, hipStreamNonBlocking), "hipStreamCreate");
+    if (!rc) rc = hip_check(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "hipEventCreate");
+    b->join.assign(n, nullptr);
+    for (int i = 0; i < n && !rc; ++i) rc = hip_check(hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming), "hipEventCreate");
+    if (rc) { cmm_batch_destroy(b); return rc; }
+    *out = b;
+    return CMM_OK;
+}
+
+int cmm_batch_destroy(cmm_batch *b) {
+    if (!b) return CMM_OK;
+    if (b->master) (void)hipStreamSynchronize(b->master);
+    for (auto &kv : b->cache) (void)hipGraphExecDestroy(kv.second);
+    for (hipEvent_t ev : b->join)
+        if (ev) (void)hipEventDestroy(ev);
+    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->master) (void)hipStreamDestroy(b->master);
+    delete b;
+    return CMM_OK;
+}
+
+int cmm_batch_step(cmm_batch *b, double dt, cmm_step_info *infos) {
+    if (!b || !(dt > 0)) { set_error("cmm_batch_step: bad argument"); return CMM_ERR_ARG; }
+    for (cmm_euler *e : b->e)
+        if (e->prof || !e->part.empty()) { set_error("cmm_batch_step: profiling and particles are not batched"); return CMM_ERR_ARG; }
+    // a graph replays the steady state: the nominal step of every engine, after its first step
+    bool graph = b->graphs;
+    for (cmm_euler *e : b->e) graph = graph && e->loop >= 1 && dt == e->p.dt;
+    if (graph) {
+        // all engines' queued work must have drained before it is captured or replayed on the master stream
+        for (cmm_euler *e : b->e)
+            if (int rc = hip_check(hipStreamSynchronize(e->st), "hipStreamSynchronize")) return rc;
+        const std::string key = batch_key(b);
+        auto it = b->cache.find(key);
+        hipGraphExec_t exec = nullptr;
+        if (it == b->cache.end()) {
+            if (int rc = batch_capture(b, dt, &exec)) return rc;
+            b->cache[key] = exec;
+            b->captured++;
+        } else {
+            exec = it->second;
+            for (cmm_euler *e : b->e) {
+                e->dry = true;
+                const int rc = step_queue(e, dt);
+                e->dry = false;
+                if (rc) return rc;
+            }
+        }
+        if (int rc = hip_check(hipGraphLaunch(exec, b->master), "hipGraphLaunch")) return rc;
+        if (int rc = hip_check(hipStreamSynchronize(b->master), "hipStreamSynchronize")) return rc;
+        b->launched++;
+    } else {
+        for (cmm_euler *e : b->e)
+            if (int rc = step_queue(e, dt)) return rc;
+        b->eager++;
+    }
+    for (size_t i = 0; i < b->e.size(); ++i) {
+        cmm_euler *e = b->e[i];
+        if (int rc = step_finish(e, infos ? &infos[i] : nullptr, graph)) return rc;
+        if (infos && infos[i].remapped)
+            if (int rc = batch_normalise(e)) return rc;
+    }
+    return CMM_OK;
+}
+
+int cmm_batch_stats(cmm_batch *b, int *graphs_captured, int *graph_launches, int *eager_steps) {
+    if (!b) { set_error("cmm_batch_stats: null batch"); return CMM_ERR_ARG; }
+    if (graphs_captured) *graphs_captured = b->captured;
+    if (graph_launches) *graph_launches = b->launched;
+    if (eager_steps) *eager_steps = b->eager;
+    return CMM_OK;
+}
+
+// The check's tile edge (4, 2, 1 where it divides the refinement ratio) and block cap, timed on the
+// engine's current map with events (the fastest of 5 runs per candidate after one untimed run)
+int cmm_euler_autotune(cmm_euler *e, int *tile_out, int *max_blocks_out, double *ms_out) {
+    if (!e || !e->initialised) { set_error("cmm_euler_autotune: engine not initialised"); return CMM_ERR_ARG; }
+    const int r = e->p.g_fine.nx / e->p.g_coarse.nx;
+    hipEvent_t a = nullptr, z = nullptr;
+    int rc = hip_check(hipEventCreate(&a), "hipEventCreate");
+    if (!rc) rc = hip_check(hipEventCreate(&z), "hipEventCreate");
+    IncompTune best = e->tune;
+    float best_ms = 1e30f;
+    for (int tile : {4, 2, 1}) {
+        if (r <= 0 || r % tile) continue;
+        for (int cap : {256, 1024, 4096, 16384}) {
+            if (rc) break;
+            e->tune.tile = tile;
+            e->tune.max_blocks = cap;
+            float m_min = 1e30f;
+            for (int rep = 0; rep < 6 && !rc; ++rep) {
+                rc = hip_check(hipEventRecord(a, e->st), "hipEventRecord");
+                if (!rc) rc = launch_incomp(e);
+                if (!rc) rc = hip_check(hipEventRecord(z, e->st), "hipEventRecord");
+                if (!rc) rc = hip_check(hipEventSynchronize(z), "hipEventSynchronize");
+                float m = 0;
+                if (!rc) rc = hip_check(hipEventElapsedTime(&m, a, z), "hipEventElapsedTime");
+                if (rep > 0) m_min = std::min(m_min, m);
+            }
+            if (!rc && m_min < best_ms) {
+                best_ms = m_min;
+                best = e->tune;
+            }
+        }
+    }
+    e->tune = best;
+    if (a) (void)hipEventDestroy(a);
+    if (z) (void)hipEventDestroy(z);
+    if (rc) return rc;
+    if (tile_out) *tile_out = best.tile;
+    if (max_blocks_out) *max_blocks_out = best.max_blocks;
+    if (ms_out) *ms_out = best_ms;
     return CMM_OK;
 }
 

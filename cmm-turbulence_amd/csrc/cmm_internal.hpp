@@ -21,8 +21,15 @@ size_t op_advect_workspace(const cmm_grid &gm, const cmm_grid &gp, const cmm_adv
 int op_map_advect(const cmm_grid &gm, const cmm_grid &gp, const double *cx, const double *cy, const PsiLevels &P,
                   const cmm_advect_coeffs &c, double *ncx, double *ncy, void *work, size_t work_bytes,
                   hipStream_t st, hipEvent_t mid = nullptr, const double *m8_in = nullptr, double *m8_out = nullptr);
+// tile (0: the largest of 4, 3, 2, 1 dividing the refinement ratio) and cap on the number of
+// blocks of the tiled check; set by the autotuner
+struct IncompTune {
+    int tile = 0, max_blocks = 8192;
+};
 int op_incomp_launch(const cmm_grid &gm, const cmm_grid &gc, const double *cx, const double *cy,
-                     unsigned long long *slot, hipStream_t st);
+                     unsigned long long *slot, hipStream_t st, IncompTune tune = IncompTune());
+// M8[8j + p] = ChiX plane p, M8[8j + 4 + p] = ChiY plane p (the advection's interleaved map)
+int op_interleave_map(const cmm_grid &g, const double *cx, const double *cy, double *M8, hipStream_t st);
 int op_incomp(const cmm_grid &gm, const cmm_grid &gc, const double *cx, const double *cy, double *out,
               hipStream_t st);
 // max |x - X_b(X_f(x))| over the cell centres of gc (forward map on gf, backward on gb)
@@ -73,12 +80,13 @@ int op_map_to32(const cmm_grid &g, const double *cx, const double *cy, float *M,
 int op_map_from32(const cmm_grid &g, const float *M, double *cx, double *cy, hipStream_t st);
 int op_interleave4_32(const cmm_grid &g, const double *H, float *H4, hipStream_t st);
 bool op_advect32_supported(const cmm_advect_coeffs &c);
-size_t op_advect32_workspace(const cmm_grid &gp, const cmm_advect_coeffs &c);
+// psi64: combined psi fields as FP64 records (flows whose stream function has a large smooth part)
+size_t op_advect32_workspace(const cmm_grid &gp, const cmm_advect_coeffs &c, bool psi64 = false);
 int op_map_advect32(const cmm_grid &gm, const cmm_grid &gp, const PsiLevels &P, const cmm_advect_coeffs &c,
                     const float *M_in, float *M_out, void *work, size_t work_bytes, hipStream_t st,
-                    hipEvent_t mid = nullptr);
+                    hipEvent_t mid = nullptr, bool psi64 = false);
 int op_incomp32_launch(const cmm_grid &gm, const cmm_grid &gc, const float *M, unsigned long long *slot,
-                       hipStream_t st);
+                       hipStream_t st, IncompTune tune = IncompTune());
 int op_incomp32(const cmm_grid &gm, const cmm_grid &gc, const float *M, double *out, hipStream_t st);
 int op_psi_from_vort32(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf, const cmm_grid &gp, const float *M,
                        const float *W4, int molly, double fcut, double *psi, void *work, size_t work_bytes,

@@ -575,6 +575,11 @@ __device__ double init_vorticity(double x, double y, const ICParams &p) {
                (exp(-(y - 3) * (y - 3) * 0.5) + exp(-(y + 3) * (y + 3) * 0.5));
     case CMM_IC_NONLIN_LANDAU_DAMPING:
         return (1 + 0.5 * cos(0.5 * x)) * exp(-y * y * 0.5) / sqrt(2 * kPi);
+    // ensemble conditions with a perturbation amplitude (no reference counterpart)
+    case CMM_IC_LANDAU_AMP:
+        return (1 + p.prm[0] * cos(0.5 * x)) * exp(-y * y * 0.5) / sqrt(2 * kPi);
+    case CMM_IC_TWO_STREAM_V2:
+        return (1 + p.prm[0] * cos(0.5 * x)) * y * y * exp(-y * y * 0.5) / sqrt(2 * kPi);
     // passive scalars, d_init_scalar (Cuda-CMM/src/simulation/cmm-init.cu:249-301)
     case CMM_IC_SCALAR_RECTANGLE: {
         x -= floor(x / kTwoPi) * kTwoPi;
@@ -1626,19 +1631,32 @@ int op_map_advect(const cmm_grid &gm, const cmm_grid &gp, const double *cx, cons
     return launch_check("k_map_advect");
 }
 
+// tile edge of the tiled check: the tuned one when it divides r, else the largest of 4, 3, 2, 1
+// that divides r
+static int incomp_tile(int r, const IncompTune &tune) {
+    if (tune.tile > 0 && r % tune.tile == 0 && (tune.tile == 1 || tune.tile == 2 || tune.tile == 3 || tune.tile == 4))
+        return tune.tile;
+    return r % 4 == 0 ? 4 : r % 3 == 0 ? 3 : r % 2 == 0 ? 2 : 1;
+}
+
+int op_interleave_map(const cmm_grid &g, const double *cx, const double *cy, double *M8, hipStream_t st) {
+    const long n = (long)g.nx * g.ny;
+    hipLaunchKernelGGL(k_interleave_map, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, n, cx, cy, M8);
+    return launch_check("k_interleave_map");
+}
+
 int op_incomp_launch(const cmm_grid &gm, const cmm_grid &gc, const double *cx, const double *cy,
-                     unsigned long long *slot, hipStream_t st) {
+                     unsigned long long *slot, hipStream_t st, IncompTune tune) {
     DGrid dm = make_dgrid(gm), dc = make_dgrid(gc);
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, slot);
     dim3 b(64, 4);
     const int r = gc.nx / gm.nx;
     if (r >= 1 && gc.nx == r * gm.nx && gc.ny == r * gm.ny && gc.x0 == gm.x0 && gc.x1 == gm.x1 &&
         gc.y0 == gm.y0 && gc.y1 == gm.y1) {
-        // tiles of T x T points, T the largest of 4, 3, 2, 1 that divides r
-        const int T = r % 4 == 0 ? 4 : r % 3 == 0 ? 3 : r % 2 == 0 ? 2 : 1;
+        const int T = incomp_tile(r, tune);
         const int ntx = gc.nx / T;
         const long ntiles = (long)ntx * (gc.ny / T);
-        const unsigned nb = (unsigned)std::min<long>((ntiles + 255) / 256, 8192);
+        const unsigned nb = (unsigned)std::max<long>(1, std::min<long>((ntiles + 255) / 256, tune.max_blocks));
         if (T == 4) hipLaunchKernelGGL((k_incomp_tensor<4, 4>), dim3(nb), dim3(256), 0, st, dm, dc, cx, cy, slot, r, ntx, ntiles);
         else if (T == 3) hipLaunchKernelGGL((k_incomp_tensor<3, 3>), dim3(nb), dim3(256), 0, st, dm, dc, cx, cy, slot, r, ntx, ntiles);
         else if (T == 2) hipLaunchKernelGGL((k_incomp_tensor<2, 2>), dim3(nb), dim3(256), 0, st, dm, dc, cx, cy, slot, r, ntx, ntiles);
@@ -2138,9 +2156,11 @@ __device__ __forceinline__ void advect_store32(const cmm_advect_coeffs &c, int n
     }
 }
 
-template <int TIN, int G, bool P2>
-__global__ void __launch_bounds__(256) k_map_advect32(DGrid gm, DGrid gp, const float4 *C1, const float4 *C12,
-                                                      const float4 *C0, const float4 *__restrict__ M,
+// PSI64: the combined psi fields are FP64 records (k_psi_combine's), for flows whose stream
+// function has a large smooth part whose FP32 rounding the velocity's 1/h would amplify
+template <int TIN, int G, bool P2, bool PSI64 = false>
+__global__ void __launch_bounds__(256) k_map_advect32(DGrid gm, DGrid gp, const void *C1, const void *C12,
+                                                      const void *C0, const float4 *__restrict__ M,
                                                       cmm_advect_coeffs c, float *__restrict__ Mout, int nblocks) {
     constexpr int NPB = 256 / G;
     const int blk = xcd_block(blockIdx.x, nblocks);
@@ -2155,7 +2175,12 @@ __global__ void __launch_bounds__(256) k_map_advect32(DGrid gm, DGrid gp, const 
         const double xe = xn + ((k >> 2) + 1) * c.eps * (1 - 2 * ((k >> 1) & 1));
         const double ye = yn + ((k >> 2) + 1) * c.eps * (1 - 2 * (((k + 1) >> 1) & 1));
         double xf, yf, px, py;
-        step_backward<TIN>(VelCombined32<P2>{gp, C1, C12, C0}, c, xe, ye, xf, yf);
+        if constexpr (PSI64)
+            step_backward<TIN>(VelCombined<P2>{gp, (const double *)C1, (const double *)C12, (const double *)C0}, c, xe,
+                               ye, xf, yf);
+        else
+            step_backward<TIN>(VelCombined32<P2>{gp, (const float4 *)C1, (const float4 *)C12, (const float4 *)C0}, c,
+                               xe, ye, xf, yf);
         diffeo32<P2>(gm, M, xf, yf, px, py);
         qx = px - xn;
         qy = py - yn;
@@ -2392,50 +2417,67 @@ int op_interleave4_32(const cmm_grid &g, const double *H, float *H4, hipStream_t
 
 bool op_advect32_supported(const cmm_advect_coeffs &c) { return c.direction != 1 && advect_combinable(c); }
 
-size_t op_advect32_workspace(const cmm_grid &gp, const cmm_advect_coeffs &c) {
+size_t op_advect32_workspace(const cmm_grid &gp, const cmm_advect_coeffs &c, bool psi64) {
     if (!op_advect32_supported(c)) return 0;
     const int fields = 1 + advect_needs_c12(c) + advect_needs_c0(c);
-    return fields * sizeof(float4) * (size_t)gp.nx * gp.ny;
+    return fields * (psi64 ? 4 * sizeof(double) : sizeof(float4)) * (size_t)gp.nx * gp.ny;
 }
 
 int op_map_advect32(const cmm_grid &gm, const cmm_grid &gp, const PsiLevels &P, const cmm_advect_coeffs &c,
-                    const float *M_in, float *M_out, void *work, size_t work_bytes, hipStream_t st, hipEvent_t mid) {
+                    const float *M_in, float *M_out, void *work, size_t work_bytes, hipStream_t st, hipEvent_t mid,
+                    bool psi64) {
     if (!op_advect32_supported(c)) {
         set_error("split-precision advection: the time scheme reads psi levels on their own (AB2, RK3Mod), "
                   "needs two or more psi levels, and is backward only");
         return CMM_ERR_ARG;
     }
-    const size_t need = op_advect32_workspace(gp, c);
+    const size_t need = op_advect32_workspace(gp, c, psi64);
     void *w;
     if (int rc = get_work(work, work_bytes, need, &w)) return rc;
     DGrid dm = make_dgrid(gm), dp = make_dgrid(gp);
     const long n = dp.n;
-    float4 *next = static_cast<float4 *>(w);
-    float4 *C1 = next, *C12 = nullptr, *C0 = nullptr;
-    next += n;
-    if (advect_needs_c12(c)) { C12 = next; next += n; }
+    const size_t rec = psi64 ? 4 * sizeof(double) : sizeof(float4);
+    char *next = static_cast<char *>(w);
+    void *C1 = next, *C12 = nullptr, *C0 = nullptr;
+    next += rec * n;
+    if (advect_needs_c12(c)) { C12 = next; next += rec * n; }
     if (advect_needs_c0(c)) C0 = next;
     const dim3 g((unsigned)((n + 255) / 256));
-    switch (c.l_order) {
-    case 2: hipLaunchKernelGGL(k_psi_combine32<2>, g, dim3(256), 0, st, n, P, c, C1, C12, C0); break;
-    case 3: hipLaunchKernelGGL(k_psi_combine32<3>, g, dim3(256), 0, st, n, P, c, C1, C12, C0); break;
-    default: hipLaunchKernelGGL(k_psi_combine32<4>, g, dim3(256), 0, st, n, P, c, C1, C12, C0); break;
+    if (psi64) {
+        double *d1 = (double *)C1, *d12 = (double *)C12, *d0 = (double *)C0;
+        switch (c.l_order) {
+        case 2: hipLaunchKernelGGL(k_psi_combine<2>, g, dim3(256), 0, st, n, P, c, d1, d12, d0); break;
+        case 3: hipLaunchKernelGGL(k_psi_combine<3>, g, dim3(256), 0, st, n, P, c, d1, d12, d0); break;
+        default: hipLaunchKernelGGL(k_psi_combine<4>, g, dim3(256), 0, st, n, P, c, d1, d12, d0); break;
+        }
+    } else {
+        float4 *f1 = (float4 *)C1, *f12 = (float4 *)C12, *f0 = (float4 *)C0;
+        switch (c.l_order) {
+        case 2: hipLaunchKernelGGL(k_psi_combine32<2>, g, dim3(256), 0, st, n, P, c, f1, f12, f0); break;
+        case 3: hipLaunchKernelGGL(k_psi_combine32<3>, g, dim3(256), 0, st, n, P, c, f1, f12, f0); break;
+        default: hipLaunchKernelGGL(k_psi_combine32<4>, g, dim3(256), 0, st, n, P, c, f1, f12, f0); break;
+        }
     }
-    if (int rc = launch_check("k_psi_combine32")) return rc;
+    if (int rc = launch_check("k_psi_combine")) return rc;
     if (mid) (void)hipEventRecord(mid, st);
     const int nfoot = 4 + 4 * c.map_update_order_num;
     const int G = nfoot <= 4 ? 4 : nfoot <= 8 ? 8 : 16;
     const int nb = (int)(((long)dm.n * G + 255) / 256);
     const bool p2 = dm.lgx >= 0 && dm.lgy >= 0 && dp.lgx >= 0 && dp.lgy >= 0;
     const float4 *Mi = (const float4 *)M_in;
-#define CMM_A32(T, GG, PP)                                                                                             \
-    hipLaunchKernelGGL((k_map_advect32<T, GG, PP>), dim3(nb), dim3(256), 0, st, dm, dp, C1, C12, C0, Mi, c, M_out, nb)
+#define CMM_A32(T, GG, PP, S)                                                                                          \
+    hipLaunchKernelGGL((k_map_advect32<T, GG, PP, S>), dim3(nb), dim3(256), 0, st, dm, dp, C1, C12, C0, Mi, c, M_out, nb)
+#define CMM_A32_S(T, S)                                                                                                \
+    do {                                                                                                               \
+        if (G == 8 && p2) CMM_A32(T, 8, true, S);                                                                      \
+        else if (G == 4) CMM_A32(T, 4, false, S);                                                                      \
+        else if (G == 8) CMM_A32(T, 8, false, S);                                                                      \
+        else CMM_A32(T, 16, false, S);                                                                                 \
+    } while (0)
 #define CMM_A32_T(T)                                                                                                   \
     do {                                                                                                               \
-        if (G == 8 && p2) CMM_A32(T, 8, true);                                                                         \
-        else if (G == 4) CMM_A32(T, 4, false);                                                                         \
-        else if (G == 8) CMM_A32(T, 8, false);                                                                         \
-        else CMM_A32(T, 16, false);                                                                                    \
+        if (psi64) CMM_A32_S(T, true);                                                                                 \
+        else CMM_A32_S(T, false);                                                                                      \
     } while (0)
     switch (c.time_integration_num) {
     case 10: CMM_A32_T(10); break;
@@ -2446,22 +2488,23 @@ int op_map_advect32(const cmm_grid &gm, const cmm_grid &gp, const PsiLevels &P, 
     default: set_error("unknown time integration"); return CMM_ERR_ARG;
     }
 #undef CMM_A32_T
+#undef CMM_A32_S
 #undef CMM_A32
     return launch_check("k_map_advect32");
 }
 
 int op_incomp32_launch(const cmm_grid &gm, const cmm_grid &gc, const float *Mf, unsigned long long *slot,
-                       hipStream_t st) {
+                       hipStream_t st, IncompTune tune) {
     DGrid dm = make_dgrid(gm), dc = make_dgrid(gc);
     const float4 *M = (const float4 *)Mf;
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, slot);
     const int r = gc.nx / gm.nx;
     if (r >= 1 && gc.nx == r * gm.nx && gc.ny == r * gm.ny && gc.x0 == gm.x0 && gc.x1 == gm.x1 &&
         gc.y0 == gm.y0 && gc.y1 == gm.y1) {
-        const int T = r % 4 == 0 ? 4 : r % 3 == 0 ? 3 : r % 2 == 0 ? 2 : 1;
+        const int T = incomp_tile(r, tune);
         const int ntx = gc.nx / T;
         const long ntiles = (long)ntx * (gc.ny / T);
-        const unsigned nb = (unsigned)std::min<long>((ntiles + 255) / 256, 8192);
+        const unsigned nb = (unsigned)std::max<long>(1, std::min<long>((ntiles + 255) / 256, tune.max_blocks));
         if (T == 4) hipLaunchKernelGGL((k_incomp_tensor32<4, 4>), dim3(nb), dim3(256), 0, st, dm, dc, M, slot, ntx, ntiles);
         else if (T == 3) hipLaunchKernelGGL((k_incomp_tensor32<3, 3>), dim3(nb), dim3(256), 0, st, dm, dc, M, slot, ntx, ntiles);
         else if (T == 2) hipLaunchKernelGGL((k_incomp_tensor32<2, 2>), dim3(nb), dim3(256), 0, st, dm, dc, M, slot, ntx, ntiles);

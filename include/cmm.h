@@ -205,8 +205,8 @@ typedef struct cmm_euler_params {
                                        of stack_capacity maps (Cuda-CMM/src/simulation/cmm-euler2d.cu:156-159,
                                        386-397, 781-811) */
     int precision;                  /* CMM_PRECISION_FP64, or CMM_PRECISION_SPLIT: the split-precision
-                                       form of the map, the combined psi fields and the fine vorticity
-                                       (Euler, backward map only; no counterpart in the reference) */
+                                       form of the map, the combined psi fields and (Euler) the fine
+                                       vorticity (backward map only; no counterpart in the reference) */
 } cmm_euler_params;
 
 /* The 1D-1V Vlasov-Poisson loop (cuda_vlasov_1d, Cuda-CMM/src/simulation/cmm-vlasov1d.cu:48-1148)
@@ -322,6 +322,25 @@ int cmm_euler_state(cmm_euler *e, int *steps, double *t, int *stack_size);
 #define CMM_PROF_SLOTS 8
 int cmm_euler_profile(cmm_euler *e, int enable);
 int cmm_euler_profile_read(cmm_euler *e, double *ms_out, int *count_out);
+
+
+/* ---- ensembles (no counterpart in the reference) ------------------------------------------------
+ * Engines of any grid sizes stepped together: each step queues every engine's advection, check,
+ * E_inc copy and psi solve, then waits once and finishes each engine (remap when E_inc exceeds its
+ * threshold, in stream order as cmm_euler_step does).  With use_graphs, the queued work of all
+ * engines is captured once per step phase into one graph (one branch per engine stream) and
+ * replayed with one launch per step; steps of another length than the engines' nominal dt (a
+ * clipped last step) and each engine's first step run without a graph.  Engines must be
+ * initialised, without particles, a forward map or profiling; results equal cmm_euler_step's. */
+typedef struct cmm_batch cmm_batch;
+int cmm_batch_create(cmm_euler *const *engines, int n, int use_graphs, cmm_batch **out);
+int cmm_batch_destroy(cmm_batch *b);
+/* one step of length dt for every engine; infos: n entries (or NULL) */
+int cmm_batch_step(cmm_batch *b, double dt, cmm_step_info *infos);
+int cmm_batch_stats(cmm_batch *b, int *graphs_captured, int *graph_launches, int *eager_steps);
+/* Autotuner: times the incompressibility check's tile edge (4, 2, 1 dividing the fine/coarse
+ * ratio) and block cap on the engine's current map and keeps the fastest for its later steps. */
+int cmm_euler_autotune(cmm_euler *e, int *tile_out, int *max_blocks_out, double *ms_out);
 
 #ifdef __cplusplus
 }

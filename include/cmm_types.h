@@ -60,6 +60,10 @@ enum {
     CMM_IC_LANDAU_DAMPING = 200,        /* (1 + 0.05 cos 0.5x) exp(-v^2/2)/sqrt(2 pi) */
     CMM_IC_TWO_STREAM = 201,            /* (1 + 0.05 cos 0.2x)(exp(-(v-3)^2/2) + exp(-(v+3)^2/2))/(2 sqrt(2 pi)) */
     CMM_IC_NONLIN_LANDAU_DAMPING = 202, /* (1 + 0.5 cos 0.5x) exp(-v^2/2)/sqrt(2 pi) */
+    /* ensemble distribution functions (no reference counterpart), amplitude a = ic_params[0]:
+       (1 + a cos 0.5x) exp(-v^2/2)/sqrt(2 pi) and (1 + a cos 0.5x) v^2 exp(-v^2/2)/sqrt(2 pi) */
+    CMM_IC_LANDAU_AMP = 203,
+    CMM_IC_TWO_STREAM_V2 = 204,
     /* passive scalars, 300 + the reference's number (Cuda-CMM/src/ui/settings.h:431-434,
      * d_init_scalar, Cuda-CMM/src/simulation/cmm-init.cu:249-301) */
     CMM_IC_SCALAR_RECTANGLE = 300,
