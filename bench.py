@@ -379,6 +379,80 @@ def run_ensemble(args):
     print(json.dumps(out), flush=True)
 
 
+def run_learned(args):
+    """The learned psi extrapolation on the double-shear-layer grids (map 128^2, psi 256^2, fine
+    512^2) with seeded random-spectrum vorticity: psi trajectories recorded every 1/256 for seeds
+    0..n_train-1 (training) and 64..71 (held out), AdamW training through the HIP kernels, the
+    extrapolation errors with and without the network on the held-out seeds, and the held-out
+    runs at dt = 1/128 with and without the correction (fallback judge at 1e-6) against runs at
+    dt = 1/256: relative L2 error of the coarse vorticity at T = 1."""
+    import numpy as np
+    import torch
+    import cmm
+    sys.path.insert(0, os.path.join(ROOT, "cmm-turbulence_amd"))
+    import learned
+    t0 = time.perf_counter()
+    trajs = [learned.trajectory(s) for s in range(args.train_seeds)]
+    held = [learned.trajectory(s) for s in range(64, 72)]
+    t_data = time.perf_counter() - t0
+    net = learned.Net(seed=0)
+    learned.fit_scales(net, trajs)
+    t0 = time.perf_counter()
+    hist = learned.train(net, trajs, iters=args.train_iters, batch=8, lr=2e-3, log=log)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    del trajs
+    p1, l1, p12, l12 = learned.evaluate(net, held)
+    del held
+    g = cmm.grid
+    rl2 = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    runs = []
+    for seed in range(64, 72):
+        def engine(dt):
+            e = cmm.Euler(g(128), g(512), g(256), g(512), dt, initial_condition="random_spectrum", seed=seed,
+                          freq_cut_psi=128.0, stack_capacity=16)
+            e.init()
+            return e
+        ref = engine(1.0 / 256)
+        for _ in range(256):
+            ref.step(1.0 / 256)
+        w_ref = ref.field("w_coarse")
+        ref.close()
+        plain = engine(1.0 / 128)
+        torch.cuda.synchronize()
+        ta = time.perf_counter()
+        for _ in range(128):
+            plain.step(1.0 / 128)
+        torch.cuda.synchronize()
+        tp = time.perf_counter() - ta
+        w_plain = plain.field("w_coarse")
+        plain.close()
+        e = engine(1.0 / 128)
+        cor = learned.Corrector(net, e, g(256), tol=1e-6)
+        ta = time.perf_counter()
+        for _ in range(128):
+            cor.step(1.0 / 128)
+        torch.cuda.synchronize()
+        tl = time.perf_counter() - ta
+        w_l = e.field("w_coarse")
+        e.close()
+        runs.append({"seed": seed, "omega_rel_l2_plain": rl2(w_plain, w_ref), "omega_rel_l2_learned": rl2(w_l, w_ref),
+                     "fallback_step": cor.fallback_step, "mean_dev_plain": float(np.mean(cor.dev_plain)),
+                     "mean_dev_learned": float(np.mean(cor.dev_learned)) if cor.dev_learned else None,
+                     "ms_per_step_plain": 1e3 * tp / 128, "ms_per_step_learned": 1e3 * tl / 128})
+        log(f"[learned] seed {seed}: {runs[-1]}")
+    out = {"workload": "learned psi extrapolation: map 128^2, psi 256^2, fine 512^2, random-spectrum seeds; "
+                       f"training seeds 0..{args.train_seeds - 1}, held out 64..71; dt 1/128 against 1/256, T 1",
+           "network": "4 layers, 3x3, 32 channels, BF16 weights/activations, FP32 MFMA accumulation, AdamW",
+           "data_s": round(t_data, 2), "train_s": round(t_train, 2), "train_loss": hist,
+           "extrapolation_rel_l2_held_out": {"to_t_plus_dt": {"plain": p1, "learned": l1, "gain": p1 / l1},
+                                             "to_t_plus_half_dt": {"plain": p12, "learned": l12, "gain": p12 / l12}},
+           "runs": runs,
+           "omega_rel_l2_mean": {"plain": float(np.mean([r["omega_rel_l2_plain"] for r in runs])),
+                                 "learned": float(np.mean([r["omega_rel_l2_learned"] for r in runs]))}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "summarize-profiles":
         kt, fe, wr, out = sys.argv[2:6]
@@ -394,12 +468,19 @@ def main():
     ap.add_argument("--ensemble", action="store_true",
                     help="run the batched Vlasov-Poisson ensemble instead (its own JSON line)")
     ap.add_argument("--instances", type=int, default=32)
+    ap.add_argument("--learned", action="store_true",
+                    help="run the learned psi-extrapolation protocol instead (its own JSON line)")
+    ap.add_argument("--train-seeds", type=int, default=64)
+    ap.add_argument("--train-iters", type=int, default=1500)
     ap.add_argument("--t-end", type=float, default=40.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 1 GiB device copy, the reference kernel for counter scaling")
     args = ap.parse_args()
     if args.ensemble:
         run_ensemble(args)
+        return
+    if args.learned:
+        run_learned(args)
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs one engine instance per GPU and has no multi-GPU "

@@ -84,7 +84,11 @@ SYMBOLS = ["cmm_version", "cmm_last_error", "cmm_device_info", "cmm_lagrange_coe
            "cmm_euler_forward_positions", "cmm_euler_forward_points", "cmm_euler_state", "cmm_euler_profile", "cmm_euler_profile_read",
            "cmm_map_to_split", "cmm_map_from_split", "cmm_hermite_to_split", "cmm_map_advect_split_workspace_bytes",
            "cmm_map_advect_split", "cmm_incomp_error_split", "cmm_psi_from_vort_split",
-           "cmm_batch_create", "cmm_batch_destroy", "cmm_batch_step", "cmm_batch_stats", "cmm_euler_autotune"]
+           "cmm_batch_create", "cmm_batch_destroy", "cmm_batch_step", "cmm_batch_stats", "cmm_euler_autotune",
+           "cmm_cnn_forward", "cmm_cnn_backward_data", "cmm_cnn_backward_weights", "cmm_cnn_relu_mask",
+           "cmm_cnn_fill", "cmm_cnn_weights_bf16", "cmm_cnn_adamw", "cmm_cnn_input", "cmm_cnn_target", "cmm_cnn_mse",
+           "cmm_cnn_correction", "cmm_extrapolate", "cmm_dev_norms", "cmm_euler_copy_psi_level",
+           "cmm_euler_psi_level", "cmm_euler_set_psi_correction"]
 # engine precision forms (CMM_PRECISION_*)
 PRECISION = {"fp64": 0, "split": 1}
 
@@ -173,6 +177,23 @@ def lib():
         L.cmm_batch_step.argtypes = [VP, C.c_double, C.POINTER(StepInfo)]
         L.cmm_batch_stats.argtypes = [VP, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.cmm_euler_autotune.argtypes = [VP, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.cmm_cnn_forward.argtypes = [C.c_int, C.c_int, C.c_int, VP, VP, VP, C.c_int, VP, VP]
+        L.cmm_cnn_backward_data.argtypes = [C.c_int, C.c_int, C.c_int, VP, VP, VP, VP]
+        L.cmm_cnn_backward_weights.argtypes = [C.c_int, C.c_int, C.c_int, VP, VP, VP, VP, VP]
+        L.cmm_cnn_relu_mask.argtypes = [C.c_long, VP, VP, VP]
+        L.cmm_cnn_fill.argtypes = [C.c_long, VP, C.c_float, VP]
+        L.cmm_cnn_weights_bf16.argtypes = [C.c_int, VP, VP, VP, VP]
+        L.cmm_cnn_adamw.argtypes = [C.c_long, VP, VP, VP, VP, C.c_float, C.c_float, C.c_float, C.c_float, C.c_float,
+                                    C.c_float, C.c_int, VP]
+        L.cmm_cnn_input.argtypes = [C.c_long, VP, VP, VP, VP, VP, VP]
+        L.cmm_cnn_target.argtypes = [C.c_long, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP]
+        L.cmm_cnn_mse.argtypes = [C.c_long, VP, VP, VP, VP, VP]
+        L.cmm_cnn_correction.argtypes = [C.c_long, VP, VP, VP, VP, VP]
+        L.cmm_extrapolate.argtypes = [C.c_long, VP, VP, VP, VP, VP, VP, VP]
+        L.cmm_dev_norms.argtypes = [C.c_long, VP, VP, VP, VP]
+        L.cmm_euler_copy_psi_level.argtypes = [VP, C.c_int, C.c_int, VP]
+        L.cmm_euler_psi_level.argtypes = [VP, C.c_int, C.POINTER(VP)]
+        L.cmm_euler_set_psi_correction.argtypes = [VP, VP, VP]
         _lib = L
     return _lib
 

@@ -337,6 +337,9 @@ struct cmm_euler {
     // graph); capturing: the step's work is being captured (no event records)
     bool dry = false, capturing = false;
     IncompTune tune;  // the check's tile and grid, from cmm_euler_autotune
+    // corrections (device Hermite fields on g_psi) added to the Lagrange extrapolations of psi
+    // to t_{n+1} and t_n + dt/2 in the next steps (cmm_euler_set_psi_correction)
+    const double *corr1 = nullptr, *corr12 = nullptr;
     int nstack = 0;
     // forward map (forward_map only): current, next, and its stack, pushed with the backward one
     double *fx = nullptr, *fy = nullptr, *nfx = nullptr, *nfy = nullptr, *fstack = nullptr;
@@ -460,7 +463,8 @@ int engine_advect(cmm_euler *e, const double *t, const double *dt, int loop_ctr,
             // combined fields stay FP64 records
             if (int rc = op_map_advect32(e->p.g_coarse, e->p.g_psi, P, c, e->m32[e->m32_cur], e->m32[1 - e->m32_cur],
                                          e->work, e->work_bytes, e->st, e->prof ? e->ev[10] : nullptr,
-                                         e->p.sim == CMM_SIM_VLASOV_POISSON_1D))
+                                         e->p.sim == CMM_SIM_VLASOV_POISSON_1D, e->initialised ? e->corr1 : nullptr,
+                                         e->initialised ? e->corr12 : nullptr))
                 return rc;
         e->m32_cur = 1 - e->m32_cur;
         e->m32_fresh = true;
@@ -479,7 +483,8 @@ int engine_advect(cmm_euler *e, const double *t, const double *dt, int loop_ctr,
     double *m8_out = comb ? e->m8[1 - e->m8_cur] : nullptr;
     if (!e->dry)
         if (int rc = op_map_advect(e->p.g_coarse, e->p.g_psi, e->cx, e->cy, P, c, e->ncx, e->ncy, e->work,
-                                   e->work_bytes, e->st, e->prof ? e->ev[10] : nullptr, m8_in, m8_out))
+                                   e->work_bytes, e->st, e->prof ? e->ev[10] : nullptr, m8_in, m8_out,
+                                   e->initialised ? e->corr1 : nullptr, e->initialised ? e->corr12 : nullptr))
             return rc;
     std::swap(e->cx, e->ncx);
     std::swap(e->cy, e->ncy);
@@ -1247,6 +1252,33 @@ int cmm_euler_autotune(cmm_euler *e, int *tile_out, int *max_blocks_out, double 
     if (tile_out) *tile_out = best.tile;
     if (max_blocks_out) *max_blocks_out = best.max_blocks;
     if (ms_out) *ms_out = best_ms;
+    return CMM_OK;
+}
+
+int cmm_euler_psi_level(cmm_euler *e, int level, const double **dev_ptr) {
+    if (!e || !dev_ptr || level < 0 || level >= e->p.l_order) { set_error("cmm_euler_psi_level: bad argument"); return CMM_ERR_ARG; }
+    *dev_ptr = e->lvl[level];
+    return CMM_OK;
+}
+
+int cmm_euler_copy_psi_level(cmm_euler *e, int level, int plane, double *dst) {
+    if (!e || !dst || level < 0 || level >= e->p.l_order || plane < 0 || plane > 3) {
+        set_error("cmm_euler_copy_psi_level: bad argument");
+        return CMM_ERR_ARG;
+    }
+    return hip_check(hipMemcpyAsync(dst, e->lvl[level] + (size_t)plane * e->np, sizeof(double) * e->np,
+                                    hipMemcpyDeviceToDevice, e->st),
+                     "hipMemcpyAsync");
+}
+
+int cmm_euler_set_psi_correction(cmm_euler *e, const double *d1, const double *d12) {
+    if (!e) { set_error("cmm_euler_set_psi_correction: null engine"); return CMM_ERR_ARG; }
+    if ((d1 || d12) && (e->p.l_order < 2 || e->p.forward_map)) {
+        set_error("cmm_euler_set_psi_correction: needs two or more psi levels and no forward map");
+        return CMM_ERR_ARG;
+    }
+    e->corr1 = d1;
+    e->corr12 = d12;
     return CMM_OK;
 }
 

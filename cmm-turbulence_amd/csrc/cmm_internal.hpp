@@ -20,7 +20,8 @@ int op_map_identity(const cmm_grid &g, double *cx, double *cy, hipStream_t st);
 size_t op_advect_workspace(const cmm_grid &gm, const cmm_grid &gp, const cmm_advect_coeffs &c);
 int op_map_advect(const cmm_grid &gm, const cmm_grid &gp, const double *cx, const double *cy, const PsiLevels &P,
                   const cmm_advect_coeffs &c, double *ncx, double *ncy, void *work, size_t work_bytes,
-                  hipStream_t st, hipEvent_t mid = nullptr, const double *m8_in = nullptr, double *m8_out = nullptr);
+                  hipStream_t st, hipEvent_t mid = nullptr, const double *m8_in = nullptr, double *m8_out = nullptr,
+                  const double *corr1 = nullptr, const double *corr12 = nullptr);
 // tile (0: the largest of 4, 3, 2, 1 dividing the refinement ratio) and cap on the number of
 // blocks of the tiled check; set by the autotuner
 struct IncompTune {
@@ -84,7 +85,8 @@ bool op_advect32_supported(const cmm_advect_coeffs &c);
 size_t op_advect32_workspace(const cmm_grid &gp, const cmm_advect_coeffs &c, bool psi64 = false);
 int op_map_advect32(const cmm_grid &gm, const cmm_grid &gp, const PsiLevels &P, const cmm_advect_coeffs &c,
                     const float *M_in, float *M_out, void *work, size_t work_bytes, hipStream_t st,
-                    hipEvent_t mid = nullptr, bool psi64 = false);
+                    hipEvent_t mid = nullptr, bool psi64 = false, const double *corr1 = nullptr,
+                    const double *corr12 = nullptr);
 int op_incomp32_launch(const cmm_grid &gm, const cmm_grid &gc, const float *M, unsigned long long *slot,
                        hipStream_t st, IncompTune tune = IncompTune());
 int op_incomp32(const cmm_grid &gm, const cmm_grid &gc, const float *M, double *out, hipStream_t st);

@@ -267,9 +267,12 @@ __device__ __forceinline__ void step_forward(const V &vel, const cmm_advect_coef
 // with L12 (when set), and C0[4j + p] = psi_0[p*n + j] (when set); sums in level order like the
 // weighted velocities.  One lane per node: plane reads coalesce across lanes, and each lane writes
 // its 32-byte records with 16-byte stores.
+// D1 / D12 (optional, plane-major Hermite fields): corrections added to the extrapolations C1 / C12
 template <int L>
 __global__ void __launch_bounds__(256) k_psi_combine(long n, PsiLevels P, cmm_advect_coeffs c, double *__restrict__ C1,
-                                                     double *__restrict__ C12, double *__restrict__ C0) {
+                                                     double *__restrict__ C12, double *__restrict__ C0,
+                                                     const double *__restrict__ D1 = nullptr,
+                                                     const double *__restrict__ D12 = nullptr) {
     const long j = (long)blockIdx.x * 256 + threadIdx.x;
     if (j >= n) return;
     double v[L][4];
@@ -277,21 +280,22 @@ __global__ void __launch_bounds__(256) k_psi_combine(long n, PsiLevels P, cmm_ad
     for (int l = 0; l < L; ++l)
 #pragma unroll
         for (int p = 0; p < 4; ++p) v[l][p] = P.p[l][p * n + j];
-    auto comb = [&](const double *w, double *dst) {
+    auto comb = [&](const double *w, double *dst, const double *D) {
         double o[4];
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
             double a = w[0] * v[0][p];
 #pragma unroll
             for (int l = 1; l < L; ++l) a += w[l] * v[l][p];
+            if (D) a += D[p * n + j];
             o[p] = a;
         }
         double2 *d = reinterpret_cast<double2 *>(dst + 4 * j);
         d[0] = double2{o[0], o[1]};
         d[1] = double2{o[2], o[3]};
     };
-    comb(c.L1, C1);
-    if (C12) comb(c.L12, C12);
+    comb(c.L1, C1, D1);
+    if (C12) comb(c.L12, C12, D12);
     if (C0) {
         double2 *d = reinterpret_cast<double2 *>(C0 + 4 * j);
         d[0] = double2{v[0][0], v[0][1]};
@@ -1539,7 +1543,8 @@ size_t op_advect_workspace(const cmm_grid &gm, const cmm_grid &gp, const cmm_adv
 
 int op_map_advect(const cmm_grid &gm, const cmm_grid &gp, const double *cx, const double *cy, const PsiLevels &P,
                   const cmm_advect_coeffs &c, double *ncx, double *ncy, void *work, size_t work_bytes,
-                  hipStream_t st, hipEvent_t mid, const double *m8_in, double *m8_out) {
+                  hipStream_t st, hipEvent_t mid, const double *m8_in, double *m8_out, const double *corr1,
+                  const double *corr12) {
     DGrid dm = make_dgrid(gm), dp = make_dgrid(gp);
     const int nfoot = 4 + 4 * c.map_update_order_num;
     const int G = nfoot <= 4 ? 4 : nfoot <= 8 ? 8 : 16;
@@ -1569,6 +1574,10 @@ int op_map_advect(const cmm_grid &gm, const cmm_grid &gp, const double *cx, cons
     }
     const size_t need = op_advect_workspace(gm, gp, c);
     const bool comb = work && need > 0 && work_bytes >= need;
+    if ((corr1 || corr12) && !comb) {
+        set_error("map advection: a psi correction needs a combinable scheme and the combination workspace");
+        return CMM_ERR_ARG;
+    }
     const double *M8 = nullptr;
     double *C1 = nullptr, *C12 = nullptr, *C0 = nullptr;
     if (comb) {
@@ -1590,9 +1599,9 @@ int op_map_advect(const cmm_grid &gm, const cmm_grid &gp, const double *cx, cons
         }
         const dim3 g((unsigned)((n + 255) / 256));
         switch (c.l_order) {
-        case 2: hipLaunchKernelGGL(k_psi_combine<2>, g, dim3(256), 0, st, n, P, c, C1, C12, C0); break;
-        case 3: hipLaunchKernelGGL(k_psi_combine<3>, g, dim3(256), 0, st, n, P, c, C1, C12, C0); break;
-        default: hipLaunchKernelGGL(k_psi_combine<4>, g, dim3(256), 0, st, n, P, c, C1, C12, C0); break;
+        case 2: hipLaunchKernelGGL(k_psi_combine<2>, g, dim3(256), 0, st, n, P, c, C1, C12, C0, corr1, corr12); break;
+        case 3: hipLaunchKernelGGL(k_psi_combine<3>, g, dim3(256), 0, st, n, P, c, C1, C12, C0, corr1, corr12); break;
+        default: hipLaunchKernelGGL(k_psi_combine<4>, g, dim3(256), 0, st, n, P, c, C1, C12, C0, corr1, corr12); break;
         }
         if (int rc = launch_check("k_psi_combine")) return rc;
     }
@@ -2095,7 +2104,8 @@ struct VelCombined32 {
 // k_psi_combine with FP32 records: the sums in FP64, rounded once
 template <int L>
 __global__ void __launch_bounds__(256) k_psi_combine32(long n, PsiLevels P, cmm_advect_coeffs c, float4 *__restrict__ C1,
-                                                       float4 *__restrict__ C12, float4 *__restrict__ C0) {
+                                                       float4 *__restrict__ C12, float4 *__restrict__ C0,
+                                                       const double *__restrict__ D1, const double *__restrict__ D12) {
     const long j = (long)blockIdx.x * 256 + threadIdx.x;
     if (j >= n) return;
     double v[L][4];
@@ -2103,19 +2113,20 @@ __global__ void __launch_bounds__(256) k_psi_combine32(long n, PsiLevels P, cmm_
     for (int l = 0; l < L; ++l)
 #pragma unroll
         for (int p = 0; p < 4; ++p) v[l][p] = P.p[l][p * n + j];
-    auto comb = [&](const double *w, float4 *dst) {
+    auto comb = [&](const double *w, float4 *dst, const double *D) {
         double o[4];
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
             double a = w[0] * v[0][p];
 #pragma unroll
             for (int l = 1; l < L; ++l) a += w[l] * v[l][p];
+            if (D) a += D[p * n + j];
             o[p] = a;
         }
         dst[j] = float4{(float)o[0], (float)o[1], (float)o[2], (float)o[3]};
     };
-    comb(c.L1, C1);
-    if (C12) comb(c.L12, C12);
+    comb(c.L1, C1, D1);
+    if (C12) comb(c.L12, C12, D12);
     if (C0) C0[j] = float4{(float)v[0][0], (float)v[0][1], (float)v[0][2], (float)v[0][3]};
 }
 
@@ -2425,7 +2436,7 @@ size_t op_advect32_workspace(const cmm_grid &gp, const cmm_advect_coeffs &c, boo
 
 int op_map_advect32(const cmm_grid &gm, const cmm_grid &gp, const PsiLevels &P, const cmm_advect_coeffs &c,
                     const float *M_in, float *M_out, void *work, size_t work_bytes, hipStream_t st, hipEvent_t mid,
-                    bool psi64) {
+                    bool psi64, const double *corr1, const double *corr12) {
     if (!op_advect32_supported(c)) {
         set_error("split-precision advection: the time scheme reads psi levels on their own (AB2, RK3Mod), "
                   "needs two or more psi levels, and is backward only");
@@ -2446,16 +2457,16 @@ int op_map_advect32(const cmm_grid &gm, const cmm_grid &gp, const PsiLevels &P, 
     if (psi64) {
         double *d1 = (double *)C1, *d12 = (double *)C12, *d0 = (double *)C0;
         switch (c.l_order) {
-        case 2: hipLaunchKernelGGL(k_psi_combine<2>, g, dim3(256), 0, st, n, P, c, d1, d12, d0); break;
-        case 3: hipLaunchKernelGGL(k_psi_combine<3>, g, dim3(256), 0, st, n, P, c, d1, d12, d0); break;
-        default: hipLaunchKernelGGL(k_psi_combine<4>, g, dim3(256), 0, st, n, P, c, d1, d12, d0); break;
+        case 2: hipLaunchKernelGGL(k_psi_combine<2>, g, dim3(256), 0, st, n, P, c, d1, d12, d0, corr1, corr12); break;
+        case 3: hipLaunchKernelGGL(k_psi_combine<3>, g, dim3(256), 0, st, n, P, c, d1, d12, d0, corr1, corr12); break;
+        default: hipLaunchKernelGGL(k_psi_combine<4>, g, dim3(256), 0, st, n, P, c, d1, d12, d0, corr1, corr12); break;
         }
     } else {
         float4 *f1 = (float4 *)C1, *f12 = (float4 *)C12, *f0 = (float4 *)C0;
         switch (c.l_order) {
-        case 2: hipLaunchKernelGGL(k_psi_combine32<2>, g, dim3(256), 0, st, n, P, c, f1, f12, f0); break;
-        case 3: hipLaunchKernelGGL(k_psi_combine32<3>, g, dim3(256), 0, st, n, P, c, f1, f12, f0); break;
-        default: hipLaunchKernelGGL(k_psi_combine32<4>, g, dim3(256), 0, st, n, P, c, f1, f12, f0); break;
+        case 2: hipLaunchKernelGGL(k_psi_combine32<2>, g, dim3(256), 0, st, n, P, c, f1, f12, f0, corr1, corr12); break;
+        case 3: hipLaunchKernelGGL(k_psi_combine32<3>, g, dim3(256), 0, st, n, P, c, f1, f12, f0, corr1, corr12); break;
+        default: hipLaunchKernelGGL(k_psi_combine32<4>, g, dim3(256), 0, st, n, P, c, f1, f12, f0, corr1, corr12); break;
         }
     }
     if (int rc = launch_check("k_psi_combine")) return rc;

@@ -1,0 +1,398 @@
+// Learned psi extrapolation (a chosen form with no counterpart in the reference): a 3x3
+// convolutional network whose layers all have 32 channels (the 3 input and 2 output channels are
+// zero-padded to 32), BF16 weights and activations in NHWC layout, FP32 accumulation on the matrix
+// cores (v_mfma_f32_32x32x8_bf16), periodic padding.  Forward, backward-data and backward-weight
+// kernels, the optimiser and the glue between the psi history and the network.
+//
+// MFMA 32x32x8 operand layout (wave of 64 lanes, i = lane % 32, h = lane / 32):
+//   A[m][k] with m = i, k = 4h..4h+3 (four bf16 per lane), B[k][n] with n = i, k = 4h..4h+3,
+//   D[m][n] with n = i and m = 8 (r / 4) + 4 h + r % 4 for the lane's 16 results r.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "cmm_internal.hpp"
+#include "../../include/cmm.h"
+
+namespace cmm {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+constexpr int kCh = 32;  // channels of every layer
+
+// round to nearest even; NaN stays NaN
+__device__ __forceinline__ unsigned short f2bf(float f) {
+    unsigned u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (unsigned short)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (unsigned short)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float((unsigned)h << 16); }
+
+__device__ __forceinline__ int wrapi(int v, int n) { return v < 0 ? v + n : v >= n ? v - n : v; }
+
+// out[n][y][x][co] = act(b[co] + sum_{t, ci} W[co][t][ci] in[n][y + dy_t][x + dx_t][ci]), tap
+// t = 3 (dy + 1) + (dx + 1), periodic in y and x.  One wave per 32 pixels of a row: M = pixels,
+// N = output channels, K = 9 taps x 32 channels (36 MFMAs).  MODE 0: bf16 out with ReLU, 1: bf16
+// out, 2: fp32 out.  The backward-data pass is this kernel on the output gradient with the
+// weights transposed in channels and flipped in taps.
+template <int MODE>
+__global__ void __launch_bounds__(256) k_conv3(const unsigned short *__restrict__ in, const unsigned short *__restrict__ W,
+                                               const float *__restrict__ bias, void *__restrict__ out, int H, int Wd,
+                                               int segs) {
+    const int lane = threadIdx.x & 63;
+    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (wave >= H * segs) return;  // uniform over the wave
+    const long img = (long)blockIdx.y * H * Wd;
+    const int y = wave / segs, x0 = (wave % segs) * 32;
+    const int i = lane & 31, h = lane >> 5;
+    f32x16 acc = {};
+    for (int t = 0; t < 9; ++t) {
+        const int yy = wrapi(y + t / 3 - 1, H), xx = wrapi(x0 + i + t % 3 - 1, Wd);
+        const unsigned short *ap = in + (img + (long)yy * Wd + xx) * kCh + 4 * h;
+        const unsigned short *bp = W + ((long)i * 9 + t) * kCh + 4 * h;
+#pragma unroll
+        for (int cb = 0; cb < kCh; cb += 8) {
+            const bf16x4 a = *reinterpret_cast<const bf16x4 *>(ap + cb);
+            const bf16x4 b = *reinterpret_cast<const bf16x4 *>(bp + cb);
+            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+        }
+    }
+    const float bb = bias ? bias[i] : 0.f;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int p = 8 * (r >> 2) + 4 * h + (r & 3);
+        float v = acc[r] + bb;
+        if (MODE == 0) v = fmaxf(v, 0.f);
+        const long o = (img + (long)y * Wd + x0 + p) * kCh + i;
+        if (MODE == 2) static_cast<float *>(out)[o] = v;
+        else static_cast<unsigned short *>(out)[o] = f2bf(v);
+    }
+}
+
+// dW[co][t][ci] += sum_p g[p][co] in[p + off_t][ci] over `chunk` pixels per wave (grid y = tap):
+// M = output channels, N = input channels, K = pixels (8 per MFMA); the wave's 32 x 32 block is
+// added to dW with FP32 atomics.
+__global__ void __launch_bounds__(256) k_conv3_wgrad(const unsigned short *__restrict__ g,
+                                                     const unsigned short *__restrict__ in, float *__restrict__ dW,
+                                                     int H, int Wd, long npix, int chunk) {
+    const int lane = threadIdx.x & 63;
+    const long wave = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const long p0 = wave * chunk;
+    if (p0 >= npix) return;  // uniform over the wave
+    const int t = blockIdx.y, dy = t / 3 - 1, dx = t % 3 - 1;
+    const int i = lane & 31, h = lane >> 5;
+    const long HW = (long)H * Wd;
+    const long pend = p0 + chunk < npix ? p0 + chunk : npix;
+    f32x16 acc = {};
+    for (long pb = p0; pb < pend; pb += 8) {
+        bf16x4 a, b;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const long p = pb + 4 * h + j;
+            const long n = p / HW;
+            const int rem = (int)(p - n * HW);
+            const int yy = wrapi(rem / Wd + dy, H), xx = wrapi(rem % Wd + dx, Wd);
+            a[j] = (short)g[p * kCh + i];
+            b[j] = (short)in[(n * HW + (long)yy * Wd + xx) * kCh + i];
+        }
+        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+        const int co = 8 * (r >> 2) + 4 * h + (r & 3);
+        atomicAdd(&dW[((long)co * 9 + t) * kCh + i], acc[r]);
+    }
+}
+
+// db[co] += sum_p g[p][co]
+__global__ void __launch_bounds__(256) k_bias_grad(const unsigned short *__restrict__ g, float *__restrict__ db,
+                                                   long npix) {
+    const int co = threadIdx.x & 31;
+    float s = 0.f;
+    for (long p = (long)blockIdx.x * 8 + (threadIdx.x >> 5); p < npix; p += (long)gridDim.x * 8)
+        s += bf2f(g[p * kCh + co]);
+    atomicAdd(&db[co], s);
+}
+
+// g = g where the layer's output was positive (ReLU derivative)
+__global__ void __launch_bounds__(256) k_relu_mask(long n, const unsigned short *__restrict__ act,
+                                                   unsigned short *__restrict__ g) {
+    const long j = (long)blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    const unsigned short a = act[j];
+    if ((a & 0x8000u) || (a & 0x7fffu) == 0) g[j] = 0;
+}
+
+__global__ void __launch_bounds__(256) k_fill_f32(long n, float *__restrict__ p, float v) {
+    const long j = (long)blockIdx.x * 256 + threadIdx.x;
+    if (j < n) p[j] = v;
+}
+
+// bf16 copies of the FP32 master weights of n_layers layers (each 32 x 9 x 32, W[co][t][ci]) and
+// their transposed, tap-flipped copies Wt[ci][8 - t][co] for the backward-data pass
+__global__ void __launch_bounds__(256) k_weights_bf16(long n, const float *__restrict__ w32,
+                                                      unsigned short *__restrict__ w, unsigned short *__restrict__ wt) {
+    const long j = (long)blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    const long per = (long)kCh * 9 * kCh;
+    const long l = j / per;
+    const int r = (int)(j - l * per);
+    const int co = r / (9 * kCh), t = (r / kCh) % 9, ci = r % kCh;
+    const unsigned short v = f2bf(w32[j]);
+    w[j] = v;
+    wt[l * per + ((long)ci * 9 + (8 - t)) * kCh + co] = v;
+}
+
+// AdamW (decoupled weight decay) on FP32 parameters; the gradient is scaled by gscale first
+__global__ void __launch_bounds__(256) k_adamw(long n, float *__restrict__ p, const float *__restrict__ g,
+                                               float *__restrict__ m, float *__restrict__ v, float lr, float b1,
+                                               float b2, float eps, float wd, float gscale, float bc1, float bc2) {
+    const long j = (long)blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    const float gj = g[j] * gscale;
+    const float mj = b1 * m[j] + (1.f - b1) * gj;
+    const float vj = b2 * v[j] + (1.f - b2) * gj * gj;
+    m[j] = mj;
+    v[j] = vj;
+    const float mh = mj / bc1, vh = vj / bc2;
+    p[j] -= lr * (mh / (sqrtf(vh) + eps) + wd * p[j]);
+}
+
+// network input from three consecutive psi value planes (newest first): psi_n / s0,
+// (psi_n - psi_{n-1}) / s1, (psi_n - 2 psi_{n-1} + psi_{n-2}) / s2 in channels 0..2, zeros after
+__global__ void __launch_bounds__(256) k_cnn_input(long n, const double *__restrict__ p0, const double *__restrict__ p1,
+                                                   const double *__restrict__ p2, float s0, float s1, float s2,
+                                                   unsigned short *__restrict__ out) {
+    const long j = (long)blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    const double a = p0[j], b = p1[j], c = p2[j];
+    unsigned short *o = out + j * kCh;
+    o[0] = f2bf((float)(a / s0));
+    o[1] = f2bf((float)((a - b) / s1));
+    o[2] = f2bf((float)((a - 2.0 * b + c) / s2));
+    for (int k = 3; k < kCh; ++k) o[k] = 0;
+}
+
+// Lagrange-weighted extrapolation of three psi value planes plus an optional correction
+__global__ void __launch_bounds__(256) k_extrapolate(long n, const double *__restrict__ p0, const double *__restrict__ p1,
+                                                     const double *__restrict__ p2, double w0, double w1, double w2,
+                                                     const double *__restrict__ d, double *__restrict__ out) {
+    const long j = (long)blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    double v = w0 * p0[j];
+    v += w1 * p1[j];
+    v += w2 * p2[j];
+    if (d) v += d[j];
+    out[j] = v;
+}
+
+// training targets: residuals of the Lagrange extrapolations to t_{n+1} and t_n + dt/2, scaled
+__global__ void __launch_bounds__(256) k_cnn_target(long n, const double *__restrict__ p0, const double *__restrict__ p1,
+                                                    const double *__restrict__ p2, const double *__restrict__ pn,
+                                                    const double *__restrict__ ph, double a0, double a1, double a2,
+                                                    double c0, double c1, double c2, float r1, float r12,
+                                                    float *__restrict__ tgt) {
+    const long j = (long)blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    double e1 = a0 * p0[j];
+    e1 += a1 * p1[j];
+    e1 += a2 * p2[j];
+    double e12 = c0 * p0[j];
+    e12 += c1 * p1[j];
+    e12 += c2 * p2[j];
+    tgt[2 * j] = (float)((pn[j] - e1) / r1);
+    tgt[2 * j + 1] = (float)((ph[j] - e12) / r12);
+}
+
+// squared-error loss of the two predicted channels and its gradient (pred - target) in bf16;
+// the sum of squares is accumulated per block and added once
+__global__ void __launch_bounds__(256) k_cnn_mse(long n, const float *__restrict__ pred, const float *__restrict__ tgt,
+                                                 unsigned short *__restrict__ g, double *__restrict__ loss) {
+    const long j = (long)blockIdx.x * 256 + threadIdx.x;
+    double s = 0;
+    if (j < n) {
+        unsigned short *o = g + j * kCh;
+        for (int c = 0; c < 2; ++c) {
+            const float d = pred[j * kCh + c] - tgt[2 * j + c];
+            o[c] = f2bf(d);
+            s += (double)d * d;
+        }
+        for (int c = 2; c < kCh; ++c) o[c] = 0;
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    __shared__ double red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(loss, red[0] + red[1] + red[2] + red[3]);
+}
+
+// correction value fields from the network output: d1 = r1 pred[.][0], d12 = r12 pred[.][1]
+__global__ void __launch_bounds__(256) k_cnn_correction(long n, const float *__restrict__ pred, float r1, float r12,
+                                                        double *__restrict__ d1, double *__restrict__ d12) {
+    const long j = (long)blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    d1[j] = (double)r1 * pred[j * kCh];
+    d12[j] = (double)r12 * pred[j * kCh + 1];
+}
+
+// sum (a - b)^2 and sum b^2 (two doubles, accumulated)
+__global__ void __launch_bounds__(256) k_dev_norms(long n, const double *__restrict__ a, const double *__restrict__ b,
+                                                   double *__restrict__ out) {
+    double s = 0, t = 0;
+    for (long j = (long)blockIdx.x * 256 + threadIdx.x; j < n; j += (long)gridDim.x * 256) {
+        const double d = a[j] - b[j];
+        s += d * d;
+        t += b[j] * b[j];
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        s += __shfl_xor(s, off, 64);
+        t += __shfl_xor(t, off, 64);
+    }
+    __shared__ double red[8];
+    if ((threadIdx.x & 63) == 0) {
+        red[threadIdx.x >> 6] = s;
+        red[4 + (threadIdx.x >> 6)] = t;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicAdd(out, red[0] + red[1] + red[2] + red[3]);
+        atomicAdd(out + 1, red[4] + red[5] + red[6] + red[7]);
+    }
+}
+
+__global__ void k_zero2(double *p) {
+    p[0] = 0.0;
+    p[1] = 0.0;
+}
+
+static unsigned blocks(long n) { return (unsigned)((n + 255) / 256); }
+static int lc(const char *what) { return hip_check(hipGetLastError(), what); }
+
+}  // namespace cmm
+
+using namespace cmm;
+
+namespace {
+hipStream_t S(void *s) { return reinterpret_cast<hipStream_t>(s); }
+bool img_ok(int n_img, int H, int W) { return n_img > 0 && H > 0 && W >= 32 && W % 32 == 0; }
+}  // namespace
+
+extern "C" {
+
+int cmm_cnn_forward(int n_img, int H, int W, const uint16_t *in, const uint16_t *w, const float *b, int mode, void *out,
+                    void *stream) {
+    if (!img_ok(n_img, H, W) || !in || !w || !out || mode < 0 || mode > 2) {
+        set_error("cmm_cnn_forward: bad argument (width a multiple of 32)");
+        return CMM_ERR_ARG;
+    }
+    const int segs = W / 32;
+    const dim3 grid((unsigned)((H * segs + 3) / 4), (unsigned)n_img);
+    if (mode == 0) hipLaunchKernelGGL(k_conv3<0>, grid, dim3(256), 0, S(stream), in, w, b, out, H, W, segs);
+    else if (mode == 1) hipLaunchKernelGGL(k_conv3<1>, grid, dim3(256), 0, S(stream), in, w, b, out, H, W, segs);
+    else hipLaunchKernelGGL(k_conv3<2>, grid, dim3(256), 0, S(stream), in, w, b, out, H, W, segs);
+    return lc("k_conv3");
+}
+
+int cmm_cnn_backward_data(int n_img, int H, int W, const uint16_t *g, const uint16_t *wt, uint16_t *gin, void *stream) {
+    return cmm_cnn_forward(n_img, H, W, g, wt, nullptr, 1, gin, stream);
+}
+
+int cmm_cnn_backward_weights(int n_img, int H, int W, const uint16_t *g, const uint16_t *in, float *dw, float *db,
+                             void *stream) {
+    if (!img_ok(n_img, H, W) || !g || !in || !dw || !db) {
+        set_error("cmm_cnn_backward_weights: bad argument");
+        return CMM_ERR_ARG;
+    }
+    const long npix = (long)n_img * H * W;
+    const int chunk = 256;
+    const long waves = (npix + chunk - 1) / chunk;
+    hipLaunchKernelGGL(k_conv3_wgrad, dim3((unsigned)((waves + 3) / 4), 9), dim3(256), 0, S(stream), g, in, dw, H, W,
+                       npix, chunk);
+    if (int rc = lc("k_conv3_wgrad")) return rc;
+    const unsigned nb = (unsigned)std::min<long>(1024, (npix + 7) / 8);
+    hipLaunchKernelGGL(k_bias_grad, dim3(nb), dim3(256), 0, S(stream), g, db, npix);
+    return lc("k_bias_grad");
+}
+
+int cmm_cnn_relu_mask(long n, const uint16_t *act, uint16_t *g, void *stream) {
+    if (n < 0 || (n > 0 && (!act || !g))) { set_error("cmm_cnn_relu_mask: bad argument"); return CMM_ERR_ARG; }
+    if (n == 0) return CMM_OK;
+    hipLaunchKernelGGL(k_relu_mask, dim3(blocks(n)), dim3(256), 0, S(stream), n, act, g);
+    return lc("k_relu_mask");
+}
+
+int cmm_cnn_fill(long n, float *p, float v, void *stream) {
+    if (n < 0 || (n > 0 && !p)) { set_error("cmm_cnn_fill: bad argument"); return CMM_ERR_ARG; }
+    if (n == 0) return CMM_OK;
+    hipLaunchKernelGGL(k_fill_f32, dim3(blocks(n)), dim3(256), 0, S(stream), n, p, v);
+    return lc("k_fill_f32");
+}
+
+int cmm_cnn_weights_bf16(int n_layers, const float *w32, uint16_t *w, uint16_t *wt, void *stream) {
+    if (n_layers <= 0 || !w32 || !w || !wt) { set_error("cmm_cnn_weights_bf16: bad argument"); return CMM_ERR_ARG; }
+    const long n = (long)n_layers * kCh * 9 * kCh;
+    hipLaunchKernelGGL(k_weights_bf16, dim3(blocks(n)), dim3(256), 0, S(stream), n, w32, w, wt);
+    return lc("k_weights_bf16");
+}
+
+int cmm_cnn_adamw(long n, float *p, const float *g, float *m, float *v, float lr, float b1, float b2, float eps,
+                  float wd, float gscale, int step, void *stream) {
+    if (n <= 0 || !p || !g || !m || !v || step < 1) { set_error("cmm_cnn_adamw: bad argument"); return CMM_ERR_ARG; }
+    const float bc1 = 1.f - std::pow(b1, (float)step), bc2 = 1.f - std::pow(b2, (float)step);
+    hipLaunchKernelGGL(k_adamw, dim3(blocks(n)), dim3(256), 0, S(stream), n, p, g, m, v, lr, b1, b2, eps, wd, gscale,
+                       bc1, bc2);
+    return lc("k_adamw");
+}
+
+int cmm_cnn_input(long n, const double *psi0, const double *psi1, const double *psi2, const float *scales,
+                  uint16_t *out, void *stream) {
+    if (n <= 0 || !psi0 || !psi1 || !psi2 || !scales || !out) { set_error("cmm_cnn_input: bad argument"); return CMM_ERR_ARG; }
+    hipLaunchKernelGGL(k_cnn_input, dim3(blocks(n)), dim3(256), 0, S(stream), n, psi0, psi1, psi2, scales[0], scales[1],
+                       scales[2], out);
+    return lc("k_cnn_input");
+}
+
+int cmm_cnn_target(long n, const double *psi0, const double *psi1, const double *psi2, const double *psi_next,
+                   const double *psi_half, const double *L1, const double *L12, const float *rscales, float *target,
+                   void *stream) {
+    if (n <= 0 || !psi0 || !psi1 || !psi2 || !psi_next || !psi_half || !L1 || !L12 || !rscales || !target) {
+        set_error("cmm_cnn_target: bad argument");
+        return CMM_ERR_ARG;
+    }
+    hipLaunchKernelGGL(k_cnn_target, dim3(blocks(n)), dim3(256), 0, S(stream), n, psi0, psi1, psi2, psi_next, psi_half,
+                       L1[0], L1[1], L1[2], L12[0], L12[1], L12[2], rscales[0], rscales[1], target);
+    return lc("k_cnn_target");
+}
+
+int cmm_cnn_mse(long n, const float *pred, const float *target, uint16_t *g, double *loss, void *stream) {
+    if (n <= 0 || !pred || !target || !g || !loss) { set_error("cmm_cnn_mse: bad argument"); return CMM_ERR_ARG; }
+    hipLaunchKernelGGL(k_cnn_mse, dim3(blocks(n)), dim3(256), 0, S(stream), n, pred, target, g, loss);
+    return lc("k_cnn_mse");
+}
+
+int cmm_cnn_correction(long n, const float *pred, const float *rscales, double *d1, double *d12, void *stream) {
+    if (n <= 0 || !pred || !rscales || !d1 || !d12) { set_error("cmm_cnn_correction: bad argument"); return CMM_ERR_ARG; }
+    hipLaunchKernelGGL(k_cnn_correction, dim3(blocks(n)), dim3(256), 0, S(stream), n, pred, rscales[0], rscales[1], d1,
+                       d12);
+    return lc("k_cnn_correction");
+}
+
+int cmm_extrapolate(long n, const double *psi0, const double *psi1, const double *psi2, const double *w,
+                    const double *corr, double *out, void *stream) {
+    if (n <= 0 || !psi0 || !psi1 || !psi2 || !w || !out) { set_error("cmm_extrapolate: bad argument"); return CMM_ERR_ARG; }
+    hipLaunchKernelGGL(k_extrapolate, dim3(blocks(n)), dim3(256), 0, S(stream), n, psi0, psi1, psi2, w[0], w[1], w[2],
+                       corr, out);
+    return lc("k_extrapolate");
+}
+
+int cmm_dev_norms(long n, const double *a, const double *b, double *out2, void *stream) {
+    if (n <= 0 || !a || !b || !out2) { set_error("cmm_dev_norms: bad argument"); return CMM_ERR_ARG; }
+    hipLaunchKernelGGL(k_zero2, dim3(1), dim3(1), 0, S(stream), out2);
+    hipLaunchKernelGGL(k_dev_norms, dim3((unsigned)std::min<long>(1024, blocks(n))), dim3(256), 0, S(stream), n, a, b,
+                       out2);
+    return lc("k_dev_norms");
+}
+
+}  // extern "C"

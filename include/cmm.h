@@ -342,6 +342,55 @@ int cmm_batch_stats(cmm_batch *b, int *graphs_captured, int *graph_launches, int
  * ratio) and block cap on the engine's current map and keeps the fastest for its later steps. */
 int cmm_euler_autotune(cmm_euler *e, int *tile_out, int *max_blocks_out, double *ms_out);
 
+
+/* ---- learned psi extrapolation (no counterpart in the reference) --------------------------------
+ * A 3x3 convolutional network with 32 channels per layer (the 3 input and 2 output channels
+ * zero-padded to 32), BF16 weights and activations in NHWC layout (uint16_t bit patterns), FP32
+ * accumulation on the matrix cores, periodic padding; widths are multiples of 32.  Weights of a
+ * layer: W[co][tap][ci], 32 x 9 x 32, tap = 3 (dy + 1) + (dx + 1); bias: 32 floats.
+ * mode 0: ReLU, bf16 out; 1: linear, bf16 out; 2: linear, float out (32 channels per pixel). */
+int cmm_cnn_forward(int n_img, int H, int W, const uint16_t *in, const uint16_t *w, const float *bias, int mode,
+                    void *out, void *stream);
+/* gradient with respect to the layer input: wt holds the transposed, tap-flipped weights
+ * (cmm_cnn_weights_bf16) */
+int cmm_cnn_backward_data(int n_img, int H, int W, const uint16_t *g, const uint16_t *wt, uint16_t *g_in, void *stream);
+/* dw (32 x 9 x 32 floats) and db (32 floats) accumulate the weight and bias gradients */
+int cmm_cnn_backward_weights(int n_img, int H, int W, const uint16_t *g, const uint16_t *in, float *dw, float *db,
+                             void *stream);
+/* g = 0 where the layer output act is not positive */
+int cmm_cnn_relu_mask(long n, const uint16_t *act, uint16_t *g, void *stream);
+int cmm_cnn_fill(long n, float *p, float value, void *stream);
+/* bf16 weights and their transposed, tap-flipped copies from FP32 master weights of n_layers layers */
+int cmm_cnn_weights_bf16(int n_layers, const float *w32, uint16_t *w, uint16_t *wt, void *stream);
+/* AdamW step (decoupled weight decay) on n FP32 parameters, gradient scaled by gscale; step >= 1 */
+int cmm_cnn_adamw(long n, float *p, const float *g, float *m, float *v, float lr, float beta1, float beta2, float eps,
+                  float weight_decay, float gscale, int step, void *stream);
+/* network input from three psi value planes (newest first): psi0/s0, (psi0-psi1)/s1,
+ * (psi0-2psi1+psi2)/s2 (scales[3]) */
+int cmm_cnn_input(long n, const double *psi0, const double *psi1, const double *psi2, const float *scales,
+                  uint16_t *out, void *stream);
+/* training targets (2 floats per pixel): (psi_next - sum L1 psi_l)/r1, (psi_half - sum L12 psi_l)/r12 */
+int cmm_cnn_target(long n, const double *psi0, const double *psi1, const double *psi2, const double *psi_next,
+                   const double *psi_half, const double *L1, const double *L12, const float *rscales, float *target,
+                   void *stream);
+/* squared-error loss (added to *loss) and its gradient pred - target in channels 0, 1 of g */
+int cmm_cnn_mse(long n, const float *pred, const float *target, uint16_t *g, double *loss, void *stream);
+/* correction value fields d1 = r1 pred[.][0], d12 = r12 pred[.][1] */
+int cmm_cnn_correction(long n, const float *pred, const float *rscales, double *d1, double *d12, void *stream);
+/* out = w0 psi0 + w1 psi1 + w2 psi2 (+ corr) */
+int cmm_extrapolate(long n, const double *psi0, const double *psi1, const double *psi2, const double *w,
+                    const double *corr, double *out, void *stream);
+/* out2[0] = sum (a - b)^2, out2[1] = sum b^2 (device doubles) */
+int cmm_dev_norms(long n, const double *a, const double *b, double *out2, void *stream);
+/* plane `plane` of psi level `level` (Npsi doubles) copied to device memory dst on the engine stream */
+int cmm_euler_copy_psi_level(cmm_euler *e, int level, int plane, double *dst);
+/* device pointer of psi level `level` (0 newest; 4*Npsi doubles) of an engine */
+int cmm_euler_psi_level(cmm_euler *e, int level, const double **dev_ptr);
+/* corrections (device Hermite fields on g_psi, 4*Npsi doubles; NULL: none) added to the Lagrange
+ * extrapolations of psi to t_{n+1} and to t_n + dt/2 in the following steps' map advection
+ * (schemes that combine the levels: 10, 21, 30, 40, 41); the caller keeps them alive */
+int cmm_euler_set_psi_correction(cmm_euler *e, const double *d1, const double *d12);
+
 #ifdef __cplusplus
 }
 #endif
