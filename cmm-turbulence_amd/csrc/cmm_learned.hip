@@ -73,10 +73,11 @@ __global__ void __launch_bounds__(256) k_conv3(const unsigned short *__restrict_
 
 // dW[co][t][ci] += sum_p g[p][co] in[p + off_t][ci] over `chunk` pixels per wave (grid y = tap):
 // M = output channels, N = input channels, K = pixels (8 per MFMA); the wave's 32 x 32 block is
-// added to dW with FP32 atomics.
+// added to dW with FP32 atomics.  The centre-tap waves also sum the A operand's rows (the bias
+// gradient db[co] = sum_p g[p][co]).
 __global__ void __launch_bounds__(256) k_conv3_wgrad(const unsigned short *__restrict__ g,
                                                      const unsigned short *__restrict__ in, float *__restrict__ dW,
-                                                     int H, int Wd, long npix, int chunk) {
+                                                     float *__restrict__ db, int H, int Wd, long npix, int chunk) {
     const int lane = threadIdx.x & 63;
     const long wave = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
     const long p0 = wave * chunk;
@@ -86,6 +87,7 @@ __global__ void __launch_bounds__(256) k_conv3_wgrad(const unsigned short *__res
     const long HW = (long)H * Wd;
     const long pend = p0 + chunk < npix ? p0 + chunk : npix;
     f32x16 acc = {};
+    float bsum = 0.f;
     for (long pb = p0; pb < pend; pb += 8) {
         bf16x4 a, b;
 #pragma unroll
@@ -94,26 +96,22 @@ __global__ void __launch_bounds__(256) k_conv3_wgrad(const unsigned short *__res
             const long n = p / HW;
             const int rem = (int)(p - n * HW);
             const int yy = wrapi(rem / Wd + dy, H), xx = wrapi(rem % Wd + dx, Wd);
-            a[j] = (short)g[p * kCh + i];
+            const unsigned short gv = g[p * kCh + i];
+            a[j] = (short)gv;
             b[j] = (short)in[(n * HW + (long)yy * Wd + xx) * kCh + i];
+            bsum += bf2f(gv);
         }
         acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+    }
+    if (t == 4) {
+        bsum += __shfl_xor(bsum, 32, 64);
+        if (h == 0) atomicAdd(&db[i], bsum);
     }
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
         const int co = 8 * (r >> 2) + 4 * h + (r & 3);
         atomicAdd(&dW[((long)co * 9 + t) * kCh + i], acc[r]);
     }
-}
-
-// db[co] += sum_p g[p][co]
-__global__ void __launch_bounds__(256) k_bias_grad(const unsigned short *__restrict__ g, float *__restrict__ db,
-                                                   long npix) {
-    const int co = threadIdx.x & 31;
-    float s = 0.f;
-    for (long p = (long)blockIdx.x * 8 + (threadIdx.x >> 5); p < npix; p += (long)gridDim.x * 8)
-        s += bf2f(g[p * kCh + co]);
-    atomicAdd(&db[co], s);
 }
 
 // g = g where the layer's output was positive (ReLU derivative)
@@ -308,12 +306,9 @@ int cmm_cnn_backward_weights(int n_img, int H, int W, const uint16_t *g, const u
     const long npix = (long)n_img * H * W;
     const int chunk = 256;
     const long waves = (npix + chunk - 1) / chunk;
-    hipLaunchKernelGGL(k_conv3_wgrad, dim3((unsigned)((waves + 3) / 4), 9), dim3(256), 0, S(stream), g, in, dw, H, W,
-                       npix, chunk);
-    if (int rc = lc("k_conv3_wgrad")) return rc;
-    const unsigned nb = (unsigned)std::min<long>(1024, (npix + 7) / 8);
-    hipLaunchKernelGGL(k_bias_grad, dim3(nb), dim3(256), 0, S(stream), g, db, npix);
-    return lc("k_bias_grad");
+    hipLaunchKernelGGL(k_conv3_wgrad, dim3((unsigned)((waves + 3) / 4), 9), dim3(256), 0, S(stream), g, in, dw, db, H,
+                       W, npix, chunk);
+    return lc("k_conv3_wgrad");
 }
 
 int cmm_cnn_relu_mask(long n, const uint16_t *act, uint16_t *g, void *stream) {
