@@ -182,3 +182,37 @@ def test_cli_vlasov_landau(dev, tmp_path):
     assert abs(m["Mass"][0] - 4 * np.pi) < 1e-9 and abs(m["Ekin"][0] - 2 * np.pi) < 1e-9
     assert abs(m["Epot"][0] - 0.01 * np.pi) < 1e-11 and abs(m["Momentum"][0]) < 1e-9
     assert abs(m["Emode1"][0] - 0.05) < 1e-11
+
+
+def test_cli_split_precision_11b(dev, tmp_path):
+    """cmm2d with map_precision=split on the reference's remapping case (4_nodes, C256/F512, dt 1/32,
+    T 2): the split form has no reference output, so its monitoring files are compared with the
+    fixture at the form's own accuracy and the errors printed: the same remap steps, the
+    conservation values within 1e-6 relative, E_inc within the noise floor that the FP32 records
+    of the combined psi fields put under the check (about 4e-5 here)."""
+    params = tmp_path / "params.txt"
+    params.write_text(FIX["remap_11b"]["params_input"])
+    ws = str(tmp_path) + "/"
+    p = subprocess.run([BIN, f"param_file={params}", f"workspace={ws}", "map_precision=split"], capture_output=True,
+                       text=True, timeout=240)
+    print(p.stdout[-1500:])
+    assert p.returncode == 0, p.stderr
+    assert "Finished simulation" in p.stdout
+    mon = os.path.join(ws, "data", "11b-4nodes-reference-remapping_4_nodes_C256_F512_t32_T2", "Monitoring_data")
+    fx = FIX["remap_11b"]
+    get = lambda k: np.fromfile(os.path.join(mon, k + ".data"), dtype="<f8")
+    einc = get("Error_incompressibility")
+    ref = np.array(fx["Error_incompressibility"])
+    # noise floor of the split check: the FP32 rounding of the combined psi records' value plane,
+    # 2^-24 max|psi| (2.17 for 4_nodes), reaches the velocity through 6/h_psi and the map's value
+    # plane through dt, and the Jacobian through 1/h_map
+    h = 2 * np.pi / 256
+    floor = 2.0 ** -24 * 2.18 * (6 / h) * (1 / 32) / h
+    print("split E_inc max abs diff", np.abs(einc - ref).max(), "noise floor", floor, "max E_inc", ref.max())
+    assert np.array_equal(get("Map_counter"), np.array(fx["Map_counter"]))
+    assert np.abs(einc - ref).max() <= floor
+    for k in ("Mesure/Energy", "Mesure/Enstrophy"):
+        g, r = get(k), np.array(fx[k])
+        e = np.max(np.abs(g - r) / np.abs(r))
+        print(k, "max rel diff", e)
+        assert e <= 1e-6
