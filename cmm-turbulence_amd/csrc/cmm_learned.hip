@@ -9,6 +9,7 @@
 //   D[m][n] with n = i and m = 8 (r / 4) + 4 h + r % 4 for the lane's 16 results r.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 
@@ -33,75 +34,90 @@ __device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float
 __device__ __forceinline__ int wrapi(int v, int n) { return v < 0 ? v + n : v >= n ? v - n : v; }
 
 // out[n][y][x][co] = act(b[co] + sum_{t, ci} W[co][t][ci] in[n][y + dy_t][x + dx_t][ci]), tap
-// t = 3 (dy + 1) + (dx + 1), periodic in y and x.  One wave per 32 pixels of a row: M = pixels,
-// N = output channels, K = 9 taps x 32 channels (36 MFMAs).  MODE 0: bf16 out with ReLU, 1: bf16
-// out, 2: fp32 out.  The backward-data pass is this kernel on the output gradient with the
-// weights transposed in channels and flipped in taps.
-template <int MODE>
+// t = 3 (dy + 1) + (dx + 1), periodic in y and x.  A wave takes SEGS consecutive 32-pixel
+// segments of a row: M = pixels, N = output channels, K = 9 taps x 32 channels (36 MFMAs per
+// segment); the wave's B operands (its 36 weight fragments) stay in registers across the
+// segments (4 for large batches, 1 where fewer waves would leave the device idle).  MODE 0: bf16
+// out with ReLU, 1: bf16 out, 2: fp32 out.  The backward-data pass is this kernel on the output
+// gradient with the weights transposed in channels and flipped in taps.
+template <int MODE, int SEGS>
 __global__ void __launch_bounds__(256) k_conv3(const unsigned short *__restrict__ in, const unsigned short *__restrict__ W,
                                                const float *__restrict__ bias, void *__restrict__ out, int H, int Wd,
                                                int segs) {
     const int lane = threadIdx.x & 63;
     const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (wave >= H * segs) return;  // uniform over the wave
+    const int groups = segs / SEGS;  // segment groups per row (segs is a multiple of SEGS)
+    if (wave >= H * groups) return;      // uniform over the wave
     const long img = (long)blockIdx.y * H * Wd;
-    const int y = wave / segs, x0 = (wave % segs) * 32;
+    const int y = wave / groups, xg = (wave % groups) * (32 * SEGS);
     const int i = lane & 31, h = lane >> 5;
-    f32x16 acc = {};
-    for (int t = 0; t < 9; ++t) {
-        const int yy = wrapi(y + t / 3 - 1, H), xx = wrapi(x0 + i + t % 3 - 1, Wd);
-        const unsigned short *ap = in + (img + (long)yy * Wd + xx) * kCh + 4 * h;
-        const unsigned short *bp = W + ((long)i * 9 + t) * kCh + 4 * h;
+    bf16x4 b[36];
 #pragma unroll
-        for (int cb = 0; cb < kCh; cb += 8) {
-            const bf16x4 a = *reinterpret_cast<const bf16x4 *>(ap + cb);
-            const bf16x4 b = *reinterpret_cast<const bf16x4 *>(bp + cb);
-            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
-        }
-    }
+    for (int q = 0; q < 36; ++q)
+        b[q] = *reinterpret_cast<const bf16x4 *>(W + ((long)i * 9 + q / 4) * kCh + 4 * h + 8 * (q % 4));
     const float bb = bias ? bias[i] : 0.f;
+    for (int sg = 0; sg < SEGS; ++sg) {
+        const int x0 = xg + 32 * sg;
+        f32x16 acc = {};
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int p = 8 * (r >> 2) + 4 * h + (r & 3);
-        float v = acc[r] + bb;
-        if (MODE == 0) v = fmaxf(v, 0.f);
-        const long o = (img + (long)y * Wd + x0 + p) * kCh + i;
-        if (MODE == 2) static_cast<float *>(out)[o] = v;
-        else static_cast<unsigned short *>(out)[o] = f2bf(v);
+        for (int t = 0; t < 9; ++t) {
+            const int yy = wrapi(y + t / 3 - 1, H), xx = wrapi(x0 + i + t % 3 - 1, Wd);
+            const unsigned short *ap = in + (img + (long)yy * Wd + xx) * kCh + 4 * h;
+#pragma unroll
+            for (int cb = 0; cb < 4; ++cb) {
+                const bf16x4 a = *reinterpret_cast<const bf16x4 *>(ap + 8 * cb);
+                acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b[4 * t + cb], acc, 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int p = 8 * (r >> 2) + 4 * h + (r & 3);
+            float v = acc[r] + bb;
+            if (MODE == 0) v = fmaxf(v, 0.f);
+            const long o = (img + (long)y * Wd + x0 + p) * kCh + i;
+            if (MODE == 2) static_cast<float *>(out)[o] = v;
+            else static_cast<unsigned short *>(out)[o] = f2bf(v);
+        }
     }
 }
 
-// dW[co][t][ci] += sum_p g[p][co] in[p + off_t][ci] over `chunk` pixels per wave (grid y = tap):
-// M = output channels, N = input channels, K = pixels (8 per MFMA); the wave's 32 x 32 block is
-// added to dW with FP32 atomics.  The centre-tap waves also sum the A operand's rows (the bias
-// gradient db[co] = sum_p g[p][co]).
+// dW[co][t][ci] += sum_p g[p][co] in[p + off_t][ci] (grid y = tap): a wave takes `segw`
+// consecutive 32-pixel row segments, M = output channels, N = input channels, K = pixels (8 per
+// MFMA, 4 MFMAs per segment); the row addressing is set once per segment, and the wave's 32 x 32
+// block is added to dW with FP32 atomics.  The centre-tap waves also sum the A operand's rows (the
+// bias gradient db[co] = sum_p g[p][co]).
 __global__ void __launch_bounds__(256) k_conv3_wgrad(const unsigned short *__restrict__ g,
                                                      const unsigned short *__restrict__ in, float *__restrict__ dW,
-                                                     float *__restrict__ db, int H, int Wd, long npix, int chunk) {
+                                                     float *__restrict__ db, int H, int Wd, int nseg, int segw) {
     const int lane = threadIdx.x & 63;
-    const long wave = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
-    const long p0 = wave * chunk;
-    if (p0 >= npix) return;  // uniform over the wave
+    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int s0 = wave * segw;
+    if (s0 >= nseg) return;  // uniform over the wave
     const int t = blockIdx.y, dy = t / 3 - 1, dx = t % 3 - 1;
     const int i = lane & 31, h = lane >> 5;
-    const long HW = (long)H * Wd;
-    const long pend = p0 + chunk < npix ? p0 + chunk : npix;
+    const int segs = Wd / 32;
+    const int s1 = s0 + segw < nseg ? s0 + segw : nseg;
     f32x16 acc = {};
     float bsum = 0.f;
-    for (long pb = p0; pb < pend; pb += 8) {
-        bf16x4 a, b;
+    for (int s = s0; s < s1; ++s) {
+        const int row = s / segs;  // n * H + y
+        const int x0 = (s - row * segs) * 32;
+        const int n = row / H, y = row - n * H;
+        const unsigned short *grow = g + (long)row * Wd * kCh + i;
+        const unsigned short *irow = in + ((long)n * H + wrapi(y + dy, H)) * Wd * kCh + i;
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            const long p = pb + 4 * h + j;
-            const long n = p / HW;
-            const int rem = (int)(p - n * HW);
-            const int yy = wrapi(rem / Wd + dy, H), xx = wrapi(rem % Wd + dx, Wd);
-            const unsigned short gv = g[p * kCh + i];
-            a[j] = (short)gv;
-            b[j] = (short)in[(n * HW + (long)yy * Wd + xx) * kCh + i];
-            bsum += bf2f(gv);
+        for (int kb = 0; kb < 32; kb += 8) {
+            bf16x4 a, b;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int x = x0 + kb + 4 * h + j;
+                const unsigned short gv = grow[(long)x * kCh];
+                a[j] = (short)gv;
+                b[j] = (short)irow[(long)wrapi(x + dx, Wd) * kCh];
+                bsum += bf2f(gv);
+            }
+            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
         }
-        acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
     }
     if (t == 4) {
         bsum += __shfl_xor(bsum, 32, 64);
@@ -286,10 +302,20 @@ int cmm_cnn_forward(int n_img, int H, int W, const uint16_t *in, const uint16_t 
         return CMM_ERR_ARG;
     }
     const int segs = W / 32;
-    const dim3 grid((unsigned)((H * segs + 3) / 4), (unsigned)n_img);
-    if (mode == 0) hipLaunchKernelGGL(k_conv3<0>, grid, dim3(256), 0, S(stream), in, w, b, out, H, W, segs);
-    else if (mode == 1) hipLaunchKernelGGL(k_conv3<1>, grid, dim3(256), 0, S(stream), in, w, b, out, H, W, segs);
-    else hipLaunchKernelGGL(k_conv3<2>, grid, dim3(256), 0, S(stream), in, w, b, out, H, W, segs);
+    // four segments per wave when that still gives 2048 waves or more
+    const bool four = segs % 4 == 0 && (long)n_img * H * (segs / 4) >= 2048;
+    const dim3 grid((unsigned)((H * (four ? segs / 4 : segs) + 3) / 4), (unsigned)n_img);
+#define CMM_CONV(M, SG) hipLaunchKernelGGL((k_conv3<M, SG>), grid, dim3(256), 0, S(stream), in, w, b, out, H, W, segs)
+    if (four) {
+        if (mode == 0) CMM_CONV(0, 4);
+        else if (mode == 1) CMM_CONV(1, 4);
+        else CMM_CONV(2, 4);
+    } else {
+        if (mode == 0) CMM_CONV(0, 1);
+        else if (mode == 1) CMM_CONV(1, 1);
+        else CMM_CONV(2, 1);
+    }
+#undef CMM_CONV
     return lc("k_conv3");
 }
 
@@ -303,11 +329,12 @@ int cmm_cnn_backward_weights(int n_img, int H, int W, const uint16_t *g, const u
         set_error("cmm_cnn_backward_weights: bad argument");
         return CMM_ERR_ARG;
     }
-    const long npix = (long)n_img * H * W;
-    const int chunk = 256;
-    const long waves = (npix + chunk - 1) / chunk;
+    // about 4096 waves over the 9 taps: fewer waves add fewer FP32 atomics to the 9216 weights
+    const int nseg = n_img * H * (W / 32);
+    const int segw = std::max(1, nseg * 9 / 4096);
+    const int waves = (nseg + segw - 1) / segw;
     hipLaunchKernelGGL(k_conv3_wgrad, dim3((unsigned)((waves + 3) / 4), 9), dim3(256), 0, S(stream), g, in, dw, db, H,
-                       W, npix, chunk);
+                       W, nseg, segw);
     return lc("k_conv3_wgrad");
 }
 

@@ -47,7 +47,7 @@ def test_conv_kernels_match_cpu(dev):
     import cmm
     import torch
     rng = np.random.default_rng(1)
-    N, H, W = 2, 6, 64
+    N, H, W = 2, 6, 128
     x = R.bf16(rng.standard_normal((N, H, W, 32)))
     w = R.bf16(rng.standard_normal((32, 9, 32)) * 0.1)
     b = rng.standard_normal(32).astype(np.float32).astype(np.float64)
@@ -94,7 +94,7 @@ def test_network_gradient_matches_cpu(dev):
     import torch
     rng = np.random.default_rng(3)
     net = learned.Net(seed=4)
-    N, H, W = 2, 8, 32
+    N, H, W = 2, 4, 128
     net.buffers(N, H, W)
     x = np.zeros((N, H, W, 32))
     x[..., :3] = R.bf16(rng.standard_normal((N, H, W, 3)))
