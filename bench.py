@@ -216,6 +216,52 @@ def summarize_profiles(kt_dir, fetch_dir, write_dir, out_dir, commands):
     print("\n".join(lines))
 
 
+def composition(cmm, torch, c, n_maps=32, reps=5):
+    """Map-stack composition at full size: the 4096^2 fine grid through the current 1024^2 map and
+    n_maps stacked maps (each a smooth shear of the identity, derivatives exact), positions written
+    (cmm_compose_positions).  Algorithmic bytes: every map read once (64 B per coarse node) and
+    16 B of positions per fine point; the kernel's time from events on its stream."""
+    import numpy as np
+    nc, nf = c["n_coarse"], c["n_fine"]
+    N = nc * nc
+    gm, gt = cmm.grid(nc), cmm.grid(nf)
+    dev = torch.device("cuda")
+    h = (2 * np.pi) / float(np.float32(nc))
+    xs = torch.arange(nc, dtype=torch.float64, device=dev) * h
+    X = xs[None, :].expand(nc, nc).reshape(-1)
+    Y = xs[:, None].expand(nc, nc).reshape(-1)
+    stack = torch.empty(n_maps * 8 * N, dtype=torch.float64, device=dev)
+    for m in range(n_maps):
+        a, ph = 0.01 * (1 + m % 3), 0.37 * m
+        sl = stack[m * 8 * N:(m + 1) * 8 * N].view(8, N)
+        sl[0] = X + a * torch.sin(Y + ph)
+        sl[1] = 1.0
+        sl[2] = a * torch.cos(Y + ph)
+        sl[3] = 0.0
+        sl[4] = Y + a * torch.sin(X - ph)
+        sl[5] = a * torch.cos(X - ph)
+        sl[6] = 1.0
+        sl[7] = 0.0
+    cx = torch.empty(4 * N, dtype=torch.float64, device=dev)
+    cy = torch.empty_like(cx)
+    cmm.map_init_identity(gm, cx, cy)
+    xy = torch.empty(2 * nf * nf, dtype=torch.float64, device=dev)
+    cmm.apply_map_stack(gt, gm, cx, cy, stack, n_maps, xy)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(reps):
+        cmm.apply_map_stack(gt, gm, cx, cy, stack, n_maps, xy)
+    ev1.record()
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / reps
+    b = (n_maps + 1) * 64 * N + 16 * nf * nf
+    gbs = b / (ms * 1e-3) / 1e9
+    del stack, xy
+    return {"kernel": "k_compose_positions", "maps": n_maps + 1, "ms_avg": round(ms, 4), "algorithmic_bytes": b,
+            "achieved_GBs": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4),
+            "point_layers_per_s": round((n_maps + 1) * nf * nf / (ms * 1e-3) / 1e9, 3)}
+
+
 def roofline(c, prof, n_stack, suffix=""):
     """Per-slot rates from the engine's event profile and the roofline object of the dominant
     kernel: algorithmic bytes and flops per launch over its average duration; traffic from the
@@ -507,6 +553,9 @@ def main():
 
     roof, per = roofline(c, prof, n_stack)
 
+    comp = composition(cmm, torch, c)
+    log(f"composition: {comp}")
+
     # the chosen form: split precision at the same step, with its errors against the FP64 run
     chosen = None
     if not args.no_split:
@@ -561,6 +610,7 @@ def main():
         "cpu_baseline": cpu,
         "kernels_ms_avg": {k: round(v["ms_avg"], 5) for k, v in per.items()},
         "chosen_form": chosen,
+        "composition": comp,
     }
     print(json.dumps(out), flush=True)
 
