@@ -536,6 +536,11 @@ double init_vorticity(double x, double y, int ic, const double *prm) {
                (std::exp(-(y - 3) * (y - 3) * 0.5) + std::exp(-(y + 3) * (y + 3) * 0.5));
     case CMM_IC_NONLIN_LANDAU_DAMPING:
         return (1 + 0.5 * std::cos(0.5 * x)) * std::exp(-y * y * 0.5) / std::sqrt(2 * kPi);
+    // ensemble distribution functions with amplitude prm[0] (no reference counterpart)
+    case CMM_IC_LANDAU_AMP:
+        return (1 + prm[0] * std::cos(0.5 * x)) * std::exp(-y * y * 0.5) / std::sqrt(2 * kPi);
+    case CMM_IC_TWO_STREAM_V2:
+        return (1 + prm[0] * std::cos(0.5 * x)) * y * y * std::exp(-y * y * 0.5) / std::sqrt(2 * kPi);
     // passive scalars, d_init_scalar (Cuda-CMM/src/simulation/cmm-init.cu:249-301)
     case CMM_IC_SCALAR_RECTANGLE: {
         x -= std::floor(x / kTwoPi) * kTwoPi;

@@ -78,3 +78,32 @@ def test_autotune_keeps_the_check(dev):
     assert ref > 0
     e.close()
     e2.close()
+
+
+@pytest.mark.parametrize("ic,code", [("landau_amp", 203), ("two_stream_v2", 204)])
+def test_ensemble_conditions_against_oracle(dev, ic, code):
+    """The ensemble's distribution functions (x in [0, 4 pi), v in [-8, 8), amplitude 0.04) through
+    the FP64 Vlasov-Poisson loop (map 64^2, fine 256^2, psi and vort 128^2, RK3, dt 1/16, T 1, remap
+    at 1e-4) against the oracle's run: the same remap steps, E_inc within 1e-10 absolute, psi within
+    1e-10 relative L-inf (the tolerances of the reference's Vlasov-Poisson case)."""
+    import cmm
+    import oracle_lib as O
+    from gpu_util import rel_linf
+    g = cmm.ensemble_grid
+    e = cmm.Euler(g(64), g(256), g(128), g(128), DT, initial_condition=ic, ic_params=[0.04],
+                  simulation_type="cmm_vlasov_poisson_1d", incomp_threshold=1e-4, stack_capacity=32)
+    e.init()
+    infos = [e.step(DT) for _ in range(16)]
+    einc = np.array([i.incomp for i in infos])
+    counter = np.array([i.stack_size for i in infos], dtype=float)
+    psi = e.field("psi")
+    e.close()
+    b = (0.0, 2 * O.TWO_PI, -8.0, 8.0)
+    og = lambda n: O.grid(n, n, b)
+    r = O.euler_run(og(64), og(256), og(128), og(128), dt=DT, final_time=1.0, ic=code, ic_params=[0.04], sim=1,
+                    threshold=1e-4, stack_capacity=32, want=("incomp", "map_counter", "psi"))
+    err = rel_linf(psi, r["psi"])
+    print(ic, "remaps", counter.max(), "max |dE_inc|", np.abs(einc - r["incomp"]).max(), "psi rel Linf", err)
+    assert np.array_equal(counter, r["map_counter"])
+    assert np.abs(einc - r["incomp"]).max() <= 1e-10
+    assert err <= 1e-10

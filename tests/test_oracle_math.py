@@ -560,3 +560,27 @@ def test_forward_points_selection_and_bounds():
     out = O.forward_points(gm, (0.5, 2.0, 0.5, 2.0), bx, by, sx, sy, 2, 2, True, q)
     assert np.isnan(out[0]) and np.isnan(out[1]) and np.isnan(out[4]) and np.isnan(out[5])
     assert abs(out[2] - 1.0) < 1e-12 and abs(out[3] - (1.0 + b * np.sin(1.0))) < 1e-5
+
+
+def test_ensemble_distribution_functions():
+    """The ensemble's Landau (203) and two-stream (204) distributions: values by their closed forms,
+    mass 4 pi on x in [0, 4 pi), v in [-8, 8) (both v-profiles integrate to 1 up to e^-32 tails),
+    and the density's first x-mode equal to the amplitude."""
+    n = 256
+    b = (0.0, 2 * O.TWO_PI, -8.0, 8.0)
+    g = O.grid(n, n, b)
+    hx, hv = 2 * O.TWO_PI / n, 16.0 / n
+    x = (np.arange(n) * hx)[None, :].repeat(n, 0).ravel()
+    v = (-8.0 + np.arange(n) * hv)[:, None].repeat(n, 1).ravel()
+    xy = np.empty(2 * n * n)
+    xy[0::2], xy[1::2] = x, v
+    for ic, prof in ((203, lambda v: np.exp(-v * v / 2)), (204, lambda v: v * v * np.exp(-v * v / 2))):
+        for a in (0.005, 0.03):
+            f = O.sample_init(g, xy, ic, [a])
+            ref = (1 + a * np.cos(0.5 * x)) * prof(v) / np.sqrt(2 * np.pi)
+            assert np.max(np.abs(f - ref)) <= 1e-15 * np.max(np.abs(ref)) + 1e-300
+            mass = hx * hv * f.sum()
+            assert abs(mass - 4 * np.pi) < 1e-9, (ic, a, mass)
+            rho = hv * f.reshape(n, n).sum(axis=0)
+            mode = np.fft.rfft(rho)[1] / (n / 2)
+            assert abs(abs(mode) - a) < 1e-9, (ic, a, mode)
