@@ -353,6 +353,10 @@ struct cmm_euler {
     // the next psi solve (which overwrites the oldest level) waits for them
     hipStream_t st2 = nullptr;
     hipEvent_t ev_psi = nullptr, ev_part = nullptr;
+    // single-engine steps (cmm_euler_step) run the check on a third stream beside the psi solve
+    // (both only read the new map); ensemble steps keep it on the engine stream
+    hipStream_t st3 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     bool part_pending = false;
     std::vector<double *> part;
     std::vector<int> part_n, part_scheme;
@@ -509,16 +513,17 @@ int engine_advect_forward(cmm_euler *e, const double *t, const double *dt, int l
     return CMM_OK;
 }
 
-// the check on whichever form of the map is current
-int launch_incomp(cmm_euler *e) {
+// the check on whichever form of the map is current, on stream s (a conversion to the FP64
+// planes, when needed, runs on the engine stream before)
+int launch_incomp(cmm_euler *e, hipStream_t s) {
     if (split(e) && e->m32_fresh)
-        return e->dry ? CMM_OK : op_incomp32_launch(e->p.g_coarse, e->p.g_fine, e->m32[e->m32_cur], e->slot, e->st, e->tune);
+        return e->dry ? CMM_OK : op_incomp32_launch(e->p.g_coarse, e->p.g_fine, e->m32[e->m32_cur], e->slot, s, e->tune);
     if (int rc = sync_planes(e)) return rc;
-    return e->dry ? CMM_OK : op_incomp_launch(e->p.g_coarse, e->p.g_fine, e->cx, e->cy, e->slot, e->st, e->tune);
+    return e->dry ? CMM_OK : op_incomp_launch(e->p.g_coarse, e->p.g_fine, e->cx, e->cy, e->slot, s, e->tune);
 }
 
 int engine_incomp(cmm_euler *e, double *out) {
-    if (int rc = launch_incomp(e)) return rc;
+    if (int rc = launch_incomp(e, e->st)) return rc;
     unsigned long long h = 0;
     if (int rc = hip_check(hipMemcpyAsync(&h, e->slot, sizeof(h), hipMemcpyDeviceToHost, e->st), "hipMemcpyAsync")) return rc;
     if (int rc = hip_check(hipStreamSynchronize(e->st), "hipStreamSynchronize")) return rc;
@@ -705,6 +710,9 @@ int cmm_euler_create(const cmm_euler_params *p, cmm_euler **out) {
     if (!rc) rc = hip_check(hipEventCreateWithFlags(&e->ev_psi, hipEventDisableTiming), "hipEventCreate");
     if (!rc) rc = hip_check(hipEventCreateWithFlags(&e->ev_part, hipEventDisableTiming), "hipEventCreate");
     if (!rc) rc = hip_check(hipStreamCreateWithFlags(&e->st2, hipStreamNonBlocking), "hipStreamCreate");
+    if (!rc) rc = hip_check(hipStreamCreateWithFlags(&e->st3, hipStreamNonBlocking), "hipStreamCreate");
+    if (!rc) rc = hip_check(hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming), "hipEventCreate");
+    if (!rc) rc = hip_check(hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming), "hipEventCreate");
     for (int s = 0; s < cmm_euler::kProfSets && !rc; ++s)
         for (int i = 0; i < 12 && !rc; ++i) rc = hip_check(hipEventCreate(&e->evs[s][i]), "hipEventCreate");
     if (!rc && p->discrete) rc = engine_setup_discrete(e);
@@ -735,6 +743,12 @@ int cmm_euler_destroy(cmm_euler *e) {
     if (e->ev_psi) (void)hipEventDestroy(e->ev_psi);
     if (e->ev_part) (void)hipEventDestroy(e->ev_part);
     if (e->st2) (void)hipStreamDestroy(e->st2);
+    if (e->st3) {
+        (void)hipStreamSynchronize(e->st3);
+        (void)hipStreamDestroy(e->st3);
+    }
+    if (e->ev_fork) (void)hipEventDestroy(e->ev_fork);
+    if (e->ev_join) (void)hipEventDestroy(e->ev_join);
     for (int s = 0; s < cmm_euler::kProfSets; ++s)
         for (int i = 0; i < 12; ++i)
             if (e->evs[s][i]) (void)hipEventDestroy(e->evs[s][i]);
@@ -916,8 +930,9 @@ namespace {
 
 // The device work of a step up to the psi solve, queued without a host wait: advection, check,
 // E_inc copied to pinned memory, psi history rotated and the new psi.  With e->dry only the host
-// state advances (the work comes from a captured graph replay).
-int step_queue(cmm_euler *e, double dt_now) {
+// state advances (the work comes from a captured graph replay).  overlap: the check and the E_inc
+// copy run on the third stream beside the psi solve (single-engine steps only).
+int step_queue(cmm_euler *e, double dt_now, bool overlap = false) {
     const cmm_euler_params &p = e->p;
     const int L = p.l_order;
     const int lc = e->loop + L - 1;
@@ -936,24 +951,36 @@ int step_queue(cmm_euler *e, double dt_now) {
     if (int rc = engine_advect(e, e->tv.data(), e->dtv.data(), e->loop, L, p.time_integration_num)) return rc;
     if (p.forward_map)
         if (int rc = engine_advect_forward(e, e->tv.data(), e->dtv.data(), e->loop, L, p.time_integration_num)) return rc;
-    prof_mark(e, 1);
     // incompressibility on the fine grid, :796: the check and the copy of E_inc to pinned host
-    // memory are queued, and the host waits for that copy only after the psi solve is queued too
-    if (int rc = launch_incomp(e)) return rc;
+    // memory are queued, and the host waits for that copy only after the psi solve is queued too.
+    // With overlap they run on the third stream beside the psi solve: both only read the new map,
+    // which the next advection does not overwrite (it writes the other buffer, after the host has
+    // waited for E_inc); a conversion of the map that the check needs runs first, on the engine
+    // stream.
+    const bool fork = overlap && e->st3 && !e->dry && !e->capturing;
+    if (fork && !(split(e) && e->m32_fresh))
+        if (int rc = sync_planes(e)) return rc;
+    const hipStream_t sc = fork ? e->st3 : e->st;
+    if (fork) {
+        if (int rc = hip_check(hipEventRecord(e->ev_fork, e->st), "hipEventRecord")) return rc;
+        if (int rc = hip_check(hipStreamWaitEvent(e->st3, e->ev_fork, 0), "hipStreamWaitEvent")) return rc;
+    }
+    if (e->prof) (void)hipEventRecord(e->ev[1], sc);
+    if (int rc = launch_incomp(e, sc)) return rc;
     // forward map: invertibility and E_inc on the coarse grid, :798-803
     if (p.forward_map) {
-        if (int rc = op_invert_launch(p.g_coarse, p.g_coarse, p.g_coarse, e->cx, e->cy, e->fx, e->fy, e->slot + 2, e->st))
+        if (int rc = op_invert_launch(p.g_coarse, p.g_coarse, p.g_coarse, e->cx, e->cy, e->fx, e->fy, e->slot + 2, sc))
             return rc;
-        if (int rc = op_incomp_launch(p.g_coarse, p.g_coarse, e->fx, e->fy, e->slot + 1, e->st)) return rc;
+        if (int rc = op_incomp_launch(p.g_coarse, p.g_coarse, e->fx, e->fy, e->slot + 1, sc)) return rc;
     }
     if (!e->dry)
         if (int rc = hip_check(hipMemcpyAsync(e->h_slot, e->slot, sizeof(unsigned long long) * (p.forward_map ? 3 : 1),
-                                              hipMemcpyDeviceToHost, e->st),
+                                              hipMemcpyDeviceToHost, sc),
                                "hipMemcpyAsync"))
             return rc;
     if (!e->dry && !e->capturing)
-        if (int rc = hip_check(hipEventRecord(e->ev_inc, e->st), "hipEventRecord")) return rc;
-    prof_mark(e, 2);
+        if (int rc = hip_check(hipEventRecord(e->ev_inc, sc), "hipEventRecord")) return rc;
+    if (e->prof) (void)hipEventRecord(e->ev[2], sc);
     // psi history: the oldest level becomes the newest (same contents as the reference's
     // copies at :865-868), then the new psi, :871.  It is queued before E_inc is known: without a
     // remap (the common case) it is the step's psi; a remap below recomputes it afterwards in
@@ -964,6 +991,11 @@ int step_queue(cmm_euler *e, double dt_now) {
     if (e->part_pending)  // the previous step's particles still read the level about to be rewritten
         if (int rc = hip_check(hipStreamWaitEvent(e->st, e->ev_part, 0), "hipStreamWaitEvent")) return rc;
     if (int rc = engine_psi(e, e->lvl[0], e->prof ? &e->ev[5] : nullptr)) return rc;
+    // rejoin: the engine stream's later work is ordered after the check
+    if (fork) {
+        if (int rc = hip_check(hipEventRecord(e->ev_join, e->st3), "hipEventRecord")) return rc;
+        if (int rc = hip_check(hipStreamWaitEvent(e->st, e->ev_join, 0), "hipStreamWaitEvent")) return rc;
+    }
     return CMM_OK;
 }
 
@@ -1057,7 +1089,7 @@ int step_finish(cmm_euler *e, cmm_step_info *info, bool waited) {
 int cmm_euler_step(cmm_euler *e, double dt_now, cmm_step_info *info) {
     if (!e || !e->initialised) { set_error("cmm_euler_step: engine not initialised"); return CMM_ERR_ARG; }
     if (!(dt_now > 0)) { set_error("cmm_euler_step: dt must be positive"); return CMM_ERR_ARG; }
-    if (int rc = step_queue(e, dt_now)) return rc;
+    if (int rc = step_queue(e, dt_now, true)) return rc;
     return step_finish(e, info, false);
 }
 
@@ -1232,7 +1264,7 @@ int cmm_euler_autotune(cmm_euler *e, int *tile_out, int *max_blocks_out, double 
             float m_min = 1e30f;
             for (int rep = 0; rep < 6 && !rc; ++rep) {
                 rc = hip_check(hipEventRecord(a, e->st), "hipEventRecord");
-                if (!rc) rc = launch_incomp(e);
+                if (!rc) rc = launch_incomp(e, e->st);
                 if (!rc) rc = hip_check(hipEventRecord(z, e->st), "hipEventRecord");
                 if (!rc) rc = hip_check(hipEventSynchronize(z), "hipEventSynchronize");
                 float m = 0;
