@@ -68,7 +68,8 @@ def algorithmic(slot, c, kx_max, n_stack=0):
     if slot == "incomp_split":
         return 48 * nc, 85.0 * nf
     if slot == "psi_row_r2c_split":
-        return 48 * nc + 16 * nf + 16 * nv + 16 * (kx_max + 1) * c["n_vort"], 270.0 * nv + 2.5 * nv * 24
+        # the sampled vorticity is written and read back as FP32
+        return 48 * nc + 16 * nf + 8 * nv + 16 * (kx_max + 1) * c["n_vort"], 270.0 * nv + 2.5 * nv * 24
     if slot == "remap":
         return (64 * nc * (n_stack + 1) + 32 * nf), 170.0 * nf * (n_stack + 1)
     return 0, 0.0

@@ -669,6 +669,13 @@ struct SampleReal {
     __device__ double operator()(int ix, int iy) const { return f[(long)iy * nx + ix]; }
 };
 
+// plain real array in FP32 (the split form's sampled vorticity)
+struct SampleRealF {
+    const float *__restrict__ f;
+    int nx;
+    __device__ double operator()(int ix, int iy) const { return (double)f[(long)iy * nx + ix]; }
+};
+
 // value of a Hermite field at the points of another grid (its own origin and spacing),
 // k_h_sample (Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:122-135)
 struct SampleHermite {
@@ -820,8 +827,8 @@ struct SampleCompose {
 // row kernel.
 constexpr int kSampleRows = 4;
 
-template <class S>
-__global__ void __launch_bounds__(256, 2) k_sample_field(S smp, int nx, int ny, double *__restrict__ f) {
+template <class S, class T = double>
+__global__ void __launch_bounds__(256, 2) k_sample_field(S smp, int nx, int ny, T *__restrict__ f) {
     constexpr int R = kSampleRows;
     const long t = (long)blockIdx.x * 256 + threadIdx.x;
     if (t >= (long)nx * (ny / R)) return;
@@ -829,7 +836,7 @@ __global__ void __launch_bounds__(256, 2) k_sample_field(S smp, int nx, int ny, 
     double v[R];
     smp.template column<R>(x, y0, v);
 #pragma unroll
-    for (int r = 0; r < R; ++r) f[(long)(y0 + r) * nx + x] = v[r];
+    for (int r = 0; r < R; ++r) f[(long)(y0 + r) * nx + x] = (T)v[r];
 }
 
 template <class S>
@@ -2547,14 +2554,15 @@ int op_psi_from_vort32(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &g
     const size_t base = sp.t_bytes + 2 * sp.g_bytes + 256;
     void *w;
     if (int rc = get_work(work, work_bytes, base + sizeof(double) * (size_t)gv.nx * gv.ny, &w)) return rc;
-    double *f = reinterpret_cast<double *>(static_cast<char *>(w) + base);
+    // the sampled vorticity in FP32: its rounding (2^-24 relative) reaches psi through 1/|k|^2
+    float *f = reinterpret_cast<float *>(static_cast<char *>(w) + base);
     const long lanes = (long)gv.nx * (gv.ny / kSampleRows);
     SampleVort32 smp{make_dgrid(gm), make_dgrid(gv), make_dgrid(gf), (const float4 *)M, (const float4 *)W4, molly};
     if (ev) (void)hipEventRecord(ev[0], st);
-    hipLaunchKernelGGL(k_sample_field<SampleVort32>, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, st, smp, gv.nx,
-                       gv.ny, f);
+    hipLaunchKernelGGL((k_sample_field<SampleVort32, float>), dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, st, smp,
+                       gv.nx, gv.ny, f);
     if (int rc = launch_check("k_sample_field")) return rc;
-    return run_spec(sp, SampleReal{f, gv.nx}, psi, w, st, ev, false);
+    return run_spec(sp, SampleRealF{f, gv.nx}, psi, w, st, ev, false);
 }
 
 int op_sample_vort32(const cmm_grid &gm, const cmm_grid &gt, const cmm_grid &gf, const float *M, const float *W4,
