@@ -148,6 +148,24 @@ int cmm_compose_positions(cmm_grid gm, cmm_grid gt, const double *cx, const doub
     return op_compose_positions(gm, gt, cx, cy, stack, ns, xy, S(stream));
 }
 
+int cmm_stack_pack_split(cmm_grid gm, const double *stack, int n_maps, void *out, void *stream) {
+    if (!grid_ok(gm) || n_maps < 0 || (n_maps > 0 && (!stack || !out))) {
+        set_error("cmm_stack_pack_split: bad argument");
+        return CMM_ERR_ARG;
+    }
+    return op_stack_pack(gm, stack, n_maps, out, S(stream));
+}
+
+int cmm_compose_positions_split(cmm_grid gm, cmm_grid gt, const double *cx, const double *cy,
+                                const void *stack_split, int ns, double *xy, void *stream) {
+    if (!grid_ok(gm) || !grid_ok(gt) || !cx || !cy || !xy || ns < 0 || (ns > 0 && !stack_split)) {
+        set_error("cmm_compose_positions_split: bad argument");
+        return CMM_ERR_ARG;
+    }
+    return op_compose_positions(gm, gt, cx, cy, static_cast<const double *>(stack_split), ns, xy, S(stream),
+                                kStackSplit);
+}
+
 int cmm_forward_positions(cmm_grid gm, cmm_grid gt, const double *cx, const double *cy, const double *stack,
                           int ns, double *xy, void *stream) {
     if (!grid_ok(gm) || !grid_ok(gt) || !cx || !cy || !xy || ns < 0 || (ns > 0 && !stack)) {
@@ -321,6 +339,9 @@ struct cmm_euler {
     double *lvl[4] = {nullptr, nullptr, nullptr, nullptr};  // level -> buffer, newest first
     double *W = nullptr, *disc = nullptr, *stack = nullptr, *tmp = nullptr, *tmp4 = nullptr;
     double *W4 = nullptr;  // W node-interleaved (Euler: read by the psi sampler)
+    // split form: the stack again in the split stack format (FP64 values, FP32 derivatives), kept
+    // beside the FP64 stack and read by every composition; the FP64 stack serves saves and readback
+    double *stack_s = nullptr;
     double *m8[2] = {nullptr, nullptr};  // the map node-interleaved (ping-pong), see engine_advect
     double *sbuf = nullptr;              // sample-grid buffers of cmm_euler_sample
     size_t sbuf_n = 0;
@@ -398,6 +419,10 @@ int falloc(float **p, size_t n) {
 
 bool split(const cmm_euler *e) { return e->p.precision == CMM_PRECISION_SPLIT; }
 
+// the stack the compositions read, and its format
+const double *comp_stack(const cmm_euler *e) { return e->stack_s ? e->stack_s : e->stack; }
+int comp_fmt(const cmm_euler *e) { return e->stack_s ? kStackSplit : kStackFp64; }
+
 int sync_planes(cmm_euler *e) {
     if (e->planes_fresh) return CMM_OK;
     if (!e->dry)
@@ -423,9 +448,9 @@ void planes_written(cmm_euler *e) {
 
 int engine_translate(cmm_euler *e) {
     if (int rc = sync_planes(e)) return rc;
-    if (int rc = op_translate_ic(e->p.g_coarse, e->p.g_fine, e->cx, e->cy, e->stack, e->nstack, e->p.ic,
+    if (int rc = op_translate_ic(e->p.g_coarse, e->p.g_fine, e->cx, e->cy, comp_stack(e), e->nstack, e->p.ic,
                                  e->p.ic_params, e->p.discrete, e->p.discrete ? e->p.g_disc : e->p.g_fine, e->disc,
-                                 e->W, e->work, e->work_bytes, e->st))
+                                 e->W, e->work, e->work_bytes, e->st, comp_fmt(e)))
         return rc;
     // the psi sampler reads the fine vorticity node-interleaved (one 32-byte record per corner,
     // 16 bytes in the split form)
@@ -683,6 +708,7 @@ int cmm_euler_create(const cmm_euler_params *p, cmm_euler **out) {
     if (!rc && sp) rc = falloc(&e->m32[1], 12 * e->nc);
     if (!rc && sp && p->sim == CMM_SIM_EULER_2D) rc = falloc(&e->W4f, 4 * e->nf);
     if (!rc && p->stack_capacity > 0) rc = dalloc(&e->stack, (size_t)8 * e->nc * p->stack_capacity);
+    if (!rc && sp && p->stack_capacity > 0) rc = dalloc(&e->stack_s, (size_t)5 * e->nc * p->stack_capacity);
     if (p->forward_map) {
         if (!rc) rc = dalloc(&e->fx, 4 * e->nc);
         if (!rc) rc = dalloc(&e->fy, 4 * e->nc);
@@ -728,7 +754,7 @@ int cmm_euler_destroy(cmm_euler *e) {
     if (!e) return CMM_OK;
     if (e->st) (void)hipStreamSynchronize(e->st);
     double *bufs[] = {e->cx, e->cy, e->ncx, e->ncy, e->W, e->W4, e->m8[0], e->m8[1], e->sbuf, e->disc, e->stack, e->tmp, e->tmp4,
-                      e->psi_buf[0], e->psi_buf[1], e->psi_buf[2], e->psi_buf[3], e->fx, e->fy, e->nfx, e->nfy, e->fstack};
+                      e->psi_buf[0], e->psi_buf[1], e->psi_buf[2], e->psi_buf[3], e->fx, e->fy, e->nfx, e->nfy, e->fstack, e->stack_s};
     for (double *b : bufs)
         if (b) (void)hipFree(b);
     for (double *b : e->part)
@@ -854,6 +880,7 @@ int cmm_euler_restart(cmm_euler *e, double t_restart, int n_maps, const double *
     // initial condition translated through them with the active map still the identity, then the
     // active map and the older psi levels
     int rc = n_maps > 0 ? put(e->stack, stack_host, (size_t)8 * e->nc * n_maps) : CMM_OK;
+    if (!rc && e->stack_s) rc = op_stack_pack(p.g_coarse, e->stack, n_maps, e->stack_s, e->st);
     e->nstack = n_maps;
     if (!rc) rc = op_map_identity(p.g_coarse, e->cx, e->cy, e->st);
     if (!rc && p.forward_map) rc = op_map_identity(p.g_coarse, e->fx, e->fy, e->st);
@@ -1026,6 +1053,9 @@ int step_finish(cmm_euler *e, cmm_step_info *info, bool waited) {
             double *slotx = e->stack + (size_t)8 * e->nc * e->nstack;
             if (int rc = hip_check(hipMemcpyAsync(slotx, e->cx, sizeof(double) * 4 * e->nc, hipMemcpyDeviceToDevice, e->st), "push map")) return rc;
             if (int rc = hip_check(hipMemcpyAsync(slotx + 4 * e->nc, e->cy, sizeof(double) * 4 * e->nc, hipMemcpyDeviceToDevice, e->st), "push map")) return rc;
+            if (e->stack_s) {
+                if (int rc = op_stack_pack(p.g_coarse, slotx, 1, e->stack_s + (size_t)5 * e->nc * e->nstack, e->st)) return rc;
+            }
             if (p.forward_map) {  // :847-851
                 double *fslot = e->fstack + (size_t)8 * e->nc * e->nstack;
                 if (int rc = hip_check(hipMemcpyAsync(fslot, e->fx, sizeof(double) * 4 * e->nc, hipMemcpyDeviceToDevice, e->st), "push forward map")) return rc;
@@ -1454,14 +1484,14 @@ int cmm_euler_sample(cmm_euler *e, cmm_grid gs, int scalar_ic, double *out) {
     auto get = [&](double *dst, const double *src, size_t cnt) {
         return hip_check(hipMemcpyAsync(dst, src, sizeof(double) * cnt, hipMemcpyDeviceToHost, e->st), "get");
     };
-    int rc = op_compose_positions(gc, gs, e->cx, e->cy, e->stack, e->nstack, xy, e->st);
+    int rc = op_compose_positions(gc, gs, e->cx, e->cy, comp_stack(e), e->nstack, xy, e->st, comp_fmt(e));
     if (!rc) rc = get(hxy.data(), xy, 2 * n);
-    if (!rc) rc = op_sample_init(gc, gs, e->cx, e->cy, e->stack, e->nstack, e->p.ic, e->p.ic_params, e->p.discrete,
-                                 e->p.discrete ? e->p.g_disc : e->p.g_fine, e->disc, w, e->st);
+    if (!rc) rc = op_sample_init(gc, gs, e->cx, e->cy, comp_stack(e), e->nstack, e->p.ic, e->p.ic_params, e->p.discrete,
+                                 e->p.discrete ? e->p.g_disc : e->p.g_fine, e->disc, w, e->st, comp_fmt(e));
     if (!rc) rc = get(out + CMM_SAMPLE_W * n, w, n);
     if (!rc && scalar_ic >= 0)
-        rc = op_sample_init(gc, gs, e->cx, e->cy, e->stack, e->nstack, scalar_ic, e->p.ic_params, 0, gs, nullptr, th,
-                            e->st);
+        rc = op_sample_init(gc, gs, e->cx, e->cy, comp_stack(e), e->nstack, scalar_ic, e->p.ic_params, 0, gs, nullptr, th,
+                            e->st, comp_fmt(e));
     if (!rc && scalar_ic >= 0) rc = get(out + CMM_SAMPLE_THETA * n, th, n);
     // spectral derivatives of the sampled field (the psi-solve workspace is free between steps)
     if (!rc) rc = op_spectral_from_real(gs, w, 2, h, nullptr, 0, e->st);
@@ -1503,13 +1533,13 @@ int cmm_euler_zoom(cmm_euler *e, cmm_grid gz, const int *ics, int n_ics, int wan
             return rc;
         return hip_check(hipStreamSynchronize(e->st), "hipStreamSynchronize");
     };
-    int rc = op_compose_positions(gc, gz, e->cx, e->cy, e->stack, e->nstack, xy, e->st);
+    int rc = op_compose_positions(gc, gz, e->cx, e->cy, comp_stack(e), e->nstack, xy, e->st, comp_fmt(e));
     if (!rc) rc = get(hxy.data(), xy, 2 * n);
     for (int k = 0; !rc && k < n_ics; ++k) {
         const bool own = ics[k] < 0;
         const int disc = own ? e->p.discrete : 0;
-        rc = op_sample_init(gc, gz, e->cx, e->cy, e->stack, e->nstack, own ? e->p.ic : ics[k], e->p.ic_params, disc,
-                            disc ? e->p.g_disc : gz, disc ? e->disc : nullptr, f, e->st);
+        rc = op_sample_init(gc, gz, e->cx, e->cy, comp_stack(e), e->nstack, own ? e->p.ic : ics[k], e->p.ic_params, disc,
+                            disc ? e->p.g_disc : gz, disc ? e->disc : nullptr, f, e->st, comp_fmt(e));
         if (!rc) rc = get(out + (2 + k) * n, f, n);
     }
     if (!rc && want_psi) {

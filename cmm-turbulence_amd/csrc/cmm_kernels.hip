@@ -770,6 +770,50 @@ using SampleVort = SampleVortT<false>;
 using SampleVortIL = SampleVortT<true>;
 
 
+// diffeo() on one map of the split stack (op_stack_pack): FP64 value planes V[0..N), V[N..2N),
+// FP32 derivative planes after them.  The gathers return 160 bytes per evaluation instead of 256,
+// and the composition at depth is bound by the bytes its gathers return per lane.
+__device__ __forceinline__ void diffeo_split_stack(const DGrid &g, const char *__restrict__ M, double x, double y,
+                                                   double &xo, double &yo) {
+    const long N = g.n;
+    const double *__restrict__ V = reinterpret_cast<const double *>(M);
+    const float *__restrict__ D = reinterpret_cast<const float *>(M + 16 * N);
+    Cell c;
+    find_cell(g, x, y, c);
+    const int id[4] = {c.i00, c.i10, c.i01, c.i11};
+    double vx[16], vy[16];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        vx[q] = V[id[q]];
+        vy[q] = V[N + id[q]];
+        vx[4 + q] = D[id[q]];
+        vx[8 + q] = D[N + id[q]];
+        vx[12 + q] = D[2 * N + id[q]];
+        vy[4 + q] = D[3 * N + id[q]];
+        vy[8 + q] = D[4 * N + id[q]];
+        vy[12 + q] = D[5 * N + id[q]];
+    }
+    diffeo_vals(g, c, vx, vy, xo, yo);
+}
+
+__global__ void __launch_bounds__(256) k_stack_pack(const double *__restrict__ stack, char *__restrict__ out, long N,
+                                                    int n_maps) {
+    const long t = (long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= N * n_maps) return;
+    const long m = t / N, i = t % N;
+    const double *S = stack + 8 * N * m;
+    double *V = reinterpret_cast<double *>(out + 40 * N * m);
+    float *D = reinterpret_cast<float *>(out + 40 * N * m + 16 * N);
+    V[i] = S[i];
+    V[N + i] = S[4 * N + i];
+    D[i] = (float)S[N + i];
+    D[N + i] = (float)S[2 * N + i];
+    D[2 * N + i] = (float)S[3 * N + i];
+    D[3 * N + i] = (float)S[5 * N + i];
+    D[4 * N + i] = (float)S[6 * N + i];
+    D[5 * N + i] = (float)S[7 * N + i];
+}
+
 // Initial condition at X_1(...X_M(X_cur(x))): the current map then the stack from newest to
 // oldest, wrapping after each layer (Cuda-CMM/src/simulation/cmm-simulation-host.cu:112-152,
 // Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:302-373).  All layers are applied in
@@ -780,6 +824,7 @@ struct SampleCompose {
     const double *__restrict__ cy;
     const double *__restrict__ stack;
     int nstack;
+    int fmt;  // stack format: kStackFp64 or kStackSplit (the backward position() only)
     ICParams ic;
     int discrete;
     const double *__restrict__ disc;
@@ -792,8 +837,12 @@ struct SampleCompose {
         y = py - floor((py - gm.y0) / gm.ly) * gm.ly;
         const long mstride = 8l * gm.n;
         for (int m = nstack - 1; m >= 0; --m) {
-            const double *sx = stack + mstride * m;
-            diffeo(gm, sx, sx + 4l * gm.n, x, y, px, py);
+            if (fmt == kStackSplit) {
+                diffeo_split_stack(gm, reinterpret_cast<const char *>(stack) + 40l * gm.n * m, x, y, px, py);
+            } else {
+                const double *sx = stack + mstride * m;
+                diffeo(gm, sx, sx + 4l * gm.n, x, y, px, py);
+            }
             x = px - floor((px - gm.x0) / gm.lx) * gm.lx;
             y = py - floor((py - gm.y0) / gm.ly) * gm.ly;
         }
@@ -1814,7 +1863,7 @@ int op_hermite_from_spectrum(const cmm_grid &g, const double *spec_cols, double 
 
 int op_translate_ic(const cmm_grid &gm, const cmm_grid &gf, const double *cx, const double *cy, const double *stack,
                     int nstack, int ic, const double *prm, int discrete, const cmm_grid &gd, const double *disc,
-                    double *herm, void *work, size_t work_bytes, hipStream_t st) {
+                    double *herm, void *work, size_t work_bytes, hipStream_t st, int stack_fmt) {
     if (int rc = fft_grid_ok(gf, "fine grid")) return rc;
     SpecPlan sp = plan_spec(gf, gf, 0, 0.0);
     void *w;
@@ -1827,14 +1876,23 @@ int op_translate_ic(const cmm_grid &gm, const cmm_grid &gf, const double *cx, co
     s.cy = cy;
     s.stack = stack;
     s.nstack = nstack;
+    s.fmt = stack_fmt;
     s.ic = make_ic(ic, prm);
     s.discrete = discrete;
     s.disc = disc;
     return run_spec(sp, s, herm, w, st);
 }
 
+int op_stack_pack(const cmm_grid &gm, const double *stack, int n_maps, void *out, hipStream_t st) {
+    if (n_maps <= 0) return CMM_OK;
+    const long N = (long)gm.nx * gm.ny;
+    hipLaunchKernelGGL(k_stack_pack, dim3((unsigned)((N * n_maps + 255) / 256)), dim3(256), 0, st, stack,
+                       static_cast<char *>(out), N, n_maps);
+    return launch_check("k_stack_pack");
+}
+
 int op_compose_positions(const cmm_grid &gm, const cmm_grid &gt, const double *cx, const double *cy,
-                         const double *stack, int nstack, double *xy, hipStream_t st) {
+                         const double *stack, int nstack, double *xy, hipStream_t st, int stack_fmt) {
     SampleCompose s;
     s.gm = make_dgrid(gm);
     s.gt = make_dgrid(gt);
@@ -1843,6 +1901,7 @@ int op_compose_positions(const cmm_grid &gm, const cmm_grid &gt, const double *c
     s.cy = cy;
     s.stack = stack;
     s.nstack = nstack;
+    s.fmt = stack_fmt;
     s.ic = make_ic(0, nullptr);
     s.discrete = 0;
     s.disc = nullptr;
@@ -1861,6 +1920,7 @@ int op_forward_positions(const cmm_grid &gm, const cmm_grid &gt, const double *c
     s.cy = cy;
     s.stack = stack;
     s.nstack = nstack;
+    s.fmt = kStackFp64;
     s.ic = make_ic(0, nullptr);
     s.discrete = 0;
     s.disc = nullptr;
@@ -1881,7 +1941,7 @@ int op_forward_points(const cmm_grid &gm, const cmm_grid &gb, const double *stac
 
 int op_sample_init(const cmm_grid &gm, const cmm_grid &gt, const double *cx, const double *cy, const double *stack,
                    int nstack, int ic, const double *prm, int discrete, const cmm_grid &gd, const double *disc,
-                   double *out, hipStream_t st) {
+                   double *out, hipStream_t st, int stack_fmt) {
     SampleCompose s;
     s.gm = make_dgrid(gm);
     s.gt = make_dgrid(gt);
@@ -1890,6 +1950,7 @@ int op_sample_init(const cmm_grid &gm, const cmm_grid &gt, const double *cx, con
     s.cy = cy;
     s.stack = stack;
     s.nstack = nstack;
+    s.fmt = stack_fmt;
     s.ic = make_ic(ic, prm);
     s.discrete = discrete;
     s.disc = disc;

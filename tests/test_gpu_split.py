@@ -14,6 +14,11 @@ Bounds written in the tests:
 - the check: |E_split - E_fp64| <= 8 U (max|D'| + h max|D''|)(1 + max|D'|) (FP32 rounding of the
   derivative planes, which enter the Jacobian with weight about one);
 - psi from the split map and vorticity: 1e-6 relative per plane (spectral scale);
+- the split stack (FP64 value planes, FP32 derivative planes): the values bit-identical, the
+  derivatives one FP32 rounding; composed positions within the sum over layers of
+  h/4 U (|dX/dx| + |dX/dy|) + h^2/16 U |d2X/dxdy| (the Hermite weights of the derivative planes sum
+  to at most 1/4 per direction, 1/16 for the cross plane), each carried through the later layers
+  by their gradient norm;
 - the split engine on the double-shear-layer workload (map 128^2, psi 256^2, fine 512^2,
   dt 1/64) to T = 2: relative L2 vorticity error <= 1e-5 against the FP64 engine.
 """
@@ -210,3 +215,57 @@ def test_split_engine_double_shear_layer(dev):
     assert ew[1] <= 1e-5
     assert ep[1] <= 1e-5
     assert ex.max() <= 1e-5
+
+
+def test_split_stack_composition(dev):
+    """positions through a stack of sheared maps, FP64 stack against the split stack"""
+    import cmm
+    import torch
+    nc, nf, nm = 128, 512, 12
+    gm, gt = cmm.grid(nc), cmm.grid(nf)
+    N = nc * nc
+    h = O.TWO_PI / float(np.float32(nc))
+    xs = np.arange(nc) * h
+    X, Y = np.meshgrid(xs, xs)
+    X, Y = X.ravel(), Y.ravel()
+    st = np.empty((nm, 8, N))
+    bound, gains = [], []
+    for m in range(nm):
+        a, b, ph = 0.02 * (1 + m % 4), 0.013 * (1 + m % 3), 0.41 * m
+        st[m, 0] = X + a * np.sin(Y + ph) + b * np.sin(X + Y)
+        st[m, 1] = 1.0 + b * np.cos(X + Y)
+        st[m, 2] = a * np.cos(Y + ph) + b * np.cos(X + Y)
+        st[m, 3] = -b * np.sin(X + Y)
+        st[m, 4] = Y + a * np.sin(X - ph)
+        st[m, 5] = a * np.cos(X - ph)
+        st[m, 6] = 1.0
+        st[m, 7] = 0.0
+        ex = h / 4 * U * (np.abs(st[m, 1]).max() + np.abs(st[m, 2]).max()) + h * h / 16 * U * np.abs(st[m, 3]).max()
+        ey = h / 4 * U * (np.abs(st[m, 5]).max() + np.abs(st[m, 6]).max()) + h * h / 16 * U * np.abs(st[m, 7]).max()
+        bound.append(max(ex, ey))
+        gains.append(max((np.abs(st[m, 1]) + np.abs(st[m, 2])).max(), (np.abs(st[m, 5]) + np.abs(st[m, 6])).max()))
+    # the layers run newest first: the error made in layer m is carried by layers m-1 .. 0
+    tot = sum(bound[m] * float(np.prod(gains[:m])) for m in range(nm))
+    stack = torch.from_numpy(st.ravel()).cuda()
+    packed = torch.zeros(nm * 5 * N, dtype=torch.float64, device="cuda")
+    cmm.stack_pack_split(gm, stack, nm, packed)
+    pb = packed.cpu().numpy().view(np.uint8).reshape(nm, 40 * N)
+    for m in range(nm):
+        v = pb[m, :16 * N].view(np.float64).reshape(2, N)
+        d = pb[m, 16 * N:].view(np.float32).reshape(6, N)
+        assert np.array_equal(v[0], st[m, 0]) and np.array_equal(v[1], st[m, 4])
+        assert np.array_equal(d, st[m, [1, 2, 3, 5, 6, 7]].astype(np.float32))
+    cx = torch.empty(4 * N, dtype=torch.float64, device="cuda")
+    cy = torch.empty_like(cx)
+    cmm.map_init_identity(gm, cx, cy)
+    xy64 = torch.empty(2 * nf * nf, dtype=torch.float64, device="cuda")
+    xys = torch.empty_like(xy64)
+    cmm.apply_map_stack(gt, gm, cx, cy, stack, nm, xy64)
+    cmm.apply_map_stack_split(gt, gm, cx, cy, packed, nm, xys)
+    d = np.abs(xys.cpu().numpy() - xy64.cpu().numpy())
+    d = np.minimum(d, O.TWO_PI - d)
+    print(f"split stack composition, {nm} maps: max position difference {d.max():.3e}, bound {tot:.3e}")
+    assert d.max() <= tot
+    cmm.apply_map_stack_split(gt, gm, cx, cy, packed, 0, xys)
+    cmm.apply_map_stack(gt, gm, cx, cy, stack, 0, xy64)
+    assert torch.equal(xys, xy64)
