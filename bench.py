@@ -276,11 +276,11 @@ def composition(cmm, torch, c, n_maps=32, reps=5):
     # map read once; it is not this kernel's traffic (positions stay in registers)
     ref_b = (n_maps + 1) * (32 * nf * nf + 64 * N)
     del stack, xy, packed, xys
-    return {"kernel": "k_compose_positions", "maps": n_maps + 1, "ms_avg": round(ms, 4), "algorithmic_bytes": b,
+    return {"kernel": "k_compose_positions<0>", "maps": n_maps + 1, "ms_avg": round(ms, 4), "algorithmic_bytes": b,
             "achieved_GBs": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4),
             "point_layers_per_s": round((n_maps + 1) * nf * nf / (ms * 1e-3) / 1e9, 3),
             "reference_dataflow_GBs": round(ref_b / (ms * 1e-3) / 1e9, 1),
-            "split_stack": {"ms_avg": round(ms_s, 4), "speedup": round(ms / ms_s, 3), "algorithmic_bytes": b_s,
+            "split_stack": {"kernel": "k_compose_positions<1>", "ms_avg": round(ms_s, 4), "speedup": round(ms / ms_s, 3), "algorithmic_bytes": b_s,
                             "achieved_GBs": round(gbs_s, 1), "frac_hbm": round(gbs_s / HBM_PEAK_GBS, 4),
                             "point_layers_per_s": round((n_maps + 1) * nf * nf / (ms_s * 1e-3) / 1e9, 3),
                             "reference_dataflow_GBs": round(ref_b / (ms_s * 1e-3) / 1e9, 1),

@@ -1111,7 +1111,10 @@ __global__ void __launch_bounds__(256) k_sample_grid(S sampler, int nx, int ny, 
     out[(long)iy * nx + ix] = sampler(ix, iy);
 }
 
+// FMT: the stack format, fixed at compile time (each format its own kernel)
+template <int FMT>
 __global__ void __launch_bounds__(256) k_compose_positions(SampleCompose sc, double *__restrict__ xy) {
+    sc.fmt = FMT;
     const int ix = blockIdx.x * blockDim.x + threadIdx.x;
     const int iy = blockIdx.y * blockDim.y + threadIdx.y;
     if (ix >= sc.gt.nx || iy >= sc.gt.ny) return;
@@ -1906,7 +1909,8 @@ int op_compose_positions(const cmm_grid &gm, const cmm_grid &gt, const double *c
     s.discrete = 0;
     s.disc = nullptr;
     dim3 b(64, 4);
-    hipLaunchKernelGGL(k_compose_positions, grid2d(gt.nx, gt.ny, b), b, 0, st, s, xy);
+    if (stack_fmt == kStackSplit) hipLaunchKernelGGL(k_compose_positions<kStackSplit>, grid2d(gt.nx, gt.ny, b), b, 0, st, s, xy);
+    else hipLaunchKernelGGL(k_compose_positions<kStackFp64>, grid2d(gt.nx, gt.ny, b), b, 0, st, s, xy);
     return launch_check("k_compose_positions");
 }
 
