@@ -194,7 +194,7 @@ int cmm_sample_init(cmm_grid gm, cmm_grid gt, const double *cx, const double *cy
     return op_sample_init(gm, gt, cx, cy, stack, ns, ic, prm, discrete, discrete ? gd : gt, disc, out, S(stream));
 }
 
-size_t cmm_translate_workspace_bytes(cmm_grid g) { return op_hermite_workspace(g); }
+size_t cmm_translate_workspace_bytes(cmm_grid g) { return grid_ok(g) ? op_translate_workspace(g) : 0; }
 
 int cmm_translate_ic(cmm_grid gm, cmm_grid gf, const double *cx, const double *cy, const double *stack, int ns, int ic,
                      const double *prm, int discrete, cmm_grid gd, const double *disc, double *W, void *work,
@@ -717,7 +717,7 @@ int cmm_euler_create(const cmm_euler_params *p, cmm_euler **out) {
         if (!rc && p->stack_capacity > 0) rc = dalloc(&e->fstack, (size_t)8 * e->nc * p->stack_capacity);
     }
     if (!rc) {
-        size_t w = std::max(op_psi_workspace(p->g_vort, p->g_psi, p->freq_cut_psi), op_hermite_workspace(p->g_fine));
+        size_t w = std::max(op_psi_workspace(p->g_vort, p->g_psi, p->freq_cut_psi), op_translate_workspace(p->g_fine));
         if (p->discrete) w = std::max(w, op_hermite_workspace(p->g_disc));
         // Lagrange-weighted psi combinations of the map advection (up to three Hermite fields on
         // g_psi) and the interleaved map

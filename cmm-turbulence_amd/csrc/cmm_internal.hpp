@@ -45,6 +45,8 @@ int op_psi_from_vort(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf,
 // H4[4j + p] = H[p*n + j] (node-interleaved copy of a Hermite field)
 int op_interleave4(const cmm_grid &g, const double *H, double *H4, hipStream_t st);
 size_t op_hermite_workspace(const cmm_grid &g);
+// op_translate_ic: the spectral buffers and the sampled initial condition (8 bytes per point)
+size_t op_translate_workspace(const cmm_grid &g);
 int op_hermite_from_real(const cmm_grid &g, const double *real, double *herm, void *work, size_t work_bytes,
                          hipStream_t st);
 int op_hermite_from_spectrum(const cmm_grid &g, const double *spec_cols, double *herm, void *work,

@@ -932,15 +932,8 @@ __device__ __forceinline__ void wg_fft(cd *s, int n, int logn, const cd *__restr
 
 // Row R2C of two rows at once (z = row_a + i row_b), samples produced by the sampler.  Writes
 // the half spectrum columns kx <= kx_max transposed: T[kx*ny + y].  Unnormalised.
-// waves per SIMD the row kernel is compiled for: 4 (two 512-lane blocks per CU) unless the
-// sampler's gathers need more registers than that leaves
 template <class S>
-constexpr int r2c_min_waves() {
-    return std::is_same_v<S, SampleCompose> ? 2 : 4;
-}
-
-template <class S>
-__global__ void __launch_bounds__(kFftThreads, r2c_min_waves<S>()) k_row_r2c(S sampler, int nx, int ny, int logn, int kx_max,
+__global__ void __launch_bounds__(kFftThreads, 4) k_row_r2c(S sampler, int nx, int ny, int logn, int kx_max,
                                                           const cd *__restrict__ tw, cd *__restrict__ T) {
     extern __shared__ cd s[];
     const int pair = xcd_block(blockIdx.x, gridDim.x);
@@ -1869,8 +1862,9 @@ int op_translate_ic(const cmm_grid &gm, const cmm_grid &gf, const double *cx, co
                     double *herm, void *work, size_t work_bytes, hipStream_t st, int stack_fmt) {
     if (int rc = fft_grid_ok(gf, "fine grid")) return rc;
     SpecPlan sp = plan_spec(gf, gf, 0, 0.0);
+    const size_t base = sp.t_bytes + 2 * sp.g_bytes + 256;
     void *w;
-    if (int rc = get_work(work, work_bytes, sp.t_bytes + 2 * sp.g_bytes + 256, &w)) return rc;
+    if (int rc = get_work(work, work_bytes, op_translate_workspace(gf), &w)) return rc;
     SampleCompose s;
     s.gm = make_dgrid(gm);
     s.gt = make_dgrid(gf);
@@ -1883,7 +1877,18 @@ int op_translate_ic(const cmm_grid &gm, const cmm_grid &gf, const double *cx, co
     s.ic = make_ic(ic, prm);
     s.discrete = discrete;
     s.disc = disc;
-    return run_spec(sp, s, herm, w, st);
+    // the initial condition at the composed positions into a real field after the spectral
+    // buffers, one point per lane at full occupancy, then the plain row transform of that field:
+    // the composition's dependent gathers need the waves that the row FFT's LDS leaves too few of
+    double *f = reinterpret_cast<double *>(static_cast<char *>(w) + base);
+    dim3 b(64, 4);
+    hipLaunchKernelGGL(k_sample_grid<SampleCompose>, grid2d(gf.nx, gf.ny, b), b, 0, st, s, gf.nx, gf.ny, f);
+    if (int rc = launch_check("k_sample_grid")) return rc;
+    return run_spec(sp, SampleReal{f, gf.nx}, herm, w, st);
+}
+
+size_t op_translate_workspace(const cmm_grid &g) {
+    return spec_workspace_bytes(g, g, 0, 0.0) + sizeof(double) * (size_t)g.nx * g.ny;
 }
 
 int op_stack_pack(const cmm_grid &gm, const double *stack, int n_maps, void *out, hipStream_t st) {

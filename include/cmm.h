@@ -105,9 +105,10 @@ int cmm_forward_points(cmm_grid g_map, cmm_grid g_bounds, const double *chix, co
 int cmm_sample_init(cmm_grid g_map, cmm_grid g_target, const double *chix, const double *chiy, const double *stack,
                     int n_stack, int ic, const double *ic_params, int discrete, cmm_grid g_disc,
                     const double *disc_herm, double *out, void *stream);
-/* New fine vorticity Hermite after a remap: composition + initial condition + R2C + /N +
- * fourier_hermite in one pass; replaces translate_initial_condition_through_map_stack
- * (Cuda-CMM/src/simulation/cmm-simulation-host.cu:299-327). */
+/* New fine vorticity Hermite after a remap: the initial condition at the composed positions
+ * (one sampling pass into the workspace), then R2C + /N + fourier_hermite; replaces
+ * translate_initial_condition_through_map_stack (Cuda-CMM/src/simulation/cmm-simulation-host.cu:299-327).
+ * The workspace holds the spectral buffers and 8 bytes per fine point. */
 size_t cmm_translate_workspace_bytes(cmm_grid g_fine);
 int cmm_translate_ic(cmm_grid g_map, cmm_grid g_fine, const double *chix, const double *chiy, const double *stack,
                      int n_stack, int ic, const double *ic_params, int discrete, cmm_grid g_disc,
