@@ -1255,6 +1255,7 @@ struct VP1D {
     double hv;           // vort grid spacing along v
     double fkx_v;        // 2 pi / (hx NX) of the vort grid
     double fky_p;        // 2 pi / (hy NY) of the psi grid
+    double fkx_p;        // 2 pi / (hx NX) of the psi grid
     double v0, hv_p, Lv; // psi grid v origin, spacing, length
 };
 
@@ -1306,7 +1307,26 @@ __device__ void inv_lap_c2r(cd *s, int n_in, int n_out, int logn_out, double sca
     wg_fft(s, n_out, logn_out, tw_out, true);
 }
 
-// one workgroup: phi (np values) then H (np values) into phiH
+// Spectral derivative of the n real values held in s[].x (any imaginary parts are ignored):
+// forward transform, times i fk k / n with the Nyquist bin dropped, inverse transform; the
+// derivative is left in s[].x.  The 2D Hermite transform of a field that depends on one
+// coordinate only reduces to this along that coordinate: its column stage takes the Nyquist
+// index positive and its C2R keeps the real part, which drops that bin's purely imaginary term.
+__device__ void spec_deriv_1d(cd *s, int n, int logn, double fk, const cd *__restrict__ tw) {
+    for (int i = threadIdx.x; i < n; i += kFftThreads) s[sw(i)].y = 0.0;
+    __syncthreads();
+    wg_fft(s, n, logn, tw, false);
+    for (int k = threadIdx.x; k < n; k += kFftThreads) {
+        const int kk = k < n / 2 ? k : (k == n / 2 ? 0 : k - n);
+        const double fac = fk * kk / (double)n;
+        const cd v = s[sw(k)];
+        s[sw(k)] = cd{-v.y * fac, v.x * fac};
+    }
+    __syncthreads();
+    wg_fft(s, n, logn, tw, true);
+}
+
+// one workgroup: phi, H, phi' and H' (np values each) into phiH
 __global__ void __launch_bounds__(kFftThreads) k_vp_1d(VP1D q, const double *__restrict__ f,
                                                        const double *__restrict__ part, const cd *__restrict__ tw_v,
                                                        const cd *__restrict__ tw_p, double *__restrict__ phiH) {
@@ -1324,6 +1344,8 @@ __global__ void __launch_bounds__(kFftThreads) k_vp_1d(VP1D q, const double *__r
     wg_fft(s, q.nv, q.lognv, tw_v, false);
     inv_lap_c2r(s, q.nv, q.np, q.lognp, 1.0 / (double)q.nv, q.fkx_v, tw_p);
     for (int x = threadIdx.x; x < q.np; x += kFftThreads) phiH[x] = s[sw(x)].x;
+    spec_deriv_1d(s, q.np, q.lognp, q.fkx_p, tw_p);
+    for (int x = threadIdx.x; x < q.np; x += kFftThreads) phiH[2 * q.np + x] = s[sw(x)].x;
     __syncthreads();
     // bump function over v on the psi grid
     double sum = 0.0;
@@ -1346,14 +1368,26 @@ __global__ void __launch_bounds__(kFftThreads) k_vp_1d(VP1D q, const double *__r
     wg_fft(s, q.np, q.lognp, tw_p, false);
     inv_lap_c2r(s, q.np, q.np, q.lognp, 1.0 / (double)q.np, q.fky_p, tw_p);
     for (int i = threadIdx.x; i < q.np; i += kFftThreads) phiH[q.np + i] = s[sw(i)].x;
+    spec_deriv_1d(s, q.np, q.lognp, q.fky_p, tw_p);
+    for (int i = threadIdx.x; i < q.np; i += kFftThreads) phiH[3 * q.np + i] = s[sw(i)].x;
 }
 
-// psi(x, v) = phi(x) - H(v) on the psi grid (k_assemble_psi_periodified)
-struct SamplePsiVP {
-    const double *__restrict__ phiH;
-    int np;
-    __device__ double operator()(int ix, int iy) const { return phiH[ix] - phiH[np + iy]; }
-};
+// The Hermite planes of psi(x, v) = phi(x) - H(v) on the psi grid (k_assemble_psi_periodified and
+// the Hermite transform of its result): psi, d/dx = phi', d/dv = -H', d2/dxdv = 0.  psi is a sum
+// of one-variable functions, so its 2D spectrum lies on the two axes and these are the planes
+// the 2D transform gives, without it.
+__global__ void __launch_bounds__(256) k_vp_psi_planes(const double *__restrict__ phiH, int np, int lognp,
+                                                       double *__restrict__ out) {
+    const long N = (long)np * np;
+    const long t = (long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= N) return;
+    const int x = (int)(t & (np - 1)), y = (int)(t >> lognp);
+    out[t] = phiH[x] - phiH[np + y];
+    out[N + t] = phiH[2 * np + x];
+    out[2 * N + t] = -phiH[3 * np + y];
+    out[3 * N + t] = 0.0;
+}
+
 
 // ------------------------------------------------------------------------------------------------
 // Host-side helpers: twiddle tables and workspaces, cached per device.
@@ -1988,8 +2022,7 @@ static int vp_check(const cmm_grid &gv, const cmm_grid &gp) {
 
 size_t op_vp_psi_workspace(const cmm_grid &gv, const cmm_grid &gp) {
     const size_t nchunks = (gv.ny + kVpRows - 1) / kVpRows;
-    const size_t head = sizeof(double) * ((size_t)gv.nx * gv.ny + nchunks * gv.nx + 2 * (size_t)gp.nx);
-    return ((head + 255) & ~(size_t)255) + spec_workspace_bytes(gp, gp, 0, 0.0);
+    return sizeof(double) * ((size_t)gv.nx * gv.ny + nchunks * gv.nx + 4 * (size_t)gp.nx);
 }
 
 int op_vp_psi_from_f(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf, const cmm_grid &gp, const double *cx,
@@ -2002,8 +2035,6 @@ int op_vp_psi_from_f(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf,
     double *f = static_cast<double *>(w);
     double *part = f + (size_t)gv.nx * gv.ny;
     double *phiH = part + (size_t)nchunks * gv.nx;
-    const size_t head = sizeof(double) * ((size_t)gv.nx * gv.ny + (size_t)nchunks * gv.nx + 2 * (size_t)gp.nx);
-    void *wspec = static_cast<char *>(w) + ((head + 255) & ~(size_t)255);
     if (int rc = op_sample_vort(gm, gv, gf, cx, cy, W, molly, f, st)) return rc;
     hipLaunchKernelGGL(k_vp_colsum, dim3((gv.nx + 255) / 256, nchunks), dim3(256), 0, st, f, gv.nx, gv.ny, kVpRows,
                        part);
@@ -2021,6 +2052,7 @@ int op_vp_psi_from_f(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf,
     q.hv = dv.hy;
     q.fkx_v = kTwoPi / (dv.hx * gv.nx);
     q.fky_p = kTwoPi / (dp.hy * gp.ny);
+    q.fkx_p = kTwoPi / (dp.hx * gp.nx);
     q.v0 = gp.y0;
     q.hv_p = dp.hy;
     q.Lv = gp.y1 - gp.y0;
@@ -2031,8 +2063,10 @@ int op_vp_psi_from_f(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf,
         return rc;
     hipLaunchKernelGGL(k_vp_1d, dim3(1), dim3(kFftThreads), lds, st, q, f, part, tw_v, tw_p, phiH);
     if (int rc = launch_check("k_vp_1d")) return rc;
-    SamplePsiVP s{phiH, gp.nx};
-    return run_spec(plan_spec(gp, gp, 0, 0.0), s, psi, wspec, st);
+    const long N = (long)gp.nx * gp.ny;
+    hipLaunchKernelGGL(k_vp_psi_planes, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, phiH, gp.nx, ilog2(gp.nx),
+                       psi);
+    return launch_check("k_vp_psi_planes");
 }
 
 int op_sample_vort(const cmm_grid &gm, const cmm_grid &gt, const cmm_grid &gf, const double *cx, const double *cy,
