@@ -1326,27 +1326,30 @@ __device__ void spec_deriv_1d(cd *s, int n, int logn, double fk, const cd *__res
     wg_fft(s, n, logn, tw, true);
 }
 
-// one workgroup: phi, H, phi' and H' (np values each) into phiH
+// two workgroups: phi and phi' (block 0), H and H' (block 1, independent of f), np values each,
+// into phiH = [phi, H, phi', H']
 __global__ void __launch_bounds__(kFftThreads) k_vp_1d(VP1D q, const double *__restrict__ f,
                                                        const double *__restrict__ part, const cd *__restrict__ tw_v,
                                                        const cd *__restrict__ tw_p, double *__restrict__ phiH) {
     extern __shared__ cd s[];
     __shared__ double red[kFftThreads];
-    // I(x): the row v0 once more, then the chunk sums in order
-    for (int x = threadIdx.x; x < q.nv; x += kFftThreads) {
-        double a = f[x];
-        for (int c = 0; c < q.nchunks; ++c) a += part[(long)c * q.nv + x];
-        a *= q.hv;
-        a += -1.0;
-        s[sw(x)] = cd{a, 0.0};
+    if (blockIdx.x == 0) {
+        // I(x): the row v0 once more, then the chunk sums in order
+        for (int x = threadIdx.x; x < q.nv; x += kFftThreads) {
+            double a = f[x];
+            for (int c = 0; c < q.nchunks; ++c) a += part[(long)c * q.nv + x];
+            a *= q.hv;
+            a += -1.0;
+            s[sw(x)] = cd{a, 0.0};
+        }
+        __syncthreads();
+        wg_fft(s, q.nv, q.lognv, tw_v, false);
+        inv_lap_c2r(s, q.nv, q.np, q.lognp, 1.0 / (double)q.nv, q.fkx_v, tw_p);
+        for (int x = threadIdx.x; x < q.np; x += kFftThreads) phiH[x] = s[sw(x)].x;
+        spec_deriv_1d(s, q.np, q.lognp, q.fkx_p, tw_p);
+        for (int x = threadIdx.x; x < q.np; x += kFftThreads) phiH[2 * q.np + x] = s[sw(x)].x;
+        return;
     }
-    __syncthreads();
-    wg_fft(s, q.nv, q.lognv, tw_v, false);
-    inv_lap_c2r(s, q.nv, q.np, q.lognp, 1.0 / (double)q.nv, q.fkx_v, tw_p);
-    for (int x = threadIdx.x; x < q.np; x += kFftThreads) phiH[x] = s[sw(x)].x;
-    spec_deriv_1d(s, q.np, q.lognp, q.fkx_p, tw_p);
-    for (int x = threadIdx.x; x < q.np; x += kFftThreads) phiH[2 * q.np + x] = s[sw(x)].x;
-    __syncthreads();
     // bump function over v on the psi grid
     double sum = 0.0;
     for (int i = threadIdx.x; i < q.np; i += kFftThreads) {
@@ -2061,7 +2064,7 @@ int op_vp_psi_from_f(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf,
     if (int rc = hip_check(hipFuncSetAttribute((const void *)k_vp_1d, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
                            "hipFuncSetAttribute"))
         return rc;
-    hipLaunchKernelGGL(k_vp_1d, dim3(1), dim3(kFftThreads), lds, st, q, f, part, tw_v, tw_p, phiH);
+    hipLaunchKernelGGL(k_vp_1d, dim3(2), dim3(kFftThreads), lds, st, q, f, part, tw_v, tw_p, phiH);
     if (int rc = launch_check("k_vp_1d")) return rc;
     const long N = (long)gp.nx * gp.ny;
     hipLaunchKernelGGL(k_vp_psi_planes, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, phiH, gp.nx, ilog2(gp.nx),
