@@ -12,6 +12,15 @@
  *   lagrange / stencil coefficients  closed forms (3,-3,1), (1.875,-1.25,0.375), weights sum
  *   map advection .................. psi=0 identity, linear psi translation, fixture
  *                                    euler2D_one_step map statistics after warm-up + 1 step
+ *   time schemes (all backward map   uniform velocity polynomial in time: exact quadrature of
+ *   schemes, particle schemes) ..... each scheme (uneven level spacing where the scheme allows
+ *                                    it); local-error order 2^(p+1) per halving of dt on a
+ *                                    time-dependent bicubic flow; forward schemes: order in a
+ *                                    steady flow, RK4Mod = the backward formula
+ *                                    (test_oracle_schemes.py)
+ *   footpoint stencils 2nd/4th/6th . exact on polynomials of total degree 1/3/5 (6th: every
+ *                                    bicubic), closed-form residuals x^2 + eps^2 and
+ *                                    x^2 y^2 - 4 eps^4 beyond (test_oracle_schemes.py)
  *   incompressibility .............. identity gives 0, fixture E_inc one-step and 11b series
  *   FFT / fourier_hermite / psi .... numpy.fft golden vectors, 4_nodes closed form, fixture
  *                                    psi / velocity statistics
@@ -1061,6 +1070,21 @@ int cmmo_map_advect(cmm_grid ggm, cmm_grid ggp, const double *cx, const double *
             for (int p = 0; p < 4; ++p) { nx_[p * gm.n + i] = ax[p]; ny_[p * gm.n + i] = ay[p]; }
         }
     }
+    return CMM_OK;
+}
+
+// One footpoint of the map advection: the backward (direction -1) or forward (+1) time
+// integrator applied to the point (x, y), Cuda-CMM/src/numerical/cmm-timestep.cu:28-470, without
+// the old map or the stencil (what k_advect_using_stream_hermite feeds them,
+// Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:232-274)
+int cmmo_footpoint(cmm_grid ggp, const double *psi, const cmm_advect_coeffs *c, double x, double y, double *out) {
+    if (!c || !psi || !out || (c->direction != -1 && c->direction != 1)) return CMM_ERR_ARG;
+    Grid gp = mk(ggp);
+    const double xin[2] = {x, y};
+    if (c->direction == -1)
+        step_backward(gp, psi, *c, xin, out);
+    else
+        step_forward(gp, psi, *c, xin, out);
     return CMM_OK;
 }
 

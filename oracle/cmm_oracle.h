@@ -40,6 +40,9 @@ int cmmo_map_init_identity(cmm_grid g, double *chix, double *chiy);
 /* Backward map advection, Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:207-298 */
 int cmmo_map_advect(cmm_grid gm, cmm_grid gpsi, const double *chix, const double *chiy,
                     const double *psi, const cmm_advect_coeffs *c, double *chix_new, double *chiy_new);
+/* One footpoint of the backward (c->direction -1) or forward (+1) time integrator at (x, y),
+ * Cuda-CMM/src/numerical/cmm-timestep.cu:28-470; out = (x_f, y_f) */
+int cmmo_footpoint(cmm_grid gpsi, const double *psi, const cmm_advect_coeffs *c, double x, double y, double *out);
 /* max |1 - det grad X| on cell centres of gcheck, Cuda-CMM/src/simulation/cmm-simulation-host.cu:44-50 */
 double cmmo_incomp_error(cmm_grid gm, cmm_grid gcheck, const double *chix, const double *chiy);
 double cmmo_invert_error(cmm_grid g_back, cmm_grid g_fwd, cmm_grid gcheck, const double *bx, const double *by,

@@ -285,3 +285,12 @@ def euler_run(g_coarse, g_fine, g_psi, g_vort, dt, final_time, tin=30, muo=1, l_
     if particles is not None:
         out["particles"] = pos
     return out
+
+
+def footpoint(gp, psi, coeffs, x, y):
+    """One footpoint of the oracle's time integrator (direction from coeffs) at (x, y)."""
+    out = np.zeros(2)
+    psi = np.ascontiguousarray(psi, dtype=np.float64)
+    rc = lib().cmmo_footpoint(gp, ptr(psi), C.byref(coeffs), C.c_double(x), C.c_double(y), ptr(out))
+    assert rc == 0
+    return out
