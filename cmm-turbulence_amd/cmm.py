@@ -78,7 +78,7 @@ SYMBOLS = ["cmm_version", "cmm_last_error", "cmm_device_info", "cmm_lagrange_coe
            "cmm_sample_init", "cmm_translate_workspace_bytes", "cmm_translate_ic", "cmm_hermite_workspace_bytes",
            "cmm_hermite_from_real", "cmm_hermite_from_spectrum", "cmm_psi_workspace_bytes", "cmm_psi_from_vort",
            "cmm_spectral_workspace_bytes", "cmm_spectral_from_real", "cmm_sample_vort", "cmm_vp_psi_workspace_bytes", "cmm_vp_psi_from_f", "cmm_particles_advect", "cmm_euler_create", "cmm_euler_destroy", "cmm_euler_stream",
-           "cmm_euler_add_particles", "cmm_euler_init", "cmm_euler_step", "cmm_euler_get_field",
+           "cmm_euler_add_particles", "cmm_euler_add_particle_set", "cmm_euler_get_particle_velocities", "cmm_euler_init", "cmm_euler_step", "cmm_euler_get_field",
            "cmm_euler_get_particles", "cmm_euler_diagnostics", "cmm_euler_vp_diagnostics", "cmm_euler_sample", "cmm_euler_zoom", "cmm_euler_restart", "cmm_euler_get_stack_map",
            "cmm_euler_restart_forward", "cmm_euler_get_forward_stack_map", "cmm_euler_request_remap",
            "cmm_euler_forward_positions", "cmm_euler_forward_points", "cmm_euler_state", "cmm_euler_profile", "cmm_euler_profile_read",
@@ -147,6 +147,9 @@ def lib():
         L.cmm_euler_stream.restype = VP
         L.cmm_euler_stream.argtypes = [VP]
         L.cmm_euler_add_particles.argtypes = [VP, C.c_int, C.c_int, DP, C.POINTER(C.c_int)]
+        L.cmm_euler_add_particle_set.argtypes = [VP, C.c_int, C.c_int, C.c_double, C.c_double, C.c_int, DP, DP,
+                                                 C.POINTER(C.c_int)]
+        L.cmm_euler_get_particle_velocities.argtypes = [VP, C.c_int, DP]
         L.cmm_euler_init.argtypes = [VP, DP, C.POINTER(C.c_int)]
         L.cmm_euler_step.argtypes = [VP, C.c_double, C.POINTER(StepInfo)]
         L.cmm_euler_get_field.argtypes = [VP, C.c_int, DP]
@@ -460,14 +463,27 @@ class Euler:
         except Exception:
             pass
 
-    def add_particles(self, pos, scheme=30):
+    def add_particles(self, pos, scheme=30, init_time=0.0, tau=0.0, init_vel=False, vel=None):
+        """A particle set: fluid particles (tau 0) or inertial ones (tau != 0), moving from
+        init_time on (cmm_euler_add_particle_set)."""
         import numpy as np
         pos = np.ascontiguousarray(pos, dtype=np.float64)
         s = C.c_int()
-        _check(lib().cmm_euler_add_particles(self.h, pos.size // 2, scheme, pos.ctypes.data_as(DP), C.byref(s)),
-               "cmm_euler_add_particles")
+        v = None if vel is None else np.ascontiguousarray(vel, dtype=np.float64)
+        _check(lib().cmm_euler_add_particle_set(self.h, pos.size // 2, scheme, float(init_time), float(tau),
+                                                int(init_vel), pos.ctypes.data_as(DP),
+                                                None if v is None else v.ctypes.data_as(DP), C.byref(s)),
+               "cmm_euler_add_particle_set")
         self.sets.append(pos.size // 2)
         return s.value
+
+    def particle_velocities(self, s):
+        """inertial sets: their velocities; fluid sets: (d psi/dx, d psi/dy) at the positions"""
+        import numpy as np
+        out = np.zeros(2 * self.sets[s])
+        _check(lib().cmm_euler_get_particle_velocities(self.h, s, out.ctypes.data_as(DP)),
+               "cmm_euler_get_particle_velocities")
+        return out
 
     def init(self):
         w = (C.c_double * 8)()

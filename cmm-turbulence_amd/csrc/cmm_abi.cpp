@@ -357,6 +357,11 @@ struct cmm_euler {
     // dry: a step's host state advances without device work (the work is replayed from a captured
     // graph); capturing: the step's work is being captured (no event records)
     bool dry = false, capturing = false;
+    // graph steps (captured or replayed): the advection reads its Lagrange weights from lw_dev
+    // (device), which the batch fills from lw_host (pinned) before each graph launch, since the
+    // weights follow the whole time history and a graph's kernel arguments are frozen
+    bool lw_graph = false;
+    double *lw_dev = nullptr, *lw_host = nullptr;
     IncompTune tune;  // the check's tile and grid, from cmm_euler_autotune
     // corrections (device Hermite fields on g_psi) added to the Lagrange extrapolations of psi
     // to t_{n+1} and t_n + dt/2 in the next steps (cmm_euler_set_psi_correction)
@@ -381,6 +386,12 @@ struct cmm_euler {
     bool part_pending = false;
     std::vector<double *> part;
     std::vector<int> part_n, part_scheme;
+    // per set: start time (the set moves once t_{n+1} + 1e-5 dt reaches it), tau (non-zero:
+    // inertial particles with velocities part_vel), the start-velocity rule and whether the caller
+    // gave the velocities
+    std::vector<double *> part_vel;
+    std::vector<double> part_t0, part_tau;
+    std::vector<int> part_init_vel, part_vel_given;
     std::vector<double> tv, dtv;
     int loop = 0;
     bool initialised = false;
@@ -485,6 +496,11 @@ int engine_advect(cmm_euler *e, const double *t, const double *dt, int loop_ctr,
         return rc;
     PsiLevels P;
     for (int l = 0; l < 4; ++l) P.p[l] = e->lvl[l < L ? l : 0];
+    if (e->lw_graph) {
+        std::memcpy(e->lw_host, c.L1, sizeof(c.L1));
+        std::memcpy(e->lw_host + 4, c.L12, sizeof(c.L12));
+    }
+    const double *lw = e->lw_graph ? e->lw_dev : nullptr;
     if (split(e) && op_advect32_supported(c)) {
         if (int rc = sync_m32(e)) return rc;
         if (!e->dry)
@@ -493,7 +509,7 @@ int engine_advect(cmm_euler *e, const double *t, const double *dt, int loop_ctr,
             if (int rc = op_map_advect32(e->p.g_coarse, e->p.g_psi, P, c, e->m32[e->m32_cur], e->m32[1 - e->m32_cur],
                                          e->work, e->work_bytes, e->st, e->prof ? e->ev[10] : nullptr,
                                          e->p.sim == CMM_SIM_VLASOV_POISSON_1D, e->initialised ? e->corr1 : nullptr,
-                                         e->initialised ? e->corr12 : nullptr))
+                                         e->initialised ? e->corr12 : nullptr, lw))
                 return rc;
         e->m32_cur = 1 - e->m32_cur;
         e->m32_fresh = true;
@@ -513,7 +529,7 @@ int engine_advect(cmm_euler *e, const double *t, const double *dt, int loop_ctr,
     if (!e->dry)
         if (int rc = op_map_advect(e->p.g_coarse, e->p.g_psi, e->cx, e->cy, P, c, e->ncx, e->ncy, e->work,
                                    e->work_bytes, e->st, e->prof ? e->ev[10] : nullptr, m8_in, m8_out,
-                                   e->initialised ? e->corr1 : nullptr, e->initialised ? e->corr12 : nullptr))
+                                   e->initialised ? e->corr1 : nullptr, e->initialised ? e->corr12 : nullptr, lw))
             return rc;
     std::swap(e->cx, e->ncx);
     std::swap(e->cy, e->ncy);
@@ -732,6 +748,10 @@ int cmm_euler_create(const cmm_euler_params *p, cmm_euler **out) {
         set_error("hipHostMalloc slot");
         rc = CMM_ERR_ALLOC;
     }
+    if (!rc) {  // bytes 64..127 of the slots: the Lagrange weights of graph steps
+        e->lw_dev = reinterpret_cast<double *>(e->slot + 8);
+        e->lw_host = reinterpret_cast<double *>(e->h_slot + 8);
+    }
     if (!rc) rc = hip_check(hipEventCreateWithFlags(&e->ev_inc, hipEventDisableTiming), "hipEventCreate");
     if (!rc) rc = hip_check(hipEventCreateWithFlags(&e->ev_psi, hipEventDisableTiming), "hipEventCreate");
     if (!rc) rc = hip_check(hipEventCreateWithFlags(&e->ev_part, hipEventDisableTiming), "hipEventCreate");
@@ -759,6 +779,8 @@ int cmm_euler_destroy(cmm_euler *e) {
         if (b) (void)hipFree(b);
     for (double *b : e->part)
         if (b) (void)hipFree(b);
+    for (double *b : e->part_vel)
+        if (b) (void)hipFree(b);
     for (float *b : {e->m32[0], e->m32[1], e->W4f})
         if (b) (void)hipFree(b);
     if (e->work) (void)hipFree(e->work);
@@ -785,22 +807,48 @@ int cmm_euler_destroy(cmm_euler *e) {
 
 void *cmm_euler_stream(cmm_euler *e) { return e ? (void *)e->st : nullptr; }
 
-int cmm_euler_add_particles(cmm_euler *e, int n, int scheme, const double *pos, int *set) {
-    if (!e || n < 0 || (n > 0 && !pos)) { set_error("cmm_euler_add_particles: bad argument"); return CMM_ERR_ARG; }
-    if (scheme != 10 && scheme != 20 && scheme != 30 && scheme != 31 && scheme != 40 && scheme != 41) {
-        set_error("cmm_euler_add_particles: unknown scheme");
+int cmm_euler_add_particle_set(cmm_euler *e, int n, int scheme, double init_time, double tau, int init_vel,
+                               const double *pos, const double *vel, int *set) {
+    if (!e || n < 0 || (n > 0 && !pos) || !std::isfinite(init_time) || !std::isfinite(tau)) {
+        set_error("cmm_euler_add_particle_set: bad argument");
         return CMM_ERR_ARG;
     }
-    double *d = nullptr;
+    if (scheme != 10 && scheme != 20 && scheme != 30 && scheme != 31 && scheme != 40 && scheme != 41) {
+        set_error("cmm_euler_add_particle_set: unknown scheme");
+        return CMM_ERR_ARG;
+    }
+    double *d = nullptr, *v = nullptr;
     if (n > 0) {
         if (int rc = dalloc(&d, 2 * (size_t)n)) return rc;
-        if (int rc = hip_check(hipMemcpy(d, pos, sizeof(double) * 2 * n, hipMemcpyHostToDevice), "upload particles")) return rc;
+        if (int rc = hip_check(hipMemcpy(d, pos, sizeof(double) * 2 * n, hipMemcpyHostToDevice), "upload particles")) {
+            (void)hipFree(d);
+            return rc;
+        }
+        if (tau != 0) {
+            if (int rc = dalloc(&v, 2 * (size_t)n)) { (void)hipFree(d); return rc; }
+            int rc = vel ? hip_check(hipMemcpy(v, vel, sizeof(double) * 2 * n, hipMemcpyHostToDevice), "upload velocities")
+                         : hip_check(hipMemset(v, 0, sizeof(double) * 2 * n), "clear velocities");
+            // a set added after the warm-up starts at the current psi, as cmm_euler_init would
+            if (!rc && !vel && e->initialised && init_vel)
+                rc = op_particles_vel_init(e->p.g_psi, e->lvl[0], n, 1, d, v, e->st);
+            if (!rc) rc = hip_check(hipStreamSynchronize(e->st), "hipStreamSynchronize");
+            if (rc) { (void)hipFree(d); (void)hipFree(v); return rc; }
+        }
     }
     e->part.push_back(d);
     e->part_n.push_back(n);
     e->part_scheme.push_back(scheme);
+    e->part_vel.push_back(v);
+    e->part_t0.push_back(init_time);
+    e->part_tau.push_back(tau);
+    e->part_init_vel.push_back(init_vel ? 1 : 0);
+    e->part_vel_given.push_back(vel ? 1 : 0);
     if (set) *set = (int)e->part.size() - 1;
     return CMM_OK;
+}
+
+int cmm_euler_add_particles(cmm_euler *e, int n, int scheme, const double *pos, int *set) {
+    return cmm_euler_add_particle_set(e, n, scheme, 0.0, 0.0, 0, pos, nullptr, set);
 }
 
 int cmm_euler_init(cmm_euler *e, double *warm, int *nwarm) {
@@ -851,6 +899,12 @@ int cmm_euler_init(cmm_euler *e, double *warm, int *nwarm) {
         }
     }
     if (nwarm) *nwarm = std::min(nw, 8);
+    // start velocities of inertial particles from psi at t = 0, Cuda-CMM/src/simulation/cmm-euler2d.cu:620-632
+    for (size_t s = 0; s < e->part.size(); ++s)
+        if (e->part_vel[s] && !e->part_vel_given[s])
+            if (int rc = op_particles_vel_init(p.g_psi, e->lvl[0], e->part_n[s], e->part_init_vel[s], e->part[s],
+                                               e->part_vel[s], e->st))
+                return rc;
     // time vectors, Cuda-CMM/src/simulation/cmm-euler2d.cu:709-715
     e->tv.assign(L, 0.0);
     e->dtv.assign(L, p.dt);
@@ -1070,22 +1124,39 @@ int step_finish(cmm_euler *e, cmm_step_info *info, bool waited) {
             remapped = 1;
         }
     }
-    // particles, :881 with Cuda-CMM/src/numerical/cmm-particles.cu:63-87, on the second stream
+    // particles, :881 with Cuda-CMM/src/numerical/cmm-particles.cu:63-87, on the second stream: a set
+    // moves once t_{n+1} + 1e-5 dt reaches its start time; an inertial set's velocity is set again
+    // at the step that reaches it (:883-895)
     bool moved = false;
     if (!e->part.empty()) {
         double Lp[4] = {0, 0, 0, 0};
         for (int i = 0; i < L; ++i) Lp[i] = cmm_lagrange_coeff(e->tv.data(), e->tv[lc] + e->dtv[lc + 1] / 2.0, lc + 1, i, L);
         PsiLevels P;
         for (int l = 0; l < 4; ++l) P.p[l] = e->lvl[l < L ? l : 0];
+        const double t1 = e->tv[lc + 1], dt1 = e->dtv[lc + 1];
         for (size_t s = 0; s < e->part.size(); ++s) {
             if (e->part_n[s] <= 0) continue;
             if (L < 2 && e->part_scheme[s] != 10 && e->part_scheme[s] != 20) continue;
+            const bool move = t1 + 1e-5 * dt1 >= e->part_t0[s];
+            const bool start = e->part_vel[s] && e->tv[lc] - e->part_t0[s] + p.dt * 1e-5 < 0 &&
+                               t1 - e->part_t0[s] + p.dt * 1e-5 > 0;
+            if (!move && !start) continue;
             if (!moved) {
                 if (int rc = hip_check(hipEventRecord(e->ev_psi, e->st), "hipEventRecord")) return rc;
                 if (int rc = hip_check(hipStreamWaitEvent(e->st2, e->ev_psi, 0), "hipStreamWaitEvent")) return rc;
                 moved = true;
             }
-            if (int rc = op_particles(p.g_psi, P, L, Lp, e->dtv[lc + 1], e->part_scheme[s], e->part_n[s], e->part[s], e->st2)) return rc;
+            if (move && !e->part_vel[s])
+                if (int rc = op_particles(p.g_psi, P, L, Lp, dt1, e->part_scheme[s], e->part_n[s], e->part[s], e->st2))
+                    return rc;
+            if (move && e->part_vel[s])
+                if (int rc = op_particles_inertial(p.g_psi, P, L, Lp, dt1, e->part_scheme[s], e->part_tau[s],
+                                                   e->part_n[s], e->part[s], e->part_vel[s], e->st2))
+                    return rc;
+            if (start)
+                if (int rc = op_particles_vel_init(p.g_psi, e->lvl[0], e->part_n[s], e->part_init_vel[s], e->part[s],
+                                                   e->part_vel[s], e->st2))
+                    return rc;
         }
     }
     if (moved) {
@@ -1237,19 +1308,31 @@ int cmm_batch_step(cmm_batch *b, double dt, cmm_step_info *infos) {
         const std::string key = batch_key(b);
         auto it = b->cache.find(key);
         hipGraphExec_t exec = nullptr;
+        for (cmm_euler *e : b->e) e->lw_graph = true;
+        int rc_q = CMM_OK;
         if (it == b->cache.end()) {
-            if (int rc = batch_capture(b, dt, &exec)) return rc;
-            b->cache[key] = exec;
-            b->captured++;
+            rc_q = batch_capture(b, dt, &exec);
+            if (!rc_q) {
+                b->cache[key] = exec;
+                b->captured++;
+            }
         } else {
             exec = it->second;
             for (cmm_euler *e : b->e) {
                 e->dry = true;
-                const int rc = step_queue(e, dt);
+                rc_q = step_queue(e, dt);
                 e->dry = false;
-                if (rc) return rc;
+                if (rc_q) break;
             }
         }
+        for (cmm_euler *e : b->e) e->lw_graph = false;
+        if (rc_q) return rc_q;
+        // this step's Lagrange weights, in stream order before the graph's kernels read them
+        for (cmm_euler *e : b->e)
+            if (int rc = hip_check(hipMemcpyAsync(e->lw_dev, e->lw_host, 8 * sizeof(double), hipMemcpyHostToDevice,
+                                                  b->master),
+                                   "hipMemcpyAsync weights"))
+                return rc;
         if (int rc = hip_check(hipGraphLaunch(exec, b->master), "hipGraphLaunch")) return rc;
         if (int rc = hip_check(hipStreamSynchronize(b->master), "hipStreamSynchronize")) return rc;
         b->launched++;
@@ -1383,6 +1466,30 @@ int cmm_euler_get_particles(cmm_euler *e, int set, double *out) {
     if (e->part_n[set] <= 0) return CMM_OK;
     if (int rc = hip_check(hipMemcpyAsync(out, e->part[set], sizeof(double) * 2 * e->part_n[set], hipMemcpyDeviceToHost, e->st2), "get")) return rc;
     return hip_check(hipStreamSynchronize(e->st2), "hipStreamSynchronize");
+}
+
+int cmm_euler_get_particle_velocities(cmm_euler *e, int set, double *out) {
+    if (!e || set < 0 || set >= (int)e->part.size() || !out) {
+        set_error("cmm_euler_get_particle_velocities: bad argument");
+        return CMM_ERR_ARG;
+    }
+    const int n = e->part_n[set];
+    if (n <= 0) return CMM_OK;
+    if (e->part_vel[set]) {
+        if (int rc = hip_check(hipMemcpyAsync(out, e->part_vel[set], sizeof(double) * 2 * n, hipMemcpyDeviceToHost, e->st2), "get"))
+            return rc;
+        return hip_check(hipStreamSynchronize(e->st2), "hipStreamSynchronize");
+    }
+    // fluid particles: psi's x and y derivatives at the positions (the engine stream's psi, after the
+    // second stream's particle step)
+    double *d = nullptr;
+    if (int rc = dalloc(&d, 2 * (size_t)n)) return rc;
+    int rc = hip_check(hipStreamSynchronize(e->st2), "hipStreamSynchronize");
+    if (!rc) rc = op_points_dxdy(e->p.g_psi, e->lvl[0], n, e->part[set], d, e->st);
+    if (!rc) rc = hip_check(hipMemcpyAsync(out, d, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, e->st), "get");
+    if (!rc) rc = hip_check(hipStreamSynchronize(e->st), "hipStreamSynchronize");
+    (void)hipFree(d);
+    return rc;
 }
 
 int cmm_euler_diagnostics(cmm_euler *e, double *out) {

@@ -21,7 +21,7 @@ size_t op_advect_workspace(const cmm_grid &gm, const cmm_grid &gp, const cmm_adv
 int op_map_advect(const cmm_grid &gm, const cmm_grid &gp, const double *cx, const double *cy, const PsiLevels &P,
                   const cmm_advect_coeffs &c, double *ncx, double *ncy, void *work, size_t work_bytes,
                   hipStream_t st, hipEvent_t mid = nullptr, const double *m8_in = nullptr, double *m8_out = nullptr,
-                  const double *corr1 = nullptr, const double *corr12 = nullptr);
+                  const double *corr1 = nullptr, const double *corr12 = nullptr, const double *lw = nullptr);
 // tile (0: the largest of 4, 3, 2, 1 dividing the refinement ratio) and cap on the number of
 // blocks of the tiled check; set by the autotuner
 struct IncompTune {
@@ -82,6 +82,13 @@ int op_sample_vort(const cmm_grid &gm, const cmm_grid &gt, const cmm_grid &gf, c
 int op_sample_hermite(const cmm_grid &gh, const double *H, const cmm_grid &gt, double *out, hipStream_t st);
 int op_particles(const cmm_grid &gp, const PsiLevels &P, int L, const double *Lp, double dt, int scheme, int n,
                  double *pos, hipStream_t st);
+// inertial particles (positions and velocities), their start velocity (fluid = 1: psi level 0's
+// velocity, else zero), and (d psi/dx, d psi/dy) at points (NaN outside the grid's bounds)
+int op_particles_inertial(const cmm_grid &gp, const PsiLevels &P, int L, const double *Lp, double dt, int scheme,
+                          double tau, int n, double *pos, double *vel, hipStream_t st);
+int op_particles_vel_init(const cmm_grid &gp, const double *psi0, int n, int fluid, const double *pos, double *vel,
+                          hipStream_t st);
+int op_points_dxdy(const cmm_grid &gp, const double *psi0, int n, const double *pos, double *out, hipStream_t st);
 
 // Split-precision form: the map as identity + FP32 displacement records (M[8j + p]: planes of
 // X - x, then of Y - y, identity derivatives removed), FP32 Hermite records for the gathered
@@ -95,7 +102,7 @@ size_t op_advect32_workspace(const cmm_grid &gp, const cmm_advect_coeffs &c, boo
 int op_map_advect32(const cmm_grid &gm, const cmm_grid &gp, const PsiLevels &P, const cmm_advect_coeffs &c,
                     const float *M_in, float *M_out, void *work, size_t work_bytes, hipStream_t st,
                     hipEvent_t mid = nullptr, bool psi64 = false, const double *corr1 = nullptr,
-                    const double *corr12 = nullptr);
+                    const double *corr12 = nullptr, const double *lw = nullptr);
 int op_incomp32_launch(const cmm_grid &gm, const cmm_grid &gc, const float *M, unsigned long long *slot,
                        hipStream_t st, IncompTune tune = IncompTune());
 int op_incomp32(const cmm_grid &gm, const cmm_grid &gc, const float *M, double *out, hipStream_t st);

@@ -79,6 +79,9 @@ __device__ __forceinline__ void lsum(const double *w, const double *u, int nl, d
 struct VelLevels {
     DGrid g;
     PsiLevels P;
+    // when set, the Lagrange weights L1[4], L12[4] read from device memory instead of the
+    // coefficient struct (steps replayed from a captured graph, whose arguments are frozen)
+    const double *LW = nullptr;
     // herm_vel over L levels followed by lsum, unrolled over the (at most 4) levels so that no
     // array is indexed at run time
     __device__ __forceinline__ void w(const double *Lw, int L, double x, double y, double &kx, double &ky) const {
@@ -107,10 +110,12 @@ struct VelLevels {
         }
     }
     __device__ __forceinline__ void w1(const cmm_advect_coeffs &c, double x, double y, double &kx, double &ky) const {
-        w(c.L1, c.l_order, x, y, kx, ky);
+        if (LW) w(LW, c.l_order, x, y, kx, ky);
+        else w(c.L1, c.l_order, x, y, kx, ky);
     }
     __device__ __forceinline__ void w12(const cmm_advect_coeffs &c, double x, double y, double &kx, double &ky) const {
-        w(c.L12, c.l_order, x, y, kx, ky);
+        if (LW) w(LW + 4, c.l_order, x, y, kx, ky);
+        else w(c.L12, c.l_order, x, y, kx, ky);
     }
     __device__ __forceinline__ void lev(int l, double x, double y, double &ux, double &uy) const {
         double u[2];
@@ -268,13 +273,21 @@ __device__ __forceinline__ void step_forward(const V &vel, const cmm_advect_coef
 // weighted velocities.  One lane per node: plane reads coalesce across lanes, and each lane writes
 // its 32-byte records with 16-byte stores.
 // D1 / D12 (optional, plane-major Hermite fields): corrections added to the extrapolations C1 / C12
+// LW (optional, device): the weights L1[4], L12[4] in place of the coefficient struct's
 template <int L>
 __global__ void __launch_bounds__(256) k_psi_combine(long n, PsiLevels P, cmm_advect_coeffs c, double *__restrict__ C1,
                                                      double *__restrict__ C12, double *__restrict__ C0,
                                                      const double *__restrict__ D1 = nullptr,
-                                                     const double *__restrict__ D12 = nullptr) {
+                                                     const double *__restrict__ D12 = nullptr,
+                                                     const double *__restrict__ LW = nullptr) {
     const long j = (long)blockIdx.x * 256 + threadIdx.x;
     if (j >= n) return;
+    double w1[L], w12[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+        w1[l] = LW ? LW[l] : c.L1[l];
+        w12[l] = LW ? LW[4 + l] : c.L12[l];
+    }
     double v[L][4];
 #pragma unroll
     for (int l = 0; l < L; ++l)
@@ -294,8 +307,8 @@ __global__ void __launch_bounds__(256) k_psi_combine(long n, PsiLevels P, cmm_ad
         d[0] = double2{o[0], o[1]};
         d[1] = double2{o[2], o[3]};
     };
-    comb(c.L1, C1, D1);
-    if (C12) comb(c.L12, C12, D12);
+    comb(w1, C1, D1);
+    if (C12) comb(w12, C12, D12);
     if (C0) {
         double2 *d = reinterpret_cast<double2 *>(C0 + 4 * j);
         d[0] = double2{v[0][0], v[0][1]};
@@ -357,7 +370,7 @@ __global__ void __launch_bounds__(256) k_map_advect(DGrid gm, DGrid gp, const do
                                                     const double *C12, const double *C0, const double *M8,
                                                     cmm_advect_coeffs c, double *__restrict__ ncx,
                                                     double *__restrict__ ncy, double *__restrict__ M8out,
-                                                    int nblocks) {
+                                                    int nblocks, const double *__restrict__ LW) {
     constexpr int NPB = 256 / G;  // nodes per block
     const int blk = xcd_block(blockIdx.x, nblocks);
     const int node = blk * NPB + threadIdx.x / G;
@@ -374,7 +387,7 @@ __global__ void __launch_bounds__(256) k_map_advect(DGrid gm, DGrid gp, const do
             step_backward<TIN>(VelCombined<P2>{gp, C1, C12, C0}, c, xe, ye, xf, yf);
             diffeo_il<P2>(gm, M8, xf, yf, px, py);
         } else {
-            step_backward<TIN>(VelLevels{gp, P}, c, xe, ye, xf, yf);
+            step_backward<TIN>(VelLevels{gp, P, LW}, c, xe, ye, xf, yf);
             diffeo(gm, cx, cy, xf, yf, px, py);
         }
     }
@@ -1229,6 +1242,165 @@ __global__ void __launch_bounds__(256) k_particles(DGrid g, PsiLevels P, PartPar
     pos[2 * i + 1] = p1 - floor((p1 - g.y0) / g.ly) * g.ly;
 }
 
+// Inertial particles, Particle_advect_inertia (Cuda-CMM/src/numerical/cmm-particles.cu:280-551):
+// position and velocity advance through the same stages, x' = v, v' = -(v - u(x, t)) / tau, with u
+// from level 1 (t_n), level 0 (t_{n+1}), level 2 (t_n - dt, RK3Mod) or the Lagrange combination of
+// the levels to t_n + dt/2 (Lp), the same sources as the fluid particles' stages.
+__global__ void __launch_bounds__(256) k_particles_inertial(DGrid g, PsiLevels P, PartParams q, double tau,
+                                                            double *__restrict__ pos, double *__restrict__ vel) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= q.n) return;
+    const double x = pos[2 * i], y = pos[2 * i + 1];
+    const double vx = vel[2 * i], vy = vel[2 * i + 1];
+    const double dt = q.dt;
+    const int L = q.L;
+    // fluid velocity of one level, or of the combination to t_n + dt/2
+    auto ulev = [&](int l, double px, double py, double &ux, double &uy) {
+        double u[2];
+        herm_vel(g, PsiLevels{{P.p[l], P.p[l], P.p[l], P.p[l]}}, 1, px, py, u);
+        ux = u[0];
+        uy = u[1];
+    };
+    auto uhalf = [&](double px, double py, double &ux, double &uy) {
+        double u[8];
+        herm_vel(g, P, L, px, py, u);
+        lsum(q.Lp, u, L, ux, uy);
+    };
+    double xn, yn, wx, wy;
+    double ux, uy;
+    switch (q.scheme) {
+    case 10: {
+        ulev(1, x, y, ux, uy);
+        xn = x + dt * vx;
+        yn = y + dt * vy;
+        wx = vx - dt * (vx - ux) / tau;
+        wy = vy - dt * (vy - uy) / tau;
+        break;
+    }
+    case 20: {
+        ulev(1, x, y, ux, uy);
+        const double a1x = -(vx - ux) / tau, a1y = -(vy - uy) / tau;
+        const double v2x = vx + dt * a1x, v2y = vy + dt * a1y;
+        ulev(0, x + dt * vx, y + dt * vy, ux, uy);
+        const double a2x = -(v2x - ux) / tau, a2y = -(v2y - uy) / tau;
+        xn = x + dt / 2.0 * (vx + v2x);
+        yn = y + dt / 2.0 * (vy + v2y);
+        wx = vx + dt / 2.0 * (a1x + a2x);
+        wy = vy + dt / 2.0 * (a1y + a2y);
+        break;
+    }
+    case 30: {
+        ulev(1, x, y, ux, uy);
+        const double a1x = -(vx - ux) / tau, a1y = -(vy - uy) / tau;
+        const double v2x = vx + dt / 2.0 * a1x, v2y = vy + dt / 2.0 * a1y;
+        uhalf(x + dt / 2.0 * vx, y + dt / 2.0 * vy, ux, uy);
+        const double a2x = -(v2x - ux) / tau, a2y = -(v2y - uy) / tau;
+        const double v3x = vx + dt * (-a1x + 2 * a2x), v3y = vy + dt * (-a1y + 2 * a2y);
+        ulev(0, x + dt * (-vx + 2 * v2x), y + dt * (-vy + 2 * v2y), ux, uy);
+        const double a3x = -(v3x - ux) / tau, a3y = -(v3y - uy) / tau;
+        xn = x + dt / 6.0 * (vx + 4 * v2x + v3x);
+        yn = y + dt / 6.0 * (vy + 4 * v2y + v3y);
+        wx = vx + dt / 6.0 * (a1x + 4 * a2x + a3x);
+        wy = vy + dt / 6.0 * (a1y + 4 * a2y + a3y);
+        break;
+    }
+    case 40: {
+        ulev(1, x, y, ux, uy);
+        const double a1x = -(vx - ux) / tau, a1y = -(vy - uy) / tau;
+        const double v2x = vx + dt / 2.0 * a1x, v2y = vy + dt / 2.0 * a1y;
+        uhalf(x + dt / 2.0 * vx, y + dt / 2.0 * vy, ux, uy);
+        const double a2x = -(v2x - ux) / tau, a2y = -(v2y - uy) / tau;
+        const double v3x = vx + dt / 2.0 * a2x, v3y = vy + dt / 2.0 * a2y;
+        uhalf(x + dt / 2.0 * v2x, y + dt / 2.0 * v2y, ux, uy);
+        const double a3x = -(v3x - ux) / tau, a3y = -(v3y - uy) / tau;
+        const double v4x = vx + dt * a3x, v4y = vy + dt * a3y;
+        ulev(0, x + dt * v3x, y + dt * v3y, ux, uy);
+        const double a4x = -(v4x - ux) / tau, a4y = -(v4y - uy) / tau;
+        xn = x + dt / 6.0 * (vx + 2 * v2x + 2 * v3x + v4x);
+        yn = y + dt / 6.0 * (vy + 2 * v2y + 2 * v3y + v4y);
+        wx = vx + dt / 6.0 * (a1x + 2 * a2x + 2 * a3x + a4x);
+        wy = vy + dt / 6.0 * (a1y + 2 * a2y + 2 * a3y + a4y);
+        break;
+    }
+    case 31: {
+        ulev(1, x, y, ux, uy);
+        const double a1x = -(vx - ux) / tau, a1y = -(vy - uy) / tau;
+        const double v2x = vx + dt * a1x, v2y = vy + dt * a1y;
+        ulev(0, x + dt * vx, y + dt * vy, ux, uy);
+        const double a2x = -(v2x - ux) / tau, a2y = -(v2y - uy) / tau;
+        const double v3x = vx + dt * (a1x - 2 * a2x), v3y = vy + dt * (a1y - 2 * a2y);
+        ulev(2, x + dt * (vx - 2 * v2x), y + dt * (vy - 2 * v2y), ux, uy);
+        const double a3x = -(v3x - ux) / tau, a3y = -(v3y - uy) / tau;
+        xn = x + dt / 12.0 * (8 * vx + 5 * v2x - v3x);
+        yn = y + dt / 12.0 * (8 * vy + 5 * v2y - v3y);
+        wx = vx + dt / 12.0 * (8 * a1x + 5 * a2x - a3x);
+        wy = vy + dt / 12.0 * (8 * a1y + 5 * a2y - a3y);
+        break;
+    }
+    case 41: {
+        ulev(1, x, y, ux, uy);
+        const double a1x = -(vx - ux) / tau, a1y = -(vy - uy) / tau;
+        const double v2x = vx + dt * a1x, v2y = vy + dt * a1y;
+        ulev(0, x + dt * vx, y + dt * vy, ux, uy);
+        const double a2x = -(v2x - ux) / tau, a2y = -(v2y - uy) / tau;
+        const double v3x = vx + dt / 8.0 * (3 * a1x + a2x), v3y = vy + dt / 8.0 * (3 * a1y + a2y);
+        uhalf(x + dt / 8.0 * (3 * vx + v2x), y + dt / 8.0 * (3 * vy + v2y), ux, uy);
+        const double a3x = -(v3x - ux) / tau, a3y = -(v3y - uy) / tau;
+        const double v4x = vx + dt / 3.0 * (-a2x + 4 * a3x), v4y = vy + dt / 3.0 * (-a2y + 4 * a3y);
+        ulev(0, x + dt / 3.0 * (-v2x + 4 * v3x), y + dt / 3.0 * (-v2y + 4 * v3y), ux, uy);
+        const double a4x = -(v4x - ux) / tau, a4y = -(v4y - uy) / tau;
+        xn = x + dt / 12.0 * (2 * vx - v2x + 8 * v3x + 3 * v4x);
+        yn = y + dt / 12.0 * (2 * vy - v2y + 8 * v3y + 3 * v4y);
+        wx = vx + dt / 12.0 * (2 * a1x - a2x + 8 * a3x + 3 * a4x);
+        wy = vy + dt / 12.0 * (2 * a1y - a2y + 8 * a3y + 3 * a4y);
+        break;
+    }
+    default:
+        xn = yn = wx = wy = 0.0;
+    }
+    pos[2 * i] = xn - floor((xn - g.x0) / g.lx) * g.lx;
+    pos[2 * i + 1] = yn - floor((yn - g.y0) / g.ly) * g.ly;
+    vel[2 * i] = wx;
+    vel[2 * i + 1] = wy;
+}
+
+// Start velocity of inertial particles: the fluid velocity of psi level 0 (fluid = 1) or zero,
+// Particle_advect_inertia_init (Cuda-CMM/src/numerical/cmm-particles.cu:49-59) and the memset of
+// Cuda-CMM/src/simulation/cmm-euler2d.cu:630
+__global__ void __launch_bounds__(256) k_particles_vel_init(DGrid g, const double *__restrict__ psi0, int n, int fluid,
+                                                            const double *__restrict__ pos, double *__restrict__ vel) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double u[2] = {0.0, 0.0};
+    if (fluid) herm_vel(g, PsiLevels{{psi0, psi0, psi0, psi0}}, 1, pos[2 * i], pos[2 * i + 1], u);
+    vel[2 * i] = u[0];
+    vel[2 * i + 1] = u[1];
+}
+
+// (d psi/dx, d psi/dy) of psi level 0 at points strictly inside the grid's bounds, NaN outside:
+// the velocity output of fluid particles, k_h_sample_points_dxdy
+// (Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:138-155) with cmm-hermite.cu:343-371
+__global__ void __launch_bounds__(256) k_points_dxdy(DGrid g, double x1, double y1, const double *__restrict__ psi0,
+                                                     int n, const double *__restrict__ pos, double *__restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double x = pos[2 * i], y = pos[2 * i + 1];
+    if (x > g.x0 && x < x1 && y > g.y0 && y < y1) {
+        Cell c;
+        find_cell(g, x, y, c);
+        double bx[4], by[4], dbx[4], dby[4], v[16];
+        basis(c.dx, bx);
+        basis(c.dy, by);
+        basis_d(c.dx, dbx);
+        basis_d(c.dy, dby);
+        load16(psi0, g.n, c, v);
+        out[2 * i] = hsum_v(v, dbx, by, g.hx, g.hy) / g.hx;
+        out[2 * i + 1] = hsum_v(v, bx, dby, g.hx, g.hy) / g.hy;
+    } else {
+        out[2 * i] = out[2 * i + 1] = __builtin_nan("");
+    }
+}
+
 // ------------------------------------------------------------------------------------------------
 // Vlasov-Poisson psi (x, v), evaluate_potential_from_density_hermite
 // (Cuda-CMM/src/simulation/cmm-simulation-host.cu:385-471, :478-494, :534-571).
@@ -1633,7 +1805,7 @@ size_t op_advect_workspace(const cmm_grid &gm, const cmm_grid &gp, const cmm_adv
 int op_map_advect(const cmm_grid &gm, const cmm_grid &gp, const double *cx, const double *cy, const PsiLevels &P,
                   const cmm_advect_coeffs &c, double *ncx, double *ncy, void *work, size_t work_bytes,
                   hipStream_t st, hipEvent_t mid, const double *m8_in, double *m8_out, const double *corr1,
-                  const double *corr12) {
+                  const double *corr12, const double *lw) {
     DGrid dm = make_dgrid(gm), dp = make_dgrid(gp);
     const int nfoot = 4 + 4 * c.map_update_order_num;
     const int G = nfoot <= 4 ? 4 : nfoot <= 8 ? 8 : 16;
@@ -1688,9 +1860,9 @@ int op_map_advect(const cmm_grid &gm, const cmm_grid &gp, const double *cx, cons
         }
         const dim3 g((unsigned)((n + 255) / 256));
         switch (c.l_order) {
-        case 2: hipLaunchKernelGGL(k_psi_combine<2>, g, dim3(256), 0, st, n, P, c, C1, C12, C0, corr1, corr12); break;
-        case 3: hipLaunchKernelGGL(k_psi_combine<3>, g, dim3(256), 0, st, n, P, c, C1, C12, C0, corr1, corr12); break;
-        default: hipLaunchKernelGGL(k_psi_combine<4>, g, dim3(256), 0, st, n, P, c, C1, C12, C0, corr1, corr12); break;
+        case 2: hipLaunchKernelGGL(k_psi_combine<2>, g, dim3(256), 0, st, n, P, c, C1, C12, C0, corr1, corr12, lw); break;
+        case 3: hipLaunchKernelGGL(k_psi_combine<3>, g, dim3(256), 0, st, n, P, c, C1, C12, C0, corr1, corr12, lw); break;
+        default: hipLaunchKernelGGL(k_psi_combine<4>, g, dim3(256), 0, st, n, P, c, C1, C12, C0, corr1, corr12, lw); break;
         }
         if (int rc = launch_check("k_psi_combine")) return rc;
     }
@@ -1698,7 +1870,7 @@ int op_map_advect(const cmm_grid &gm, const cmm_grid &gp, const double *cx, cons
     const bool p2 = dm.lgx >= 0 && dm.lgy >= 0 && dp.lgx >= 0 && dp.lgy >= 0;
 #define CMM_ADV(T, GG, CB)                                                                                             \
     hipLaunchKernelGGL((k_map_advect<T, GG, CB>), dim3(nb), dim3(256), 0, st, dm, dp, cx, cy, P, C1, C12, C0, M8, c, ncx, \
-                       ncy, m8_out, nb)
+                       ncy, m8_out, nb, lw)
 #define CMM_ADV_G(T, CB)                                                                                               \
     do {                                                                                                               \
         if (G == 4) CMM_ADV(T, 4, CB);                                                                                 \
@@ -1709,7 +1881,7 @@ int op_map_advect(const cmm_grid &gm, const cmm_grid &gp, const double *cx, cons
     do {                                                                                                               \
         if (comb && p2 && G == 8)                                                                                      \
             hipLaunchKernelGGL((k_map_advect<T, 8, true, true>), dim3(nb), dim3(256), 0, st, dm, dp, cx, cy, P, C1,  \
-                               C12, C0, M8, c, ncx, ncy, m8_out, nb);                                                 \
+                               C12, C0, M8, c, ncx, ncy, m8_out, nb, lw);                                             \
         else if (comb) CMM_ADV_G(T, true);                                                                             \
         else CMM_ADV_G(T, false);                                                                                      \
     } while (0)
@@ -2107,6 +2279,34 @@ int op_particles(const cmm_grid &gp, const PsiLevels &P, int L, const double *Lp
     return launch_check("k_particles");
 }
 
+int op_particles_inertial(const cmm_grid &gp, const PsiLevels &P, int L, const double *Lp, double dt, int scheme,
+                          double tau, int n, double *pos, double *vel, hipStream_t st) {
+    if (n <= 0) return CMM_OK;
+    PartParams q;
+    for (int i = 0; i < 4; ++i) q.Lp[i] = (Lp && i < L) ? Lp[i] : 0.0;
+    q.dt = dt;
+    q.L = L;
+    q.scheme = scheme;
+    q.n = n;
+    hipLaunchKernelGGL(k_particles_inertial, dim3((n + 255) / 256), dim3(256), 0, st, make_dgrid(gp), P, q, tau, pos, vel);
+    return launch_check("k_particles_inertial");
+}
+
+int op_particles_vel_init(const cmm_grid &gp, const double *psi0, int n, int fluid, const double *pos, double *vel,
+                          hipStream_t st) {
+    if (n <= 0) return CMM_OK;
+    hipLaunchKernelGGL(k_particles_vel_init, dim3((n + 255) / 256), dim3(256), 0, st, make_dgrid(gp), psi0, n, fluid,
+                       pos, vel);
+    return launch_check("k_particles_vel_init");
+}
+
+int op_points_dxdy(const cmm_grid &gp, const double *psi0, int n, const double *pos, double *out, hipStream_t st) {
+    if (n <= 0) return CMM_OK;
+    hipLaunchKernelGGL(k_points_dxdy, dim3((n + 255) / 256), dim3(256), 0, st, make_dgrid(gp), gp.x1, gp.y1, psi0, n,
+                       pos, out);
+    return launch_check("k_points_dxdy");
+}
+
 // ================================================================================================
 // Split-precision form.
 //
@@ -2219,9 +2419,16 @@ struct VelCombined32 {
 template <int L>
 __global__ void __launch_bounds__(256) k_psi_combine32(long n, PsiLevels P, cmm_advect_coeffs c, float4 *__restrict__ C1,
                                                        float4 *__restrict__ C12, float4 *__restrict__ C0,
-                                                       const double *__restrict__ D1, const double *__restrict__ D12) {
+                                                       const double *__restrict__ D1, const double *__restrict__ D12,
+                                                       const double *__restrict__ LW) {
     const long j = (long)blockIdx.x * 256 + threadIdx.x;
     if (j >= n) return;
+    double w1[L], w12[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+        w1[l] = LW ? LW[l] : c.L1[l];
+        w12[l] = LW ? LW[4 + l] : c.L12[l];
+    }
     double v[L][4];
 #pragma unroll
     for (int l = 0; l < L; ++l)
@@ -2239,8 +2446,8 @@ __global__ void __launch_bounds__(256) k_psi_combine32(long n, PsiLevels P, cmm_
         }
         dst[j] = float4{(float)o[0], (float)o[1], (float)o[2], (float)o[3]};
     };
-    comb(c.L1, C1, D1);
-    if (C12) comb(c.L12, C12, D12);
+    comb(w1, C1, D1);
+    if (C12) comb(w12, C12, D12);
     if (C0) C0[j] = float4{(float)v[0][0], (float)v[0][1], (float)v[0][2], (float)v[0][3]};
 }
 
@@ -2550,7 +2757,7 @@ size_t op_advect32_workspace(const cmm_grid &gp, const cmm_advect_coeffs &c, boo
 
 int op_map_advect32(const cmm_grid &gm, const cmm_grid &gp, const PsiLevels &P, const cmm_advect_coeffs &c,
                     const float *M_in, float *M_out, void *work, size_t work_bytes, hipStream_t st, hipEvent_t mid,
-                    bool psi64, const double *corr1, const double *corr12) {
+                    bool psi64, const double *corr1, const double *corr12, const double *lw) {
     if (!op_advect32_supported(c)) {
         set_error("split-precision advection: the time scheme reads psi levels on their own (AB2, RK3Mod), "
                   "needs two or more psi levels, and is backward only");
@@ -2571,16 +2778,16 @@ int op_map_advect32(const cmm_grid &gm, const cmm_grid &gp, const PsiLevels &P, 
     if (psi64) {
         double *d1 = (double *)C1, *d12 = (double *)C12, *d0 = (double *)C0;
         switch (c.l_order) {
-        case 2: hipLaunchKernelGGL(k_psi_combine<2>, g, dim3(256), 0, st, n, P, c, d1, d12, d0, corr1, corr12); break;
-        case 3: hipLaunchKernelGGL(k_psi_combine<3>, g, dim3(256), 0, st, n, P, c, d1, d12, d0, corr1, corr12); break;
-        default: hipLaunchKernelGGL(k_psi_combine<4>, g, dim3(256), 0, st, n, P, c, d1, d12, d0, corr1, corr12); break;
+        case 2: hipLaunchKernelGGL(k_psi_combine<2>, g, dim3(256), 0, st, n, P, c, d1, d12, d0, corr1, corr12, lw); break;
+        case 3: hipLaunchKernelGGL(k_psi_combine<3>, g, dim3(256), 0, st, n, P, c, d1, d12, d0, corr1, corr12, lw); break;
+        default: hipLaunchKernelGGL(k_psi_combine<4>, g, dim3(256), 0, st, n, P, c, d1, d12, d0, corr1, corr12, lw); break;
         }
     } else {
         float4 *f1 = (float4 *)C1, *f12 = (float4 *)C12, *f0 = (float4 *)C0;
         switch (c.l_order) {
-        case 2: hipLaunchKernelGGL(k_psi_combine32<2>, g, dim3(256), 0, st, n, P, c, f1, f12, f0, corr1, corr12); break;
-        case 3: hipLaunchKernelGGL(k_psi_combine32<3>, g, dim3(256), 0, st, n, P, c, f1, f12, f0, corr1, corr12); break;
-        default: hipLaunchKernelGGL(k_psi_combine32<4>, g, dim3(256), 0, st, n, P, c, f1, f12, f0, corr1, corr12); break;
+        case 2: hipLaunchKernelGGL(k_psi_combine32<2>, g, dim3(256), 0, st, n, P, c, f1, f12, f0, corr1, corr12, lw); break;
+        case 3: hipLaunchKernelGGL(k_psi_combine32<3>, g, dim3(256), 0, st, n, P, c, f1, f12, f0, corr1, corr12, lw); break;
+        default: hipLaunchKernelGGL(k_psi_combine32<4>, g, dim3(256), 0, st, n, P, c, f1, f12, f0, corr1, corr12, lw); break;
         }
     }
     if (int rc = launch_check("k_psi_combine")) return rc;

@@ -347,6 +347,14 @@ struct Run {
                 if (s.verbose < 4) write_file(s, psub + "/Particles_advected_pos_P" + tag, pos.data(), pos.size(), false);
                 else debug_globals(s, "/Particles_advected_pos_P" + tag, pos.data(), pos.size(), 0);
             }
+            // velocities: an inertial set's own, else psi's x and y derivatives at the positions
+            // (Cuda-CMM/src/ui/cmm-io.cu:418-430)
+            if ((has(("PartA_Vel_" + tag).c_str()) || all) && part_set[i] >= 0) {
+                std::vector<double> vel(2 * (size_t)s.particles_advected[i].num);
+                check(cmm_euler_get_particle_velocities(e, part_set[i], vel.data()), "cmm_euler_get_particle_velocities");
+                if (s.verbose < 4) write_file(s, psub + "/Particles_advected_vel_U" + tag, vel.data(), vel.size(), false);
+                else debug_globals(s, "/Particles_advected_vel_U" + tag, vel.data(), vel.size(), 0);
+            }
         }
         return msg;
     }
@@ -755,17 +763,14 @@ int run_euler(Settings &s) {
                 log.push("Particles set F" + zpad((int)i + 1, 2) + " : Num = " + to_s(s.particles_forwarded[i].num));
         }
 
-    // particles, Cuda-CMM/src/simulation/cmm-euler2d.cu:423-450 (fluid particles)
+    // particles, Cuda-CMM/src/simulation/cmm-euler2d.cu:423-450: fluid (tau 0) and inertial sets, each
+    // moving from its init_time on
     for (size_t i = 0; i < s.particles_advected.size(); ++i) {
         const auto &pa = s.particles_advected[i];
         r.part_set.push_back(-1);
-        if (pa.tau != 0) {
-            if (s.verbose >= 1) log.push("Particles set P" + zpad((int)i + 1, 2) + " : inertial particles (tau != 0) are not advected by this engine");
-            continue;
-        }
-        std::vector<double> pos;
+        std::vector<double> pos, vel;
         if (s.restart_time != 0) {
-            // readParticlesState, Cuda-CMM/src/ui/cmm-io.cu:659-688
+            // readParticlesState, Cuda-CMM/src/ui/cmm-io.cu:659-688: positions, and velocities if tau != 0
             pos.resize(2 * (size_t)pa.num);
             const std::string f = restart_dir(s) + "/Particles_advected_pos_P" + zpad((int)i + 1, 2) + ".data";
             if (!read_doubles(f, pos.data(), pos.size())) {
@@ -773,11 +778,22 @@ int run_euler(Settings &s) {
                 log.push("Exitting ..");
                 std::exit(0);
             }
+            if (pa.tau != 0) {
+                vel.resize(2 * (size_t)pa.num);
+                const std::string fv = restart_dir(s) + "/Particles_advected_vel_U" + zpad((int)i + 1, 2) + ".data";
+                if (!read_doubles(fv, vel.data(), vel.size())) {
+                    log.push("Failed to read in particles vel " + to_s((int)i) + " at location " + fv);
+                    log.push("Exitting ..");
+                    std::exit(0);
+                }
+            }
         } else {
             pos = init_particles(pa, b);
         }
         int set = -1;
-        r.check(cmm_euler_add_particles(r.e, pa.num, pa.time_integration_num, pos.data(), &set), "cmm_euler_add_particles");
+        r.check(cmm_euler_add_particle_set(r.e, pa.num, pa.time_integration_num, pa.init_time, pa.tau, pa.init_vel ? 1 : 0,
+                                           pos.data(), vel.empty() ? nullptr : vel.data(), &set),
+                "cmm_euler_add_particle_set");
         r.part_set.back() = set;
         if (s.verbose >= 1) {
             std::ostringstream os;
@@ -959,6 +975,12 @@ int run_euler(Settings &s) {
             std::vector<double> pos(2 * (size_t)s.particles_advected[i].num);
             r.check(cmm_euler_get_particles(r.e, r.part_set[i], pos.data()), "cmm_euler_get_particles");
             write_file(s, "/MapStack/Particles_advected_pos_P" + zpad((int)i + 1, 2), pos.data(), pos.size(), false);
+            // writeParticlesState, Cuda-CMM/src/ui/cmm-io.cu:638-657: velocities of inertial sets
+            if (s.particles_advected[i].tau != 0) {
+                std::vector<double> vel(pos.size());
+                r.check(cmm_euler_get_particle_velocities(r.e, r.part_set[i], vel.data()), "cmm_euler_get_particle_velocities");
+                write_file(s, "/MapStack/Particles_advected_vel_U" + zpad((int)i + 1, 2), vel.data(), vel.size(), false);
+            }
         }
     }
     cmm_euler_destroy(r.e);

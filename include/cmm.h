@@ -250,8 +250,22 @@ int cmm_euler_destroy(cmm_euler *e);
 /* Stream the engine runs on (hipStream_t as void*).  Particle sets advance on a second engine
  * stream, ordered against this one with events; cmm_euler_get_particles waits for them. */
 void *cmm_euler_stream(cmm_euler *e);
-/* Adds a fluid-particle set (positions copied from host); returns its index in *set. */
+/* Adds a fluid-particle set (positions copied from host) that moves from the first step; returns
+ * its index in *set.  Same as cmm_euler_add_particle_set with init_time 0 and tau 0. */
 int cmm_euler_add_particles(cmm_euler *e, int n, int scheme, const double *pos_host, int *set);
+/* Adds a particle set with the reference's per-set settings (ParticlesAdvected: tau, init_time,
+ * init_vel, time_integration; Cuda-CMM/src/ui/settings.h, parsed at Cuda-CMM/src/ui/settings.cu):
+ * - the set moves in every step with t_{n+1} + 1e-5 dt >= init_time
+ *   (particles_advect, Cuda-CMM/src/numerical/cmm-particles.cu:63-87);
+ * - tau == 0: fluid particles (Particle_advect, :119-276);
+ * - tau != 0: inertial particles x' = v, v' = -(v - u)/tau (Particle_advect_inertia, :280-551).
+ *   Their velocity starts as the fluid velocity of psi after the warm-up (init_vel != 0) or zero,
+ *   and is set again by that rule in the step that reaches init_time
+ *   (Cuda-CMM/src/simulation/cmm-euler2d.cu:620-632, :883-895); vel_host (2n doubles, may be NULL)
+ *   gives the velocities instead, as a restart reads them (readParticlesState,
+ *   Cuda-CMM/src/ui/cmm-io.cu:659-688). */
+int cmm_euler_add_particle_set(cmm_euler *e, int n, int scheme, double init_time, double tau, int init_vel,
+                               const double *pos_host, const double *vel_host, int *set);
 /* Initial state and Lagrange warm-up; warmup_incomp[8] receives the logged errors. */
 int cmm_euler_init(cmm_euler *e, double *warmup_incomp, int *n_warmup);
 /* Restart from a saved state instead of cmm_euler_init: readMapStack and the restart branch of the
@@ -282,6 +296,11 @@ int cmm_euler_step(cmm_euler *e, double dt_now, cmm_step_info *info);
 /* Copies a field to host memory (size per CMM_FIELD_*). */
 int cmm_euler_get_field(cmm_euler *e, int field, double *host_out);
 int cmm_euler_get_particles(cmm_euler *e, int set, double *host_out);
+/* The velocity output of a particle set (writeParticles' PartA_Vel_, Cuda-CMM/src/ui/cmm-io.cu:418-430):
+ * an inertial set's velocities; for fluid particles (d psi/dx, d psi/dy) of the newest psi at the
+ * positions, NaN outside the psi grid's bounds (k_h_sample_points_dxdy,
+ * Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:138-155).  2n doubles. */
+int cmm_euler_get_particle_velocities(cmm_euler *e, int set, double *host_out);
 /* Conservation diagnostics of compute_conservation_targets
  * (Cuda-CMM/src/simulation/cmm-simulation-host.cu:615-698): out[0] energy 0.5 h^2 sum |grad psi|^2,
  * out[1] enstrophy and out[2] palinstrophy of the coarse vorticity, out[3] max |omega|. */

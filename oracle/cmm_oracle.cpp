@@ -961,6 +961,86 @@ void particle_step(const Grid &g, const double *psi, int L, const double *Lp, do
     p[1] = p[1] - std::floor((p[1] - g.y0) / LY) * LY;
 }
 
+// Inertial particles, Particle_advect_inertia (Cuda-CMM/src/numerical/cmm-particles.cu:280-551), written
+// here as explicit Runge-Kutta tableaus over the state (x, y, vx, vy) with
+//   F(state) = (vx, vy, -(vx - ux)/tau, -(vy - uy)/tau),
+// where each stage reads the fluid velocity u from its own source: psi level 1 (t_n), level 0
+// (t_{n+1}), level 2 (t_n - dt) or the Lagrange combination Lp of the levels to t_n + dt/2.
+// Tableaus (a = stage offsets, b = weights, src per stage):
+//   10 EulerExp  src (1)          b (1)
+//   20 Heun      src (1, 0)       a21 = 1                         b (1, 1)/2
+//   30 RK3       src (1, h, 0)    a21 = 1/2; a31 = -1, a32 = 2    b (1, 4, 1)/6
+//   40 RK4       src (1, h, h, 0) a21 = a32 = 1/2, a43 = 1        b (1, 2, 2, 1)/6
+//   31 RK3Mod    src (1, 0, 2)    a21 = 1; a31 = 1, a32 = -2      b (8, 5, -1)/12
+//   41 RK4Mod    src (1, 0, h, 0) a21 = 1; a31 = 3/8, a32 = 1/8; a42 = -1/3, a43 = 4/3
+//                                                                 b (2, -1, 8, 3)/12
+void particle_step_inertial(const Grid &g, const double *psi, int L, const double *Lp, double dt, int scheme, double tau,
+                            double p[2], double v[2]) {
+    const long N4 = 4 * g.n;
+    enum { HALF = -1 };
+    int ns = 0;
+    int src[4] = {1, 1, 1, 1};
+    double a[4][4] = {{0}}, b[4] = {0};
+    switch (scheme) {
+    case 10: ns = 1; b[0] = 1; break;
+    case 20: ns = 2; src[1] = 0; a[1][0] = 1; b[0] = b[1] = 0.5; break;
+    case 30:
+        ns = 3; src[1] = HALF; src[2] = 0;
+        a[1][0] = 0.5; a[2][0] = -1; a[2][1] = 2;
+        b[0] = 1.0 / 6; b[1] = 4.0 / 6; b[2] = 1.0 / 6;
+        break;
+    case 40:
+        ns = 4; src[1] = src[2] = HALF; src[3] = 0;
+        a[1][0] = 0.5; a[2][1] = 0.5; a[3][2] = 1;
+        b[0] = b[3] = 1.0 / 6; b[1] = b[2] = 2.0 / 6;
+        break;
+    case 31:
+        ns = 3; src[1] = 0; src[2] = 2;
+        a[1][0] = 1; a[2][0] = 1; a[2][1] = -2;
+        b[0] = 8.0 / 12; b[1] = 5.0 / 12; b[2] = -1.0 / 12;
+        break;
+    case 41:
+        ns = 4; src[1] = 0; src[2] = HALF; src[3] = 0;
+        a[1][0] = 1; a[2][0] = 3.0 / 8; a[2][1] = 1.0 / 8; a[3][1] = -1.0 / 3; a[3][2] = 4.0 / 3;
+        b[0] = 2.0 / 12; b[1] = -1.0 / 12; b[2] = 8.0 / 12; b[3] = 3.0 / 12;
+        break;
+    default:
+        p[0] = p[1] = v[0] = v[1] = 0;
+        break;
+    }
+    if (ns > 0) {
+        double K[4][4];  // stage derivatives of (x, y, vx, vy)
+        for (int st = 0; st < ns; ++st) {
+            double z[4] = {p[0], p[1], v[0], v[1]};
+            for (int j = 0; j < st; ++j)
+                for (int q = 0; q < 4; ++q) z[q] += dt * a[st][j] * K[j][q];
+            double u[8], ux, uy;
+            if (src[st] == HALF) {
+                herm_grad(g, psi, L, z[0], z[1], u);
+                lsum(Lp, u, L, ux, uy);
+            } else {
+                herm_grad(g, psi + src[st] * N4, 1, z[0], z[1], u);
+                ux = u[0];
+                uy = u[1];
+            }
+            K[st][0] = z[2];
+            K[st][1] = z[3];
+            K[st][2] = -(z[2] - ux) / tau;
+            K[st][3] = -(z[3] - uy) / tau;
+        }
+        double dz[4] = {0, 0, 0, 0};
+        for (int st = 0; st < ns; ++st)
+            for (int q = 0; q < 4; ++q) dz[q] += b[st] * K[st][q];
+        p[0] += dt * dz[0];
+        p[1] += dt * dz[1];
+        v[0] += dt * dz[2];
+        v[1] += dt * dz[3];
+    }
+    const double LX = g.x1 - g.x0, LY = g.y1 - g.y0;
+    p[0] = p[0] - std::floor((p[0] - g.x0) / LX) * LX;
+    p[1] = p[1] - std::floor((p[1] - g.y0) / LY) * LY;
+}
+
 // SplitMix64 finaliser (Steele, Lea, Flood 2014)
 inline uint64_t mix64(uint64_t z) {
     z += 0x9E3779B97F4A7C15ULL;
@@ -1218,6 +1298,27 @@ int cmmo_particles_advect(cmm_grid gg, const double *psi, int L, const double *L
     return CMM_OK;
 }
 
+int cmmo_particles_inertial(cmm_grid gg, const double *psi, int L, const double *Lp, double dt, int scheme, double tau,
+                            int n, double *pos, double *vel) {
+    Grid g = mk(gg);
+#pragma omp parallel for
+    for (int i = 0; i < n; ++i) particle_step_inertial(g, psi, L, Lp, dt, scheme, tau, pos + 2 * i, vel + 2 * i);
+    return CMM_OK;
+}
+
+// start velocity of inertial particles: the velocity of psi (level 0) or zero,
+// Cuda-CMM/src/numerical/cmm-particles.cu:49-59 and Cuda-CMM/src/simulation/cmm-euler2d.cu:620-632
+int cmmo_particles_vel_init(cmm_grid gg, const double *psi0, int n, int fluid, const double *pos, double *vel) {
+    Grid g = mk(gg);
+    for (int i = 0; i < n; ++i) {
+        double u[2] = {0, 0};
+        if (fluid) herm_grad(g, psi0, 1, pos[2 * i], pos[2 * i + 1], u);
+        vel[2 * i] = u[0];
+        vel[2 * i + 1] = u[1];
+    }
+    return CMM_OK;
+}
+
 /*
  * Random-spectrum field on an n*n grid over [0,2pi)^2 (no reference counterpart):
  *   omega_hat(kx,ky) = A sqrt(k E(k)/pi) exp(i theta),  E(k) = k^4 exp(-2 (k/6)^2),
@@ -1327,6 +1428,12 @@ int cmmo_euler_run(const cmmo_euler_params *p, cmmo_euler_result *r) {
         }
     }
 
+    // start velocities of inertial particles, Cuda-CMM/src/simulation/cmm-euler2d.cu:620-632
+    const bool inertial = p->n_particles > 0 && p->particles && p->particle_tau != 0 && p->particle_vel;
+    if (inertial && !p->particle_vel_given)
+        cmmo_particles_vel_init(p->g_psi, psi.data(), p->n_particles, p->particle_init_vel, p->particles,
+                                p->particle_vel);
+
     // main loop, Cuda-CMM/src/simulation/cmm-euler2d.cu:709-960
     const int iter_max = (int)(1.05 * std::ceil(p->final_time / p->dt));
     std::vector<double> tv(iter_max + L + 1), dtv(iter_max + L + 1);
@@ -1380,11 +1487,23 @@ int cmmo_euler_run(const cmmo_euler_params *p, cmmo_euler_result *r) {
         for (int il = 1; il < L; ++il)
             std::memcpy(&psi[(size_t)4 * Np * (L - il)], &psi[(size_t)4 * Np * (L - il - 1)], sizeof(double) * 4 * Np);
         eval_psi();
+        // particles from their start time on (Cuda-CMM/src/numerical/cmm-particles.cu:63-87); inertial
+        // ones get their start velocity again in the step reaching it (cmm-euler2d.cu:883-895)
         if (p->n_particles > 0 && p->particles) {
             double Lp[4] = {0, 0, 0, 0};
             for (int i = 0; i < L; ++i) Lp[i] = cmmo_lagrange_coeff(tv.data(), tv[lc] + dtv[lc + 1] / 2.0, lc + 1, i, L);
-            cmmo_particles_advect(p->g_psi, psi.data(), L, Lp, dtv[lc + 1], p->particle_scheme, p->n_particles,
-                                  p->particles);
+            const double t0 = p->particle_init_time;
+            if (tv[lc + 1] + 1e-5 * dtv[lc + 1] >= t0) {
+                if (inertial)
+                    cmmo_particles_inertial(p->g_psi, psi.data(), L, Lp, dtv[lc + 1], p->particle_scheme, p->particle_tau,
+                                            p->n_particles, p->particles, p->particle_vel);
+                else
+                    cmmo_particles_advect(p->g_psi, psi.data(), L, Lp, dtv[lc + 1], p->particle_scheme,
+                                          p->n_particles, p->particles);
+            }
+            if (inertial && tv[lc] - t0 + p->dt * 1e-5 < 0 && tv[lc + 1] - t0 + p->dt * 1e-5 > 0)
+                cmmo_particles_vel_init(p->g_psi, psi.data(), p->n_particles, p->particle_init_vel, p->particles,
+                                        p->particle_vel);
         }
         if (r->time) r->time[loop] = tv[lc + 1];
         if (r->incomp) r->incomp[loop] = einc;

@@ -86,6 +86,11 @@ int cmmo_vp_psi_from_f(cmm_grid gm, cmm_grid gvort, cmm_grid gfine, cmm_grid gps
 /* Fluid particles, Cuda-CMM/src/numerical/cmm-particles.cu:119-276 */
 int cmmo_particles_advect(cmm_grid gpsi, const double *psi, int l_order, const double *Lp12,
                           double dt, int scheme, int n, double *pos);
+/* Inertial particles, Cuda-CMM/src/numerical/cmm-particles.cu:280-551 (vel in/out, 2n), and their
+ * start velocity (fluid != 0: velocity of psi0, else zero), :49-59 */
+int cmmo_particles_inertial(cmm_grid gpsi, const double *psi, int l_order, const double *Lp, double dt, int scheme,
+                            double tau, int n, double *pos, double *vel);
+int cmmo_particles_vel_init(cmm_grid gpsi, const double *psi0, int n, int fluid, const double *pos, double *vel);
 /* Random-spectrum initial field on an n*n grid (see cmm_oracle.cpp) */
 int cmmo_ic_random_spectrum(int n, uint64_t seed, double *out);
 
@@ -108,6 +113,11 @@ typedef struct cmmo_euler_params {
     int max_steps;             /* stop after this many steps (<=0: run to final_time) */
     int sim;                   /* 0: 2D Euler (cmm-euler2d.cu), 1: 1D-1V Vlasov-Poisson (cmm-vlasov1d.cu) */
     int forward_map;           /* also advect the forward map (cmm-euler2d.cu:780-785) */
+    double particle_init_time; /* the particle set moves once t_{n+1} + 1e-5 dt reaches it */
+    double particle_tau;       /* != 0: inertial particles with velocities particle_vel */
+    int particle_init_vel;     /* inertial start velocity: psi's velocity (1) or zero (0) */
+    int particle_vel_given;    /* particle_vel holds start velocities (no start rule after the warm-up) */
+    double *particle_vel;      /* in/out, 2*n_particles (inertial only) */
 } cmmo_euler_params;
 
 typedef struct cmmo_euler_result {

@@ -34,7 +34,9 @@ class EulerParams(C.Structure):
                 ("discrete", C.c_int), ("g_disc", Grid), ("disc_herm", C.POINTER(C.c_double)),
                 ("stack_capacity", C.c_int), ("n_particles", C.c_int), ("particle_scheme", C.c_int),
                 ("particles", C.POINTER(C.c_double)), ("max_steps", C.c_int), ("sim", C.c_int),
-                ("forward_map", C.c_int)]
+                ("forward_map", C.c_int), ("particle_init_time", C.c_double), ("particle_tau", C.c_double),
+                ("particle_init_vel", C.c_int), ("particle_vel_given", C.c_int),
+                ("particle_vel", C.POINTER(C.c_double))]
 
 
 class EulerResult(C.Structure):
@@ -200,6 +202,16 @@ def particles_advect(gp, psi, l_order, Lp12, dt, scheme, pos):
     return pos
 
 
+def particles_inertial(gp, psi, l_order, Lp12, dt, scheme, tau, pos, vel):
+    pos = np.ascontiguousarray(pos, dtype=np.float64).copy()
+    vel = np.ascontiguousarray(vel, dtype=np.float64).copy()
+    Lp = np.zeros(4)
+    Lp[:len(Lp12)] = Lp12
+    lib().cmmo_particles_inertial(gp, ptr(psi), l_order, ptr(Lp), C.c_double(dt), scheme, C.c_double(tau),
+                                  len(pos) // 2, ptr(pos), ptr(vel))
+    return pos, vel
+
+
 def ic_random_spectrum(n, seed):
     out = np.zeros(n * n)
     rc = lib().cmmo_ic_random_spectrum(n, seed, ptr(out))
@@ -230,7 +242,8 @@ def vp_grid(n, nv=None):
 def euler_run(g_coarse, g_fine, g_psi, g_vort, dt, final_time, tin=30, muo=1, l_order=3, lihi=1,
               eps=1e-3, threshold=1e-4, fcut=128.0, molly=0, skip_remapping=0, ic=0, ic_params=None,
               disc=None, g_disc=None, stack_capacity=1000, particles=None, particle_scheme=30,
-              max_steps=0, want=("incomp", "time", "map_counter"), sim=0, forward_map=0):
+              max_steps=0, want=("incomp", "time", "map_counter"), sim=0, forward_map=0, particle_init_time=0.0,
+              particle_tau=0.0, particle_init_vel=0, particle_vel=None):
     p = EulerParams()
     p.sim = sim
     p.forward_map = forward_map
@@ -257,6 +270,15 @@ def euler_run(g_coarse, g_fine, g_psi, g_vort, dt, final_time, tin=30, muo=1, l_
         p.n_particles = len(pos) // 2
         p.particle_scheme = particle_scheme
         p.particles = ptr(pos)
+        p.particle_init_time = particle_init_time
+        p.particle_tau = particle_tau
+        p.particle_init_vel = int(particle_init_vel)
+        if particle_tau != 0:
+            vel = np.zeros(len(pos)) if particle_vel is None else np.ascontiguousarray(particle_vel,
+                                                                                     dtype=np.float64).copy()
+            p.particle_vel_given = int(particle_vel is not None)
+            keep.append(vel)
+            p.particle_vel = ptr(vel)
     p.max_steps = max_steps
     iter_max = int(1.05 * np.ceil(final_time / dt)) + 1
     res = EulerResult()
@@ -284,6 +306,8 @@ def euler_run(g_coarse, g_fine, g_psi, g_vort, dt, final_time, tin=30, muo=1, l_
     out["warmup_incomp"] = [res.warmup_incomp[i] for i in range(res.n_warmup)]
     if particles is not None:
         out["particles"] = pos
+        if particle_tau != 0:
+            out["particle_vel"] = vel
     return out
 
 

@@ -107,3 +107,86 @@ def test_ensemble_conditions_against_oracle(dev, ic, code):
     assert np.array_equal(counter, r["map_counter"])
     assert np.abs(einc - r["incomp"]).max() <= 1e-10
     assert err <= 1e-10
+
+
+def run_pair(cmm, specs, dt, dts, precision="fp64", grid_of=None):
+    """the same engines stepped alone (cmm_euler_step) and together in a graph batch, with the step
+    sequence dts; returns both sets of step infos and engines"""
+    mk = grid_of or (lambda n: (n, 4 * n, 2 * n, 2 * n))
+
+    def make_e(n, ic, amp):
+        g = cmm.ensemble_grid
+        nc, nf, np_, nv = mk(n)
+        e = cmm.Euler(g(nc), g(nf), g(np_), g(nv), dt, initial_condition=ic, ic_params=[amp],
+                      simulation_type="cmm_vlasov_poisson_1d", incomp_threshold=1e-4, stack_capacity=32,
+                      freq_cut_psi=1e9, precision=precision)
+        e.init()
+        return e
+
+    alone = [make_e(*sp) for sp in specs]
+    ref = [[e.step(d) for d in dts] for e in alone]
+    batched = [make_e(*sp) for sp in specs]
+    b = cmm.Batch(batched, use_graphs=True)
+    got = [b.step(d) for d in dts]
+    return alone, ref, batched, got, b
+
+
+def assert_same(alone, ref, batched, got, steps):
+    for k, (ea, eb) in enumerate(zip(alone, batched)):
+        einc_a = np.array([r.incomp for r in ref[k]])
+        einc_b = np.array([got[s][k].incomp for s in range(steps)])
+        assert np.array_equal(einc_a, einc_b), (k, np.abs(einc_a - einc_b).max())
+        assert [r.stack_size for r in ref[k]] == [got[s][k].stack_size for s in range(steps)]
+        for f in ("chix", "chiy", "psi"):
+            assert np.array_equal(ea.field(f), eb.field(f)), (k, f)
+
+
+def test_batch_graph_after_a_shorter_step_and_at_a_non_dyadic_step(dev):
+    """Graph replays take their Lagrange weights from the time history of the step they replay:
+    after a shortened step (eager) the nominal steps replay graphs captured under an even history,
+    and at dt = 1/20 the weights change in their last bits from step to step.  Every instance must
+    still equal its single-engine run bit for bit."""
+    import cmm
+    specs = SPECS[:2]
+    for dt, dts in ((DT, [DT] * 14 + [DT / 2] + [DT] * 14), (1.0 / 20, [1.0 / 20] * 40)):
+        alone, ref, batched, got, b = run_pair(cmm, specs, dt, dts)
+        st = b.stats()
+        print("dt", dt, "batch stats", st)
+        assert st["graph_launches"] >= len(dts) - 3
+        assert_same(alone, ref, batched, got, len(dts))
+        b.close()
+        for e in alone + batched:
+            e.close()
+
+
+def test_batch_mixed_grid_sizes_of_the_ensemble_workload(dev):
+    """One instance each of the ensemble workload's grid sizes (maps 256^2, 512^2, 1024^2; fine 4x,
+    psi and vort 2x; x in [0, 4 pi), v in [-8, 8); dt 1/20; Landau and two-stream with seeded
+    amplitudes), 20 steps: the graph batch equals the instances run alone bit for bit, and the
+    256^2 instance follows the oracle's run (remap steps, E_inc within 1e-10, psi within 1e-10)."""
+    import cmm
+    import oracle_lib as O
+    from gpu_util import rel_linf
+    amps = np.random.default_rng(0).uniform(0.005, 0.05, 3)
+    specs = [(256, "landau_amp", amps[0]), (512, "two_stream_v2", amps[1]), (1024, "landau_amp", amps[2])]
+    dt = 1.0 / 20
+    steps = 20
+    alone, ref, batched, got, b = run_pair(cmm, specs, dt, [dt] * steps)
+    print("batch stats", b.stats(), "remaps", [sum(i.remapped for i in r) for r in ref])
+    assert_same(alone, ref, batched, got, steps)
+    psi = alone[0].field("psi")
+    bb = (0.0, 2 * O.TWO_PI, -8.0, 8.0)
+    og = lambda n: O.grid(n, n, bb)
+    r = O.euler_run(og(256), og(1024), og(512), og(512), dt=dt, final_time=steps * dt, ic=203, ic_params=[amps[0]],
+                    sim=1, threshold=1e-4, stack_capacity=32, fcut=1e9, want=("incomp", "map_counter", "psi"))
+    einc = np.array([i.incomp for i in ref[0]])
+    counter = np.array([i.stack_size for i in ref[0]], dtype=float)
+    print("256^2 instance: remaps", counter.max(), "max |dE_inc|", np.abs(einc - r["incomp"]).max(),
+          "psi", rel_linf(psi, r["psi"]))
+    assert r["steps"] == steps
+    assert np.array_equal(counter, r["map_counter"])
+    assert np.abs(einc - r["incomp"]).max() <= 1e-10
+    assert rel_linf(psi, r["psi"]) <= 1e-10
+    b.close()
+    for e in alone + batched:
+        e.close()

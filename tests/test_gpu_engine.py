@@ -300,3 +300,60 @@ def test_forward_map_in_the_loop(dev):
     assert np.array_equal(e.field("chix_f"), O.identity_map(ogc)[0])
     assert not np.array_equal(e.forward_stack_map(n_st)[0], O.identity_map(ogc)[0])
     e.close()
+
+
+@pytest.mark.parametrize("tin,L,schemes", [("RK3", 3, (30, 30)), ("RK4", 4, (41, 40))])
+def test_particle_sets_start_time_and_inertia(dev, tin, L, schemes):
+    """Particle sets with the reference's per-set settings in the device loop (4_nodes, map 64^2,
+    fine = psi = vort 128^2, dt 1/16, 16 steps, remapping at 1e-4) against the oracle's run of each
+    set: a fluid set starting at t = 0.5, an inertial set (tau 0.1, fluid start velocity) from t = 0
+    and an inertial set (tau 0.05, zero start velocity) starting at t = 0.25; positions and
+    velocities within 1e-12 relative, and the fluid set's velocity output (d psi/dx, d psi/dy at the
+    positions) against the oracle's psi."""
+    import cmm
+    rng = np.random.default_rng(11)
+    dt = 1.0 / 16
+    sets = [dict(scheme=schemes[0], init_time=0.5, tau=0.0, init_vel=False),
+            dict(scheme=schemes[1], init_time=0.0, tau=0.1, init_vel=True),
+            dict(scheme=schemes[1], init_time=0.25, tau=0.05, init_vel=False)]
+    pos = [rng.uniform(0, TWO_PI, 400) for _ in sets]
+    e = cmm.Euler(cmm.grid(64), cmm.grid(128), cmm.grid(128), cmm.grid(128), dt, time_integration=tin,
+                  incomp_threshold=1e-4, stack_capacity=64)
+    ids = [e.add_particles(p, **kw) for p, kw in zip(pos, sets)]
+    e.init()
+    mid = None
+    for k in range(16):
+        e.step(dt)
+        if k == 5:
+            mid = [e.particles(i) for i in ids]
+    got = [(e.particles(i), e.particle_velocities(i)) for i in ids]
+    psi = e.field("psi")
+    e.close()
+    # the fluid set has not moved before its start (t = 6/16 < 0.5)
+    assert np.array_equal(mid[0], pos[0])
+    og = O.grid(128)
+    for k, (kw, p0) in enumerate(zip(sets, pos)):
+        r = O.euler_run(O.grid(64), og, og, og, dt=dt, final_time=1.0, tin={3: 30, 4: 40}[L], l_order=L,
+                        threshold=1e-4, stack_capacity=64, particles=p0,
+                        particle_scheme=kw["scheme"], particle_init_time=kw["init_time"], particle_tau=kw["tau"],
+                        particle_init_vel=int(kw["init_vel"]), want=("incomp", "map_counter", "psi"))
+        ep = rel_linf(got[k][0], r["particles"])
+        print(tin, "set", k, kw, "positions", ep)
+        assert ep <= 1e-12
+        if kw["tau"] != 0:
+            ev = rel_linf(got[k][1], r["particle_vel"])
+            print("   velocities", ev)
+            assert ev <= 1e-12
+        else:
+            # (d psi/dx, d psi/dy) of the newest psi at the positions
+            p0r = np.ascontiguousarray(r["psi"][:4 * 128 * 128])
+            ref = np.zeros_like(got[k][1])
+            u = np.zeros(2)
+            xy = r["particles"]
+            for j in range(len(xy) // 2):
+                O.lib().cmmo_hermite_grad(og, O.ptr(p0r), 1, O.C.c_double(xy[2 * j]), O.C.c_double(xy[2 * j + 1]),
+                                          O.ptr(u))
+                ref[2 * j], ref[2 * j + 1] = -u[1], u[0]
+            ok = np.isfinite(got[k][1])
+            print("   velocity output", rel_linf(got[k][1][ok], ref[ok]), "outside", (~ok).sum())
+            assert rel_linf(got[k][1][ok], ref[ok]) <= 1e-12

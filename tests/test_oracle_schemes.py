@@ -370,3 +370,83 @@ def test_stencil_6th_order():
     mx, my = map_planes(*fX), map_planes(*fY)
     nx, _ = stencil_apply(1, eps, mx, my)
     assert np.max(np.abs(plane(nx, 0) - plane(mx, 0))) > 1e-7
+
+
+# ---- inertial particles -------------------------------------------------------------------------
+# x' = v, v' = -(v - u(t))/tau in a uniform flow u(t) = u0 + u1 t has the closed form
+#   v(t) = u(t) - u1 tau + (v0 - u0(t0) + u1 tau) e^{-(t-t0)/tau},
+#   x(t) = x0 + int_{t0}^{t} v,
+# so one step of a scheme of order p misses it by O(dt^(p+1)), and the stage velocities must come
+# from the right time levels for that order to hold.
+
+def inertial_exact(x0, v0, u0, u1, tau, t0, t1):
+    c = v0 - (u0 + u1 * t0) + u1 * tau
+    T = t1 - t0
+    dx = u0 * T + u1 * (t1 ** 2 - t0 ** 2) / 2 - u1 * tau * T + c * tau * (1 - np.exp(-T / tau))
+    v = u0 + u1 * t1 - u1 * tau + c * np.exp(-T / tau)
+    return x0 + dx, v
+
+
+@pytest.mark.parametrize("scheme,L,order", [(10, 2, 1), (20, 2, 2), (30, 3, 3), (31, 3, 3), (40, 4, 4), (41, 4, 4)])
+def test_inertial_particles_local_error_order(scheme, L, order):
+    tau = 0.5
+    u0 = np.array([0.3, -0.2])
+    u1 = np.array([0.4, 0.25])
+    x0 = np.array([C0 + 0.1, C0 - 0.3])
+    v0 = np.array([-0.5, 0.7])
+    errs = []
+    for dt in (0.2, 0.1, 0.05, 0.025):
+        levels = np.array([1.0 - (L - 2 - i) * dt for i in range(L - 1)] + [1.0 + dt])   # ascending, last t_{n+1}
+        psi = np.concatenate([psi_uniform(u0[0] + u1[0] * t, u0[1] + u1[1] * t) for t in levels[::-1]])
+        Lp = lagrange_weights(levels[::-1], 1.0 + dt / 2)
+        pos, vel = O.particles_inertial(G, psi, L, Lp, dt, scheme, tau, x0, v0)
+        ex, ev = inertial_exact(x0, v0, u0, u1, tau, 1.0, 1.0 + dt)
+        errs.append(max(np.max(np.abs(pos - ex)), np.max(np.abs(vel - ev))))
+    orders = observed_orders(errs)
+    print(scheme, errs, orders)
+    assert abs(orders[-1] - (order + 1)) < 0.3 and abs(orders[-2] - (order + 1)) < 0.45
+
+
+def test_inertial_particles_follow_the_fluid_as_tau_vanishes():
+    # with tau -> 0 an inertial particle released at the fluid velocity follows the fluid particle:
+    # the lag is u'(t) tau to first order
+    dt, steps = 2e-3, 200
+    L = 3
+    z = np.array([C0 + 0.3, C0 - 0.2])
+    lag = []
+    for tau in (4e-2, 2e-2, 1e-2):
+        p_in = z.copy()
+        p_fl = z.copy()
+        X, Y = p_in[0] - C0, p_in[1] - C0
+        v = s(1.0) * np.array([P_y(X, Y), -P_x(X, Y)])
+        for k in range(steps):
+            t = 1.0 + k * dt
+            levels = np.array([t - (L - 2 - i) * dt for i in range(L - 1)] + [t + dt])
+            psi = np.concatenate([psi_bicubic(s(tl)) for tl in levels[::-1]])
+            Lp = lagrange_weights(levels[::-1], t + dt / 2)
+            p_in, v = O.particles_inertial(G, psi, L, Lp, dt, 30, tau, p_in, v)
+            p_fl = O.particles_advect(G, psi, L, Lp, dt, 30, p_fl)
+        lag.append(np.max(np.abs(p_in - p_fl)))
+    print("lag", lag)
+    r1, r2 = lag[0] / lag[1], lag[1] / lag[2]
+    assert 1.7 < r1 < 2.3 and 1.7 < r2 < 2.3 and abs(r2 - 2) < abs(r1 - 2)
+
+
+def test_particle_start_time_in_the_loop():
+    # a set with init_time = 4 dt stays put for the first three steps and then moves; an inertial set
+    # keeps its start velocity rule: zero velocity until the step reaching init_time sets the fluid one
+    ogc, ogf = O.grid(16), O.grid(32)
+    dt = 1.0 / 16
+    pos = np.array([1.0, 2.0, 3.0, 1.5])
+    r3 = O.euler_run(ogc, ogf, ogf, ogf, dt=dt, final_time=3 * dt, particles=pos, particle_init_time=4 * dt,
+                     want=("incomp",))
+    assert np.array_equal(r3["particles"], pos)
+    r4 = O.euler_run(ogc, ogf, ogf, ogf, dt=dt, final_time=4 * dt, particles=pos, particle_init_time=4 * dt,
+                     want=("incomp",))
+    assert not np.array_equal(r4["particles"], pos)
+    # inertial, init_vel = 0: the velocity stays zero (and the particle still, since v = 0 and the
+    # first stage moves it by dt * v) until the start; then the set moves
+    ri = O.euler_run(ogc, ogf, ogf, ogf, dt=dt, final_time=3 * dt, particles=pos, particle_init_time=4 * dt,
+                     particle_tau=0.1, particle_init_vel=1, want=("incomp",))
+    assert np.array_equal(ri["particles"], pos)
+    assert np.all(ri["particle_vel"] != 0)          # init_vel = 1: fluid velocity after the warm-up
