@@ -76,7 +76,7 @@ SYMBOLS = ["cmm_version", "cmm_last_error", "cmm_device_info", "cmm_lagrange_coe
            "cmm_map_init_identity", "cmm_advect_workspace_bytes", "cmm_map_advect", "cmm_incomp_error", "cmm_invert_error", "cmm_compose_positions",
            "cmm_stack_pack_split", "cmm_compose_positions_split", "cmm_forward_positions", "cmm_forward_points",
            "cmm_sample_init", "cmm_translate_workspace_bytes", "cmm_translate_ic", "cmm_hermite_workspace_bytes",
-           "cmm_hermite_from_real", "cmm_hermite_from_spectrum", "cmm_psi_workspace_bytes", "cmm_psi_from_vort",
+           "cmm_hermite_from_real", "cmm_hermite_from_spectrum", "cmm_psi_workspace_bytes", "cmm_psi_from_vort", "cmm_psi_from_vort_noalloc",
            "cmm_spectral_workspace_bytes", "cmm_spectral_from_real", "cmm_sample_vort", "cmm_vp_psi_workspace_bytes", "cmm_vp_psi_from_f", "cmm_particles_advect", "cmm_euler_create", "cmm_euler_destroy", "cmm_euler_stream",
            "cmm_euler_add_particles", "cmm_euler_add_particle_set", "cmm_euler_get_particle_velocities", "cmm_euler_init", "cmm_euler_step", "cmm_euler_get_field",
            "cmm_euler_get_particles", "cmm_euler_diagnostics", "cmm_euler_vp_diagnostics", "cmm_euler_sample", "cmm_euler_zoom", "cmm_euler_restart", "cmm_euler_get_stack_map",
@@ -134,6 +134,7 @@ def lib():
         L.cmm_psi_workspace_bytes.argtypes = [Grid, Grid, C.c_double]
         L.cmm_psi_from_vort.argtypes = [Grid, Grid, Grid, Grid, VP, VP, VP, C.c_int, C.c_double, VP, VP,
                                         C.c_size_t, VP]
+        L.cmm_psi_from_vort_noalloc.argtypes = [Grid, Grid, Grid, Grid, VP, VP, VP, C.c_int, C.c_double, VP, VP]
         L.cmm_sample_vort.argtypes = [Grid, Grid, Grid, VP, VP, VP, C.c_int, VP, VP]
         L.cmm_spectral_workspace_bytes.restype = C.c_size_t
         L.cmm_spectral_workspace_bytes.argtypes = [Grid]

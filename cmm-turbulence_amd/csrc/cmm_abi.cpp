@@ -252,6 +252,15 @@ int cmm_psi_from_vort(cmm_grid gm, cmm_grid gv, cmm_grid gf, cmm_grid gp, const 
     return op_psi_from_vort(gm, gv, gf, gp, cx, cy, W, molly, fcut, psi, work, wb, S(stream));
 }
 
+int cmm_psi_from_vort_noalloc(cmm_grid gm, cmm_grid gv, cmm_grid gf, cmm_grid gp, const double *cx, const double *cy,
+                              const double *W, int molly, double fcut, double *psi, void *stream) {
+    if (!grid_ok(gm) || !grid_ok(gf) || !cx || !cy || !W || !psi || (molly != 0 && molly != 4 && molly != 8)) {
+        set_error("cmm_psi_from_vort_noalloc: bad argument");
+        return CMM_ERR_ARG;
+    }
+    return op_psi_from_vort_noalloc(gm, gv, gf, gp, cx, cy, W, molly, fcut, psi, S(stream));
+}
+
 int cmm_sample_vort(cmm_grid gm, cmm_grid gt, cmm_grid gf, const double *cx, const double *cy, const double *W,
                     int molly, double *out, void *stream) {
     if (!grid_ok(gm) || !grid_ok(gt) || !grid_ok(gf) || !cx || !cy || !W || !out) {

@@ -42,6 +42,12 @@ size_t op_psi_workspace(const cmm_grid &gv, const cmm_grid &gp, double fcut);
 int op_psi_from_vort(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf, const cmm_grid &gp, const double *cx,
                      const double *cy, const double *W, int molly, double fcut, double *psi, void *work,
                      size_t work_bytes, hipStream_t st, hipEvent_t *ev = nullptr, const double *W4 = nullptr);
+// op_psi_from_vort with its workspace and twiddle tables in a static device array (no allocation);
+// op_psi_noalloc_max_bytes() is the largest workspace it takes
+size_t op_psi_noalloc_max_bytes();
+int op_psi_from_vort_noalloc(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf, const cmm_grid &gp,
+                             const double *cx, const double *cy, const double *W, int molly, double fcut, double *psi,
+                             hipStream_t st);
 // H4[4j + p] = H[p*n + j] (node-interleaved copy of a Hermite field)
 int op_interleave4(const cmm_grid &g, const double *H, double *H4, hipStream_t st);
 size_t op_hermite_workspace(const cmm_grid &g);

@@ -12,6 +12,7 @@
 #include <type_traits>
 #include <vector>
 
+#include "../../include/cmm.h"
 #include "cmm_core.hpp"
 #include "cmm_internal.hpp"
 #include "../../include/cmm.h"
@@ -1696,14 +1697,23 @@ size_t spec_workspace_bytes(const cmm_grid &gin, const cmm_grid &gout, int mode,
 }
 
 // ev (optional): four events recorded before the row transform and after each of the three kernels
+// tw_table (optional): twiddle tables of all power-of-two sizes 8..4096, table n at entry n - 8,
+// in place of the cached ones (which are allocated on first use)
 template <class S>
 static int run_spec(const SpecPlan &sp, const S &sampler, double *out, void *work, hipStream_t st,
-                    hipEvent_t *ev = nullptr, bool ev0 = true) {
+                    hipEvent_t *ev = nullptr, bool ev0 = true, const cd *tw_table = nullptr) {
     const cd *tw_in, *tw_out, *tw_x_in, *tw_x_out;
-    if (int rc = twiddles(sp.gin.nx, &tw_x_in)) return rc;
-    if (int rc = twiddles(sp.gin.ny, &tw_in)) return rc;
-    if (int rc = twiddles(sp.gout.ny, &tw_out)) return rc;
-    if (int rc = twiddles(sp.gout.nx, &tw_x_out)) return rc;
+    if (tw_table) {
+        tw_x_in = tw_table + (sp.gin.nx - 8);
+        tw_in = tw_table + (sp.gin.ny - 8);
+        tw_out = tw_table + (sp.gout.ny - 8);
+        tw_x_out = tw_table + (sp.gout.nx - 8);
+    } else {
+        if (int rc = twiddles(sp.gin.nx, &tw_x_in)) return rc;
+        if (int rc = twiddles(sp.gin.ny, &tw_in)) return rc;
+        if (int rc = twiddles(sp.gout.ny, &tw_out)) return rc;
+        if (int rc = twiddles(sp.gout.nx, &tw_x_out)) return rc;
+    }
     cd *T = (cd *)work;
     cd *G0 = (cd *)((char *)work + sp.t_bytes);
     cd *G1 = (cd *)((char *)work + sp.t_bytes + sp.g_bytes);
@@ -2008,6 +2018,82 @@ int op_psi_from_vort(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf,
     else launch(SampleVort{}, W);
     if (int rc = launch_check("k_sample_field")) return rc;
     return run_spec(sp, SampleReal{f, gv.nx}, psi, w, st, ev, false);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Allocation-free psi solve (op_psi_from_vort_noalloc).  `scratch` is a file-scope device array of
+// kScratchBytes that every GPU the library runs on holds until the process ends: the twiddle tables
+// of every power-of-two size 8..4096 (table n at entry n - 8, exp(-2 pi i m/n), m < n, from long
+// double as twiddles() makes them), then the psi solve's workspace.  Its device address on a GPU
+// comes from hipGetSymbolAddress, taken once per device with that device current, when the first
+// call on it sets the tables up; the array's name is never passed to a kernel.  The workspace needs
+// no clearing: every kernel writes what the next one reads.  Calls that overlap on one GPU race.
+constexpr size_t kScratchTwBytes = 131072;  // 8184 complex doubles, padded to 256-byte blocks
+constexpr size_t kScratchBytes = (size_t)192 << 20;
+static_assert(kScratchBytes - kScratchTwBytes == CMM_PSI_NOALLOC_MAX_WORKSPACE, "scratch limit of include/cmm.h");
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kScratchDevices = 64;
+static char *g_scratch_ptr[kScratchDevices];  // device address of scratch per device (under g_mutex)
+
+static int scratch_setup(char **out) {
+    int dev;
+    if (int rc = current_device(dev)) return rc;
+    if (dev < 0 || dev >= kScratchDevices) {
+        set_error("allocation-free psi solve: device ordinal beyond 63");
+        return CMM_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(g_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void *ptr = nullptr;
+        if (int rc = hip_check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress scratch")) return rc;
+        static cd h_tw[8184];  // host copy of the tables (static: no allocation on this path)
+        for (int n = 8; n <= kFftMaxN; n *= 2)
+            for (int m = 0; m < n; ++m) {
+                long double a = -2.0L * 3.141592653589793238462643383279502884L * (long double)m / (long double)n;
+                h_tw[n - 8 + m] = cd{(double)cosl(a), (double)sinl(a)};
+            }
+        if (int rc = hip_check(hipMemcpy(ptr, h_tw, sizeof(h_tw), hipMemcpyHostToDevice), "hipMemcpy scratch twiddles"))
+            return rc;
+        g_scratch_ptr[dev] = static_cast<char *>(ptr);
+    }
+    *out = g_scratch_ptr[dev];
+    return CMM_OK;
+}
+
+size_t op_psi_noalloc_max_bytes() { return kScratchBytes - kScratchTwBytes; }
+
+int op_psi_from_vort_noalloc(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf, const cmm_grid &gp,
+                             const double *cx, const double *cy, const double *W, int molly, double fcut, double *psi,
+                             hipStream_t st) {
+    if (int rc = fft_grid_ok(gv, "vort grid")) return rc;
+    if (int rc = fft_grid_ok(gp, "psi grid")) return rc;
+    SpecPlan sp = plan_spec(gv, gp, 1, fcut);
+    const size_t base = sp.t_bytes + 2 * sp.g_bytes + 256;
+    const size_t need = base + sizeof(double) * (size_t)gv.nx * gv.ny;
+    if (need > op_psi_noalloc_max_bytes()) {
+        set_error("allocation-free psi solve: the workspace of these grids and cut-off (" + std::to_string(need) +
+                  " bytes) exceeds the static scratch array (" + std::to_string(op_psi_noalloc_max_bytes()) + " bytes)");
+        return CMM_ERR_ARG;
+    }
+    char *base_ptr = nullptr;
+    if (int rc = scratch_setup(&base_ptr)) return rc;
+    const cd *tw = reinterpret_cast<const cd *>(base_ptr);
+    void *w = base_ptr + kScratchTwBytes;
+    double *f = reinterpret_cast<double *>(static_cast<char *>(w) + base);
+    const long lanes = (long)gv.nx * (gv.ny / kSampleRows);
+    SampleVort smp;
+    smp.gm = make_dgrid(gm);
+    smp.gv = make_dgrid(gv);
+    smp.gf = make_dgrid(gf);
+    smp.cx = cx;
+    smp.cy = cy;
+    smp.W = W;
+    smp.molly = molly;
+    hipLaunchKernelGGL(k_sample_field<SampleVort>, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, st, smp, gv.nx,
+                       gv.ny, f);
+    if (int rc = launch_check("k_sample_field")) return rc;
+    return run_spec(sp, SampleReal{f, gv.nx}, psi, w, st, nullptr, false, tw);
 }
 
 // node-interleaved copy of a Hermite field: H4[4j + p] = H[p*n + j]

@@ -142,6 +142,20 @@ size_t cmm_psi_workspace_bytes(cmm_grid g_vort, cmm_grid g_psi, double freq_cut)
 int cmm_psi_from_vort(cmm_grid g_map, cmm_grid g_vort, cmm_grid g_fine, cmm_grid g_psi, const double *chix,
                       const double *chiy, const double *w_herm, int molly_stencil, double freq_cut, double *psi_out,
                       void *work, size_t work_bytes, void *stream);
+/* cmm_psi_from_vort without any allocation.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same kernels and results as cmm_psi_from_vort.  Its workspace and the FFT
+ * twiddle tables live in a static device array the library holds on every GPU it runs on until the
+ * process ends (192 MiB); a call whose workspace, cmm_psi_workspace_bytes(g_vort, g_psi, freq_cut),
+ * exceeds CMM_PSI_NOALLOC_MAX_WORKSPACE returns CMM_ERR_ARG without launching anything (the
+ * 4096^2 vorticity grid fits for freq_cut up to 511).  The first call on a device looks the
+ * array's address up and copies the twiddle tables with a synchronous copy: make it outside graph
+ * capture.  All calls on one GPU share the array: calls that overlap on one GPU (on different
+ * streams or host threads) race. */
+#define CMM_PSI_NOALLOC_MAX_WORKSPACE ((size_t)201195520)
+int cmm_psi_from_vort_noalloc(cmm_grid g_map, cmm_grid g_vort, cmm_grid g_fine, cmm_grid g_psi, const double *chix,
+                              const double *chiy, const double *w_herm, int molly_stencil, double freq_cut,
+                              double *psi_out, void *stream);
 /* omega(X(x)) on g_target without transform (coarse vorticity output); k_apply_map_and_sample_from_hermite
  * with padd_inline_fft = false (Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:402-461). */
 int cmm_sample_vort(cmm_grid g_map, cmm_grid g_target, cmm_grid g_fine, const double *chix, const double *chiy,
