@@ -7,7 +7,14 @@ vorticity Hermite), psi solve, and RK3 advection of 4096 particles.  The run sta
 random-spectrum vorticity (E(k) ~ k^4 exp(-2 (k/6)^2), enstrophy 1), dt = 1/256; W untimed
 steps then K timed steps (default W + K = 512 steps = T 2).  Prints one JSON line.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--copy-calibration]
+The JSON line also carries: the split-precision form at the same step and both forms at twice the
+step run to the same end time (speed-up in simulated time and omega error against plain FP64 at the
+nominal step); map-stack composition at depth; the double-shear-layer workload (128^2 / 256^2 /
+512^2) stepped singly and from captured graphs; a 100-step window of the 32-instance Vlasov-Poisson
+ensemble, graph batch against one by one; and the CPU baseline (the C++/OpenMP oracle's whole steps
+of the same workload and one timed remap, with the thread count, nproc and CPU model).
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu-baseline] [--no-ensemble] [--copy-calibration]
   python bench.py summarize-profiles KT_DIR FETCH_DIR WRITE_DIR OUT_DIR "kt cmd" "fetch cmd" "write cmd"
 
 roofline.traffic is the dominant kernel's HBM bytes per launch from the committed counter summary
@@ -80,45 +87,68 @@ def uniform_particles(n, seed):
     return cmm.uniform_particles(n, seed)
 
 
-def cpu_baseline(c, threads, budget_s=12.0):
-    """The oracle (C++/OpenMP, FP64, same algorithm) timed on the hot-path operators of one step at
-    full size (advect 1024^2 with psi 2048^2, check on 4096^2, psi solve 4096^2 -> 2048^2, 4096 RK3
-    particles) from a random-spectrum state, repeated until budget_s of CPU work has run (at least
-    two repetitions); returns (steps/s, repetitions, seconds)."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def stack_maps_host(nc, n_maps):
+    """n_maps smooth shears of the identity on an nc^2 grid (derivative planes exact), as the
+    composition leg stacks them: (stack_x, stack_y) plane-major, oldest first"""
+    import numpy as np
+    h = (2 * np.pi) / float(np.float32(nc))
+    xs = np.arange(nc) * h
+    X, Y = np.meshgrid(xs, xs)
+    X, Y = X.ravel(), Y.ravel()
+    sx, sy = [], []
+    for m in range(n_maps):
+        a, ph = 0.01 * (1 + m % 3), 0.37 * m
+        sx.append(np.concatenate([X + a * np.sin(Y + ph), np.ones_like(X), a * np.cos(Y + ph), np.zeros_like(X)]))
+        sy.append(np.concatenate([Y + a * np.sin(X - ph), a * np.cos(X - ph), np.ones_like(X), np.zeros_like(X)]))
+    return np.concatenate(sx), np.concatenate(sy)
+
+
+def cpu_baseline(c, threads, steps=16, remap_maps=32):
+    """The oracle (C++/OpenMP, FP64, the same algorithm) on the host cores, whole steps of the same
+    workload: one run through the Lagrange warm-up only and one through the warm-up and `steps`
+    steps (advection, check on the 4096^2 grid, remap when due, psi solve, 4096 RK3 particles);
+    the difference is the steady stepping.  Then one remap timed on its own: the 4096^2 fine grid
+    composed through the current map and remap_maps stacked 1024^2 maps, the initial condition
+    sampled there and its Hermite form built (translate_initial_condition_through_map_stack)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     os.environ["OMP_NUM_THREADS"] = str(threads)
     import numpy as np
     import oracle_lib as O
     gc, gp, gf = O.grid(c["n_coarse"]), O.grid(c["n_psi"]), O.grid(c["n_fine"])
-    w = O.ic_random_spectrum(c["n_fine"], c["seed"])
-    W = O.hermite_from_real(gf, w)
-    cx, cy = O.identity_map(gc)
-    psi1 = O.psi_from_vort(gc, gf, gf, gp, cx, cy, W, 0, c["freq_cut_psi"])
-    psi = np.concatenate([psi1, psi1, psi1])
+    t0 = time.perf_counter()
+    H = O.hermite_from_real(gf, O.ic_random_spectrum(c["n_fine"], c["seed"]))
     pos = uniform_particles(c["n_particles"], c["seed"])
-    dt = c["dt"]
-    t = np.array([-2 * dt, -dt, 0.0, dt])
-    coeff = O.advect_coeffs(t, np.full(4, dt), 0, 3, 30, 1, c["eps"])
-    parts = np.zeros(4)
-    reps = 0
-    t_start = time.perf_counter()
-    while reps < 2 or time.perf_counter() - t_start < budget_s:
-        t0 = time.perf_counter()
-        nx, ny = O.map_advect(gc, gp, cx, cy, psi, coeff)
-        t1 = time.perf_counter()
-        O.incomp_error(gc, gf, nx, ny)
-        t2 = time.perf_counter()
-        O.psi_from_vort(gc, gf, gf, gp, nx, ny, W, 0, c["freq_cut_psi"])
-        t3 = time.perf_counter()
-        O.particles_advect(gp, psi, 3, [0.375, 0.75, -0.125], dt, 30, pos.copy())
-        t4 = time.perf_counter()
-        parts += np.array([t1 - t0, t2 - t1, t3 - t2, t4 - t3])
-        reps += 1
-    total = time.perf_counter() - t_start
-    parts /= reps
-    log(f"cpu baseline ({threads} threads, {reps} reps, {total:.1f}s): advect {parts[0]:.3f}s incomp {parts[1]:.3f}s "
-        f"psi {parts[2]:.3f}s particles {parts[3]:.4f}s per step")
-    return reps / total, reps, total
+    kw = dict(dt=c["dt"], final_time=c["final_time"], fcut=c["freq_cut_psi"], eps=c["eps"], threshold=c["threshold"],
+              disc=H, g_disc=gf, stack_capacity=c["stack_capacity"], particles=pos, particle_scheme=30,
+              want=("incomp",))
+    t1 = time.perf_counter()
+    O.euler_run(gc, gf, gp, gf, max_steps=-1, **kw)
+    t2 = time.perf_counter()
+    r = O.euler_run(gc, gf, gp, gf, max_steps=steps, **kw)
+    t3 = time.perf_counter()
+    assert r["steps"] == steps
+    sps = steps / ((t3 - t2) - (t2 - t1))
+    # one remap at depth remap_maps + 1
+    sx, sy = stack_maps_host(c["n_coarse"], remap_maps)
+    cx, cy = O.identity_map(gc)
+    t4 = time.perf_counter()
+    xy = O.compose_positions(gc, gf, cx, cy, sx, sy, remap_maps)
+    w = O.sample_init(gf, xy, 101, disc=H, gdisc=gf, discrete=1)
+    O.hermite_from_real(gf, w)
+    t5 = time.perf_counter()
+    log(f"cpu baseline ({threads} threads): initial field {t1 - t0:.2f}s, warm-up {t2 - t1:.2f}s, warm-up + {steps} "
+        f"steps {t3 - t2:.2f}s -> {sps:.3f} steps/s; remap through {remap_maps + 1} maps {t5 - t4:.2f}s")
+    return sps, (t3 - t2) - (t2 - t1), t5 - t4, (t1 - t0) + (t3 - t1) + (t5 - t4)
 
 
 # bench slot -> kernel-name patterns of the launches it times (the row-transform slot of the psi solve
@@ -308,7 +338,7 @@ def roofline(c, prof, n_stack, suffix=""):
     gbs = d["bytes"] / (d["ms_avg"] * 1e-3) / 1e9
     tfs = d["flops"] / (d["ms_avg"] * 1e-3) / 1e12
     if tfs / FP64_PEAK_TFLOPS > gbs / HBM_PEAK_GBS:
-        roof = {"bound": "mfma", "achieved": round(tfs, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+        roof = {"bound": "fp64", "achieved": round(tfs, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(tfs / FP64_PEAK_TFLOPS, 4)}
     else:
         roof = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -323,10 +353,11 @@ def roofline(c, prof, n_stack, suffix=""):
     return roof, per
 
 
-def run_form(cmm, torch, c, precision, warmup, steps, dt=None):
+def run_form(cmm, torch, c, precision, warmup, steps, dt=None, t_end=None):
     """One engine run of the workload: W untimed steps, then K steps timed between two device
     synchronisations with per-kernel events; returns the rate, the profile and the final fields
-    (coarse vorticity, newest psi level, particles) for the error comparison."""
+    (coarse vorticity, newest psi level, particles) for the error comparison.  With t_end the timed
+    steps instead run until t_end, the last one shortened to land on it."""
     import numpy as np
     g = cmm.grid
     dt = c["dt"] if dt is None else dt
@@ -345,21 +376,116 @@ def run_form(cmm, torch, c, precision, warmup, steps, dt=None):
     torch.cuda.synchronize()
     eng.profile(True)
     remaps = 0
+    _, t_start, _ = eng.state()
+    t = t_start
+    i = 0
     t0 = time.perf_counter()
-    for i in range(steps):
-        info = eng.step(dt)
+    while (i < steps) if t_end is None else (t_end - t > 1e-5 * dt):
+        info = eng.step(dt if t_end is None else min(dt, t_end - t))
+        t = info.t
         remaps += info.remapped
         einc.append(info.incomp)
-        if (i + 1) % 128 == 0:
-            log(f"[{precision}] step {warmup + i + 1}: t={info.t:.4f} E_inc={info.incomp:.3e} maps={info.stack_size}")
+        i += 1
+        if i % 128 == 0:
+            log(f"[{precision}] step {warmup + i}: t={info.t:.4f} E_inc={info.incomp:.3e} maps={info.stack_size}")
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     prof = eng.profile_read()
     _, t_now, n_stack = eng.state()
-    out = {"sps": steps / (t1 - t0), "elapsed": t1 - t0, "prof": prof, "maps": n_stack, "t_end": t_now,
+    out = {"sps": i / (t1 - t0), "elapsed": t1 - t0, "steps": i, "sim_time": t_now - t_start,
+           "sim_per_s": (t_now - t_start) / (t1 - t0), "prof": prof, "maps": n_stack, "t_end": t_now,
            "remaps": remaps, "init_s": t_init, "einc": np.array(einc), "w": eng.field("w_coarse"),
            "psi": eng.field("psi")[:4 * c["n_psi"] ** 2], "part": eng.particles(0)}
     eng.close()
+    return out
+
+
+def run_c1(cmm, torch, cpu_threads=None):
+    """The double-shear-layer workload (map 128^2, psi 256^2, fine 512^2, dt 1/64, RK3, remap at
+    1e-4) to T = 1: steps/s of single-engine steps (cmm_euler_step, one host wait per step) and of
+    the same engine stepped from captured graphs (a batch of one engine); the oracle's steady steps
+    on the host as its CPU baseline."""
+    g = cmm.grid
+    dt, n = 1.0 / 64, 64
+
+    def engine():
+        e = cmm.Euler(g(128), g(512), g(256), g(512), dt, freq_cut_psi=128.0, initial_condition="double_shear_layer",
+                      incomp_threshold=1e-4, stack_capacity=64)
+        e.init()
+        return e
+    out = {"workload": "double shear layer, map 128^2, psi 256^2, fine 512^2, dt 1/64, RK3, remap 1e-4, T 1"}
+    for mode in ("eager", "graph"):
+        e = engine()
+        b = cmm.Batch([e], use_graphs=True) if mode == "graph" else None
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        remaps = 0
+        for _ in range(n):
+            info = b.step(dt)[0] if b else e.step(dt)
+            remaps += info.remapped
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        out[mode] = {"steps_per_s": round(n / el, 1), "ms_per_step": round(1e3 * el / n, 4), "remaps": remaps,
+                     "t_end": info.t}
+        if b:
+            out[mode]["graph"] = b.stats()
+            b.close()
+        e.close()
+    out["graph_vs_eager"] = round(out["graph"]["steps_per_s"] / out["eager"]["steps_per_s"], 3)
+    if cpu_threads:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+        kw = dict(dt=dt, final_time=1.0, ic=100, threshold=1e-4, fcut=128.0, stack_capacity=64, want=("incomp",))
+        og = O.grid
+        t0 = time.perf_counter()
+        O.euler_run(og(128), og(512), og(256), og(512), max_steps=-1, **kw)
+        t1 = time.perf_counter()
+        O.euler_run(og(128), og(512), og(256), og(512), **kw)
+        t2 = time.perf_counter()
+        out["cpu_oracle_steps_per_s"] = round(n / ((t2 - t1) - (t1 - t0)), 2)
+        out["cpu_threads"] = cpu_threads
+    log(f"config 1: {out}")
+    return out
+
+
+def ensemble_leg(cmm, torch, n_inst=32, steps=100):
+    """The 32-instance Vlasov-Poisson ensemble for a bounded window of steps after the
+    initialisation, FP64: the captured-graph batch against the instances stepped one by one
+    (cmm_euler_step each), each mode on freshly built, identical engines over the same steps;
+    saturated instances (stack full, remap skipped) are counted."""
+    dt = 1.0 / 20
+    specs = ensemble_specs(n_inst, 0)
+    g = cmm.ensemble_grid
+    out = {"workload": f"{n_inst} Vlasov-Poisson instances, maps 256^2/512^2/1024^2 (fine 4x, psi and vort 2x), "
+                       f"Landau / two-stream, amplitudes U[0.005, 0.05] seed 0, dt 1/20, steps 1..{steps}"}
+    for mode in ("graph_batch", "one_by_one"):
+        es = []
+        for nn, ic, amp in specs:
+            e = cmm.Euler(g(nn), g(4 * nn), g(2 * nn), g(2 * nn), dt, initial_condition=ic, ic_params=[amp],
+                          simulation_type="cmm_vlasov_poisson_1d", incomp_threshold=1e-4, stack_capacity=64)
+            e.init()
+            es.append(e)
+        b = cmm.Batch(es, use_graphs=True) if mode == "graph_batch" else None
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        remaps = saturated = 0
+        for _ in range(steps):
+            infos = b.step(dt) if b else [e.step(dt) for e in es]
+            remaps += sum(x.remapped for x in infos)
+            saturated += sum(x.saturated for x in infos)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        out[mode] = {"ensemble_steps_per_s": round(steps / el, 2), "instance_steps_per_s": round(steps * n_inst / el, 1),
+                     "remaps": remaps, "saturated_steps": saturated}
+        if b:
+            out[mode]["graph"] = b.stats()
+            b.close()
+        for e in es:
+            e.close()
+        torch.cuda.synchronize()
+    out["graph_batch_vs_one_by_one"] = round(out["graph_batch"]["ensemble_steps_per_s"] /
+                                             out["one_by_one"]["ensemble_steps_per_s"], 3)
+    log(f"ensemble: {out}")
     return out
 
 
@@ -536,6 +662,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=8)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-split", action="store_true", help="skip the split-precision form")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the short ensemble leg")
     ap.add_argument("--ensemble", action="store_true",
                     help="run the batched Vlasov-Poisson ensemble instead (its own JSON line)")
     ap.add_argument("--instances", type=int, default=32)
@@ -602,25 +729,43 @@ def main():
                   "roofline": roofline(c, sp["prof"], sp["maps"], "_split")[0],
                   "kernels_ms_avg": {k: round(ms / cnt, 5) for k, (ms, cnt) in sp["prof"].items() if cnt}}
         log(f"split form: {sp['sps']:.1f} steps/s ({chosen['speedup']:.2f}x), errors {chosen['errors_vs_fp64_at_t_end']}")
-        # the same forms at twice the step (plain Lagrange extrapolation of psi), to the same end
-        # time: their error against the FP64 run at the nominal step, and the rate in simulated time
+        # both forms at twice the step (plain Lagrange extrapolation of psi) to the same end time as
+        # the nominal run; the error of each against the FP64 run at the nominal step, and the rate
+        # in simulated time against it
         big = {}
         for prec in ("fp64", "split"):
-            r = run_form(cmm, torch, c, prec, args.warmup // 2, args.steps // 2, dt=2 * c["dt"])
-            big[prec] = {"dt": 2 * c["dt"], "steps_per_s": round(r["sps"], 3),
-                         "speedup_in_simulated_time": round(2 * r["sps"] / sps, 3), "t_end": r["t_end"],
+            r = run_form(cmm, torch, c, prec, args.warmup // 2, 0, dt=2 * c["dt"], t_end=t_now)
+            assert abs(r["t_end"] - t_now) <= 1e-12, (r["t_end"], t_now)
+            big[prec] = {"dt": 2 * c["dt"], "steps_per_s": round(r["sps"], 3), "t_end": r["t_end"],
+                         "speedup_in_simulated_time": round(r["sim_per_s"] / plain["sim_per_s"], 3),
                          "omega_coarse_rel_l2": rl2(r["w"], plain["w"]), "psi_rel_l2": rl2(r["psi"], plain["psi"])}
             log(f"[{prec} at 2 dt] {big[prec]}")
         chosen["at_twice_the_step"] = big
+        # the three readings at the common end time: each against plain FP64 at the nominal step
+        chosen["readings"] = {
+            "fp64_at_2dt_vs_fp64_at_dt": {"speedup_in_simulated_time": big["fp64"]["speedup_in_simulated_time"],
+                                          "omega_coarse_rel_l2": big["fp64"]["omega_coarse_rel_l2"]},
+            "split_at_2dt_vs_fp64_at_dt": {"speedup_in_simulated_time": big["split"]["speedup_in_simulated_time"],
+                                           "omega_coarse_rel_l2": big["split"]["omega_coarse_rel_l2"]},
+            "split_vs_fp64_at_the_same_dt": {"speedup": round(sp["sps"] / sps, 3),
+                                             "omega_coarse_rel_l2": rl2(sp["w"], plain["w"])},
+            "split_at_2dt_vs_fp64_at_2dt": {"speedup": round(big["split"]["steps_per_s"] / big["fp64"]["steps_per_s"], 3)}}
 
     cpu = None
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
-        v, reps, secs = cpu_baseline(c, threads)
-        cpu = {"value": round(v, 5), "unit": "steps/s", "cores": threads, "kind": "port",
-               "sample": f"{reps} repetitions of one step's hot-path operators at full size (advect 1024^2 map with "
-                         f"2048^2 psi, check on 4096^2, psi solve 4096^2->2048^2, 4096 RK3 particles) by the "
-                         f"OpenMP oracle, {secs:.1f} s of CPU work"}
+        # the host's share of threads for one GPU (OpenMP threads as set for the job, at most 16)
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+        threads = min(threads, 16)
+        v, secs, remap_s, total = cpu_baseline(c, threads)
+        cpu = {"value": round(v, 4), "unit": "steps/s", "cores": threads, "kind": "port",
+               "nproc": os.cpu_count(), "cpu_model": cpu_model(), "remap_s": round(remap_s, 3),
+               "sample": f"16 whole steps of the same workload (advection, check on 4096^2, psi solve, 4096 RK3 "
+                         f"particles; no remap falls due) after the Lagrange warm-up, by the C++/OpenMP oracle on "
+                         f"{threads} threads ({secs:.1f} s), and one remap through 33 maps on the 4096^2 grid "
+                         f"timed alone; {total:.1f} s of CPU work in all"}
+
+    c1 = run_c1(cmm, torch, cpu["cores"] if cpu else None)
+    ens = None if args.no_ensemble else ensemble_leg(cmm, torch)
 
     out = {
         "metric": METRIC, "value": round(sps, 3), "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
@@ -636,6 +781,8 @@ def main():
         "kernels_ms_avg": {k: round(v["ms_avg"], 5) for k, v in per.items()},
         "chosen_form": chosen,
         "composition": comp,
+        "config1": c1,
+        "ensemble": ens,
     }
     print(json.dumps(out), flush=True)
 
