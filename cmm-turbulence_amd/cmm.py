@@ -74,7 +74,7 @@ FIELD = {"chix": 0, "chiy": 1, "psi": 2, "w_fine": 3, "w_coarse": 4, "disc": 5, 
 # every symbol include/cmm.h declares
 SYMBOLS = ["cmm_version", "cmm_last_error", "cmm_device_info", "cmm_lagrange_coeff", "cmm_advect_coeffs_fill",
            "cmm_map_init_identity", "cmm_advect_workspace_bytes", "cmm_map_advect", "cmm_incomp_error", "cmm_invert_error", "cmm_compose_positions",
-           "cmm_stack_pack_split", "cmm_compose_positions_split", "cmm_forward_positions", "cmm_forward_points",
+           "cmm_stack_pack", "cmm_compose_positions_packed", "cmm_forward_positions", "cmm_forward_points",
            "cmm_sample_init", "cmm_translate_workspace_bytes", "cmm_translate_ic", "cmm_hermite_workspace_bytes",
            "cmm_hermite_from_real", "cmm_hermite_from_spectrum", "cmm_psi_workspace_bytes", "cmm_psi_from_vort", "cmm_psi_from_vort_noalloc",
            "cmm_spectral_workspace_bytes", "cmm_spectral_from_real", "cmm_sample_vort", "cmm_vp_psi_workspace_bytes", "cmm_vp_psi_from_f", "cmm_particles_advect", "cmm_euler_create", "cmm_euler_destroy", "cmm_euler_stream",
@@ -118,8 +118,8 @@ def lib():
         L.cmm_invert_error.argtypes = [Grid, Grid, Grid, VP, VP, VP, VP, DP, VP]
         L.cmm_compose_positions.argtypes = [Grid, Grid, VP, VP, VP, C.c_int, VP, VP]
         L.cmm_forward_positions.argtypes = [Grid, Grid, VP, VP, VP, C.c_int, VP, VP]
-        L.cmm_stack_pack_split.argtypes = [Grid, VP, C.c_int, VP, VP]
-        L.cmm_compose_positions_split.argtypes = [Grid, Grid, VP, VP, VP, C.c_int, VP, VP]
+        L.cmm_stack_pack.argtypes = [Grid, VP, C.c_int, C.c_int, VP, VP]
+        L.cmm_compose_positions_packed.argtypes = [Grid, Grid, VP, VP, VP, C.c_int, C.c_int, VP, VP]
         L.cmm_forward_points.argtypes = [Grid, Grid, VP, VP, VP, C.c_int, C.c_int, C.c_int, C.c_int, VP, VP, VP]
         L.cmm_sample_init.argtypes = [Grid, Grid, VP, VP, VP, C.c_int, C.c_int, DP, C.c_int, Grid, VP, VP, VP]
         L.cmm_translate_workspace_bytes.restype = C.c_size_t
@@ -287,16 +287,21 @@ def apply_map_stack(g_target, g_map, chix, chiy, stack, n_stack, xy):
            "cmm_compose_positions")
 
 
-def stack_pack_split(g_map, stack, n_maps, out):
-    """FP64 stacked maps into the split stack: value planes FP64, derivative planes FP32 (40 bytes
-    per node; out holds at least 40 * Nc * n_maps bytes)"""
-    _check(lib().cmm_stack_pack_split(g_map, _p(stack), n_maps, _p(out), _stream()), "cmm_stack_pack_split")
+# stack record formats (CMM_STACK_RECORDS_*): 64 bytes per node FP64, 48 bytes with FP32 derivatives
+STACK_RECORDS = {"fp64": 1, "split": 2}
+STACK_RECORD_BYTES = {"fp64": 64, "split": 48}
 
 
-def apply_map_stack_split(g_target, g_map, chix, chiy, stack_split, n_stack, xy):
-    """apply_map_stack through a split stack (stack_pack_split)"""
-    _check(lib().cmm_compose_positions_split(g_map, g_target, _p(chix), _p(chiy), _p(stack_split), n_stack, _p(xy),
-                                             _stream()), "cmm_compose_positions_split")
+def stack_pack(g_map, stack, n_maps, fmt, out):
+    """plane-major stacked maps into node records (out holds STACK_RECORD_BYTES[fmt] * Nc * n_maps
+    bytes)"""
+    _check(lib().cmm_stack_pack(g_map, _p(stack), n_maps, STACK_RECORDS[fmt], _p(out), _stream()), "cmm_stack_pack")
+
+
+def apply_map_stack_packed(g_target, g_map, chix, chiy, packed, n_stack, fmt, xy):
+    """apply_map_stack through a stack packed by stack_pack"""
+    _check(lib().cmm_compose_positions_packed(g_map, g_target, _p(chix), _p(chiy), _p(packed), n_stack,
+                                              STACK_RECORDS[fmt], _p(xy), _stream()), "cmm_compose_positions_packed")
 
 
 def apply_map_stack_forward(g_target, g_map, chix, chiy, stack, n_stack, xy):

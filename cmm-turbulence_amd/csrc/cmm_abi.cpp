@@ -148,22 +148,24 @@ int cmm_compose_positions(cmm_grid gm, cmm_grid gt, const double *cx, const doub
     return op_compose_positions(gm, gt, cx, cy, stack, ns, xy, S(stream));
 }
 
-int cmm_stack_pack_split(cmm_grid gm, const double *stack, int n_maps, void *out, void *stream) {
-    if (!grid_ok(gm) || n_maps < 0 || (n_maps > 0 && (!stack || !out))) {
-        set_error("cmm_stack_pack_split: bad argument");
+int cmm_stack_pack(cmm_grid gm, const double *stack, int n_maps, int format, void *packed, void *stream) {
+    if (!grid_ok(gm) || n_maps < 0 || (n_maps > 0 && (!stack || !packed)) ||
+        (format != CMM_STACK_RECORDS_FP64 && format != CMM_STACK_RECORDS_SPLIT)) {
+        set_error("cmm_stack_pack: bad argument");
         return CMM_ERR_ARG;
     }
-    return op_stack_pack(gm, stack, n_maps, out, S(stream));
+    const long N = (long)gm.nx * gm.ny;
+    return op_pack_maps(gm, stack, stack + 4 * N, 8 * N, n_maps, format, packed, S(stream));
 }
 
-int cmm_compose_positions_split(cmm_grid gm, cmm_grid gt, const double *cx, const double *cy,
-                                const void *stack_split, int ns, double *xy, void *stream) {
-    if (!grid_ok(gm) || !grid_ok(gt) || !cx || !cy || !xy || ns < 0 || (ns > 0 && !stack_split)) {
-        set_error("cmm_compose_positions_split: bad argument");
+int cmm_compose_positions_packed(cmm_grid gm, cmm_grid gt, const double *cx, const double *cy, const void *packed,
+                                 int ns, int format, double *xy, void *stream) {
+    if (!grid_ok(gm) || !grid_ok(gt) || !cx || !cy || !xy || ns < 0 || (ns > 0 && !packed) ||
+        (format != CMM_STACK_RECORDS_FP64 && format != CMM_STACK_RECORDS_SPLIT)) {
+        set_error("cmm_compose_positions_packed: bad argument");
         return CMM_ERR_ARG;
     }
-    return op_compose_positions(gm, gt, cx, cy, static_cast<const double *>(stack_split), ns, xy, S(stream),
-                                kStackSplit);
+    return op_compose_positions(gm, gt, cx, cy, static_cast<const double *>(packed), ns, xy, S(stream), format);
 }
 
 int cmm_forward_positions(cmm_grid gm, cmm_grid gt, const double *cx, const double *cy, const double *stack,
@@ -441,7 +443,7 @@ bool split(const cmm_euler *e) { return e->p.precision == CMM_PRECISION_SPLIT; }
 
 // the stack the compositions read, and its format
 const double *comp_stack(const cmm_euler *e) { return e->stack_s ? e->stack_s : e->stack; }
-int comp_fmt(const cmm_euler *e) { return e->stack_s ? kStackSplit : kStackFp64; }
+int comp_fmt(const cmm_euler *e) { return e->stack_s ? kStackRec48 : kStackRec64; }
 
 int sync_planes(cmm_euler *e) {
     if (e->planes_fresh) return CMM_OK;
@@ -733,7 +735,7 @@ int cmm_euler_create(const cmm_euler_params *p, cmm_euler **out) {
     if (!rc && sp) rc = falloc(&e->m32[1], 12 * e->nc);
     if (!rc && sp && p->sim == CMM_SIM_EULER_2D) rc = falloc(&e->W4f, 4 * e->nf);
     if (!rc && p->stack_capacity > 0) rc = dalloc(&e->stack, (size_t)8 * e->nc * p->stack_capacity);
-    if (!rc && sp && p->stack_capacity > 0) rc = dalloc(&e->stack_s, (size_t)5 * e->nc * p->stack_capacity);
+    if (!rc && sp && p->stack_capacity > 0) rc = dalloc(&e->stack_s, (size_t)6 * e->nc * p->stack_capacity);
     if (p->forward_map) {
         if (!rc) rc = dalloc(&e->fx, 4 * e->nc);
         if (!rc) rc = dalloc(&e->fy, 4 * e->nc);
@@ -942,8 +944,21 @@ int cmm_euler_restart(cmm_euler *e, double t_restart, int n_maps, const double *
     // readMapStack (Cuda-CMM/src/simulation/cmm-simulation-host.cu:1255-1302): the stacked maps, the
     // initial condition translated through them with the active map still the identity, then the
     // active map and the older psi levels
-    int rc = n_maps > 0 ? put(e->stack, stack_host, (size_t)8 * e->nc * n_maps) : CMM_OK;
-    if (!rc && e->stack_s) rc = op_stack_pack(p.g_coarse, e->stack, n_maps, e->stack_s, e->st);
+    // each map goes through a device buffer into the stack's node records (and the split records)
+    int rc = CMM_OK;
+    if (n_maps > 0) {
+        double *tmp = nullptr;
+        rc = dalloc(&tmp, 8 * (size_t)e->nc);
+        for (int m = 0; m < n_maps && !rc; ++m) {
+            rc = put(tmp, stack_host + (size_t)8 * e->nc * m, (size_t)8 * e->nc);
+            if (!rc) rc = op_pack_maps(p.g_coarse, tmp, tmp + 4 * e->nc, 0, 1, kStackRec64, e->stack + (size_t)8 * e->nc * m, e->st);
+            if (!rc && e->stack_s)
+                rc = op_pack_maps(p.g_coarse, tmp, tmp + 4 * e->nc, 0, 1, kStackRec48, e->stack_s + (size_t)6 * e->nc * m, e->st);
+            // the buffer is reused for the next map after these kernels read it
+            if (!rc) rc = hip_check(hipStreamSynchronize(e->st), "hipStreamSynchronize");
+        }
+        if (tmp) (void)hipFree(tmp);
+    }
     e->nstack = n_maps;
     if (!rc) rc = op_map_identity(p.g_coarse, e->cx, e->cy, e->st);
     if (!rc && p.forward_map) rc = op_map_identity(p.g_coarse, e->fx, e->fy, e->st);
@@ -971,13 +986,18 @@ int cmm_euler_get_stack_map(cmm_euler *e, int m, double *chix_host, double *chiy
         set_error("cmm_euler_get_stack_map: bad argument");
         return CMM_ERR_ARG;
     }
+    // the stack holds node records (8 doubles per node): copied out, then split into the planes
     const double *slot = e->stack + (size_t)8 * e->nc * m;
-    int rc = hip_check(hipMemcpyAsync(chix_host, slot, sizeof(double) * 4 * e->nc, hipMemcpyDeviceToHost, e->st), "get");
-    if (!rc)
-        rc = hip_check(hipMemcpyAsync(chiy_host, slot + 4 * e->nc, sizeof(double) * 4 * e->nc, hipMemcpyDeviceToHost, e->st),
-                       "get");
+    std::vector<double> rec(8 * (size_t)e->nc);
+    int rc = hip_check(hipMemcpyAsync(rec.data(), slot, sizeof(double) * 8 * e->nc, hipMemcpyDeviceToHost, e->st), "get");
+    if (!rc) rc = hip_check(hipStreamSynchronize(e->st), "hipStreamSynchronize");
     if (rc) return rc;
-    return hip_check(hipStreamSynchronize(e->st), "hipStreamSynchronize");
+    for (long i = 0; i < e->nc; ++i)
+        for (int q = 0; q < 4; ++q) {
+            chix_host[q * e->nc + i] = rec[8 * i + q];
+            chiy_host[q * e->nc + i] = rec[8 * i + 4 + q];
+        }
+    return CMM_OK;
 }
 
 int cmm_euler_restart_forward(cmm_euler *e, int n_maps, const double *stack_host, const double *chix_host,
@@ -1113,12 +1133,12 @@ int step_finish(cmm_euler *e, cmm_step_info *info, bool waited) {
         } else {
             prof_mark(e, 3);
             if (int rc = engine_translate(e)) return rc;
-            double *slotx = e->stack + (size_t)8 * e->nc * e->nstack;
-            if (int rc = hip_check(hipMemcpyAsync(slotx, e->cx, sizeof(double) * 4 * e->nc, hipMemcpyDeviceToDevice, e->st), "push map")) return rc;
-            if (int rc = hip_check(hipMemcpyAsync(slotx + 4 * e->nc, e->cy, sizeof(double) * 4 * e->nc, hipMemcpyDeviceToDevice, e->st), "push map")) return rc;
-            if (e->stack_s) {
-                if (int rc = op_stack_pack(p.g_coarse, slotx, 1, e->stack_s + (size_t)5 * e->nc * e->nstack, e->st)) return rc;
-            }
+            // the pushed map as node records (and split records for the split form's compositions)
+            if (int rc = op_pack_maps(p.g_coarse, e->cx, e->cy, 0, 1, kStackRec64, e->stack + (size_t)8 * e->nc * e->nstack, e->st))
+                return rc;
+            if (e->stack_s)
+                if (int rc = op_pack_maps(p.g_coarse, e->cx, e->cy, 0, 1, kStackRec48, e->stack_s + (size_t)6 * e->nc * e->nstack, e->st))
+                    return rc;
             if (p.forward_map) {  // :847-851
                 double *fslot = e->fstack + (size_t)8 * e->nc * e->nstack;
                 if (int rc = hip_check(hipMemcpyAsync(fslot, e->fx, sizeof(double) * 4 * e->nc, hipMemcpyDeviceToDevice, e->st), "push forward map")) return rc;

@@ -57,18 +57,19 @@ int op_hermite_from_real(const cmm_grid &g, const double *real, double *herm, vo
                          hipStream_t st);
 int op_hermite_from_spectrum(const cmm_grid &g, const double *spec_cols, double *herm, void *work,
                              size_t work_bytes, hipStream_t st);
-// Stack formats: 0 = eight FP64 planes per map (ChiX planes, then ChiY planes), map m at
-// stack + 8*Nc*m; 1 = the split stack of op_stack_pack, map m at byte 40*Nc*m.
-constexpr int kStackFp64 = 0, kStackSplit = 1;
+// Stack formats: kStackFp64 = eight FP64 planes per map (ChiX planes, then ChiY planes), map m at
+// stack + 8*Nc*m; kStackRec64 / kStackRec48 = node records of op_pack_maps (64 bytes FP64; 48 bytes
+// with FP32 derivatives), map m at byte m*Nc*64 / m*Nc*48.  The values equal CMM_STACK_RECORDS_*.
+constexpr int kStackFp64 = 0, kStackRec64 = 1, kStackRec48 = 2;
 int op_translate_ic(const cmm_grid &gm, const cmm_grid &gf, const double *cx, const double *cy, const double *stack,
                     int nstack, int ic, const double *prm, int discrete, const cmm_grid &gd, const double *disc,
                     double *herm, void *work, size_t work_bytes, hipStream_t st, int stack_fmt = kStackFp64);
 int op_compose_positions(const cmm_grid &gm, const cmm_grid &gt, const double *cx, const double *cy,
                          const double *stack, int nstack, double *xy, hipStream_t st, int stack_fmt = kStackFp64);
-// n_maps FP64 stacked maps (format 0) into the split stack (format 1): per map the X and Y value
-// planes in FP64, then the six derivative planes dX/dx, dX/dy, d2X/dxdy, dY/dx, dY/dy, d2Y/dxdy
-// rounded to FP32 (40 bytes per node)
-int op_stack_pack(const cmm_grid &gm, const double *stack, int n_maps, void *out, hipStream_t st);
+// n_maps maps read plane-major from (cx + m*stride, cy + m*stride) into node records of format fmt
+// (kStackRec64 or kStackRec48) at out
+int op_pack_maps(const cmm_grid &gm, const double *cx, const double *cy, long stride, int n_maps, int fmt, void *out,
+                 hipStream_t st);
 int op_forward_positions(const cmm_grid &gm, const cmm_grid &gt, const double *cx, const double *cy,
                          const double *stack, int nstack, double *xy, hipStream_t st);
 int op_forward_points(const cmm_grid &gm, const cmm_grid &gb, const double *stack, int first, int nstack,

@@ -784,61 +784,72 @@ using SampleVort = SampleVortT<false>;
 using SampleVortIL = SampleVortT<true>;
 
 
-// diffeo() on one map of the split stack (op_stack_pack): FP64 value planes V[0..N), V[N..2N),
-// FP32 derivative planes after them.  The gathers return 160 bytes per evaluation instead of 256,
-// and the composition at depth is bound by the bytes its gathers return per lane.
-__device__ __forceinline__ void diffeo_split_stack(const DGrid &g, const char *__restrict__ M, double x, double y,
-                                                   double &xo, double &yo) {
-    const long N = g.n;
-    const double *__restrict__ V = reinterpret_cast<const double *>(M);
-    const float *__restrict__ D = reinterpret_cast<const float *>(M + 16 * N);
-    Cell c;
-    find_cell(g, x, y, c);
-    const int id[4] = {c.i00, c.i10, c.i01, c.i11};
-    double vx[16], vy[16];
+// Stacked maps as node records (op_pack_maps): one record per coarse node, so that a map
+// evaluation gathers 4 records, 2 to 4 cache lines, instead of 32 values from 8 planes.
+//   kStackRec64: 64 bytes, the ChiX planes then the ChiY planes (the advection's interleaved map);
+//   kStackRec48: 48 bytes, the X and Y values in FP64, then dX/dx, dX/dy, d2X/dxdy, dY/dx, dY/dy,
+//                d2Y/dxdy in FP32 and 8 unused bytes (split-precision engines).
+template <int FMT>
+__device__ __forceinline__ void load_map_rec(const char *__restrict__ M, const Cell &c, double vx[16], double vy[16]) {
+    if constexpr (FMT == kStackRec64) {
+        load_map_il(reinterpret_cast<const double *>(M), c, vx, vy);
+    } else {
+        const int idx[4] = {c.i00, c.i10, c.i01, c.i11};
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        vx[q] = V[id[q]];
-        vy[q] = V[N + id[q]];
-        vx[4 + q] = D[id[q]];
-        vx[8 + q] = D[N + id[q]];
-        vx[12 + q] = D[2 * N + id[q]];
-        vy[4 + q] = D[3 * N + id[q]];
-        vy[8 + q] = D[4 * N + id[q]];
-        vy[12 + q] = D[5 * N + id[q]];
+        for (int q = 0; q < 4; ++q) {
+            const char *r = M + 48 * (long)idx[q];
+            const double2 v = *reinterpret_cast<const double2 *>(r);
+            const float4 d = *reinterpret_cast<const float4 *>(r + 16);
+            const float2 e = *reinterpret_cast<const float2 *>(r + 32);
+            vx[q] = v.x;
+            vy[q] = v.y;
+            vx[4 + q] = d.x;
+            vx[8 + q] = d.y;
+            vx[12 + q] = d.z;
+            vy[4 + q] = d.w;
+            vy[8 + q] = e.x;
+            vy[12 + q] = e.y;
+        }
     }
-    diffeo_vals(g, c, vx, vy, xo, yo);
 }
 
-__global__ void __launch_bounds__(256) k_stack_pack(const double *__restrict__ stack, char *__restrict__ out, long N,
-                                                    int n_maps) {
+// bytes per node of a stack format
+__host__ __device__ constexpr long stack_rec_bytes(int fmt) { return fmt == kStackRec48 ? 48 : 64; }
+
+// maps m < n_maps read plane-major from (cx + m stride, cy + m stride) into records of format fmt at
+// out + m N rec_bytes
+__global__ void __launch_bounds__(256) k_pack_maps(const double *__restrict__ cx, const double *__restrict__ cy,
+                                                   long stride, char *__restrict__ out, long N, int n_maps, int fmt) {
     const long t = (long)blockIdx.x * 256 + threadIdx.x;
     if (t >= N * n_maps) return;
     const long m = t / N, i = t % N;
-    const double *S = stack + 8 * N * m;
-    double *V = reinterpret_cast<double *>(out + 40 * N * m);
-    float *D = reinterpret_cast<float *>(out + 40 * N * m + 16 * N);
-    V[i] = S[i];
-    V[N + i] = S[4 * N + i];
-    D[i] = (float)S[N + i];
-    D[N + i] = (float)S[2 * N + i];
-    D[2 * N + i] = (float)S[3 * N + i];
-    D[3 * N + i] = (float)S[5 * N + i];
-    D[4 * N + i] = (float)S[6 * N + i];
-    D[5 * N + i] = (float)S[7 * N + i];
+    const double *X = cx + stride * m, *Y = cy + stride * m;
+    char *r = out + (m * N + i) * stack_rec_bytes(fmt);
+    if (fmt == kStackRec48) {
+        *reinterpret_cast<double2 *>(r) = double2{X[i], Y[i]};
+        *reinterpret_cast<float4 *>(r + 16) = float4{(float)X[N + i], (float)X[2 * N + i], (float)X[3 * N + i], (float)Y[N + i]};
+        *reinterpret_cast<float4 *>(r + 32) = float4{(float)Y[2 * N + i], (float)Y[3 * N + i], 0.0f, 0.0f};
+    } else {
+        double2 *d = reinterpret_cast<double2 *>(r);
+        d[0] = double2{X[i], X[N + i]};
+        d[1] = double2{X[2 * N + i], X[3 * N + i]};
+        d[2] = double2{Y[i], Y[N + i]};
+        d[3] = double2{Y[2 * N + i], Y[3 * N + i]};
+    }
 }
 
 // Initial condition at X_1(...X_M(X_cur(x))): the current map then the stack from newest to
 // oldest, wrapping after each layer (Cuda-CMM/src/simulation/cmm-simulation-host.cu:112-152,
 // Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:302-373).  All layers are applied in
-// registers; the stack is device resident, map m at stack + 8*Nc*m (X planes, then Y planes).
+// registers; the stack is device resident, map m at stack + 8*Nc*m (X planes, then Y planes) or,
+// as node records, at byte m*Nc*rec_bytes.
 struct SampleCompose {
     DGrid gm, gt, gd;
     const double *__restrict__ cx;
     const double *__restrict__ cy;
     const double *__restrict__ stack;
     int nstack;
-    int fmt;  // stack format: kStackFp64 or kStackSplit (the backward position() only)
+    int fmt;  // stack format: kStackFp64, kStackRec64 or kStackRec48 (the backward position() only)
     ICParams ic;
     int discrete;
     const double *__restrict__ disc;
@@ -850,12 +861,18 @@ struct SampleCompose {
         x = px - floor((px - gm.x0) / gm.lx) * gm.lx;
         y = py - floor((py - gm.y0) / gm.ly) * gm.ly;
         const long mstride = 8l * gm.n;
+        const char *recs = reinterpret_cast<const char *>(stack);
         for (int m = nstack - 1; m >= 0; --m) {
-            if (fmt == kStackSplit) {
-                diffeo_split_stack(gm, reinterpret_cast<const char *>(stack) + 40l * gm.n * m, x, y, px, py);
-            } else {
+            if (fmt == kStackFp64) {
                 const double *sx = stack + mstride * m;
                 diffeo(gm, sx, sx + 4l * gm.n, x, y, px, py);
+            } else {
+                Cell c;
+                find_cell(gm, x, y, c);
+                double vx[16], vy[16];
+                if (fmt == kStackRec48) load_map_rec<kStackRec48>(recs + 48l * gm.n * m, c, vx, vy);
+                else load_map_rec<kStackRec64>(recs + 64l * gm.n * m, c, vx, vy);
+                diffeo_vals(gm, c, vx, vy, px, py);
             }
             x = px - floor((px - gm.x0) / gm.lx) * gm.lx;
             y = py - floor((py - gm.y0) / gm.ly) * gm.ly;
@@ -2186,12 +2203,17 @@ size_t op_translate_workspace(const cmm_grid &g) {
     return spec_workspace_bytes(g, g, 0, 0.0) + sizeof(double) * (size_t)g.nx * g.ny;
 }
 
-int op_stack_pack(const cmm_grid &gm, const double *stack, int n_maps, void *out, hipStream_t st) {
+int op_pack_maps(const cmm_grid &gm, const double *cx, const double *cy, long stride, int n_maps, int fmt, void *out,
+                 hipStream_t st) {
     if (n_maps <= 0) return CMM_OK;
+    if (fmt != kStackRec64 && fmt != kStackRec48) {
+        set_error("stack records: unknown format");
+        return CMM_ERR_ARG;
+    }
     const long N = (long)gm.nx * gm.ny;
-    hipLaunchKernelGGL(k_stack_pack, dim3((unsigned)((N * n_maps + 255) / 256)), dim3(256), 0, st, stack,
-                       static_cast<char *>(out), N, n_maps);
-    return launch_check("k_stack_pack");
+    hipLaunchKernelGGL(k_pack_maps, dim3((unsigned)((N * n_maps + 255) / 256)), dim3(256), 0, st, cx, cy, stride,
+                       static_cast<char *>(out), N, n_maps, fmt);
+    return launch_check("k_pack_maps");
 }
 
 int op_compose_positions(const cmm_grid &gm, const cmm_grid &gt, const double *cx, const double *cy,
@@ -2209,7 +2231,8 @@ int op_compose_positions(const cmm_grid &gm, const cmm_grid &gt, const double *c
     s.discrete = 0;
     s.disc = nullptr;
     dim3 b(64, 4);
-    if (stack_fmt == kStackSplit) hipLaunchKernelGGL(k_compose_positions<kStackSplit>, grid2d(gt.nx, gt.ny, b), b, 0, st, s, xy);
+    if (stack_fmt == kStackRec48) hipLaunchKernelGGL(k_compose_positions<kStackRec48>, grid2d(gt.nx, gt.ny, b), b, 0, st, s, xy);
+    else if (stack_fmt == kStackRec64) hipLaunchKernelGGL(k_compose_positions<kStackRec64>, grid2d(gt.nx, gt.ny, b), b, 0, st, s, xy);
     else hipLaunchKernelGGL(k_compose_positions<kStackFp64>, grid2d(gt.nx, gt.ny, b), b, 0, st, s, xy);
     return launch_check("k_compose_positions");
 }

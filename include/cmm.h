@@ -75,17 +75,22 @@ int cmm_invert_error(cmm_grid g_back, cmm_grid g_fwd, cmm_grid g_check, const do
  * (Cuda-CMM/src/simulation/cmm-simulation-host.cu:112-152). xy: 2*Nt interleaved. */
 int cmm_compose_positions(cmm_grid g_map, cmm_grid g_target, const double *chix, const double *chiy,
                           const double *stack, int n_stack, double *xy, void *stream);
-/* The split stack: n_maps stacked maps (layout above) repacked with the X and Y value planes in
- * FP64 and the six derivative planes (dX/dx, dX/dy, d2X/dxdy, dY/dx, dY/dy, d2Y/dxdy) in FP32, 40
- * bytes per node, map m at byte 40*Nc*m of out.  Split-precision engines keep their stack in this
- * form as well.  No counterpart in the reference (its stack is FP64 only). */
-int cmm_stack_pack_split(cmm_grid g_map, const double *stack, int n_maps, void *out, void *stream);
-/* cmm_compose_positions through a split stack: the gathers return 160 instead of 256 bytes per
- * point and layer.  Positions differ from the FP64 stack's by the FP32 rounding of the derivative
- * planes: per layer at most h/4 (|dX/dx| + |dX/dy|) 2^-24 + h^2/16 |d2X/dxdy| 2^-24 (likewise Y),
- * carried through the later layers by their gradients. */
-int cmm_compose_positions_split(cmm_grid g_map, cmm_grid g_target, const double *chix, const double *chiy,
-                                const void *stack_split, int n_stack, double *xy, void *stream);
+/* Stacked maps repacked as node records, the form the engine keeps its stack in: a map evaluation
+ * then gathers four records (two to four cache lines) instead of 32 values from eight planes.  No
+ * counterpart in the reference (its MapStack is plane-major FP64, the layout above).
+ *   CMM_STACK_RECORDS_FP64:  64 bytes per node, ChiX planes then ChiY planes; map m at byte 64*Nc*m.
+ *   CMM_STACK_RECORDS_SPLIT: 48 bytes per node, X and Y values in FP64, then dX/dx, dX/dy, d2X/dxdy,
+ *     dY/dx, dY/dy, d2Y/dxdy in FP32 and 8 unused bytes; map m at byte 48*Nc*m (split-precision
+ *     engines).  Positions differ from the FP64 stack's by the FP32 rounding of the derivative
+ *     planes: per layer at most h/4 (|dX/dx| + |dX/dy|) 2^-24 + h^2/16 |d2X/dxdy| 2^-24 (likewise Y),
+ *     carried through the later layers by their gradients. */
+#define CMM_STACK_RECORDS_FP64 1
+#define CMM_STACK_RECORDS_SPLIT 2
+int cmm_stack_pack(cmm_grid g_map, const double *stack, int n_maps, int format, void *packed, void *stream);
+/* cmm_compose_positions through a stack packed by cmm_stack_pack; with CMM_STACK_RECORDS_FP64 the
+ * positions equal cmm_compose_positions' bit for bit. */
+int cmm_compose_positions_packed(cmm_grid g_map, cmm_grid g_target, const double *chix, const double *chiy,
+                                 const void *packed, int n_stack, int format, double *xy, void *stream);
 /* Forward positions X_cur(X_{M-1}(...X_0(x))) on g_target through a stack of forward maps (same
  * layout), apply_map_stack with direction 1 (Cuda-CMM/src/simulation/cmm-simulation-host.cu:128-148).
  * xy: 2*Nt interleaved. */

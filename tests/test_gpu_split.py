@@ -218,7 +218,8 @@ def test_split_engine_double_shear_layer(dev):
 
 
 def test_split_stack_composition(dev):
-    """positions through a stack of sheared maps, FP64 stack against the split stack"""
+    """positions through a stack of sheared maps: the plane-major FP64 stack against its FP64 node
+    records (bit for bit) and its split records (within the FP32 rounding bound)"""
     import cmm
     import torch
     nc, nf, nm = 128, 512, 12
@@ -247,25 +248,34 @@ def test_split_stack_composition(dev):
     # the layers run newest first: the error made in layer m is carried by layers m-1 .. 0
     tot = sum(bound[m] * float(np.prod(gains[:m])) for m in range(nm))
     stack = torch.from_numpy(st.ravel()).cuda()
-    packed = torch.zeros(nm * 5 * N, dtype=torch.float64, device="cuda")
-    cmm.stack_pack_split(gm, stack, nm, packed)
-    pb = packed.cpu().numpy().view(np.uint8).reshape(nm, 40 * N)
+    # both record formats: their bytes against the planes, then the compositions
+    packed = torch.zeros(nm * 6 * N, dtype=torch.float64, device="cuda")
+    cmm.stack_pack(gm, stack, nm, "split", packed)
+    rec64 = torch.zeros(nm * 8 * N, dtype=torch.float64, device="cuda")
+    cmm.stack_pack(gm, stack, nm, "fp64", rec64)
+    pb = packed.cpu().numpy().view(np.uint8).reshape(nm, N, 48)
+    r64 = rec64.cpu().numpy().reshape(nm, N, 8)
     for m in range(nm):
-        v = pb[m, :16 * N].view(np.float64).reshape(2, N)
-        d = pb[m, 16 * N:].view(np.float32).reshape(6, N)
-        assert np.array_equal(v[0], st[m, 0]) and np.array_equal(v[1], st[m, 4])
-        assert np.array_equal(d, st[m, [1, 2, 3, 5, 6, 7]].astype(np.float32))
+        v = pb[m, :, :16].copy().view(np.float64)
+        d = pb[m, :, 16:40].copy().view(np.float32)
+        assert np.array_equal(v[:, 0], st[m, 0]) and np.array_equal(v[:, 1], st[m, 4])
+        assert np.array_equal(d, st[m, [1, 2, 3, 5, 6, 7]].T.astype(np.float32))
+        assert np.array_equal(r64[m], st[m].T)
     cx = torch.empty(4 * N, dtype=torch.float64, device="cuda")
     cy = torch.empty_like(cx)
     cmm.map_init_identity(gm, cx, cy)
     xy64 = torch.empty(2 * nf * nf, dtype=torch.float64, device="cuda")
     xys = torch.empty_like(xy64)
+    xyr = torch.empty_like(xy64)
     cmm.apply_map_stack(gt, gm, cx, cy, stack, nm, xy64)
-    cmm.apply_map_stack_split(gt, gm, cx, cy, packed, nm, xys)
+    cmm.apply_map_stack_packed(gt, gm, cx, cy, packed, nm, "split", xys)
+    cmm.apply_map_stack_packed(gt, gm, cx, cy, rec64, nm, "fp64", xyr)
+    # FP64 records: the same values in the same arithmetic, so the same positions bit for bit
+    assert torch.equal(xyr, xy64)
     d = np.abs(xys.cpu().numpy() - xy64.cpu().numpy())
     d = np.minimum(d, O.TWO_PI - d)
     print(f"split stack composition, {nm} maps: max position difference {d.max():.3e}, bound {tot:.3e}")
     assert d.max() <= tot
-    cmm.apply_map_stack_split(gt, gm, cx, cy, packed, 0, xys)
+    cmm.apply_map_stack_packed(gt, gm, cx, cy, packed, 0, "split", xys)
     cmm.apply_map_stack(gt, gm, cx, cy, stack, 0, xy64)
     assert torch.equal(xys, xy64)
