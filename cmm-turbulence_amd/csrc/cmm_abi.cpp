@@ -373,6 +373,13 @@ struct cmm_euler {
     // weights follow the whole time history and a graph's kernel arguments are frozen
     bool lw_graph = false;
     double *lw_dev = nullptr, *lw_host = nullptr;
+    // Vlasov-Poisson: H(v) and H'(v) of the bump function (they depend on the grid only), and, in
+    // captured batch steps (vp_defer), the phi solve described in vp_job for one batched launch
+    // with the planes of psi written afterwards into vp_out
+    double *vpH = nullptr;
+    bool vp_defer = false;
+    VPJob vp_job{};
+    double *vp_out = nullptr;
     IncompTune tune;  // the check's tile and grid, from cmm_euler_autotune
     // corrections (device Hermite fields on g_psi) added to the Lagrange extrapolations of psi
     // to t_{n+1} and t_n + dt/2 in the next steps (cmm_euler_set_psi_correction)
@@ -487,7 +494,9 @@ int engine_psi(cmm_euler *e, double *out, hipEvent_t *ev = nullptr) {
         if (int rc = sync_planes(e)) return rc;
         if (ev) (void)hipEventRecord(ev[0], e->st);
         int rc = op_vp_psi_from_f(e->p.g_coarse, e->p.g_vort, e->p.g_fine, e->p.g_psi, e->cx, e->cy, e->W,
-                                  e->p.molly_stencil, out, e->work, e->work_bytes, e->st);
+                                  e->p.molly_stencil, out, e->work, e->work_bytes, e->st, e->W4, e->vpH,
+                                  e->vp_defer ? &e->vp_job : nullptr);
+        if (e->vp_defer) e->vp_out = out;
         if (ev)
             for (int i = 1; i < 4; ++i) (void)hipEventRecord(ev[i], e->st);
         return rc;
@@ -730,7 +739,13 @@ int cmm_euler_create(const cmm_euler_params *p, cmm_euler **out) {
     const bool sp = p->precision == CMM_PRECISION_SPLIT;
     if (!rc && p->l_order >= 2 && !sp) rc = dalloc(&e->m8[0], 8 * e->nc);
     if (!rc && p->l_order >= 2 && !sp) rc = dalloc(&e->m8[1], 8 * e->nc);
-    if (!rc && p->sim == CMM_SIM_EULER_2D && !sp) rc = dalloc(&e->W4, 4 * e->nf);
+    // the fine field node-interleaved for the column samplers (the Vlasov-Poisson flow samples FP64
+    // records in both forms)
+    if (!rc && (!sp || p->sim == CMM_SIM_VLASOV_POISSON_1D)) rc = dalloc(&e->W4, 4 * e->nf);
+    if (!rc && p->sim == CMM_SIM_VLASOV_POISSON_1D) {
+        rc = dalloc(&e->vpH, 3 * (size_t)p->g_psi.nx);
+        if (!rc) rc = op_vp_bump(p->g_vort, p->g_psi, e->vpH, e->st);
+    }
     if (!rc && sp) rc = falloc(&e->m32[0], 12 * e->nc);
     if (!rc && sp) rc = falloc(&e->m32[1], 12 * e->nc);
     if (!rc && sp && p->sim == CMM_SIM_EULER_2D) rc = falloc(&e->W4f, 4 * e->nf);
@@ -784,7 +799,7 @@ int cmm_euler_create(const cmm_euler_params *p, cmm_euler **out) {
 int cmm_euler_destroy(cmm_euler *e) {
     if (!e) return CMM_OK;
     if (e->st) (void)hipStreamSynchronize(e->st);
-    double *bufs[] = {e->cx, e->cy, e->ncx, e->ncy, e->W, e->W4, e->m8[0], e->m8[1], e->sbuf, e->disc, e->stack, e->tmp, e->tmp4,
+    double *bufs[] = {e->cx, e->cy, e->ncx, e->ncy, e->W, e->W4, e->vpH, e->m8[0], e->m8[1], e->sbuf, e->disc, e->stack, e->tmp, e->tmp4,
                       e->psi_buf[0], e->psi_buf[1], e->psi_buf[2], e->psi_buf[3], e->fx, e->fy, e->nfx, e->nfy, e->fstack, e->stack_s};
     for (double *b : bufs)
         if (b) (void)hipFree(b);
@@ -1237,6 +1252,13 @@ struct cmm_batch {
     // the step it was captured on
     std::map<std::string, hipGraphExec_t> cache;
     int captured = 0, launched = 0, eager = 0;
+    // Vlasov-Poisson instances: their phi solves in one launch per captured step (one workgroup
+    // each), between a second fork and join of the engine streams
+    hipEvent_t fork2 = nullptr;
+    std::vector<hipEvent_t> join2;
+    VPJob *jobs_dev = nullptr;
+    std::vector<VPJob> jobs_up;  // what jobs_dev holds (once jobs_uploaded)
+    bool jobs_uploaded = false;
 };
 
 namespace {
@@ -1264,18 +1286,63 @@ int batch_normalise(cmm_euler *e) {
     return CMM_OK;
 }
 
+bool is_vp(const cmm_euler *e) { return e->p.sim == CMM_SIM_VLASOV_POISSON_1D; }
+
 int batch_capture(cmm_batch *b, double dt, hipGraphExec_t *out) {
     int rc = hip_check(hipStreamBeginCapture(b->master, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
     if (rc) return rc;
     rc = hip_check(hipEventRecord(b->fork, b->master), "hipEventRecord");
+    std::vector<VPJob> jobs;
+    int max_n = 0;
     for (size_t i = 0; i < b->e.size() && !rc; ++i) {
         cmm_euler *e = b->e[i];
         rc = hip_check(hipStreamWaitEvent(e->st, b->fork, 0), "hipStreamWaitEvent");
         e->capturing = true;
+        e->vp_defer = is_vp(e);
+        e->vp_out = nullptr;
         if (!rc) rc = step_queue(e, dt);
         e->capturing = false;
+        e->vp_defer = false;
+        if (!rc && is_vp(e)) {
+            if (!e->vp_out) { set_error("cmm_batch_step: psi solve not deferred"); rc = CMM_ERR_ARG; }
+            jobs.push_back(e->vp_job);
+            max_n = std::max({max_n, e->vp_job.q.nv, e->vp_job.q.np});
+        }
         if (!rc) rc = hip_check(hipEventRecord(b->join[i], e->st), "hipEventRecord");
         if (!rc) rc = hip_check(hipStreamWaitEvent(b->master, b->join[i], 0), "hipStreamWaitEvent");
+    }
+    // the phi solves of all Vlasov-Poisson instances in one launch, then each instance's psi planes
+    // on its own stream again.  The job table is the same at every step (workspace and twiddle
+    // addresses); it is uploaded before the first launch of the graph
+    if (!rc && !jobs.empty()) {
+        if (!b->jobs_dev || b->jobs_up.size() != jobs.size()) {
+            if (b->jobs_dev) (void)hipFree(b->jobs_dev);
+            b->jobs_dev = nullptr;
+            b->jobs_up.clear();
+            if (hipMalloc(&b->jobs_dev, sizeof(VPJob) * jobs.size()) != hipSuccess) {
+                b->jobs_dev = nullptr;
+                set_error("hipMalloc batch jobs");
+                rc = CMM_ERR_ALLOC;
+            }
+        }
+        if (!rc) rc = op_vp_phi_batch(b->jobs_dev, (int)jobs.size(), max_n, b->master);
+        if (!rc) rc = hip_check(hipEventRecord(b->fork2, b->master), "hipEventRecord");
+        for (size_t i = 0; i < b->e.size() && !rc; ++i) {
+            cmm_euler *e = b->e[i];
+            if (!is_vp(e)) continue;
+            rc = hip_check(hipStreamWaitEvent(e->st, b->fork2, 0), "hipStreamWaitEvent");
+            if (!rc) rc = op_vp_psi_planes(e->p.g_psi, e->vp_job.phi, e->vpH, e->vp_out, e->st);
+            if (!rc) rc = hip_check(hipEventRecord(b->join2[i], e->st), "hipEventRecord");
+            if (!rc) rc = hip_check(hipStreamWaitEvent(b->master, b->join2[i], 0), "hipStreamWaitEvent");
+        }
+    }
+    if (!rc && !jobs.empty()) {
+        bool same = b->jobs_up.size() == jobs.size() &&
+                    std::memcmp(b->jobs_up.data(), jobs.data(), sizeof(VPJob) * jobs.size()) == 0;
+        if (!same) {
+            b->jobs_up = jobs;
+            b->jobs_uploaded = false;
+        }
     }
     hipGraph_t g = nullptr;
     const int rc_end = hip_check(hipStreamEndCapture(b->master, &g), "hipStreamEndCapture");
@@ -1305,7 +1372,10 @@ int cmm_batch_create(cmm_euler *const *engines, int n, int use_graphs, cmm_batch
     int rc = hip_check(hipStreamCreateWithFlags(&b->master, hipStreamNonBlocking), "hipStreamCreate");
     if (!rc) rc = hip_check(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "hipEventCreate");
     b->join.assign(n, nullptr);
+    b->join2.assign(n, nullptr);
     for (int i = 0; i < n && !rc; ++i) rc = hip_check(hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming), "hipEventCreate");
+    for (int i = 0; i < n && !rc; ++i) rc = hip_check(hipEventCreateWithFlags(&b->join2[i], hipEventDisableTiming), "hipEventCreate");
+    if (!rc) rc = hip_check(hipEventCreateWithFlags(&b->fork2, hipEventDisableTiming), "hipEventCreate");
     if (rc) { cmm_batch_destroy(b); return rc; }
     *out = b;
     return CMM_OK;
@@ -1317,6 +1387,10 @@ int cmm_batch_destroy(cmm_batch *b) {
     for (auto &kv : b->cache) (void)hipGraphExecDestroy(kv.second);
     for (hipEvent_t ev : b->join)
         if (ev) (void)hipEventDestroy(ev);
+    for (hipEvent_t ev : b->join2)
+        if (ev) (void)hipEventDestroy(ev);
+    if (b->fork2) (void)hipEventDestroy(b->fork2);
+    if (b->jobs_dev) (void)hipFree(b->jobs_dev);
     if (b->fork) (void)hipEventDestroy(b->fork);
     if (b->master) (void)hipStreamDestroy(b->master);
     delete b;
@@ -1356,6 +1430,16 @@ int cmm_batch_step(cmm_batch *b, double dt, cmm_step_info *infos) {
         }
         for (cmm_euler *e : b->e) e->lw_graph = false;
         if (rc_q) return rc_q;
+        // the phi job table (written at capture, the same for every step): uploaded in stream order
+        // before the graph's first use of it
+        if (!b->jobs_up.empty() && !b->jobs_uploaded) {
+            if (int rc = hip_check(hipMemcpyAsync(b->jobs_dev, b->jobs_up.data(), sizeof(VPJob) * b->jobs_up.size(),
+                                                  hipMemcpyHostToDevice, b->master),
+                                   "hipMemcpyAsync jobs"))
+                return rc;
+            if (int rc = hip_check(hipStreamSynchronize(b->master), "hipStreamSynchronize")) return rc;
+            b->jobs_uploaded = true;
+        }
         // this step's Lagrange weights, in stream order before the graph's kernels read them
         for (cmm_euler *e : b->e)
             if (int rc = hip_check(hipMemcpyAsync(e->lw_dev, e->lw_host, 8 * sizeof(double), hipMemcpyHostToDevice,

@@ -77,10 +77,39 @@ int op_forward_points(const cmm_grid &gm, const cmm_grid &gb, const double *stac
 int op_sample_init(const cmm_grid &gm, const cmm_grid &gt, const double *cx, const double *cy, const double *stack,
                    int nstack, int ic, const double *prm, int discrete, const cmm_grid &gd, const double *disc,
                    double *out, hipStream_t st, int stack_fmt = kStackFp64);
+// Vlasov-Poisson psi: the 1D problems of one instance
+struct VP1D {
+    int nv, lognv;       // vort grid along x
+    int np, lognp;       // psi grid along x (= along v)
+    int nchunks;
+    double hv;           // vort grid spacing along v
+    double fkx_v;        // 2 pi / (hx NX) of the vort grid
+    double fky_p;        // 2 pi / (hy NY) of the psi grid
+    double fkx_p;        // 2 pi / (hx NX) of the psi grid
+    double v0, hv_p, Lv; // psi grid v origin, spacing, length
+};
+// phi(x) of one instance from its column sums: f (the sampled field, row v0 read), part (chunk
+// sums), twiddles of the vort and psi grids; phi written at [0, np), phi' at [2 np, 3 np)
+struct VPJob {
+    VP1D q;
+    const double *f, *part;
+    const cd *tw_v, *tw_p;
+    double *phi;
+};
 size_t op_vp_psi_workspace(const cmm_grid &gv, const cmm_grid &gp);
+// W4 (optional): the fine field node-interleaved, for the column sampler; H (optional): the bump
+// function's H and H' (op_vp_bump), else computed into the workspace; defer (optional): the phi
+// solve is not launched but described in *defer for op_vp_phi_batch, and the planes are left to
+// op_vp_psi_planes
 int op_vp_psi_from_f(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf, const cmm_grid &gp, const double *cx,
                      const double *cy, const double *W, int molly, double *psi, void *work, size_t work_bytes,
-                     hipStream_t st);
+                     hipStream_t st, const double *W4 = nullptr, const double *H = nullptr, VPJob *defer = nullptr);
+// H(v) at [0, np) and H'(v) at [2 np, 3 np) of H (3 np doubles): the grid-only part of psi
+int op_vp_bump(const cmm_grid &gv, const cmm_grid &gp, double *H, hipStream_t st);
+int op_vp_psi_planes(const cmm_grid &gp, const double *phi, const double *H, double *psi, hipStream_t st);
+// phi of n_jobs instances (jobs_dev: device array of VPJob), one workgroup each; max_n the largest
+// vort or psi grid width among them
+int op_vp_phi_batch(const VPJob *jobs_dev, int n_jobs, int max_n, hipStream_t st);
 int op_spectral_from_real(const cmm_grid &g, const double *real, int op, double *herm, void *work, size_t work_bytes,
                           hipStream_t st);
 int op_sample_vort(const cmm_grid &gm, const cmm_grid &gt, const cmm_grid &gf, const double *cx, const double *cy,

@@ -853,30 +853,52 @@ struct SampleCompose {
     ICParams ic;
     int discrete;
     const double *__restrict__ disc;
-    __device__ void position(int ix, int iy, double &x, double &y) const {
+    // One layer: the map's value at (x, y), wrapped into [x0, x0 + L).  On power-of-two map grids the
+    // cell lookup takes its wrap counts by shifts, and the wrap after the layer multiplies by 1/L:
+    // floor((p - x0) / L) and floor((p - x0) (1/L)) differ only for a point within rounding of a
+    // period boundary, and then by one period, the same point of the torus.
+    template <bool P2>
+    __device__ __forceinline__ void layer(const double *planes, const char *rec, double ilx, double ily, double &x,
+                                          double &y) const {
+        Cell c;
+        find_cell<P2>(gm, x, y, c);
+        double vx[16], vy[16], px, py;
+        if (rec) {
+            if (fmt == kStackRec48) load_map_rec<kStackRec48>(rec, c, vx, vy);
+            else load_map_rec<kStackRec64>(rec, c, vx, vy);
+        } else {
+            load16(planes, gm.n, c, vx);
+            load16(planes + 4l * gm.n, gm.n, c, vy);
+        }
+        diffeo_vals(gm, c, vx, vy, px, py);
+        x = px - floor((px - gm.x0) * ilx) * gm.lx;
+        y = py - floor((py - gm.y0) * ily) * gm.ly;
+    }
+    template <bool P2>
+    __device__ __forceinline__ void position_t(int ix, int iy, double &x, double &y) const {
+        const double ilx = 1.0 / gm.lx, ily = 1.0 / gm.ly;
         x = gt.x0 + ix * gt.hx;
         y = gt.y0 + iy * gt.hy;
-        double px, py;
-        diffeo(gm, cx, cy, x, y, px, py);
-        x = px - floor((px - gm.x0) / gm.lx) * gm.lx;
-        y = py - floor((py - gm.y0) / gm.ly) * gm.ly;
-        const long mstride = 8l * gm.n;
+        // the current map (two plane-major fields)
+        {
+            Cell c;
+            find_cell<P2>(gm, x, y, c);
+            double vx[16], vy[16], px, py;
+            load16(cx, gm.n, c, vx);
+            load16(cy, gm.n, c, vy);
+            diffeo_vals(gm, c, vx, vy, px, py);
+            x = px - floor((px - gm.x0) * ilx) * gm.lx;
+            y = py - floor((py - gm.y0) * ily) * gm.ly;
+        }
         const char *recs = reinterpret_cast<const char *>(stack);
         for (int m = nstack - 1; m >= 0; --m) {
-            if (fmt == kStackFp64) {
-                const double *sx = stack + mstride * m;
-                diffeo(gm, sx, sx + 4l * gm.n, x, y, px, py);
-            } else {
-                Cell c;
-                find_cell(gm, x, y, c);
-                double vx[16], vy[16];
-                if (fmt == kStackRec48) load_map_rec<kStackRec48>(recs + 48l * gm.n * m, c, vx, vy);
-                else load_map_rec<kStackRec64>(recs + 64l * gm.n * m, c, vx, vy);
-                diffeo_vals(gm, c, vx, vy, px, py);
-            }
-            x = px - floor((px - gm.x0) / gm.lx) * gm.lx;
-            y = py - floor((py - gm.y0) / gm.ly) * gm.ly;
+            if (fmt == kStackFp64) layer<P2>(stack + 8l * gm.n * m, nullptr, ilx, ily, x, y);
+            else layer<P2>(nullptr, recs + stack_rec_bytes(fmt) * gm.n * m, ilx, ily, x, y);
         }
+    }
+    __device__ void position(int ix, int iy, double &x, double &y) const {
+        if (gm.lgx >= 0 && gm.lgy >= 0) position_t<true>(ix, iy, x, y);
+        else position_t<false>(ix, iy, x, y);
     }
     // forward order, Cuda-CMM/src/simulation/cmm-simulation-host.cu:128-148: the oldest stacked
     // forward map first, the current forward map last
@@ -907,9 +929,9 @@ struct SampleCompose {
 // row kernel.
 constexpr int kSampleRows = 4;
 
-template <class S, class T = double>
+// R: rows per lane (kSampleRows, or 2 where the vorticity grid refines the map grid only twice)
+template <class S, class T = double, int R = kSampleRows>
 __global__ void __launch_bounds__(256, 2) k_sample_field(S smp, int nx, int ny, T *__restrict__ f) {
-    constexpr int R = kSampleRows;
     const long t = (long)blockIdx.x * 256 + threadIdx.x;
     if (t >= (long)nx * (ny / R)) return;
     const int x = (int)(t % nx), y0 = R * (int)(t / nx);
@@ -1438,17 +1460,6 @@ __global__ void __launch_bounds__(256) k_vp_colsum(const double *__restrict__ f,
     part[(long)blockIdx.y * nx + x] = a;
 }
 
-struct VP1D {
-    int nv, lognv;       // vort grid along x
-    int np, lognp;       // psi grid along x (= along v)
-    int nchunks;
-    double hv;           // vort grid spacing along v
-    double fkx_v;        // 2 pi / (hx NX) of the vort grid
-    double fky_p;        // 2 pi / (hy NY) of the psi grid
-    double fkx_p;        // 2 pi / (hx NX) of the psi grid
-    double v0, hv_p, Lv; // psi grid v origin, spacing, length
-};
-
 // block-wide sum / max over kFftThreads lanes (fixed tree order)
 __device__ double block_reduce(double v, double *red, bool is_max) {
     red[threadIdx.x] = v;
@@ -1518,29 +1529,45 @@ __device__ void spec_deriv_1d(cd *s, int n, int logn, double fk, const cd *__res
 
 // two workgroups: phi and phi' (block 0), H and H' (block 1, independent of f), np values each,
 // into phiH = [phi, H, phi', H']
-__global__ void __launch_bounds__(kFftThreads) k_vp_1d(VP1D q, const double *__restrict__ f,
-                                                       const double *__restrict__ part, const cd *__restrict__ tw_v,
-                                                       const cd *__restrict__ tw_p, double *__restrict__ phiH) {
+// phi(x) and phi'(x) of one instance: I(x) = hv (f(x, v0) + sum_v f) - 1 (the row v0 once more,
+// then the chunk sums in order), the 1D Poisson solve and zero padding to the psi grid, then the
+// spectral derivative; into job.phi[0, np) and job.phi[2 np, 3 np)
+__device__ void vp_phi_block(const VPJob &job, cd *s) {
+    const VP1D &q = job.q;
+    for (int x = threadIdx.x; x < q.nv; x += kFftThreads) {
+        double a = job.f[x];
+        for (int c = 0; c < q.nchunks; ++c) a += job.part[(long)c * q.nv + x];
+        a *= q.hv;
+        a += -1.0;
+        s[sw(x)] = cd{a, 0.0};
+    }
+    __syncthreads();
+    wg_fft(s, q.nv, q.lognv, job.tw_v, false);
+    inv_lap_c2r(s, q.nv, q.np, q.lognp, 1.0 / (double)q.nv, q.fkx_v, job.tw_p);
+    for (int x = threadIdx.x; x < q.np; x += kFftThreads) job.phi[x] = s[sw(x)].x;
+    spec_deriv_1d(s, q.np, q.lognp, q.fkx_p, job.tw_p);
+    for (int x = threadIdx.x; x < q.np; x += kFftThreads) job.phi[2 * q.np + x] = s[sw(x)].x;
+}
+
+__global__ void __launch_bounds__(kFftThreads) k_vp_phi(VPJob job) {
+    extern __shared__ cd s[];
+    vp_phi_block(job, s);
+}
+
+// phi of several instances in one launch, one workgroup per instance: the 1D solves are latency
+// bound, and one at a time they leave the GPU idle
+__global__ void __launch_bounds__(kFftThreads) k_vp_phi_batch(const VPJob *__restrict__ jobs) {
+    extern __shared__ cd s[];
+    const VPJob job = jobs[blockIdx.x];
+    vp_phi_block(job, s);
+}
+
+// the bump function over v on the psi grid, centred and scaled to maximum 1, its 1D Poisson solve
+// H and the derivative H'; into H[0, np) and H[2 np, 3 np) (the layout of phi's buffer).  It
+// depends on the grid only.
+__global__ void __launch_bounds__(kFftThreads) k_vp_bump(VP1D q, const cd *__restrict__ tw_p, double *__restrict__ H) {
     extern __shared__ cd s[];
     __shared__ double red[kFftThreads];
-    if (blockIdx.x == 0) {
-        // I(x): the row v0 once more, then the chunk sums in order
-        for (int x = threadIdx.x; x < q.nv; x += kFftThreads) {
-            double a = f[x];
-            for (int c = 0; c < q.nchunks; ++c) a += part[(long)c * q.nv + x];
-            a *= q.hv;
-            a += -1.0;
-            s[sw(x)] = cd{a, 0.0};
-        }
-        __syncthreads();
-        wg_fft(s, q.nv, q.lognv, tw_v, false);
-        inv_lap_c2r(s, q.nv, q.np, q.lognp, 1.0 / (double)q.nv, q.fkx_v, tw_p);
-        for (int x = threadIdx.x; x < q.np; x += kFftThreads) phiH[x] = s[sw(x)].x;
-        spec_deriv_1d(s, q.np, q.lognp, q.fkx_p, tw_p);
-        for (int x = threadIdx.x; x < q.np; x += kFftThreads) phiH[2 * q.np + x] = s[sw(x)].x;
-        return;
-    }
-    // bump function over v on the psi grid
     double sum = 0.0;
     for (int i = threadIdx.x; i < q.np; i += kFftThreads) {
         const double v = q.v0 + i * q.hv_p;
@@ -1560,24 +1587,25 @@ __global__ void __launch_bounds__(kFftThreads) k_vp_1d(VP1D q, const double *__r
     __syncthreads();
     wg_fft(s, q.np, q.lognp, tw_p, false);
     inv_lap_c2r(s, q.np, q.np, q.lognp, 1.0 / (double)q.np, q.fky_p, tw_p);
-    for (int i = threadIdx.x; i < q.np; i += kFftThreads) phiH[q.np + i] = s[sw(i)].x;
+    for (int i = threadIdx.x; i < q.np; i += kFftThreads) H[i] = s[sw(i)].x;
     spec_deriv_1d(s, q.np, q.lognp, q.fky_p, tw_p);
-    for (int i = threadIdx.x; i < q.np; i += kFftThreads) phiH[3 * q.np + i] = s[sw(i)].x;
+    for (int i = threadIdx.x; i < q.np; i += kFftThreads) H[2 * q.np + i] = s[sw(i)].x;
 }
 
 // The Hermite planes of psi(x, v) = phi(x) - H(v) on the psi grid (k_assemble_psi_periodified and
 // the Hermite transform of its result): psi, d/dx = phi', d/dv = -H', d2/dxdv = 0.  psi is a sum
 // of one-variable functions, so its 2D spectrum lies on the two axes and these are the planes
 // the 2D transform gives, without it.
-__global__ void __launch_bounds__(256) k_vp_psi_planes(const double *__restrict__ phiH, int np, int lognp,
-                                                       double *__restrict__ out) {
+// phi: phi at [0, np), phi' at [2 np, 3 np); H: H at [0, np), H' at [2 np, 3 np)
+__global__ void __launch_bounds__(256) k_vp_psi_planes(const double *__restrict__ phi, const double *__restrict__ H,
+                                                       int np, int lognp, double *__restrict__ out) {
     const long N = (long)np * np;
     const long t = (long)blockIdx.x * 256 + threadIdx.x;
     if (t >= N) return;
     const int x = (int)(t & (np - 1)), y = (int)(t >> lognp);
-    out[t] = phiH[x] - phiH[np + y];
-    out[N + t] = phiH[2 * np + x];
-    out[2 * N + t] = -phiH[3 * np + y];
+    out[t] = phi[x] - H[y];
+    out[N + t] = phi[2 * np + x];
+    out[2 * N + t] = -H[2 * np + y];
     out[3 * N + t] = 0.0;
 }
 
@@ -2309,30 +2337,14 @@ size_t op_vp_psi_workspace(const cmm_grid &gv, const cmm_grid &gp) {
     return sizeof(double) * ((size_t)gv.nx * gv.ny + nchunks * gv.nx + 4 * (size_t)gp.nx);
 }
 
-int op_vp_psi_from_f(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf, const cmm_grid &gp, const double *cx,
-                     const double *cy, const double *W, int molly, double *psi, void *work, size_t work_bytes,
-                     hipStream_t st) {
-    if (int rc = vp_check(gv, gp)) return rc;
-    void *w;
-    if (int rc = get_work(work, work_bytes, op_vp_psi_workspace(gv, gp), &w)) return rc;
-    const int nchunks = (gv.ny + kVpRows - 1) / kVpRows;
-    double *f = static_cast<double *>(w);
-    double *part = f + (size_t)gv.nx * gv.ny;
-    double *phiH = part + (size_t)nchunks * gv.nx;
-    if (int rc = op_sample_vort(gm, gv, gf, cx, cy, W, molly, f, st)) return rc;
-    hipLaunchKernelGGL(k_vp_colsum, dim3((gv.nx + 255) / 256, nchunks), dim3(256), 0, st, f, gv.nx, gv.ny, kVpRows,
-                       part);
-    if (int rc = launch_check("k_vp_colsum")) return rc;
-    const cd *tw_v, *tw_p;
-    if (int rc = twiddles(gv.nx, &tw_v)) return rc;
-    if (int rc = twiddles(gp.nx, &tw_p)) return rc;
+static VP1D vp_params(const cmm_grid &gv, const cmm_grid &gp) {
     const DGrid dv = make_dgrid(gv), dp = make_dgrid(gp);
     VP1D q;
     q.nv = gv.nx;
     q.lognv = ilog2(gv.nx);
     q.np = gp.nx;
     q.lognp = ilog2(gp.nx);
-    q.nchunks = nchunks;
+    q.nchunks = (gv.ny + kVpRows - 1) / kVpRows;
     q.hv = dv.hy;
     q.fkx_v = kTwoPi / (dv.hx * gv.nx);
     q.fky_p = kTwoPi / (dp.hy * gp.ny);
@@ -2340,17 +2352,93 @@ int op_vp_psi_from_f(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf,
     q.v0 = gp.y0;
     q.hv_p = dp.hy;
     q.Lv = gp.y1 - gp.y0;
+    return q;
+}
+
+int op_vp_bump(const cmm_grid &gv, const cmm_grid &gp, double *H, hipStream_t st) {
+    if (int rc = vp_check(gv, gp)) return rc;
+    const cd *tw_p;
+    if (int rc = twiddles(gp.nx, &tw_p)) return rc;
+    const VP1D q = vp_params(gv, gp);
+    const size_t lds = sizeof(cd) * gp.nx;
+    if (int rc = set_lds((const void *)k_vp_bump, lds)) return rc;
+    hipLaunchKernelGGL(k_vp_bump, dim3(1), dim3(kFftThreads), lds, st, q, tw_p, H);
+    return launch_check("k_vp_bump");
+}
+
+int op_vp_psi_from_f(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf, const cmm_grid &gp, const double *cx,
+                     const double *cy, const double *W, int molly, double *psi, void *work, size_t work_bytes,
+                     hipStream_t st, const double *W4, const double *H, VPJob *defer) {
+    if (int rc = vp_check(gv, gp)) return rc;
+    void *w;
+    if (int rc = get_work(work, work_bytes, op_vp_psi_workspace(gv, gp), &w)) return rc;
+    const VP1D q = vp_params(gv, gp);
+    double *f = static_cast<double *>(w);
+    double *part = f + (size_t)gv.nx * gv.ny;
+    double *phiH = part + (size_t)q.nchunks * gv.nx;  // phi, H, phi', H' (H, H' when not given)
+    const int r = gv.ny / gm.ny;
+    if (W4 && molly == 0 && gv.ny == r * gm.ny && r >= 2 && gv.x0 == gm.x0 && gv.y0 == gm.y0) {
+        // omega(X(x)) by the column sampler on the node-interleaved field: 4 rows per lane, or 2
+        // where the vorticity grid refines the map grid twice, so a lane's rows share a map cell
+        SampleVortIL smp;
+        smp.gm = make_dgrid(gm);
+        smp.gv = make_dgrid(gv);
+        smp.gf = make_dgrid(gf);
+        smp.cx = cx;
+        smp.cy = cy;
+        smp.W = W4;
+        smp.molly = 0;
+        if (r >= kSampleRows) {
+            const long lanes = (long)gv.nx * (gv.ny / kSampleRows);
+            hipLaunchKernelGGL((k_sample_field<SampleVortIL, double, kSampleRows>), dim3((unsigned)((lanes + 255) / 256)),
+                               dim3(256), 0, st, smp, gv.nx, gv.ny, f);
+        } else {
+            const long lanes = (long)gv.nx * (gv.ny / 2);
+            hipLaunchKernelGGL((k_sample_field<SampleVortIL, double, 2>), dim3((unsigned)((lanes + 255) / 256)),
+                               dim3(256), 0, st, smp, gv.nx, gv.ny, f);
+        }
+        if (int rc = launch_check("k_sample_field")) return rc;
+    } else {
+        if (int rc = op_sample_vort(gm, gv, gf, cx, cy, W, molly, f, st)) return rc;
+    }
+    hipLaunchKernelGGL(k_vp_colsum, dim3((gv.nx + 255) / 256, q.nchunks), dim3(256), 0, st, f, gv.nx, gv.ny, kVpRows,
+                       part);
+    if (int rc = launch_check("k_vp_colsum")) return rc;
+    const cd *tw_v, *tw_p;
+    if (int rc = twiddles(gv.nx, &tw_v)) return rc;
+    if (int rc = twiddles(gp.nx, &tw_p)) return rc;
+    if (!H) {  // the bump function into the workspace: H at phiH + np, H' at phiH + 3 np
+        const size_t lds = sizeof(cd) * gp.nx;
+        if (int rc = set_lds((const void *)k_vp_bump, lds)) return rc;
+        hipLaunchKernelGGL(k_vp_bump, dim3(1), dim3(kFftThreads), lds, st, q, tw_p, phiH + gp.nx);
+        if (int rc = launch_check("k_vp_bump")) return rc;
+    }
+    VPJob job{q, f, part, tw_v, tw_p, phiH};
+    if (defer) {
+        // phi(x) of this instance goes into a batched launch (op_vp_phi_batch) with the others
+        *defer = job;
+        return CMM_OK;
+    }
     const size_t lds = sizeof(cd) * std::max(gv.nx, gp.nx);
-    // 64 KiB of dynamic LDS plus the static reduction array: raise the limit explicitly
-    if (int rc = hip_check(hipFuncSetAttribute((const void *)k_vp_1d, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
-                           "hipFuncSetAttribute"))
-        return rc;
-    hipLaunchKernelGGL(k_vp_1d, dim3(2), dim3(kFftThreads), lds, st, q, f, part, tw_v, tw_p, phiH);
-    if (int rc = launch_check("k_vp_1d")) return rc;
+    if (int rc = set_lds((const void *)k_vp_phi, lds)) return rc;
+    hipLaunchKernelGGL(k_vp_phi, dim3(1), dim3(kFftThreads), lds, st, job);
+    if (int rc = launch_check("k_vp_phi")) return rc;
+    return op_vp_psi_planes(gp, phiH, H ? H : phiH + gp.nx, psi, st);
+}
+
+int op_vp_psi_planes(const cmm_grid &gp, const double *phi, const double *H, double *psi, hipStream_t st) {
     const long N = (long)gp.nx * gp.ny;
-    hipLaunchKernelGGL(k_vp_psi_planes, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, phiH, gp.nx, ilog2(gp.nx),
-                       psi);
+    hipLaunchKernelGGL(k_vp_psi_planes, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, phi, H, gp.nx,
+                       ilog2(gp.nx), psi);
     return launch_check("k_vp_psi_planes");
+}
+
+int op_vp_phi_batch(const VPJob *jobs_dev, int n_jobs, int max_n, hipStream_t st) {
+    if (n_jobs <= 0) return CMM_OK;
+    const size_t lds = sizeof(cd) * max_n;
+    if (int rc = set_lds((const void *)k_vp_phi_batch, lds)) return rc;
+    hipLaunchKernelGGL(k_vp_phi_batch, dim3(n_jobs), dim3(kFftThreads), lds, st, jobs_dev);
+    return launch_check("k_vp_phi_batch");
 }
 
 int op_sample_vort(const cmm_grid &gm, const cmm_grid &gt, const cmm_grid &gf, const double *cx, const double *cy,
