@@ -1057,7 +1057,9 @@ namespace {
 // E_inc copied to pinned memory, psi history rotated and the new psi.  With e->dry only the host
 // state advances (the work comes from a captured graph replay).  overlap: the check and the E_inc
 // copy run on the third stream beside the psi solve (single-engine steps only).
-int step_queue(cmm_euler *e, double dt_now, bool overlap = false) {
+// The step's first part: time vectors, advection, the check and the copy of E_inc to pinned memory.
+// *forked: the check runs on the third stream (overlap, single-engine steps), joined in part b.
+int step_queue_a(cmm_euler *e, double dt_now, bool overlap, bool *forked) {
     const cmm_euler_params &p = e->p;
     const int L = p.l_order;
     const int lc = e->loop + L - 1;
@@ -1106,6 +1108,14 @@ int step_queue(cmm_euler *e, double dt_now, bool overlap = false) {
     if (!e->dry && !e->capturing)
         if (int rc = hip_check(hipEventRecord(e->ev_inc, sc), "hipEventRecord")) return rc;
     if (e->prof) (void)hipEventRecord(e->ev[2], sc);
+    *forked = fork;
+    return CMM_OK;
+}
+
+// The step's second part: the psi history rotated and the new psi, and the rejoin of the check's
+// stream (forked: part a put the check on the third stream).
+int step_queue_b(cmm_euler *e, bool fork) {
+    const int L = e->p.l_order;
     // psi history: the oldest level becomes the newest (same contents as the reference's
     // copies at :865-868), then the new psi, :871.  It is queued before E_inc is known: without a
     // remap (the common case) it is the step's psi; a remap below recomputes it afterwards in
@@ -1122,6 +1132,12 @@ int step_queue(cmm_euler *e, double dt_now, bool overlap = false) {
         if (int rc = hip_check(hipStreamWaitEvent(e->st, e->ev_join, 0), "hipStreamWaitEvent")) return rc;
     }
     return CMM_OK;
+}
+
+int step_queue(cmm_euler *e, double dt_now, bool overlap = false) {
+    bool fork = false;
+    if (int rc = step_queue_a(e, dt_now, overlap, &fork)) return rc;
+    return step_queue_b(e, fork);
 }
 
 // The rest of a step: E_inc (waited for unless the caller has synchronised), remap, particles,
@@ -1247,10 +1263,15 @@ struct cmm_batch {
     hipStream_t master = nullptr;
     hipEvent_t fork = nullptr;
     std::vector<hipEvent_t> join;
-    // one graph per (step phase, state flags of every engine): the psi ring and the map ping-pong
-    // rotate device pointers with periods dividing 12, so a graph replays exactly the launches of
-    // the step it was captured on
+    // A graph step is two captured graphs launched back to back on the master stream: phase A
+    // (advection, check, E_inc copy of every engine) and phase B (psi history and psi solve).
+    // The host waits for ev_a only: with E_inc known, and no remap due, the next step is
+    // prepared and launched while phase B still runs.  One graph per (phase, step phase, state
+    // flags of every engine): the psi ring and the map ping-pong rotate device pointers with
+    // periods dividing 12, so a graph replays exactly the launches of the step it was captured on.
     std::map<std::string, hipGraphExec_t> cache;
+    hipEvent_t ev_a = nullptr, ev_b = nullptr;
+    std::vector<hipEvent_t> tail;  // per engine: the end of its stream's work before a graph step
     int captured = 0, launched = 0, eager = 0;
     // Vlasov-Poisson instances: their phi solves in one launch per captured step (one workgroup
     // each), between a second fork and join of the engine streams
@@ -1259,12 +1280,15 @@ struct cmm_batch {
     VPJob *jobs_dev = nullptr;
     std::vector<VPJob> jobs_up;  // what jobs_dev holds (once jobs_uploaded)
     bool jobs_uploaded = false;
+    // every engine's Lagrange weights (8 doubles each) in one pinned and one device array, so
+    // that a graph step copies them with one transfer; the engines point into them while batched
+    double *lw_host = nullptr, *lw_dev = nullptr;
 };
 
 namespace {
 
-std::string batch_key(const cmm_batch *b) {
-    std::string k;
+std::string batch_key(const cmm_batch *b, char phase) {
+    std::string k(1, phase);
     for (const cmm_euler *e : b->e) {
         k += std::to_string(e->loop % 12);
         k += e->m8_valid ? 'v' : '-';
@@ -1288,7 +1312,10 @@ int batch_normalise(cmm_euler *e) {
 
 bool is_vp(const cmm_euler *e) { return e->p.sim == CMM_SIM_VLASOV_POISSON_1D; }
 
-int batch_capture(cmm_batch *b, double dt, hipGraphExec_t *out) {
+// Phase A or B of every engine's step, captured from the master stream into one graph: the master
+// forks to every engine stream (one branch per engine) and joins them back.  In phase B the
+// Vlasov-Poisson instances' phi solves run as one launch after a join, then their planes.
+int batch_capture(cmm_batch *b, double dt, char phase, hipGraphExec_t *out) {
     int rc = hip_check(hipStreamBeginCapture(b->master, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
     if (rc) return rc;
     rc = hip_check(hipEventRecord(b->fork, b->master), "hipEventRecord");
@@ -1298,16 +1325,21 @@ int batch_capture(cmm_batch *b, double dt, hipGraphExec_t *out) {
         cmm_euler *e = b->e[i];
         rc = hip_check(hipStreamWaitEvent(e->st, b->fork, 0), "hipStreamWaitEvent");
         e->capturing = true;
-        e->vp_defer = is_vp(e);
-        e->vp_out = nullptr;
-        if (!rc) rc = step_queue(e, dt);
-        e->capturing = false;
-        e->vp_defer = false;
-        if (!rc && is_vp(e)) {
-            if (!e->vp_out) { set_error("cmm_batch_step: psi solve not deferred"); rc = CMM_ERR_ARG; }
-            jobs.push_back(e->vp_job);
-            max_n = std::max({max_n, e->vp_job.q.nv, e->vp_job.q.np});
+        if (phase == 'A') {
+            bool fork = false;
+            if (!rc) rc = step_queue_a(e, dt, false, &fork);
+        } else {
+            e->vp_defer = is_vp(e);
+            e->vp_out = nullptr;
+            if (!rc) rc = step_queue_b(e, false);
+            e->vp_defer = false;
+            if (!rc && is_vp(e)) {
+                if (!e->vp_out) { set_error("cmm_batch_step: psi solve not deferred"); rc = CMM_ERR_ARG; }
+                jobs.push_back(e->vp_job);
+                max_n = std::max({max_n, e->vp_job.q.nv, e->vp_job.q.np});
+            }
         }
+        e->capturing = false;
         if (!rc) rc = hip_check(hipEventRecord(b->join[i], e->st), "hipEventRecord");
         if (!rc) rc = hip_check(hipStreamWaitEvent(b->master, b->join[i], 0), "hipStreamWaitEvent");
     }
@@ -1315,11 +1347,8 @@ int batch_capture(cmm_batch *b, double dt, hipGraphExec_t *out) {
     // on its own stream again.  The job table is the same at every step (workspace and twiddle
     // addresses); it is uploaded before the first launch of the graph
     if (!rc && !jobs.empty()) {
-        if (!b->jobs_dev || b->jobs_up.size() != jobs.size()) {
-            if (b->jobs_dev) (void)hipFree(b->jobs_dev);
-            b->jobs_dev = nullptr;
-            b->jobs_up.clear();
-            if (hipMalloc(&b->jobs_dev, sizeof(VPJob) * jobs.size()) != hipSuccess) {
+        if (!b->jobs_dev) {
+            if (hipMalloc(&b->jobs_dev, sizeof(VPJob) * b->e.size()) != hipSuccess) {
                 b->jobs_dev = nullptr;
                 set_error("hipMalloc batch jobs");
                 rc = CMM_ERR_ALLOC;
@@ -1335,11 +1364,8 @@ int batch_capture(cmm_batch *b, double dt, hipGraphExec_t *out) {
             if (!rc) rc = hip_check(hipEventRecord(b->join2[i], e->st), "hipEventRecord");
             if (!rc) rc = hip_check(hipStreamWaitEvent(b->master, b->join2[i], 0), "hipStreamWaitEvent");
         }
-    }
-    if (!rc && !jobs.empty()) {
-        bool same = b->jobs_up.size() == jobs.size() &&
-                    std::memcmp(b->jobs_up.data(), jobs.data(), sizeof(VPJob) * jobs.size()) == 0;
-        if (!same) {
+        if (!rc && (b->jobs_up.size() != jobs.size() ||
+                    std::memcmp(b->jobs_up.data(), jobs.data(), sizeof(VPJob) * jobs.size()) != 0)) {
             b->jobs_up = jobs;
             b->jobs_uploaded = false;
         }
@@ -1349,6 +1375,37 @@ int batch_capture(cmm_batch *b, double dt, hipGraphExec_t *out) {
     if (!rc) rc = rc_end;
     if (!rc) rc = hip_check(hipGraphInstantiate(out, g, nullptr, nullptr, 0), "hipGraphInstantiate");
     if (g) (void)hipGraphDestroy(g);
+    return rc;
+}
+
+// one phase of a graph step: the cached graph of this state, or a capture of it; with a cached graph
+// the engines only advance their host state (dry pass) and the graph carries the launches
+int batch_phase(cmm_batch *b, double dt, char phase, hipGraphExec_t *exec) {
+    const std::string key = batch_key(b, phase);
+    auto it = b->cache.find(key);
+    for (cmm_euler *e : b->e) e->lw_graph = phase == 'A';
+    int rc = CMM_OK;
+    if (it == b->cache.end()) {
+        rc = batch_capture(b, dt, phase, exec);
+        if (!rc) {
+            b->cache[key] = *exec;
+            b->captured++;
+        }
+    } else {
+        *exec = it->second;
+        for (cmm_euler *e : b->e) {
+            e->dry = true;
+            if (phase == 'A') {
+                bool fork = false;
+                rc = step_queue_a(e, dt, false, &fork);
+            } else {
+                rc = step_queue_b(e, false);
+            }
+            e->dry = false;
+            if (rc) break;
+        }
+    }
+    for (cmm_euler *e : b->e) e->lw_graph = false;
     return rc;
 }
 
@@ -1371,12 +1428,30 @@ int cmm_batch_create(cmm_euler *const *engines, int n, int use_graphs, cmm_batch
     b->graphs = use_graphs != 0;
     int rc = hip_check(hipStreamCreateWithFlags(&b->master, hipStreamNonBlocking), "hipStreamCreate");
     if (!rc) rc = hip_check(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "hipEventCreate");
+    if (!rc) rc = hip_check(hipEventCreateWithFlags(&b->fork2, hipEventDisableTiming), "hipEventCreate");
+    if (!rc) rc = hip_check(hipEventCreateWithFlags(&b->ev_a, hipEventDisableTiming), "hipEventCreate");
+    if (!rc) rc = hip_check(hipEventCreateWithFlags(&b->ev_b, hipEventDisableTiming), "hipEventCreate");
     b->join.assign(n, nullptr);
     b->join2.assign(n, nullptr);
+    b->tail.assign(n, nullptr);
     for (int i = 0; i < n && !rc; ++i) rc = hip_check(hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming), "hipEventCreate");
     for (int i = 0; i < n && !rc; ++i) rc = hip_check(hipEventCreateWithFlags(&b->join2[i], hipEventDisableTiming), "hipEventCreate");
-    if (!rc) rc = hip_check(hipEventCreateWithFlags(&b->fork2, hipEventDisableTiming), "hipEventCreate");
+    for (int i = 0; i < n && !rc; ++i) rc = hip_check(hipEventCreateWithFlags(&b->tail[i], hipEventDisableTiming), "hipEventCreate");
+    if (!rc && hipHostMalloc((void **)&b->lw_host, sizeof(double) * 8 * n, hipHostMallocDefault) != hipSuccess) {
+        b->lw_host = nullptr;
+        set_error("hipHostMalloc batch weights");
+        rc = CMM_ERR_ALLOC;
+    }
+    if (!rc && hipMalloc(&b->lw_dev, sizeof(double) * 8 * n) != hipSuccess) {
+        b->lw_dev = nullptr;
+        set_error("hipMalloc batch weights");
+        rc = CMM_ERR_ALLOC;
+    }
     if (rc) { cmm_batch_destroy(b); return rc; }
+    for (int i = 0; i < n; ++i) {
+        b->e[i]->lw_host = b->lw_host + 8 * i;
+        b->e[i]->lw_dev = b->lw_dev + 8 * i;
+    }
     *out = b;
     return CMM_OK;
 }
@@ -1384,14 +1459,21 @@ int cmm_batch_create(cmm_euler *const *engines, int n, int use_graphs, cmm_batch
 int cmm_batch_destroy(cmm_batch *b) {
     if (!b) return CMM_OK;
     if (b->master) (void)hipStreamSynchronize(b->master);
+    // the engines' own weight slots again (bytes 64..127 of their slots)
+    if (b->lw_host)
+        for (cmm_euler *e : b->e) {
+            e->lw_dev = reinterpret_cast<double *>(e->slot + 8);
+            e->lw_host = reinterpret_cast<double *>(e->h_slot + 8);
+        }
     for (auto &kv : b->cache) (void)hipGraphExecDestroy(kv.second);
-    for (hipEvent_t ev : b->join)
+    for (auto *v : {&b->join, &b->join2, &b->tail})
+        for (hipEvent_t ev : *v)
+            if (ev) (void)hipEventDestroy(ev);
+    for (hipEvent_t ev : {b->fork, b->fork2, b->ev_a, b->ev_b})
         if (ev) (void)hipEventDestroy(ev);
-    for (hipEvent_t ev : b->join2)
-        if (ev) (void)hipEventDestroy(ev);
-    if (b->fork2) (void)hipEventDestroy(b->fork2);
     if (b->jobs_dev) (void)hipFree(b->jobs_dev);
-    if (b->fork) (void)hipEventDestroy(b->fork);
+    if (b->lw_dev) (void)hipFree(b->lw_dev);
+    if (b->lw_host) (void)hipHostFree(b->lw_host);
     if (b->master) (void)hipStreamDestroy(b->master);
     delete b;
     return CMM_OK;
@@ -1405,49 +1487,47 @@ int cmm_batch_step(cmm_batch *b, double dt, cmm_step_info *infos) {
     bool graph = b->graphs;
     for (cmm_euler *e : b->e) graph = graph && e->loop >= 1 && dt == e->p.dt;
     if (graph) {
-        // all engines' queued work must have drained before it is captured or replayed on the master stream
-        for (cmm_euler *e : b->e)
-            if (int rc = hip_check(hipStreamSynchronize(e->st), "hipStreamSynchronize")) return rc;
-        const std::string key = batch_key(b);
-        auto it = b->cache.find(key);
-        hipGraphExec_t exec = nullptr;
-        for (cmm_euler *e : b->e) e->lw_graph = true;
-        int rc_q = CMM_OK;
-        if (it == b->cache.end()) {
-            rc_q = batch_capture(b, dt, &exec);
-            if (!rc_q) {
-                b->cache[key] = exec;
-                b->captured++;
-            }
-        } else {
-            exec = it->second;
-            for (cmm_euler *e : b->e) {
-                e->dry = true;
-                rc_q = step_queue(e, dt);
-                e->dry = false;
-                if (rc_q) break;
-            }
+        // the master stream runs after every engine stream's earlier work (remaps, the previous
+        // step's phase B), ordered on the device
+        for (size_t i = 0; i < b->e.size(); ++i) {
+            if (int rc = hip_check(hipEventRecord(b->tail[i], b->e[i]->st), "hipEventRecord")) return rc;
+            if (int rc = hip_check(hipStreamWaitEvent(b->master, b->tail[i], 0), "hipStreamWaitEvent")) return rc;
         }
-        for (cmm_euler *e : b->e) e->lw_graph = false;
-        if (rc_q) return rc_q;
-        // the phi job table (written at capture, the same for every step): uploaded in stream order
-        // before the graph's first use of it
+        hipGraphExec_t ga = nullptr, gb = nullptr;
+        if (int rc = batch_phase(b, dt, 'A', &ga)) return rc;
+        // the phi job table, written at capture and the same for every step
         if (!b->jobs_up.empty() && !b->jobs_uploaded) {
-            if (int rc = hip_check(hipMemcpyAsync(b->jobs_dev, b->jobs_up.data(), sizeof(VPJob) * b->jobs_up.size(),
-                                                  hipMemcpyHostToDevice, b->master),
-                                   "hipMemcpyAsync jobs"))
+            if (int rc = hip_check(hipMemcpy(b->jobs_dev, b->jobs_up.data(), sizeof(VPJob) * b->jobs_up.size(),
+                                             hipMemcpyHostToDevice),
+                                   "hipMemcpy jobs"))
                 return rc;
-            if (int rc = hip_check(hipStreamSynchronize(b->master), "hipStreamSynchronize")) return rc;
             b->jobs_uploaded = true;
         }
-        // this step's Lagrange weights, in stream order before the graph's kernels read them
-        for (cmm_euler *e : b->e)
-            if (int rc = hip_check(hipMemcpyAsync(e->lw_dev, e->lw_host, 8 * sizeof(double), hipMemcpyHostToDevice,
-                                                  b->master),
-                                   "hipMemcpyAsync weights"))
+        // this step's Lagrange weights, in stream order before phase A reads them (the previous
+        // step's phase A, their last reader, has completed: the host waited for it)
+        if (int rc = hip_check(hipMemcpyAsync(b->lw_dev, b->lw_host, sizeof(double) * 8 * b->e.size(),
+                                              hipMemcpyHostToDevice, b->master),
+                               "hipMemcpyAsync weights"))
+            return rc;
+        if (int rc = hip_check(hipGraphLaunch(ga, b->master), "hipGraphLaunch")) return rc;
+        if (int rc = hip_check(hipEventRecord(b->ev_a, b->master), "hipEventRecord")) return rc;
+        if (int rc = batch_phase(b, dt, 'B', &gb)) return rc;
+        if (!b->jobs_up.empty() && !b->jobs_uploaded) {
+            // first capture of phase B: the table goes up before the graph's first launch
+            if (int rc = hip_check(hipStreamSynchronize(b->master), "hipStreamSynchronize")) return rc;
+            if (int rc = hip_check(hipMemcpy(b->jobs_dev, b->jobs_up.data(), sizeof(VPJob) * b->jobs_up.size(),
+                                             hipMemcpyHostToDevice),
+                                   "hipMemcpy jobs"))
                 return rc;
-        if (int rc = hip_check(hipGraphLaunch(exec, b->master), "hipGraphLaunch")) return rc;
-        if (int rc = hip_check(hipStreamSynchronize(b->master), "hipStreamSynchronize")) return rc;
+            b->jobs_uploaded = true;
+        }
+        if (int rc = hip_check(hipGraphLaunch(gb, b->master), "hipGraphLaunch")) return rc;
+        if (int rc = hip_check(hipEventRecord(b->ev_b, b->master), "hipEventRecord")) return rc;
+        // every engine stream's later work (remaps, field reads, the next eager step) after phase B
+        for (cmm_euler *e : b->e)
+            if (int rc = hip_check(hipStreamWaitEvent(e->st, b->ev_b, 0), "hipStreamWaitEvent")) return rc;
+        // E_inc of every engine is in pinned memory once phase A has run
+        if (int rc = hip_check(hipEventSynchronize(b->ev_a), "hipEventSynchronize")) return rc;
         b->launched++;
     } else {
         for (cmm_euler *e : b->e)
