@@ -743,8 +743,9 @@ def main():
         rli = lambda a, b: float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
         dpart = np.abs(sp["part"] - plain["part"])
         dpart = np.minimum(dpart, 2 * np.pi - dpart)
-        chosen = {"form": "split precision (map = identity + FP32 displacement, FP32 combined psi and fine omega "
-                          "records, FP64 arithmetic, FP64 psi levels / FFT / Poisson divide)",
+        chosen = {"form": "split precision (map = identity + FP32 displacement, FP32 combined psi records with "
+                          "low-part value planes, FP32 fine omega records, FP64 arithmetic, FP64 psi levels / FFT / "
+                          "Poisson divide)",
                   "value": round(sp["sps"], 3), "ms_per_step": round(1e3 * sp["elapsed"] / args.steps, 4),
                   "speedup": round(sp["sps"] / sps, 3),
                   "errors_vs_fp64_at_t_end": {

@@ -2591,33 +2591,48 @@ __device__ __forceinline__ double incomp32_point(const DGrid &g, const float4 *_
     return fabs(a + d + (a * d - b * cc));
 }
 
-// velocity sources of the footpoint integration on FP32 combined fields (see VelCombined)
-template <bool P2>
+// velocity sources of the footpoint integration on FP32 combined fields (see VelCombined).  LO:
+// each field carries a plane of the low parts of its values (FP32 records hold the value's
+// leading FP32 part): the value plane of a stream function with |psi| of order one rounded to
+// FP32 would move the velocity by 2^-24 |psi| 6/h, which the check's 1/h amplifies again.
+template <bool P2, bool LO>
 struct VelCombined32 {
     DGrid g;
     const float4 *C1, *C12, *C0;
-    __device__ __forceinline__ void ev(const float4 *__restrict__ H, double x, double y, double &ux, double &uy) const {
+    const float *L1, *L12, *L0;
+    __device__ __forceinline__ void ev(const float4 *__restrict__ H, const float *__restrict__ Lo, double x, double y,
+                                       double &ux, double &uy) const {
         Cell c;
         find_cell<P2>(g, x, y, c);
         double v[16];
         load16_f4(H, c, v);
+        if constexpr (LO) {
+            v[0] += (double)Lo[c.i00];
+            v[1] += (double)Lo[c.i10];
+            v[2] += (double)Lo[c.i01];
+            v[3] += (double)Lo[c.i11];
+        }
         herm_vel_vals(g, c, v, ux, uy);
     }
     __device__ __forceinline__ void w1(const cmm_advect_coeffs &, double x, double y, double &kx, double &ky) const {
-        ev(C1, x, y, kx, ky);
+        ev(C1, L1, x, y, kx, ky);
     }
     __device__ __forceinline__ void w12(const cmm_advect_coeffs &, double x, double y, double &kx, double &ky) const {
-        ev(C12, x, y, kx, ky);
+        ev(C12, L12, x, y, kx, ky);
     }
-    __device__ __forceinline__ void lev(int, double x, double y, double &ux, double &uy) const { ev(C0, x, y, ux, uy); }
+    __device__ __forceinline__ void lev(int, double x, double y, double &ux, double &uy) const {
+        ev(C0, L0, x, y, ux, uy);
+    }
 };
 
-// k_psi_combine with FP32 records: the sums in FP64, rounded once
+// k_psi_combine with FP32 records: the sums in FP64, rounded once; with low-part planes (lo1,
+// lo12, lo0 not null) the value's rounding error goes to them
 template <int L>
 __global__ void __launch_bounds__(256) k_psi_combine32(long n, PsiLevels P, cmm_advect_coeffs c, float4 *__restrict__ C1,
                                                        float4 *__restrict__ C12, float4 *__restrict__ C0,
-                                                       const double *__restrict__ D1, const double *__restrict__ D12,
-                                                       const double *__restrict__ LW) {
+                                                       float *__restrict__ lo1, float *__restrict__ lo12,
+                                                       float *__restrict__ lo0, const double *__restrict__ D1,
+                                                       const double *__restrict__ D12, const double *__restrict__ LW) {
     const long j = (long)blockIdx.x * 256 + threadIdx.x;
     if (j >= n) return;
     double w1[L], w12[L];
@@ -2631,7 +2646,12 @@ __global__ void __launch_bounds__(256) k_psi_combine32(long n, PsiLevels P, cmm_
     for (int l = 0; l < L; ++l)
 #pragma unroll
         for (int p = 0; p < 4; ++p) v[l][p] = P.p[l][p * n + j];
-    auto comb = [&](const double *w, float4 *dst, const double *D) {
+    auto put = [&](const double *o, float4 *dst, float *lo) {
+        const float hi = (float)o[0];
+        dst[j] = float4{hi, (float)o[1], (float)o[2], (float)o[3]};
+        if (lo) lo[j] = (float)(o[0] - (double)hi);
+    };
+    auto comb = [&](const double *w, float4 *dst, float *lo, const double *D) {
         double o[4];
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
@@ -2641,11 +2661,11 @@ __global__ void __launch_bounds__(256) k_psi_combine32(long n, PsiLevels P, cmm_
             if (D) a += D[p * n + j];
             o[p] = a;
         }
-        dst[j] = float4{(float)o[0], (float)o[1], (float)o[2], (float)o[3]};
+        put(o, dst, lo);
     };
-    comb(w1, C1, D1);
-    if (C12) comb(w12, C12, D12);
-    if (C0) C0[j] = float4{(float)v[0][0], (float)v[0][1], (float)v[0][2], (float)v[0][3]};
+    comb(w1, C1, lo1, D1);
+    if (C12) comb(w12, C12, lo12, D12);
+    if (C0) put(v[0], C0, lo0);
 }
 
 // Output stage of the split advection: lane k of a node's group forms displacement plane k (and
@@ -2685,11 +2705,11 @@ __device__ __forceinline__ void advect_store32(const cmm_advect_coeffs &c, int n
     }
 }
 
-// PSI64: the combined psi fields are FP64 records (k_psi_combine's), for flows whose stream
-// function has a large smooth part whose FP32 rounding the velocity's 1/h would amplify
-template <int TIN, int G, bool P2, bool PSI64 = false>
+// PSI: the form of the combined psi fields (a PsiRecForm)
+template <int TIN, int G, bool P2, int PSI>
 __global__ void __launch_bounds__(256) k_map_advect32(DGrid gm, DGrid gp, const void *C1, const void *C12,
-                                                      const void *C0, const float4 *__restrict__ M,
+                                                      const void *C0, const float *L1, const float *L12,
+                                                      const float *L0, const float4 *__restrict__ M,
                                                       cmm_advect_coeffs c, float *__restrict__ Mout, int nblocks) {
     constexpr int NPB = 256 / G;
     const int blk = xcd_block(blockIdx.x, nblocks);
@@ -2704,13 +2724,14 @@ __global__ void __launch_bounds__(256) k_map_advect32(DGrid gm, DGrid gp, const 
         const double xe = xn + ((k >> 2) + 1) * c.eps * (1 - 2 * ((k >> 1) & 1));
         const double ye = yn + ((k >> 2) + 1) * c.eps * (1 - 2 * (((k + 1) >> 1) & 1));
         double xf, yf, px, py;
-        if constexpr (PSI64)
+        if constexpr (PSI == PsiRec64)
             step_backward<TIN>(VelCombined<P2>{gp, (const double *)C1, (const double *)C12, (const double *)C0}, c, xe,
                                ye, xf, yf);
         else
-            step_backward<TIN>(VelCombined32<P2>{gp, (const float4 *)C1, (const float4 *)C12, (const float4 *)C0}, c,
-                               xe, ye, xf, yf);
-        diffeo32<P2>(gm, M, xf, yf, px, py);
+            step_backward<TIN>(VelCombined32<P2, PSI == PsiRec32Lo>{gp, (const float4 *)C1, (const float4 *)C12,
+                                                                    (const float4 *)C0, L1, L12, L0},
+                               c, xe, ye, xf, yf);
+    diffeo32<P2>(gm, M, xf, yf, px, py);
         qx = px - xn;
         qy = py - yn;
     }
@@ -2946,33 +2967,47 @@ int op_interleave4_32(const cmm_grid &g, const double *H, float *H4, hipStream_t
 
 bool op_advect32_supported(const cmm_advect_coeffs &c) { return c.direction != 1 && advect_combinable(c); }
 
-size_t op_advect32_workspace(const cmm_grid &gp, const cmm_advect_coeffs &c, bool psi64) {
+// bytes of one combined psi field in form f on n nodes, rounded up to 256 (FP32 records are
+// followed by their low-part plane)
+size_t psi_rec_bytes(int f, long n) {
+    const size_t per = f == PsiRec64 ? 4 * sizeof(double) : f == PsiRec32Lo ? sizeof(float4) + sizeof(float) : sizeof(float4);
+    return (per * (size_t)n + 255) / 256 * 256;
+}
+
+size_t op_advect32_workspace(const cmm_grid &gp, const cmm_advect_coeffs &c, int psi_form) {
     if (!op_advect32_supported(c)) return 0;
     const int fields = 1 + advect_needs_c12(c) + advect_needs_c0(c);
-    return fields * (psi64 ? 4 * sizeof(double) : sizeof(float4)) * (size_t)gp.nx * gp.ny;
+    return fields * psi_rec_bytes(psi_form, (long)gp.nx * gp.ny);
 }
 
 int op_map_advect32(const cmm_grid &gm, const cmm_grid &gp, const PsiLevels &P, const cmm_advect_coeffs &c,
                     const float *M_in, float *M_out, void *work, size_t work_bytes, hipStream_t st, hipEvent_t mid,
-                    bool psi64, const double *corr1, const double *corr12, const double *lw) {
+                    int psi_form, const double *corr1, const double *corr12, const double *lw) {
     if (!op_advect32_supported(c)) {
         set_error("split-precision advection: the time scheme reads psi levels on their own (AB2, RK3Mod), "
                   "needs two or more psi levels, and is backward only");
         return CMM_ERR_ARG;
     }
-    const size_t need = op_advect32_workspace(gp, c, psi64);
+    if (psi_form != PsiRec32 && psi_form != PsiRec32Lo && psi_form != PsiRec64) {
+        set_error("split-precision advection: unknown psi record form");
+        return CMM_ERR_ARG;
+    }
+    const size_t need = op_advect32_workspace(gp, c, psi_form);
     void *w;
     if (int rc = get_work(work, work_bytes, need, &w)) return rc;
     DGrid dm = make_dgrid(gm), dp = make_dgrid(gp);
     const long n = dp.n;
-    const size_t rec = psi64 ? 4 * sizeof(double) : sizeof(float4);
+    const size_t rec = psi_rec_bytes(psi_form, n);
     char *next = static_cast<char *>(w);
     void *C1 = next, *C12 = nullptr, *C0 = nullptr;
-    next += rec * n;
-    if (advect_needs_c12(c)) { C12 = next; next += rec * n; }
+    next += rec;
+    if (advect_needs_c12(c)) { C12 = next; next += rec; }
     if (advect_needs_c0(c)) C0 = next;
+    // the low-part plane of an FP32 field follows its n records
+    auto lo_of = [&](void *f) { return psi_form == PsiRec32Lo && f ? (float *)((float4 *)f + n) : nullptr; };
+    float *L1 = lo_of(C1), *L12 = lo_of(C12), *L0 = lo_of(C0);
     const dim3 g((unsigned)((n + 255) / 256));
-    if (psi64) {
+    if (psi_form == PsiRec64) {
         double *d1 = (double *)C1, *d12 = (double *)C12, *d0 = (double *)C0;
         switch (c.l_order) {
         case 2: hipLaunchKernelGGL(k_psi_combine<2>, g, dim3(256), 0, st, n, P, c, d1, d12, d0, corr1, corr12, lw); break;
@@ -2981,11 +3016,14 @@ int op_map_advect32(const cmm_grid &gm, const cmm_grid &gp, const PsiLevels &P, 
         }
     } else {
         float4 *f1 = (float4 *)C1, *f12 = (float4 *)C12, *f0 = (float4 *)C0;
+#define CMM_C32(LL)                                                                                                    \
+    hipLaunchKernelGGL(k_psi_combine32<LL>, g, dim3(256), 0, st, n, P, c, f1, f12, f0, L1, L12, L0, corr1, corr12, lw)
         switch (c.l_order) {
-        case 2: hipLaunchKernelGGL(k_psi_combine32<2>, g, dim3(256), 0, st, n, P, c, f1, f12, f0, corr1, corr12, lw); break;
-        case 3: hipLaunchKernelGGL(k_psi_combine32<3>, g, dim3(256), 0, st, n, P, c, f1, f12, f0, corr1, corr12, lw); break;
-        default: hipLaunchKernelGGL(k_psi_combine32<4>, g, dim3(256), 0, st, n, P, c, f1, f12, f0, corr1, corr12, lw); break;
+        case 2: CMM_C32(2); break;
+        case 3: CMM_C32(3); break;
+        default: CMM_C32(4); break;
         }
+#undef CMM_C32
     }
     if (int rc = launch_check("k_psi_combine")) return rc;
     if (mid) (void)hipEventRecord(mid, st);
@@ -2995,7 +3033,8 @@ int op_map_advect32(const cmm_grid &gm, const cmm_grid &gp, const PsiLevels &P, 
     const bool p2 = dm.lgx >= 0 && dm.lgy >= 0 && dp.lgx >= 0 && dp.lgy >= 0;
     const float4 *Mi = (const float4 *)M_in;
 #define CMM_A32(T, GG, PP, S)                                                                                          \
-    hipLaunchKernelGGL((k_map_advect32<T, GG, PP, S>), dim3(nb), dim3(256), 0, st, dm, dp, C1, C12, C0, Mi, c, M_out, nb)
+    hipLaunchKernelGGL((k_map_advect32<T, GG, PP, S>), dim3(nb), dim3(256), 0, st, dm, dp, C1, C12, C0, L1, L12, L0,  \
+                       Mi, c, M_out, nb)
 #define CMM_A32_S(T, S)                                                                                                \
     do {                                                                                                               \
         if (G == 8 && p2) CMM_A32(T, 8, true, S);                                                                      \
@@ -3005,8 +3044,9 @@ int op_map_advect32(const cmm_grid &gm, const cmm_grid &gp, const PsiLevels &P, 
     } while (0)
 #define CMM_A32_T(T)                                                                                                   \
     do {                                                                                                               \
-        if (psi64) CMM_A32_S(T, true);                                                                                 \
-        else CMM_A32_S(T, false);                                                                                      \
+        if (psi_form == PsiRec64) CMM_A32_S(T, PsiRec64);                                                              \
+        else if (psi_form == PsiRec32Lo) CMM_A32_S(T, PsiRec32Lo);                                                     \
+        else CMM_A32_S(T, PsiRec32);                                                                                   \
     } while (0)
     switch (c.time_integration_num) {
     case 10: CMM_A32_T(10); break;

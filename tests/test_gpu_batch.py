@@ -190,3 +190,28 @@ def test_batch_mixed_grid_sizes_of_the_ensemble_workload(dev):
     b.close()
     for e in alone + batched:
         e.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "split"])
+def test_engines_alternate_between_single_steps_and_the_batch(dev, precision):
+    """Batched engines also step alone between graph steps: a single-engine step runs the check and
+    its E_inc copy on the engine's third stream (forked and joined by events), a graph step captures
+    or replays the engine streams' work from the batch's master stream, and neither may leave state
+    that the other reads wrongly (the check's events re-recorded outside a capture, the Lagrange
+    weights taken by value again, graphs keyed on the step phase that the single steps advanced).
+    Every instance must equal its run through single-engine steps alone, bit for bit."""
+    import cmm
+    specs = SPECS[:2]
+    plan = ["b"] * 10 + ["s"] * 5 + ["b"] * 15 + ["s"] * 5 + ["b"] * 5
+    alone = [make(cmm, *s, precision) for s in specs]
+    ref = [[e.step(DT) for _ in plan] for e in alone]
+    mixed = [make(cmm, *s, precision) for s in specs]
+    b = cmm.Batch(mixed, use_graphs=True)
+    got = [b.step(DT) if how == "b" else [e.step(DT) for e in mixed] for how in plan]
+    st = b.stats()
+    print("batch stats", st, "remaps", [sum(i.remapped for i in r) for r in ref])
+    assert st["graph_launches"] >= 20
+    assert_same(alone, ref, mixed, got, len(plan))
+    b.close()
+    for e in alone + mixed:
+        e.close()
