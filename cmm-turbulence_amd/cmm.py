@@ -79,7 +79,7 @@ SYMBOLS = ["cmm_version", "cmm_last_error", "cmm_device_info", "cmm_lagrange_coe
            "cmm_hermite_from_real", "cmm_hermite_from_spectrum", "cmm_psi_workspace_bytes", "cmm_psi_from_vort", "cmm_psi_from_vort_noalloc",
            "cmm_spectral_workspace_bytes", "cmm_spectral_from_real", "cmm_sample_vort", "cmm_vp_psi_workspace_bytes", "cmm_vp_psi_from_f", "cmm_particles_advect", "cmm_euler_create", "cmm_euler_destroy", "cmm_euler_stream",
            "cmm_euler_add_particles", "cmm_euler_add_particle_set", "cmm_euler_get_particle_velocities", "cmm_euler_init", "cmm_euler_step", "cmm_euler_get_field",
-           "cmm_euler_get_particles", "cmm_euler_diagnostics", "cmm_euler_vp_diagnostics", "cmm_euler_sample", "cmm_euler_zoom", "cmm_euler_restart", "cmm_euler_get_stack_map",
+           "cmm_euler_get_particles", "cmm_euler_diagnostics", "cmm_euler_vp_diagnostics", "cmm_euler_sample", "cmm_euler_vp_sample", "cmm_euler_zoom", "cmm_euler_restart", "cmm_euler_get_stack_map",
            "cmm_euler_restart_forward", "cmm_euler_get_forward_stack_map", "cmm_euler_request_remap",
            "cmm_euler_forward_positions", "cmm_euler_forward_points", "cmm_euler_state", "cmm_euler_profile", "cmm_euler_profile_read",
            "cmm_map_to_split", "cmm_map_from_split", "cmm_hermite_to_split", "cmm_map_advect_split_workspace_bytes",
@@ -158,6 +158,7 @@ def lib():
         L.cmm_euler_diagnostics.argtypes = [VP, DP]
         L.cmm_euler_vp_diagnostics.argtypes = [VP, DP]
         L.cmm_euler_sample.argtypes = [VP, Grid, C.c_int, DP]
+        L.cmm_euler_vp_sample.argtypes = [VP, Grid, DP]
         L.cmm_euler_zoom.argtypes = [VP, Grid, C.POINTER(C.c_int), C.c_int, C.c_int, DP]
         L.cmm_euler_restart.argtypes = [VP, C.c_double, C.c_int, DP, DP, DP, DP]
         L.cmm_euler_get_stack_map.argtypes = [VP, C.c_int, DP, DP]
@@ -533,6 +534,18 @@ class Euler:
         _check(lib().cmm_euler_sample(self.h, g_sample, scalar_ic, out.ctypes.data_as(DP)), "cmm_euler_sample")
         names = ("chix", "chiy", "w", "theta", "lap_w", "gradx_w", "grady_w", "psi", "ux", "uy")
         return {k: out[i * n:(i + 1) * n] for i, k in enumerate(names)}
+
+    def vp_sample(self, g_sample):
+        """Vlasov-Poisson sample outputs on g_sample (cmm_euler_vp_sample): positions x, v through
+        the whole stack, f there, the sample's electric field e (dphi/dx, nx values) and the
+        measures etot, ekin, epot, mass, momentum, emodes (4)."""
+        import numpy as np
+        n, nx = g_sample.nx * g_sample.ny, g_sample.nx
+        out = np.zeros(3 * n + nx + 9)
+        _check(lib().cmm_euler_vp_sample(self.h, g_sample, out.ctypes.data_as(DP)), "cmm_euler_vp_sample")
+        m = out[3 * n + nx:]
+        return {"x": out[:n], "v": out[n:2 * n], "f": out[2 * n:3 * n], "e": out[3 * n:3 * n + nx], "etot": m[0],
+                "ekin": m[1], "epot": m[2], "mass": m[3], "momentum": m[4], "emodes": m[5:9].copy()}
 
     def zoom(self, g_zoom, ics=(), want_psi=True):
         """A zoom window (cmm_euler_zoom): positions x, y through the whole stack on g_zoom, the

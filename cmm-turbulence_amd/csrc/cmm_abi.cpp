@@ -1811,6 +1811,76 @@ int cmm_euler_sample(cmm_euler *e, cmm_grid gs, int scalar_ic, double *out) {
     return CMM_OK;
 }
 
+int cmm_euler_vp_sample(cmm_euler *e, cmm_grid gs, double *out) {
+    if (!e || !out || !grid_ok(gs)) { set_error("cmm_euler_vp_sample: bad argument"); return CMM_ERR_ARG; }
+    if (e->p.sim != CMM_SIM_VLASOV_POISSON_1D) {
+        set_error("cmm_euler_vp_sample: not a Vlasov-Poisson engine");
+        return CMM_ERR_ARG;
+    }
+    const cmm_grid &gc = e->p.g_coarse;
+    if (gs.x0 != gc.x0 || gs.x1 != gc.x1 || gs.y0 != gc.y0 || gs.y1 != gc.y1) {
+        set_error("cmm_euler_vp_sample: the sample grid must have the map's bounds");
+        return CMM_ERR_ARG;
+    }
+    // the per-x Fourier solve wants a square power-of-two grid
+    if (gs.nx != gs.ny) { set_error("cmm_euler_vp_sample: the sample grid must be square"); return CMM_ERR_ARG; }
+    const size_t n = (size_t)gs.nx * gs.ny, nx = gs.nx;
+    const size_t wbytes = op_vp_efield_workspace(gs);
+    // device buffer: positions (2n), f (n), phi / phi' (3 nx), the field solve's workspace
+    const size_t need = 3 * n + 3 * nx + (wbytes + sizeof(double) - 1) / sizeof(double);
+    if (e->sbuf_n * 8 < need) {  // the buffer holds 8 sbuf_n doubles
+        if (e->sbuf) (void)hipFree(e->sbuf);
+        e->sbuf = nullptr;
+        e->sbuf_n = 0;
+        if (int rc = dalloc(&e->sbuf, 8 * ((need + 7) / 8))) return rc;
+        e->sbuf_n = (need + 7) / 8;
+    }
+    if (int rc = sync_planes(e)) return rc;
+    double *xy = e->sbuf, *f = xy + 2 * n, *phi3 = f + n, *work = phi3 + 3 * nx;
+    std::vector<double> hxy(2 * n);
+    auto get = [&](double *dst, const double *src, size_t cnt) {
+        return hip_check(hipMemcpyAsync(dst, src, sizeof(double) * cnt, hipMemcpyDeviceToHost, e->st), "get");
+    };
+    int rc = op_compose_positions(gc, gs, e->cx, e->cy, comp_stack(e), e->nstack, xy, e->st, comp_fmt(e));
+    if (!rc) rc = get(hxy.data(), xy, 2 * n);
+    if (!rc) rc = op_sample_init(gc, gs, e->cx, e->cy, comp_stack(e), e->nstack, e->p.ic, e->p.ic_params, e->p.discrete,
+                                 e->p.discrete ? e->p.g_disc : e->p.g_fine, e->disc, f, e->st, comp_fmt(e));
+    if (!rc) rc = get(out + 2 * n, f, n);
+    if (!rc) rc = op_vp_efield(gs, f, phi3, work, wbytes, e->st);
+    if (!rc) rc = get(out + 3 * n, phi3 + 2 * nx, nx);
+    if (!rc) rc = hip_check(hipStreamSynchronize(e->st), "hipStreamSynchronize");
+    if (rc) return rc;
+    for (size_t i = 0; i < n; ++i) {
+        out[i] = hxy[2 * i];
+        out[n + i] = hxy[2 * i + 1];
+    }
+    // Compute_Mass, Compute_Kinetic_Energy, Compute_Momentum, Compute_Potential_Energy on the sample
+    const DGrid d = make_dgrid(gs);
+    const double *fv = out + 2 * n, *E = out + 3 * n;
+    double mass = 0, kin = 0, mom = 0, pot = 0;
+    for (int iy = 0; iy < gs.ny; ++iy) {
+        const double v = gs.y0 + iy * d.hy;
+        for (int ix = 0; ix < gs.nx; ++ix) {
+            const double a = fv[(size_t)iy * nx + ix];
+            mass += a;
+            kin += v * v * a;
+            mom += v * a;
+        }
+    }
+    for (size_t ix = 0; ix < nx; ++ix) pot += E[ix] * E[ix];
+    double *m = out + 3 * n + nx;
+    m[1] = 0.5 * d.hy * d.hx * kin;
+    m[2] = 0.5 * d.hx * pot;
+    m[0] = m[1] + m[2];
+    m[3] = d.hy * d.hx * mass;
+    m[4] = d.hy * d.hx * mom;
+    // the modes come from the engine's psi, as in the conservation diagnostics
+    double diag[9];
+    if (int rc2 = cmm_euler_vp_diagnostics(e, diag)) return rc2;
+    for (int k = 0; k < 4; ++k) m[5 + k] = diag[5 + k];
+    return CMM_OK;
+}
+
 int cmm_euler_zoom(cmm_euler *e, cmm_grid gz, const int *ics, int n_ics, int want_psi, double *out) {
     if (!e || !out || gz.nx < 1 || gz.ny < 1 || n_ics < 0 || (n_ics > 0 && !ics) || !(gz.x1 > gz.x0) ||
         !(gz.y1 > gz.y0)) {

@@ -107,6 +107,10 @@ int op_vp_psi_from_f(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf,
 // H(v) at [0, np) and H'(v) at [2 np, 3 np) of H (3 np doubles): the grid-only part of psi
 int op_vp_bump(const cmm_grid &gv, const cmm_grid &gp, double *H, hipStream_t st);
 int op_vp_psi_planes(const cmm_grid &gp, const double *phi, const double *H, double *psi, hipStream_t st);
+// phi and phi' of a distribution given on the grid g (sample outputs): into phi3[0, nx) and
+// phi3[2 nx, 3 nx); workspace op_vp_efield_workspace(g)
+size_t op_vp_efield_workspace(const cmm_grid &g);
+int op_vp_efield(const cmm_grid &g, const double *f, double *phi3, void *work, size_t work_bytes, hipStream_t st);
 // phi of n_jobs instances (jobs_dev: device array of VPJob), one workgroup each; max_n the largest
 // vort or psi grid width among them
 int op_vp_phi_batch(const VPJob *jobs_dev, int n_jobs, int max_n, hipStream_t st);

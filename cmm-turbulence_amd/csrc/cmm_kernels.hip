@@ -425,6 +425,11 @@ __global__ void __launch_bounds__(256) k_map_advect_fwd(DGrid gm, DGrid gp, cons
 // Cuda-CMM/src/simulation/cmm-simulation-host.cu:44-50.
 __global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
 
+__global__ void __launch_bounds__(256) k_fill(double *p, long n, double v) {
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 __global__ void __launch_bounds__(256) k_incomp(DGrid gm, DGrid gc, const double *__restrict__ cx,
                                                 const double *__restrict__ cy, unsigned long long *__restrict__ out,
                                                 int tiles_x, int tiles_total) {
@@ -2424,6 +2429,36 @@ int op_vp_psi_from_f(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf,
     hipLaunchKernelGGL(k_vp_phi, dim3(1), dim3(kFftThreads), lds, st, job);
     if (int rc = launch_check("k_vp_phi")) return rc;
     return op_vp_psi_planes(gp, phiH, H ? H : phiH + gp.nx, psi, st);
+}
+
+size_t op_vp_efield_workspace(const cmm_grid &g) {
+    const size_t nchunks = (g.ny + kVpRows - 1) / kVpRows;
+    return sizeof(double) * (nchunks * g.nx + 5 * (size_t)g.nx);
+}
+
+// phi(x) and phi'(x) of a distribution f given on the grid g itself (the sample outputs): the
+// column integral I(x) = hv sum_v f - 1 from a zeroed accumulator, the 1D Poisson solve and the
+// spectral derivative on g (vp_phi_block with an all-zero extra row); phi into phi3[0, nx),
+// phi' into phi3[2 nx, 3 nx)
+int op_vp_efield(const cmm_grid &g, const double *f, double *phi3, void *work, size_t work_bytes, hipStream_t st) {
+    if (int rc = vp_check(g, g)) return rc;
+    void *w;
+    if (int rc = get_work(work, work_bytes, op_vp_efield_workspace(g), &w)) return rc;
+    const VP1D q = vp_params(g, g);
+    double *part = static_cast<double *>(w);
+    double *zero = part + (size_t)q.nchunks * g.nx;
+    hipLaunchKernelGGL(k_fill, dim3((g.nx + 255) / 256), dim3(256), 0, st, zero, (long)g.nx, 0.0);
+    if (int rc = launch_check("k_fill")) return rc;
+    hipLaunchKernelGGL(k_vp_colsum, dim3((g.nx + 255) / 256, q.nchunks), dim3(256), 0, st, f, g.nx, g.ny, kVpRows,
+                       part);
+    if (int rc = launch_check("k_vp_colsum")) return rc;
+    const cd *tw;
+    if (int rc = twiddles(g.nx, &tw)) return rc;
+    VPJob job{q, zero, part, tw, tw, phi3};
+    const size_t lds = sizeof(cd) * g.nx;
+    if (int rc = set_lds((const void *)k_vp_phi, lds)) return rc;
+    hipLaunchKernelGGL(k_vp_phi, dim3(1), dim3(kFftThreads), lds, st, job);
+    return launch_check("k_vp_phi");
 }
 
 int op_vp_psi_planes(const cmm_grid &gp, const double *phi, const double *H, double *psi, hipStream_t st) {

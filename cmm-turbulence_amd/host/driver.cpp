@@ -399,6 +399,7 @@ struct Run {
     // through the whole map stack on each due sample grid, written like the computational data,
     // with conservation values on that grid
     std::string sample(double t_now, double dt_now, double dt) {
+        if (s.simulation_type == "cmm_vlasov_poisson_1d") return sample_vlasov(t_now, dt_now, dt);
         if (s.simulation_type != "cmm_euler_2d") return "";
         std::string msg;
         auto f8 = [](double v) {
@@ -492,6 +493,70 @@ struct Run {
             append1(s, "/Monitoring_data/Mesure/Max_vorticity_" + tag, m[3]);
             msg += "Processed sample data " + to_s((int)i + 1) + " on grid " + tag + ", Cons : Energ = " + f8(m[0]) +
                    " \t Enstr = " + f8(m[1]) + " \t Palinstr = " + f8(m[2]) + " \t Wmax = " + f8(m[3]);
+        }
+        return msg;
+    }
+
+    // sample_compute_and_write_vlasov, Cuda-CMM/src/simulation/cmm-simulation-host.cu:898-1030: on
+    // each due sample grid the map through the whole stack, the distribution there, the measures
+    // of that sample (the potential energy from the sample's own phi) and the modes of psi.
+    // Stream_function_Psi_<g>: the reference writes its reused sample buffer, whose first nx values
+    // hold dphi/dx and whose tail holds f; here the first nx values are dphi/dx and the rest f
+    // (the reference holds FFT scratch between nx and n/8).
+    std::string sample_vlasov(double t_now, double dt_now, double dt) {
+        std::string msg;
+        auto f8 = [](double v) {
+            std::ostringstream os;
+            os.precision(8);
+            os << v;
+            return os.str();
+        };
+        const double *b = s.domain_bounds;
+        for (size_t i = 0; i < s.save_sample.size(); ++i) {
+            const auto &c = s.save_sample[i];
+            if (!save_due(c.is_instant, c.time_start, c.time_end, c.time_step, t_now, dt_now, dt)) continue;
+            const int N = c.grid;
+            const std::string tag = to_s(N);
+            const cmm_grid gs{N, N, b[0], b[1], b[2], b[3]};
+            const size_t n = (size_t)N * N;
+            const double area = ((b[1] - b[0]) / (float)N) * ((b[3] - b[2]) / (float)N);
+            const std::string &var = c.var;
+            const bool all = var.find("All") != std::string::npos;
+            auto has = [&](const char *k) { return var.find(k) != std::string::npos; };
+            if (s.verbose >= 4) append1(s, "/Monitoring_data/Debug_globals/Time_s_" + tag, t_now);
+            std::vector<double> out(3 * n + N + 9);
+            check(cmm_euler_vp_sample(e, gs, out.data()), "cmm_euler_vp_sample");
+            std::string ts = std::abs(t_now - kTMax) < 1 ? "final" : fmt3(t_now);
+            const std::string sub = "/Time_data/Time_" + ts;
+            mkdir_p(s.workspace + "data/" + s.file_name + sub);
+            if (s.forward_map && (has("Map_f") || has("Chi_f") || all)) {
+                std::vector<double> fxy(2 * n);
+                check(cmm_euler_forward_positions(e, gs, fxy.data()), "cmm_euler_forward_positions");
+                write_var(sub, "Map_ChiX_f_" + tag, fxy.data(), n, area);
+                write_var(sub, "Map_ChiY_f_" + tag, fxy.data() + n, n, area);
+            }
+            if (has("Map_b") || has("Chi_b") || all) {
+                write_var(sub, "Map_ChiX_" + tag, out.data(), n, area);
+                write_var(sub, "Map_ChiY_" + tag, out.data() + n, n, area);
+            }
+            const double *fv = out.data() + 2 * n;
+            if (has("Dist") || has("F") || all) write_var(sub, "Distribution_F_" + tag, fv, n, area);
+            if (has("Stream") || has("Psi") || all) {
+                std::vector<double> buf(fv, fv + n);
+                std::copy(out.begin() + 3 * n, out.begin() + 3 * n + N, buf.begin());
+                write_var(sub, "Stream_function_Psi_" + tag, buf.data(), n, area);
+            }
+            const double *m = out.data() + 3 * n + N;
+            append1(s, "/Monitoring_data/Mesure/Time_s_" + tag, t_now);
+            append1(s, "/Monitoring_data/Mesure/Etot_" + tag, m[0]);
+            append1(s, "/Monitoring_data/Mesure/Ekin_" + tag, m[1]);
+            append1(s, "/Monitoring_data/Mesure/Epot_" + tag, m[2]);
+            append1(s, "/Monitoring_data/Mesure/Mass_" + tag, m[3]);
+            append1(s, "/Monitoring_data/Mesure/Momentum_" + tag, m[4]);
+            for (int k = 0; k < 4; ++k) append1(s, "/Monitoring_data/Mesure/Emode" + to_s(k + 1) + "_" + tag, m[5 + k]);
+            msg += "Saved sample data " + to_s((int)i + 1) + " on grid " + tag + ", Cons : Etot = " + f8(m[0]) +
+                   " \t Momenta = " + f8(m[4]) + " \t Ekin = " + f8(m[1]) + " \t Epot = " + f8(m[2]) +
+                   " \t Mass = " + f8(m[3]);
         }
         return msg;
     }
@@ -689,8 +754,6 @@ int run_euler(Settings &s) {
     // parts of the reference that are not on this engine's path stop here with a message
     if (!vp && s.initial_condition_num >= 7 && s.initial_condition_num <= 8)
         fail("initial condition " + s.initial_condition + " draws from cuRAND/std distributions and is not provided", 230305);
-    if (vp && !s.save_sample.empty() && s.verbose >= 2)
-        log.push("Note: save_sample outputs of the Vlasov-Poisson flow are not produced by this engine");
 
 
     const double *b = s.domain_bounds;

@@ -346,6 +346,17 @@ enum {
     CMM_SAMPLE_FIELDS = 10
 };
 int cmm_euler_sample(cmm_euler *e, cmm_grid g_sample, int scalar_ic, double *host_out);
+/* The Vlasov-Poisson sample outputs of sample_compute_and_write_vlasov
+ * (Cuda-CMM/src/simulation/cmm-simulation-host.cu:898-1030) on g_sample (n = nx*ny points, nx ==
+ * ny a power of two, 8..4096, the map's bounds): host_out receives [0, n) and [n, 2n) the positions
+ * X and V through the current map and the whole stack, [2n, 3n) the distribution f0 at those
+ * positions, [3n, 3n + nx) E(x) = dphi/dx of the sample's own potential (phi_xx = hv sum_v f - 1
+ * in Fourier space, Cuda-CMM/src/simulation/cmm-simulation-host.cu:436-471; the column integral
+ * starts from zero), then 9 values: Etot, Ekin = 1/2 hx hv sum f v^2, Epot = 1/2 hx sum E^2,
+ * Mass = hx hv sum f, Momentum = hx hv sum f v (Cuda-CMM/src/numerical/cmm-mesure.cu:97-183), and
+ * |E_k|, k = 1..4, of psi level 0 (Compute_kth_Mode, as cmm_euler_vp_diagnostics).  Vlasov-Poisson
+ * engines only. */
+int cmm_euler_vp_sample(cmm_euler *e, cmm_grid g_sample, double *host_out);
 /* A zoom window of compute_zoom (Cuda-CMM/src/simulation/cmm-simulation-host.cu:1039-1227) on
  * g_zoom (any size >= 1, any bounds): host_out receives, each g_zoom.nx*g_zoom.ny doubles, the
  * positions X and Y through the current map and the whole stack (wrapped into the domain after

@@ -193,7 +193,8 @@ def test_cli_vlasov_landau(dev, tmp_path):
     come from an earlier psi (psi = phi - v^2/2, not periodified: Velocity_UY_psi_Max = 2.5 pi
     exactly), so they are not compared.  Checked: the E_inc series against the oracle's run of the
     same settings, and the t=0 diagnostics against closed forms (mass 4 pi, kinetic energy 2 pi,
-    potential energy 1/2 int (0.1 sin(x/2))^2 dx = 0.01 pi)."""
+    potential energy 1/2 int (0.1 sin(x/2))^2 dx = 0.01 pi), on the computational grids and on the
+    file's 256^2 sample grid."""
     params = tmp_path / "params.txt"
     # saves every 16 steps (not every 0.01) so that the driver never shortens a step for a save and
     # the step times equal the oracle's
@@ -227,6 +228,23 @@ def test_cli_vlasov_landau(dev, tmp_path):
     assert abs(m["Mass"][0] - 4 * np.pi) < 1e-9 and abs(m["Ekin"][0] - 2 * np.pi) < 1e-9
     assert abs(m["Epot"][0] - 0.01 * np.pi) < 1e-11 and abs(m["Momentum"][0]) < 1e-9
     assert abs(m["Emode1"][0] - 0.05) < 1e-11
+    # the sample outputs on the 256^2 grid (save_sample of the same file): the same save times, the
+    # closed forms at t = 0 on the sample, the psi modes of the computational run, and the sampled
+    # distribution and field written under Time_data
+    ms = {k: np.fromfile(os.path.join(mon, "Mesure", k + "_256.data"), dtype="<f8")
+          for k in ("Time_s", "Etot", "Ekin", "Epot", "Mass", "Momentum", "Emode1")}
+    print("sample", {k: v[:2] for k, v in ms.items()})
+    assert np.array_equal(ms["Time_s"], m["Time_s"])
+    assert abs(ms["Mass"][0] - 4 * np.pi) < 1e-9 and abs(ms["Ekin"][0] - 2 * np.pi) < 1e-9
+    assert abs(ms["Epot"][0] - 0.01 * np.pi) < 1e-11 and abs(ms["Momentum"][0]) < 1e-9
+    assert np.array_equal(ms["Emode1"], m["Emode1"])
+    t0 = os.path.join(d, "Time_data", "Time_0")
+    fdist = np.fromfile(os.path.join(t0, "Distribution_F_256.data"), dtype="<f8")
+    fpsi = np.fromfile(os.path.join(t0, "Stream_function_Psi_256.data"), dtype="<f8")
+    assert fdist.size == fpsi.size == 256 * 256
+    x = np.arange(256) * (4 * np.pi / float(np.float32(256)))
+    assert np.abs(np.abs(fpsi[:256]) - 0.1 * np.abs(np.sin(x / 2))).max() < 1e-9
+    assert np.array_equal(fpsi[256:], fdist[256:])
 
 
 def run_11b(tmp_path, tag, *extra):
