@@ -44,6 +44,17 @@ __device__ __forceinline__ int xcd_block(int bid, int nblocks) {
     return xcd * per + (xcd < extra ? xcd : extra) + slot;
 }
 
+// The tiles [t_begin, t_end) of this block when ntiles tiles are split into gridDim.x contiguous
+// ranges (whole multiples of 256) and the ranges are dealt to the XCDs in contiguous runs
+// (xcd_block): an XCD then reads one band of the map rows, so the rows shared by neighbouring
+// tiles are fetched into one L2 rather than into several.
+__device__ __forceinline__ void tile_range(long ntiles, long &t_begin, long &t_end) {
+    const long per = ((ntiles + gridDim.x - 1) / gridDim.x + 255) / 256 * 256;
+    const long lb = xcd_block(blockIdx.x, gridDim.x);
+    t_begin = lb * per;
+    t_end = min(ntiles, t_begin + per);
+}
+
 // ------------------------------------------------------------------------------------------------
 // Map kernels
 
@@ -336,6 +347,9 @@ __global__ void __launch_bounds__(256) k_interleave_map(long n, const double *__
     d[3] = double2{cy[2 * n + j], cy[3 * n + j]};
 }
 
+// most footpoints of a stencil (6th order: 12)
+constexpr int kMaxFoot = 12;
+
 // P2 (with COMB): both grids have power-of-two point counts
 // Output stage of the advection: lane k of a node's group forms output plane k (and k + G) by
 // reading the group's traced positions with cross-lane shuffles and summing them in footpoint
@@ -345,17 +359,22 @@ __device__ __forceinline__ void advect_store(const DGrid &gm, const cmm_advect_c
                                              bool active, int k, double px, double py, double *__restrict__ ncx,
                                              double *__restrict__ ncy, double *__restrict__ M8out) {
     const int base = (threadIdx.x & 63) - k;  // first lane of the group within the wavefront
-    // every lane takes part in every shuffle (the same trip count for all lanes)
+    // every lane takes part in every shuffle (the same trip count for all lanes); the footpoint
+    // loop is unrolled to its maximum so that the stencil weights are read at constant indices
+    // (a weight array indexed at run time would be copied to private memory)
 #pragma unroll
     for (int r = 0; r < (8 + G - 1) / G; ++r) {
         const int o = k + r * G;
         const int plane = o & 3;
-        const double *w = plane == 0 ? c.c1 : plane == 1 ? c.cx : plane == 2 ? c.cy : c.cxy;
         double acc = 0.0;
-        for (int m = 0; m < nfoot; ++m) {
-            const double vx = __shfl(px, base + m, 64);
-            const double vy = __shfl(py, base + m, 64);
-            acc += (o < 4 ? vx : vy) * w[m];
+#pragma unroll
+        for (int m = 0; m < kMaxFoot; ++m) {
+            if (m < nfoot) {
+                const double vx = __shfl(px, base + m, 64);
+                const double vy = __shfl(py, base + m, 64);
+                const double wm = plane == 0 ? c.c1[m] : plane == 1 ? c.cx[m] : plane == 2 ? c.cy[m] : c.cxy[m];
+                acc += (o < 4 ? vx : vy) * wm;
+            }
         }
         if (active && o < 8) {
             (o < 4 ? ncx : ncy)[(long)plane * gm.n + node] = acc;
@@ -497,9 +516,11 @@ __global__ void __launch_bounds__(256) k_incomp_tensor(DGrid gm, DGrid gc, const
                                                        unsigned long long *__restrict__ out, int r, int ntx,
                                                        long ntiles) {
     double v = 0.0;
-    for (long t0 = (long)blockIdx.x * 256; t0 < ntiles; t0 += (long)gridDim.x * 256) {
+    long t_begin, t_end;
+    tile_range(ntiles, t_begin, t_end);
+    for (long t0 = t_begin; t0 < t_end; t0 += 256) {
         const long t = t0 + threadIdx.x;
-        if (t >= ntiles) continue;
+        if (t >= t_end) continue;
         const int ix0 = (int)(t % ntx) * TX, iy0 = (int)(t / ntx) * TY;
         double xs[TX], ys[TY];
 #pragma unroll
@@ -2722,12 +2743,15 @@ __device__ __forceinline__ void advect_store32(const cmm_advect_coeffs &c, int n
     for (int r = 0; r < (8 + G - 1) / G; ++r) {
         const int o = k + r * G;
         const int plane = o & 3;
-        const double *w = plane == 0 ? c.c1 : plane == 1 ? c.cx : plane == 2 ? c.cy : c.cxy;
         double acc = 0.0;
-        for (int m = 0; m < nfoot; ++m) {
-            const double vx = __shfl(qx, base + m, 64);
-            const double vy = __shfl(qy, base + m, 64);
-            acc += (o < 4 ? vx : vy) * w[m];
+#pragma unroll
+        for (int m = 0; m < kMaxFoot; ++m) {
+            if (m < nfoot) {
+                const double vx = __shfl(qx, base + m, 64);
+                const double vy = __shfl(qy, base + m, 64);
+                const double wm = plane == 0 ? c.c1[m] : plane == 1 ? c.cx[m] : plane == 2 ? c.cy[m] : c.cxy[m];
+                acc += (o < 4 ? vx : vy) * wm;
+            }
         }
         if (o == 1 || o == 6) acc -= 1.0;
         if (active && o < 8) {
@@ -2778,9 +2802,11 @@ template <int TX, int TY>
 __global__ void __launch_bounds__(256) k_incomp_tensor32(DGrid gm, DGrid gc, const float4 *__restrict__ M,
                                                          unsigned long long *__restrict__ out, int ntx, long ntiles) {
     double v = 0.0;
-    for (long t0 = (long)blockIdx.x * 256; t0 < ntiles; t0 += (long)gridDim.x * 256) {
+    long t_begin, t_end;
+    tile_range(ntiles, t_begin, t_end);
+    for (long t0 = t_begin; t0 < t_end; t0 += 256) {
         const long t = t0 + threadIdx.x;
-        if (t >= ntiles) continue;
+        if (t >= t_end) continue;
         const int ix0 = (int)(t % ntx) * TX, iy0 = (int)(t / ntx) * TY;
         double xs[TX], ys[TY];
 #pragma unroll
