@@ -341,6 +341,10 @@ int cmm_psi_from_vort_split(cmm_grid gm, cmm_grid gv, cmm_grid gf, cmm_grid gp, 
 // ================================================================================================
 // Engine
 
+// Events that only order streams on the device (never waited on by the host) need only a
+// device-scope release.
+constexpr unsigned kDeviceEvent = hipEventDisableTiming | hipEventReleaseToDevice;
+
 struct cmm_euler {
     cmm_euler_params p;
     hipStream_t st = nullptr;
@@ -400,7 +404,7 @@ struct cmm_euler {
     // single-engine steps (cmm_euler_step) run the check on a third stream beside the psi solve
     // (both only read the new map); ensemble steps keep it on the engine stream
     hipStream_t st3 = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipEvent_t ev_fork = nullptr;
     bool part_pending = false;
     std::vector<double *> part;
     std::vector<int> part_n, part_scheme;
@@ -780,12 +784,11 @@ int cmm_euler_create(const cmm_euler_params *p, cmm_euler **out) {
         e->lw_host = reinterpret_cast<double *>(e->h_slot + 8);
     }
     if (!rc) rc = hip_check(hipEventCreateWithFlags(&e->ev_inc, hipEventDisableTiming), "hipEventCreate");
-    if (!rc) rc = hip_check(hipEventCreateWithFlags(&e->ev_psi, hipEventDisableTiming), "hipEventCreate");
-    if (!rc) rc = hip_check(hipEventCreateWithFlags(&e->ev_part, hipEventDisableTiming), "hipEventCreate");
+    if (!rc) rc = hip_check(hipEventCreateWithFlags(&e->ev_psi, kDeviceEvent), "hipEventCreate");
+    if (!rc) rc = hip_check(hipEventCreateWithFlags(&e->ev_part, kDeviceEvent), "hipEventCreate");
     if (!rc) rc = hip_check(hipStreamCreateWithFlags(&e->st2, hipStreamNonBlocking), "hipStreamCreate");
     if (!rc) rc = hip_check(hipStreamCreateWithFlags(&e->st3, hipStreamNonBlocking), "hipStreamCreate");
-    if (!rc) rc = hip_check(hipEventCreateWithFlags(&e->ev_fork, hipEventDisableTiming), "hipEventCreate");
-    if (!rc) rc = hip_check(hipEventCreateWithFlags(&e->ev_join, hipEventDisableTiming), "hipEventCreate");
+    if (!rc) rc = hip_check(hipEventCreateWithFlags(&e->ev_fork, kDeviceEvent), "hipEventCreate");
     for (int s = 0; s < cmm_euler::kProfSets && !rc; ++s)
         for (int i = 0; i < 12 && !rc; ++i) rc = hip_check(hipEventCreate(&e->evs[s][i]), "hipEventCreate");
     if (!rc && p->discrete) rc = engine_setup_discrete(e);
@@ -823,7 +826,6 @@ int cmm_euler_destroy(cmm_euler *e) {
         (void)hipStreamDestroy(e->st3);
     }
     if (e->ev_fork) (void)hipEventDestroy(e->ev_fork);
-    if (e->ev_join) (void)hipEventDestroy(e->ev_join);
     for (int s = 0; s < cmm_euler::kProfSets; ++s)
         for (int i = 0; i < 12; ++i)
             if (e->evs[s][i]) (void)hipEventDestroy(e->evs[s][i]);
@@ -1059,7 +1061,8 @@ namespace {
 // state advances (the work comes from a captured graph replay).  overlap: the check and the E_inc
 // copy run on the third stream beside the psi solve (single-engine steps only).
 // The step's first part: time vectors, advection, the check and the copy of E_inc to pinned memory.
-// *forked: the check runs on the third stream (overlap, single-engine steps), joined in part b.
+// *forked: the check runs on the third stream (overlap, single-engine steps; the caller waits for
+// ev_inc before it queues more work for the engine).
 int step_queue_a(cmm_euler *e, double dt_now, bool overlap, bool *forked) {
     const cmm_euler_params &p = e->p;
     const int L = p.l_order;
@@ -1113,8 +1116,8 @@ int step_queue_a(cmm_euler *e, double dt_now, bool overlap, bool *forked) {
     return CMM_OK;
 }
 
-// The step's second part: the psi history rotated and the new psi, and the rejoin of the check's
-// stream (forked: part a put the check on the third stream).
+// The step's second part: the psi history rotated and the new psi (forked: part a put the check
+// on the third stream).
 int step_queue_b(cmm_euler *e, bool fork) {
     const int L = e->p.l_order;
     // psi history: the oldest level becomes the newest (same contents as the reference's
@@ -1127,11 +1130,11 @@ int step_queue_b(cmm_euler *e, bool fork) {
     if (e->part_pending)  // the previous step's particles still read the level about to be rewritten
         if (int rc = hip_check(hipStreamWaitEvent(e->st, e->ev_part, 0), "hipStreamWaitEvent")) return rc;
     if (int rc = engine_psi(e, e->lvl[0], e->prof ? &e->ev[5] : nullptr)) return rc;
-    // rejoin: the engine stream's later work is ordered after the check
-    if (fork) {
-        if (int rc = hip_check(hipEventRecord(e->ev_join, e->st3), "hipEventRecord")) return rc;
-        if (int rc = hip_check(hipStreamWaitEvent(e->st, e->ev_join, 0), "hipStreamWaitEvent")) return rc;
-    }
+    // no rejoin of the check's stream: the host waits for ev_inc, recorded there after the check,
+    // before it queues anything else for this engine (step_finish), so all later work on the
+    // engine stream follows the check; a stream wait would only add a barrier packet (about 12 us
+    // idle) in front of the next step
+    (void)fork;
     return CMM_OK;
 }
 
@@ -1428,16 +1431,16 @@ int cmm_batch_create(cmm_euler *const *engines, int n, int use_graphs, cmm_batch
     b->e.assign(engines, engines + n);
     b->graphs = use_graphs != 0;
     int rc = hip_check(hipStreamCreateWithFlags(&b->master, hipStreamNonBlocking), "hipStreamCreate");
-    if (!rc) rc = hip_check(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "hipEventCreate");
-    if (!rc) rc = hip_check(hipEventCreateWithFlags(&b->fork2, hipEventDisableTiming), "hipEventCreate");
+    if (!rc) rc = hip_check(hipEventCreateWithFlags(&b->fork, kDeviceEvent), "hipEventCreate");
+    if (!rc) rc = hip_check(hipEventCreateWithFlags(&b->fork2, kDeviceEvent), "hipEventCreate");
     if (!rc) rc = hip_check(hipEventCreateWithFlags(&b->ev_a, hipEventDisableTiming), "hipEventCreate");
-    if (!rc) rc = hip_check(hipEventCreateWithFlags(&b->ev_b, hipEventDisableTiming), "hipEventCreate");
+    if (!rc) rc = hip_check(hipEventCreateWithFlags(&b->ev_b, kDeviceEvent), "hipEventCreate");
     b->join.assign(n, nullptr);
     b->join2.assign(n, nullptr);
     b->tail.assign(n, nullptr);
-    for (int i = 0; i < n && !rc; ++i) rc = hip_check(hipEventCreateWithFlags(&b->join[i], hipEventDisableTiming), "hipEventCreate");
-    for (int i = 0; i < n && !rc; ++i) rc = hip_check(hipEventCreateWithFlags(&b->join2[i], hipEventDisableTiming), "hipEventCreate");
-    for (int i = 0; i < n && !rc; ++i) rc = hip_check(hipEventCreateWithFlags(&b->tail[i], hipEventDisableTiming), "hipEventCreate");
+    for (int i = 0; i < n && !rc; ++i) rc = hip_check(hipEventCreateWithFlags(&b->join[i], kDeviceEvent), "hipEventCreate");
+    for (int i = 0; i < n && !rc; ++i) rc = hip_check(hipEventCreateWithFlags(&b->join2[i], kDeviceEvent), "hipEventCreate");
+    for (int i = 0; i < n && !rc; ++i) rc = hip_check(hipEventCreateWithFlags(&b->tail[i], kDeviceEvent), "hipEventCreate");
     if (!rc && hipHostMalloc((void **)&b->lw_host, sizeof(double) * 8 * n, hipHostMallocDefault) != hipSuccess) {
         b->lw_host = nullptr;
         set_error("hipHostMalloc batch weights");
