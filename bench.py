@@ -372,11 +372,13 @@ def roofline(c, prof, n_stack, suffix=""):
     return roof, per
 
 
-def run_form(cmm, torch, c, precision, warmup, steps, dt=None, t_end=None):
+def run_form(cmm, torch, c, precision, warmup, steps, dt=None, t_end=None, profile_every=8):
     """One engine run of the workload: W untimed steps, then K steps timed between two device
-    synchronisations with per-kernel events; returns the rate, the profile and the final fields
-    (coarse vorticity, newest psi level, particles) for the error comparison.  With t_end the timed
-    steps instead run until t_end, the last one shortened to land on it."""
+    synchronisations; returns the rate, the kernel profile and the final fields (coarse vorticity,
+    newest psi level, particles) for the error comparison.  With t_end the timed steps instead run
+    until t_end, the last one shortened to land on it.  The kernel profile comes from per-kernel
+    events on every profile_every-th timed step (each event adds a packet to the stream, about
+    1 % of the step at one step in eight; 0: no events)."""
     import numpy as np
     g = cmm.grid
     dt = c["dt"] if dt is None else dt
@@ -393,7 +395,8 @@ def run_form(cmm, torch, c, precision, warmup, steps, dt=None, t_end=None):
     for i in range(warmup):
         einc.append(eng.step(dt).incomp)
     torch.cuda.synchronize()
-    eng.profile(True)
+    if profile_every:
+        eng.profile(True, every=profile_every)
     remaps = 0
     _, t_start, _ = eng.state()
     t = t_start
@@ -409,7 +412,7 @@ def run_form(cmm, torch, c, precision, warmup, steps, dt=None, t_end=None):
             log(f"[{precision}] step {warmup + i}: t={info.t:.4f} E_inc={info.incomp:.3e} maps={info.stack_size}")
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    prof = eng.profile_read()
+    prof = eng.profile_read() if profile_every else {}
     _, t_now, n_stack = eng.state()
     out = {"sps": i / (t1 - t0), "elapsed": t1 - t0, "steps": i, "sim_time": t_now - t_start,
            "sim_per_s": (t_now - t_start) / (t1 - t0), "prof": prof, "maps": n_stack, "t_end": t_now,
@@ -762,7 +765,7 @@ def main():
         # in simulated time against it
         big = {}
         for prec in ("fp64", "split"):
-            r = run_form(cmm, torch, c, prec, args.warmup // 2, 0, dt=2 * c["dt"], t_end=t_now)
+            r = run_form(cmm, torch, c, prec, args.warmup // 2, 0, dt=2 * c["dt"], t_end=t_now, profile_every=0)
             assert abs(r["t_end"] - t_now) <= 1e-12, (r["t_end"], t_now)
             big[prec] = {"dt": 2 * c["dt"], "steps_per_s": round(r["sps"], 3), "t_end": r["t_end"],
                          "speedup_in_simulated_time": round(r["sim_per_s"] / plain["sim_per_s"], 3),

@@ -634,8 +634,9 @@ class Euler:
         _check(lib().cmm_euler_state(self.h, C.byref(st), C.byref(t), C.byref(ns)), "cmm_euler_state")
         return st.value, t.value, ns.value
 
-    def profile(self, enable=True):
-        _check(lib().cmm_euler_profile(self.h, int(enable)), "cmm_euler_profile")
+    def profile(self, enable=True, every=1):
+        """per-kernel events on every `every`-th step (enable False: none); clears the counters"""
+        _check(lib().cmm_euler_profile(self.h, int(every) if enable else 0), "cmm_euler_profile")
 
     def profile_read(self):
         ms = (C.c_double * PROF_SLOTS)()

@@ -418,7 +418,8 @@ struct cmm_euler {
     int loop = 0;
     bool initialised = false;
     // per-kernel event timing: [advect, incomp, psi_row, psi_col, psi_c2r, particles, remap, psi_combine]
-    bool prof = false;
+    bool prof = false;     // events in the current step
+    int prof_every = 0;    // events on every prof_every-th step (0: none)
     // event sets of the last kProfSets steps; a set's intervals are read when the set comes round
     // again (its events long complete), so profiling never makes the host wait inside a step
     static constexpr int kProfSets = 4;
@@ -1071,6 +1072,7 @@ int step_queue_a(cmm_euler *e, double dt_now, bool overlap, bool *forked) {
     e->dtv.resize(lc + 2);
     e->tv[lc + 1] = e->tv[lc] + dt_now;
     e->dtv[lc + 1] = dt_now;
+    e->prof = e->prof_every > 0 && e->loop % e->prof_every == 0;
     if (e->prof) {
         const int s = e->loop % cmm_euler::kProfSets;
         prof_drain(e, s);
@@ -1486,7 +1488,7 @@ int cmm_batch_destroy(cmm_batch *b) {
 int cmm_batch_step(cmm_batch *b, double dt, cmm_step_info *infos) {
     if (!b || !(dt > 0)) { set_error("cmm_batch_step: bad argument"); return CMM_ERR_ARG; }
     for (cmm_euler *e : b->e)
-        if (e->prof || !e->part.empty()) { set_error("cmm_batch_step: profiling and particles are not batched"); return CMM_ERR_ARG; }
+        if (e->prof_every || !e->part.empty()) { set_error("cmm_batch_step: profiling and particles are not batched"); return CMM_ERR_ARG; }
     // a graph replays the steady state: the nominal step of every engine, after its first step
     bool graph = b->graphs;
     for (cmm_euler *e : b->e) graph = graph && e->loop >= 1 && dt == e->p.dt;
@@ -1982,7 +1984,11 @@ int cmm_euler_state(cmm_euler *e, int *steps, double *t, int *stack_size) {
 
 int cmm_euler_profile(cmm_euler *e, int enable) {
     if (!e) { set_error("cmm_euler_profile: null engine"); return CMM_ERR_ARG; }
+    if (enable < 0) { set_error("cmm_euler_profile: negative sampling period"); return CMM_ERR_ARG; }
     for (int s = 0; s < cmm_euler::kProfSets; ++s) prof_drain(e, s);
+    e->prof_every = enable;
+    // the next step decides whether it records (step_queue_a); until then calls outside a step
+    // record as the setting says
     e->prof = enable != 0;
     for (int i = 0; i < CMM_PROF_SLOTS; ++i) { e->ms[i] = 0; e->cnt[i] = 0; }
     return CMM_OK;

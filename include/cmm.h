@@ -378,8 +378,9 @@ int cmm_euler_forward_points(cmm_euler *e, cmm_grid g_bounds, int n, const doubl
                              int apply_current, double *host_out);
 /* Steps taken, current time, stack size. */
 int cmm_euler_state(cmm_euler *e, int *steps, double *t, int *stack_size);
-/* Per-kernel timing with events on the engine stream: enable (clears the counters), then read
- * the accumulated milliseconds and launch counts of CMM_PROF_SLOTS slots:
+/* Per-kernel timing with events on the engine stream: enable (clears the counters; 0 turns the
+ * events off, n >= 1 records them on every n-th step, since each event adds a packet to the
+ * stream), then read the accumulated milliseconds and launch counts of CMM_PROF_SLOTS slots:
  * [map advection, incompressibility check, psi sample+row R2C, psi column stage, psi row C2R,
  *  particles, remap (translate + push + the psi recomputed after it), Lagrange-weighted psi
  *  combination before the advection]. */
