@@ -320,15 +320,15 @@ def composition(cmm, torch, c, n_maps=32, reps=5):
     ref_b = (n_maps + 1) * (32 * nf * nf + 64 * N)
     del stack, xy, packed, xys, rec, xyr
     return {"maps": n_maps + 1, "algorithmic_bytes": b,
-            "plane_major": {"kernel": "k_compose_positions<0>", "ms_avg": round(ms, 4), "achieved_GBs": round(gbs, 1),
+            "plane_major": {"kernel": "k_compose_multi<0, 2>", "ms_avg": round(ms, 4), "achieved_GBs": round(gbs, 1),
                             "frac_hbm": round(gbs / HBM_PEAK_GBS, 4),
                             "point_layers_per_s": round((n_maps + 1) * nf * nf / (ms * 1e-3) / 1e9, 3)},
-            "records_fp64": {"kernel": "k_compose_positions<1>", "ms_avg": round(ms_r, 4), "speedup": round(ms / ms_r, 3),
+            "records_fp64": {"kernel": "k_compose_multi<1, 2>", "ms_avg": round(ms_r, 4), "speedup": round(ms / ms_r, 3),
                              "achieved_GBs": round(gbs_r, 1), "frac_hbm": round(gbs_r / HBM_PEAK_GBS, 4),
                              "point_layers_per_s": round((n_maps + 1) * nf * nf / (ms_r * 1e-3) / 1e9, 3),
                              "reference_dataflow_GBs": round(ref_b / (ms_r * 1e-3) / 1e9, 1),
                              "positions_equal_plane_major": same},
-            "records_split": {"kernel": "k_compose_positions<2>", "ms_avg": round(ms_s, 4), "speedup": round(ms / ms_s, 3),
+            "records_split": {"kernel": "k_compose_multi<2, 2>", "ms_avg": round(ms_s, 4), "speedup": round(ms / ms_s, 3),
                               "algorithmic_bytes": b_s, "achieved_GBs": round(gbs_s, 1),
                               "frac_hbm": round(gbs_s / HBM_PEAK_GBS, 4),
                               "point_layers_per_s": round((n_maps + 1) * nf * nf / (ms_s * 1e-3) / 1e9, 3),
