@@ -157,20 +157,30 @@ struct VelCombined {
     }
 };
 
+// Two footpoints held by one lane: arithmetic component by component, in the order the scalar
+// expressions give (so each component's result is that of the scalar code)
+struct PointPair {
+    double a, b;
+};
+__device__ __forceinline__ PointPair operator+(PointPair p, PointPair q) { return {p.a + q.a, p.b + q.b}; }
+__device__ __forceinline__ PointPair operator-(PointPair p, PointPair q) { return {p.a - q.a, p.b - q.b}; }
+__device__ __forceinline__ PointPair operator*(double s, PointPair p) { return {s * p.a, s * p.b}; }
+__device__ __forceinline__ PointPair operator/(PointPair p, double s) { return {p.a / s, p.b / s}; }
+
 // Backward footpoint integration for one start point; schemes of
 // Cuda-CMM/src/numerical/cmm-timestep.cu (EulerExp :35-45, AB2 :115-137, Heun :68-85,
 // RK3 :175-207, RK3Mod :357-386, RK4 :261-304, RK4Mod :441-470).
-template <int TIN, class V>
-__device__ __forceinline__ void step_backward(const V &vel, const cmm_advect_coeffs &c, double x0, double y0,
-                                              double &xo, double &yo) {
+// T: double, or PointPair (two footpoints of one lane, component by component the same arithmetic)
+template <int TIN, class V, class T = double>
+__device__ __forceinline__ void step_backward(const V &vel, const cmm_advect_coeffs &c, T x0, T y0, T &xo, T &yo) {
     const double dt = c.dt;
     if constexpr (TIN == 10) {
-        double kx, ky;
+        T kx, ky;
         vel.w1(c, x0, y0, kx, ky);
         xo = x0 - dt * kx;
         yo = y0 - dt * ky;
     } else if constexpr (TIN == 20) {
-        double u0x, u0y, u1x, u1y, xp = x0, yp = y0;
+        T u0x, u0y, u1x, u1y, xp = x0, yp = y0;
         xo = x0;
         yo = y0;
         for (int it = 0; it < 10; ++it) {
@@ -183,27 +193,27 @@ __device__ __forceinline__ void step_backward(const V &vel, const cmm_advect_coe
             yo = y0 - dt * (1.5 * u0y - 0.5 * u1y);
         }
     } else if constexpr (TIN == 21) {
-        double k1x, k1y, ux, uy;
+        T k1x, k1y, ux, uy;
         vel.w1(c, x0, y0, k1x, k1y);
         vel.lev(0, x0 - dt * k1x, y0 - dt * k1y, ux, uy);
         xo = x0 - dt * (k1x + ux) / 2;
         yo = y0 - dt * (k1y + uy) / 2;
     } else if constexpr (TIN == 30) {
-        double k[6];
+        T k[6];
         vel.w1(c, x0, y0, k[0], k[1]);
         vel.w12(c, x0 - dt * k[0] / 2.0, y0 - dt * k[1] / 2.0, k[2], k[3]);
         vel.lev(0, x0 + dt * k[0] - 2 * dt * k[2], y0 + dt * k[1] - 2 * dt * k[3], k[4], k[5]);
         xo = x0 - dt * (k[0] + 4 * k[2] + k[4]) / 6.0;
         yo = y0 - dt * (k[1] + 4 * k[3] + k[5]) / 6.0;
     } else if constexpr (TIN == 31) {
-        double k[6];
+        T k[6];
         vel.w1(c, x0, y0, k[0], k[1]);
         vel.lev(0, x0 - dt * k[0], y0 - dt * k[1], k[2], k[3]);
         vel.lev(1, x0 + 2 * dt * (-2 * k[0] + k[2]), y0 + 2 * dt * (-2 * k[1] + k[3]), k[4], k[5]);
         xo = x0 - dt * (5 * k[0] + 8 * k[2] - k[4]) / 12.0;
         yo = y0 - dt * (5 * k[1] + 8 * k[3] - k[5]) / 12.0;
     } else if constexpr (TIN == 40) {
-        double k[8];
+        T k[8];
         vel.w1(c, x0, y0, k[0], k[1]);
         vel.w12(c, x0 - dt * k[0] / 2.0, y0 - dt * k[1] / 2.0, k[2], k[3]);
         vel.w12(c, x0 - dt * k[2] / 2.0, y0 - dt * k[3] / 2.0, k[4], k[5]);
@@ -211,7 +221,7 @@ __device__ __forceinline__ void step_backward(const V &vel, const cmm_advect_coe
         xo = x0 - dt * (k[0] + 2 * k[2] + 2 * k[4] + k[6]) / 6.0;
         yo = y0 - dt * (k[1] + 2 * k[3] + 2 * k[5] + k[7]) / 6.0;
     } else {  // 41
-        double k[8];
+        T k[8];
         vel.w1(c, x0, y0, k[0], k[1]);
         vel.lev(0, x0 - dt * k[0], y0 - dt * k[1], k[2], k[3]);
         vel.w12(c, x0 - dt * (3 * k[0] + k[2]) / 8.0, y0 - dt * (3 * k[1] + k[3]) / 8.0, k[4], k[5]);
@@ -412,6 +422,111 @@ __global__ void __launch_bounds__(256) k_map_advect(DGrid gm, DGrid gp, const do
         }
     }
     advect_store<G>(gm, c, nfoot, node, active, k, px, py, ncx, ncy, M8out);
+}
+
+// ---- two footpoints per lane ----------------------------------------------------------------
+// With the 8-footpoint stencil, lane k of a node's 4 lanes traces footpoints k and k + 4 (the
+// same diagonal at eps and 2 eps from the node), which fall in the same psi cell, and then in the
+// same map cell, most of the time: the second footpoint's corner records are loaded only when its
+// cell differs, which spares the texture path about a third of its addresses.  The second load is
+// held back behind the first footpoint's result (a zero index offset the optimiser cannot see
+// through), so that it does not take registers of its own and cost occupancy.  Per footpoint the
+// values and the operations are those of k_map_advect (the compiler's contraction of the pair's
+// expressions can differ from the single footpoint's in the last bit).
+
+// a zero that depends on v (issued after v is computed)
+__device__ __forceinline__ int zero_after(double v) {
+    int z = 0;
+    asm volatile("" : "+v"(z) : "v"(__double2loint(v)));
+    return z;
+}
+
+__device__ __forceinline__ Cell cell_after(Cell c, double v) {
+    const int z = zero_after(v);
+    c.i00 += z;
+    c.i10 += z;
+    c.i01 += z;
+    c.i11 += z;
+    return c;
+}
+
+template <bool P2>
+struct VelPair {
+    DGrid g;
+    const double *C1, *C12, *C0;
+    __device__ __forceinline__ void ev(const double *__restrict__ H4, PointPair x, PointPair y, PointPair &ux,
+                                       PointPair &uy) const {
+        Cell ca, cb;
+        find_cell<P2>(g, x.a, y.a, ca);
+        find_cell<P2>(g, x.b, y.b, cb);
+        double v[16];
+        load16_il(H4, ca, v);
+        herm_vel_vals(g, ca, v, ux.a, uy.a);
+        if (!same_cell(ca, cb)) load16_il(H4, cell_after(cb, ux.a), v);
+        herm_vel_vals(g, cb, v, ux.b, uy.b);
+    }
+    __device__ __forceinline__ void w1(const cmm_advect_coeffs &, PointPair x, PointPair y, PointPair &kx,
+                                       PointPair &ky) const {
+        ev(C1, x, y, kx, ky);
+    }
+    __device__ __forceinline__ void w12(const cmm_advect_coeffs &, PointPair x, PointPair y, PointPair &kx,
+                                        PointPair &ky) const {
+        ev(C12, x, y, kx, ky);
+    }
+    __device__ __forceinline__ void lev(int, PointPair x, PointPair y, PointPair &ux, PointPair &uy) const {
+        ev(C0, x, y, ux, uy);
+    }
+};
+
+template <int TIN, bool P2>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_map_advect_pair(DGrid gm, DGrid gp, const double *C1, const double *C12,
+                                                         const double *C0, const double *M8, cmm_advect_coeffs c,
+                                                         double *__restrict__ ncx, double *__restrict__ ncy,
+                                                         double *__restrict__ M8out, int nblocks) {
+    constexpr int G = 4, NPB = 256 / G;
+    const int blk = xcd_block(blockIdx.x, nblocks);
+    const int node = blk * NPB + threadIdx.x / G;
+    const int k = threadIdx.x % G;
+    const bool active = node < gm.n;
+    PointPair px{0.0, 0.0}, py{0.0, 0.0};
+    if (active) {
+        const int ix = node % gm.nx, iy = node / gm.nx;
+        const int fa = k, fb = k + 4;
+        const PointPair xe{gm.x0 + ix * gm.hx + ((fa >> 2) + 1) * c.eps * (1 - 2 * ((fa >> 1) & 1)),
+                           gm.x0 + ix * gm.hx + ((fb >> 2) + 1) * c.eps * (1 - 2 * ((fb >> 1) & 1))};
+        const PointPair ye{gm.y0 + iy * gm.hy + ((fa >> 2) + 1) * c.eps * (1 - 2 * (((fa + 1) >> 1) & 1)),
+                           gm.y0 + iy * gm.hy + ((fb >> 2) + 1) * c.eps * (1 - 2 * (((fb + 1) >> 1) & 1))};
+        PointPair xf, yf;
+        step_backward<TIN>(VelPair<P2>{gp, C1, C12, C0}, c, xe, ye, xf, yf);
+        Cell ca, cb;
+        find_cell<P2>(gm, xf.a, yf.a, ca);
+        find_cell<P2>(gm, xf.b, yf.b, cb);
+        double vx[16], vy[16];
+        load_map_il(M8, ca, vx, vy);
+        diffeo_vals(gm, ca, vx, vy, px.a, py.a);
+        if (!same_cell(ca, cb)) load_map_il(M8, cell_after(cb, px.a), vx, vy);
+        diffeo_vals(gm, cb, vx, vy, px.b, py.b);
+    }
+    // output planes: footpoint m sits in lane m & 3 of the group, component m >> 2; lane k forms
+    // plane k of X and of Y, summing in footpoint order like advect_store
+    const int base = (threadIdx.x & 63) - k;
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const int o = k + r * G;
+        const int plane = o & 3;
+        double acc = 0.0;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            const double vx = __shfl(m < 4 ? px.a : px.b, base + (m & 3), 64);
+            const double vy = __shfl(m < 4 ? py.a : py.b, base + (m & 3), 64);
+            const double wm = plane == 0 ? c.c1[m] : plane == 1 ? c.cx[m] : plane == 2 ? c.cy[m] : c.cxy[m];
+            acc += (o < 4 ? vx : vy) * wm;
+        }
+        if (active) {
+            (o < 4 ? ncx : ncy)[(long)plane * gm.n + node] = acc;
+            if (M8out) M8out[8L * node + o] = acc;
+        }
+    }
 }
 
 // The forward map (Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:254-274): the old forward map
@@ -2040,6 +2155,7 @@ int op_map_advect(const cmm_grid &gm, const cmm_grid &gp, const double *cx, cons
     }
     if (mid) (void)hipEventRecord(mid, st);
     const bool p2 = dm.lgx >= 0 && dm.lgy >= 0 && dp.lgx >= 0 && dp.lgy >= 0;
+    const int nb2 = (int)(((long)dm.n * 4 + 255) / 256);  // two footpoints per lane
 #define CMM_ADV(T, GG, CB)                                                                                             \
     hipLaunchKernelGGL((k_map_advect<T, GG, CB>), dim3(nb), dim3(256), 0, st, dm, dp, cx, cy, P, C1, C12, C0, M8, c, ncx, \
                        ncy, m8_out, nb, lw)
@@ -2052,8 +2168,8 @@ int op_map_advect(const cmm_grid &gm, const cmm_grid &gp, const double *cx, cons
 #define CMM_ADV_T(T)                                                                                                   \
     do {                                                                                                               \
         if (comb && p2 && G == 8)                                                                                      \
-            hipLaunchKernelGGL((k_map_advect<T, 8, true, true>), dim3(nb), dim3(256), 0, st, dm, dp, cx, cy, P, C1,  \
-                               C12, C0, M8, c, ncx, ncy, m8_out, nb, lw);                                             \
+            hipLaunchKernelGGL((k_map_advect_pair<T, true>), dim3(nb2), dim3(256), 0, st, dm, dp, C1, C12, C0, M8, c,  \
+                               ncx, ncy, m8_out, nb2);                                                                \
         else if (comb) CMM_ADV_G(T, true);                                                                             \
         else CMM_ADV_G(T, false);                                                                                      \
     } while (0)
@@ -2908,6 +3024,107 @@ __global__ void __launch_bounds__(256) k_map_advect32(DGrid gm, DGrid gp, const 
     advect_store32<G>(c, nfoot, node, active, k, qx, qy, Mout);
 }
 
+// k_map_advect32 with two footpoints per lane (footpoints k and k + 4 of the 8-footpoint stencil),
+// as k_map_advect_pair: the second footpoint's psi and map records are loaded only when its cell
+// differs from the first one's
+template <bool P2, bool LO>
+struct VelPair32 {
+    DGrid g;
+    const float4 *C1, *C12, *C0;
+    const float *L1, *L12, *L0;
+    __device__ __forceinline__ void load(const float4 *__restrict__ H, const float *__restrict__ Lo, const Cell &c,
+                                         double v[16]) const {
+        load16_f4(H, c, v);
+        if constexpr (LO) {
+            v[0] += (double)Lo[c.i00];
+            v[1] += (double)Lo[c.i10];
+            v[2] += (double)Lo[c.i01];
+            v[3] += (double)Lo[c.i11];
+        }
+    }
+    __device__ __forceinline__ void ev(const float4 *__restrict__ H, const float *__restrict__ Lo, PointPair x,
+                                       PointPair y, PointPair &ux, PointPair &uy) const {
+        Cell ca, cb;
+        find_cell<P2>(g, x.a, y.a, ca);
+        find_cell<P2>(g, x.b, y.b, cb);
+        double v[16];
+        load(H, Lo, ca, v);
+        herm_vel_vals(g, ca, v, ux.a, uy.a);
+        if (!same_cell(ca, cb)) load(H, Lo, cell_after(cb, ux.a), v);
+        herm_vel_vals(g, cb, v, ux.b, uy.b);
+    }
+    __device__ __forceinline__ void w1(const cmm_advect_coeffs &, PointPair x, PointPair y, PointPair &kx,
+                                       PointPair &ky) const {
+        ev(C1, L1, x, y, kx, ky);
+    }
+    __device__ __forceinline__ void w12(const cmm_advect_coeffs &, PointPair x, PointPair y, PointPair &kx,
+                                        PointPair &ky) const {
+        ev(C12, L12, x, y, kx, ky);
+    }
+    __device__ __forceinline__ void lev(int, PointPair x, PointPair y, PointPair &ux, PointPair &uy) const {
+        ev(C0, L0, x, y, ux, uy);
+    }
+};
+
+template <int TIN, bool LO>
+__global__ void __launch_bounds__(256)
+k_map_advect32_pair(DGrid gm, DGrid gp, const float4 *C1, const float4 *C12, const float4 *C0, const float *L1,
+                    const float *L12, const float *L0, const float4 *__restrict__ M, cmm_advect_coeffs c,
+                    float *__restrict__ Mout, int nblocks) {
+    constexpr int G = 4, NPB = 256 / G;
+    const int blk = xcd_block(blockIdx.x, nblocks);
+    const int node = blk * NPB + threadIdx.x / G;
+    const int k = threadIdx.x % G;
+    const bool active = node < gm.n;
+    PointPair qx{0.0, 0.0}, qy{0.0, 0.0};
+    if (active) {
+        const int ix = node % gm.nx, iy = node / gm.nx;
+        const double xn = gm.x0 + ix * gm.hx, yn = gm.y0 + iy * gm.hy;
+        const int fa = k, fb = k + 4;
+        const PointPair xe{xn + ((fa >> 2) + 1) * c.eps * (1 - 2 * ((fa >> 1) & 1)),
+                           xn + ((fb >> 2) + 1) * c.eps * (1 - 2 * ((fb >> 1) & 1))};
+        const PointPair ye{yn + ((fa >> 2) + 1) * c.eps * (1 - 2 * (((fa + 1) >> 1) & 1)),
+                           yn + ((fb >> 2) + 1) * c.eps * (1 - 2 * (((fb + 1) >> 1) & 1))};
+        PointPair xf, yf;
+        step_backward<TIN>(VelPair32<true, LO>{gp, C1, C12, C0, L1, L12, L0}, c, xe, ye, xf, yf);
+        Cell ca, cb;
+        find_cell<true>(gm, xf.a, yf.a, ca);
+        find_cell<true>(gm, xf.b, yf.b, cb);
+        double vx[16], vy[16], px, py;
+        load_map32(M, ca, vx, vy);
+        diffeo32_vals(gm, ca, vx, vy, xf.a, yf.a, px, py);
+        qx.a = px - xn;
+        qy.a = py - yn;
+        if (!same_cell(ca, cb)) load_map32(M, cell_after(cb, qx.a), vx, vy);
+        diffeo32_vals(gm, cb, vx, vy, xf.b, yf.b, px, py);
+        qx.b = px - xn;
+        qy.b = py - yn;
+    }
+    // output planes as advect_store32: footpoint m in lane m & 3, component m >> 2
+    const int base = (threadIdx.x & 63) - k;
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const int o = k + r * G;
+        const int plane = o & 3;
+        double acc = 0.0;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            const double vx = __shfl(m < 4 ? qx.a : qx.b, base + (m & 3), 64);
+            const double vy = __shfl(m < 4 ? qy.a : qy.b, base + (m & 3), 64);
+            const double wm = plane == 0 ? c.c1[m] : plane == 1 ? c.cx[m] : plane == 2 ? c.cy[m] : c.cxy[m];
+            acc += (o < 4 ? vx : vy) * wm;
+        }
+        if (o == 1 || o == 6) acc -= 1.0;
+        if (active) {
+            const float hi = (float)acc;
+            float *rec = Mout + 12L * node;
+            rec[split_slot(o)] = hi;
+            if (plane == 0) rec[split_slot(o) + 1] = (float)(acc - (double)hi);
+            else if (plane == 3) rec[10 + (o >> 2)] = 0.0f;
+        }
+    }
+}
+
 // k_incomp_tensor on the split map: the same separable sums over the displacement records
 template <int TX, int TY>
 __global__ void __launch_bounds__(256) k_incomp_tensor32(DGrid gm, DGrid gc, const float4 *__restrict__ M,
@@ -3207,9 +3424,15 @@ int op_map_advect32(const cmm_grid &gm, const cmm_grid &gp, const PsiLevels &P, 
 #define CMM_A32(T, GG, PP, S)                                                                                          \
     hipLaunchKernelGGL((k_map_advect32<T, GG, PP, S>), dim3(nb), dim3(256), 0, st, dm, dp, C1, C12, C0, L1, L12, L0,  \
                        Mi, c, M_out, nb)
+    const int nb2 = (int)(((long)dm.n * 4 + 255) / 256);  // two footpoints per lane
+#define CMM_A32P(T, LOV)                                                                                               \
+    hipLaunchKernelGGL((k_map_advect32_pair<T, LOV>), dim3(nb2), dim3(256), 0, st, dm, dp, (const float4 *)C1,          \
+                       (const float4 *)C12, (const float4 *)C0, L1, L12, L0, Mi, c, M_out, nb2)
 #define CMM_A32_S(T, S)                                                                                                \
     do {                                                                                                               \
-        if (G == 8 && p2) CMM_A32(T, 8, true, S);                                                                      \
+        if (G == 8 && p2 && S == PsiRec32Lo) CMM_A32P(T, true);                                                       \
+        else if (G == 8 && p2 && S == PsiRec32) CMM_A32P(T, false);                                                   \
+        else if (G == 8 && p2) CMM_A32(T, 8, true, S);                                                                 \
         else if (G == 4) CMM_A32(T, 4, false, S);                                                                      \
         else if (G == 8) CMM_A32(T, 8, false, S);                                                                      \
         else CMM_A32(T, 16, false, S);                                                                                 \
@@ -3230,6 +3453,7 @@ int op_map_advect32(const cmm_grid &gm, const cmm_grid &gp, const PsiLevels &P, 
     }
 #undef CMM_A32_T
 #undef CMM_A32_S
+#undef CMM_A32P
 #undef CMM_A32
     return launch_check("k_map_advect32");
 }
