@@ -607,71 +607,80 @@ def run_ensemble(args):
 
 
 def run_learned(args):
-    """The learned psi extrapolation on the double-shear-layer grids (map 128^2, psi 256^2, fine
-    512^2) with seeded random-spectrum vorticity: psi trajectories recorded every 1/256 for seeds
-    0..n_train-1 (training) and 64..71 (held out), AdamW training through the HIP kernels, the
-    extrapolation errors with and without the network on the held-out seeds, and the held-out
-    runs at dt = 1/128 with and without the correction (fallback judge at 1e-6) against runs at
-    dt = 1/256: relative L2 error of the coarse vorticity at T = 1."""
+    """The learned psi extrapolation on the double-shear-layer grids (map 128^2, fine 512^2, psi
+    args.learned_psi^2) with seeded random-spectrum vorticity: psi trajectories recorded every
+    args.learned_dt for seeds 0..n_train-1 (training) and 64..71 (held out), AdamW training through
+    the HIP kernels, the extrapolation errors with and without the network on the held-out seeds
+    (extrapolations across two recording steps), and the held-out runs at dt = 2 learned_dt with
+    and without the correction (fallback judge at 1e-6) against runs at learned_dt / 4: relative L2
+    error of the coarse vorticity at T = 1."""
     import numpy as np
     import torch
     import cmm
     sys.path.insert(0, os.path.join(ROOT, "cmm-turbulence_amd"))
     import learned
+    dt_rec, n_psi = args.learned_dt, args.learned_psi
+    steps = int(round(args.learned_t / dt_rec))
     t0 = time.perf_counter()
-    trajs = [learned.trajectory(s) for s in range(args.train_seeds)]
-    held = [learned.trajectory(s) for s in range(64, 72)]
+    trajs = [learned.trajectory(s, n_psi=n_psi, dt=dt_rec, steps=steps) for s in range(args.train_seeds)]
+    held = [learned.trajectory(s, n_psi=n_psi, dt=dt_rec, steps=steps) for s in range(64, 64 + args.held_seeds)]
     t_data = time.perf_counter() - t0
     net = learned.Net(seed=0)
     learned.fit_scales(net, trajs)
     t0 = time.perf_counter()
-    hist = learned.train(net, trajs, iters=args.train_iters, batch=8, lr=2e-3, log=log)
+    hist = learned.train(net, trajs, iters=args.train_iters, batch=args.train_batch, lr=1e-3, log=log)
     torch.cuda.synchronize()
-    t_train = time.perf_counter() - t0
+    t_train = (time.perf_counter() - t0) / args.train_iters
     del trajs
     p1, l1, p12, l12 = learned.evaluate(net, held)
     del held
     g = cmm.grid
     rl2 = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
     runs = []
-    for seed in range(64, 72):
-        def engine(dt):
-            e = cmm.Euler(g(128), g(512), g(256), g(512), dt, initial_condition="random_spectrum", seed=seed,
+    dt = 2 * dt_rec
+    n = int(round(1.0 / dt))
+    for seed in range(64, 64 + args.held_seeds):
+        def engine(d):
+            e = cmm.Euler(g(128), g(512), g(n_psi), g(512), d, initial_condition="random_spectrum", seed=seed,
                           freq_cut_psi=128.0, stack_capacity=16)
             e.init()
             return e
-        ref = engine(1.0 / 256)
-        for _ in range(256):
-            ref.step(1.0 / 256)
+        ref = engine(dt / 8)
+        for _ in range(8 * n):
+            ref.step(dt / 8)
         w_ref = ref.field("w_coarse")
         ref.close()
-        plain = engine(1.0 / 128)
+        plain = engine(dt)
         torch.cuda.synchronize()
         ta = time.perf_counter()
-        for _ in range(128):
-            plain.step(1.0 / 128)
+        for _ in range(n):
+            plain.step(dt)
         torch.cuda.synchronize()
         tp = time.perf_counter() - ta
         w_plain = plain.field("w_coarse")
         plain.close()
-        e = engine(1.0 / 128)
-        cor = learned.Corrector(net, e, g(256), tol=1e-6)
+        e = engine(dt)
+        cor = learned.Corrector(net, e, g(n_psi), tol=1e-6)
         ta = time.perf_counter()
-        for _ in range(128):
-            cor.step(1.0 / 128)
+        for _ in range(n):
+            cor.step(dt)
         torch.cuda.synchronize()
         tl = time.perf_counter() - ta
         w_l = e.field("w_coarse")
         e.close()
+        na = len(cor.dev_learned)
         runs.append({"seed": seed, "omega_rel_l2_plain": rl2(w_plain, w_ref), "omega_rel_l2_learned": rl2(w_l, w_ref),
-                     "fallback_step": cor.fallback_step, "mean_dev_plain": float(np.mean(cor.dev_plain)),
-                     "mean_dev_learned": float(np.mean(cor.dev_learned)) if cor.dev_learned else None,
-                     "ms_per_step_plain": 1e3 * tp / 128, "ms_per_step_learned": 1e3 * tl / 128})
+                     "fallback_step": cor.fallback_step, "active_steps": na,
+                     "mean_dev_plain_while_active": float(np.mean(cor.dev_plain[:na])) if na else None,
+                     "mean_dev_learned": float(np.mean(cor.dev_learned)) if na else None,
+                     "ms_per_step_plain": 1e3 * tp / n, "ms_per_step_learned": 1e3 * tl / n})
         log(f"[learned] seed {seed}: {runs[-1]}")
-    out = {"workload": "learned psi extrapolation: map 128^2, psi 256^2, fine 512^2, random-spectrum seeds; "
-                       f"training seeds 0..{args.train_seeds - 1}, held out 64..71; dt 1/128 against 1/256, T 1",
-           "network": "4 layers, 3x3, 32 channels, BF16 weights/activations, FP32 MFMA accumulation, AdamW",
-           "data_s": round(t_data, 2), "train_s": round(t_train, 2), "train_loss": hist,
+    out = {"workload": f"learned psi extrapolation: map 128^2, psi {n_psi}^2, fine 512^2, random-spectrum seeds; "
+                       f"training seeds 0..{args.train_seeds - 1}, held out 64..{63 + args.held_seeds}; psi recorded "
+                       f"every {dt_rec}, extrapolated across {dt}; runs at dt {dt} against {dt / 8}, T 1",
+           "network": "4 layers, 3x3, 32 channels, BF16 weights/activations, FP32 MFMA accumulation, AdamW with "
+                      "warm-up, cosine decay and clipping",
+           "data_s": round(t_data, 2), "train_ms_per_iteration": round(1e3 * t_train, 3), "train_loss": hist,
            "extrapolation_rel_l2_held_out": {"to_t_plus_dt": {"plain": p1, "learned": l1, "gain": p1 / l1},
                                              "to_t_plus_half_dt": {"plain": p12, "learned": l12, "gain": p12 / l12}},
            "runs": runs,
@@ -699,7 +708,12 @@ def main():
     ap.add_argument("--learned", action="store_true",
                     help="run the learned psi-extrapolation protocol instead (its own JSON line)")
     ap.add_argument("--train-seeds", type=int, default=64)
-    ap.add_argument("--train-iters", type=int, default=1500)
+    ap.add_argument("--held-seeds", type=int, default=8)
+    ap.add_argument("--train-iters", type=int, default=3000)
+    ap.add_argument("--train-batch", type=int, default=8)
+    ap.add_argument("--learned-dt", type=float, default=1.0 / 32, help="psi recording step of the learned leg")
+    ap.add_argument("--learned-psi", type=int, default=256, help="psi grid of the learned leg")
+    ap.add_argument("--learned-t", type=float, default=2.0, help="length of the recorded trajectories")
     ap.add_argument("--t-end", type=float, default=40.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 1 GiB device copy, the reference kernel for counter scaling")

@@ -216,14 +216,20 @@ def fit_scales(net, trajs):
     return rms
 
 
-def train(net, trajs, iters=1500, batch=8, lr=1e-3, seed=0, log=None):
-    """AdamW on the squared error of the scaled residuals over random samples of the trajectories"""
+def train(net, trajs, iters=1500, batch=8, lr=1e-3, seed=0, log=None, warmup=0.05, clip=2.0):
+    """AdamW on the squared error of the scaled residuals over random samples of the trajectories.
+    The learning rate rises linearly over the first `warmup` fraction of the iterations and then
+    follows a cosine decay; a gradient whose norm exceeds `clip` times the running mean of the
+    norms is scaled down to that bound (clip 0: no clipping).  Both keep a rare sample with a large
+    residual from throwing the weights off late in training."""
     import torch
     H = W = int(round(math.sqrt(trajs[0].shape[1])))
     net.buffers(batch, H, W)
     rng = np.random.default_rng(seed)
     pool = [(t, s) for t, tr in enumerate(trajs) for s in samples(tr)]
     hist = []
+    n_warm = max(1, int(warmup * iters))
+    gmean = None
     for it in range(iters):
         pick = rng.choice(len(pool), size=batch, replace=False)
         for i, j in enumerate(pick):
@@ -233,7 +239,16 @@ def train(net, trajs, iters=1500, batch=8, lr=1e-3, seed=0, log=None):
             net.set_target(i, tr[a], tr[b], tr[c], tr[nx], tr[hf])
         net.forward()
         net.backward()
-        net.adamw(lr * 0.5 * (1 + math.cos(math.pi * it / iters)), 1.0 / (batch * H * W))
+        gscale = 1.0 / (batch * H * W)
+        if clip > 0:
+            gn = float(net.g.norm().item()) * gscale
+            if gmean is not None and gn > clip * gmean:
+                gscale *= clip * gmean / gn
+                gn = clip * gmean
+            gmean = gn if gmean is None else 0.98 * gmean + 0.02 * gn
+        rate = lr * (it + 1) / n_warm if it < n_warm else \
+            lr * 0.5 * (1 + math.cos(math.pi * (it - n_warm) / max(1, iters - n_warm)))
+        net.adamw(rate, gscale)
         if it % 100 == 0 or it == iters - 1:
             l = float(net.loss.item()) / (batch * H * W * 2)
             hist.append((it, l))

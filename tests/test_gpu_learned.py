@@ -122,32 +122,39 @@ def test_network_gradient_matches_cpu(dev):
 
 
 def test_training_gain_and_fallback(dev):
-    """trains on seeded trajectories (map 128^2, psi 256^2, fine 512^2, psi recorded every 1/256),
-    then compares the extrapolation errors with and without the network on held-out seeds, and
-    runs the corrected time loop at dt = 1/128 with the fallback judge"""
+    """trains on seeded trajectories (map 128^2, psi 256^2, fine 512^2, psi recorded every 1/32, so
+    the extrapolations span 1/16, where the Lagrange extrapolation's residual is large enough to
+    learn), then compares the extrapolation errors with and without the network on held-out seeds:
+    the corrected extrapolation to t + dt must be at least 5 % closer to the recorded psi (measured
+    1.13x with these 6 training seeds and 300 iterations, 1.13x with 64 seeds and 3000).  Then the
+    corrected time loop at dt = 1/16 with the fallback judge (1e-6): its extrapolations must be
+    closer to the new Poisson psi than the plain ones while it is active."""
     import cmm
     import learned
     train_seeds, test_seeds = range(0, 6), range(64, 66)
-    trajs = [learned.trajectory(s, steps=128) for s in train_seeds]
-    held = [learned.trajectory(s, steps=128) for s in test_seeds]
+    trajs = [learned.trajectory(s, dt=1.0 / 32, steps=64) for s in train_seeds]
+    held = [learned.trajectory(s, dt=1.0 / 32, steps=64) for s in test_seeds]
     net = learned.Net(seed=0)
     rms = learned.fit_scales(net, trajs)
     print("scales (psi, d1, d2, r1, r12):", rms)
     hist = learned.train(net, trajs, iters=300, batch=4, lr=2e-3, log=print)
-    assert hist[-1][1] < hist[0][1]
+    assert all(np.isfinite(l) for _, l in hist)
     p1, l1, p12, l12 = learned.evaluate(net, held)
     print(f"held-out relative L2 extrapolation error to t+dt: plain {p1:.3e}, learned {l1:.3e} "
           f"(gain {p1 / l1:.3f}); to t+dt/2: plain {p12:.3e}, learned {l12:.3e} (gain {p12 / l12:.3f})")
+    assert p1 / l1 >= 1.05 and p12 / l12 > 1.0
     # the corrected loop at twice the recording step, with the judge
     g = cmm.grid
-    e = cmm.Euler(g(128), g(512), g(256), g(512), 1.0 / 128, initial_condition="random_spectrum", seed=64,
+    e = cmm.Euler(g(128), g(512), g(256), g(512), 1.0 / 16, initial_condition="random_spectrum", seed=64,
                   freq_cut_psi=128.0, stack_capacity=16)
     e.init()
     cor = learned.Corrector(net, e, g(256), tol=1e-6)
-    for _ in range(32):
-        cor.step(1.0 / 128)
+    for _ in range(16):
+        cor.step(1.0 / 16)
     print(f"in the loop: learned deviations {np.round(np.log10(cor.dev_learned[:8]), 2)} ..., plain "
           f"{np.round(np.log10(cor.dev_plain[:8]), 2)} ..., fallback at step {cor.fallback_step}")
-    assert len(cor.dev_plain) == 32
+    assert len(cor.dev_plain) == 16
     assert cor.active or cor.fallback_step is not None
+    n_active = len(cor.dev_learned)
+    assert n_active >= 1 and np.mean(cor.dev_learned) < np.mean(cor.dev_plain[:n_active])
     e.close()
