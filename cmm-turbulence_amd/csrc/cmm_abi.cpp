@@ -545,6 +545,23 @@ int engine_advect(cmm_euler *e, const double *t, const double *dt, int loop_ctr,
     // the split form runs the schemes that read psi levels on their own (and the one-level
     // warm-up steps) on the FP64 planes
     if (int rc = sync_planes(e)) return rc;
+    // the Vlasov-Poisson flow: the velocities from the separable psi levels (1D tables of the
+    // combined phi and of the bump function instead of 2D gathers)
+    if (e->p.sim == CMM_SIM_VLASOV_POISSON_1D && e->m8[0] && e->vpH &&
+        op_vp_advect_supported(e->p.g_coarse, e->p.g_psi, c) && !(e->initialised && (e->corr1 || e->corr12))) {
+        const double *m8_in = e->m8_valid ? e->m8[e->m8_cur] : nullptr;
+        double *m8_out = e->m8[1 - e->m8_cur];
+        if (!e->dry)
+            if (int rc = op_map_advect_vp(e->p.g_coarse, e->p.g_psi, P, c, e->cx, e->cy, m8_in, m8_out, e->vpH, e->ncx,
+                                          e->ncy, e->work, e->work_bytes, e->st, e->prof ? e->ev[10] : nullptr, lw))
+                return rc;
+        std::swap(e->cx, e->ncx);
+        std::swap(e->cy, e->ncy);
+        e->m8_valid = true;
+        e->m8_cur = 1 - e->m8_cur;
+        e->m32_fresh = false;
+        return CMM_OK;
+    }
     // the workspace is shared with the psi solve, which runs later on the same stream
     // event 10 separates the psi combination from the advection kernel in the profile
     // m8: the map node-interleaved, written by the previous advection (valid until the map is

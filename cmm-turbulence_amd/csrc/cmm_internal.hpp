@@ -111,6 +111,13 @@ int op_vp_psi_planes(const cmm_grid &gp, const double *phi, const double *H, dou
 // phi3[2 nx, 3 nx); workspace op_vp_efield_workspace(g)
 size_t op_vp_efield_workspace(const cmm_grid &g);
 int op_vp_efield(const cmm_grid &g, const double *f, double *phi3, void *work, size_t work_bytes, hipStream_t st);
+// Map advection of a Vlasov-Poisson engine from the separable psi levels (1D tables of the combined
+// phi and of the bump H instead of 2D gathers); H: op_vp_bump's array
+bool op_vp_advect_supported(const cmm_grid &gm, const cmm_grid &gp, const cmm_advect_coeffs &c);
+int op_map_advect_vp(const cmm_grid &gm, const cmm_grid &gp, const PsiLevels &P, const cmm_advect_coeffs &c,
+                     const double *cx, const double *cy, const double *m8_in, double *m8_out, const double *H,
+                     double *ncx, double *ncy, void *work, size_t work_bytes, hipStream_t st, hipEvent_t mid,
+                     const double *lw);
 // phi of n_jobs instances (jobs_dev: device array of VPJob), one workgroup each; max_n the largest
 // vort or psi grid width among them
 int op_vp_phi_batch(const VPJob *jobs_dev, int n_jobs, int max_n, hipStream_t st);

@@ -529,6 +529,163 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
     }
 }
 
+// ---- Vlasov-Poisson: the separable stream function --------------------------------------------
+// Every psi level of the Vlasov-Poisson flow is psi_l(x, v) = phi_l(x) - H(v) on the psi grid
+// (k_vp_psi_planes: the planes phi - H, phi', -H', 0).  The bicubic Hermite interpolant of such
+// planes is the sum of the cubic Hermite interpolants of phi_l in x and of -H in v (the basis
+// weights of each direction sum to one and their derivatives to zero), so the velocity of a
+// Lagrange combination sum_l w_l psi_l is
+//   u_x = d psi / dv = -(sum_l w_l) H'(v),   u_v = -d psi / dx = -(sum_l w_l phi_l)'(x),
+// two 1D interpolations in tables of the psi grid's width.  Row 0 of a level's value plane holds
+// phi_l(x) - H(v_0) (the constant drops out of the x derivative) and row 0 of its d/dx plane
+// phi_l'(x).  Against the 2D evaluation of the planes the results differ by rounding only.
+
+// the 1D tables of the combined fields: R (values along x) and D (d/dx along x) of C1 (w = L1),
+// C12 (w = L12) and C0 (level 0), each np doubles at T + {0, 1, 2, 3, 4, 5} np, and the weight
+// sums S[0] = sum L1, S[1] = sum L12 (LW: the weights from device memory, as k_psi_combine)
+template <int L>
+__global__ void __launch_bounds__(256) k_vp_combine1d(int np, long N, PsiLevels P, cmm_advect_coeffs c,
+                                                      const double *__restrict__ LW, double *__restrict__ T,
+                                                      double *__restrict__ S) {
+    const int x = blockIdx.x * 256 + threadIdx.x;
+    double w1[L], w12[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+        w1[l] = LW ? LW[l] : c.L1[l];
+        w12[l] = LW ? LW[4 + l] : c.L12[l];
+    }
+    if (x == 0) {
+        double s1 = w1[0], s12 = w12[0];
+#pragma unroll
+        for (int l = 1; l < L; ++l) {
+            s1 += w1[l];
+            s12 += w12[l];
+        }
+        S[0] = s1;
+        S[1] = s12;
+    }
+    if (x >= np) return;
+    double r[L], d[L];
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+        r[l] = P.p[l][x];
+        d[l] = P.p[l][N + x];
+    }
+    double r1 = w1[0] * r[0], d1 = w1[0] * d[0], r12 = w12[0] * r[0], d12 = w12[0] * d[0];
+#pragma unroll
+    for (int l = 1; l < L; ++l) {
+        r1 += w1[l] * r[l];
+        d1 += w1[l] * d[l];
+        r12 += w12[l] * r[l];
+        d12 += w12[l] * d[l];
+    }
+    T[x] = r1;
+    T[np + x] = d1;
+    T[2 * np + x] = r12;
+    T[3 * np + x] = d12;
+    T[4 * np + x] = r[0];
+    T[5 * np + x] = d[0];
+}
+
+// one coordinate's cell, as find_cell takes it: lower node, upper node (wrapped) and offset
+__device__ __forceinline__ void cell_1d(double x, double x0, double ih, int n, int lg, int &i0, int &i1, double &d) {
+    const double f = (x - x0) * ih;
+    double fl = floor(f);
+    d = f - fl;
+    fl = fmin(fmax(fl, -1.0e9), 1.0e9);
+    const int i = (int)fl;
+    i0 = i - floordiv_g(i, n, lg) * n;
+    i1 = i0 + 1 == n ? 0 : i0 + 1;
+}
+
+struct VelSep {
+    DGrid g;
+    const double *T;  // k_vp_combine1d's tables
+    const double *S;  // weight sums
+    const double *H;  // H at [0, np), H' at [2 np, 3 np)
+    __device__ __forceinline__ void ev1(const double *R, const double *D, double s, double x, double v, double &ux,
+                                        double &uy) const {
+        int i0, i1, j0, j1;
+        double dx, dv;
+        cell_1d(x, g.x0, g.ihx, g.nx, g.lgx, i0, i1, dx);
+        cell_1d(v, g.y0, g.ihy, g.ny, g.lgy, j0, j1, dv);
+        double dbx[4], dbv[4];
+        basis_d(dx, dbx);
+        basis_d(dv, dbv);
+        const double *Hd = H + 2 * g.nx;
+        uy = -(dbx[0] * R[i0] + dbx[1] * R[i1] + g.hx * (dbx[2] * D[i0] + dbx[3] * D[i1])) * g.ihx;
+        ux = -s * (dbv[0] * H[j0] + dbv[1] * H[j1] + g.hy * (dbv[2] * Hd[j0] + dbv[3] * Hd[j1])) * g.ihy;
+    }
+    __device__ __forceinline__ void ev(int f, PointPair x, PointPair y, PointPair &ux, PointPair &uy) const {
+        const int np = g.nx;
+        const double s = f == 0 ? S[0] : f == 1 ? S[1] : 1.0;
+        ev1(T + 2 * f * np, T + (2 * f + 1) * np, s, x.a, y.a, ux.a, uy.a);
+        ev1(T + 2 * f * np, T + (2 * f + 1) * np, s, x.b, y.b, ux.b, uy.b);
+    }
+    __device__ __forceinline__ void w1(const cmm_advect_coeffs &, PointPair x, PointPair y, PointPair &kx,
+                                       PointPair &ky) const {
+        ev(0, x, y, kx, ky);
+    }
+    __device__ __forceinline__ void w12(const cmm_advect_coeffs &, PointPair x, PointPair y, PointPair &kx,
+                                        PointPair &ky) const {
+        ev(1, x, y, kx, ky);
+    }
+    __device__ __forceinline__ void lev(int, PointPair x, PointPair y, PointPair &ux, PointPair &uy) const {
+        ev(2, x, y, ux, uy);
+    }
+};
+
+// the 8-footpoint advection of a Vlasov-Poisson map with the separable velocity (two footpoints
+// per lane and the map records as k_map_advect_pair)
+template <int TIN>
+__global__ void __launch_bounds__(256) k_map_advect_vp(DGrid gm, DGrid gp, const double *T, const double *S,
+                                                       const double *H, const double *M8, cmm_advect_coeffs c,
+                                                       double *__restrict__ ncx, double *__restrict__ ncy,
+                                                       double *__restrict__ M8out, int nblocks) {
+    constexpr int G = 4, NPB = 256 / G;
+    const int blk = xcd_block(blockIdx.x, nblocks);
+    const int node = blk * NPB + threadIdx.x / G;
+    const int k = threadIdx.x % G;
+    const bool active = node < gm.n;
+    PointPair px{0.0, 0.0}, py{0.0, 0.0};
+    if (active) {
+        const int ix = node % gm.nx, iy = node / gm.nx;
+        const int fa = k, fb = k + 4;
+        const PointPair xe{gm.x0 + ix * gm.hx + ((fa >> 2) + 1) * c.eps * (1 - 2 * ((fa >> 1) & 1)),
+                           gm.x0 + ix * gm.hx + ((fb >> 2) + 1) * c.eps * (1 - 2 * ((fb >> 1) & 1))};
+        const PointPair ye{gm.y0 + iy * gm.hy + ((fa >> 2) + 1) * c.eps * (1 - 2 * (((fa + 1) >> 1) & 1)),
+                           gm.y0 + iy * gm.hy + ((fb >> 2) + 1) * c.eps * (1 - 2 * (((fb + 1) >> 1) & 1))};
+        PointPair xf, yf;
+        step_backward<TIN>(VelSep{gp, T, S, H}, c, xe, ye, xf, yf);
+        Cell ca, cb;
+        find_cell<true>(gm, xf.a, yf.a, ca);
+        find_cell<true>(gm, xf.b, yf.b, cb);
+        double vx[16], vy[16];
+        load_map_il(M8, ca, vx, vy);
+        diffeo_vals(gm, ca, vx, vy, px.a, py.a);
+        if (!same_cell(ca, cb)) load_map_il(M8, cell_after(cb, px.a), vx, vy);
+        diffeo_vals(gm, cb, vx, vy, px.b, py.b);
+    }
+    const int base = (threadIdx.x & 63) - k;
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const int o = k + r * G;
+        const int plane = o & 3;
+        double acc = 0.0;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            const double vx = __shfl(m < 4 ? px.a : px.b, base + (m & 3), 64);
+            const double vy = __shfl(m < 4 ? py.a : py.b, base + (m & 3), 64);
+            const double wm = plane == 0 ? c.c1[m] : plane == 1 ? c.cx[m] : plane == 2 ? c.cy[m] : c.cxy[m];
+            acc += (o < 4 ? vx : vy) * wm;
+        }
+        if (active) {
+            (o < 4 ? ncx : ncy)[(long)plane * gm.n + node] = acc;
+            if (M8out) M8out[8L * node + o] = acc;
+        }
+    }
+}
+
 // The forward map (Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:254-274): the old forward map
 // is evaluated at the footpoint first, then the point moves forward in time; psi per level.
 template <int TIN, int G>
@@ -2187,6 +2344,57 @@ int op_map_advect(const cmm_grid &gm, const cmm_grid &gp, const double *cx, cons
 #undef CMM_ADV_G
 #undef CMM_ADV
     return launch_check("k_map_advect");
+}
+
+// The Vlasov-Poisson advection with the separable velocity: the 8-footpoint stencil on power-of-two
+// grids, a combinable scheme, levels built by k_vp_psi_planes; H: the bump function's H and H'
+// (op_vp_bump).  Workspace: the 1D tables and, without m8_in, the interleaved map.
+bool op_vp_advect_supported(const cmm_grid &gm, const cmm_grid &gp, const cmm_advect_coeffs &c) {
+    const DGrid dm = make_dgrid(gm), dp = make_dgrid(gp);
+    return c.direction != 1 && advect_combinable(c) && 4 + 4 * c.map_update_order_num == 8 && dm.lgx >= 0 &&
+           dm.lgy >= 0 && dp.lgx >= 0 && dp.lgy >= 0 && c.l_order >= 2 && c.l_order <= 4;
+}
+
+int op_map_advect_vp(const cmm_grid &gm, const cmm_grid &gp, const PsiLevels &P, const cmm_advect_coeffs &c,
+                     const double *cx, const double *cy, const double *m8_in, double *m8_out, const double *H,
+                     double *ncx, double *ncy, void *work, size_t work_bytes, hipStream_t st, hipEvent_t mid,
+                     const double *lw) {
+    if (!op_vp_advect_supported(gm, gp, c)) {
+        set_error("separable Vlasov-Poisson advection: needs the 8-footpoint stencil, a combinable scheme and "
+                  "power-of-two grids");
+        return CMM_ERR_ARG;
+    }
+    const DGrid dm = make_dgrid(gm), dp = make_dgrid(gp);
+    const size_t need = sizeof(double) * (6 * (size_t)gp.nx + 2 + (m8_in ? 0 : 8 * (size_t)dm.n));
+    void *w;
+    if (int rc = get_work(work, work_bytes, need, &w)) return rc;
+    double *T = static_cast<double *>(w), *S = T + 6 * gp.nx;
+    const double *M8 = m8_in;
+    if (!M8) {
+        double *m8w = S + 2;
+        hipLaunchKernelGGL(k_interleave_map, dim3((unsigned)((dm.n + 255) / 256)), dim3(256), 0, st, (long)dm.n, cx, cy,
+                           m8w);
+        if (int rc = launch_check("k_interleave_map")) return rc;
+        M8 = m8w;
+    }
+    const dim3 g1((unsigned)((gp.nx + 255) / 256));
+    switch (c.l_order) {
+    case 2: hipLaunchKernelGGL(k_vp_combine1d<2>, g1, dim3(256), 0, st, gp.nx, (long)dp.n, P, c, lw, T, S); break;
+    case 3: hipLaunchKernelGGL(k_vp_combine1d<3>, g1, dim3(256), 0, st, gp.nx, (long)dp.n, P, c, lw, T, S); break;
+    default: hipLaunchKernelGGL(k_vp_combine1d<4>, g1, dim3(256), 0, st, gp.nx, (long)dp.n, P, c, lw, T, S); break;
+    }
+    if (int rc = launch_check("k_vp_combine1d")) return rc;
+    if (mid) (void)hipEventRecord(mid, st);
+    const int nb = (int)(((long)dm.n * 4 + 255) / 256);
+    switch (c.time_integration_num) {
+    case 10: hipLaunchKernelGGL(k_map_advect_vp<10>, dim3(nb), dim3(256), 0, st, dm, dp, T, S, H, M8, c, ncx, ncy, m8_out, nb); break;
+    case 21: hipLaunchKernelGGL(k_map_advect_vp<21>, dim3(nb), dim3(256), 0, st, dm, dp, T, S, H, M8, c, ncx, ncy, m8_out, nb); break;
+    case 30: hipLaunchKernelGGL(k_map_advect_vp<30>, dim3(nb), dim3(256), 0, st, dm, dp, T, S, H, M8, c, ncx, ncy, m8_out, nb); break;
+    case 40: hipLaunchKernelGGL(k_map_advect_vp<40>, dim3(nb), dim3(256), 0, st, dm, dp, T, S, H, M8, c, ncx, ncy, m8_out, nb); break;
+    case 41: hipLaunchKernelGGL(k_map_advect_vp<41>, dim3(nb), dim3(256), 0, st, dm, dp, T, S, H, M8, c, ncx, ncy, m8_out, nb); break;
+    default: set_error("unknown time integration"); return CMM_ERR_ARG;
+    }
+    return launch_check("k_map_advect_vp");
 }
 
 // tile edge of the tiled check: the tuned one when it divides r, else the largest of 4, 3, 2, 1
