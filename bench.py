@@ -153,12 +153,12 @@ def cpu_baseline(c, threads, steps=16, remap_maps=32):
 
 # bench slot -> kernel-name patterns of the launches it times (the row-transform slot of the psi solve
 # is the sampling of omega(X(x)) followed by the row transform of the sampled field)
-SLOT_KERNELS = {"advect": ("k_map_advect<30, 8, true",), "psi_combine": ("k_psi_combine<3>",),
+SLOT_KERNELS = {"advect": ("k_map_advect_pair<30, true>",), "psi_combine": ("k_psi_combine<3>",),
                 "incomp": ("k_incomp_tensor<",),
                 "psi_row_r2c": ("k_sample_field<cmm::SampleVortT<true", "k_row_r2c<cmm::SampleReal>"),
                 "psi_col": ("k_col(",), "psi_row_c2r": ("k_row_c2r4(",), "particles": ("k_particles(",),
                 # the split-precision form's own kernels (its column and C2R stages are the FP64 ones)
-                "advect_split": ("k_map_advect32<30, 8, true",), "psi_combine_split": ("k_psi_combine32<3>",),
+                "advect_split": ("k_map_advect32_pair<30, true>",), "psi_combine_split": ("k_psi_combine32<3>",),
                 "incomp_split": ("k_incomp_tensor32<",),
                 "psi_row_r2c_split": ("k_sample_field<cmm::SampleVort32", "k_row_r2c<cmm::SampleReal>")}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "cmm_euler_c2_hbm_traffic.json")
