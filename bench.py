@@ -702,6 +702,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-split", action="store_true", help="skip the split-precision form")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the short ensemble leg")
+    ap.add_argument("--no-config1", action="store_true", help="skip the double-shear-layer leg")
     ap.add_argument("--ensemble", action="store_true",
                     help="run the batched Vlasov-Poisson ensemble instead (its own JSON line)")
     ap.add_argument("--instances", type=int, default=32)
@@ -809,7 +810,7 @@ def main():
                          f"{threads} threads ({secs:.1f} s), and one remap through 33 maps on the 4096^2 grid "
                          f"timed alone; {total:.1f} s of CPU work in all"}
 
-    c1 = run_c1(cmm, torch, cpu["cores"] if cpu else None)
+    c1 = None if args.no_config1 else run_c1(cmm, torch, cpu["cores"] if cpu else None)
     ens = None if args.no_ensemble else ensemble_leg(cmm, torch)
 
     out = {

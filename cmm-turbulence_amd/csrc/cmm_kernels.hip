@@ -510,6 +510,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
     // output planes: footpoint m sits in lane m & 3 of the group, component m >> 2; lane k forms
     // plane k of X and of Y, summing in footpoint order like advect_store
     const int base = (threadIdx.x & 63) - k;
+    double out[2];
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
         const int o = k + r * G;
@@ -522,10 +523,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
             const double wm = plane == 0 ? c.c1[m] : plane == 1 ? c.cx[m] : plane == 2 ? c.cy[m] : c.cxy[m];
             acc += (o < 4 ? vx : vy) * wm;
         }
-        if (active) {
-            (o < 4 ? ncx : ncy)[(long)plane * gm.n + node] = acc;
-            if (M8out) M8out[8L * node + o] = acc;
-        }
+        if (active) (o < 4 ? ncx : ncy)[(long)plane * gm.n + node] = acc;
+        out[r] = acc;
+    }
+    // the record of the node (X planes, then Y planes) with one 16-byte store per lane: lane k
+    // stores entries 2k and 2k + 1, which lanes (2k) & 3 and (2k + 1) & 3 hold in component k >> 1
+    if (M8out) {
+        const double e0 = __shfl(out[0], base + ((2 * k) & 3), 64), e1 = __shfl(out[1], base + ((2 * k) & 3), 64);
+        const double f0 = __shfl(out[0], base + ((2 * k + 1) & 3), 64), f1 = __shfl(out[1], base + ((2 * k + 1) & 3), 64);
+        if (active)
+            reinterpret_cast<double2 *>(M8out + 8L * node)[k] = double2{k < 2 ? e0 : e1, k < 2 ? f0 : f1};
     }
 }
 
@@ -667,6 +674,7 @@ __global__ void __launch_bounds__(256) k_map_advect_vp(DGrid gm, DGrid gp, const
         diffeo_vals(gm, cb, vx, vy, px.b, py.b);
     }
     const int base = (threadIdx.x & 63) - k;
+    double out[2];
 #pragma unroll
     for (int r = 0; r < 2; ++r) {
         const int o = k + r * G;
@@ -679,10 +687,16 @@ __global__ void __launch_bounds__(256) k_map_advect_vp(DGrid gm, DGrid gp, const
             const double wm = plane == 0 ? c.c1[m] : plane == 1 ? c.cx[m] : plane == 2 ? c.cy[m] : c.cxy[m];
             acc += (o < 4 ? vx : vy) * wm;
         }
-        if (active) {
-            (o < 4 ? ncx : ncy)[(long)plane * gm.n + node] = acc;
-            if (M8out) M8out[8L * node + o] = acc;
-        }
+        if (active) (o < 4 ? ncx : ncy)[(long)plane * gm.n + node] = acc;
+        out[r] = acc;
+    }
+    // the record of the node (X planes, then Y planes) with one 16-byte store per lane: lane k
+    // stores entries 2k and 2k + 1, which lanes (2k) & 3 and (2k + 1) & 3 hold in component k >> 1
+    if (M8out) {
+        const double e0 = __shfl(out[0], base + ((2 * k) & 3), 64), e1 = __shfl(out[1], base + ((2 * k) & 3), 64);
+        const double f0 = __shfl(out[0], base + ((2 * k + 1) & 3), 64), f1 = __shfl(out[1], base + ((2 * k + 1) & 3), 64);
+        if (active)
+            reinterpret_cast<double2 *>(M8out + 8L * node)[k] = double2{k < 2 ? e0 : e1, k < 2 ? f0 : f1};
     }
 }
 
