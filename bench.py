@@ -482,14 +482,14 @@ def ensemble_leg(cmm, torch, n_inst=32, steps=100, warmup=24):
     out = {"workload": f"{n_inst} Vlasov-Poisson instances, maps 256^2/512^2/1024^2 (fine 4x, psi and vort 2x), "
                        f"Landau / two-stream, amplitudes U[0.005, 0.05] seed 0, dt 1/20, steps "
                        f"{warmup + 1}..{warmup + steps} after {warmup} untimed steps"}
-    for mode in ("graph_batch", "one_by_one"):
+    for mode in ("graph_batch", "eager_batch", "one_by_one"):
         es = []
         for nn, ic, amp in specs:
             e = cmm.Euler(g(nn), g(4 * nn), g(2 * nn), g(2 * nn), dt, initial_condition=ic, ic_params=[amp],
                           simulation_type="cmm_vlasov_poisson_1d", incomp_threshold=1e-4, stack_capacity=64)
             e.init()
             es.append(e)
-        b = cmm.Batch(es, use_graphs=True) if mode == "graph_batch" else None
+        b = cmm.Batch(es, use_graphs=mode == "graph_batch") if mode != "one_by_one" else None
         for _ in range(warmup):
             if b:
                 b.step(dt)
@@ -514,6 +514,8 @@ def ensemble_leg(cmm, torch, n_inst=32, steps=100, warmup=24):
             e.close()
         torch.cuda.synchronize()
     out["graph_batch_vs_one_by_one"] = round(out["graph_batch"]["ensemble_steps_per_s"] /
+                                             out["one_by_one"]["ensemble_steps_per_s"], 3)
+    out["eager_batch_vs_one_by_one"] = round(out["eager_batch"]["ensemble_steps_per_s"] /
                                              out["one_by_one"]["ensemble_steps_per_s"], 3)
     log(f"ensemble: {out}")
     return out

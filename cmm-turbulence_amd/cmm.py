@@ -684,7 +684,7 @@ def uniform_particles(n, seed, center=(TWO_PI / 2, TWO_PI / 2), width=(TWO_PI, T
 
 class Batch:
     """Engines stepped together (cmm_batch_*): one wait per step, and with graphs the queued work
-    of all engines replayed from one captured graph per step phase."""
+    of all engines replayed from one captured graph per step phase (a branch per engine)."""
 
     def __init__(self, engines, use_graphs=True):
         self.engines = list(engines)

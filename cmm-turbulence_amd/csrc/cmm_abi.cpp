@@ -377,13 +377,8 @@ struct cmm_euler {
     // weights follow the whole time history and a graph's kernel arguments are frozen
     bool lw_graph = false;
     double *lw_dev = nullptr, *lw_host = nullptr;
-    // Vlasov-Poisson: H(v) and H'(v) of the bump function (they depend on the grid only), and, in
-    // captured batch steps (vp_defer), the phi solve described in vp_job for one batched launch
-    // with the planes of psi written afterwards into vp_out
+    // Vlasov-Poisson: H(v) and H'(v) of the bump function (they depend on the grid only)
     double *vpH = nullptr;
-    bool vp_defer = false;
-    VPJob vp_job{};
-    double *vp_out = nullptr;
     IncompTune tune;  // the check's tile and grid, from cmm_euler_autotune
     // corrections (device Hermite fields on g_psi) added to the Lagrange extrapolations of psi
     // to t_{n+1} and t_n + dt/2 in the next steps (cmm_euler_set_psi_correction)
@@ -499,9 +494,7 @@ int engine_psi(cmm_euler *e, double *out, hipEvent_t *ev = nullptr) {
         if (int rc = sync_planes(e)) return rc;
         if (ev) (void)hipEventRecord(ev[0], e->st);
         int rc = op_vp_psi_from_f(e->p.g_coarse, e->p.g_vort, e->p.g_fine, e->p.g_psi, e->cx, e->cy, e->W,
-                                  e->p.molly_stencil, out, e->work, e->work_bytes, e->st, e->W4, e->vpH,
-                                  e->vp_defer ? &e->vp_job : nullptr);
-        if (e->vp_defer) e->vp_out = out;
+                                  e->p.molly_stencil, out, e->work, e->work_bytes, e->st, e->W4, e->vpH);
         if (ev)
             for (int i = 1; i < 4; ++i) (void)hipEventRecord(ev[i], e->st);
         return rc;
@@ -1283,26 +1276,22 @@ int cmm_euler_step(cmm_euler *e, double dt_now, cmm_step_info *info) {
 struct cmm_batch {
     std::vector<cmm_euler *> e;
     bool graphs = true;
+    // A graph step is one captured graph launched on `master`: it forks to every engine stream
+    // (one branch per engine) and joins them back.  A branch holds the engine's whole step: it
+    // waits for tail[i] (the engine stream's work before the step: the previous step, a remap),
+    // queues the advection, the check and the E_inc copy, records ev_ak[i] (external event
+    // node, which the host waits for), queues the psi history and psi solve and records ev_bk[i],
+    // which the engine stream waits for.  So each engine's kernels follow each other as in an
+    // eager step, the host learns E_inc engine by engine while psi solves still run, and an
+    // engine's next step starts when its own psi solve is done.  One graph per (step phase, state
+    // flags of every engine): the psi ring and the map ping-pong rotate device pointers with
+    // periods dividing 12, so a graph replays exactly the launches of the step it was captured on.
     hipStream_t master = nullptr;
     hipEvent_t fork = nullptr;
     std::vector<hipEvent_t> join;
-    // A graph step is two captured graphs launched back to back on the master stream: phase A
-    // (advection, check, E_inc copy of every engine) and phase B (psi history and psi solve).
-    // The host waits for ev_a only: with E_inc known, and no remap due, the next step is
-    // prepared and launched while phase B still runs.  One graph per (phase, step phase, state
-    // flags of every engine): the psi ring and the map ping-pong rotate device pointers with
-    // periods dividing 12, so a graph replays exactly the launches of the step it was captured on.
     std::map<std::string, hipGraphExec_t> cache;
-    hipEvent_t ev_a = nullptr, ev_b = nullptr;
-    std::vector<hipEvent_t> tail;  // per engine: the end of its stream's work before a graph step
+    std::vector<hipEvent_t> tail, ev_ak, ev_bk;
     int captured = 0, launched = 0, eager = 0;
-    // Vlasov-Poisson instances: their phi solves in one launch per captured step (one workgroup
-    // each), between a second fork and join of the engine streams
-    hipEvent_t fork2 = nullptr;
-    std::vector<hipEvent_t> join2;
-    VPJob *jobs_dev = nullptr;
-    std::vector<VPJob> jobs_up;  // what jobs_dev holds (once jobs_uploaded)
-    bool jobs_uploaded = false;
     // every engine's Lagrange weights (8 doubles each) in one pinned and one device array, so
     // that a graph step copies them with one transfer; the engines point into them while batched
     double *lw_host = nullptr, *lw_dev = nullptr;
@@ -1310,8 +1299,8 @@ struct cmm_batch {
 
 namespace {
 
-std::string batch_key(const cmm_batch *b, char phase) {
-    std::string k(1, phase);
+std::string batch_key(const cmm_batch *b) {
+    std::string k;
     for (const cmm_euler *e : b->e) {
         k += std::to_string(e->loop % 12);
         k += e->m8_valid ? 'v' : '-';
@@ -1333,83 +1322,74 @@ int batch_normalise(cmm_euler *e) {
     return CMM_OK;
 }
 
-bool is_vp(const cmm_euler *e) { return e->p.sim == CMM_SIM_VLASOV_POISSON_1D; }
+// An event record (record) or wait node on the capture of stream st, depending on the stream's
+// current capture leaves and becoming its only leaf: the engines' branches of the two phase graphs
+// are chained by such nodes (they record and wait in launch order, across graph launches)
+int capture_event_node(hipStream_t st, hipEvent_t ev, bool record) {
+    hipStreamCaptureStatus status = hipStreamCaptureStatusNone;
+    unsigned long long id = 0;
+    hipGraph_t graph = nullptr;
+    const hipGraphNode_t *deps = nullptr;
+    size_t n_deps = 0;
+    if (int rc = hip_check(hipStreamGetCaptureInfo_v2(st, &status, &id, &graph, &deps, &n_deps), "hipStreamGetCaptureInfo"))
+        return rc;
+    if (status != hipStreamCaptureStatusActive || !graph) {
+        set_error("batch capture: engine stream not capturing");
+        return CMM_ERR_HIP;
+    }
+    std::vector<hipGraphNode_t> d(deps, deps + n_deps);
+    hipGraphNode_t node = nullptr;
+    if (record) {
+        if (int rc = hip_check(hipGraphAddEventRecordNode(&node, graph, d.data(), d.size(), ev), "hipGraphAddEventRecordNode"))
+            return rc;
+    } else {
+        if (int rc = hip_check(hipGraphAddEventWaitNode(&node, graph, d.data(), d.size(), ev), "hipGraphAddEventWaitNode"))
+            return rc;
+    }
+    return hip_check(hipStreamUpdateCaptureDependencies(st, &node, 1, hipStreamSetCaptureDependencies),
+                     "hipStreamUpdateCaptureDependencies");
+}
 
-// Phase A or B of every engine's step, captured from the master stream into one graph: the master
-// forks to every engine stream (one branch per engine) and joins them back.  In phase B the
-// Vlasov-Poisson instances' phi solves run as one launch after a join, then their planes.
-int batch_capture(cmm_batch *b, double dt, char phase, hipGraphExec_t *out) {
+// The step of every engine captured into one graph from the master stream: a branch per engine
+// stream, opened by a wait for the engine's tail event, with a record node after the E_inc copy
+// and one at the end (external event nodes), joined back into the master stream.
+int batch_capture(cmm_batch *b, double dt, hipGraphExec_t *out) {
     int rc = hip_check(hipStreamBeginCapture(b->master, hipStreamCaptureModeRelaxed), "hipStreamBeginCapture");
     if (rc) return rc;
-    rc = hip_check(hipEventRecord(b->fork, b->master), "hipEventRecord");
-    std::vector<VPJob> jobs;
-    int max_n = 0;
+    rc = hip_check(hipEventRecord(b->fork, b->master), "hipEventRecord (fork)");
     for (size_t i = 0; i < b->e.size() && !rc; ++i) {
         cmm_euler *e = b->e[i];
-        rc = hip_check(hipStreamWaitEvent(e->st, b->fork, 0), "hipStreamWaitEvent");
+        rc = hip_check(hipStreamWaitEvent(e->st, b->fork, 0), "hipStreamWaitEvent (fork)");
+        if (!rc) rc = capture_event_node(e->st, b->tail[i], false);
         e->capturing = true;
-        if (phase == 'A') {
-            bool fork = false;
-            if (!rc) rc = step_queue_a(e, dt, false, &fork);
-        } else {
-            e->vp_defer = is_vp(e);
-            e->vp_out = nullptr;
-            if (!rc) rc = step_queue_b(e, false);
-            e->vp_defer = false;
-            if (!rc && is_vp(e)) {
-                if (!e->vp_out) { set_error("cmm_batch_step: psi solve not deferred"); rc = CMM_ERR_ARG; }
-                jobs.push_back(e->vp_job);
-                max_n = std::max({max_n, e->vp_job.q.nv, e->vp_job.q.np});
-            }
-        }
+        bool fork = false;
+        if (!rc) rc = step_queue_a(e, dt, false, &fork);
+        e->lw_graph = false;
+        if (!rc) rc = capture_event_node(e->st, b->ev_ak[i], true);
+        if (!rc) rc = step_queue_b(e, false);
         e->capturing = false;
-        if (!rc) rc = hip_check(hipEventRecord(b->join[i], e->st), "hipEventRecord");
-        if (!rc) rc = hip_check(hipStreamWaitEvent(b->master, b->join[i], 0), "hipStreamWaitEvent");
+        if (!rc) rc = capture_event_node(e->st, b->ev_bk[i], true);
+        if (!rc) rc = hip_check(hipEventRecord(b->join[i], e->st), "hipEventRecord (join)");
+        if (!rc) rc = hip_check(hipStreamWaitEvent(b->master, b->join[i], 0), "hipStreamWaitEvent (join)");
     }
-    // the phi solves of all Vlasov-Poisson instances in one launch, then each instance's psi planes
-    // on its own stream again.  The job table is the same at every step (workspace and twiddle
-    // addresses); it is uploaded before the first launch of the graph
-    if (!rc && !jobs.empty()) {
-        if (!b->jobs_dev) {
-            if (hipMalloc(&b->jobs_dev, sizeof(VPJob) * b->e.size()) != hipSuccess) {
-                b->jobs_dev = nullptr;
-                set_error("hipMalloc batch jobs");
-                rc = CMM_ERR_ALLOC;
-            }
-        }
-        if (!rc) rc = op_vp_phi_batch(b->jobs_dev, (int)jobs.size(), max_n, b->master);
-        if (!rc) rc = hip_check(hipEventRecord(b->fork2, b->master), "hipEventRecord");
-        for (size_t i = 0; i < b->e.size() && !rc; ++i) {
-            cmm_euler *e = b->e[i];
-            if (!is_vp(e)) continue;
-            rc = hip_check(hipStreamWaitEvent(e->st, b->fork2, 0), "hipStreamWaitEvent");
-            if (!rc) rc = op_vp_psi_planes(e->p.g_psi, e->vp_job.phi, e->vpH, e->vp_out, e->st);
-            if (!rc) rc = hip_check(hipEventRecord(b->join2[i], e->st), "hipEventRecord");
-            if (!rc) rc = hip_check(hipStreamWaitEvent(b->master, b->join2[i], 0), "hipStreamWaitEvent");
-        }
-        if (!rc && (b->jobs_up.size() != jobs.size() ||
-                    std::memcmp(b->jobs_up.data(), jobs.data(), sizeof(VPJob) * jobs.size()) != 0)) {
-            b->jobs_up = jobs;
-            b->jobs_uploaded = false;
-        }
-    }
+    // the capture always ends; a failure inside it keeps its own message
     hipGraph_t g = nullptr;
-    const int rc_end = hip_check(hipStreamEndCapture(b->master, &g), "hipStreamEndCapture");
-    if (!rc) rc = rc_end;
+    const hipError_t er = hipStreamEndCapture(b->master, &g);
+    if (!rc) rc = hip_check(er, "hipStreamEndCapture");
     if (!rc) rc = hip_check(hipGraphInstantiate(out, g, nullptr, nullptr, 0), "hipGraphInstantiate");
     if (g) (void)hipGraphDestroy(g);
     return rc;
 }
 
-// one phase of a graph step: the cached graph of this state, or a capture of it; with a cached graph
-// the engines only advance their host state (dry pass) and the graph carries the launches
-int batch_phase(cmm_batch *b, double dt, char phase, hipGraphExec_t *exec) {
-    const std::string key = batch_key(b, phase);
+// the graph of this step's state, cached or captured; with a cached graph the engines only advance
+// their host state (dry pass) and the graph carries the launches
+int batch_graph(cmm_batch *b, double dt, hipGraphExec_t *exec) {
+    const std::string key = batch_key(b);
     auto it = b->cache.find(key);
-    for (cmm_euler *e : b->e) e->lw_graph = phase == 'A';
+    for (cmm_euler *e : b->e) e->lw_graph = true;
     int rc = CMM_OK;
     if (it == b->cache.end()) {
-        rc = batch_capture(b, dt, phase, exec);
+        rc = batch_capture(b, dt, exec);
         if (!rc) {
             b->cache[key] = *exec;
             b->captured++;
@@ -1418,12 +1398,10 @@ int batch_phase(cmm_batch *b, double dt, char phase, hipGraphExec_t *exec) {
         *exec = it->second;
         for (cmm_euler *e : b->e) {
             e->dry = true;
-            if (phase == 'A') {
-                bool fork = false;
-                rc = step_queue_a(e, dt, false, &fork);
-            } else {
-                rc = step_queue_b(e, false);
-            }
+            bool fork = false;
+            rc = step_queue_a(e, dt, false, &fork);
+            e->lw_graph = false;
+            if (!rc) rc = step_queue_b(e, false);
             e->dry = false;
             if (rc) break;
         }
@@ -1451,15 +1429,13 @@ int cmm_batch_create(cmm_euler *const *engines, int n, int use_graphs, cmm_batch
     b->graphs = use_graphs != 0;
     int rc = hip_check(hipStreamCreateWithFlags(&b->master, hipStreamNonBlocking), "hipStreamCreate");
     if (!rc) rc = hip_check(hipEventCreateWithFlags(&b->fork, kDeviceEvent), "hipEventCreate");
-    if (!rc) rc = hip_check(hipEventCreateWithFlags(&b->fork2, kDeviceEvent), "hipEventCreate");
-    if (!rc) rc = hip_check(hipEventCreateWithFlags(&b->ev_a, hipEventDisableTiming), "hipEventCreate");
-    if (!rc) rc = hip_check(hipEventCreateWithFlags(&b->ev_b, kDeviceEvent), "hipEventCreate");
-    b->join.assign(n, nullptr);
-    b->join2.assign(n, nullptr);
-    b->tail.assign(n, nullptr);
-    for (int i = 0; i < n && !rc; ++i) rc = hip_check(hipEventCreateWithFlags(&b->join[i], kDeviceEvent), "hipEventCreate");
-    for (int i = 0; i < n && !rc; ++i) rc = hip_check(hipEventCreateWithFlags(&b->join2[i], kDeviceEvent), "hipEventCreate");
-    for (int i = 0; i < n && !rc; ++i) rc = hip_check(hipEventCreateWithFlags(&b->tail[i], kDeviceEvent), "hipEventCreate");
+    for (auto *v : {&b->join, &b->tail, &b->ev_bk}) {
+        v->assign(n, nullptr);
+        for (int i = 0; i < n && !rc; ++i) rc = hip_check(hipEventCreateWithFlags(&(*v)[i], kDeviceEvent), "hipEventCreate");
+    }
+    // the host waits for these after the E_inc copies to pinned memory: system-scope release
+    b->ev_ak.assign(n, nullptr);
+    for (int i = 0; i < n && !rc; ++i) rc = hip_check(hipEventCreateWithFlags(&b->ev_ak[i], hipEventDisableTiming), "hipEventCreate");
     if (!rc && hipHostMalloc((void **)&b->lw_host, sizeof(double) * 8 * n, hipHostMallocDefault) != hipSuccess) {
         b->lw_host = nullptr;
         set_error("hipHostMalloc batch weights");
@@ -1489,12 +1465,10 @@ int cmm_batch_destroy(cmm_batch *b) {
             e->lw_host = reinterpret_cast<double *>(e->h_slot + 8);
         }
     for (auto &kv : b->cache) (void)hipGraphExecDestroy(kv.second);
-    for (auto *v : {&b->join, &b->join2, &b->tail})
+    for (auto *v : {&b->join, &b->tail, &b->ev_ak, &b->ev_bk})
         for (hipEvent_t ev : *v)
             if (ev) (void)hipEventDestroy(ev);
-    for (hipEvent_t ev : {b->fork, b->fork2, b->ev_a, b->ev_b})
-        if (ev) (void)hipEventDestroy(ev);
-    if (b->jobs_dev) (void)hipFree(b->jobs_dev);
+    if (b->fork) (void)hipEventDestroy(b->fork);
     if (b->lw_dev) (void)hipFree(b->lw_dev);
     if (b->lw_host) (void)hipHostFree(b->lw_host);
     if (b->master) (void)hipStreamDestroy(b->master);
@@ -1510,47 +1484,23 @@ int cmm_batch_step(cmm_batch *b, double dt, cmm_step_info *infos) {
     bool graph = b->graphs;
     for (cmm_euler *e : b->e) graph = graph && e->loop >= 1 && dt == e->p.dt;
     if (graph) {
-        // the master stream runs after every engine stream's earlier work (remaps, the previous
-        // step's phase B), ordered on the device
-        for (size_t i = 0; i < b->e.size(); ++i) {
-            if (int rc = hip_check(hipEventRecord(b->tail[i], b->e[i]->st), "hipEventRecord")) return rc;
-            if (int rc = hip_check(hipStreamWaitEvent(b->master, b->tail[i], 0), "hipStreamWaitEvent")) return rc;
-        }
-        hipGraphExec_t ga = nullptr, gb = nullptr;
-        if (int rc = batch_phase(b, dt, 'A', &ga)) return rc;
-        // the phi job table, written at capture and the same for every step
-        if (!b->jobs_up.empty() && !b->jobs_uploaded) {
-            if (int rc = hip_check(hipMemcpy(b->jobs_dev, b->jobs_up.data(), sizeof(VPJob) * b->jobs_up.size(),
-                                             hipMemcpyHostToDevice),
-                                   "hipMemcpy jobs"))
-                return rc;
-            b->jobs_uploaded = true;
-        }
-        // this step's Lagrange weights, in stream order before phase A reads them (the previous
-        // step's phase A, their last reader, has completed: the host waited for it)
+        // each engine's stream work so far (the previous step, a remap): the engine's branch
+        // waits for it
+        for (size_t i = 0; i < b->e.size(); ++i)
+            if (int rc = hip_check(hipEventRecord(b->tail[i], b->e[i]->st), "hipEventRecord (tail)")) return rc;
+        hipGraphExec_t g = nullptr;
+        if (int rc = batch_graph(b, dt, &g)) return rc;
+        // this step's Lagrange weights, in stream order before the graph reads them (their last
+        // readers, the previous step's advections, have completed: the host waited for ev_ak)
         if (int rc = hip_check(hipMemcpyAsync(b->lw_dev, b->lw_host, sizeof(double) * 8 * b->e.size(),
                                               hipMemcpyHostToDevice, b->master),
                                "hipMemcpyAsync weights"))
             return rc;
-        if (int rc = hip_check(hipGraphLaunch(ga, b->master), "hipGraphLaunch")) return rc;
-        if (int rc = hip_check(hipEventRecord(b->ev_a, b->master), "hipEventRecord")) return rc;
-        if (int rc = batch_phase(b, dt, 'B', &gb)) return rc;
-        if (!b->jobs_up.empty() && !b->jobs_uploaded) {
-            // first capture of phase B: the table goes up before the graph's first launch
-            if (int rc = hip_check(hipStreamSynchronize(b->master), "hipStreamSynchronize")) return rc;
-            if (int rc = hip_check(hipMemcpy(b->jobs_dev, b->jobs_up.data(), sizeof(VPJob) * b->jobs_up.size(),
-                                             hipMemcpyHostToDevice),
-                                   "hipMemcpy jobs"))
-                return rc;
-            b->jobs_uploaded = true;
-        }
-        if (int rc = hip_check(hipGraphLaunch(gb, b->master), "hipGraphLaunch")) return rc;
-        if (int rc = hip_check(hipEventRecord(b->ev_b, b->master), "hipEventRecord")) return rc;
-        // every engine stream's later work (remaps, field reads, the next eager step) after phase B
-        for (cmm_euler *e : b->e)
-            if (int rc = hip_check(hipStreamWaitEvent(e->st, b->ev_b, 0), "hipStreamWaitEvent")) return rc;
-        // E_inc of every engine is in pinned memory once phase A has run
-        if (int rc = hip_check(hipEventSynchronize(b->ev_a), "hipEventSynchronize")) return rc;
+        if (int rc = hip_check(hipGraphLaunch(g, b->master), "hipGraphLaunch")) return rc;
+        // every engine stream's later work (a remap, field reads, the next eager step) after its
+        // own branch
+        for (size_t i = 0; i < b->e.size(); ++i)
+            if (int rc = hip_check(hipStreamWaitEvent(b->e[i]->st, b->ev_bk[i], 0), "hipStreamWaitEvent")) return rc;
         b->launched++;
     } else {
         for (cmm_euler *e : b->e)
@@ -1559,6 +1509,9 @@ int cmm_batch_step(cmm_batch *b, double dt, cmm_step_info *infos) {
     }
     for (size_t i = 0; i < b->e.size(); ++i) {
         cmm_euler *e = b->e[i];
+        // E_inc of the engine is in pinned memory once its branch has passed ev_ak
+        if (graph)
+            if (int rc = hip_check(hipEventSynchronize(b->ev_ak[i]), "hipEventSynchronize")) return rc;
         if (int rc = step_finish(e, infos ? &infos[i] : nullptr, graph)) return rc;
         if (infos && infos[i].remapped)
             if (int rc = batch_normalise(e)) return rc;

@@ -98,12 +98,10 @@ struct VPJob {
 };
 size_t op_vp_psi_workspace(const cmm_grid &gv, const cmm_grid &gp);
 // W4 (optional): the fine field node-interleaved, for the column sampler; H (optional): the bump
-// function's H and H' (op_vp_bump), else computed into the workspace; defer (optional): the phi
-// solve is not launched but described in *defer for op_vp_phi_batch, and the planes are left to
-// op_vp_psi_planes
+// function's H and H' (op_vp_bump), else computed into the workspace
 int op_vp_psi_from_f(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf, const cmm_grid &gp, const double *cx,
                      const double *cy, const double *W, int molly, double *psi, void *work, size_t work_bytes,
-                     hipStream_t st, const double *W4 = nullptr, const double *H = nullptr, VPJob *defer = nullptr);
+                     hipStream_t st, const double *W4 = nullptr, const double *H = nullptr);
 // H(v) at [0, np) and H'(v) at [2 np, 3 np) of H (3 np doubles): the grid-only part of psi
 int op_vp_bump(const cmm_grid &gv, const cmm_grid &gp, double *H, hipStream_t st);
 int op_vp_psi_planes(const cmm_grid &gp, const double *phi, const double *H, double *psi, hipStream_t st);
@@ -118,9 +116,6 @@ int op_map_advect_vp(const cmm_grid &gm, const cmm_grid &gp, const PsiLevels &P,
                      const double *cx, const double *cy, const double *m8_in, double *m8_out, const double *H,
                      double *ncx, double *ncy, void *work, size_t work_bytes, hipStream_t st, hipEvent_t mid,
                      const double *lw);
-// phi of n_jobs instances (jobs_dev: device array of VPJob), one workgroup each; max_n the largest
-// vort or psi grid width among them
-int op_vp_phi_batch(const VPJob *jobs_dev, int n_jobs, int max_n, hipStream_t st);
 int op_spectral_from_real(const cmm_grid &g, const double *real, int op, double *herm, void *work, size_t work_bytes,
                           hipStream_t st);
 int op_sample_vort(const cmm_grid &gm, const cmm_grid &gt, const cmm_grid &gf, const double *cx, const double *cy,

@@ -1957,14 +1957,6 @@ __global__ void __launch_bounds__(kFftThreads) k_vp_phi(VPJob job) {
     vp_phi_block(job, s);
 }
 
-// phi of several instances in one launch, one workgroup per instance: the 1D solves are latency
-// bound, and one at a time they leave the GPU idle
-__global__ void __launch_bounds__(kFftThreads) k_vp_phi_batch(const VPJob *__restrict__ jobs) {
-    extern __shared__ cd s[];
-    const VPJob job = jobs[blockIdx.x];
-    vp_phi_block(job, s);
-}
-
 // the bump function over v on the psi grid, centred and scaled to maximum 1, its 1D Poisson solve
 // H and the derivative H'; into H[0, np) and H[2 np, 3 np) (the layout of phi's buffer).  It
 // depends on the grid only.
@@ -2843,7 +2835,7 @@ int op_vp_bump(const cmm_grid &gv, const cmm_grid &gp, double *H, hipStream_t st
 
 int op_vp_psi_from_f(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf, const cmm_grid &gp, const double *cx,
                      const double *cy, const double *W, int molly, double *psi, void *work, size_t work_bytes,
-                     hipStream_t st, const double *W4, const double *H, VPJob *defer) {
+                     hipStream_t st, const double *W4, const double *H) {
     if (int rc = vp_check(gv, gp)) return rc;
     void *w;
     if (int rc = get_work(work, work_bytes, op_vp_psi_workspace(gv, gp), &w)) return rc;
@@ -2889,11 +2881,6 @@ int op_vp_psi_from_f(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf,
         if (int rc = launch_check("k_vp_bump")) return rc;
     }
     VPJob job{q, f, part, tw_v, tw_p, phiH};
-    if (defer) {
-        // phi(x) of this instance goes into a batched launch (op_vp_phi_batch) with the others
-        *defer = job;
-        return CMM_OK;
-    }
     const size_t lds = sizeof(cd) * std::max(gv.nx, gp.nx);
     if (int rc = set_lds((const void *)k_vp_phi, lds)) return rc;
     hipLaunchKernelGGL(k_vp_phi, dim3(1), dim3(kFftThreads), lds, st, job);
@@ -2936,14 +2923,6 @@ int op_vp_psi_planes(const cmm_grid &gp, const double *phi, const double *H, dou
     hipLaunchKernelGGL(k_vp_psi_planes, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, st, phi, H, gp.nx,
                        ilog2(gp.nx), psi);
     return launch_check("k_vp_psi_planes");
-}
-
-int op_vp_phi_batch(const VPJob *jobs_dev, int n_jobs, int max_n, hipStream_t st) {
-    if (n_jobs <= 0) return CMM_OK;
-    const size_t lds = sizeof(cd) * max_n;
-    if (int rc = set_lds((const void *)k_vp_phi_batch, lds)) return rc;
-    hipLaunchKernelGGL(k_vp_phi_batch, dim3(n_jobs), dim3(kFftThreads), lds, st, jobs_dev);
-    return launch_check("k_vp_phi_batch");
 }
 
 int op_sample_vort(const cmm_grid &gm, const cmm_grid &gt, const cmm_grid &gf, const double *cx, const double *cy,

@@ -392,12 +392,12 @@ int cmm_euler_profile_read(cmm_euler *e, double *ms_out, int *count_out);
 /* ---- ensembles (no counterpart in the reference) ------------------------------------------------
  * Engines of any grid sizes stepped together: each step queues every engine's advection, check,
  * E_inc copy and psi solve, then waits once and finishes each engine (remap when E_inc exceeds its
- * threshold, in stream order as cmm_euler_step does).  With use_graphs, a step is two captured
- * graphs (one branch per engine stream), kept per step phase and replayed: the advection, check and
- * E_inc copies, then the psi solves, the Vlasov-Poisson instances' 1D solves in one launch.  The
- * call returns once E_inc is known, while the psi solves may still run: every later use of an
- * engine (its calls, the next step) is ordered after them on the device.  Steps of another length
- * than the engines' nominal dt (a clipped last step) and each engine's first step run without a
+ * threshold, in stream order as cmm_euler_step does).  With use_graphs, a step is one captured
+ * graph (one branch per engine stream, holding that engine's whole step), kept per step phase
+ * and replayed; the host waits for each engine's E_inc copy inside its branch, while the psi
+ * solves may still run, and every later use of an engine (its calls, the next step) is ordered
+ * after its own branch on the device.  Steps of another length than the engines' nominal dt (a
+ * clipped last step) and each engine's first step run without a
  * graph.  Engines must be initialised, without particles, a forward map or profiling; results equal
  * cmm_euler_step's. */
 typedef struct cmm_batch cmm_batch;
