@@ -66,12 +66,13 @@ def algorithmic(slot, c, kx_max, n_stack=0):
         return 2 * 16 * (kx_max + 1) * c["n_psi"] + 32 * npsi, 2 * 5.0 * npsi * 11
     if slot == "particles":
         return 16 * c["n_particles"], 0.0
-    # split-precision form: 48-byte map records, 16-byte records of the combined psi fields plus
-    # their 4-byte low-part planes, 16-byte records of the fine vorticity
+    # split-precision form: 48-byte map records, 16-byte records of the combined psi fields (the
+    # value's low part coded in the cross derivative's low byte), 16-byte records of the fine
+    # vorticity
     if slot == "advect_split":
-        return 48 * nc + 20 * 3 * npsi + 48 * nc, 9.2e3 * nc
+        return 48 * nc + 16 * 3 * npsi + 48 * nc, 9.2e3 * nc
     if slot == "psi_combine_split":
-        return 32 * L * npsi + 20 * 3 * npsi, 2 * 2 * L * 4 * npsi
+        return 32 * L * npsi + 16 * 3 * npsi, 2 * 2 * L * 4 * npsi
     if slot == "incomp_split":
         return 48 * nc, 85.0 * nf
     if slot == "psi_row_r2c_split":
@@ -763,8 +764,8 @@ def main():
         rli = lambda a, b: float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
         dpart = np.abs(sp["part"] - plain["part"])
         dpart = np.minimum(dpart, 2 * np.pi - dpart)
-        chosen = {"form": "split precision (map = identity + FP32 displacement, FP32 combined psi records with "
-                          "low-part value planes, FP32 fine omega records, FP64 arithmetic, FP64 psi levels / FFT / "
+        chosen = {"form": "split precision (map = identity + FP32 displacement, FP32 combined psi records "
+                          "keeping the values' low parts, FP32 fine omega records, FP64 arithmetic, FP64 psi levels / FFT / "
                           "Poisson divide)",
                   "value": round(sp["sps"], 3), "ms_per_step": round(1e3 * sp["elapsed"] / args.steps, 4),
                   "speedup": round(sp["sps"] / sps, 3),

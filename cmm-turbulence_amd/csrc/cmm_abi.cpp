@@ -523,10 +523,11 @@ int engine_advect(cmm_euler *e, const double *t, const double *dt, int loop_ctr,
         if (int rc = sync_m32(e)) return rc;
         if (!e->dry)
             // the Vlasov-Poisson stream function -H(v) + phi(x) has a large smooth part in v: its
-            // combined fields stay FP64 records; Euler's are FP32 records with low-part planes
+            // combined fields stay FP64 records; Euler's are packed FP32 records keeping the values'
+            // low parts
             if (int rc = op_map_advect32(e->p.g_coarse, e->p.g_psi, P, c, e->m32[e->m32_cur], e->m32[1 - e->m32_cur],
                                          e->work, e->work_bytes, e->st, e->prof ? e->ev[10] : nullptr,
-                                         e->p.sim == CMM_SIM_VLASOV_POISSON_1D ? PsiRec64 : PsiRec32Lo,
+                                         e->p.sim == CMM_SIM_VLASOV_POISSON_1D ? PsiRec64 : PsiRec32Pk,
                                          e->initialised ? e->corr1 : nullptr,
                                          e->initialised ? e->corr12 : nullptr, lw))
                 return rc;

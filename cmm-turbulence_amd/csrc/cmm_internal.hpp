@@ -140,14 +140,15 @@ int op_map_from32(const cmm_grid &g, const float *M, double *cx, double *cy, hip
 int op_interleave4_32(const cmm_grid &g, const double *H, float *H4, hipStream_t st);
 bool op_advect32_supported(const cmm_advect_coeffs &c);
 // The form of the Lagrange-combined psi fields that the split advection gathers: FP32 records
-// (value, d/dx, d/dy, d2/dxdy), the same with a plane of the values' low parts (the value of a
-// stream function of order one loses 2^-24 |psi| in FP32, which the velocity's 6/h amplifies), or
+// (value, d/dx, d/dy, d2/dxdy), the same with an 8-bit code of the value's low part in the low byte
+// of d2/dxdy (the value of a stream function of order one loses 2^-24 |psi| in FP32, which the
+// velocity's 6/h amplifies), or
 // FP64 records (the Vlasov-Poisson flow, whose -H(v) is large and smooth).
-enum PsiRecForm { PsiRec32 = 0, PsiRec32Lo = 1, PsiRec64 = 2 };
-size_t op_advect32_workspace(const cmm_grid &gp, const cmm_advect_coeffs &c, int psi_form = PsiRec32Lo);
+enum PsiRecForm { PsiRec32 = 0, PsiRec32Pk = 1, PsiRec64 = 2 };
+size_t op_advect32_workspace(const cmm_grid &gp, const cmm_advect_coeffs &c, int psi_form = PsiRec32Pk);
 int op_map_advect32(const cmm_grid &gm, const cmm_grid &gp, const PsiLevels &P, const cmm_advect_coeffs &c,
                     const float *M_in, float *M_out, void *work, size_t work_bytes, hipStream_t st,
-                    hipEvent_t mid = nullptr, int psi_form = PsiRec32Lo, const double *corr1 = nullptr,
+                    hipEvent_t mid = nullptr, int psi_form = PsiRec32Pk, const double *corr1 = nullptr,
                     const double *corr12 = nullptr, const double *lw = nullptr);
 int op_incomp32_launch(const cmm_grid &gm, const cmm_grid &gc, const float *M, unsigned long long *slot,
                        hipStream_t st, IncompTune tune = IncompTune());

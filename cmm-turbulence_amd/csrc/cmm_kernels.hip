@@ -3075,48 +3075,120 @@ __device__ __forceinline__ double incomp32_point(const DGrid &g, const float4 *_
     return fabs(a + d + (a * d - b * cc));
 }
 
-// velocity sources of the footpoint integration on FP32 combined fields (see VelCombined).  LO:
-// each field carries a plane of the low parts of its values (FP32 records hold the value's
-// leading FP32 part): the value plane of a stream function with |psi| of order one rounded to
-// FP32 would move the velocity by 2^-24 |psi| 6/h, which the check's 1/h amplifies again.
-template <bool P2, bool LO>
+// FP32 Hermite records whose value carries a low part (PsiRec32Pk): the value's leading FP32
+// part sits in .x and an 8-bit signed code q of its remainder in the low byte of .w, whose upper
+// 24 bits hold the cross derivative rounded to 15 explicit mantissa bits.  The remainder
+// f - (float)f is at most half an FP32 ulp of the leading part; q counts it in units of 1/256 of
+// that ulp (clamped to 127), so the value is kept to 2^-31 of itself or better.  The value plane
+// of a stream function with |psi| of order one rounded to FP32 alone would move the velocity by
+// 2^-24 |psi| 6/h, which the check's 1/h amplifies again; the cross derivative enters the
+// velocity with weight h/7 or less, where its 2^-16 rounding stays at the level of the first
+// derivatives' FP32 rounding.
+__device__ __forceinline__ float4 pack_psi_rec(const double o[4]) {
+    const float hi = (float)o[0];
+    const unsigned e = (__float_as_uint(hi) >> 23) & 0xFFu;
+    // 256 / ulp(hi) = 2^(158 - e), exact as a double for every biased exponent e
+    const double inv = __longlong_as_double((long long)(1023 + 158 - (int)e) << 52);
+    int q = (int)rint((o[0] - (double)hi) * inv);
+    q = q < -128 ? -128 : q > 127 ? 127 : q;
+    unsigned w = __float_as_uint((float)o[3]);
+    w = ((w + 0x80u) & 0xFFFFFF00u) | ((unsigned)q & 0xFFu);
+    return float4{hi, (float)o[1], (float)o[2], __uint_as_float(w)};
+}
+
+__device__ __forceinline__ void unpack_psi_rec(const float4 a, double &f, double &fxy) {
+    const unsigned w = __float_as_uint(a.w);
+    const unsigned e = (__float_as_uint(a.x) >> 23) & 0xFFu;
+    const double s = __longlong_as_double((long long)(e + 865u) << 52);  // ulp(hi) / 256 = 2^(e - 158)
+    f = (double)a.x + (double)(int)(signed char)(w & 0xFFu) * s;
+    fxy = (double)__uint_as_float(w & 0xFFFFFF00u);
+}
+
+// the four corner records of a cell
+__device__ __forceinline__ void load4_rec(const float4 *__restrict__ H, const Cell &c, float4 r[4]) {
+    r[0] = H[c.i00];
+    r[1] = H[c.i10];
+    r[2] = H[c.i01];
+    r[3] = H[c.i11];
+}
+
+// herm_vel_vals on the four corner records of a cell, each converted where it is used: the same
+// terms summed in the same order (corners 00, 10, 01, 11 within each plane's partial sum), with
+// four float4 records live instead of 16 doubles; PK: packed records (pack_psi_rec)
+template <bool PK>
+__device__ __forceinline__ void herm_vel_rec(const DGrid &g, const Cell &c, const float4 r[4], double &ux, double &uy) {
+    double bx[4], by[4], dbx[4], dby[4];
+    basis(c.dx, bx);
+    basis(c.dy, by);
+    basis_d(c.dx, dbx);
+    basis_d(c.dy, dby);
+    double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0, b0 = 0.0, b1 = 0.0, b2 = 0.0, b3 = 0.0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int xa = q & 1, yb = q >> 1;
+        double f, fxy;
+        if constexpr (PK) {
+            unpack_psi_rec(r[q], f, fxy);
+        } else {
+            f = r[q].x;
+            fxy = r[q].w;
+        }
+        const double fx = r[q].y, fy = r[q].z;
+        a0 += by[yb] * dbx[xa] * f;
+        a1 += by[yb] * dbx[2 + xa] * fx;
+        a2 += by[2 + yb] * dbx[xa] * fy;
+        a3 += by[2 + yb] * dbx[2 + xa] * fxy;
+        b0 += dby[yb] * bx[xa] * f;
+        b1 += dby[yb] * bx[2 + xa] * fx;
+        b2 += dby[2 + yb] * bx[xa] * fy;
+        b3 += dby[2 + yb] * bx[2 + xa] * fxy;
+    }
+    double oa = a0, ob = b0;
+    oa += a1 * g.hx;
+    ob += b1 * g.hx;
+    oa += a2 * g.hy;
+    ob += b2 * g.hy;
+    a3 *= g.hx;
+    b3 *= g.hx;
+    a3 *= g.hy;
+    b3 *= g.hy;
+    oa += a3;
+    ob += b3;
+    uy = -oa * g.ihx;
+    ux = ob * g.ihy;
+}
+
+// velocity sources of the footpoint integration on FP32 combined fields (see VelCombined); PK:
+// packed records carrying the values' low parts
+template <bool P2, bool PK>
 struct VelCombined32 {
     DGrid g;
     const float4 *C1, *C12, *C0;
-    const float *L1, *L12, *L0;
-    __device__ __forceinline__ void ev(const float4 *__restrict__ H, const float *__restrict__ Lo, double x, double y,
-                                       double &ux, double &uy) const {
+    __device__ __forceinline__ void ev(const float4 *__restrict__ H, double x, double y, double &ux, double &uy) const {
         Cell c;
         find_cell<P2>(g, x, y, c);
-        double v[16];
-        load16_f4(H, c, v);
-        if constexpr (LO) {
-            v[0] += (double)Lo[c.i00];
-            v[1] += (double)Lo[c.i10];
-            v[2] += (double)Lo[c.i01];
-            v[3] += (double)Lo[c.i11];
-        }
-        herm_vel_vals(g, c, v, ux, uy);
+        float4 r[4];
+        load4_rec(H, c, r);
+        herm_vel_rec<PK>(g, c, r, ux, uy);
     }
     __device__ __forceinline__ void w1(const cmm_advect_coeffs &, double x, double y, double &kx, double &ky) const {
-        ev(C1, L1, x, y, kx, ky);
+        ev(C1, x, y, kx, ky);
     }
     __device__ __forceinline__ void w12(const cmm_advect_coeffs &, double x, double y, double &kx, double &ky) const {
-        ev(C12, L12, x, y, kx, ky);
+        ev(C12, x, y, kx, ky);
     }
     __device__ __forceinline__ void lev(int, double x, double y, double &ux, double &uy) const {
-        ev(C0, L0, x, y, ux, uy);
+        ev(C0, x, y, ux, uy);
     }
 };
 
-// k_psi_combine with FP32 records: the sums in FP64, rounded once; with low-part planes (lo1,
-// lo12, lo0 not null) the value's rounding error goes to them
-template <int L>
+// k_psi_combine with FP32 records: the sums in FP64, rounded once; PK: packed records that keep
+// the values' low parts (pack_psi_rec)
+template <int L, bool PK>
 __global__ void __launch_bounds__(256) k_psi_combine32(long n, PsiLevels P, cmm_advect_coeffs c, float4 *__restrict__ C1,
                                                        float4 *__restrict__ C12, float4 *__restrict__ C0,
-                                                       float *__restrict__ lo1, float *__restrict__ lo12,
-                                                       float *__restrict__ lo0, const double *__restrict__ D1,
-                                                       const double *__restrict__ D12, const double *__restrict__ LW) {
+                                                       const double *__restrict__ D1, const double *__restrict__ D12,
+                                                       const double *__restrict__ LW) {
     const long j = (long)blockIdx.x * 256 + threadIdx.x;
     if (j >= n) return;
     double w1[L], w12[L];
@@ -3130,12 +3202,11 @@ __global__ void __launch_bounds__(256) k_psi_combine32(long n, PsiLevels P, cmm_
     for (int l = 0; l < L; ++l)
 #pragma unroll
         for (int p = 0; p < 4; ++p) v[l][p] = P.p[l][p * n + j];
-    auto put = [&](const double *o, float4 *dst, float *lo) {
-        const float hi = (float)o[0];
-        dst[j] = float4{hi, (float)o[1], (float)o[2], (float)o[3]};
-        if (lo) lo[j] = (float)(o[0] - (double)hi);
+    auto put = [&](const double *o, float4 *dst) {
+        if constexpr (PK) dst[j] = pack_psi_rec(o);
+        else dst[j] = float4{(float)o[0], (float)o[1], (float)o[2], (float)o[3]};
     };
-    auto comb = [&](const double *w, float4 *dst, float *lo, const double *D) {
+    auto comb = [&](const double *w, float4 *dst, const double *D) {
         double o[4];
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
@@ -3145,11 +3216,11 @@ __global__ void __launch_bounds__(256) k_psi_combine32(long n, PsiLevels P, cmm_
             if (D) a += D[p * n + j];
             o[p] = a;
         }
-        put(o, dst, lo);
+        put(o, dst);
     };
-    comb(w1, C1, lo1, D1);
-    if (C12) comb(w12, C12, lo12, D12);
-    if (C0) put(v[0], C0, lo0);
+    comb(w1, C1, D1);
+    if (C12) comb(w12, C12, D12);
+    if (C0) put(v[0], C0);
 }
 
 // Output stage of the split advection: lane k of a node's group forms displacement plane k (and
@@ -3195,8 +3266,7 @@ __device__ __forceinline__ void advect_store32(const cmm_advect_coeffs &c, int n
 // PSI: the form of the combined psi fields (a PsiRecForm)
 template <int TIN, int G, bool P2, int PSI>
 __global__ void __launch_bounds__(256) k_map_advect32(DGrid gm, DGrid gp, const void *C1, const void *C12,
-                                                      const void *C0, const float *L1, const float *L12,
-                                                      const float *L0, const float4 *__restrict__ M,
+                                                      const void *C0, const float4 *__restrict__ M,
                                                       cmm_advect_coeffs c, float *__restrict__ Mout, int nblocks) {
     constexpr int NPB = 256 / G;
     const int blk = xcd_block(blockIdx.x, nblocks);
@@ -3215,8 +3285,8 @@ __global__ void __launch_bounds__(256) k_map_advect32(DGrid gm, DGrid gp, const 
             step_backward<TIN>(VelCombined<P2>{gp, (const double *)C1, (const double *)C12, (const double *)C0}, c, xe,
                                ye, xf, yf);
         else
-            step_backward<TIN>(VelCombined32<P2, PSI == PsiRec32Lo>{gp, (const float4 *)C1, (const float4 *)C12,
-                                                                    (const float4 *)C0, L1, L12, L0},
+            step_backward<TIN>(VelCombined32<P2, PSI == PsiRec32Pk>{gp, (const float4 *)C1, (const float4 *)C12,
+                                                                    (const float4 *)C0},
                                c, xe, ye, xf, yf);
     diffeo32<P2>(gm, M, xf, yf, px, py);
         qx = px - xn;
@@ -3228,49 +3298,38 @@ __global__ void __launch_bounds__(256) k_map_advect32(DGrid gm, DGrid gp, const 
 // k_map_advect32 with two footpoints per lane (footpoints k and k + 4 of the 8-footpoint stencil),
 // as k_map_advect_pair: the second footpoint's psi and map records are loaded only when its cell
 // differs from the first one's
-template <bool P2, bool LO>
+template <bool P2, bool PK>
 struct VelPair32 {
     DGrid g;
     const float4 *C1, *C12, *C0;
-    const float *L1, *L12, *L0;
-    __device__ __forceinline__ void load(const float4 *__restrict__ H, const float *__restrict__ Lo, const Cell &c,
-                                         double v[16]) const {
-        load16_f4(H, c, v);
-        if constexpr (LO) {
-            v[0] += (double)Lo[c.i00];
-            v[1] += (double)Lo[c.i10];
-            v[2] += (double)Lo[c.i01];
-            v[3] += (double)Lo[c.i11];
-        }
-    }
-    __device__ __forceinline__ void ev(const float4 *__restrict__ H, const float *__restrict__ Lo, PointPair x,
-                                       PointPair y, PointPair &ux, PointPair &uy) const {
+    __device__ __forceinline__ void ev(const float4 *__restrict__ H, PointPair x, PointPair y, PointPair &ux,
+                                       PointPair &uy) const {
         Cell ca, cb;
         find_cell<P2>(g, x.a, y.a, ca);
         find_cell<P2>(g, x.b, y.b, cb);
-        double v[16];
-        load(H, Lo, ca, v);
-        herm_vel_vals(g, ca, v, ux.a, uy.a);
-        if (!same_cell(ca, cb)) load(H, Lo, cell_after(cb, ux.a), v);
-        herm_vel_vals(g, cb, v, ux.b, uy.b);
+        float4 r[4];
+        load4_rec(H, ca, r);
+        herm_vel_rec<PK>(g, ca, r, ux.a, uy.a);
+        if (!same_cell(ca, cb)) load4_rec(H, cell_after(cb, ux.a), r);
+        herm_vel_rec<PK>(g, cb, r, ux.b, uy.b);
     }
     __device__ __forceinline__ void w1(const cmm_advect_coeffs &, PointPair x, PointPair y, PointPair &kx,
                                        PointPair &ky) const {
-        ev(C1, L1, x, y, kx, ky);
+        ev(C1, x, y, kx, ky);
     }
     __device__ __forceinline__ void w12(const cmm_advect_coeffs &, PointPair x, PointPair y, PointPair &kx,
                                         PointPair &ky) const {
-        ev(C12, L12, x, y, kx, ky);
+        ev(C12, x, y, kx, ky);
     }
     __device__ __forceinline__ void lev(int, PointPair x, PointPair y, PointPair &ux, PointPair &uy) const {
-        ev(C0, L0, x, y, ux, uy);
+        ev(C0, x, y, ux, uy);
     }
 };
 
-template <int TIN, bool LO>
+template <int TIN, bool PK>
 __global__ void __launch_bounds__(256)
-k_map_advect32_pair(DGrid gm, DGrid gp, const float4 *C1, const float4 *C12, const float4 *C0, const float *L1,
-                    const float *L12, const float *L0, const float4 *__restrict__ M, cmm_advect_coeffs c,
+k_map_advect32_pair(DGrid gm, DGrid gp, const float4 *C1, const float4 *C12, const float4 *C0,
+                    const float4 *__restrict__ M, cmm_advect_coeffs c,
                     float *__restrict__ Mout, int nblocks) {
     constexpr int G = 4, NPB = 256 / G;
     const int blk = xcd_block(blockIdx.x, nblocks);
@@ -3287,7 +3346,7 @@ k_map_advect32_pair(DGrid gm, DGrid gp, const float4 *C1, const float4 *C12, con
         const PointPair ye{yn + ((fa >> 2) + 1) * c.eps * (1 - 2 * (((fa + 1) >> 1) & 1)),
                            yn + ((fb >> 2) + 1) * c.eps * (1 - 2 * (((fb + 1) >> 1) & 1))};
         PointPair xf, yf;
-        step_backward<TIN>(VelPair32<true, LO>{gp, C1, C12, C0, L1, L12, L0}, c, xe, ye, xf, yf);
+        step_backward<TIN>(VelPair32<true, PK>{gp, C1, C12, C0}, c, xe, ye, xf, yf);
         Cell ca, cb;
         find_cell<true>(gm, xf.a, yf.a, ca);
         find_cell<true>(gm, xf.b, yf.b, cb);
@@ -3557,10 +3616,9 @@ int op_interleave4_32(const cmm_grid &g, const double *H, float *H4, hipStream_t
 
 bool op_advect32_supported(const cmm_advect_coeffs &c) { return c.direction != 1 && advect_combinable(c); }
 
-// bytes of one combined psi field in form f on n nodes, rounded up to 256 (FP32 records are
-// followed by their low-part plane)
+// bytes of one combined psi field in form f on n nodes, rounded up to 256
 size_t psi_rec_bytes(int f, long n) {
-    const size_t per = f == PsiRec64 ? 4 * sizeof(double) : f == PsiRec32Lo ? sizeof(float4) + sizeof(float) : sizeof(float4);
+    const size_t per = f == PsiRec64 ? 4 * sizeof(double) : sizeof(float4);
     return (per * (size_t)n + 255) / 256 * 256;
 }
 
@@ -3578,7 +3636,7 @@ int op_map_advect32(const cmm_grid &gm, const cmm_grid &gp, const PsiLevels &P, 
                   "needs two or more psi levels, and is backward only");
         return CMM_ERR_ARG;
     }
-    if (psi_form != PsiRec32 && psi_form != PsiRec32Lo && psi_form != PsiRec64) {
+    if (psi_form != PsiRec32 && psi_form != PsiRec32Pk && psi_form != PsiRec64) {
         set_error("split-precision advection: unknown psi record form");
         return CMM_ERR_ARG;
     }
@@ -3593,9 +3651,6 @@ int op_map_advect32(const cmm_grid &gm, const cmm_grid &gp, const PsiLevels &P, 
     next += rec;
     if (advect_needs_c12(c)) { C12 = next; next += rec; }
     if (advect_needs_c0(c)) C0 = next;
-    // the low-part plane of an FP32 field follows its n records
-    auto lo_of = [&](void *f) { return psi_form == PsiRec32Lo && f ? (float *)((float4 *)f + n) : nullptr; };
-    float *L1 = lo_of(C1), *L12 = lo_of(C12), *L0 = lo_of(C0);
     const dim3 g((unsigned)((n + 255) / 256));
     if (psi_form == PsiRec64) {
         double *d1 = (double *)C1, *d12 = (double *)C12, *d0 = (double *)C0;
@@ -3606,12 +3661,13 @@ int op_map_advect32(const cmm_grid &gm, const cmm_grid &gp, const PsiLevels &P, 
         }
     } else {
         float4 *f1 = (float4 *)C1, *f12 = (float4 *)C12, *f0 = (float4 *)C0;
-#define CMM_C32(LL)                                                                                                    \
-    hipLaunchKernelGGL(k_psi_combine32<LL>, g, dim3(256), 0, st, n, P, c, f1, f12, f0, L1, L12, L0, corr1, corr12, lw)
+#define CMM_C32(LL, PK)                                                                                                \
+    hipLaunchKernelGGL((k_psi_combine32<LL, PK>), g, dim3(256), 0, st, n, P, c, f1, f12, f0, corr1, corr12, lw)
+        const bool pk = psi_form == PsiRec32Pk;
         switch (c.l_order) {
-        case 2: CMM_C32(2); break;
-        case 3: CMM_C32(3); break;
-        default: CMM_C32(4); break;
+        case 2: if (pk) CMM_C32(2, true); else CMM_C32(2, false); break;
+        case 3: if (pk) CMM_C32(3, true); else CMM_C32(3, false); break;
+        default: if (pk) CMM_C32(4, true); else CMM_C32(4, false); break;
         }
 #undef CMM_C32
     }
@@ -3623,15 +3679,15 @@ int op_map_advect32(const cmm_grid &gm, const cmm_grid &gp, const PsiLevels &P, 
     const bool p2 = dm.lgx >= 0 && dm.lgy >= 0 && dp.lgx >= 0 && dp.lgy >= 0;
     const float4 *Mi = (const float4 *)M_in;
 #define CMM_A32(T, GG, PP, S)                                                                                          \
-    hipLaunchKernelGGL((k_map_advect32<T, GG, PP, S>), dim3(nb), dim3(256), 0, st, dm, dp, C1, C12, C0, L1, L12, L0,  \
-                       Mi, c, M_out, nb)
+    hipLaunchKernelGGL((k_map_advect32<T, GG, PP, S>), dim3(nb), dim3(256), 0, st, dm, dp, C1, C12, C0, Mi, c, M_out, \
+                       nb)
     const int nb2 = (int)(((long)dm.n * 4 + 255) / 256);  // two footpoints per lane
 #define CMM_A32P(T, LOV)                                                                                               \
     hipLaunchKernelGGL((k_map_advect32_pair<T, LOV>), dim3(nb2), dim3(256), 0, st, dm, dp, (const float4 *)C1,          \
-                       (const float4 *)C12, (const float4 *)C0, L1, L12, L0, Mi, c, M_out, nb2)
+                       (const float4 *)C12, (const float4 *)C0, Mi, c, M_out, nb2)
 #define CMM_A32_S(T, S)                                                                                                \
     do {                                                                                                               \
-        if (G == 8 && p2 && S == PsiRec32Lo) CMM_A32P(T, true);                                                       \
+        if (G == 8 && p2 && S == PsiRec32Pk) CMM_A32P(T, true);                                                       \
         else if (G == 8 && p2 && S == PsiRec32) CMM_A32P(T, false);                                                   \
         else if (G == 8 && p2) CMM_A32(T, 8, true, S);                                                                 \
         else if (G == 4) CMM_A32(T, 4, false, S);                                                                      \
@@ -3641,7 +3697,7 @@ int op_map_advect32(const cmm_grid &gm, const cmm_grid &gp, const PsiLevels &P, 
 #define CMM_A32_T(T)                                                                                                   \
     do {                                                                                                               \
         if (psi_form == PsiRec64) CMM_A32_S(T, PsiRec64);                                                              \
-        else if (psi_form == PsiRec32Lo) CMM_A32_S(T, PsiRec32Lo);                                                     \
+        else if (psi_form == PsiRec32Pk) CMM_A32_S(T, PsiRec32Pk);                                                     \
         else CMM_A32_S(T, PsiRec32);                                                                                   \
     } while (0)
     switch (c.time_integration_num) {

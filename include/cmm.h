@@ -192,9 +192,10 @@ int cmm_particles_advect(cmm_grid g_psi, const double *const *psi_levels, int l_
  * 12 floats per node: (Dx hi, Dx lo, Dy hi, Dy lo), the value planes as FP32 pairs hi + lo;
  * (dX/dx - 1, dX/dy, dY/dx, dY/dy - 1); (d2X/dxdy, d2Y/dxdy, 0, 0).  The gathered Hermite fields
  * (the Lagrange-combined psi fields, the fine vorticity) are FP32 node records
- * (f, fx, fy, fxy); each combined psi field also carries a plane of its values' FP32 low parts
- * (f = f_hi + f_lo), since the value of a stream function of order one rounded to FP32 would move
- * the velocity by 2^-24 |psi| 6/h.  Arithmetic, the footpoint integration, the Jacobian check, the psi levels,
+ * (f, fx, fy, fxy); the combined psi fields keep their values to 2^-31 relative or better, since the
+ * value of a stream function of order one rounded to FP32 would move the velocity by
+ * 2^-24 |psi| 6/h: an 8-bit code of f - (float)f takes the low byte of fxy, which keeps 15
+ * explicit mantissa bits.  Arithmetic, the footpoint integration, the Jacobian check, the psi levels,
  * the FFTs and the Poisson divide stay FP64.  These operators are the split forms of
  * cmm_map_advect, cmm_incomp_error and cmm_psi_from_vort. */
 enum { CMM_PRECISION_FP64 = 0, CMM_PRECISION_SPLIT = 1 };
@@ -204,8 +205,8 @@ int cmm_map_from_split(cmm_grid g, const float *map_split, double *chix, double 
 /* FP32 node records (4*N floats) of a plane-major FP64 Hermite field */
 int cmm_hermite_to_split(cmm_grid g, const double *herm, float *herm_split, void *stream);
 /* Backward advection of the split map (schemes 10, 21, 30, 40, 41 with l_order >= 2); the
- * workspace (cmm_map_advect_split_workspace_bytes) holds the combined psi fields (20 bytes per
- * psi node each: the FP32 records, then the low-part plane). */
+ * workspace (cmm_map_advect_split_workspace_bytes) holds the combined psi fields (one 16-byte
+ * record per psi node each). */
 size_t cmm_map_advect_split_workspace_bytes(cmm_grid g_psi, const cmm_advect_coeffs *c);
 int cmm_map_advect_split(cmm_grid g_map, cmm_grid g_psi, const float *map_in, const double *const *psi_levels,
                          const cmm_advect_coeffs *c, float *map_out, void *work, size_t work_bytes, void *stream);

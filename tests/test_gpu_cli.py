@@ -268,8 +268,9 @@ def test_cli_split_precision_11b(dev, tmp_path):
     T 2): the split form has no reference output, so its monitoring files are compared with the
     fixture at the form's own accuracy and the errors printed: the same remap steps, the
     conservation values within 1e-6 relative, E_inc within the noise floor of the split records.
-    The combined psi fields carry the low parts of their values, so that floor comes from the FP32
-    derivative planes: 2^-24 max|u| (|u| <= 2.2 for 4_nodes, twice that for the Hermite weights)
+    The combined psi fields keep their values to 2^-31 relative or better, so that floor comes from the
+    FP32 derivative planes (the cross derivative's 15-bit mantissa enters the velocity with weight
+    h/7 or less): 2^-24 max|u| (|u| <= 2.2 for 4_nodes, twice that for the Hermite weights)
     reaches the map's value planes through dt each step and the Jacobian through 1/h_map, summed
     over the 64 steps as rounding errors of random sign (sqrt(64)); measured: 3.2e-7 against that
     bound's 2.7e-6.  At the reference's preset threshold, 1e-5, the split run must remap on
