@@ -60,6 +60,11 @@ def algorithmic(slot, c, kx_max, n_stack=0):
         # sampling: map + fine omega Hermite read, omega(X(x)) written; row transform: that field read,
         # transposed half-spectrum columns kx <= kx_max written
         return 64 * nc + 32 * nf + 16 * nv + 16 * (kx_max + 1) * c["n_vort"], 270.0 * nv + 2.5 * nv * 24
+    if slot == "psi_row_r2c_fused":
+        # the same with the incompressibility check in the sampler's pass (fine grid = vorticity
+        # grid): the check's flops at the cell centres, the map read once for both
+        b, f = algorithmic("psi_row_r2c", c, kx_max, n_stack)
+        return b, f + 85.0 * nf
     if slot == "psi_col":
         return 16 * (kx_max + 1) * (c["n_vort"] + 2 * c["n_psi"]), 5.0 * (kx_max + 1) * 36 * c["n_psi"]
     if slot == "psi_row_c2r":
@@ -337,15 +342,20 @@ def composition(cmm, torch, c, n_maps=32, reps=5):
                               "max_position_difference": dmax}}
 
 
-def roofline(c, prof, n_stack, suffix=""):
+def roofline(c, prof, n_stack, suffix="", fused=False):
     """Per-slot rates from the engine's event profile and the roofline object of the dominant
     kernel: algorithmic bytes and flops per launch over its average duration; traffic from the
-    committed counter summary.  suffix selects the algorithmic model of a form's slots."""
+    committed counter summary.  suffix selects the algorithmic model of a form's slots; fused: the
+    check ran in the psi sampler's pass (its own slot then closes at once and is left out)."""
     kx_max = int(min(c["n_psi"] // 2, c["n_vort"] // 2, c["freq_cut_psi"]))
     per = {}
     for name, (ms, cnt) in prof.items():
+        if fused and name == "incomp":
+            continue
         if cnt:
             model = name + suffix if algorithmic(name + suffix, c, kx_max, n_stack)[0] else name
+            if fused and name == "psi_row_r2c":
+                model = "psi_row_r2c_fused"
             b, f = algorithmic(model, c, kx_max, n_stack)
             per[name] = dict(ms_total=ms, launches=cnt, ms_avg=ms / cnt, bytes=b, flops=f, model=model)
     for name, d in sorted(per.items(), key=lambda kv: -kv[1]["ms_total"]):
@@ -390,6 +400,7 @@ def run_form(cmm, torch, c, precision, warmup, steps, dt=None, t_end=None, profi
                     precision=precision)
     eng.add_particles(uniform_particles(c["n_particles"], c["seed"]), scheme=30)
     warm = eng.init()
+    fused = eng.check_fusion()
     t_init = time.perf_counter() - t_setup
     log(f"[{precision}] init + warm-up {t_init:.2f}s, warm-up E_inc {warm}")
     einc = []
@@ -415,7 +426,7 @@ def run_form(cmm, torch, c, precision, warmup, steps, dt=None, t_end=None, profi
     t1 = time.perf_counter()
     prof = eng.profile_read() if profile_every else {}
     _, t_now, n_stack = eng.state()
-    out = {"sps": i / (t1 - t0), "elapsed": t1 - t0, "steps": i, "sim_time": t_now - t_start,
+    out = {"sps": i / (t1 - t0), "elapsed": t1 - t0, "steps": i, "sim_time": t_now - t_start, "fused": fused,
            "sim_per_s": (t_now - t_start) / (t1 - t0), "prof": prof, "maps": n_stack, "t_end": t_now,
            "remaps": remaps, "init_s": t_init, "einc": np.array(einc), "w": eng.field("w_coarse"),
            "psi": eng.field("psi")[:4 * c["n_psi"] ** 2], "part": eng.particles(0)}
@@ -751,7 +762,7 @@ def main():
     sps, elapsed, prof, n_stack, t_now, remaps, t_init = (plain[k] for k in (
         "sps", "elapsed", "prof", "maps", "t_end", "remaps", "init_s"))
 
-    roof, per = roofline(c, prof, n_stack)
+    roof, per = roofline(c, prof, n_stack, fused=plain["fused"])
 
     comp = composition(cmm, torch, c)
     log(f"composition: {comp}")
@@ -775,7 +786,7 @@ def main():
                       "particles_max_abs": float(dpart.max()),
                       "e_inc_max_abs_diff": float(np.max(np.abs(sp["einc"] - plain["einc"])))},
                   "remaps_timed": sp["remaps"], "maps": sp["maps"],
-                  "roofline": roofline(c, sp["prof"], sp["maps"], "_split")[0],
+                  "roofline": roofline(c, sp["prof"], sp["maps"], "_split", fused=sp["fused"])[0],
                   "kernels_ms_avg": {k: round(ms / cnt, 5) for k, (ms, cnt) in sp["prof"].items() if cnt}}
         log(f"split form: {sp['sps']:.1f} steps/s ({chosen['speedup']:.2f}x), errors {chosen['errors_vs_fp64_at_t_end']}")
         # both forms at twice the step (plain Lagrange extrapolation of psi) to the same end time as
@@ -824,7 +835,7 @@ def main():
                                "IC seed 0, dt 1/256, RK3, 4th-order stencil, eps 1e-3, remap 1e-4, "
                                "freq_cut_psi 256, 4096 RK3 particles",
                    "form": "plain FP64", "t_end": round(t_now, 6), "remaps_timed": remaps, "maps": n_stack,
-                   "init_s": round(t_init, 3)},
+                   "check_in_psi_sampler": plain["fused"], "init_s": round(t_init, 3)},
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels_ms_avg": {k: round(v["ms_avg"], 5) for k, v in per.items()},

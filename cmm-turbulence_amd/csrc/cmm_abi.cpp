@@ -380,6 +380,9 @@ struct cmm_euler {
     // Vlasov-Poisson: H(v) and H'(v) of the bump function (they depend on the grid only)
     double *vpH = nullptr;
     IncompTune tune;  // the check's tile and grid, from cmm_euler_autotune
+    // the step's check runs inside the psi sampler's pass (the check grid is the vorticity grid);
+    // sampled: the current step's sampling is queued, its spectral part not yet
+    bool fuse_check = false, sampled = false;
     // corrections (device Hermite fields on g_psi) added to the Lagrange extrapolations of psi
     // to t_{n+1} and t_n + dt/2 in the next steps (cmm_euler_set_psi_correction)
     const double *corr1 = nullptr, *corr12 = nullptr;
@@ -506,6 +509,30 @@ int engine_psi(cmm_euler *e, double *out, hipEvent_t *ev = nullptr) {
     }
     return op_psi_from_vort(e->p.g_coarse, e->p.g_vort, e->p.g_fine, e->p.g_psi, e->cx, e->cy, e->W,
                             e->p.molly_stencil, e->p.freq_cut_psi, out, e->work, e->work_bytes, e->st, ev, e->W4);
+}
+
+bool check_fusable(const cmm_euler *e) {
+    return e->p.sim == CMM_SIM_EULER_2D && !e->p.forward_map && op_check_fusable(e->p.g_coarse, e->p.g_vort, e->p.g_fine);
+}
+
+// the psi solve's sampling with the step's check in the same pass (fuse_check), and its spectral
+// part into out
+int engine_psi_sample(cmm_euler *e, hipEvent_t *ev) {
+    if (split(e)) {
+        if (int rc = sync_m32(e)) return rc;
+        if (e->dry) return CMM_OK;
+        return op_psi_sample32(e->p.g_coarse, e->p.g_vort, e->p.g_fine, e->p.g_psi, e->m32[e->m32_cur], e->W4f,
+                               e->p.molly_stencil, e->p.freq_cut_psi, e->work, e->work_bytes, e->st, ev, e->slot);
+    }
+    if (e->dry) return CMM_OK;
+    return op_psi_sample(e->p.g_coarse, e->p.g_vort, e->p.g_fine, e->p.g_psi, e->cx, e->cy, e->W, e->p.molly_stencil,
+                         e->p.freq_cut_psi, e->work, e->work_bytes, e->st, ev, e->W4, e->slot);
+}
+
+int engine_psi_spectral(cmm_euler *e, double *out, hipEvent_t *ev) {
+    if (e->dry) return CMM_OK;
+    if (split(e)) return op_psi_spectral32(e->p.g_vort, e->p.g_psi, e->p.freq_cut_psi, out, e->work, e->work_bytes, e->st, ev);
+    return op_psi_spectral(e->p.g_vort, e->p.g_psi, e->p.freq_cut_psi, out, e->work, e->work_bytes, e->st, ev);
 }
 
 int engine_advect(cmm_euler *e, const double *t, const double *dt, int loop_ctr, int L, int tin) {
@@ -808,7 +835,27 @@ int cmm_euler_create(const cmm_euler_params *p, cmm_euler **out) {
         cmm_euler_destroy(e);
         return rc;
     }
+    // the split form's sampler keeps fewer registers without the check, and its check overlaps
+    // the sampler from the third stream: measured faster unfused
+    e->fuse_check = check_fusable(e) && !split(e);
     *out = e;
+    return CMM_OK;
+}
+
+int cmm_euler_set_check_fusion(cmm_euler *e, int on) {
+    if (!e) { set_error("cmm_euler_set_check_fusion: null engine"); return CMM_ERR_ARG; }
+    if (on && !check_fusable(e)) {
+        set_error("cmm_euler_set_check_fusion: the check grid must be the vorticity grid, refining the map grid by a "
+                  "multiple of 4 over the same domain, in an Euler run without a forward map");
+        return CMM_ERR_ARG;
+    }
+    e->fuse_check = on != 0;
+    return CMM_OK;
+}
+
+int cmm_euler_get_check_fusion(cmm_euler *e, int *on) {
+    if (!e || !on) { set_error("cmm_euler_get_check_fusion: bad argument"); return CMM_ERR_ARG; }
+    *on = e->fuse_check ? 1 : 0;
     return CMM_OK;
 }
 
@@ -1101,7 +1148,10 @@ int step_queue_a(cmm_euler *e, double dt_now, bool overlap, bool *forked) {
     // which the next advection does not overwrite (it writes the other buffer, after the host has
     // waited for E_inc); a conversion of the map that the check needs runs first, on the engine
     // stream.
-    const bool fork = overlap && e->st3 && !e->dry && !e->capturing;
+    // With fuse_check the check runs in the pass of the psi solve's sampler over the same grid,
+    // which is queued here; the step's second part queues the spectral rest of the psi solve.
+    const bool fuse = e->fuse_check;
+    const bool fork = overlap && e->st3 && !e->dry && !e->capturing && !fuse;
     if (fork && !(split(e) && e->m32_fresh))
         if (int rc = sync_planes(e)) return rc;
     const hipStream_t sc = fork ? e->st3 : e->st;
@@ -1110,7 +1160,15 @@ int step_queue_a(cmm_euler *e, double dt_now, bool overlap, bool *forked) {
         if (int rc = hip_check(hipStreamWaitEvent(e->st3, e->ev_fork, 0), "hipStreamWaitEvent")) return rc;
     }
     if (e->prof) (void)hipEventRecord(e->ev[1], sc);
-    if (int rc = launch_incomp(e, sc)) return rc;
+    if (fuse) {
+        // the check has no time of its own: its slot closes at once, the sampling counts to the
+        // psi row slot as without the fusion
+        if (e->prof) (void)hipEventRecord(e->ev[2], sc);
+        if (int rc = engine_psi_sample(e, e->prof ? &e->ev[5] : nullptr)) return rc;
+        e->sampled = true;
+    } else if (int rc = launch_incomp(e, sc)) {
+        return rc;
+    }
     // forward map: invertibility and E_inc on the coarse grid, :798-803
     if (p.forward_map) {
         if (int rc = op_invert_launch(p.g_coarse, p.g_coarse, p.g_coarse, e->cx, e->cy, e->fx, e->fy, e->slot + 2, sc))
@@ -1124,7 +1182,7 @@ int step_queue_a(cmm_euler *e, double dt_now, bool overlap, bool *forked) {
             return rc;
     if (!e->dry && !e->capturing)
         if (int rc = hip_check(hipEventRecord(e->ev_inc, sc), "hipEventRecord")) return rc;
-    if (e->prof) (void)hipEventRecord(e->ev[2], sc);
+    if (e->prof && !fuse) (void)hipEventRecord(e->ev[2], sc);
     *forked = fork;
     return CMM_OK;
 }
@@ -1142,7 +1200,12 @@ int step_queue_b(cmm_euler *e, bool fork) {
     e->lvl[0] = oldest;
     if (e->part_pending)  // the previous step's particles still read the level about to be rewritten
         if (int rc = hip_check(hipStreamWaitEvent(e->st, e->ev_part, 0), "hipStreamWaitEvent")) return rc;
-    if (int rc = engine_psi(e, e->lvl[0], e->prof ? &e->ev[5] : nullptr)) return rc;
+    if (e->sampled) {
+        e->sampled = false;
+        if (int rc = engine_psi_spectral(e, e->lvl[0], e->prof ? &e->ev[5] : nullptr)) return rc;
+    } else if (int rc = engine_psi(e, e->lvl[0], e->prof ? &e->ev[5] : nullptr)) {
+        return rc;
+    }
     // no rejoin of the check's stream: the host waits for ev_inc, recorded there after the check,
     // before it queues anything else for this engine (step_finish), so all later work on the
     // engine stream follows the check; a stream wait would only add a barrier packet (about 12 us

@@ -42,6 +42,15 @@ size_t op_psi_workspace(const cmm_grid &gv, const cmm_grid &gp, double fcut);
 int op_psi_from_vort(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf, const cmm_grid &gp, const double *cx,
                      const double *cy, const double *W, int molly, double fcut, double *psi, void *work,
                      size_t work_bytes, hipStream_t st, hipEvent_t *ev = nullptr, const double *W4 = nullptr);
+// op_psi_from_vort in two parts: the sampling of omega(X(x)) into the workspace, optionally with
+// the incompressibility check on the same grid in the same pass (chk_slot, cleared by the call;
+// possible when op_check_fusable(gm, gv, gc) holds for the check grid gc), and the spectral part
+bool op_check_fusable(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gc);
+int op_psi_sample(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf, const cmm_grid &gp, const double *cx,
+                  const double *cy, const double *W, int molly, double fcut, void *work, size_t work_bytes,
+                  hipStream_t st, hipEvent_t *ev, const double *W4, unsigned long long *chk_slot);
+int op_psi_spectral(const cmm_grid &gv, const cmm_grid &gp, double fcut, double *psi, void *work, size_t work_bytes,
+                    hipStream_t st, hipEvent_t *ev);
 // op_psi_from_vort with its workspace and twiddle tables in a static device array (no allocation);
 // op_psi_noalloc_max_bytes() is the largest workspace it takes
 size_t op_psi_noalloc_max_bytes();
@@ -156,6 +165,11 @@ int op_incomp32(const cmm_grid &gm, const cmm_grid &gc, const float *M, double *
 int op_psi_from_vort32(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf, const cmm_grid &gp, const float *M,
                        const float *W4, int molly, double fcut, double *psi, void *work, size_t work_bytes,
                        hipStream_t st, hipEvent_t *ev = nullptr);
+int op_psi_sample32(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf, const cmm_grid &gp, const float *M,
+                    const float *W4, int molly, double fcut, void *work, size_t work_bytes, hipStream_t st,
+                    hipEvent_t *ev, unsigned long long *chk_slot);
+int op_psi_spectral32(const cmm_grid &gv, const cmm_grid &gp, double fcut, double *psi, void *work, size_t work_bytes,
+                      hipStream_t st, hipEvent_t *ev);
 int op_sample_vort32(const cmm_grid &gm, const cmm_grid &gt, const cmm_grid &gf, const float *M, const float *W4,
                      int molly, double *out, hipStream_t st);
 

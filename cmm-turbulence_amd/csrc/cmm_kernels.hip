@@ -1015,6 +1015,54 @@ struct SampleHermite {
 // omega(X(x)) through the current map and the fine Hermite vorticity (mollifier 0/4/8),
 // Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:402-461; positions use the map grid's origin.
 // IL: the fine vorticity is given node-interleaved (W[4*i + plane]) instead of plane-major.
+// The incompressibility check at R check points of one map column: the cell centres
+// (xc, yc[r]) of the check grid, all inside map cell c (whose corner values vx, vy of the two map
+// components are loaded as load16 gives them, without the wrap jumps).  The arithmetic of
+// k_incomp_tensor: the value planes with their wrap jumps, contracted along x at xc, then along y
+// per point; returns the largest |1 - det grad X|.
+template <int R>
+__device__ __forceinline__ double incomp_column(const DGrid &gm, const Cell &c, const double vx[16], const double vy[16],
+                                                double xc, const double yc[R]) {
+    const double wjx[4] = {c.wx0 * gm.jx, c.wx1 * gm.jx, c.wx0 * gm.jx, c.wx1 * gm.jx};
+    const double wjy[4] = {c.wy0 * gm.jy, c.wy0 * gm.jy, c.wy1 * gm.jy, c.wy1 * gm.jy};
+    const double fx = (xc - gm.x0) * gm.ihx;
+    const double dx = fx - floor(fx);
+    double bx[4], dbx[4];
+    basis(dx, bx);
+    basis_d(dx, dbx);
+    double P[4][4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const double *w = (k & 1) ? bx : dbx;
+        const double *f = (k < 2) ? vx : vy;
+        const double *wj = (k < 2) ? wjx : wjy;
+#pragma unroll
+        for (int n = 0; n < 2; ++n) {
+            const double f0 = f[2 * n] + wj[2 * n], f1 = f[2 * n + 1] + wj[2 * n + 1];
+            P[k][n] = w[0] * f0 + w[1] * f1 + gm.hx * (w[2] * f[4 + 2 * n] + w[3] * f[5 + 2 * n]);
+            P[k][2 + n] = w[0] * f[8 + 2 * n] + w[1] * f[9 + 2 * n] + gm.hx * (w[2] * f[12 + 2 * n] + w[3] * f[13 + 2 * n]);
+        }
+    }
+    double v = 0.0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const double fy = (yc[r] - gm.y0) * gm.ihy;
+        const double d = fy - floor(fy);
+        double by[4], dby[4];
+        basis(d, by);
+        basis_d(d, dby);
+        auto ysum = [&](const double *q, const double *cw) {
+            return cw[0] * q[0] + cw[1] * q[1] + gm.hy * (cw[2] * q[2] + cw[3] * q[3]);
+        };
+        const double Xx = ysum(P[0], by) * gm.ihx;
+        const double Xy = ysum(P[1], dby) * gm.ihx;
+        const double Yx = ysum(P[2], by) * gm.ihy;
+        const double Yy = ysum(P[3], dby) * gm.ihy;
+        v = fmax(v, fabs(1 - (Xx * Yy - Xy * Yx)));
+    }
+    return v;
+}
+
 template <bool IL>
 struct SampleVortT {
     DGrid gm, gv, gf;
@@ -1054,15 +1102,30 @@ struct SampleVortT {
         }
         return v;
     }
+    // the check at the R cell centres (x + 1/2, y0 + r + 1/2) of the column, point by point
+    template <int R>
+    __device__ __forceinline__ double check_points(int x, int y0) const {
+        double v = 0.0;
+        const double xc = gm.x0 + x * gv.hx + 0.5 * gv.hx;
+#pragma unroll 1
+        for (int r = 0; r < R; ++r)
+            v = fmax(v, fabs(1 - diffeo_det(gm, cx, cy, xc, gm.y0 + (y0 + r) * gv.hy + 0.5 * gv.hy)));
+        return v;
+    }
     // R points (x, y0 .. y0 + R - 1) of one column: the map cell corners are loaded once when all
     // R share the cell (the vorticity grid refines the map grid R times or more along y), and the
-    // R vorticity gathers are independent.  The values equal operator().
-    template <int R>
-    __device__ __forceinline__ void column(int x, int y0, double *out) const {
+    // R vorticity gathers are independent.  The values equal operator().  CHK: also the
+    // incompressibility check at the column's R cell centres of the same grid (the check grid
+    // equals the vorticity grid), whose largest |1 - det grad X| goes to chk; the centres lie in
+    // the nodes' map cell.  Returns false where the points were taken one by one (a sampling
+    // stencil, or a column across map cells); the caller then checks the centres with
+    // check_points, after the column's values are stored.
+    template <int R, bool CHK = false>
+    __device__ __forceinline__ bool column(int x, int y0, double *out, double &chk) const {
         if (molly != 0) {
 #pragma unroll
             for (int r = 0; r < R; ++r) out[r] = (*this)(x, y0 + r);
-            return;
+            return false;
         }
         const double xp = gm.x0 + x * gv.hx;
         Cell cf, cl;
@@ -1071,11 +1134,17 @@ struct SampleVortT {
         if (!same_cell(cf, cl)) {
 #pragma unroll
             for (int r = 0; r < R; ++r) out[r] = (*this)(x, y0 + r);
-            return;
+            return false;
         }
         double vx[16], vy[16];
         load16(cx, gm.n, cf, vx);
         load16(cy, gm.n, cf, vy);
+        if constexpr (CHK) {
+            double yc[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) yc[r] = gm.y0 + (y0 + r) * gv.hy + 0.5 * gv.hy;
+            chk = fmax(chk, incomp_column<R>(gm, cf, vx, vy, xp + 0.5 * gv.hx, yc));
+        }
         Cell wc[R];
         double wv[R][16];
 #pragma unroll
@@ -1090,6 +1159,12 @@ struct SampleVortT {
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) out[r] = herm_eval_vals(gf, wc[r], wv[r]);
+        return true;
+    }
+    template <int R>
+    __device__ __forceinline__ void column(int x, int y0, double *out) const {
+        double chk = 0.0;
+        column<R, false>(x, y0, out, chk);
     }
 };
 using SampleVort = SampleVortT<false>;
@@ -1296,16 +1371,34 @@ struct SampleCompose {
 // row kernel.
 constexpr int kSampleRows = 4;
 
-// R: rows per lane (kSampleRows, or 2 where the vorticity grid refines the map grid only twice)
-template <class S, class T = double, int R = kSampleRows>
-__global__ void __launch_bounds__(256, 2) k_sample_field(S smp, int nx, int ny, T *__restrict__ f) {
+// R: rows per lane (kSampleRows, or 2 where the vorticity grid refines the map grid only twice).
+// CHK: the incompressibility check on the same grid's cell centres in the same pass (the sampler's
+// column<R, true>), its maximum into chk_out with one 64-bit atomic max per block as k_incomp
+// (chk_out cleared beforehand).
+template <class S, class T = double, int R = kSampleRows, bool CHK = false>
+__global__ void __launch_bounds__(256, 2) k_sample_field(S smp, int nx, int ny, T *__restrict__ f,
+                                                         unsigned long long *__restrict__ chk_out) {
     const long t = (long)blockIdx.x * 256 + threadIdx.x;
-    if (t >= (long)nx * (ny / R)) return;
-    const int x = (int)(t % nx), y0 = R * (int)(t / nx);
-    double v[R];
-    smp.template column<R>(x, y0, v);
+    double chk = 0.0;
+    if (t < (long)nx * (ny / R)) {
+        const int x = (int)(t % nx), y0 = R * (int)(t / nx);
+        double v[R];
+        const bool fast = smp.template column<R, CHK>(x, y0, v, chk);
 #pragma unroll
-    for (int r = 0; r < R; ++r) f[(long)(y0 + r) * nx + x] = (T)v[r];
+        for (int r = 0; r < R; ++r) f[(long)(y0 + r) * nx + x] = (T)v[r];
+        if constexpr (CHK)
+            if (!fast) chk = fmax(chk, smp.template check_points<R>(x, y0));
+    }
+    if constexpr (CHK) {
+        for (int o = 32; o > 0; o >>= 1) chk = fmax(chk, __shfl_xor(chk, o, 64));
+        __shared__ double red[4];
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = chk;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const double m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
+            atomicMax(chk_out, (unsigned long long)__double_as_longlong(m));
+        }
+    }
 }
 
 template <class S>
@@ -2481,17 +2574,35 @@ size_t op_psi_workspace(const cmm_grid &gv, const cmm_grid &gp, double fcut) {
     return spec_workspace_bytes(gv, gp, 1, fcut) + sizeof(double) * (size_t)gv.nx * gv.ny;
 }
 
-int op_psi_from_vort(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf, const cmm_grid &gp, const double *cx,
-                     const double *cy, const double *W, int molly, double fcut, double *psi, void *work,
-                     size_t work_bytes, hipStream_t st, hipEvent_t *ev, const double *W4) {
+// whether the incompressibility check on grid gc can run inside the psi sampler on grid gv: the same
+// grid, refining the map grid gm by a multiple of the sampler's rows per lane over the same domain
+bool op_check_fusable(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gc) {
+    const int r = gv.ny / gm.ny;
+    return gc.nx == gv.nx && gc.ny == gv.ny && gc.x0 == gv.x0 && gc.x1 == gv.x1 && gc.y0 == gv.y0 &&
+           gc.y1 == gv.y1 && gv.x0 == gm.x0 && gv.x1 == gm.x1 && gv.y0 == gm.y0 && gv.y1 == gm.y1 &&
+           gv.nx % gm.nx == 0 && gv.ny == r * gm.ny && r % kSampleRows == 0;
+}
+
+// The psi solve in two parts: op_psi_sample writes omega(X(x)) on the vorticity grid into the
+// workspace (with chk_slot: the incompressibility check of the same map on the same grid in the
+// same pass, op_check_fusable, its maximum as k_incomp's bit pattern in *chk_slot);
+// op_psi_spectral transforms that field into psi.  Work on the same stream in between must leave
+// the field alone.
+int op_psi_sample(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf, const cmm_grid &gp, const double *cx,
+                  const double *cy, const double *W, int molly, double fcut, void *work, size_t work_bytes,
+                  hipStream_t st, hipEvent_t *ev, const double *W4, unsigned long long *chk_slot) {
     if (int rc = fft_grid_ok(gv, "vort grid")) return rc;
     if (int rc = fft_grid_ok(gp, "psi grid")) return rc;
     SpecPlan sp = plan_spec(gv, gp, 1, fcut);
     const size_t base = sp.t_bytes + 2 * sp.g_bytes + 256;
     void *w;
     if (int rc = get_work(work, work_bytes, base + sizeof(double) * (size_t)gv.nx * gv.ny, &w)) return rc;
-    // omega(X(x)) on the vorticity grid into a real field after the spectral buffers, then the
-    // plain row transform of that field; the sampling counts to the row transform's profile slot
+    if (chk_slot && !op_check_fusable(gm, gv, gv)) {
+        set_error("psi sampler: the check cannot run in the sampler's pass on these grids");
+        return CMM_ERR_ARG;
+    }
+    // omega(X(x)) on the vorticity grid into a real field after the spectral buffers, for the
+    // plain row transform; the sampling counts to the row transform's profile slot
     double *f = reinterpret_cast<double *>(static_cast<char *>(w) + base);
     const long lanes = (long)gv.nx * (gv.ny / kSampleRows);
     const dim3 grid((unsigned)((lanes + 255) / 256));
@@ -2503,13 +2614,36 @@ int op_psi_from_vort(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf,
         smp.cy = cy;
         smp.W = field;
         smp.molly = molly;
-        hipLaunchKernelGGL(k_sample_field<decltype(smp)>, grid, dim3(256), 0, st, smp, gv.nx, gv.ny, f);
+        if (chk_slot) {
+            hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, chk_slot);
+            hipLaunchKernelGGL((k_sample_field<decltype(smp), double, kSampleRows, true>), grid, dim3(256), 0, st, smp,
+                               gv.nx, gv.ny, f, chk_slot);
+        } else {
+            hipLaunchKernelGGL(k_sample_field<decltype(smp)>, grid, dim3(256), 0, st, smp, gv.nx, gv.ny, f, nullptr);
+        }
     };
     if (ev) (void)hipEventRecord(ev[0], st);
     if (W4) launch(SampleVortIL{}, W4);
     else launch(SampleVort{}, W);
-    if (int rc = launch_check("k_sample_field")) return rc;
+    return launch_check("k_sample_field");
+}
+
+int op_psi_spectral(const cmm_grid &gv, const cmm_grid &gp, double fcut, double *psi, void *work, size_t work_bytes,
+                    hipStream_t st, hipEvent_t *ev) {
+    SpecPlan sp = plan_spec(gv, gp, 1, fcut);
+    const size_t base = sp.t_bytes + 2 * sp.g_bytes + 256;
+    void *w;
+    if (int rc = get_work(work, work_bytes, base + sizeof(double) * (size_t)gv.nx * gv.ny, &w)) return rc;
+    double *f = reinterpret_cast<double *>(static_cast<char *>(w) + base);
     return run_spec(sp, SampleReal{f, gv.nx}, psi, w, st, ev, false);
+}
+
+int op_psi_from_vort(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf, const cmm_grid &gp, const double *cx,
+                     const double *cy, const double *W, int molly, double fcut, double *psi, void *work,
+                     size_t work_bytes, hipStream_t st, hipEvent_t *ev, const double *W4) {
+    if (int rc = op_psi_sample(gm, gv, gf, gp, cx, cy, W, molly, fcut, work, work_bytes, st, ev, W4, nullptr))
+        return rc;
+    return op_psi_spectral(gv, gp, fcut, psi, work, work_bytes, st, ev);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -2583,7 +2717,7 @@ int op_psi_from_vort_noalloc(const cmm_grid &gm, const cmm_grid &gv, const cmm_g
     smp.W = W;
     smp.molly = molly;
     hipLaunchKernelGGL(k_sample_field<SampleVort>, dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, st, smp, gv.nx,
-                       gv.ny, f);
+                       gv.ny, f, nullptr);
     if (int rc = launch_check("k_sample_field")) return rc;
     return run_spec(sp, SampleReal{f, gv.nx}, psi, w, st, nullptr, false, tw);
 }
@@ -2858,11 +2992,11 @@ int op_vp_psi_from_f(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf,
         if (r >= kSampleRows) {
             const long lanes = (long)gv.nx * (gv.ny / kSampleRows);
             hipLaunchKernelGGL((k_sample_field<SampleVortIL, double, kSampleRows>), dim3((unsigned)((lanes + 255) / 256)),
-                               dim3(256), 0, st, smp, gv.nx, gv.ny, f);
+                               dim3(256), 0, st, smp, gv.nx, gv.ny, f, nullptr);
         } else {
             const long lanes = (long)gv.nx * (gv.ny / 2);
             hipLaunchKernelGGL((k_sample_field<SampleVortIL, double, 2>), dim3((unsigned)((lanes + 255) / 256)),
-                               dim3(256), 0, st, smp, gv.nx, gv.ny, f);
+                               dim3(256), 0, st, smp, gv.nx, gv.ny, f, nullptr);
         }
         if (int rc = launch_check("k_sample_field")) return rc;
     } else {
@@ -3485,6 +3619,47 @@ __global__ void __launch_bounds__(256) k_incomp32(DGrid gm, DGrid gc, const floa
 // omega(X(x)) through the split map and the FP32 fine vorticity records; SampleVortT's mollifiers
 // and column sampling with the split evaluations.  The positions use the displacement without its
 // low parts: their error, 2^-24 |X - x|, moves the sampled vorticity by that times |grad omega|.
+// incomp_column on the split map: the corner values as load_map32 gives them (displacement, value
+// planes with their low parts), the arithmetic of k_incomp_tensor32
+template <int R>
+__device__ __forceinline__ double incomp32_column(const DGrid &gm, const double vx[16], const double vy[16], double xc,
+                                                  const double yc[R]) {
+    const double fx = (xc - gm.x0) * gm.ihx;
+    const double dx = fx - floor(fx);
+    double bx[4], dbx[4];
+    basis(dx, bx);
+    basis_d(dx, dbx);
+    double P[4][4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const double *w = (k & 1) ? bx : dbx;
+        const double *f = (k < 2) ? vx : vy;
+#pragma unroll
+        for (int n = 0; n < 2; ++n) {
+            P[k][n] = w[0] * f[2 * n] + w[1] * f[2 * n + 1] + gm.hx * (w[2] * f[4 + 2 * n] + w[3] * f[5 + 2 * n]);
+            P[k][2 + n] = w[0] * f[8 + 2 * n] + w[1] * f[9 + 2 * n] + gm.hx * (w[2] * f[12 + 2 * n] + w[3] * f[13 + 2 * n]);
+        }
+    }
+    double v = 0.0;
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const double fy = (yc[r] - gm.y0) * gm.ihy;
+        const double d = fy - floor(fy);
+        double by[4], dby[4];
+        basis(d, by);
+        basis_d(d, dby);
+        auto ysum = [&](const double *q, const double *cw) {
+            return cw[0] * q[0] + cw[1] * q[1] + gm.hy * (cw[2] * q[2] + cw[3] * q[3]);
+        };
+        const double ea = ysum(P[0], by) * gm.ihx;
+        const double eb = ysum(P[1], dby) * gm.ihx;
+        const double ec = ysum(P[2], by) * gm.ihy;
+        const double ed = ysum(P[3], dby) * gm.ihy;
+        v = fmax(v, fabs(ea + ed + (ea * ed - eb * ec)));
+    }
+    return v;
+}
+
 struct SampleVort32 {
     DGrid gm, gv, gf;
     const float4 *__restrict__ M;
@@ -3501,12 +3676,12 @@ struct SampleVort32 {
         const double x = gm.x0 + ix * gv.hx, y = gm.y0 + iy * gv.hy;
         double x2, y2, v;
         if (molly == 4) {
-            diffeo32<false, false>(gm, M, x, y, x2, y2);
+            diffeo32<false, true>(gm, M, x, y, x2, y2);
             v = wval(x2, y2) / 3.0;
             for (int m = 0; m < 4; ++m) {
                 x2 = x + gv.hx / 2.0 * ((m >> 2) & 1) * (-1 + 2 * (m & 1));
                 y2 = y + gv.hy / 2.0 * ((m >> 1) & 1) * (-1 + 2 * (m & 1));
-                diffeo32<false, false>(gm, M, x2, y2, x2, y2);
+                diffeo32<false, true>(gm, M, x2, y2, x2, y2);
                 v += wval(x2, y2) / 6.0;
             }
         } else if (molly == 8) {
@@ -3514,21 +3689,32 @@ struct SampleVort32 {
             for (int m = 0; m < 9; ++m) {
                 x2 = x + gv.hx * (-1 + m % 3) / 2.0;
                 y2 = y + gv.hy * (-1 + m / 3) / 2.0;
-                diffeo32<false, false>(gm, M, x2, y2, x2, y2);
+                diffeo32<false, true>(gm, M, x2, y2, x2, y2);
                 v += ((1 + (m % 3)) & 1) * ((1 + (m / 3)) & 1) * wval(x2, y2) / 16.0;
             }
         } else {
-            diffeo32<false, false>(gm, M, x, y, x2, y2);
+            diffeo32<false, true>(gm, M, x, y, x2, y2);
             v = wval(x2, y2);
         }
         return v;
     }
     template <int R>
-    __device__ __forceinline__ void column(int x, int y0, double *out) const {
+    __device__ __forceinline__ double check_points(int x, int y0) const {
+        double v = 0.0;
+        const double xc = gm.x0 + x * gv.hx + 0.5 * gv.hx;
+#pragma unroll 1
+        for (int r = 0; r < R; ++r) v = fmax(v, incomp32_point(gm, M, xc, gm.y0 + (y0 + r) * gv.hy + 0.5 * gv.hy));
+        return v;
+    }
+    // as SampleVortT::column (also its return value); CHK: the check at the column's cell
+    // centres.  The map's value planes are taken with their low parts (the check needs them: they
+    // reach the Jacobian through the basis derivative).
+    template <int R, bool CHK = false>
+    __device__ __forceinline__ bool column(int x, int y0, double *out, double &chk) const {
         if (molly != 0) {
 #pragma unroll
             for (int r = 0; r < R; ++r) out[r] = (*this)(x, y0 + r);
-            return;
+            return false;
         }
         const double xp = gm.x0 + x * gv.hx;
         Cell cf, cl;
@@ -3537,10 +3723,16 @@ struct SampleVort32 {
         if (!same_cell(cf, cl)) {
 #pragma unroll
             for (int r = 0; r < R; ++r) out[r] = (*this)(x, y0 + r);
-            return;
+            return false;
         }
         double vx[16], vy[16];
-        load_map32<false>(M, cf, vx, vy);
+        load_map32(M, cf, vx, vy);
+        if constexpr (CHK) {
+            double yc[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) yc[r] = gm.y0 + (y0 + r) * gv.hy + 0.5 * gv.hy;
+            chk = fmax(chk, incomp32_column<R>(gm, vx, vy, xp + 0.5 * gv.hx, yc));
+        }
         Cell wc[R];
         double wv[R][16];
 #pragma unroll
@@ -3555,6 +3747,12 @@ struct SampleVort32 {
         }
 #pragma unroll
         for (int r = 0; r < R; ++r) out[r] = herm_eval_vals(gf, wc[r], wv[r]);
+        return true;
+    }
+    template <int R>
+    __device__ __forceinline__ void column(int x, int y0, double *out) const {
+        double chk = 0.0;
+        column<R, false>(x, y0, out, chk);
     }
 };
 
@@ -3749,24 +3947,51 @@ int op_incomp32(const cmm_grid &gm, const cmm_grid &gc, const float *M, double *
     return CMM_OK;
 }
 
-int op_psi_from_vort32(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf, const cmm_grid &gp, const float *M,
-                       const float *W4, int molly, double fcut, double *psi, void *work, size_t work_bytes,
-                       hipStream_t st, hipEvent_t *ev) {
+// the split psi solve in the two parts of op_psi_sample / op_psi_spectral
+int op_psi_sample32(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf, const cmm_grid &gp, const float *M,
+                    const float *W4, int molly, double fcut, void *work, size_t work_bytes, hipStream_t st,
+                    hipEvent_t *ev, unsigned long long *chk_slot) {
     if (int rc = fft_grid_ok(gv, "vort grid")) return rc;
     if (int rc = fft_grid_ok(gp, "psi grid")) return rc;
     SpecPlan sp = plan_spec(gv, gp, 1, fcut);
     const size_t base = sp.t_bytes + 2 * sp.g_bytes + 256;
     void *w;
     if (int rc = get_work(work, work_bytes, base + sizeof(double) * (size_t)gv.nx * gv.ny, &w)) return rc;
+    if (chk_slot && !op_check_fusable(gm, gv, gv)) {
+        set_error("psi sampler: the check cannot run in the sampler's pass on these grids");
+        return CMM_ERR_ARG;
+    }
     // the sampled vorticity in FP32: its rounding (2^-24 relative) reaches psi through 1/|k|^2
     float *f = reinterpret_cast<float *>(static_cast<char *>(w) + base);
     const long lanes = (long)gv.nx * (gv.ny / kSampleRows);
+    const dim3 grid((unsigned)((lanes + 255) / 256));
     SampleVort32 smp{make_dgrid(gm), make_dgrid(gv), make_dgrid(gf), (const float4 *)M, (const float4 *)W4, molly};
     if (ev) (void)hipEventRecord(ev[0], st);
-    hipLaunchKernelGGL((k_sample_field<SampleVort32, float>), dim3((unsigned)((lanes + 255) / 256)), dim3(256), 0, st, smp,
-                       gv.nx, gv.ny, f);
-    if (int rc = launch_check("k_sample_field")) return rc;
+    if (chk_slot) {
+        hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, st, chk_slot);
+        hipLaunchKernelGGL((k_sample_field<SampleVort32, float, kSampleRows, true>), grid, dim3(256), 0, st, smp, gv.nx,
+                           gv.ny, f, chk_slot);
+    } else {
+        hipLaunchKernelGGL((k_sample_field<SampleVort32, float>), grid, dim3(256), 0, st, smp, gv.nx, gv.ny, f, nullptr);
+    }
+    return launch_check("k_sample_field");
+}
+
+int op_psi_spectral32(const cmm_grid &gv, const cmm_grid &gp, double fcut, double *psi, void *work, size_t work_bytes,
+                      hipStream_t st, hipEvent_t *ev) {
+    SpecPlan sp = plan_spec(gv, gp, 1, fcut);
+    const size_t base = sp.t_bytes + 2 * sp.g_bytes + 256;
+    void *w;
+    if (int rc = get_work(work, work_bytes, base + sizeof(double) * (size_t)gv.nx * gv.ny, &w)) return rc;
+    float *f = reinterpret_cast<float *>(static_cast<char *>(w) + base);
     return run_spec(sp, SampleRealF{f, gv.nx}, psi, w, st, ev, false);
+}
+
+int op_psi_from_vort32(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf, const cmm_grid &gp, const float *M,
+                       const float *W4, int molly, double fcut, double *psi, void *work, size_t work_bytes,
+                       hipStream_t st, hipEvent_t *ev) {
+    if (int rc = op_psi_sample32(gm, gv, gf, gp, M, W4, molly, fcut, work, work_bytes, st, ev, nullptr)) return rc;
+    return op_psi_spectral32(gv, gp, fcut, psi, work, work_bytes, st, ev);
 }
 
 int op_sample_vort32(const cmm_grid &gm, const cmm_grid &gt, const cmm_grid &gf, const float *M, const float *W4,

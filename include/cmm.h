@@ -410,6 +410,14 @@ int cmm_batch_stats(cmm_batch *b, int *graphs_captured, int *graph_launches, int
 /* Autotuner: times the incompressibility check's tile edge (4, 2, 1 dividing the fine/coarse
  * ratio) and block cap on the engine's current map and keeps the fastest for its later steps. */
 int cmm_euler_autotune(cmm_euler *e, int *tile_out, int *max_blocks_out, double *ms_out);
+/* Whether a step's incompressibility check (Cuda-CMM/src/simulation/cmm-euler2d.cu:796) runs in the
+ * pass of the psi solve's vorticity sampler instead of a kernel of its own: possible when the check
+ * grid is the vorticity grid, refining the map grid by a multiple of 4 over the same domain, in an
+ * Euler run without a forward map, and then on by default in the FP64 form (off in the split form,
+ * whose sampler runs faster without it).  The check points, their arithmetic and the remap decision
+ * are the same; the map is read once for both. */
+int cmm_euler_set_check_fusion(cmm_euler *e, int on);
+int cmm_euler_get_check_fusion(cmm_euler *e, int *on);
 
 
 /* ---- learned psi extrapolation (no counterpart in the reference) --------------------------------

@@ -357,3 +357,30 @@ def test_particle_sets_start_time_and_inertia(dev, tin, L, schemes):
             ok = np.isfinite(got[k][1])
             print("   velocity output", rel_linf(got[k][1][ok], ref[ok]), "outside", (~ok).sum())
             assert rel_linf(got[k][1][ok], ref[ok]) <= 1e-12
+
+
+@pytest.mark.parametrize("precision", ["fp64", "split"])
+def test_check_in_the_sampler_pass(dev, precision):
+    """The step's incompressibility check inside the psi sampler's pass (cmm_euler_set_check_fusion)
+    against the check kernel of its own, on the double shear layer (map 128^2, fine = vort 512^2,
+    psi 256^2, dt 1/64) with a threshold that remaps every few steps: the same remap steps, the same
+    E_inc and the same psi.  The check points and the terms of the sums are the same; the compiler
+    may contract products into fused multiply-adds differently in the two kernels, and the sums
+    cancel about 6/h of magnitude to h, so E_inc is held to 1e-12 absolute (measured 7e-15).  psi
+    is bit for bit the same."""
+    import cmm
+    runs = []
+    for fused in (False, True):
+        e = cmm.Euler(cmm.grid(128), cmm.grid(512), cmm.grid(256), cmm.grid(512), 1.0 / 64, freq_cut_psi=128.0,
+                      initial_condition="double_shear_layer", incomp_threshold=8e-6, stack_capacity=32,
+                      precision=precision)
+        e.set_check_fusion(fused)
+        e.init()
+        infos = [e.step(1.0 / 64) for _ in range(32)]
+        runs.append(([x.incomp for x in infos], [x.remapped for x in infos], e.field("psi")))
+        e.close()
+    (e0, r0, p0), (e1, r1, p1) = runs
+    print(precision, "remaps", sum(r0), "E_inc max", max(e0), "max diff", max(abs(a - b) for a, b in zip(e0, e1)))
+    assert sum(r0) >= 2 and r0 == r1
+    assert all(abs(a - b) <= 1e-12 for a, b in zip(e0, e1))
+    assert np.array_equal(p0, p1)
