@@ -65,6 +65,13 @@ def algorithmic(slot, c, kx_max, n_stack=0):
         # grid): the check's flops at the cell centres, the map read once for both
         b, f = algorithmic("psi_row_r2c", c, kx_max, n_stack)
         return b, f + 85.0 * nf
+    if slot in ("psi_row_c2r_pre", "psi_row_c2r_pre_split"):
+        # the transform also writes the next step's combined psi fields: the older L - 1 levels
+        # read, C1, C12 and C0 written (32-byte FP64 records, 16-byte packed records in the split
+        # form), the sums' flops
+        b, f = algorithmic("psi_row_c2r", c, kx_max, n_stack)
+        rec = 16 if slot.endswith("_split") else 32
+        return b + 32 * (L - 1) * npsi + 3 * rec * npsi, f + 2 * 2 * L * 4 * npsi
     if slot == "psi_col":
         return 16 * (kx_max + 1) * (c["n_vort"] + 2 * c["n_psi"]), 5.0 * (kx_max + 1) * 36 * c["n_psi"]
     if slot == "psi_row_c2r":
@@ -157,16 +164,22 @@ def cpu_baseline(c, threads, steps=16, remap_maps=32):
     return sps, (t3 - t2) - (t2 - t1), t5 - t4, (t1 - t0) + (t3 - t1) + (t5 - t4)
 
 
-# bench slot -> kernel-name patterns of the launches it times (the row-transform slot of the psi solve
-# is the sampling of omega(X(x)) followed by the row transform of the sampled field)
+# slot model (algorithmic()) -> kernel-name patterns of the launches the slot times (the row-transform
+# slot of the psi solve is the sampling of omega(X(x)) followed by the row transform of the sampled
+# field; "_fused": the sampler with the check in its pass; "_pre": the C2R transform that also writes
+# the next step's combined psi fields)
 SLOT_KERNELS = {"advect": ("k_map_advect_pair<30, true>",), "psi_combine": ("k_psi_combine<3>",),
                 "incomp": ("k_incomp_tensor<",),
-                "psi_row_r2c": ("k_sample_field<cmm::SampleVortT<true", "k_row_r2c<cmm::SampleReal>"),
-                "psi_col": ("k_col(",), "psi_row_c2r": ("k_row_c2r4(",), "particles": ("k_particles(",),
-                # the split-precision form's own kernels (its column and C2R stages are the FP64 ones)
-                "advect_split": ("k_map_advect32_pair<30, true>",), "psi_combine_split": ("k_psi_combine32<3>",),
+                "psi_row_r2c": ("k_sample_field<cmm::SampleVortT<true>, double, 4, false>", "k_row_r2c<cmm::SampleReal>"),
+                "psi_row_r2c_fused": ("k_sample_field<cmm::SampleVortT<true>, double, 4, true>",
+                                      "k_row_r2c<cmm::SampleReal>"),
+                "psi_col": ("k_col(",), "psi_row_c2r": ("k_row_c2r4(",), "psi_row_c2r_pre": ("k_row_c2r4_pre<0, 3>",),
+                "particles": ("k_particles(",),
+                # the split-precision form's own kernels (its column stage is the FP64 one)
+                "advect_split": ("k_map_advect32_pair<30, true>",), "psi_combine_split": ("k_psi_combine32<3, true>",),
                 "incomp_split": ("k_incomp_tensor32<",),
-                "psi_row_r2c_split": ("k_sample_field<cmm::SampleVort32", "k_row_r2c<cmm::SampleReal>")}
+                "psi_row_r2c_split": ("k_sample_field<cmm::SampleVort32, float, 4, false>", "k_row_r2c<cmm::SampleRealF>"),
+                "psi_row_c2r_pre_split": ("k_row_c2r4_pre<1, 3>",)}
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "cmm_euler_c2_hbm_traffic.json")
 FETCH_SCALE_DEFAULT, WRITE_SCALE_DEFAULT = 2.0, 1.0
 
@@ -223,7 +236,7 @@ def summarize_profiles(kt_dir, fetch_dir, write_dir, out_dir, commands):
              "", "fetch command: " + commands["fetch"], "write command: " + commands["write"], "",
              f"copy calibration: FETCH_SIZE {max(cf) if cf else float('nan'):.1f} KiB, WRITE_SIZE "
              f"{max(cw) if cw else float('nan'):.1f} KiB for 1048576 KiB -> scale fetch x{fs:.3f}, write x{ws:.3f}", "",
-             f"{'slot':12s} {'kernel':40s} {'n':>4s} {'fetch_MiB':>10s} {'write_MiB':>10s} {'alg_MiB':>10s} "
+             f"{'slot':22s} {'kernel':40s} {'n':>4s} {'fetch_MiB':>10s} {'write_MiB':>10s} {'alg_MiB':>10s} "
              f"{'ratio':>6s}"]
     out = {"fetch_scale": fs, "write_scale": ws, "slots": {}}
     for slot, pats in SLOT_KERNELS.items():
@@ -244,7 +257,7 @@ def summarize_profiles(kt_dir, fetch_dir, write_dir, out_dir, commands):
             alg = algorithmic(slot, c, kx_max)[0]
             out["slots"][slot] = {"fetch_bytes": fb, "write_bytes": wb, "algorithmic_bytes": alg, "dispatches": n,
                                   "kernels": names}
-            lines.append(f"{slot:12s} {' + '.join(names)[:40]:40s} {n:4d} {fb / 2**20:10.1f} {wb / 2**20:10.1f} "
+            lines.append(f"{slot:22s} {' + '.join(names)[:40]:40s} {n:4d} {fb / 2**20:10.1f} {wb / 2**20:10.1f} "
                          f"{alg / 2**20:10.1f} {(fb + wb) / alg:6.2f}")
     with open(os.path.join(out_dir, "cmm_euler_c2_hbm_traffic.txt"), "w") as f:
         f.write("\n".join(lines) + "\n")
@@ -342,20 +355,24 @@ def composition(cmm, torch, c, n_maps=32, reps=5):
                               "max_position_difference": dmax}}
 
 
-def roofline(c, prof, n_stack, suffix="", fused=False):
+def roofline(c, prof, n_stack, suffix="", fused=False, pre=False):
     """Per-slot rates from the engine's event profile and the roofline object of the dominant
     kernel: algorithmic bytes and flops per launch over its average duration; traffic from the
     committed counter summary.  suffix selects the algorithmic model of a form's slots; fused: the
-    check ran in the psi sampler's pass (its own slot then closes at once and is left out)."""
+    check ran in the psi sampler's pass (its own slot then closes at once and is left out); pre:
+    the psi solve's last transform wrote the next step's combined fields (the combination's slot
+    then closes at once and is left out)."""
     kx_max = int(min(c["n_psi"] // 2, c["n_vort"] // 2, c["freq_cut_psi"]))
     per = {}
     for name, (ms, cnt) in prof.items():
-        if fused and name == "incomp":
+        if (fused and name == "incomp") or (pre and name == "psi_combine"):
             continue
         if cnt:
             model = name + suffix if algorithmic(name + suffix, c, kx_max, n_stack)[0] else name
             if fused and name == "psi_row_r2c":
                 model = "psi_row_r2c_fused"
+            if pre and name == "psi_row_c2r":
+                model = "psi_row_c2r_pre" + suffix
             b, f = algorithmic(model, c, kx_max, n_stack)
             per[name] = dict(ms_total=ms, launches=cnt, ms_avg=ms / cnt, bytes=b, flops=f, model=model)
     for name, d in sorted(per.items(), key=lambda kv: -kv[1]["ms_total"]):
@@ -401,6 +418,7 @@ def run_form(cmm, torch, c, precision, warmup, steps, dt=None, t_end=None, profi
     eng.add_particles(uniform_particles(c["n_particles"], c["seed"]), scheme=30)
     warm = eng.init()
     fused = eng.check_fusion()
+    pre = eng.psi_precombine()
     t_init = time.perf_counter() - t_setup
     log(f"[{precision}] init + warm-up {t_init:.2f}s, warm-up E_inc {warm}")
     einc = []
@@ -426,7 +444,7 @@ def run_form(cmm, torch, c, precision, warmup, steps, dt=None, t_end=None, profi
     t1 = time.perf_counter()
     prof = eng.profile_read() if profile_every else {}
     _, t_now, n_stack = eng.state()
-    out = {"sps": i / (t1 - t0), "elapsed": t1 - t0, "steps": i, "sim_time": t_now - t_start, "fused": fused,
+    out = {"sps": i / (t1 - t0), "elapsed": t1 - t0, "steps": i, "sim_time": t_now - t_start, "fused": fused, "pre": pre,
            "sim_per_s": (t_now - t_start) / (t1 - t0), "prof": prof, "maps": n_stack, "t_end": t_now,
            "remaps": remaps, "init_s": t_init, "einc": np.array(einc), "w": eng.field("w_coarse"),
            "psi": eng.field("psi")[:4 * c["n_psi"] ** 2], "part": eng.particles(0)}
@@ -762,7 +780,7 @@ def main():
     sps, elapsed, prof, n_stack, t_now, remaps, t_init = (plain[k] for k in (
         "sps", "elapsed", "prof", "maps", "t_end", "remaps", "init_s"))
 
-    roof, per = roofline(c, prof, n_stack, fused=plain["fused"])
+    roof, per = roofline(c, prof, n_stack, fused=plain["fused"], pre=plain["pre"])
 
     comp = composition(cmm, torch, c)
     log(f"composition: {comp}")
@@ -786,7 +804,7 @@ def main():
                       "particles_max_abs": float(dpart.max()),
                       "e_inc_max_abs_diff": float(np.max(np.abs(sp["einc"] - plain["einc"])))},
                   "remaps_timed": sp["remaps"], "maps": sp["maps"],
-                  "roofline": roofline(c, sp["prof"], sp["maps"], "_split", fused=sp["fused"])[0],
+                  "roofline": roofline(c, sp["prof"], sp["maps"], "_split", fused=sp["fused"], pre=sp["pre"])[0],
                   "kernels_ms_avg": {k: round(ms / cnt, 5) for k, (ms, cnt) in sp["prof"].items() if cnt}}
         log(f"split form: {sp['sps']:.1f} steps/s ({chosen['speedup']:.2f}x), errors {chosen['errors_vs_fp64_at_t_end']}")
         # both forms at twice the step (plain Lagrange extrapolation of psi) to the same end time as
@@ -835,7 +853,8 @@ def main():
                                "IC seed 0, dt 1/256, RK3, 4th-order stencil, eps 1e-3, remap 1e-4, "
                                "freq_cut_psi 256, 4096 RK3 particles",
                    "form": "plain FP64", "t_end": round(t_now, 6), "remaps_timed": remaps, "maps": n_stack,
-                   "check_in_psi_sampler": plain["fused"], "init_s": round(t_init, 3)},
+                   "check_in_psi_sampler": plain["fused"], "psi_precombined": plain["pre"],
+                   "init_s": round(t_init, 3)},
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernels_ms_avg": {k: round(v["ms_avg"], 5) for k, v in per.items()},

@@ -85,7 +85,8 @@ SYMBOLS = ["cmm_version", "cmm_last_error", "cmm_device_info", "cmm_lagrange_coe
            "cmm_map_to_split", "cmm_map_from_split", "cmm_hermite_to_split", "cmm_map_advect_split_workspace_bytes",
            "cmm_map_advect_split", "cmm_incomp_error_split", "cmm_psi_from_vort_split",
            "cmm_batch_create", "cmm_batch_destroy", "cmm_batch_step", "cmm_batch_stats", "cmm_euler_autotune", "cmm_euler_set_check_fusion",
-           "cmm_euler_get_check_fusion",
+           "cmm_euler_get_check_fusion", "cmm_euler_set_psi_precombine",
+           "cmm_euler_get_psi_precombine",
            "cmm_cnn_forward", "cmm_cnn_backward_data", "cmm_cnn_backward_weights", "cmm_cnn_relu_mask",
            "cmm_cnn_fill", "cmm_cnn_weights_bf16", "cmm_cnn_adamw", "cmm_cnn_input", "cmm_cnn_target", "cmm_cnn_mse",
            "cmm_cnn_correction", "cmm_extrapolate", "cmm_dev_norms", "cmm_euler_copy_psi_level",
@@ -187,6 +188,8 @@ def lib():
         L.cmm_euler_autotune.argtypes = [VP, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]
         L.cmm_euler_set_check_fusion.argtypes = [VP, C.c_int]
         L.cmm_euler_get_check_fusion.argtypes = [VP, C.POINTER(C.c_int)]
+        L.cmm_euler_set_psi_precombine.argtypes = [VP, C.c_int]
+        L.cmm_euler_get_psi_precombine.argtypes = [VP, C.POINTER(C.c_int)]
         L.cmm_cnn_forward.argtypes = [C.c_int, C.c_int, C.c_int, VP, VP, VP, C.c_int, VP, VP]
         L.cmm_cnn_backward_data.argtypes = [C.c_int, C.c_int, C.c_int, VP, VP, VP, VP]
         L.cmm_cnn_backward_weights.argtypes = [C.c_int, C.c_int, C.c_int, VP, VP, VP, VP, VP]
@@ -640,6 +643,15 @@ class Euler:
     def set_check_fusion(self, on):
         """the step's incompressibility check in the psi sampler's pass (on by default where possible)"""
         _check(lib().cmm_euler_set_check_fusion(self.h, int(on)), "cmm_euler_set_check_fusion")
+
+    def set_psi_precombine(self, on):
+        """the psi solve also writes the next step's combined psi fields (on by default)"""
+        _check(lib().cmm_euler_set_psi_precombine(self.h, int(on)), "cmm_euler_set_psi_precombine")
+
+    def psi_precombine(self):
+        on = C.c_int()
+        _check(lib().cmm_euler_get_psi_precombine(self.h, C.byref(on)), "cmm_euler_get_psi_precombine")
+        return bool(on.value)
 
     def check_fusion(self):
         on = C.c_int()

@@ -383,6 +383,13 @@ struct cmm_euler {
     // the step's check runs inside the psi sampler's pass (the check grid is the vorticity grid);
     // sampled: the current step's sampling is queued, its spectral part not yet
     bool fuse_check = false, sampled = false;
+    // the next step's combined psi fields (cpre, three fields), written by the psi solve's last
+    // transform for the weights of a next step as long as this one (pre); the next advection reads
+    // them in place of combining when its own weights, levels and scheme are those
+    void *cpre = nullptr;
+    PsiPre pre{};
+    int pre_tin = 0;
+    bool pre_valid = false, pre_enabled = true;
     // corrections (device Hermite fields on g_psi) added to the Lagrange extrapolations of psi
     // to t_{n+1} and t_n + dt/2 in the next steps (cmm_euler_set_psi_correction)
     const double *corr1 = nullptr, *corr12 = nullptr;
@@ -490,7 +497,7 @@ int engine_translate(cmm_euler *e) {
     return e->W4 ? op_interleave4(e->p.g_fine, e->W, e->W4, e->st) : CMM_OK;
 }
 
-int engine_psi(cmm_euler *e, double *out, hipEvent_t *ev = nullptr) {
+int engine_psi(cmm_euler *e, double *out, hipEvent_t *ev = nullptr, const PsiPre *pre = nullptr) {
     if (e->dry) return split(e) ? (e->p.sim == CMM_SIM_VLASOV_POISSON_1D ? sync_planes(e) : sync_m32(e)) : CMM_OK;
     if (e->p.sim == CMM_SIM_VLASOV_POISSON_1D) {
         // one operator; its whole time goes to the first psi profile slot
@@ -505,10 +512,60 @@ int engine_psi(cmm_euler *e, double *out, hipEvent_t *ev = nullptr) {
     if (split(e)) {
         if (int rc = sync_m32(e)) return rc;
         return op_psi_from_vort32(e->p.g_coarse, e->p.g_vort, e->p.g_fine, e->p.g_psi, e->m32[e->m32_cur], e->W4f,
-                                  e->p.molly_stencil, e->p.freq_cut_psi, out, e->work, e->work_bytes, e->st, ev);
+                                  e->p.molly_stencil, e->p.freq_cut_psi, out, e->work, e->work_bytes, e->st, ev, pre);
     }
     return op_psi_from_vort(e->p.g_coarse, e->p.g_vort, e->p.g_fine, e->p.g_psi, e->cx, e->cy, e->W,
-                            e->p.molly_stencil, e->p.freq_cut_psi, out, e->work, e->work_bytes, e->st, ev, e->W4);
+                            e->p.molly_stencil, e->p.freq_cut_psi, out, e->work, e->work_bytes, e->st, ev, e->W4, pre);
+}
+
+// the next step's combined fields to be written by this step's psi solve, for a next step as long
+// as this one (null where the engine state does not allow them)
+const PsiPre *prepare_pre(cmm_euler *e) {
+    const cmm_euler_params &p = e->p;
+    e->pre_valid = false;
+    if (!e->pre_enabled || !e->cpre || e->dry || e->capturing || e->lw_graph || e->corr1 || e->corr12 ||
+        p.sim != CMM_SIM_EULER_2D || !e->initialised)
+        return nullptr;
+    const int L = p.l_order, lc = e->loop + L - 1;
+    if ((int)e->tv.size() < lc + 2) return nullptr;
+    std::vector<double> t2(e->tv.begin(), e->tv.begin() + lc + 2), d2(e->dtv.begin(), e->dtv.begin() + lc + 2);
+    t2.push_back(t2[lc + 1] + d2[lc + 1]);
+    d2.push_back(d2[lc + 1]);
+    cmm_advect_coeffs c;
+    if (cmm_advect_coeffs_fill(t2.data(), d2.data(), e->loop + 1, L, p.time_integration_num, p.map_update_order_num,
+                               p.map_epsilon, -1, &c))
+        return nullptr;
+    const int f = op_advect_fields(c);
+    if (!f) return nullptr;
+    const bool sp = split(e) && op_advect32_supported(c);
+    if (!sp && op_advect_workspace(p.g_coarse, p.g_psi, c) == 0) return nullptr;
+    const size_t field = (sp ? sizeof(float) : sizeof(double)) * 4 * (size_t)e->np;
+    PsiPre &q = e->pre;
+    q = PsiPre{};
+    for (int l = 1; l < L; ++l) q.P.p[l] = e->lvl[l];
+    q.L = L;
+    q.form = sp ? 1 : 0;
+    std::memcpy(q.w1, c.L1, sizeof(q.w1));
+    std::memcpy(q.w12, c.L12, sizeof(q.w12));
+    char *base = static_cast<char *>(e->cpre);
+    q.C1 = base;
+    q.C12 = (f & 2) ? base + field : nullptr;
+    q.C0 = (f & 4) ? base + 2 * field : nullptr;
+    e->pre_tin = c.time_integration_num;
+    return &e->pre;
+}
+
+// the precombined fields for this advection's coefficients, or null
+const PsiPre *take_pre(cmm_euler *e, const cmm_advect_coeffs &c, int form) {
+    const bool ok = e->pre_valid && !e->lw_graph && !(e->initialised && (e->corr1 || e->corr12)) &&
+                    e->pre.form == form && e->pre.L == c.l_order && e->pre_tin == c.time_integration_num &&
+                    std::memcmp(e->pre.w1, c.L1, sizeof(double) * c.l_order) == 0 &&
+                    std::memcmp(e->pre.w12, c.L12, sizeof(double) * c.l_order) == 0 &&
+                    (op_advect_fields(c) & ~(1 | (e->pre.C12 ? 2 : 0) | (e->pre.C0 ? 4 : 0))) == 0;
+    bool levels = true;
+    for (int l = 1; l < c.l_order; ++l) levels = levels && e->pre.P.p[l] == e->lvl[l];
+    e->pre_valid = false;
+    return ok && levels ? &e->pre : nullptr;
 }
 
 bool check_fusable(const cmm_euler *e) {
@@ -529,10 +586,11 @@ int engine_psi_sample(cmm_euler *e, hipEvent_t *ev) {
                          e->p.freq_cut_psi, e->work, e->work_bytes, e->st, ev, e->W4, e->slot);
 }
 
-int engine_psi_spectral(cmm_euler *e, double *out, hipEvent_t *ev) {
+int engine_psi_spectral(cmm_euler *e, double *out, hipEvent_t *ev, const PsiPre *pre = nullptr) {
     if (e->dry) return CMM_OK;
-    if (split(e)) return op_psi_spectral32(e->p.g_vort, e->p.g_psi, e->p.freq_cut_psi, out, e->work, e->work_bytes, e->st, ev);
-    return op_psi_spectral(e->p.g_vort, e->p.g_psi, e->p.freq_cut_psi, out, e->work, e->work_bytes, e->st, ev);
+    if (split(e))
+        return op_psi_spectral32(e->p.g_vort, e->p.g_psi, e->p.freq_cut_psi, out, e->work, e->work_bytes, e->st, ev, pre);
+    return op_psi_spectral(e->p.g_vort, e->p.g_psi, e->p.freq_cut_psi, out, e->work, e->work_bytes, e->st, ev, pre);
 }
 
 int engine_advect(cmm_euler *e, const double *t, const double *dt, int loop_ctr, int L, int tin) {
@@ -546,6 +604,8 @@ int engine_advect(cmm_euler *e, const double *t, const double *dt, int loop_ctr,
         std::memcpy(e->lw_host + 4, c.L12, sizeof(c.L12));
     }
     const double *lw = e->lw_graph ? e->lw_dev : nullptr;
+    // the combined fields the previous step's psi solve wrote for these coefficients, if any
+    const PsiPre *pre_in = take_pre(e, c, split(e) && op_advect32_supported(c) ? 1 : 0);
     if (split(e) && op_advect32_supported(c)) {
         if (int rc = sync_m32(e)) return rc;
         if (!e->dry)
@@ -556,7 +616,7 @@ int engine_advect(cmm_euler *e, const double *t, const double *dt, int loop_ctr,
                                          e->work, e->work_bytes, e->st, e->prof ? e->ev[10] : nullptr,
                                          e->p.sim == CMM_SIM_VLASOV_POISSON_1D ? PsiRec64 : PsiRec32Pk,
                                          e->initialised ? e->corr1 : nullptr,
-                                         e->initialised ? e->corr12 : nullptr, lw))
+                                         e->initialised ? e->corr12 : nullptr, lw, pre_in))
                 return rc;
         e->m32_cur = 1 - e->m32_cur;
         e->m32_fresh = true;
@@ -593,7 +653,8 @@ int engine_advect(cmm_euler *e, const double *t, const double *dt, int loop_ctr,
     if (!e->dry)
         if (int rc = op_map_advect(e->p.g_coarse, e->p.g_psi, e->cx, e->cy, P, c, e->ncx, e->ncy, e->work,
                                    e->work_bytes, e->st, e->prof ? e->ev[10] : nullptr, m8_in, m8_out,
-                                   e->initialised ? e->corr1 : nullptr, e->initialised ? e->corr12 : nullptr, lw))
+                                   e->initialised ? e->corr1 : nullptr, e->initialised ? e->corr12 : nullptr, lw,
+                                   comb ? pre_in : nullptr))
             return rc;
     std::swap(e->cx, e->ncx);
     std::swap(e->cy, e->ncy);
@@ -793,6 +854,11 @@ int cmm_euler_create(const cmm_euler_params *p, cmm_euler **out) {
     if (!rc && sp) rc = falloc(&e->m32[0], 12 * e->nc);
     if (!rc && sp) rc = falloc(&e->m32[1], 12 * e->nc);
     if (!rc && sp && p->sim == CMM_SIM_EULER_2D) rc = falloc(&e->W4f, 4 * e->nf);
+    // the next step's combined psi fields written by the psi solve (three fields of FP64 records,
+    // or of packed FP32 records in the split form)
+    if (!rc && p->sim == CMM_SIM_EULER_2D)
+        rc = hip_check(hipMalloc(&e->cpre, 3 * (sp ? sizeof(float) : sizeof(double)) * 4 * (size_t)e->np),
+                       "hipMalloc (combined psi fields)");
     if (!rc && p->stack_capacity > 0) rc = dalloc(&e->stack, (size_t)8 * e->nc * p->stack_capacity);
     if (!rc && sp && p->stack_capacity > 0) rc = dalloc(&e->stack_s, (size_t)6 * e->nc * p->stack_capacity);
     if (p->forward_map) {
@@ -853,6 +919,19 @@ int cmm_euler_set_check_fusion(cmm_euler *e, int on) {
     return CMM_OK;
 }
 
+int cmm_euler_set_psi_precombine(cmm_euler *e, int on) {
+    if (!e) { set_error("cmm_euler_set_psi_precombine: null engine"); return CMM_ERR_ARG; }
+    e->pre_enabled = on != 0;
+    e->pre_valid = false;
+    return CMM_OK;
+}
+
+int cmm_euler_get_psi_precombine(cmm_euler *e, int *on) {
+    if (!e || !on) { set_error("cmm_euler_get_psi_precombine: bad argument"); return CMM_ERR_ARG; }
+    *on = e->pre_enabled && e->cpre ? 1 : 0;
+    return CMM_OK;
+}
+
 int cmm_euler_get_check_fusion(cmm_euler *e, int *on) {
     if (!e || !on) { set_error("cmm_euler_get_check_fusion: bad argument"); return CMM_ERR_ARG; }
     *on = e->fuse_check ? 1 : 0;
@@ -866,6 +945,7 @@ int cmm_euler_destroy(cmm_euler *e) {
                       e->psi_buf[0], e->psi_buf[1], e->psi_buf[2], e->psi_buf[3], e->fx, e->fy, e->nfx, e->nfy, e->fstack, e->stack_s};
     for (double *b : bufs)
         if (b) (void)hipFree(b);
+    if (e->cpre) (void)hipFree(e->cpre);
     for (double *b : e->part)
         if (b) (void)hipFree(b);
     for (double *b : e->part_vel)
@@ -944,6 +1024,7 @@ int cmm_euler_init(cmm_euler *e, double *warm, int *nwarm) {
     const cmm_euler_params &p = e->p;
     const int L = p.l_order;
     int nw = 0;
+    e->pre_valid = false;
     // Cuda-CMM/src/simulation/cmm-euler2d.cu:495-512
     if (int rc = op_map_identity(p.g_coarse, e->cx, e->cy, e->st)) return rc;
     if (p.forward_map)
@@ -1011,6 +1092,7 @@ int cmm_euler_restart(cmm_euler *e, double t_restart, int n_maps, const double *
     }
     const cmm_euler_params &p = e->p;
     const int L = p.l_order;
+    e->pre_valid = false;
     if (n_maps > p.stack_capacity) {
         set_error("cmm_euler_restart: more maps than the stack holds");
         return CMM_ERR_ARG;
@@ -1200,12 +1282,14 @@ int step_queue_b(cmm_euler *e, bool fork) {
     e->lvl[0] = oldest;
     if (e->part_pending)  // the previous step's particles still read the level about to be rewritten
         if (int rc = hip_check(hipStreamWaitEvent(e->st, e->ev_part, 0), "hipStreamWaitEvent")) return rc;
+    const PsiPre *pre = prepare_pre(e);
     if (e->sampled) {
         e->sampled = false;
-        if (int rc = engine_psi_spectral(e, e->lvl[0], e->prof ? &e->ev[5] : nullptr)) return rc;
-    } else if (int rc = engine_psi(e, e->lvl[0], e->prof ? &e->ev[5] : nullptr)) {
+        if (int rc = engine_psi_spectral(e, e->lvl[0], e->prof ? &e->ev[5] : nullptr, pre)) return rc;
+    } else if (int rc = engine_psi(e, e->lvl[0], e->prof ? &e->ev[5] : nullptr, pre)) {
         return rc;
     }
+    e->pre_valid = pre != nullptr;
     // no rejoin of the check's stream: the host waits for ev_inc, recorded there after the check,
     // before it queues anything else for this engine (step_finish), so all later work on the
     // engine stream follows the check; a stream wait would only add a barrier packet (about 12 us
@@ -1259,7 +1343,10 @@ int step_finish(cmm_euler *e, cmm_step_info *info, bool waited) {
             e->nstack++;
             if (int rc = op_map_identity(p.g_coarse, e->cx, e->cy, e->st)) return rc;
             planes_written(e);
-            if (int rc = engine_psi(e, e->lvl[0])) return rc;
+            // the new psi also rewrites the next step's combined fields
+            const PsiPre *pre = prepare_pre(e);
+            if (int rc = engine_psi(e, e->lvl[0], nullptr, pre)) return rc;
+            e->pre_valid = pre != nullptr;
             prof_mark(e, 4);
             remapped = 1;
         }
@@ -1657,6 +1744,7 @@ int cmm_euler_set_psi_correction(cmm_euler *e, const double *d1, const double *d
     }
     e->corr1 = d1;
     e->corr12 = d12;
+    e->pre_valid = false;
     return CMM_OK;
 }
 

@@ -210,6 +210,18 @@ struct PsiLevels {
     const double *p[4];
 };
 
+// The next step's Lagrange-combined psi fields written by the psi solve's last transform
+// (k_row_c2r4_pre): P.p[l], l = 1 .. L - 1, the older levels (level 0 is the transform's output),
+// the weights of C1 (w1) and C12 (w12), and the fields (C12, C0 may be null); form 0: FP64
+// records, 1: packed FP32 records.
+struct PsiPre {
+    PsiLevels P;
+    int L;
+    int form;
+    double w1[4], w12[4];
+    void *C1, *C12, *C0;
+};
+
 CMM_HD void herm_vel(const DGrid &g, const PsiLevels &P, int nl, double x, double y, double *u) {
     Cell c;
     find_cell(g, x, y, c);

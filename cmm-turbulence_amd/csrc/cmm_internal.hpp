@@ -21,7 +21,13 @@ size_t op_advect_workspace(const cmm_grid &gm, const cmm_grid &gp, const cmm_adv
 int op_map_advect(const cmm_grid &gm, const cmm_grid &gp, const double *cx, const double *cy, const PsiLevels &P,
                   const cmm_advect_coeffs &c, double *ncx, double *ncy, void *work, size_t work_bytes,
                   hipStream_t st, hipEvent_t mid = nullptr, const double *m8_in = nullptr, double *m8_out = nullptr,
-                  const double *corr1 = nullptr, const double *corr12 = nullptr, const double *lw = nullptr);
+                  const double *corr1 = nullptr, const double *corr12 = nullptr, const double *lw = nullptr,
+                  const PsiPre *pre_in = nullptr);
+// (pre_in: combined fields written by the previous psi solve for these weights, used in place of
+// the combination kernel)
+// the combined psi fields a scheme's advection reads: 0 (the scheme reads the levels on their own),
+// else 1 (C1) | 2 (C12) | 4 (C0)
+int op_advect_fields(const cmm_advect_coeffs &c);
 // tile (0: the largest of 4, 3, 2, 1 dividing the refinement ratio) and cap on the number of
 // blocks of the tiled check; set by the autotuner
 struct IncompTune {
@@ -41,7 +47,8 @@ int op_invert(const cmm_grid &gb, const cmm_grid &gf, const cmm_grid &gc, const 
 size_t op_psi_workspace(const cmm_grid &gv, const cmm_grid &gp, double fcut);
 int op_psi_from_vort(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf, const cmm_grid &gp, const double *cx,
                      const double *cy, const double *W, int molly, double fcut, double *psi, void *work,
-                     size_t work_bytes, hipStream_t st, hipEvent_t *ev = nullptr, const double *W4 = nullptr);
+                     size_t work_bytes, hipStream_t st, hipEvent_t *ev = nullptr, const double *W4 = nullptr,
+                     const PsiPre *pre = nullptr);
 // op_psi_from_vort in two parts: the sampling of omega(X(x)) into the workspace, optionally with
 // the incompressibility check on the same grid in the same pass (chk_slot, cleared by the call;
 // possible when op_check_fusable(gm, gv, gc) holds for the check grid gc), and the spectral part
@@ -49,8 +56,9 @@ bool op_check_fusable(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gc
 int op_psi_sample(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf, const cmm_grid &gp, const double *cx,
                   const double *cy, const double *W, int molly, double fcut, void *work, size_t work_bytes,
                   hipStream_t st, hipEvent_t *ev, const double *W4, unsigned long long *chk_slot);
+// (pre: the transform also writes the next step's combined fields, k_row_c2r4_pre)
 int op_psi_spectral(const cmm_grid &gv, const cmm_grid &gp, double fcut, double *psi, void *work, size_t work_bytes,
-                    hipStream_t st, hipEvent_t *ev);
+                    hipStream_t st, hipEvent_t *ev, const PsiPre *pre = nullptr);
 // op_psi_from_vort with its workspace and twiddle tables in a static device array (no allocation);
 // op_psi_noalloc_max_bytes() is the largest workspace it takes
 size_t op_psi_noalloc_max_bytes();
@@ -158,18 +166,18 @@ size_t op_advect32_workspace(const cmm_grid &gp, const cmm_advect_coeffs &c, int
 int op_map_advect32(const cmm_grid &gm, const cmm_grid &gp, const PsiLevels &P, const cmm_advect_coeffs &c,
                     const float *M_in, float *M_out, void *work, size_t work_bytes, hipStream_t st,
                     hipEvent_t mid = nullptr, int psi_form = PsiRec32Pk, const double *corr1 = nullptr,
-                    const double *corr12 = nullptr, const double *lw = nullptr);
+                    const double *corr12 = nullptr, const double *lw = nullptr, const PsiPre *pre_in = nullptr);
 int op_incomp32_launch(const cmm_grid &gm, const cmm_grid &gc, const float *M, unsigned long long *slot,
                        hipStream_t st, IncompTune tune = IncompTune());
 int op_incomp32(const cmm_grid &gm, const cmm_grid &gc, const float *M, double *out, hipStream_t st);
 int op_psi_from_vort32(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf, const cmm_grid &gp, const float *M,
                        const float *W4, int molly, double fcut, double *psi, void *work, size_t work_bytes,
-                       hipStream_t st, hipEvent_t *ev = nullptr);
+                       hipStream_t st, hipEvent_t *ev = nullptr, const PsiPre *pre = nullptr);
 int op_psi_sample32(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf, const cmm_grid &gp, const float *M,
                     const float *W4, int molly, double fcut, void *work, size_t work_bytes, hipStream_t st,
                     hipEvent_t *ev, unsigned long long *chk_slot);
 int op_psi_spectral32(const cmm_grid &gv, const cmm_grid &gp, double fcut, double *psi, void *work, size_t work_bytes,
-                      hipStream_t st, hipEvent_t *ev);
+                      hipStream_t st, hipEvent_t *ev, const PsiPre *pre = nullptr);
 int op_sample_vort32(const cmm_grid &gm, const cmm_grid &gt, const cmm_grid &gf, const float *M, const float *W4,
                      int molly, double *out, hipStream_t st);
 

@@ -1606,6 +1606,123 @@ __global__ void __launch_bounds__(kFftThreads, 4) k_row_c2r4(int nx, int ny, int
     }
 }
 
+// FP32 Hermite records whose value carries a low part (PsiRec32Pk): the value's leading FP32
+// part sits in .x and an 8-bit signed code q of its remainder in the low byte of .w, whose upper
+// 24 bits hold the cross derivative rounded to 15 explicit mantissa bits.  The remainder
+// f - (float)f is at most half an FP32 ulp of the leading part; q counts it in units of 1/256 of
+// that ulp (clamped to 127), so the value is kept to 2^-31 of itself or better.  The value plane
+// of a stream function with |psi| of order one rounded to FP32 alone would move the velocity by
+// 2^-24 |psi| 6/h, which the check's 1/h amplifies again; the cross derivative enters the
+// velocity with weight h/7 or less, where its 2^-16 rounding stays at the level of the first
+// derivatives' FP32 rounding.
+__device__ __forceinline__ float4 pack_psi_rec(const double o[4]) {
+    const float hi = (float)o[0];
+    const unsigned e = (__float_as_uint(hi) >> 23) & 0xFFu;
+    // 256 / ulp(hi) = 2^(158 - e), exact as a double for every biased exponent e
+    const double inv = __longlong_as_double((long long)(1023 + 158 - (int)e) << 52);
+    int q = (int)rint((o[0] - (double)hi) * inv);
+    q = q < -128 ? -128 : q > 127 ? 127 : q;
+    unsigned w = __float_as_uint((float)o[3]);
+    w = ((w + 0x80u) & 0xFFFFFF00u) | ((unsigned)q & 0xFFu);
+    return float4{hi, (float)o[1], (float)o[2], __uint_as_float(w)};
+}
+
+__device__ __forceinline__ void unpack_psi_rec(const float4 a, double &f, double &fxy) {
+    const unsigned w = __float_as_uint(a.w);
+    const unsigned e = (__float_as_uint(a.x) >> 23) & 0xFFu;
+    const double s = __longlong_as_double((long long)(e + 865u) << 52);  // ulp(hi) / 256 = 2^(e - 158)
+    f = (double)a.x + (double)(int)(signed char)(w & 0xFFu) * s;
+    fxy = (double)__uint_as_float(w & 0xFFFFFF00u);
+}
+
+// k_row_c2r4 writing, besides the four planes of the new psi level, the next step's Lagrange-combined
+// fields (k_psi_combine's C1, C12, C0, same weights and summation order) from the new level and
+// the older ones: once a lane has formed the pass-1 planes of a node it reads back the pass-0 planes
+// it wrote itself (its own earlier stores, in order) and combines.  FORM 0: FP64 node records (32
+// bytes), 1: packed FP32 records (pack_psi_rec).  L: Lagrange order.
+template <int FORM, int L>
+__global__ void __launch_bounds__(kFftThreads, 4) k_row_c2r4_pre(int nx, int ny, int logn, int kx_max, double fkx,
+                                                              const cd *__restrict__ G0, const cd *__restrict__ G1,
+                                                              const cd *__restrict__ tw, double *__restrict__ out,
+                                                              PsiPre pre) {
+    extern __shared__ cd s[];
+    constexpr int KM = kFftMaxN / kFftThreads;
+    const int y = xcd_block(blockIdx.x, gridDim.x);
+    const long N = (long)nx * ny;
+    const int h = nx / 2;
+    int z = 0;  // zero, made to depend on the previous node's result (see below)
+    for (int pass = 0; pass < 2; ++pass) {
+        const cd *G = pass == 0 ? G0 : G1;
+        for (int k = threadIdx.x; k <= h; k += kFftThreads) {
+            cd A = {0.0, 0.0};
+            if (k <= kx_max) A = G[(long)k * ny + y];
+            const double kv = fkx * k;
+            cd B = {-A.y * kv, A.x * kv};  // i kx A
+            if (k == 0 || k == h) {
+                s[sw(k)] = cd{A.x, B.x};
+            } else {
+                s[sw(k)] = cd{A.x - B.y, A.y + B.x};       // A + iB
+                s[sw(nx - k)] = cd{A.x + B.y, -A.y + B.x};  // conj(A) + i conj(B)
+            }
+        }
+        __syncthreads();
+        wg_fft(s, nx, logn, tw, true);
+        double *o0 = out + (2 * pass) * N + (long)y * nx;
+        double *o1 = out + (2 * pass + 1) * N + (long)y * nx;
+#pragma unroll
+        for (int q = 0; q < KM; ++q) {
+            const int x = threadIdx.x + q * kFftThreads;
+            if (x >= nx) continue;
+            const cd v = s[sw(x)];
+            o0[x] = v.x;
+            o1[x] = v.y;
+            if (pass == 0) continue;
+            // the older levels' loads of a node are issued after the previous node's fields are
+            // formed, so that they do not all take registers at once
+            const long j = (long)y * nx + x + z;
+            double lv[L][4];
+            lv[0][0] = out[j];
+            lv[0][1] = out[N + j];
+            lv[0][2] = v.x;
+            lv[0][3] = v.y;
+#pragma unroll
+            for (int l = 1; l < L; ++l)
+#pragma unroll
+                for (int p = 0; p < 4; ++p) lv[l][p] = pre.P.p[l][p * N + j];
+            auto comb = [&](const double *w, void *dst) {
+                double o[4];
+#pragma unroll
+                for (int p = 0; p < 4; ++p) {
+                    double a = w[0] * lv[0][p];
+#pragma unroll
+                    for (int l = 1; l < L; ++l) a += w[l] * lv[l][p];
+                    o[p] = a;
+                }
+                if constexpr (FORM == 0) {
+                    double2 *d = reinterpret_cast<double2 *>(dst) + 2 * j;
+                    d[0] = double2{o[0], o[1]};
+                    d[1] = double2{o[2], o[3]};
+                } else {
+                    reinterpret_cast<float4 *>(dst)[j] = pack_psi_rec(o);
+                }
+            };
+            comb(pre.w1, pre.C1);
+            z = zero_after(lv[L - 1][3]);
+            if (pre.C12) comb(pre.w12, pre.C12);
+            if (pre.C0) {
+                if constexpr (FORM == 0) {
+                    double2 *d = reinterpret_cast<double2 *>(pre.C0) + 2 * j;
+                    d[0] = double2{lv[0][0], lv[0][1]};
+                    d[1] = double2{lv[0][2], lv[0][3]};
+                } else {
+                    reinterpret_cast<float4 *>(pre.C0)[j] = pack_psi_rec(lv[0]);
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // ------------------------------------------------------------------------------------------------
 // Point kernels used outside the fused transforms
 
@@ -2234,7 +2351,8 @@ size_t spec_workspace_bytes(const cmm_grid &gin, const cmm_grid &gout, int mode,
 // in place of the cached ones (which are allocated on first use)
 template <class S>
 static int run_spec(const SpecPlan &sp, const S &sampler, double *out, void *work, hipStream_t st,
-                    hipEvent_t *ev = nullptr, bool ev0 = true, const cd *tw_table = nullptr) {
+                    hipEvent_t *ev = nullptr, bool ev0 = true, const cd *tw_table = nullptr,
+                    const PsiPre *pre = nullptr) {
     const cd *tw_in, *tw_out, *tw_x_in, *tw_x_out;
     if (tw_table) {
         tw_x_in = tw_table + (sp.gin.nx - 8);
@@ -2283,7 +2401,30 @@ static int run_spec(const SpecPlan &sp, const S &sampler, double *out, void *wor
         (void)hipEventRecord(ev[1], st);
     }
     if (ev) (void)hipEventRecord(ev[2], st);
-    {
+    if (pre) {
+        // the new level and the next step's combined fields (pre->form, pre->L)
+        size_t lds = sizeof(cd) * sp.gout.nx;
+        const void *kern = nullptr;
+#define CMM_PRE_K(F, LL) (const void *)k_row_c2r4_pre<F, LL>
+        if (pre->form == 0) kern = pre->L == 2 ? CMM_PRE_K(0, 2) : pre->L == 3 ? CMM_PRE_K(0, 3) : CMM_PRE_K(0, 4);
+        else kern = pre->L == 2 ? CMM_PRE_K(1, 2) : pre->L == 3 ? CMM_PRE_K(1, 3) : CMM_PRE_K(1, 4);
+#undef CMM_PRE_K
+        if (int rc = set_lds(kern, lds)) return rc;
+#define CMM_PRE_L(F, LL)                                                                                               \
+    hipLaunchKernelGGL((k_row_c2r4_pre<F, LL>), dim3(sp.gout.ny), dim3(kFftThreads), lds, st, sp.gout.nx, sp.gout.ny,  \
+                       ilog2(sp.gout.nx), kxm, kTwoPi / (dout.hx * sp.gout.nx), G0, G1, tw_x_out, out, *pre)
+        if (pre->form == 0) {
+            if (pre->L == 2) CMM_PRE_L(0, 2);
+            else if (pre->L == 3) CMM_PRE_L(0, 3);
+            else CMM_PRE_L(0, 4);
+        } else {
+            if (pre->L == 2) CMM_PRE_L(1, 2);
+            else if (pre->L == 3) CMM_PRE_L(1, 3);
+            else CMM_PRE_L(1, 4);
+        }
+#undef CMM_PRE_L
+        if (int rc = launch_check("k_row_c2r4_pre")) return rc;
+    } else {
         size_t lds = sizeof(cd) * sp.gout.nx;
         if (int rc = set_lds((const void *)k_row_c2r4, lds)) return rc;
         hipLaunchKernelGGL(k_row_c2r4, dim3(sp.gout.ny), dim3(kFftThreads), lds, st, sp.gout.nx, sp.gout.ny,
@@ -2339,6 +2480,11 @@ static bool advect_needs_c12(const cmm_advect_coeffs &c) {
 
 static bool advect_needs_c0(const cmm_advect_coeffs &c) { return c.time_integration_num != 10; }
 
+int op_advect_fields(const cmm_advect_coeffs &c) {
+    if (!advect_combinable(c)) return 0;
+    return 1 | (advect_needs_c12(c) ? 2 : 0) | (advect_needs_c0(c) ? 4 : 0);
+}
+
 size_t op_advect_workspace(const cmm_grid &gm, const cmm_grid &gp, const cmm_advect_coeffs &c) {
     if (!advect_combinable(c)) return 0;
     const int fields = 1 + advect_needs_c12(c) + advect_needs_c0(c);
@@ -2348,7 +2494,7 @@ size_t op_advect_workspace(const cmm_grid &gm, const cmm_grid &gp, const cmm_adv
 int op_map_advect(const cmm_grid &gm, const cmm_grid &gp, const double *cx, const double *cy, const PsiLevels &P,
                   const cmm_advect_coeffs &c, double *ncx, double *ncy, void *work, size_t work_bytes,
                   hipStream_t st, hipEvent_t mid, const double *m8_in, double *m8_out, const double *corr1,
-                  const double *corr12, const double *lw) {
+                  const double *corr12, const double *lw, const PsiPre *pre_in) {
     DGrid dm = make_dgrid(gm), dp = make_dgrid(gp);
     const int nfoot = 4 + 4 * c.map_update_order_num;
     const int G = nfoot <= 4 ? 4 : nfoot <= 8 ? 8 : 16;
@@ -2382,6 +2528,10 @@ int op_map_advect(const cmm_grid &gm, const cmm_grid &gp, const double *cx, cons
         set_error("map advection: a psi correction needs a combinable scheme and the combination workspace");
         return CMM_ERR_ARG;
     }
+    if (pre_in && (!comb || corr1 || corr12 || lw || pre_in->form != 0)) {
+        set_error("map advection: precombined fields need the combination path without corrections");
+        return CMM_ERR_ARG;
+    }
     const double *M8 = nullptr;
     double *C1 = nullptr, *C12 = nullptr, *C0 = nullptr;
     if (comb) {
@@ -2393,6 +2543,11 @@ int op_map_advect(const cmm_grid &gm, const cmm_grid &gp, const double *cx, cons
         next += n4;
         if (advect_needs_c12(c)) { C12 = next; next += n4; }
         if (advect_needs_c0(c)) C0 = next;
+        if (pre_in) {  // the fields the previous psi solve wrote for this step's weights
+            C1 = static_cast<double *>(pre_in->C1);
+            C12 = advect_needs_c12(c) ? static_cast<double *>(pre_in->C12) : nullptr;
+            C0 = advect_needs_c0(c) ? static_cast<double *>(pre_in->C0) : nullptr;
+        }
         if (m8_in) {
             M8 = m8_in;
         } else {
@@ -2402,12 +2557,14 @@ int op_map_advect(const cmm_grid &gm, const cmm_grid &gp, const double *cx, cons
             M8 = m8_work;
         }
         const dim3 g((unsigned)((n + 255) / 256));
-        switch (c.l_order) {
-        case 2: hipLaunchKernelGGL(k_psi_combine<2>, g, dim3(256), 0, st, n, P, c, C1, C12, C0, corr1, corr12, lw); break;
-        case 3: hipLaunchKernelGGL(k_psi_combine<3>, g, dim3(256), 0, st, n, P, c, C1, C12, C0, corr1, corr12, lw); break;
-        default: hipLaunchKernelGGL(k_psi_combine<4>, g, dim3(256), 0, st, n, P, c, C1, C12, C0, corr1, corr12, lw); break;
+        if (!pre_in) {
+            switch (c.l_order) {
+            case 2: hipLaunchKernelGGL(k_psi_combine<2>, g, dim3(256), 0, st, n, P, c, C1, C12, C0, corr1, corr12, lw); break;
+            case 3: hipLaunchKernelGGL(k_psi_combine<3>, g, dim3(256), 0, st, n, P, c, C1, C12, C0, corr1, corr12, lw); break;
+            default: hipLaunchKernelGGL(k_psi_combine<4>, g, dim3(256), 0, st, n, P, c, C1, C12, C0, corr1, corr12, lw); break;
+            }
+            if (int rc = launch_check("k_psi_combine")) return rc;
         }
-        if (int rc = launch_check("k_psi_combine")) return rc;
     }
     if (mid) (void)hipEventRecord(mid, st);
     const bool p2 = dm.lgx >= 0 && dm.lgy >= 0 && dp.lgx >= 0 && dp.lgy >= 0;
@@ -2629,21 +2786,21 @@ int op_psi_sample(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf, co
 }
 
 int op_psi_spectral(const cmm_grid &gv, const cmm_grid &gp, double fcut, double *psi, void *work, size_t work_bytes,
-                    hipStream_t st, hipEvent_t *ev) {
+                    hipStream_t st, hipEvent_t *ev, const PsiPre *pre) {
     SpecPlan sp = plan_spec(gv, gp, 1, fcut);
     const size_t base = sp.t_bytes + 2 * sp.g_bytes + 256;
     void *w;
     if (int rc = get_work(work, work_bytes, base + sizeof(double) * (size_t)gv.nx * gv.ny, &w)) return rc;
     double *f = reinterpret_cast<double *>(static_cast<char *>(w) + base);
-    return run_spec(sp, SampleReal{f, gv.nx}, psi, w, st, ev, false);
+    return run_spec(sp, SampleReal{f, gv.nx}, psi, w, st, ev, false, nullptr, pre);
 }
 
 int op_psi_from_vort(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf, const cmm_grid &gp, const double *cx,
                      const double *cy, const double *W, int molly, double fcut, double *psi, void *work,
-                     size_t work_bytes, hipStream_t st, hipEvent_t *ev, const double *W4) {
+                     size_t work_bytes, hipStream_t st, hipEvent_t *ev, const double *W4, const PsiPre *pre) {
     if (int rc = op_psi_sample(gm, gv, gf, gp, cx, cy, W, molly, fcut, work, work_bytes, st, ev, W4, nullptr))
         return rc;
-    return op_psi_spectral(gv, gp, fcut, psi, work, work_bytes, st, ev);
+    return op_psi_spectral(gv, gp, fcut, psi, work, work_bytes, st, ev, pre);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -3207,35 +3364,6 @@ __device__ __forceinline__ double incomp32_point(const DGrid &g, const float4 *_
     const double b = hsum_v(vx, bx, dby, g.hx, g.hy) * g.ihx;
     const double d = hsum_v(vy, bx, dby, g.hx, g.hy) * g.ihy;
     return fabs(a + d + (a * d - b * cc));
-}
-
-// FP32 Hermite records whose value carries a low part (PsiRec32Pk): the value's leading FP32
-// part sits in .x and an 8-bit signed code q of its remainder in the low byte of .w, whose upper
-// 24 bits hold the cross derivative rounded to 15 explicit mantissa bits.  The remainder
-// f - (float)f is at most half an FP32 ulp of the leading part; q counts it in units of 1/256 of
-// that ulp (clamped to 127), so the value is kept to 2^-31 of itself or better.  The value plane
-// of a stream function with |psi| of order one rounded to FP32 alone would move the velocity by
-// 2^-24 |psi| 6/h, which the check's 1/h amplifies again; the cross derivative enters the
-// velocity with weight h/7 or less, where its 2^-16 rounding stays at the level of the first
-// derivatives' FP32 rounding.
-__device__ __forceinline__ float4 pack_psi_rec(const double o[4]) {
-    const float hi = (float)o[0];
-    const unsigned e = (__float_as_uint(hi) >> 23) & 0xFFu;
-    // 256 / ulp(hi) = 2^(158 - e), exact as a double for every biased exponent e
-    const double inv = __longlong_as_double((long long)(1023 + 158 - (int)e) << 52);
-    int q = (int)rint((o[0] - (double)hi) * inv);
-    q = q < -128 ? -128 : q > 127 ? 127 : q;
-    unsigned w = __float_as_uint((float)o[3]);
-    w = ((w + 0x80u) & 0xFFFFFF00u) | ((unsigned)q & 0xFFu);
-    return float4{hi, (float)o[1], (float)o[2], __uint_as_float(w)};
-}
-
-__device__ __forceinline__ void unpack_psi_rec(const float4 a, double &f, double &fxy) {
-    const unsigned w = __float_as_uint(a.w);
-    const unsigned e = (__float_as_uint(a.x) >> 23) & 0xFFu;
-    const double s = __longlong_as_double((long long)(e + 865u) << 52);  // ulp(hi) / 256 = 2^(e - 158)
-    f = (double)a.x + (double)(int)(signed char)(w & 0xFFu) * s;
-    fxy = (double)__uint_as_float(w & 0xFFFFFF00u);
 }
 
 // the four corner records of a cell
@@ -3828,7 +3956,7 @@ size_t op_advect32_workspace(const cmm_grid &gp, const cmm_advect_coeffs &c, int
 
 int op_map_advect32(const cmm_grid &gm, const cmm_grid &gp, const PsiLevels &P, const cmm_advect_coeffs &c,
                     const float *M_in, float *M_out, void *work, size_t work_bytes, hipStream_t st, hipEvent_t mid,
-                    int psi_form, const double *corr1, const double *corr12, const double *lw) {
+                    int psi_form, const double *corr1, const double *corr12, const double *lw, const PsiPre *pre_in) {
     if (!op_advect32_supported(c)) {
         set_error("split-precision advection: the time scheme reads psi levels on their own (AB2, RK3Mod), "
                   "needs two or more psi levels, and is backward only");
@@ -3849,15 +3977,24 @@ int op_map_advect32(const cmm_grid &gm, const cmm_grid &gp, const PsiLevels &P, 
     next += rec;
     if (advect_needs_c12(c)) { C12 = next; next += rec; }
     if (advect_needs_c0(c)) C0 = next;
+    if (pre_in) {  // the fields the previous psi solve wrote for this step's weights
+        if (corr1 || corr12 || lw || pre_in->form != (psi_form == PsiRec64 ? 0 : 1) || psi_form == PsiRec32) {
+            set_error("split advection: precombined fields need the packed or FP64 form without corrections");
+            return CMM_ERR_ARG;
+        }
+        C1 = pre_in->C1;
+        C12 = advect_needs_c12(c) ? pre_in->C12 : nullptr;
+        C0 = advect_needs_c0(c) ? pre_in->C0 : nullptr;
+    }
     const dim3 g((unsigned)((n + 255) / 256));
-    if (psi_form == PsiRec64) {
+    if (!pre_in && psi_form == PsiRec64) {
         double *d1 = (double *)C1, *d12 = (double *)C12, *d0 = (double *)C0;
         switch (c.l_order) {
         case 2: hipLaunchKernelGGL(k_psi_combine<2>, g, dim3(256), 0, st, n, P, c, d1, d12, d0, corr1, corr12, lw); break;
         case 3: hipLaunchKernelGGL(k_psi_combine<3>, g, dim3(256), 0, st, n, P, c, d1, d12, d0, corr1, corr12, lw); break;
         default: hipLaunchKernelGGL(k_psi_combine<4>, g, dim3(256), 0, st, n, P, c, d1, d12, d0, corr1, corr12, lw); break;
         }
-    } else {
+    } else if (!pre_in) {
         float4 *f1 = (float4 *)C1, *f12 = (float4 *)C12, *f0 = (float4 *)C0;
 #define CMM_C32(LL, PK)                                                                                                \
     hipLaunchKernelGGL((k_psi_combine32<LL, PK>), g, dim3(256), 0, st, n, P, c, f1, f12, f0, corr1, corr12, lw)
@@ -3978,20 +4115,20 @@ int op_psi_sample32(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf, 
 }
 
 int op_psi_spectral32(const cmm_grid &gv, const cmm_grid &gp, double fcut, double *psi, void *work, size_t work_bytes,
-                      hipStream_t st, hipEvent_t *ev) {
+                      hipStream_t st, hipEvent_t *ev, const PsiPre *pre) {
     SpecPlan sp = plan_spec(gv, gp, 1, fcut);
     const size_t base = sp.t_bytes + 2 * sp.g_bytes + 256;
     void *w;
     if (int rc = get_work(work, work_bytes, base + sizeof(double) * (size_t)gv.nx * gv.ny, &w)) return rc;
     float *f = reinterpret_cast<float *>(static_cast<char *>(w) + base);
-    return run_spec(sp, SampleRealF{f, gv.nx}, psi, w, st, ev, false);
+    return run_spec(sp, SampleRealF{f, gv.nx}, psi, w, st, ev, false, nullptr, pre);
 }
 
 int op_psi_from_vort32(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf, const cmm_grid &gp, const float *M,
                        const float *W4, int molly, double fcut, double *psi, void *work, size_t work_bytes,
-                       hipStream_t st, hipEvent_t *ev) {
+                       hipStream_t st, hipEvent_t *ev, const PsiPre *pre) {
     if (int rc = op_psi_sample32(gm, gv, gf, gp, M, W4, molly, fcut, work, work_bytes, st, ev, nullptr)) return rc;
-    return op_psi_spectral32(gv, gp, fcut, psi, work, work_bytes, st, ev);
+    return op_psi_spectral32(gv, gp, fcut, psi, work, work_bytes, st, ev, pre);
 }
 
 int op_sample_vort32(const cmm_grid &gm, const cmm_grid &gt, const cmm_grid &gf, const float *M, const float *W4,

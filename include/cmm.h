@@ -418,6 +418,13 @@ int cmm_euler_autotune(cmm_euler *e, int *tile_out, int *max_blocks_out, double 
  * are the same; the map is read once for both. */
 int cmm_euler_set_check_fusion(cmm_euler *e, int on);
 int cmm_euler_get_check_fusion(cmm_euler *e, int *on);
+/* Whether a step's psi solve also writes the next step's Lagrange-combined psi fields (the
+ * combination inside advect_using_stream_hermite, Cuda-CMM/src/simulation/cmm-simulation-host.cu:64-104)
+ * from its last transform, for a next step of the same length: the next advection then reads them
+ * instead of combining the levels again, when its weights, levels and scheme are those.  The sums
+ * are the same.  On by default (Euler runs). */
+int cmm_euler_set_psi_precombine(cmm_euler *e, int on);
+int cmm_euler_get_psi_precombine(cmm_euler *e, int *on);
 
 
 /* ---- learned psi extrapolation (no counterpart in the reference) --------------------------------

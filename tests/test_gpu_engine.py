@@ -384,3 +384,33 @@ def test_check_in_the_sampler_pass(dev, precision):
     assert sum(r0) >= 2 and r0 == r1
     assert all(abs(a - b) <= 1e-12 for a, b in zip(e0, e1))
     assert np.array_equal(p0, p1)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "split"])
+def test_psi_precombined_by_the_transform(dev, precision):
+    """The next step's Lagrange-combined psi fields written by the psi solve's last transform
+    (cmm_euler_set_psi_precombine) against the combination kernel of the advection: the double
+    shear layer (map 128^2, fine = vort 512^2, psi 256^2, dt 1/64) with remaps (the remap's psi
+    solve writes them again), a shortened step (whose weights differ, so the advection combines
+    anew) and the steps after it.  The sums are the same, so the runs agree bit for bit: map, psi
+    levels and every E_inc."""
+    import cmm
+    dt = 1.0 / 64
+    plan = [dt] * 12 + [dt / 2] + [dt] * 12
+    runs = []
+    for pre in (False, True):
+        e = cmm.Euler(cmm.grid(128), cmm.grid(512), cmm.grid(256), cmm.grid(512), dt, freq_cut_psi=128.0,
+                      initial_condition="double_shear_layer", incomp_threshold=8e-6, stack_capacity=32,
+                      precision=precision)
+        e.set_psi_precombine(pre)
+        e.init()
+        infos = [e.step(d) for d in plan]
+        runs.append(([x.incomp for x in infos], [x.remapped for x in infos], e.field("psi"), e.field("chix"),
+                     e.field("chiy")))
+        e.close()
+    (e0, r0, *f0), (e1, r1, *f1) = runs
+    print(precision, "remaps", sum(r0))
+    assert sum(r0) >= 1 and r0 == r1
+    assert e0 == e1
+    for a, b in zip(f0, f1):
+        assert np.array_equal(a, b)
