@@ -84,7 +84,8 @@ SYMBOLS = ["cmm_version", "cmm_last_error", "cmm_device_info", "cmm_lagrange_coe
            "cmm_euler_forward_positions", "cmm_euler_forward_points", "cmm_euler_state", "cmm_euler_profile", "cmm_euler_profile_read",
            "cmm_map_to_split", "cmm_map_from_split", "cmm_hermite_to_split", "cmm_map_advect_split_workspace_bytes",
            "cmm_map_advect_split", "cmm_incomp_error_split", "cmm_psi_from_vort_split",
-           "cmm_batch_create", "cmm_batch_destroy", "cmm_batch_step", "cmm_batch_stats", "cmm_euler_autotune", "cmm_euler_set_check_fusion",
+           "cmm_batch_create", "cmm_batch_destroy", "cmm_batch_step", "cmm_batch_stats", "cmm_batch_set_grouping", "cmm_batch_grouped_steps",
+           "cmm_batch_autotune", "cmm_euler_autotune", "cmm_euler_set_check_fusion",
            "cmm_euler_get_check_fusion", "cmm_euler_set_psi_precombine",
            "cmm_euler_get_psi_precombine",
            "cmm_cnn_forward", "cmm_cnn_backward_data", "cmm_cnn_backward_weights", "cmm_cnn_relu_mask",
@@ -185,6 +186,9 @@ def lib():
         L.cmm_batch_destroy.argtypes = [VP]
         L.cmm_batch_step.argtypes = [VP, C.c_double, C.POINTER(StepInfo)]
         L.cmm_batch_stats.argtypes = [VP, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.cmm_batch_set_grouping.argtypes = [VP, C.c_int]
+        L.cmm_batch_grouped_steps.argtypes = [VP, C.POINTER(C.c_int)]
+        L.cmm_batch_autotune.argtypes = [VP, C.c_double, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
         L.cmm_euler_autotune.argtypes = [VP, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]
         L.cmm_euler_set_check_fusion.argtypes = [VP, C.c_int]
         L.cmm_euler_get_check_fusion.argtypes = [VP, C.POINTER(C.c_int)]
@@ -707,25 +711,38 @@ def uniform_particles(n, seed, center=(TWO_PI / 2, TWO_PI / 2), width=(TWO_PI, T
 
 
 class Batch:
-    """Engines stepped together (cmm_batch_*): one wait per step, and with graphs the queued work
-    of all engines replayed from one captured graph per step phase (a branch per engine)."""
+    """Engines stepped together (cmm_batch_*): one wait per step; with grouping (FP64 Vlasov-Poisson
+    engines in their steady state) every kind of launch of all engines as one persistent launch,
+    else with graphs the queued work of all engines replayed from one captured graph per step
+    phase (a branch per engine)."""
 
-    def __init__(self, engines, use_graphs=True):
+    def __init__(self, engines, use_graphs=True, grouping=False):
         self.engines = list(engines)
         arr = (VP * len(self.engines))(*[e.h.value for e in self.engines])
         h = C.c_void_p()
         _check(lib().cmm_batch_create(arr, len(self.engines), int(use_graphs), C.byref(h)), "cmm_batch_create")
         self.h = h
+        _check(lib().cmm_batch_set_grouping(self.h, int(grouping)), "cmm_batch_set_grouping")
 
     def step(self, dt):
         infos = (StepInfo * len(self.engines))()
         _check(lib().cmm_batch_step(self.h, dt, infos), "cmm_batch_step")
         return list(infos)
 
+    MODES = {0: "eager", 1: "graphs", 2: "grouped", 3: "grouped_persistent"}
+
+    def autotune(self, dt, steps_per_mode=4):
+        """cmm_batch_autotune: the fastest launch form over real steps (the run advances)"""
+        m, ms = C.c_int(), C.c_double()
+        _check(lib().cmm_batch_autotune(self.h, dt, int(steps_per_mode), C.byref(m), C.byref(ms)), "cmm_batch_autotune")
+        return {"mode": self.MODES.get(m.value, str(m.value)), "ms_per_step": ms.value}
+
     def stats(self):
         a, b, c = C.c_int(), C.c_int(), C.c_int()
         _check(lib().cmm_batch_stats(self.h, C.byref(a), C.byref(b), C.byref(c)), "cmm_batch_stats")
-        return {"graphs_captured": a.value, "graph_launches": b.value, "eager_steps": c.value}
+        g = C.c_int()
+        _check(lib().cmm_batch_grouped_steps(self.h, C.byref(g)), "cmm_batch_grouped_steps")
+        return {"graphs_captured": a.value, "graph_launches": b.value, "eager_steps": c.value, "grouped_steps": g.value}
 
     def close(self):
         if self.h:

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <complex>
 #include <map>
@@ -390,6 +391,9 @@ struct cmm_euler {
     PsiPre pre{};
     int pre_tin = 0;
     bool pre_valid = false, pre_enabled = true;
+    // grouped ensemble steps: while set, the step records the arguments of its launches here instead
+    // of launching them (the batch launches every engine's work together, op_group_vp_step)
+    GroupInstance *grp = nullptr;
     // corrections (device Hermite fields on g_psi) added to the Lagrange extrapolations of psi
     // to t_{n+1} and t_n + dt/2 in the next steps (cmm_euler_set_psi_correction)
     const double *corr1 = nullptr, *corr12 = nullptr;
@@ -498,6 +502,23 @@ int engine_translate(cmm_euler *e) {
 }
 
 int engine_psi(cmm_euler *e, double *out, hipEvent_t *ev = nullptr, const PsiPre *pre = nullptr) {
+    if (e->grp) {
+        if (e->p.sim != CMM_SIM_VLASOV_POISSON_1D || split(e)) {
+            set_error("grouped step: psi of an engine that is not an FP64 Vlasov-Poisson engine");
+            return CMM_ERR_UNSUPPORTED;
+        }
+        GroupInstance &g = *e->grp;
+        g.gv = e->p.g_vort;
+        g.gf = e->p.g_fine;
+        g.pcx = e->cx;
+        g.pcy = e->cy;
+        g.W4 = e->W4;
+        g.psi = out;
+        g.psi_work = e->work;
+        g.psi_work_bytes = e->work_bytes;
+        g.parts |= 4;
+        return CMM_OK;
+    }
     if (e->dry) return split(e) ? (e->p.sim == CMM_SIM_VLASOV_POISSON_1D ? sync_planes(e) : sync_m32(e)) : CMM_OK;
     if (e->p.sim == CMM_SIM_VLASOV_POISSON_1D) {
         // one operator; its whole time goes to the first psi profile slot
@@ -626,13 +647,32 @@ int engine_advect(cmm_euler *e, const double *t, const double *dt, int loop_ctr,
     // the split form runs the schemes that read psi levels on their own (and the one-level
     // warm-up steps) on the FP64 planes
     if (int rc = sync_planes(e)) return rc;
+    if (e->grp && !(e->p.sim == CMM_SIM_VLASOV_POISSON_1D && e->m8[0] && e->vpH &&
+                    op_vp_advect_supported(e->p.g_coarse, e->p.g_psi, c) && !(e->initialised && (e->corr1 || e->corr12)))) {
+        set_error("grouped step: an advection that is not the separable Vlasov-Poisson one");
+        return CMM_ERR_UNSUPPORTED;
+    }
     // the Vlasov-Poisson flow: the velocities from the separable psi levels (1D tables of the
     // combined phi and of the bump function instead of 2D gathers)
     if (e->p.sim == CMM_SIM_VLASOV_POISSON_1D && e->m8[0] && e->vpH &&
         op_vp_advect_supported(e->p.g_coarse, e->p.g_psi, c) && !(e->initialised && (e->corr1 || e->corr12))) {
         const double *m8_in = e->m8_valid ? e->m8[e->m8_cur] : nullptr;
         double *m8_out = e->m8[1 - e->m8_cur];
-        if (!e->dry)
+        if (e->grp) {
+            GroupInstance &g = *e->grp;
+            g.gm = e->p.g_coarse;
+            g.gp = e->p.g_psi;
+            g.P = P;
+            g.c = c;
+            g.m8_in = m8_in;
+            g.m8_out = m8_out;
+            g.ncx = e->ncx;
+            g.ncy = e->ncy;
+            g.H = e->vpH;
+            g.adv_work = e->work;
+            g.adv_work_bytes = e->work_bytes;
+            g.parts |= 1;
+        } else if (!e->dry)
             if (int rc = op_map_advect_vp(e->p.g_coarse, e->p.g_psi, P, c, e->cx, e->cy, m8_in, m8_out, e->vpH, e->ncx,
                                           e->ncy, e->work, e->work_bytes, e->st, e->prof ? e->ev[10] : nullptr, lw))
                 return rc;
@@ -682,9 +722,20 @@ int engine_advect_forward(cmm_euler *e, const double *t, const double *dt, int l
 // the check on whichever form of the map is current, on stream s (a conversion to the FP64
 // planes, when needed, runs on the engine stream before)
 int launch_incomp(cmm_euler *e, hipStream_t s) {
-    if (split(e) && e->m32_fresh)
+    if (split(e) && e->m32_fresh) {
+        if (e->grp) { set_error("grouped step: a split-form check"); return CMM_ERR_UNSUPPORTED; }
         return e->dry ? CMM_OK : op_incomp32_launch(e->p.g_coarse, e->p.g_fine, e->m32[e->m32_cur], e->slot, s, e->tune);
+    }
     if (int rc = sync_planes(e)) return rc;
+    if (e->grp) {
+        GroupInstance &g = *e->grp;
+        g.gc = e->p.g_fine;
+        g.ccx = e->cx;
+        g.ccy = e->cy;
+        g.tune = e->tune;
+        g.parts |= 2;
+        return CMM_OK;
+    }
     return e->dry ? CMM_OK : op_incomp_launch(e->p.g_coarse, e->p.g_fine, e->cx, e->cy, e->slot, s, e->tune);
 }
 
@@ -1257,12 +1308,12 @@ int step_queue_a(cmm_euler *e, double dt_now, bool overlap, bool *forked) {
             return rc;
         if (int rc = op_incomp_launch(p.g_coarse, p.g_coarse, e->fx, e->fy, e->slot + 1, sc)) return rc;
     }
-    if (!e->dry)
+    if (!e->dry && !e->grp)
         if (int rc = hip_check(hipMemcpyAsync(e->h_slot, e->slot, sizeof(unsigned long long) * (p.forward_map ? 3 : 1),
                                               hipMemcpyDeviceToHost, sc),
                                "hipMemcpyAsync"))
             return rc;
-    if (!e->dry && !e->capturing)
+    if (!e->dry && !e->capturing && !e->grp)
         if (int rc = hip_check(hipEventRecord(e->ev_inc, sc), "hipEventRecord")) return rc;
     if (e->prof && !fuse) (void)hipEventRecord(e->ev[2], sc);
     *forked = fork;
@@ -1280,7 +1331,7 @@ int step_queue_b(cmm_euler *e, bool fork) {
     double *oldest = e->lvl[L - 1];
     for (int l = L - 1; l > 0; --l) e->lvl[l] = e->lvl[l - 1];
     e->lvl[0] = oldest;
-    if (e->part_pending)  // the previous step's particles still read the level about to be rewritten
+    if (e->part_pending && !e->grp)  // the previous step's particles still read the level about to be rewritten
         if (int rc = hip_check(hipStreamWaitEvent(e->st, e->ev_part, 0), "hipStreamWaitEvent")) return rc;
     const PsiPre *pre = prepare_pre(e);
     if (e->sampled) {
@@ -1426,7 +1477,7 @@ int cmm_euler_step(cmm_euler *e, double dt_now, cmm_step_info *info) {
 
 struct cmm_batch {
     std::vector<cmm_euler *> e;
-    bool graphs = true;
+    bool graphs = true, graphs_allowed = true;
     // A graph step is one captured graph launched on `master`: it forks to every engine stream
     // (one branch per engine) and joins them back.  A branch holds the engine's whole step: it
     // waits for tail[i] (the engine stream's work before the step: the previous step, a remap),
@@ -1446,6 +1497,15 @@ struct cmm_batch {
     // every engine's Lagrange weights (8 doubles each) in one pinned and one device array, so
     // that a graph step copies them with one transfer; the engines point into them while batched
     double *lw_host = nullptr, *lw_dev = nullptr;
+    // grouped steps (op_group_vp_step): the recorded launches of every engine, the job tables
+    // (pinned staging, device copy), E_inc of every engine (device, pinned) and the step's end
+    bool grouping = false;
+    int grouped = 0;
+    int g_blocks = 0;  // grid cap of the grouped launches (0: one workgroup per block of work)
+    std::vector<GroupInstance> ginst;
+    void *g_host = nullptr, *g_dev = nullptr;
+    unsigned long long *g_hslots = nullptr, *g_dslots = nullptr;
+    hipEvent_t g_done = nullptr;
 };
 
 namespace {
@@ -1460,6 +1520,78 @@ std::string batch_key(const cmm_batch *b) {
         k += ';';
     }
     return k;
+}
+
+// Whether an engine's next step of length dt can run in a grouped step: an initialised FP64
+// Vlasov-Poisson engine in its steady state (interleaved map current, bump function cached, the
+// column sampler's node-interleaved field) whose scheme takes the separable advection.
+bool group_ready(const cmm_euler *e, double dt) {
+    const cmm_euler_params &p = e->p;
+    if (p.sim != CMM_SIM_VLASOV_POISSON_1D || split(e) || p.forward_map || !e->initialised || !e->m8[0] ||
+        !e->m8_valid || !e->vpH || !e->W4 || p.molly_stencil != 0 || e->corr1 || e->corr12 || e->prof_every)
+        return false;
+    const int L = p.l_order, lc = e->loop + L - 1;
+    if ((int)e->tv.size() < lc + 1) return false;
+    std::vector<double> t2(e->tv.begin(), e->tv.begin() + lc + 1), d2(e->dtv.begin(), e->dtv.begin() + lc + 1);
+    t2.push_back(t2[lc] + dt);
+    d2.push_back(dt);
+    cmm_advect_coeffs c;
+    if (cmm_advect_coeffs_fill(t2.data(), d2.data(), e->loop, L, p.time_integration_num, p.map_update_order_num,
+                               p.map_epsilon, -1, &c))
+        return false;
+    return op_vp_advect_supported(p.g_coarse, p.g_psi, c);
+}
+
+int batch_normalise(cmm_euler *e);
+
+// One grouped step of every engine: each engine's step is run with its launches recorded
+// (step_queue_a/b with grp set), then op_group_vp_step launches all of them together on the master
+// stream, ordered after every engine stream's earlier work; the engines' streams are ordered after
+// it, and each engine finishes its step (E_inc, remap, bookkeeping) as after its own.
+int batch_step_grouped(cmm_batch *b, double dt, cmm_step_info *infos) {
+    const int n = (int)b->e.size();
+    if (!b->g_dev) {
+        const size_t jb = op_group_vp_job_bytes(n);
+        if (hipMalloc(&b->g_dev, jb) != hipSuccess || hipHostMalloc(&b->g_host, jb, hipHostMallocDefault) != hipSuccess ||
+            hipMalloc((void **)&b->g_dslots, sizeof(unsigned long long) * n) != hipSuccess ||
+            hipHostMalloc((void **)&b->g_hslots, sizeof(unsigned long long) * n, hipHostMallocDefault) != hipSuccess) {
+            set_error("grouped step: allocation of the job tables");
+            return CMM_ERR_ALLOC;
+        }
+        if (int rc = hip_check(hipEventCreateWithFlags(&b->g_done, hipEventDisableTiming), "hipEventCreate")) return rc;
+    }
+    for (int i = 0; i < n; ++i) {
+        if (int rc = hip_check(hipEventRecord(b->tail[i], b->e[i]->st), "hipEventRecord (tail)")) return rc;
+        if (int rc = hip_check(hipStreamWaitEvent(b->master, b->tail[i], 0), "hipStreamWaitEvent (tail)")) return rc;
+    }
+    b->ginst.assign(n, GroupInstance{});
+    for (int i = 0; i < n; ++i) {
+        cmm_euler *e = b->e[i];
+        e->grp = &b->ginst[i];
+        bool fork = false;
+        int rc = step_queue_a(e, dt, false, &fork);
+        if (!rc) rc = step_queue_b(e, false);
+        e->grp = nullptr;
+        if (rc) return rc;
+    }
+    if (int rc = op_group_vp_step(b->ginst.data(), n, b->g_dev, b->g_host, b->g_dslots, b->g_hslots, b->master,
+                                  b->g_blocks))
+        return rc;
+    if (int rc = hip_check(hipEventRecord(b->g_done, b->master), "hipEventRecord")) return rc;
+    if (int rc = hip_check(hipEventSynchronize(b->g_done), "hipEventSynchronize")) return rc;
+    for (int i = 0; i < n; ++i) {
+        cmm_euler *e = b->e[i];
+        std::memcpy(e->h_slot, b->g_hslots + i, sizeof(unsigned long long));
+        if (int rc = hip_check(hipStreamWaitEvent(e->st, b->g_done, 0), "hipStreamWaitEvent")) return rc;
+        // the interleaved map is brought back after a remap whether or not the caller takes infos
+        cmm_step_info own{};
+        cmm_step_info &info = infos ? infos[i] : own;
+        if (int rc = step_finish(e, &info, true)) return rc;
+        if (info.remapped)
+            if (int rc = batch_normalise(e)) return rc;
+    }
+    b->grouped++;
+    return CMM_OK;
 }
 
 // after a remap, bring the forms of the map a replayed step reads up to date, so that the next
@@ -1577,7 +1709,7 @@ int cmm_batch_create(cmm_euler *const *engines, int n, int use_graphs, cmm_batch
     }
     cmm_batch *b = new cmm_batch();
     b->e.assign(engines, engines + n);
-    b->graphs = use_graphs != 0;
+    b->graphs = b->graphs_allowed = use_graphs != 0;
     int rc = hip_check(hipStreamCreateWithFlags(&b->master, hipStreamNonBlocking), "hipStreamCreate");
     if (!rc) rc = hip_check(hipEventCreateWithFlags(&b->fork, kDeviceEvent), "hipEventCreate");
     for (auto *v : {&b->join, &b->tail, &b->ev_bk}) {
@@ -1616,6 +1748,11 @@ int cmm_batch_destroy(cmm_batch *b) {
             e->lw_host = reinterpret_cast<double *>(e->h_slot + 8);
         }
     for (auto &kv : b->cache) (void)hipGraphExecDestroy(kv.second);
+    if (b->g_dev) (void)hipFree(b->g_dev);
+    if (b->g_dslots) (void)hipFree(b->g_dslots);
+    if (b->g_host) (void)hipHostFree(b->g_host);
+    if (b->g_hslots) (void)hipHostFree(b->g_hslots);
+    if (b->g_done) (void)hipEventDestroy(b->g_done);
     for (auto *v : {&b->join, &b->tail, &b->ev_ak, &b->ev_bk})
         for (hipEvent_t ev : *v)
             if (ev) (void)hipEventDestroy(ev);
@@ -1631,6 +1768,10 @@ int cmm_batch_step(cmm_batch *b, double dt, cmm_step_info *infos) {
     if (!b || !(dt > 0)) { set_error("cmm_batch_step: bad argument"); return CMM_ERR_ARG; }
     for (cmm_euler *e : b->e)
         if (e->prof_every || !e->part.empty()) { set_error("cmm_batch_step: profiling and particles are not batched"); return CMM_ERR_ARG; }
+    // grouped launches where every engine is ready for them
+    bool grouped = b->grouping && (int)b->e.size() <= kGroupMax;
+    for (cmm_euler *e : b->e) grouped = grouped && group_ready(e, dt);
+    if (grouped) return batch_step_grouped(b, dt, infos);
     // a graph replays the steady state: the nominal step of every engine, after its first step
     bool graph = b->graphs;
     for (cmm_euler *e : b->e) graph = graph && e->loop >= 1 && dt == e->p.dt;
@@ -1663,10 +1804,92 @@ int cmm_batch_step(cmm_batch *b, double dt, cmm_step_info *infos) {
         // E_inc of the engine is in pinned memory once its branch has passed ev_ak
         if (graph)
             if (int rc = hip_check(hipEventSynchronize(b->ev_ak[i]), "hipEventSynchronize")) return rc;
-        if (int rc = step_finish(e, infos ? &infos[i] : nullptr, graph)) return rc;
-        if (infos && infos[i].remapped)
+        cmm_step_info own{};
+        cmm_step_info &info = infos ? infos[i] : own;
+        if (int rc = step_finish(e, &info, graph)) return rc;
+        if (info.remapped)
             if (int rc = batch_normalise(e)) return rc;
     }
+    return CMM_OK;
+}
+
+int cmm_batch_set_grouping(cmm_batch *b, int on) {
+    if (!b) { set_error("cmm_batch_set_grouping: null batch"); return CMM_ERR_ARG; }
+    b->grouping = on != 0;
+    return CMM_OK;
+}
+
+namespace {
+
+// every engine's streams and the batch's stream drained
+int batch_sync(cmm_batch *b) {
+    if (int rc = hip_check(hipStreamSynchronize(b->master), "hipStreamSynchronize")) return rc;
+    for (cmm_euler *e : b->e) {
+        if (int rc = hip_check(hipStreamSynchronize(e->st), "hipStreamSynchronize")) return rc;
+        if (int rc = hip_check(hipStreamSynchronize(e->st2), "hipStreamSynchronize")) return rc;
+    }
+    return CMM_OK;
+}
+
+void batch_set_mode(cmm_batch *b, int mode) {
+    b->grouping = mode == CMM_BATCH_GROUPED || mode == CMM_BATCH_GROUPED_PERSISTENT;
+    b->g_blocks = mode == CMM_BATCH_GROUPED_PERSISTENT ? 2048 : 0;
+    b->graphs = mode == CMM_BATCH_GRAPHS;
+}
+
+}  // namespace
+
+// The batch's launch form chosen by timing: each candidate (grouped launches with one workgroup per
+// block of work, persistent grouped launches of 2048 workgroups, captured graphs, eager queues)
+// runs steps_per_mode real steps of length dt back to back (the run advances; captured graphs
+// first take the 12 steps that capture them, untimed), between full synchronisations; the fastest
+// stays.  All forms give the same results bit for bit.
+int cmm_batch_autotune(cmm_batch *b, double dt, int steps_per_mode, int *mode_out, double *ms_out) {
+    if (!b || !(dt > 0) || steps_per_mode < 1) { set_error("cmm_batch_autotune: bad argument"); return CMM_ERR_ARG; }
+    // the first step of fresh engines interleaves their maps (grouped steps start after it)
+    bool ready = (int)b->e.size() <= kGroupMax;
+    for (cmm_euler *e : b->e) ready = ready && group_ready(e, dt);
+    if (!ready) {
+        batch_set_mode(b, CMM_BATCH_EAGER);
+        if (int rc = cmm_batch_step(b, dt, nullptr)) return rc;
+        ready = (int)b->e.size() <= kGroupMax;
+        for (cmm_euler *e : b->e) ready = ready && group_ready(e, dt);
+    }
+    std::vector<int> cand;
+    if (ready) {
+        cand.push_back(CMM_BATCH_GROUPED);
+        cand.push_back(CMM_BATCH_GROUPED_PERSISTENT);
+    }
+    if (b->graphs_allowed) cand.push_back(CMM_BATCH_GRAPHS);
+    cand.push_back(CMM_BATCH_EAGER);
+    int best = CMM_BATCH_EAGER;
+    double best_ms = 1e300;
+    for (int mode : cand) {
+        batch_set_mode(b, mode);
+        if (mode == CMM_BATCH_GRAPHS)
+            for (int i = 0; i < 12; ++i)
+                if (int rc = cmm_batch_step(b, dt, nullptr)) return rc;
+        if (int rc = batch_sync(b)) return rc;
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int i = 0; i < steps_per_mode; ++i)
+            if (int rc = cmm_batch_step(b, dt, nullptr)) return rc;
+        if (int rc = batch_sync(b)) return rc;
+        const double ms =
+            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / steps_per_mode;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = mode;
+        }
+    }
+    batch_set_mode(b, best);
+    if (mode_out) *mode_out = best;
+    if (ms_out) *ms_out = best_ms;
+    return CMM_OK;
+}
+
+int cmm_batch_grouped_steps(cmm_batch *b, int *grouped_steps) {
+    if (!b || !grouped_steps) { set_error("cmm_batch_grouped_steps: bad argument"); return CMM_ERR_ARG; }
+    *grouped_steps = b->grouped;
     return CMM_OK;
 }
 

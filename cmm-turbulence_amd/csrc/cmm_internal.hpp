@@ -119,6 +119,44 @@ size_t op_vp_psi_workspace(const cmm_grid &gv, const cmm_grid &gp);
 int op_vp_psi_from_f(const cmm_grid &gm, const cmm_grid &gv, const cmm_grid &gf, const cmm_grid &gp, const double *cx,
                      const double *cy, const double *W, int molly, double *psi, void *work, size_t work_bytes,
                      hipStream_t st, const double *W4 = nullptr, const double *H = nullptr);
+// ---- Ensembles: one Vlasov-Poisson step of many instances in grouped, persistent launches ----
+// The arguments one instance's step gives its launches (those of op_map_advect_vp,
+// op_incomp_launch and op_vp_psi_from_f), recorded by the engine instead of launched.
+constexpr int kGroupMax = 64;
+struct GroupInstance {
+    // advection
+    cmm_grid gm, gp;
+    PsiLevels P;
+    cmm_advect_coeffs c;
+    const double *m8_in;
+    double *m8_out, *ncx, *ncy;
+    const double *H;
+    void *adv_work;
+    size_t adv_work_bytes;
+    // check of the advected map on the fine grid
+    cmm_grid gc;
+    const double *ccx, *ccy;
+    IncompTune tune;
+    // psi of the advected map
+    cmm_grid gv, gf;
+    const double *pcx, *pcy, *W4;
+    double *psi;
+    void *psi_work;
+    size_t psi_work_bytes;
+    int parts;  // recorded parts: 1 advection, 2 check, 4 psi
+};
+// whether n recorded instances can run in one grouped step (the same time scheme, Lagrange order,
+// check tile and sampler rows; the separable advection, the tensor check and the column sampler)
+bool op_group_vp_eligible(const GroupInstance *inst, int n);
+size_t op_group_vp_job_bytes(int n);
+// The step of the n instances: every kind of launch of every instance in one launch, in the order
+// of a single instance's step (1D combination, advection, check, sampling, column sums, phi, psi
+// planes); max_blocks > 0 caps the grid (persistent workgroups), else one workgroup per block of
+// work.  The job tables go up in one copy from host_jobs (pinned) to dev_jobs; E_inc of instance
+// i lands in dev_slots[i] and is copied to host_slots[i] (pinned) after the check.
+int op_group_vp_step(const GroupInstance *inst, int n, void *dev_jobs, void *host_jobs,
+                     unsigned long long *dev_slots, unsigned long long *host_slots, hipStream_t st,
+                     int max_blocks);
 // H(v) at [0, np) and H'(v) at [2 np, 3 np) of H (3 np doubles): the grid-only part of psi
 int op_vp_bump(const cmm_grid &gv, const cmm_grid &gp, double *H, hipStream_t st);
 int op_vp_psi_planes(const cmm_grid &gp, const double *phi, const double *H, double *psi, hipStream_t st);

@@ -48,11 +48,14 @@ __device__ __forceinline__ int xcd_block(int bid, int nblocks) {
 // ranges (whole multiples of 256) and the ranges are dealt to the XCDs in contiguous runs
 // (xcd_block): an XCD then reads one band of the map rows, so the rows shared by neighbouring
 // tiles are fetched into one L2 rather than into several.
-__device__ __forceinline__ void tile_range(long ntiles, long &t_begin, long &t_end) {
-    const long per = ((ntiles + gridDim.x - 1) / gridDim.x + 255) / 256 * 256;
-    const long lb = xcd_block(blockIdx.x, gridDim.x);
+__device__ __forceinline__ void tile_range_of(long ntiles, int lb, int nblocks, long &t_begin, long &t_end) {
+    const long per = ((ntiles + nblocks - 1) / nblocks + 255) / 256 * 256;
     t_begin = lb * per;
     t_end = min(ntiles, t_begin + per);
+}
+
+__device__ __forceinline__ void tile_range(long ntiles, long &t_begin, long &t_end) {
+    tile_range_of(ntiles, xcd_block(blockIdx.x, gridDim.x), gridDim.x, t_begin, t_end);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -551,10 +554,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
 // C12 (w = L12) and C0 (level 0), each np doubles at T + {0, 1, 2, 3, 4, 5} np, and the weight
 // sums S[0] = sum L1, S[1] = sum L12 (LW: the weights from device memory, as k_psi_combine)
 template <int L>
-__global__ void __launch_bounds__(256) k_vp_combine1d(int np, long N, PsiLevels P, cmm_advect_coeffs c,
-                                                      const double *__restrict__ LW, double *__restrict__ T,
-                                                      double *__restrict__ S) {
-    const int x = blockIdx.x * 256 + threadIdx.x;
+__device__ __forceinline__ void vp_combine1d_at(int x, int np, long N, const PsiLevels &P, const cmm_advect_coeffs &c,
+                                                const double *__restrict__ LW, double *__restrict__ T,
+                                                double *__restrict__ S) {
     double w1[L], w12[L];
 #pragma unroll
     for (int l = 0; l < L; ++l) {
@@ -592,6 +594,13 @@ __global__ void __launch_bounds__(256) k_vp_combine1d(int np, long N, PsiLevels 
     T[3 * np + x] = d12;
     T[4 * np + x] = r[0];
     T[5 * np + x] = d[0];
+}
+
+template <int L>
+__global__ void __launch_bounds__(256) k_vp_combine1d(int np, long N, PsiLevels P, cmm_advect_coeffs c,
+                                                      const double *__restrict__ LW, double *__restrict__ T,
+                                                      double *__restrict__ S) {
+    vp_combine1d_at<L>(blockIdx.x * 256 + threadIdx.x, np, N, P, c, LW, T, S);
 }
 
 // one coordinate's cell, as find_cell takes it: lower node, upper node (wrapped) and offset
@@ -645,12 +654,11 @@ struct VelSep {
 // the 8-footpoint advection of a Vlasov-Poisson map with the separable velocity (two footpoints
 // per lane and the map records as k_map_advect_pair)
 template <int TIN>
-__global__ void __launch_bounds__(256) k_map_advect_vp(DGrid gm, DGrid gp, const double *T, const double *S,
-                                                       const double *H, const double *M8, cmm_advect_coeffs c,
-                                                       double *__restrict__ ncx, double *__restrict__ ncy,
-                                                       double *__restrict__ M8out, int nblocks) {
+__device__ __forceinline__ void advect_vp_block(int blk, const DGrid &gm, const DGrid &gp, const double *T,
+                                                const double *S, const double *H, const double *M8,
+                                                const cmm_advect_coeffs &c, double *__restrict__ ncx,
+                                                double *__restrict__ ncy, double *__restrict__ M8out) {
     constexpr int G = 4, NPB = 256 / G;
-    const int blk = xcd_block(blockIdx.x, nblocks);
     const int node = blk * NPB + threadIdx.x / G;
     const int k = threadIdx.x % G;
     const bool active = node < gm.n;
@@ -698,6 +706,14 @@ __global__ void __launch_bounds__(256) k_map_advect_vp(DGrid gm, DGrid gp, const
         if (active)
             reinterpret_cast<double2 *>(M8out + 8L * node)[k] = double2{k < 2 ? e0 : e1, k < 2 ? f0 : f1};
     }
+}
+
+template <int TIN>
+__global__ void __launch_bounds__(256) k_map_advect_vp(DGrid gm, DGrid gp, const double *T, const double *S,
+                                                       const double *H, const double *M8, cmm_advect_coeffs c,
+                                                       double *__restrict__ ncx, double *__restrict__ ncy,
+                                                       double *__restrict__ M8out, int nblocks) {
+    advect_vp_block<TIN>(xcd_block(blockIdx.x, nblocks), gm, gp, T, S, H, M8, c, ncx, ncy, M8out);
 }
 
 // The forward map (Cuda-CMM/src/simulation/cmm-simulation-kernel.cu:254-274): the old forward map
@@ -797,13 +813,10 @@ __global__ void __launch_bounds__(256) k_invert(DGrid gb, DGrid gf, DGrid gc, co
 // wrap jumps) grouped by the x basis first, so the values agree with diffeo_det to rounding.  A
 // tile whose first and last points fall in different cells is evaluated point by point.
 template <int TX, int TY>
-__global__ void __launch_bounds__(256) k_incomp_tensor(DGrid gm, DGrid gc, const double *__restrict__ cx,
-                                                       const double *__restrict__ cy,
-                                                       unsigned long long *__restrict__ out, int r, int ntx,
-                                                       long ntiles) {
+__device__ __forceinline__ double incomp_tensor_tiles(const DGrid &gm, const DGrid &gc, const double *__restrict__ cx,
+                                                      const double *__restrict__ cy, int ntx, long t_begin,
+                                                      long t_end) {
     double v = 0.0;
-    long t_begin, t_end;
-    tile_range(ntiles, t_begin, t_end);
     for (long t0 = t_begin; t0 < t_end; t0 += 256) {
         const long t = t0 + threadIdx.x;
         if (t >= t_end) continue;
@@ -878,14 +891,31 @@ __global__ void __launch_bounds__(256) k_incomp_tensor(DGrid gm, DGrid gc, const
             }
         }
     }
+    return v;
+}
+
+// the block's maximum of v into *out (one 64-bit atomic max on the bit pattern; for non-negative
+// doubles it preserves order); red: 4 doubles of LDS
+__device__ __forceinline__ void block_max_to(double v, double *red, unsigned long long *out) {
     for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-    __shared__ double red[4];
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
     __syncthreads();
     if (threadIdx.x == 0) {
         double m = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
         atomicMax(out, (unsigned long long)__double_as_longlong(m));
     }
+}
+
+template <int TX, int TY>
+__global__ void __launch_bounds__(256) k_incomp_tensor(DGrid gm, DGrid gc, const double *__restrict__ cx,
+                                                       const double *__restrict__ cy,
+                                                       unsigned long long *__restrict__ out, int r, int ntx,
+                                                       long ntiles) {
+    long t_begin, t_end;
+    tile_range(ntiles, t_begin, t_end);
+    const double v = incomp_tensor_tiles<TX, TY>(gm, gc, cx, cy, ntx, t_begin, t_end);
+    __shared__ double red[4];
+    block_max_to(v, red, out);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -4137,6 +4167,324 @@ int op_sample_vort32(const cmm_grid &gm, const cmm_grid &gt, const cmm_grid &gf,
     dim3 b(64, 4);
     hipLaunchKernelGGL(k_sample_grid<SampleVort32>, grid2d(gt.nx, gt.ny, b), b, 0, st, s, gt.nx, gt.ny, out);
     return launch_check("k_sample_grid");
+}
+
+// ------------------------------------------------------------------------------------------------
+// Ensembles: grouped launches.  One launch per kind of work covers every instance of a
+// Vlasov-Poisson ensemble: its workgroups walk the concatenated blocks of all instances (the blocks
+// of instance j are items start[j] .. start[j+1] - 1), each workgroup a contiguous range of items,
+// the ranges dealt to the XCDs in contiguous runs (xcd_block).  With a capped grid the launch is
+// persistent (each workgroup takes several items); with one workgroup per item it is a plain
+// grouped launch.  Each item runs the block body of the single-instance kernel with that
+// instance's arguments, so every instance computes what its own launches compute, bit for bit,
+// whatever the grid.
+
+struct GroupMap {
+    int n;
+    int start[kGroupMax + 1];
+};
+
+
+// the instance of an item; the same for the whole workgroup, which readfirstlane states, so that
+// the instance's arguments are read with scalar loads
+__device__ __forceinline__ int group_find(const GroupMap &g, int item) {
+    int j = 0;
+    while (j + 1 < g.n && g.start[j + 1] <= item) ++j;
+    return __builtin_amdgcn_readfirstlane(j);
+}
+
+// the items [lo, hi) of this workgroup
+__device__ __forceinline__ void group_items(int total, int &lo, int &hi) {
+    const int per = (total + (int)gridDim.x - 1) / (int)gridDim.x;
+    lo = xcd_block(blockIdx.x, gridDim.x) * per;
+    hi = min(total, lo + per);
+}
+
+struct GAdvVP {
+    DGrid gm, gp;
+    const double *T, *S, *H, *M8;
+    cmm_advect_coeffs c;
+    double *ncx, *ncy, *M8out;
+};
+struct GComb {
+    int np;
+    long N;
+    PsiLevels P;
+    cmm_advect_coeffs c;
+    double *T, *S;
+};
+struct GChk {
+    DGrid gm, gc;
+    const double *cx, *cy;
+    unsigned long long *slot;
+    int ntx, nblocks;
+    long ntiles;
+};
+struct GSample {
+    SampleVortIL smp;
+    int nx, ny;
+    double *f;
+};
+struct GColsum {
+    const double *f;
+    int nx, ny, nbx;
+    double *part;
+};
+struct GPlanes {
+    const double *phi, *H;
+    int np, lognp;
+    double *out;
+};
+
+template <int L>
+__global__ void __launch_bounds__(256) k_grp_vp_combine1d(const GComb *__restrict__ jobs, GroupMap g) {
+    int lo, hi;
+    group_items(g.start[g.n], lo, hi);
+    for (int it = lo; it < hi; ++it) {
+        const int j = group_find(g, it);
+        const GComb &a = jobs[j];
+        vp_combine1d_at<L>((it - g.start[j]) * 256 + threadIdx.x, a.np, a.N, a.P, a.c, nullptr, a.T, a.S);
+    }
+}
+
+template <int TIN>
+__global__ void __launch_bounds__(256) k_grp_advect_vp(const GAdvVP *__restrict__ jobs, GroupMap g) {
+    int lo, hi;
+    group_items(g.start[g.n], lo, hi);
+    for (int it = lo; it < hi; ++it) {
+        const int j = group_find(g, it);
+        const GAdvVP &a = jobs[j];
+        advect_vp_block<TIN>(it - g.start[j], a.gm, a.gp, a.T, a.S, a.H, a.M8, a.c, a.ncx, a.ncy, a.M8out);
+    }
+}
+
+__global__ void k_grp_zero(unsigned long long *slots, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) slots[i] = 0ull;
+}
+
+template <int TX, int TY>
+__global__ void __launch_bounds__(256) k_grp_incomp(const GChk *__restrict__ jobs, GroupMap g) {
+    __shared__ double red[4];
+    int lo, hi;
+    group_items(g.start[g.n], lo, hi);
+    for (int it = lo; it < hi; ++it) {
+        const int j = group_find(g, it);
+        const GChk &a = jobs[j];
+        long t_begin, t_end;
+        tile_range_of(a.ntiles, it - g.start[j], a.nblocks, t_begin, t_end);
+        const double v = incomp_tensor_tiles<TX, TY>(a.gm, a.gc, a.cx, a.cy, a.ntx, t_begin, t_end);
+        block_max_to(v, red, a.slot);
+        __syncthreads();  // red is written again by the next item
+    }
+}
+
+template <int R>
+__global__ void __launch_bounds__(256, 2) k_grp_sample(const GSample *__restrict__ jobs, GroupMap g) {
+    int lo, hi;
+    group_items(g.start[g.n], lo, hi);
+    for (int it = lo; it < hi; ++it) {
+        const int j = group_find(g, it);
+        const SampleVortIL smp = jobs[j].smp;
+        const int nx = jobs[j].nx, ny = jobs[j].ny;
+        double *f = jobs[j].f;
+        const long t = (long)(it - g.start[j]) * 256 + threadIdx.x;
+        if (t < (long)nx * (ny / R)) {
+            const int x = (int)(t % nx), y0 = R * (int)(t / nx);
+            double v[R];
+            smp.template column<R>(x, y0, v);
+#pragma unroll
+            for (int r = 0; r < R; ++r) f[(long)(y0 + r) * nx + x] = v[r];
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_grp_colsum(const GColsum *__restrict__ jobs, GroupMap g) {
+    int lo, hi;
+    group_items(g.start[g.n], lo, hi);
+    for (int it = lo; it < hi; ++it) {
+        const int j = group_find(g, it);
+        const GColsum &a = jobs[j];
+        const int lb = it - g.start[j];
+        const int x = (lb % a.nbx) * 256 + threadIdx.x, c = lb / a.nbx;
+        if (x >= a.nx) continue;
+        const int y0 = c * kVpRows, y1 = min(a.ny, y0 + kVpRows);
+        double s = 0.0;
+        for (int y = y0; y < y1; ++y) s += a.f[(long)y * a.nx + x];
+        a.part[(long)c * a.nx + x] = s;
+    }
+}
+
+// one workgroup per instance
+__global__ void __launch_bounds__(kFftThreads) k_grp_vp_phi(const VPJob *__restrict__ jobs) {
+    extern __shared__ cd s[];
+    vp_phi_block(jobs[blockIdx.x], s);
+}
+
+__global__ void __launch_bounds__(256) k_grp_psi_planes(const GPlanes *__restrict__ jobs, GroupMap g) {
+    int lo, hi;
+    group_items(g.start[g.n], lo, hi);
+    for (int it = lo; it < hi; ++it) {
+        const int j = group_find(g, it);
+        const GPlanes &a = jobs[j];
+        const long N = (long)a.np * a.np;
+        const long t = (long)(it - g.start[j]) * 256 + threadIdx.x;
+        if (t >= N) continue;
+        const int x = (int)(t & (a.np - 1)), y = (int)(t >> a.lognp);
+        a.out[t] = a.phi[x] - a.H[y];
+        a.out[N + t] = a.phi[2 * a.np + x];
+        a.out[2 * N + t] = -a.H[2 * a.np + y];
+        a.out[3 * N + t] = 0.0;
+    }
+}
+
+static size_t group_align(size_t b) { return (b + 255) / 256 * 256; }
+
+size_t op_group_vp_job_bytes(int n) {
+    return group_align(sizeof(GAdvVP) * n) + group_align(sizeof(GComb) * n) + group_align(sizeof(GChk) * n) +
+           group_align(sizeof(GSample) * n) + group_align(sizeof(GColsum) * n) + group_align(sizeof(VPJob) * n) +
+           group_align(sizeof(GPlanes) * n);
+}
+
+// the sampler's rows per lane and the check's tile edge of one instance (0 where its grids do not
+// take the column sampler or the tensor check)
+static int group_rows(const GroupInstance &a) {
+    const int r = a.gv.ny / a.gm.ny;
+    if (!a.W4 || a.gv.ny != r * a.gm.ny || r < 2 || a.gv.x0 != a.gm.x0 || a.gv.y0 != a.gm.y0) return 0;
+    return r >= kSampleRows ? kSampleRows : 2;
+}
+
+static int group_tile(const GroupInstance &a) {
+    const int r = a.gc.nx / a.gm.nx;
+    if (!(r >= 1 && a.gc.nx == r * a.gm.nx && a.gc.ny == r * a.gm.ny && a.gc.x0 == a.gm.x0 && a.gc.x1 == a.gm.x1 &&
+          a.gc.y0 == a.gm.y0 && a.gc.y1 == a.gm.y1))
+        return 0;
+    return incomp_tile(r, a.tune);
+}
+
+bool op_group_vp_eligible(const GroupInstance *inst, int n) {
+    if (n < 1 || n > kGroupMax) return false;
+    for (int i = 0; i < n; ++i) {
+        const GroupInstance &a = inst[i];
+        if (a.parts != 7 || !a.m8_in || !a.H || !op_vp_advect_supported(a.gm, a.gp, a.c) || vp_check(a.gv, a.gp) ||
+            group_rows(a) == 0 || group_tile(a) == 0)
+            return false;
+        if (a.c.time_integration_num != inst[0].c.time_integration_num || a.c.l_order != inst[0].c.l_order ||
+            group_rows(a) != group_rows(inst[0]) || group_tile(a) != group_tile(inst[0]))
+            return false;
+        // the workspaces of op_map_advect_vp (with its interleaved input map) and op_vp_psi_from_f
+        if (a.adv_work_bytes < sizeof(double) * (6 * (size_t)a.gp.nx + 2) ||
+            a.psi_work_bytes < op_vp_psi_workspace(a.gv, a.gp))
+            return false;
+    }
+    set_error("");
+    return true;
+}
+
+int op_group_vp_step(const GroupInstance *inst, int n, void *dev_jobs, void *host_jobs,
+                     unsigned long long *dev_slots, unsigned long long *host_slots, hipStream_t st, int max_blocks) {
+    if (!op_group_vp_eligible(inst, n)) {
+        set_error("grouped ensemble step: the instances differ in scheme, order or grid ratios");
+        return CMM_ERR_ARG;
+    }
+    char *h = static_cast<char *>(host_jobs);
+    GAdvVP *adv = reinterpret_cast<GAdvVP *>(h);
+    GComb *comb = reinterpret_cast<GComb *>(h + group_align(sizeof(GAdvVP) * n));
+    GChk *chk = reinterpret_cast<GChk *>(reinterpret_cast<char *>(comb) + group_align(sizeof(GComb) * n));
+    GSample *smp = reinterpret_cast<GSample *>(reinterpret_cast<char *>(chk) + group_align(sizeof(GChk) * n));
+    GColsum *col = reinterpret_cast<GColsum *>(reinterpret_cast<char *>(smp) + group_align(sizeof(GSample) * n));
+    VPJob *phi = reinterpret_cast<VPJob *>(reinterpret_cast<char *>(col) + group_align(sizeof(GColsum) * n));
+    GPlanes *pl = reinterpret_cast<GPlanes *>(reinterpret_cast<char *>(phi) + group_align(sizeof(VPJob) * n));
+    GroupMap g_comb{n, {}}, g_adv{n, {}}, g_chk{n, {}}, g_smp{n, {}}, g_col{n, {}}, g_pl{n, {}};
+    const int R = group_rows(inst[0]), TT = group_tile(inst[0]);
+    size_t lds_phi = 0;
+    for (int i = 0; i < n; ++i) {
+        const GroupInstance &a = inst[i];
+        const DGrid dm = make_dgrid(a.gm), dp = make_dgrid(a.gp), dc = make_dgrid(a.gc);
+        // op_map_advect_vp's tables in its workspace
+        double *T = static_cast<double *>(a.adv_work), *S = T + 6 * a.gp.nx;
+        comb[i] = GComb{a.gp.nx, (long)dp.n, a.P, a.c, T, S};
+        g_comb.start[i + 1] = g_comb.start[i] + (a.gp.nx + 255) / 256;
+        adv[i] = GAdvVP{dm, dp, T, S, a.H, a.m8_in, a.c, a.ncx, a.ncy, a.m8_out};
+        g_adv.start[i + 1] = g_adv.start[i] + (int)(((long)dm.n * 4 + 255) / 256);
+        // op_incomp_launch's tile split
+        const int ntx = a.gc.nx / TT;
+        const long ntiles = (long)ntx * (a.gc.ny / TT);
+        const int nb = (int)std::max<long>(1, std::min<long>((ntiles + 255) / 256, a.tune.max_blocks));
+        chk[i] = GChk{dm, dc, a.ccx, a.ccy, dev_slots + i, ntx, nb, ntiles};
+        g_chk.start[i + 1] = g_chk.start[i] + nb;
+        // op_vp_psi_from_f's workspace: f, the chunk sums, phi
+        const VP1D q = vp_params(a.gv, a.gp);
+        double *f = static_cast<double *>(a.psi_work);
+        double *part = f + (size_t)a.gv.nx * a.gv.ny;
+        double *phiH = part + (size_t)q.nchunks * a.gv.nx;
+        SampleVortIL sv;
+        sv.gm = dm;
+        sv.gv = make_dgrid(a.gv);
+        sv.gf = make_dgrid(a.gf);
+        sv.cx = a.pcx;
+        sv.cy = a.pcy;
+        sv.W = a.W4;
+        sv.molly = 0;
+        smp[i] = GSample{sv, a.gv.nx, a.gv.ny, f};
+        g_smp.start[i + 1] = g_smp.start[i] + (int)(((long)a.gv.nx * (a.gv.ny / R) + 255) / 256);
+        const int nbx = (a.gv.nx + 255) / 256;
+        col[i] = GColsum{f, a.gv.nx, a.gv.ny, nbx, part};
+        g_col.start[i + 1] = g_col.start[i] + nbx * q.nchunks;
+        const cd *tw_v, *tw_p;
+        if (int rc = twiddles(a.gv.nx, &tw_v)) return rc;
+        if (int rc = twiddles(a.gp.nx, &tw_p)) return rc;
+        phi[i] = VPJob{q, f, part, tw_v, tw_p, phiH};
+        lds_phi = std::max(lds_phi, sizeof(cd) * std::max(a.gv.nx, a.gp.nx));
+        pl[i] = GPlanes{phiH, a.H, a.gp.nx, ilog2(a.gp.nx), a.psi};
+        g_pl.start[i + 1] = g_pl.start[i] + (int)(((long)a.gp.nx * a.gp.ny + 255) / 256);
+    }
+    const size_t bytes = op_group_vp_job_bytes(n);
+    if (int rc = hip_check(hipMemcpyAsync(dev_jobs, host_jobs, bytes, hipMemcpyHostToDevice, st), "hipMemcpyAsync (jobs)"))
+        return rc;
+    char *d = static_cast<char *>(dev_jobs);
+    auto dev = [&](const void *hp) { return d + (static_cast<const char *>(hp) - h); };
+    // max_blocks > 0: persistent launches of at most that many workgroups; else one per item
+    auto grid = [&](const GroupMap &g) {
+        return dim3((unsigned)std::max(1, max_blocks > 0 ? std::min(g.start[g.n], max_blocks) : g.start[g.n]));
+    };
+    switch (inst[0].c.l_order) {
+    case 2: hipLaunchKernelGGL(k_grp_vp_combine1d<2>, grid(g_comb), dim3(256), 0, st, (const GComb *)dev(comb), g_comb); break;
+    case 3: hipLaunchKernelGGL(k_grp_vp_combine1d<3>, grid(g_comb), dim3(256), 0, st, (const GComb *)dev(comb), g_comb); break;
+    default: hipLaunchKernelGGL(k_grp_vp_combine1d<4>, grid(g_comb), dim3(256), 0, st, (const GComb *)dev(comb), g_comb); break;
+    }
+    if (int rc = launch_check("k_grp_vp_combine1d")) return rc;
+    const GAdvVP *da = (const GAdvVP *)dev(adv);
+    switch (inst[0].c.time_integration_num) {
+    case 10: hipLaunchKernelGGL(k_grp_advect_vp<10>, grid(g_adv), dim3(256), 0, st, da, g_adv); break;
+    case 21: hipLaunchKernelGGL(k_grp_advect_vp<21>, grid(g_adv), dim3(256), 0, st, da, g_adv); break;
+    case 30: hipLaunchKernelGGL(k_grp_advect_vp<30>, grid(g_adv), dim3(256), 0, st, da, g_adv); break;
+    case 40: hipLaunchKernelGGL(k_grp_advect_vp<40>, grid(g_adv), dim3(256), 0, st, da, g_adv); break;
+    case 41: hipLaunchKernelGGL(k_grp_advect_vp<41>, grid(g_adv), dim3(256), 0, st, da, g_adv); break;
+    default: set_error("unknown time integration"); return CMM_ERR_ARG;
+    }
+    if (int rc = launch_check("k_grp_advect_vp")) return rc;
+    hipLaunchKernelGGL(k_grp_zero, dim3((n + 63) / 64), dim3(64), 0, st, dev_slots, n);
+    const GChk *dc = (const GChk *)dev(chk);
+    if (TT == 4) hipLaunchKernelGGL((k_grp_incomp<4, 4>), grid(g_chk), dim3(256), 0, st, dc, g_chk);
+    else if (TT == 3) hipLaunchKernelGGL((k_grp_incomp<3, 3>), grid(g_chk), dim3(256), 0, st, dc, g_chk);
+    else if (TT == 2) hipLaunchKernelGGL((k_grp_incomp<2, 2>), grid(g_chk), dim3(256), 0, st, dc, g_chk);
+    else hipLaunchKernelGGL((k_grp_incomp<1, 1>), grid(g_chk), dim3(256), 0, st, dc, g_chk);
+    if (int rc = launch_check("k_grp_incomp")) return rc;
+    if (int rc = hip_check(hipMemcpyAsync(host_slots, dev_slots, sizeof(unsigned long long) * n, hipMemcpyDeviceToHost, st),
+                           "hipMemcpyAsync (E_inc)"))
+        return rc;
+    const GSample *ds = (const GSample *)dev(smp);
+    if (R == kSampleRows) hipLaunchKernelGGL((k_grp_sample<kSampleRows>), grid(g_smp), dim3(256), 0, st, ds, g_smp);
+    else hipLaunchKernelGGL((k_grp_sample<2>), grid(g_smp), dim3(256), 0, st, ds, g_smp);
+    if (int rc = launch_check("k_grp_sample")) return rc;
+    hipLaunchKernelGGL(k_grp_colsum, grid(g_col), dim3(256), 0, st, (const GColsum *)dev(col), g_col);
+    if (int rc = launch_check("k_grp_colsum")) return rc;
+    if (int rc = set_lds((const void *)k_grp_vp_phi, lds_phi)) return rc;
+    hipLaunchKernelGGL(k_grp_vp_phi, dim3(n), dim3(kFftThreads), lds_phi, st, (const VPJob *)dev(phi));
+    if (int rc = launch_check("k_grp_vp_phi")) return rc;
+    hipLaunchKernelGGL(k_grp_psi_planes, grid(g_pl), dim3(256), 0, st, (const GPlanes *)dev(pl), g_pl);
+    return launch_check("k_grp_psi_planes");
 }
 
 }  // namespace cmm

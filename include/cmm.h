@@ -407,6 +407,20 @@ int cmm_batch_destroy(cmm_batch *b);
 /* one step of length dt for every engine; infos: n entries (or NULL) */
 int cmm_batch_step(cmm_batch *b, double dt, cmm_step_info *infos);
 int cmm_batch_stats(cmm_batch *b, int *graphs_captured, int *graph_launches, int *eager_steps);
+/* Grouped steps (on by default): when every engine of the batch is an FP64 Vlasov-Poisson engine in
+ * its steady state whose scheme takes the separable advection (the ensembles of
+ * Cuda-CMM/src/simulation/cmm-vlasov1d.cu:48, one instance per engine), a step runs each kind of
+ * launch of all engines as one launch over the concatenated blocks of every instance, with the same
+ * per-instance results as their own launches; other steps run as above.  Off by default;
+ * cmm_batch_autotune chooses. */
+int cmm_batch_set_grouping(cmm_batch *b, int on);
+int cmm_batch_grouped_steps(cmm_batch *b, int *grouped_steps);
+/* Autotuner of the batch's launch form: grouped launches (one workgroup per block of work, or
+ * persistent ones of 2048 workgroups), captured graphs (when the batch was created with them) and
+ * eager queues each run steps_per_mode real steps of length dt (the run advances; graphs first
+ * capture theirs in 12 untimed steps); the fastest stays.  Every form gives the same results. */
+enum { CMM_BATCH_EAGER = 0, CMM_BATCH_GRAPHS = 1, CMM_BATCH_GROUPED = 2, CMM_BATCH_GROUPED_PERSISTENT = 3 };
+int cmm_batch_autotune(cmm_batch *b, double dt, int steps_per_mode, int *mode_out, double *ms_out);
 /* Autotuner: times the incompressibility check's tile edge (4, 2, 1 dividing the fine/coarse
  * ratio) and block cap on the engine's current map and keeps the fastest for its later steps. */
 int cmm_euler_autotune(cmm_euler *e, int *tile_out, int *max_blocks_out, double *ms_out);

@@ -1,8 +1,8 @@
 """Ensembles of Vlasov-Poisson instances of mixed grid sizes stepped together (cmm_batch_*):
-with the queued step work of all instances captured into one graph per step phase, every
-instance must follow exactly the steps it takes alone through cmm_euler_step (same kernels, same
-order, same coefficients for a step that is a binary fraction), including remaps, in both
-precision forms.  The autotuner's choice must leave the check's value unchanged up to the
+with the queued step work of all instances captured into one graph per step phase, or with every
+kind of launch of all instances in one grouped, persistent launch (FP64), every instance must
+follow exactly the steps it takes alone through cmm_euler_step (same block bodies, same order,
+same coefficients), including remaps, in both precision forms.  The autotuner's choice must leave the check's value unchanged up to the
 rounding of det near 1 (1e-13 absolute)."""
 import numpy as np
 import pytest
@@ -24,20 +24,24 @@ def make(cmm, n, ic, amp, precision):
 SPECS = [(32, "landau_amp", 0.05), (64, "two_stream_v2", 0.02), (32, "two_stream_v2", 0.04), (64, "landau_amp", 0.01)]
 
 
-@pytest.mark.parametrize("precision", ["fp64", "split"])
-def test_batch_graph_matches_single_engines(dev, precision):
+@pytest.mark.parametrize("precision,grouping", [("fp64", False), ("split", False), ("fp64", True)])
+def test_batch_graph_matches_single_engines(dev, precision, grouping):
     import cmm
     steps = 60
     alone = [make(cmm, *s, precision) for s in SPECS]
     ref_infos = [[e.step(DT) for _ in range(steps)] for e in alone]
     batched = [make(cmm, *s, precision) for s in SPECS]
-    b = cmm.Batch(batched, use_graphs=True)
+    b = cmm.Batch(batched, use_graphs=True, grouping=grouping)
     infos = [b.step(DT) for _ in range(steps)]
     st = b.stats()
     print("batch stats", st)
     remaps = [sum(i.remapped for i in r) for r in ref_infos]
     print("remaps per instance", remaps)
-    assert st["graph_launches"] == steps - 1 and st["eager_steps"] == 1
+    if grouping:
+        assert st["grouped_steps"] >= steps - 1
+        assert st["grouped_steps"] + st["graph_launches"] + st["eager_steps"] == steps
+    else:
+        assert st["graph_launches"] == steps - 1 and st["eager_steps"] == 1 and st["grouped_steps"] == 0
     for k, (ea, eb) in enumerate(zip(alone, batched)):
         einc_a = np.array([r.incomp for r in ref_infos[k]])
         einc_b = np.array([infos[s][k].incomp for s in range(steps)])
@@ -109,9 +113,9 @@ def test_ensemble_conditions_against_oracle(dev, ic, code):
     assert err <= 1e-10
 
 
-def run_pair(cmm, specs, dt, dts, precision="fp64", grid_of=None):
-    """the same engines stepped alone (cmm_euler_step) and together in a graph batch, with the step
-    sequence dts; returns both sets of step infos and engines"""
+def run_pair(cmm, specs, dt, dts, precision="fp64", grid_of=None, grouping=False):
+    """the same engines stepped alone (cmm_euler_step) and together in a batch (graphs, or grouped
+    launches), with the step sequence dts; returns both sets of step infos and engines"""
     mk = grid_of or (lambda n: (n, 4 * n, 2 * n, 2 * n))
 
     def make_e(n, ic, amp):
@@ -126,7 +130,7 @@ def run_pair(cmm, specs, dt, dts, precision="fp64", grid_of=None):
     alone = [make_e(*sp) for sp in specs]
     ref = [[e.step(d) for d in dts] for e in alone]
     batched = [make_e(*sp) for sp in specs]
-    b = cmm.Batch(batched, use_graphs=True)
+    b = cmm.Batch(batched, use_graphs=True, grouping=grouping)
     got = [b.step(d) for d in dts]
     return alone, ref, batched, got, b
 
@@ -141,29 +145,33 @@ def assert_same(alone, ref, batched, got, steps):
             assert np.array_equal(ea.field(f), eb.field(f)), (k, f)
 
 
-def test_batch_graph_after_a_shorter_step_and_at_a_non_dyadic_step(dev):
+@pytest.mark.parametrize("grouping", [False, True])
+def test_batch_graph_after_a_shorter_step_and_at_a_non_dyadic_step(dev, grouping):
     """Graph replays take their Lagrange weights from the time history of the step they replay:
     after a shortened step (eager) the nominal steps replay graphs captured under an even history,
-    and at dt = 1/20 the weights change in their last bits from step to step.  Every instance must
-    still equal its single-engine run bit for bit."""
+    and at dt = 1/20 the weights change in their last bits from step to step.  Grouped steps take
+    the coefficients of each step (the shortened one too).  Every instance must still equal its
+    single-engine run bit for bit."""
     import cmm
     specs = SPECS[:2]
     for dt, dts in ((DT, [DT] * 14 + [DT / 2] + [DT] * 14), (1.0 / 20, [1.0 / 20] * 40)):
-        alone, ref, batched, got, b = run_pair(cmm, specs, dt, dts)
+        alone, ref, batched, got, b = run_pair(cmm, specs, dt, dts, grouping=grouping)
         st = b.stats()
         print("dt", dt, "batch stats", st)
-        assert st["graph_launches"] >= len(dts) - 3
+        assert (st["grouped_steps"] if grouping else st["graph_launches"]) >= len(dts) - 3
         assert_same(alone, ref, batched, got, len(dts))
         b.close()
         for e in alone + batched:
             e.close()
 
 
-def test_batch_mixed_grid_sizes_of_the_ensemble_workload(dev):
+@pytest.mark.parametrize("grouping", [False, True])
+def test_batch_mixed_grid_sizes_of_the_ensemble_workload(dev, grouping):
     """One instance each of the ensemble workload's grid sizes (maps 256^2, 512^2, 1024^2; fine 4x,
     psi and vort 2x; x in [0, 4 pi), v in [-8, 8); dt 1/20; Landau and two-stream with seeded
-    amplitudes), 20 steps: the graph batch equals the instances run alone bit for bit, and the
-    256^2 instance follows the oracle's run (remap steps, E_inc within 1e-10, psi within 1e-10)."""
+    amplitudes), 20 steps: the graph batch and the grouped batch equal the instances run alone bit
+    for bit, and the 256^2 instance follows the oracle's run (remap steps, E_inc within 1e-10, psi
+    within 1e-10)."""
     import cmm
     import oracle_lib as O
     from gpu_util import rel_linf
@@ -171,8 +179,10 @@ def test_batch_mixed_grid_sizes_of_the_ensemble_workload(dev):
     specs = [(256, "landau_amp", amps[0]), (512, "two_stream_v2", amps[1]), (1024, "landau_amp", amps[2])]
     dt = 1.0 / 20
     steps = 20
-    alone, ref, batched, got, b = run_pair(cmm, specs, dt, [dt] * steps)
+    alone, ref, batched, got, b = run_pair(cmm, specs, dt, [dt] * steps, grouping=grouping)
     print("batch stats", b.stats(), "remaps", [sum(i.remapped for i in r) for r in ref])
+    if grouping:
+        assert b.stats()["grouped_steps"] >= steps - 1
     assert_same(alone, ref, batched, got, steps)
     psi = alone[0].field("psi")
     bb = (0.0, 2 * O.TWO_PI, -8.0, 8.0)
@@ -192,8 +202,8 @@ def test_batch_mixed_grid_sizes_of_the_ensemble_workload(dev):
         e.close()
 
 
-@pytest.mark.parametrize("precision", ["fp64", "split"])
-def test_engines_alternate_between_single_steps_and_the_batch(dev, precision):
+@pytest.mark.parametrize("precision,grouping", [("fp64", False), ("split", False), ("fp64", True)])
+def test_engines_alternate_between_single_steps_and_the_batch(dev, precision, grouping):
     """Batched engines also step alone between graph steps: a single-engine step runs the check and
     its E_inc copy on the engine's third stream (forked and joined by events), a graph step captures
     or replays the engine streams' work from the batch's master stream, and neither may leave state
@@ -206,12 +216,37 @@ def test_engines_alternate_between_single_steps_and_the_batch(dev, precision):
     alone = [make(cmm, *s, precision) for s in specs]
     ref = [[e.step(DT) for _ in plan] for e in alone]
     mixed = [make(cmm, *s, precision) for s in specs]
-    b = cmm.Batch(mixed, use_graphs=True)
+    b = cmm.Batch(mixed, use_graphs=True, grouping=grouping)
     got = [b.step(DT) if how == "b" else [e.step(DT) for e in mixed] for how in plan]
     st = b.stats()
     print("batch stats", st, "remaps", [sum(i.remapped for i in r) for r in ref])
-    assert st["graph_launches"] >= 20
+    assert (st["grouped_steps"] if grouping else st["graph_launches"]) >= 20
     assert_same(alone, ref, mixed, got, len(plan))
     b.close()
     for e in alone + mixed:
+        e.close()
+
+
+def test_batch_autotune_keeps_the_results(dev):
+    """cmm_batch_autotune runs real steps in each launch form (grouped, grouped persistent, graphs,
+    eager) and keeps the fastest; the steps it takes and the ones after it must equal the engines'
+    own steps bit for bit."""
+    import cmm
+    specs = SPECS[:3]
+    alone = [make(cmm, *s, "fp64") for s in specs]
+    batched = [make(cmm, *s, "fp64") for s in specs]
+    b = cmm.Batch(batched, use_graphs=True)
+    t = b.autotune(DT, 2)
+    steps_done = batched[0].state()[0]
+    print("autotune", t, "steps taken", steps_done, "stats", b.stats())
+    assert t["mode"] in ("eager", "graphs", "grouped", "grouped_persistent") and t["ms_per_step"] > 0
+    assert steps_done >= 1 + 2 + 2 + 12 + 2 + 2
+    got = [b.step(DT) for _ in range(10)]
+    ref = [[e.step(DT) for _ in range(steps_done + 10)][-10:] for e in alone]
+    for k, (ea, eb) in enumerate(zip(alone, batched)):
+        assert [r.incomp for r in ref[k]] == [got[s][k].incomp for s in range(10)]
+        for f in ("chix", "chiy", "psi"):
+            assert np.array_equal(ea.field(f), eb.field(f)), (k, f)
+    b.close()
+    for e in alone + batched:
         e.close()
