@@ -85,7 +85,7 @@ SYMBOLS = ["cmm_version", "cmm_last_error", "cmm_device_info", "cmm_lagrange_coe
            "cmm_map_to_split", "cmm_map_from_split", "cmm_hermite_to_split", "cmm_map_advect_split_workspace_bytes",
            "cmm_map_advect_split", "cmm_incomp_error_split", "cmm_psi_from_vort_split",
            "cmm_batch_create", "cmm_batch_destroy", "cmm_batch_step", "cmm_batch_stats", "cmm_batch_set_grouping", "cmm_batch_grouped_steps",
-           "cmm_batch_autotune", "cmm_euler_autotune", "cmm_euler_set_check_fusion",
+           "cmm_batch_autotune", "cmm_batch_set_mode", "cmm_euler_autotune", "cmm_euler_set_check_fusion",
            "cmm_euler_get_check_fusion", "cmm_euler_set_psi_precombine",
            "cmm_euler_get_psi_precombine",
            "cmm_cnn_forward", "cmm_cnn_backward_data", "cmm_cnn_backward_weights", "cmm_cnn_relu_mask",
@@ -189,6 +189,7 @@ def lib():
         L.cmm_batch_set_grouping.argtypes = [VP, C.c_int]
         L.cmm_batch_grouped_steps.argtypes = [VP, C.POINTER(C.c_int)]
         L.cmm_batch_autotune.argtypes = [VP, C.c_double, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.cmm_batch_set_mode.argtypes = [VP, C.c_int]
         L.cmm_euler_autotune.argtypes = [VP, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]
         L.cmm_euler_set_check_fusion.argtypes = [VP, C.c_int]
         L.cmm_euler_get_check_fusion.argtypes = [VP, C.POINTER(C.c_int)]
@@ -729,7 +730,12 @@ class Batch:
         _check(lib().cmm_batch_step(self.h, dt, infos), "cmm_batch_step")
         return list(infos)
 
-    MODES = {0: "eager", 1: "graphs", 2: "grouped", 3: "grouped_persistent"}
+    MODES = {0: "eager", 1: "graphs", 2: "grouped", 3: "grouped_persistent", 4: "grouped_lanes"}
+
+    def set_mode(self, name):
+        """the launch form by name (MODES)"""
+        code = {v: k for k, v in self.MODES.items()}[name]
+        _check(lib().cmm_batch_set_mode(self.h, code), "cmm_batch_set_mode")
 
     def autotune(self, dt, steps_per_mode=4):
         """cmm_batch_autotune: the fastest launch form over real steps (the run advances)"""

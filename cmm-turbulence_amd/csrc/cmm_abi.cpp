@@ -1502,10 +1502,20 @@ struct cmm_batch {
     bool grouping = false;
     int grouped = 0;
     int g_blocks = 0;  // grid cap of the grouped launches (0: one workgroup per block of work)
-    std::vector<GroupInstance> ginst;
-    void *g_host = nullptr, *g_dev = nullptr;
-    unsigned long long *g_hslots = nullptr, *g_dslots = nullptr;
-    hipEvent_t g_done = nullptr;
+    // the engines split into lanes of about equal work, each lane's grouped launches on its own
+    // stream (lane 0: master), so that the lanes' kernels overlap
+    struct Lane {
+        std::vector<int> members;
+        std::vector<GroupInstance> inst;
+        hipStream_t st = nullptr;
+        bool own_stream = false;
+        void *host = nullptr, *dev = nullptr;
+        unsigned long long *hslots = nullptr, *dslots = nullptr;
+        hipEvent_t done = nullptr;
+    };
+    int g_lanes = 1;
+    std::vector<Lane> lanes;
+    std::vector<std::pair<int, int>> lane_of;  // engine -> (lane, position in it)
 };
 
 namespace {
@@ -1548,41 +1558,90 @@ int batch_normalise(cmm_euler *e);
 // (step_queue_a/b with grp set), then op_group_vp_step launches all of them together on the master
 // stream, ordered after every engine stream's earlier work; the engines' streams are ordered after
 // it, and each engine finishes its step (E_inc, remap, bookkeeping) as after its own.
-int batch_step_grouped(cmm_batch *b, double dt, cmm_step_info *infos) {
-    const int n = (int)b->e.size();
-    if (!b->g_dev) {
-        const size_t jb = op_group_vp_job_bytes(n);
-        if (hipMalloc(&b->g_dev, jb) != hipSuccess || hipHostMalloc(&b->g_host, jb, hipHostMallocDefault) != hipSuccess ||
-            hipMalloc((void **)&b->g_dslots, sizeof(unsigned long long) * n) != hipSuccess ||
-            hipHostMalloc((void **)&b->g_hslots, sizeof(unsigned long long) * n, hipHostMallocDefault) != hipSuccess) {
+void batch_free_lanes(cmm_batch *b) {
+    for (cmm_batch::Lane &l : b->lanes) {
+        if (l.st) (void)hipStreamSynchronize(l.st);
+        if (l.dev) (void)hipFree(l.dev);
+        if (l.dslots) (void)hipFree(l.dslots);
+        if (l.host) (void)hipHostFree(l.host);
+        if (l.hslots) (void)hipHostFree(l.hslots);
+        if (l.done) (void)hipEventDestroy(l.done);
+        if (l.own_stream && l.st) (void)hipStreamDestroy(l.st);
+    }
+    b->lanes.clear();
+}
+
+// b->g_lanes lanes (fewer where there are fewer engines): the engines dealt largest first to the
+// lane with the least work so far (work: fine-grid nodes), each lane with its job tables and slots
+int batch_lanes(cmm_batch *b) {
+    const int n = (int)b->e.size(), nl = std::max(1, std::min(b->g_lanes, n));
+    if ((int)b->lanes.size() == nl) return CMM_OK;
+    batch_free_lanes(b);
+    b->lanes.resize(nl);
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i) order[i] = i;
+    auto work = [&](int i) { return (double)b->e[i]->nf; };
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return work(x) > work(y); });
+    std::vector<double> load(nl, 0.0);
+    b->lane_of.assign(n, {0, 0});
+    for (int i : order) {
+        const int l = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+        b->lane_of[i] = {l, (int)b->lanes[l].members.size()};
+        b->lanes[l].members.push_back(i);
+        load[l] += work(i);
+    }
+    for (int l = 0; l < nl; ++l) {
+        cmm_batch::Lane &ln = b->lanes[l];
+        const int m = (int)ln.members.size();
+        const size_t jb = op_group_vp_job_bytes(m);
+        if (hipMalloc(&ln.dev, jb) != hipSuccess || hipHostMalloc(&ln.host, jb, hipHostMallocDefault) != hipSuccess ||
+            hipMalloc((void **)&ln.dslots, sizeof(unsigned long long) * m) != hipSuccess ||
+            hipHostMalloc((void **)&ln.hslots, sizeof(unsigned long long) * m, hipHostMallocDefault) != hipSuccess) {
             set_error("grouped step: allocation of the job tables");
             return CMM_ERR_ALLOC;
         }
-        if (int rc = hip_check(hipEventCreateWithFlags(&b->g_done, hipEventDisableTiming), "hipEventCreate")) return rc;
+        if (int rc = hip_check(hipEventCreateWithFlags(&ln.done, hipEventDisableTiming), "hipEventCreate")) return rc;
+        if (l == 0) {
+            ln.st = b->master;
+        } else {
+            if (int rc = hip_check(hipStreamCreateWithFlags(&ln.st, hipStreamNonBlocking), "hipStreamCreate")) return rc;
+            ln.own_stream = true;
+        }
     }
+    return CMM_OK;
+}
+
+int batch_step_grouped(cmm_batch *b, double dt, cmm_step_info *infos) {
+    const int n = (int)b->e.size();
+    if (int rc = batch_lanes(b)) return rc;
     for (int i = 0; i < n; ++i) {
+        cmm_batch::Lane &ln = b->lanes[b->lane_of[i].first];
         if (int rc = hip_check(hipEventRecord(b->tail[i], b->e[i]->st), "hipEventRecord (tail)")) return rc;
-        if (int rc = hip_check(hipStreamWaitEvent(b->master, b->tail[i], 0), "hipStreamWaitEvent (tail)")) return rc;
+        if (int rc = hip_check(hipStreamWaitEvent(ln.st, b->tail[i], 0), "hipStreamWaitEvent (tail)")) return rc;
     }
-    b->ginst.assign(n, GroupInstance{});
+    for (cmm_batch::Lane &ln : b->lanes) ln.inst.assign(ln.members.size(), GroupInstance{});
     for (int i = 0; i < n; ++i) {
         cmm_euler *e = b->e[i];
-        e->grp = &b->ginst[i];
+        e->grp = &b->lanes[b->lane_of[i].first].inst[b->lane_of[i].second];
         bool fork = false;
         int rc = step_queue_a(e, dt, false, &fork);
         if (!rc) rc = step_queue_b(e, false);
         e->grp = nullptr;
         if (rc) return rc;
     }
-    if (int rc = op_group_vp_step(b->ginst.data(), n, b->g_dev, b->g_host, b->g_dslots, b->g_hslots, b->master,
-                                  b->g_blocks))
-        return rc;
-    if (int rc = hip_check(hipEventRecord(b->g_done, b->master), "hipEventRecord")) return rc;
-    if (int rc = hip_check(hipEventSynchronize(b->g_done), "hipEventSynchronize")) return rc;
+    for (cmm_batch::Lane &ln : b->lanes) {
+        if (int rc = op_group_vp_step(ln.inst.data(), (int)ln.inst.size(), ln.dev, ln.host, ln.dslots, ln.hslots, ln.st,
+                                      b->g_blocks))
+            return rc;
+        if (int rc = hip_check(hipEventRecord(ln.done, ln.st), "hipEventRecord")) return rc;
+    }
+    for (cmm_batch::Lane &ln : b->lanes)
+        if (int rc = hip_check(hipEventSynchronize(ln.done), "hipEventSynchronize")) return rc;
     for (int i = 0; i < n; ++i) {
         cmm_euler *e = b->e[i];
-        std::memcpy(e->h_slot, b->g_hslots + i, sizeof(unsigned long long));
-        if (int rc = hip_check(hipStreamWaitEvent(e->st, b->g_done, 0), "hipStreamWaitEvent")) return rc;
+        const cmm_batch::Lane &ln = b->lanes[b->lane_of[i].first];
+        std::memcpy(e->h_slot, ln.hslots + b->lane_of[i].second, sizeof(unsigned long long));
+        if (int rc = hip_check(hipStreamWaitEvent(e->st, ln.done, 0), "hipStreamWaitEvent")) return rc;
         // the interleaved map is brought back after a remap whether or not the caller takes infos
         cmm_step_info own{};
         cmm_step_info &info = infos ? infos[i] : own;
@@ -1748,11 +1807,7 @@ int cmm_batch_destroy(cmm_batch *b) {
             e->lw_host = reinterpret_cast<double *>(e->h_slot + 8);
         }
     for (auto &kv : b->cache) (void)hipGraphExecDestroy(kv.second);
-    if (b->g_dev) (void)hipFree(b->g_dev);
-    if (b->g_dslots) (void)hipFree(b->g_dslots);
-    if (b->g_host) (void)hipHostFree(b->g_host);
-    if (b->g_hslots) (void)hipHostFree(b->g_hslots);
-    if (b->g_done) (void)hipEventDestroy(b->g_done);
+    batch_free_lanes(b);
     for (auto *v : {&b->join, &b->tail, &b->ev_ak, &b->ev_bk})
         for (hipEvent_t ev : *v)
             if (ev) (void)hipEventDestroy(ev);
@@ -1832,15 +1887,26 @@ int batch_sync(cmm_batch *b) {
 }
 
 void batch_set_mode(cmm_batch *b, int mode) {
-    b->grouping = mode == CMM_BATCH_GROUPED || mode == CMM_BATCH_GROUPED_PERSISTENT;
+    b->grouping = mode == CMM_BATCH_GROUPED || mode == CMM_BATCH_GROUPED_PERSISTENT || mode == CMM_BATCH_GROUPED_LANES;
     b->g_blocks = mode == CMM_BATCH_GROUPED_PERSISTENT ? 2048 : 0;
+    b->g_lanes = mode == CMM_BATCH_GROUPED_LANES ? 4 : 1;
     b->graphs = mode == CMM_BATCH_GRAPHS;
 }
 
 }  // namespace
 
+int cmm_batch_set_mode(cmm_batch *b, int mode) {
+    if (!b || mode < CMM_BATCH_EAGER || mode > CMM_BATCH_GROUPED_LANES || (mode == CMM_BATCH_GRAPHS && !b->graphs_allowed)) {
+        set_error("cmm_batch_set_mode: bad argument");
+        return CMM_ERR_ARG;
+    }
+    batch_set_mode(b, mode);
+    return CMM_OK;
+}
+
 // The batch's launch form chosen by timing: each candidate (grouped launches with one workgroup per
-// block of work, persistent grouped launches of 2048 workgroups, captured graphs, eager queues)
+// block of work, persistent grouped launches of 2048 workgroups, grouped launches in 4 lanes of
+// engines on their own streams, captured graphs, eager queues)
 // runs steps_per_mode real steps of length dt back to back (the run advances; captured graphs
 // first take the 12 steps that capture them, untimed), between full synchronisations; the fastest
 // stays.  All forms give the same results bit for bit.
@@ -1859,6 +1925,7 @@ int cmm_batch_autotune(cmm_batch *b, double dt, int steps_per_mode, int *mode_ou
     if (ready) {
         cand.push_back(CMM_BATCH_GROUPED);
         cand.push_back(CMM_BATCH_GROUPED_PERSISTENT);
+        if (b->e.size() > 1) cand.push_back(CMM_BATCH_GROUPED_LANES);
     }
     if (b->graphs_allowed) cand.push_back(CMM_BATCH_GRAPHS);
     cand.push_back(CMM_BATCH_EAGER);

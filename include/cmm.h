@@ -415,12 +415,16 @@ int cmm_batch_stats(cmm_batch *b, int *graphs_captured, int *graph_launches, int
  * cmm_batch_autotune chooses. */
 int cmm_batch_set_grouping(cmm_batch *b, int on);
 int cmm_batch_grouped_steps(cmm_batch *b, int *grouped_steps);
-/* Autotuner of the batch's launch form: grouped launches (one workgroup per block of work, or
- * persistent ones of 2048 workgroups), captured graphs (when the batch was created with them) and
+/* Autotuner of the batch's launch form: grouped launches (one workgroup per block of work,
+ * persistent ones of 2048 workgroups, or grouped launches per lane of engines, 4 lanes of about
+ * equal work on their own streams), captured graphs (when the batch was created with them) and
  * eager queues each run steps_per_mode real steps of length dt (the run advances; graphs first
  * capture theirs in 12 untimed steps); the fastest stays.  Every form gives the same results. */
-enum { CMM_BATCH_EAGER = 0, CMM_BATCH_GRAPHS = 1, CMM_BATCH_GROUPED = 2, CMM_BATCH_GROUPED_PERSISTENT = 3 };
+enum { CMM_BATCH_EAGER = 0, CMM_BATCH_GRAPHS = 1, CMM_BATCH_GROUPED = 2, CMM_BATCH_GROUPED_PERSISTENT = 3,
+       CMM_BATCH_GROUPED_LANES = 4 };
 int cmm_batch_autotune(cmm_batch *b, double dt, int steps_per_mode, int *mode_out, double *ms_out);
+/* the launch form set directly (CMM_BATCH_*; graphs only for a batch created with them) */
+int cmm_batch_set_mode(cmm_batch *b, int mode);
 /* Autotuner: times the incompressibility check's tile edge (4, 2, 1 dividing the fine/coarse
  * ratio) and block cap on the engine's current map and keeps the fastest for its later steps. */
 int cmm_euler_autotune(cmm_euler *e, int *tile_out, int *max_blocks_out, double *ms_out);

@@ -113,7 +113,7 @@ def test_ensemble_conditions_against_oracle(dev, ic, code):
     assert err <= 1e-10
 
 
-def run_pair(cmm, specs, dt, dts, precision="fp64", grid_of=None, grouping=False):
+def run_pair(cmm, specs, dt, dts, precision="fp64", grid_of=None, grouping=False, mode=None):
     """the same engines stepped alone (cmm_euler_step) and together in a batch (graphs, or grouped
     launches), with the step sequence dts; returns both sets of step infos and engines"""
     mk = grid_of or (lambda n: (n, 4 * n, 2 * n, 2 * n))
@@ -131,6 +131,8 @@ def run_pair(cmm, specs, dt, dts, precision="fp64", grid_of=None, grouping=False
     ref = [[e.step(d) for d in dts] for e in alone]
     batched = [make_e(*sp) for sp in specs]
     b = cmm.Batch(batched, use_graphs=True, grouping=grouping)
+    if mode:
+        b.set_mode(mode)
     got = [b.step(d) for d in dts]
     return alone, ref, batched, got, b
 
@@ -165,7 +167,7 @@ def test_batch_graph_after_a_shorter_step_and_at_a_non_dyadic_step(dev, grouping
             e.close()
 
 
-@pytest.mark.parametrize("grouping", [False, True])
+@pytest.mark.parametrize("grouping", [False, True, "grouped_lanes", "grouped_persistent"])
 def test_batch_mixed_grid_sizes_of_the_ensemble_workload(dev, grouping):
     """One instance each of the ensemble workload's grid sizes (maps 256^2, 512^2, 1024^2; fine 4x,
     psi and vort 2x; x in [0, 4 pi), v in [-8, 8); dt 1/20; Landau and two-stream with seeded
@@ -179,7 +181,8 @@ def test_batch_mixed_grid_sizes_of_the_ensemble_workload(dev, grouping):
     specs = [(256, "landau_amp", amps[0]), (512, "two_stream_v2", amps[1]), (1024, "landau_amp", amps[2])]
     dt = 1.0 / 20
     steps = 20
-    alone, ref, batched, got, b = run_pair(cmm, specs, dt, [dt] * steps, grouping=grouping)
+    alone, ref, batched, got, b = run_pair(cmm, specs, dt, [dt] * steps, grouping=bool(grouping),
+                                           mode=grouping if isinstance(grouping, str) else None)
     print("batch stats", b.stats(), "remaps", [sum(i.remapped for i in r) for r in ref])
     if grouping:
         assert b.stats()["grouped_steps"] >= steps - 1
