@@ -660,23 +660,26 @@ def run_ensemble(args):
 
 
 def run_learned(args):
-    """The learned psi extrapolation on the double-shear-layer grids (map 128^2, fine 512^2, psi
-    args.learned_psi^2) with seeded random-spectrum vorticity: psi trajectories recorded every
-    args.learned_dt for seeds 0..n_train-1 (training) and 64..71 (held out), AdamW training through
-    the HIP kernels, the extrapolation errors with and without the network on the held-out seeds
-    (extrapolations across two recording steps), and the held-out runs at dt = 2 learned_dt with
-    and without the correction (fallback judge at 1e-6) against runs at learned_dt / 4: relative L2
-    error of the coarse vorticity at T = 1."""
+    """The learned psi extrapolation (map args.learned_map^2, psi args.learned_psi^2, fine and
+    vorticity args.learned_fine^2; by default the double-shear-layer grids 128^2 / 256^2 / 512^2)
+    with seeded random-spectrum vorticity: psi trajectories recorded every args.learned_dt for seeds
+    0..n_train-1 (training) and 64.. (held out), AdamW training through the HIP kernels, the
+    extrapolation errors with and without the network on the held-out seeds (extrapolations across
+    two recording steps), and the held-out runs at dt = 2 learned_dt with and without the
+    correction (fallback judge at 1e-6) against runs at learned_dt / 4: relative L2 error of the
+    coarse vorticity at T = args.learned_run_t."""
     import numpy as np
     import torch
     import cmm
     sys.path.insert(0, os.path.join(ROOT, "cmm-turbulence_amd"))
     import learned
-    dt_rec, n_psi = args.learned_dt, args.learned_psi
+    dt_rec, n_psi, n_map, n_fine = args.learned_dt, args.learned_psi, args.learned_map, args.learned_fine
+    fcut = args.learned_fcut if args.learned_fcut else n_psi / 2
     steps = int(round(args.learned_t / dt_rec))
     t0 = time.perf_counter()
-    trajs = [learned.trajectory(s, n_psi=n_psi, dt=dt_rec, steps=steps) for s in range(args.train_seeds)]
-    held = [learned.trajectory(s, n_psi=n_psi, dt=dt_rec, steps=steps) for s in range(64, 64 + args.held_seeds)]
+    traj = lambda s: learned.trajectory(s, n_map=n_map, n_psi=n_psi, n_fine=n_fine, dt=dt_rec, steps=steps, fcut=fcut)
+    trajs = [traj(s) for s in range(args.train_seeds)]
+    held = [traj(s) for s in range(64, 64 + args.held_seeds)]
     t_data = time.perf_counter() - t0
     net = learned.Net(seed=0)
     learned.fit_scales(net, trajs)
@@ -691,11 +694,11 @@ def run_learned(args):
     rl2 = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
     runs = []
     dt = 2 * dt_rec
-    n = int(round(1.0 / dt))
+    n = int(round(args.learned_run_t / dt))
     for seed in range(64, 64 + args.held_seeds):
         def engine(d):
-            e = cmm.Euler(g(128), g(512), g(n_psi), g(512), d, initial_condition="random_spectrum", seed=seed,
-                          freq_cut_psi=128.0, stack_capacity=16)
+            e = cmm.Euler(g(n_map), g(n_fine), g(n_psi), g(n_fine), d, initial_condition="random_spectrum", seed=seed,
+                          freq_cut_psi=fcut, stack_capacity=16)
             e.init()
             return e
         ref = engine(dt / 8)
@@ -728,9 +731,10 @@ def run_learned(args):
                      "mean_dev_learned": float(np.mean(cor.dev_learned)) if na else None,
                      "ms_per_step_plain": 1e3 * tp / n, "ms_per_step_learned": 1e3 * tl / n})
         log(f"[learned] seed {seed}: {runs[-1]}")
-    out = {"workload": f"learned psi extrapolation: map 128^2, psi {n_psi}^2, fine 512^2, random-spectrum seeds; "
-                       f"training seeds 0..{args.train_seeds - 1}, held out 64..{63 + args.held_seeds}; psi recorded "
-                       f"every {dt_rec}, extrapolated across {dt}; runs at dt {dt} against {dt / 8}, T 1",
+    out = {"workload": f"learned psi extrapolation: map {n_map}^2, psi {n_psi}^2, fine {n_fine}^2, psi cut-off {fcut}, "
+                       f"random-spectrum seeds; training seeds 0..{args.train_seeds - 1}, held out "
+                       f"64..{63 + args.held_seeds}; psi recorded every {dt_rec} to T {args.learned_t}, extrapolated "
+                       f"across {dt}; runs at dt {dt} against {dt / 8}, T {args.learned_run_t}",
            "network": "4 layers, 3x3, 32 channels, BF16 weights/activations, FP32 MFMA accumulation, AdamW with "
                       "warm-up, cosine decay and clipping",
            "data_s": round(t_data, 2), "train_ms_per_iteration": round(1e3 * t_train, 3), "train_loss": hist,
@@ -768,6 +772,10 @@ def main():
     ap.add_argument("--learned-dt", type=float, default=1.0 / 32, help="psi recording step of the learned leg")
     ap.add_argument("--learned-psi", type=int, default=256, help="psi grid of the learned leg")
     ap.add_argument("--learned-t", type=float, default=2.0, help="length of the recorded trajectories")
+    ap.add_argument("--learned-map", type=int, default=128, help="map grid of the learned leg")
+    ap.add_argument("--learned-fine", type=int, default=512, help="fine (and vorticity) grid of the learned leg")
+    ap.add_argument("--learned-fcut", type=float, default=None, help="psi cut-off of the learned leg (psi grid / 2)")
+    ap.add_argument("--learned-run-t", type=float, default=1.0, help="length of the learned leg's in-loop runs")
     ap.add_argument("--t-end", type=float, default=40.0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 1 GiB device copy, the reference kernel for counter scaling")

@@ -88,7 +88,8 @@ SYMBOLS = ["cmm_version", "cmm_last_error", "cmm_device_info", "cmm_lagrange_coe
            "cmm_batch_autotune", "cmm_batch_set_mode", "cmm_euler_autotune", "cmm_euler_set_check_fusion",
            "cmm_euler_get_check_fusion", "cmm_euler_set_psi_precombine",
            "cmm_euler_get_psi_precombine",
-           "cmm_cnn_forward", "cmm_cnn_backward_data", "cmm_cnn_backward_weights", "cmm_cnn_relu_mask",
+           "cmm_cnn_forward", "cmm_cnn_backward_data", "cmm_cnn_backward_data_relu", "cmm_cnn_backward_weights",
+           "cmm_cnn_relu_mask",
            "cmm_cnn_fill", "cmm_cnn_weights_bf16", "cmm_cnn_adamw", "cmm_cnn_input", "cmm_cnn_target", "cmm_cnn_mse",
            "cmm_cnn_correction", "cmm_extrapolate", "cmm_dev_norms", "cmm_euler_copy_psi_level",
            "cmm_euler_psi_level", "cmm_euler_set_psi_correction"]
@@ -197,6 +198,7 @@ def lib():
         L.cmm_euler_get_psi_precombine.argtypes = [VP, C.POINTER(C.c_int)]
         L.cmm_cnn_forward.argtypes = [C.c_int, C.c_int, C.c_int, VP, VP, VP, C.c_int, VP, VP]
         L.cmm_cnn_backward_data.argtypes = [C.c_int, C.c_int, C.c_int, VP, VP, VP, VP]
+        L.cmm_cnn_backward_data_relu.argtypes = [C.c_int, C.c_int, C.c_int, VP, VP, VP, VP, VP]
         L.cmm_cnn_backward_weights.argtypes = [C.c_int, C.c_int, C.c_int, VP, VP, VP, VP, VP]
         L.cmm_cnn_relu_mask.argtypes = [C.c_long, VP, VP, VP]
         L.cmm_cnn_fill.argtypes = [C.c_long, VP, C.c_float, VP]

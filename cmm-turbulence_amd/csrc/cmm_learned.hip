@@ -1,12 +1,14 @@
 // Learned psi extrapolation (a chosen form with no counterpart in the reference): a 3x3
 // convolutional network whose layers all have 32 channels (the 3 input and 2 output channels are
 // zero-padded to 32), BF16 weights and activations in NHWC layout, FP32 accumulation on the matrix
-// cores (v_mfma_f32_32x32x8_bf16), periodic padding.  Forward, backward-data and backward-weight
+// cores (v_mfma_f32_32x32x16_bf16), periodic padding.  Forward, backward-data and backward-weight
 // kernels, the optimiser and the glue between the psi history and the network.
 //
-// MFMA 32x32x8 operand layout (wave of 64 lanes, i = lane % 32, h = lane / 32):
-//   A[m][k] with m = i, k = 4h..4h+3 (four bf16 per lane), B[k][n] with n = i, k = 4h..4h+3,
-//   D[m][n] with n = i and m = 8 (r / 4) + 4 h + r % 4 for the lane's 16 results r.
+// MFMA 32x32x16 operands (wave of 64 lanes, i = lane % 32, h = lane / 32): lane (i, h) gives row
+// m = i of A and column n = i of B, eight bf16 each (operand slots e = 0..7); slot e of lane (i, h)
+// in A and in B meet in the same K index, so the kernels put the same pixel or channel, 8 h + e of
+// the K-block, into both and the product sums over the block whatever the instruction's K order.
+// D[m][n] with n = i and m = 8 (r / 4) + 4 h + r % 4 for the lane's 16 results r.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -18,7 +20,7 @@
 
 namespace cmm {
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 constexpr int kCh = 32;  // channels of every layer
 
@@ -35,15 +37,17 @@ __device__ __forceinline__ int wrapi(int v, int n) { return v < 0 ? v + n : v >=
 
 // out[n][y][x][co] = act(b[co] + sum_{t, ci} W[co][t][ci] in[n][y + dy_t][x + dx_t][ci]), tap
 // t = 3 (dy + 1) + (dx + 1), periodic in y and x.  A wave takes SEGS consecutive 32-pixel
-// segments of a row: M = pixels, N = output channels, K = 9 taps x 32 channels (36 MFMAs per
-// segment); the wave's B operands (its 36 weight fragments) stay in registers across the
-// segments (4 for large batches, 1 where fewer waves would leave the device idle).  MODE 0: bf16
-// out with ReLU, 1: bf16 out, 2: fp32 out.  The backward-data pass is this kernel on the output
-// gradient with the weights transposed in channels and flipped in taps.
+// segments of a row: M = output channels, N = pixels, K = 9 taps x 32 channels (18 MFMAs per
+// segment, the bias as the accumulators' start); the wave's A operands (its 18 weight fragments)
+// stay in registers across the segments (4 for large batches, 1 where fewer waves would leave the
+// device idle).  MODE 0: bf16 out with ReLU, 1: bf16 out, 2: fp32 out, 3: bf16 out, zero where
+// mask (bf16, same layout as out) is not positive.  The backward-data pass is this kernel on the output gradient with the weights
+// transposed in channels and flipped in taps; mode 3 also applies the ReLU derivative of the layer
+// below.
 template <int MODE, int SEGS>
 __global__ void __launch_bounds__(256) k_conv3(const unsigned short *__restrict__ in, const unsigned short *__restrict__ W,
                                                const float *__restrict__ bias, void *__restrict__ out, int H, int Wd,
-                                               int segs) {
+                                               int segs, const unsigned short *__restrict__ mask) {
     const int lane = threadIdx.x & 63;
     const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
     const int groups = segs / SEGS;  // segment groups per row (segs is a multiple of SEGS)
@@ -51,82 +55,199 @@ __global__ void __launch_bounds__(256) k_conv3(const unsigned short *__restrict_
     const long img = (long)blockIdx.y * H * Wd;
     const int y = wave / groups, xg = (wave % groups) * (32 * SEGS);
     const int i = lane & 31, h = lane >> 5;
-    bf16x4 b[36];
+    // A = weights (row m = output channel i), B = input (column n = pixel x0 + i); operand slot e of
+    // lane (i, h) in K-block kb of tap t: input channel 16 kb + 8 h + e in both.  The lane's 16
+    // results are pixel x0 + i, output channels 8 g + 4 h .. 8 g + 4 h + 3 for g = 0..3: four runs
+    // of 4 consecutive channels, written with 8-byte (16-byte for FP32) stores.
+    bf16x8 wf[18];
 #pragma unroll
-    for (int q = 0; q < 36; ++q)
-        b[q] = *reinterpret_cast<const bf16x4 *>(W + ((long)i * 9 + q / 4) * kCh + 4 * h + 8 * (q % 4));
-    const float bb = bias ? bias[i] : 0.f;
+    for (int q = 0; q < 18; ++q)
+        wf[q] = *reinterpret_cast<const bf16x8 *>(W + ((long)i * 9 + q / 2) * kCh + 8 * h + 16 * (q % 2));
     for (int sg = 0; sg < SEGS; ++sg) {
         const int x0 = xg + 32 * sg;
-        f32x16 acc = {};
+        const long o = (img + (long)y * Wd + x0 + i) * kCh + 4 * h;  // + 8 g: channel run g
+        f32x16 acc;
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const float4 bv = bias ? *reinterpret_cast<const float4 *>(bias + 8 * g + 4 * h) : float4{0.f, 0.f, 0.f, 0.f};
+            acc[4 * g] = bv.x;
+            acc[4 * g + 1] = bv.y;
+            acc[4 * g + 2] = bv.z;
+            acc[4 * g + 3] = bv.w;
+        }
+        uint2 mk[4];
+        if (MODE == 3)
+#pragma unroll
+            for (int g = 0; g < 4; ++g) mk[g] = *reinterpret_cast<const uint2 *>(mask + o + 8 * g);
 #pragma unroll
         for (int t = 0; t < 9; ++t) {
             const int yy = wrapi(y + t / 3 - 1, H), xx = wrapi(x0 + i + t % 3 - 1, Wd);
-            const unsigned short *ap = in + (img + (long)yy * Wd + xx) * kCh + 4 * h;
+            const unsigned short *bp = in + (img + (long)yy * Wd + xx) * kCh + 8 * h;
 #pragma unroll
-            for (int cb = 0; cb < 4; ++cb) {
-                const bf16x4 a = *reinterpret_cast<const bf16x4 *>(ap + 8 * cb);
-                acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b[4 * t + cb], acc, 0, 0, 0);
+            for (int kb = 0; kb < 2; ++kb) {
+                const bf16x8 bx = *reinterpret_cast<const bf16x8 *>(bp + 16 * kb);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wf[2 * t + kb], bx, acc, 0, 0, 0);
             }
         }
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int p = 8 * (r >> 2) + 4 * h + (r & 3);
-            float v = acc[r] + bb;
-            if (MODE == 0) v = fmaxf(v, 0.f);
-            const long o = (img + (long)y * Wd + x0 + p) * kCh + i;
-            if (MODE == 2) static_cast<float *>(out)[o] = v;
-            else static_cast<unsigned short *>(out)[o] = f2bf(v);
+        for (int g = 0; g < 4; ++g) {
+            float v[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                v[j] = acc[4 * g + j];
+                if (MODE == 0) v[j] = fmaxf(v[j], 0.f);
+                if (MODE == 3) {
+                    const unsigned a = ((j < 2 ? mk[g].x : mk[g].y) >> (16 * (j & 1))) & 0xffffu;
+                    if ((a & 0x8000u) || (a & 0x7fffu) == 0) v[j] = 0.f;
+                }
+            }
+            if (MODE == 2) {
+                *reinterpret_cast<float4 *>(static_cast<float *>(out) + o + 8 * g) = float4{v[0], v[1], v[2], v[3]};
+            } else {
+                const uint2 pk = {(unsigned)f2bf(v[0]) | ((unsigned)f2bf(v[1]) << 16),
+                                  (unsigned)f2bf(v[2]) | ((unsigned)f2bf(v[3]) << 16)};
+                *reinterpret_cast<uint2 *>(static_cast<unsigned short *>(out) + o + 8 * g) = pk;
+            }
         }
     }
 }
 
-// dW[co][t][ci] += sum_p g[p][co] in[p + off_t][ci] (grid y = tap): a wave takes `segw`
-// consecutive 32-pixel row segments, M = output channels, N = input channels, K = pixels (8 per
-// MFMA, 4 MFMAs per segment); the row addressing is set once per segment, and the wave's 32 x 32
-// block is added to dW with FP32 atomics.  The centre-tap waves also sum the A operand's rows (the
-// bias gradient db[co] = sum_p g[p][co]).
-__global__ void __launch_bounds__(256) k_conv3_wgrad(const unsigned short *__restrict__ g,
-                                                     const unsigned short *__restrict__ in, float *__restrict__ dW,
-                                                     float *__restrict__ db, int H, int Wd, int nseg, int segw) {
-    const int lane = threadIdx.x & 63;
-    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int s0 = wave * segw;
-    if (s0 >= nseg) return;  // uniform over the wave
-    const int t = blockIdx.y, dy = t / 3 - 1, dx = t % 3 - 1;
+// dW[co][t][ci] += sum_p g[p][co] in[p + off_t][ci], db[co] += sum_p g[p][co].  A unit of work
+// is a 32-pixel column segment of kWgRows rows of g in one image; a wave takes units in turn
+// (stride: all waves of the grid) and keeps the 9 taps' 32 x 32 sums in its accumulators.  It
+// walks the input rows r = y0 - 1 .. y1 of its unit: row r's 34 pixels (with the periodic halo)
+// and g row r + 1 go from global memory (16-byte loads, the next row's issued before this row's
+// products) into the wave's own LDS tiles, transposed to [channel][pixel] so that a lane reads the
+// 8 consecutive pixels of one channel that an MFMA operand takes; the input row is written three
+// times, shifted by dx = -1, 0, 1.  g rows r - 1, r, r + 1 stay in a window of three tiles, so
+// every g row and input row is read from global memory once per unit.  Per row: 2 K-blocks of 16
+// pixels x (3 taps dy) x (3 taps dx) v_mfma_f32_32x32x16_bf16 (M = output channels from g, N =
+// input channels, K = pixels; A and B take the same pixel for the same operand slot, so the sum is
+// over pixels whatever the instruction's K order).  The block's 4 waves add their sums through LDS
+// and the block adds them to dW and db with FP32 atomics.
+constexpr int kWgRows = 32;              // g rows per unit
+constexpr int kWgPitch = 40;             // LDS pitch (pixels) of a transposed tile
+constexpr int kWgTile = kCh * kWgPitch;  // elements of one tile
+constexpr int kWgWaves = 4;
+
+__device__ __forceinline__ void wg_tile_put(unsigned short *t, int p, int q, const uint4 v) {
+    const unsigned w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+    for (int j = 0; j < 8; ++j) t[(8 * q + j) * kWgPitch + p] = (unsigned short)(w[j >> 1] >> (16 * (j & 1)));
+}
+
+__global__ void __launch_bounds__(64 * kWgWaves) __attribute__((amdgpu_waves_per_eu(2))) k_conv3_wgrad(const unsigned short *__restrict__ g,
+                                                               const unsigned short *__restrict__ in,
+                                                               float *__restrict__ dW, float *__restrict__ db, int H,
+                                                               int Wd, int bands, long units) {
+    __shared__ __align__(16) unsigned short lds[kWgWaves * 6 * kWgTile];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    unsigned short *gT = lds + wid * 6 * kWgTile;  // 3 g rows
+    unsigned short *iT = gT + 3 * kWgTile;         // the input row, shifted by dx = -1, 0, 1
     const int i = lane & 31, h = lane >> 5;
     const int segs = Wd / 32;
-    const int s1 = s0 + segw < nseg ? s0 + segw : nseg;
-    f32x16 acc = {};
-    float bsum = 0.f;
-    for (int s = s0; s < s1; ++s) {
-        const int row = s / segs;  // n * H + y
-        const int x0 = (s - row * segs) * 32;
-        const int n = row / H, y = row - n * H;
-        const unsigned short *grow = g + (long)row * Wd * kCh + i;
-        const unsigned short *irow = in + ((long)n * H + wrapi(y + dy, H)) * Wd * kCh + i;
+    // this lane's 16-byte chunks of a 32-pixel row: pixels p0 and p0 + 16, channels 8q .. 8q + 7;
+    // lanes 0..7 also take the halo pixels (x0 - 1 for lanes 0..3, x0 + 32 for lanes 4..7)
+    const int p0 = lane >> 2, q = lane & 3;
+    f32x16 acc[9];
 #pragma unroll
-        for (int kb = 0; kb < 32; kb += 8) {
-            bf16x4 a, b;
+    for (int t = 0; t < 9; ++t) acc[t] = f32x16{};
+    float bs[8] = {};
+    for (long u = (long)blockIdx.x * kWgWaves + wid; u < units; u += (long)gridDim.x * kWgWaves) {
+        const int seg = (int)(u % segs);
+        const long rest = u / segs;
+        const int band = (int)(rest % bands);
+        const long n = rest / bands;
+        const int x0 = 32 * seg, y0 = band * kWgRows, y1 = min(H, y0 + kWgRows);
+        const int xh = lane < 4 ? wrapi(x0 - 1, Wd) : wrapi(x0 + 32, Wd);
+        const unsigned short *gimg = g + n * H * Wd * kCh;
+        const unsigned short *iimg = in + n * H * Wd * kCh;
+        uint4 gv[2], iv[2], hv = {};
+        auto load = [&](int r) {
+            const unsigned short *irow = iimg + (long)wrapi(r, H) * Wd * kCh;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int x = x0 + kb + 4 * h + j;
-                const unsigned short gv = grow[(long)x * kCh];
-                a[j] = (short)gv;
-                b[j] = (short)irow[(long)wrapi(x + dx, Wd) * kCh];
-                bsum += bf2f(gv);
+            for (int k = 0; k < 2; ++k)
+                iv[k] = *reinterpret_cast<const uint4 *>(irow + (long)(x0 + p0 + 16 * k) * kCh + 8 * q);
+            if (lane < 8) hv = *reinterpret_cast<const uint4 *>(irow + (long)xh * kCh + 8 * q);
+            if (r + 1 < y1) {
+                const unsigned short *grow = gimg + (long)(r + 1) * Wd * kCh;
+#pragma unroll
+                for (int k = 0; k < 2; ++k)
+                    gv[k] = *reinterpret_cast<const uint4 *>(grow + (long)(x0 + p0 + 16 * k) * kCh + 8 * q);
             }
-            acc = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(a, b, acc, 0, 0, 0);
+        };
+        load(y0 - 1);
+        for (int r = y0 - 1; r <= y1; ++r) {
+            // this row into the tiles (the previous row's reads of them are done: LDS operations of
+            // a wave complete in order, and the fence keeps the compiler from moving them)
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                const int p = p0 + 16 * k;
+                if (p <= 30) wg_tile_put(iT, p + 1, q, iv[k]);  // dx = -1: position m holds pixel x0 + m - 1
+                wg_tile_put(iT + kWgTile, p, q, iv[k]);
+                if (p >= 1) wg_tile_put(iT + 2 * kWgTile, p - 1, q, iv[k]);
+            }
+            if (lane < 4) wg_tile_put(iT, 0, q, hv);
+            else if (lane < 8) wg_tile_put(iT + 2 * kWgTile, 31, q, hv);
+            if (r + 1 < y1) {
+                unsigned short *gs = gT + ((r + 1 - y0) % 3) * kWgTile;
+#pragma unroll
+                for (int k = 0; k < 2; ++k) {
+                    wg_tile_put(gs, p0 + 16 * k, q, gv[k]);
+                    const unsigned w[4] = {gv[k].x, gv[k].y, gv[k].z, gv[k].w};
+#pragma unroll
+                    for (int j = 0; j < 8; ++j) bs[j] += bf2f((unsigned short)(w[j >> 1] >> (16 * (j & 1))));
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+            if (r < y1) load(r + 1);  // the next row in flight during this row's products
+#pragma unroll
+            for (int kb = 0; kb < 32; kb += 16) {
+                const int m0 = kb + 8 * h;
+                bf16x8 B[3];
+#pragma unroll
+                for (int c = 0; c < 3; ++c) B[c] = *reinterpret_cast<const bf16x8 *>(iT + c * kWgTile + i * kWgPitch + m0);
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    const int y = r - (d - 1);  // g row of tap dy = d - 1
+                    if (y < y0 || y >= y1) continue;
+                    const bf16x8 A = *reinterpret_cast<const bf16x8 *>(gT + ((y - y0) % 3) * kWgTile + i * kWgPitch + m0);
+#pragma unroll
+                    for (int c = 0; c < 3; ++c)
+                        acc[3 * d + c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(A, B[c], acc[3 * d + c], 0, 0, 0);
+                }
+            }
         }
     }
-    if (t == 4) {
-        bsum += __shfl_xor(bsum, 32, 64);
-        if (h == 0) atomicAdd(&db[i], bsum);
-    }
+    // bias sums: lanes with the same channel group q add up
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-        const int co = 8 * (r >> 2) + 4 * h + (r & 3);
-        atomicAdd(&dW[((long)co * 9 + t) * kCh + i], acc[r]);
+    for (int j = 0; j < 8; ++j)
+        for (int o = 4; o < 64; o <<= 1) bs[j] += __shfl_xor(bs[j], o, 64);
+    __syncthreads();
+    float *red = reinterpret_cast<float *>(lds);  // kWgWaves x 1024 sums, then kWgWaves x 32 bias sums
+#pragma unroll
+    for (int t = 0; t < 9; ++t) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) red[wid * 1024 + (8 * (r >> 2) + 4 * h + (r & 3)) * kCh + i] = acc[t][r];
+        __syncthreads();
+        for (int e = threadIdx.x; e < 1024; e += 64 * kWgWaves) {
+            float v = red[e];
+#pragma unroll
+            for (int w = 1; w < kWgWaves; ++w) v += red[w * 1024 + e];
+            atomicAdd(&dW[((long)(e >> 5) * 9 + t) * kCh + (e & 31)], v);
+        }
+        __syncthreads();
+    }
+    if (lane < 4)
+#pragma unroll
+        for (int j = 0; j < 8; ++j) red[wid * 32 + 8 * q + j] = bs[j];
+    __syncthreads();
+    if (threadIdx.x < kCh) {
+        float v = red[threadIdx.x];
+#pragma unroll
+        for (int w = 1; w < kWgWaves; ++w) v += red[w * 32 + threadIdx.x];
+        atomicAdd(&db[threadIdx.x], v);
     }
 }
 
@@ -295,32 +416,51 @@ bool img_ok(int n_img, int H, int W) { return n_img > 0 && H > 0 && W >= 32 && W
 
 extern "C" {
 
+namespace {
+int conv3(int n_img, int H, int W, const uint16_t *in, const uint16_t *w, const float *b, int mode, void *out,
+          const uint16_t *mask, void *stream) {
+    const int segs = W / 32;
+    // four segments per wave when that still gives 2048 waves or more
+    const bool four = segs % 4 == 0 && (long)n_img * H * (segs / 4) >= 2048;
+    const dim3 grid((unsigned)((H * (four ? segs / 4 : segs) + 3) / 4), (unsigned)n_img);
+#define CMM_CONV(M, SG) \
+    hipLaunchKernelGGL((k_conv3<M, SG>), grid, dim3(256), 0, S(stream), in, w, b, out, H, W, segs, mask)
+#define CMM_CONV_MODES(SG)            \
+    if (mode == 0) CMM_CONV(0, SG);   \
+    else if (mode == 1) CMM_CONV(1, SG); \
+    else if (mode == 2) CMM_CONV(2, SG); \
+    else CMM_CONV(3, SG);
+    if (four) {
+        CMM_CONV_MODES(4)
+    } else {
+        CMM_CONV_MODES(1)
+    }
+#undef CMM_CONV_MODES
+#undef CMM_CONV
+    return lc("k_conv3");
+}
+}  // namespace
+
 int cmm_cnn_forward(int n_img, int H, int W, const uint16_t *in, const uint16_t *w, const float *b, int mode, void *out,
                     void *stream) {
     if (!img_ok(n_img, H, W) || !in || !w || !out || mode < 0 || mode > 2) {
         set_error("cmm_cnn_forward: bad argument (width a multiple of 32)");
         return CMM_ERR_ARG;
     }
-    const int segs = W / 32;
-    // four segments per wave when that still gives 2048 waves or more
-    const bool four = segs % 4 == 0 && (long)n_img * H * (segs / 4) >= 2048;
-    const dim3 grid((unsigned)((H * (four ? segs / 4 : segs) + 3) / 4), (unsigned)n_img);
-#define CMM_CONV(M, SG) hipLaunchKernelGGL((k_conv3<M, SG>), grid, dim3(256), 0, S(stream), in, w, b, out, H, W, segs)
-    if (four) {
-        if (mode == 0) CMM_CONV(0, 4);
-        else if (mode == 1) CMM_CONV(1, 4);
-        else CMM_CONV(2, 4);
-    } else {
-        if (mode == 0) CMM_CONV(0, 1);
-        else if (mode == 1) CMM_CONV(1, 1);
-        else CMM_CONV(2, 1);
-    }
-#undef CMM_CONV
-    return lc("k_conv3");
+    return conv3(n_img, H, W, in, w, b, mode, out, nullptr, stream);
 }
 
 int cmm_cnn_backward_data(int n_img, int H, int W, const uint16_t *g, const uint16_t *wt, uint16_t *gin, void *stream) {
     return cmm_cnn_forward(n_img, H, W, g, wt, nullptr, 1, gin, stream);
+}
+
+int cmm_cnn_backward_data_relu(int n_img, int H, int W, const uint16_t *g, const uint16_t *wt, const uint16_t *act,
+                               uint16_t *gin, void *stream) {
+    if (!img_ok(n_img, H, W) || !g || !wt || !act || !gin) {
+        set_error("cmm_cnn_backward_data_relu: bad argument (width a multiple of 32)");
+        return CMM_ERR_ARG;
+    }
+    return conv3(n_img, H, W, g, wt, nullptr, 3, gin, act, stream);
 }
 
 int cmm_cnn_backward_weights(int n_img, int H, int W, const uint16_t *g, const uint16_t *in, float *dw, float *db,
@@ -329,12 +469,12 @@ int cmm_cnn_backward_weights(int n_img, int H, int W, const uint16_t *g, const u
         set_error("cmm_cnn_backward_weights: bad argument");
         return CMM_ERR_ARG;
     }
-    // about 4096 waves over the 9 taps: fewer waves add fewer FP32 atomics to the 9216 weights
-    const int nseg = n_img * H * (W / 32);
-    const int segw = std::max(1, nseg * 9 / 4096);
-    const int waves = (nseg + segw - 1) / segw;
-    hipLaunchKernelGGL(k_conv3_wgrad, dim3((unsigned)((waves + 3) / 4), 9), dim3(256), 0, S(stream), g, in, dw, db, H,
-                       W, nseg, segw);
+    // at most 2 blocks per CU: each block adds its sums with 9248 FP32 atomics
+    const int bands = (H + kWgRows - 1) / kWgRows;
+    const long units = (long)n_img * bands * (W / 32);
+    const long blocks = std::min<long>((units + kWgWaves - 1) / kWgWaves, 512);
+    hipLaunchKernelGGL(k_conv3_wgrad, dim3((unsigned)blocks), dim3(64 * kWgWaves), 0, S(stream), g, in, dw, db, H, W,
+                       bands, units);
     return lc("k_conv3_wgrad");
 }
 

@@ -134,8 +134,8 @@ class Net:
                 "cmm_cnn_backward_weights")
             if l == 0:
                 break
-            _ck(L.cmm_cnn_backward_data(n, H, W, _vp(g), self.wt_(l), _vp(gn), _st()), "cmm_cnn_backward_data")
-            _ck(L.cmm_cnn_relu_mask(npx * CH, _vp(self.act[l]), _vp(gn), _st()), "cmm_cnn_relu_mask")
+            _ck(L.cmm_cnn_backward_data_relu(n, H, W, _vp(g), self.wt_(l), _vp(self.act[l]), _vp(gn), _st()),
+                "cmm_cnn_backward_data_relu")
             g, gn = gn, g
 
     def adamw(self, lr, gscale, wd=1e-4):
@@ -171,13 +171,14 @@ def _ptr(x):
 
 # ---- trajectories -----------------------------------------------------------------------------
 
-def trajectory(seed, n_map=128, n_psi=256, n_fine=512, dt=1.0 / 256, steps=256):
+def trajectory(seed, n_map=128, n_psi=256, n_fine=512, dt=1.0 / 256, steps=256, fcut=None):
     """psi value planes of a random-spectrum Euler run (seeded), one per step of length dt:
-    a device tensor [steps + 1, n_psi^2] (row 0 after the warm-up)."""
+    a device tensor [steps + 1, n_psi^2] (row 0 after the warm-up).  fcut: the psi cut-off
+    (default n_psi / 2)."""
     import torch
     g = cmm.grid
     e = cmm.Euler(g(n_map), g(n_fine), g(n_psi), g(n_fine), dt, initial_condition="random_spectrum", seed=seed,
-                  freq_cut_psi=n_psi / 2, stack_capacity=16)
+                  freq_cut_psi=fcut if fcut else n_psi / 2, stack_capacity=16)
     e.init()
     out = torch.zeros((steps + 1, n_psi * n_psi), dtype=torch.float64, device="cuda")
     _ck(cmm.lib().cmm_euler_copy_psi_level(e.h, 0, 0, _vp(out[0])), "cmm_euler_copy_psi_level")

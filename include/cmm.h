@@ -456,6 +456,11 @@ int cmm_cnn_forward(int n_img, int H, int W, const uint16_t *in, const uint16_t 
 /* gradient with respect to the layer input: wt holds the transposed, tap-flipped weights
  * (cmm_cnn_weights_bf16) */
 int cmm_cnn_backward_data(int n_img, int H, int W, const uint16_t *g, const uint16_t *wt, uint16_t *g_in, void *stream);
+/* cmm_cnn_backward_data with the ReLU derivative of the layer below in the same pass: g_in is zero
+ * where act (the layer input, a ReLU output, same layout) is not positive, as cmm_cnn_relu_mask
+ * after cmm_cnn_backward_data leaves it */
+int cmm_cnn_backward_data_relu(int n_img, int H, int W, const uint16_t *g, const uint16_t *wt, const uint16_t *act,
+                               uint16_t *g_in, void *stream);
 /* dw (32 x 9 x 32 floats) and db (32 floats) accumulate the weight and bias gradients */
 int cmm_cnn_backward_weights(int n_img, int H, int W, const uint16_t *g, const uint16_t *in, float *dw, float *db,
                              void *stream);

@@ -79,6 +79,13 @@ def test_conv_kernels_match_cpu(dev):
     e = np.abs(got - refd).max() / np.abs(refd).max()
     print("backward data: rel err", e)
     assert e <= 2.0 ** -8 + 1e-5
+    # with the ReLU derivative of the layer below in the same pass: the separate mask's result
+    act = dev16(R.bf16(rng.standard_normal((N, H, W, 32))))
+    gin_r = torch.zeros_like(gin)
+    cmm._check(L.cmm_cnn_backward_data_relu(N, H, W, vp(dev16(g)), vp(wt), vp(act), vp(gin_r), stream()),
+               "backward data relu")
+    cmm._check(L.cmm_cnn_relu_mask(N * H * W * 32, vp(act), vp(gin), stream()), "relu mask")
+    assert np.array_equal(host16(gin_r), host16(gin))
     dw = torch.zeros(32 * 9 * 32, dtype=torch.float32, device="cuda")
     db = torch.zeros(32, dtype=torch.float32, device="cuda")
     cmm._check(L.cmm_cnn_backward_weights(N, H, W, vp(dev16(g)), vp(xin), vp(dw), vp(db), stream()), "backward w")
