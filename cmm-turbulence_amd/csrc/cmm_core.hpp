@@ -230,7 +230,11 @@ CMM_HD void herm_vel(const DGrid &g, const PsiLevels &P, int nl, double x, doubl
     basis(c.dy, by);
     basis_d(c.dx, dbx);
     basis_d(c.dy, dby);
-    for (int l = 0; l < nl; ++l) {
+    // unrolled to the 4 levels a PsiLevels holds, so that u is written at constant indices (an
+    // array indexed at run time would go to private memory)
+#pragma unroll
+    for (int l = 0; l < 4; ++l) {
+        if (l >= nl) break;
         double v[16];
         load16(P.p[l], g.n, c, v);
         u[2 * l + 1] = -hsum_v(v, dbx, by, g.hx, g.hy) * g.ihx;

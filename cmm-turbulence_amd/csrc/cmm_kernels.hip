@@ -80,7 +80,9 @@ __global__ void k_map_identity(DGrid g, double *__restrict__ cx, double *__restr
 __device__ __forceinline__ void lsum(const double *w, const double *u, int nl, double &kx, double &ky) {
     kx = w[0] * u[0];
     ky = w[0] * u[1];
-    for (int l = 1; l < nl; ++l) {
+#pragma unroll
+    for (int l = 1; l < 4; ++l) {  // at most 4 levels; constant indices keep u in registers
+        if (l >= nl) break;
         kx += w[l] * u[2 * l];
         ky += w[l] * u[2 * l + 1];
     }
@@ -374,18 +376,21 @@ __device__ __forceinline__ void advect_store(const DGrid &gm, const cmm_advect_c
     const int base = (threadIdx.x & 63) - k;  // first lane of the group within the wavefront
     // every lane takes part in every shuffle (the same trip count for all lanes); the footpoint
     // loop is unrolled to its maximum so that the stencil weights are read at constant indices
-    // (a weight array indexed at run time would be copied to private memory)
+    // (a weight array indexed at run time would be copied to private memory).  The lane's plane
+    // picks its weight by 0/1 factors, not by a select, which the compiler can turn into an index
+    // into the four weight arrays at run time (0 * w + w' is w' exactly for finite weights).
 #pragma unroll
     for (int r = 0; r < (8 + G - 1) / G; ++r) {
         const int o = k + r * G;
         const int plane = o & 3;
+        const double f0 = plane == 0, f1 = plane == 1, f2 = plane == 2, f3 = plane == 3;
         double acc = 0.0;
 #pragma unroll
         for (int m = 0; m < kMaxFoot; ++m) {
             if (m < nfoot) {
                 const double vx = __shfl(px, base + m, 64);
                 const double vy = __shfl(py, base + m, 64);
-                const double wm = plane == 0 ? c.c1[m] : plane == 1 ? c.cx[m] : plane == 2 ? c.cy[m] : c.cxy[m];
+                const double wm = ((f0 * c.c1[m] + f1 * c.cx[m]) + f2 * c.cy[m]) + f3 * c.cxy[m];
                 acc += (o < 4 ? vx : vy) * wm;
             }
         }
