@@ -500,25 +500,29 @@ def run_c1(cmm, torch, cpu_threads=None):
     return out
 
 
-def ensemble_leg(cmm, torch, n_inst=32, steps=100, warmup=32):
+def ensemble_leg(cmm, torch, n_inst=32, steps=100, warmup=32, precision="fp64"):
     """The 32-instance Vlasov-Poisson ensemble for a bounded window of steps after `warmup`
     untimed steps (the graph batch captures its graphs in the first 12 steps; the autotuned batch
-    chooses its launch form by timing them in the first 29), FP64: the autotuned batch, the grouped
-    batch (every kind of launch of all instances in one launch) and the captured-graph batch
-    against the instances stepped one by one (cmm_euler_step each), each mode on freshly built,
-    identical engines over the same steps; saturated instances (stack full, remap skipped) are
-    counted."""
+    chooses its launch form by timing them in the first 29): the autotuned batch, the grouped
+    batch (every kind of launch of all instances in one launch; FP64 only) and the captured-graph
+    and eager batches against the instances stepped one by one (cmm_euler_step each), each mode on
+    freshly built, identical engines over the same steps; saturated instances (stack full, remap
+    skipped) are counted."""
     dt = 1.0 / 20
     specs = ensemble_specs(n_inst, 0)
     g = cmm.ensemble_grid
     out = {"workload": f"{n_inst} Vlasov-Poisson instances, maps 256^2/512^2/1024^2 (fine 4x, psi and vort 2x), "
-                       f"Landau / two-stream, amplitudes U[0.005, 0.05] seed 0, dt 1/20, steps "
+                       f"Landau / two-stream, amplitudes U[0.005, 0.05] seed 0, dt 1/20, {precision}, steps "
                        f"{warmup + 1}..{warmup + steps} after {warmup} untimed steps"}
-    for mode in ("autotuned_batch", "grouped_batch", "graph_batch", "eager_batch", "one_by_one"):
+    modes = ["autotuned_batch", "grouped_batch", "graph_batch", "eager_batch", "one_by_one"]
+    if precision != "fp64":
+        modes.remove("grouped_batch")
+    for mode in modes:
         es = []
         for nn, ic, amp in specs:
             e = cmm.Euler(g(nn), g(4 * nn), g(2 * nn), g(2 * nn), dt, initial_condition=ic, ic_params=[amp],
-                          simulation_type="cmm_vlasov_poisson_1d", incomp_threshold=1e-4, stack_capacity=64)
+                          simulation_type="cmm_vlasov_poisson_1d", incomp_threshold=1e-4, stack_capacity=64,
+                          precision=precision)
             e.init()
             es.append(e)
         b = (cmm.Batch(es, use_graphs=mode != "eager_batch", grouping=mode == "grouped_batch")
@@ -549,13 +553,8 @@ def ensemble_leg(cmm, torch, n_inst=32, steps=100, warmup=32):
         for e in es:
             e.close()
         torch.cuda.synchronize()
-    out["autotuned_batch_vs_one_by_one"] = round(out["autotuned_batch"]["ensemble_steps_per_s"] /
-                                                 out["one_by_one"]["ensemble_steps_per_s"], 3)
-    out["grouped_batch_vs_one_by_one"] = round(out["grouped_batch"]["ensemble_steps_per_s"] /
-                                               out["one_by_one"]["ensemble_steps_per_s"], 3)
-    out["graph_batch_vs_one_by_one"] = round(out["graph_batch"]["ensemble_steps_per_s"] /
-                                             out["one_by_one"]["ensemble_steps_per_s"], 3)
-    out["eager_batch_vs_one_by_one"] = round(out["eager_batch"]["ensemble_steps_per_s"] /
+    for mode in modes[:-1]:
+        out[f"{mode}_vs_one_by_one"] = round(out[mode]["ensemble_steps_per_s"] /
                                              out["one_by_one"]["ensemble_steps_per_s"], 3)
     log(f"ensemble: {out}")
     return out
@@ -575,8 +574,9 @@ def ensemble_specs(n_inst, seed):
 def run_ensemble(args):
     """Batched 1D-1V Vlasov-Poisson ensemble to T, in the FP64 and split forms: the batch's launch
     form (grouped launches, persistent grouped launches, captured graphs, eager queues) chosen by
-    the autotuner in the run's first steps; the rate against the other forms and against stepping
-    the instances one by one over a bounded window, and the split form's errors."""
+    the autotuner in the run's first steps, saturated instances counted; the rate of every form
+    against stepping the instances one by one over a bounded window of steps on freshly built
+    engines (ensemble_leg), and the split form's errors at T."""
     import numpy as np
     import torch
     import cmm
@@ -607,10 +607,11 @@ def run_ensemble(args):
         t0 = time.perf_counter()
         # the launch form chosen by timing in the run's first steps (part of the timed run)
         form = b.autotune(dt, 4)
-        remaps = 0
+        remaps = saturated = 0
         for i in range(es[0].state()[0], steps):
             infos = b.step(dt)
             remaps += sum(x.remapped for x in infos)
+            saturated += sum(x.saturated for x in infos)
             if (i + 1) % 200 == 0:
                 log(f"[ensemble {prec}] step {i + 1}/{steps}, remaps so far {remaps}")
         torch.cuda.synchronize()
@@ -619,39 +620,13 @@ def run_ensemble(args):
         b.close()
         fields[prec] = [(e.field("w_coarse"), e.field("psi")[:4 * (2 * n) ** 2]) for e, (n, _, _) in zip(es, specs)]
         out["forms"][prec] = {"ensemble_steps_per_s": round(steps / el, 3), "instance_steps_per_s": round(
-            steps * len(es) / el, 1), "remaps_after_autotune": remaps, "graph": st, "autotune": form,
-            "check_tiles": sorted({(t["tile"], t["max_blocks"]) for t in tunes})}
+            steps * len(es) / el, 1), "remaps_after_autotune": remaps, "saturated_steps": saturated, "graph": st,
+            "autotune": form, "check_tiles": sorted({(t["tile"], t["max_blocks"]) for t in tunes})}
         log(f"[ensemble {prec}] {steps} steps in {el:.2f}s: {out['forms'][prec]}")
-        # the same batch with captured graphs, without graphs (each instance's work queued on its
-        # own stream, one wait per ensemble step), then the instances stepped one by one
-        # (cmm_euler_step each, one wait per instance step), each for a bounded number of steps
-        ns = min(steps, 100)
-        b = cmm.Batch(es, use_graphs=True, grouping=False)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for i in range(ns):
-            b.step(dt)
-        torch.cuda.synchronize()
-        out["forms"][prec]["graph_batch_ensemble_steps_per_s"] = round(ns / (time.perf_counter() - t0), 3)
-        b.close()
-        b = cmm.Batch(es, use_graphs=False, grouping=False)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for i in range(ns):
-            b.step(dt)
-        torch.cuda.synchronize()
-        out["forms"][prec]["eager_batch_ensemble_steps_per_s"] = round(ns / (time.perf_counter() - t0), 3)
-        b.close()
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for i in range(ns):
-            for e in es:
-                e.step(dt)
-        torch.cuda.synchronize()
-        el1 = time.perf_counter() - t0
-        out["forms"][prec]["one_by_one_ensemble_steps_per_s"] = round(ns / el1, 3)
         for e in es:
             e.close()
+        # every launch form and the instances one by one, each on fresh engines over one window
+        out["forms"][prec]["windows"] = ensemble_leg(cmm, torch, n_inst=args.instances, precision=prec)
     rl2 = lambda a, b: float(np.linalg.norm(a - b) / np.linalg.norm(b))
     errs = [(rl2(fs[0], f6[0]), rl2(fs[1], f6[1])) for fs, f6 in zip(fields["split"], fields["fp64"])]
     out["split_vs_fp64_at_t_end"] = {"f_coarse_rel_l2_max": max(e[0] for e in errs),

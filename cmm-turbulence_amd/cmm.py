@@ -85,7 +85,8 @@ SYMBOLS = ["cmm_version", "cmm_last_error", "cmm_device_info", "cmm_lagrange_coe
            "cmm_map_to_split", "cmm_map_from_split", "cmm_hermite_to_split", "cmm_map_advect_split_workspace_bytes",
            "cmm_map_advect_split", "cmm_incomp_error_split", "cmm_psi_from_vort_split",
            "cmm_batch_create", "cmm_batch_destroy", "cmm_batch_step", "cmm_batch_stats", "cmm_batch_set_grouping", "cmm_batch_grouped_steps",
-           "cmm_batch_autotune", "cmm_batch_set_mode", "cmm_euler_autotune", "cmm_euler_set_check_fusion",
+           "cmm_batch_autotune", "cmm_batch_set_mode", "cmm_euler_autotune", "cmm_euler_set_check_tile",
+           "cmm_euler_set_check_fusion",
            "cmm_euler_get_check_fusion", "cmm_euler_set_psi_precombine",
            "cmm_euler_get_psi_precombine",
            "cmm_cnn_forward", "cmm_cnn_backward_data", "cmm_cnn_backward_data_relu", "cmm_cnn_backward_weights",
@@ -192,6 +193,7 @@ def lib():
         L.cmm_batch_autotune.argtypes = [VP, C.c_double, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double)]
         L.cmm_batch_set_mode.argtypes = [VP, C.c_int]
         L.cmm_euler_autotune.argtypes = [VP, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.cmm_euler_set_check_tile.argtypes = [VP, C.c_int, C.c_int]
         L.cmm_euler_set_check_fusion.argtypes = [VP, C.c_int]
         L.cmm_euler_get_check_fusion.argtypes = [VP, C.POINTER(C.c_int)]
         L.cmm_euler_set_psi_precombine.argtypes = [VP, C.c_int]
@@ -646,6 +648,12 @@ class Euler:
         st, t, ns = C.c_int(), C.c_double(), C.c_int()
         _check(lib().cmm_euler_state(self.h, C.byref(st), C.byref(t), C.byref(ns)), "cmm_euler_state")
         return st.value, t.value, ns.value
+
+    def set_check_tile(self, tile, max_blocks=8192):
+        """the check's tile edge and block cap, as the autotuner chooses them (E_inc does not
+        depend on the cap; tile edges round differently near 1e-14, and grouped batch steps need
+        one tile edge in all engines)"""
+        _check(lib().cmm_euler_set_check_tile(self.h, int(tile), int(max_blocks)), "cmm_euler_set_check_tile")
 
     def set_check_fusion(self, on):
         """the step's incompressibility check in the psi sampler's pass (on by default where possible)"""

@@ -1554,6 +1554,10 @@ bool group_ready(const cmm_euler *e, double dt) {
 
 int batch_normalise(cmm_euler *e);
 
+// whether the batch's engines can share grouped launches (op_group_vp_alike on what the engines'
+// parameters fix), checked before any engine records a step
+bool batch_group_alike(const cmm_batch *b);
+
 // One grouped step of every engine: each engine's step is run with its launches recorded
 // (step_queue_a/b with grp set), then op_group_vp_step launches all of them together on the master
 // stream, ordered after every engine stream's earlier work; the engines' streams are ordered after
@@ -1609,6 +1613,22 @@ int batch_lanes(cmm_batch *b) {
         }
     }
     return CMM_OK;
+}
+
+bool batch_group_alike(const cmm_batch *b) {
+    std::vector<GroupInstance> g(b->e.size());
+    for (size_t i = 0; i < b->e.size(); ++i) {
+        const cmm_euler *e = b->e[i];
+        g[i].gm = e->p.g_coarse;
+        g[i].gp = e->p.g_psi;
+        g[i].gv = e->p.g_vort;
+        g[i].gc = e->p.g_fine;
+        g[i].W4 = e->W4;
+        g[i].tune = e->tune;
+        g[i].c.time_integration_num = e->p.time_integration_num;
+        g[i].c.l_order = e->p.l_order;
+    }
+    return op_group_vp_alike(g.data(), (int)g.size());
 }
 
 int batch_step_grouped(cmm_batch *b, double dt, cmm_step_info *infos) {
@@ -1826,6 +1846,7 @@ int cmm_batch_step(cmm_batch *b, double dt, cmm_step_info *infos) {
     // grouped launches where every engine is ready for them
     bool grouped = b->grouping && (int)b->e.size() <= kGroupMax;
     for (cmm_euler *e : b->e) grouped = grouped && group_ready(e, dt);
+    grouped = grouped && batch_group_alike(b);
     if (grouped) return batch_step_grouped(b, dt, infos);
     // a graph replays the steady state: the nominal step of every engine, after its first step
     bool graph = b->graphs;
@@ -1921,6 +1942,7 @@ int cmm_batch_autotune(cmm_batch *b, double dt, int steps_per_mode, int *mode_ou
         ready = (int)b->e.size() <= kGroupMax;
         for (cmm_euler *e : b->e) ready = ready && group_ready(e, dt);
     }
+    ready = ready && batch_group_alike(b);
     std::vector<int> cand;
     if (ready) {
         cand.push_back(CMM_BATCH_GROUPED);
@@ -1965,6 +1987,16 @@ int cmm_batch_stats(cmm_batch *b, int *graphs_captured, int *graph_launches, int
     if (graphs_captured) *graphs_captured = b->captured;
     if (graph_launches) *graph_launches = b->launched;
     if (eager_steps) *eager_steps = b->eager;
+    return CMM_OK;
+}
+
+int cmm_euler_set_check_tile(cmm_euler *e, int tile, int max_blocks) {
+    if (!e || tile < 0 || tile > 4 || max_blocks < 1) {
+        set_error("cmm_euler_set_check_tile: bad argument (tile 0 to 4; max_blocks >= 1)");
+        return CMM_ERR_ARG;
+    }
+    e->tune.tile = tile;
+    e->tune.max_blocks = max_blocks;
     return CMM_OK;
 }
 

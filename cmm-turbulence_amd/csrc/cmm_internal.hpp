@@ -145,8 +145,12 @@ struct GroupInstance {
     size_t psi_work_bytes;
     int parts;  // recorded parts: 1 advection, 2 check, 4 psi
 };
-// whether n recorded instances can run in one grouped step (the same time scheme, Lagrange order,
-// check tile and sampler rows; the separable advection, the tensor check and the column sampler)
+// whether n instances can share grouped launches, from their grids, check tiles, time scheme and
+// Lagrange order alone (gm, gp, gv, gc, W4, tune, c.time_integration_num, c.l_order): the same
+// scheme, order, sampler rows and check tile edge
+bool op_group_vp_alike(const GroupInstance *inst, int n);
+// whether n recorded instances can run in one grouped step (op_group_vp_alike; the separable
+// advection, the tensor check and the column sampler, their workspaces)
 bool op_group_vp_eligible(const GroupInstance *inst, int n);
 size_t op_group_vp_job_bytes(int n);
 // The step of the n instances: every kind of launch of every instance in one launch, in the order

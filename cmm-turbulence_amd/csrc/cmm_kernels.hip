@@ -4367,15 +4367,27 @@ static int group_tile(const GroupInstance &a) {
     return incomp_tile(r, a.tune);
 }
 
-bool op_group_vp_eligible(const GroupInstance *inst, int n) {
+// The grouped check runs every instance with one tile edge, and the tile kernels round the
+// determinant near 1 differently (about 1e-14), so the instances' own tiles must agree for each to
+// get its own launch's value.
+bool op_group_vp_alike(const GroupInstance *inst, int n) {
     if (n < 1 || n > kGroupMax) return false;
+    const int R = group_rows(inst[0]), TT = group_tile(inst[0]);
+    if (R == 0 || TT == 0) return false;
     for (int i = 0; i < n; ++i) {
         const GroupInstance &a = inst[i];
-        if (a.parts != 7 || !a.m8_in || !a.H || !op_vp_advect_supported(a.gm, a.gp, a.c) || vp_check(a.gv, a.gp) ||
-            group_rows(a) == 0 || group_tile(a) == 0)
+        if (group_rows(a) != R || group_tile(a) != TT) return false;
+        if (a.c.time_integration_num != inst[0].c.time_integration_num || a.c.l_order != inst[0].c.l_order)
             return false;
-        if (a.c.time_integration_num != inst[0].c.time_integration_num || a.c.l_order != inst[0].c.l_order ||
-            group_rows(a) != group_rows(inst[0]) || group_tile(a) != group_tile(inst[0]))
+    }
+    return true;
+}
+
+bool op_group_vp_eligible(const GroupInstance *inst, int n) {
+    if (!op_group_vp_alike(inst, n)) return false;
+    for (int i = 0; i < n; ++i) {
+        const GroupInstance &a = inst[i];
+        if (a.parts != 7 || !a.m8_in || !a.H || !op_vp_advect_supported(a.gm, a.gp, a.c) || vp_check(a.gv, a.gp))
             return false;
         // the workspaces of op_map_advect_vp (with its interleaved input map) and op_vp_psi_from_f
         if (a.adv_work_bytes < sizeof(double) * (6 * (size_t)a.gp.nx + 2) ||

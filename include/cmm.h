@@ -428,6 +428,12 @@ int cmm_batch_set_mode(cmm_batch *b, int mode);
 /* Autotuner: times the incompressibility check's tile edge (4, 2, 1 dividing the fine/coarse
  * ratio) and block cap on the engine's current map and keeps the fastest for its later steps. */
 int cmm_euler_autotune(cmm_euler *e, int *tile_out, int *max_blocks_out, double *ms_out);
+/* Sets the check's tile edge (1 to 4; 0 or one that does not divide the fine/coarse ratio: the
+ * largest of 4, 3, 2, 1 that does) and block cap, as cmm_euler_autotune chooses them (to replay a
+ * tuned launch shape). E_inc does not depend on the cap; tile edges round the determinant near 1
+ * differently (about 1e-14). Grouped batch steps need the same tile edge in every engine; a batch
+ * whose engines differ steps in its other forms. */
+int cmm_euler_set_check_tile(cmm_euler *e, int tile, int max_blocks);
 /* Whether a step's incompressibility check (Cuda-CMM/src/simulation/cmm-euler2d.cu:796) runs in the
  * pass of the psi solve's vorticity sampler instead of a kernel of its own: possible when the check
  * grid is the vorticity grid, refining the map grid by a multiple of 4 over the same domain, in an

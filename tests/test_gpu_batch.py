@@ -137,6 +137,52 @@ def run_pair(cmm, specs, dt, dts, precision="fp64", grid_of=None, grouping=False
     return alone, ref, batched, got, b
 
 
+def test_grouped_batch_with_set_and_mixed_check_tiles(dev):
+    """Engines with one tile edge set for their checks (different block caps) step in grouped
+    launches bit for bit as alone; engines whose tile edges differ (as separately autotuned engines
+    can) or whose time schemes differ step in the other forms, decided before any engine records a
+    step, again bit for bit as alone."""
+    import cmm
+    dt = 1.0 / 16
+    steps = 24
+    g = cmm.ensemble_grid
+    tiles = [(2, 256), (2, 1024), (2, 64), (2, 8192)]
+
+    def make_t(n, ic, amp, tile, cap, scheme="RK3"):
+        e = cmm.Euler(g(n), g(4 * n), g(2 * n), g(2 * n), dt, initial_condition=ic, ic_params=[amp],
+                      simulation_type="cmm_vlasov_poisson_1d", incomp_threshold=1e-4, stack_capacity=32,
+                      freq_cut_psi=1e9, time_integration=scheme)
+        e.init()
+        e.set_check_tile(tile, cap)
+        return e
+
+    alone = [make_t(*sp, *t) for sp, t in zip(SPECS, tiles)]
+    ref = [[e.step(dt) for _ in range(steps)] for e in alone]
+    batched = [make_t(*sp, *t) for sp, t in zip(SPECS, tiles)]
+    b = cmm.Batch(batched, use_graphs=True, grouping=True)
+    got = [b.step(dt) for _ in range(steps)]
+    print("batch stats", b.stats())
+    assert b.stats()["grouped_steps"] >= steps - 1
+    assert_same(alone, ref, batched, got, steps)
+    b.close()
+    # unlike engines: no grouped step, the same results as alone
+    unlike = [[(SPECS[0], 4, "RK3"), (SPECS[1], 2, "RK3"), (SPECS[2], 1, "RK3")],
+              [(SPECS[0], 4, "RK3"), (SPECS[1], 4, "RK4")]]
+    others = []
+    for cfg in unlike:
+        mixed = [make_t(*sp, t, 256, scheme=sc) for sp, t, sc in cfg]
+        solo = [make_t(*sp, t, 256, scheme=sc) for sp, t, sc in cfg]
+        b2 = cmm.Batch(mixed, use_graphs=True, grouping=True)
+        got2 = [b2.step(dt) for _ in range(8)]
+        ref2 = [[e.step(dt) for _ in range(8)] for e in solo]
+        assert b2.stats()["grouped_steps"] == 0
+        assert_same(solo, ref2, mixed, got2, 8)
+        b2.close()
+        others += mixed + solo
+    for e in alone + batched + others:
+        e.close()
+
+
 def assert_same(alone, ref, batched, got, steps):
     for k, (ea, eb) in enumerate(zip(alone, batched)):
         einc_a = np.array([r.incomp for r in ref[k]])
