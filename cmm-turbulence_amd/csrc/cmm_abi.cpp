@@ -629,7 +629,15 @@ int engine_advect(cmm_euler *e, const double *t, const double *dt, int loop_ctr,
     const PsiPre *pre_in = take_pre(e, c, split(e) && op_advect32_supported(c) ? 1 : 0);
     if (split(e) && op_advect32_supported(c)) {
         if (int rc = sync_m32(e)) return rc;
-        if (!e->dry)
+        // the Vlasov-Poisson flow: the velocity from the separable psi levels, as the FP64 form
+        const bool sep = e->p.sim == CMM_SIM_VLASOV_POISSON_1D && e->vpH && !pre_in &&
+                         op_vp_advect_supported(e->p.g_coarse, e->p.g_psi, c) &&
+                         !(e->initialised && (e->corr1 || e->corr12));
+        if (sep && !e->dry) {
+            if (int rc = op_map_advect32_vp(e->p.g_coarse, e->p.g_psi, P, c, e->m32[e->m32_cur], e->m32[1 - e->m32_cur],
+                                            e->vpH, e->work, e->work_bytes, e->st, e->prof ? e->ev[10] : nullptr, lw))
+                return rc;
+        } else if (!e->dry)
             // the Vlasov-Poisson stream function -H(v) + phi(x) has a large smooth part in v: its
             // combined fields stay FP64 records; Euler's are packed FP32 records keeping the values'
             // low parts

@@ -175,6 +175,10 @@ int op_map_advect_vp(const cmm_grid &gm, const cmm_grid &gp, const PsiLevels &P,
                      const double *cx, const double *cy, const double *m8_in, double *m8_out, const double *H,
                      double *ncx, double *ncy, void *work, size_t work_bytes, hipStream_t st, hipEvent_t mid,
                      const double *lw);
+// the same separable advection of a split map (identity + FP32 displacement records)
+int op_map_advect32_vp(const cmm_grid &gm, const cmm_grid &gp, const PsiLevels &P, const cmm_advect_coeffs &c,
+                       const float *M_in, float *M_out, const double *H, void *work, size_t work_bytes,
+                       hipStream_t st, hipEvent_t mid, const double *lw);
 int op_spectral_from_real(const cmm_grid &g, const double *real, int op, double *herm, void *work, size_t work_bytes,
                           hipStream_t st);
 int op_sample_vort(const cmm_grid &gm, const cmm_grid &gt, const cmm_grid &gf, const double *cx, const double *cy,

@@ -3623,6 +3623,65 @@ struct VelPair32 {
     }
 };
 
+// k_map_advect32_pair's block body with the velocity source V (k_map_advect32_pair keeps its own
+// copy: as a shared body the compiler schedules it a few per cent slower)
+template <int TIN, class V>
+__device__ __forceinline__ void advect32_pair_block(int blk, const DGrid &gm, const V &vel,
+                                                    const float4 *__restrict__ M, const cmm_advect_coeffs &c,
+                                                    float *__restrict__ Mout) {
+    constexpr int G = 4, NPB = 256 / G;
+    const int node = blk * NPB + threadIdx.x / G;
+    const int k = threadIdx.x % G;
+    const bool active = node < gm.n;
+    PointPair qx{0.0, 0.0}, qy{0.0, 0.0};
+    if (active) {
+        const int ix = node % gm.nx, iy = node / gm.nx;
+        const double xn = gm.x0 + ix * gm.hx, yn = gm.y0 + iy * gm.hy;
+        const int fa = k, fb = k + 4;
+        const PointPair xe{xn + ((fa >> 2) + 1) * c.eps * (1 - 2 * ((fa >> 1) & 1)),
+                           xn + ((fb >> 2) + 1) * c.eps * (1 - 2 * ((fb >> 1) & 1))};
+        const PointPair ye{yn + ((fa >> 2) + 1) * c.eps * (1 - 2 * (((fa + 1) >> 1) & 1)),
+                           yn + ((fb >> 2) + 1) * c.eps * (1 - 2 * (((fb + 1) >> 1) & 1))};
+        PointPair xf, yf;
+        step_backward<TIN>(vel, c, xe, ye, xf, yf);
+        Cell ca, cb;
+        find_cell<true>(gm, xf.a, yf.a, ca);
+        find_cell<true>(gm, xf.b, yf.b, cb);
+        double vx[16], vy[16], px, py;
+        load_map32(M, ca, vx, vy);
+        diffeo32_vals(gm, ca, vx, vy, xf.a, yf.a, px, py);
+        qx.a = px - xn;
+        qy.a = py - yn;
+        if (!same_cell(ca, cb)) load_map32(M, cell_after(cb, qx.a), vx, vy);
+        diffeo32_vals(gm, cb, vx, vy, xf.b, yf.b, px, py);
+        qx.b = px - xn;
+        qy.b = py - yn;
+    }
+    // output planes as advect_store32: footpoint m in lane m & 3, component m >> 2
+    const int base = (threadIdx.x & 63) - k;
+#pragma unroll
+    for (int r = 0; r < 2; ++r) {
+        const int o = k + r * G;
+        const int plane = o & 3;
+        double acc = 0.0;
+#pragma unroll
+        for (int m = 0; m < 8; ++m) {
+            const double vx = __shfl(m < 4 ? qx.a : qx.b, base + (m & 3), 64);
+            const double vy = __shfl(m < 4 ? qy.a : qy.b, base + (m & 3), 64);
+            const double wm = plane == 0 ? c.c1[m] : plane == 1 ? c.cx[m] : plane == 2 ? c.cy[m] : c.cxy[m];
+            acc += (o < 4 ? vx : vy) * wm;
+        }
+        if (o == 1 || o == 6) acc -= 1.0;
+        if (active) {
+            const float hi = (float)acc;
+            float *rec = Mout + 12L * node;
+            rec[split_slot(o)] = hi;
+            if (plane == 0) rec[split_slot(o) + 1] = (float)(acc - (double)hi);
+            else if (plane == 3) rec[10 + (o >> 2)] = 0.0f;
+        }
+    }
+}
+
 template <int TIN, bool PK>
 __global__ void __launch_bounds__(256)
 k_map_advect32_pair(DGrid gm, DGrid gp, const float4 *C1, const float4 *C12, const float4 *C0,
@@ -3680,6 +3739,15 @@ k_map_advect32_pair(DGrid gm, DGrid gp, const float4 *C1, const float4 *C12, con
             else if (plane == 3) rec[10 + (o >> 2)] = 0.0f;
         }
     }
+}
+
+// the split map of a Vlasov-Poisson flow advected with the separable velocity (k_map_advect_vp's
+// velocity from k_vp_combine1d's tables, the split records as k_map_advect32_pair)
+template <int TIN>
+__global__ void __launch_bounds__(256) k_map_advect32_vp(DGrid gm, DGrid gp, const double *T, const double *S,
+                                                         const double *H, const float4 *__restrict__ M,
+                                                         cmm_advect_coeffs c, float *__restrict__ Mout, int nblocks) {
+    advect32_pair_block<TIN>(xcd_block(blockIdx.x, nblocks), gm, VelSep{gp, T, S, H}, M, c, Mout);
 }
 
 // k_incomp_tensor on the split map: the same separable sums over the displacement records
@@ -4083,6 +4151,42 @@ int op_map_advect32(const cmm_grid &gm, const cmm_grid &gp, const PsiLevels &P, 
 #undef CMM_A32P
 #undef CMM_A32
     return launch_check("k_map_advect32");
+}
+
+int op_map_advect32_vp(const cmm_grid &gm, const cmm_grid &gp, const PsiLevels &P, const cmm_advect_coeffs &c,
+                       const float *M_in, float *M_out, const double *H, void *work, size_t work_bytes,
+                       hipStream_t st, hipEvent_t mid, const double *lw) {
+    if (!op_vp_advect_supported(gm, gp, c)) {
+        set_error("separable Vlasov-Poisson advection: needs the 8-footpoint stencil, a combinable scheme and "
+                  "power-of-two grids");
+        return CMM_ERR_ARG;
+    }
+    const DGrid dm = make_dgrid(gm), dp = make_dgrid(gp);
+    void *w;
+    if (int rc = get_work(work, work_bytes, sizeof(double) * (6 * (size_t)gp.nx + 2), &w)) return rc;
+    double *T = static_cast<double *>(w), *S = T + 6 * gp.nx;
+    const dim3 g1((unsigned)((gp.nx + 255) / 256));
+    switch (c.l_order) {
+    case 2: hipLaunchKernelGGL(k_vp_combine1d<2>, g1, dim3(256), 0, st, gp.nx, (long)dp.n, P, c, lw, T, S); break;
+    case 3: hipLaunchKernelGGL(k_vp_combine1d<3>, g1, dim3(256), 0, st, gp.nx, (long)dp.n, P, c, lw, T, S); break;
+    default: hipLaunchKernelGGL(k_vp_combine1d<4>, g1, dim3(256), 0, st, gp.nx, (long)dp.n, P, c, lw, T, S); break;
+    }
+    if (int rc = launch_check("k_vp_combine1d")) return rc;
+    if (mid) (void)hipEventRecord(mid, st);
+    const int nb = (int)(((long)dm.n * 4 + 255) / 256);
+    const float4 *Mi = reinterpret_cast<const float4 *>(M_in);
+#define CMM_A32VP(T_)                                                                                                  \
+    hipLaunchKernelGGL(k_map_advect32_vp<T_>, dim3(nb), dim3(256), 0, st, dm, dp, T, S, H, Mi, c, M_out, nb)
+    switch (c.time_integration_num) {
+    case 10: CMM_A32VP(10); break;
+    case 21: CMM_A32VP(21); break;
+    case 30: CMM_A32VP(30); break;
+    case 40: CMM_A32VP(40); break;
+    case 41: CMM_A32VP(41); break;
+    default: set_error("unknown time integration"); return CMM_ERR_ARG;
+    }
+#undef CMM_A32VP
+    return launch_check("k_map_advect32_vp");
 }
 
 int op_incomp32_launch(const cmm_grid &gm, const cmm_grid &gc, const float *Mf, unsigned long long *slot,
