@@ -36,36 +36,55 @@ __device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float
 __device__ __forceinline__ int wrapi(int v, int n) { return v < 0 ? v + n : v >= n ? v - n : v; }
 
 // out[n][y][x][co] = act(b[co] + sum_{t, ci} W[co][t][ci] in[n][y + dy_t][x + dx_t][ci]), tap
-// t = 3 (dy + 1) + (dx + 1), periodic in y and x.  A wave takes SEGS consecutive 32-pixel
-// segments of a row: M = output channels, N = pixels, K = 9 taps x 32 channels (18 MFMAs per
-// segment, the bias as the accumulators' start); the wave's A operands (its 18 weight fragments)
-// stay in registers across the segments (4 for large batches, 1 where fewer waves would leave the
-// device idle).  MODE 0: bf16 out with ReLU, 1: bf16 out, 2: fp32 out, 3: bf16 out, zero where
-// mask (bf16, same layout as out) is not positive.  The backward-data pass is this kernel on the output gradient with the weights
+// t = 3 (dy + 1) + (dx + 1), periodic in y and x.  A block takes kCvRows rows of up to kCvSegs
+// 32-pixel segments: its input rows y0 - 1 .. y0 + kCvRows (with the periodic halo pixels) go
+// into LDS once, with coalesced 16-byte loads, at a pixel pitch of 80 bytes (the 16 lanes of a
+// 16-byte LDS read pass then cover all 64 banks once).  Wave w computes row y0 + w: per segment
+// M = output channels, N = pixels, K = 9 taps x 32 channels (18 MFMAs, B from LDS), the bias as
+// the accumulators' start; its A operands (18 weight fragments) stay in registers.  A lane's 16
+// results are pixel x0 + i, output channels 8 g + 4 h .. 8 g + 4 h + 3 for g = 0..3: four runs of
+// 4 consecutive channels, written with 8-byte (16-byte for FP32) stores.  MODE 0: bf16 out with
+// ReLU, 1: bf16 out, 2: fp32 out, 3: bf16 out, zero where mask (bf16, same layout as out) is not
+// positive.  The backward-data pass is this kernel on the output gradient with the weights
 // transposed in channels and flipped in taps; mode 3 also applies the ReLU derivative of the layer
 // below.
-template <int MODE, int SEGS>
+constexpr int kCvRows = 4;                   // output rows per block (one per wave)
+constexpr int kCvSegs = 4;                   // 32-pixel segments per block and row
+constexpr int kCvPix = 32 * kCvSegs + 2;     // staged pixels per row
+constexpr int kCvPitch = 40;                 // LDS elements per staged pixel (32 channels + 8 pad)
+
+template <int MODE>
 __global__ void __launch_bounds__(256) k_conv3(const unsigned short *__restrict__ in, const unsigned short *__restrict__ W,
                                                const float *__restrict__ bias, void *__restrict__ out, int H, int Wd,
-                                               int segs, const unsigned short *__restrict__ mask) {
-    const int lane = threadIdx.x & 63;
-    const int wave = blockIdx.x * 4 + (threadIdx.x >> 6);
-    const int groups = segs / SEGS;  // segment groups per row (segs is a multiple of SEGS)
-    if (wave >= H * groups) return;      // uniform over the wave
+                                               int groups, const unsigned short *__restrict__ mask) {
+    __shared__ __align__(16) unsigned short tile[(kCvRows + 2) * kCvPix * kCvPitch];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int y0 = (blockIdx.x / groups) * kCvRows;
+    const int x0 = (blockIdx.x % groups) * (32 * kCvSegs);
+    const int nseg = min(kCvSegs, (Wd - x0) / 32);
+    const int npix = 32 * nseg + 2;
     const long img = (long)blockIdx.y * H * Wd;
-    const int y = wave / groups, xg = (wave % groups) * (32 * SEGS);
+    // stage rows y0 - 1 .. y0 + kCvRows, pixels x0 - 1 .. x0 + 32 nseg: chunk c is 8 channels
+    const int nchunk = (kCvRows + 2) * npix * 4;
+    for (int c = threadIdx.x; c < nchunk; c += 256) {
+        const int q = c & 3, px = (c >> 2) % npix, r = (c >> 2) / npix;
+        const int yy = ((y0 - 1 + r) % H + H) % H;
+        const int xx = wrapi(x0 - 1 + px, Wd);
+        const uint4 v = *reinterpret_cast<const uint4 *>(in + (img + (long)yy * Wd + xx) * kCh + 8 * q);
+        *reinterpret_cast<uint4 *>(tile + (r * kCvPix + px) * kCvPitch + 8 * q) = v;
+    }
     const int i = lane & 31, h = lane >> 5;
     // A = weights (row m = output channel i), B = input (column n = pixel x0 + i); operand slot e of
-    // lane (i, h) in K-block kb of tap t: input channel 16 kb + 8 h + e in both.  The lane's 16
-    // results are pixel x0 + i, output channels 8 g + 4 h .. 8 g + 4 h + 3 for g = 0..3: four runs
-    // of 4 consecutive channels, written with 8-byte (16-byte for FP32) stores.
+    // lane (i, h) in K-block kb of tap t: input channel 16 kb + 8 h + e in both
     bf16x8 wf[18];
 #pragma unroll
     for (int q = 0; q < 18; ++q)
         wf[q] = *reinterpret_cast<const bf16x8 *>(W + ((long)i * 9 + q / 2) * kCh + 8 * h + 16 * (q % 2));
-    for (int sg = 0; sg < SEGS; ++sg) {
-        const int x0 = xg + 32 * sg;
-        const long o = (img + (long)y * Wd + x0 + i) * kCh + 4 * h;  // + 8 g: channel run g
+    __syncthreads();
+    const int y = y0 + wid;
+    if (y >= H) return;  // uniform over the wave; no barrier follows
+    for (int sg = 0; sg < nseg; ++sg) {
+        const long o = (img + (long)y * Wd + x0 + 32 * sg + i) * kCh + 4 * h;  // + 8 g: channel run g
         f32x16 acc;
 #pragma unroll
         for (int g = 0; g < 4; ++g) {
@@ -81,8 +100,8 @@ __global__ void __launch_bounds__(256) k_conv3(const unsigned short *__restrict_
             for (int g = 0; g < 4; ++g) mk[g] = *reinterpret_cast<const uint2 *>(mask + o + 8 * g);
 #pragma unroll
         for (int t = 0; t < 9; ++t) {
-            const int yy = wrapi(y + t / 3 - 1, H), xx = wrapi(x0 + i + t % 3 - 1, Wd);
-            const unsigned short *bp = in + (img + (long)yy * Wd + xx) * kCh + 8 * h;
+            // staged row wid + 1 + dy, staged pixel 32 sg + i + dx + 1
+            const unsigned short *bp = tile + ((wid + t / 3) * kCvPix + 32 * sg + i + t % 3) * kCvPitch + 8 * h;
 #pragma unroll
             for (int kb = 0; kb < 2; ++kb) {
                 const bf16x8 bx = *reinterpret_cast<const bf16x8 *>(bp + 16 * kb);
@@ -419,23 +438,13 @@ extern "C" {
 namespace {
 int conv3(int n_img, int H, int W, const uint16_t *in, const uint16_t *w, const float *b, int mode, void *out,
           const uint16_t *mask, void *stream) {
-    const int segs = W / 32;
-    // four segments per wave when that still gives 2048 waves or more
-    const bool four = segs % 4 == 0 && (long)n_img * H * (segs / 4) >= 2048;
-    const dim3 grid((unsigned)((H * (four ? segs / 4 : segs) + 3) / 4), (unsigned)n_img);
-#define CMM_CONV(M, SG) \
-    hipLaunchKernelGGL((k_conv3<M, SG>), grid, dim3(256), 0, S(stream), in, w, b, out, H, W, segs, mask)
-#define CMM_CONV_MODES(SG)            \
-    if (mode == 0) CMM_CONV(0, SG);   \
-    else if (mode == 1) CMM_CONV(1, SG); \
-    else if (mode == 2) CMM_CONV(2, SG); \
-    else CMM_CONV(3, SG);
-    if (four) {
-        CMM_CONV_MODES(4)
-    } else {
-        CMM_CONV_MODES(1)
-    }
-#undef CMM_CONV_MODES
+    const int groups = (W / 32 + kCvSegs - 1) / kCvSegs;  // segment groups per row
+    const dim3 grid((unsigned)(((H + kCvRows - 1) / kCvRows) * groups), (unsigned)n_img);
+#define CMM_CONV(M) hipLaunchKernelGGL((k_conv3<M>), grid, dim3(256), 0, S(stream), in, w, b, out, H, W, groups, mask)
+    if (mode == 0) CMM_CONV(0);
+    else if (mode == 1) CMM_CONV(1);
+    else if (mode == 2) CMM_CONV(2);
+    else CMM_CONV(3);
 #undef CMM_CONV
     return lc("k_conv3");
 }
