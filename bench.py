@@ -210,6 +210,30 @@ def summarize_profiles(kt_dir, fetch_dir, write_dir, out_dir, commands):
     for r in sorted(stats, key=lambda r: -float(r["TotalDurationNs"])):
         lines.append(f"{r['Name'][:78]:78s} {int(r['Calls']):6d} {float(r['TotalDurationNs']) / 1e6:10.3f} "
                      f"{float(r['AverageNs']) / 1e3:10.2f} {float(r['Percentage']):6.2f}")
+    # per slot, from the kernel trace: each kernel's launches at its most frequent grid size (the
+    # time loop's), their average duration, summed over the slot's kernels (the bench's slot times,
+    # taken with events, also hold the gaps between the kernels)
+    dur = collections.defaultdict(list)
+    for r in _read_csv(_find(kt_dir, "kernel_trace.csv")):
+        key = (r["Kernel_Name"], r["Grid_Size_X"], r["Grid_Size_Y"])
+        dur[key].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+    best = {}
+    for (k, _, _), v in dur.items():
+        if k not in best or len(v) > len(best[k]):
+            best[k] = v
+    lines += ["", "slots from the kernel trace (each kernel's launches at its most frequent grid size; average us, "
+              "the slot's kernels summed)", f"{'slot':22s} {'kernels':64s} {'launches':>8s} {'avg_us':>10s}"]
+    for slot, pats in SLOT_KERNELS.items():
+        parts = []
+        for pat in pats:
+            ks = [k for k in best if pat in k]
+            if not ks:
+                break
+            v = best[ks[0]]
+            parts.append((ks[0].split("(")[0].replace("void ", "").replace("cmm::", ""), len(v), sum(v) / len(v)))
+        else:
+            lines.append(f"{slot:22s} {' + '.join(p[0] for p in parts)[:64]:64s} {min(p[1] for p in parts):8d} "
+                         f"{sum(p[2] for p in parts):10.2f}")
     with open(os.path.join(out_dir, "cmm_euler_c2_kernel_stats.txt"), "w") as f:
         f.write("\n".join(lines) + "\n")
 
@@ -395,8 +419,11 @@ def roofline(c, prof, n_stack, suffix="", fused=False, pre=False):
         tr = json.load(open(TRAFFIC_FILE))["slots"].get(d["model"])
         if tr:
             traffic = int(tr["fetch_bytes"] + tr["write_bytes"])
-    roof.update({"kernel": d["model"], "traffic": traffic, "ms_avg": round(d["ms_avg"], 5),
-                 "algorithmic_bytes": d["bytes"], "algorithmic_flops": d["flops"]})
+    # the slot's kernels (its events also hold the launches and gaps between them: the check's
+    # clearing launch and E_inc copy in the fused sampler's slot)
+    roof.update({"kernel": d["model"], "kernels": [k.split("(")[0] for k in SLOT_KERNELS.get(d["model"], ())],
+                 "traffic": traffic, "ms_avg": round(d["ms_avg"], 5), "algorithmic_bytes": d["bytes"],
+                 "algorithmic_flops": d["flops"]})
     return roof, per
 
 

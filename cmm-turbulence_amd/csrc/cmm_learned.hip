@@ -36,7 +36,7 @@ __device__ __forceinline__ float bf2f(unsigned short h) { return __uint_as_float
 __device__ __forceinline__ int wrapi(int v, int n) { return v < 0 ? v + n : v >= n ? v - n : v; }
 
 // out[n][y][x][co] = act(b[co] + sum_{t, ci} W[co][t][ci] in[n][y + dy_t][x + dx_t][ci]), tap
-// t = 3 (dy + 1) + (dx + 1), periodic in y and x.  A block takes kCvRows rows of up to kCvSegs
+// t = 3 (dy + 1) + (dx + 1), periodic in y and x.  A block takes kCvRows rows of up to SEGS
 // 32-pixel segments: its input rows y0 - 1 .. y0 + kCvRows (with the periodic halo pixels) go
 // into LDS once, with coalesced 16-byte loads, at a pixel pitch of 80 bytes (the 16 lanes of a
 // 16-byte LDS read pass then cover all 64 banks once).  Wave w computes row y0 + w: per segment
@@ -48,20 +48,20 @@ __device__ __forceinline__ int wrapi(int v, int n) { return v < 0 ? v + n : v >=
 // positive.  The backward-data pass is this kernel on the output gradient with the weights
 // transposed in channels and flipped in taps; mode 3 also applies the ReLU derivative of the layer
 // below.
-constexpr int kCvRows = 4;                   // output rows per block (one per wave)
-constexpr int kCvSegs = 4;                   // 32-pixel segments per block and row
-constexpr int kCvPix = 32 * kCvSegs + 2;     // staged pixels per row
-constexpr int kCvPitch = 40;                 // LDS elements per staged pixel (32 channels + 8 pad)
+constexpr int kCvRows = 4;    // output rows per block (one per wave)
+constexpr int kCvPitch = 40;  // LDS elements per staged pixel (32 channels + 8 pad)
 
-template <int MODE>
+// SEGS: 32-pixel segments per block and row (4, or 1 where 4 would leave too few blocks)
+template <int MODE, int SEGS>
 __global__ void __launch_bounds__(256) k_conv3(const unsigned short *__restrict__ in, const unsigned short *__restrict__ W,
                                                const float *__restrict__ bias, void *__restrict__ out, int H, int Wd,
                                                int groups, const unsigned short *__restrict__ mask) {
+    constexpr int kCvPix = 32 * SEGS + 2;  // staged pixels per row
     __shared__ __align__(16) unsigned short tile[(kCvRows + 2) * kCvPix * kCvPitch];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     const int y0 = (blockIdx.x / groups) * kCvRows;
-    const int x0 = (blockIdx.x % groups) * (32 * kCvSegs);
-    const int nseg = min(kCvSegs, (Wd - x0) / 32);
+    const int x0 = (blockIdx.x % groups) * (32 * SEGS);
+    const int nseg = min(SEGS, (Wd - x0) / 32);
     const int npix = 32 * nseg + 2;
     const long img = (long)blockIdx.y * H * Wd;
     // stage rows y0 - 1 .. y0 + kCvRows, pixels x0 - 1 .. x0 + 32 nseg: chunk c is 8 channels
@@ -438,13 +438,24 @@ extern "C" {
 namespace {
 int conv3(int n_img, int H, int W, const uint16_t *in, const uint16_t *w, const float *b, int mode, void *out,
           const uint16_t *mask, void *stream) {
-    const int groups = (W / 32 + kCvSegs - 1) / kCvSegs;  // segment groups per row
-    const dim3 grid((unsigned)(((H + kCvRows - 1) / kCvRows) * groups), (unsigned)n_img);
-#define CMM_CONV(M) hipLaunchKernelGGL((k_conv3<M>), grid, dim3(256), 0, S(stream), in, w, b, out, H, W, groups, mask)
-    if (mode == 0) CMM_CONV(0);
-    else if (mode == 1) CMM_CONV(1);
-    else if (mode == 2) CMM_CONV(2);
-    else CMM_CONV(3);
+    // 4 segments per block where that still gives 1024 blocks or more
+    const long rows = (H + kCvRows - 1) / kCvRows;
+    const bool four = rows * ((W / 32 + 3) / 4) * n_img >= 1024;
+    const int groups = four ? (W / 32 + 3) / 4 : W / 32;  // segment groups per row
+    const dim3 grid((unsigned)(rows * groups), (unsigned)n_img);
+#define CMM_CONV(M, SG) \
+    hipLaunchKernelGGL((k_conv3<M, SG>), grid, dim3(256), 0, S(stream), in, w, b, out, H, W, groups, mask)
+#define CMM_CONV_MODES(SG)               \
+    if (mode == 0) CMM_CONV(0, SG);      \
+    else if (mode == 1) CMM_CONV(1, SG); \
+    else if (mode == 2) CMM_CONV(2, SG); \
+    else CMM_CONV(3, SG);
+    if (four) {
+        CMM_CONV_MODES(4)
+    } else {
+        CMM_CONV_MODES(1)
+    }
+#undef CMM_CONV_MODES
 #undef CMM_CONV
     return lc("k_conv3");
 }
