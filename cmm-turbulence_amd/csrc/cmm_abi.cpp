@@ -1967,12 +1967,21 @@ int cmm_batch_autotune(cmm_batch *b, double dt, int steps_per_mode, int *mode_ou
             for (int i = 0; i < 12; ++i)
                 if (int rc = cmm_batch_step(b, dt, nullptr)) return rc;
         if (int rc = batch_sync(b)) return rc;
-        const auto t0 = std::chrono::steady_clock::now();
-        for (int i = 0; i < steps_per_mode; ++i)
+        // each step's time (a step returns once every engine's E_inc is in and its remap queued);
+        // the median keeps a step that remaps from deciding the choice
+        std::vector<double> t(steps_per_mode);
+        auto t0 = std::chrono::steady_clock::now();
+        for (int i = 0; i < steps_per_mode; ++i) {
             if (int rc = cmm_batch_step(b, dt, nullptr)) return rc;
-        if (int rc = batch_sync(b)) return rc;
-        const double ms =
-            std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count() / steps_per_mode;
+            if (i == steps_per_mode - 1)
+                if (int rc = batch_sync(b)) return rc;
+            const auto t1 = std::chrono::steady_clock::now();
+            t[i] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+            t0 = t1;
+        }
+        std::sort(t.begin(), t.end());
+        const int m = steps_per_mode / 2;
+        const double ms = steps_per_mode % 2 ? t[m] : 0.5 * (t[m - 1] + t[m]);
         if (ms < best_ms) {
             best_ms = ms;
             best = mode;

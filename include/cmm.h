@@ -419,7 +419,8 @@ int cmm_batch_grouped_steps(cmm_batch *b, int *grouped_steps);
  * persistent ones of 2048 workgroups, or grouped launches per lane of engines, 4 lanes of about
  * equal work on their own streams), captured graphs (when the batch was created with them) and
  * eager queues each run steps_per_mode real steps of length dt (the run advances; graphs first
- * capture theirs in 12 untimed steps); the fastest stays.  Every form gives the same results. */
+ * capture theirs in 12 untimed steps); the form with the shortest median step stays (ms_out: that
+ * median).  Every form gives the same results. */
 enum { CMM_BATCH_EAGER = 0, CMM_BATCH_GRAPHS = 1, CMM_BATCH_GROUPED = 2, CMM_BATCH_GROUPED_PERSISTENT = 3,
        CMM_BATCH_GROUPED_LANES = 4 };
 int cmm_batch_autotune(cmm_batch *b, double dt, int steps_per_mode, int *mode_out, double *ms_out);
