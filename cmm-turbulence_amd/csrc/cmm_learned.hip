@@ -21,6 +21,7 @@
 namespace cmm {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 constexpr int kCh = 32;  // channels of every layer
 
@@ -127,6 +128,122 @@ __global__ void __launch_bounds__(256) k_conv3(const unsigned short *__restrict_
                                   (unsigned)f2bf(v[2]) | ((unsigned)f2bf(v[3]) << 16)};
                 *reinterpret_cast<uint2 *>(static_cast<unsigned short *>(out) + o + 8 * g) = pk;
             }
+        }
+    }
+}
+
+// k_conv3 for large images with blocks that stay over a strip of 4 segments (128 pixels) and a
+// range of row groups: the input rows of the group being computed sit in a ring of 6 LDS rows
+// (row r in slot (r + 1) mod 6), and while the group's products run, the next group's 4 new rows
+// come from global memory into registers; after a barrier they replace the 4 rows no longer
+// needed.  Same products and epilogue as k_conv3 (bit for bit).
+template <int MODE>
+__global__ void __launch_bounds__(256) k_conv3_ring(const unsigned short *__restrict__ in,
+                                                    const unsigned short *__restrict__ W, const float *__restrict__ bias,
+                                                    void *__restrict__ out, int H, int Wd, int strips, int rgroups,
+                                                    int gper, const unsigned short *__restrict__ mask) {
+    constexpr int SEGS = 4, PIX = 32 * SEGS + 2;
+    constexpr int NPF = (4 * PIX * 4 + 255) / 256;  // 16-byte chunks of 4 rows per thread
+    __shared__ __align__(16) unsigned short tile[6 * PIX * kCvPitch];
+    const int g0 = (blockIdx.x / strips) * gper, g1 = min(rgroups, g0 + gper);
+    if (g0 >= g1) return;  // uniform over the block, before any barrier
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int x0 = (blockIdx.x % strips) * (32 * SEGS);
+    const int nseg = min(SEGS, (Wd - x0) / 32);
+    const int npix = 32 * nseg + 2;
+    const long img = (long)blockIdx.y * H * Wd;
+    auto slot = [](int r) { return (r + 7) % 6; };  // (r + 1) mod 6 for r >= -1
+    auto src = [=](int r, int px, int q) {
+        const int yy = (r % H + H) % H, xx = wrapi(x0 - 1 + px, Wd);
+        return in + (img + (long)yy * Wd + xx) * kCh + 8 * q;
+    };
+    // the first group's 6 rows
+    {
+        const int ya = 4 * g0;
+        for (int c = threadIdx.x; c < 6 * npix * 4; c += 256) {
+            const int q = c & 3, px = (c >> 2) % npix, r = ya - 1 + (c >> 2) / npix;
+            *reinterpret_cast<uint4 *>(tile + (slot(r) * PIX + px) * kCvPitch + 8 * q) =
+                *reinterpret_cast<const uint4 *>(src(r, px, q));
+        }
+    }
+    const int i = lane & 31, h = lane >> 5;
+    bf16x8 wf[18];
+#pragma unroll
+    for (int q = 0; q < 18; ++q)
+        wf[q] = *reinterpret_cast<const bf16x8 *>(W + ((long)i * 9 + q / 2) * kCh + 8 * h + 16 * (q % 2));
+    __syncthreads();
+    for (int g = g0; g < g1; ++g) {
+        const int ya = 4 * g;
+        const bool more = g + 1 < g1;  // uniform over the block
+        u32x4 pf[NPF];  // a native vector: struct copies would go through private memory
+#pragma unroll
+        for (int j = 0; j < NPF; ++j) {
+            // a chunk past the 4 rows repeats the last one (after the last group: chunk 0, unused)
+            const int c = min(threadIdx.x + 256 * j, 4 * npix * 4 - 1) * (more ? 1 : 0);
+            const int q = c & 3, px = (c >> 2) % npix, r = ya + 5 + (c >> 2) / npix;
+            pf[j] = *reinterpret_cast<const u32x4 *>(src(r, px, q));
+        }
+        const int y = ya + wid;
+        if (y < H) {
+            for (int sg = 0; sg < nseg; ++sg) {
+                const long o = (img + (long)y * Wd + x0 + 32 * sg + i) * kCh + 4 * h;
+                f32x16 acc;
+#pragma unroll
+                for (int gg = 0; gg < 4; ++gg) {
+                    const float4 bv =
+                        bias ? *reinterpret_cast<const float4 *>(bias + 8 * gg + 4 * h) : float4{0.f, 0.f, 0.f, 0.f};
+                    acc[4 * gg] = bv.x;
+                    acc[4 * gg + 1] = bv.y;
+                    acc[4 * gg + 2] = bv.z;
+                    acc[4 * gg + 3] = bv.w;
+                }
+                uint2 mk[4];
+                if (MODE == 3)
+#pragma unroll
+                    for (int gg = 0; gg < 4; ++gg) mk[gg] = *reinterpret_cast<const uint2 *>(mask + o + 8 * gg);
+#pragma unroll
+                for (int t = 0; t < 9; ++t) {
+                    const unsigned short *bp =
+                        tile + (slot(y + t / 3 - 1) * PIX + 32 * sg + i + t % 3) * kCvPitch + 8 * h;
+#pragma unroll
+                    for (int kb = 0; kb < 2; ++kb) {
+                        const bf16x8 bx = *reinterpret_cast<const bf16x8 *>(bp + 16 * kb);
+                        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(wf[2 * t + kb], bx, acc, 0, 0, 0);
+                    }
+                }
+#pragma unroll
+                for (int gg = 0; gg < 4; ++gg) {
+                    float v[4];
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        v[j] = acc[4 * gg + j];
+                        if (MODE == 0) v[j] = fmaxf(v[j], 0.f);
+                        if (MODE == 3) {
+                            const unsigned a = ((j < 2 ? mk[gg].x : mk[gg].y) >> (16 * (j & 1))) & 0xffffu;
+                            if ((a & 0x8000u) || (a & 0x7fffu) == 0) v[j] = 0.f;
+                        }
+                    }
+                    if (MODE == 2) {
+                        *reinterpret_cast<float4 *>(static_cast<float *>(out) + o + 8 * gg) = float4{v[0], v[1], v[2], v[3]};
+                    } else {
+                        const uint2 pk = {(unsigned)f2bf(v[0]) | ((unsigned)f2bf(v[1]) << 16),
+                                          (unsigned)f2bf(v[2]) | ((unsigned)f2bf(v[3]) << 16)};
+                        *reinterpret_cast<uint2 *>(static_cast<unsigned short *>(out) + o + 8 * gg) = pk;
+                    }
+                }
+            }
+        }
+        __syncthreads();  // the group's reads of the ring are done
+        if (more) {
+            // the same chunk indices as the loads: a chunk past the 4 rows repeats the last one,
+            // the same data to the same place
+#pragma unroll
+            for (int j = 0; j < NPF; ++j) {
+                const int c = min(threadIdx.x + 256 * j, 4 * npix * 4 - 1);
+                const int q = c & 3, px = (c >> 2) % npix, r = ya + 5 + (c >> 2) / npix;
+                *reinterpret_cast<u32x4 *>(tile + (slot(r) * PIX + px) * kCvPitch + 8 * q) = pf[j];
+            }
+            __syncthreads();
         }
     }
 }
@@ -451,7 +568,20 @@ int conv3(int n_img, int H, int W, const uint16_t *in, const uint16_t *w, const 
     else if (mode == 2) CMM_CONV(2, SG); \
     else CMM_CONV(3, SG);
     if (four) {
-        CMM_CONV_MODES(4)
+        // blocks over strips of 128 pixels and ranges of row groups, about 2 per CU
+        const int strips = (W / 32 + 3) / 4;
+        const int rgroups = (H + kCvRows - 1) / kCvRows;
+        const long per = (long)strips * n_img;
+        const int parts = (int)std::max<long>(1, std::min<long>(rgroups, 512 / per));
+        const int gper = (rgroups + parts - 1) / parts;
+        const dim3 gr((unsigned)(strips * parts), (unsigned)n_img);
+#define CMM_RING(M) \
+    hipLaunchKernelGGL((k_conv3_ring<M>), gr, dim3(256), 0, S(stream), in, w, b, out, H, W, strips, rgroups, gper, mask)
+        if (mode == 0) CMM_RING(0);
+        else if (mode == 1) CMM_RING(1);
+        else if (mode == 2) CMM_RING(2);
+        else CMM_RING(3);
+#undef CMM_RING
     } else {
         CMM_CONV_MODES(1)
     }

@@ -73,7 +73,10 @@ def test_conv_kernels_match_cpu(dev):
     cmm._check(L.cmm_cnn_weights_bf16(1, vp(w32), vp(wb), vp(wt), stream()), "weights")
     assert np.array_equal(host16(wb), w.ravel())
     gin = torch.zeros(N * H * W * 32, dtype=torch.int16, device="cuda")
-    cmm._check(L.cmm_cnn_backward_data(N, H, W, vp(dev16(g)), vp(wt), vp(gin), stream()), "backward data")
+    # every device input is held in a variable while a call reads it (a temporary's memory goes
+    # back to the allocator, which can hand it to the next tensor before the kernel runs)
+    gd = dev16(g)
+    cmm._check(L.cmm_cnn_backward_data(N, H, W, vp(gd), vp(wt), vp(gin), stream()), "backward data")
     refd = R.conv_back_data(g, w)
     got = host16(gin).reshape(refd.shape)
     e = np.abs(got - refd).max() / np.abs(refd).max()
@@ -82,18 +85,62 @@ def test_conv_kernels_match_cpu(dev):
     # with the ReLU derivative of the layer below in the same pass: the separate mask's result
     act = dev16(R.bf16(rng.standard_normal((N, H, W, 32))))
     gin_r = torch.zeros_like(gin)
-    cmm._check(L.cmm_cnn_backward_data_relu(N, H, W, vp(dev16(g)), vp(wt), vp(act), vp(gin_r), stream()),
+    cmm._check(L.cmm_cnn_backward_data_relu(N, H, W, vp(gd), vp(wt), vp(act), vp(gin_r), stream()),
                "backward data relu")
     cmm._check(L.cmm_cnn_relu_mask(N * H * W * 32, vp(act), vp(gin), stream()), "relu mask")
     assert np.array_equal(host16(gin_r), host16(gin))
     dw = torch.zeros(32 * 9 * 32, dtype=torch.float32, device="cuda")
     db = torch.zeros(32, dtype=torch.float32, device="cuda")
-    cmm._check(L.cmm_cnn_backward_weights(N, H, W, vp(dev16(g)), vp(xin), vp(dw), vp(db), stream()), "backward w")
+    cmm._check(L.cmm_cnn_backward_weights(N, H, W, vp(gd), vp(xin), vp(dw), vp(db), stream()), "backward w")
     rw, rb = R.conv_back_weights(g, x)
     ew = np.abs(dw.cpu().numpy().reshape(rw.shape) - rw).max() / np.abs(rw).max()
     eb = np.abs(db.cpu().numpy() - rb).max() / np.abs(rb).max()
     print("backward weights: rel err", ew, "bias", eb)
     assert ew <= 1e-5 and eb <= 1e-5
+
+
+def test_conv_kernels_on_large_images_match_cpu(dev):
+    """Images large enough for the strip-resident launch (2046 x 160 x 32 channels, where blocks
+    stay over a 128-pixel strip and stage each next row group during the products, including a
+    strip of one segment and a height not a multiple of the 4-row groups): forward with FP32 and
+    ReLU outputs and the backward-data pass with the ReLU derivative, against the CPU reference
+    with the small test's bounds."""
+    import cmm
+    import torch
+    rng = np.random.default_rng(2)
+    N, H, W = 1, 2046, 160
+    x = R.bf16(rng.standard_normal((N, H, W, 32)))
+    w = R.bf16(rng.standard_normal((32, 9, 32)) * 0.1)
+    b = rng.standard_normal(32).astype(np.float32).astype(np.float64)
+    L = cmm.lib()
+    xin, wd, bd = dev16(x), dev16(w), dev32(b)
+    ref = R.conv(x, w, b)
+    out32 = torch.zeros(N * H * W * 32, dtype=torch.float32, device="cuda")
+    cmm._check(L.cmm_cnn_forward(N, H, W, vp(xin), vp(wd), vp(bd), 2, vp(out32), stream()), "forward")
+    got = out32.cpu().numpy().reshape(ref.shape)
+    e = np.abs(got - ref).max() / np.abs(ref).max()
+    print("large forward fp32 out: rel err", e)
+    assert e <= 1e-5
+    out16 = torch.zeros(N * H * W * 32, dtype=torch.int16, device="cuda")
+    cmm._check(L.cmm_cnn_forward(N, H, W, vp(xin), vp(wd), vp(bd), 0, vp(out16), stream()), "forward relu")
+    got = host16(out16).reshape(ref.shape)
+    refr = np.maximum(ref, 0)
+    assert np.all(np.abs(got - refr) <= 2.0 ** -8 * np.abs(refr) + 1e-5 * np.abs(ref).max())
+    g = R.bf16(rng.standard_normal((N, H, W, 32)))
+    act = R.bf16(rng.standard_normal((N, H, W, 32)))
+    w32 = dev32(w.ravel())
+    wb = torch.zeros(32 * 9 * 32, dtype=torch.int16, device="cuda")
+    wt = torch.zeros_like(wb)
+    cmm._check(L.cmm_cnn_weights_bf16(1, vp(w32), vp(wb), vp(wt), stream()), "weights")
+    gin = torch.zeros(N * H * W * 32, dtype=torch.int16, device="cuda")
+    gd, ad = dev16(g), dev16(act)
+    cmm._check(L.cmm_cnn_backward_data_relu(N, H, W, vp(gd), vp(wt), vp(ad), vp(gin), stream()),
+               "backward data relu")
+    refd = R.conv_back_data(g, w) * (act > 0)
+    got = host16(gin).reshape(refd.shape)
+    e = np.abs(got - refd).max() / np.abs(refd).max()
+    print("large backward data relu: rel err", e)
+    assert e <= 2.0 ** -8 + 1e-5
 
 
 def test_network_gradient_matches_cpu(dev):
