@@ -52,7 +52,8 @@ __device__ __forceinline__ int wrapi(int v, int n) { return v < 0 ? v + n : v >=
 constexpr int kCvRows = 4;    // output rows per block (one per wave)
 constexpr int kCvPitch = 40;  // LDS elements per staged pixel (32 channels + 8 pad)
 
-// SEGS: 32-pixel segments per block and row (4, or 1 where 4 would leave too few blocks)
+// SEGS: 32-pixel segments per block and row (launched with 1: the images too small for
+// k_conv3_ring's strips of 4, where blocks of 4 segments would leave the device idle)
 template <int MODE, int SEGS>
 __global__ void __launch_bounds__(256) k_conv3(const unsigned short *__restrict__ in, const unsigned short *__restrict__ W,
                                                const float *__restrict__ bias, void *__restrict__ out, int H, int Wd,
