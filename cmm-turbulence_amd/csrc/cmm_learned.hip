@@ -435,16 +435,20 @@ __global__ void __launch_bounds__(256) k_adamw(long n, float *__restrict__ p, co
 // network input from three consecutive psi value planes (newest first): psi_n / s0,
 // (psi_n - psi_{n-1}) / s1, (psi_n - 2 psi_{n-1} + psi_{n-2}) / s2 in channels 0..2, zeros after
 __global__ void __launch_bounds__(256) k_cnn_input(long n, const double *__restrict__ p0, const double *__restrict__ p1,
-                                                   const double *__restrict__ p2, float s0, float s1, float s2,
+                                                   const double *__restrict__ p2, const float *__restrict__ scales,
                                                    unsigned short *__restrict__ out) {
     const long j = (long)blockIdx.x * 256 + threadIdx.x;
     if (j >= n) return;
     const double a = p0[j], b = p1[j], c = p2[j];
-    unsigned short *o = out + j * kCh;
-    o[0] = f2bf((float)(a / s0));
-    o[1] = f2bf((float)((a - b) / s1));
-    o[2] = f2bf((float)((a - 2.0 * b + c) / s2));
-    for (int k = 3; k < kCh; ++k) o[k] = 0;
+    const float s0 = scales[0], s1 = scales[1], s2 = scales[2];
+    const unsigned c0 = f2bf((float)(a / s0)), c1 = f2bf((float)((a - b) / s1)),
+                   c2 = f2bf((float)((a - 2.0 * b + c) / s2));
+    // the pixel's 32 channels (64 bytes) as four 16-byte stores
+    uint4 *o = reinterpret_cast<uint4 *>(out + j * kCh);
+    o[0] = uint4{c0 | (c1 << 16), c2, 0u, 0u};
+    o[1] = uint4{0u, 0u, 0u, 0u};
+    o[2] = uint4{0u, 0u, 0u, 0u};
+    o[3] = uint4{0u, 0u, 0u, 0u};
 }
 
 // Lagrange-weighted extrapolation of three psi value planes plus an optional correction
@@ -464,10 +468,11 @@ __global__ void __launch_bounds__(256) k_extrapolate(long n, const double *__res
 __global__ void __launch_bounds__(256) k_cnn_target(long n, const double *__restrict__ p0, const double *__restrict__ p1,
                                                     const double *__restrict__ p2, const double *__restrict__ pn,
                                                     const double *__restrict__ ph, double a0, double a1, double a2,
-                                                    double c0, double c1, double c2, float r1, float r12,
+                                                    double c0, double c1, double c2, const float *__restrict__ rs,
                                                     float *__restrict__ tgt) {
     const long j = (long)blockIdx.x * 256 + threadIdx.x;
     if (j >= n) return;
+    const float r1 = rs[0], r12 = rs[1];
     double e1 = a0 * p0[j];
     e1 += a1 * p1[j];
     e1 += a2 * p2[j];
@@ -501,10 +506,12 @@ __global__ void __launch_bounds__(256) k_cnn_mse(long n, const float *__restrict
 }
 
 // correction value fields from the network output: d1 = r1 pred[.][0], d12 = r12 pred[.][1]
-__global__ void __launch_bounds__(256) k_cnn_correction(long n, const float *__restrict__ pred, float r1, float r12,
-                                                        double *__restrict__ d1, double *__restrict__ d12) {
+__global__ void __launch_bounds__(256) k_cnn_correction(long n, const float *__restrict__ pred,
+                                                        const float *__restrict__ rs, double *__restrict__ d1,
+                                                        double *__restrict__ d12) {
     const long j = (long)blockIdx.x * 256 + threadIdx.x;
     if (j >= n) return;
+    const float r1 = rs[0], r12 = rs[1];
     d1[j] = (double)r1 * pred[j * kCh];
     d12[j] = (double)r12 * pred[j * kCh + 1];
 }
@@ -662,8 +669,7 @@ int cmm_cnn_adamw(long n, float *p, const float *g, float *m, float *v, float lr
 int cmm_cnn_input(long n, const double *psi0, const double *psi1, const double *psi2, const float *scales,
                   uint16_t *out, void *stream) {
     if (n <= 0 || !psi0 || !psi1 || !psi2 || !scales || !out) { set_error("cmm_cnn_input: bad argument"); return CMM_ERR_ARG; }
-    hipLaunchKernelGGL(k_cnn_input, dim3(blocks(n)), dim3(256), 0, S(stream), n, psi0, psi1, psi2, scales[0], scales[1],
-                       scales[2], out);
+    hipLaunchKernelGGL(k_cnn_input, dim3(blocks(n)), dim3(256), 0, S(stream), n, psi0, psi1, psi2, scales, out);
     return lc("k_cnn_input");
 }
 
@@ -675,7 +681,7 @@ int cmm_cnn_target(long n, const double *psi0, const double *psi1, const double 
         return CMM_ERR_ARG;
     }
     hipLaunchKernelGGL(k_cnn_target, dim3(blocks(n)), dim3(256), 0, S(stream), n, psi0, psi1, psi2, psi_next, psi_half,
-                       L1[0], L1[1], L1[2], L12[0], L12[1], L12[2], rscales[0], rscales[1], target);
+                       L1[0], L1[1], L1[2], L12[0], L12[1], L12[2], rscales, target);
     return lc("k_cnn_target");
 }
 
@@ -687,8 +693,7 @@ int cmm_cnn_mse(long n, const float *pred, const float *target, uint16_t *g, dou
 
 int cmm_cnn_correction(long n, const float *pred, const float *rscales, double *d1, double *d12, void *stream) {
     if (n <= 0 || !pred || !rscales || !d1 || !d12) { set_error("cmm_cnn_correction: bad argument"); return CMM_ERR_ARG; }
-    hipLaunchKernelGGL(k_cnn_correction, dim3(blocks(n)), dim3(256), 0, S(stream), n, pred, rscales[0], rscales[1], d1,
-                       d12);
+    hipLaunchKernelGGL(k_cnn_correction, dim3(blocks(n)), dim3(256), 0, S(stream), n, pred, rscales, d1, d12);
     return lc("k_cnn_correction");
 }
 

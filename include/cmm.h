@@ -480,16 +480,18 @@ int cmm_cnn_weights_bf16(int n_layers, const float *w32, uint16_t *w, uint16_t *
 int cmm_cnn_adamw(long n, float *p, const float *g, float *m, float *v, float lr, float beta1, float beta2, float eps,
                   float weight_decay, float gscale, int step, void *stream);
 /* network input from three psi value planes (newest first): psi0/s0, (psi0-psi1)/s1,
- * (psi0-2psi1+psi2)/s2 (scales[3]) */
+ * (psi0-2psi1+psi2)/s2 (scales: 3 floats in device memory, read by the kernel in stream order) */
 int cmm_cnn_input(long n, const double *psi0, const double *psi1, const double *psi2, const float *scales,
                   uint16_t *out, void *stream);
-/* training targets (2 floats per pixel): (psi_next - sum L1 psi_l)/r1, (psi_half - sum L12 psi_l)/r12 */
+/* training targets (2 floats per pixel): (psi_next - sum L1 psi_l)/r1, (psi_half - sum L12 psi_l)/r12
+ * (L1, L12: 3 doubles each in host memory; rscales = {r1, r12}: 2 floats in device memory) */
 int cmm_cnn_target(long n, const double *psi0, const double *psi1, const double *psi2, const double *psi_next,
                    const double *psi_half, const double *L1, const double *L12, const float *rscales, float *target,
                    void *stream);
 /* squared-error loss (added to *loss) and its gradient pred - target in channels 0, 1 of g */
 int cmm_cnn_mse(long n, const float *pred, const float *target, uint16_t *g, double *loss, void *stream);
-/* correction value fields d1 = r1 pred[.][0], d12 = r12 pred[.][1] */
+/* correction value fields d1 = r1 pred[.][0], d12 = r12 pred[.][1] (rscales = {r1, r12}: 2 floats
+ * in device memory) */
 int cmm_cnn_correction(long n, const float *pred, const float *rscales, double *d1, double *d12, void *stream);
 /* out = w0 psi0 + w1 psi1 + w2 psi2 (+ corr) */
 int cmm_extrapolate(long n, const double *psi0, const double *psi1, const double *psi2, const double *w,
