@@ -493,7 +493,7 @@ int cmm_cnn_mse(long n, const float *pred, const float *target, uint16_t *g, dou
 /* correction value fields d1 = r1 pred[.][0], d12 = r12 pred[.][1] (rscales = {r1, r12}: 2 floats
  * in device memory) */
 int cmm_cnn_correction(long n, const float *pred, const float *rscales, double *d1, double *d12, void *stream);
-/* out = w0 psi0 + w1 psi1 + w2 psi2 (+ corr) */
+/* out = w0 psi0 + w1 psi1 + w2 psi2 (+ corr) (w: 3 doubles in host memory) */
 int cmm_extrapolate(long n, const double *psi0, const double *psi1, const double *psi2, const double *w,
                     const double *corr, double *out, void *stream);
 /* out2[0] = sum (a - b)^2, out2[1] = sum b^2 (device doubles) */
