@@ -487,16 +487,21 @@ __global__ void __launch_bounds__(256) k_cnn_target(long n, const double *__rest
 // the sum of squares is accumulated per block and added once
 __global__ void __launch_bounds__(256) k_cnn_mse(long n, const float *__restrict__ pred, const float *__restrict__ tgt,
                                                  unsigned short *__restrict__ g, double *__restrict__ loss) {
-    const long j = (long)blockIdx.x * 256 + threadIdx.x;
+    // grid-stride: a capped grid adds one partial sum per block to *loss (one address: every
+    // block's atomic add waits its turn)
     double s = 0;
-    if (j < n) {
-        unsigned short *o = g + j * kCh;
-        for (int c = 0; c < 2; ++c) {
-            const float d = pred[j * kCh + c] - tgt[2 * j + c];
-            o[c] = f2bf(d);
-            s += (double)d * d;
-        }
-        for (int c = 2; c < kCh; ++c) o[c] = 0;
+    for (long j = (long)blockIdx.x * 256 + threadIdx.x; j < n; j += (long)gridDim.x * 256) {
+        const float2 p = *reinterpret_cast<const float2 *>(pred + j * kCh);
+        const float2 t = *reinterpret_cast<const float2 *>(tgt + 2 * j);
+        const float d0 = p.x - t.x, d1 = p.y - t.y;
+        s += (double)d0 * d0;
+        s += (double)d1 * d1;
+        // the pixel's 32 gradient channels (64 bytes) as four 16-byte stores
+        uint4 *o = reinterpret_cast<uint4 *>(g + j * kCh);
+        o[0] = uint4{(unsigned)f2bf(d0) | ((unsigned)f2bf(d1) << 16), 0u, 0u, 0u};
+        o[1] = uint4{0u, 0u, 0u, 0u};
+        o[2] = uint4{0u, 0u, 0u, 0u};
+        o[3] = uint4{0u, 0u, 0u, 0u};
     }
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
     __shared__ double red[4];
@@ -687,7 +692,8 @@ int cmm_cnn_target(long n, const double *psi0, const double *psi1, const double 
 
 int cmm_cnn_mse(long n, const float *pred, const float *target, uint16_t *g, double *loss, void *stream) {
     if (n <= 0 || !pred || !target || !g || !loss) { set_error("cmm_cnn_mse: bad argument"); return CMM_ERR_ARG; }
-    hipLaunchKernelGGL(k_cnn_mse, dim3(blocks(n)), dim3(256), 0, S(stream), n, pred, target, g, loss);
+    hipLaunchKernelGGL(k_cnn_mse, dim3((unsigned)std::min<long>(blocks(n), 4096)), dim3(256), 0, S(stream), n, pred,
+                       target, g, loss);
     return lc("k_cnn_mse");
 }
 
